@@ -1,0 +1,122 @@
+// Device-resident window layout shared by the HIP kernels and the host driver.
+//
+// One window = one sliding window of one session. Observation streams are stored SoA in
+// planes of length F, sorted by landmark (CSR offsets per landmark), so a group of lanes walks
+// one landmark's factors with coalesced loads. State lives in HBM for the whole solve; the
+// host only reads a handful of scalars per LM iteration.
+//
+// Batched kernels walk a work list of Chunks (a contiguous landmark range of one window);
+// per-chunk partial sums are reduced in a fixed order by the per-window kernels, so a
+// window's result does not depend on which other windows share the launch.
+#pragma once
+
+#include <cstdint>
+
+namespace vib {
+
+constexpr int kMaxFrames = 20;      // LDS accumulator sized for nv = 6*20 + 7 = 127
+constexpr int kMaxCameras = 4;
+constexpr int kMaxVisual = 6 * kMaxFrames + 7;
+constexpr int kMaxRetained = 15 * kMaxFrames + 7;
+constexpr int kChunkLanes = 8;      // lanes per landmark in the per-landmark kernels
+constexpr int kBlock = 256;         // threads per chunk workgroup
+
+// observation planes (doubles, each F long)
+enum ObsPlane { OB_BX, OB_BY, OB_BZ, OB_VX, OB_VY, OB_VZ, OB_T, OB_S00, OB_S10, OB_S11, OB_PLANES };
+// landmark planes (doubles, each L long)
+enum LmPlane { LM_BX, LM_BY, LM_BZ, LM_VX, LM_VY, LM_VZ, LM_T, LM_PLANES };
+// per-landmark linearization outputs (doubles, each L long)
+enum LmOut { LO_D, LO_G, LO_CR0, LO_CR1, LO_CR2, LO_CR3, LO_CR4, LO_CR5, LO_CT, LO_PLANES };
+// per-chunk scalars
+enum ChunkScalar { CS_COST, CS_INVALID, CS_PRED, CS_MAXINC, CS_TRIAL, CS_TINVALID, CS_PLANES };
+
+// LM bookkeeping per window (BackendOptimizer.cpp:167-307), updated on the device.
+struct LmState {
+    double lambda, increase, cost, initial_cost;
+    double lin_cost;        // cost reported by the last linearization
+    double predicted, trial_cost, max_increment;
+    double last_gain;
+    int32_t new_point;      // linearize at the start of the next iteration
+    int32_t status;         // 0 running, 1 finished (converged / budget), <0 -status error
+    int32_t iteration;      // iterations started
+    int32_t accepted;
+    int32_t converged;
+    int32_t lin_valid, chol_ok, trial_valid;
+    int32_t max_iterations;
+    int32_t evaluate_only;  // piecewise API: compute metrics, never accept
+    int32_t pad[2];
+};
+
+struct PriorBlockDev {
+    int32_t kind;      // VIB_KEY_*
+    int32_t slot;      // frame slot (POSE/SB)
+    int32_t col;       // tangent column or -1
+    int32_t width;     // free width (0 constant)
+    int32_t tsize;     // tangent size 6/9/6/1
+    int32_t chart;     // chart offset in A's columns
+    int32_t ref;       // offset into the reference values
+    int32_t pad;
+};
+
+struct WinDev {
+    int32_t K, C, L, F, E;
+    int32_t nv, nr, ne;
+    int32_t rig_col, td_col;
+    uint32_t flags;
+    int32_t tri;               // nv*(nv+1)/2
+    double huber, min_rho, td_lo, td_hi;
+    int32_t pose_col[kMaxFrames], sb_col[kMaxFrames], sb_w[kMaxFrames];
+    // state and trial candidate
+    double* pose; double* sb; double* rig; double* td; double* rho;
+    double* cpose; double* csb; double* crig; double* ctd; double* crho;
+    const double* cam;         // C x 12: R (9, row-major R_RC) then t (3)
+    // landmarks
+    const int32_t* lm_fc;      // anchor frame << 8 | camera
+    const int32_t* lm_off;     // L + 1 observation offsets
+    const int32_t* lm_col;     // rho column or -1
+    const double* lmd;         // LM_PLANES x L
+    // observations (sorted by landmark)
+    const int32_t* ob_fc;      // observer frame << 8 | camera
+    const double* ob;          // OB_PLANES x F
+    // linearization outputs
+    double* fac_u;             // 6 x F : J_anchor^T J_rho per factor (weighted)
+    double* lm_out;            // LO_PLANES x L
+    double* lm_delta;          // L (tangent step of rho; 0 for constant)
+    // IMU
+    const int32_t* imu_from; const int32_t* imu_to; const double* imu;
+    // prior
+    int32_t prior_rows, prior_cols, prior_blocks, pad0;
+    const PriorBlockDev* pblk;
+    const double* pA;          // rows x cols
+    const double* pr0;
+    const double* pG;          // cols x cols = A^T A
+    const double* pref;
+    const int32_t* pfree;      // free prior columns: block << 8 | index within the block
+    int32_t prior_free, pad2;
+    // dense per-window buffers
+    double* Hret;              // nr x nr retained (IMU + prior) normal matrix
+    double* gret;              // nr
+    double* Hfull;             // nr x nr undamped full normal matrix of the states
+    double* gfull;             // nr
+    double* S;                 // nr x nr damped reduced camera system / Cholesky factor
+    double* Scopy;             // nr x nr damped reduced system kept for refinement
+    double* rhs;               // nr
+    double* dstate;            // nr state step
+    double* scratch;           // 4 * nr
+    // partials
+    double* lin_part;          // nchunks x (tri + nv)
+    double* schur_part;        // nchunks x (tri + nv)
+    double* chunk_sc;          // nchunks x CS_PLANES
+    int32_t chunk0, nchunks;
+    double ret_cost;           // retained cost at the linearization point (device-written)
+    int32_t ret_invalid, pad1;
+    LmState* lm;
+};
+
+struct Chunk {
+    int32_t win;        // window index in the batch
+    int32_t local;      // chunk index within the window
+    int32_t lm_begin, lm_end;
+};
+
+}  // namespace vib

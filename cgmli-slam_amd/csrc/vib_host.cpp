@@ -1,0 +1,777 @@
+// Host driver of the MI355X Schur backend: window slots in HBM, the batched LM loop and the
+// C-ABI of include/vib.h.
+//
+// Counterparts in the reference:
+//   window upload     VisualCudaBatch::linearizeSchur(newTopology) + configureRetained +
+//                     configureResident (src/backend/VisualCudaBatch.cu:433-511,606-624,673-691)
+//   LM loop           solveVisualSchur (src/backend/BackendOptimizer.cpp:27-331), here with the
+//                     accept/reject decision evaluated on the device for every window at once
+//   piecewise API     linearizeResident / solveOnline / acceptResident / readResident
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vib.h"
+#include "vib_device.hpp"
+#include "vib_host.hpp"
+
+extern "C" {
+int vibk_retained_shared_bytes(int E, int prior_rows, int prior_cols, int prior_blocks);
+hipError_t vibk_linearize(vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, size_t ret_shared,
+                          hipStream_t s);
+hipError_t vibk_reduce(vib::WinDev* wins, int nwin, int which, double** outs, int max_len, hipStream_t s);
+hipError_t vibk_schur(vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, hipStream_t s);
+hipError_t vibk_assemble(vib::WinDev* wins, int nwin, double* const* reduced, hipStream_t s);
+hipError_t vibk_cholesky(vib::WinDev* wins, int nwin, int refinement, hipStream_t s);
+hipError_t vibk_backsub_trial(vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, hipStream_t s);
+hipError_t vibk_lm_update(vib::WinDev* wins, int nwin, int mode, size_t ret_shared, hipStream_t s);
+hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
+}
+
+namespace vib {
+
+namespace {
+
+int tangent_size(int kind) { return kind == VIB_KEY_POSE || kind == VIB_KEY_RIG ? 6 : (kind == VIB_KEY_SPEED_BIAS ? 9 : 1); }
+int ambient_size(int kind) { return kind == VIB_KEY_POSE || kind == VIB_KEY_RIG ? 7 : (kind == VIB_KEY_SPEED_BIAS ? 9 : 1); }
+
+// Bump allocator over one hipMalloc arena per slot.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    template <class T>
+    T* take(size_t n) {
+        used = (used + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + used);
+        used += sizeof(T) * std::max<size_t>(n, 1);
+        return p;
+    }
+};
+
+}  // namespace
+
+int HostWindow::chunk_size = 512;
+
+// ------------------------------------------------------------------------------------------
+// Host-side window copy with derived layout.
+// ------------------------------------------------------------------------------------------
+bool HostWindow::build(const vib_window& w, std::string& err) {
+    *this = HostWindow();
+    K = w.num_frames; C = w.num_cameras; L = w.num_landmarks; F = w.num_observations; E = w.num_imu;
+    flags = w.flags; huber = w.huber_delta; min_rho = w.min_inverse_depth;
+    td_lo = w.td_lower; td_hi = w.td_upper;
+    if (K < 1 || K > kMaxFrames) { err = "window must hold 1..20 frames"; return false; }
+    if (C < 1 || C > kMaxCameras) { err = "window must hold 1..4 cameras"; return false; }
+    if (L < 0 || F < 0 || E < 0) { err = "negative sizes"; return false; }
+    if (!w.pose || !w.speed_bias || !w.rig || !w.td || !w.cam_rotation || !w.cam_translation) { err = "missing state"; return false; }
+    if (L && (!w.lm_anchor_frame || !w.lm_anchor_cam || !w.lm_bearing || !w.lm_velocity || !w.lm_tref || !w.lm_rho)) { err = "missing landmarks"; return false; }
+    if (F && (!w.obs_landmark || !w.obs_frame || !w.obs_cam || !w.obs_bearing || !w.obs_velocity || !w.obs_tref || !w.obs_sqrt_info)) { err = "missing observations"; return false; }
+    if (E && (!w.imu_from || !w.imu_to || !w.imu_data)) { err = "missing IMU"; return false; }
+    if (!(huber >= 0) || !(min_rho > 0)) { err = "invalid huber / inverse depth bound"; return false; }
+    frame_id.assign(K, 0);
+    for (int k = 0; k < K; ++k) frame_id[k] = w.frame_id ? w.frame_id[k] : k;
+    pose.assign(w.pose, w.pose + 7 * K);
+    sb.assign(w.speed_bias, w.speed_bias + 9 * K);
+    pose_free.assign(K, 1);
+    sb_width.assign(K, 9);
+    for (int k = 0; k < K; ++k) {
+        if (w.pose_free) pose_free[k] = w.pose_free[k] ? 1 : 0;
+        if (w.sb_width) sb_width[k] = w.sb_width[k];
+        if (sb_width[k] != 0 && sb_width[k] != 3 && sb_width[k] != 9) { err = "sb_width must be 0, 3 or 9"; return false; }
+    }
+    std::copy_n(w.rig, 7, rig);
+    td = *w.td;
+    cam.assign(12 * C, 0);
+    for (int c = 0; c < C; ++c) {
+        for (int i = 0; i < 9; ++i) cam[12 * c + i] = w.cam_rotation[9 * c + i];
+        for (int i = 0; i < 3; ++i) cam[12 * c + 9 + i] = w.cam_translation[3 * c + i];
+    }
+    lm_frame.assign(L, 0); lm_cam.assign(L, 0); lm_free.assign(L, 1);
+    lmd.assign((size_t)LM_PLANES * L, 0);
+    rho.assign(L, 0);
+    for (int l = 0; l < L; ++l) {
+        lm_frame[l] = w.lm_anchor_frame[l]; lm_cam[l] = w.lm_anchor_cam[l];
+        if (lm_frame[l] < 0 || lm_frame[l] >= K || lm_cam[l] < 0 || lm_cam[l] >= C) { err = "landmark anchor out of range"; return false; }
+        for (int i = 0; i < 3; ++i) {
+            lmd[(size_t)(LM_BX + i) * L + l] = w.lm_bearing[3 * l + i];
+            lmd[(size_t)(LM_VX + i) * L + l] = w.lm_velocity[3 * l + i];
+        }
+        lmd[(size_t)LM_T * L + l] = w.lm_tref[l];
+        rho[l] = w.lm_rho[l];
+        if (w.lm_free) lm_free[l] = w.lm_free[l] ? 1 : 0;
+    }
+    // observations sorted by landmark (stable counting sort)
+    lm_off.assign(L + 1, 0);
+    for (int f = 0; f < F; ++f) {
+        const int l = w.obs_landmark[f];
+        if (l < 0 || l >= L) { err = "observation landmark out of range"; return false; }
+        if (w.obs_frame[f] < 0 || w.obs_frame[f] >= K || w.obs_cam[f] < 0 || w.obs_cam[f] >= C) { err = "observation frame/camera out of range"; return false; }
+        lm_off[l + 1]++;
+    }
+    for (int l = 0; l < L; ++l) {
+        if (lm_off[l + 1] > kMaxFactorsPerLandmark) { err = "a landmark has more than 13 factors"; return false; }
+        lm_off[l + 1] += lm_off[l];
+    }
+    std::vector<int> cursor(lm_off.begin(), lm_off.end() - 1);
+    ob_fc.assign(F, 0); ob_lm.assign(F, 0);
+    ob.assign((size_t)OB_PLANES * F, 0);
+    for (int f = 0; f < F; ++f) {
+        const int l = w.obs_landmark[f];
+        const int d = cursor[l]++;
+        ob_fc[d] = (w.obs_frame[f] << 8) | w.obs_cam[f];
+        ob_lm[d] = l;
+        for (int i = 0; i < 3; ++i) {
+            ob[(size_t)(OB_BX + i) * F + d] = w.obs_bearing[3 * f + i];
+            ob[(size_t)(OB_VX + i) * F + d] = w.obs_velocity[3 * f + i];
+            ob[(size_t)(OB_S00 + i) * F + d] = w.obs_sqrt_info[3 * f + i];
+        }
+        ob[(size_t)OB_T * F + d] = w.obs_tref[f];
+    }
+    imu_from.assign(w.imu_from, w.imu_from + E);
+    imu_to.assign(w.imu_to, w.imu_to + E);
+    imu.assign(w.imu_data, w.imu_data + (size_t)VIB_IMU_STRIDE * E);
+    for (int e = 0; e < E; ++e)
+        if (imu_from[e] < 0 || imu_from[e] >= K || imu_to[e] < 0 || imu_to[e] >= K || imu_from[e] == imu_to[e]) {
+            err = "IMU edge out of range"; return false;
+        }
+    // prior
+    prior_rows = w.prior_rows > 0 ? w.prior_rows : 0;
+    prior_cols = 0;
+    if (prior_rows) {
+        const int B = w.prior_blocks;
+        if (B <= 0 || !w.prior_key_kind || !w.prior_ref || !w.prior_A || !w.prior_r0) { err = "incomplete prior"; return false; }
+        int refs = 0;
+        for (int b = 0; b < B; ++b) {
+            const int k = w.prior_key_kind[b];
+            if (k < 0 || k > 3) { err = "invalid prior key kind"; return false; }
+            prior_kind.push_back(k);
+            prior_id.push_back(w.prior_key_id ? w.prior_key_id[b] : 0);
+            refs += ambient_size(k);
+            prior_cols += tangent_size(k);
+        }
+        prior_ref.assign(w.prior_ref, w.prior_ref + refs);
+        prior_A.assign(w.prior_A, w.prior_A + (size_t)prior_rows * prior_cols);
+        prior_r0.assign(w.prior_r0, w.prior_r0 + prior_rows);
+    }
+    return layout(err);
+}
+
+bool HostWindow::layout(std::string& err) {
+    int c = 0;
+    pose_col.assign(K, -1); sb_col.assign(K, -1);
+    for (int k = 0; k < K; ++k) if (pose_free[k]) { pose_col[k] = c; c += 6; }
+    rig_col = (flags & VIB_RIG_FREE) ? c : -1;
+    if (rig_col >= 0) c += 6;
+    td_col = (flags & VIB_TD_FREE) ? c : -1;
+    if (td_col >= 0) c += 1;
+    nv = c;
+    for (int k = 0; k < K; ++k) if (sb_width[k]) { sb_col[k] = c; c += sb_width[k]; }
+    nr = c;
+    lm_col.assign(L, -1);
+    ne = 0;
+    for (int l = 0; l < L; ++l) if (lm_free[l]) lm_col[l] = nr + ne++;
+    if (nr <= 0) { err = "no retained state to solve"; return false; }
+    // prior blocks
+    pblk.clear(); pfree.clear();
+    int chart = 0, ref = 0;
+    for (size_t b = 0; b < prior_kind.size(); ++b) {
+        PriorBlockDev d{};
+        d.kind = prior_kind[b];
+        d.slot = -1;
+        if (d.kind == VIB_KEY_POSE || d.kind == VIB_KEY_SPEED_BIAS) {
+            for (int k = 0; k < K; ++k) if (frame_id[k] == prior_id[b]) d.slot = k;
+            if (d.slot < 0) { err = "prior key refers to a frame outside the window"; return false; }
+        }
+        d.tsize = tangent_size(d.kind);
+        d.chart = chart; d.ref = ref;
+        chart += d.tsize; ref += ambient_size(d.kind);
+        if (d.kind == VIB_KEY_POSE) { d.col = pose_col[d.slot]; d.width = d.col >= 0 ? 6 : 0; }
+        else if (d.kind == VIB_KEY_SPEED_BIAS) { d.col = sb_col[d.slot]; d.width = d.col >= 0 ? sb_width[d.slot] : 0; }
+        else if (d.kind == VIB_KEY_RIG) { d.col = rig_col; d.width = rig_col >= 0 ? 6 : 0; }
+        else { d.col = td_col; d.width = td_col >= 0 ? 1 : 0; }
+        for (int i = 0; i < d.width; ++i) pfree.push_back(((int)b << 8) | i);
+        pblk.push_back(d);
+    }
+    // chunks
+    chunks.clear();
+    const int cs = std::max(1, chunk_size);
+    for (int l0 = 0, i = 0; l0 < L; l0 += cs, ++i) chunks.push_back(Chunk{0, i, l0, std::min(L, l0 + cs)});
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------
+// Slot: device arena holding one window.
+// ------------------------------------------------------------------------------------------
+Slot::~Slot() { release(); }
+
+void Slot::release() {
+    if (arena) hipFree(arena);
+    arena = nullptr;
+    bytes = 0;
+    used = false;
+}
+
+size_t Slot::footprint(const HostWindow& h) {
+    const size_t nr = h.nr, nv = h.nv, tri = nv * (nv + 1) / 2, nch = h.chunks.size();
+    const size_t B = h.pblk.size(), m = h.prior_rows, n = h.prior_cols;
+    size_t d = 0;
+    d += 2 * (16 * h.K + 8 + h.L);                     // state + candidate
+    d += 12 * h.C + LM_PLANES * h.L + OB_PLANES * h.F;  // streams
+    d += 6 * h.F + LO_PLANES * h.L + h.L;                // linearization outputs
+    d += (size_t)VIB_IMU_STRIDE * h.E;
+    d += m * n + m + n * n + h.prior_ref.size();
+    d += 4 * nr * nr + 8 * nr;
+    d += 2 * nch * (tri + nv) + nch * CS_PLANES + 2 * (tri + nv);
+    size_t i = 3 * h.L + 1 + h.F + 2 * h.E + h.pfree.size();
+    return 8 * d + 4 * i + sizeof(PriorBlockDev) * B + sizeof(LmState) + 64 * 256;
+}
+
+bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
+    release();
+    host = h;
+    bytes = footprint(h);
+    if (hipMalloc(&arena, bytes) != hipSuccess) { arena = nullptr; err = "device allocation failed"; return false; }
+    Arena a{reinterpret_cast<char*>(arena), bytes, 0};
+    WinDev& d = dev;
+    d = WinDev{};
+    d.K = h.K; d.C = h.C; d.L = h.L; d.F = h.F; d.E = h.E;
+    d.nv = h.nv; d.nr = h.nr; d.ne = h.ne; d.rig_col = h.rig_col; d.td_col = h.td_col; d.flags = h.flags;
+    d.tri = h.nv * (h.nv + 1) / 2;
+    d.huber = h.huber; d.min_rho = h.min_rho; d.td_lo = h.td_lo; d.td_hi = h.td_hi;
+    for (int k = 0; k < kMaxFrames; ++k) {
+        d.pose_col[k] = k < h.K ? h.pose_col[k] : -1;
+        d.sb_col[k] = k < h.K ? h.sb_col[k] : -1;
+        d.sb_w[k] = k < h.K ? h.sb_width[k] : 0;
+    }
+    d.pose = a.take<double>(7 * h.K); d.sb = a.take<double>(9 * h.K); d.rig = a.take<double>(7);
+    d.td = a.take<double>(1); d.rho = a.take<double>(h.L);
+    d.cpose = a.take<double>(7 * h.K); d.csb = a.take<double>(9 * h.K); d.crig = a.take<double>(7);
+    d.ctd = a.take<double>(1); d.crho = a.take<double>(h.L);
+    double* cam = a.take<double>(12 * h.C);
+    int32_t* lm_fc = a.take<int32_t>(h.L);
+    int32_t* lm_off = a.take<int32_t>(h.L + 1);
+    int32_t* lm_col = a.take<int32_t>(h.L);
+    double* lmd = a.take<double>((size_t)LM_PLANES * h.L);
+    int32_t* ob_fc = a.take<int32_t>(h.F);
+    double* ob = a.take<double>((size_t)OB_PLANES * h.F);
+    d.cam = cam; d.lm_fc = lm_fc; d.lm_off = lm_off; d.lm_col = lm_col; d.lmd = lmd; d.ob_fc = ob_fc; d.ob = ob;
+    d.fac_u = a.take<double>((size_t)6 * h.F);
+    d.lm_out = a.take<double>((size_t)LO_PLANES * h.L);
+    d.lm_delta = a.take<double>(h.L);
+    int32_t* imu_from = a.take<int32_t>(h.E);
+    int32_t* imu_to = a.take<int32_t>(h.E);
+    double* imu = a.take<double>((size_t)VIB_IMU_STRIDE * h.E);
+    d.imu_from = imu_from; d.imu_to = imu_to; d.imu = imu;
+    d.prior_rows = h.prior_rows; d.prior_cols = h.prior_cols; d.prior_blocks = (int)h.pblk.size();
+    PriorBlockDev* pblk = a.take<PriorBlockDev>(h.pblk.size());
+    double* pA = a.take<double>((size_t)h.prior_rows * h.prior_cols);
+    double* pr0 = a.take<double>(h.prior_rows);
+    double* pG = a.take<double>((size_t)h.prior_cols * h.prior_cols);
+    double* pref = a.take<double>(h.prior_ref.size());
+    int32_t* pfree = a.take<int32_t>(h.pfree.size());
+    d.pblk = pblk; d.pA = pA; d.pr0 = pr0; d.pG = pG; d.pref = pref; d.pfree = pfree;
+    d.prior_free = (int)h.pfree.size();
+    const size_t nr = h.nr;
+    d.Hret = a.take<double>(nr * nr); d.gret = a.take<double>(nr);
+    d.Hfull = a.take<double>(nr * nr); d.gfull = a.take<double>(nr);
+    d.S = a.take<double>(nr * nr); d.Scopy = a.take<double>(nr * nr);
+    d.rhs = a.take<double>(nr); d.dstate = a.take<double>(nr); d.scratch = a.take<double>(4 * nr);
+    const size_t len = d.tri + h.nv, nch = h.chunks.size();
+    d.lin_part = a.take<double>(nch * len); d.schur_part = a.take<double>(nch * len);
+    d.chunk_sc = a.take<double>(nch * CS_PLANES);
+    d.nchunks = (int)nch;
+    reduced_lin = a.take<double>(len); reduced_schur = a.take<double>(len);
+    d.lm = a.take<LmState>(1);
+    if (a.used > bytes) { err = "arena overflow"; return false; }
+    auto up = [&](void* dst, const void* src, size_t n) {
+        return n == 0 || hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
+    };
+    std::vector<int32_t> fc(h.L);
+    for (int l = 0; l < h.L; ++l) fc[l] = (h.lm_frame[l] << 8) | h.lm_cam[l];
+    bool ok = up(d.pose, h.pose.data(), 8 * h.pose.size()) && up(d.sb, h.sb.data(), 8 * h.sb.size()) &&
+              up(d.rig, h.rig, 56) && up(d.td, &h.td, 8) && up(d.rho, h.rho.data(), 8 * h.rho.size()) &&
+              up(cam, h.cam.data(), 8 * h.cam.size()) && up(lm_fc, fc.data(), 4 * fc.size()) &&
+              up(lm_off, h.lm_off.data(), 4 * h.lm_off.size()) && up(lm_col, h.lm_col.data(), 4 * h.lm_col.size()) &&
+              up(lmd, h.lmd.data(), 8 * h.lmd.size()) && up(ob_fc, h.ob_fc.data(), 4 * h.ob_fc.size()) &&
+              up(ob, h.ob.data(), 8 * h.ob.size()) && up(imu_from, h.imu_from.data(), 4 * h.E) &&
+              up(imu_to, h.imu_to.data(), 4 * h.E) && up(imu, h.imu.data(), 8 * h.imu.size()) &&
+              up(pblk, h.pblk.data(), sizeof(PriorBlockDev) * h.pblk.size()) &&
+              up(pA, h.prior_A.data(), 8 * h.prior_A.size()) && up(pr0, h.prior_r0.data(), 8 * h.prior_r0.size()) &&
+              up(pref, h.prior_ref.data(), 8 * h.prior_ref.size()) &&
+              up(pfree, h.pfree.data(), 4 * h.pfree.size());
+    // candidate starts equal to the state
+    ok = ok && up(d.cpose, h.pose.data(), 8 * h.pose.size()) && up(d.csb, h.sb.data(), 8 * h.sb.size()) &&
+         up(d.crig, h.rig, 56) && up(d.ctd, &h.td, 8) && up(d.crho, h.rho.data(), 8 * h.rho.size());
+    LmState lm{};
+    lm.lambda = 1e-4; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1;
+    ok = ok && up(d.lm, &lm, sizeof(lm));
+    if (!ok) { err = "upload failed"; return false; }
+    if (h.prior_rows) {
+        // G = A^T A once per prior (one-window launch)
+        WinDev* dw = nullptr;
+        double** gp = nullptr;
+        if (hipMalloc(&dw, sizeof(WinDev)) != hipSuccess || hipMalloc(&gp, sizeof(double*)) != hipSuccess) {
+            err = "device allocation failed"; return false;
+        }
+        ok = up(dw, &d, sizeof(WinDev)) && up(gp, &pG, sizeof(double*)) &&
+             vibk_prior_gram(dw, 1, gp, stream) == hipSuccess && hipStreamSynchronize(stream) == hipSuccess;
+        hipFree(dw);
+        hipFree(gp);
+        if (!ok) { err = "prior Gram matrix failed"; return false; }
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess) { err = "upload synchronization failed"; return false; }
+    used = true;
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------
+// Batch: device copies of the WinDev descriptors and the concatenated chunk list.
+// ------------------------------------------------------------------------------------------
+Batch::~Batch() {
+    if (d_wins) hipFree(d_wins);
+    if (d_chunks) hipFree(d_chunks);
+    if (d_red) hipFree(d_red);
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    if (graph) hipGraphDestroy(graph);
+}
+
+bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err) {
+    nwin = (int)slots.size();
+    std::vector<WinDev> wins(nwin);
+    std::vector<Chunk> all;
+    std::vector<double*> red(2 * nwin);
+    max_len = 1;
+    ret_shared = 8;
+    for (int i = 0; i < nwin; ++i) {
+        Slot& s = *slots[i];
+        wins[i] = s.dev;
+        for (Chunk c : s.host.chunks) { c.win = i; all.push_back(c); }
+        red[2 * i] = s.reduced_lin;
+        red[2 * i + 1] = s.reduced_schur;
+        max_len = std::max(max_len, wins[i].tri + wins[i].nv);
+        ret_shared = std::max<size_t>(ret_shared, vibk_retained_shared_bytes(s.host.E, s.host.prior_rows, s.host.prior_cols,
+                                                                          (int)s.host.pblk.size()));
+    }
+    nchunks = (int)all.size();
+    if (hipMalloc(&d_wins, sizeof(WinDev) * nwin) != hipSuccess ||
+        hipMalloc(&d_chunks, sizeof(Chunk) * std::max(1, nchunks)) != hipSuccess ||
+        hipMalloc(&d_red, sizeof(double*) * 2 * nwin) != hipSuccess) { err = "batch allocation failed"; return false; }
+    if (hipMemcpyAsync(d_wins, wins.data(), sizeof(WinDev) * nwin, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        (nchunks && hipMemcpyAsync(d_chunks, all.data(), sizeof(Chunk) * nchunks, hipMemcpyHostToDevice, stream) != hipSuccess) ||
+        hipMemcpyAsync(d_red, red.data(), sizeof(double*) * 2 * nwin, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) { err = "batch upload failed"; return false; }
+    return true;
+}
+
+bool Batch::linearize(hipStream_t s) {
+    return vibk_linearize(d_wins, nwin, d_chunks, nchunks, ret_shared, s) == hipSuccess &&
+           vibk_reduce(d_wins, nwin, 0, d_red, max_len, s) == hipSuccess;
+}
+
+bool Batch::step(hipStream_t s, int refinement, int mode) {
+    return vibk_schur(d_wins, d_chunks, nchunks, s) == hipSuccess &&
+           vibk_reduce(d_wins, nwin, 1, d_red, max_len, s) == hipSuccess &&
+           vibk_assemble(d_wins, nwin, d_red, s) == hipSuccess &&
+           vibk_cholesky(d_wins, nwin, refinement, s) == hipSuccess &&
+           vibk_backsub_trial(d_wins, d_chunks, nchunks, s) == hipSuccess &&
+           vibk_lm_update(d_wins, nwin, mode, ret_shared, s) == hipSuccess;
+}
+
+bool Batch::iteration(hipStream_t s, int refinement) { return linearize(s) && step(s, refinement, 0); }
+
+bool Batch::capture(hipStream_t s, int refinement) {
+    if (graph_exec) return true;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+    const bool ok = iteration(s, refinement);
+    if (hipStreamEndCapture(s, &graph) != hipSuccess || !ok) return false;
+    return hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0) == hipSuccess;
+}
+
+}  // namespace vib
+
+// ============================================================================================
+// C-ABI
+// ============================================================================================
+using namespace vib;
+
+struct vib_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    std::map<int, std::unique_ptr<Slot>> slots;
+    std::mutex mutex;
+    bool use_graph = true;
+};
+
+namespace {
+
+int fail(vib_context* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->error = msg;
+    return code;
+}
+
+Slot* find_slot(vib_context* ctx, int slot) {
+    auto it = ctx->slots.find(slot);
+    return it == ctx->slots.end() || !it->second->used ? nullptr : it->second.get();
+}
+
+int status_from_lm(int s) {
+    if (s >= 0) return VIB_OK;
+    return -s;
+}
+
+const char* status_message(int code) {
+    switch (code) {
+        case VIB_ERR_INVALID_RESIDUAL: return "invalid residual during linearization";
+        case VIB_ERR_NOT_POSITIVE_DEFINITE: return "device Schur step failed: damped reduced system not positive definite";
+        case VIB_ERR_COST_MISMATCH: return "joint cost disagrees with accepted trial";
+        case VIB_ERR_NO_ACCEPTABLE_STEP: return "no acceptable device Schur trial";
+        default: return "device solve failed";
+    }
+}
+
+bool read_lm(vib_context* ctx, Slot& s, LmState& lm) {
+    return hipMemcpyAsync(&lm, s.dev.lm, sizeof(LmState), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+           hipStreamSynchronize(ctx->stream) == hipSuccess;
+}
+
+bool write_lm(vib_context* ctx, Slot& s, const LmState& lm) {
+    return hipMemcpyAsync(s.dev.lm, &lm, sizeof(LmState), hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
+}
+
+// Runs the LM loop for a set of slots. All windows iterate in lock-step; finished windows are
+// skipped on the device. Returns the per-window reports.
+int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options& o, std::vector<vib_solve_report>& reps) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (o.precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    const int maxit = std::max(1, o.max_iterations);
+    for (Slot* s : slots) {
+        LmState lm{};
+        lm.lambda = o.initial_lambda > 0 ? o.initial_lambda : 1e-4;
+        lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = maxit;
+        if (!write_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
+    }
+    Batch batch;
+    std::string err;
+    if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    const bool graph = ctx->use_graph && batch.capture(ctx->stream, o.refinement_steps);
+    std::vector<LmState> states(slots.size());
+    for (int it = 0; it < maxit; ++it) {
+        bool ok;
+        if (graph) ok = hipGraphLaunch(batch.graph_exec, ctx->stream) == hipSuccess;
+        else ok = batch.iteration(ctx->stream, o.refinement_steps);
+        if (!ok) return fail(ctx, VIB_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+        // all done?
+        bool running = false;
+        for (size_t i = 0; i < slots.size(); ++i) {
+            if (hipMemcpyAsync(&states[i], slots[i]->dev.lm, sizeof(LmState), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+                return fail(ctx, VIB_ERR_DEVICE, "LM state read failed");
+        }
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+            return fail(ctx, VIB_ERR_DEVICE, std::string("device failure: ") + hipGetErrorString(hipGetLastError()));
+        for (auto& s : states) running = running || s.status == 0;
+        if (!running) break;
+        if (o.time_limit_seconds > 0 && it + 1 >= o.min_iterations &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= o.time_limit_seconds)
+            break;
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    reps.assign(slots.size(), vib_solve_report{});
+    int first_error = VIB_OK;
+    for (size_t i = 0; i < slots.size(); ++i) {
+        const LmState& s = states[i];
+        vib_solve_report& r = reps[i];
+        int st = status_from_lm(s.status);
+        if (s.status == 0 && !s.accepted && !s.converged) st = VIB_ERR_NO_ACCEPTABLE_STEP;
+        r.status = st;
+        r.iterations = s.iteration;
+        r.accepted_steps = s.accepted;
+        r.converged = s.converged;
+        r.effective_parameters = slots[i]->host.nr + slots[i]->host.ne;
+        r.retained_dimension = slots[i]->host.nr;
+        r.initial_cost = s.initial_cost;
+        r.final_cost = s.cost;
+        r.final_lambda = s.lambda;
+        r.seconds = secs;
+        if (st != VIB_OK && first_error == VIB_OK) first_error = st;
+    }
+    if (first_error != VIB_OK) ctx->error = status_message(first_error);
+    return first_error;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vib_abi_version(void) { return VIB_ABI_VERSION; }
+
+int vib_create(vib_context** out, int device) {
+    if (!out) return VIB_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return VIB_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return VIB_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return VIB_ERR_DEVICE;
+    auto* ctx = new vib_context();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return VIB_ERR_DEVICE;
+    }
+    const char* g = std::getenv("VIB_NO_GRAPH");
+    ctx->use_graph = !(g && g[0] == '1');
+    *out = ctx;
+    return VIB_OK;
+}
+
+int vib_destroy(vib_context* ctx) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    ctx->slots.clear();
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return VIB_OK;
+}
+
+const char* vib_last_error(const vib_context* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int64_t vib_window_footprint(const vib_window* w, int32_t precision) {
+    if (!w) return -1;
+    HostWindow h;
+    std::string err;
+    if (!h.build(*w, err)) return -1;
+    (void)precision;
+    return (int64_t)Slot::footprint(h);
+}
+
+int vib_window_upload(vib_context* ctx, int32_t slot, const vib_window* w) {
+    if (!ctx || !w || slot < 0) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    auto h = std::make_unique<HostWindow>();
+    std::string err;
+    if (!h->build(*w, err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
+    auto& s = ctx->slots[slot];
+    if (!s) s = std::make_unique<Slot>();
+    if (!s->upload(*h, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    return VIB_OK;
+}
+
+int vib_window_release(vib_context* ctx, int32_t slot) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    auto it = ctx->slots.find(slot);
+    if (it == ctx->slots.end()) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    ctx->slots.erase(it);
+    return VIB_OK;
+}
+
+int vib_read_state(vib_context* ctx, int32_t slot, vib_window* w) {
+    if (!ctx || !w) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    const WinDev& d = s->dev;
+    if (w->num_frames != d.K || w->num_landmarks != d.L) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "window shape mismatch");
+    bool ok = hipMemcpyAsync(w->pose, d.pose, 56 * d.K, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              hipMemcpyAsync(w->speed_bias, d.sb, 72 * d.K, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              hipMemcpyAsync(w->rig, d.rig, 56, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              hipMemcpyAsync(w->td, d.td, 8, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              (d.L == 0 || hipMemcpyAsync(w->lm_rho, d.rho, 8 * d.L, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
+              hipStreamSynchronize(ctx->stream) == hipSuccess;
+    return ok ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "state download failed");
+}
+
+int vib_window_shape(vib_context* ctx, int32_t slot, int32_t* frames, int32_t* landmarks, int32_t* observations,
+                     int32_t* imu, int32_t* prior_rows, int32_t* prior_cols) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    if (frames) *frames = s->host.K;
+    if (landmarks) *landmarks = s->host.L;
+    if (observations) *observations = s->host.F;
+    if (imu) *imu = s->host.E;
+    if (prior_rows) *prior_rows = s->host.prior_rows;
+    if (prior_cols) *prior_cols = s->host.prior_cols;
+    return VIB_OK;
+}
+
+int vib_retained_dimension(vib_context* ctx, int32_t slot, int32_t* nr, int32_t* nv) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    if (nr) *nr = s->host.nr;
+    if (nv) *nv = s->host.nv;
+    return VIB_OK;
+}
+
+int vib_linearize(vib_context* ctx, int32_t slot, int32_t precision) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    if (precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    LmState lm;
+    if (!read_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state read failed");
+    lm.new_point = 1; lm.status = 0; lm.iteration = 0;
+    std::vector<Slot*> one{s};
+    Batch b;
+    std::string err;
+    if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, err) || !b.linearize(ctx->stream) ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(ctx, VIB_ERR_DEVICE, "linearization launch failed");
+    return VIB_OK;
+}
+
+int vib_solve_step(vib_context* ctx, int32_t slot, double lambda, int32_t precision, int32_t refinement,
+                   vib_step_report* rep) {
+    if (!ctx || !rep || !(lambda > 0) || !std::isfinite(lambda)) return VIB_ERR_INVALID_ARGUMENT;
+    if (precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    *rep = vib_step_report{};
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    LmState lm;
+    if (!read_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state read failed");
+    lm.lambda = lambda; lm.status = 0; lm.iteration = 0;
+    std::vector<Slot*> one{s};
+    Batch b;
+    std::string err;
+    if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, err) || !b.step(ctx->stream, refinement, 1) ||
+        !read_lm(ctx, *s, lm))
+        return fail(ctx, VIB_ERR_DEVICE, "step launch failed");
+    rep->cost = lm.cost;
+    rep->predicted_reduction = lm.predicted;
+    rep->trial_cost = lm.trial_cost;
+    rep->max_increment = lm.max_increment;
+    rep->trial_valid = lm.trial_valid;
+    rep->status = status_from_lm(lm.status);
+    if (rep->status != VIB_OK) return fail(ctx, rep->status, status_message(rep->status));
+    return VIB_OK;
+}
+
+int vib_accept(vib_context* ctx, int32_t slot) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    std::vector<Slot*> one{s};
+    Batch b;
+    std::string err;
+    if (!b.build(one, ctx->stream, err) || vibk_lm_update(b.d_wins, 1, 2, b.ret_shared, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(ctx, VIB_ERR_DEVICE, "accept failed");
+    return VIB_OK;
+}
+
+int vib_read_step(vib_context* ctx, int32_t slot, double* delta, int32_t capacity) {
+    if (!ctx || !delta) return VIB_ERR_INVALID_ARGUMENT;
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    const HostWindow& h = s->host;
+    if (capacity < h.nr + h.ne) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "capacity too small");
+    std::vector<double> dl(h.L);
+    bool ok = hipMemcpyAsync(delta, s->dev.dstate, 8 * h.nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              (h.L == 0 || hipMemcpyAsync(dl.data(), s->dev.lm_delta, 8 * h.L, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
+              hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "step download failed");
+    for (int l = 0; l < h.L; ++l) if (h.lm_col[l] >= 0) delta[h.lm_col[l]] = dl[l];
+    return VIB_OK;
+}
+
+int vib_read_reduced(vib_context* ctx, int32_t slot, double* S, double* rhs, int32_t capacity) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    const int nr = s->host.nr;
+    if (capacity < nr) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "capacity too small");
+    bool ok = (!S || hipMemcpyAsync(S, s->dev.Scopy, 8 * (size_t)nr * nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
+              (!rhs || hipMemcpyAsync(rhs, s->dev.rhs, 8 * nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
+              hipStreamSynchronize(ctx->stream) == hipSuccess;
+    return ok ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "download failed");
+}
+
+int vib_read_normal(vib_context* ctx, int32_t slot, double* H, double* g, double* cost, int32_t capacity) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    const int nr = s->host.nr;
+    if (capacity < nr) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "capacity too small");
+    LmState lm;
+    bool ok = (!H || hipMemcpyAsync(H, s->dev.Hfull, 8 * (size_t)nr * nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
+              (!g || hipMemcpyAsync(g, s->dev.gfull, 8 * nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
+              read_lm(ctx, *s, lm);
+    if (cost) *cost = lm.lin_cost;
+    return ok ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "download failed");
+}
+
+int vib_solve(vib_context* ctx, int32_t slot, const vib_solve_options* o, vib_solve_report* rep) {
+    if (!ctx || !o || !rep) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    std::vector<Slot*> one{s};
+    std::vector<vib_solve_report> reps;
+    const int st = run_lm(ctx, one, *o, reps);
+    if (!reps.empty()) *rep = reps[0];
+    else rep->status = st;
+    return st;
+}
+
+int vib_solve_batch(vib_context* ctx, int32_t count, const int32_t* ids, const vib_solve_options* o,
+                    vib_solve_report* reports) {
+    if (!ctx || !o || !ids || !reports || count <= 0) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    std::vector<Slot*> slots;
+    for (int i = 0; i < count; ++i) {
+        Slot* s = find_slot(ctx, ids[i]);
+        if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+        slots.push_back(s);
+    }
+    std::vector<vib_solve_report> reps;
+    const int st = run_lm(ctx, slots, *o, reps);
+    for (size_t i = 0; i < reps.size(); ++i) reports[i] = reps[i];
+    return st;
+}
+
+int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* o, vib_solve_report* rep) {
+    (void)slot; (void)o; (void)rep;
+    return fail(ctx, VIB_ERR_UNSUPPORTED, "two-frame solve not available in this build");
+}
+
+int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double* samples, int32_t count,
+                    const vib_imu_params* params, vib_marginal_report* rep) {
+    (void)slot; (void)policy; (void)samples; (void)count; (void)params; (void)rep;
+    return fail(ctx, VIB_ERR_UNSUPPORTED, "marginalization not available in this build");
+}
+
+int vib_read_prior(vib_context* ctx, int32_t slot, int32_t* rows, int32_t* cols, int32_t* blocks, int32_t* kind,
+                   int64_t* id, double* ref, double* A, double* r0, int32_t cap_rows, int32_t cap_cols, int32_t cap_blocks) {
+    if (!ctx || !rows || !cols || !blocks) return VIB_ERR_INVALID_ARGUMENT;
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    const HostWindow& h = s->host;
+    *rows = h.prior_rows; *cols = h.prior_cols; *blocks = (int)h.prior_kind.size();
+    if (h.prior_rows > cap_rows || h.prior_cols > cap_cols || (int)h.prior_kind.size() > cap_blocks)
+        return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "capacity too small");
+    for (size_t b = 0; b < h.prior_kind.size(); ++b) {
+        if (kind) kind[b] = h.prior_kind[b];
+        if (id) id[b] = h.prior_id[b];
+    }
+    if (ref) std::copy(h.prior_ref.begin(), h.prior_ref.end(), ref);
+    if (A) std::copy(h.prior_A.begin(), h.prior_A.end(), A);
+    if (r0) std::copy(h.prior_r0.begin(), h.prior_r0.end(), r0);
+    return VIB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Host-side window bookkeeping of the Schur backend (topology, layout, device slots, batches).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vib.h"
+#include "vib_device.hpp"
+
+namespace vib {
+
+constexpr int kMaxFactorsPerLandmark = 13;
+
+// Host copy of one window: observation streams sorted by landmark, tangent column layout
+// [pose (6, free) ..., rig (6), td (1), speed/bias (width) ...] then free inverse ranges,
+// prior block table and the chunk work list.
+struct HostWindow {
+    static int chunk_size;  // landmarks per chunk workgroup
+    int K = 0, C = 0, L = 0, F = 0, E = 0;
+    uint32_t flags = 0;
+    double huber = 1, min_rho = 1e-8, td_lo = -1e300, td_hi = 1e300;
+    std::vector<int64_t> frame_id;
+    std::vector<double> pose, sb;
+    std::vector<uint8_t> pose_free, sb_width;
+    double rig[7]{}, td = 0;
+    std::vector<double> cam;  // C x 12
+    std::vector<int> lm_frame, lm_cam, lm_off;
+    std::vector<uint8_t> lm_free;
+    std::vector<double> lmd, rho;
+    std::vector<int> ob_fc, ob_lm;
+    std::vector<double> ob;
+    std::vector<int> imu_from, imu_to;
+    std::vector<double> imu;
+    int prior_rows = 0, prior_cols = 0;
+    std::vector<int> prior_kind;
+    std::vector<int64_t> prior_id;
+    std::vector<double> prior_ref, prior_A, prior_r0;
+    // layout
+    std::vector<int> pose_col, sb_col, lm_col;
+    int rig_col = -1, td_col = -1, nv = 0, nr = 0, ne = 0;
+    std::vector<PriorBlockDev> pblk;
+    std::vector<int32_t> pfree;
+    std::vector<Chunk> chunks;
+
+    bool build(const vib_window& w, std::string& err);
+    bool layout(std::string& err);
+};
+
+// One device-resident window (a session's window and prior live here across solves).
+struct Slot {
+    bool used = false;
+    HostWindow host;
+    WinDev dev{};
+    void* arena = nullptr;
+    size_t bytes = 0;
+    double* reduced_lin = nullptr;
+    double* reduced_schur = nullptr;
+
+    ~Slot();
+    void release();
+    static size_t footprint(const HostWindow& h);
+    bool upload(const HostWindow& h, hipStream_t stream, std::string& err);
+};
+
+// Descriptors of a set of windows solved together.
+struct Batch {
+    int nwin = 0, nchunks = 0, max_len = 1;
+    size_t ret_shared = 8;
+    WinDev* d_wins = nullptr;
+    Chunk* d_chunks = nullptr;
+    double** d_red = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    ~Batch();
+    bool build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err);
+    bool linearize(hipStream_t s);
+    bool step(hipStream_t s, int refinement, int mode);
+    bool iteration(hipStream_t s, int refinement);
+    bool capture(hipStream_t s, int refinement);
+};
+
+}  // namespace vib

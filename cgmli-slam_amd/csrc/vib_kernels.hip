@@ -1,0 +1,1390 @@
+// HIP kernels of the sliding-window Schur backend for gfx950 (CDNA4), FP64 path.
+//
+// One LM iteration over a batch of windows is the launch sequence
+//   k_lin_visual   (chunks)  : S^2 factors -> visual normal block partials, per-factor
+//                              J_a^T J_rho, per-landmark (d, g_rho, c_rig, c_td), cost
+//   k_lin_retained (windows) : IMU + prior rows -> dense retained normal matrix and cost
+//   k_reduce       (slices)  : fixed-order sum of the chunk partials
+//   k_schur        (chunks)  : -c c^T / (d + lambda max(d,1e-6)) partials
+//   k_assemble     (windows) : damped reduced camera system + rhs
+//   k_cholesky     (windows) : FP64 Cholesky, triangular solves, refinement, state trial
+//   k_backsub      (chunks)  : inverse-range back-substitution, model decrease, rho trial
+//   k_trial_cost   (chunks)  : robust cost at the trial state
+//   k_lm_update    (windows) : retained trial cost, gain, lambda schedule, accept
+// mirroring VisualCudaBatch::linearizeSchur / solveOnline / launchTrial
+// (src/backend/VisualCudaBatch.cu:433-604,715-721) and the host LM of
+// src/backend/BackendOptimizer.cpp:167-307, with no atomics to global memory.
+#include <hip/hip_runtime.h>
+
+#include "vib_device.hpp"
+#include "vib_math.hpp"
+
+namespace vib {
+
+// ---------------------------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int tri_index(int i, int j, int nv) {  // i <= j, packed upper rows
+    return i * nv - (i * (i - 1)) / 2 + (j - i);
+}
+
+// Deterministic block reductions (fixed tree order).
+__device__ double block_sum(double v, double* red) {
+    const int t = threadIdx.x, n = blockDim.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = n / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+__device__ double block_max(double v, double* red) {
+    const int t = threadIdx.x, n = blockDim.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = n / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] = fmax(red[t], red[t + s]);
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Orders LDS traffic between the lanes of one wavefront (single-wave phases of a workgroup).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Sum over the kChunkLanes lanes of a landmark group (xor tree, deterministic).
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int m = kChunkLanes / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, kChunkLanes);
+    return v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Frame/camera table (LDS): world rotation/translation of every (frame, camera).
+// ---------------------------------------------------------------------------------------------
+struct FrameTable {
+    double Rb[kMaxFrames][9];
+    double pb[kMaxFrames][3];
+    double Rc[kMaxFrames][kMaxCameras][9];
+    double tc[kMaxFrames][kMaxCameras][3];
+    int valid[kMaxFrames];  // finite and |q|^2 > 1e-16
+    int unit[kMaxFrames];   // ||q|^2 - 1| <= 1e-6 (required for a pose Jacobian)
+    int rig_valid, rig_unit;
+};
+
+__device__ void build_frame_table(const WinDev& w, const double* pose, const double* rig, FrameTable& ft) {
+    const int t = threadIdx.x;
+    const double* qe = rig + 3;
+    if (t == 0) {
+        bool ok = true;
+        for (int i = 0; i < 7; ++i) ok = ok && isfinite(rig[i]);
+        const double n2 = qe[0] * qe[0] + qe[1] * qe[1] + qe[2] * qe[2] + qe[3] * qe[3];
+        ft.rig_valid = ok && n2 > 1e-16;
+        ft.rig_unit = isfinite(n2) && fabs(n2 - 1) <= 1e-6;
+    }
+    if (t < w.K) {
+        const double* x = pose + 7 * t;
+        bool ok = true;
+        for (int i = 0; i < 7; ++i) ok = ok && isfinite(x[i]);
+        const double n2 = x[3] * x[3] + x[4] * x[4] + x[5] * x[5] + x[6] * x[6];
+        ft.valid[t] = ok && n2 > 1e-16;
+        ft.unit[t] = isfinite(n2) && fabs(n2 - 1) <= 1e-6;
+        const M3<double> R = quat_to_matrix(x + 3);
+        for (int i = 0; i < 9; ++i) ft.Rb[t][i] = R.m[i];
+        ft.pb[t][0] = x[0]; ft.pb[t][1] = x[1]; ft.pb[t][2] = x[2];
+    }
+    if (t < w.K * w.C) {
+        const int k = t / w.C, c = t % w.C;
+        const double* x = pose + 7 * k;
+        const M3<double> Rb = quat_to_matrix(x + 3);
+        const M3<double> Re = quat_to_matrix(qe);
+        M3<double> Rcam;
+        for (int i = 0; i < 9; ++i) Rcam.m[i] = w.cam[12 * c + i];
+        const V3<double> tcam = load3(w.cam + 12 * c + 9);
+        const M3<double> Rwc = mul(mul(Rb, Re), Rcam);
+        const V3<double> twc = mul(Rb, mul(Re, tcam) + load3(rig)) + load3(x);
+        for (int i = 0; i < 9; ++i) ft.Rc[k][c][i] = Rwc.m[i];
+        ft.tc[k][c][0] = twc.x; ft.tc[k][c][1] = twc.y; ft.tc[k][c][2] = twc.z;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// One S^2 visual factor (VisualSphereBatchMath.hpp:67-174) in tangent coordinates.
+// ---------------------------------------------------------------------------------------------
+struct LmData {
+    V3<double> raw, vel;
+    double tref, rho;
+    int frame, cam;
+};
+struct FactorJ {
+    double r[2];
+    double Ja[12];   // 2 x 6 anchor tangent (observer = -Ja)
+    double Jr[12];   // 2 x 6 rig tangent
+    double Jt[2];
+    double Jrho[2];
+    double w;        // Huber weight
+};
+
+template <bool WANT_J>
+__device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData& lm, int f, double td,
+                            bool want_td, FactorJ& o) {
+    const int fc = w.ob_fc[f];
+    const int b = fc >> 8, cb = fc & 255;
+    const int a = lm.frame, ca = lm.cam;
+    const double rho = lm.rho;
+    if (!(rho > 0) || !isfinite(rho) || !isfinite(td)) return false;
+    if (!ft.valid[a] || !ft.valid[b] || !ft.rig_valid) return false;
+    const bool same = a == b;
+    const double* ob = w.ob;
+    const size_t F = (size_t)w.F;
+    const V3<double> braw = {ob[OB_BX * F + f], ob[OB_BY * F + f], ob[OB_BZ * F + f]};
+    const V3<double> bvel = {ob[OB_VX * F + f], ob[OB_VY * F + f], ob[OB_VZ * F + f]};
+    const double btref = ob[OB_T * F + f];
+    const double s00 = ob[OB_S00 * F + f], s10 = ob[OB_S10 * F + f], s11 = ob[OB_S11 * F + f];
+    const bool movingA = sqnorm(lm.vel) > 0, movingB = sqnorm(bvel) > 0;
+    const bool tdd = WANT_J && want_td;
+    V3<double> da = {0, 0, 0}, db = {0, 0, 0}, ba, bb;
+    if (movingA) ba = time_bearing(lm.raw, lm.vel, lm.tref - td, tdd, da);
+    else { const double n = sqrt(sqnorm(lm.raw)); ba = {lm.raw.x / n, lm.raw.y / n, lm.raw.z / n}; }
+    if (movingB) bb = time_bearing(braw, bvel, btref - td, tdd, db);
+    else { const double n = sqrt(sqnorm(braw)); bb = {braw.x / n, braw.y / n, braw.z / n}; }
+    if (!isfinite(ba.x) || !isfinite(ba.y) || !isfinite(ba.z) || !isfinite(bb.x) || !isfinite(bb.y) || !isfinite(bb.z))
+        return false;
+    // camera chain
+    M3<double> Rwa, Rwb;
+    for (int i = 0; i < 9; ++i) { Rwa.m[i] = ft.Rc[a][ca][i]; Rwb.m[i] = ft.Rc[b][cb][i]; }
+    const V3<double> twa = {ft.tc[a][ca][0], ft.tc[a][ca][1], ft.tc[a][ca][2]};
+    const V3<double> twb = {ft.tc[b][cb][0], ft.tc[b][cb][1], ft.tc[b][cb][2]};
+    const V3<double> pw = mul(Rwa, (1.0 / rho) * ba) + twa;
+    const V3<double> pc = mulT(Rwb, pw - twb);
+    // measurement projection: P = S B^T (transported when the bearing moves)
+    V3<double> c0, c1;
+    sphere_basis(braw, c0, c1);
+    double M[6] = {s00 * c0.x, s00 * c0.y, s00 * c0.z,
+                   s10 * c0.x + s11 * c1.x, s10 * c0.y + s11 * c1.y, s10 * c0.z + s11 * c1.z};
+    double P[6];
+    double den = 2;
+    double K[9];
+    if (movingB) {
+        den = 1 + dot(braw, bb);
+        const V3<double> k = cross(braw, bb);
+        K[0] = 0; K[1] = -k.z; K[2] = k.y; K[3] = k.z; K[4] = 0; K[5] = -k.x; K[6] = -k.y; K[7] = k.x; K[8] = 0;
+        double Tr[9];  // transport = I + K + K K / den
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double kk = K[r * 3] * K[c] + K[r * 3 + 1] * K[3 + c] + K[r * 3 + 2] * K[6 + c];
+                Tr[r * 3 + c] = (r == c ? 1.0 : 0.0) + K[r * 3 + c] + kk / den;
+            }
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < 3; ++c)  // P = M * Tr^T
+                P[r * 3 + c] = M[r * 3] * Tr[c * 3] + M[r * 3 + 1] * Tr[c * 3 + 1] + M[r * 3 + 2] * Tr[c * 3 + 2];
+    } else {
+        for (int i = 0; i < 6; ++i) P[i] = M[i];
+    }
+    SphereEval<double> e;
+    if (!sphere_residual(pc, bb, P, den, WANT_J, e)) return false;
+    o.r[0] = e.r[0]; o.r[1] = e.r[1];
+    const double sq = e.r[0] * e.r[0] + e.r[1] * e.r[1];
+    o.w = huber_weight(w.huber, sq);
+    if (!WANT_J) return true;
+    const double wt = o.w;
+    // D = reduce * Rwb^T (2 x 3)
+    double D[6];
+    for (int r = 0; r < 2; ++r)
+        for (int k = 0; k < 3; ++k)
+            D[r * 3 + k] = e.reduce[r * 3] * Rwb.m[k * 3] + e.reduce[r * 3 + 1] * Rwb.m[k * 3 + 1] +
+                           e.reduce[r * 3 + 2] * Rwb.m[k * 3 + 2];
+    // anchor pose: [D, -D [p_w]x] ; rows: [D_r, p_w x D_r]
+    const int ac = w.pose_col[a], bc = w.pose_col[b];
+    if (same) {
+        for (int i = 0; i < 12; ++i) { o.Ja[i] = 0; o.Jr[i] = 0; }
+    } else {
+        if ((ac >= 0 && !ft.unit[a]) || (bc >= 0 && !ft.unit[b]) || (w.rig_col >= 0 && !ft.rig_unit)) return false;
+        for (int r = 0; r < 2; ++r) {
+            const V3<double> Dr = {D[r * 3], D[r * 3 + 1], D[r * 3 + 2]};
+            const V3<double> rot = cross(pw, Dr);
+            o.Ja[r * 6 + 0] = wt * Dr.x; o.Ja[r * 6 + 1] = wt * Dr.y; o.Ja[r * 6 + 2] = wt * Dr.z;
+            o.Ja[r * 6 + 3] = wt * rot.x; o.Ja[r * 6 + 4] = wt * rot.y; o.Ja[r * 6 + 5] = wt * rot.z;
+        }
+        if (w.rig_col >= 0) {
+            M3<double> Ra, Rb;
+            for (int i = 0; i < 9; ++i) { Ra.m[i] = ft.Rb[a][i]; Rb.m[i] = ft.Rb[b][i]; }
+            const V3<double> pa = {ft.pb[a][0], ft.pb[a][1], ft.pb[a][2]};
+            const V3<double> pb = {ft.pb[b][0], ft.pb[b][1], ft.pb[b][2]};
+            const V3<double> body = mulT(Ra, pw - pa);   // point in the anchor body frame
+            const V3<double> q = mulT(Rb, pw - pb);      // point in the observer body frame
+            for (int r = 0; r < 2; ++r) {
+                const V3<double> Dr = {D[r * 3], D[r * 3 + 1], D[r * 3 + 2]};
+                const V3<double> DRa = mulT(Ra, Dr), DRb = mulT(Rb, Dr);  // rows of D R_a, D R_b
+                const V3<double> tr = DRa - DRb;
+                const V3<double> rot = cross(DRb, q) - cross(DRa, body);
+                o.Jr[r * 6 + 0] = wt * tr.x; o.Jr[r * 6 + 1] = wt * tr.y; o.Jr[r * 6 + 2] = wt * tr.z;
+                o.Jr[r * 6 + 3] = wt * rot.x; o.Jr[r * 6 + 4] = wt * rot.y; o.Jr[r * 6 + 5] = wt * rot.z;
+            }
+        }
+    }
+    // rho: D Rwa (-ba / rho^2)
+    {
+        const V3<double> v = mul(Rwa, (-1.0 / (rho * rho)) * ba);
+        o.Jrho[0] = wt * (D[0] * v.x + D[1] * v.y + D[2] * v.z);
+        o.Jrho[1] = wt * (D[3] * v.x + D[4] * v.y + D[5] * v.z);
+        if (!isfinite(o.Jrho[0]) || !isfinite(o.Jrho[1])) return false;
+    }
+    o.Jt[0] = o.Jt[1] = 0;
+    if (tdd && (movingA || movingB)) {
+        // J = D Rwa (da/rho) + P Jb db + M dR^T log  (VisualSphereBatchMath.hpp:162-171)
+        const V3<double> v = mul(Rwa, (1.0 / rho) * da);
+        double t1[2] = {D[0] * v.x + D[1] * v.y + D[2] * v.z, D[3] * v.x + D[4] * v.y + D[5] * v.z};
+        // Jb db = f(-bb (n.db) - c db) + g u (n.db)
+        const double ndb = dot(e.n, db);
+        const V3<double> jbdb = e.f * ((-ndb) * bb - e.c * db) + (e.g * ndb) * e.u;
+        double t2[2] = {P[0] * jbdb.x + P[1] * jbdb.y + P[2] * jbdb.z, P[3] * jbdb.x + P[4] * jbdb.y + P[5] * jbdb.z};
+        double t3[2] = {0, 0};
+        if (movingB) {
+            const V3<double> kd = cross(braw, db);
+            double dK[9] = {0, -kd.z, kd.y, kd.z, 0, -kd.x, -kd.y, kd.x, 0};
+            const double mdb = dot(braw, db);
+            double dR[9];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    double a1 = 0, a2 = 0, kk = 0;
+                    for (int m = 0; m < 3; ++m) {
+                        a1 += dK[r * 3 + m] * K[m * 3 + c];
+                        a2 += K[r * 3 + m] * dK[m * 3 + c];
+                        kk += K[r * 3 + m] * K[m * 3 + c];
+                    }
+                    dR[r * 3 + c] = dK[r * 3 + c] + (a1 + a2) / den - kk * (mdb / (den * den));
+                }
+            // dR^T log
+            const V3<double> lg = e.logv;
+            const V3<double> dl = {dR[0] * lg.x + dR[3] * lg.y + dR[6] * lg.z, dR[1] * lg.x + dR[4] * lg.y + dR[7] * lg.z,
+                                   dR[2] * lg.x + dR[5] * lg.y + dR[8] * lg.z};
+            t3[0] = M[0] * dl.x + M[1] * dl.y + M[2] * dl.z;
+            t3[1] = M[3] * dl.x + M[4] * dl.y + M[5] * dl.z;
+        }
+        o.Jt[0] = wt * (t1[0] + t2[0] + t3[0]);
+        o.Jt[1] = wt * (t1[1] + t2[1] + t3[1]);
+        if (!isfinite(o.Jt[0]) || !isfinite(o.Jt[1])) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ LmData load_landmark(const WinDev& w, const double* rho, int l) {
+    LmData d;
+    const size_t L = (size_t)w.L;
+    d.raw = {w.lmd[LM_BX * L + l], w.lmd[LM_BY * L + l], w.lmd[LM_BZ * L + l]};
+    d.vel = {w.lmd[LM_VX * L + l], w.lmd[LM_VY * L + l], w.lmd[LM_VZ * L + l]};
+    d.tref = w.lmd[LM_T * L + l];
+    d.rho = rho[l];
+    const int fc = w.lm_fc[l];
+    d.frame = fc >> 8;
+    d.cam = fc & 255;
+    return d;
+}
+
+// Add sign * (A^T B) for 2-row Jacobian blocks A (2 x na) at column ca and B (2 x nb) at cb into
+// the packed upper triangle `acc` (nv wide). ca == cb adds only the upper part of A^T A.
+__device__ __forceinline__ void acc_block(double* acc, int nv, int ca, const double* A, int na, int cb,
+                                          const double* B, int nb, double sign) {
+    if (ca < 0 || cb < 0) return;
+    if (ca == cb) {
+        for (int i = 0; i < na; ++i)
+            for (int j = i; j < nb; ++j)
+                atomicAdd(&acc[tri_index(ca + i, cb + j, nv)], sign * (A[i] * B[j] + A[na + i] * B[nb + j]));
+    } else if (ca < cb) {
+        for (int i = 0; i < na; ++i)
+            for (int j = 0; j < nb; ++j)
+                atomicAdd(&acc[tri_index(ca + i, cb + j, nv)], sign * (A[i] * B[j] + A[na + i] * B[nb + j]));
+    } else {
+        for (int j = 0; j < nb; ++j)
+            for (int i = 0; i < na; ++i)
+                atomicAdd(&acc[tri_index(cb + j, ca + i, nv)], sign * (A[i] * B[j] + A[na + i] * B[nb + j]));
+    }
+}
+
+__device__ __forceinline__ bool window_active(const WinDev& w) { return w.lm->status == 0; }
+
+constexpr int kAccSize = kMaxVisual * (kMaxVisual + 1) / 2 + kMaxVisual;
+
+// ---------------------------------------------------------------------------------------------
+// k_lin_visual: one chunk of landmarks per workgroup, 8 lanes per landmark.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* chunks) {
+    __shared__ double acc[kAccSize];
+    __shared__ FrameTable ft;
+    __shared__ double red[kBlock];
+    const Chunk ch = chunks[blockIdx.x];
+    const WinDev& w = wins[ch.win];
+    if (!window_active(w) || !w.lm->new_point) return;
+    const int nv = w.nv, tri = w.tri;
+    for (int i = threadIdx.x; i < tri + nv; i += kBlock) acc[i] = 0;
+    build_frame_table(w, w.pose, w.rig, ft);
+    __syncthreads();
+    double* gacc = acc + tri;
+    const double td = *w.td;
+    const bool want_td = w.td_col >= 0;
+    const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
+    constexpr int groups = kBlock / kChunkLanes;
+    double cost = 0;
+    int invalid = 0;
+    const size_t L = (size_t)w.L, F = (size_t)w.F;
+    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
+        const int l = base + group;
+        const bool live = l < ch.lm_end;
+        double d = 0, gr = 0, cr[6] = {0, 0, 0, 0, 0, 0}, ct = 0;
+        int col = -1;
+        if (live) {
+            const LmData lm = load_landmark(w, w.rho, l);
+            col = w.lm_col[l];
+            const int a = lm.frame;
+            const int ac = w.pose_col[a];
+            for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
+                FactorJ J;
+                if (!eval_factor<true>(w, ft, lm, f, td, want_td, J)) {
+                    invalid = 1;
+                    for (int k = 0; k < 6; ++k) w.fac_u[k * F + f] = 0;
+                    continue;
+                }
+                const double sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
+                if (!isfinite(sq)) { invalid = 1; continue; }
+                cost += huber_cost(w.huber, sq);
+                const double r0 = J.w * J.r[0], r1 = J.w * J.r[1];
+                const int b = w.ob_fc[f] >> 8;
+                const bool same = a == b;
+                const int bc = same ? -1 : w.pose_col[b];
+                const int acol = same ? -1 : ac;
+                // pose-pose blocks: J_o = -J_a
+                acc_block(acc, nv, acol, J.Ja, 6, acol, J.Ja, 6, 1.0);
+                acc_block(acc, nv, bc, J.Ja, 6, bc, J.Ja, 6, 1.0);
+                if (acol >= 0 && bc >= 0) acc_block(acc, nv, acol, J.Ja, 6, bc, J.Ja, 6, -1.0);
+                if (w.rig_col >= 0 && !same) {
+                    acc_block(acc, nv, acol, J.Ja, 6, w.rig_col, J.Jr, 6, 1.0);
+                    acc_block(acc, nv, bc, J.Ja, 6, w.rig_col, J.Jr, 6, -1.0);
+                    acc_block(acc, nv, w.rig_col, J.Jr, 6, w.rig_col, J.Jr, 6, 1.0);
+                }
+                if (want_td) {
+                    if (!same) {
+                        acc_block(acc, nv, acol, J.Ja, 6, w.td_col, J.Jt, 1, 1.0);
+                        acc_block(acc, nv, bc, J.Ja, 6, w.td_col, J.Jt, 1, -1.0);
+                        if (w.rig_col >= 0) acc_block(acc, nv, w.rig_col, J.Jr, 6, w.td_col, J.Jt, 1, 1.0);
+                    }
+                    acc_block(acc, nv, w.td_col, J.Jt, 1, w.td_col, J.Jt, 1, 1.0);
+                }
+                // gradient
+                for (int k = 0; k < 6; ++k) {
+                    const double ga = J.Ja[k] * r0 + J.Ja[6 + k] * r1;
+                    if (acol >= 0) atomicAdd(&gacc[acol + k], ga);
+                    if (bc >= 0) atomicAdd(&gacc[bc + k], -ga);
+                    if (w.rig_col >= 0 && !same) atomicAdd(&gacc[w.rig_col + k], J.Jr[k] * r0 + J.Jr[6 + k] * r1);
+                }
+                if (want_td) atomicAdd(&gacc[w.td_col], J.Jt[0] * r0 + J.Jt[1] * r1);
+                // landmark coupling pieces
+                double u[6];
+                for (int k = 0; k < 6; ++k) u[k] = J.Ja[k] * J.Jrho[0] + J.Ja[6 + k] * J.Jrho[1];
+                for (int k = 0; k < 6; ++k) w.fac_u[k * F + f] = same ? 0.0 : u[k];
+                d += J.Jrho[0] * J.Jrho[0] + J.Jrho[1] * J.Jrho[1];
+                gr += J.Jrho[0] * r0 + J.Jrho[1] * r1;
+                if (!same)
+                    for (int k = 0; k < 6; ++k) cr[k] += J.Jr[k] * J.Jrho[0] + J.Jr[6 + k] * J.Jrho[1];
+                ct += J.Jt[0] * J.Jrho[0] + J.Jt[1] * J.Jrho[1];
+            }
+        }
+        d = group_sum(d);
+        gr = group_sum(gr);
+        for (int k = 0; k < 6; ++k) cr[k] = group_sum(cr[k]);
+        ct = group_sum(ct);
+        if (live && lane == 0) {
+            w.lm_out[LO_D * L + l] = d;
+            w.lm_out[LO_G * L + l] = gr;
+            for (int k = 0; k < 6; ++k) w.lm_out[(LO_CR0 + k) * L + l] = cr[k];
+            w.lm_out[LO_CT * L + l] = ct;
+        }
+        (void)col;
+    }
+    __syncthreads();
+    double* part = w.lin_part + (size_t)ch.local * (tri + nv);
+    for (int i = threadIdx.x; i < tri + nv; i += kBlock) part[i] = acc[i];
+    const double c = block_sum(cost, red);
+    const double inv = block_max(invalid ? 1.0 : 0.0, red);
+    if (threadIdx.x == 0) {
+        w.chunk_sc[(size_t)ch.local * CS_PLANES + CS_COST] = c;
+        w.chunk_sc[(size_t)ch.local * CS_PLANES + CS_INVALID] = inv;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Retained factors: IMU (GpuRetainedFactorMath.hpp:53-140) and the square-root prior
+// (VisualCudaBatch.cu:216-268, MarginalizationPrior.cpp:20-66).
+// ---------------------------------------------------------------------------------------------
+struct ImuSensor {
+    V3<double> p, v, arm, lever;
+    Qt<double> q;
+};
+__device__ ImuSensor imu_sensor(const double* d, const double* pose, const double* sb, int end) {
+    const Qt<double> q = qnormalized(qload(pose + 3));
+    ImuSensor s;
+    s.q = qnormalized(qmul(q, qload(d + 5)));
+    s.arm = qrotate(q, load3(d + 2));
+    s.lever = cross(qrotate(s.q, load3(d + 9 + 3 * end) - load3(sb + 3)), s.arm);
+    s.p = load3(pose) + s.arm;
+    s.v = load3(sb) + s.lever;
+    return s;
+}
+__device__ __forceinline__ V3<double> m3v(const double* M, V3<double> v) {  // row-major 3x3 * v
+    return {M[0] * v.x + M[1] * v.y + M[2] * v.z, M[3] * v.x + M[4] * v.y + M[5] * v.z,
+            M[6] * v.x + M[7] * v.y + M[8] * v.z};
+}
+__device__ __forceinline__ void skew_of(V3<double> v, double* S) {
+    S[0] = 0; S[1] = -v.z; S[2] = v.y; S[3] = v.z; S[4] = 0; S[5] = -v.x; S[6] = -v.y; S[7] = v.x; S[8] = 0;
+}
+__device__ __forceinline__ void mm3(const double* A, const double* B, double* C) {
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) C[r * 3 + c] = A[r * 3] * B[c] + A[r * 3 + 1] * B[3 + c] + A[r * 3 + 2] * B[6 + c];
+}
+
+// Evaluates one IMU edge: r (15) and, if J, the 15 x 30 tangent Jacobian [pose0 6 | sb0 9 | pose1 6 | sb1 9]
+// (row-major, whitened). Returns false on non-finite results.
+__device__ bool eval_imu(const double* d, const double* pose0, const double* sb0, const double* pose1,
+                         const double* sb1, double* r, double* J) {
+    const ImuSensor s0 = imu_sensor(d, pose0, sb0, 0), s1 = imu_sensor(d, pose1, sb1, 1);
+    const M3<double> R = qmatrix(qconj(s0.q));
+    const double dt = d[0];
+    const V3<double> grav = {0, 0, d[1]};
+    const V3<double> bg = load3(sb0 + 3) - load3(d + 25), ba = load3(sb0 + 6) - load3(d + 28);
+    const V3<double> dp = s0.p - s1.p + dt * s0.v - (0.5 * dt * dt) * grav;
+    const V3<double> dv = s0.v - s1.v - dt * grav;
+    const V3<double> phi = -1.0 * m3v(d + 31, bg);
+    const Qt<double> corr = rotation_exp(phi);
+    const Qt<double> Dq = qload(d + 15);
+    const Qt<double> dq = qmul(corr, Dq);
+    const Qt<double> rel = qmul(qconj(s1.q), s0.q);
+    const Qt<double> tail = qmul(qmul(Dq, qconj(s1.q)), s0.q);
+    double e[15];
+    const V3<double> ep = mul(R, dp) + load3(d + 22) + m3v(d + 49, bg) - m3v(d + 67, ba);
+    const V3<double> eq = 2.0 * qvec(qmul(dq, rel));
+    const V3<double> ev = mul(R, dv) + load3(d + 19) + m3v(d + 40, bg) - m3v(d + 58, ba);
+    e[0] = ep.x; e[1] = ep.y; e[2] = ep.z; e[3] = eq.x; e[4] = eq.y; e[5] = eq.z; e[6] = ev.x; e[7] = ev.y; e[8] = ev.z;
+    for (int i = 0; i < 6; ++i) e[9 + i] = sb0[3 + i] - sb1[3 + i];
+    const double* W = d + 76;
+    bool ok = true;
+    for (int i = 0; i < 15; ++i) {
+        double s = 0;
+        for (int k = 0; k < 15; ++k) s += W[i * 15 + k] * e[k];
+        r[i] = s;
+        ok = ok && isfinite(s);
+    }
+    if (!ok || !J) return ok;
+    // gyroAngle = -J_r(-phi)... rotationJacobian(-phi) * A, as columns
+    double GA[9];
+    {
+        // rotationJacobian(v) = I + c1 [v]x + c2 [v]x^2 at v = -phi
+        const V3<double> v = -1.0 * phi;
+        const double a2 = sqnorm(v), n = sqrt(a2);
+        const double c1 = a2 < 1e-10 ? 0.5 - a2 / 24 : (1 - cos(n)) / a2;
+        const double c2 = a2 < 1e-10 ? 1.0 / 6 - a2 / 120 : (n - sin(n)) / (a2 * n);
+        double Sk[9], S2[9], Jl[9];
+        skew_of(v, Sk);
+        mm3(Sk, Sk, S2);
+        for (int i = 0; i < 9; ++i) Jl[i] = (i % 4 == 0 ? 1.0 : 0.0) + c1 * Sk[i] + c2 * S2[i];
+        mm3(Jl, d + 31, GA);
+        for (int i = 0; i < 9; ++i) GA[i] = -GA[i];
+    }
+    double Jrot[9], Jbias[9];
+    const Qt<double> left = qmul(dq, qconj(s1.q));
+    for (int j = 0; j < 3; ++j) {
+        const Qt<double> ax = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0, 0.0};
+        const V3<double> c1 = qvec(qmul(qmul(left, ax), s0.q));
+        const Qt<double> dbq = {GA[j], GA[3 + j], GA[6 + j], 0.0};
+        const V3<double> c2 = qvec(qmul(qmul(corr, dbq), tail));
+        Jrot[j] = c1.x; Jrot[3 + j] = c1.y; Jrot[6 + j] = c1.z;
+        Jbias[j] = c2.x; Jbias[3 + j] = c2.y; Jbias[6 + j] = c2.z;
+    }
+    // raw (unwhitened) Jacobian blocks, 15 x 30
+    double Jr[15 * 30];
+    for (int i = 0; i < 15 * 30; ++i) Jr[i] = 0;
+    auto put = [&](int r0, int c0, const double* m, double s) {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Jr[(r0 + a) * 30 + c0 + b] = s * m[a * 3 + b];
+    };
+    auto add = [&](int r0, int c0, const double* m, double s) {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Jr[(r0 + a) * 30 + c0 + b] += s * m[a * 3 + b];
+    };
+    double tmp[9], tmp2[9], Sk[9];
+    for (int k = 0; k < 4; ++k) {
+        const bool first = k < 2;
+        const double sign = first ? 1.0 : -1.0;
+        const ImuSensor& s = first ? s0 : s1;
+        const int c0 = k == 0 ? 0 : (k == 1 ? 6 : (k == 2 ? 15 : 21));
+        if (k % 2 == 0) {
+            put(0, c0, R.m, sign);
+            skew_of(s.p, Sk); mm3(R.m, Sk, tmp); put(0, c0 + 3, tmp, -sign);
+            put(3, c0 + 3, Jrot, sign);
+            skew_of(s.lever, Sk); mm3(R.m, Sk, tmp); put(6, c0 + 3, tmp, -sign);
+            if (first) {
+                double Sdp[9], Sl[9];
+                skew_of(dp, Sdp); skew_of(s.lever, Sl);
+                for (int i = 0; i < 9; ++i) tmp2[i] = Sdp[i] - dt * Sl[i];
+                mm3(R.m, tmp2, tmp); add(0, c0 + 3, tmp, 1.0);
+                skew_of(dv, Sk); mm3(R.m, Sk, tmp); add(6, c0 + 3, tmp, 1.0);
+            }
+        } else {
+            double lb[9];
+            const M3<double> Rs = qmatrix(s.q);
+            skew_of(s.arm, Sk); mm3(Sk, Rs.m, lb);
+            put(6, c0, R.m, sign);
+            mm3(R.m, lb, tmp); put(6, c0 + 3, tmp, sign);
+            double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            put(9, c0 + 3, I3, sign);
+            put(12, c0 + 6, I3, sign);
+            if (first) {
+                put(0, c0, R.m, dt);
+                mm3(R.m, lb, tmp);
+                for (int i = 0; i < 9; ++i) tmp[i] = dt * tmp[i] + d[49 + i];
+                put(0, c0 + 3, tmp, 1.0);
+                put(0, c0 + 6, d + 67, -1.0);
+                put(3, c0 + 3, Jbias, 1.0);
+                add(6, c0 + 3, d + 40, 1.0);
+                put(6, c0 + 6, d + 58, -1.0);
+            }
+        }
+    }
+    for (int i = 0; i < 15; ++i)
+        for (int c = 0; c < 30; ++c) {
+            double s = 0;
+            for (int m = 0; m < 15; ++m) s += W[i * 15 + m] * Jr[m * 30 + c];
+            J[i * 30 + c] = s;
+            ok = ok && isfinite(s);
+        }
+    return ok;
+}
+
+// Prior chart of block b at state x, with the central-difference chart derivative (6x6, h=1e-7)
+// for SE(3) blocks (VisualCudaBatch.cu:216-246).
+__device__ const double* prior_state(const WinDev& w, const PriorBlockDev& b, const double* pose, const double* sb,
+                                     const double* rig, const double* td) {
+    if (b.kind == 0) return pose + 7 * b.slot;
+    if (b.kind == 1) return sb + 9 * b.slot;
+    if (b.kind == 2) return rig;
+    return td;
+}
+
+// Retained linearization (IMU + prior) of one window into Hret/gret; cost and validity in LmState.
+// If `cost_only`, evaluates the retained cost at the trial state and returns it via `out_cost`.
+__device__ void retained_eval(const WinDev& w, const double* pose, const double* sb, const double* rig,
+                              const double* td, bool cost_only, double* sh, double* red, double& out_cost,
+                              int& out_invalid) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int nr = w.nr;
+    double cost = 0;
+    int invalid = 0;
+    // IMU edges: each thread evaluates edges t, t+nt, ... residuals (+ Jacobian into sh)
+    // sh layout: E x (15 + 450)
+    const int E = w.E;
+    for (int e = t; e < E; e += nt) {
+        double* rr = sh + (size_t)e * 465;
+        const int s0 = w.imu_from[e], s1 = w.imu_to[e];
+        const bool ok = eval_imu(w.imu + (size_t)e * 301, pose + 7 * s0, sb + 9 * s0, pose + 7 * s1, sb + 9 * s1, rr,
+                                 cost_only ? nullptr : rr + 15);
+        if (!ok) invalid = 1;
+        for (int i = 0; i < 15; ++i) cost += 0.5 * rr[i] * rr[i];
+    }
+    __syncthreads();
+    if (!cost_only) {
+        for (size_t i = t; i < (size_t)nr * nr; i += nt) w.Hret[i] = 0;
+        for (int i = t; i < nr; i += nt) w.gret[i] = 0;
+        __syncthreads();
+        for (int e = 0; e < E; ++e) {
+            const double* rr = sh + (size_t)e * 465;
+            const double* J = rr + 15;
+            const int s0 = w.imu_from[e], s1 = w.imu_to[e];
+            int col[4] = {w.pose_col[s0], w.sb_col[s0], w.pose_col[s1], w.sb_col[s1]};
+            int wid[4] = {col[0] >= 0 ? 6 : 0, col[1] >= 0 ? w.sb_w[s0] : 0, col[2] >= 0 ? 6 : 0,
+                          col[3] >= 0 ? w.sb_w[s1] : 0};
+            const int off[4] = {0, 6, 15, 21};
+            // 30 x 30 entries over the free columns
+            for (int p = t; p < 900; p += nt) {
+                const int i = p / 30, j = p % 30;
+                const int bi = i < 6 ? 0 : (i < 15 ? 1 : (i < 21 ? 2 : 3));
+                const int bj = j < 6 ? 0 : (j < 15 ? 1 : (j < 21 ? 2 : 3));
+                const int ii = i - off[bi], jj = j - off[bj];
+                if (ii >= wid[bi] || jj >= wid[bj]) continue;
+                double s = 0;
+                for (int m = 0; m < 15; ++m) s += J[m * 30 + i] * J[m * 30 + j];
+                w.Hret[(size_t)(col[bi] + ii) * nr + col[bj] + jj] += s;
+            }
+            for (int i = t; i < 30; i += nt) {
+                const int bi = i < 6 ? 0 : (i < 15 ? 1 : (i < 21 ? 2 : 3));
+                const int ii = i - off[bi];
+                if (ii >= wid[bi]) continue;
+                double s = 0;
+                for (int m = 0; m < 15; ++m) s += J[m * 30 + i] * rr[m];
+                w.gret[col[bi] + ii] += s;
+            }
+            __syncthreads();
+        }
+    }
+    // prior
+    if (w.prior_rows > 0) {
+        const int m = w.prior_rows, n = w.prior_cols, B = w.prior_blocks;
+        double* chart = sh;            // n
+        double* D = sh + n;            // B x 36 chart derivatives
+        double* res = D + 36 * B;      // m
+        double* atr = res + m;         // n : A^T r
+        __syncthreads();
+        for (int bi = t; bi < B; bi += nt) {
+            const PriorBlockDev b = w.pblk[bi];
+            const double* x = prior_state(w, b, pose, sb, rig, td);
+            const double* ref = w.pref + b.ref;
+            const bool se3 = b.kind == 0 || b.kind == 2;
+            if (se3) pose_minus(x, ref, chart + b.chart);
+            else for (int c = 0; c < b.tsize; ++c) chart[b.chart + c] = x[c] - ref[c];
+            if (!cost_only && se3 && b.width > 0) {
+                const double h = 1e-7;
+                double step[6] = {0, 0, 0, 0, 0, 0}, pp[7], pm[7], dp[6], dm[6];
+                for (int c = 0; c < 6; ++c) {
+                    step[c] = h; pose_plus(x, step, pp);
+                    step[c] = -h; pose_plus(x, step, pm);
+                    step[c] = 0;
+                    pose_minus(pp, ref, dp); pose_minus(pm, ref, dm);
+                    for (int r = 0; r < 6; ++r) D[bi * 36 + r * 6 + c] = (dp[r] - dm[r]) / (2 * h);
+                }
+            }
+        }
+        __syncthreads();
+        for (int r = t; r < m; r += nt) {
+            const double* A = w.pA + (size_t)r * n;
+            double v = w.pr0[r];
+            for (int c = 0; c < n; ++c) v += A[c] * chart[c];
+            res[r] = v;
+            if (!isfinite(v)) invalid = 1;
+            cost += 0.5 * v * v;
+        }
+        __syncthreads();
+        if (!cost_only) {
+            for (int c = t; c < n; c += nt) {
+                double s = 0;
+                for (int r = 0; r < m; ++r) s += w.pA[(size_t)r * n + c] * res[r];
+                atr[c] = s;
+            }
+            __syncthreads();
+            // gradient D_b^T (A^T r)_b and H = D^T G D over the free prior columns
+            // (pfree[k] = block << 8 | index within the block)
+            const int nf = w.prior_free;
+            for (int k = t; k < nf; k += nt) {
+                const int bi = w.pfree[k] >> 8, ii = w.pfree[k] & 255;
+                const PriorBlockDev b = w.pblk[bi];
+                const bool se3 = b.kind == 0 || b.kind == 2;
+                double s = 0;
+                for (int p = 0; p < b.tsize; ++p) s += (se3 ? D[bi * 36 + p * 6 + ii] : (p == ii ? 1.0 : 0.0)) * atr[b.chart + p];
+                w.gret[b.col + ii] += s;
+            }
+            for (int p = t; p < nf * nf; p += nt) {
+                const int k1 = p / nf, k2 = p % nf;
+                const int bi = w.pfree[k1] >> 8, ii = w.pfree[k1] & 255;
+                const int bj = w.pfree[k2] >> 8, jj = w.pfree[k2] & 255;
+                const PriorBlockDev b1 = w.pblk[bi], b2 = w.pblk[bj];
+                const bool se1 = b1.kind == 0 || b1.kind == 2, se2 = b2.kind == 0 || b2.kind == 2;
+                double s = 0;
+                for (int x = 0; x < b1.tsize; ++x) {
+                    const double d1 = se1 ? D[bi * 36 + x * 6 + ii] : (x == ii ? 1.0 : 0.0);
+                    if (d1 == 0) continue;
+                    const double* G = w.pG + (size_t)(b1.chart + x) * n + b2.chart;
+                    double inner = 0;
+                    if (se2) {
+                        for (int y = 0; y < b2.tsize; ++y) inner += G[y] * D[bj * 36 + y * 6 + jj];
+                    } else {
+                        inner = G[jj];
+                    }
+                    s += d1 * inner;
+                }
+                w.Hret[(size_t)(b1.col + ii) * nr + b2.col + jj] += s;
+            }
+            __syncthreads();
+        }
+    }
+    const double c = block_sum(cost, red);
+    const double inv = block_max(invalid ? 1.0 : 0.0, red);
+    out_cost = c;
+    out_invalid = inv > 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lin_retained(WinDev* wins) {
+    __shared__ double red[kBlock];
+    extern __shared__ double sh[];
+    WinDev& w = wins[blockIdx.x];
+    if (!window_active(w) || !w.lm->new_point) return;
+    double cost;
+    int invalid;
+    retained_eval(w, w.pose, w.sb, w.rig, w.td, false, sh, red, cost, invalid);
+    if (threadIdx.x == 0) {
+        w.ret_cost = cost;
+        w.ret_invalid = invalid;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fixed-order reduction of chunk partials: out[i] = sum_c part[c][i] (slices of 256 entries).
+// grid.x = window, grid.y = slice
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_reduce(const WinDev* wins, int which, double** outs) {
+    const WinDev& w = wins[blockIdx.x];
+    if (!window_active(w)) return;
+    if (which == 0 && !w.lm->new_point) return;
+    const int len = w.tri + w.nv;
+    const int i = blockIdx.y * kBlock + threadIdx.x;
+    if (i >= len) return;
+    const double* part = which == 0 ? w.lin_part : w.schur_part;
+    double s = 0;
+    for (int c = 0; c < w.nchunks; ++c) s += part[(size_t)c * len + i];
+    outs[2 * blockIdx.x + which][i] = s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_schur: -c c^T / damped(d) over each landmark's support (VisualCudaBatch.cu:145-161).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double damped_point(double d, double lambda) { return d + lambda * fmax(d, 1e-6); }
+
+__global__ __launch_bounds__(kBlock) void k_schur(const WinDev* wins, const Chunk* chunks) {
+    __shared__ double acc[kAccSize];
+    constexpr int groups = kBlock / kChunkLanes;
+    constexpr int kSlots = 16;  // anchor + up to 13 observers + rig + td
+    __shared__ double cv[groups][kSlots][6];
+    __shared__ int ccol[groups][kSlots];
+    __shared__ int cw[groups][kSlots];
+    const Chunk ch = chunks[blockIdx.x];
+    const WinDev& w = wins[ch.win];
+    if (!window_active(w)) return;
+    const int nv = w.nv, tri = w.tri;
+    for (int i = threadIdx.x; i < tri + nv; i += kBlock) acc[i] = 0;
+    __syncthreads();
+    double* gacc = acc + tri;
+    const double lambda = w.lm->lambda;
+    const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
+    const size_t L = (size_t)w.L, F = (size_t)w.F;
+    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
+        const int l = base + group;
+        const bool live = l < ch.lm_end && w.lm_col[l] >= 0;
+        int nslots = 0;
+        double inv = 0, gk = 0;
+        double ca[6] = {0, 0, 0, 0, 0, 0};
+        if (live) {
+            const int a = w.lm_fc[l] >> 8;
+            const int f0 = w.lm_off[l], f1 = w.lm_off[l + 1];
+            const int m = f1 - f0;
+            inv = 1.0 / damped_point(w.lm_out[LO_D * L + l], lambda);
+            gk = w.lm_out[LO_G * L + l];
+            // observer slots 1..m : -u_f at observer frame
+            for (int j = lane; j < m && j < kSlots - 3; j += kChunkLanes) {
+                const int f = f0 + j;
+                const int b = w.ob_fc[f] >> 8;
+                const int col = b == a ? -1 : w.pose_col[b];
+                ccol[group][1 + j] = col;
+                cw[group][1 + j] = 6;
+                for (int k = 0; k < 6; ++k) {
+                    const double u = w.fac_u[k * F + f];
+                    cv[group][1 + j][k] = -u;
+                    ca[k] += u;
+                }
+            }
+            nslots = 1 + (m < kSlots - 3 ? m : kSlots - 3);
+        }
+        for (int k = 0; k < 6; ++k) ca[k] = group_sum(ca[k]);
+        if (live && lane == 0) {
+            const int a = w.lm_fc[l] >> 8;
+            ccol[group][0] = w.pose_col[a];
+            cw[group][0] = 6;
+            for (int k = 0; k < 6; ++k) cv[group][0][k] = ca[k];
+            int s = nslots;
+            if (w.rig_col >= 0) {
+                ccol[group][s] = w.rig_col; cw[group][s] = 6;
+                for (int k = 0; k < 6; ++k) cv[group][s][k] = w.lm_out[(LO_CR0 + k) * L + l];
+                ++s;
+            }
+            if (w.td_col >= 0) {
+                ccol[group][s] = w.td_col; cw[group][s] = 1;
+                cv[group][s][0] = w.lm_out[LO_CT * L + l];
+                ++s;
+            }
+        }
+        if (live) nslots += (w.rig_col >= 0) + (w.td_col >= 0);
+        __syncthreads();
+        if (live) {
+            // gradient: g_r -= c g_k / damped
+            for (int s = lane; s < nslots; s += kChunkLanes) {
+                const int col = ccol[group][s];
+                if (col < 0) continue;
+                for (int k = 0; k < cw[group][s]; ++k) atomicAdd(&gacc[col + k], -inv * cv[group][s][k] * gk);
+            }
+            // unordered slot pairs (s <= t)
+            const int npairs = nslots * (nslots + 1) / 2;
+            for (int p = lane; p < npairs; p += kChunkLanes) {
+                int s = 0, rem = p;
+                while (rem >= nslots - s) { rem -= nslots - s; ++s; }
+                const int t2 = s + rem;
+                const int c1 = ccol[group][s], c2 = ccol[group][t2];
+                if (c1 < 0 || c2 < 0) continue;
+                const int w1 = cw[group][s], w2 = cw[group][t2];
+                const double* v1 = cv[group][s];
+                const double* v2 = cv[group][t2];
+                if (c1 == c2) {
+                    if (s == t2) {
+                        for (int i = 0; i < w1; ++i)
+                            for (int j = i; j < w1; ++j) atomicAdd(&acc[tri_index(c1 + i, c1 + j, nv)], -inv * v1[i] * v1[j]);
+                    } else {
+                        for (int i = 0; i < w1; ++i)
+                            for (int j = i; j < w1; ++j)
+                                atomicAdd(&acc[tri_index(c1 + i, c1 + j, nv)], -inv * (v1[i] * v2[j] + v2[i] * v1[j]));
+                    }
+                } else if (c1 < c2) {
+                    for (int i = 0; i < w1; ++i)
+                        for (int j = 0; j < w2; ++j) atomicAdd(&acc[tri_index(c1 + i, c2 + j, nv)], -inv * v1[i] * v2[j]);
+                } else {
+                    for (int j = 0; j < w2; ++j)
+                        for (int i = 0; i < w1; ++i) atomicAdd(&acc[tri_index(c2 + j, c1 + i, nv)], -inv * v1[i] * v2[j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* part = w.schur_part + (size_t)ch.local * (tri + nv);
+    for (int i = threadIdx.x; i < tri + nv; i += kBlock) part[i] = acc[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_assemble: full undamped normal matrix of the states (when newly linearized), then the damped
+// reduced camera system S and rhs (prepareStateSolve, VisualCudaBatch.cu:163-172).
+// red_lin / red_schur: reduced partials (tri + nv) per window.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_assemble(WinDev* wins, double* const* reduced) {
+    __shared__ double red[1024];
+    WinDev& w = wins[blockIdx.x];
+    if (!window_active(w)) return;
+    const int nr = w.nr, nv = w.nv;
+    const double* rl = reduced[2 * blockIdx.x + 0];
+    const double* rs = reduced[2 * blockIdx.x + 1];
+    const int t = threadIdx.x, nt = blockDim.x;
+    if (w.lm->new_point) {
+        for (size_t p = t; p < (size_t)nr * nr; p += nt) {
+            const int i = (int)(p / nr), j = (int)(p % nr);
+            double h = w.Hret[p];
+            if (i < nv && j < nv) h += rl[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
+            w.Hfull[p] = h;
+        }
+        for (int i = t; i < nr; i += nt) w.gfull[i] = w.gret[i] + (i < nv ? rl[w.tri + i] : 0.0);
+        if (t == 0) {
+            double cost = 0;
+            int invalid = w.ret_invalid;
+            for (int k = 0; k < w.nchunks; ++k) {
+                cost += w.chunk_sc[(size_t)k * CS_PLANES + CS_COST];
+                invalid |= w.chunk_sc[(size_t)k * CS_PLANES + CS_INVALID] > 0;
+            }
+            cost += w.ret_cost;
+            w.lm->lin_cost = cost;
+            w.lm->lin_valid = !invalid && isfinite(cost);
+        }
+        __syncthreads();
+    }
+    const double lambda = w.lm->lambda;
+    for (size_t p = t; p < (size_t)nr * nr; p += nt) {
+        const int i = (int)(p / nr), j = (int)(p % nr);
+        double s = w.Hfull[p];
+        if (i < nv && j < nv) s += rs[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
+        if (i == j) s += lambda * fmax(w.Hfull[p], 1e-6);
+        w.S[p] = s;
+        w.Scopy[p] = s;
+    }
+    for (int i = t; i < nr; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
+    (void)red;
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_cholesky: FP64 blocked right-looking Cholesky of S (lower, in place), triangular solves,
+// optional iterative refinement against the saved damped system, state step clamp, model
+// decrease of the states and the state trial x [+] delta (VisualCudaBatch.cu:174-189,323-340).
+// One workgroup per window.
+// ---------------------------------------------------------------------------------------------
+constexpr int kNb = 32;
+
+__device__ bool cholesky_inplace(double* A, int n, double* panel, double* diag, int* flag) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    for (int k0 = 0; k0 < n; k0 += kNb) {
+        const int kb = min(kNb, n - k0);
+        // diagonal block -> LDS
+        for (int p = t; p < kb * kb; p += nt) diag[p] = A[(size_t)(k0 + p / kb) * n + k0 + p % kb];
+        __syncthreads();
+        // unblocked factorization of the diagonal block by the first wave
+        if (t < 64) {
+            for (int j = 0; j < kb; ++j) {
+                if (t == 0) {
+                    const double d = diag[j * kb + j];
+                    if (!(d > 0) || !isfinite(d)) *flag = 1;
+                    diag[j * kb + j] = sqrt(d);
+                }
+                wave_sync();
+                const double djj = diag[j * kb + j];
+                for (int i = j + 1 + t; i < kb; i += 64) diag[i * kb + j] /= djj;
+                wave_sync();
+                for (int p = t; p < (kb - j - 1) * (kb - j - 1); p += 64) {
+                    const int i = j + 1 + p / (kb - j - 1), c = j + 1 + p % (kb - j - 1);
+                    if (c <= i) diag[i * kb + c] -= diag[i * kb + j] * diag[c * kb + j];
+                }
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        if (*flag) return false;
+        for (int p = t; p < kb * kb; p += nt) {
+            const int i = p / kb, c = p % kb;
+            if (c <= i) A[(size_t)(k0 + i) * n + k0 + c] = diag[p];
+        }
+        // panel: rows below, L21 = A21 L11^-T  (row-wise forward substitution)
+        const int rows = n - k0 - kb;
+        for (int r = t; r < rows; r += nt) {
+            double* row = panel + (size_t)r * kNb;
+            const double* src = A + (size_t)(k0 + kb + r) * n + k0;
+            for (int c = 0; c < kb; ++c) {
+                double s = src[c];
+                for (int q = 0; q < c; ++q) s -= row[q] * diag[c * kb + q];
+                row[c] = s / diag[c * kb + c];
+            }
+        }
+        __syncthreads();
+        for (int p = t; p < rows * kb; p += nt) {
+            const int r = p / kb, c = p % kb;
+            A[(size_t)(k0 + kb + r) * n + k0 + c] = panel[(size_t)r * kNb + c];
+        }
+        // trailing update of the lower triangle, 4x4 register tiles
+        const int tiles = (rows + 3) / 4;
+        const int ntile = tiles * (tiles + 1) / 2;
+        for (int p = t; p < ntile; p += nt) {
+            int ti = 0, rem = p;
+            while (rem > ti) { rem -= ti + 1; ++ti; }
+            const int tj = rem;  // tj <= ti
+            double accv[4][4];
+            for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) accv[a][b] = 0;
+            for (int q = 0; q < kb; ++q) {
+                double x[4], y[4];
+                for (int a = 0; a < 4; ++a) {
+                    const int ri = 4 * ti + a, rj = 4 * tj + a;
+                    x[a] = ri < rows ? panel[(size_t)ri * kNb + q] : 0.0;
+                    y[a] = rj < rows ? panel[(size_t)rj * kNb + q] : 0.0;
+                }
+                for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) accv[a][b] += x[a] * y[b];
+            }
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) {
+                    const int ri = 4 * ti + a, rj = 4 * tj + b;
+                    if (ri < rows && rj < rows && rj <= ri)
+                        A[(size_t)(k0 + kb + ri) * n + k0 + kb + rj] -= accv[a][b];
+                }
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// Solve L L^T x = b in place (x in LDS vector), lower factor in A.
+__device__ void cholesky_solve(const double* A, int n, double* x) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    // forward: L y = b, column oriented in blocks of 64 rows handled by wave 0
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int jb = min(64, n - j0);
+        if (t < 64) {
+            for (int j = 0; j < jb; ++j) {
+                if (t == 0) x[j0 + j] /= A[(size_t)(j0 + j) * n + j0 + j];
+                wave_sync();
+                const double yj = x[j0 + j];
+                const int i = j0 + t;
+                if (t > j && t < jb) x[i] -= A[(size_t)i * n + j0 + j] * yj;
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        for (int i = j0 + jb + t; i < n; i += nt) {
+            double s = 0;
+            for (int j = 0; j < jb; ++j) s += A[(size_t)i * n + j0 + j] * x[j0 + j];
+            x[i] -= s;
+        }
+        __syncthreads();
+    }
+    // backward: L^T x = y
+    for (int j1 = n; j1 > 0; j1 -= 64) {
+        const int j0 = max(0, j1 - 64), jb = j1 - j0;
+        if (t < 64) {
+            for (int j = jb - 1; j >= 0; --j) {
+                if (t == 0) x[j0 + j] /= A[(size_t)(j0 + j) * n + j0 + j];
+                wave_sync();
+                const double xj = x[j0 + j];
+                if (t < j) x[j0 + t] -= A[(size_t)(j0 + j) * n + j0 + t] * xj;
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        for (int i = t; i < j0; i += nt) {
+            double s = 0;
+            for (int j = 0; j < jb; ++j) s += A[(size_t)(j0 + j) * n + i] * x[j0 + j];
+            x[i] -= s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cholesky(WinDev* wins, int refinement) {
+    __shared__ double panel[kMaxRetained * kNb];
+    __shared__ double diag[kNb * kNb];
+    __shared__ double x[kMaxRetained];
+    __shared__ double rres[kMaxRetained];
+    __shared__ double red[kBlock];
+    __shared__ int flag;
+    WinDev& w = wins[blockIdx.x];
+    if (!window_active(w)) return;
+    const int n = w.nr, t = threadIdx.x, nt = blockDim.x;
+    if (t == 0) flag = 0;
+    __syncthreads();
+    const bool ok = cholesky_inplace(w.S, n, panel, diag, &flag);
+    if (!ok) {
+        if (t == 0) { w.lm->chol_ok = 0; }
+        return;
+    }
+    for (int i = t; i < n; i += nt) x[i] = w.rhs[i];
+    __syncthreads();
+    cholesky_solve(w.S, n, x);
+    for (int it = 0; it < refinement; ++it) {
+        for (int i = t; i < n; i += nt) {
+            double s = w.rhs[i];
+            const double* row = w.Scopy + (size_t)i * n;
+            for (int j = 0; j < n; ++j) s -= row[j] * x[j];
+            rres[i] = s;
+        }
+        __syncthreads();
+        cholesky_solve(w.S, n, rres);
+        for (int i = t; i < n; i += nt) x[i] += rres[i];
+        __syncthreads();
+    }
+    // clamp to relative bounds: only td is bounded among the states
+    if (t == 0 && w.td_col >= 0 && isfinite(x[w.td_col])) {
+        const double tdv = *w.td;
+        x[w.td_col] = fmax(w.td_lo - tdv, fmin(w.td_hi - tdv, x[w.td_col]));
+    }
+    __syncthreads();
+    // state model decrease: -dx^T (g + 0.5 H dx), undamped H
+    double pred = 0;
+    for (int i = t; i < n; i += nt) {
+        const double* row = w.Hfull + (size_t)i * n;
+        double hd = 0;
+        for (int j = 0; j < n; ++j) hd += row[j] * x[j];
+        pred += -x[i] * (w.gfull[i] + 0.5 * hd);
+    }
+    const double ps = block_sum(pred, red);
+    double mx = 0;
+    int bad = 0;
+    for (int i = t; i < n; i += nt) {
+        w.dstate[i] = x[i];
+        if (!isfinite(x[i])) bad = 1;
+        mx = fmax(mx, fabs(x[i]));
+    }
+    const double maxinc = block_max(mx, red);
+    const double nonfinite = block_max(bad ? 1.0 : 0.0, red);
+    // state trial
+    for (int k = t; k < w.K; k += nt) {
+        const double* xp = w.pose + 7 * k;
+        double* yp = w.cpose + 7 * k;
+        if (w.pose_col[k] >= 0) pose_plus(xp, x + w.pose_col[k], yp);
+        else for (int i = 0; i < 7; ++i) yp[i] = xp[i];
+        for (int i = 0; i < 9; ++i) {
+            const double v = w.sb[9 * k + i];
+            w.csb[9 * k + i] = (w.sb_col[k] >= 0 && i < w.sb_w[k]) ? v + x[w.sb_col[k] + i] : v;
+        }
+    }
+    if (t == 0) {
+        if (w.rig_col >= 0) pose_plus(w.rig, x + w.rig_col, w.crig);
+        else for (int i = 0; i < 7; ++i) w.crig[i] = w.rig[i];
+        *w.ctd = w.td_col >= 0 ? fmax(w.td_lo, fmin(w.td_hi, *w.td + x[w.td_col])) : *w.td;
+        w.lm->chol_ok = 1;
+        w.lm->predicted = ps;
+        w.lm->max_increment = maxinc;
+        w.lm->trial_valid = nonfinite > 0 ? 0 : 1;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_backsub: delta_k = -(g_k + c_k^T dx) / damped(d_k), clamped; model decrease; rho trial
+// (VisualCudaBatch.cu:191-205).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Chunk* chunks) {
+    __shared__ double red[kBlock];
+    const Chunk ch = chunks[blockIdx.x];
+    const WinDev& w = wins[ch.win];
+    if (!window_active(w)) return;
+    const double lambda = w.lm->lambda;
+    const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
+    constexpr int groups = kBlock / kChunkLanes;
+    const size_t L = (size_t)w.L, F = (size_t)w.F;
+    const double* dx = w.dstate;
+    double pred = 0, mx = 0;
+    int bad = 0;
+    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
+        const int l = base + group;
+        const bool live = l < ch.lm_end;
+        const bool free = live && w.lm_col[l] >= 0;
+        double cd = 0;
+        if (free) {
+            const int a = w.lm_fc[l] >> 8;
+            const int ac = w.pose_col[a];
+            for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
+                const int b = w.ob_fc[f] >> 8;
+                if (b == a) continue;
+                const int bc = w.pose_col[b];
+                for (int k = 0; k < 6; ++k) {
+                    const double u = w.fac_u[k * F + f];
+                    cd += u * ((ac >= 0 ? dx[ac + k] : 0.0) - (bc >= 0 ? dx[bc + k] : 0.0));
+                }
+            }
+        }
+        cd = group_sum(cd);
+        if (live && lane == 0) {
+            const double rho = w.rho[l];
+            if (free) {
+                if (w.rig_col >= 0)
+                    for (int k = 0; k < 6; ++k) cd += w.lm_out[(LO_CR0 + k) * L + l] * dx[w.rig_col + k];
+                if (w.td_col >= 0) cd += w.lm_out[LO_CT * L + l] * dx[w.td_col];
+                const double dd = w.lm_out[LO_D * L + l], gk = w.lm_out[LO_G * L + l];
+                const double d = -(gk + cd) / damped_point(dd, lambda);
+                const double delta = isfinite(d) ? fmax(w.min_rho - rho, d) : d;
+                pred += -delta * (gk + cd + 0.5 * dd * delta);
+                if (!isfinite(delta)) bad = 1;
+                mx = fmax(mx, fabs(delta));
+                w.lm_delta[l] = delta;
+                w.crho[l] = fmax(w.min_rho, rho + delta);
+            } else {
+                w.lm_delta[l] = 0;
+                w.crho[l] = rho;
+            }
+        }
+    }
+    const double ps = block_sum(pred, red);
+    const double m = block_max(mx, red);
+    const double b = block_max(bad ? 1.0 : 0.0, red);
+    if (threadIdx.x == 0) {
+        double* sc = w.chunk_sc + (size_t)ch.local * CS_PLANES;
+        sc[CS_PRED] = ps;
+        sc[CS_MAXINC] = m;
+        sc[CS_TINVALID] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_trial_cost: robust visual cost at the trial state.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const Chunk* chunks) {
+    __shared__ FrameTable ft;
+    __shared__ double red[kBlock];
+    const Chunk ch = chunks[blockIdx.x];
+    const WinDev& w = wins[ch.win];
+    if (!window_active(w)) return;
+    build_frame_table(w, w.cpose, w.crig, ft);
+    __syncthreads();
+    const double td = *w.ctd;
+    double cost = 0;
+    int invalid = 0;
+    const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
+    constexpr int groups = kBlock / kChunkLanes;
+    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
+        const int l = base + group;
+        if (l >= ch.lm_end) continue;
+        const LmData lm = load_landmark(w, w.crho, l);
+        for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
+            FactorJ J;
+            if (!eval_factor<false>(w, ft, lm, f, td, false, J)) { invalid = 1; continue; }
+            const double sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
+            if (!isfinite(sq)) { invalid = 1; continue; }
+            cost += huber_cost(w.huber, sq);
+        }
+    }
+    const double c = block_sum(cost, red);
+    const double b = block_max(invalid ? 1.0 : 0.0, red);
+    if (threadIdx.x == 0) {
+        double* sc = w.chunk_sc + (size_t)ch.local * CS_PLANES;
+        sc[CS_TRIAL] = c;
+        sc[CS_TINVALID] = fmax(sc[CS_TINVALID], b);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_lm_update: retained trial cost, then the native LM decision (BackendOptimizer.cpp:232-306).
+// mode 0: LM step; 1: evaluate only (metrics, no state change); 2: accept the trial state.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
+    __shared__ double red[kBlock];
+    extern __shared__ double sh[];
+    __shared__ int accept_flag;
+    WinDev& w = wins[blockIdx.x];
+    LmState* lm = w.lm;
+    if (mode != 2 && lm->status != 0) return;
+    const int t = threadIdx.x;
+    if (mode == 2) {
+        if (t == 0) accept_flag = 1;
+    } else {
+        double rcost;
+        int rinv;
+        retained_eval(w, w.cpose, w.csb, w.crig, w.ctd, true, sh, red, rcost, rinv);
+        if (t == 0) {
+            double pred = lm->predicted, trial = rcost, mx = lm->max_increment;
+            int tinv = rinv || !lm->trial_valid;
+            for (int k = 0; k < w.nchunks; ++k) {
+                const double* sc = w.chunk_sc + (size_t)k * CS_PLANES;
+                pred += sc[CS_PRED];
+                trial += sc[CS_TRIAL];
+                mx = fmax(mx, sc[CS_MAXINC]);
+                tinv |= sc[CS_TINVALID] > 0;
+            }
+            const bool trial_valid = !tinv && isfinite(trial);
+            accept_flag = 0;
+            const int i = lm->iteration;
+            int status = 0;
+            if (lm->new_point) {
+                if (!lm->lin_valid) status = -3;
+                else {
+                    const double lc = lm->lin_cost;
+                    if (!isfinite(lc) || (i > 0 && mode == 0 && fabs(lc - lm->cost) > 1e-8 * fmax(1.0, lm->cost))) status = -5;
+                    else if (i == 0 || mode == 1) { lm->cost = lc; if (i == 0) lm->initial_cost = lc; }
+                }
+            }
+            if (status == 0 && !lm->chol_ok) status = -4;
+            lm->predicted = pred;
+            lm->trial_cost = trial;
+            lm->max_increment = mx;
+            lm->trial_valid = trial_valid;
+            if (mode == 1) {
+                lm->status = status;
+            } else if (status != 0) {
+                lm->status = status;
+            } else {
+                if (trial_valid && mx < 1e-10) {
+                    lm->converged = 1;
+                    lm->status = 1;
+                } else {
+                    double gain = -1;
+                    if (trial_valid && pred > 0 && isfinite(pred)) gain = (lm->cost - trial) / pred;
+                    lm->last_gain = gain;
+                    if (gain > 1e-3) {
+                        accept_flag = 1;
+                        lm->accepted += 1;
+                        const bool conv = lm->cost - trial <= 1e-6 * fmax(1.0, lm->cost);
+                        lm->cost = trial;
+                        const double c = 2 * gain - 1;
+                        lm->lambda = fmax(1e-12, lm->lambda * fmax(1.0 / 3, 1 - c * c * c));
+                        lm->increase = 2;
+                        lm->new_point = 1;
+                        if (conv) { lm->converged = 1; lm->status = 1; }
+                    } else {
+                        lm->lambda *= lm->increase;
+                        lm->increase *= 2;
+                        lm->new_point = 0;
+                    }
+                }
+                lm->iteration = i + 1;
+                if (lm->status == 0 && lm->iteration >= lm->max_iterations) lm->status = 1;
+                if (lm->status == 1 && !lm->accepted && !lm->converged) lm->status = -6;
+            }
+        }
+    }
+    __syncthreads();
+    if (!accept_flag) return;
+    for (int i = t; i < 7 * w.K; i += blockDim.x) w.pose[i] = w.cpose[i];
+    for (int i = t; i < 9 * w.K; i += blockDim.x) w.sb[i] = w.csb[i];
+    if (t < 7) w.rig[t] = w.crig[t];
+    if (t == 0) *w.td = *w.ctd;
+    for (int i = t; i < w.L; i += blockDim.x) w.rho[i] = w.crho[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Prior Gram matrix G = A^T A (cols x cols), one workgroup per window, after a prior upload.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_prior_gram(const WinDev* wins, double* const* G) {
+    const WinDev& w = wins[blockIdx.x];
+    const int m = w.prior_rows, n = w.prior_cols;
+    double* out = G[blockIdx.x];
+    for (size_t p = threadIdx.x; p < (size_t)n * n; p += blockDim.x) {
+        const int i = (int)(p / n), j = (int)(p % n);
+        double s = 0;
+        for (int r = 0; r < m; ++r) s += w.pA[(size_t)r * n + i] * w.pA[(size_t)r * n + j];
+        out[p] = s;
+    }
+}
+
+}  // namespace vib
+
+// ---------------------------------------------------------------------------------------------
+// Launch wrappers used by the host driver.
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+int vibk_retained_shared_bytes(int E, int prior_rows, int prior_cols, int prior_blocks) {
+    const int imu = E * 465;
+    const int prior = prior_cols + 36 * prior_blocks + prior_rows + prior_cols;
+    return 8 * (imu > prior ? imu : prior);
+}
+
+static void allow_dynamic_lds(const void* fn, size_t bytes) {
+    if (bytes > 48 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t vibk_linearize(vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, size_t ret_shared,
+                          hipStream_t s) {
+    allow_dynamic_lds((const void*)vib::k_lin_retained, ret_shared);
+    if (nchunks) hipLaunchKernelGGL(vib::k_lin_visual, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks);
+    hipLaunchKernelGGL(vib::k_lin_retained, dim3(nwin), dim3(vib::kBlock), ret_shared, s, wins);
+    return hipGetLastError();
+}
+
+hipError_t vibk_reduce(vib::WinDev* wins, int nwin, int which, double** outs, int max_len, hipStream_t s) {
+    const int slices = (max_len + vib::kBlock - 1) / vib::kBlock;
+    hipLaunchKernelGGL(vib::k_reduce, dim3(nwin, slices), dim3(vib::kBlock), 0, s, wins, which, outs);
+    return hipGetLastError();
+}
+
+hipError_t vibk_schur(vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, hipStream_t s) {
+    if (nchunks) hipLaunchKernelGGL(vib::k_schur, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks);
+    return hipGetLastError();
+}
+
+hipError_t vibk_assemble(vib::WinDev* wins, int nwin, double* const* reduced, hipStream_t s) {
+    hipLaunchKernelGGL(vib::k_assemble, dim3(nwin), dim3(1024), 0, s, wins, reduced);
+    return hipGetLastError();
+}
+
+hipError_t vibk_cholesky(vib::WinDev* wins, int nwin, int refinement, hipStream_t s) {
+    hipLaunchKernelGGL(vib::k_cholesky, dim3(nwin), dim3(vib::kBlock), 0, s, wins, refinement);
+    return hipGetLastError();
+}
+
+hipError_t vibk_backsub_trial(vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, hipStream_t s) {
+    if (nchunks) {
+        hipLaunchKernelGGL(vib::k_backsub, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks);
+        hipLaunchKernelGGL(vib::k_trial_cost, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks);
+    }
+    return hipGetLastError();
+}
+
+hipError_t vibk_lm_update(vib::WinDev* wins, int nwin, int mode, size_t ret_shared, hipStream_t s) {
+    allow_dynamic_lds((const void*)vib::k_lm_update, ret_shared);
+    hipLaunchKernelGGL(vib::k_lm_update, dim3(nwin), dim3(vib::kBlock), ret_shared, s, wins, mode);
+    return hipGetLastError();
+}
+
+hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s) {
+    hipLaunchKernelGGL(vib::k_prior_gram, dim3(nwin), dim3(vib::kBlock), 0, s, wins, G);
+    return hipGetLastError();
+}
+
+}  // extern "C"

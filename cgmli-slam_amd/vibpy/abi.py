@@ -1,0 +1,457 @@
+"""ctypes binding of include/vib.h.
+
+`Window` owns numpy arrays for every field of `vib_window`; `Window.cstruct()` returns a
+`VibWindow` whose pointers alias those arrays (the Window must outlive the call).
+`load_library()` opens the in-tree `libvib.so` and fails loudly if it is missing.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "build", "libvib.so")
+
+VIB_OK = 0
+STATUS_NAMES = {
+    0: "OK", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED_DIMENSIONS", 3: "INVALID_RESIDUAL",
+    4: "NOT_POSITIVE_DEFINITE", 5: "COST_MISMATCH", 6: "NO_ACCEPTABLE_STEP", 7: "UNSUPPORTED_LOSS",
+    8: "DEVICE", 9: "OUT_OF_MEMORY", 10: "NONFINITE_STEP", 11: "INDEFINITE", 12: "EIGEN_FAILED",
+    13: "NO_DEVICE", 14: "NOT_FOUND", 15: "UNSUPPORTED",
+}
+VIB_RIG_FREE = 1
+VIB_TD_FREE = 2
+VIB_KEY_POSE, VIB_KEY_SPEED_BIAS, VIB_KEY_RIG, VIB_KEY_TD = 0, 1, 2, 3
+VIB_FP64, VIB_MIXED = 0, 1
+VIB_MARGIN_OLD, VIB_MARGIN_SECOND_NEW = 0, 1
+IMU_STRIDE = 301
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int32)
+lp = C.POINTER(C.c_int64)
+up = C.POINTER(C.c_uint8)
+
+
+class VibImuParams(C.Structure):
+    _fields_ = [("sigma_g_c", C.c_double), ("sigma_a_c", C.c_double), ("sigma_gw_c", C.c_double),
+                ("sigma_aw_c", C.c_double), ("g", C.c_double), ("g_max", C.c_double), ("a_max", C.c_double),
+                ("p_BS", C.c_double * 3), ("q_BS", C.c_double * 4)]
+
+
+class VibWindow(C.Structure):
+    _fields_ = [
+        ("num_frames", C.c_int32), ("num_cameras", C.c_int32), ("num_landmarks", C.c_int32),
+        ("num_observations", C.c_int32), ("num_imu", C.c_int32), ("flags", C.c_uint32),
+        ("huber_delta", C.c_double), ("min_inverse_depth", C.c_double),
+        ("td_lower", C.c_double), ("td_upper", C.c_double),
+        ("frame_id", lp), ("pose", dp), ("speed_bias", dp), ("pose_free", up), ("sb_width", up),
+        ("rig", dp), ("td", dp), ("cam_rotation", dp), ("cam_translation", dp),
+        ("lm_anchor_frame", ip), ("lm_anchor_cam", ip), ("lm_bearing", dp), ("lm_velocity", dp),
+        ("lm_tref", dp), ("lm_rho", dp), ("lm_free", up),
+        ("obs_landmark", ip), ("obs_frame", ip), ("obs_cam", ip), ("obs_bearing", dp),
+        ("obs_velocity", dp), ("obs_tref", dp), ("obs_sqrt_info", dp),
+        ("imu_from", ip), ("imu_to", ip), ("imu_data", dp),
+        ("prior_rows", C.c_int32), ("prior_blocks", C.c_int32), ("prior_key_kind", ip),
+        ("prior_key_id", lp), ("prior_ref", dp), ("prior_A", dp), ("prior_r0", dp),
+    ]
+
+
+class VibSolveOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("min_iterations", C.c_int32),
+                ("initial_lambda", C.c_double), ("time_limit_seconds", C.c_double),
+                ("precision", C.c_int32), ("refinement_steps", C.c_int32)]
+
+
+class VibStepReport(C.Structure):
+    _fields_ = [("cost", C.c_double), ("predicted_reduction", C.c_double), ("trial_cost", C.c_double),
+                ("max_increment", C.c_double), ("trial_valid", C.c_int32), ("status", C.c_int32)]
+
+
+class VibSolveReport(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("accepted_steps", C.c_int32),
+                ("converged", C.c_int32), ("effective_parameters", C.c_int32),
+                ("retained_dimension", C.c_int32), ("initial_cost", C.c_double),
+                ("final_cost", C.c_double), ("final_lambda", C.c_double), ("seconds", C.c_double)]
+
+
+class VibMarginalReport(C.Structure):
+    _fields_ = [("status", C.c_int32), ("prior_reused", C.c_int32), ("eliminated_dimension", C.c_int32),
+                ("retained_dimension", C.c_int32), ("prior_rank", C.c_int32),
+                ("eliminated_landmarks", C.c_int32), ("consumed_factors", C.c_int32),
+                ("eigen_cutoff", C.c_double)]
+
+
+def options(max_iterations=8, min_iterations=0, initial_lambda=1e-4, time_limit=0.0,
+            precision=VIB_FP64, refinement_steps=0):
+    return VibSolveOptions(max_iterations, min_iterations, initial_lambda, time_limit, precision,
+                           refinement_steps)
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class Window:
+    """One sliding window in the C-ABI layout (all arrays C-contiguous)."""
+    frame_id: np.ndarray
+    pose: np.ndarray            # (K, 7)
+    speed_bias: np.ndarray      # (K, 9)
+    rig: np.ndarray             # (7,)
+    td: np.ndarray              # (1,)
+    cam_rotation: np.ndarray    # (C, 3, 3)
+    cam_translation: np.ndarray  # (C, 3)
+    lm_anchor_frame: np.ndarray
+    lm_anchor_cam: np.ndarray
+    lm_bearing: np.ndarray      # (L, 3)
+    lm_velocity: np.ndarray     # (L, 3)
+    lm_tref: np.ndarray
+    lm_rho: np.ndarray
+    obs_landmark: np.ndarray
+    obs_frame: np.ndarray
+    obs_cam: np.ndarray
+    obs_bearing: np.ndarray     # (F, 3)
+    obs_velocity: np.ndarray    # (F, 3)
+    obs_tref: np.ndarray
+    obs_sqrt_info: np.ndarray   # (F, 3)
+    imu_from: np.ndarray
+    imu_to: np.ndarray
+    imu_data: np.ndarray        # (E, 301)
+    flags: int = 0
+    huber_delta: float = 1.0
+    min_inverse_depth: float = 1e-8
+    td_lower: float = -0.05
+    td_upper: float = 0.05
+    pose_free: np.ndarray = None
+    sb_width: np.ndarray = None
+    lm_free: np.ndarray = None
+    prior_key_kind: np.ndarray = None
+    prior_key_id: np.ndarray = None
+    prior_ref: np.ndarray = None
+    prior_A: np.ndarray = None   # (m, n)
+    prior_r0: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        i32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+        i64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.int64)
+        u8 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.uint8)
+        for name in ("pose", "speed_bias", "rig", "td", "cam_rotation", "cam_translation", "lm_bearing",
+                     "lm_velocity", "lm_tref", "lm_rho", "obs_bearing", "obs_velocity", "obs_tref",
+                     "obs_sqrt_info", "imu_data", "prior_ref", "prior_A", "prior_r0"):
+            setattr(self, name, f64(getattr(self, name)))
+        for name in ("lm_anchor_frame", "lm_anchor_cam", "obs_landmark", "obs_frame", "obs_cam", "imu_from",
+                     "imu_to", "prior_key_kind"):
+            setattr(self, name, i32(getattr(self, name)))
+        for name in ("frame_id", "prior_key_id"):
+            setattr(self, name, i64(getattr(self, name)))
+        for name in ("pose_free", "sb_width", "lm_free"):
+            setattr(self, name, u8(getattr(self, name)))
+
+    # sizes
+    @property
+    def K(self):
+        return int(self.pose.shape[0])
+
+    @property
+    def L(self):
+        return int(self.lm_rho.shape[0])
+
+    @property
+    def F(self):
+        return int(self.obs_landmark.shape[0])
+
+    @property
+    def E(self):
+        return int(self.imu_from.shape[0])
+
+    @property
+    def prior_rows(self):
+        return 0 if self.prior_A is None else int(self.prior_A.shape[0])
+
+    def copy(self):
+        kw = {}
+        for k, v in self.__dict__.items():
+            kw[k] = v.copy() if isinstance(v, np.ndarray) else (dict(v) if isinstance(v, dict) else v)
+        return Window(**kw)
+
+    def layout(self):
+        """Tangent column layout shared by the library and the oracle:
+        [pose_k (6, free), rig (6), td (1), sb_k (width)] then free inverse ranges."""
+        K = self.K
+        pf = np.ones(K, np.uint8) if self.pose_free is None else self.pose_free
+        sw = np.full(K, 9, np.uint8) if self.sb_width is None else self.sb_width
+        lf = np.ones(self.L, np.uint8) if self.lm_free is None else self.lm_free
+        c = 0
+        pose_col = np.full(K, -1)
+        for k in range(K):
+            if pf[k]:
+                pose_col[k] = c
+                c += 6
+        rig_col = td_col = -1
+        if self.flags & VIB_RIG_FREE:
+            rig_col = c
+            c += 6
+        if self.flags & VIB_TD_FREE:
+            td_col = c
+            c += 1
+        nv = c
+        sb_col = np.full(K, -1)
+        for k in range(K):
+            if sw[k]:
+                sb_col[k] = c
+                c += int(sw[k])
+        nr = c
+        lm_col = np.full(self.L, -1)
+        idx = np.nonzero(lf)[0]
+        lm_col[idx] = nr + np.arange(idx.size)
+        return dict(pose=pose_col, rig=rig_col, td=td_col, sb=sb_col, lm=lm_col, nv=nv, nr=nr,
+                    ne=int(idx.size))
+
+    def cstruct(self):
+        w = VibWindow()
+        w.num_frames = self.K
+        w.num_cameras = int(self.cam_rotation.shape[0])
+        w.num_landmarks = self.L
+        w.num_observations = self.F
+        w.num_imu = self.E
+        w.flags = self.flags
+        w.huber_delta = self.huber_delta
+        w.min_inverse_depth = self.min_inverse_depth
+        w.td_lower, w.td_upper = self.td_lower, self.td_upper
+        w.frame_id = _ptr(self.frame_id, C.c_int64)
+        w.pose = _ptr(self.pose, C.c_double)
+        w.speed_bias = _ptr(self.speed_bias, C.c_double)
+        w.pose_free = _ptr(self.pose_free, C.c_uint8)
+        w.sb_width = _ptr(self.sb_width, C.c_uint8)
+        w.rig = _ptr(self.rig, C.c_double)
+        w.td = _ptr(self.td, C.c_double)
+        w.cam_rotation = _ptr(self.cam_rotation, C.c_double)
+        w.cam_translation = _ptr(self.cam_translation, C.c_double)
+        w.lm_anchor_frame = _ptr(self.lm_anchor_frame, C.c_int32)
+        w.lm_anchor_cam = _ptr(self.lm_anchor_cam, C.c_int32)
+        w.lm_bearing = _ptr(self.lm_bearing, C.c_double)
+        w.lm_velocity = _ptr(self.lm_velocity, C.c_double)
+        w.lm_tref = _ptr(self.lm_tref, C.c_double)
+        w.lm_rho = _ptr(self.lm_rho, C.c_double)
+        w.lm_free = _ptr(self.lm_free, C.c_uint8)
+        w.obs_landmark = _ptr(self.obs_landmark, C.c_int32)
+        w.obs_frame = _ptr(self.obs_frame, C.c_int32)
+        w.obs_cam = _ptr(self.obs_cam, C.c_int32)
+        w.obs_bearing = _ptr(self.obs_bearing, C.c_double)
+        w.obs_velocity = _ptr(self.obs_velocity, C.c_double)
+        w.obs_tref = _ptr(self.obs_tref, C.c_double)
+        w.obs_sqrt_info = _ptr(self.obs_sqrt_info, C.c_double)
+        w.imu_from = _ptr(self.imu_from, C.c_int32)
+        w.imu_to = _ptr(self.imu_to, C.c_int32)
+        w.imu_data = _ptr(self.imu_data, C.c_double)
+        m = self.prior_rows
+        w.prior_rows = m
+        w.prior_blocks = 0 if not m else int(self.prior_key_kind.shape[0])
+        w.prior_key_kind = _ptr(self.prior_key_kind, C.c_int32)
+        w.prior_key_id = _ptr(self.prior_key_id, C.c_int64)
+        w.prior_ref = _ptr(self.prior_ref, C.c_double)
+        w.prior_A = _ptr(self.prior_A, C.c_double)
+        w.prior_r0 = _ptr(self.prior_r0, C.c_double)
+        return w
+
+
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Open libvib.so. Raises if the native library has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"native library missing: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    sig = {
+        "vib_abi_version": ([], C.c_int),
+        "vib_create": ([C.POINTER(vp), C.c_int], C.c_int),
+        "vib_destroy": ([vp], C.c_int),
+        "vib_last_error": ([vp], C.c_char_p),
+        "vib_window_upload": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
+        "vib_window_footprint": ([C.POINTER(VibWindow), C.c_int32], C.c_int64),
+        "vib_window_release": ([vp, C.c_int32], C.c_int),
+        "vib_read_state": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
+        "vib_window_shape": ([vp, C.c_int32] + [ip] * 6, C.c_int),
+        "vib_linearize": ([vp, C.c_int32, C.c_int32], C.c_int),
+        "vib_solve_step": ([vp, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.POINTER(VibStepReport)], C.c_int),
+        "vib_accept": ([vp, C.c_int32], C.c_int),
+        "vib_read_step": ([vp, C.c_int32, dp, C.c_int32], C.c_int),
+        "vib_read_reduced": ([vp, C.c_int32, dp, dp, C.c_int32], C.c_int),
+        "vib_read_normal": ([vp, C.c_int32, dp, dp, dp, C.c_int32], C.c_int),
+        "vib_retained_dimension": ([vp, C.c_int32, ip, ip], C.c_int),
+        "vib_solve": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.POINTER(VibSolveReport)], C.c_int),
+        "vib_solve_batch": ([vp, C.c_int32, ip, C.POINTER(VibSolveOptions), C.POINTER(VibSolveReport)], C.c_int),
+        "vib_solve_latest_two": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.POINTER(VibSolveReport)], C.c_int),
+        "vib_marginalize": ([vp, C.c_int32, C.c_int32, dp, C.c_int32, C.POINTER(VibImuParams),
+                             C.POINTER(VibMarginalReport)], C.c_int),
+        "vib_read_prior": ([vp, C.c_int32, ip, ip, ip, ip, lp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "vib_preintegrate": ([dp, C.c_int32, C.c_double, C.c_double, dp, C.POINTER(VibImuParams), dp, dp, dp], C.c_int),
+        "vib_pinhole_observations": ([C.c_double] * 5 + [C.c_int32, dp, dp, dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = lib
+    return lib
+
+
+def f64p(a):
+    return a.ctypes.data_as(dp)
+
+
+def imu_params(sigma_g_c=1e-3, sigma_a_c=1e-2, sigma_gw_c=2.6867875817855036e-06,
+               sigma_aw_c=4.8555923907289822e-06, g=9.81, g_max=35.0, a_max=160.0,
+               p_BS=(0.0, 0.0, 0.0), q_BS=(0.0, 0.0, 0.0, 1.0)):
+    p = VibImuParams()
+    p.sigma_g_c, p.sigma_a_c, p.sigma_gw_c, p.sigma_aw_c = sigma_g_c, sigma_a_c, sigma_gw_c, sigma_aw_c
+    p.g, p.g_max, p.a_max = g, g_max, a_max
+    p.p_BS[:] = list(p_BS)
+    p.q_BS[:] = list(q_BS)
+    return p
+
+
+class Context:
+    """Owning wrapper of a vib_context (device `device`)."""
+
+    def __init__(self, device=0, lib=None):
+        self.lib = lib or load_library()
+        h = C.c_void_p()
+        st = self.lib.vib_create(C.byref(h), device)
+        if st != VIB_OK:
+            raise RuntimeError(f"vib_create failed: {STATUS_NAMES.get(st, st)}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.vib_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self):
+        m = self.lib.vib_last_error(self.h)
+        return m.decode() if m else ""
+
+    def check(self, st, what):
+        if st != VIB_OK:
+            raise RuntimeError(f"{what}: {STATUS_NAMES.get(st, st)}: {self.error()}")
+        return st
+
+    def upload(self, slot, win):
+        s = win.cstruct()
+        return self.check(self.lib.vib_window_upload(self.h, slot, C.byref(s)), "vib_window_upload")
+
+    def read_state(self, slot, win):
+        s = win.cstruct()
+        return self.check(self.lib.vib_read_state(self.h, slot, C.byref(s)), "vib_read_state")
+
+    def linearize(self, slot, precision=VIB_FP64):
+        return self.lib.vib_linearize(self.h, slot, precision)
+
+    def solve_step(self, slot, lam, precision=VIB_FP64, refinement=0):
+        rep = VibStepReport()
+        st = self.lib.vib_solve_step(self.h, slot, lam, precision, refinement, C.byref(rep))
+        return st, rep
+
+    def dims(self, slot):
+        nr, nv = C.c_int32(), C.c_int32()
+        self.check(self.lib.vib_retained_dimension(self.h, slot, C.byref(nr), C.byref(nv)), "dims")
+        return nr.value, nv.value
+
+    def read_step(self, slot, n):
+        out = np.zeros(n)
+        self.check(self.lib.vib_read_step(self.h, slot, f64p(out), n), "vib_read_step")
+        return out
+
+    def read_reduced(self, slot, nr):
+        S = np.zeros((nr, nr))
+        rhs = np.zeros(nr)
+        self.check(self.lib.vib_read_reduced(self.h, slot, f64p(S), f64p(rhs), nr), "vib_read_reduced")
+        return S, rhs
+
+    def read_normal(self, slot, nr):
+        H = np.zeros((nr, nr))
+        g = np.zeros(nr)
+        cost = C.c_double()
+        self.check(self.lib.vib_read_normal(self.h, slot, f64p(H), f64p(g), C.byref(cost), nr), "vib_read_normal")
+        return H, g, cost.value
+
+    def solve(self, slot, opts):
+        rep = VibSolveReport()
+        st = self.lib.vib_solve(self.h, slot, C.byref(opts), C.byref(rep))
+        return st, rep
+
+    def solve_batch(self, slots, opts):
+        arr = np.ascontiguousarray(slots, dtype=np.int32)
+        reps = (VibSolveReport * len(arr))()
+        st = self.lib.vib_solve_batch(self.h, len(arr), arr.ctypes.data_as(ip), C.byref(opts), reps)
+        return st, list(reps)
+
+    def solve_latest_two(self, slot, opts):
+        rep = VibSolveReport()
+        st = self.lib.vib_solve_latest_two(self.h, slot, C.byref(opts), C.byref(rep))
+        return st, rep
+
+    def marginalize(self, slot, policy, samples=None, params=None):
+        rep = VibMarginalReport()
+        sp = None if samples is None else f64p(np.ascontiguousarray(samples, np.float64))
+        cnt = 0 if samples is None else int(samples.shape[0])
+        st = self.lib.vib_marginalize(self.h, slot, policy, sp, cnt,
+                                      None if params is None else C.byref(params), C.byref(rep))
+        return st, rep
+
+    def read_prior(self, slot, max_rows=2048, max_cols=2048, max_blocks=256):
+        rows, cols, blocks = C.c_int32(), C.c_int32(), C.c_int32()
+        kind = np.zeros(max_blocks, np.int32)
+        kid = np.zeros(max_blocks, np.int64)
+        ref = np.zeros(max_blocks * 9)
+        A = np.zeros(max_rows * max_cols)
+        r0 = np.zeros(max_rows)
+        st = self.lib.vib_read_prior(self.h, slot, C.byref(rows), C.byref(cols), C.byref(blocks),
+                                     kind.ctypes.data_as(ip), kid.ctypes.data_as(lp), f64p(ref), f64p(A),
+                                     f64p(r0), max_rows, max_cols, max_blocks)
+        self.check(st, "vib_read_prior")
+        m, n, b = rows.value, cols.value, blocks.value
+        return dict(A=A[:m * n].reshape(m, n), r0=r0[:m], kind=kind[:b], id=kid[:b], ref=ref)
+
+
+def preintegrate(samples, t0, t1, speed_bias, params, gyro_start, gyro_end, lib=None):
+    lib = lib or load_library()
+    out = np.zeros(IMU_STRIDE)
+    s = np.ascontiguousarray(samples, np.float64)
+    sb = np.ascontiguousarray(speed_bias, np.float64)
+    g0 = np.ascontiguousarray(gyro_start, np.float64)
+    g1 = np.ascontiguousarray(gyro_end, np.float64)
+    st = lib.vib_preintegrate(f64p(s), int(s.shape[0]), t0, t1, f64p(sb), C.byref(params), f64p(g0), f64p(g1),
+                              f64p(out))
+    if st != VIB_OK:
+        raise RuntimeError(f"vib_preintegrate: {STATUS_NAMES.get(st, st)}")
+    return out
+
+
+def pinhole_observations(fx, fy, cx, cy, sigma, pixels, lib=None):
+    """Vectorized whitening through the native library: returns (bearings (n,3), sqrt_info (n,3))."""
+    lib = lib or load_library()
+    px = np.ascontiguousarray(pixels, np.float64).reshape(-1, 2)
+    n = px.shape[0]
+    b = np.zeros((n, 3))
+    s = np.zeros((n, 3))
+    st = lib.vib_pinhole_observations(fx, fy, cx, cy, sigma, n, f64p(px), f64p(b), f64p(s))
+    if st != VIB_OK:
+        raise RuntimeError("vib_pinhole_observations failed")
+    return b, s

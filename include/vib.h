@@ -1,0 +1,275 @@
+/*
+ * vib.h -- C-ABI of the MI355X sliding-window visual-inertial Schur backend.
+ *
+ * This library is a drop-in for the `backend_visual_batch: schur` estimator option of
+ * CGMLI-SLAM (config/hilti_robot_estimator.yaml:224-225, parsed at
+ * src/common/ViParametersReader.cpp:354-359, mapped at src/pipeline/ThreadedSlam.cpp:33-38).
+ * It replaces the device layer `cgmli::backend::detail::VisualCudaBatch` /
+ * `MarginalCudaWorkspace` (src/backend/VisualCudaBatch.hpp:43-104) together with the host
+ * drivers that call it: the native LM loop `solveVisualSchur`
+ * (src/backend/BackendOptimizer.cpp:27-331), the two-frame solve `optimizeLatestTwo`
+ * (src/backend/BackendOptimizer.cpp:476-590) and the marginalizer
+ * (src/backend/Marginalizer.cpp:284-474).
+ *
+ * Conventions (identical to the reference, src/backend/README.md:45-54,111):
+ *   pose        7 doubles  [p_x p_y p_z q_x q_y q_z q_w]  (T_WB, Eigen xyzw quaternion)
+ *   speed/bias  9 doubles  [v_W(3) b_g(3) b_a(3)]
+ *   tangent     pose [rho(3) theta(3)] left perturbation; everything else additive
+ *   rig         T_BR, one shared extrinsic; fixed per-camera T_RC
+ *   landmark    inverse range rho = 1/|p_C| along the anchor bearing, rho >= 1e-8
+ *   td          camera-IMU time offset, t_IMU = t_sensor + td
+ *
+ * Every function returns an int status (VIB_OK = 0). A non-zero status corresponds to
+ * one of the reference's fallback reasons (src/backend/BackendOptimizer.cpp:32-308);
+ * vib_last_error() gives the message. No function throws.
+ *
+ * Threading: one context may be driven by one host thread at a time; the serving loop
+ * (vib_serve_*) is the only multi-request client and serializes internally.
+ */
+#ifndef VIB_H
+#define VIB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VIB_ABI_VERSION 1
+
+/* ---- status codes (reference fallback classes, src/backend/BackendOptimizer.cpp) ---- */
+enum {
+    VIB_OK = 0,
+    VIB_ERR_INVALID_ARGUMENT = 1,      /* malformed window / options                       */
+    VIB_ERR_UNSUPPORTED_DIMENSIONS = 2,/* "unsupported Schur dimensions"  (:67-68)          */
+    VIB_ERR_INVALID_RESIDUAL = 3,      /* invalid factor (rho<=0, |p_C|<1e-10, ...) (:231)   */
+    VIB_ERR_NOT_POSITIVE_DEFINITE = 4, /* damped RCS Cholesky failed (potrf info != 0)       */
+    VIB_ERR_COST_MISMATCH = 5,         /* "joint cost disagrees with accepted trial" (:236)  */
+    VIB_ERR_NO_ACCEPTABLE_STEP = 6,    /* "no acceptable CUDA Schur trial" (:308)            */
+    VIB_ERR_UNSUPPORTED_LOSS = 7,      /* "unsupported visual loss" (:112)                   */
+    VIB_ERR_DEVICE = 8,                /* HIP runtime failure                               */
+    VIB_ERR_OUT_OF_MEMORY = 9,         /* HBM budget / allocation failure                   */
+    VIB_ERR_NONFINITE_STEP = 10,       /* "nonfinite CUDA Schur step" (:254)                 */
+    VIB_ERR_INDEFINITE = 11,           /* "Schur prior is indefinite" (Marginalizer.cpp:444) */
+    VIB_ERR_EIGEN_FAILED = 12,         /* eigensolver did not converge                      */
+    VIB_ERR_NO_DEVICE = 13,            /* no HIP device                                     */
+    VIB_ERR_NOT_FOUND = 14,            /* unknown slot / session                            */
+    VIB_ERR_UNSUPPORTED = 15           /* unsupported configuration                          */
+};
+
+/* window flags */
+#define VIB_RIG_FREE 1u   /* estimate the shared T_BR online */
+#define VIB_TD_FREE 2u    /* estimate the camera-IMU time offset online */
+
+/* prior key kinds (reference BlockKind order, src/backend/BackendTypes.hpp:30-31) */
+#define VIB_KEY_POSE 0
+#define VIB_KEY_SPEED_BIAS 1
+#define VIB_KEY_RIG 2
+#define VIB_KEY_TD 3
+
+/* precision of a solve */
+#define VIB_FP64 0   /* plain FP64 everywhere */
+#define VIB_MIXED 1  /* FP32 streams/Jacobian products, FP64 accumulation, FP64 LDLT + refinement */
+
+/* marginalization policies (src/backend/SlidingWindowManager.cpp:503) */
+#define VIB_MARGIN_OLD 0
+#define VIB_MARGIN_SECOND_NEW 1
+
+/* Frozen IMU preintegration record, 301 doubles (reference FrozenImuData,
+ * src/backend/GpuRetainedFactors.hpp:7-14), every matrix row-major:
+ *   [0] duration  [1] gravity  [2..4] p_BS  [5..8] q_BS(xyzw)  [9..14] gyro at start/end
+ *   [15..18] dq(xyzw) [19..21] dv  [22..24] dp  [25..30] reference bias (b_g, b_a)
+ *   [31..39] d(alpha)/db_g  [40..48] dv/db_g  [49..57] dp/db_g  [58..66] dv/db_a (C integral)
+ *   [67..75] dp/db_a (C double integral)  [76..300] whitening W (15x15)                       */
+#define VIB_IMU_STRIDE 301
+
+/* Raw IMU sample: time [s], gyro (3) [rad/s], accel (3) [m/s^2]. */
+#define VIB_IMU_SAMPLE_STRIDE 7
+
+/* IMU noise model (reference ImuParameters, config/hilti_robot_estimator.yaml:120-140). */
+typedef struct vib_imu_params {
+    double sigma_g_c, sigma_a_c, sigma_gw_c, sigma_aw_c;
+    double g;                 /* gravity magnitude (9.81) */
+    double g_max, a_max;      /* saturation thresholds */
+    double p_BS[3];
+    double q_BS[4];           /* xyzw */
+} vib_imu_params;
+
+/* One sliding window, described by host pointers (copied on upload). Observations are the
+ * non-anchor visual factors (one per (landmark, observing camera) pair other than the anchor).
+ * Arrays marked (in/out) receive the solved state in vib_read_state(). */
+typedef struct vib_window {
+    int32_t num_frames;        /* K (window slots, oldest first)                      */
+    int32_t num_cameras;       /* C (fixed T_RC per camera)                           */
+    int32_t num_landmarks;     /* L                                                   */
+    int32_t num_observations;  /* F                                                   */
+    int32_t num_imu;           /* E (IMU edges)                                       */
+    uint32_t flags;            /* VIB_RIG_FREE | VIB_TD_FREE                          */
+    double huber_delta;        /* whitened Huber scale; 0 disables the robust loss    */
+    double min_inverse_depth;  /* lower bound of rho (reference default 1e-8)         */
+    double td_lower, td_upper; /* bounds of td when free                              */
+    /* frames */
+    const int64_t* frame_id;   /* [K] stable ids; prior keys refer to them            */
+    double* pose;              /* [K*7] (in/out)                                      */
+    double* speed_bias;        /* [K*9] (in/out)                                      */
+    const uint8_t* pose_free;  /* [K] 1 free / 0 constant; NULL = all free            */
+    const uint8_t* sb_width;   /* [K] 9 full, 3 velocity only, 0 constant; NULL = 9   */
+    /* calibration */
+    double* rig;               /* [7] T_BR (in/out)                                   */
+    double* td;                /* [1] (in/out)                                        */
+    const double* cam_rotation;    /* [C*9] R_RC row-major                            */
+    const double* cam_translation; /* [C*3] t_RC                                      */
+    /* landmarks (anchor observation + inverse range) */
+    const int32_t* lm_anchor_frame; /* [L] window slot of the anchor                   */
+    const int32_t* lm_anchor_cam;   /* [L]                                            */
+    const double* lm_bearing;       /* [L*3] raw anchor bearing                       */
+    const double* lm_velocity;      /* [L*3] anchor bearing velocity                  */
+    const double* lm_tref;          /* [L] anchor reference time offset               */
+    double* lm_rho;                 /* [L] (in/out)                                   */
+    const uint8_t* lm_free;         /* [L] NULL = all free                            */
+    /* visual factors */
+    const int32_t* obs_landmark;    /* [F]                                            */
+    const int32_t* obs_frame;       /* [F] window slot of the observer                */
+    const int32_t* obs_cam;         /* [F]                                            */
+    const double* obs_bearing;      /* [F*3] raw measured bearing                     */
+    const double* obs_velocity;     /* [F*3] measured bearing velocity                */
+    const double* obs_tref;         /* [F] reference time offset                      */
+    const double* obs_sqrt_info;    /* [F*3] (s00, s10, s11) of the lower-triangular 2x2
+                                       square-root information in the sphere basis      */
+    /* IMU factors */
+    const int32_t* imu_from;        /* [E] window slot of the first state             */
+    const int32_t* imu_to;          /* [E] window slot of the second state            */
+    const double* imu_data;         /* [E*VIB_IMU_STRIDE]                             */
+    /* square-root prior r = r0 + A * (x [-] x_ref) (MarginalizationPrior.cpp:20-66) */
+    int32_t prior_rows;             /* m (0 = no prior)                               */
+    int32_t prior_blocks;           /* number of keys                                 */
+    const int32_t* prior_key_kind;  /* [B] VIB_KEY_*                                  */
+    const int64_t* prior_key_id;    /* [B] frame id (POSE/SPEED_BIAS), 0 otherwise    */
+    const double* prior_ref;        /* concatenated reference values (7/9/7/1 per key) */
+    const double* prior_A;          /* [m * n] row-major, n = sum of tangent sizes     */
+    const double* prior_r0;         /* [m]                                            */
+} vib_window;
+
+typedef struct vib_solve_options {
+    int32_t max_iterations;     /* LM iteration budget (reference: maxIterations)       */
+    int32_t min_iterations;
+    double initial_lambda;      /* 1e-4 in the reference (:167)                          */
+    double time_limit_seconds;  /* <= 0: unlimited                                       */
+    int32_t precision;          /* VIB_FP64 / VIB_MIXED                                  */
+    int32_t refinement_steps;   /* iterative-refinement steps of the RCS solve           */
+} vib_solve_options;
+
+typedef struct vib_step_report {    /* reference VisualSchurStep (VisualCudaBatch.hpp:38-42) */
+    double cost;                /* robust cost at the linearization point              */
+    double predicted_reduction; /* model decrease of the clamped step                  */
+    double trial_cost;          /* cost at x [+] delta                                 */
+    double max_increment;       /* max |delta|                                         */
+    int32_t trial_valid;
+    int32_t status;
+} vib_step_report;
+
+typedef struct vib_solve_report {
+    int32_t status;
+    int32_t iterations;
+    int32_t accepted_steps;
+    int32_t converged;
+    int32_t effective_parameters; /* retained + eliminated tangent dimension */
+    int32_t retained_dimension;   /* nr */
+    double initial_cost;
+    double final_cost;
+    double final_lambda;
+    double seconds;               /* wall time of the solve on the host */
+} vib_solve_report;
+
+typedef struct vib_marginal_report {
+    int32_t status;
+    int32_t prior_reused;        /* SECOND_NEW with an untouched prior                   */
+    int32_t eliminated_dimension;
+    int32_t retained_dimension;
+    int32_t prior_rank;
+    int32_t eliminated_landmarks;
+    int32_t consumed_factors;
+    double eigen_cutoff;
+} vib_marginal_report;
+
+typedef struct vib_context vib_context;
+
+/* ---------------- context ---------------- */
+int vib_abi_version(void);
+int vib_create(vib_context** ctx, int device);
+int vib_destroy(vib_context* ctx);
+/* Last error message of the context (reference takeVisualCudaRuntimeError, VisualCudaBatch.cu:399-402). */
+const char* vib_last_error(const vib_context* ctx);
+
+/* ---------------- window slots (device-resident sessions) ---------------- */
+/* Upload a window into slot `slot` (>=0); replaces any previous content. Topology, measurement
+ * streams, IMU and prior are copied to HBM (reference linearizeSchur(newTopology) +
+ * configureRetained + configureResident, VisualCudaBatch.cu:433-511,606-624,673-691). */
+int vib_window_upload(vib_context* ctx, int32_t slot, const vib_window* window);
+/* Bytes of HBM a window of this shape occupies (admission accounting). */
+int64_t vib_window_footprint(const vib_window* window, int32_t precision);
+int vib_window_release(vib_context* ctx, int32_t slot);
+/* Copy the resident state back into the (in/out) arrays of `window` (readResident, :704-713). */
+int vib_read_state(vib_context* ctx, int32_t slot, vib_window* window);
+/* Shape of the resident window after marginalizations. */
+int vib_window_shape(vib_context* ctx, int32_t slot, int32_t* frames, int32_t* landmarks,
+                     int32_t* observations, int32_t* imu, int32_t* prior_rows, int32_t* prior_cols);
+
+/* ---------------- one LM iteration, exposed piecewise (VisualCudaBatch API) ---------------- */
+/* Linearize visual + IMU + prior factors at the resident state (linearizeResident, :693-700). */
+int vib_linearize(vib_context* ctx, int32_t slot, int32_t precision);
+/* Damped Schur step, back-substitution, resident trial and trial cost (solveOnline, :530-604). */
+int vib_solve_step(vib_context* ctx, int32_t slot, double lambda, int32_t precision,
+                   int32_t refinement_steps, vib_step_report* report);
+/* Adopt the last trial state (acceptResident, :702). */
+int vib_accept(vib_context* ctx, int32_t slot);
+/* Debug/parity read-outs of the last step: full tangent step [states, rho] (length nr + L)
+ * and the damped reduced camera system (nr*nr row-major, upper triangle meaningful) + rhs. */
+int vib_read_step(vib_context* ctx, int32_t slot, double* delta, int32_t capacity);
+int vib_read_reduced(vib_context* ctx, int32_t slot, double* S, double* rhs, int32_t capacity);
+/* Undamped normal equations of the retained block (readNormal, :651-671): H (nr*nr) and g. */
+int vib_read_normal(vib_context* ctx, int32_t slot, double* H, double* g, double* cost, int32_t capacity);
+int vib_retained_dimension(vib_context* ctx, int32_t slot, int32_t* nr, int32_t* nv);
+
+/* ---------------- full solves ---------------- */
+/* Native LM (solveVisualSchur, BackendOptimizer.cpp:27-331). */
+int vib_solve(vib_context* ctx, int32_t slot, const vib_solve_options* options, vib_solve_report* report);
+/* Many independent windows in lock-step batched launches (one graph per LM iteration). */
+int vib_solve_batch(vib_context* ctx, int32_t count, const int32_t* slots,
+                    const vib_solve_options* options, vib_solve_report* reports);
+/* Two-frame lightweight solve (optimizeLatestTwo, BackendOptimizer.cpp:476-590): free blocks
+ * pose/velocity of slots K-2 and K-1, landmarks and biases constant, <= 3 iterations. */
+int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* options,
+                         vib_solve_report* report);
+
+/* ---------------- marginalization ---------------- */
+/* OLD: Schur-eliminate slot 0 (pose, speed/bias) plus every landmark anchored in or observed
+ * by it into a new square-root prior (Marginalizer::compute, Marginalizer.cpp:284-474).
+ * SECOND_NEW: drop slot K-2's visual factors, merge its two IMU edges by re-integrating the
+ * raw samples given here, reuse the prior (SlidingWindowManager.cpp:275-311). */
+int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy,
+                    const double* imu_samples, int32_t sample_count,
+                    const vib_imu_params* imu_params, vib_marginal_report* report);
+/* Read the resident prior: keys, references, A (rows x cols row-major) and r0. */
+int vib_read_prior(vib_context* ctx, int32_t slot, int32_t* rows, int32_t* cols, int32_t* blocks,
+                   int32_t* key_kind, int64_t* key_id, double* ref, double* A, double* r0,
+                   int32_t capacity_rows, int32_t capacity_cols, int32_t capacity_blocks);
+
+/* ---------------- host-side helpers (CPU only, no device) ---------------- */
+/* IMU preintegration between t0 and t1 at the given speed/bias linearization point,
+ * written as one VIB_IMU_STRIDE record (ImuError::redoPreintegration, ImuError.cpp:254-462). */
+int vib_preintegrate(const double* samples, int32_t count, double t0, double t1,
+                     const double* speed_bias, const vib_imu_params* params,
+                     const double* gyro_start, const double* gyro_end, double* out);
+/* Observation whitening for `count` pixels (u, v interleaved) of an undistorted pinhole camera
+ * with pixel sigma: unit bearings (count*3) and (s00, s10, s11) of the lower-triangular
+ * square-root information in the sphere basis (FrameStore::observation, FrameStore.cpp:102-124;
+ * PinholeCamera::backProject, PinholeCamera.hpp:778-807). */
+int vib_pinhole_observations(double fx, double fy, double cx, double cy, double sigma, int32_t count,
+                             const double* pixels, double* bearings, double* sqrt_info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VIB_H */

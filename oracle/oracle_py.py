@@ -1,0 +1,181 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/build/liboracle.so).
+
+Test infrastructure only: loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg to check (or time, as a reported baseline) the native HIP path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-B", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def load(abi):
+    """Open liboracle.so; `abi` is the vibpy.abi module (struct definitions)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    lp = C.POINTER(C.c_int64)
+    W = C.POINTER(abi.VibWindow)
+    lib.oracle_visual_factor.argtypes = [W, C.c_int, dp, dp, dp]
+    lib.oracle_visual_residual_reference.argtypes = [W, C.c_int, dp]
+    lib.oracle_imu_factor.argtypes = [W, C.c_int, dp, dp]
+    lib.oracle_linearize.argtypes = [W, dp, dp, dp, dp, dp, ip]
+    lib.oracle_solve_step.argtypes = [W, C.c_double, dp, dp, dp, dp]
+    lib.oracle_cost.argtypes = [W, dp]
+    lib.oracle_solve.argtypes = [W, C.POINTER(abi.VibSolveOptions), C.POINTER(abi.VibSolveReport)]
+    lib.oracle_marginalize_old.argtypes = [W, ip, dp, dp, dp, dp, ip, lp, dp, dp]
+    lib.oracle_preintegrate.argtypes = [dp, C.c_int, C.c_double, C.c_double, dp, C.POINTER(abi.VibImuParams), dp, dp, dp]
+    lib.oracle_pinhole_observation.argtypes = [C.c_double] * 7 + [dp, dp]
+    lib.oracle_eigen_sym.argtypes = [dp, C.c_int, dp, dp]
+    lib.oracle_pose_plus.argtypes = [dp, dp, dp]
+    lib.oracle_pose_minus.argtypes = [dp, dp, dp]
+    for f in ("oracle_pose_plus", "oracle_pose_minus"):
+        getattr(lib, f).restype = None
+    _LIB = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    def __init__(self, abi):
+        self.abi = abi
+        self.lib = load(abi)
+
+    def visual_factor(self, win, f):
+        s = win.cstruct()
+        r = np.zeros(2)
+        J = np.zeros((2, 20))
+        amb = np.zeros((2, 23))
+        ok = self.lib.oracle_visual_factor(C.byref(s), f, _p(r), _p(J), _p(amb))
+        return bool(ok), r, J, amb
+
+    def visual_residual_reference(self, win, f):
+        s = win.cstruct()
+        r = np.zeros(2)
+        ok = self.lib.oracle_visual_residual_reference(C.byref(s), f, _p(r))
+        return bool(ok), r
+
+    def imu_factor(self, win, e):
+        s = win.cstruct()
+        r = np.zeros(15)
+        J = np.zeros((4, 15, 9))
+        ok = self.lib.oracle_imu_factor(C.byref(s), e, _p(r), _p(J))
+        return bool(ok), r, J
+
+    def linearize(self, win):
+        s = win.cstruct()
+        ne = C.c_int32()
+        nr = self.lib.oracle_linearize(C.byref(s), None, None, None, None, None, C.byref(ne))
+        ne = ne.value
+        H = np.zeros((nr, nr))
+        cross = np.zeros((ne, nr))
+        diag = np.zeros(ne)
+        g = np.zeros(nr + ne)
+        cost = C.c_double()
+        st = self.lib.oracle_linearize(C.byref(s), _p(H), _p(cross), _p(diag), _p(g), C.byref(cost), None)
+        if st < 0:
+            raise RuntimeError("oracle linearization invalid")
+        return dict(H=H, cross=cross, diag=diag, g=g, cost=cost.value, nr=nr, ne=ne)
+
+    def solve_step(self, win, lam):
+        lay = win.layout()
+        nr, ne = lay["nr"], lay["ne"]
+        delta = np.zeros(nr + ne)
+        S = np.zeros((nr, nr))
+        rhs = np.zeros(nr)
+        m = np.zeros(5)
+        s = win.cstruct()
+        st = self.lib.oracle_solve_step(C.byref(s), lam, _p(delta), _p(S), _p(rhs), _p(m))
+        return st, dict(delta=delta, S=S, rhs=rhs, cost=m[0], predicted=m[1], trial_cost=m[2], max_increment=m[3],
+                        trial_valid=bool(m[4]))
+
+    def cost(self, win):
+        s = win.cstruct()
+        c = C.c_double()
+        st = self.lib.oracle_cost(C.byref(s), C.byref(c))
+        return st, c.value
+
+    def solve(self, win, opts):
+        """Solves in place (win's state arrays receive the result)."""
+        s = win.cstruct()
+        rep = self.abi.VibSolveReport()
+        st = self.lib.oracle_solve(C.byref(s), C.byref(opts), C.byref(rep))
+        return st, rep
+
+    def marginalize_old(self, win):
+        s = win.cstruct()
+        sizes = np.zeros(5, np.int32)
+        ip = C.POINTER(C.c_int32)
+        st = self.lib.oracle_marginalize_old(C.byref(s), sizes.ctypes.data_as(ip), None, None, None, None, None, None,
+                                             None, None)
+        if st != 0:
+            return st, None
+        n, rank, blocks, reflen = int(sizes[0]), int(sizes[1]), int(sizes[2]), int(sizes[3])
+        H = np.zeros((n, n))
+        b = np.zeros(n)
+        A = np.zeros((rank, n))
+        r0 = np.zeros(rank)
+        kind = np.zeros(blocks, np.int32)
+        kid = np.zeros(blocks, np.int64)
+        ref = np.zeros(reflen)
+        cut = C.c_double()
+        st = self.lib.oracle_marginalize_old(C.byref(s), sizes.ctypes.data_as(ip), _p(H), _p(b), _p(A), _p(r0),
+                                             kind.ctypes.data_as(ip), kid.ctypes.data_as(C.POINTER(C.c_int64)), _p(ref),
+                                             C.byref(cut))
+        return st, dict(H=H, b=b, A=A, r0=r0, kind=kind, id=kid, ref=ref, cutoff=cut.value,
+                        landmarks=int(sizes[4]))
+
+    def preintegrate(self, samples, t0, t1, sb, params, g0, g1):
+        out = np.zeros(self.abi.IMU_STRIDE)
+        s = np.ascontiguousarray(samples, np.float64)
+        sb = np.ascontiguousarray(sb, np.float64)
+        g0 = np.ascontiguousarray(g0, np.float64)
+        g1 = np.ascontiguousarray(g1, np.float64)
+        st = self.lib.oracle_preintegrate(_p(s), s.shape[0], t0, t1, _p(sb), C.byref(params), _p(g0), _p(g1), _p(out))
+        return st, out
+
+    def pinhole_observation(self, fx, fy, cx, cy, u, v, sigma):
+        b = np.zeros(3)
+        s = np.zeros(3)
+        st = self.lib.oracle_pinhole_observation(fx, fy, cx, cy, u, v, sigma, _p(b), _p(s))
+        return st, b, s
+
+    def eigen(self, A):
+        A = np.ascontiguousarray(A, np.float64)
+        n = A.shape[0]
+        ev = np.zeros(n)
+        V = np.zeros((n, n))
+        st = self.lib.oracle_eigen_sym(_p(A), n, _p(ev), _p(V))
+        return st, ev, V
+
+    def pose_plus(self, x, d):
+        x = np.ascontiguousarray(x, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        y = np.zeros(7)
+        self.lib.oracle_pose_plus(_p(x), _p(d), _p(y))
+        return y
+
+    def pose_minus(self, x, ref):
+        x = np.ascontiguousarray(x, np.float64)
+        ref = np.ascontiguousarray(ref, np.float64)
+        d = np.zeros(6)
+        self.lib.oracle_pose_minus(_p(x), _p(ref), _p(d))
+        return d

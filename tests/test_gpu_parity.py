@@ -1,0 +1,141 @@
+"""HIP path vs the FP64 oracle on identical seeded windows (parity tests proper).
+
+Every call goes through the C-ABI of libvib.so. Tolerances (FP64 on both sides; the device
+forms tangent Jacobians directly and sums in a different order, and the prior chart derivative
+is a central difference with h = 1e-7 evaluated with the device's own sin/cos/atan2):
+  normal equations   ||dH|| / ||H|| <= 1e-8, ||dg|| / ||g|| <= 1e-8, cost rel 1e-10
+  reduced system     ||dS|| / ||S|| <= 1e-8
+  LM step            ||dx_gpu - dx_cpu|| / ||dx_cpu|| <= 1e-6
+  full solve         final cost rel 1e-8, states within 1e-6 (reference onlineSchurMatchesJoint-
+                     CpuSolve criteria: cost 1e-7, state 1e-4)
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx(abi):
+    c = abi.Context(0)
+    yield c
+    c.close()
+
+
+def small_windows():
+    from vibpy import synth
+    return [
+        ("k5_l150", dict(K=5, L=150, n_cams=4, obs_range=(2, 6), seed=11)),
+        ("k8_l400_calib", dict(K=8, L=400, n_cams=4, obs_range=(2, 8), seed=12, rig_free=True, td_free=True)),
+        ("k10_l600_c2", dict(K=10, L=600, n_cams=2, obs_range=(2, 10), seed=13, outlier_fraction=0.05)),
+    ]
+
+
+def rel(a, b):
+    d = np.linalg.norm(a - b)
+    n = max(np.linalg.norm(b), 1e-300)
+    return d / n
+
+
+@pytest.mark.parametrize("name,cfg", small_windows())
+def test_linearize_matches_oracle(ctx, oracle, abi, name, cfg):
+    from vibpy import synth
+    w = synth.generate(**cfg)
+    say(f"[gpu] linearize {name}: K={w.K} L={w.L} F={w.F}")
+    ctx.upload(0, w)
+    ctx.check(ctx.linearize(0), "linearize")
+    st, rep = ctx.solve_step(0, 1e-4)
+    assert st == 0, ctx.error()
+    nr, nv = ctx.dims(0)
+    H, g, cost = ctx.read_normal(0, nr)
+    ref = oracle.linearize(w)
+    assert ref["nr"] == nr
+    assert rel(H, ref["H"]) < 1e-8
+    assert rel(g, ref["g"][:nr]) < 1e-8
+    assert abs(cost - ref["cost"]) <= 1e-10 * abs(ref["cost"])
+
+
+@pytest.mark.parametrize("name,cfg", small_windows())
+def test_schur_step_matches_oracle(ctx, oracle, abi, name, cfg):
+    from vibpy import synth
+    w = synth.generate(**cfg)
+    lay = w.layout()
+    for lam in (1e-4, 1e-1):
+        say(f"[gpu] step {name} lambda={lam}")
+        ctx.upload(0, w)
+        ctx.check(ctx.linearize(0), "linearize")
+        st, rep = ctx.solve_step(0, lam)
+        assert st == 0, ctx.error()
+        nr = lay["nr"]
+        S, rhs = ctx.read_reduced(0, nr)
+        delta = ctx.read_step(0, nr + lay["ne"])
+        ost, ref = oracle.solve_step(w, lam)
+        assert ost == 0
+        assert rel(S, ref["S"]) < 1e-8
+        assert rel(rhs, ref["rhs"]) < 1e-8
+        assert rel(delta, ref["delta"]) < 1e-6
+        assert abs(rep.cost - ref["cost"]) <= 1e-10 * ref["cost"]
+        assert abs(rep.predicted_reduction - ref["predicted"]) <= 1e-6 * abs(ref["predicted"])
+        assert rep.trial_valid == ref["trial_valid"]
+        assert abs(rep.trial_cost - ref["trial_cost"]) <= 1e-6 * abs(ref["trial_cost"]) + 1e-9
+        assert abs(rep.max_increment - ref["max_increment"]) <= 1e-6 * ref["max_increment"]
+
+
+@pytest.mark.parametrize("name,cfg", small_windows())
+def test_full_solve_matches_oracle(ctx, oracle, abi, name, cfg):
+    from vibpy import synth
+    w = synth.generate(**cfg)
+    opts = abi.options(max_iterations=6)
+    say(f"[gpu] solve {name}")
+    ctx.upload(0, w)
+    st, rep = ctx.solve(0, opts)
+    assert st == 0, ctx.error()
+    got = w.copy()
+    ctx.read_state(0, got)
+    exp = w.copy()
+    ost, orep = oracle.solve(exp, opts)
+    assert ost == 0
+    assert rep.iterations == orep.iterations
+    assert rep.accepted_steps == orep.accepted_steps
+    assert abs(rep.final_cost - orep.final_cost) <= 1e-8 * orep.final_cost
+    assert abs(rep.initial_cost - orep.initial_cost) <= 1e-10 * orep.initial_cost
+    assert np.abs(got.pose - exp.pose).max() < 1e-6
+    assert np.abs(got.speed_bias - exp.speed_bias).max() < 1e-6
+    assert np.abs(got.lm_rho - exp.lm_rho).max() < 1e-6 * max(1.0, np.abs(exp.lm_rho).max())
+
+
+def test_batch_alone_equals_mixed(ctx, abi):
+    """Each window's result is the same solved alone and inside a batch of uneven windows."""
+    from vibpy import synth
+    cfgs = [dict(K=6, L=300, seed=21), dict(K=9, L=700, n_cams=3, seed=22, obs_range=(2, 9)),
+            dict(K=5, L=120, n_cams=2, seed=23), dict(K=7, L=500, seed=24, rig_free=True, td_free=True)]
+    wins = [synth.generate(**c) for c in cfgs]
+    opts = abi.options(max_iterations=4)
+    alone = []
+    for i, w in enumerate(wins):
+        say(f"[gpu] alone {i}")
+        ctx.upload(10 + i, w)
+        st, rep = ctx.solve(10 + i, opts)
+        assert st == 0, ctx.error()
+        out = w.copy()
+        ctx.read_state(10 + i, out)
+        alone.append((rep.final_cost, out))
+    for i, w in enumerate(wins):
+        ctx.upload(10 + i, w)
+    say("[gpu] batch of 4")
+    st, reps = ctx.solve_batch([10, 11, 12, 13], opts)
+    assert st == 0, ctx.error()
+    for i, w in enumerate(wins):
+        out = w.copy()
+        ctx.read_state(10 + i, out)
+        c0, o0 = alone[i]
+        assert abs(reps[i].final_cost - c0) <= 1e-10 * c0
+        assert np.abs(out.pose - o0.pose).max() < 1e-9
+        assert np.abs(out.lm_rho - o0.lm_rho).max() < 1e-9 * max(1.0, np.abs(o0.lm_rho).max())
