@@ -44,7 +44,8 @@ struct LmState {
     int32_t lin_valid, chol_ok, trial_valid;
     int32_t max_iterations;
     int32_t evaluate_only;  // piecewise API: compute metrics, never accept
-    int32_t pad[2];
+    int32_t continuous;     // benchmark mode: relinearize every iteration, never terminate
+    int32_t pad;
 };
 
 struct PriorBlockDev {

@@ -26,14 +26,9 @@
 
 extern "C" {
 int vibk_retained_shared_bytes(int E, int prior_rows, int prior_cols, int prior_blocks);
-hipError_t vibk_linearize(vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, size_t ret_shared,
-                          hipStream_t s);
-hipError_t vibk_reduce(vib::WinDev* wins, int nwin, int which, double** outs, int max_len, hipStream_t s);
-hipError_t vibk_schur(vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, hipStream_t s);
-hipError_t vibk_assemble(vib::WinDev* wins, int nwin, double* const* reduced, hipStream_t s);
-hipError_t vibk_cholesky(vib::WinDev* wins, int nwin, int refinement, hipStream_t s);
-hipError_t vibk_backsub_trial(vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, hipStream_t s);
-hipError_t vibk_lm_update(vib::WinDev* wins, int nwin, int mode, size_t ret_shared, hipStream_t s);
+const char* vibk_name(int k);
+hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
+                       int max_len, size_t ret_shared, int refinement, int mode, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 }
 
@@ -372,18 +367,16 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     return true;
 }
 
-bool Batch::linearize(hipStream_t s) {
-    return vibk_linearize(d_wins, nwin, d_chunks, nchunks, ret_shared, s) == hipSuccess &&
-           vibk_reduce(d_wins, nwin, 0, d_red, max_len, s) == hipSuccess;
+bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
+    return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_red, max_len, ret_shared, refinement, mode, s) == hipSuccess;
 }
 
+bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(1, s, 0, 0) && launch(2, s, 0, 0); }
+
 bool Batch::step(hipStream_t s, int refinement, int mode) {
-    return vibk_schur(d_wins, d_chunks, nchunks, s) == hipSuccess &&
-           vibk_reduce(d_wins, nwin, 1, d_red, max_len, s) == hipSuccess &&
-           vibk_assemble(d_wins, nwin, d_red, s) == hipSuccess &&
-           vibk_cholesky(d_wins, nwin, refinement, s) == hipSuccess &&
-           vibk_backsub_trial(d_wins, d_chunks, nchunks, s) == hipSuccess &&
-           vibk_lm_update(d_wins, nwin, mode, ret_shared, s) == hipSuccess;
+    for (int k = 3; k < kKernelCount; ++k)
+        if (!launch(k, s, refinement, mode)) return false;
+    return true;
 }
 
 bool Batch::iteration(hipStream_t s, int refinement) { return linearize(s) && step(s, refinement, 0); }
@@ -458,6 +451,7 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
         LmState lm{};
         lm.lambda = o.initial_lambda > 0 ? o.initial_lambda : 1e-4;
         lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = maxit;
+        lm.continuous = (o.flags & VIB_SOLVE_CONTINUOUS) ? 1 : 0;
         if (!write_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
     }
     Batch batch;
@@ -668,7 +662,7 @@ int vib_accept(vib_context* ctx, int32_t slot) {
     std::vector<Slot*> one{s};
     Batch b;
     std::string err;
-    if (!b.build(one, ctx->stream, err) || vibk_lm_update(b.d_wins, 1, 2, b.ret_shared, ctx->stream) != hipSuccess ||
+    if (!b.build(one, ctx->stream, err) || !b.launch(9, ctx->stream, 0, 2) ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(ctx, VIB_ERR_DEVICE, "accept failed");
     return VIB_OK;
@@ -742,6 +736,71 @@ int vib_solve_batch(vib_context* ctx, int32_t count, const int32_t* ids, const v
     const int st = run_lm(ctx, slots, *o, reps);
     for (size_t i = 0; i < reps.size(); ++i) reports[i] = reps[i];
     return st;
+}
+
+int vib_kernel_count(void) { return kKernelCount; }
+
+const char* vib_kernel_name(int32_t k) { return vibk_name(k); }
+
+int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_solve_options* o, int32_t iterations,
+              double* iteration_ms, double* kernel_ms) {
+    if (!ctx || !o || !ids || count <= 0 || iterations <= 0) return VIB_ERR_INVALID_ARGUMENT;
+    if (o->precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    std::vector<Slot*> slots;
+    for (int i = 0; i < count; ++i) {
+        Slot* s = find_slot(ctx, ids[i]);
+        if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+        slots.push_back(s);
+    }
+    for (Slot* s : slots) {
+        LmState lm{};
+        lm.lambda = o->initial_lambda > 0 ? o->initial_lambda : 1e-4;
+        lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1 << 30; lm.continuous = 1;
+        if (!write_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
+    }
+    Batch batch;
+    std::string err;
+    if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    if (!batch.capture(ctx->stream, o->refinement_steps)) return fail(ctx, VIB_ERR_DEVICE, "graph capture failed");
+    std::vector<hipEvent_t> ev(iterations + 1);
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(ctx, VIB_ERR_DEVICE, "event creation failed");
+    bool ok = hipEventRecord(ev[0], ctx->stream) == hipSuccess;
+    for (int it = 0; ok && it < iterations; ++it)
+        ok = hipGraphLaunch(batch.graph_exec, ctx->stream) == hipSuccess && hipEventRecord(ev[it + 1], ctx->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
+    for (int it = 0; ok && it < iterations; ++it) {
+        float ms = 0;
+        ok = hipEventElapsedTime(&ms, ev[it], ev[it + 1]) == hipSuccess;
+        if (iteration_ms) iteration_ms[it] = ms;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, std::string("timed iterations failed: ") + hipGetErrorString(hipGetLastError()));
+    if (kernel_ms) {
+        std::vector<hipEvent_t> kev(2 * kKernelCount * iterations);
+        for (auto& e : kev)
+            if (hipEventCreate(&e) != hipSuccess) return fail(ctx, VIB_ERR_DEVICE, "event creation failed");
+        for (int it = 0; ok && it < iterations; ++it)
+            for (int k = 0; ok && k < kKernelCount; ++k) {
+                hipEvent_t* e = &kev[2 * (it * kKernelCount + k)];
+                ok = hipEventRecord(e[0], ctx->stream) == hipSuccess && batch.launch(k, ctx->stream, o->refinement_steps, 0) &&
+                     hipEventRecord(e[1], ctx->stream) == hipSuccess;
+            }
+        ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
+        for (int k = 0; k < kKernelCount; ++k) kernel_ms[k] = 0;
+        for (int it = 0; ok && it < iterations; ++it)
+            for (int k = 0; ok && k < kKernelCount; ++k) {
+                float ms = 0;
+                const hipEvent_t* e = &kev[2 * (it * kKernelCount + k)];
+                ok = hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess;
+                kernel_ms[k] += ms / iterations;
+            }
+        for (auto& e : kev) (void)hipEventDestroy(e);
+        if (!ok) return fail(ctx, VIB_ERR_DEVICE, "kernel profiling failed");
+    }
+    return VIB_OK;
 }
 
 int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* o, vib_solve_report* rep) {

@@ -13,6 +13,7 @@
 namespace vib {
 
 constexpr int kMaxFactorsPerLandmark = 13;
+constexpr int kKernelCount = 10;  // kernels of one LM iteration (vibk_launch ids 0..9)
 
 // Host copy of one window: observation streams sorted by landmark, tangent column layout
 // [pose (6, free) ..., rig (6), td (1), speed/bias (width) ...] then free inverse ranges,
@@ -77,6 +78,7 @@ struct Batch {
 
     ~Batch();
     bool build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err);
+    bool launch(int kernel, hipStream_t s, int refinement, int mode);
     bool linearize(hipStream_t s);
     bool step(hipStream_t s, int refinement, int mode);
     bool iteration(hipStream_t s, int refinement);

@@ -1293,8 +1293,13 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
                     }
                 }
                 lm->iteration = i + 1;
-                if (lm->status == 0 && lm->iteration >= lm->max_iterations) lm->status = 1;
-                if (lm->status == 1 && !lm->accepted && !lm->converged) lm->status = -6;
+                if (lm->continuous) {
+                    if (lm->status > 0) lm->status = 0;
+                    lm->new_point = 1;
+                } else {
+                    if (lm->status == 0 && lm->iteration >= lm->max_iterations) lm->status = 1;
+                    if (lm->status == 1 && !lm->accepted && !lm->converged) lm->status = -6;
+                }
             }
         }
     }
@@ -1327,6 +1332,10 @@ __global__ __launch_bounds__(kBlock) void k_prior_gram(const WinDev* wins, doubl
 // ---------------------------------------------------------------------------------------------
 // Launch wrappers used by the host driver.
 // ---------------------------------------------------------------------------------------------
+static void allow_dynamic_lds(const void* fn, size_t bytes) {
+    if (bytes > 48 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 extern "C" {
 
 int vibk_retained_shared_bytes(int E, int prior_rows, int prior_cols, int prior_blocks) {
@@ -1335,50 +1344,44 @@ int vibk_retained_shared_bytes(int E, int prior_rows, int prior_cols, int prior_
     return 8 * (imu > prior ? imu : prior);
 }
 
-static void allow_dynamic_lds(const void* fn, size_t bytes) {
-    if (bytes > 48 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+// One launch per kernel id (the LM iteration is the sequence 0..9).
+const char* vibk_name(int k) {
+    static const char* names[] = {"k_lin_visual", "k_lin_retained", "k_reduce_lin", "k_schur", "k_reduce_schur",
+                                  "k_assemble", "k_cholesky", "k_backsub", "k_trial_cost", "k_lm_update"};
+    return k >= 0 && k < 10 ? names[k] : "";
 }
 
-hipError_t vibk_linearize(vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, size_t ret_shared,
-                          hipStream_t s) {
-    allow_dynamic_lds((const void*)vib::k_lin_retained, ret_shared);
-    if (nchunks) hipLaunchKernelGGL(vib::k_lin_visual, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks);
-    hipLaunchKernelGGL(vib::k_lin_retained, dim3(nwin), dim3(vib::kBlock), ret_shared, s, wins);
-    return hipGetLastError();
-}
-
-hipError_t vibk_reduce(vib::WinDev* wins, int nwin, int which, double** outs, int max_len, hipStream_t s) {
-    const int slices = (max_len + vib::kBlock - 1) / vib::kBlock;
-    hipLaunchKernelGGL(vib::k_reduce, dim3(nwin, slices), dim3(vib::kBlock), 0, s, wins, which, outs);
-    return hipGetLastError();
-}
-
-hipError_t vibk_schur(vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, hipStream_t s) {
-    if (nchunks) hipLaunchKernelGGL(vib::k_schur, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks);
-    return hipGetLastError();
-}
-
-hipError_t vibk_assemble(vib::WinDev* wins, int nwin, double* const* reduced, hipStream_t s) {
-    hipLaunchKernelGGL(vib::k_assemble, dim3(nwin), dim3(1024), 0, s, wins, reduced);
-    return hipGetLastError();
-}
-
-hipError_t vibk_cholesky(vib::WinDev* wins, int nwin, int refinement, hipStream_t s) {
-    hipLaunchKernelGGL(vib::k_cholesky, dim3(nwin), dim3(vib::kBlock), 0, s, wins, refinement);
-    return hipGetLastError();
-}
-
-hipError_t vibk_backsub_trial(vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, hipStream_t s) {
-    if (nchunks) {
-        hipLaunchKernelGGL(vib::k_backsub, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks);
-        hipLaunchKernelGGL(vib::k_trial_cost, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks);
+hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
+                       int max_len, size_t ret_shared, int refinement, int mode, hipStream_t s) {
+    using namespace vib;
+    const int slices = (max_len + kBlock - 1) / kBlock;
+    switch (k) {
+        case 0:
+            if (nchunks) hipLaunchKernelGGL(k_lin_visual, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            break;
+        case 1:
+            allow_dynamic_lds((const void*)k_lin_retained, ret_shared);
+            hipLaunchKernelGGL(k_lin_retained, dim3(nwin), dim3(kBlock), ret_shared, s, wins);
+            break;
+        case 2: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 0, red); break;
+        case 3:
+            if (nchunks) hipLaunchKernelGGL(k_schur, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            break;
+        case 4: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 1, red); break;
+        case 5: hipLaunchKernelGGL(k_assemble, dim3(nwin), dim3(1024), 0, s, wins, red); break;
+        case 6: hipLaunchKernelGGL(k_cholesky, dim3(nwin), dim3(kBlock), 0, s, wins, refinement); break;
+        case 7:
+            if (nchunks) hipLaunchKernelGGL(k_backsub, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            break;
+        case 8:
+            if (nchunks) hipLaunchKernelGGL(k_trial_cost, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            break;
+        case 9:
+            allow_dynamic_lds((const void*)k_lm_update, ret_shared);
+            hipLaunchKernelGGL(k_lm_update, dim3(nwin), dim3(kBlock), ret_shared, s, wins, mode);
+            break;
+        default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
-}
-
-hipError_t vibk_lm_update(vib::WinDev* wins, int nwin, int mode, size_t ret_shared, hipStream_t s) {
-    allow_dynamic_lds((const void*)vib::k_lm_update, ret_shared);
-    hipLaunchKernelGGL(vib::k_lm_update, dim3(nwin), dim3(vib::kBlock), ret_shared, s, wins, mode);
     return hipGetLastError();
 }
 

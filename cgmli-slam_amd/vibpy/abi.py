@@ -60,7 +60,7 @@ class VibWindow(C.Structure):
 class VibSolveOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_int32), ("min_iterations", C.c_int32),
                 ("initial_lambda", C.c_double), ("time_limit_seconds", C.c_double),
-                ("precision", C.c_int32), ("refinement_steps", C.c_int32)]
+                ("precision", C.c_int32), ("refinement_steps", C.c_int32), ("flags", C.c_uint32)]
 
 
 class VibStepReport(C.Structure):
@@ -82,10 +82,13 @@ class VibMarginalReport(C.Structure):
                 ("eigen_cutoff", C.c_double)]
 
 
+VIB_SOLVE_CONTINUOUS = 1
+
+
 def options(max_iterations=8, min_iterations=0, initial_lambda=1e-4, time_limit=0.0,
-            precision=VIB_FP64, refinement_steps=0):
+            precision=VIB_FP64, refinement_steps=0, flags=0):
     return VibSolveOptions(max_iterations, min_iterations, initial_lambda, time_limit, precision,
-                           refinement_steps)
+                           refinement_steps, flags)
 
 
 def _ptr(a, ctype):
@@ -295,6 +298,9 @@ def load_library(path=LIB_PATH):
         "vib_marginalize": ([vp, C.c_int32, C.c_int32, dp, C.c_int32, C.POINTER(VibImuParams),
                              C.POINTER(VibMarginalReport)], C.c_int),
         "vib_read_prior": ([vp, C.c_int32, ip, ip, ip, ip, lp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "vib_kernel_count": ([], C.c_int),
+        "vib_kernel_name": ([C.c_int32], C.c_char_p),
+        "vib_bench": ([vp, C.c_int32, ip, C.POINTER(VibSolveOptions), C.c_int32, dp, dp], C.c_int),
         "vib_preintegrate": ([dp, C.c_int32, C.c_double, C.c_double, dp, C.POINTER(VibImuParams), dp, dp, dp], C.c_int),
         "vib_pinhole_observations": ([C.c_double] * 5 + [C.c_int32, dp, dp, dp], C.c_int),
     }
@@ -401,6 +407,17 @@ class Context:
         reps = (VibSolveReport * len(arr))()
         st = self.lib.vib_solve_batch(self.h, len(arr), arr.ctypes.data_as(ip), C.byref(opts), reps)
         return st, list(reps)
+
+    def bench(self, slots, opts, iterations, kernels=True):
+        arr = np.ascontiguousarray(slots, dtype=np.int32)
+        it_ms = np.zeros(iterations)
+        nk = self.lib.vib_kernel_count()
+        k_ms = np.zeros(nk)
+        st = self.lib.vib_bench(self.h, len(arr), arr.ctypes.data_as(ip), C.byref(opts), iterations, f64p(it_ms),
+                                f64p(k_ms) if kernels else None)
+        self.check(st, "vib_bench")
+        names = [self.lib.vib_kernel_name(i).decode() for i in range(nk)]
+        return it_ms, dict(zip(names, k_ms))
 
     def solve_latest_two(self, slot, opts):
         rep = VibSolveReport()

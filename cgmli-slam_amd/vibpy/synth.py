@@ -296,6 +296,8 @@ def generate(K=10, L=2000, n_cams=4, obs_range=(2, 8), seed=20261204, rig_free=F
     rn = np.linalg.norm(pc, axis=1)
     bt = pc / rn[:, None]
     bvel = (pdot - bt * (bt * pdot).sum(1, keepdims=True)) / rn[:, None]
+    # a feature-track bearing derivative is tangent to the measured bearing itself
+    bvel = bvel - bearing * (bearing * bvel).sum(1, keepdims=True)
     # split anchors / factors
     first_idx = np.concatenate([[0], np.cumsum([len(o) for o in obs_list])[:-1]])
     is_anchor = np.zeros(len(all_pairs), bool)

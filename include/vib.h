@@ -150,6 +150,10 @@ typedef struct vib_window {
     const double* prior_r0;         /* [m]                                            */
 } vib_window;
 
+/* solve option flags */
+#define VIB_SOLVE_CONTINUOUS 1u  /* benchmark mode: relinearize every iteration, run the full
+                                    iteration budget without terminating on convergence  */
+
 typedef struct vib_solve_options {
     int32_t max_iterations;     /* LM iteration budget (reference: maxIterations)       */
     int32_t min_iterations;
@@ -157,6 +161,7 @@ typedef struct vib_solve_options {
     double time_limit_seconds;  /* <= 0: unlimited                                       */
     int32_t precision;          /* VIB_FP64 / VIB_MIXED                                  */
     int32_t refinement_steps;   /* iterative-refinement steps of the RCS solve           */
+    uint32_t flags;             /* VIB_SOLVE_*                                           */
 } vib_solve_options;
 
 typedef struct vib_step_report {    /* reference VisualSchurStep (VisualCudaBatch.hpp:38-42) */
@@ -241,6 +246,18 @@ int vib_solve_batch(vib_context* ctx, int32_t count, const int32_t* slots,
  * pose/velocity of slots K-2 and K-1, landmarks and biases constant, <= 3 iterations. */
 int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* options,
                          vib_solve_report* report);
+
+/* ---------------- measurement harness (no counterpart in the reference) ---------------- */
+/* Number of distinct kernels of one LM iteration and their names (for per-kernel timing). */
+int vib_kernel_count(void);
+const char* vib_kernel_name(int32_t kernel);
+/* Runs `iterations` continuous-mode LM iterations on the given slots (one graph replay per
+ * iteration, no host synchronization in between) and writes each iteration's device time,
+ * measured with events on the library's stream, to iteration_ms. Then runs `iterations` more
+ * iterations kernel by kernel with events around every launch and writes the average duration
+ * of each kernel (vib_kernel_count entries) to kernel_ms. Either output may be NULL. */
+int vib_bench(vib_context* ctx, int32_t count, const int32_t* slots, const vib_solve_options* options,
+              int32_t iterations, double* iteration_ms, double* kernel_ms);
 
 /* ---------------- marginalization ---------------- */
 /* OLD: Schur-eliminate slot 0 (pose, speed/bias) plus every landmark anchored in or observed
