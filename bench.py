@@ -98,18 +98,18 @@ def main():
     cfg = synth.CONFIGS[args.workload]
     win = synth.generate(**cfg)
     prec = abi.VIB_FP64
-    import torch  # device-wide synchronization around the timed region
     ctx = abi.Context(0)
     ctx.upload(0, win)
     opts = abi.options(max_iterations=1, precision=prec, flags=abi.VIB_SOLVE_CONTINUOUS)
     ctx.bench([0], opts, args.warmup, kernels=False)
-    torch.cuda.synchronize()
+    ctx.synchronize()  # device-wide barrier on both sides of the timed region
     t0 = time.perf_counter()
     it_ms, _ = ctx.bench([0], opts, args.steps, kernels=False)
-    torch.cuda.synchronize()
+    ctx.synchronize()
     wall = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
     _, k_ms = ctx.bench([0], opts, max(5, min(args.steps, 20)), kernels=True)
-    torch.cuda.synchronize()
+    ctx.synchronize()
     ctx.close()
 
     dom = max(k_ms, key=k_ms.get)

@@ -36,6 +36,8 @@ struct LmState {
     double lin_cost;        // cost reported by the last linearization
     double predicted, trial_cost, max_increment;
     double last_gain;
+    double ret_cost;        // retained (IMU + prior) cost at the linearization point
+    int32_t ret_invalid;
     int32_t new_point;      // linearize at the start of the next iteration
     int32_t status;         // 0 running, 1 finished (converged / budget), <0 -status error
     int32_t iteration;      // iterations started
@@ -45,7 +47,6 @@ struct LmState {
     int32_t max_iterations;
     int32_t evaluate_only;  // piecewise API: compute metrics, never accept
     int32_t continuous;     // benchmark mode: relinearize every iteration, never terminate
-    int32_t pad;
 };
 
 struct PriorBlockDev {
@@ -109,9 +110,7 @@ struct WinDev {
     double* schur_part;        // nchunks x (tri + nv)
     double* chunk_sc;          // nchunks x CS_PLANES
     int32_t chunk0, nchunks;
-    double ret_cost;           // retained cost at the linearization point (device-written)
-    int32_t ret_invalid, pad1;
-    LmState* lm;
+    LmState* lm;               // persistent per-window LM state (device)
 };
 
 struct Chunk {

@@ -523,6 +523,7 @@ int vib_create(vib_context** out, int device) {
     }
     const char* g = std::getenv("VIB_NO_GRAPH");
     ctx->use_graph = !(g && g[0] == '1');
+    if (const char* c = std::getenv("VIB_CHUNK")) HostWindow::chunk_size = std::max(8, std::atoi(c));
     *out = ctx;
     return VIB_OK;
 }
@@ -535,6 +536,12 @@ int vib_destroy(vib_context* ctx) {
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return VIB_OK;
+}
+
+int vib_device_synchronize(vib_context* ctx) {
+    if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
+    hipSetDevice(ctx->device);
+    return hipDeviceSynchronize() == hipSuccess ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "device synchronization failed");
 }
 
 const char* vib_last_error(const vib_context* ctx) { return ctx ? ctx->error.c_str() : "null context"; }

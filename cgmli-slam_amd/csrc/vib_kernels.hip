@@ -567,6 +567,135 @@ __device__ bool eval_imu(const double* d, const double* pose0, const double* sb0
     return ok;
 }
 
+// Quantities of one IMU edge shared by all Jacobian columns (GpuRetainedFactorMath.hpp:71-101).
+struct ImuShared {
+    ImuSensor s0, s1;
+    M3<double> R;
+    double dt;
+    V3<double> dp, dv;
+    double Jrot[9], Jbias[9];
+    double r[15];
+    bool ok;
+};
+
+__device__ void imu_shared(const double* d, const double* pose0, const double* sb0, const double* pose1,
+                           const double* sb1, bool jac, ImuShared& o) {
+    o.s0 = imu_sensor(d, pose0, sb0, 0);
+    o.s1 = imu_sensor(d, pose1, sb1, 1);
+    o.R = qmatrix(qconj(o.s0.q));
+    o.dt = d[0];
+    const V3<double> grav = {0, 0, d[1]};
+    const V3<double> bg = load3(sb0 + 3) - load3(d + 25), ba = load3(sb0 + 6) - load3(d + 28);
+    o.dp = o.s0.p - o.s1.p + o.dt * o.s0.v - (0.5 * o.dt * o.dt) * grav;
+    o.dv = o.s0.v - o.s1.v - o.dt * grav;
+    const V3<double> phi = -1.0 * m3v(d + 31, bg);
+    const Qt<double> corr = rotation_exp(phi);
+    const Qt<double> Dq = qload(d + 15);
+    const Qt<double> dq = qmul(corr, Dq);
+    const Qt<double> rel = qmul(qconj(o.s1.q), o.s0.q);
+    double e[15];
+    const V3<double> ep = mul(o.R, o.dp) + load3(d + 22) + m3v(d + 49, bg) - m3v(d + 67, ba);
+    const V3<double> eq = 2.0 * qvec(qmul(dq, rel));
+    const V3<double> ev = mul(o.R, o.dv) + load3(d + 19) + m3v(d + 40, bg) - m3v(d + 58, ba);
+    e[0] = ep.x; e[1] = ep.y; e[2] = ep.z; e[3] = eq.x; e[4] = eq.y; e[5] = eq.z; e[6] = ev.x; e[7] = ev.y; e[8] = ev.z;
+    for (int i = 0; i < 6; ++i) e[9 + i] = sb0[3 + i] - sb1[3 + i];
+    const double* W = d + 76;
+    o.ok = true;
+    for (int i = 0; i < 15; ++i) {
+        double v = 0;
+        for (int k = 0; k < 15; ++k) v += W[i * 15 + k] * e[k];
+        o.r[i] = v;
+        o.ok = o.ok && isfinite(v);
+    }
+    if (!jac) return;
+    double GA[9];
+    {
+        const V3<double> v = phi;  // rotationJacobian(-phi) * A, negated
+        const V3<double> mv = -1.0 * v;
+        const double a2 = sqnorm(mv), n = sqrt(a2);
+        const double c1 = a2 < 1e-10 ? 0.5 - a2 / 24 : (1 - cos(n)) / a2;
+        const double c2 = a2 < 1e-10 ? 1.0 / 6 - a2 / 120 : (n - sin(n)) / (a2 * n);
+        double Sk[9], S2[9], Jl[9];
+        skew_of(mv, Sk);
+        mm3(Sk, Sk, S2);
+        for (int i = 0; i < 9; ++i) Jl[i] = (i % 4 == 0 ? 1.0 : 0.0) + c1 * Sk[i] + c2 * S2[i];
+        mm3(Jl, d + 31, GA);
+        for (int i = 0; i < 9; ++i) GA[i] = -GA[i];
+    }
+    const Qt<double> tail = qmul(qmul(Dq, qconj(o.s1.q)), o.s0.q);
+    const Qt<double> left = qmul(dq, qconj(o.s1.q));
+    for (int j = 0; j < 3; ++j) {
+        const Qt<double> ax = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0, 0.0};
+        const V3<double> c1 = qvec(qmul(qmul(left, ax), o.s0.q));
+        const Qt<double> dbq = {GA[j], GA[3 + j], GA[6 + j], 0.0};
+        const V3<double> c2 = qvec(qmul(qmul(corr, dbq), tail));
+        o.Jrot[j] = c1.x; o.Jrot[3 + j] = c1.y; o.Jrot[6 + j] = c1.z;
+        o.Jbias[j] = c2.x; o.Jbias[3 + j] = c2.y; o.Jbias[6 + j] = c2.z;
+    }
+}
+
+// Column c (0..29) of the unwhitened IMU Jacobian [pose0 6 | sb0 9 | pose1 6 | sb1 9].
+__device__ void imu_raw_column(const double* d, const ImuShared& o, int c, double* col) {
+    for (int i = 0; i < 15; ++i) col[i] = 0;
+    const int k = c < 6 ? 0 : (c < 15 ? 1 : (c < 21 ? 2 : 3));
+    const int cc = c - (k == 0 ? 0 : (k == 1 ? 6 : (k == 2 ? 15 : 21)));
+    const bool first = k < 2;
+    const double sign = first ? 1.0 : -1.0;
+    const ImuSensor& s = first ? o.s0 : o.s1;
+    const double* R = o.R.m;
+    auto mcol = [&](const double* M, int j, double* out3) { out3[0] = M[j]; out3[1] = M[3 + j]; out3[2] = M[6 + j]; };
+    double v[3], tmp[9], Sk[9];
+    if (k % 2 == 0) {
+        if (cc < 3) {
+            mcol(R, cc, v);
+            for (int i = 0; i < 3; ++i) col[i] = sign * v[i];
+        } else {
+            const int j = cc - 3;
+            skew_of(s.p, Sk); mm3(R, Sk, tmp); mcol(tmp, j, v);
+            for (int i = 0; i < 3; ++i) col[i] = -sign * v[i];
+            mcol(o.Jrot, j, v);
+            for (int i = 0; i < 3; ++i) col[3 + i] = sign * v[i];
+            skew_of(s.lever, Sk); mm3(R, Sk, tmp); mcol(tmp, j, v);
+            for (int i = 0; i < 3; ++i) col[6 + i] = -sign * v[i];
+            if (first) {
+                double Sdp[9], Sl[9], M2[9];
+                skew_of(o.dp, Sdp); skew_of(s.lever, Sl);
+                for (int i = 0; i < 9; ++i) M2[i] = Sdp[i] - o.dt * Sl[i];
+                mm3(R, M2, tmp); mcol(tmp, j, v);
+                for (int i = 0; i < 3; ++i) col[i] += v[i];
+                skew_of(o.dv, Sk); mm3(R, Sk, tmp); mcol(tmp, j, v);
+                for (int i = 0; i < 3; ++i) col[6 + i] += v[i];
+            }
+        }
+    } else {
+        double lb[9];
+        const M3<double> Rs = qmatrix(s.q);
+        skew_of(s.arm, Sk); mm3(Sk, Rs.m, lb);
+        if (cc < 3) {
+            mcol(R, cc, v);
+            for (int i = 0; i < 3; ++i) col[6 + i] = sign * v[i];
+            if (first) for (int i = 0; i < 3; ++i) col[i] = o.dt * v[i];
+        } else if (cc < 6) {
+            const int j = cc - 3;
+            mm3(R, lb, tmp); mcol(tmp, j, v);
+            for (int i = 0; i < 3; ++i) col[6 + i] = sign * v[i];
+            col[9 + j] = sign;
+            if (first) {
+                for (int i = 0; i < 3; ++i) col[i] = o.dt * v[i] + d[49 + i * 3 + j];
+                for (int i = 0; i < 3; ++i) col[3 + i] = o.Jbias[i * 3 + j];
+                for (int i = 0; i < 3; ++i) col[6 + i] += d[40 + i * 3 + j];
+            }
+        } else {
+            const int j = cc - 6;
+            col[12 + j] = sign;
+            if (first) {
+                for (int i = 0; i < 3; ++i) col[i] = -d[67 + i * 3 + j];
+                for (int i = 0; i < 3; ++i) col[6 + i] = -d[58 + i * 3 + j];
+            }
+        }
+    }
+}
+
 // Prior chart of block b at state x, with the central-difference chart derivative (6x6, h=1e-7)
 // for SE(3) blocks (VisualCudaBatch.cu:216-246).
 __device__ const double* prior_state(const WinDev& w, const PriorBlockDev& b, const double* pose, const double* sb,
@@ -586,48 +715,67 @@ __device__ void retained_eval(const WinDev& w, const double* pose, const double*
     const int nr = w.nr;
     double cost = 0;
     int invalid = 0;
-    // IMU edges: each thread evaluates edges t, t+nt, ... residuals (+ Jacobian into sh)
-    // sh layout: E x (15 + 450)
+    // IMU edges: one wave per edge; lane c < 30 builds Jacobian column c (sh: E x (15 + 450))
     const int E = w.E;
-    for (int e = t; e < E; e += nt) {
+    const int wave = t / 64, lane = t % 64, nw = nt / 64;
+    for (int e = wave; e < E; e += nw) {
         double* rr = sh + (size_t)e * 465;
         const int s0 = w.imu_from[e], s1 = w.imu_to[e];
-        const bool ok = eval_imu(w.imu + (size_t)e * 301, pose + 7 * s0, sb + 9 * s0, pose + 7 * s1, sb + 9 * s1, rr,
-                                 cost_only ? nullptr : rr + 15);
-        if (!ok) invalid = 1;
-        for (int i = 0; i < 15; ++i) cost += 0.5 * rr[i] * rr[i];
+        const double* d = w.imu + (size_t)e * 301;
+        ImuShared o;
+        imu_shared(d, pose + 7 * s0, sb + 9 * s0, pose + 7 * s1, sb + 9 * s1, !cost_only, o);
+        if (lane == 0) {
+            if (!o.ok) invalid = 1;
+            for (int i = 0; i < 15; ++i) {
+                rr[i] = o.r[i];
+                cost += 0.5 * o.r[i] * o.r[i];
+            }
+        }
+        if (!cost_only && lane < 30) {
+            double col[15];
+            imu_raw_column(d, o, lane, col);
+            const double* W = d + 76;
+            for (int i = 0; i < 15; ++i) {
+                double v = 0;
+                for (int m = 0; m < 15; ++m) v += W[i * 15 + m] * col[m];
+                rr[15 + i * 30 + lane] = v;
+                if (!isfinite(v)) invalid = 1;
+            }
+        }
     }
     __syncthreads();
     if (!cost_only) {
         for (size_t i = t; i < (size_t)nr * nr; i += nt) w.Hret[i] = 0;
         for (int i = t; i < nr; i += nt) w.gret[i] = 0;
         __syncthreads();
-        for (int e = 0; e < E; ++e) {
-            const double* rr = sh + (size_t)e * 465;
-            const double* J = rr + 15;
-            const int s0 = w.imu_from[e], s1 = w.imu_to[e];
-            int col[4] = {w.pose_col[s0], w.sb_col[s0], w.pose_col[s1], w.sb_col[s1]};
-            int wid[4] = {col[0] >= 0 ? 6 : 0, col[1] >= 0 ? w.sb_w[s0] : 0, col[2] >= 0 ? 6 : 0,
-                          col[3] >= 0 ? w.sb_w[s1] : 0};
-            const int off[4] = {0, 6, 15, 21};
-            // 30 x 30 entries over the free columns
-            for (int p = t; p < 900; p += nt) {
-                const int i = p / 30, j = p % 30;
-                const int bi = i < 6 ? 0 : (i < 15 ? 1 : (i < 21 ? 2 : 3));
-                const int bj = j < 6 ? 0 : (j < 15 ? 1 : (j < 21 ? 2 : 3));
-                const int ii = i - off[bi], jj = j - off[bj];
-                if (ii >= wid[bi] || jj >= wid[bj]) continue;
-                double s = 0;
-                for (int m = 0; m < 15; ++m) s += J[m * 30 + i] * J[m * 30 + j];
-                w.Hret[(size_t)(col[bi] + ii) * nr + col[bj] + jj] += s;
-            }
-            for (int i = t; i < 30; i += nt) {
-                const int bi = i < 6 ? 0 : (i < 15 ? 1 : (i < 21 ? 2 : 3));
-                const int ii = i - off[bi];
-                if (ii >= wid[bi]) continue;
-                double s = 0;
-                for (int m = 0; m < 15; ++m) s += J[m * 30 + i] * rr[m];
-                w.gret[col[bi] + ii] += s;
+        // even edges, then odd edges: edges of one parity touch disjoint frame blocks
+        for (int parity = 0; parity < 2; ++parity) {
+            for (int e = parity; e < E; e += 2) {
+                const double* rr = sh + (size_t)e * 465;
+                const double* J = rr + 15;
+                const int s0 = w.imu_from[e], s1 = w.imu_to[e];
+                int col[4] = {w.pose_col[s0], w.sb_col[s0], w.pose_col[s1], w.sb_col[s1]};
+                int wid[4] = {col[0] >= 0 ? 6 : 0, col[1] >= 0 ? w.sb_w[s0] : 0, col[2] >= 0 ? 6 : 0,
+                              col[3] >= 0 ? w.sb_w[s1] : 0};
+                const int off[4] = {0, 6, 15, 21};
+                for (int p = t; p < 900; p += nt) {
+                    const int i = p / 30, j = p % 30;
+                    const int bi = i < 6 ? 0 : (i < 15 ? 1 : (i < 21 ? 2 : 3));
+                    const int bj = j < 6 ? 0 : (j < 15 ? 1 : (j < 21 ? 2 : 3));
+                    const int ii = i - off[bi], jj = j - off[bj];
+                    if (ii >= wid[bi] || jj >= wid[bj]) continue;
+                    double v = 0;
+                    for (int m = 0; m < 15; ++m) v += J[m * 30 + i] * J[m * 30 + j];
+                    w.Hret[(size_t)(col[bi] + ii) * nr + col[bj] + jj] += v;
+                }
+                for (int i = t; i < 30; i += nt) {
+                    const int bi = i < 6 ? 0 : (i < 15 ? 1 : (i < 21 ? 2 : 3));
+                    const int ii = i - off[bi];
+                    if (ii >= wid[bi]) continue;
+                    double v = 0;
+                    for (int m = 0; m < 15; ++m) v += J[m * 30 + i] * rr[m];
+                    w.gret[col[bi] + ii] += v;
+                }
             }
             __syncthreads();
         }
@@ -717,17 +865,17 @@ __device__ void retained_eval(const WinDev& w, const double* pose, const double*
     out_invalid = inv > 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_lin_retained(WinDev* wins) {
+__global__ __launch_bounds__(kBlock) void k_lin_retained(const WinDev* wins) {
     __shared__ double red[kBlock];
     extern __shared__ double sh[];
-    WinDev& w = wins[blockIdx.x];
+    const WinDev& w = wins[blockIdx.x];
     if (!window_active(w) || !w.lm->new_point) return;
     double cost;
     int invalid;
     retained_eval(w, w.pose, w.sb, w.rig, w.td, false, sh, red, cost, invalid);
     if (threadIdx.x == 0) {
-        w.ret_cost = cost;
-        w.ret_invalid = invalid;
+        w.lm->ret_cost = cost;
+        w.lm->ret_invalid = invalid;
     }
 }
 
@@ -864,209 +1012,217 @@ __global__ __launch_bounds__(kBlock) void k_schur(const WinDev* wins, const Chun
 // reduced camera system S and rhs (prepareStateSolve, VisualCudaBatch.cu:163-172).
 // red_lin / red_schur: reduced partials (tri + nv) per window.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_assemble(WinDev* wins, double* const* reduced) {
-    __shared__ double red[1024];
+constexpr int kAssembleRows = 16;  // rows of the reduced system per workgroup
+
+__global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const* reduced) {
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
     const int nr = w.nr, nv = w.nv;
+    const int row0 = blockIdx.y * kAssembleRows;
+    if (row0 >= nr) return;
+    const int row1 = min(nr, row0 + kAssembleRows);
     const double* rl = reduced[2 * blockIdx.x + 0];
     const double* rs = reduced[2 * blockIdx.x + 1];
     const int t = threadIdx.x, nt = blockDim.x;
-    if (w.lm->new_point) {
-        for (size_t p = t; p < (size_t)nr * nr; p += nt) {
+    const bool fresh = w.lm->new_point;
+    if (fresh) {
+        for (size_t p = (size_t)row0 * nr + t; p < (size_t)row1 * nr; p += nt) {
             const int i = (int)(p / nr), j = (int)(p % nr);
             double h = w.Hret[p];
             if (i < nv && j < nv) h += rl[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
             w.Hfull[p] = h;
         }
-        for (int i = t; i < nr; i += nt) w.gfull[i] = w.gret[i] + (i < nv ? rl[w.tri + i] : 0.0);
-        if (t == 0) {
+        for (int i = row0 + t; i < row1; i += nt) w.gfull[i] = w.gret[i] + (i < nv ? rl[w.tri + i] : 0.0);
+        if (blockIdx.y == 0 && t == 0) {
             double cost = 0;
-            int invalid = w.ret_invalid;
+            int invalid = w.lm->ret_invalid;
             for (int k = 0; k < w.nchunks; ++k) {
                 cost += w.chunk_sc[(size_t)k * CS_PLANES + CS_COST];
                 invalid |= w.chunk_sc[(size_t)k * CS_PLANES + CS_INVALID] > 0;
             }
-            cost += w.ret_cost;
+            cost += w.lm->ret_cost;
             w.lm->lin_cost = cost;
             w.lm->lin_valid = !invalid && isfinite(cost);
         }
         __syncthreads();
     }
     const double lambda = w.lm->lambda;
-    for (size_t p = t; p < (size_t)nr * nr; p += nt) {
+    for (size_t p = (size_t)row0 * nr + t; p < (size_t)row1 * nr; p += nt) {
         const int i = (int)(p / nr), j = (int)(p % nr);
-        double s = w.Hfull[p];
-        if (i < nv && j < nv) s += rs[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
-        if (i == j) s += lambda * fmax(w.Hfull[p], 1e-6);
-        w.S[p] = s;
-        w.Scopy[p] = s;
+        const double h = w.Hfull[p];
+        double v = h;
+        if (i < nv && j < nv) v += rs[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
+        if (i == j) v += lambda * fmax(h, 1e-6);
+        w.S[p] = v;
+        w.Scopy[p] = v;
     }
-    for (int i = t; i < nr; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
-    (void)red;
+    for (int i = row0 + t; i < row1; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_cholesky: FP64 blocked right-looking Cholesky of S (lower, in place), triangular solves,
-// optional iterative refinement against the saved damped system, state step clamp, model
-// decrease of the states and the state trial x [+] delta (VisualCudaBatch.cu:174-189,323-340).
-// One workgroup per window.
+// k_cholesky: FP64 Cholesky of the damped reduced camera system on 16x16 tiles (right-looking):
+// the diagonal tile is factored by one wave in LDS, the panel below by a row-parallel triangular
+// solve in LDS, and the trailing tiles are updated with v_mfma_f64_16x16x4f64 (accumulator
+// layout: D[i][j] in lane j + 16 (i % 4), register i / 4; A[i][k] in lane i + 16 k; B[k][j] in
+// lane j + 16 k). Then blocked forward/backward substitution, optional iterative refinement
+// against the saved damped system, the step clamp, the model decrease of the states and the
+// state trial x [+] delta (VisualCudaBatch.cu:163-189,323-340). One workgroup per window.
 // ---------------------------------------------------------------------------------------------
-constexpr int kNb = 32;
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int kTile = 16;
+constexpr int kMaxTiles = (kMaxRetained + kTile - 1) / kTile;
+constexpr int kCholThreads = 1024;
 
-__device__ bool cholesky_inplace(double* A, int n, double* panel, double* diag, int* flag) {
-    const int t = threadIdx.x, nt = blockDim.x;
-    for (int k0 = 0; k0 < n; k0 += kNb) {
-        const int kb = min(kNb, n - k0);
-        // diagonal block -> LDS
-        for (int p = t; p < kb * kb; p += nt) diag[p] = A[(size_t)(k0 + p / kb) * n + k0 + p % kb];
+__device__ __forceinline__ double padded(const double* S, int n, int r, int c) {
+    return (r < n && c < n) ? S[(size_t)r * n + c] : (r == c ? 1.0 : 0.0);
+}
+
+__device__ double wave_reduce_sum(double v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// In-place lower Cholesky of S (n x n, row-major full). The diagonal tiles of L are kept in dg.
+__device__ bool cholesky_tiles(double* S, int n, double* panel, double* dg, int* flag) {
+    const int T = (n + kTile - 1) / kTile;
+    const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
+    for (int k = 0; k < T; ++k) {
+        const int r0 = kTile * k, rows = kTile * T - r0;
+        for (int p = t; p < rows * kTile; p += nt) panel[p] = padded(S, n, r0 + p / kTile, r0 + p % kTile);
         __syncthreads();
-        // unblocked factorization of the diagonal block by the first wave
-        if (t < 64) {
-            for (int j = 0; j < kb; ++j) {
-                if (t == 0) {
-                    const double d = diag[j * kb + j];
+        if (wave == 0) {
+            for (int j = 0; j < kTile; ++j) {
+                if (lane == 0) {
+                    const double d = panel[j * kTile + j];
                     if (!(d > 0) || !isfinite(d)) *flag = 1;
-                    diag[j * kb + j] = sqrt(d);
+                    panel[j * kTile + j] = sqrt(d);
                 }
                 wave_sync();
-                const double djj = diag[j * kb + j];
-                for (int i = j + 1 + t; i < kb; i += 64) diag[i * kb + j] /= djj;
+                if (lane > j && lane < kTile) panel[lane * kTile + j] /= panel[j * kTile + j];
                 wave_sync();
-                for (int p = t; p < (kb - j - 1) * (kb - j - 1); p += 64) {
-                    const int i = j + 1 + p / (kb - j - 1), c = j + 1 + p % (kb - j - 1);
-                    if (c <= i) diag[i * kb + c] -= diag[i * kb + j] * diag[c * kb + j];
+                for (int p = lane; p < kTile * kTile; p += 64) {
+                    const int r = p / kTile, c = p % kTile;
+                    if (c > j && r >= c) panel[p] -= panel[r * kTile + j] * panel[c * kTile + j];
                 }
                 wave_sync();
             }
         }
         __syncthreads();
         if (*flag) return false;
-        for (int p = t; p < kb * kb; p += nt) {
-            const int i = p / kb, c = p % kb;
-            if (c <= i) A[(size_t)(k0 + i) * n + k0 + c] = diag[p];
-        }
-        // panel: rows below, L21 = A21 L11^-T  (row-wise forward substitution)
-        const int rows = n - k0 - kb;
-        for (int r = t; r < rows; r += nt) {
-            double* row = panel + (size_t)r * kNb;
-            const double* src = A + (size_t)(k0 + kb + r) * n + k0;
-            for (int c = 0; c < kb; ++c) {
-                double s = src[c];
-                for (int q = 0; q < c; ++q) s -= row[q] * diag[c * kb + q];
-                row[c] = s / diag[c * kb + c];
+        for (int pr = kTile + t; pr < rows; pr += nt) {
+            double* row = panel + pr * kTile;
+            for (int c = 0; c < kTile; ++c) {
+                double v = row[c];
+                for (int q = 0; q < c; ++q) v -= row[q] * panel[c * kTile + q];
+                row[c] = v / panel[c * kTile + c];
             }
         }
+        for (int p = t; p < kTile * kTile; p += nt) dg[k * kTile * kTile + p] = panel[p];
         __syncthreads();
-        for (int p = t; p < rows * kb; p += nt) {
-            const int r = p / kb, c = p % kb;
-            A[(size_t)(k0 + kb + r) * n + k0 + c] = panel[(size_t)r * kNb + c];
+        for (int p = t; p < rows * kTile; p += nt) {
+            const int r = r0 + p / kTile, c = r0 + p % kTile;
+            if (r < n && c < n && c <= r) S[(size_t)r * n + c] = panel[p];
         }
-        // trailing update of the lower triangle, 4x4 register tiles
-        const int tiles = (rows + 3) / 4;
-        const int ntile = tiles * (tiles + 1) / 2;
-        for (int p = t; p < ntile; p += nt) {
-            int ti = 0, rem = p;
-            while (rem > ti) { rem -= ti + 1; ++ti; }
-            const int tj = rem;  // tj <= ti
-            double accv[4][4];
-            for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) accv[a][b] = 0;
-            for (int q = 0; q < kb; ++q) {
-                double x[4], y[4];
-                for (int a = 0; a < 4; ++a) {
-                    const int ri = 4 * ti + a, rj = 4 * tj + a;
-                    x[a] = ri < rows ? panel[(size_t)ri * kNb + q] : 0.0;
-                    y[a] = rj < rows ? panel[(size_t)rj * kNb + q] : 0.0;
-                }
-                for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) accv[a][b] += x[a] * y[b];
+        // trailing tiles (ti, tj), k < tj <= ti < T
+        const int m = T - k - 1;
+        const int ntiles = m * (m + 1) / 2;
+        for (int q = wave; q < ntiles; q += nw) {
+            int ii = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+            while (ii * (ii + 1) / 2 > q) --ii;
+            while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
+            const int jj = q - ii * (ii + 1) / 2;
+            const int ti = k + 1 + ii, tj = k + 1 + jj;
+            d4 acc;
+            for (int rr = 0; rr < 4; ++rr) acc[rr] = padded(S, n, kTile * ti + lane / 16 + 4 * rr, kTile * tj + lane % 16);
+            for (int kk = 0; kk < 4; ++kk) {
+                const double a = -panel[(kTile * (ti - k) + lane % 16) * kTile + 4 * kk + lane / 16];
+                const double b = panel[(kTile * (tj - k) + lane % 16) * kTile + 4 * kk + lane / 16];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
             }
-            for (int a = 0; a < 4; ++a)
-                for (int b = 0; b < 4; ++b) {
-                    const int ri = 4 * ti + a, rj = 4 * tj + b;
-                    if (ri < rows && rj < rows && rj <= ri)
-                        A[(size_t)(k0 + kb + ri) * n + k0 + kb + rj] -= accv[a][b];
-                }
+            for (int rr = 0; rr < 4; ++rr) {
+                const int r = kTile * ti + lane / 16 + 4 * rr, c = kTile * tj + lane % 16;
+                if (r < n && c < n) S[(size_t)r * n + c] = acc[rr];
+            }
         }
         __syncthreads();
     }
     return true;
 }
 
-// Solve L L^T x = b in place (x in LDS vector), lower factor in A.
-__device__ void cholesky_solve(const double* A, int n, double* x) {
-    const int t = threadIdx.x, nt = blockDim.x;
-    // forward: L y = b, column oriented in blocks of 64 rows handled by wave 0
-    for (int j0 = 0; j0 < n; j0 += 64) {
-        const int jb = min(64, n - j0);
-        if (t < 64) {
-            for (int j = 0; j < jb; ++j) {
-                if (t == 0) x[j0 + j] /= A[(size_t)(j0 + j) * n + j0 + j];
+// Solve L L^T x = x in place (x in LDS, length n) with the lower factor in S and its diagonal
+// tiles in dg. tmp: 16 doubles of LDS.
+__device__ void cholesky_solve_tiles(const double* S, int n, const double* dg, double* x, double* tmp) {
+    const int T = (n + kTile - 1) / kTile;
+    const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64;
+    for (int k = 0; k < T; ++k) {  // forward
+        const int r0 = kTile * k, kb = min(kTile, n - r0);
+        if (wave == 0) {
+            for (int c = 0; c < kb; ++c) {
+                if (lane == 0) x[r0 + c] /= dg[k * 256 + c * kTile + c];
                 wave_sync();
-                const double yj = x[j0 + j];
-                const int i = j0 + t;
-                if (t > j && t < jb) x[i] -= A[(size_t)i * n + j0 + j] * yj;
+                if (lane > c && lane < kb) x[r0 + lane] -= dg[k * 256 + lane * kTile + c] * x[r0 + c];
                 wave_sync();
             }
         }
         __syncthreads();
-        for (int i = j0 + jb + t; i < n; i += nt) {
-            double s = 0;
-            for (int j = 0; j < jb; ++j) s += A[(size_t)i * n + j0 + j] * x[j0 + j];
-            x[i] -= s;
+        for (int r = r0 + kTile + t; r < n; r += nt) {
+            const double* row = S + (size_t)r * n + r0;
+            double v = 0;
+            for (int c = 0; c < kb; ++c) v += row[c] * x[r0 + c];
+            x[r] -= v;
         }
         __syncthreads();
     }
-    // backward: L^T x = y
-    for (int j1 = n; j1 > 0; j1 -= 64) {
-        const int j0 = max(0, j1 - 64), jb = j1 - j0;
-        if (t < 64) {
-            for (int j = jb - 1; j >= 0; --j) {
-                if (t == 0) x[j0 + j] /= A[(size_t)(j0 + j) * n + j0 + j];
-                wave_sync();
-                const double xj = x[j0 + j];
-                if (t < j) x[j0 + t] -= A[(size_t)(j0 + j) * n + j0 + t] * xj;
-                wave_sync();
-            }
+    for (int k = T - 1; k >= 0; --k) {  // backward
+        const int r0 = kTile * k, kb = min(kTile, n - r0);
+        if (wave < kb) {
+            double v = 0;
+            for (int r = r0 + kTile + lane; r < n; r += 64) v += S[(size_t)r * n + r0 + wave] * x[r];
+            v = wave_reduce_sum(v);
+            if (lane == 0) tmp[wave] = v;
         }
         __syncthreads();
-        for (int i = t; i < j0; i += nt) {
-            double s = 0;
-            for (int j = 0; j < jb; ++j) s += A[(size_t)(j0 + j) * n + i] * x[j0 + j];
-            x[i] -= s;
+        if (t == 0) {
+            for (int c = kb - 1; c >= 0; --c) {
+                double v = x[r0 + c] - tmp[c];
+                for (int q = c + 1; q < kb; ++q) v -= dg[k * 256 + q * kTile + c] * x[r0 + q];
+                x[r0 + c] = v / dg[k * 256 + c * kTile + c];
+            }
         }
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cholesky(WinDev* wins, int refinement) {
-    __shared__ double panel[kMaxRetained * kNb];
-    __shared__ double diag[kNb * kNb];
-    __shared__ double x[kMaxRetained];
-    __shared__ double rres[kMaxRetained];
-    __shared__ double red[kBlock];
+__global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int refinement) {
+    __shared__ double panel[kMaxTiles * kTile * kTile];
+    __shared__ double dg[kMaxTiles * kTile * kTile];
+    __shared__ double x[kMaxTiles * kTile];
+    __shared__ double rres[kMaxTiles * kTile];
+    __shared__ double red[kCholThreads];
+    __shared__ double tmp[kTile];
     __shared__ int flag;
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
     const int n = w.nr, t = threadIdx.x, nt = blockDim.x;
     if (t == 0) flag = 0;
     __syncthreads();
-    const bool ok = cholesky_inplace(w.S, n, panel, diag, &flag);
+    const bool ok = cholesky_tiles(w.S, n, panel, dg, &flag);
     if (!ok) {
-        if (t == 0) { w.lm->chol_ok = 0; }
+        if (t == 0) w.lm->chol_ok = 0;
         return;
     }
     for (int i = t; i < n; i += nt) x[i] = w.rhs[i];
     __syncthreads();
-    cholesky_solve(w.S, n, x);
+    cholesky_solve_tiles(w.S, n, dg, x, tmp);
     for (int it = 0; it < refinement; ++it) {
         for (int i = t; i < n; i += nt) {
-            double s = w.rhs[i];
+            double v = w.rhs[i];
             const double* row = w.Scopy + (size_t)i * n;
-            for (int j = 0; j < n; ++j) s -= row[j] * x[j];
-            rres[i] = s;
+            for (int j = 0; j < n; ++j) v -= row[j] * x[j];
+            rres[i] = v;
         }
         __syncthreads();
-        cholesky_solve(w.S, n, rres);
+        cholesky_solve_tiles(w.S, n, dg, rres, tmp);
         for (int i = t; i < n; i += nt) x[i] += rres[i];
         __syncthreads();
     }
@@ -1368,8 +1524,11 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             if (nchunks) hipLaunchKernelGGL(k_schur, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
             break;
         case 4: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 1, red); break;
-        case 5: hipLaunchKernelGGL(k_assemble, dim3(nwin), dim3(1024), 0, s, wins, red); break;
-        case 6: hipLaunchKernelGGL(k_cholesky, dim3(nwin), dim3(kBlock), 0, s, wins, refinement); break;
+        case 5:
+            hipLaunchKernelGGL(k_assemble, dim3(nwin, (kMaxRetained + kAssembleRows - 1) / kAssembleRows), dim3(kBlock), 0, s,
+                               wins, red);
+            break;
+        case 6: hipLaunchKernelGGL(k_cholesky, dim3(nwin), dim3(kCholThreads), 0, s, wins, refinement); break;
         case 7:
             if (nchunks) hipLaunchKernelGGL(k_backsub, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
             break;

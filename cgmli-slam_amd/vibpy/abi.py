@@ -280,6 +280,7 @@ def load_library(path=LIB_PATH):
         "vib_create": ([C.POINTER(vp), C.c_int], C.c_int),
         "vib_destroy": ([vp], C.c_int),
         "vib_last_error": ([vp], C.c_char_p),
+        "vib_device_synchronize": ([vp], C.c_int),
         "vib_window_upload": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
         "vib_window_footprint": ([C.POINTER(VibWindow), C.c_int32], C.c_int64),
         "vib_window_release": ([vp, C.c_int32], C.c_int),
@@ -348,6 +349,9 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def synchronize(self):
+        return self.check(self.lib.vib_device_synchronize(self.h), "vib_device_synchronize")
 
     def error(self):
         m = self.lib.vib_last_error(self.h)

@@ -203,6 +203,8 @@ typedef struct vib_context vib_context;
 int vib_abi_version(void);
 int vib_create(vib_context** ctx, int device);
 int vib_destroy(vib_context* ctx);
+/* Wait for all work of the context's device (hipDeviceSynchronize). */
+int vib_device_synchronize(vib_context* ctx);
 /* Last error message of the context (reference takeVisualCudaRuntimeError, VisualCudaBatch.cu:399-402). */
 const char* vib_last_error(const vib_context* ctx);
 
