@@ -95,9 +95,14 @@ struct WinDev {
     const double* pref;
     const int32_t* pfree;      // free prior columns: block << 8 | index within the block
     int32_t prior_free, pad2;
+    // retained-factor linearization (written by k_retained, assembled row-parallel)
+    double* imu_J;             // E x 15 x 30 whitened Jacobians [pose0 | sb0 | pose1 | sb1]
+    double* imu_r;             // E x 15 whitened residuals
+    double* pD;                // prior_blocks x 36 chart derivatives (6 x 6, row-major)
+    double* patr;              // prior_cols : A^T r
+    double* ret_sc;            // 4 E + 4 : IMU costs (lin, trial), invalid (lin, trial); prior same
+    const int32_t* colmap;     // nr x 4 : IMU (edge << 8 | local) x2, prior (block << 8 | index), -1
     // dense per-window buffers
-    double* Hret;              // nr x nr retained (IMU + prior) normal matrix
-    double* gret;              // nr
     double* Hfull;             // nr x nr undamped full normal matrix of the states
     double* gfull;             // nr
     double* S;                 // nr x nr damped reduced camera system / Cholesky factor

@@ -25,10 +25,9 @@
 #include "vib_host.hpp"
 
 extern "C" {
-int vibk_retained_shared_bytes(int E, int prior_rows, int prior_cols, int prior_blocks);
 const char* vibk_name(int k);
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
-                       int max_len, size_t ret_shared, int refinement, int mode, hipStream_t s);
+                       int max_len, int max_edges, int refinement, int mode, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 }
 
@@ -54,7 +53,7 @@ struct Arena {
 
 }  // namespace
 
-int HostWindow::chunk_size = 512;
+int HostWindow::chunk_size = 128;
 
 // ------------------------------------------------------------------------------------------
 // Host-side window copy with derived layout.
@@ -196,6 +195,26 @@ bool HostWindow::layout(std::string& err) {
         for (int i = 0; i < d.width; ++i) pfree.push_back(((int)b << 8) | i);
         pblk.push_back(d);
     }
+    // static column map of the retained factors: IMU (edge << 8 | local column) x2, prior
+    colmap.assign(4 * (size_t)nr, -1);
+    for (int e = 0; e < E; ++e) {
+        const int s0 = imu_from[e], s1 = imu_to[e];
+        const int cols[4] = {pose_col[s0], sb_col[s0], pose_col[s1], sb_col[s1]};
+        const int width[4] = {cols[0] >= 0 ? 6 : 0, cols[1] >= 0 ? sb_width[s0] : 0, cols[2] >= 0 ? 6 : 0,
+                              cols[3] >= 0 ? sb_width[s1] : 0};
+        const int off[4] = {0, 6, 15, 21};
+        for (int b = 0; b < 4; ++b)
+            for (int i = 0; i < width[b]; ++i) {
+                int* slot = &colmap[4 * (size_t)(cols[b] + i)];
+                if (slot[0] < 0) slot[0] = (e << 8) | (off[b] + i);
+                else if (slot[1] < 0) slot[1] = (e << 8) | (off[b] + i);
+                else { err = "a state block is shared by more than two IMU edges"; return false; }
+            }
+    }
+    for (const int32_t f : pfree) {
+        const PriorBlockDev& d = pblk[f >> 8];
+        colmap[4 * (size_t)(d.col + (f & 255)) + 2] = f;
+    }
     // chunks
     chunks.clear();
     const int cs = std::max(1, chunk_size);
@@ -224,9 +243,10 @@ size_t Slot::footprint(const HostWindow& h) {
     d += 6 * h.F + LO_PLANES * h.L + h.L;                // linearization outputs
     d += (size_t)VIB_IMU_STRIDE * h.E;
     d += m * n + m + n * n + h.prior_ref.size();
-    d += 4 * nr * nr + 8 * nr;
+    d += 3 * nr * nr + 8 * nr;
+    d += 465 * (size_t)h.E + 36 * B + n + 4 * (size_t)h.E + 4;
     d += 2 * nch * (tri + nv) + nch * CS_PLANES + 2 * (tri + nv);
-    size_t i = 3 * h.L + 1 + h.F + 2 * h.E + h.pfree.size();
+    size_t i = 3 * h.L + 1 + h.F + 2 * h.E + h.pfree.size() + 4 * nr;
     return 8 * d + 4 * i + sizeof(PriorBlockDev) * B + sizeof(LmState) + 64 * 256;
 }
 
@@ -276,7 +296,11 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     d.pblk = pblk; d.pA = pA; d.pr0 = pr0; d.pG = pG; d.pref = pref; d.pfree = pfree;
     d.prior_free = (int)h.pfree.size();
     const size_t nr = h.nr;
-    d.Hret = a.take<double>(nr * nr); d.gret = a.take<double>(nr);
+    d.imu_J = a.take<double>((size_t)450 * h.E); d.imu_r = a.take<double>((size_t)15 * h.E);
+    d.pD = a.take<double>(36 * h.pblk.size()); d.patr = a.take<double>(h.prior_cols);
+    d.ret_sc = a.take<double>(4 * (size_t)h.E + 4);
+    int32_t* colmap = a.take<int32_t>(4 * nr);
+    d.colmap = colmap;
     d.Hfull = a.take<double>(nr * nr); d.gfull = a.take<double>(nr);
     d.S = a.take<double>(nr * nr); d.Scopy = a.take<double>(nr * nr);
     d.rhs = a.take<double>(nr); d.dstate = a.take<double>(nr); d.scratch = a.take<double>(4 * nr);
@@ -302,7 +326,7 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
               up(pblk, h.pblk.data(), sizeof(PriorBlockDev) * h.pblk.size()) &&
               up(pA, h.prior_A.data(), 8 * h.prior_A.size()) && up(pr0, h.prior_r0.data(), 8 * h.prior_r0.size()) &&
               up(pref, h.prior_ref.data(), 8 * h.prior_ref.size()) &&
-              up(pfree, h.pfree.data(), 4 * h.pfree.size());
+              up(pfree, h.pfree.data(), 4 * h.pfree.size()) && up(colmap, h.colmap.data(), 4 * h.colmap.size());
     // candidate starts equal to the state
     ok = ok && up(d.cpose, h.pose.data(), 8 * h.pose.size()) && up(d.csb, h.sb.data(), 8 * h.sb.size()) &&
          up(d.crig, h.rig, 56) && up(d.ctd, &h.td, 8) && up(d.crho, h.rho.data(), 8 * h.rho.size());
@@ -345,7 +369,7 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     std::vector<Chunk> all;
     std::vector<double*> red(2 * nwin);
     max_len = 1;
-    ret_shared = 8;
+    max_edges = 0;
     for (int i = 0; i < nwin; ++i) {
         Slot& s = *slots[i];
         wins[i] = s.dev;
@@ -353,8 +377,7 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
         red[2 * i] = s.reduced_lin;
         red[2 * i + 1] = s.reduced_schur;
         max_len = std::max(max_len, wins[i].tri + wins[i].nv);
-        ret_shared = std::max<size_t>(ret_shared, vibk_retained_shared_bytes(s.host.E, s.host.prior_rows, s.host.prior_cols,
-                                                                          (int)s.host.pblk.size()));
+        max_edges = std::max(max_edges, s.host.E);
     }
     nchunks = (int)all.size();
     if (hipMalloc(&d_wins, sizeof(WinDev) * nwin) != hipSuccess ||
@@ -368,7 +391,7 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
 }
 
 bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
-    return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_red, max_len, ret_shared, refinement, mode, s) == hipSuccess;
+    return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_red, max_len, max_edges, refinement, mode, s) == hipSuccess;
 }
 
 bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(1, s, 0, 0) && launch(2, s, 0, 0); }
@@ -669,7 +692,7 @@ int vib_accept(vib_context* ctx, int32_t slot) {
     std::vector<Slot*> one{s};
     Batch b;
     std::string err;
-    if (!b.build(one, ctx->stream, err) || !b.launch(9, ctx->stream, 0, 2) ||
+    if (!b.build(one, ctx->stream, err) || !b.launch(10, ctx->stream, 0, 2) ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(ctx, VIB_ERR_DEVICE, "accept failed");
     return VIB_OK;

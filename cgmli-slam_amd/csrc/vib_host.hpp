@@ -13,7 +13,7 @@
 namespace vib {
 
 constexpr int kMaxFactorsPerLandmark = 13;
-constexpr int kKernelCount = 10;  // kernels of one LM iteration (vibk_launch ids 0..9)
+constexpr int kKernelCount = 11;  // kernels of one LM iteration (vibk_launch ids 0..10)
 
 // Host copy of one window: observation streams sorted by landmark, tangent column layout
 // [pose (6, free) ..., rig (6), td (1), speed/bias (width) ...] then free inverse ranges,
@@ -44,6 +44,7 @@ struct HostWindow {
     int rig_col = -1, td_col = -1, nv = 0, nr = 0, ne = 0;
     std::vector<PriorBlockDev> pblk;
     std::vector<int32_t> pfree;
+    std::vector<int32_t> colmap;  // nr x 4 retained-factor column map
     std::vector<Chunk> chunks;
 
     bool build(const vib_window& w, std::string& err);
@@ -69,7 +70,7 @@ struct Slot {
 // Descriptors of a set of windows solved together.
 struct Batch {
     int nwin = 0, nchunks = 0, max_len = 1;
-    size_t ret_shared = 8;
+    int max_edges = 0;
     WinDev* d_wins = nullptr;
     Chunk* d_chunks = nullptr;
     double** d_red = nullptr;

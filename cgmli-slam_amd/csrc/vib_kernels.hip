@@ -60,6 +60,11 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+__device__ double wave_reduce_sum(double v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
 // Sum over the kChunkLanes lanes of a landmark group (xor tree, deterministic).
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
@@ -706,176 +711,105 @@ __device__ const double* prior_state(const WinDev& w, const PriorBlockDev& b, co
     return td;
 }
 
-// Retained linearization (IMU + prior) of one window into Hret/gret; cost and validity in LmState.
-// If `cost_only`, evaluates the retained cost at the trial state and returns it via `out_cost`.
-__device__ void retained_eval(const WinDev& w, const double* pose, const double* sb, const double* rig,
-                              const double* td, bool cost_only, double* sh, double* red, double& out_cost,
-                              int& out_invalid) {
-    const int t = threadIdx.x, nt = blockDim.x;
-    const int nr = w.nr;
-    double cost = 0;
-    int invalid = 0;
-    // IMU edges: one wave per edge; lane c < 30 builds Jacobian column c (sh: E x (15 + 450))
+// Retained factors of one window, one wavefront per IMU edge plus one for the prior.
+// mode 0 linearizes at the state (residuals, whitened Jacobians, chart derivatives, A^T r);
+// mode 1 evaluates the cost at the trial state. Results go to per-edge slots that k_assemble /
+// k_lm_update combine in a fixed order. grid = (windows, max_edges + 1), block = one wave.
+constexpr int kMaxPriorBlocks = 2 * kMaxFrames + 2;
+
+__global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
+    const WinDev& w = wins[blockIdx.x];
+    if (!window_active(w)) return;
+    if (mode == 0 && !w.lm->new_point) return;
+    const int y = blockIdx.y, lane = threadIdx.x;
     const int E = w.E;
-    const int wave = t / 64, lane = t % 64, nw = nt / 64;
-    for (int e = wave; e < E; e += nw) {
-        double* rr = sh + (size_t)e * 465;
-        const int s0 = w.imu_from[e], s1 = w.imu_to[e];
-        const double* d = w.imu + (size_t)e * 301;
+    const double* pose = mode == 0 ? w.pose : w.cpose;
+    const double* sb = mode == 0 ? w.sb : w.csb;
+    const double* rig = mode == 0 ? w.rig : w.crig;
+    const double* td = mode == 0 ? w.td : w.ctd;
+    double* sc = w.ret_sc;
+    if (y < E) {
+        const int s0 = w.imu_from[y], s1 = w.imu_to[y];
+        const double* d = w.imu + (size_t)y * 301;
         ImuShared o;
-        imu_shared(d, pose + 7 * s0, sb + 9 * s0, pose + 7 * s1, sb + 9 * s1, !cost_only, o);
-        if (lane == 0) {
-            if (!o.ok) invalid = 1;
-            for (int i = 0; i < 15; ++i) {
-                rr[i] = o.r[i];
-                cost += 0.5 * o.r[i] * o.r[i];
-            }
-        }
-        if (!cost_only && lane < 30) {
+        imu_shared(d, pose + 7 * s0, sb + 9 * s0, pose + 7 * s1, sb + 9 * s1, mode == 0, o);
+        int bad = !o.ok;
+        if (mode == 0 && lane < 30) {
             double col[15];
             imu_raw_column(d, o, lane, col);
             const double* W = d + 76;
+            double* J = w.imu_J + (size_t)y * 450;
             for (int i = 0; i < 15; ++i) {
                 double v = 0;
                 for (int m = 0; m < 15; ++m) v += W[i * 15 + m] * col[m];
-                rr[15 + i * 30 + lane] = v;
-                if (!isfinite(v)) invalid = 1;
+                J[i * 30 + lane] = v;
+                if (!isfinite(v)) bad = 1;
+            }
+        }
+        for (int m = 32; m > 0; m >>= 1) bad |= __shfl_xor(bad, m, 64);
+        if (lane == 0) {
+            double c = 0;
+            for (int i = 0; i < 15; ++i) c += 0.5 * o.r[i] * o.r[i];
+            sc[mode * E + y] = c;
+            sc[2 * E + mode * E + y] = bad ? 1.0 : 0.0;
+            if (mode == 0)
+                for (int i = 0; i < 15; ++i) w.imu_r[(size_t)y * 15 + i] = o.r[i];
+        }
+        return;
+    }
+    if (y != E) return;
+    double* psc = sc + 4 * E;  // prior: [lin cost, trial cost, lin invalid, trial invalid]
+    if (w.prior_rows <= 0) {
+        if (lane == 0) { psc[mode] = 0; psc[2 + mode] = 0; }
+        return;
+    }
+    __shared__ double chart[kMaxRetained];
+    __shared__ double res[kMaxRetained];
+    const int m = w.prior_rows, n = w.prior_cols, B = w.prior_blocks;
+    for (int bi = lane; bi < B; bi += 64) {
+        const PriorBlockDev b = w.pblk[bi];
+        const double* x = prior_state(w, b, pose, sb, rig, td);
+        const double* ref = w.pref + b.ref;
+        const bool se3 = b.kind == 0 || b.kind == 2;
+        if (se3) pose_minus(x, ref, chart + b.chart);
+        else for (int c = 0; c < b.tsize; ++c) chart[b.chart + c] = x[c] - ref[c];
+        if (mode == 0 && se3 && b.width > 0) {
+            const double h = 1e-7;
+            double step[6] = {0, 0, 0, 0, 0, 0}, pp[7], pm[7], dp[6], dm[6];
+            double* D = w.pD + (size_t)bi * 36;
+            for (int c = 0; c < 6; ++c) {
+                step[c] = h; pose_plus(x, step, pp);
+                step[c] = -h; pose_plus(x, step, pm);
+                step[c] = 0;
+                pose_minus(pp, ref, dp); pose_minus(pm, ref, dm);
+                for (int r = 0; r < 6; ++r) D[r * 6 + c] = (dp[r] - dm[r]) / (2 * h);
             }
         }
     }
-    __syncthreads();
-    if (!cost_only) {
-        for (size_t i = t; i < (size_t)nr * nr; i += nt) w.Hret[i] = 0;
-        for (int i = t; i < nr; i += nt) w.gret[i] = 0;
-        __syncthreads();
-        // even edges, then odd edges: edges of one parity touch disjoint frame blocks
-        for (int parity = 0; parity < 2; ++parity) {
-            for (int e = parity; e < E; e += 2) {
-                const double* rr = sh + (size_t)e * 465;
-                const double* J = rr + 15;
-                const int s0 = w.imu_from[e], s1 = w.imu_to[e];
-                int col[4] = {w.pose_col[s0], w.sb_col[s0], w.pose_col[s1], w.sb_col[s1]};
-                int wid[4] = {col[0] >= 0 ? 6 : 0, col[1] >= 0 ? w.sb_w[s0] : 0, col[2] >= 0 ? 6 : 0,
-                              col[3] >= 0 ? w.sb_w[s1] : 0};
-                const int off[4] = {0, 6, 15, 21};
-                for (int p = t; p < 900; p += nt) {
-                    const int i = p / 30, j = p % 30;
-                    const int bi = i < 6 ? 0 : (i < 15 ? 1 : (i < 21 ? 2 : 3));
-                    const int bj = j < 6 ? 0 : (j < 15 ? 1 : (j < 21 ? 2 : 3));
-                    const int ii = i - off[bi], jj = j - off[bj];
-                    if (ii >= wid[bi] || jj >= wid[bj]) continue;
-                    double v = 0;
-                    for (int m = 0; m < 15; ++m) v += J[m * 30 + i] * J[m * 30 + j];
-                    w.Hret[(size_t)(col[bi] + ii) * nr + col[bj] + jj] += v;
-                }
-                for (int i = t; i < 30; i += nt) {
-                    const int bi = i < 6 ? 0 : (i < 15 ? 1 : (i < 21 ? 2 : 3));
-                    const int ii = i - off[bi];
-                    if (ii >= wid[bi]) continue;
-                    double v = 0;
-                    for (int m = 0; m < 15; ++m) v += J[m * 30 + i] * rr[m];
-                    w.gret[col[bi] + ii] += v;
-                }
-            }
-            __syncthreads();
+    wave_sync();
+    double cost = 0;
+    int bad = 0;
+    for (int r = lane; r < m; r += 64) {
+        const double* A = w.pA + (size_t)r * n;
+        double v = w.pr0[r];
+        for (int c = 0; c < n; ++c) v += A[c] * chart[c];
+        res[r] = v;
+        if (!isfinite(v)) bad = 1;
+        cost += 0.5 * v * v;
+    }
+    wave_sync();
+    if (mode == 0) {
+        for (int c = lane; c < n; c += 64) {
+            double v = 0;
+            for (int r = 0; r < m; ++r) v += w.pA[(size_t)r * n + c] * res[r];
+            w.patr[c] = v;
         }
     }
-    // prior
-    if (w.prior_rows > 0) {
-        const int m = w.prior_rows, n = w.prior_cols, B = w.prior_blocks;
-        double* chart = sh;            // n
-        double* D = sh + n;            // B x 36 chart derivatives
-        double* res = D + 36 * B;      // m
-        double* atr = res + m;         // n : A^T r
-        __syncthreads();
-        for (int bi = t; bi < B; bi += nt) {
-            const PriorBlockDev b = w.pblk[bi];
-            const double* x = prior_state(w, b, pose, sb, rig, td);
-            const double* ref = w.pref + b.ref;
-            const bool se3 = b.kind == 0 || b.kind == 2;
-            if (se3) pose_minus(x, ref, chart + b.chart);
-            else for (int c = 0; c < b.tsize; ++c) chart[b.chart + c] = x[c] - ref[c];
-            if (!cost_only && se3 && b.width > 0) {
-                const double h = 1e-7;
-                double step[6] = {0, 0, 0, 0, 0, 0}, pp[7], pm[7], dp[6], dm[6];
-                for (int c = 0; c < 6; ++c) {
-                    step[c] = h; pose_plus(x, step, pp);
-                    step[c] = -h; pose_plus(x, step, pm);
-                    step[c] = 0;
-                    pose_minus(pp, ref, dp); pose_minus(pm, ref, dm);
-                    for (int r = 0; r < 6; ++r) D[bi * 36 + r * 6 + c] = (dp[r] - dm[r]) / (2 * h);
-                }
-            }
-        }
-        __syncthreads();
-        for (int r = t; r < m; r += nt) {
-            const double* A = w.pA + (size_t)r * n;
-            double v = w.pr0[r];
-            for (int c = 0; c < n; ++c) v += A[c] * chart[c];
-            res[r] = v;
-            if (!isfinite(v)) invalid = 1;
-            cost += 0.5 * v * v;
-        }
-        __syncthreads();
-        if (!cost_only) {
-            for (int c = t; c < n; c += nt) {
-                double s = 0;
-                for (int r = 0; r < m; ++r) s += w.pA[(size_t)r * n + c] * res[r];
-                atr[c] = s;
-            }
-            __syncthreads();
-            // gradient D_b^T (A^T r)_b and H = D^T G D over the free prior columns
-            // (pfree[k] = block << 8 | index within the block)
-            const int nf = w.prior_free;
-            for (int k = t; k < nf; k += nt) {
-                const int bi = w.pfree[k] >> 8, ii = w.pfree[k] & 255;
-                const PriorBlockDev b = w.pblk[bi];
-                const bool se3 = b.kind == 0 || b.kind == 2;
-                double s = 0;
-                for (int p = 0; p < b.tsize; ++p) s += (se3 ? D[bi * 36 + p * 6 + ii] : (p == ii ? 1.0 : 0.0)) * atr[b.chart + p];
-                w.gret[b.col + ii] += s;
-            }
-            for (int p = t; p < nf * nf; p += nt) {
-                const int k1 = p / nf, k2 = p % nf;
-                const int bi = w.pfree[k1] >> 8, ii = w.pfree[k1] & 255;
-                const int bj = w.pfree[k2] >> 8, jj = w.pfree[k2] & 255;
-                const PriorBlockDev b1 = w.pblk[bi], b2 = w.pblk[bj];
-                const bool se1 = b1.kind == 0 || b1.kind == 2, se2 = b2.kind == 0 || b2.kind == 2;
-                double s = 0;
-                for (int x = 0; x < b1.tsize; ++x) {
-                    const double d1 = se1 ? D[bi * 36 + x * 6 + ii] : (x == ii ? 1.0 : 0.0);
-                    if (d1 == 0) continue;
-                    const double* G = w.pG + (size_t)(b1.chart + x) * n + b2.chart;
-                    double inner = 0;
-                    if (se2) {
-                        for (int y = 0; y < b2.tsize; ++y) inner += G[y] * D[bj * 36 + y * 6 + jj];
-                    } else {
-                        inner = G[jj];
-                    }
-                    s += d1 * inner;
-                }
-                w.Hret[(size_t)(b1.col + ii) * nr + b2.col + jj] += s;
-            }
-            __syncthreads();
-        }
-    }
-    const double c = block_sum(cost, red);
-    const double inv = block_max(invalid ? 1.0 : 0.0, red);
-    out_cost = c;
-    out_invalid = inv > 0;
-}
-
-__global__ __launch_bounds__(kBlock) void k_lin_retained(const WinDev* wins) {
-    __shared__ double red[kBlock];
-    extern __shared__ double sh[];
-    const WinDev& w = wins[blockIdx.x];
-    if (!window_active(w) || !w.lm->new_point) return;
-    double cost;
-    int invalid;
-    retained_eval(w, w.pose, w.sb, w.rig, w.td, false, sh, red, cost, invalid);
-    if (threadIdx.x == 0) {
-        w.lm->ret_cost = cost;
-        w.lm->ret_invalid = invalid;
+    cost = wave_reduce_sum(cost);
+    for (int k = 32; k > 0; k >>= 1) bad |= __shfl_xor(bad, k, 64);
+    if (lane == 0) {
+        psc[mode] = cost;
+        psc[2 + mode] = bad ? 1.0 : 0.0;
     }
 }
 
@@ -1014,6 +948,43 @@ __global__ __launch_bounds__(kBlock) void k_schur(const WinDev* wins, const Chun
 // ---------------------------------------------------------------------------------------------
 constexpr int kAssembleRows = 16;  // rows of the reduced system per workgroup
 
+// Retained (IMU + prior) normal-matrix entry (i, j) from the per-edge Jacobians and the prior
+// chart derivatives / Gram matrix, using the static column map.
+__device__ __forceinline__ double retained_entry(const WinDev& w, const int* ci, const int* cj) {
+    double h = 0;
+    for (int a = 0; a < 2; ++a) {
+        if (ci[a] < 0) continue;
+        const int e = ci[a] >> 8, li = ci[a] & 255;
+        for (int b = 0; b < 2; ++b) {
+            if (cj[b] < 0 || (cj[b] >> 8) != e) continue;
+            const int lj = cj[b] & 255;
+            const double* J = w.imu_J + (size_t)e * 450;
+            double v = 0;
+            for (int m = 0; m < 15; ++m) v += J[m * 30 + li] * J[m * 30 + lj];
+            h += v;
+        }
+    }
+    if (ci[2] >= 0 && cj[2] >= 0) {
+        const int bi = ci[2] >> 8, ii = ci[2] & 255, bj = cj[2] >> 8, jj = cj[2] & 255;
+        const PriorBlockDev b1 = w.pblk[bi], b2 = w.pblk[bj];
+        const bool se1 = b1.kind == 0 || b1.kind == 2, se2 = b2.kind == 0 || b2.kind == 2;
+        const int n = w.prior_cols;
+        for (int x = 0; x < b1.tsize; ++x) {
+            const double d1 = se1 ? w.pD[bi * 36 + x * 6 + ii] : (x == ii ? 1.0 : 0.0);
+            if (d1 == 0) continue;
+            const double* G = w.pG + (size_t)(b1.chart + x) * n + b2.chart;
+            double inner = 0;
+            if (se2) {
+                for (int yy = 0; yy < 6; ++yy) inner += G[yy] * w.pD[bj * 36 + yy * 6 + jj];
+            } else {
+                inner = G[jj];
+            }
+            h += d1 * inner;
+        }
+    }
+    return h;
+}
+
 __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const* reduced) {
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
@@ -1028,19 +999,42 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
     if (fresh) {
         for (size_t p = (size_t)row0 * nr + t; p < (size_t)row1 * nr; p += nt) {
             const int i = (int)(p / nr), j = (int)(p % nr);
-            double h = w.Hret[p];
+            double h = retained_entry(w, w.colmap + 4 * i, w.colmap + 4 * j);
             if (i < nv && j < nv) h += rl[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
             w.Hfull[p] = h;
         }
-        for (int i = row0 + t; i < row1; i += nt) w.gfull[i] = w.gret[i] + (i < nv ? rl[w.tri + i] : 0.0);
+        for (int i = row0 + t; i < row1; i += nt) {
+            const int* ci = w.colmap + 4 * i;
+            double g = i < nv ? rl[w.tri + i] : 0.0;
+            for (int a = 0; a < 2; ++a) {
+                if (ci[a] < 0) continue;
+                const int e = ci[a] >> 8, li = ci[a] & 255;
+                const double* J = w.imu_J + (size_t)e * 450;
+                const double* r = w.imu_r + (size_t)e * 15;
+                for (int m = 0; m < 15; ++m) g += J[m * 30 + li] * r[m];
+            }
+            if (ci[2] >= 0) {
+                const int bi = ci[2] >> 8, ii = ci[2] & 255;
+                const PriorBlockDev b = w.pblk[bi];
+                const bool se3 = b.kind == 0 || b.kind == 2;
+                for (int x = 0; x < b.tsize; ++x)
+                    g += (se3 ? w.pD[bi * 36 + x * 6 + ii] : (x == ii ? 1.0 : 0.0)) * w.patr[b.chart + x];
+            }
+            w.gfull[i] = g;
+        }
         if (blockIdx.y == 0 && t == 0) {
             double cost = 0;
-            int invalid = w.lm->ret_invalid;
+            int invalid = 0;
             for (int k = 0; k < w.nchunks; ++k) {
                 cost += w.chunk_sc[(size_t)k * CS_PLANES + CS_COST];
                 invalid |= w.chunk_sc[(size_t)k * CS_PLANES + CS_INVALID] > 0;
             }
-            cost += w.lm->ret_cost;
+            for (int e = 0; e < w.E; ++e) {
+                cost += w.ret_sc[e];
+                invalid |= w.ret_sc[2 * w.E + e] > 0;
+            }
+            cost += w.ret_sc[4 * w.E];
+            invalid |= w.ret_sc[4 * w.E + 2] > 0;
             w.lm->lin_cost = cost;
             w.lm->lin_valid = !invalid && isfinite(cost);
         }
@@ -1077,19 +1071,23 @@ __device__ __forceinline__ double padded(const double* S, int n, int r, int c) {
     return (r < n && c < n) ? S[(size_t)r * n + c] : (r == c ? 1.0 : 0.0);
 }
 
-__device__ double wave_reduce_sum(double v) {
-    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
 
-// In-place lower Cholesky of S (n x n, row-major full). The diagonal tiles of L are kept in dg.
-__device__ bool cholesky_tiles(double* S, int n, double* panel, double* dg, int* flag) {
+// Lower Cholesky of S (n x n, row-major full, lower factor written back to S) fused with the
+// forward substitution of y (LDS, length 16 T, padding zero). The tile column k lives in the
+// LDS panel buffer pnl[k % 2]; trailing tiles are updated in registers and the tiles of
+// column k + 1 are handed to the next panel buffer without a round trip through memory.
+constexpr int kTileBatch = 8;
+
+__device__ bool cholesky_forward(double* S, int n, double* pnl, double* dg, double* y, int* flag) {
     const int T = (n + kTile - 1) / kTile;
     const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
+    constexpr int PANEL = kMaxTiles * kTile * kTile;
+    for (int p = t; p < T * kTile * kTile; p += nt) pnl[p] = padded(S, n, p / kTile, p % kTile);
+    __syncthreads();
     for (int k = 0; k < T; ++k) {
+        double* panel = pnl + (k & 1) * PANEL;
+        double* next = pnl + ((k + 1) & 1) * PANEL;
         const int r0 = kTile * k, rows = kTile * T - r0;
-        for (int p = t; p < rows * kTile; p += nt) panel[p] = padded(S, n, r0 + p / kTile, r0 + p % kTile);
-        __syncthreads();
         if (wave == 0) {
             for (int j = 0; j < kTile; ++j) {
                 if (lane == 0) {
@@ -1111,37 +1109,76 @@ __device__ bool cholesky_tiles(double* S, int n, double* panel, double* dg, int*
         if (*flag) return false;
         for (int pr = kTile + t; pr < rows; pr += nt) {
             double* row = panel + pr * kTile;
+            double v[kTile];
+            for (int c = 0; c < kTile; ++c) v[c] = row[c];
             for (int c = 0; c < kTile; ++c) {
-                double v = row[c];
-                for (int q = 0; q < c; ++q) v -= row[q] * panel[c * kTile + q];
-                row[c] = v / panel[c * kTile + c];
+                for (int q = 0; q < c; ++q) v[c] -= v[q] * panel[c * kTile + q];
+                v[c] /= panel[c * kTile + c];
+            }
+            for (int c = 0; c < kTile; ++c) row[c] = v[c];
+        }
+        __syncthreads();
+        // forward substitution of the diagonal block, done by wave 0 while the rest store L
+        if (wave == 0) {
+            for (int c = 0; c < kTile; ++c) {
+                if (lane == 0) y[r0 + c] /= panel[c * kTile + c];
+                wave_sync();
+                if (lane > c && lane < kTile) y[r0 + lane] -= panel[lane * kTile + c] * y[r0 + c];
+                wave_sync();
             }
         }
         for (int p = t; p < kTile * kTile; p += nt) dg[k * kTile * kTile + p] = panel[p];
-        __syncthreads();
         for (int p = t; p < rows * kTile; p += nt) {
             const int r = r0 + p / kTile, c = r0 + p % kTile;
             if (r < n && c < n && c <= r) S[(size_t)r * n + c] = panel[p];
         }
-        // trailing tiles (ti, tj), k < tj <= ti < T
+        __syncthreads();
+        for (int pr = kTile + t; pr < rows; pr += nt) {
+            double v = 0;
+            for (int c = 0; c < kTile; ++c) v += panel[pr * kTile + c] * y[r0 + c];
+            y[r0 + pr] -= v;
+        }
+        // trailing tiles (ti, tj), k < tj <= ti < T, processed in register batches
         const int m = T - k - 1;
         const int ntiles = m * (m + 1) / 2;
-        for (int q = wave; q < ntiles; q += nw) {
-            int ii = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
-            while (ii * (ii + 1) / 2 > q) --ii;
-            while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
-            const int jj = q - ii * (ii + 1) / 2;
-            const int ti = k + 1 + ii, tj = k + 1 + jj;
-            d4 acc;
-            for (int rr = 0; rr < 4; ++rr) acc[rr] = padded(S, n, kTile * ti + lane / 16 + 4 * rr, kTile * tj + lane % 16);
-            for (int kk = 0; kk < 4; ++kk) {
-                const double a = -panel[(kTile * (ti - k) + lane % 16) * kTile + 4 * kk + lane / 16];
-                const double b = panel[(kTile * (tj - k) + lane % 16) * kTile + 4 * kk + lane / 16];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        for (int q0 = wave; q0 < ntiles; q0 += kTileBatch * nw) {
+            d4 acc[kTileBatch];
+            int tti[kTileBatch], ttj[kTileBatch];
+#pragma unroll
+            for (int b = 0; b < kTileBatch; ++b) {
+                const int q = q0 + b * nw;
+                tti[b] = -1;
+                if (q < ntiles) {
+                    int ii = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+                    while (ii * (ii + 1) / 2 > q) --ii;
+                    while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
+                    tti[b] = k + 1 + ii;
+                    ttj[b] = k + 1 + (q - ii * (ii + 1) / 2);
+                    for (int rr = 0; rr < 4; ++rr)
+                        acc[b][rr] = padded(S, n, kTile * tti[b] + lane / 16 + 4 * rr, kTile * ttj[b] + lane % 16);
+                }
             }
-            for (int rr = 0; rr < 4; ++rr) {
-                const int r = kTile * ti + lane / 16 + 4 * rr, c = kTile * tj + lane % 16;
-                if (r < n && c < n) S[(size_t)r * n + c] = acc[rr];
+#pragma unroll
+            for (int b = 0; b < kTileBatch; ++b) {
+                if (tti[b] < 0) continue;
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double a = -panel[(kTile * (tti[b] - k) + lane % 16) * kTile + 4 * kk + lane / 16];
+                    const double bb = panel[(kTile * (ttj[b] - k) + lane % 16) * kTile + 4 * kk + lane / 16];
+                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[b], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < kTileBatch; ++b) {
+                if (tti[b] < 0) continue;
+                if (ttj[b] == k + 1) {
+                    for (int rr = 0; rr < 4; ++rr)
+                        next[(kTile * (tti[b] - k - 1) + lane / 16 + 4 * rr) * kTile + lane % 16] = acc[b][rr];
+                } else {
+                    for (int rr = 0; rr < 4; ++rr) {
+                        const int r = kTile * tti[b] + lane / 16 + 4 * rr, c = kTile * ttj[b] + lane % 16;
+                        if (r < n && c < n) S[(size_t)r * n + c] = acc[b][rr];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -1149,18 +1186,50 @@ __device__ bool cholesky_tiles(double* S, int n, double* panel, double* dg, int*
     return true;
 }
 
-// Solve L L^T x = x in place (x in LDS, length n) with the lower factor in S and its diagonal
-// tiles in dg. tmp: 16 doubles of LDS.
-__device__ void cholesky_solve_tiles(const double* S, int n, const double* dg, double* x, double* tmp) {
+// Backward substitution L^T x = y in place (x in LDS) with the factor in S and diagonal tiles in
+// dg. part: 16 x 16 doubles of LDS for cross-wave partial sums.
+__device__ void cholesky_backward(const double* S, int n, const double* dg, double* x, double* part) {
+    const int T = (n + kTile - 1) / kTile;
+    const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
+    const int rsub = lane / 16, c = lane % 16;
+    for (int k = T - 1; k >= 0; --k) {
+        const int r0 = kTile * k, kb = min(kTile, n - r0);
+        double v = 0;
+        if (c < kb)
+            for (int r = r0 + kTile + 4 * wave + rsub; r < n; r += 4 * nw) v += S[(size_t)r * n + r0 + c] * x[r];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (lane < kTile) part[wave * kTile + lane] = v;
+        __syncthreads();
+        if (wave == 0) {
+            if (lane < kb) {
+                double s = 0;
+                for (int q = 0; q < nw; ++q) s += part[q * kTile + lane];
+                x[r0 + lane] -= s;
+            }
+            wave_sync();
+            for (int cc = kb - 1; cc >= 0; --cc) {
+                if (lane == 0) x[r0 + cc] /= dg[k * 256 + cc * kTile + cc];
+                wave_sync();
+                if (lane < cc) x[r0 + lane] -= dg[k * 256 + cc * kTile + lane] * x[r0 + cc];
+                wave_sync();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Forward + backward solve with an existing factor (iterative refinement).
+__device__ void cholesky_solve_again(const double* S, int n, const double* dg, double* x, double* part) {
     const int T = (n + kTile - 1) / kTile;
     const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64;
-    for (int k = 0; k < T; ++k) {  // forward
+    for (int k = 0; k < T; ++k) {
         const int r0 = kTile * k, kb = min(kTile, n - r0);
         if (wave == 0) {
-            for (int c = 0; c < kb; ++c) {
-                if (lane == 0) x[r0 + c] /= dg[k * 256 + c * kTile + c];
+            for (int cc = 0; cc < kb; ++cc) {
+                if (lane == 0) x[r0 + cc] /= dg[k * 256 + cc * kTile + cc];
                 wave_sync();
-                if (lane > c && lane < kb) x[r0 + lane] -= dg[k * 256 + lane * kTile + c] * x[r0 + c];
+                if (lane > cc && lane < kb) x[r0 + lane] -= dg[k * 256 + lane * kTile + cc] * x[r0 + cc];
                 wave_sync();
             }
         }
@@ -1168,61 +1237,47 @@ __device__ void cholesky_solve_tiles(const double* S, int n, const double* dg, d
         for (int r = r0 + kTile + t; r < n; r += nt) {
             const double* row = S + (size_t)r * n + r0;
             double v = 0;
-            for (int c = 0; c < kb; ++c) v += row[c] * x[r0 + c];
+            for (int cc = 0; cc < kb; ++cc) v += row[cc] * x[r0 + cc];
             x[r] -= v;
         }
         __syncthreads();
     }
-    for (int k = T - 1; k >= 0; --k) {  // backward
-        const int r0 = kTile * k, kb = min(kTile, n - r0);
-        if (wave < kb) {
-            double v = 0;
-            for (int r = r0 + kTile + lane; r < n; r += 64) v += S[(size_t)r * n + r0 + wave] * x[r];
-            v = wave_reduce_sum(v);
-            if (lane == 0) tmp[wave] = v;
-        }
-        __syncthreads();
-        if (t == 0) {
-            for (int c = kb - 1; c >= 0; --c) {
-                double v = x[r0 + c] - tmp[c];
-                for (int q = c + 1; q < kb; ++q) v -= dg[k * 256 + q * kTile + c] * x[r0 + q];
-                x[r0 + c] = v / dg[k * 256 + c * kTile + c];
-            }
-        }
-        __syncthreads();
-    }
+    cholesky_backward(S, n, dg, x, part);
 }
 
 __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int refinement) {
-    __shared__ double panel[kMaxTiles * kTile * kTile];
+    __shared__ double pnl[2 * kMaxTiles * kTile * kTile];
     __shared__ double dg[kMaxTiles * kTile * kTile];
     __shared__ double x[kMaxTiles * kTile];
     __shared__ double rres[kMaxTiles * kTile];
+    __shared__ double part[kTile * kTile];
     __shared__ double red[kCholThreads];
-    __shared__ double tmp[kTile];
     __shared__ int flag;
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
     const int n = w.nr, t = threadIdx.x, nt = blockDim.x;
+    const int N = kTile * ((n + kTile - 1) / kTile);
     if (t == 0) flag = 0;
+    for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[i] : 0.0;
     __syncthreads();
-    const bool ok = cholesky_tiles(w.S, n, panel, dg, &flag);
+    const bool ok = cholesky_forward(w.S, n, pnl, dg, x, &flag);
     if (!ok) {
         if (t == 0) w.lm->chol_ok = 0;
         return;
     }
-    for (int i = t; i < n; i += nt) x[i] = w.rhs[i];
-    __syncthreads();
-    cholesky_solve_tiles(w.S, n, dg, x, tmp);
+    cholesky_backward(w.S, n, dg, x, part);
     for (int it = 0; it < refinement; ++it) {
-        for (int i = t; i < n; i += nt) {
-            double v = w.rhs[i];
-            const double* row = w.Scopy + (size_t)i * n;
-            for (int j = 0; j < n; ++j) v -= row[j] * x[j];
+        for (int i = t; i < N; i += nt) {
+            double v = 0;
+            if (i < n) {
+                v = w.rhs[i];
+                const double* row = w.Scopy + (size_t)i * n;
+                for (int j = 0; j < n; ++j) v -= row[j] * x[j];
+            }
             rres[i] = v;
         }
         __syncthreads();
-        cholesky_solve_tiles(w.S, n, dg, rres, tmp);
+        cholesky_solve_again(w.S, n, dg, rres, part);
         for (int i = t; i < n; i += nt) x[i] += rres[i];
         __syncthreads();
     }
@@ -1380,8 +1435,6 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
 // mode 0: LM step; 1: evaluate only (metrics, no state change); 2: accept the trial state.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
-    __shared__ double red[kBlock];
-    extern __shared__ double sh[];
     __shared__ int accept_flag;
     WinDev& w = wins[blockIdx.x];
     LmState* lm = w.lm;
@@ -1390,12 +1443,16 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
     if (mode == 2) {
         if (t == 0) accept_flag = 1;
     } else {
-        double rcost;
-        int rinv;
-        retained_eval(w, w.cpose, w.csb, w.crig, w.ctd, true, sh, red, rcost, rinv);
         if (t == 0) {
-            double pred = lm->predicted, trial = rcost, mx = lm->max_increment;
-            int tinv = rinv || !lm->trial_valid;
+            const int E = w.E;
+            double pred = lm->predicted, trial = 0, mx = lm->max_increment;
+            int tinv = !lm->trial_valid;
+            for (int e = 0; e < E; ++e) {
+                trial += w.ret_sc[E + e];
+                tinv |= w.ret_sc[3 * E + e] > 0;
+            }
+            trial += w.ret_sc[4 * E + 1];
+            tinv |= w.ret_sc[4 * E + 3] > 0;
             for (int k = 0; k < w.nchunks; ++k) {
                 const double* sc = w.chunk_sc + (size_t)k * CS_PLANES;
                 pred += sc[CS_PRED];
@@ -1488,37 +1545,26 @@ __global__ __launch_bounds__(kBlock) void k_prior_gram(const WinDev* wins, doubl
 // ---------------------------------------------------------------------------------------------
 // Launch wrappers used by the host driver.
 // ---------------------------------------------------------------------------------------------
-static void allow_dynamic_lds(const void* fn, size_t bytes) {
-    if (bytes > 48 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
 extern "C" {
 
-int vibk_retained_shared_bytes(int E, int prior_rows, int prior_cols, int prior_blocks) {
-    const int imu = E * 465;
-    const int prior = prior_cols + 36 * prior_blocks + prior_rows + prior_cols;
-    return 8 * (imu > prior ? imu : prior);
-}
 
 // One launch per kernel id (the LM iteration is the sequence 0..9).
 const char* vibk_name(int k) {
-    static const char* names[] = {"k_lin_visual", "k_lin_retained", "k_reduce_lin", "k_schur", "k_reduce_schur",
-                                  "k_assemble", "k_cholesky", "k_backsub", "k_trial_cost", "k_lm_update"};
-    return k >= 0 && k < 10 ? names[k] : "";
+    static const char* names[] = {"k_lin_visual", "k_retained_lin", "k_reduce_lin", "k_schur", "k_reduce_schur",
+                                  "k_assemble", "k_cholesky", "k_backsub", "k_trial_cost", "k_retained_trial",
+                                  "k_lm_update"};
+    return k >= 0 && k < 11 ? names[k] : "";
 }
 
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
-                       int max_len, size_t ret_shared, int refinement, int mode, hipStream_t s) {
+                       int max_len, int max_edges, int refinement, int mode, hipStream_t s) {
     using namespace vib;
     const int slices = (max_len + kBlock - 1) / kBlock;
     switch (k) {
         case 0:
             if (nchunks) hipLaunchKernelGGL(k_lin_visual, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
             break;
-        case 1:
-            allow_dynamic_lds((const void*)k_lin_retained, ret_shared);
-            hipLaunchKernelGGL(k_lin_retained, dim3(nwin), dim3(kBlock), ret_shared, s, wins);
-            break;
+        case 1: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 0); break;
         case 2: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 0, red); break;
         case 3:
             if (nchunks) hipLaunchKernelGGL(k_schur, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
@@ -1535,10 +1581,8 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
         case 8:
             if (nchunks) hipLaunchKernelGGL(k_trial_cost, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
             break;
-        case 9:
-            allow_dynamic_lds((const void*)k_lm_update, ret_shared);
-            hipLaunchKernelGGL(k_lm_update, dim3(nwin), dim3(kBlock), ret_shared, s, wins, mode);
-            break;
+        case 9: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 1); break;
+        case 10: hipLaunchKernelGGL(k_lm_update, dim3(nwin), dim3(kBlock), 0, s, wins, mode); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

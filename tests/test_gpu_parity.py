@@ -112,7 +112,9 @@ def test_full_solve_matches_oracle(ctx, oracle, abi, name, cfg):
 
 
 def test_batch_alone_equals_mixed(ctx, abi):
-    """Each window's result is the same solved alone and inside a batch of uneven windows."""
+    """Each window's result is the same solved alone and inside a batch of uneven windows, within
+    1e-10 relative cost and 1e-7 absolute state: the per-workgroup LDS accumulation adds in
+    hardware order, so sums differ at the last bit and LM iterations carry that forward."""
     from vibpy import synth
     cfgs = [dict(K=6, L=300, seed=21), dict(K=9, L=700, n_cams=3, seed=22, obs_range=(2, 9)),
             dict(K=5, L=120, n_cams=2, seed=23), dict(K=7, L=500, seed=24, rig_free=True, td_free=True)]
@@ -126,7 +128,7 @@ def test_batch_alone_equals_mixed(ctx, abi):
         assert st == 0, ctx.error()
         out = w.copy()
         ctx.read_state(10 + i, out)
-        alone.append((rep.final_cost, out))
+        alone.append((rep.final_cost, out, rep))
     for i, w in enumerate(wins):
         ctx.upload(10 + i, w)
     say("[gpu] batch of 4")
@@ -135,7 +137,9 @@ def test_batch_alone_equals_mixed(ctx, abi):
     for i, w in enumerate(wins):
         out = w.copy()
         ctx.read_state(10 + i, out)
-        c0, o0 = alone[i]
+        c0, o0, _ = alone[i]
+        assert reps[i].iterations == alone[i][2].iterations
+        assert reps[i].accepted_steps == alone[i][2].accepted_steps
         assert abs(reps[i].final_cost - c0) <= 1e-10 * c0
-        assert np.abs(out.pose - o0.pose).max() < 1e-9
-        assert np.abs(out.lm_rho - o0.lm_rho).max() < 1e-9 * max(1.0, np.abs(o0.lm_rho).max())
+        assert np.abs(out.pose - o0.pose).max() < 1e-7
+        assert np.abs(out.lm_rho - o0.lm_rho).max() < 1e-7 * max(1.0, np.abs(o0.lm_rho).max())
