@@ -20,6 +20,18 @@ constexpr int kMaxVisual = 6 * kMaxFrames + 7;
 constexpr int kMaxRetained = 15 * kMaxFrames + 7;
 constexpr int kChunkLanes = 8;      // lanes per landmark in the per-landmark kernels
 constexpr int kBlock = 256;         // threads per chunk workgroup
+constexpr int kTile = 16;           // tile edge of the reduced-system factorization
+
+// Tile store of the reduced system / Cholesky factor: the lower-triangle 16x16 tiles of the
+// padded system (16 T x 16 T, T = ceil(nr / 16), padding rows and columns = identity), tile
+// (ti, tj <= ti) at (ti (ti + 1) / 2 + tj) * 256, element (i, j) of a tile at
+// (i / 4) * 64 + 16 (i % 4) + j: the accumulator layout of v_mfma_f64_16x16x4f64, so a wave
+// moves a tile with four fully coalesced 512-byte accesses.
+__host__ __device__ inline int tile_base(int ti, int tj) { return (ti * (ti + 1) / 2 + tj) * kTile * kTile; }
+__host__ __device__ inline size_t factor_doubles(int nr) {
+    const size_t T = (size_t)(nr + kTile - 1) / kTile;
+    return T * (T + 1) / 2 * kTile * kTile;
+}
 
 // observation planes (doubles, each F long)
 enum ObsPlane { OB_BX, OB_BY, OB_BZ, OB_VX, OB_VY, OB_VZ, OB_T, OB_S00, OB_S10, OB_S11, OB_PLANES };
@@ -105,8 +117,9 @@ struct WinDev {
     // dense per-window buffers
     double* Hfull;             // nr x nr undamped full normal matrix of the states
     double* gfull;             // nr
-    double* S;                 // nr x nr damped reduced camera system / Cholesky factor
-    double* Scopy;             // nr x nr damped reduced system kept for refinement
+    double* fac;               // factor_doubles(nr): tile store of the damped reduced system,
+                                // overwritten by its lower Cholesky factor
+    double* Scopy;             // nr x nr damped reduced system (row-major), kept for refinement
     double* rhs;               // nr
     double* dstate;            // nr state step
     double* scratch;           // 4 * nr

@@ -243,7 +243,7 @@ size_t Slot::footprint(const HostWindow& h) {
     d += 6 * h.F + LO_PLANES * h.L + h.L;                // linearization outputs
     d += (size_t)VIB_IMU_STRIDE * h.E;
     d += m * n + m + n * n + h.prior_ref.size();
-    d += 3 * nr * nr + 8 * nr;
+    d += 2 * nr * nr + factor_doubles((int)nr) + 8 * nr;
     d += 465 * (size_t)h.E + 36 * B + n + 4 * (size_t)h.E + 4;
     d += 2 * nch * (tri + nv) + nch * CS_PLANES + 2 * (tri + nv);
     size_t i = 3 * h.L + 1 + h.F + 2 * h.E + h.pfree.size() + 4 * nr;
@@ -302,7 +302,7 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     int32_t* colmap = a.take<int32_t>(4 * nr);
     d.colmap = colmap;
     d.Hfull = a.take<double>(nr * nr); d.gfull = a.take<double>(nr);
-    d.S = a.take<double>(nr * nr); d.Scopy = a.take<double>(nr * nr);
+    d.fac = a.take<double>(factor_doubles((int)nr)); d.Scopy = a.take<double>(nr * nr);
     d.rhs = a.take<double>(nr); d.dstate = a.take<double>(nr); d.scratch = a.take<double>(4 * nr);
     const size_t len = d.tri + h.nv, nch = h.chunks.size();
     d.lin_part = a.take<double>(nch * len); d.schur_part = a.take<double>(nch * len);

@@ -946,7 +946,7 @@ __global__ __launch_bounds__(kBlock) void k_schur(const WinDev* wins, const Chun
 // reduced camera system S and rhs (prepareStateSolve, VisualCudaBatch.cu:163-172).
 // red_lin / red_schur: reduced partials (tri + nv) per window.
 // ---------------------------------------------------------------------------------------------
-constexpr int kAssembleRows = 16;  // rows of the reduced system per workgroup
+constexpr int kAssembleRows = kTile;  // rows of the reduced system per workgroup (one tile row)
 
 // Retained (IMU + prior) normal-matrix entry (i, j) from the per-edge Jacobians and the prior
 // chart derivatives / Gram matrix, using the static column map.
@@ -1041,142 +1041,237 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
         __syncthreads();
     }
     const double lambda = w.lm->lambda;
-    for (size_t p = (size_t)row0 * nr + t; p < (size_t)row1 * nr; p += nt) {
-        const int i = (int)(p / nr), j = (int)(p % nr);
-        const double h = w.Hfull[p];
+    auto damped = [&](int i, int j) {
+        const double h = w.Hfull[(size_t)i * nr + j];
         double v = h;
         if (i < nv && j < nv) v += rs[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
         if (i == j) v += lambda * fmax(h, 1e-6);
-        w.S[p] = v;
-        w.Scopy[p] = v;
+        return v;
+    };
+    for (size_t p = (size_t)row0 * nr + t; p < (size_t)row1 * nr; p += nt) {
+        const int i = (int)(p / nr), j = (int)(p % nr);
+        w.Scopy[p] = damped(i, j);
     }
     for (int i = row0 + t; i < row1; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
+    // tile row blockIdx.y of the factor tile store (lower triangle, padding identity)
+    const int ti = blockIdx.y;
+    for (int p = t; p < (ti + 1) * kTile * kTile; p += nt) {
+        const int tj = p / (kTile * kTile), e = p % (kTile * kTile);
+        const int r = kTile * ti + 4 * (e / 64) + (e % 64) / 16, c = kTile * tj + e % 16;
+        w.fac[tile_base(ti, tj) + e] = (r < nr && c < nr) ? damped(r, c) : (r == c ? 1.0 : 0.0);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_cholesky: FP64 Cholesky of the damped reduced camera system on 16x16 tiles (right-looking):
-// the diagonal tile is factored by one wave in LDS, the panel below by a row-parallel triangular
-// solve in LDS, and the trailing tiles are updated with v_mfma_f64_16x16x4f64 (accumulator
-// layout: D[i][j] in lane j + 16 (i % 4), register i / 4; A[i][k] in lane i + 16 k; B[k][j] in
-// lane j + 16 k). Then blocked forward/backward substitution, optional iterative refinement
-// against the saved damped system, the step clamp, the model decrease of the states and the
-// state trial x [+] delta (VisualCudaBatch.cu:163-189,323-340). One workgroup per window.
+// k_cholesky: FP64 Cholesky of the damped reduced camera system on 16x16 tiles (right-looking),
+// fused with the forward substitution, then the backward substitution, optional iterative
+// refinement against the saved damped system, the step clamp, the model decrease of the states
+// and the state trial x [+] delta (VisualCudaBatch.cu:163-189,323-340). One workgroup per window.
+//
+// The factor lives in the window's tile store (lower-triangle tiles in the accumulator layout of
+// v_mfma_f64_16x16x4f64, written by k_assemble, padding rows = identity). Per tile column k:
+//   A  every wave solves its panel tiles X = A L_kk^-T with MFMA against the inverse diagonal
+//      tile (LDS, operand layout), stores them to the tile store; wave 0 first applies
+//      L_kk^-1 to the right-hand side block.
+//   B  waves 1.. update the trailing tiles T_ij -= X_i X_j^T with MFMA (operands from LDS, the
+//      tile itself from the tile store; tiles of column k + 1 go to the next LDS panel).
+//      Wave 0 updates the next diagonal tile, factors and inverts it in registers (lookahead)
+//      and updates the right-hand side below the block.
+// Two workgroup barriers per tile column.
 // ---------------------------------------------------------------------------------------------
 typedef double d4 __attribute__((ext_vector_type(4)));
-constexpr int kTile = 16;
+typedef __attribute__((address_space(1))) double gdouble;
 constexpr int kMaxTiles = (kMaxRetained + kTile - 1) / kTile;
+constexpr int kTileDoubles = kTile * kTile;
 constexpr int kCholThreads = 1024;
-
-__device__ __forceinline__ double padded(const double* S, int n, int r, int c) {
-    return (r < n && c < n) ? S[(size_t)r * n + c] : (r == c ? 1.0 : 0.0);
-}
-
-
-// Lower Cholesky of S (n x n, row-major full, lower factor written back to S) fused with the
-// forward substitution of y (LDS, length 16 T, padding zero). The tile column k lives in the
-// LDS panel buffer pnl[k % 2]; trailing tiles are updated in registers and the tiles of
-// column k + 1 are handed to the next panel buffer without a round trip through memory.
 constexpr int kTileBatch = 8;
 
-__device__ bool cholesky_forward(double* S, int n, double* pnl, double* dg, double* y, int* flag) {
-    const int T = (n + kTile - 1) / kTile;
+// LDS operand layout of a 16x16 tile: element (i, c) at (c / 4) * 64 + i + 16 (c % 4), so the
+// 64 doubles of column group c / 4 are the A or B operand of one MFMA (lane i + 16 k).
+__device__ __forceinline__ int opnd_pos(int i, int c) { return (c >> 2) * 64 + i + 16 * (c & 3); }
+
+// Value of v in lane src (src wave-uniform).
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Cholesky factor and inverse of one SPD 16x16 tile held row-wise in lanes 0..15 of a wave
+// (v[c] = A[lane][c], c <= lane used). Writes L^-1 in operand layout to inv; returns false when
+// a pivot is not positive and finite (wave-uniform).
+__device__ bool factor_diag_tile(double (&v)[kTile], double* inv) {
+    const int r = threadIdx.x % 64;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < kTile; ++j) {
+        const double d = lane_bcast(v[j], j);
+        if (!(d > 0) || !isfinite(d)) ok = false;
+        const double s = sqrt(d);
+        v[j] = r == j ? s : (r > j ? v[j] / s : 0.0);
+#pragma unroll
+        for (int c = j + 1; c < kTile; ++c) {
+            const double lc = lane_bcast(v[j], c);
+            v[c] = r >= c ? v[c] - v[j] * lc : v[c];
+        }
+    }
+    double m[kTile];
+#pragma unroll
+    for (int c = 0; c < kTile; ++c) m[c] = r == c ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < kTile; ++q) {
+        const double lqq = lane_bcast(v[q], q);
+        const double il = 1.0 / lqq;
+#pragma unroll
+        for (int c = 0; c <= q; ++c) m[c] = r == q ? m[c] * il : m[c];
+#pragma unroll
+        for (int c = 0; c <= q; ++c) {
+            const double mq = lane_bcast(m[c], q);
+            m[c] = r > q ? m[c] - v[q] * mq : m[c];
+        }
+    }
+    if (r < kTile) {
+#pragma unroll
+        for (int c = 0; c < kTile; ++c) inv[opnd_pos(r, c)] = m[c];
+    }
+    return ok;
+}
+
+// Updates the tile (ti, tj) of the trailing matrix with the solved panel of column k:
+// acc -= X_ti X_tj^T (panel tiles indexed relative to k).
+__device__ __forceinline__ d4 trailing_mfma(d4 acc, const double* panel, int ai, int aj, int lane) {
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const double a = -panel[(ai * 4 + kk) * 64 + lane];
+        const double b = panel[(aj * 4 + kk) * 64 + lane];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Accumulator register rr of a lane -> tile element (i, j).
+__device__ __forceinline__ void acc_elem(int rr, int lane, int& i, int& j) { i = 4 * rr + lane / 16; j = lane % 16; }
+
+// Lower Cholesky of the padded tile store Lt (T x T tiles) with the forward substitution of y
+// (LDS, 16 T entries, padding zero) fused. pnl: two LDS panels; dinv: inverse diagonal tiles.
+__device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, double* y, int* flag,
+                                 unsigned char* tri_row) {
+    gdouble* Lt = (gdouble*)Lt_;
     const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
-    constexpr int PANEL = kMaxTiles * kTile * kTile;
-    for (int p = t; p < T * kTile * kTile; p += nt) pnl[p] = padded(S, n, p / kTile, p % kTile);
+    constexpr int PANEL = kMaxTiles * kTileDoubles;
+    for (int q = t; q < (kMaxTiles * (kMaxTiles - 1)) / 2; q += nt) {
+        int ii = 0;
+        while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
+        tri_row[q] = (unsigned char)ii;
+    }
+    for (int p = t; p < T * kTileDoubles; p += nt) {
+        const int a = p / kTileDoubles, e = p % kTileDoubles;
+        int i, j;
+        acc_elem(e / 64, e % 64, i, j);
+        pnl[a * kTileDoubles + opnd_pos(i, j)] = Lt[tile_base(a, 0) + e];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        double v[kTile];
+#pragma unroll
+        for (int c = 0; c < kTile; ++c) v[c] = lane < kTile ? pnl[opnd_pos(lane, c)] : 0.0;
+        if (!factor_diag_tile(v, dinv) && lane == 0) *flag = 1;
+    }
     __syncthreads();
     for (int k = 0; k < T; ++k) {
+        if (*flag) return false;
         double* panel = pnl + (k & 1) * PANEL;
         double* next = pnl + ((k + 1) & 1) * PANEL;
-        const int r0 = kTile * k, rows = kTile * T - r0;
-        if (wave == 0) {
-            for (int j = 0; j < kTile; ++j) {
-                if (lane == 0) {
-                    const double d = panel[j * kTile + j];
-                    if (!(d > 0) || !isfinite(d)) *flag = 1;
-                    panel[j * kTile + j] = sqrt(d);
-                }
-                wave_sync();
-                if (lane > j && lane < kTile) panel[lane * kTile + j] /= panel[j * kTile + j];
-                wave_sync();
-                for (int p = lane; p < kTile * kTile; p += 64) {
-                    const int r = p / kTile, c = p % kTile;
-                    if (c > j && r >= c) panel[p] -= panel[r * kTile + j] * panel[c * kTile + j];
-                }
-                wave_sync();
-            }
-        }
-        __syncthreads();
-        if (*flag) return false;
-        for (int pr = kTile + t; pr < rows; pr += nt) {
-            double* row = panel + pr * kTile;
-            double v[kTile];
-            for (int c = 0; c < kTile; ++c) v[c] = row[c];
-            for (int c = 0; c < kTile; ++c) {
-                for (int q = 0; q < c; ++q) v[c] -= v[q] * panel[c * kTile + q];
-                v[c] /= panel[c * kTile + c];
-            }
-            for (int c = 0; c < kTile; ++c) row[c] = v[c];
-        }
-        __syncthreads();
-        // forward substitution of the diagonal block, done by wave 0 while the rest store L
-        if (wave == 0) {
-            for (int c = 0; c < kTile; ++c) {
-                if (lane == 0) y[r0 + c] /= panel[c * kTile + c];
-                wave_sync();
-                if (lane > c && lane < kTile) y[r0 + lane] -= panel[lane * kTile + c] * y[r0 + c];
-                wave_sync();
-            }
-        }
-        for (int p = t; p < kTile * kTile; p += nt) dg[k * kTile * kTile + p] = panel[p];
-        for (int p = t; p < rows * kTile; p += nt) {
-            const int r = r0 + p / kTile, c = r0 + p % kTile;
-            if (r < n && c < n && c <= r) S[(size_t)r * n + c] = panel[p];
-        }
-        __syncthreads();
-        for (int pr = kTile + t; pr < rows; pr += nt) {
-            double v = 0;
-            for (int c = 0; c < kTile; ++c) v += panel[pr * kTile + c] * y[r0 + c];
-            y[r0 + pr] -= v;
-        }
-        // trailing tiles (ti, tj), k < tj <= ti < T, processed in register batches
+        const double* inv = dinv + k * kTileDoubles;
         const int m = T - k - 1;
-        const int ntiles = m * (m + 1) / 2;
-        for (int q0 = wave; q0 < ntiles; q0 += kTileBatch * nw) {
-            d4 acc[kTileBatch];
-            int tti[kTileBatch], ttj[kTileBatch];
+        // A: y_k <- L_kk^-1 y_k; panel solve
+        if (wave == 0) {
+            double yr = 0;
+            if (lane < kTile) {
 #pragma unroll
-            for (int b = 0; b < kTileBatch; ++b) {
-                const int q = q0 + b * nw;
-                tti[b] = -1;
-                if (q < ntiles) {
-                    int ii = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
-                    while (ii * (ii + 1) / 2 > q) --ii;
-                    while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
-                    tti[b] = k + 1 + ii;
-                    ttj[b] = k + 1 + (q - ii * (ii + 1) / 2);
-                    for (int rr = 0; rr < 4; ++rr)
-                        acc[b][rr] = padded(S, n, kTile * tti[b] + lane / 16 + 4 * rr, kTile * ttj[b] + lane % 16);
-                }
+                for (int c = 0; c < kTile; ++c) yr += inv[opnd_pos(lane, c)] * y[kTile * k + c];
             }
+            wave_sync();
+            if (lane < kTile) y[kTile * k + lane] = yr;
+        }
+        for (int a = 1 + wave; a <= m; a += nw) {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int b = 0; b < kTileBatch; ++b) {
-                if (tti[b] < 0) continue;
-                for (int kk = 0; kk < 4; ++kk) {
-                    const double a = -panel[(kTile * (tti[b] - k) + lane % 16) * kTile + 4 * kk + lane / 16];
-                    const double bb = panel[(kTile * (ttj[b] - k) + lane % 16) * kTile + 4 * kk + lane / 16];
-                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[b], 0, 0, 0);
-                }
+            for (int kk = 0; kk < 4; ++kk)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(panel[(a * 4 + kk) * 64 + lane], inv[kk * 64 + lane], acc, 0, 0, 0);
+            const int base = tile_base(k + a, k);
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                int i, j;
+                acc_elem(rr, lane, i, j);
+                panel[a * kTileDoubles + opnd_pos(i, j)] = acc[rr];
+                Lt[base + rr * 64 + lane] = acc[rr];
             }
+        }
+        __syncthreads();
+        // B: lookahead diagonal + right-hand side on wave 0, trailing tiles on the other waves
+        if (wave == 0) {
+            if (m > 0) {
+                const int base = tile_base(k + 1, k + 1);
+                d4 acc;
 #pragma unroll
-            for (int b = 0; b < kTileBatch; ++b) {
-                if (tti[b] < 0) continue;
-                if (ttj[b] == k + 1) {
-                    for (int rr = 0; rr < 4; ++rr)
-                        next[(kTile * (tti[b] - k - 1) + lane / 16 + 4 * rr) * kTile + lane % 16] = acc[b][rr];
-                } else {
-                    for (int rr = 0; rr < 4; ++rr) {
-                        const int r = kTile * tti[b] + lane / 16 + 4 * rr, c = kTile * ttj[b] + lane % 16;
-                        if (r < n && c < n) S[(size_t)r * n + c] = acc[b][rr];
+                for (int rr = 0; rr < 4; ++rr) acc[rr] = Lt[base + rr * 64 + lane];
+                acc = trailing_mfma(acc, panel, 1, 1, lane);
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    int i, j;
+                    acc_elem(rr, lane, i, j);
+                    next[opnd_pos(i, j)] = acc[rr];
+                }
+                wave_sync();
+                double v[kTile];
+#pragma unroll
+                for (int c = 0; c < kTile; ++c) v[c] = lane < kTile ? next[opnd_pos(lane, c)] : 0.0;
+                if (!factor_diag_tile(v, dinv + (k + 1) * kTileDoubles) && lane == 0) *flag = 1;
+            }
+            for (int rr = lane; rr < kTile * m; rr += 64) {
+                const int a = 1 + rr / kTile, i = rr % kTile;
+                double s = 0;
+#pragma unroll
+                for (int c = 0; c < kTile; ++c) s += panel[a * kTileDoubles + opnd_pos(i, c)] * y[kTile * k + c];
+                y[kTile * (k + a) + i] -= s;
+            }
+        } else {
+            const int ntiles = m * (m + 1) / 2, nwt = nw - 1;
+            for (int q0 = wave; q0 < ntiles; q0 += kTileBatch * nwt) {
+                d4 acc[kTileBatch];
+                int ai[kTileBatch], aj[kTileBatch];
+#pragma unroll
+                for (int b = 0; b < kTileBatch; ++b) {
+                    const int q = q0 + b * nwt;
+                    ai[b] = -1;
+                    if (q < ntiles) {
+                        const int ii = tri_row[q];
+                        ai[b] = 1 + ii;
+                        aj[b] = 1 + q - ii * (ii + 1) / 2;
+                        const int base = tile_base(k + ai[b], k + aj[b]);
+#pragma unroll
+                        for (int rr = 0; rr < 4; ++rr) acc[b][rr] = Lt[base + rr * 64 + lane];
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < kTileBatch; ++b)
+                    if (ai[b] >= 0) acc[b] = trailing_mfma(acc[b], panel, ai[b], aj[b], lane);
+#pragma unroll
+                for (int b = 0; b < kTileBatch; ++b) {
+                    if (ai[b] < 0) continue;
+                    if (aj[b] == 1) {
+#pragma unroll
+                        for (int rr = 0; rr < 4; ++rr) {
+                            int i, j;
+                            acc_elem(rr, lane, i, j);
+                            next[(ai[b] - 1) * kTileDoubles + opnd_pos(i, j)] = acc[b][rr];
+                        }
+                    } else {
+                        const int base = tile_base(k + ai[b], k + aj[b]);
+#pragma unroll
+                        for (int rr = 0; rr < 4; ++rr) Lt[base + rr * 64 + lane] = acc[b][rr];
                     }
                 }
             }
@@ -1186,98 +1281,149 @@ __device__ bool cholesky_forward(double* S, int n, double* pnl, double* dg, doub
     return true;
 }
 
-// Backward substitution L^T x = y in place (x in LDS) with the factor in S and diagonal tiles in
-// dg. part: 16 x 16 doubles of LDS for cross-wave partial sums.
-__device__ void cholesky_backward(const double* S, int n, const double* dg, double* x, double* part) {
-    const int T = (n + kTile - 1) / kTile;
+// L^T x = y in place (x in LDS, 16 T entries) with the panel tiles from the tile store and the
+// inverse diagonal tiles from dinv. part: 16 doubles per wave of LDS.
+__device__ void cholesky_backward(const double* Lt_, int T, const double* dinv, double* x, double* part) {
+    const gdouble* Lt = (const gdouble*)Lt_;
     const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
-    const int rsub = lane / 16, c = lane % 16;
+    d4 cur[2], nxt[2];
     for (int k = T - 1; k >= 0; --k) {
-        const int r0 = kTile * k, kb = min(kTile, n - r0);
-        double v = 0;
-        if (c < kb)
-            for (int r = r0 + kTile + 4 * wave + rsub; r < n; r += 4 * nw) v += S[(size_t)r * n + r0 + c] * x[r];
-        v += __shfl_xor(v, 16, 64);
-        v += __shfl_xor(v, 32, 64);
-        if (lane < kTile) part[wave * kTile + lane] = v;
-        __syncthreads();
-        if (wave == 0) {
-            if (lane < kb) {
-                double s = 0;
-                for (int q = 0; q < nw; ++q) s += part[q * kTile + lane];
-                x[r0 + lane] -= s;
-            }
-            wave_sync();
-            for (int cc = kb - 1; cc >= 0; --cc) {
-                if (lane == 0) x[r0 + cc] /= dg[k * 256 + cc * kTile + cc];
-                wave_sync();
-                if (lane < cc) x[r0 + lane] -= dg[k * 256 + cc * kTile + lane] * x[r0 + cc];
-                wave_sync();
+        if (k == T - 1) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int a = k + 1 + wave + s * nw;
+                if (a < T)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) cur[s][rr] = Lt[tile_base(a, k) + rr * 64 + lane];
             }
         }
+        if (k > 0) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int a = k + wave + s * nw;
+                if (a < T)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) nxt[s][rr] = Lt[tile_base(a, k - 1) + rr * 64 + lane];
+            }
+        }
+        double p = 0;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int a = k + 1 + wave + s * nw;
+            if (a < T)
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) p += cur[s][rr] * x[kTile * a + 4 * rr + lane / 16];
+        }
+        p += __shfl_xor(p, 16, 64);
+        p += __shfl_xor(p, 32, 64);
+        if (lane < kTile) part[wave * kTile + lane] = p;
         __syncthreads();
+        if (wave == 0) {
+            const double* inv = dinv + k * kTileDoubles;
+            double u = 0;
+            if (lane < kTile) {
+                u = x[kTile * k + lane];
+                for (int w = 0; w < nw; ++w) u -= part[w * kTile + lane];
+            }
+            double xr = 0;
+#pragma unroll
+            for (int c = 0; c < kTile; ++c) xr += inv[opnd_pos(c, lane % kTile)] * lane_bcast(u, c);
+            if (lane < kTile) x[kTile * k + lane] = xr;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) cur[s] = nxt[s];
     }
 }
 
 // Forward + backward solve with an existing factor (iterative refinement).
-__device__ void cholesky_solve_again(const double* S, int n, const double* dg, double* x, double* part) {
-    const int T = (n + kTile - 1) / kTile;
-    const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64;
+__device__ void cholesky_solve_again(const double* Lt_, int T, const double* dinv, double* x, double* part) {
+    const gdouble* Lt = (const gdouble*)Lt_;
+    const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
     for (int k = 0; k < T; ++k) {
-        const int r0 = kTile * k, kb = min(kTile, n - r0);
         if (wave == 0) {
-            for (int cc = 0; cc < kb; ++cc) {
-                if (lane == 0) x[r0 + cc] /= dg[k * 256 + cc * kTile + cc];
-                wave_sync();
-                if (lane > cc && lane < kb) x[r0 + lane] -= dg[k * 256 + lane * kTile + cc] * x[r0 + cc];
-                wave_sync();
+            const double* inv = dinv + k * kTileDoubles;
+            const double u = lane < kTile ? x[kTile * k + lane] : 0.0;
+            double xr = 0;
+#pragma unroll
+            for (int c = 0; c < kTile; ++c) xr += inv[opnd_pos(lane % kTile, c)] * lane_bcast(u, c);
+            if (lane < kTile) x[kTile * k + lane] = xr;
+        }
+        __syncthreads();
+        for (int a = k + 1 + wave; a < T; a += nw) {
+            const int base = tile_base(a, k);
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                double v = Lt[base + rr * 64 + lane] * x[kTile * k + lane % 16];
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                v += __shfl_xor(v, 4, 64);
+                v += __shfl_xor(v, 8, 64);
+                if (lane % 16 == 0) x[kTile * a + 4 * rr + lane / 16] -= v;
             }
         }
         __syncthreads();
-        for (int r = r0 + kTile + t; r < n; r += nt) {
-            const double* row = S + (size_t)r * n + r0;
-            double v = 0;
-            for (int cc = 0; cc < kb; ++cc) v += row[cc] * x[r0 + cc];
-            x[r] -= v;
-        }
-        __syncthreads();
     }
-    cholesky_backward(S, n, dg, x, part);
+    cholesky_backward(Lt_, T, dinv, x, part);
+}
+
+// Sum over a wave, then over the waves of the block in fixed order (deterministic).
+__device__ double block_sum_waves(double v, double* red) {
+    const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
+    v = wave_reduce_sum(v);
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0;
+    for (int w = 0; w < nw; ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+__device__ double block_max_waves(double v, double* red) {
+    const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
+    for (int m = 32; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int w = 1; w < nw; ++w) s = fmax(s, red[w]);
+    __syncthreads();
+    return s;
 }
 
 __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int refinement) {
-    __shared__ double pnl[2 * kMaxTiles * kTile * kTile];
-    __shared__ double dg[kMaxTiles * kTile * kTile];
+    __shared__ double pnl[2 * kMaxTiles * kTileDoubles];
+    __shared__ double dinv[kMaxTiles * kTileDoubles];
     __shared__ double x[kMaxTiles * kTile];
     __shared__ double rres[kMaxTiles * kTile];
-    __shared__ double part[kTile * kTile];
-    __shared__ double red[kCholThreads];
+    __shared__ double part[kCholThreads / 64 * kTile];
+    __shared__ double red[kCholThreads / 64];
+    __shared__ unsigned char tri_row[(kMaxTiles * (kMaxTiles - 1)) / 2];
     __shared__ int flag;
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
-    const int n = w.nr, t = threadIdx.x, nt = blockDim.x;
-    const int N = kTile * ((n + kTile - 1) / kTile);
+    const int n = w.nr, t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
+    const int T = (n + kTile - 1) / kTile, N = kTile * T;
     if (t == 0) flag = 0;
     for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[i] : 0.0;
     __syncthreads();
-    const bool ok = cholesky_forward(w.S, n, pnl, dg, x, &flag);
+    const bool ok = cholesky_forward(w.fac, T, pnl, dinv, x, &flag, tri_row);
     if (!ok) {
         if (t == 0) w.lm->chol_ok = 0;
         return;
     }
-    cholesky_backward(w.S, n, dg, x, part);
+    cholesky_backward(w.fac, T, dinv, x, part);
     for (int it = 0; it < refinement; ++it) {
-        for (int i = t; i < N; i += nt) {
+        for (int i = wave; i < N; i += nw) {
             double v = 0;
             if (i < n) {
-                v = w.rhs[i];
                 const double* row = w.Scopy + (size_t)i * n;
-                for (int j = 0; j < n; ++j) v -= row[j] * x[j];
+                for (int j = lane; j < n; j += 64) v += row[j] * x[j];
             }
-            rres[i] = v;
+            v = wave_reduce_sum(v);
+            if (lane == 0) rres[i] = i < n ? w.rhs[i] - v : 0.0;
         }
         __syncthreads();
-        cholesky_solve_again(w.S, n, dg, rres, part);
+        cholesky_solve_again(w.fac, T, dinv, rres, part);
         for (int i = t; i < n; i += nt) x[i] += rres[i];
         __syncthreads();
     }
@@ -1287,24 +1433,24 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
         x[w.td_col] = fmax(w.td_lo - tdv, fmin(w.td_hi - tdv, x[w.td_col]));
     }
     __syncthreads();
-    // state model decrease: -dx^T (g + 0.5 H dx), undamped H
+    // state model decrease: -dx^T (g + 0.5 H dx), undamped H (one wave per row)
     double pred = 0;
-    for (int i = t; i < n; i += nt) {
+    for (int i = wave; i < n; i += nw) {
         const double* row = w.Hfull + (size_t)i * n;
         double hd = 0;
-        for (int j = 0; j < n; ++j) hd += row[j] * x[j];
-        pred += -x[i] * (w.gfull[i] + 0.5 * hd);
+        for (int j = lane; j < n; j += 64) hd += row[j] * x[j];
+        hd = wave_reduce_sum(hd);
+        if (lane == 0) pred += -x[i] * (w.gfull[i] + 0.5 * hd);
     }
-    const double ps = block_sum(pred, red);
-    double mx = 0;
-    int bad = 0;
+    const double ps = block_sum_waves(pred, red);
+    double mx = 0, bad = 0;
     for (int i = t; i < n; i += nt) {
         w.dstate[i] = x[i];
         if (!isfinite(x[i])) bad = 1;
         mx = fmax(mx, fabs(x[i]));
     }
-    const double maxinc = block_max(mx, red);
-    const double nonfinite = block_max(bad ? 1.0 : 0.0, red);
+    const double maxinc = block_max_waves(mx, red);
+    const double nonfinite = block_max_waves(bad, red);
     // state trial
     for (int k = t; k < w.K; k += nt) {
         const double* xp = w.pose + 7 * k;
