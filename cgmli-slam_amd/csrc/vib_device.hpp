@@ -28,6 +28,19 @@ constexpr int kTile = 16;           // tile edge of the reduced-system factoriza
 // (i / 4) * 64 + 16 (i % 4) + j: the accumulator layout of v_mfma_f64_16x16x4f64, so a wave
 // moves a tile with four fully coalesced 512-byte accesses.
 __host__ __device__ inline int tile_base(int ti, int tj) { return (ti * (ti + 1) / 2 + tj) * kTile * kTile; }
+// Off-diagonal tiles below the first tile column (ti > tj >= 1) of a T x T tile system in the
+// order of decreasing tj, then increasing ti: index e -> (ti, tj). The factorization keeps the
+// leading ones in registers for the whole elimination (they are updated most often).
+__host__ __device__ inline void register_tile(int T, int e, int& ti, int& tj) {
+    int d = 1;
+    while (d * (d + 1) / 2 <= e) ++d;
+    tj = T - 1 - d;
+    ti = tj + 1 + (e - d * (d - 1) / 2);
+}
+// The same tiles in reverse order (increasing tj, then decreasing ti): index f -> (ti, tj).
+__host__ __device__ inline void store_tile(int T, int f, int& ti, int& tj) {
+    register_tile(T, (T - 1) * (T - 2) / 2 - 1 - f, ti, tj);
+}
 __host__ __device__ inline size_t factor_doubles(int nr) {
     const size_t T = (size_t)(nr + kTile - 1) / kTile;
     return T * (T + 1) / 2 * kTile * kTile;

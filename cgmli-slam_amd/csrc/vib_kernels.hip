@@ -1098,43 +1098,63 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Cholesky factor and inverse of one SPD 16x16 tile held row-wise in lanes 0..15 of a wave
-// (v[c] = A[lane][c], c <= lane used). Writes L^-1 in operand layout to inv; returns false when
-// a pivot is not positive and finite (wave-uniform).
+// Value of v in lane Q of the lane's own 16-lane row (DPP row_newbcast).
+template <int Q>
+__device__ __forceinline__ double row_bcast(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x150 + Q, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + Q, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1 / s from the hardware reciprocal and two Newton steps (error below one unit in the last place).
+__device__ __forceinline__ double recip(double s) {
+    double q = __builtin_amdgcn_rcp(s);
+    q = fma(fma(-s, q, 1.0), q, q);
+    q = fma(fma(-s, q, 1.0), q, q);
+    return q;
+}
+
+// Rows Q.. of M = L^-1 (forward elimination, one row per step): lane (r, g) of the wave holds
+// columns 4 cc + g of row r of M in m[cc]; v is row r of L.
+template <int Q>
+__device__ __forceinline__ void inverse_rows(double (&m)[4], const double (&v)[kTile], int r) {
+    if constexpr (Q < kTile) {
+        const double sc = r == Q ? recip(lane_bcast(v[Q], Q)) : 1.0;
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc)
+            if (4 * cc <= Q) m[cc] *= sc;
+        const double lq = r > Q ? v[Q] : 0.0;
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc)
+            if (4 * cc <= Q) m[cc] -= lq * row_bcast<Q>(m[cc]);
+        inverse_rows<Q + 1>(m, v, r);
+    }
+}
+
+// Cholesky factor and inverse of one SPD 16x16 tile; every 16-lane row of the wave holds the
+// tile row-wise (v[c] = A[lane % 16][c], c <= lane % 16 used). The factor is formed redundantly
+// in the four rows, the inverse split over them by column (c % 4 = row index of the lane group);
+// L^-1 goes to inv in operand layout. Returns false when a pivot is not positive and finite.
 __device__ bool factor_diag_tile(double (&v)[kTile], double* inv) {
-    const int r = threadIdx.x % 64;
+    const int lane = threadIdx.x % 64, r = lane % kTile, g = lane / kTile;
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < kTile; ++j) {
         const double d = lane_bcast(v[j], j);
         if (!(d > 0) || !isfinite(d)) ok = false;
         const double s = sqrt(d);
-        v[j] = r == j ? s : (r > j ? v[j] / s : 0.0);
+        v[j] = r == j ? s : (r > j ? v[j] * recip(s) : 0.0);
+        // rows above j hold zeros in column j; entries right of the diagonal are never read
 #pragma unroll
-        for (int c = j + 1; c < kTile; ++c) {
-            const double lc = lane_bcast(v[j], c);
-            v[c] = r >= c ? v[c] - v[j] * lc : v[c];
-        }
+        for (int c = j + 1; c < kTile; ++c) v[c] -= v[j] * lane_bcast(v[j], c);
     }
-    double m[kTile];
+    double m[4];
 #pragma unroll
-    for (int c = 0; c < kTile; ++c) m[c] = r == c ? 1.0 : 0.0;
+    for (int cc = 0; cc < 4; ++cc) m[cc] = r == 4 * cc + g ? 1.0 : 0.0;
+    inverse_rows<0>(m, v, r);
 #pragma unroll
-    for (int q = 0; q < kTile; ++q) {
-        const double lqq = lane_bcast(v[q], q);
-        const double il = 1.0 / lqq;
-#pragma unroll
-        for (int c = 0; c <= q; ++c) m[c] = r == q ? m[c] * il : m[c];
-#pragma unroll
-        for (int c = 0; c <= q; ++c) {
-            const double mq = lane_bcast(m[c], q);
-            m[c] = r > q ? m[c] - v[q] * mq : m[c];
-        }
-    }
-    if (r < kTile) {
-#pragma unroll
-        for (int c = 0; c < kTile; ++c) inv[opnd_pos(r, c)] = m[c];
-    }
+    for (int cc = 0; cc < 4; ++cc) inv[cc * 64 + lane] = m[cc];
     return ok;
 }
 
@@ -1153,18 +1173,57 @@ __device__ __forceinline__ d4 trailing_mfma(d4 acc, const double* panel, int ai,
 // Accumulator register rr of a lane -> tile element (i, j).
 __device__ __forceinline__ void acc_elem(int rr, int lane, int& i, int& j) { i = 4 * rr + lane / 16; j = lane % 16; }
 
+__device__ __forceinline__ void acc_to_panel(const d4& acc, double* tile, int lane) {
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        int i, j;
+        acc_elem(rr, lane, i, j);
+        tile[opnd_pos(i, j)] = acc[rr];
+    }
+}
+
+constexpr int kRegSlots = 7;   // register-resident trailing tiles per trailing wave
+constexpr int kGlobBatch = 3;  // tile-store trailing tiles in flight per wave
+
+// Panel solve of tile column k: X_a = A_a L_kk^-T for the panel tiles a = 1.. (relative to k),
+// written to the LDS panel (operand layout) and to the tile store; wave 0 first applies
+// L_kk^-1 to the right-hand side block y_k.
+__device__ __forceinline__ void panel_solve(gdouble* Lt, double* panel, const double* inv, double* y, int k, int m,
+                                            int wave, int lane, int nw) {
+    if (wave == 0) {
+        double yr = 0;
+        if (lane < kTile) {
+#pragma unroll
+            for (int c = 0; c < kTile; ++c) yr += inv[opnd_pos(lane, c)] * y[kTile * k + c];
+        }
+        wave_sync();
+        if (lane < kTile) y[kTile * k + lane] = yr;
+    }
+    for (int a = 1 + wave; a <= m; a += nw) {
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(panel[(a * 4 + kk) * 64 + lane], inv[kk * 64 + lane], acc, 0, 0, 0);
+        const int base = tile_base(k + a, k);
+        acc_to_panel(acc, panel + a * kTileDoubles, lane);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) Lt[base + rr * 64 + lane] = acc[rr];
+    }
+}
+
 // Lower Cholesky of the padded tile store Lt (T x T tiles) with the forward substitution of y
 // (LDS, 16 T entries, padding zero) fused. pnl: two LDS panels; dinv: inverse diagonal tiles.
-__device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, double* y, int* flag,
-                                 unsigned char* tri_row) {
+// Wave 0 runs the diagonal tiles (update, factor, inverse: one step ahead of the trailing
+// update); the other waves run the trailing update. The off-diagonal trailing tiles with the
+// largest column index (register_tile order) stay in accumulator registers of the trailing
+// waves for the whole factorization; the others and the diagonal tiles are read from and
+// written back to the tile store at every update. The two roles run separate step loops with
+// the same barrier sequence, so the resident tiles do not constrain the factor wave's registers.
+__device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, double* y, int* flag) {
     gdouble* Lt = (gdouble*)Lt_;
-    const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
+    const int t = threadIdx.x, nt = blockDim.x, lane = t % 64, nw = nt / 64, nwt = nw - 1;
+    const int wave = __builtin_amdgcn_readfirstlane(t / 64);
     constexpr int PANEL = kMaxTiles * kTileDoubles;
-    for (int q = t; q < (kMaxTiles * (kMaxTiles - 1)) / 2; q += nt) {
-        int ii = 0;
-        while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
-        tri_row[q] = (unsigned char)ii;
-    }
     for (int p = t; p < T * kTileDoubles; p += nt) {
         const int a = p / kTileDoubles, e = p % kTileDoubles;
         int i, j;
@@ -1173,108 +1232,118 @@ __device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, 
     }
     __syncthreads();
     if (wave == 0) {
-        double v[kTile];
+        {
+            double v[kTile];
 #pragma unroll
-        for (int c = 0; c < kTile; ++c) v[c] = lane < kTile ? pnl[opnd_pos(lane, c)] : 0.0;
-        if (!factor_diag_tile(v, dinv) && lane == 0) *flag = 1;
-    }
-    __syncthreads();
-    for (int k = 0; k < T; ++k) {
-        if (*flag) return false;
-        double* panel = pnl + (k & 1) * PANEL;
-        double* next = pnl + ((k + 1) & 1) * PANEL;
-        const double* inv = dinv + k * kTileDoubles;
-        const int m = T - k - 1;
-        // A: y_k <- L_kk^-1 y_k; panel solve
-        if (wave == 0) {
-            double yr = 0;
-            if (lane < kTile) {
-#pragma unroll
-                for (int c = 0; c < kTile; ++c) yr += inv[opnd_pos(lane, c)] * y[kTile * k + c];
-            }
-            wave_sync();
-            if (lane < kTile) y[kTile * k + lane] = yr;
-        }
-        for (int a = 1 + wave; a <= m; a += nw) {
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(panel[(a * 4 + kk) * 64 + lane], inv[kk * 64 + lane], acc, 0, 0, 0);
-            const int base = tile_base(k + a, k);
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                int i, j;
-                acc_elem(rr, lane, i, j);
-                panel[a * kTileDoubles + opnd_pos(i, j)] = acc[rr];
-                Lt[base + rr * 64 + lane] = acc[rr];
-            }
+            for (int c = 0; c < kTile; ++c) v[c] = pnl[opnd_pos(lane % kTile, c)];
+            if (!factor_diag_tile(v, dinv) && lane == 0) *flag = 1;
         }
         __syncthreads();
-        // B: lookahead diagonal + right-hand side on wave 0, trailing tiles on the other waves
-        if (wave == 0) {
+        for (int k = 0; k < T; ++k) {
+            if (*flag) return false;
+            double* panel = pnl + (k & 1) * PANEL;
+            double* next = pnl + ((k + 1) & 1) * PANEL;
+            const int m = T - k - 1;
+            panel_solve(Lt, panel, dinv + k * kTileDoubles, y, k, m, wave, lane, nw);
+            __syncthreads();
             if (m > 0) {
                 const int base = tile_base(k + 1, k + 1);
                 d4 acc;
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) acc[rr] = Lt[base + rr * 64 + lane];
                 acc = trailing_mfma(acc, panel, 1, 1, lane);
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) {
-                    int i, j;
-                    acc_elem(rr, lane, i, j);
-                    next[opnd_pos(i, j)] = acc[rr];
-                }
+                acc_to_panel(acc, next, lane);
                 wave_sync();
                 double v[kTile];
 #pragma unroll
-                for (int c = 0; c < kTile; ++c) v[c] = lane < kTile ? next[opnd_pos(lane, c)] : 0.0;
+                for (int c = 0; c < kTile; ++c) v[c] = next[opnd_pos(lane % kTile, c)];
                 if (!factor_diag_tile(v, dinv + (k + 1) * kTileDoubles) && lane == 0) *flag = 1;
             }
-            for (int rr = lane; rr < kTile * m; rr += 64) {
-                const int a = 1 + rr / kTile, i = rr % kTile;
-                double s = 0;
+            __syncthreads();
+        }
+        return true;
+    }
+    const int noff = (T - 1) * (T - 2) / 2;
+    const int nreg = min(noff, kRegSlots * nwt);
+    const int G = noff - nreg;
+    d4 racc[kRegSlots];
+    int rti[kRegSlots], rtj[kRegSlots];
 #pragma unroll
-                for (int c = 0; c < kTile; ++c) s += panel[a * kTileDoubles + opnd_pos(i, c)] * y[kTile * k + c];
-                y[kTile * (k + a) + i] -= s;
-            }
-        } else {
-            const int ntiles = m * (m + 1) / 2, nwt = nw - 1;
-            for (int q0 = wave; q0 < ntiles; q0 += kTileBatch * nwt) {
-                d4 acc[kTileBatch];
-                int ai[kTileBatch], aj[kTileBatch];
+    for (int s = 0; s < kRegSlots; ++s) {
+        const int e = s * nwt + wave - 1;
+        rti[s] = 0;
+        rtj[s] = -1;
+        if (e < nreg) {
+            register_tile(T, e, rti[s], rtj[s]);
+            const int base = tile_base(rti[s], rtj[s]);
 #pragma unroll
-                for (int b = 0; b < kTileBatch; ++b) {
-                    const int q = q0 + b * nwt;
-                    ai[b] = -1;
-                    if (q < ntiles) {
-                        const int ii = tri_row[q];
-                        ai[b] = 1 + ii;
-                        aj[b] = 1 + q - ii * (ii + 1) / 2;
-                        const int base = tile_base(k + ai[b], k + aj[b]);
+            for (int rr = 0; rr < 4; ++rr) racc[s][rr] = Lt[base + rr * 64 + lane];
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < T; ++k) {
+        if (*flag) return false;
+        double* panel = pnl + (k & 1) * PANEL;
+        double* next = pnl + ((k + 1) & 1) * PANEL;
+        const int m = T - k - 1;
+        panel_solve(Lt, panel, dinv + k * kTileDoubles, y, k, m, wave, lane, nw);
+        __syncthreads();
+        // right-hand side rows below the block
+        for (int rr = (wave - 1) * 64 + lane; rr < kTile * m; rr += nwt * 64) {
+            const int a = 1 + rr / kTile, i = rr % kTile;
+            double s = 0;
 #pragma unroll
-                        for (int rr = 0; rr < 4; ++rr) acc[b][rr] = Lt[base + rr * 64 + lane];
-                    }
+            for (int c = 0; c < kTile; ++c) s += panel[a * kTileDoubles + opnd_pos(i, c)] * y[kTile * k + c];
+            y[kTile * (k + a) + i] -= s;
+        }
+        // tile-store items of this step: diagonal tiles k + 2.., then the off-diagonal tiles
+        // outside the register set with column > k
+        const int nd = max(0, T - k - 2);
+        const int Fk = k * (T - 1) - k * (k + 1) / 2;
+        const int nitems = nd + max(0, G - Fk);
+        d4 gacc[kGlobBatch];
+        int gi[kGlobBatch], gj[kGlobBatch];
+        int w0 = wave - 1;
+        auto load_batch = [&]() {
+#pragma unroll
+            for (int b = 0; b < kGlobBatch; ++b) {
+                const int w = w0 + b * nwt;
+                gi[b] = -1;
+                gj[b] = 0;
+                if (w < nitems) {
+                    if (w < nd) gi[b] = gj[b] = k + 2 + w;
+                    else store_tile(T, Fk + w - nd, gi[b], gj[b]);
+                    const int base = tile_base(gi[b], gj[b]);
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) gacc[b][rr] = Lt[base + rr * 64 + lane];
                 }
+            }
+        };
+        auto finish_batch = [&]() {
 #pragma unroll
-                for (int b = 0; b < kTileBatch; ++b)
-                    if (ai[b] >= 0) acc[b] = trailing_mfma(acc[b], panel, ai[b], aj[b], lane);
+            for (int b = 0; b < kGlobBatch; ++b) {
+                if (gi[b] < 0) continue;
+                gacc[b] = trailing_mfma(gacc[b], panel, gi[b] - k, gj[b] - k, lane);
+                if (gj[b] == k + 1) {
+                    acc_to_panel(gacc[b], next + (gi[b] - k - 1) * kTileDoubles, lane);
+                } else {
+                    const int base = tile_base(gi[b], gj[b]);
 #pragma unroll
-                for (int b = 0; b < kTileBatch; ++b) {
-                    if (ai[b] < 0) continue;
-                    if (aj[b] == 1) {
-#pragma unroll
-                        for (int rr = 0; rr < 4; ++rr) {
-                            int i, j;
-                            acc_elem(rr, lane, i, j);
-                            next[(ai[b] - 1) * kTileDoubles + opnd_pos(i, j)] = acc[b][rr];
-                        }
-                    } else {
-                        const int base = tile_base(k + ai[b], k + aj[b]);
-#pragma unroll
-                        for (int rr = 0; rr < 4; ++rr) Lt[base + rr * 64 + lane] = acc[b][rr];
-                    }
+                    for (int rr = 0; rr < 4; ++rr) Lt[base + rr * 64 + lane] = gacc[b][rr];
                 }
             }
+        };
+        load_batch();
+#pragma unroll
+        for (int s = 0; s < kRegSlots; ++s) {
+            if (rtj[s] <= k) continue;
+            racc[s] = trailing_mfma(racc[s], panel, rti[s] - k, rtj[s] - k, lane);
+            if (rtj[s] == k + 1) acc_to_panel(racc[s], next + (rti[s] - k - 1) * kTileDoubles, lane);
+        }
+        finish_batch();
+        for (w0 += kGlobBatch * nwt; w0 < nitems; w0 += kGlobBatch * nwt) {
+            load_batch();
+            finish_batch();
         }
         __syncthreads();
     }
@@ -1397,7 +1466,6 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     __shared__ double rres[kMaxTiles * kTile];
     __shared__ double part[kCholThreads / 64 * kTile];
     __shared__ double red[kCholThreads / 64];
-    __shared__ unsigned char tri_row[(kMaxTiles * (kMaxTiles - 1)) / 2];
     __shared__ int flag;
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
@@ -1406,7 +1474,7 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     if (t == 0) flag = 0;
     for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[i] : 0.0;
     __syncthreads();
-    const bool ok = cholesky_forward(w.fac, T, pnl, dinv, x, &flag, tri_row);
+    const bool ok = cholesky_forward(w.fac, T, pnl, dinv, x, &flag);
     if (!ok) {
         if (t == 0) w.lm->chol_ok = 0;
         return;
