@@ -1098,57 +1098,70 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Value of v in lane Q of the lane's own 16-lane row (DPP row_newbcast).
+// Value of v in lane Q of the lane's own 16-lane row (one v_mov_b64_dpp row_newbcast).
 template <int Q>
 __device__ __forceinline__ double row_bcast(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x150 + Q, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + Q, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + Q, 0xF, 0xF, false);
 }
 
-// 1 / s from the hardware reciprocal and two Newton steps (error below one unit in the last place).
-__device__ __forceinline__ double recip(double s) {
-    double q = __builtin_amdgcn_rcp(s);
-    q = fma(fma(-s, q, 1.0), q, q);
-    q = fma(fma(-s, q, 1.0), q, q);
+// 1 / sqrt(d) from the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double q = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    q = q * fma(-h * q, q, 1.5);
+    q = q * fma(-h * q, q, 1.5);
     return q;
 }
 
+// Rank-1 update of columns C.. of the rows with column J of L: A[r][c] -= L[r][J] L[c][J].
+template <int J, int C>
+__device__ __forceinline__ void factor_update(double (&v)[kTile]) {
+    if constexpr (C < kTile) {
+        v[C] = fma(-v[J], row_bcast<C>(v[J]), v[C]);
+        factor_update<J, C + 1>(v);
+    }
+}
+
+// Columns J.. of the factor (right-looking): lane r of a 16-lane row holds row r of the tile.
+template <int J>
+__device__ __forceinline__ void factor_columns(double (&v)[kTile], int r, bool& ok) {
+    if constexpr (J < kTile) {
+        const double d = row_bcast<J>(v[J]);
+        if (!(d > 0) || !isfinite(d)) ok = false;
+        const double is = rsqrt_nr(d);
+        v[J] = r == J ? is : (r > J ? v[J] * is : 0.0);
+        // rows above J hold zeros in column J; entries right of the diagonal are never read
+        factor_update<J, J + 1>(v);
+        factor_columns<J + 1>(v, r, ok);
+    }
+}
+
 // Rows Q.. of M = L^-1 (forward elimination, one row per step): lane (r, g) of the wave holds
-// columns 4 cc + g of row r of M in m[cc]; v is row r of L.
+// columns 4 cc + g of row r of M in m[cc]; v is row r of L with 1 / L[r][r] on the diagonal.
 template <int Q>
 __device__ __forceinline__ void inverse_rows(double (&m)[4], const double (&v)[kTile], int r) {
     if constexpr (Q < kTile) {
-        const double sc = r == Q ? recip(lane_bcast(v[Q], Q)) : 1.0;
+        const double sc = r == Q ? row_bcast<Q>(v[Q]) : 1.0;
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc)
             if (4 * cc <= Q) m[cc] *= sc;
         const double lq = r > Q ? v[Q] : 0.0;
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc)
-            if (4 * cc <= Q) m[cc] -= lq * row_bcast<Q>(m[cc]);
+            if (4 * cc <= Q) m[cc] = fma(-lq, row_bcast<Q>(m[cc]), m[cc]);
         inverse_rows<Q + 1>(m, v, r);
     }
 }
 
 // Cholesky factor and inverse of one SPD 16x16 tile; every 16-lane row of the wave holds the
 // tile row-wise (v[c] = A[lane % 16][c], c <= lane % 16 used). The factor is formed redundantly
-// in the four rows, the inverse split over them by column (c % 4 = row index of the lane group);
-// L^-1 goes to inv in operand layout. Returns false when a pivot is not positive and finite.
+// in the four rows (the diagonal keeps 1 / L[j][j]), the inverse split over them by column
+// (c % 4 = row index of the lane group); L^-1 goes to inv in operand layout. Returns false when
+// a pivot is not positive and finite.
 __device__ bool factor_diag_tile(double (&v)[kTile], double* inv) {
     const int lane = threadIdx.x % 64, r = lane % kTile, g = lane / kTile;
     bool ok = true;
-#pragma unroll
-    for (int j = 0; j < kTile; ++j) {
-        const double d = lane_bcast(v[j], j);
-        if (!(d > 0) || !isfinite(d)) ok = false;
-        const double s = sqrt(d);
-        v[j] = r == j ? s : (r > j ? v[j] * recip(s) : 0.0);
-        // rows above j hold zeros in column j; entries right of the diagonal are never read
-#pragma unroll
-        for (int c = j + 1; c < kTile; ++c) v[c] -= v[j] * lane_bcast(v[j], c);
-    }
+    factor_columns<0>(v, r, ok);
     double m[4];
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) m[cc] = r == 4 * cc + g ? 1.0 : 0.0;
@@ -1194,7 +1207,7 @@ __device__ __forceinline__ void panel_solve(gdouble* Lt, double* panel, const do
         double yr = 0;
         if (lane < kTile) {
 #pragma unroll
-            for (int c = 0; c < kTile; ++c) yr += inv[opnd_pos(lane, c)] * y[kTile * k + c];
+            for (int c = 0; c < kTile; ++c) yr = fma(inv[opnd_pos(lane, c)], y[kTile * k + c], yr);
         }
         wave_sync();
         if (lane < kTile) y[kTile * k + lane] = yr;
@@ -1244,13 +1257,15 @@ __device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, 
             double* panel = pnl + (k & 1) * PANEL;
             double* next = pnl + ((k + 1) & 1) * PANEL;
             const int m = T - k - 1;
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            if (m > 0) {  // the next diagonal tile is final since the last step: fetch it early
+                const int base = tile_base(k + 1, k + 1);
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) acc[rr] = Lt[base + rr * 64 + lane];
+            }
             panel_solve(Lt, panel, dinv + k * kTileDoubles, y, k, m, wave, lane, nw);
             __syncthreads();
             if (m > 0) {
-                const int base = tile_base(k + 1, k + 1);
-                d4 acc;
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) acc[rr] = Lt[base + rr * 64 + lane];
                 acc = trailing_mfma(acc, panel, 1, 1, lane);
                 acc_to_panel(acc, next, lane);
                 wave_sync();
@@ -1293,7 +1308,7 @@ __device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, 
             const int a = 1 + rr / kTile, i = rr % kTile;
             double s = 0;
 #pragma unroll
-            for (int c = 0; c < kTile; ++c) s += panel[a * kTileDoubles + opnd_pos(i, c)] * y[kTile * k + c];
+            for (int c = 0; c < kTile; ++c) s = fma(panel[a * kTileDoubles + opnd_pos(i, c)], y[kTile * k + c], s);
             y[kTile * (k + a) + i] -= s;
         }
         // tile-store items of this step: diagonal tiles k + 2.., then the off-diagonal tiles
@@ -1381,7 +1396,7 @@ __device__ void cholesky_backward(const double* Lt_, int T, const double* dinv, 
             const int a = k + 1 + wave + s * nw;
             if (a < T)
 #pragma unroll
-                for (int rr = 0; rr < 4; ++rr) p += cur[s][rr] * x[kTile * a + 4 * rr + lane / 16];
+                for (int rr = 0; rr < 4; ++rr) p = fma(cur[s][rr], x[kTile * a + 4 * rr + lane / 16], p);
         }
         p += __shfl_xor(p, 16, 64);
         p += __shfl_xor(p, 32, 64);
@@ -1396,7 +1411,7 @@ __device__ void cholesky_backward(const double* Lt_, int T, const double* dinv, 
             }
             double xr = 0;
 #pragma unroll
-            for (int c = 0; c < kTile; ++c) xr += inv[opnd_pos(c, lane % kTile)] * lane_bcast(u, c);
+            for (int c = 0; c < kTile; ++c) xr = fma(inv[opnd_pos(c, lane % kTile)], lane_bcast(u, c), xr);
             if (lane < kTile) x[kTile * k + lane] = xr;
         }
         __syncthreads();
@@ -1415,7 +1430,7 @@ __device__ void cholesky_solve_again(const double* Lt_, int T, const double* din
             const double u = lane < kTile ? x[kTile * k + lane] : 0.0;
             double xr = 0;
 #pragma unroll
-            for (int c = 0; c < kTile; ++c) xr += inv[opnd_pos(lane % kTile, c)] * lane_bcast(u, c);
+            for (int c = 0; c < kTile; ++c) xr = fma(inv[opnd_pos(lane % kTile, c)], lane_bcast(u, c), xr);
             if (lane < kTile) x[kTile * k + lane] = xr;
         }
         __syncthreads();
