@@ -57,6 +57,18 @@ def kernel_flops(name, win):
     return None
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE profile of this
+    workload (counters scaled by a copy of known size), or None when no profile exists."""
+    path = os.path.join(ROOT, "profiles", f"schur_step_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        prof = json.load(f)
+    k = prof.get("kernels", {}).get(kernel)
+    return None if k is None else k["read_bytes"] + k["write_bytes"]
+
+
 def cpu_baseline(win, seconds=12.0):
     """Oracle FP64 restatement of the reference CPU path, one thread: steps/s on a bounded sample
     (repeated LM steps on the same window until ~`seconds` of CPU work)."""
@@ -88,6 +100,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="after the measurement, launch one plain device copy of this many MiB "
+                         "(reference dispatch for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); windows and sessions are "
@@ -110,6 +125,8 @@ def main():
     print(f"[bench] timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
     _, k_ms = ctx.bench([0], opts, max(5, min(args.steps, 20)), kernels=True)
     ctx.synchronize()
+    if args.copy_probe_mb > 0:
+        ctx.copy_probe(args.copy_probe_mb << 20)
     ctx.close()
 
     dom = max(k_ms, key=k_ms.get)
@@ -118,12 +135,14 @@ def main():
     kf = kernel_flops(dom, win)
     if kb is not None:
         ach = kb / (k_ms[dom] * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None,
-                    kernel=dom, kernel_ms=k_ms[dom])
+        roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                    traffic=measured_traffic(args.workload, dom), kernel=dom, kernel_ms=k_ms[dom],
+                    algorithmic_bytes=kb)
     elif kf is not None:
         ach = kf / (k_ms[dom] * 1e-3) / 1e12
         roof = dict(bound="mfma", achieved=ach, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=ach / FP64_PEAK_TFLOPS,
-                    traffic=None, kernel=dom, kernel_ms=k_ms[dom])
+                    traffic=measured_traffic(args.workload, dom), kernel=dom, kernel_ms=k_ms[dom],
+                    algorithmic_flops=kf)
     value = args.steps / wall
     out = {
         "metric": "Schur window steps/sec + p50/p99 latency, mixed vs FP64 GPU at Δx rel err≤1e-6",

@@ -29,6 +29,7 @@ const char* vibk_name(int k);
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
                        int max_len, int max_edges, int refinement, int mode, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
+hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 }
 
 namespace vib {
@@ -766,6 +767,23 @@ int vib_solve_batch(vib_context* ctx, int32_t count, const int32_t* ids, const v
     const int st = run_lm(ctx, slots, *o, reps);
     for (size_t i = 0; i < reps.size(); ++i) reports[i] = reps[i];
     return st;
+}
+
+int vib_copy_probe(vib_context* ctx, int64_t bytes) {
+    if (!ctx || bytes <= 0 || bytes % 16) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    void *src = nullptr, *dst = nullptr;
+    if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) {
+        if (src) (void)hipFree(src);
+        return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "copy probe allocation failed");
+    }
+    bool ok = hipMemsetAsync(src, 0, bytes, ctx->stream) == hipSuccess &&
+              vibk_copy_probe(src, dst, (size_t)bytes, ctx->stream) == hipSuccess &&
+              hipStreamSynchronize(ctx->stream) == hipSuccess;
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return ok ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "copy probe failed");
 }
 
 int vib_kernel_count(void) { return kKernelCount; }

@@ -1817,6 +1817,18 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
     return hipGetLastError();
 }
 
+// Plain streaming copy of n 16-byte words (traffic calibration of the HBM counters).
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError();
+}
+
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s) {
     hipLaunchKernelGGL(vib::k_prior_gram, dim3(nwin), dim3(vib::kBlock), 0, s, wins, G);
     return hipGetLastError();

@@ -302,6 +302,7 @@ def load_library(path=LIB_PATH):
         "vib_kernel_count": ([], C.c_int),
         "vib_kernel_name": ([C.c_int32], C.c_char_p),
         "vib_bench": ([vp, C.c_int32, ip, C.POINTER(VibSolveOptions), C.c_int32, dp, dp], C.c_int),
+        "vib_copy_probe": ([vp, C.c_int64], C.c_int),
         "vib_preintegrate": ([dp, C.c_int32, C.c_double, C.c_double, dp, C.POINTER(VibImuParams), dp, dp, dp], C.c_int),
         "vib_pinhole_observations": ([C.c_double] * 5 + [C.c_int32, dp, dp, dp], C.c_int),
     }
@@ -352,6 +353,10 @@ class Context:
 
     def synchronize(self):
         return self.check(self.lib.vib_device_synchronize(self.h), "vib_device_synchronize")
+
+    def copy_probe(self, nbytes):
+        """One plain device copy of nbytes (counter calibration)."""
+        return self.check(self.lib.vib_copy_probe(self.h, int(nbytes)), "vib_copy_probe")
 
     def error(self):
         m = self.lib.vib_last_error(self.h)

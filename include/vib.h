@@ -260,6 +260,9 @@ const char* vib_kernel_name(int32_t kernel);
  * of each kernel (vib_kernel_count entries) to kernel_ms. Either output may be NULL. */
 int vib_bench(vib_context* ctx, int32_t count, const int32_t* slots, const vib_solve_options* options,
               int32_t iterations, double* iteration_ms, double* kernel_ms);
+/* One launch of a plain streaming copy of `bytes` (multiple of 16) between two fresh device
+ * buffers: the known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters. */
+int vib_copy_probe(vib_context* ctx, int64_t bytes);
 
 /* ---------------- marginalization ---------------- */
 /* OLD: Schur-eliminate slot 0 (pose, speed/bias) plus every landmark anchored in or observed
