@@ -483,6 +483,7 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
     if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     const bool graph = ctx->use_graph && batch.capture(ctx->stream, o.refinement_steps);
     std::vector<LmState> states(slots.size());
+    const auto t_iter = std::chrono::steady_clock::now();  // the time budget covers the iterations
     for (int it = 0; it < maxit; ++it) {
         bool ok;
         if (graph) ok = hipGraphLaunch(batch.graph_exec, ctx->stream) == hipSuccess;
@@ -499,7 +500,7 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
         for (auto& s : states) running = running || s.status == 0;
         if (!running) break;
         if (o.time_limit_seconds > 0 && it + 1 >= o.min_iterations &&
-            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= o.time_limit_seconds)
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count() >= o.time_limit_seconds)
             break;
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -524,6 +525,124 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
     }
     if (first_error != VIB_OK) ctx->error = status_message(first_error);
     return first_error;
+}
+
+// Host copy of a slot with the state it holds on the device now.
+bool pull_state(vib_context* ctx, Slot& s, HostWindow& h) {
+    h = s.host;
+    const WinDev& d = s.dev;
+    return hipMemcpyAsync(h.pose.data(), d.pose, 56 * (size_t)d.K, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+           hipMemcpyAsync(h.sb.data(), d.sb, 72 * (size_t)d.K, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+           hipMemcpyAsync(h.rig, d.rig, 56, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+           hipMemcpyAsync(&h.td, d.td, 8, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+           (d.L == 0 || hipMemcpyAsync(h.rho.data(), d.rho, 8 * (size_t)d.L, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
+           hipStreamSynchronize(ctx->stream) == hipSuccess;
+}
+
+struct LatestTwo {
+    HostWindow win;
+    int edges[2] = {-1, -1};  // local IMU edge ending at K-2, edge K-2 -> K-1
+    int support = 0;
+};
+
+// Local graph of the two-frame solve (BackendOptimizer.cpp:476-538): factors touching the pose
+// or speed/bias of slots K-2, K-1; those blocks free with the velocity-only speed/bias tangent,
+// everything else constant.
+int latest_two_window(const HostWindow& h, LatestTwo& out, std::string& err) {
+    if (h.K < 2) { err = "missing pose or velocity block in two-frame solve"; return VIB_ERR_LIGHTWEIGHT_SUPPORT; }
+    const int prev = h.K - 2, last = h.K - 1;
+    auto is_free = [&](int k) { return k == prev || k == last; };
+    HostWindow& w = out.win;
+    w = HostWindow();
+    w.K = h.K; w.C = h.C; w.E = 0;
+    w.flags = h.flags & ~(VIB_RIG_FREE | VIB_TD_FREE);
+    w.huber = h.huber; w.min_rho = h.min_rho; w.td_lo = h.td_lo; w.td_hi = h.td_hi;
+    w.frame_id = h.frame_id; w.pose = h.pose; w.sb = h.sb;
+    std::copy_n(h.rig, 7, w.rig);
+    w.td = h.td; w.cam = h.cam;
+    w.pose_free.assign(h.K, 0);
+    w.sb_width.assign(h.K, 0);
+    w.pose_free[prev] = w.pose_free[last] = 1;
+    w.sb_width[prev] = w.sb_width[last] = 3;
+    // visual factors touching a free pose, and their landmarks (held constant)
+    std::vector<int> keep_lm, keep_f;
+    std::vector<int> off{0};
+    std::vector<char> seen(h.L, 0);
+    for (int l = 0; l < h.L; ++l) {
+        const size_t before = keep_f.size();
+        for (int f = h.lm_off[l]; f < h.lm_off[l + 1]; ++f)
+            if (is_free(h.lm_frame[l]) || is_free(h.ob_fc[f] >> 8)) {
+                keep_f.push_back(f);
+                if ((h.ob_fc[f] >> 8) == last && !seen[l]) { seen[l] = 1; ++out.support; }
+            }
+        if (keep_f.size() > before) {
+            keep_lm.push_back(l);
+            off.push_back((int)keep_f.size());
+        }
+    }
+    if (out.support < 6) { err = "fewer than six current landmarks in two-frame solve"; return VIB_ERR_LIGHTWEIGHT_SUPPORT; }
+    w.L = (int)keep_lm.size();
+    w.F = (int)keep_f.size();
+    w.lm_off = off;
+    w.lm_frame.resize(w.L); w.lm_cam.resize(w.L); w.lm_free.assign(w.L, 0); w.rho.resize(w.L);
+    w.lmd.assign((size_t)LM_PLANES * w.L, 0);
+    for (int i = 0; i < w.L; ++i) {
+        const int l = keep_lm[i];
+        w.lm_frame[i] = h.lm_frame[l]; w.lm_cam[i] = h.lm_cam[l]; w.rho[i] = h.rho[l];
+        for (int p = 0; p < LM_PLANES; ++p) w.lmd[(size_t)p * w.L + i] = h.lmd[(size_t)p * h.L + l];
+    }
+    w.ob_fc.resize(w.F); w.ob_lm.resize(w.F);
+    w.ob.assign((size_t)OB_PLANES * w.F, 0);
+    for (int i = 0, li = 0; i < w.F; ++i) {
+        while (i >= w.lm_off[li + 1]) ++li;
+        const int f = keep_f[i];
+        w.ob_fc[i] = h.ob_fc[f];
+        w.ob_lm[i] = li;
+        for (int p = 0; p < OB_PLANES; ++p) w.ob[(size_t)p * w.F + i] = h.ob[(size_t)p * h.F + f];
+    }
+    // the IMU bridge: exactly the edge ending at K-2 and the edge K-2 -> K-1
+    for (int e = 0; e < h.E; ++e) {
+        if (!is_free(h.imu_from[e]) && !is_free(h.imu_to[e])) continue;
+        const int i = w.E++;
+        w.imu_from.push_back(h.imu_from[e]);
+        w.imu_to.push_back(h.imu_to[e]);
+        w.imu.insert(w.imu.end(), h.imu.begin() + (size_t)VIB_IMU_STRIDE * e, h.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+        if (h.imu_to[e] == prev) out.edges[0] = i;
+        if (h.imu_from[e] == prev && h.imu_to[e] == last) out.edges[1] = i;
+    }
+    if (w.E != 2 || out.edges[0] < 0 || out.edges[1] < 0) { err = "two-frame IMU bridge missing"; return VIB_ERR_IMU_BRIDGE; }
+    // the prior takes part when one of its blocks is a free block
+    bool touches = false;
+    for (size_t b = 0; b < h.prior_kind.size(); ++b)
+        if ((h.prior_kind[b] == VIB_KEY_POSE || h.prior_kind[b] == VIB_KEY_SPEED_BIAS) &&
+            (h.prior_id[b] == h.frame_id[prev] || h.prior_id[b] == h.frame_id[last]))
+            touches = true;
+    if (touches) {
+        w.prior_rows = h.prior_rows; w.prior_cols = h.prior_cols;
+        w.prior_kind = h.prior_kind; w.prior_id = h.prior_id;
+        w.prior_ref = h.prior_ref; w.prior_A = h.prior_A; w.prior_r0 = h.prior_r0;
+    }
+    return w.layout(err) ? VIB_OK : VIB_ERR_INVALID_ARGUMENT;
+}
+
+// Squared whitened residuals (NIS) of two IMU edges of a slot at its resident state.
+bool imu_nis(vib_context* ctx, Slot& s, const int edges[2], double nis[2]) {
+    LmState lm{};
+    lm.lambda = 1e-4; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1;
+    std::vector<Slot*> one{&s};
+    Batch b;
+    std::string err;
+    std::vector<double> r((size_t)15 * s.host.E);
+    if (!write_lm(ctx, s, lm) || !b.build(one, ctx->stream, err) || !b.launch(1, ctx->stream, 0, 0) ||
+        hipMemcpyAsync(r.data(), s.dev.imu_r, r.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return false;
+    for (int i = 0; i < 2; ++i) {
+        double v = 0;
+        for (int m = 0; m < 15; ++m) v += r[(size_t)15 * edges[i] + m] * r[(size_t)15 * edges[i] + m];
+        nis[i] = v;
+    }
+    return true;
 }
 
 }  // namespace
@@ -851,9 +970,56 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
     return VIB_OK;
 }
 
-int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* o, vib_solve_report* rep) {
-    (void)slot; (void)o; (void)rep;
-    return fail(ctx, VIB_ERR_UNSUPPORTED, "two-frame solve not available in this build");
+int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* o, int32_t gate,
+                         vib_latest_two_report* rep) {
+    if (!ctx || !o || !rep) return VIB_ERR_INVALID_ARGUMENT;
+    if (o->precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    *rep = vib_latest_two_report{};
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    HostWindow cur;
+    if (!pull_state(ctx, *s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    LatestTwo lt;
+    std::string err;
+    int st = latest_two_window(cur, lt, err);
+    rep->support_landmarks = lt.support;
+    if (st != VIB_OK) return fail(ctx, st, err);
+    Slot local;
+    if (!local.upload(lt.win, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    double nis[2];
+    if (!imu_nis(ctx, local, lt.edges, nis)) return fail(ctx, VIB_ERR_DEVICE, "IMU residual evaluation failed");
+    rep->old_imu_nis_before = nis[0];
+    rep->new_imu_nis_before = nis[1];
+    if (!std::isfinite(nis[0]) || !std::isfinite(nis[1]))
+        return fail(ctx, VIB_ERR_IMU_BRIDGE, "two-frame IMU residual invalid before solve");
+    vib_solve_options lo = *o;  // budget of the lightweight solve (BackendOptimizer.cpp:549-553)
+    lo.max_iterations = std::min(3, std::max(1, o->max_iterations));
+    lo.min_iterations = std::min(1, std::max(0, o->min_iterations));
+    lo.time_limit_seconds = o->time_limit_seconds > 0 ? std::min(0.008, o->time_limit_seconds) : 0.008;
+    lo.flags &= ~VIB_SOLVE_CONTINUOUS;
+    std::vector<Slot*> one{&local};
+    std::vector<vib_solve_report> reps;
+    st = run_lm(ctx, one, lo, reps);
+    if (!reps.empty()) rep->solve = reps[0];
+    if (st != VIB_OK) return st;
+    if (!imu_nis(ctx, local, lt.edges, nis)) return fail(ctx, VIB_ERR_DEVICE, "IMU residual evaluation failed");
+    rep->old_imu_nis_after = nis[0];
+    rep->new_imu_nis_after = nis[1];
+    if (!std::isfinite(nis[0]) || !std::isfinite(nis[1]))
+        return fail(ctx, VIB_ERR_IMU_BRIDGE, "two-frame IMU bridge invalid after solve");
+    if (gate && nis[1] > 60.0)
+        return fail(ctx, VIB_ERR_IMU_GATE, "new IMU edge residual exceeded lightweight health gate");
+    // write back poses and velocities of K-2 and K-1 (biases are constant in this solve)
+    const int prev = cur.K - 2;
+    bool ok = hipMemcpyAsync(s->dev.pose + 7 * prev, local.dev.pose + 7 * prev, 14 * sizeof(double),
+                             hipMemcpyDeviceToDevice, ctx->stream) == hipSuccess;
+    for (int k = prev; k < prev + 2; ++k)
+        ok = ok && hipMemcpyAsync(s->dev.sb + 9 * k, local.dev.sb + 9 * k, 3 * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
+    return ok ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "state write-back failed");
 }
 
 int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double* samples, int32_t count,

@@ -18,8 +18,13 @@ STATUS_NAMES = {
     0: "OK", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED_DIMENSIONS", 3: "INVALID_RESIDUAL",
     4: "NOT_POSITIVE_DEFINITE", 5: "COST_MISMATCH", 6: "NO_ACCEPTABLE_STEP", 7: "UNSUPPORTED_LOSS",
     8: "DEVICE", 9: "OUT_OF_MEMORY", 10: "NONFINITE_STEP", 11: "INDEFINITE", 12: "EIGEN_FAILED",
-    13: "NO_DEVICE", 14: "NOT_FOUND", 15: "UNSUPPORTED",
+    13: "NO_DEVICE", 14: "NOT_FOUND", 15: "UNSUPPORTED", 16: "LIGHTWEIGHT_SUPPORT", 17: "IMU_BRIDGE",
+    18: "IMU_GATE",
 }
+VIB_ERR_UNSUPPORTED = 15
+VIB_ERR_LIGHTWEIGHT_SUPPORT = 16
+VIB_ERR_IMU_BRIDGE = 17
+VIB_ERR_IMU_GATE = 18
 VIB_RIG_FREE = 1
 VIB_TD_FREE = 2
 VIB_KEY_POSE, VIB_KEY_SPEED_BIAS, VIB_KEY_RIG, VIB_KEY_TD = 0, 1, 2, 3
@@ -73,6 +78,12 @@ class VibSolveReport(C.Structure):
                 ("converged", C.c_int32), ("effective_parameters", C.c_int32),
                 ("retained_dimension", C.c_int32), ("initial_cost", C.c_double),
                 ("final_cost", C.c_double), ("final_lambda", C.c_double), ("seconds", C.c_double)]
+
+
+class VibLatestTwoReport(C.Structure):
+    _fields_ = [("solve", VibSolveReport), ("old_imu_nis_before", C.c_double), ("old_imu_nis_after", C.c_double),
+                ("new_imu_nis_before", C.c_double), ("new_imu_nis_after", C.c_double),
+                ("support_landmarks", C.c_int32)]
 
 
 class VibMarginalReport(C.Structure):
@@ -295,7 +306,8 @@ def load_library(path=LIB_PATH):
         "vib_retained_dimension": ([vp, C.c_int32, ip, ip], C.c_int),
         "vib_solve": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.POINTER(VibSolveReport)], C.c_int),
         "vib_solve_batch": ([vp, C.c_int32, ip, C.POINTER(VibSolveOptions), C.POINTER(VibSolveReport)], C.c_int),
-        "vib_solve_latest_two": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.POINTER(VibSolveReport)], C.c_int),
+        "vib_solve_latest_two": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.c_int32, C.POINTER(VibLatestTwoReport)],
+                                 C.c_int),
         "vib_marginalize": ([vp, C.c_int32, C.c_int32, dp, C.c_int32, C.POINTER(VibImuParams),
                              C.POINTER(VibMarginalReport)], C.c_int),
         "vib_read_prior": ([vp, C.c_int32, ip, ip, ip, ip, lp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
@@ -411,6 +423,11 @@ class Context:
         st = self.lib.vib_solve(self.h, slot, C.byref(opts), C.byref(rep))
         return st, rep
 
+    def solve_latest_two(self, slot, opts, gate=False):
+        rep = VibLatestTwoReport()
+        st = self.lib.vib_solve_latest_two(self.h, slot, C.byref(opts), 1 if gate else 0, C.byref(rep))
+        return st, rep
+
     def solve_batch(self, slots, opts):
         arr = np.ascontiguousarray(slots, dtype=np.int32)
         reps = (VibSolveReport * len(arr))()
@@ -427,11 +444,6 @@ class Context:
         self.check(st, "vib_bench")
         names = [self.lib.vib_kernel_name(i).decode() for i in range(nk)]
         return it_ms, dict(zip(names, k_ms))
-
-    def solve_latest_two(self, slot, opts):
-        rep = VibSolveReport()
-        st = self.lib.vib_solve_latest_two(self.h, slot, C.byref(opts), C.byref(rep))
-        return st, rep
 
     def marginalize(self, slot, policy, samples=None, params=None):
         rep = VibMarginalReport()

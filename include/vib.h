@@ -54,7 +54,13 @@ enum {
     VIB_ERR_EIGEN_FAILED = 12,         /* eigensolver did not converge                      */
     VIB_ERR_NO_DEVICE = 13,            /* no HIP device                                     */
     VIB_ERR_NOT_FOUND = 14,            /* unknown slot / session                            */
-    VIB_ERR_UNSUPPORTED = 15           /* unsupported configuration                          */
+    VIB_ERR_UNSUPPORTED = 15,          /* unsupported configuration                          */
+    VIB_ERR_LIGHTWEIGHT_SUPPORT = 16,  /* "missing pose or velocity block" / "fewer than six
+                                          current landmarks in two-frame solve" (:500-514)   */
+    VIB_ERR_IMU_BRIDGE = 17,           /* "two-frame IMU bridge missing" / "... invalid before
+                                          / after solve" (:528-567)                          */
+    VIB_ERR_IMU_GATE = 18              /* "new IMU edge residual exceeded lightweight health
+                                          gate" (NIS > 60, :572-577)                         */
 };
 
 /* window flags */
@@ -197,6 +203,15 @@ typedef struct vib_marginal_report {
     double eigen_cutoff;
 } vib_marginal_report;
 
+typedef struct vib_latest_two_report {  /* SolveReport lightweight fields (:541-562) */
+    vib_solve_report solve;
+    double old_imu_nis_before;   /* squared whitened residual of the edge ending at K-2   */
+    double old_imu_nis_after;
+    double new_imu_nis_before;   /* edge K-2 -> K-1                                        */
+    double new_imu_nis_after;
+    int32_t support_landmarks;   /* distinct landmarks observed in the latest frame       */
+} vib_latest_two_report;
+
 typedef struct vib_context vib_context;
 
 /* ---------------- context ---------------- */
@@ -244,10 +259,15 @@ int vib_solve(vib_context* ctx, int32_t slot, const vib_solve_options* options, 
 /* Many independent windows in lock-step batched launches (one graph per LM iteration). */
 int vib_solve_batch(vib_context* ctx, int32_t count, const int32_t* slots,
                     const vib_solve_options* options, vib_solve_report* reports);
-/* Two-frame lightweight solve (optimizeLatestTwo, BackendOptimizer.cpp:476-590): free blocks
- * pose/velocity of slots K-2 and K-1, landmarks and biases constant, <= 3 iterations. */
+/* Two-frame lightweight solve (optimizeLatestTwo, BackendOptimizer.cpp:476-590, scoped
+ * pose/velocity manifold :594,783-790): free blocks are the poses and velocities of slots K-2
+ * and K-1; landmarks, biases, rig, td and the other frames are constant; only factors touching
+ * a free block take part; >= 6 distinct landmarks observed in frame K-1 and exactly the two IMU
+ * edges ending at K-2 and K-2 -> K-1 are required; at most 3 iterations and 8 ms. With
+ * gate_new_imu_nis the result is rejected when the new edge's NIS exceeds 60. On success the
+ * poses and velocities of K-2 and K-1 in the resident window are updated. */
 int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* options,
-                         vib_solve_report* report);
+                         int32_t gate_new_imu_nis, vib_latest_two_report* report);
 
 /* ---------------- measurement harness (no counterpart in the reference) ---------------- */
 /* Number of distinct kernels of one LM iteration and their names (for per-kernel timing). */

@@ -1408,6 +1408,97 @@ int preintegrate(const double* samples, int count, double t0, double t1, const d
 }
 
 }  // namespace
+// ------------------------------------------------------------------------------------
+// Two-frame lightweight solve (BackendOptimizer.cpp:476-590; the velocity-only speed/bias
+// tangent of the scoped solve, :594,783-790). Free: pose and velocity of the two newest slots;
+// everything else constant; only factors with a key among the free blocks.
+// ------------------------------------------------------------------------------------
+double imuNis(const Window& w, const State& x, int e) {
+    const Frozen imu = unpackImu(&w.imu[(size_t)VIB_IMU_STRIDE * e]);
+    const int s0 = w.imuFrom[e], s1 = w.imuTo[e];
+    double r[15];
+    if (!imuEvaluate(imu, &x.pose[7 * s0], &x.sb[9 * s0], &x.pose[7 * s1], &x.sb[9 * s1], r, nullptr))
+        return std::numeric_limits<double>::quiet_NaN();
+    double v = 0;
+    for (double ri : r) v += ri * ri;
+    return v;
+}
+
+int latestTwo(Window& w, const vib_solve_options& opt, int gate, double* nis, int& support, vib_solve_report& rep) {
+    for (int i = 0; i < 4; ++i) nis[i] = std::numeric_limits<double>::quiet_NaN();
+    support = 0;
+    if (w.K < 2) return VIB_ERR_LIGHTWEIGHT_SUPPORT;
+    const int previous = w.K - 2, latest = w.K - 1;
+    auto isFree = [&](int k) { return k == previous || k == latest; };
+    Window loc = w;
+    loc.flags &= ~(VIB_RIG_FREE | VIB_TD_FREE);
+    for (int k = 0; k < w.K; ++k) { loc.poseFree[k] = isFree(k); loc.sbWidth[k] = isFree(k) ? 3 : 0; }
+    // visual factors: keys are anchor pose, observer pose, rig, td, landmark
+    std::vector<int> lmMap(w.L, -1);
+    loc.L = 0; loc.lmFrame.clear(); loc.lmCam.clear(); loc.lmBearing.clear(); loc.lmVel.clear();
+    loc.lmTref.clear(); loc.rho.clear(); loc.lmFree.clear();
+    loc.F = 0; loc.obsLm.clear(); loc.obsFrame.clear(); loc.obsCam.clear(); loc.obsBearing.clear();
+    loc.obsVel.clear(); loc.obsTref.clear(); loc.obsSqrt.clear();
+    std::set<int> current;
+    for (int f = 0; f < w.F; ++f) {
+        const int l = w.obsLm[f];
+        if (!isFree(w.lmFrame[l]) && !isFree(w.obsFrame[f])) continue;
+        if (lmMap[l] < 0) {
+            lmMap[l] = loc.L++;
+            loc.lmFrame.push_back(w.lmFrame[l]); loc.lmCam.push_back(w.lmCam[l]);
+            for (int i = 0; i < 3; ++i) { loc.lmBearing.push_back(w.lmBearing[3 * l + i]); loc.lmVel.push_back(w.lmVel[3 * l + i]); }
+            loc.lmTref.push_back(w.lmTref[l]); loc.rho.push_back(w.rho[l]); loc.lmFree.push_back(0);
+        }
+        ++loc.F;
+        loc.obsLm.push_back(lmMap[l]); loc.obsFrame.push_back(w.obsFrame[f]); loc.obsCam.push_back(w.obsCam[f]);
+        for (int i = 0; i < 3; ++i) {
+            loc.obsBearing.push_back(w.obsBearing[3 * f + i]); loc.obsVel.push_back(w.obsVel[3 * f + i]);
+            loc.obsSqrt.push_back(w.obsSqrt[3 * f + i]);
+        }
+        loc.obsTref.push_back(w.obsTref[f]);
+        if (w.obsFrame[f] == latest) current.insert(l);
+    }
+    support = (int)current.size();
+    if (support < 6) return VIB_ERR_LIGHTWEIGHT_SUPPORT;
+    // IMU factors touching a free block: exactly the edge into `previous` and previous -> latest
+    int oldEdge = -1, newEdge = -1;
+    loc.E = 0; loc.imuFrom.clear(); loc.imuTo.clear(); loc.imu.clear();
+    for (int e = 0; e < w.E; ++e) {
+        if (!isFree(w.imuFrom[e]) && !isFree(w.imuTo[e])) continue;
+        if (w.imuTo[e] == previous) oldEdge = loc.E;
+        if (w.imuFrom[e] == previous && w.imuTo[e] == latest) newEdge = loc.E;
+        ++loc.E;
+        loc.imuFrom.push_back(w.imuFrom[e]); loc.imuTo.push_back(w.imuTo[e]);
+        loc.imu.insert(loc.imu.end(), w.imu.begin() + (size_t)VIB_IMU_STRIDE * e, w.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+    }
+    if (loc.E != 2 || oldEdge < 0 || newEdge < 0) return VIB_ERR_IMU_BRIDGE;
+    // the prior participates when it has a free key
+    bool touches = false;
+    for (int b = 0; b < w.priorBlocks; ++b)
+        if ((w.priorKind[b] == VIB_KEY_POSE || w.priorKind[b] == VIB_KEY_SPEED_BIAS) && isFree(slotOfFrameId(w, w.priorId[b])))
+            touches = true;
+    if (!touches) { loc.priorRows = 0; loc.priorBlocks = 0; }
+    const State x0 = stateOf(loc);
+    nis[0] = imuNis(loc, x0, oldEdge);
+    nis[2] = imuNis(loc, x0, newEdge);
+    if (!std::isfinite(nis[0]) || !std::isfinite(nis[2])) return VIB_ERR_IMU_BRIDGE;
+    vib_solve_options o = opt;
+    o.max_iterations = std::min(3, std::max(1, opt.max_iterations));
+    o.min_iterations = std::min(1, std::max(0, opt.min_iterations));
+    State x;
+    const int st = lmSolve(loc, o, rep, x);
+    if (st != VIB_OK) return st;
+    nis[1] = imuNis(loc, x, oldEdge);
+    nis[3] = imuNis(loc, x, newEdge);
+    if (!std::isfinite(nis[1]) || !std::isfinite(nis[3])) return VIB_ERR_IMU_BRIDGE;
+    if (gate && nis[3] > 60.0) return VIB_ERR_IMU_GATE;
+    for (int k : {previous, latest}) {
+        for (int i = 0; i < 7; ++i) w.pose[7 * k + i] = x.pose[7 * k + i];
+        for (int i = 0; i < 3; ++i) w.sb[9 * k + i] = x.sb[9 * k + i];
+    }
+    return VIB_OK;
+}
+
 }  // namespace oracle
 
 using namespace oracle;
@@ -1553,6 +1644,21 @@ int oracle_cost(const vib_window* win, double* cost) {
 }
 
 // Full native LM solve; writes the final state into the window's in/out arrays.
+// Two-frame solve; nis = {old before, old after, new before, new after}. Writes the poses and
+// velocities of the two newest slots back into win on success.
+int oracle_solve_latest_two(vib_window* win, const vib_solve_options* opt, int gate, double* nis, int* support,
+                            vib_solve_report* rep) {
+    Window w = load(*win);
+    int sup = 0;
+    const int st = latestTwo(w, *opt, gate, nis, sup, *rep);
+    if (support) *support = sup;
+    if (st == VIB_OK) {
+        std::copy(w.pose.begin(), w.pose.end(), win->pose);
+        std::copy(w.sb.begin(), w.sb.end(), win->speed_bias);
+    }
+    return st;
+}
+
 int oracle_solve(vib_window* win, const vib_solve_options* opt, vib_solve_report* rep) {
     Window w = load(*win);
     State s;

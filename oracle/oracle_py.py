@@ -38,6 +38,8 @@ def load(abi):
     lib.oracle_solve_step.argtypes = [W, C.c_double, dp, dp, dp, dp]
     lib.oracle_cost.argtypes = [W, dp]
     lib.oracle_solve.argtypes = [W, C.POINTER(abi.VibSolveOptions), C.POINTER(abi.VibSolveReport)]
+    lib.oracle_solve_latest_two.argtypes = [W, C.POINTER(abi.VibSolveOptions), C.c_int, dp, ip,
+                                            C.POINTER(abi.VibSolveReport)]
     lib.oracle_marginalize_old.argtypes = [W, ip, dp, dp, dp, dp, ip, lp, dp, dp]
     lib.oracle_preintegrate.argtypes = [dp, C.c_int, C.c_double, C.c_double, dp, C.POINTER(abi.VibImuParams), dp, dp, dp]
     lib.oracle_pinhole_observation.argtypes = [C.c_double] * 7 + [dp, dp]
@@ -119,6 +121,17 @@ class Oracle:
         rep = self.abi.VibSolveReport()
         st = self.lib.oracle_solve(C.byref(s), C.byref(opts), C.byref(rep))
         return st, rep
+
+    def solve_latest_two(self, win, opts, gate=False):
+        """Two-frame solve in place (poses and velocities of the two newest slots).
+        Returns status, report, NIS {old before, old after, new before, new after}, support."""
+        s = win.cstruct()
+        rep = self.abi.VibSolveReport()
+        nis = np.zeros(4)
+        sup = C.c_int32()
+        st = self.lib.oracle_solve_latest_two(C.byref(s), C.byref(opts), 1 if gate else 0, _p(nis), C.byref(sup),
+                                              C.byref(rep))
+        return st, rep, nis, sup.value
 
     def marginalize_old(self, win):
         s = win.cstruct()

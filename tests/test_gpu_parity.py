@@ -143,3 +143,47 @@ def test_batch_alone_equals_mixed(ctx, abi):
         assert abs(reps[i].final_cost - c0) <= 1e-10 * c0
         assert np.abs(out.pose - o0.pose).max() < 1e-7
         assert np.abs(out.lm_rho - o0.lm_rho).max() < 1e-7 * max(1.0, np.abs(o0.lm_rho).max())
+
+
+def test_latest_two_matches_oracle(ctx, oracle, abi):
+    """Two-frame lightweight solve on the device vs the oracle restatement of optimizeLatestTwo:
+    same iteration/accept counts, poses and velocities of the two newest frames within 1e-6,
+    IMU NIS before/after within 1e-6 relative, nothing else changed."""
+    from vibpy import synth
+    w = synth.generate(K=8, L=400, n_cams=4, obs_range=(2, 8), seed=31)
+    st, _ = oracle.solve(w, abi.options(max_iterations=12))
+    assert st == 0
+    pert = w.copy()
+    pert.pose[-2:, :3] += 0.03
+    pert.speed_bias[-2:, :3] += 0.05
+    opts = abi.options(max_iterations=5)
+    say("[gpu] latest two")
+    ctx.upload(0, pert)
+    st, rep = ctx.solve_latest_two(0, opts)
+    assert st == 0, ctx.error()
+    got = pert.copy()
+    ctx.read_state(0, got)
+    exp = pert.copy()
+    ost, orep, nis, support = oracle.solve_latest_two(exp, opts)
+    assert ost == 0
+    assert rep.solve.iterations == orep.iterations
+    assert rep.solve.accepted_steps == orep.accepted_steps
+    assert rep.solve.retained_dimension == 18
+    assert rep.support_landmarks == support
+    assert np.abs(got.pose - exp.pose).max() < 1e-6
+    assert np.abs(got.speed_bias - exp.speed_bias).max() < 1e-6
+    assert np.array_equal(got.lm_rho, pert.lm_rho)
+    dev = [rep.old_imu_nis_before, rep.old_imu_nis_after, rep.new_imu_nis_before, rep.new_imu_nis_after]
+    for a, b in zip(dev, nis):
+        assert abs(a - b) <= 1e-6 * max(1.0, abs(b)), (dev, nis)
+
+
+def test_latest_two_rejects_missing_bridge(ctx, abi):
+    from vibpy import synth
+    w = synth.generate(K=6, L=300, seed=32)
+    keep = w.imu_to != w.K - 2
+    w.imu_from, w.imu_to, w.imu_data = w.imu_from[keep], w.imu_to[keep], w.imu_data[keep]
+    say("[gpu] latest two without bridge")
+    ctx.upload(0, w)
+    st, _ = ctx.solve_latest_two(0, abi.options(max_iterations=3))
+    assert st == abi.VIB_ERR_IMU_BRIDGE

@@ -352,3 +352,54 @@ def test_lm_reduces_cost_and_recovers_truth(oracle, abi):
     assert st == 0
     assert rep.final_cost < 1e-2 * rep.initial_cost
     assert np.abs(out.pose[:, :3] - w.meta["true_pose"][:, :3]).max() < 0.02
+
+
+def _latest_two_case(oracle, abi, seed=31):
+    """A window solved jointly (its optimum), then the two newest poses/velocities perturbed."""
+    from vibpy import synth
+    w = synth.generate(K=8, L=400, n_cams=4, obs_range=(2, 8), seed=seed)
+    st, _ = oracle.solve(w, abi.options(max_iterations=12))
+    assert st == 0
+    pert = w.copy()
+    pert.pose[-2:, :3] += 0.03
+    pert.speed_bias[-2:, :3] += 0.05
+    return w, pert
+
+
+def test_latest_two_returns_to_joint_optimum(oracle, abi):
+    """Restricted to the newest two poses and velocities, the solve must return them to the joint
+    optimum (every other block is held at that optimum); nothing else may change."""
+    opt, pert = _latest_two_case(oracle, abi)
+    out = pert.copy()
+    st, rep, nis, support = oracle.solve_latest_two(out, abi.options(max_iterations=5))
+    assert st == 0 and rep.iterations <= 3 and rep.retained_dimension == 18
+    assert support >= 6
+    assert np.abs(out.pose[-2:, :3] - opt.pose[-2:, :3]).max() < 1e-4
+    assert np.abs(out.speed_bias[-2:, :3] - opt.speed_bias[-2:, :3]).max() < 1e-3
+    assert np.array_equal(out.pose[:-2], pert.pose[:-2])
+    assert np.array_equal(out.speed_bias[:, 3:], pert.speed_bias[:, 3:])
+    assert np.array_equal(out.lm_rho, pert.lm_rho)
+    assert nis[1] < nis[0] and nis[3] < nis[2]
+
+
+def test_latest_two_rejections(oracle, abi):
+    """Reference rejection reasons: no IMU edge into K-2, too few landmarks seen in K-1, and the
+    NIS gate of the new edge."""
+    from vibpy import synth
+    opt, pert = _latest_two_case(oracle, abi)
+    w = pert.copy()
+    keep = w.imu_to != w.K - 2
+    w.imu_from, w.imu_to, w.imu_data = w.imu_from[keep], w.imu_to[keep], w.imu_data[keep]
+    st, _, _, _ = oracle.solve_latest_two(w, abi.options(max_iterations=3))
+    assert st == abi.VIB_ERR_IMU_BRIDGE
+    w = pert.copy()
+    latest = w.obs_frame == w.K - 1
+    w.obs_frame = np.where(latest, 0, w.obs_frame).astype(w.obs_frame.dtype)
+    st, _, _, support = oracle.solve_latest_two(w, abi.options(max_iterations=3))
+    assert st == abi.VIB_ERR_LIGHTWEIGHT_SUPPORT and support == 0
+    w = pert.copy()
+    w.speed_bias[-1, :3] += 50.0  # a velocity far off makes the new edge's NIS explode
+    st, _, nis, _ = oracle.solve_latest_two(w, abi.options(max_iterations=1), gate=True)
+    assert st in (abi.VIB_ERR_IMU_GATE, 0)
+    if st == abi.VIB_ERR_IMU_GATE:
+        assert nis[3] > 60
