@@ -30,6 +30,7 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        int max_len, int max_edges, int refinement, int mode, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t vibk_marg_cholesky(vib::WinDev* wins, int nwin, hipStream_t s);
 }
 
 namespace vib {
@@ -244,7 +245,7 @@ size_t Slot::footprint(const HostWindow& h) {
     d += 6 * h.F + LO_PLANES * h.L + h.L;                // linearization outputs
     d += (size_t)VIB_IMU_STRIDE * h.E;
     d += m * n + m + n * n + h.prior_ref.size();
-    d += 2 * nr * nr + factor_doubles((int)nr) + 8 * nr;
+    d += 2 * nr * nr + factor_doubles((int)nr + kTile) + 8 * nr + 64;
     d += 465 * (size_t)h.E + 36 * B + n + 4 * (size_t)h.E + 4;
     d += 2 * nch * (tri + nv) + nch * CS_PLANES + 2 * (tri + nv);
     size_t i = 3 * h.L + 1 + h.F + 2 * h.E + h.pfree.size() + 4 * nr;
@@ -303,8 +304,8 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     int32_t* colmap = a.take<int32_t>(4 * nr);
     d.colmap = colmap;
     d.Hfull = a.take<double>(nr * nr); d.gfull = a.take<double>(nr);
-    d.fac = a.take<double>(factor_doubles((int)nr)); d.Scopy = a.take<double>(nr * nr);
-    d.rhs = a.take<double>(nr); d.dstate = a.take<double>(nr); d.scratch = a.take<double>(4 * nr);
+    d.fac = a.take<double>(factor_doubles((int)nr + kTile)); d.Scopy = a.take<double>(nr * nr);
+    d.rhs = a.take<double>(nr); d.dstate = a.take<double>(nr); d.scratch = a.take<double>(4 * nr + 64);
     const size_t len = d.tri + h.nv, nch = h.chunks.size();
     d.lin_part = a.take<double>(nch * len); d.schur_part = a.take<double>(nch * len);
     d.chunk_sc = a.take<double>(nch * CS_PLANES);
@@ -643,6 +644,298 @@ bool imu_nis(vib_context* ctx, Slot& s, const int edges[2], double nis[2]) {
         nis[i] = v;
     }
     return true;
+}
+
+// OLD marginalization (Marginalizer::compute, Marginalizer.cpp:284-474, with the elimination set
+// of SlidingWindowManager.cpp:627-652): slot 0's pose and speed/bias and every landmark anchored
+// in or observed by slot 0 are eliminated from the factors they touch (all blocks of those
+// factors as variables). On the device: linearization of the consumed factors and the landmark
+// Schur complement through the diagonal pseudo-inverse. On the host: the 15 x 15 state block's
+// eigen pseudo-inverse and the rank-15 update to H_prior, b_prior. On the device again: a
+// semidefinite MFMA tile Cholesky H_prior = L L^T with r0 = L^-1 b_prior, so A = L^T gives
+// A^T A = H_prior and A^T r0 = b_prior (dropped pivots drop rows). The window then slides.
+int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
+    HostWindow cur;
+    if (!pull_state(ctx, s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    const int K = cur.K, victim = 0;
+    if (K < 2) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "marginalization needs at least two frames");
+    std::vector<char> elim(cur.L, 0), tpose(K, 0), tsb(K, 0);
+    bool rig_used = false, td_used = false;
+    for (int l = 0; l < cur.L; ++l) {
+        bool e = cur.lm_frame[l] == victim;
+        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) e = e || (cur.ob_fc[f] >> 8) == victim;
+        if (!e) continue;
+        elim[l] = 1;
+        tpose[cur.lm_frame[l]] = 1;
+        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) tpose[cur.ob_fc[f] >> 8] = 1;
+        if (cur.lm_off[l + 1] > cur.lm_off[l]) rig_used = td_used = true;
+    }
+    std::vector<int> imu_used;
+    for (int e = 0; e < cur.E; ++e)
+        if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) {
+            imu_used.push_back(e);
+            tpose[cur.imu_from[e]] = tpose[cur.imu_to[e]] = 1;
+            tsb[cur.imu_from[e]] = tsb[cur.imu_to[e]] = 1;
+        }
+    for (size_t b = 0; b < cur.prior_kind.size(); ++b) {
+        int slot = -1;
+        for (int k = 0; k < K; ++k) if (cur.frame_id[k] == cur.prior_id[b]) slot = k;
+        switch (cur.prior_kind[b]) {
+            case VIB_KEY_POSE: if (slot >= 0) tpose[slot] = 1; break;
+            case VIB_KEY_SPEED_BIAS: if (slot >= 0) tsb[slot] = 1; break;
+            case VIB_KEY_RIG: rig_used = true; break;
+            default: td_used = true;
+        }
+    }
+    // the marginal system: consumed factors only, every touched block a variable
+    HostWindow m;
+    m.K = K; m.C = cur.C;
+    m.flags = (rig_used ? VIB_RIG_FREE : 0u) | (td_used ? VIB_TD_FREE : 0u);
+    m.huber = cur.huber; m.min_rho = cur.min_rho; m.td_lo = -1e300; m.td_hi = 1e300;
+    m.frame_id = cur.frame_id; m.pose = cur.pose; m.sb = cur.sb;
+    std::copy_n(cur.rig, 7, m.rig);
+    m.td = cur.td; m.cam = cur.cam;
+    m.pose_free.assign(tpose.begin(), tpose.end());
+    m.sb_width.assign(K, 0);
+    for (int k = 0; k < K; ++k) m.sb_width[k] = tsb[k] ? 9 : 0;
+    std::vector<int> keep_lm;
+    for (int l = 0; l < cur.L; ++l) if (elim[l]) keep_lm.push_back(l);
+    m.L = (int)keep_lm.size();
+    m.lm_off.assign(1, 0);
+    m.lm_frame.resize(m.L); m.lm_cam.resize(m.L); m.lm_free.assign(m.L, 1); m.rho.resize(m.L);
+    m.lmd.assign((size_t)LM_PLANES * m.L, 0);
+    std::vector<int> keep_f;
+    for (int i = 0; i < m.L; ++i) {
+        const int l = keep_lm[i];
+        m.lm_frame[i] = cur.lm_frame[l]; m.lm_cam[i] = cur.lm_cam[l]; m.rho[i] = cur.rho[l];
+        for (int p = 0; p < LM_PLANES; ++p) m.lmd[(size_t)p * m.L + i] = cur.lmd[(size_t)p * cur.L + l];
+        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) keep_f.push_back(f);
+        m.lm_off.push_back((int)keep_f.size());
+    }
+    m.F = (int)keep_f.size();
+    m.ob_fc.resize(m.F); m.ob_lm.resize(m.F);
+    m.ob.assign((size_t)OB_PLANES * m.F, 0);
+    for (int i = 0, li = 0; i < m.F; ++i) {
+        while (i >= m.lm_off[li + 1]) ++li;
+        const int f = keep_f[i];
+        m.ob_fc[i] = cur.ob_fc[f];
+        m.ob_lm[i] = li;
+        for (int p = 0; p < OB_PLANES; ++p) m.ob[(size_t)p * m.F + i] = cur.ob[(size_t)p * cur.F + f];
+    }
+    for (int e : imu_used) {
+        m.imu_from.push_back(cur.imu_from[e]);
+        m.imu_to.push_back(cur.imu_to[e]);
+        m.imu.insert(m.imu.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e,
+                     cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+    }
+    m.E = (int)imu_used.size();
+    m.prior_rows = cur.prior_rows; m.prior_cols = cur.prior_cols;
+    m.prior_kind = cur.prior_kind; m.prior_id = cur.prior_id;
+    m.prior_ref = cur.prior_ref; m.prior_A = cur.prior_A; m.prior_r0 = cur.prior_r0;
+    std::string err;
+    if (!m.layout(err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
+    const int nr = m.nr;
+    // eliminated state columns (slot 0 pose and speed/bias) and retained columns in layout order
+    std::vector<int> victim_cols, kept_cols;
+    std::vector<char> is_victim(nr, 0);
+    if (m.pose_col[victim] >= 0) for (int i = 0; i < 6; ++i) victim_cols.push_back(m.pose_col[victim] + i);
+    if (m.sb_col[victim] >= 0) for (int i = 0; i < 9; ++i) victim_cols.push_back(m.sb_col[victim] + i);
+    for (int c : victim_cols) is_victim[c] = 1;
+    for (int c = 0; c < nr; ++c) if (!is_victim[c]) kept_cols.push_back(c);
+    const int nvict = (int)victim_cols.size(), np = (int)kept_cols.size();
+    if (nvict == 0) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "no eliminated state block");
+    // device: linearize the consumed factors, landmark pseudo-inverse Schur, reduced system
+    Slot tmp;
+    if (!tmp.upload(m, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    LmState lm{};
+    lm.lambda = 0; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1; lm.evaluate_only = 1;
+    lm.marginal = 1;
+    std::vector<Slot*> one{&tmp};
+    Batch b;
+    if (!write_lm(ctx, tmp, lm) || !b.build(one, ctx->stream, err) || !b.linearize(ctx->stream))
+        return fail(ctx, VIB_ERR_DEVICE, "marginal linearization failed");
+    std::vector<double> dvals(m.L);
+    if ((m.L && hipMemcpyAsync(dvals.data(), tmp.dev.lm_out + (size_t)LO_D * m.L, 8 * (size_t)m.L, hipMemcpyDeviceToHost,
+                               ctx->stream) != hipSuccess) ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
+    double dmax = 0;
+    for (double d : dvals) dmax = std::max(dmax, d);
+    lm.marg_cut = std::max(1e-12, 1e-9 * dmax);
+    std::vector<double> S((size_t)nr * nr), rhs(nr);
+    if (!write_lm(ctx, tmp, lm) || !b.launch(3, ctx->stream, 0, 0) || !b.launch(4, ctx->stream, 0, 0) ||
+        !b.launch(5, ctx->stream, 0, 0) ||
+        hipMemcpyAsync(S.data(), tmp.dev.Scopy, 8 * S.size(), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(rhs.data(), tmp.dev.rhs, 8 * (size_t)nr, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(ctx, VIB_ERR_DEVICE, "marginal Schur failed");
+    {
+        LmState chk;
+        if (!read_lm(ctx, tmp, chk)) return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
+        if (!chk.lin_valid) return fail(ctx, VIB_ERR_INVALID_RESIDUAL, "invalid residual in a consumed factor");
+    }
+    // eliminated state block: eigen pseudo-inverse with cutoff max(1e-12, 1e-9 lambda_max)
+    // (Marginalizer::psdSolve, Marginalizer.cpp:240-283), H_prior = S_rr - S_rm S_mm^+ S_mr
+    std::vector<double> Smm((size_t)nvict * nvict), ev, U;
+    for (int i = 0; i < nvict; ++i)
+        for (int j = 0; j < nvict; ++j)
+            Smm[(size_t)i * nvict + j] = 0.5 * (S[(size_t)victim_cols[i] * nr + victim_cols[j]] +
+                                                 S[(size_t)victim_cols[j] * nr + victim_cols[i]]);
+    symmetric_eigen(Smm, nvict, ev, U);
+    const double cut_v = std::max(1e-12, 1e-9 * std::max(0.0, ev[nvict - 1]));
+    if (ev[0] < -10.0 * cut_v) return fail(ctx, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
+    std::vector<double> Z, zb;  // U_k^T S_mr / sqrt(lambda_k), U_k^T b_m / sqrt(lambda_k)
+    for (int q = 0; q < nvict; ++q) {
+        if (!(ev[q] > cut_v)) continue;
+        const double is = 1.0 / std::sqrt(ev[q]);
+        double bz = 0;
+        for (int i = 0; i < nvict; ++i) bz += U[(size_t)i * nvict + q] * -rhs[victim_cols[i]];
+        zb.push_back(bz * is);
+        for (int c = 0; c < np; ++c) {
+            double v = 0;
+            for (int i = 0; i < nvict; ++i) v += U[(size_t)i * nvict + q] * S[(size_t)victim_cols[i] * nr + kept_cols[c]];
+            Z.push_back(v * is);
+        }
+    }
+    const int nk = (int)zb.size();
+    std::vector<double> Hp((size_t)np * np), bp(np);
+    for (int i = 0; i < np; ++i) {
+        double g = -rhs[kept_cols[i]];
+        for (int q = 0; q < nk; ++q) g -= Z[(size_t)q * np + i] * zb[q];
+        bp[i] = g;
+        for (int j = 0; j <= i; ++j) {
+            double h = 0.5 * (S[(size_t)kept_cols[i] * nr + kept_cols[j]] + S[(size_t)kept_cols[j] * nr + kept_cols[i]]);
+            for (int q = 0; q < nk; ++q) h -= Z[(size_t)q * np + i] * Z[(size_t)q * np + j];
+            Hp[(size_t)i * np + j] = Hp[(size_t)j * np + i] = h;
+        }
+    }
+    // largest eigenvalue of H_prior (power iteration) for the relative pivot cutoff
+    double lmax = 0;
+    {
+        std::vector<double> v(np, 1.0), u(np);
+        for (int it = 0; it < 60; ++it) {
+            double nrm = 0;
+            for (int i = 0; i < np; ++i) {
+                double acc = 0;
+                for (int j = 0; j < np; ++j) acc += Hp[(size_t)i * np + j] * v[j];
+                u[i] = acc;
+                nrm += acc * acc;
+            }
+            nrm = std::sqrt(nrm);
+            if (!(nrm > 0)) break;
+            double ray = 0;
+            for (int i = 0; i < np; ++i) { ray += v[i] * u[i]; v[i] = u[i] / nrm; }
+            lmax = ray;
+        }
+    }
+    // square root of H_prior: semidefinite MFMA tile Cholesky, r0 = L^-1 b_prior
+    const int T2 = (np + kTile - 1) / kTile, N2 = kTile * T2;
+    lm.marg_n = np;
+    lm.marg_cut_victim = lm.marg_cut_prior = std::max(1e-12, 1e-9 * std::max(0.0, lmax));
+    const size_t tiles = (size_t)T2 * (T2 + 1) / 2;
+    std::vector<double> fac(tiles * kTile * kTile), y(N2);
+    for (int ti = 0; ti < T2; ++ti)
+        for (int tj = 0; tj <= ti; ++tj)
+            for (int e = 0; e < kTile * kTile; ++e) {
+                const int r = kTile * ti + 4 * (e / 64) + (e % 64) / 16, c = kTile * tj + e % 16;
+                fac[tile_base(ti, tj) + e] = (r < np && c < np) ? Hp[(size_t)r * np + c] : (r == c ? 1.0 : 0.0);
+            }
+    if (hipMemcpyAsync(tmp.dev.fac, fac.data(), 8 * fac.size(), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(tmp.dev.scratch, bp.data(), 8 * bp.size(), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        !write_lm(ctx, tmp, lm) || vibk_marg_cholesky(b.d_wins, 1, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(fac.data(), tmp.dev.fac, 8 * fac.size(), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(y.data(), tmp.dev.scratch + N2, 8 * (size_t)N2, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        !read_lm(ctx, tmp, lm))
+        return fail(ctx, VIB_ERR_DEVICE, "marginal factorization failed");
+    if (!lm.chol_ok) return fail(ctx, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
+    auto Lat = [&](int r, int c) {  // factor element (r >= c)
+        return fac[tile_base(r / kTile, c / kTile) + 64 * ((r % kTile) / 4) + 16 * (r % 4) + c % kTile];
+    };
+    std::vector<int> rows;
+    for (int q = 0; q < np; ++q) if (Lat(q, q) > 0) rows.push_back(q);
+    const int rank = (int)rows.size();
+    std::vector<double> A((size_t)rank * np, 0.0), r0(rank);
+    for (int i = 0; i < rank; ++i) {
+        const int q = rows[i];
+        for (int a = q; a < np; ++a) A[(size_t)i * np + a] = Lat(a, q);
+        r0[i] = y[q];
+    }
+    // new prior keys in column order: poses, rig, td, speed/bias (the eliminated slot excluded)
+    HostWindow next;
+    next.prior_rows = rank; next.prior_cols = np;
+    next.prior_A = A; next.prior_r0 = r0;
+    for (int k = 0; k < K; ++k)
+        if (k != victim && m.pose_col[k] >= 0) {
+            next.prior_kind.push_back(VIB_KEY_POSE); next.prior_id.push_back(cur.frame_id[k]);
+            next.prior_ref.insert(next.prior_ref.end(), cur.pose.begin() + 7 * k, cur.pose.begin() + 7 * k + 7);
+        }
+    if (m.rig_col >= 0) {
+        next.prior_kind.push_back(VIB_KEY_RIG); next.prior_id.push_back(0);
+        next.prior_ref.insert(next.prior_ref.end(), cur.rig, cur.rig + 7);
+    }
+    if (m.td_col >= 0) {
+        next.prior_kind.push_back(VIB_KEY_TD); next.prior_id.push_back(0);
+        next.prior_ref.push_back(cur.td);
+    }
+    for (int k = 0; k < K; ++k)
+        if (k != victim && m.sb_col[k] >= 0) {
+            next.prior_kind.push_back(VIB_KEY_SPEED_BIAS); next.prior_id.push_back(cur.frame_id[k]);
+            next.prior_ref.insert(next.prior_ref.end(), cur.sb.begin() + 9 * k, cur.sb.begin() + 9 * k + 9);
+        }
+    // slide: drop slot 0, its landmarks, factors and IMU edges
+    next.K = K - 1; next.C = cur.C; next.flags = cur.flags;
+    next.huber = cur.huber; next.min_rho = cur.min_rho; next.td_lo = cur.td_lo; next.td_hi = cur.td_hi;
+    next.frame_id.assign(cur.frame_id.begin() + 1, cur.frame_id.end());
+    next.pose.assign(cur.pose.begin() + 7, cur.pose.end());
+    next.sb.assign(cur.sb.begin() + 9, cur.sb.end());
+    next.pose_free.assign(cur.pose_free.begin() + 1, cur.pose_free.end());
+    next.sb_width.assign(cur.sb_width.begin() + 1, cur.sb_width.end());
+    std::copy_n(cur.rig, 7, next.rig);
+    next.td = cur.td; next.cam = cur.cam;
+    std::vector<int> live;
+    for (int l = 0; l < cur.L; ++l) if (!elim[l]) live.push_back(l);
+    next.L = (int)live.size();
+    next.lm_off.assign(1, 0);
+    next.lm_frame.resize(next.L); next.lm_cam.resize(next.L); next.lm_free.resize(next.L); next.rho.resize(next.L);
+    next.lmd.assign((size_t)LM_PLANES * next.L, 0);
+    std::vector<int> live_f;
+    for (int i = 0; i < next.L; ++i) {
+        const int l = live[i];
+        next.lm_frame[i] = cur.lm_frame[l] - 1; next.lm_cam[i] = cur.lm_cam[l];
+        next.lm_free[i] = cur.lm_free[l]; next.rho[i] = cur.rho[l];
+        for (int p = 0; p < LM_PLANES; ++p) next.lmd[(size_t)p * next.L + i] = cur.lmd[(size_t)p * cur.L + l];
+        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) live_f.push_back(f);
+        next.lm_off.push_back((int)live_f.size());
+    }
+    next.F = (int)live_f.size();
+    next.ob_fc.resize(next.F); next.ob_lm.resize(next.F);
+    next.ob.assign((size_t)OB_PLANES * next.F, 0);
+    for (int i = 0, li = 0; i < next.F; ++i) {
+        while (i >= next.lm_off[li + 1]) ++li;
+        const int f = live_f[i];
+        next.ob_fc[i] = cur.ob_fc[f] - (1 << 8);
+        next.ob_lm[i] = li;
+        for (int p = 0; p < OB_PLANES; ++p) next.ob[(size_t)p * next.F + i] = cur.ob[(size_t)p * cur.F + f];
+    }
+    for (int e = 0; e < cur.E; ++e) {
+        if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) continue;
+        next.imu_from.push_back(cur.imu_from[e] - 1);
+        next.imu_to.push_back(cur.imu_to[e] - 1);
+        next.imu.insert(next.imu.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e,
+                        cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+    }
+    next.E = (int)next.imu_from.size();
+    if (!next.layout(err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
+    if (!s.upload(next, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    rep->status = VIB_OK;
+    rep->prior_reused = 0;
+    rep->eliminated_dimension = nvict + m.L;
+    rep->retained_dimension = np;
+    rep->prior_rank = rank;
+    rep->eliminated_landmarks = m.L;
+    rep->consumed_factors = m.F + m.E + (cur.prior_rows ? 1 : 0);
+    rep->eigen_cutoff = lm.marg_cut_prior;
+    return VIB_OK;
 }
 
 }  // namespace
@@ -1024,8 +1317,19 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
 
 int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double* samples, int32_t count,
                     const vib_imu_params* params, vib_marginal_report* rep) {
-    (void)slot; (void)policy; (void)samples; (void)count; (void)params; (void)rep;
-    return fail(ctx, VIB_ERR_UNSUPPORTED, "marginalization not available in this build");
+    if (!ctx || !rep) return VIB_ERR_INVALID_ARGUMENT;
+    (void)samples; (void)count; (void)params;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    *rep = vib_marginal_report{};
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    if (policy == VIB_MARGIN_OLD) {
+        const int st = marginalize_old(ctx, *s, rep);
+        rep->status = st;
+        return st;
+    }
+    return fail(ctx, VIB_ERR_UNSUPPORTED, "SECOND_NEW marginalization not available in this build");
 }
 
 int vib_read_prior(vib_context* ctx, int32_t slot, int32_t* rows, int32_t* cols, int32_t* blocks, int32_t* kind,

@@ -13,6 +13,10 @@
 namespace vib {
 
 constexpr int kMaxFactorsPerLandmark = 13;
+
+// Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major, small n); eigenvalues
+// ascending, eigenvectors as columns of V (vib_preint.cpp).
+void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V);
 constexpr int kKernelCount = 11;  // kernels of one LM iteration (vibk_launch ids 0..10)
 
 // Host copy of one window: observation streams sorted by landmark, tangent column layout

@@ -862,7 +862,9 @@ __global__ __launch_bounds__(kBlock) void k_schur(const WinDev* wins, const Chun
             const int a = w.lm_fc[l] >> 8;
             const int f0 = w.lm_off[l], f1 = w.lm_off[l + 1];
             const int m = f1 - f0;
-            inv = 1.0 / damped_point(w.lm_out[LO_D * L + l], lambda);
+            const double dl = w.lm_out[LO_D * L + l];
+            // marginalization: pseudo-inverse of the diagonal landmark block (Marginalizer.cpp:373-409)
+            inv = w.lm->marginal ? (dl > w.lm->marg_cut ? 1.0 / dl : 0.0) : 1.0 / damped_point(dl, lambda);
             gk = w.lm_out[LO_G * L + l];
             // observer slots 1..m : -u_f at observer frame
             for (int j = lane; j < m && j < kSlots - 3; j += kChunkLanes) {
@@ -1123,16 +1125,24 @@ __device__ __forceinline__ void factor_update(double (&v)[kTile]) {
 }
 
 // Columns J.. of the factor (right-looking): lane r of a 16-lane row holds row r of the tile.
-template <int J>
-__device__ __forceinline__ void factor_columns(double (&v)[kTile], int r, bool& ok) {
+// PSD: pivots at or below `cut` are dropped (their column of L is zero, the pseudo-inverse of a
+// semidefinite block), pivots below -10 cut fail; otherwise every pivot must be positive.
+template <int J, bool PSD>
+__device__ __forceinline__ void factor_columns(double (&v)[kTile], int r, bool& ok, double cut) {
     if constexpr (J < kTile) {
         const double d = row_bcast<J>(v[J]);
-        if (!(d > 0) || !isfinite(d)) ok = false;
-        const double is = rsqrt_nr(d);
+        double is;
+        if constexpr (PSD) {
+            if (!isfinite(d) || d < -10.0 * cut) ok = false;
+            is = d > cut ? rsqrt_nr(d) : 0.0;
+        } else {
+            if (!(d > 0) || !isfinite(d)) ok = false;
+            is = rsqrt_nr(d);
+        }
         v[J] = r == J ? is : (r > J ? v[J] * is : 0.0);
         // rows above J hold zeros in column J; entries right of the diagonal are never read
         factor_update<J, J + 1>(v);
-        factor_columns<J + 1>(v, r, ok);
+        factor_columns<J + 1, PSD>(v, r, ok, cut);
     }
 }
 
@@ -1158,10 +1168,11 @@ __device__ __forceinline__ void inverse_rows(double (&m)[4], const double (&v)[k
 // in the four rows (the diagonal keeps 1 / L[j][j]), the inverse split over them by column
 // (c % 4 = row index of the lane group); L^-1 goes to inv in operand layout. Returns false when
 // a pivot is not positive and finite.
-__device__ bool factor_diag_tile(double (&v)[kTile], double* inv) {
+template <bool PSD = false>
+__device__ bool factor_diag_tile(double (&v)[kTile], double* inv, double cut = 0.0) {
     const int lane = threadIdx.x % 64, r = lane % kTile, g = lane / kTile;
     bool ok = true;
-    factor_columns<0>(v, r, ok);
+    factor_columns<0, PSD>(v, r, ok, cut);
     double m[4];
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) m[cc] = r == 4 * cc + g ? 1.0 : 0.0;
@@ -1232,7 +1243,9 @@ __device__ __forceinline__ void panel_solve(gdouble* Lt, double* panel, const do
 // waves for the whole factorization; the others and the diagonal tiles are read from and
 // written back to the tile store at every update. The two roles run separate step loops with
 // the same barrier sequence, so the resident tiles do not constrain the factor wave's registers.
-__device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, double* y, int* flag) {
+template <bool PSD>
+__device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, double* y, int* flag,
+                                                 double cut_first = 0.0, double cut_rest = 0.0) {
     gdouble* Lt = (gdouble*)Lt_;
     const int t = threadIdx.x, nt = blockDim.x, lane = t % 64, nw = nt / 64, nwt = nw - 1;
     const int wave = __builtin_amdgcn_readfirstlane(t / 64);
@@ -1245,11 +1258,24 @@ __device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, 
     }
     __syncthreads();
     if (wave == 0) {
+        // diagonal tile k: factor and inverse; in the semidefinite mode the factor tile is also
+        // stored (diagonal L_jj, zero for dropped pivots) for the read-out of the square root
+        auto factor = [&](double (&v)[kTile], int k) {
+            if constexpr (!PSD) return factor_diag_tile<false>(v, dinv + k * kTileDoubles);
+            const bool ok = factor_diag_tile<true>(v, dinv + k * kTileDoubles, k == 0 ? cut_first : cut_rest);
+            if (lane < kTile) {
+                const int base = tile_base(k, k) + (lane / 4) * 64 + 16 * (lane % 4);
+#pragma unroll
+                for (int c = 0; c < kTile; ++c)
+                    Lt[base + c] = c < lane ? v[c] : (c == lane && v[c] > 0 ? 1.0 / v[c] : 0.0);
+            }
+            return ok;
+        };
         {
             double v[kTile];
 #pragma unroll
             for (int c = 0; c < kTile; ++c) v[c] = pnl[opnd_pos(lane % kTile, c)];
-            if (!factor_diag_tile(v, dinv) && lane == 0) *flag = 1;
+            if (!factor(v, 0) && lane == 0) *flag = 1;
         }
         __syncthreads();
         for (int k = 0; k < T; ++k) {
@@ -1272,7 +1298,7 @@ __device__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, 
                 double v[kTile];
 #pragma unroll
                 for (int c = 0; c < kTile; ++c) v[c] = next[opnd_pos(lane % kTile, c)];
-                if (!factor_diag_tile(v, dinv + (k + 1) * kTileDoubles) && lane == 0) *flag = 1;
+                if (!factor(v, k + 1) && lane == 0) *flag = 1;
             }
             __syncthreads();
         }
@@ -1489,7 +1515,7 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     if (t == 0) flag = 0;
     for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[i] : 0.0;
     __syncthreads();
-    const bool ok = cholesky_forward(w.fac, T, pnl, dinv, x, &flag);
+    const bool ok = cholesky_forward<false>(w.fac, T, pnl, dinv, x, &flag);
     if (!ok) {
         if (t == 0) w.lm->chol_ok = 0;
         return;
@@ -1554,6 +1580,32 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
         w.lm->max_increment = maxinc;
         w.lm->trial_valid = nonfinite > 0 ? 0 : 1;
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_marg_cholesky: semidefinite factor of the reordered marginal system held in the tile store
+// (eliminated state block first), right-hand side in scratch[0, N), forward solution written to
+// scratch[N, 2N). Dropped pivots leave zero columns (pseudo-inverse, Marginalizer.cpp:240-283).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kCholThreads) void k_marg_cholesky(WinDev* wins) {
+    __shared__ double pnl[2 * kMaxTiles * kTileDoubles];
+    __shared__ double dinv[kMaxTiles * kTileDoubles];
+    __shared__ double x[kMaxTiles * kTile];
+    __shared__ int flag;
+    WinDev& w = wins[blockIdx.x];
+    if (!window_active(w) || !w.lm->marginal) return;
+    const int n = w.lm->marg_n, t = threadIdx.x, nt = blockDim.x;
+    const int T = (n + kTile - 1) / kTile, N = kTile * T;
+    if (t == 0) flag = 0;
+    for (int i = t; i < N; i += nt) x[i] = i < n ? w.scratch[i] : 0.0;
+    __syncthreads();
+    const bool ok = cholesky_forward<true>(w.fac, T, pnl, dinv, x, &flag, w.lm->marg_cut_victim, w.lm->marg_cut_prior);
+    if (!ok) {
+        if (t == 0) w.lm->chol_ok = 0;
+        return;
+    }
+    for (int i = t; i < N; i += nt) w.scratch[N + i] = x[i];
+    if (t == 0) w.lm->chol_ok = 1;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1826,6 +1878,11 @@ __global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ sr
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const size_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError();
+}
+
+hipError_t vibk_marg_cholesky(vib::WinDev* wins, int nwin, hipStream_t s) {
+    hipLaunchKernelGGL(vib::k_marg_cholesky, dim3(nwin), dim3(vib::kCholThreads), 0, s, wins);
     return hipGetLastError();
 }
 

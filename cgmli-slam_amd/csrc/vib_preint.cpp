@@ -70,6 +70,28 @@ Mat3 right_jacobian(const double* phi) {  // Forster et al. RSS 2015 eq. (8)
     return Mat3::eye() + scale(-(1.0 - std::cos(n)) / (n * n), px) + scale((n - std::sin(n)) / (n * n * n), px2);
 }
 
+// 15x15 covariance propagation helper: dP <- F dP F^T + Q
+void propagate(const double* F, double* P, const double* Q) {
+    double T[225], O[225];
+    for (int i = 0; i < 15; ++i)
+        for (int j = 0; j < 15; ++j) {
+            double s = 0;
+            for (int k = 0; k < 15; ++k) s += F[i * 15 + k] * P[k * 15 + j];
+            T[i * 15 + j] = s;
+        }
+    for (int i = 0; i < 15; ++i)
+        for (int j = 0; j < 15; ++j) {
+            double s = 0;
+            for (int k = 0; k < 15; ++k) s += T[i * 15 + k] * F[j * 15 + k];
+            O[i * 15 + j] = s + Q[i * 15 + j];
+        }
+    std::memcpy(P, O, sizeof(O));
+}
+
+}  // namespace
+
+namespace vib {
+
 // Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major); eigenvalues
 // ascending, eigenvectors as columns of V.
 void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V) {
@@ -110,7 +132,7 @@ void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std:
     }
     ev.assign(n, 0);
     for (int i = 0; i < n; ++i) ev[i] = A[(size_t)i * n + i];
-    // selection sort ascending (n = 15)
+    // selection sort ascending
     for (int i = 0; i < n; ++i) {
         int m = i;
         for (int j = i + 1; j < n; ++j) if (ev[j] < ev[m]) m = j;
@@ -121,25 +143,7 @@ void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std:
     }
 }
 
-// 15x15 covariance propagation helper: dP <- F dP F^T + Q
-void propagate(const double* F, double* P, const double* Q) {
-    double T[225], O[225];
-    for (int i = 0; i < 15; ++i)
-        for (int j = 0; j < 15; ++j) {
-            double s = 0;
-            for (int k = 0; k < 15; ++k) s += F[i * 15 + k] * P[k * 15 + j];
-            T[i * 15 + j] = s;
-        }
-    for (int i = 0; i < 15; ++i)
-        for (int j = 0; j < 15; ++j) {
-            double s = 0;
-            for (int k = 0; k < 15; ++k) s += T[i * 15 + k] * F[j * 15 + k];
-            O[i * 15 + j] = s + Q[i * 15 + j];
-        }
-    std::memcpy(P, O, sizeof(O));
-}
-
-}  // namespace
+}  // namespace vib
 
 extern "C" {
 
@@ -248,7 +252,7 @@ int vib_preintegrate(const double* smp, int32_t count, double t0, double t1, con
             for (int c = 0; c < 15; ++c)
                 P[r * 15 + c] += (0.5 * dP[225 * j + r * 15 + c] + 0.5 * dP[225 * j + c * 15 + r]) * sig[j] * sig[j];
     std::vector<double> ev, U;
-    symmetric_eigen(P, 15, ev, U);
+    vib::symmetric_eigen(P, 15, ev, U);
     const double eps = std::numeric_limits<double>::epsilon();
     const double tol = std::max(eps, eps * 15 * ev[14]);
     std::memset(out, 0, sizeof(double) * VIB_IMU_STRIDE);

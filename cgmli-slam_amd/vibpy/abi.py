@@ -400,6 +400,12 @@ class Context:
         self.check(self.lib.vib_retained_dimension(self.h, slot, C.byref(nr), C.byref(nv)), "dims")
         return nr.value, nv.value
 
+    def shape(self, slot):
+        """(frames, landmarks, observations, imu edges, prior rows, prior cols) of a resident window."""
+        v = [C.c_int32() for _ in range(6)]
+        self.check(self.lib.vib_window_shape(self.h, slot, *[C.byref(x) for x in v]), "vib_window_shape")
+        return tuple(x.value for x in v)
+
     def read_step(self, slot, n):
         out = np.zeros(n)
         self.check(self.lib.vib_read_step(self.h, slot, f64p(out), n), "vib_read_step")

@@ -1023,6 +1023,8 @@ struct MargResult {
     std::vector<int64_t> keyId;
     std::vector<double> ref, A, r0;
     std::vector<double> Hprior, bprior;  // retained Schur system before the eigen cut
+    std::vector<double> Hfull, bfull;    // whole system of the consumed factors (n x n, n)
+    std::vector<int> elimMask;           // 1 for eliminated columns of the whole system
     int cols = 0;
 };
 
@@ -1160,6 +1162,9 @@ MargResult marginalizeOld(const Window& w) {
     }
     const int nm = (int)m.size(), nrr = (int)rr.size();
     out.nm = nm; out.nr = nrr; out.landmarks = (int)lmKeys.size();
+    out.Hfull = H; out.bfull = b;
+    out.elimMask.assign(n, 0);
+    for (int i : m) out.elimMask[i] = 1;
     std::vector<double> Hmm((size_t)nm * nm), Hmr((size_t)nm * nrr), Hrr((size_t)nrr * nrr), bm(nm), br(nrr);
     for (int i = 0; i < nm; ++i) {
         bm[i] = b[m[i]];
@@ -1684,6 +1689,20 @@ int oracle_marginalize_old(const vib_window* win, int* sizes, double* H, double*
     if (keyId) std::copy(m.keyId.begin(), m.keyId.end(), keyId);
     if (ref) std::copy(m.ref.begin(), m.ref.end(), ref);
     if (cutoff) *cutoff = m.cutoff;
+    return VIB_OK;
+}
+
+// The whole linear system of the consumed factors before any elimination (size in *n; H, b,
+// mask may be null for a size query); the retained columns of mask == 0 are in the order of the
+// prior keys.
+int oracle_marginal_system(const vib_window* win, int* n, double* H, double* b, int* mask) {
+    Window w = load(*win);
+    MargResult m = marginalizeOld(w);
+    if (m.status != VIB_OK) return m.status;
+    *n = (int)m.bfull.size();
+    if (H) std::copy(m.Hfull.begin(), m.Hfull.end(), H);
+    if (b) std::copy(m.bfull.begin(), m.bfull.end(), b);
+    if (mask) std::copy(m.elimMask.begin(), m.elimMask.end(), mask);
     return VIB_OK;
 }
 

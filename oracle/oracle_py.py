@@ -41,6 +41,7 @@ def load(abi):
     lib.oracle_solve_latest_two.argtypes = [W, C.POINTER(abi.VibSolveOptions), C.c_int, dp, ip,
                                             C.POINTER(abi.VibSolveReport)]
     lib.oracle_marginalize_old.argtypes = [W, ip, dp, dp, dp, dp, ip, lp, dp, dp]
+    lib.oracle_marginal_system.argtypes = [W, ip, dp, dp, ip]
     lib.oracle_preintegrate.argtypes = [dp, C.c_int, C.c_double, C.c_double, dp, C.POINTER(abi.VibImuParams), dp, dp, dp]
     lib.oracle_pinhole_observation.argtypes = [C.c_double] * 7 + [dp, dp]
     lib.oracle_eigen_sym.argtypes = [dp, C.c_int, dp, dp]
@@ -155,6 +156,19 @@ class Oracle:
                                              C.byref(cut))
         return st, dict(H=H, b=b, A=A, r0=r0, kind=kind, id=kid, ref=ref, cutoff=cut.value,
                         landmarks=int(sizes[4]))
+
+    def marginal_system(self, win):
+        """Whole system (H, b) of the factors an OLD marginalization consumes, eliminated mask."""
+        s = win.cstruct()
+        n = C.c_int32()
+        st = self.lib.oracle_marginal_system(C.byref(s), C.byref(n), None, None, None)
+        if st != 0:
+            return st, None
+        H = np.zeros((n.value, n.value))
+        b = np.zeros(n.value)
+        mask = np.zeros(n.value, np.int32)
+        st = self.lib.oracle_marginal_system(C.byref(s), C.byref(n), _p(H), _p(b), mask.ctypes.data_as(C.POINTER(C.c_int32)))
+        return st, dict(H=H, b=b, mask=mask.astype(bool))
 
     def preintegrate(self, samples, t0, t1, sb, params, g0, g1):
         out = np.zeros(self.abi.IMU_STRIDE)

@@ -187,3 +187,46 @@ def test_latest_two_rejects_missing_bridge(ctx, abi):
     ctx.upload(0, w)
     st, _ = ctx.solve_latest_two(0, abi.options(max_iterations=3))
     assert st == abi.VIB_ERR_IMU_BRIDGE
+
+
+def _prior_columns(kind, ids):
+    widths = {0: 6, 1: 9, 2: 6, 3: 1}
+    out, c = {}, 0
+    for k, i in zip(kind, ids):
+        key = (int(k), int(i) if k in (0, 1) else 0)
+        out[key] = (c, widths[int(k)])
+        c += widths[int(k)]
+    return out, c
+
+
+def test_marginalize_old_matches_oracle(ctx, oracle, abi):
+    """OLD marginalization on the device (landmark pseudo-inverse Schur, semidefinite MFMA tile
+    Cholesky of the reordered system) vs the oracle's eigen-based prior: A^T A and A^T r0 agree
+    within 1e-6 relative on the oracle's kept subspace (the metric's prior bound); the window
+    slides by one frame and solves afterwards."""
+    from vibpy import synth
+    w = synth.generate(K=6, L=200, n_cams=4, obs_range=(2, 6), seed=41, rig_free=True, td_free=True)
+    st, pr = oracle.marginalize_old(w)
+    assert st == 0
+    say("[gpu] marginalize OLD")
+    ctx.upload(0, w)
+    st, rep = ctx.marginalize(0, abi.VIB_MARGIN_OLD)
+    assert st == 0, ctx.error()
+    dev = ctx.read_prior(0)
+    co, no = _prior_columns(pr["kind"], pr["id"])
+    cd, nd = _prior_columns(dev["kind"], dev["id"])
+    assert set(co) == set(cd) and no == nd
+    perm = np.concatenate([np.arange(cd[k][0], cd[k][0] + cd[k][1]) for k in co])
+    Ad = dev["A"][:, perm]
+    Gd, hd = Ad.T @ Ad, Ad.T @ dev["r0"]
+    Ao = pr["A"]
+    Go, ho = Ao.T @ Ao, Ao.T @ pr["r0"]
+    P = np.linalg.pinv(Ao) @ Ao
+    assert np.linalg.norm(P @ Gd @ P - Go) <= 1e-6 * np.linalg.norm(Go)
+    assert np.linalg.norm(P @ hd - ho) <= 1e-6 * np.linalg.norm(ho)
+    assert rep.eliminated_landmarks == pr["landmarks"]
+    K, L, F, E, m, n = ctx.shape(0)
+    assert K == w.K - 1 and E == w.E - 1 and L == w.L - pr["landmarks"] and n == no
+    say("[gpu] solve after marginalization")
+    st, srep = ctx.solve(0, abi.options(max_iterations=3))
+    assert st == 0, ctx.error()

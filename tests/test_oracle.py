@@ -403,3 +403,46 @@ def test_latest_two_rejections(oracle, abi):
     assert st in (abi.VIB_ERR_IMU_GATE, 0)
     if st == abi.VIB_ERR_IMU_GATE:
         assert nis[3] > 60
+
+
+def test_marginalize_old_is_the_schur_complement(oracle, abi):
+    """OLD marginalization: the prior's quadratic (A^T A, A^T r0) equals the Schur complement of
+    the eliminated block (victim pose/speed-bias and its landmarks) in the whole system of the
+    consumed factors, computed here directly with numpy; the eigen cut drops nothing here."""
+    from vibpy import synth
+    w = synth.generate(K=6, L=200, n_cams=4, obs_range=(2, 6), seed=41, rig_free=True, td_free=True)
+    st, sysm = oracle.marginal_system(w)
+    assert st == 0
+    st, pr = oracle.marginalize_old(w)
+    assert st == 0
+    H, b, e = sysm["H"], sysm["b"], sysm["mask"]
+    r = ~e
+    Hmm, Hmr, Hrr = H[np.ix_(e, e)], H[np.ix_(e, r)], H[np.ix_(r, r)]
+    X = np.linalg.solve(Hmm, np.column_stack([Hmr, b[e]]))
+    S = Hrr - Hmr.T @ X[:, :-1]
+    g = b[r] - Hmr.T @ X[:, -1]
+    A, r0 = pr["A"], pr["r0"]
+    # eigen cut max(1e-12, 1e-9 lambda_max) (Marginalizer.cpp:441-444): the rank and the kept
+    # subspace follow numpy's eigendecomposition of the Schur complement
+    lam, U = np.linalg.eigh(0.5 * (S + S.T))
+    cut = max(1e-12, 1e-9 * lam[-1])
+    assert abs(pr["cutoff"] - cut) <= 1e-6 * cut
+    keep = lam > cut
+    assert A.shape == (keep.sum(), S.shape[0])
+    P = U[:, keep] @ U[:, keep].T
+    assert np.linalg.norm(A.T @ A - P @ S @ P) <= 1e-8 * np.linalg.norm(S)
+    assert np.linalg.norm(A.T @ r0 - P @ g) <= 1e-8 * np.linalg.norm(g)
+    # linear-quadratic agreement: min over the eliminated block of the whole quadratic
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        dr = rng.normal(size=r.sum()) * 1e-2
+        dm = -np.linalg.solve(Hmm, Hmr @ dr + b[e])
+        d = np.zeros(len(b)); d[e] = dm; d[r] = dr
+        full = 0.5 * d @ H @ d + b @ d
+        dr = P @ dr  # kept subspace
+        dm = -np.linalg.solve(Hmm, Hmr @ dr + b[e])
+        d = np.zeros(len(b)); d[e] = dm; d[r] = dr
+        full = 0.5 * d @ H @ d + b @ d
+        base = -0.5 * b[e] @ np.linalg.solve(Hmm, b[e])
+        prior = 0.5 * np.sum((r0 + A @ dr) ** 2) - 0.5 * np.sum(r0 ** 2) + base
+        assert abs(full - prior) <= 1e-7 * max(1.0, abs(full))
