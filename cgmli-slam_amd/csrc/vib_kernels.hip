@@ -16,6 +16,8 @@
 // src/backend/BackendOptimizer.cpp:167-307, with no atomics to global memory.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "vib_device.hpp"
 #include "vib_math.hpp"
 
@@ -27,6 +29,8 @@ namespace vib {
 __device__ __forceinline__ int tri_index(int i, int j, int nv) {  // i <= j, packed upper rows
     return i * nv - (i * (i - 1)) / 2 + (j - i);
 }
+
+typedef double d4 __attribute__((ext_vector_type(4)));  // v_mfma_f64_16x16x4f64 accumulator
 
 // Deterministic block reductions (fixed tree order).
 __device__ double block_sum(double v, double* red) {
@@ -322,22 +326,23 @@ constexpr int kAccSize = kMaxVisual * (kMaxVisual + 1) / 2 + kMaxVisual;
 // ---------------------------------------------------------------------------------------------
 // k_lin_visual: one chunk of landmarks per workgroup, 8 lanes per landmark.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* chunks) {
+template <int NT>
+__global__ __launch_bounds__(NT) void k_lin_visual(const WinDev* wins, const Chunk* chunks) {
     __shared__ double acc[kAccSize];
     __shared__ FrameTable ft;
-    __shared__ double red[kBlock];
+    __shared__ double red[NT];
     const Chunk ch = chunks[blockIdx.x];
     const WinDev& w = wins[ch.win];
     if (!window_active(w) || !w.lm->new_point) return;
     const int nv = w.nv, tri = w.tri;
-    for (int i = threadIdx.x; i < tri + nv; i += kBlock) acc[i] = 0;
+    for (int i = threadIdx.x; i < tri + nv; i += NT) acc[i] = 0;
     build_frame_table(w, w.pose, w.rig, ft);
     __syncthreads();
     double* gacc = acc + tri;
     const double td = *w.td;
     const bool want_td = w.td_col >= 0;
     const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
-    constexpr int groups = kBlock / kChunkLanes;
+    constexpr int groups = NT / kChunkLanes;
     double cost = 0;
     int invalid = 0;
     const size_t L = (size_t)w.L, F = (size_t)w.F;
@@ -416,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     }
     __syncthreads();
     double* part = w.lin_part + (size_t)ch.local * (tri + nv);
-    for (int i = threadIdx.x; i < tri + nv; i += kBlock) part[i] = acc[i];
+    for (int i = threadIdx.x; i < tri + nv; i += NT) part[i] = acc[i];
     const double c = block_sum(cost, red);
     const double inv = block_max(invalid ? 1.0 : 0.0, red);
     if (threadIdx.x == 0) {
@@ -814,20 +819,33 @@ __global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Fixed-order reduction of chunk partials: out[i] = sum_c part[c][i] (slices of 256 entries).
+// Fixed-order reduction of chunk partials: out[i] = sum_c part[c][i]; 16 entries per workgroup,
+// chunks split over 16 lanes per entry and combined in lane order.
 // grid.x = window, grid.y = slice
 // ---------------------------------------------------------------------------------------------
+constexpr int kRedEntries = 16;               // entries per workgroup (one 128-byte line per chunk)
+constexpr int kRedLanes = kBlock / kRedEntries;  // chunk lanes per entry
+
 __global__ __launch_bounds__(kBlock) void k_reduce(const WinDev* wins, int which, double** outs) {
+    __shared__ double red[kRedLanes][kRedEntries];
     const WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
     if (which == 0 && !w.lm->new_point) return;
     const int len = w.tri + w.nv;
-    const int i = blockIdx.y * kBlock + threadIdx.x;
-    if (i >= len) return;
+    if ((int)blockIdx.y * kRedEntries >= len) return;
+    const int e = threadIdx.x % kRedEntries, cl = threadIdx.x / kRedEntries;
+    const int i = blockIdx.y * kRedEntries + e;
     const double* part = which == 0 ? w.lin_part : w.schur_part;
     double s = 0;
-    for (int c = 0; c < w.nchunks; ++c) s += part[(size_t)c * len + i];
-    outs[2 * blockIdx.x + which][i] = s;
+    if (i < len)
+        for (int c = cl; c < w.nchunks; c += kRedLanes) s += part[(size_t)c * len + i];
+    red[cl][e] = s;
+    __syncthreads();
+    if (cl == 0 && i < len) {
+        double t = 0;
+        for (int q = 0; q < kRedLanes; ++q) t += red[q][e];
+        outs[2 * blockIdx.x + which][i] = t;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -835,112 +853,113 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const WinDev* wins, int which
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ double damped_point(double d, double lambda) { return d + lambda * fmax(d, 1e-6); }
 
+// The correction -sum_k c_k c_k^T / d_k is a product C^T diag(1/d) C of the chunk's dense
+// coupling rows (c_k over the nv visual columns: anchor pose sum_f u_f, observer poses -u_f,
+// rig, td), formed 64 landmarks at a time in LDS and multiplied on MFMA (v_mfma_f64_16x16x4f64,
+// upper 16x16 tiles of the padded 128 x 128 block, 9 tiles per wave).
+constexpr int kSchurRows = 64;                 // landmarks per LDS pass
+constexpr int kSchurLd = 144;                  // LDS row stride (doubles; offsets rows by 128 B)
+constexpr int kVisTiles = 8;                   // 128 / 16
+constexpr int kSchurTiles = kVisTiles * (kVisTiles + 1) / 2;
+constexpr int kSchurTilesPerWave = (kSchurTiles + kBlock / 64 - 1) / (kBlock / 64);
+
 __global__ __launch_bounds__(kBlock) void k_schur(const WinDev* wins, const Chunk* chunks) {
-    __shared__ double acc[kAccSize];
-    constexpr int groups = kBlock / kChunkLanes;
-    constexpr int kSlots = 16;  // anchor + up to 13 observers + rig + td
-    __shared__ double cv[groups][kSlots][6];
-    __shared__ int ccol[groups][kSlots];
-    __shared__ int cw[groups][kSlots];
+    __shared__ double crow[kSchurRows * kSchurLd];
+    __shared__ double sc[kSchurRows], qg[kSchurRows];
     const Chunk ch = chunks[blockIdx.x];
     const WinDev& w = wins[ch.win];
     if (!window_active(w)) return;
     const int nv = w.nv, tri = w.tri;
-    for (int i = threadIdx.x; i < tri + nv; i += kBlock) acc[i] = 0;
-    __syncthreads();
-    double* gacc = acc + tri;
     const double lambda = w.lm->lambda;
-    const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
+    const int t = threadIdx.x, lane = t % 64, wave = t / 64;
+    const int lgroup = t % kChunkLanes, group = t / kChunkLanes;
+    constexpr int groups = kBlock / kChunkLanes;
     const size_t L = (size_t)w.L, F = (size_t)w.F;
-    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
-        const int l = base + group;
-        const bool live = l < ch.lm_end && w.lm_col[l] >= 0;
-        int nslots = 0;
-        double inv = 0, gk = 0;
-        double ca[6] = {0, 0, 0, 0, 0, 0};
-        if (live) {
+    // this wave's output tiles (ti <= tj) of the padded block
+    int tti[kSchurTilesPerWave], ttj[kSchurTilesPerWave];
+    d4 acc[kSchurTilesPerWave];
+#pragma unroll
+    for (int q = 0; q < kSchurTilesPerWave; ++q) {
+        const int idx = wave + q * (kBlock / 64);
+        tti[q] = -1;
+        ttj[q] = 0;
+        acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+        if (idx < kSchurTiles) {
+            int ti = 0, r = idx;
+            while (r >= kVisTiles - ti) { r -= kVisTiles - ti; ++ti; }
+            tti[q] = ti;
+            ttj[q] = ti + r;
+            if (16 * ttj[q] >= nv) tti[q] = -1;
+        }
+    }
+    double gsum = 0;  // gradient correction of column t (t < nv)
+    for (int base = ch.lm_begin; base < ch.lm_end; base += kSchurRows) {
+        for (int i = t; i < kSchurRows * kSchurLd; i += kBlock) crow[i] = 0.0;
+        __syncthreads();
+        // dense coupling rows, scale 1 / damped(d) and gradient coefficient g / damped(d)
+        for (int r0 = 0; r0 < kSchurRows; r0 += groups) {
+            const int r = r0 + group, l = base + r;
+            const bool live = r < kSchurRows && l < ch.lm_end && w.lm_col[l] >= 0;
+            if (r < kSchurRows && lgroup == 0) {
+                double inv = 0, gk = 0;
+                if (live) {
+                    const double dl = w.lm_out[LO_D * L + l];
+                    // marginalization: pseudo-inverse of the diagonal landmark block (Marginalizer.cpp:373-409)
+                    inv = w.lm->marginal ? (dl > w.lm->marg_cut ? 1.0 / dl : 0.0) : 1.0 / damped_point(dl, lambda);
+                    gk = w.lm_out[LO_G * L + l];
+                }
+                sc[r] = inv;
+                qg[r] = inv * gk;
+            }
+            if (!live) continue;
+            double* row = crow + r * kSchurLd;
             const int a = w.lm_fc[l] >> 8;
+            const int ac = w.pose_col[a];
             const int f0 = w.lm_off[l], f1 = w.lm_off[l + 1];
-            const int m = f1 - f0;
-            const double dl = w.lm_out[LO_D * L + l];
-            // marginalization: pseudo-inverse of the diagonal landmark block (Marginalizer.cpp:373-409)
-            inv = w.lm->marginal ? (dl > w.lm->marg_cut ? 1.0 / dl : 0.0) : 1.0 / damped_point(dl, lambda);
-            gk = w.lm_out[LO_G * L + l];
-            // observer slots 1..m : -u_f at observer frame
-            for (int j = lane; j < m && j < kSlots - 3; j += kChunkLanes) {
-                const int f = f0 + j;
+            for (int f = f0 + lgroup; f < f1; f += kChunkLanes) {
                 const int b = w.ob_fc[f] >> 8;
-                const int col = b == a ? -1 : w.pose_col[b];
-                ccol[group][1 + j] = col;
-                cw[group][1 + j] = 6;
+                const int bc = b == a ? -1 : w.pose_col[b];
                 for (int k = 0; k < 6; ++k) {
                     const double u = w.fac_u[k * F + f];
-                    cv[group][1 + j][k] = -u;
-                    ca[k] += u;
+                    if (ac >= 0) atomicAdd(&row[ac + k], u);
+                    if (bc >= 0) atomicAdd(&row[bc + k], -u);
                 }
             }
-            nslots = 1 + (m < kSlots - 3 ? m : kSlots - 3);
-        }
-        for (int k = 0; k < 6; ++k) ca[k] = group_sum(ca[k]);
-        if (live && lane == 0) {
-            const int a = w.lm_fc[l] >> 8;
-            ccol[group][0] = w.pose_col[a];
-            cw[group][0] = 6;
-            for (int k = 0; k < 6; ++k) cv[group][0][k] = ca[k];
-            int s = nslots;
-            if (w.rig_col >= 0) {
-                ccol[group][s] = w.rig_col; cw[group][s] = 6;
-                for (int k = 0; k < 6; ++k) cv[group][s][k] = w.lm_out[(LO_CR0 + k) * L + l];
-                ++s;
-            }
-            if (w.td_col >= 0) {
-                ccol[group][s] = w.td_col; cw[group][s] = 1;
-                cv[group][s][0] = w.lm_out[LO_CT * L + l];
-                ++s;
+            if (lgroup == 0) {
+                if (w.rig_col >= 0)
+                    for (int k = 0; k < 6; ++k) row[w.rig_col + k] = w.lm_out[(LO_CR0 + k) * L + l];
+                if (w.td_col >= 0) row[w.td_col] = w.lm_out[LO_CT * L + l];
             }
         }
-        if (live) nslots += (w.rig_col >= 0) + (w.td_col >= 0);
         __syncthreads();
-        if (live) {
-            // gradient: g_r -= c g_k / damped
-            for (int s = lane; s < nslots; s += kChunkLanes) {
-                const int col = ccol[group][s];
-                if (col < 0) continue;
-                for (int k = 0; k < cw[group][s]; ++k) atomicAdd(&gacc[col + k], -inv * cv[group][s][k] * gk);
-            }
-            // unordered slot pairs (s <= t)
-            const int npairs = nslots * (nslots + 1) / 2;
-            for (int p = lane; p < npairs; p += kChunkLanes) {
-                int s = 0, rem = p;
-                while (rem >= nslots - s) { rem -= nslots - s; ++s; }
-                const int t2 = s + rem;
-                const int c1 = ccol[group][s], c2 = ccol[group][t2];
-                if (c1 < 0 || c2 < 0) continue;
-                const int w1 = cw[group][s], w2 = cw[group][t2];
-                const double* v1 = cv[group][s];
-                const double* v2 = cv[group][t2];
-                if (c1 == c2) {
-                    if (s == t2) {
-                        for (int i = 0; i < w1; ++i)
-                            for (int j = i; j < w1; ++j) atomicAdd(&acc[tri_index(c1 + i, c1 + j, nv)], -inv * v1[i] * v1[j]);
-                    } else {
-                        for (int i = 0; i < w1; ++i)
-                            for (int j = i; j < w1; ++j)
-                                atomicAdd(&acc[tri_index(c1 + i, c1 + j, nv)], -inv * (v1[i] * v2[j] + v2[i] * v1[j]));
-                    }
-                } else if (c1 < c2) {
-                    for (int i = 0; i < w1; ++i)
-                        for (int j = 0; j < w2; ++j) atomicAdd(&acc[tri_index(c1 + i, c2 + j, nv)], -inv * v1[i] * v2[j]);
-                } else {
-                    for (int j = 0; j < w2; ++j)
-                        for (int i = 0; i < w1; ++i) atomicAdd(&acc[tri_index(c2 + j, c1 + i, nv)], -inv * v1[i] * v2[j]);
-                }
+        // C^T diag(s) C on MFMA: A[i][k] = s_k C[k][16 ti + i] (lane i + 16 k), B[k][j] = C[k][16 tj + j]
+#pragma unroll
+        for (int q = 0; q < kSchurTilesPerWave; ++q) {
+            if (tti[q] < 0) continue;
+            const int ci = 16 * tti[q] + lane % 16, cj = 16 * ttj[q] + lane % 16;
+            for (int k0 = 0; k0 < kSchurRows; k0 += 4) {
+                const int k = k0 + lane / 16;
+                const double av = sc[k] * crow[k * kSchurLd + ci];
+                const double bv = crow[k * kSchurLd + cj];
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
             }
         }
+        if (t < nv)
+            for (int k = 0; k < kSchurRows; ++k) gsum += qg[k] * crow[k * kSchurLd + t];
         __syncthreads();
     }
+    // chunk partial: packed upper triangle of -C^T diag(s) C, then -C^T (s g)
     double* part = w.schur_part + (size_t)ch.local * (tri + nv);
-    for (int i = threadIdx.x; i < tri + nv; i += kBlock) part[i] = acc[i];
+#pragma unroll
+    for (int q = 0; q < kSchurTilesPerWave; ++q) {
+        if (tti[q] < 0) continue;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int i = 16 * tti[q] + 4 * rr + lane / 16, j = 16 * ttj[q] + lane % 16;
+            if (i <= j && j < nv) part[tri_index(i, j, nv)] = -acc[q][rr];
+        }
+    }
+    if (t < nv) part[tri + t] = -gsum;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1081,7 +1100,6 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
 //      and updates the right-hand side below the block.
 // Two workgroup barriers per tile column.
 // ---------------------------------------------------------------------------------------------
-typedef double d4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) double gdouble;
 constexpr int kMaxTiles = (kMaxRetained + kTile - 1) / kTile;
 constexpr int kTileDoubles = kTile * kTile;
@@ -1837,13 +1855,21 @@ const char* vibk_name(int k) {
     return k >= 0 && k < 11 ? names[k] : "";
 }
 
+// Threads per k_lin_visual workgroup (256 or 512; environment VIB_LIN_THREADS).
+static int lin_threads = [] {
+    const char* e = std::getenv("VIB_LIN_THREADS");
+    return e && std::atoi(e) == 512 ? 512 : 256;
+}();
+
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
                        int max_len, int max_edges, int refinement, int mode, hipStream_t s) {
     using namespace vib;
-    const int slices = (max_len + kBlock - 1) / kBlock;
+    const int slices = (max_len + kRedEntries - 1) / kRedEntries;
     switch (k) {
         case 0:
-            if (nchunks) hipLaunchKernelGGL(k_lin_visual, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            if (!nchunks) break;
+            if (lin_threads == 512) hipLaunchKernelGGL(k_lin_visual<512>, dim3(nchunks), dim3(512), 0, s, wins, chunks);
+            else hipLaunchKernelGGL(k_lin_visual<kBlock>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
             break;
         case 1: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 0); break;
         case 2: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 0, red); break;
