@@ -72,6 +72,7 @@ struct LmState {
     int32_t max_iterations;
     int32_t evaluate_only;  // piecewise API: compute metrics, never accept
     int32_t continuous;     // benchmark mode: relinearize every iteration, never terminate
+    int32_t accept_pending; // the last update accepted the trial state (k_accept copies it)
     // marginalization (landmark-first Schur with a pseudo-inverse cutoff, then a positive
     // semidefinite tile Cholesky of the prior's information; Marginalizer.cpp:356-474)
     int32_t marginal;       // 1: build the marginal system instead of an LM step

@@ -27,7 +27,7 @@
 extern "C" {
 const char* vibk_name(int k);
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
-                       int max_len, int max_edges, int refinement, int mode, hipStream_t s);
+                       int max_len, int max_edges, int max_landmarks, int refinement, int mode, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t vibk_marg_cholesky(vib::WinDev* wins, int nwin, hipStream_t s);
@@ -372,6 +372,7 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     std::vector<double*> red(2 * nwin);
     max_len = 1;
     max_edges = 0;
+    max_landmarks = 0;
     for (int i = 0; i < nwin; ++i) {
         Slot& s = *slots[i];
         wins[i] = s.dev;
@@ -380,6 +381,7 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
         red[2 * i + 1] = s.reduced_schur;
         max_len = std::max(max_len, wins[i].tri + wins[i].nv);
         max_edges = std::max(max_edges, s.host.E);
+        max_landmarks = std::max(max_landmarks, s.host.L);
     }
     nchunks = (int)all.size();
     if (hipMalloc(&d_wins, sizeof(WinDev) * nwin) != hipSuccess ||
@@ -393,7 +395,8 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
 }
 
 bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
-    return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_red, max_len, max_edges, refinement, mode, s) == hipSuccess;
+    return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_red, max_len, max_edges, max_landmarks, refinement, mode, s) ==
+           hipSuccess;
 }
 
 bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(1, s, 0, 0) && launch(2, s, 0, 0); }
@@ -1105,7 +1108,7 @@ int vib_accept(vib_context* ctx, int32_t slot) {
     std::vector<Slot*> one{s};
     Batch b;
     std::string err;
-    if (!b.build(one, ctx->stream, err) || !b.launch(10, ctx->stream, 0, 2) ||
+    if (!b.build(one, ctx->stream, err) || !b.launch(10, ctx->stream, 0, 2) || !b.launch(11, ctx->stream, 0, 0) ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(ctx, VIB_ERR_DEVICE, "accept failed");
     return VIB_OK;

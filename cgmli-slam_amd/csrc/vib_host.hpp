@@ -17,7 +17,7 @@ constexpr int kMaxFactorsPerLandmark = 13;
 // Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major, small n); eigenvalues
 // ascending, eigenvectors as columns of V (vib_preint.cpp).
 void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V);
-constexpr int kKernelCount = 11;  // kernels of one LM iteration (vibk_launch ids 0..10)
+constexpr int kKernelCount = 12;  // kernels of one LM iteration (vibk_launch ids 0..11)
 
 // Host copy of one window: observation streams sorted by landmark, tangent column layout
 // [pose (6, free) ..., rig (6), td (1), speed/bias (width) ...] then free inverse ranges,
@@ -75,6 +75,7 @@ struct Slot {
 struct Batch {
     int nwin = 0, nchunks = 0, max_len = 1;
     int max_edges = 0;
+    int max_landmarks = 0;
     WinDev* d_wins = nullptr;
     Chunk* d_chunks = nullptr;
     double** d_red = nullptr;

@@ -967,7 +967,8 @@ __global__ __launch_bounds__(kBlock) void k_schur(const WinDev* wins, const Chun
 // reduced camera system S and rhs (prepareStateSolve, VisualCudaBatch.cu:163-172).
 // red_lin / red_schur: reduced partials (tri + nv) per window.
 // ---------------------------------------------------------------------------------------------
-constexpr int kAssembleRows = kTile;  // rows of the reduced system per workgroup (one tile row)
+constexpr int kAssembleRows = 4;                       // rows per workgroup (one register row of a tile row)
+constexpr int kAssembleSplit = kTile / kAssembleRows;  // workgroups per tile row
 
 // Retained (IMU + prior) normal-matrix entry (i, j) from the per-edge Jacobians and the prior
 // chart derivatives / Gram matrix, using the static column map.
@@ -1007,22 +1008,24 @@ __device__ __forceinline__ double retained_entry(const WinDev& w, const int* ci,
 }
 
 __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const* reduced) {
+    __shared__ double red[kBlock];
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
     const int nr = w.nr, nv = w.nv;
-    const int row0 = blockIdx.y * kAssembleRows;
-    if (row0 >= nr) return;
+    const int ti = blockIdx.y / kAssembleSplit, q = blockIdx.y % kAssembleSplit;
+    if (kTile * ti >= nr) return;
+    const int row0 = kTile * ti + kAssembleRows * q;
     const int row1 = min(nr, row0 + kAssembleRows);
     const double* rl = reduced[2 * blockIdx.x + 0];
     const double* rs = reduced[2 * blockIdx.x + 1];
     const int t = threadIdx.x, nt = blockDim.x;
     const bool fresh = w.lm->new_point;
     if (fresh) {
-        for (size_t p = (size_t)row0 * nr + t; p < (size_t)row1 * nr; p += nt) {
-            const int i = (int)(p / nr), j = (int)(p % nr);
+        for (int p = t; p < (row1 - row0) * nr; p += nt) {
+            const int i = row0 + p / nr, j = p % nr;
             double h = retained_entry(w, w.colmap + 4 * i, w.colmap + 4 * j);
             if (i < nv && j < nv) h += rl[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
-            w.Hfull[p] = h;
+            w.Hfull[(size_t)i * nr + j] = h;
         }
         for (int i = row0 + t; i < row1; i += nt) {
             const int* ci = w.colmap + 4 * i;
@@ -1043,21 +1046,26 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
             }
             w.gfull[i] = g;
         }
-        if (blockIdx.y == 0 && t == 0) {
-            double cost = 0;
-            int invalid = 0;
-            for (int k = 0; k < w.nchunks; ++k) {
+        if (blockIdx.y == 0) {  // cost at the linearization point (fixed-order sums)
+            double cost = 0, invalid = 0;
+            for (int k = t; k < w.nchunks; k += nt) {
                 cost += w.chunk_sc[(size_t)k * CS_PLANES + CS_COST];
-                invalid |= w.chunk_sc[(size_t)k * CS_PLANES + CS_INVALID] > 0;
+                invalid = fmax(invalid, w.chunk_sc[(size_t)k * CS_PLANES + CS_INVALID]);
             }
-            for (int e = 0; e < w.E; ++e) {
+            for (int e = t; e < w.E; e += nt) {
                 cost += w.ret_sc[e];
-                invalid |= w.ret_sc[2 * w.E + e] > 0;
+                invalid = fmax(invalid, w.ret_sc[2 * w.E + e]);
             }
-            cost += w.ret_sc[4 * w.E];
-            invalid |= w.ret_sc[4 * w.E + 2] > 0;
-            w.lm->lin_cost = cost;
-            w.lm->lin_valid = !invalid && isfinite(cost);
+            if (t == 0) {
+                cost += w.ret_sc[4 * w.E];
+                invalid = fmax(invalid, w.ret_sc[4 * w.E + 2]);
+            }
+            cost = block_sum(cost, red);
+            invalid = block_max(invalid, red);
+            if (t == 0) {
+                w.lm->lin_cost = cost;
+                w.lm->lin_valid = !(invalid > 0) && isfinite(cost);
+            }
         }
         __syncthreads();
     }
@@ -1069,17 +1077,17 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
         if (i == j) v += lambda * fmax(h, 1e-6);
         return v;
     };
-    for (size_t p = (size_t)row0 * nr + t; p < (size_t)row1 * nr; p += nt) {
-        const int i = (int)(p / nr), j = (int)(p % nr);
-        w.Scopy[p] = damped(i, j);
+    for (int p = t; p < (row1 - row0) * nr; p += nt) {
+        const int i = row0 + p / nr, j = p % nr;
+        w.Scopy[(size_t)i * nr + j] = damped(i, j);
     }
     for (int i = row0 + t; i < row1; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
-    // tile row blockIdx.y of the factor tile store (lower triangle, padding identity)
-    const int ti = blockIdx.y;
-    for (int p = t; p < (ti + 1) * kTile * kTile; p += nt) {
-        const int tj = p / (kTile * kTile), e = p % (kTile * kTile);
-        const int r = kTile * ti + 4 * (e / 64) + (e % 64) / 16, c = kTile * tj + e % 16;
-        w.fac[tile_base(ti, tj) + e] = (r < nr && c < nr) ? damped(r, c) : (r == c ? 1.0 : 0.0);
+    // this workgroup's four rows of tile row ti of the factor tile store (register q of the
+    // accumulator layout; padding identity)
+    for (int p = t; p < (ti + 1) * 64; p += nt) {
+        const int tj = p / 64, e = p % 64;
+        const int r = row0 + e / 16, c = kTile * tj + e % 16;
+        w.fac[tile_base(ti, tj) + 64 * q + e] = (r < nr && c < nr) ? damped(r, c) : (r == c ? 1.0 : 0.0);
     }
 }
 
@@ -1734,33 +1742,41 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
 // mode 0: LM step; 1: evaluate only (metrics, no state change); 2: accept the trial state.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
-    __shared__ int accept_flag;
+    __shared__ double red[kBlock];
     WinDev& w = wins[blockIdx.x];
     LmState* lm = w.lm;
-    if (mode != 2 && lm->status != 0) return;
     const int t = threadIdx.x;
+    if (mode != 2 && lm->status != 0) {
+        if (t == 0) lm->accept_pending = 0;
+        return;
+    }
     if (mode == 2) {
-        if (t == 0) accept_flag = 1;
+        if (t == 0) lm->accept_pending = 1;
     } else {
+        // per-chunk and retained scalars of the trial (fixed-order block sums)
+        const int E = w.E;
+        double pred = 0, trial = 0, mx = 0, tinv = 0;
+        for (int e = t; e < E; e += blockDim.x) {
+            trial += w.ret_sc[E + e];
+            tinv = fmax(tinv, w.ret_sc[3 * E + e]);
+        }
+        for (int k = t; k < w.nchunks; k += blockDim.x) {
+            const double* sc = w.chunk_sc + (size_t)k * CS_PLANES;
+            pred += sc[CS_PRED];
+            trial += sc[CS_TRIAL];
+            mx = fmax(mx, sc[CS_MAXINC]);
+            tinv = fmax(tinv, sc[CS_TINVALID]);
+        }
+        pred = block_sum(pred, red);
+        trial = block_sum(trial, red);
+        mx = block_max(mx, red);
+        tinv = block_max(tinv, red);
         if (t == 0) {
-            const int E = w.E;
-            double pred = lm->predicted, trial = 0, mx = lm->max_increment;
-            int tinv = !lm->trial_valid;
-            for (int e = 0; e < E; ++e) {
-                trial += w.ret_sc[E + e];
-                tinv |= w.ret_sc[3 * E + e] > 0;
-            }
+            int accept_flag = 0;
+            pred += lm->predicted;
+            mx = fmax(mx, lm->max_increment);
             trial += w.ret_sc[4 * E + 1];
-            tinv |= w.ret_sc[4 * E + 3] > 0;
-            for (int k = 0; k < w.nchunks; ++k) {
-                const double* sc = w.chunk_sc + (size_t)k * CS_PLANES;
-                pred += sc[CS_PRED];
-                trial += sc[CS_TRIAL];
-                mx = fmax(mx, sc[CS_MAXINC]);
-                tinv |= sc[CS_TINVALID] > 0;
-            }
-            const bool trial_valid = !tinv && isfinite(trial);
-            accept_flag = 0;
+            const bool trial_valid = !(tinv > 0) && !(w.ret_sc[4 * E + 3] > 0) && lm->trial_valid && isfinite(trial);
             const int i = lm->iteration;
             int status = 0;
             if (lm->new_point) {
@@ -1813,16 +1829,33 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
                     if (lm->status == 1 && !lm->accepted && !lm->converged) lm->status = -6;
                 }
             }
+            lm->accept_pending = accept_flag;
         }
     }
-    __syncthreads();
-    if (!accept_flag) return;
-    for (int i = t; i < 7 * w.K; i += blockDim.x) w.pose[i] = w.cpose[i];
-    for (int i = t; i < 9 * w.K; i += blockDim.x) w.sb[i] = w.csb[i];
-    if (t < 7) w.rig[t] = w.crig[t];
-    if (t == 0) *w.td = *w.ctd;
-    for (int i = t; i < w.L; i += blockDim.x) w.rho[i] = w.crho[i];
 }
+
+// ---------------------------------------------------------------------------------------------
+// k_accept: adopt the trial state after an accepted step (acceptResident, VisualCudaBatch.cu:702);
+// slice 0 copies the states, every slice 4096 inverse depths.
+// ---------------------------------------------------------------------------------------------
+constexpr int kAcceptSlice = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_accept(WinDev* wins) {
+    WinDev& w = wins[blockIdx.x];
+    if (!w.lm->accept_pending) return;
+    const int t = threadIdx.x;
+    const int l0 = blockIdx.y * kAcceptSlice;
+    if (l0 >= w.L && blockIdx.y > 0) return;
+    if (blockIdx.y == 0) {
+        for (int i = t; i < 7 * w.K; i += blockDim.x) w.pose[i] = w.cpose[i];
+        for (int i = t; i < 9 * w.K; i += blockDim.x) w.sb[i] = w.csb[i];
+        if (t < 7) w.rig[t] = w.crig[t];
+        if (t == 0) *w.td = *w.ctd;
+    }
+    const int l1 = min(w.L, l0 + kAcceptSlice);
+    for (int i = l0 + t; i < l1; i += blockDim.x) w.rho[i] = w.crho[i];
+}
+
 
 // ---------------------------------------------------------------------------------------------
 // Prior Gram matrix G = A^T A (cols x cols), one workgroup per window, after a prior upload.
@@ -1851,8 +1884,8 @@ extern "C" {
 const char* vibk_name(int k) {
     static const char* names[] = {"k_lin_visual", "k_retained_lin", "k_reduce_lin", "k_schur", "k_reduce_schur",
                                   "k_assemble", "k_cholesky", "k_backsub", "k_trial_cost", "k_retained_trial",
-                                  "k_lm_update"};
-    return k >= 0 && k < 11 ? names[k] : "";
+                                  "k_lm_update", "k_accept"};
+    return k >= 0 && k < 12 ? names[k] : "";
 }
 
 // Threads per k_lin_visual workgroup (256 or 512; environment VIB_LIN_THREADS).
@@ -1862,7 +1895,7 @@ static int lin_threads = [] {
 }();
 
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
-                       int max_len, int max_edges, int refinement, int mode, hipStream_t s) {
+                       int max_len, int max_edges, int max_landmarks, int refinement, int mode, hipStream_t s) {
     using namespace vib;
     const int slices = (max_len + kRedEntries - 1) / kRedEntries;
     switch (k) {
@@ -1878,8 +1911,8 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             break;
         case 4: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 1, red); break;
         case 5:
-            hipLaunchKernelGGL(k_assemble, dim3(nwin, (kMaxRetained + kAssembleRows - 1) / kAssembleRows), dim3(kBlock), 0, s,
-                               wins, red);
+            hipLaunchKernelGGL(k_assemble, dim3(nwin, kAssembleSplit * ((kMaxRetained + kTile - 1) / kTile)), dim3(kBlock), 0,
+                               s, wins, red);
             break;
         case 6: hipLaunchKernelGGL(k_cholesky, dim3(nwin), dim3(kCholThreads), 0, s, wins, refinement); break;
         case 7:
@@ -1890,6 +1923,10 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             break;
         case 9: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 1); break;
         case 10: hipLaunchKernelGGL(k_lm_update, dim3(nwin), dim3(kBlock), 0, s, wins, mode); break;
+        case 11:
+            hipLaunchKernelGGL(k_accept, dim3(nwin, (max_landmarks + kAcceptSlice - 1) / kAcceptSlice + 1), dim3(kBlock), 0, s,
+                               wins);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
