@@ -52,6 +52,8 @@ enum ObsPlane { OB_BX, OB_BY, OB_BZ, OB_VX, OB_VY, OB_VZ, OB_T, OB_S00, OB_S10, 
 enum LmPlane { LM_BX, LM_BY, LM_BZ, LM_VX, LM_VY, LM_VZ, LM_T, LM_PLANES };
 // per-landmark linearization outputs (doubles, each L long)
 enum LmOut { LO_D, LO_G, LO_CR0, LO_CR1, LO_CR2, LO_CR3, LO_CR4, LO_CR5, LO_CT, LO_PLANES };
+// per-factor landmark terms, summed per landmark into LmOut (doubles, each F long)
+enum FacLm { FL_D, FL_G, FL_CR0, FL_CR1, FL_CR2, FL_CR3, FL_CR4, FL_CR5, FL_CT, FL_PLANES };
 // per-chunk scalars
 enum ChunkScalar { CS_COST, CS_INVALID, CS_PRED, CS_MAXINC, CS_TRIAL, CS_TINVALID, CS_PLANES };
 
@@ -113,8 +115,16 @@ struct WinDev {
     // observations (sorted by landmark)
     const int32_t* ob_fc;      // observer frame << 8 | camera
     const double* ob;          // OB_PLANES x F
+    // observations again in (anchor frame, observer frame) pair order, for the visual normal
+    // equations (runs of one pair reduce on MFMA)
+    const double* pf_ob;       // OB_PLANES x F
+    const double* pf_an;       // LM_PLANES x F : the factor's anchor bearing, velocity, t_ref
+    const int32_t* pf_ids;     // F : anchor frame << 24 | anchor cam << 16 | observer frame << 8 | cam
+    const int32_t* pf_lm;      // F : landmark
+    const int32_t* pf_f;       // F : position of the factor in landmark order
     // linearization outputs
-    double* fac_u;             // 6 x F : J_anchor^T J_rho per factor (weighted)
+    double* fac_u;             // 6 x F : J_anchor^T J_rho per factor (weighted, landmark order)
+    double* fac_lm;            // FL_PLANES x F : per-factor landmark terms (landmark order)
     double* lm_out;            // LO_PLANES x L
     double* lm_delta;          // L (tangent step of rho; 0 for constant)
     // IMU
@@ -145,17 +155,20 @@ struct WinDev {
     double* dstate;            // nr state step
     double* scratch;           // 4 * nr
     // partials
-    double* lin_part;          // nchunks x (tri + nv)
+    double* lin_part;          // nlin x (tri + nv) (one per factor range of the visual kernel)
+    double* lin_sc;            // nlin x CS_PLANES
     double* schur_part;        // nchunks x (tri + nv)
     double* chunk_sc;          // nchunks x CS_PLANES
-    int32_t chunk0, nchunks;
+    int32_t nlin, nchunks;
     LmState* lm;               // persistent per-window LM state (device)
 };
 
 struct Chunk {
     int32_t win;        // window index in the batch
     int32_t local;      // chunk index within the window
-    int32_t lm_begin, lm_end;
+    int32_t lm_begin, lm_end;  // landmark range (factor range for the visual-normal work list)
 };
+
+constexpr int kLinFactors = 512;  // factors per visual-normal workgroup (pair order)
 
 }  // namespace vib

@@ -26,8 +26,9 @@
 
 extern "C" {
 const char* vibk_name(int k);
-hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
-                       int max_len, int max_edges, int max_landmarks, int refinement, int mode, hipStream_t s);
+hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, const vib::Chunk* lin,
+                       int nlin, double** red, int max_len, int max_edges, int max_landmarks, int refinement, int mode,
+                       hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t vibk_marg_cholesky(vib::WinDev* wins, int nwin, hipStream_t s);
@@ -221,6 +222,25 @@ bool HostWindow::layout(std::string& err) {
     chunks.clear();
     const int cs = std::max(1, chunk_size);
     for (int l0 = 0, i = 0; l0 < L; l0 += cs, ++i) chunks.push_back(Chunk{0, i, l0, std::min(L, l0 + cs)});
+    // factors in (anchor frame, observer frame) pair order (stable counting sort)
+    std::vector<int> start(kMaxFrames * kMaxFrames + 1, 0), order(F);
+    auto pair_key = [&](int f) { return lm_frame[ob_lm[f]] * kMaxFrames + (ob_fc[f] >> 8); };
+    for (int f = 0; f < F; ++f) start[pair_key(f) + 1]++;
+    for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
+    for (int f = 0; f < F; ++f) order[start[pair_key(f)]++] = f;
+    pf_ob.assign((size_t)OB_PLANES * F, 0);
+    pf_an.assign((size_t)LM_PLANES * F, 0);
+    pf_ids.assign(F, 0); pf_lm.assign(F, 0); pf_f.assign(F, 0);
+    for (int p = 0; p < F; ++p) {
+        const int f = order[p], l = ob_lm[f];
+        for (int q = 0; q < OB_PLANES; ++q) pf_ob[(size_t)q * F + p] = ob[(size_t)q * F + f];
+        for (int q = 0; q < LM_PLANES; ++q) pf_an[(size_t)q * F + p] = lmd[(size_t)q * L + l];
+        pf_ids[p] = (lm_frame[l] << 24) | (lm_cam[l] << 16) | ob_fc[f];
+        pf_lm[p] = l;
+        pf_f[p] = f;
+    }
+    lin_chunks.clear();
+    for (int p0 = 0, i = 0; p0 < F; p0 += kLinFactors, ++i) lin_chunks.push_back(Chunk{0, i, p0, std::min(F, p0 + kLinFactors)});
     return true;
 }
 
@@ -247,8 +267,11 @@ size_t Slot::footprint(const HostWindow& h) {
     d += m * n + m + n * n + h.prior_ref.size();
     d += 2 * nr * nr + factor_doubles((int)nr + kTile) + 8 * nr + 64;
     d += 465 * (size_t)h.E + 36 * B + n + 4 * (size_t)h.E + 4;
+    const size_t nlin = h.lin_chunks.size();
     d += 2 * nch * (tri + nv) + nch * CS_PLANES + 2 * (tri + nv);
-    size_t i = 3 * h.L + 1 + h.F + 2 * h.E + h.pfree.size() + 4 * nr;
+    d += nlin * (tri + nv + CS_PLANES);
+    d += (size_t)(OB_PLANES + LM_PLANES + FL_PLANES) * h.F;
+    size_t i = 3 * h.L + 1 + 4 * h.F + 2 * h.E + h.pfree.size() + 4 * nr;
     return 8 * d + 4 * i + sizeof(PriorBlockDev) * B + sizeof(LmState) + 64 * 256;
 }
 
@@ -282,6 +305,13 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     double* ob = a.take<double>((size_t)OB_PLANES * h.F);
     d.cam = cam; d.lm_fc = lm_fc; d.lm_off = lm_off; d.lm_col = lm_col; d.lmd = lmd; d.ob_fc = ob_fc; d.ob = ob;
     d.fac_u = a.take<double>((size_t)6 * h.F);
+    d.fac_lm = a.take<double>((size_t)FL_PLANES * h.F);
+    double* pf_ob = a.take<double>((size_t)OB_PLANES * h.F);
+    double* pf_an = a.take<double>((size_t)LM_PLANES * h.F);
+    int32_t* pf_ids = a.take<int32_t>(h.F);
+    int32_t* pf_lm = a.take<int32_t>(h.F);
+    int32_t* pf_f = a.take<int32_t>(h.F);
+    d.pf_ob = pf_ob; d.pf_an = pf_an; d.pf_ids = pf_ids; d.pf_lm = pf_lm; d.pf_f = pf_f;
     d.lm_out = a.take<double>((size_t)LO_PLANES * h.L);
     d.lm_delta = a.take<double>(h.L);
     int32_t* imu_from = a.take<int32_t>(h.E);
@@ -307,9 +337,12 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     d.fac = a.take<double>(factor_doubles((int)nr + kTile)); d.Scopy = a.take<double>(nr * nr);
     d.rhs = a.take<double>(nr); d.dstate = a.take<double>(nr); d.scratch = a.take<double>(4 * nr + 64);
     const size_t len = d.tri + h.nv, nch = h.chunks.size();
-    d.lin_part = a.take<double>(nch * len); d.schur_part = a.take<double>(nch * len);
+    const size_t nlin = h.lin_chunks.size();
+    d.lin_part = a.take<double>(nlin * len); d.schur_part = a.take<double>(nch * len);
+    d.lin_sc = a.take<double>(nlin * CS_PLANES);
     d.chunk_sc = a.take<double>(nch * CS_PLANES);
     d.nchunks = (int)nch;
+    d.nlin = (int)nlin;
     reduced_lin = a.take<double>(len); reduced_schur = a.take<double>(len);
     d.lm = a.take<LmState>(1);
     if (a.used > bytes) { err = "arena overflow"; return false; }
@@ -328,7 +361,10 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
               up(pblk, h.pblk.data(), sizeof(PriorBlockDev) * h.pblk.size()) &&
               up(pA, h.prior_A.data(), 8 * h.prior_A.size()) && up(pr0, h.prior_r0.data(), 8 * h.prior_r0.size()) &&
               up(pref, h.prior_ref.data(), 8 * h.prior_ref.size()) &&
-              up(pfree, h.pfree.data(), 4 * h.pfree.size()) && up(colmap, h.colmap.data(), 4 * h.colmap.size());
+              up(pfree, h.pfree.data(), 4 * h.pfree.size()) && up(colmap, h.colmap.data(), 4 * h.colmap.size()) &&
+              up(pf_ob, h.pf_ob.data(), 8 * h.pf_ob.size()) && up(pf_an, h.pf_an.data(), 8 * h.pf_an.size()) &&
+              up(pf_ids, h.pf_ids.data(), 4 * h.pf_ids.size()) && up(pf_lm, h.pf_lm.data(), 4 * h.pf_lm.size()) &&
+              up(pf_f, h.pf_f.data(), 4 * h.pf_f.size());
     // candidate starts equal to the state
     ok = ok && up(d.cpose, h.pose.data(), 8 * h.pose.size()) && up(d.csb, h.sb.data(), 8 * h.sb.size()) &&
          up(d.crig, h.rig, 56) && up(d.ctd, &h.td, 8) && up(d.crho, h.rho.data(), 8 * h.rho.size());
@@ -360,6 +396,7 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
 Batch::~Batch() {
     if (d_wins) hipFree(d_wins);
     if (d_chunks) hipFree(d_chunks);
+    if (d_lin) hipFree(d_lin);
     if (d_red) hipFree(d_red);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
@@ -373,10 +410,12 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     max_len = 1;
     max_edges = 0;
     max_landmarks = 0;
+    std::vector<Chunk> lin;
     for (int i = 0; i < nwin; ++i) {
         Slot& s = *slots[i];
         wins[i] = s.dev;
         for (Chunk c : s.host.chunks) { c.win = i; all.push_back(c); }
+        for (Chunk c : s.host.lin_chunks) { c.win = i; lin.push_back(c); }
         red[2 * i] = s.reduced_lin;
         red[2 * i + 1] = s.reduced_schur;
         max_len = std::max(max_len, wins[i].tri + wins[i].nv);
@@ -384,19 +423,22 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
         max_landmarks = std::max(max_landmarks, s.host.L);
     }
     nchunks = (int)all.size();
+    nlin = (int)lin.size();
     if (hipMalloc(&d_wins, sizeof(WinDev) * nwin) != hipSuccess ||
         hipMalloc(&d_chunks, sizeof(Chunk) * std::max(1, nchunks)) != hipSuccess ||
+        hipMalloc(&d_lin, sizeof(Chunk) * std::max(1, nlin)) != hipSuccess ||
         hipMalloc(&d_red, sizeof(double*) * 2 * nwin) != hipSuccess) { err = "batch allocation failed"; return false; }
     if (hipMemcpyAsync(d_wins, wins.data(), sizeof(WinDev) * nwin, hipMemcpyHostToDevice, stream) != hipSuccess ||
         (nchunks && hipMemcpyAsync(d_chunks, all.data(), sizeof(Chunk) * nchunks, hipMemcpyHostToDevice, stream) != hipSuccess) ||
+        (nlin && hipMemcpyAsync(d_lin, lin.data(), sizeof(Chunk) * nlin, hipMemcpyHostToDevice, stream) != hipSuccess) ||
         hipMemcpyAsync(d_red, red.data(), sizeof(double*) * 2 * nwin, hipMemcpyHostToDevice, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess) { err = "batch upload failed"; return false; }
     return true;
 }
 
 bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
-    return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_red, max_len, max_edges, max_landmarks, refinement, mode, s) ==
-           hipSuccess;
+    return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_lin, nlin, d_red, max_len, max_edges, max_landmarks, refinement,
+                       mode, s) == hipSuccess;
 }
 
 bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(1, s, 0, 0) && launch(2, s, 0, 0); }

@@ -50,6 +50,10 @@ struct HostWindow {
     std::vector<int32_t> pfree;
     std::vector<int32_t> colmap;  // nr x 4 retained-factor column map
     std::vector<Chunk> chunks;
+    // factors in (anchor frame, observer frame) pair order
+    std::vector<double> pf_ob, pf_an;
+    std::vector<int32_t> pf_ids, pf_lm, pf_f;
+    std::vector<Chunk> lin_chunks;  // factor ranges of the visual-normal kernel
 
     bool build(const vib_window& w, std::string& err);
     bool layout(std::string& err);
@@ -73,11 +77,12 @@ struct Slot {
 
 // Descriptors of a set of windows solved together.
 struct Batch {
-    int nwin = 0, nchunks = 0, max_len = 1;
+    int nwin = 0, nchunks = 0, nlin = 0, max_len = 1;
     int max_edges = 0;
     int max_landmarks = 0;
     WinDev* d_wins = nullptr;
     Chunk* d_chunks = nullptr;
+    Chunk* d_lin = nullptr;
     double** d_red = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

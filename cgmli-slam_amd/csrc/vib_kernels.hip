@@ -142,18 +142,16 @@ struct FactorJ {
     double w;        // Huber weight
 };
 
+// One visual factor: observation record f of the planes `ob` (stride F), observer frame/camera fc.
 template <bool WANT_J>
-__device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData& lm, int f, double td,
-                            bool want_td, FactorJ& o) {
-    const int fc = w.ob_fc[f];
+__device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData& lm, const double* ob, size_t F,
+                            int f, int fc, double td, bool want_td, FactorJ& o) {
     const int b = fc >> 8, cb = fc & 255;
     const int a = lm.frame, ca = lm.cam;
     const double rho = lm.rho;
     if (!(rho > 0) || !isfinite(rho) || !isfinite(td)) return false;
     if (!ft.valid[a] || !ft.valid[b] || !ft.rig_valid) return false;
     const bool same = a == b;
-    const double* ob = w.ob;
-    const size_t F = (size_t)w.F;
     const V3<double> braw = {ob[OB_BX * F + f], ob[OB_BY * F + f], ob[OB_BZ * F + f]};
     const V3<double> bvel = {ob[OB_VX * F + f], ob[OB_VY * F + f], ob[OB_VZ * F + f]};
     const double btref = ob[OB_T * F + f];
@@ -299,134 +297,188 @@ __device__ __forceinline__ LmData load_landmark(const WinDev& w, const double* r
     return d;
 }
 
-// Add sign * (A^T B) for 2-row Jacobian blocks A (2 x na) at column ca and B (2 x nb) at cb into
-// the packed upper triangle `acc` (nv wide). ca == cb adds only the upper part of A^T A.
-__device__ __forceinline__ void acc_block(double* acc, int nv, int ca, const double* A, int na, int cb,
-                                          const double* B, int nb, double sign) {
-    if (ca < 0 || cb < 0) return;
-    if (ca == cb) {
-        for (int i = 0; i < na; ++i)
-            for (int j = i; j < nb; ++j)
-                atomicAdd(&acc[tri_index(ca + i, cb + j, nv)], sign * (A[i] * B[j] + A[na + i] * B[nb + j]));
-    } else if (ca < cb) {
-        for (int i = 0; i < na; ++i)
-            for (int j = 0; j < nb; ++j)
-                atomicAdd(&acc[tri_index(ca + i, cb + j, nv)], sign * (A[i] * B[j] + A[na + i] * B[nb + j]));
-    } else {
-        for (int j = 0; j < nb; ++j)
-            for (int i = 0; i < na; ++i)
-                atomicAdd(&acc[tri_index(cb + j, ca + i, nv)], sign * (A[i] * B[j] + A[na + i] * B[nb + j]));
-    }
-}
-
 __device__ __forceinline__ bool window_active(const WinDev& w) { return w.lm->status == 0; }
 
 constexpr int kAccSize = kMaxVisual * (kMaxVisual + 1) / 2 + kMaxVisual;
 
 // ---------------------------------------------------------------------------------------------
-// k_lin_visual: one chunk of landmarks per workgroup, 8 lanes per landmark.
+// k_lin_visual: visual factors in (anchor frame, observer frame) pair order, one factor per lane
+// (VisualCudaBatch.cu:46-136). Every run of one pair inside a wave reduces to the Gram matrix of
+// its stacked weighted rows [J_anchor (6) | J_rig (6) | J_td | r] on v_mfma_f64_16x16x4f64
+// (32 steps over the wave's 128 rows); the Gram matrix is scattered once into the packed visual
+// block (anchor, observer, rig and td blocks follow from it with signs, J_observer = -J_anchor).
+// Per-factor landmark terms go to fac_u / fac_lm in landmark order (k_lm_sums adds them up).
 // ---------------------------------------------------------------------------------------------
-template <int NT>
-__global__ __launch_bounds__(NT) void k_lin_visual(const WinDev* wins, const Chunk* chunks) {
+constexpr int kGramCols = 16;
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ void acc_add(double* acc, int nv, int ci, int cj, double v) {
+    if (ci < 0 || cj < 0) return;
+    atomicAdd(&acc[ci <= cj ? tri_index(ci, cj, nv) : tri_index(cj, ci, nv)], v);
+}
+
+// Gram entry G[i][j] (i <= j) of pair (a, b) into the visual block and gradient.
+__device__ __forceinline__ void scatter_gram(double* acc, double* gacc, const WinDev& w, int a, int b, int i, int j,
+                                             double g) {
+    const int nv = w.nv;
+    const bool same = a == b;
+    const int ac = same ? -1 : w.pose_col[a], bc = same ? -1 : w.pose_col[b];
+    const int rc = w.rig_col, tc = w.td_col;
+    if (j < 6) {  // anchor/observer pose blocks
+        acc_add(acc, nv, ac >= 0 ? ac + i : -1, ac >= 0 ? ac + j : -1, g);
+        acc_add(acc, nv, bc >= 0 ? bc + i : -1, bc >= 0 ? bc + j : -1, g);
+        if (ac >= 0 && bc >= 0) {
+            acc_add(acc, nv, ac + i, bc + j, -g);
+            if (i != j) acc_add(acc, nv, ac + j, bc + i, -g);
+        }
+    } else if (j < 12) {
+        if (i < 6) {
+            if (rc >= 0 && !same) {
+                acc_add(acc, nv, ac >= 0 ? ac + i : -1, rc + j - 6, g);
+                acc_add(acc, nv, bc >= 0 ? bc + i : -1, rc + j - 6, -g);
+            }
+        } else if (rc >= 0) {
+            acc_add(acc, nv, rc + i - 6, rc + j - 6, g);
+        }
+    } else if (j == 12) {
+        if (tc < 0) return;
+        if (i < 6) {
+            acc_add(acc, nv, ac >= 0 ? ac + i : -1, tc, g);
+            acc_add(acc, nv, bc >= 0 ? bc + i : -1, tc, -g);
+        } else if (i < 12) {
+            if (rc >= 0 && !same) acc_add(acc, nv, rc + i - 6, tc, g);
+        } else {
+            acc_add(acc, nv, tc, tc, g);
+        }
+    } else if (j == 13) {  // gradient J^T r
+        if (i < 6) {
+            if (ac >= 0) atomicAdd(&gacc[ac + i], g);
+            if (bc >= 0) atomicAdd(&gacc[bc + i], -g);
+        } else if (i < 12) {
+            if (rc >= 0 && !same) atomicAdd(&gacc[rc + i - 6], g);
+        } else if (i == 12) {
+            if (tc >= 0) atomicAdd(&gacc[tc], g);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* lin) {
     __shared__ double acc[kAccSize];
     __shared__ FrameTable ft;
-    __shared__ double red[NT];
-    const Chunk ch = chunks[blockIdx.x];
+    __shared__ double rows[kWaves][128 * kGramCols];  // per wave: 2 rows per lane x 16 columns
+    __shared__ int keys[kWaves][64];
+    __shared__ double red[kBlock];
+    const Chunk ch = lin[blockIdx.x];
     const WinDev& w = wins[ch.win];
     if (!window_active(w) || !w.lm->new_point) return;
     const int nv = w.nv, tri = w.tri;
-    for (int i = threadIdx.x; i < tri + nv; i += NT) acc[i] = 0;
+    for (int i = threadIdx.x; i < tri + nv; i += kBlock) acc[i] = 0;
     build_frame_table(w, w.pose, w.rig, ft);
     __syncthreads();
     double* gacc = acc + tri;
     const double td = *w.td;
     const bool want_td = w.td_col >= 0;
-    const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
-    constexpr int groups = NT / kChunkLanes;
+    const int t = threadIdx.x, lane = t % 64, wave = t / 64;
+    double* V = rows[wave];
+    const size_t F = (size_t)w.F;
     double cost = 0;
     int invalid = 0;
-    const size_t L = (size_t)w.L, F = (size_t)w.F;
-    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
-        const int l = base + group;
-        const bool live = l < ch.lm_end;
-        double d = 0, gr = 0, cr[6] = {0, 0, 0, 0, 0, 0}, ct = 0;
-        int col = -1;
-        if (live) {
-            const LmData lm = load_landmark(w, w.rho, l);
-            col = w.lm_col[l];
-            const int a = lm.frame;
-            const int ac = w.pose_col[a];
-            for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
-                FactorJ J;
-                if (!eval_factor<true>(w, ft, lm, f, td, want_td, J)) {
-                    invalid = 1;
-                    for (int k = 0; k < 6; ++k) w.fac_u[k * F + f] = 0;
-                    continue;
-                }
-                const double sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
-                if (!isfinite(sq)) { invalid = 1; continue; }
+    for (int p0 = ch.lm_begin + 64 * wave; p0 < ch.lm_end; p0 += kBlock) {
+        const int p = p0 + lane;
+        int key = -1;
+#pragma unroll
+        for (int c = 0; c < 2 * kGramCols; ++c) V[2 * kGramCols * lane + c] = 0.0;
+        if (p < ch.lm_end) {
+            const int ids = w.pf_ids[p];
+            const int f = w.pf_f[p];
+            LmData lm;
+            lm.raw = {w.pf_an[LM_BX * F + p], w.pf_an[LM_BY * F + p], w.pf_an[LM_BZ * F + p]};
+            lm.vel = {w.pf_an[LM_VX * F + p], w.pf_an[LM_VY * F + p], w.pf_an[LM_VZ * F + p]};
+            lm.tref = w.pf_an[LM_T * F + p];
+            lm.rho = w.rho[w.pf_lm[p]];
+            lm.frame = ids >> 24;
+            lm.cam = (ids >> 16) & 255;
+            const int fc = ids & 0xffff;
+            FactorJ J;
+            bool ok = eval_factor<true>(w, ft, lm, w.pf_ob, F, p, fc, td, want_td, J);
+            const double sq = ok ? J.r[0] * J.r[0] + J.r[1] * J.r[1] : 0.0;
+            ok = ok && isfinite(sq);
+            double u[6] = {0, 0, 0, 0, 0, 0}, fl[FL_PLANES] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (ok) {
                 cost += huber_cost(w.huber, sq);
                 const double r0 = J.w * J.r[0], r1 = J.w * J.r[1];
-                const int b = w.ob_fc[f] >> 8;
-                const bool same = a == b;
-                const int bc = same ? -1 : w.pose_col[b];
-                const int acol = same ? -1 : ac;
-                // pose-pose blocks: J_o = -J_a
-                acc_block(acc, nv, acol, J.Ja, 6, acol, J.Ja, 6, 1.0);
-                acc_block(acc, nv, bc, J.Ja, 6, bc, J.Ja, 6, 1.0);
-                if (acol >= 0 && bc >= 0) acc_block(acc, nv, acol, J.Ja, 6, bc, J.Ja, 6, -1.0);
-                if (w.rig_col >= 0 && !same) {
-                    acc_block(acc, nv, acol, J.Ja, 6, w.rig_col, J.Jr, 6, 1.0);
-                    acc_block(acc, nv, bc, J.Ja, 6, w.rig_col, J.Jr, 6, -1.0);
-                    acc_block(acc, nv, w.rig_col, J.Jr, 6, w.rig_col, J.Jr, 6, 1.0);
-                }
-                if (want_td) {
-                    if (!same) {
-                        acc_block(acc, nv, acol, J.Ja, 6, w.td_col, J.Jt, 1, 1.0);
-                        acc_block(acc, nv, bc, J.Ja, 6, w.td_col, J.Jt, 1, -1.0);
-                        if (w.rig_col >= 0) acc_block(acc, nv, w.rig_col, J.Jr, 6, w.td_col, J.Jt, 1, 1.0);
-                    }
-                    acc_block(acc, nv, w.td_col, J.Jt, 1, w.td_col, J.Jt, 1, 1.0);
-                }
-                // gradient
+                const bool same = lm.frame == (fc >> 8);
+                double* v0 = V + 2 * kGramCols * lane;
+                double* v1 = v0 + kGramCols;
+#pragma unroll
                 for (int k = 0; k < 6; ++k) {
-                    const double ga = J.Ja[k] * r0 + J.Ja[6 + k] * r1;
-                    if (acol >= 0) atomicAdd(&gacc[acol + k], ga);
-                    if (bc >= 0) atomicAdd(&gacc[bc + k], -ga);
-                    if (w.rig_col >= 0 && !same) atomicAdd(&gacc[w.rig_col + k], J.Jr[k] * r0 + J.Jr[6 + k] * r1);
+                    v0[k] = J.Ja[k]; v1[k] = J.Ja[6 + k];
+                    v0[6 + k] = J.Jr[k]; v1[6 + k] = J.Jr[6 + k];
                 }
-                if (want_td) atomicAdd(&gacc[w.td_col], J.Jt[0] * r0 + J.Jt[1] * r1);
-                // landmark coupling pieces
-                double u[6];
-                for (int k = 0; k < 6; ++k) u[k] = J.Ja[k] * J.Jrho[0] + J.Ja[6 + k] * J.Jrho[1];
-                for (int k = 0; k < 6; ++k) w.fac_u[k * F + f] = same ? 0.0 : u[k];
-                d += J.Jrho[0] * J.Jrho[0] + J.Jrho[1] * J.Jrho[1];
-                gr += J.Jrho[0] * r0 + J.Jrho[1] * r1;
+                v0[12] = J.Jt[0]; v1[12] = J.Jt[1];
+                v0[13] = r0; v1[13] = r1;
+                key = lm.frame * 32 + (fc >> 8);
                 if (!same)
-                    for (int k = 0; k < 6; ++k) cr[k] += J.Jr[k] * J.Jrho[0] + J.Jr[6 + k] * J.Jrho[1];
-                ct += J.Jt[0] * J.Jrho[0] + J.Jt[1] * J.Jrho[1];
+                    for (int k = 0; k < 6; ++k) u[k] = J.Ja[k] * J.Jrho[0] + J.Ja[6 + k] * J.Jrho[1];
+                fl[FL_D] = J.Jrho[0] * J.Jrho[0] + J.Jrho[1] * J.Jrho[1];
+                fl[FL_G] = J.Jrho[0] * r0 + J.Jrho[1] * r1;
+                if (!same)
+                    for (int k = 0; k < 6; ++k) fl[FL_CR0 + k] = J.Jr[k] * J.Jrho[0] + J.Jr[6 + k] * J.Jrho[1];
+                fl[FL_CT] = J.Jt[0] * J.Jrho[0] + J.Jt[1] * J.Jrho[1];
+            } else {
+                invalid = 1;
             }
+            for (int k = 0; k < 6; ++k) w.fac_u[k * F + f] = u[k];
+            for (int q = 0; q < FL_PLANES; ++q) w.fac_lm[q * F + f] = fl[q];
         }
-        d = group_sum(d);
-        gr = group_sum(gr);
-        for (int k = 0; k < 6; ++k) cr[k] = group_sum(cr[k]);
-        ct = group_sum(ct);
-        if (live && lane == 0) {
-            w.lm_out[LO_D * L + l] = d;
-            w.lm_out[LO_G * L + l] = gr;
-            for (int k = 0; k < 6; ++k) w.lm_out[(LO_CR0 + k) * L + l] = cr[k];
-            w.lm_out[LO_CT * L + l] = ct;
+        keys[wave][lane] = key;
+        wave_sync();
+        // runs of one pair: Gram matrix on MFMA, then one scatter per run
+        unsigned long long pending = __ballot(key >= 0);
+        while (pending) {
+            const int leader = __ffsll(pending) - 1;
+            const int pk = __shfl(key, leader, 64);
+            d4 G = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+            for (int st = 0; st < 32; ++st) {
+                const int row = 4 * st + lane / 16;
+                const double x = keys[wave][row >> 1] == pk ? V[row * kGramCols + lane % 16] : 0.0;
+                G = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, G, 0, 0, 0);
+            }
+            const int a = pk / 32, b = pk % 32;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int i = 4 * rr + lane / 16, j = lane % 16;
+                if (i <= j && j < 14) scatter_gram(acc, gacc, w, a, b, i, j, G[rr]);
+            }
+            pending &= ~__ballot(key == pk);
         }
-        (void)col;
+        wave_sync();
     }
     __syncthreads();
     double* part = w.lin_part + (size_t)ch.local * (tri + nv);
-    for (int i = threadIdx.x; i < tri + nv; i += NT) part[i] = acc[i];
+    for (int i = threadIdx.x; i < tri + nv; i += kBlock) part[i] = acc[i];
     const double c = block_sum(cost, red);
     const double inv = block_max(invalid ? 1.0 : 0.0, red);
     if (threadIdx.x == 0) {
-        w.chunk_sc[(size_t)ch.local * CS_PLANES + CS_COST] = c;
-        w.chunk_sc[(size_t)ch.local * CS_PLANES + CS_INVALID] = inv;
+        w.lin_sc[(size_t)ch.local * CS_PLANES + CS_COST] = c;
+        w.lin_sc[(size_t)ch.local * CS_PLANES + CS_INVALID] = inv;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_lm_sums: per-landmark d, g_rho, c_rig, c_td from the per-factor terms (fixed order).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_lm_sums(const WinDev* wins, const Chunk* chunks) {
+    const Chunk ch = chunks[blockIdx.x];
+    const WinDev& w = wins[ch.win];
+    if (!window_active(w) || !w.lm->new_point) return;
+    const size_t L = (size_t)w.L, F = (size_t)w.F;
+    for (int l = ch.lm_begin + threadIdx.x; l < ch.lm_end; l += blockDim.x) {
+        double sum[FL_PLANES] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int f = w.lm_off[l]; f < w.lm_off[l + 1]; ++f)
+            for (int q = 0; q < FL_PLANES; ++q) sum[q] += w.fac_lm[q * F + f];
+        for (int q = 0; q < FL_PLANES; ++q) w.lm_out[q * L + l] = sum[q];
     }
 }
 
@@ -836,9 +888,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const WinDev* wins, int which
     const int e = threadIdx.x % kRedEntries, cl = threadIdx.x / kRedEntries;
     const int i = blockIdx.y * kRedEntries + e;
     const double* part = which == 0 ? w.lin_part : w.schur_part;
+    const int nparts = which == 0 ? w.nlin : w.nchunks;
     double s = 0;
     if (i < len)
-        for (int c = cl; c < w.nchunks; c += kRedLanes) s += part[(size_t)c * len + i];
+        for (int c = cl; c < nparts; c += kRedLanes) s += part[(size_t)c * len + i];
     red[cl][e] = s;
     __syncthreads();
     if (cl == 0 && i < len) {
@@ -1048,9 +1101,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
         }
         if (blockIdx.y == 0) {  // cost at the linearization point (fixed-order sums)
             double cost = 0, invalid = 0;
-            for (int k = t; k < w.nchunks; k += nt) {
-                cost += w.chunk_sc[(size_t)k * CS_PLANES + CS_COST];
-                invalid = fmax(invalid, w.chunk_sc[(size_t)k * CS_PLANES + CS_INVALID]);
+            for (int k = t; k < w.nlin; k += nt) {
+                cost += w.lin_sc[(size_t)k * CS_PLANES + CS_COST];
+                invalid = fmax(invalid, w.lin_sc[(size_t)k * CS_PLANES + CS_INVALID]);
             }
             for (int e = t; e < w.E; e += nt) {
                 cost += w.ret_sc[e];
@@ -1722,7 +1775,7 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
         const LmData lm = load_landmark(w, w.crho, l);
         for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
             FactorJ J;
-            if (!eval_factor<false>(w, ft, lm, f, td, false, J)) { invalid = 1; continue; }
+            if (!eval_factor<false>(w, ft, lm, w.ob, (size_t)w.F, f, w.ob_fc[f], td, false, J)) { invalid = 1; continue; }
             const double sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
             if (!isfinite(sq)) { invalid = 1; continue; }
             cost += huber_cost(w.huber, sq);
@@ -1880,7 +1933,8 @@ __global__ __launch_bounds__(kBlock) void k_prior_gram(const WinDev* wins, doubl
 extern "C" {
 
 
-// One launch per kernel id (the LM iteration is the sequence 0..9).
+// One launch per kernel id (the LM iteration is the sequence 0..11; id 0 is the visual
+// normal equations plus the per-landmark sums).
 const char* vibk_name(int k) {
     static const char* names[] = {"k_lin_visual", "k_retained_lin", "k_reduce_lin", "k_schur", "k_reduce_schur",
                                   "k_assemble", "k_cholesky", "k_backsub", "k_trial_cost", "k_retained_trial",
@@ -1888,21 +1942,15 @@ const char* vibk_name(int k) {
     return k >= 0 && k < 12 ? names[k] : "";
 }
 
-// Threads per k_lin_visual workgroup (256 or 512; environment VIB_LIN_THREADS).
-static int lin_threads = [] {
-    const char* e = std::getenv("VIB_LIN_THREADS");
-    return e && std::atoi(e) == 512 ? 512 : 256;
-}();
-
-hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, double** red,
-                       int max_len, int max_edges, int max_landmarks, int refinement, int mode, hipStream_t s) {
+hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, const vib::Chunk* lin,
+                       int nlin, double** red, int max_len, int max_edges, int max_landmarks, int refinement, int mode,
+                       hipStream_t s) {
     using namespace vib;
     const int slices = (max_len + kRedEntries - 1) / kRedEntries;
     switch (k) {
         case 0:
-            if (!nchunks) break;
-            if (lin_threads == 512) hipLaunchKernelGGL(k_lin_visual<512>, dim3(nchunks), dim3(512), 0, s, wins, chunks);
-            else hipLaunchKernelGGL(k_lin_visual<kBlock>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            if (nlin) hipLaunchKernelGGL(k_lin_visual, dim3(nlin), dim3(kBlock), 0, s, wins, lin);
+            if (nchunks) hipLaunchKernelGGL(k_lm_sums, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
             break;
         case 1: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 0); break;
         case 2: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 0, red); break;
