@@ -69,6 +69,44 @@ def measured_traffic(workload, kernel):
     return None if k is None else k["read_bytes"] + k["write_bytes"]
 
 
+def _batch_window(i):
+    from vibpy import synth
+    return synth.generate(**synth.batch_config(i))
+
+
+def batch_windows(n, procs):
+    """The n windows of the batched workload (sizes drawn per window from the seeded stream),
+    generated in `procs` spawned worker processes (no GPU state is inherited)."""
+    import multiprocessing as mp
+    with mp.get_context("spawn").Pool(procs) as pool:
+        return pool.map(_batch_window, range(n))
+
+
+def batched_line(ctx, abi, wins, steps, warmup):
+    """Batched throughput: all windows in one graph-captured launch sequence per LM iteration,
+    against the same windows solved one after another (one window per launch sequence)."""
+    slots = list(range(1000, 1000 + len(wins)))
+    for s, w in zip(slots, wins):
+        ctx.upload(s, w)
+    opts = abi.options(max_iterations=1, flags=abi.VIB_SOLVE_CONTINUOUS)
+    ctx.bench(slots, opts, warmup, kernels=False)
+    ctx.synchronize()
+    it_ms, _ = ctx.bench(slots, opts, steps, kernels=False)
+    ctx.synchronize()
+    seq = []
+    for s in slots:
+        ms, _ = ctx.bench([s], opts, 3, kernels=False)
+        seq.append(float(np.median(ms)))
+    for s in slots:
+        ctx.release(s)
+    batch_ms = float(np.median(it_ms))
+    seq_ms = float(np.sum(seq))
+    return dict(windows=len(wins), factors=int(sum(w.F for w in wins)), landmarks=int(sum(w.L for w in wins)),
+                value=len(wins) / (batch_ms * 1e-3), unit="window-steps/s", ms_per_batched_step=batch_ms,
+                p99_ms=float(np.percentile(it_ms, 99)), sequential_value=len(wins) / (seq_ms * 1e-3),
+                sequential_ms=seq_ms, speedup_vs_sequential=seq_ms / batch_ms)
+
+
 def cpu_baseline(win, seconds=12.0):
     """Oracle FP64 restatement of the reference CPU path, one thread: steps/s on a bounded sample
     (repeated LM steps on the same window until ~`seconds` of CPU work)."""
@@ -100,6 +138,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--batch-windows", type=int, default=256,
+                    help="windows of the batched workload line (0: skip)")
+    ap.add_argument("--batch-procs", type=int, default=16)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the measurement, launch one plain device copy of this many MiB "
                          "(reference dispatch for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -112,6 +153,7 @@ def main():
     from vibpy import abi, synth
     cfg = synth.CONFIGS[args.workload]
     win = synth.generate(**cfg)
+    bwins = batch_windows(args.batch_windows, args.batch_procs) if args.batch_windows > 0 else []
     prec = abi.VIB_FP64
     ctx = abi.Context(0)
     ctx.upload(0, win)
@@ -125,6 +167,10 @@ def main():
     print(f"[bench] timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
     _, k_ms = ctx.bench([0], opts, max(5, min(args.steps, 20)), kernels=True)
     ctx.synchronize()
+    batched = None
+    if bwins:
+        print(f"[bench] batched workload: {len(bwins)} windows", file=sys.stderr, flush=True)
+        batched = batched_line(ctx, abi, bwins, max(3, min(args.steps, 10)), max(1, args.warmup))
     if args.copy_probe_mb > 0:
         ctx.copy_probe(args.copy_probe_mb << 20)
     ctx.close()
@@ -166,6 +212,9 @@ def main():
         "kernel_ms": k_ms,
         "roofline": roof,
     }
+    if batched is not None:
+        out["batched"] = dict(batched, workload=f"{batched['windows']} independent windows, K 5-20, L 500-30000, "
+                                               "2-4 cameras, 2-10 observations per landmark, 5% outliers at 20 px, FP64")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(win, args.cpu_seconds)
     print(json.dumps(out))

@@ -383,6 +383,9 @@ class Context:
         s = win.cstruct()
         return self.check(self.lib.vib_window_upload(self.h, slot, C.byref(s)), "vib_window_upload")
 
+    def release(self, slot):
+        return self.check(self.lib.vib_window_release(self.h, slot), "vib_window_release")
+
     def read_state(self, slot, win):
         s = win.cstruct()
         return self.check(self.lib.vib_read_state(self.h, slot, C.byref(s)), "vib_read_state")
