@@ -1621,14 +1621,22 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
         x[w.td_col] = fmax(w.td_lo - tdv, fmin(w.td_hi - tdv, x[w.td_col]));
     }
     __syncthreads();
-    // state model decrease: -dx^T (g + 0.5 H dx), undamped H (one wave per row)
+    // state model decrease -dx^T (g + 0.5 H dx) with the undamped H: k_backsub spreads its rows
+    // over the chunk blocks; only a window without landmarks (no chunks) sums it here
     double pred = 0;
-    for (int i = wave; i < n; i += nw) {
-        const double* row = w.Hfull + (size_t)i * n;
-        double hd = 0;
-        for (int j = lane; j < n; j += 64) hd += row[j] * x[j];
-        hd = wave_reduce_sum(hd);
-        if (lane == 0) pred += -x[i] * (w.gfull[i] + 0.5 * hd);
+    for (int i0 = 4 * wave; w.nchunks == 0 && i0 < n; i0 += 4 * nw) {
+        double hd[4] = {0, 0, 0, 0};
+        for (int j = lane; j < n; j += 64) {
+            const double xj = x[j];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (i0 + q < n) hd[q] += w.Hfull[(size_t)(i0 + q) * n + j] * xj;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double h = wave_reduce_sum(hd[q]);
+            if (lane == 0 && i0 + q < n) pred += -x[i0 + q] * (w.gfull[i0 + q] + 0.5 * h);
+        }
     }
     const double ps = block_sum_waves(pred, red);
     double mx = 0, bad = 0;
@@ -1740,6 +1748,18 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
                 w.lm_delta[l] = 0;
                 w.crho[l] = rho;
             }
+        }
+    }
+    // this block's share of the state rows of -dx^T (g + 0.5 H dx), undamped H, one row per wave
+    {
+        const int n = w.nr, wave = threadIdx.x / 64, l64 = threadIdx.x % 64;
+        const int r0 = (int)((long)ch.local * n / w.nchunks), r1 = (int)((long)(ch.local + 1) * n / w.nchunks);
+        for (int i = r0 + wave; i < r1; i += kBlock / 64) {
+            const double* row = w.Hfull + (size_t)i * n;
+            double h = 0;
+            for (int j = l64; j < n; j += 64) h += row[j] * dx[j];
+            h = wave_reduce_sum(h);
+            if (l64 == 0) pred += -dx[i] * (w.gfull[i] + 0.5 * h);
         }
     }
     const double ps = block_sum(pred, red);
