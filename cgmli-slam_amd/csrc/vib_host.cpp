@@ -411,10 +411,22 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     max_edges = 0;
     max_landmarks = 0;
     std::vector<Chunk> lin;
+    // Landmark chunks are merged `merge` at a time when the batch alone fills the device many
+    // times over: fewer per-chunk Schur partials to write and reduce. The per-slot arrays are
+    // sized for the unmerged count, so any merge fits.
+    size_t fine = 0;
+    for (Slot* s : slots) fine += s->host.chunks.size();
+    int merge = fine >= kMergeChunksAbove ? kChunkMerge : 1;
+    if (const char* m = std::getenv("VIB_CHUNK_MERGE")) merge = std::max(1, std::atoi(m));
     for (int i = 0; i < nwin; ++i) {
         Slot& s = *slots[i];
         wins[i] = s.dev;
-        for (Chunk c : s.host.chunks) { c.win = i; all.push_back(c); }
+        const std::vector<Chunk>& hc = s.host.chunks;
+        for (size_t c0 = 0; c0 < hc.size(); c0 += merge) {
+            const size_t c1 = std::min(hc.size(), c0 + merge);
+            all.push_back(Chunk{i, (int)(c0 / merge), hc[c0].lm_begin, hc[c1 - 1].lm_end});
+        }
+        wins[i].nchunks = (int)((hc.size() + merge - 1) / merge);
         for (Chunk c : s.host.lin_chunks) { c.win = i; lin.push_back(c); }
         red[2 * i] = s.reduced_lin;
         red[2 * i + 1] = s.reduced_schur;

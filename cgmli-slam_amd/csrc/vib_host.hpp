@@ -17,7 +17,11 @@ constexpr int kMaxFactorsPerLandmark = 13;
 // Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major, small n); eigenvalues
 // ascending, eigenvectors as columns of V (vib_preint.cpp).
 void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V);
-constexpr int kKernelCount = 12;  // kernels of one LM iteration (vibk_launch ids 0..11)
+constexpr int kKernelCount = 12;
+// Batches with at least this many landmark chunks (about 16 per CU) merge kChunkMerge of them
+// per workgroup (measured on the 256-window batched workload).
+constexpr size_t kMergeChunksAbove = 4096;
+constexpr int kChunkMerge = 4;  // kernels of one LM iteration (vibk_launch ids 0..11)
 
 // Host copy of one window: observation streams sorted by landmark, tangent column layout
 // [pose (6, free) ..., rig (6), td (1), speed/bias (width) ...] then free inverse ranges,

@@ -916,7 +916,8 @@ constexpr int kVisTiles = 8;                   // 128 / 16
 constexpr int kSchurTiles = kVisTiles * (kVisTiles + 1) / 2;
 constexpr int kSchurTilesPerWave = (kSchurTiles + kBlock / 64 - 1) / (kBlock / 64);
 
-__global__ __launch_bounds__(kBlock) void k_schur(const WinDev* wins, const Chunk* chunks) {
+// two workgroups per CU: 256 VGPRs (no AGPRs) and 74 KB of LDS each
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_schur(const WinDev* wins, const Chunk* chunks) {
     __shared__ double crow[kSchurRows * kSchurLd];
     __shared__ double sc[kSchurRows], qg[kSchurRows];
     const Chunk ch = chunks[blockIdx.x];

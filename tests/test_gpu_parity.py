@@ -111,6 +111,30 @@ def test_full_solve_matches_oracle(ctx, oracle, abi, name, cfg):
     assert np.abs(got.lm_rho - exp.lm_rho).max() < 1e-6 * max(1.0, np.abs(exp.lm_rho).max())
 
 
+def test_merged_chunks_match_oracle(ctx, oracle, abi, monkeypatch):
+    """Landmark chunks merged three at a time per workgroup (the large-batch launch shape, here
+    forced on a small window with a ragged last group) give the same LM step as the oracle."""
+    from vibpy import synth
+    monkeypatch.setenv("VIB_CHUNK_MERGE", "3")
+    w = synth.generate(K=8, L=900, n_cams=4, obs_range=(2, 8), seed=31, rig_free=True, td_free=True)
+    lay = w.layout()
+    say("[gpu] step with merged chunks")
+    ctx.upload(0, w)
+    ctx.check(ctx.linearize(0), "linearize")
+    st, rep = ctx.solve_step(0, 1e-4)
+    assert st == 0, ctx.error()
+    nr = lay["nr"]
+    S, rhs = ctx.read_reduced(0, nr)
+    delta = ctx.read_step(0, nr + lay["ne"])
+    ost, ref = oracle.solve_step(w, 1e-4)
+    assert ost == 0
+    assert rel(S, ref["S"]) < 1e-8
+    assert rel(rhs, ref["rhs"]) < 1e-8
+    assert rel(delta, ref["delta"]) < 1e-6
+    assert abs(rep.predicted_reduction - ref["predicted"]) <= 1e-6 * abs(ref["predicted"])
+    assert abs(rep.trial_cost - ref["trial_cost"]) <= 1e-6 * abs(ref["trial_cost"]) + 1e-9
+
+
 def test_batch_alone_equals_mixed(ctx, abi):
     """Each window's result is the same solved alone and inside a batch of uneven windows, within
     1e-10 relative cost and 1e-7 absolute state: the per-workgroup LDS accumulation adds in
