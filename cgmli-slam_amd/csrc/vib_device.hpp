@@ -75,13 +75,10 @@ struct LmState {
     int32_t evaluate_only;  // piecewise API: compute metrics, never accept
     int32_t continuous;     // benchmark mode: relinearize every iteration, never terminate
     int32_t accept_pending; // the last update accepted the trial state (k_accept copies it)
-    // marginalization (landmark-first Schur with a pseudo-inverse cutoff, then a positive
-    // semidefinite tile Cholesky of the prior's information; Marginalizer.cpp:356-474)
+    // marginalization (landmark-first Schur with a pseudo-inverse cutoff; the state block and
+    // the prior's square root follow from k_sym_eigen; Marginalizer.cpp:356-474)
     int32_t marginal;       // 1: build the marginal system instead of an LM step
-    int32_t marg_n;         // dimension of the system factored by k_marg_cholesky
     double marg_cut;        // landmark cutoff max(1e-12, 1e-9 max d)
-    double marg_cut_victim; // pivot cutoff of the first diagonal tile
-    double marg_cut_prior;  // pivot cutoff of the other diagonal tiles
 };
 
 struct PriorBlockDev {

@@ -18,11 +18,13 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/vib.h"
 #include "vib_device.hpp"
 #include "vib_host.hpp"
+#include "vib_math.hpp"
 
 extern "C" {
 const char* vibk_name(int k);
@@ -31,7 +33,7 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
-hipError_t vibk_marg_cholesky(vib::WinDev* wins, int nwin, hipStream_t s);
+hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, hipStream_t s);
 }
 
 namespace vib {
@@ -711,29 +713,191 @@ bool imu_nis(vib_context* ctx, Slot& s, const int edges[2], double nis[2]) {
 // eigen pseudo-inverse and the rank-15 update to H_prior, b_prior. On the device again: a
 // semidefinite MFMA tile Cholesky H_prior = L L^T with r0 = L^-1 b_prior, so A = L^T gives
 // A^T A = H_prior and A^T r0 = b_prior (dropped pivots drop rows). The window then slides.
-int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
-    HostWindow cur;
-    if (!pull_state(ctx, s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
-    const int K = cur.K, victim = 0;
-    if (K < 2) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "marginalization needs at least two frames");
-    std::vector<char> elim(cur.L, 0), tpose(K, 0), tsb(K, 0);
+// Editable copy of a window in the C-ABI layout; view() points a vib_window into it.
+struct WindowCopy {
+    int K = 0, C = 0, L = 0, F = 0, E = 0;
+    uint32_t flags = 0;
+    double huber = 1, min_rho = 1e-8, td_lo = -1e300, td_hi = 1e300;
+    std::vector<int64_t> frame_id;
+    std::vector<double> pose, sb;
+    std::vector<uint8_t> pose_free, sb_width;
+    double rig[7]{}, td = 0;
+    std::vector<double> cam_R, cam_t;
+    std::vector<int32_t> lm_frame, lm_cam;
+    std::vector<double> lm_bearing, lm_vel, lm_tref, rho;
+    std::vector<uint8_t> lm_free;
+    std::vector<int32_t> ob_lm, ob_frame, ob_cam;
+    std::vector<double> ob_bearing, ob_vel, ob_tref, ob_sqrt;
+    std::vector<int32_t> imu_from, imu_to;
+    std::vector<double> imu;
+    int prior_rows = 0;
+    std::vector<int32_t> prior_kind;
+    std::vector<int64_t> prior_id;
+    std::vector<double> prior_ref, prior_A, prior_r0;
+
+    vib_window view() {
+        vib_window w{};
+        w.num_frames = K; w.num_cameras = C; w.num_landmarks = L; w.num_observations = F; w.num_imu = E;
+        w.flags = flags; w.huber_delta = huber; w.min_inverse_depth = min_rho; w.td_lower = td_lo; w.td_upper = td_hi;
+        w.frame_id = frame_id.data(); w.pose = pose.data(); w.speed_bias = sb.data();
+        w.pose_free = pose_free.data(); w.sb_width = sb_width.data();
+        w.rig = rig; w.td = &td; w.cam_rotation = cam_R.data(); w.cam_translation = cam_t.data();
+        w.lm_anchor_frame = lm_frame.data(); w.lm_anchor_cam = lm_cam.data(); w.lm_bearing = lm_bearing.data();
+        w.lm_velocity = lm_vel.data(); w.lm_tref = lm_tref.data(); w.lm_rho = rho.data(); w.lm_free = lm_free.data();
+        w.obs_landmark = ob_lm.data(); w.obs_frame = ob_frame.data(); w.obs_cam = ob_cam.data();
+        w.obs_bearing = ob_bearing.data(); w.obs_velocity = ob_vel.data(); w.obs_tref = ob_tref.data();
+        w.obs_sqrt_info = ob_sqrt.data();
+        w.imu_from = imu_from.data(); w.imu_to = imu_to.data(); w.imu_data = imu.data();
+        w.prior_rows = prior_rows; w.prior_blocks = (int32_t)prior_kind.size();
+        w.prior_key_kind = prior_kind.data(); w.prior_key_id = prior_id.data(); w.prior_ref = prior_ref.data();
+        w.prior_A = prior_A.data(); w.prior_r0 = prior_r0.data();
+        return w;
+    }
+};
+
+// The resident window as a copy (observations in landmark order).
+WindowCopy copy_of(const HostWindow& h) {
+    WindowCopy w;
+    w.K = h.K; w.C = h.C; w.L = h.L; w.F = h.F; w.E = h.E;
+    w.flags = h.flags; w.huber = h.huber; w.min_rho = h.min_rho; w.td_lo = h.td_lo; w.td_hi = h.td_hi;
+    w.frame_id = h.frame_id; w.pose = h.pose; w.sb = h.sb; w.pose_free = h.pose_free; w.sb_width = h.sb_width;
+    std::copy_n(h.rig, 7, w.rig);
+    w.td = h.td;
+    for (int c = 0; c < h.C; ++c) {
+        w.cam_R.insert(w.cam_R.end(), h.cam.begin() + 12 * c, h.cam.begin() + 12 * c + 9);
+        w.cam_t.insert(w.cam_t.end(), h.cam.begin() + 12 * c + 9, h.cam.begin() + 12 * c + 12);
+    }
+    w.lm_frame = h.lm_frame; w.lm_cam = h.lm_cam; w.rho = h.rho; w.lm_free = h.lm_free;
+    const size_t L = h.L, F = h.F;
+    for (size_t l = 0; l < L; ++l) {
+        for (int i = 0; i < 3; ++i) {
+            w.lm_bearing.push_back(h.lmd[(LM_BX + i) * L + l]);
+            w.lm_vel.push_back(h.lmd[(LM_VX + i) * L + l]);
+        }
+        w.lm_tref.push_back(h.lmd[LM_T * L + l]);
+    }
+    for (size_t f = 0; f < F; ++f) {
+        w.ob_lm.push_back(h.ob_lm[f]); w.ob_frame.push_back(h.ob_fc[f] >> 8); w.ob_cam.push_back(h.ob_fc[f] & 255);
+        for (int i = 0; i < 3; ++i) {
+            w.ob_bearing.push_back(h.ob[(OB_BX + i) * F + f]);
+            w.ob_vel.push_back(h.ob[(OB_VX + i) * F + f]);
+            w.ob_sqrt.push_back(h.ob[(OB_S00 + i) * F + f]);
+        }
+        w.ob_tref.push_back(h.ob[OB_T * F + f]);
+    }
+    w.imu_from = h.imu_from; w.imu_to = h.imu_to; w.imu = h.imu;
+    w.prior_rows = h.prior_rows; w.prior_kind = h.prior_kind; w.prior_id = h.prior_id;
+    w.prior_ref = h.prior_ref; w.prior_A = h.prior_A; w.prior_r0 = h.prior_r0;
+    return w;
+}
+
+// The window without slot `drop` (-1: none), restricted to the kept landmarks, factors and IMU
+// edges (factors and edges of the dropped slot go with it); later slots move down by one.
+WindowCopy filter_window(const WindowCopy& w, int drop, const std::vector<char>& keep_lm,
+                         const std::vector<char>& keep_f, const std::vector<char>& keep_e) {
+    WindowCopy o = w;
+    auto slot = [&](int k) { return drop >= 0 && k > drop ? k - 1 : k; };
+    if (drop >= 0) {
+        o.K = w.K - 1;
+        o.frame_id.erase(o.frame_id.begin() + drop);
+        o.pose.erase(o.pose.begin() + 7 * drop, o.pose.begin() + 7 * drop + 7);
+        o.sb.erase(o.sb.begin() + 9 * drop, o.sb.begin() + 9 * drop + 9);
+        o.pose_free.erase(o.pose_free.begin() + drop);
+        o.sb_width.erase(o.sb_width.begin() + drop);
+    }
+    std::vector<int> nl(w.L, -1);
+    o.lm_frame.clear(); o.lm_cam.clear(); o.lm_bearing.clear(); o.lm_vel.clear(); o.lm_tref.clear();
+    o.rho.clear(); o.lm_free.clear();
+    for (int l = 0; l < w.L; ++l) {
+        if (!keep_lm[l]) continue;
+        nl[l] = (int)o.lm_frame.size();
+        o.lm_frame.push_back(slot(w.lm_frame[l])); o.lm_cam.push_back(w.lm_cam[l]);
+        o.lm_bearing.insert(o.lm_bearing.end(), w.lm_bearing.begin() + 3 * l, w.lm_bearing.begin() + 3 * l + 3);
+        o.lm_vel.insert(o.lm_vel.end(), w.lm_vel.begin() + 3 * l, w.lm_vel.begin() + 3 * l + 3);
+        o.lm_tref.push_back(w.lm_tref[l]); o.rho.push_back(w.rho[l]); o.lm_free.push_back(w.lm_free[l]);
+    }
+    o.L = (int)o.lm_frame.size();
+    o.ob_lm.clear(); o.ob_frame.clear(); o.ob_cam.clear(); o.ob_bearing.clear(); o.ob_vel.clear();
+    o.ob_tref.clear(); o.ob_sqrt.clear();
+    for (int f = 0; f < w.F; ++f) {
+        if (!keep_f[f] || nl[w.ob_lm[f]] < 0 || w.ob_frame[f] == drop) continue;
+        o.ob_lm.push_back(nl[w.ob_lm[f]]); o.ob_frame.push_back(slot(w.ob_frame[f])); o.ob_cam.push_back(w.ob_cam[f]);
+        o.ob_bearing.insert(o.ob_bearing.end(), w.ob_bearing.begin() + 3 * f, w.ob_bearing.begin() + 3 * f + 3);
+        o.ob_vel.insert(o.ob_vel.end(), w.ob_vel.begin() + 3 * f, w.ob_vel.begin() + 3 * f + 3);
+        o.ob_sqrt.insert(o.ob_sqrt.end(), w.ob_sqrt.begin() + 3 * f, w.ob_sqrt.begin() + 3 * f + 3);
+        o.ob_tref.push_back(w.ob_tref[f]);
+    }
+    o.F = (int)o.ob_lm.size();
+    o.imu_from.clear(); o.imu_to.clear(); o.imu.clear();
+    for (int e = 0; e < w.E; ++e) {
+        if (!keep_e[e] || w.imu_from[e] == drop || w.imu_to[e] == drop) continue;
+        o.imu_from.push_back(slot(w.imu_from[e])); o.imu_to.push_back(slot(w.imu_to[e]));
+        o.imu.insert(o.imu.end(), w.imu.begin() + (size_t)VIB_IMU_STRIDE * e, w.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+    }
+    o.E = (int)o.imu_from.size();
+    return o;
+}
+
+// Eigendecomposition of a symmetric n x n matrix on the device (k_sym_eigen): eigenvalues
+// ascending, eigenvectors as the columns of U (row-major).
+int device_eigen(vib_context* ctx, const std::vector<double>& M, int n, std::vector<double>& ev, std::vector<double>& U) {
+    double *dA = nullptr, *dV = nullptr, *de = nullptr;
+    const size_t bytes = 8 * (size_t)n * n;
+    std::vector<double> V((size_t)n * n), e(n);
+    const bool ok = hipMalloc(&dA, bytes) == hipSuccess && hipMalloc(&dV, bytes) == hipSuccess &&
+                    hipMalloc(&de, 8 * (size_t)n) == hipSuccess &&
+                    hipMemcpyAsync(dA, M.data(), bytes, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+                    vibk_sym_eigen(dA, dV, de, n, n, ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(V.data(), dV, bytes, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(e.data(), de, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                    hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (dA) hipFree(dA);
+    if (dV) hipFree(dV);
+    if (de) hipFree(de);
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "device eigendecomposition failed");
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e[a] < e[b]; });
+    ev.resize(n);
+    U.resize((size_t)n * n);
+    for (int c = 0; c < n; ++c) {
+        ev[c] = e[order[c]];
+        for (int r = 0; r < n; ++r) U[(size_t)r * n + c] = V[(size_t)r * n + order[c]];
+    }
+    for (double v : ev) if (!std::isfinite(v)) return fail(ctx, VIB_ERR_EIGEN_FAILED, "eigensolver did not converge");
+    return VIB_OK;
+}
+
+struct SchurPrior {
+    int rows = 0, cols = 0, eliminated = 0, landmarks = 0, consumed = 0;
+    double cutoff = 0;
+    std::vector<int32_t> kind;
+    std::vector<int64_t> id;
+    std::vector<double> ref, A, r0;
+};
+
+// Schur elimination of slot `victim` (pose, speed/bias) together with the landmarks of `elim`
+// (all their factors) and the IMU edges `imu_used` into a square-root prior on the separator;
+// the previous prior always takes part (Marginalizer::compute, Marginalizer.cpp:284-474).
+// Consumed visual factors linearize on the device, the landmarks are eliminated there with the
+// diagonal pseudo-inverse and the reduced system is assembled on the device; the victim block
+// is eliminated on the host (eigen pseudo-inverse) and the prior's square root is the
+// semidefinite tile Cholesky on the device.
+int schur_prior(vib_context* ctx, const HostWindow& cur, int victim, const std::vector<char>& elim,
+                const std::vector<int>& imu_used, SchurPrior& out) {
+    const int K = cur.K;
+    std::vector<char> tpose(K, 0), tsb(K, 0);
     bool rig_used = false, td_used = false;
     for (int l = 0; l < cur.L; ++l) {
-        bool e = cur.lm_frame[l] == victim;
-        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) e = e || (cur.ob_fc[f] >> 8) == victim;
-        if (!e) continue;
-        elim[l] = 1;
+        if (!elim[l]) continue;
         tpose[cur.lm_frame[l]] = 1;
         for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) tpose[cur.ob_fc[f] >> 8] = 1;
         if (cur.lm_off[l + 1] > cur.lm_off[l]) rig_used = td_used = true;
     }
-    std::vector<int> imu_used;
-    for (int e = 0; e < cur.E; ++e)
-        if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) {
-            imu_used.push_back(e);
-            tpose[cur.imu_from[e]] = tpose[cur.imu_to[e]] = 1;
-            tsb[cur.imu_from[e]] = tsb[cur.imu_to[e]] = 1;
-        }
+    for (int e : imu_used) {
+        tpose[cur.imu_from[e]] = tpose[cur.imu_to[e]] = 1;
+        tsb[cur.imu_from[e]] = tsb[cur.imu_to[e]] = 1;
+    }
     for (size_t b = 0; b < cur.prior_kind.size(); ++b) {
         int slot = -1;
         for (int k = 0; k < K; ++k) if (cur.frame_id[k] == cur.prior_id[b]) slot = k;
@@ -792,7 +956,7 @@ int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
     std::string err;
     if (!m.layout(err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
     const int nr = m.nr;
-    // eliminated state columns (slot 0 pose and speed/bias) and retained columns in layout order
+    // eliminated state columns (victim pose and speed/bias) and retained columns in layout order
     std::vector<int> victim_cols, kept_cols;
     std::vector<char> is_victim(nr, 0);
     if (m.pose_col[victim] >= 0) for (int i = 0; i < 6; ++i) victim_cols.push_back(m.pose_col[victim] + i);
@@ -838,7 +1002,10 @@ int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
         for (int j = 0; j < nvict; ++j)
             Smm[(size_t)i * nvict + j] = 0.5 * (S[(size_t)victim_cols[i] * nr + victim_cols[j]] +
                                                  S[(size_t)victim_cols[j] * nr + victim_cols[i]]);
-    symmetric_eigen(Smm, nvict, ev, U);
+    {
+        const int st = device_eigen(ctx, Smm, nvict, ev, U);
+        if (st != VIB_OK) return st;
+    }
     const double cut_v = std::max(1e-12, 1e-9 * std::max(0.0, ev[nvict - 1]));
     if (ev[0] < -10.0 * cut_v) return fail(ctx, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
     std::vector<double> Z, zb;  // U_k^T S_mr / sqrt(lambda_k), U_k^T b_m / sqrt(lambda_k)
@@ -866,132 +1033,339 @@ int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
             Hp[(size_t)i * np + j] = Hp[(size_t)j * np + i] = h;
         }
     }
-    // largest eigenvalue of H_prior (power iteration) for the relative pivot cutoff
-    double lmax = 0;
+    // square root of H_prior: A = sqrt(lambda) u^T, r0 = u^T b / sqrt(lambda) for the eigenvalues
+    // above max(1e-12, 1e-9 lambda_max); indefinite below -10 cutoff (Marginalizer.cpp:430-466)
+    std::vector<double> pev, PU;
     {
-        std::vector<double> v(np, 1.0), u(np);
-        for (int it = 0; it < 60; ++it) {
-            double nrm = 0;
-            for (int i = 0; i < np; ++i) {
-                double acc = 0;
-                for (int j = 0; j < np; ++j) acc += Hp[(size_t)i * np + j] * v[j];
-                u[i] = acc;
-                nrm += acc * acc;
-            }
-            nrm = std::sqrt(nrm);
-            if (!(nrm > 0)) break;
-            double ray = 0;
-            for (int i = 0; i < np; ++i) { ray += v[i] * u[i]; v[i] = u[i] / nrm; }
-            lmax = ray;
+        const int st = device_eigen(ctx, Hp, np, pev, PU);
+        if (st != VIB_OK) return st;
+    }
+    const double cut = std::max(1e-12, 1e-9 * std::max(0.0, pev[np - 1]));
+    if (pev[0] < -10.0 * cut) return fail(ctx, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
+    out.cols = np;
+    out.rows = 0;
+    for (int q = 0; q < np; ++q) {
+        if (!(pev[q] > cut)) continue;
+        const double root = std::sqrt(pev[q]);
+        double ub = 0;
+        for (int k = 0; k < np; ++k) {
+            out.A.push_back(root * PU[(size_t)k * np + q]);
+            ub += PU[(size_t)k * np + q] * bp[k];
         }
+        out.r0.push_back(ub / root);
+        ++out.rows;
     }
-    // square root of H_prior: semidefinite MFMA tile Cholesky, r0 = L^-1 b_prior
-    const int T2 = (np + kTile - 1) / kTile, N2 = kTile * T2;
-    lm.marg_n = np;
-    lm.marg_cut_victim = lm.marg_cut_prior = std::max(1e-12, 1e-9 * std::max(0.0, lmax));
-    const size_t tiles = (size_t)T2 * (T2 + 1) / 2;
-    std::vector<double> fac(tiles * kTile * kTile), y(N2);
-    for (int ti = 0; ti < T2; ++ti)
-        for (int tj = 0; tj <= ti; ++tj)
-            for (int e = 0; e < kTile * kTile; ++e) {
-                const int r = kTile * ti + 4 * (e / 64) + (e % 64) / 16, c = kTile * tj + e % 16;
-                fac[tile_base(ti, tj) + e] = (r < np && c < np) ? Hp[(size_t)r * np + c] : (r == c ? 1.0 : 0.0);
-            }
-    if (hipMemcpyAsync(tmp.dev.fac, fac.data(), 8 * fac.size(), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(tmp.dev.scratch, bp.data(), 8 * bp.size(), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-        !write_lm(ctx, tmp, lm) || vibk_marg_cholesky(b.d_wins, 1, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(fac.data(), tmp.dev.fac, 8 * fac.size(), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(y.data(), tmp.dev.scratch + N2, 8 * (size_t)N2, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        !read_lm(ctx, tmp, lm))
-        return fail(ctx, VIB_ERR_DEVICE, "marginal factorization failed");
-    if (!lm.chol_ok) return fail(ctx, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
-    auto Lat = [&](int r, int c) {  // factor element (r >= c)
-        return fac[tile_base(r / kTile, c / kTile) + 64 * ((r % kTile) / 4) + 16 * (r % 4) + c % kTile];
-    };
-    std::vector<int> rows;
-    for (int q = 0; q < np; ++q) if (Lat(q, q) > 0) rows.push_back(q);
-    const int rank = (int)rows.size();
-    std::vector<double> A((size_t)rank * np, 0.0), r0(rank);
-    for (int i = 0; i < rank; ++i) {
-        const int q = rows[i];
-        for (int a = q; a < np; ++a) A[(size_t)i * np + a] = Lat(a, q);
-        r0[i] = y[q];
-    }
-    // new prior keys in column order: poses, rig, td, speed/bias (the eliminated slot excluded)
-    HostWindow next;
-    next.prior_rows = rank; next.prior_cols = np;
-    next.prior_A = A; next.prior_r0 = r0;
+    // prior keys in column order: poses, rig, td, speed/bias (the victim excluded)
     for (int k = 0; k < K; ++k)
         if (k != victim && m.pose_col[k] >= 0) {
-            next.prior_kind.push_back(VIB_KEY_POSE); next.prior_id.push_back(cur.frame_id[k]);
-            next.prior_ref.insert(next.prior_ref.end(), cur.pose.begin() + 7 * k, cur.pose.begin() + 7 * k + 7);
+            out.kind.push_back(VIB_KEY_POSE); out.id.push_back(cur.frame_id[k]);
+            out.ref.insert(out.ref.end(), cur.pose.begin() + 7 * k, cur.pose.begin() + 7 * k + 7);
         }
     if (m.rig_col >= 0) {
-        next.prior_kind.push_back(VIB_KEY_RIG); next.prior_id.push_back(0);
-        next.prior_ref.insert(next.prior_ref.end(), cur.rig, cur.rig + 7);
+        out.kind.push_back(VIB_KEY_RIG); out.id.push_back(0);
+        out.ref.insert(out.ref.end(), cur.rig, cur.rig + 7);
     }
     if (m.td_col >= 0) {
-        next.prior_kind.push_back(VIB_KEY_TD); next.prior_id.push_back(0);
-        next.prior_ref.push_back(cur.td);
+        out.kind.push_back(VIB_KEY_TD); out.id.push_back(0);
+        out.ref.push_back(cur.td);
     }
     for (int k = 0; k < K; ++k)
         if (k != victim && m.sb_col[k] >= 0) {
-            next.prior_kind.push_back(VIB_KEY_SPEED_BIAS); next.prior_id.push_back(cur.frame_id[k]);
-            next.prior_ref.insert(next.prior_ref.end(), cur.sb.begin() + 9 * k, cur.sb.begin() + 9 * k + 9);
+            out.kind.push_back(VIB_KEY_SPEED_BIAS); out.id.push_back(cur.frame_id[k]);
+            out.ref.insert(out.ref.end(), cur.sb.begin() + 9 * k, cur.sb.begin() + 9 * k + 9);
         }
+    out.eliminated = nvict + m.L;
+    out.landmarks = m.L;
+    out.consumed = m.F + m.E + (cur.prior_rows ? 1 : 0);
+    out.cutoff = cut;
+    return VIB_OK;
+}
+
+void set_prior(WindowCopy& w, const SchurPrior& p) {
+    w.prior_rows = p.rows; w.prior_kind = p.kind; w.prior_id = p.id; w.prior_ref = p.ref;
+    w.prior_A = p.A; w.prior_r0 = p.r0;
+}
+
+// Makes `w` the resident window of slot s.
+int install(vib_context* ctx, Slot& s, WindowCopy& w) {
+    HostWindow h;
+    std::string err;
+    const vib_window v = w.view();
+    if (!h.build(v, err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
+    if (!s.upload(h, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    return VIB_OK;
+}
+
+// OLD: slot 0 with every landmark anchored in or observed by it and its IMU edges.
+int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
+    HostWindow cur;
+    if (!pull_state(ctx, s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    const int victim = 0;
+    if (cur.K < 2) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "marginalization needs at least two frames");
+    std::vector<char> elim(cur.L, 0);
+    for (int l = 0; l < cur.L; ++l) {
+        bool e = cur.lm_frame[l] == victim;
+        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) e = e || (cur.ob_fc[f] >> 8) == victim;
+        elim[l] = e;
+    }
+    std::vector<int> imu_used;
+    for (int e = 0; e < cur.E; ++e) if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) imu_used.push_back(e);
+    SchurPrior p;
+    const int st = schur_prior(ctx, cur, victim, elim, imu_used, p);
+    if (st != VIB_OK) return st;
     // slide: drop slot 0, its landmarks, factors and IMU edges
-    next.K = K - 1; next.C = cur.C; next.flags = cur.flags;
-    next.huber = cur.huber; next.min_rho = cur.min_rho; next.td_lo = cur.td_lo; next.td_hi = cur.td_hi;
-    next.frame_id.assign(cur.frame_id.begin() + 1, cur.frame_id.end());
-    next.pose.assign(cur.pose.begin() + 7, cur.pose.end());
-    next.sb.assign(cur.sb.begin() + 9, cur.sb.end());
-    next.pose_free.assign(cur.pose_free.begin() + 1, cur.pose_free.end());
-    next.sb_width.assign(cur.sb_width.begin() + 1, cur.sb_width.end());
-    std::copy_n(cur.rig, 7, next.rig);
-    next.td = cur.td; next.cam = cur.cam;
-    std::vector<int> live;
-    for (int l = 0; l < cur.L; ++l) if (!elim[l]) live.push_back(l);
-    next.L = (int)live.size();
-    next.lm_off.assign(1, 0);
-    next.lm_frame.resize(next.L); next.lm_cam.resize(next.L); next.lm_free.resize(next.L); next.rho.resize(next.L);
-    next.lmd.assign((size_t)LM_PLANES * next.L, 0);
-    std::vector<int> live_f;
-    for (int i = 0; i < next.L; ++i) {
-        const int l = live[i];
-        next.lm_frame[i] = cur.lm_frame[l] - 1; next.lm_cam[i] = cur.lm_cam[l];
-        next.lm_free[i] = cur.lm_free[l]; next.rho[i] = cur.rho[l];
-        for (int p = 0; p < LM_PLANES; ++p) next.lmd[(size_t)p * next.L + i] = cur.lmd[(size_t)p * cur.L + l];
-        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) live_f.push_back(f);
-        next.lm_off.push_back((int)live_f.size());
-    }
-    next.F = (int)live_f.size();
-    next.ob_fc.resize(next.F); next.ob_lm.resize(next.F);
-    next.ob.assign((size_t)OB_PLANES * next.F, 0);
-    for (int i = 0, li = 0; i < next.F; ++i) {
-        while (i >= next.lm_off[li + 1]) ++li;
-        const int f = live_f[i];
-        next.ob_fc[i] = cur.ob_fc[f] - (1 << 8);
-        next.ob_lm[i] = li;
-        for (int p = 0; p < OB_PLANES; ++p) next.ob[(size_t)p * next.F + i] = cur.ob[(size_t)p * cur.F + f];
-    }
-    for (int e = 0; e < cur.E; ++e) {
-        if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) continue;
-        next.imu_from.push_back(cur.imu_from[e] - 1);
-        next.imu_to.push_back(cur.imu_to[e] - 1);
-        next.imu.insert(next.imu.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e,
-                        cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
-    }
-    next.E = (int)next.imu_from.size();
-    if (!next.layout(err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
-    if (!s.upload(next, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    WindowCopy w = copy_of(cur);
+    std::vector<char> keep_lm(cur.L), keep_f(cur.F, 1), keep_e(cur.E, 1);
+    for (int l = 0; l < cur.L; ++l) keep_lm[l] = !elim[l];
+    WindowCopy next = filter_window(w, victim, keep_lm, keep_f, keep_e);
+    set_prior(next, p);
+    const int up = install(ctx, s, next);
+    if (up != VIB_OK) return up;
     rep->status = VIB_OK;
     rep->prior_reused = 0;
-    rep->eliminated_dimension = nvict + m.L;
-    rep->retained_dimension = np;
-    rep->prior_rank = rank;
-    rep->eliminated_landmarks = m.L;
-    rep->consumed_factors = m.F + m.E + (cur.prior_rows ? 1 : 0);
-    rep->eigen_cutoff = lm.marg_cut_prior;
+    rep->eliminated_dimension = p.eliminated;
+    rep->retained_dimension = p.cols;
+    rep->prior_rank = p.rows;
+    rep->eliminated_landmarks = p.landmarks;
+    rep->consumed_factors = p.consumed;
+    rep->eigen_cutoff = p.cutoff;
+    rep->imu_mode = imu_used.empty() ? VIB_IMU_MERGE_NONE : VIB_IMU_MERGE_BRIDGE;
+    return VIB_OK;
+}
+
+// World pose of camera c of slot k: T_WB T_BR T_RC.
+void camera_world(const WindowCopy& w, int k, int c, M3<double>& R, V3<double>& t) {
+    const M3<double> Rb = quat_to_matrix(&w.pose[7 * k + 3]), Re = quat_to_matrix(w.rig + 3);
+    M3<double> Rc;
+    for (int i = 0; i < 9; ++i) Rc.m[i] = w.cam_R[9 * c + i];
+    R = mul(mul(Rb, Re), Rc);
+    t = mul(Rb, mul(Re, load3(&w.cam_t[3 * c])) + load3(w.rig)) + load3(&w.pose[7 * k]);
+}
+
+// Whether factor f of landmark l evaluates (validity rules of VisualSphereBatchMath.hpp:67-174)
+// with the landmark anchored at (frame, cam) on `bearing` with inverse range rho; the bearings
+// of the tracks this is used for do not move.
+bool static_factor_valid(const WindowCopy& w, int f, int frame, int cam, V3<double> bearing, double rho) {
+    if (!(rho > 0) || !std::isfinite(rho)) return false;
+    M3<double> Ra, Rb;
+    V3<double> ta, tb;
+    camera_world(w, frame, cam, Ra, ta);
+    camera_world(w, w.ob_frame[f], w.ob_cam[f], Rb, tb);
+    const V3<double> ba = (1.0 / std::sqrt(sqnorm(bearing))) * bearing;
+    const V3<double> raw = load3(&w.ob_bearing[3 * f]);
+    const V3<double> bb = (1.0 / std::sqrt(sqnorm(raw))) * raw;
+    const V3<double> pc = mulT(Rb, mul(Ra, (1.0 / rho) * ba) + ta - tb);
+    // sphere_residual's rules: a finite p_C of length >= 1e-10, not within 1e-8 of antipodal,
+    // and a finite residual P f(theta) (n - c b)
+    const double range = std::sqrt(sqnorm(pc));
+    if (!std::isfinite(pc.x) || !std::isfinite(pc.y) || !std::isfinite(pc.z) || !std::isfinite(range) || range < 1e-10)
+        return false;
+    const V3<double> n = (1.0 / range) * pc;
+    const double c = std::fmin(1.0, std::fmax(-1.0, dot(bb, n))), sn = std::sqrt(sqnorm(cross(bb, n)));
+    if (c < -1 + 1e-8) return false;
+    const double f_theta = sn < 1e-8 ? 1.0 : std::atan2(sn, c) / sn;
+    const V3<double> lg = f_theta * (n - c * bb);
+    V3<double> c0, c1;
+    sphere_basis(raw, c0, c1);
+    const double s00 = w.ob_sqrt[3 * f], s10 = w.ob_sqrt[3 * f + 1], s11 = w.ob_sqrt[3 * f + 2];
+    const double r0 = s00 * dot(c0, lg), r1 = s10 * dot(c0, lg) + s11 * dot(c1, lg);
+    return std::isfinite(r0) && std::isfinite(r1);
+}
+
+// Linear interpolation of the gyroscope at t (gyroAt, BodyImuFactor.cpp:29-38).
+void gyro_at(const double* samples, int count, double t, double* out) {
+    int next = 0;
+    while (next < count && samples[7 * next] < t) ++next;
+    if (next == 0 || next == count) {
+        const double* g = &samples[7 * (next == 0 ? 0 : count - 1) + 1];
+        for (int i = 0; i < 3; ++i) out[i] = g[i];
+        return;
+    }
+    const double t0 = samples[7 * (next - 1)], a = (t - t0) / (samples[7 * next] - t0);
+    for (int i = 0; i < 3; ++i) out[i] = (1 - a) * samples[7 * (next - 1) + 1 + i] + a * samples[7 * next + 1 + i];
+}
+
+// SECOND_NEW, the redundant-frame path (SlidingWindowManager.cpp:263-482, 587-680): the two IMU
+// edges of slot K-2 become one re-integrated edge (mergeBodyImuRecords, BodyImuFactor.cpp:306-388),
+// landmarks anchored in K-2 are re-anchored on a surviving view, demoted (one surviving ray) or
+// retired into the prior, the slot's other image measurements are discarded, and the prior is
+// reused when nothing else touches the slot.
+int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int count, double t0, double t1,
+                           const vib_imu_params* params, vib_marginal_report* rep) {
+    HostWindow cur;
+    if (!pull_state(ctx, s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    if (cur.K < 3) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "SECOND_NEW needs at least three frames");
+    WindowCopy w = copy_of(cur);
+    const int K = w.K, v = K - 2, left = K - 3, right = K - 1;
+    // IMU edges of the slot (SlidingWindowManager.cpp:275-311)
+    std::vector<int> incident;
+    for (int e = 0; e < w.E; ++e) if (w.imu_from[e] == v || w.imu_to[e] == v) incident.push_back(e);
+    bool sb_in_prior = false, prior_touches = false;
+    for (size_t b = 0; b < w.prior_kind.size(); ++b)
+        if ((w.prior_kind[b] == VIB_KEY_POSE || w.prior_kind[b] == VIB_KEY_SPEED_BIAS) && w.prior_id[b] == w.frame_id[v]) {
+            prior_touches = true;
+            if (w.prior_kind[b] == VIB_KEY_SPEED_BIAS) sb_in_prior = true;
+        }
+    int imu_mode = incident.empty() ? VIB_IMU_MERGE_NONE : VIB_IMU_MERGE_BRIDGE;
+    std::vector<char> keep_e(w.E, 1);
+    std::vector<double> merged;
+    if (incident.size() == 2 && !sb_in_prior && samples && count > 0 && params) {
+        int a = incident[0], b = incident[1];
+        if (w.imu_from[a] != left) std::swap(a, b);
+        if (w.imu_from[a] != left || w.imu_to[a] != v || w.imu_from[b] != v || w.imu_to[b] != right)
+            return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "IMU merge endpoint/layout mismatch");
+        const double* da = &w.imu[(size_t)VIB_IMU_STRIDE * a];
+        const double* db = &w.imu[(size_t)VIB_IMU_STRIDE * b];
+        if (!(t1 > t0) || std::fabs((t1 - t0) - (da[0] + db[0])) > 1e-9)
+            return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "IMU merge interval does not span the two edges");
+        if (count > 10000) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "merged IMU interval exceeds sample budget");
+        for (int i = 0; i < count; ++i) {
+            for (int c = 0; c < 7; ++c)
+                if (!std::isfinite(samples[7 * i + c])) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "non-finite IMU sample");
+            if (i && !(samples[7 * i] > samples[7 * (i - 1)]))
+                return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "non-increasing IMU samples");
+        }
+        keep_e[a] = keep_e[b] = 0;
+        if (t1 - t0 > 10.0) {
+            imu_mode = VIB_IMU_MERGE_SUPPRESSED;  // maxImuIntervalSeconds: kept out of the solve
+        } else {
+            // the left edge's reference bias is kept while the estimate is within 0.0003 rad/s
+            // (gyro) and 0.02 m/s^2 (accel) of it (BodyImuFactor.cpp:146-170)
+            double lin[9];
+            std::copy_n(&w.sb[9 * left], 9, lin);
+            double dg = 0, dacc = 0;
+            for (int i = 0; i < 3; ++i) {
+                dg += (lin[3 + i] - da[25 + i]) * (lin[3 + i] - da[25 + i]);
+                dacc += (lin[6 + i] - da[28 + i]) * (lin[6 + i] - da[28 + i]);
+            }
+            const bool incremental = std::sqrt(dg) <= 3e-4 && std::sqrt(dacc) <= 0.02;
+            if (incremental) for (int i = 0; i < 6; ++i) lin[3 + i] = da[25 + i];
+            double g0[3], g1[3];
+            gyro_at(samples, count, t0, g0);
+            gyro_at(samples, count, t1, g1);
+            merged.assign(VIB_IMU_STRIDE, 0);
+            const int st = vib_preintegrate(samples, count, t0, t1, lin, params, g0, g1, merged.data());
+            if (st != VIB_OK) return fail(ctx, st, "IMU re-integration failed");
+            imu_mode = incremental ? VIB_IMU_MERGE_INCREMENTAL : VIB_IMU_MERGE_REINTEGRATED;
+        }
+    }
+    if (!merged.empty()) {
+        w.imu_from.push_back(left); w.imu_to.push_back(right);
+        w.imu.insert(w.imu.end(), merged.begin(), merged.end());
+        w.E += 1;
+        keep_e.push_back(1);
+    }
+    // landmarks anchored in the slot (SlidingWindowManager.cpp:312-457)
+    std::vector<std::vector<int>> of_lm(w.L);
+    for (int f = 0; f < w.F; ++f) of_lm[w.ob_lm[f]].push_back(f);
+    std::vector<char> keep_lm(w.L, 1), keep_f(w.F, 1), retired(w.L, 0);
+    int reset = 0, demoted = 0, dropped = 0;
+    for (int l = 0; l < w.L; ++l) {
+        if (w.lm_frame[l] != v) continue;
+        std::vector<int> surviving;
+        bool eligible = sqnorm(load3(&w.lm_vel[3 * l])) == 0;
+        for (int f : of_lm[l]) {
+            eligible = eligible && sqnorm(load3(&w.ob_vel[3 * f])) == 0;
+            if (w.ob_frame[f] != v) surviving.push_back(f);
+        }
+        if (surviving.empty() || !eligible || !w.lm_free[l]) { retired[l] = 1; continue; }
+        if (surviving.size() == 1) {
+            keep_lm[l] = 0;
+            for (int f : of_lm[l]) keep_f[f] = 0;
+            ++demoted;
+            continue;
+        }
+        // preferred anchors: newer frames first, then lower camera index (every retained frame
+        // counts as a keyframe)
+        std::stable_sort(surviving.begin(), surviving.end(), [&](int a, int b) {
+            if (w.ob_frame[a] != w.ob_frame[b]) return w.ob_frame[a] > w.ob_frame[b];
+            return w.ob_cam[a] < w.ob_cam[b];
+        });
+        M3<double> Ra;
+        V3<double> ta;
+        camera_world(w, v, w.lm_cam[l], Ra, ta);
+        const V3<double> raw = load3(&w.lm_bearing[3 * l]);
+        const V3<double> world = mul(Ra, (1.0 / (w.rho[l] * std::sqrt(sqnorm(raw)))) * raw) + ta;
+        int chosen = -1;
+        double new_rho = 0;
+        for (int cand : surviving) {
+            M3<double> Rc;
+            V3<double> tc;
+            camera_world(w, w.ob_frame[cand], w.ob_cam[cand], Rc, tc);
+            const V3<double> delta = mulT(Rc, world - tc);
+            const V3<double> ray = load3(&w.ob_bearing[3 * cand]);
+            const double range = dot(delta, ray);
+            if (!(range > 0) || !std::isfinite(range) || 1 / range < w.min_rho ||
+                std::atan2(std::sqrt(sqnorm(cross(delta, ray))), range) > 0.03) continue;
+            bool valid = true;
+            for (int f : surviving)
+                if (f != cand && !static_factor_valid(w, f, w.ob_frame[cand], w.ob_cam[cand], ray, 1 / range)) {
+                    valid = false;
+                    break;
+                }
+            if (valid) { chosen = cand; new_rho = 1 / range; break; }
+        }
+        if (chosen < 0) { retired[l] = 1; continue; }
+        w.lm_frame[l] = w.ob_frame[chosen]; w.lm_cam[l] = w.ob_cam[chosen];
+        for (int i = 0; i < 3; ++i) {
+            w.lm_bearing[3 * l + i] = w.ob_bearing[3 * chosen + i];
+            w.lm_vel[3 * l + i] = w.ob_vel[3 * chosen + i];
+        }
+        w.lm_tref[l] = w.ob_tref[chosen];
+        w.rho[l] = new_rho;
+        keep_f[chosen] = 0;
+        for (int f : of_lm[l]) if (w.ob_frame[f] == v) keep_f[f] = 0;
+        ++reset;
+    }
+    // the slot's image measurements of landmarks anchored elsewhere are discarded (:458-470)
+    for (int f = 0; f < w.F; ++f)
+        if (w.ob_frame[f] == v && w.lm_frame[w.ob_lm[f]] != v && keep_f[f]) { keep_f[f] = 0; ++dropped; }
+    WindowCopy w1 = filter_window(w, -1, keep_lm, keep_f, keep_e);
+    std::vector<char> ret1;
+    for (int l = 0; l < w.L; ++l) if (keep_lm[l]) ret1.push_back(retired[l]);
+    std::vector<int> imus;
+    for (int e = 0; e < w1.E; ++e) if (w1.imu_from[e] == v || w1.imu_to[e] == v) imus.push_back(e);
+    bool consumed = !imus.empty();
+    int nret = 0;
+    for (int f = 0; f < w1.F; ++f) consumed = consumed || ret1[w1.ob_lm[f]];
+    for (char r : ret1) nret += r;
+    std::vector<char> keep_lm1(w1.L), keep_f1(w1.F, 1), keep_e1(w1.E, 1);
+    for (int l = 0; l < w1.L; ++l) keep_lm1[l] = !ret1[l];
+    WindowCopy next = filter_window(w1, v, keep_lm1, keep_f1, keep_e1);
+    *rep = vib_marginal_report{};
+    rep->imu_mode = imu_mode;
+    rep->reset_landmarks = reset;
+    rep->demoted_landmarks = demoted;
+    rep->dropped_factors = dropped;
+    rep->eliminated_landmarks = nret;
+    if (!consumed && !prior_touches) {
+        // Marginalizer.cpp:299-310: nothing incident is left, the frozen prior is kept as it is
+        const int up = install(ctx, s, next);
+        if (up != VIB_OK) return up;
+        rep->status = VIB_OK;
+        rep->prior_reused = 1;
+        rep->retained_dimension = next.prior_rows ? (int)(next.prior_A.size() / next.prior_rows) : 0;
+        rep->prior_rank = next.prior_rows;
+        return VIB_OK;
+    }
+    HostWindow h1;
+    std::string err;
+    {
+        const vib_window view = w1.view();
+        if (!h1.build(view, err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
+    }
+    SchurPrior p;
+    const int st = schur_prior(ctx, h1, v, ret1, imus, p);
+    if (st != VIB_OK) return st;
+    set_prior(next, p);
+    const int up = install(ctx, s, next);
+    if (up != VIB_OK) return up;
+    rep->status = VIB_OK;
+    rep->eliminated_dimension = p.eliminated;
+    rep->retained_dimension = p.cols;
+    rep->prior_rank = p.rows;
+    rep->consumed_factors = p.consumed;
+    rep->eigen_cutoff = p.cutoff;
     return VIB_OK;
 }
 
@@ -1373,20 +1747,54 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
 }
 
 int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double* samples, int32_t count,
-                    const vib_imu_params* params, vib_marginal_report* rep) {
+                    double t_start, double t_end, const vib_imu_params* params, vib_marginal_report* rep) {
     if (!ctx || !rep) return VIB_ERR_INVALID_ARGUMENT;
-    (void)samples; (void)count; (void)params;
     std::lock_guard<std::mutex> lock(ctx->mutex);
     hipSetDevice(ctx->device);
     *rep = vib_marginal_report{};
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
-    if (policy == VIB_MARGIN_OLD) {
-        const int st = marginalize_old(ctx, *s, rep);
-        rep->status = st;
-        return st;
-    }
-    return fail(ctx, VIB_ERR_UNSUPPORTED, "SECOND_NEW marginalization not available in this build");
+    int st;
+    if (policy == VIB_MARGIN_OLD) st = marginalize_old(ctx, *s, rep);
+    else if (policy == VIB_MARGIN_SECOND_NEW) st = marginalize_second_new(ctx, *s, samples, count, t_start, t_end, params, rep);
+    else return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown marginalization policy");
+    rep->status = st;
+    return st;
+}
+
+int vib_read_window(vib_context* ctx, int32_t slot, vib_window* out) {
+    if (!ctx || !out) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    HostWindow h;
+    if (!pull_state(ctx, *s, h)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    if (out->num_frames != h.K || out->num_cameras != h.C || out->num_landmarks != h.L ||
+        out->num_observations != h.F || out->num_imu != h.E || out->prior_rows != h.prior_rows)
+        return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "window shape mismatch");
+    WindowCopy w = copy_of(h);
+    auto put = [](const auto& v, const auto* p) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(p)>>;
+        if (p && !v.empty()) std::copy(v.begin(), v.end(), const_cast<T*>(p));
+    };
+    out->flags = w.flags; out->huber_delta = w.huber; out->min_inverse_depth = w.min_rho;
+    out->td_lower = w.td_lo; out->td_upper = w.td_hi;
+    put(w.frame_id, out->frame_id); put(w.pose, out->pose); put(w.sb, out->speed_bias);
+    put(w.pose_free, out->pose_free); put(w.sb_width, out->sb_width);
+    if (out->rig) std::copy_n(w.rig, 7, out->rig);
+    if (out->td) *out->td = w.td;
+    put(w.cam_R, out->cam_rotation); put(w.cam_t, out->cam_translation);
+    put(w.lm_frame, out->lm_anchor_frame); put(w.lm_cam, out->lm_anchor_cam); put(w.lm_bearing, out->lm_bearing);
+    put(w.lm_vel, out->lm_velocity); put(w.lm_tref, out->lm_tref); put(w.rho, out->lm_rho); put(w.lm_free, out->lm_free);
+    put(w.ob_lm, out->obs_landmark); put(w.ob_frame, out->obs_frame); put(w.ob_cam, out->obs_cam);
+    put(w.ob_bearing, out->obs_bearing); put(w.ob_vel, out->obs_velocity); put(w.ob_tref, out->obs_tref);
+    put(w.ob_sqrt, out->obs_sqrt_info);
+    put(w.imu_from, out->imu_from); put(w.imu_to, out->imu_to); put(w.imu, out->imu_data);
+    out->prior_blocks = (int32_t)w.prior_kind.size();
+    put(w.prior_kind, out->prior_key_kind); put(w.prior_id, out->prior_key_id); put(w.prior_ref, out->prior_ref);
+    put(w.prior_A, out->prior_A); put(w.prior_r0, out->prior_r0);
+    return VIB_OK;
 }
 
 int vib_read_prior(vib_context* ctx, int32_t slot, int32_t* rows, int32_t* cols, int32_t* blocks, int32_t* kind,

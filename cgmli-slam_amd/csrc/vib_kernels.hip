@@ -772,7 +772,6 @@ __device__ const double* prior_state(const WinDev& w, const PriorBlockDev& b, co
 // mode 0 linearizes at the state (residuals, whitened Jacobians, chart derivatives, A^T r);
 // mode 1 evaluates the cost at the trial state. Results go to per-edge slots that k_assemble /
 // k_lm_update combine in a fixed order. grid = (windows, max_edges + 1), block = one wave.
-constexpr int kMaxPriorBlocks = 2 * kMaxFrames + 2;
 
 __global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
     const WinDev& w = wins[blockIdx.x];
@@ -1166,7 +1165,6 @@ typedef __attribute__((address_space(1))) double gdouble;
 constexpr int kMaxTiles = (kMaxRetained + kTile - 1) / kTile;
 constexpr int kTileDoubles = kTile * kTile;
 constexpr int kCholThreads = 1024;
-constexpr int kTileBatch = 8;
 
 // LDS operand layout of a 16x16 tile: element (i, c) at (c / 4) * 64 + i + 16 (c % 4), so the
 // 64 doubles of column group c / 4 are the A or B operand of one MFMA (lane i + 16 k).
@@ -1204,25 +1202,18 @@ __device__ __forceinline__ void factor_update(double (&v)[kTile]) {
     }
 }
 
-// Columns J.. of the factor (right-looking): lane r of a 16-lane row holds row r of the tile.
-// PSD: pivots at or below `cut` are dropped (their column of L is zero, the pseudo-inverse of a
-// semidefinite block), pivots below -10 cut fail; otherwise every pivot must be positive.
-template <int J, bool PSD>
-__device__ __forceinline__ void factor_columns(double (&v)[kTile], int r, bool& ok, double cut) {
+// Columns J.. of the factor (right-looking): lane r of a 16-lane row holds row r of the tile;
+// every pivot must be positive and finite.
+template <int J>
+__device__ __forceinline__ void factor_columns(double (&v)[kTile], int r, bool& ok) {
     if constexpr (J < kTile) {
         const double d = row_bcast<J>(v[J]);
-        double is;
-        if constexpr (PSD) {
-            if (!isfinite(d) || d < -10.0 * cut) ok = false;
-            is = d > cut ? rsqrt_nr(d) : 0.0;
-        } else {
-            if (!(d > 0) || !isfinite(d)) ok = false;
-            is = rsqrt_nr(d);
-        }
+        if (!(d > 0) || !isfinite(d)) ok = false;
+        const double is = rsqrt_nr(d);
         v[J] = r == J ? is : (r > J ? v[J] * is : 0.0);
         // rows above J hold zeros in column J; entries right of the diagonal are never read
         factor_update<J, J + 1>(v);
-        factor_columns<J + 1, PSD>(v, r, ok, cut);
+        factor_columns<J + 1>(v, r, ok);
     }
 }
 
@@ -1248,11 +1239,10 @@ __device__ __forceinline__ void inverse_rows(double (&m)[4], const double (&v)[k
 // in the four rows (the diagonal keeps 1 / L[j][j]), the inverse split over them by column
 // (c % 4 = row index of the lane group); L^-1 goes to inv in operand layout. Returns false when
 // a pivot is not positive and finite.
-template <bool PSD = false>
-__device__ bool factor_diag_tile(double (&v)[kTile], double* inv, double cut = 0.0) {
+__device__ bool factor_diag_tile(double (&v)[kTile], double* inv) {
     const int lane = threadIdx.x % 64, r = lane % kTile, g = lane / kTile;
     bool ok = true;
-    factor_columns<0, PSD>(v, r, ok, cut);
+    factor_columns<0>(v, r, ok);
     double m[4];
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) m[cc] = r == 4 * cc + g ? 1.0 : 0.0;
@@ -1323,9 +1313,7 @@ __device__ __forceinline__ void panel_solve(gdouble* Lt, double* panel, const do
 // waves for the whole factorization; the others and the diagonal tiles are read from and
 // written back to the tile store at every update. The two roles run separate step loops with
 // the same barrier sequence, so the resident tiles do not constrain the factor wave's registers.
-template <bool PSD>
-__device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, double* y, int* flag,
-                                                 double cut_first = 0.0, double cut_rest = 0.0) {
+__device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, double* y, int* flag) {
     gdouble* Lt = (gdouble*)Lt_;
     const int t = threadIdx.x, nt = blockDim.x, lane = t % 64, nw = nt / 64, nwt = nw - 1;
     const int wave = __builtin_amdgcn_readfirstlane(t / 64);
@@ -1338,19 +1326,8 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
     }
     __syncthreads();
     if (wave == 0) {
-        // diagonal tile k: factor and inverse; in the semidefinite mode the factor tile is also
-        // stored (diagonal L_jj, zero for dropped pivots) for the read-out of the square root
-        auto factor = [&](double (&v)[kTile], int k) {
-            if constexpr (!PSD) return factor_diag_tile<false>(v, dinv + k * kTileDoubles);
-            const bool ok = factor_diag_tile<true>(v, dinv + k * kTileDoubles, k == 0 ? cut_first : cut_rest);
-            if (lane < kTile) {
-                const int base = tile_base(k, k) + (lane / 4) * 64 + 16 * (lane % 4);
-#pragma unroll
-                for (int c = 0; c < kTile; ++c)
-                    Lt[base + c] = c < lane ? v[c] : (c == lane && v[c] > 0 ? 1.0 / v[c] : 0.0);
-            }
-            return ok;
-        };
+        // diagonal tile k: factor and inverse
+        auto factor = [&](double (&v)[kTile], int k) { return factor_diag_tile(v, dinv + k * kTileDoubles); };
         {
             double v[kTile];
 #pragma unroll
@@ -1595,7 +1572,7 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     if (t == 0) flag = 0;
     for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[i] : 0.0;
     __syncthreads();
-    const bool ok = cholesky_forward<false>(w.fac, T, pnl, dinv, x, &flag);
+    const bool ok = cholesky_forward(w.fac, T, pnl, dinv, x, &flag);
     if (!ok) {
         if (t == 0) w.lm->chol_ok = 0;
         return;
@@ -1625,20 +1602,13 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     // state model decrease -dx^T (g + 0.5 H dx) with the undamped H: k_backsub spreads its rows
     // over the chunk blocks; only a window without landmarks (no chunks) sums it here
     double pred = 0;
-    for (int i0 = 4 * wave; w.nchunks == 0 && i0 < n; i0 += 4 * nw) {
-        double hd[4] = {0, 0, 0, 0};
-        for (int j = lane; j < n; j += 64) {
-            const double xj = x[j];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (i0 + q < n) hd[q] += w.Hfull[(size_t)(i0 + q) * n + j] * xj;
+    if (w.nchunks == 0)
+        for (int i = wave; i < n; i += nw) {
+            double h = 0;
+            for (int j = lane; j < n; j += 64) h += w.Hfull[(size_t)i * n + j] * x[j];
+            h = wave_reduce_sum(h);
+            if (lane == 0) pred += -x[i] * (w.gfull[i] + 0.5 * h);
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double h = wave_reduce_sum(hd[q]);
-            if (lane == 0 && i0 + q < n) pred += -x[i0 + q] * (w.gfull[i0 + q] + 0.5 * h);
-        }
-    }
     const double ps = block_sum_waves(pred, red);
     double mx = 0, bad = 0;
     for (int i = t; i < n; i += nt) {
@@ -1671,29 +1641,82 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_marg_cholesky: semidefinite factor of the reordered marginal system held in the tile store
-// (eliminated state block first), right-hand side in scratch[0, N), forward solution written to
-// scratch[N, 2N). Dropped pivots leave zero columns (pseudo-inverse, Marginalizer.cpp:240-283).
+// k_sym_eigen: A = V diag(ev) V^T for one symmetric n x n matrix per workgroup (row-major, leading
+// dimension ld; A is overwritten, ev is unsorted, eigenvectors are the columns of V). Parallel
+// cyclic Jacobi: every step rotates the n/2 disjoint index pairs of one round of a round-robin
+// schedule at once, so each 2x2 block of A and each row pair of V is updated by one thread
+// without conflicts (A <- G^T A G, V <- V G); n-1 steps form a sweep; sweeps repeat until the
+// off-diagonal mass falls below 1e-32 of the total, at most 60. The marginalization's
+// pseudo-inverse of the eliminated state block and the prior's eigen cut and square root use it
+// (Marginalizer::psdSolve / compute, Marginalizer.cpp:240-283,430-466, whose GPU form is
+// MarginalCudaWorkspace::eigen, VisualCudaBatch.cu:887-957).
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kCholThreads) void k_marg_cholesky(WinDev* wins) {
-    __shared__ double pnl[2 * kMaxTiles * kTileDoubles];
-    __shared__ double dinv[kMaxTiles * kTileDoubles];
-    __shared__ double x[kMaxTiles * kTile];
-    __shared__ int flag;
-    WinDev& w = wins[blockIdx.x];
-    if (!window_active(w) || !w.lm->marginal) return;
-    const int n = w.lm->marg_n, t = threadIdx.x, nt = blockDim.x;
-    const int T = (n + kTile - 1) / kTile, N = kTile * T;
-    if (t == 0) flag = 0;
-    for (int i = t; i < N; i += nt) x[i] = i < n ? w.scratch[i] : 0.0;
+constexpr int kEigMax = 512;
+constexpr int kEigThreads = 1024;
+__global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V, double* ev, int n, int ld) {
+    __shared__ double cs[kEigMax / 2], sn[kEigMax / 2];
+    __shared__ int pp[kEigMax / 2], qq[kEigMax / 2];
+    __shared__ double red[kEigThreads / 64];
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int m = n + (n & 1), half = m / 2;  // odd n: index n is a zero dummy row/column
+    for (int i = t; i < n * n; i += nt) V[(size_t)(i / n) * ld + i % n] = (i / n == i % n) ? 1.0 : 0.0;
     __syncthreads();
-    const bool ok = cholesky_forward<true>(w.fac, T, pnl, dinv, x, &flag, w.lm->marg_cut_victim, w.lm->marg_cut_prior);
-    if (!ok) {
-        if (t == 0) w.lm->chol_ok = 0;
-        return;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = t; i < n * n; i += nt) {
+            const double v = A[(size_t)(i / n) * ld + i % n];
+            tot += v * v;
+            if (i / n != i % n) off += v * v;
+        }
+        off = block_sum_waves(off, red);
+        tot = block_sum_waves(tot, red);
+        if (off == 0 || off <= 1e-32 * tot) break;
+        for (int r = 0; r < m - 1; ++r) {
+            if (t < half) {  // round r of the circle schedule (index m-1 fixed)
+                const int a = t == 0 ? r : (r + t) % (m - 1), b = t == 0 ? m - 1 : (r - t + (m - 1)) % (m - 1);
+                const int p = min(a, b), q = max(a, b);
+                double c = 1, s = 0;
+                if (q < n) {
+                    const double apq = A[(size_t)p * ld + q];
+                    if (apq != 0) {
+                        const double theta = (A[(size_t)q * ld + q] - A[(size_t)p * ld + p]) / (2 * apq);
+                        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                        c = 1 / sqrt(tt * tt + 1);
+                        s = tt * c;
+                    }
+                }
+                pp[t] = p; qq[t] = q; cs[t] = c; sn[t] = s;
+            }
+            __syncthreads();
+            for (int blk = t; blk < half * half; blk += nt) {
+                const int P = blk / half, R = blk % half;
+                const int p = pp[P], q = qq[P], u = pp[R], v = qq[R];
+                const double c1 = cs[P], s1 = sn[P], c2 = cs[R], s2 = sn[R];
+                const bool qn = q < n, vn = v < n;
+                const double apu = A[(size_t)p * ld + u];
+                const double apv = vn ? A[(size_t)p * ld + v] : 0.0;
+                const double aqu = qn ? A[(size_t)q * ld + u] : 0.0;
+                const double aqv = qn && vn ? A[(size_t)q * ld + v] : 0.0;
+                // rows: G^T (p: c1 p - s1 q, q: s1 p + c1 q), then columns by G of pair R
+                const double bpu = c1 * apu - s1 * aqu, bpv = c1 * apv - s1 * aqv;
+                const double bqu = s1 * apu + c1 * aqu, bqv = s1 * apv + c1 * aqv;
+                A[(size_t)p * ld + u] = c2 * bpu - s2 * bpv;
+                if (vn) A[(size_t)p * ld + v] = s2 * bpu + c2 * bpv;
+                if (qn) A[(size_t)q * ld + u] = c2 * bqu - s2 * bqv;
+                if (qn && vn) A[(size_t)q * ld + v] = s2 * bqu + c2 * bqv;
+            }
+            for (int e = t; e < n * half; e += nt) {
+                const int k = e / half, P = e % half;
+                const int p = pp[P], q = qq[P];
+                if (q >= n) continue;
+                const double x = V[(size_t)k * ld + p], y = V[(size_t)k * ld + q];
+                V[(size_t)k * ld + p] = cs[P] * x - sn[P] * y;
+                V[(size_t)k * ld + q] = sn[P] * x + cs[P] * y;
+            }
+            __syncthreads();
+        }
     }
-    for (int i = t; i < N; i += nt) w.scratch[N + i] = x[i];
-    if (t == 0) w.lm->chol_ok = 1;
+    for (int i = t; i < n; i += nt) ev[i] = A[(size_t)i * ld + i];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2013,8 +2036,9 @@ hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t vibk_marg_cholesky(vib::WinDev* wins, int nwin, hipStream_t s) {
-    hipLaunchKernelGGL(vib::k_marg_cholesky, dim3(nwin), dim3(vib::kCholThreads), 0, s, wins);
+hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, hipStream_t s) {
+    if (n < 1 || n > vib::kEigMax || ld < n) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(vib::k_sym_eigen, dim3(1), dim3(vib::kEigThreads), 0, s, A, V, ev, n, ld);
     return hipGetLastError();
 }
 

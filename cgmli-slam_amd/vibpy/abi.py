@@ -90,7 +90,12 @@ class VibMarginalReport(C.Structure):
     _fields_ = [("status", C.c_int32), ("prior_reused", C.c_int32), ("eliminated_dimension", C.c_int32),
                 ("retained_dimension", C.c_int32), ("prior_rank", C.c_int32),
                 ("eliminated_landmarks", C.c_int32), ("consumed_factors", C.c_int32),
-                ("eigen_cutoff", C.c_double)]
+                ("eigen_cutoff", C.c_double), ("imu_mode", C.c_int32), ("reset_landmarks", C.c_int32),
+                ("demoted_landmarks", C.c_int32), ("dropped_factors", C.c_int32)]
+
+
+VIB_IMU_MERGE_NONE, VIB_IMU_MERGE_REINTEGRATED, VIB_IMU_MERGE_INCREMENTAL = 0, 1, 2
+VIB_IMU_MERGE_SUPPRESSED, VIB_IMU_MERGE_BRIDGE = 3, 4
 
 
 VIB_SOLVE_CONTINUOUS = 1
@@ -308,8 +313,9 @@ def load_library(path=LIB_PATH):
         "vib_solve_batch": ([vp, C.c_int32, ip, C.POINTER(VibSolveOptions), C.POINTER(VibSolveReport)], C.c_int),
         "vib_solve_latest_two": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.c_int32, C.POINTER(VibLatestTwoReport)],
                                  C.c_int),
-        "vib_marginalize": ([vp, C.c_int32, C.c_int32, dp, C.c_int32, C.POINTER(VibImuParams),
-                             C.POINTER(VibMarginalReport)], C.c_int),
+        "vib_marginalize": ([vp, C.c_int32, C.c_int32, dp, C.c_int32, C.c_double, C.c_double,
+                             C.POINTER(VibImuParams), C.POINTER(VibMarginalReport)], C.c_int),
+        "vib_read_window": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
         "vib_read_prior": ([vp, C.c_int32, ip, ip, ip, ip, lp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "vib_kernel_count": ([], C.c_int),
         "vib_kernel_name": ([C.c_int32], C.c_char_p),
@@ -328,6 +334,45 @@ def load_library(path=LIB_PATH):
 
 def f64p(a):
     return a.ctypes.data_as(dp)
+
+
+def window_buffers(K, C_, L, F, E, prior_rows, prior_cols, prior_blocks, prior_ref):
+    """Zeroed arrays of a window at the given capacities, for the calls that write a window."""
+    z = np.zeros
+    return Window(frame_id=z(K, np.int64), pose=z((K, 7)), speed_bias=z((K, 9)), rig=z(7), td=z(1),
+                  cam_rotation=z((C_, 3, 3)), cam_translation=z((C_, 3)),
+                  lm_anchor_frame=z(L, np.int32), lm_anchor_cam=z(L, np.int32), lm_bearing=z((L, 3)),
+                  lm_velocity=z((L, 3)), lm_tref=z(L), lm_rho=z(L),
+                  obs_landmark=z(F, np.int32), obs_frame=z(F, np.int32), obs_cam=z(F, np.int32),
+                  obs_bearing=z((F, 3)), obs_velocity=z((F, 3)), obs_tref=z(F), obs_sqrt_info=z((F, 3)),
+                  imu_from=z(E, np.int32), imu_to=z(E, np.int32), imu_data=z((E, IMU_STRIDE)),
+                  pose_free=z(K, np.uint8), sb_width=z(K, np.uint8), lm_free=z(L, np.uint8),
+                  prior_key_kind=z(max(prior_blocks, 1), np.int32), prior_key_id=z(max(prior_blocks, 1), np.int64),
+                  prior_ref=z(max(prior_ref, 1)), prior_A=z((max(prior_rows, 1), max(prior_cols, 1))),
+                  prior_r0=z(max(prior_rows, 1)))
+
+
+def trimmed_window(buf, cs, K, L, F, E, rows, cols, blocks, reflen):
+    """The window of `buf` (filled through its C struct `cs`) cut to the written sizes."""
+    prior = {}
+    if rows > 0:
+        prior = dict(prior_key_kind=buf.prior_key_kind[:blocks].copy(), prior_key_id=buf.prior_key_id[:blocks].copy(),
+                     prior_ref=buf.prior_ref[:reflen].copy(),
+                     prior_A=buf.prior_A.reshape(-1)[:rows * cols].reshape(rows, cols).copy(),
+                     prior_r0=buf.prior_r0[:rows].copy())
+    return Window(frame_id=buf.frame_id[:K].copy(), pose=buf.pose[:K].copy(), speed_bias=buf.speed_bias[:K].copy(),
+                  rig=buf.rig.copy(), td=buf.td.copy(), cam_rotation=buf.cam_rotation.copy(),
+                  cam_translation=buf.cam_translation.copy(), lm_anchor_frame=buf.lm_anchor_frame[:L].copy(),
+                  lm_anchor_cam=buf.lm_anchor_cam[:L].copy(), lm_bearing=buf.lm_bearing[:L].copy(),
+                  lm_velocity=buf.lm_velocity[:L].copy(), lm_tref=buf.lm_tref[:L].copy(), lm_rho=buf.lm_rho[:L].copy(),
+                  obs_landmark=buf.obs_landmark[:F].copy(), obs_frame=buf.obs_frame[:F].copy(),
+                  obs_cam=buf.obs_cam[:F].copy(), obs_bearing=buf.obs_bearing[:F].copy(),
+                  obs_velocity=buf.obs_velocity[:F].copy(), obs_tref=buf.obs_tref[:F].copy(),
+                  obs_sqrt_info=buf.obs_sqrt_info[:F].copy(), imu_from=buf.imu_from[:E].copy(),
+                  imu_to=buf.imu_to[:E].copy(), imu_data=buf.imu_data[:E].copy(), flags=int(cs.flags),
+                  huber_delta=cs.huber_delta, min_inverse_depth=cs.min_inverse_depth, td_lower=cs.td_lower,
+                  td_upper=cs.td_upper, pose_free=buf.pose_free[:K].copy(), sb_width=buf.sb_width[:K].copy(),
+                  lm_free=buf.lm_free[:L].copy(), **prior)
 
 
 def imu_params(sigma_g_c=1e-3, sigma_a_c=1e-2, sigma_gw_c=2.6867875817855036e-06,
@@ -351,6 +396,7 @@ class Context:
         if st != VIB_OK:
             raise RuntimeError(f"vib_create failed: {STATUS_NAMES.get(st, st)}")
         self.h = h
+        self._cams = {}  # cameras per uploaded slot (the shape query does not report them)
 
     def close(self):
         if self.h:
@@ -381,6 +427,7 @@ class Context:
 
     def upload(self, slot, win):
         s = win.cstruct()
+        self._cams[slot] = int(win.cam_rotation.shape[0])
         return self.check(self.lib.vib_window_upload(self.h, slot, C.byref(s)), "vib_window_upload")
 
     def release(self, slot):
@@ -454,13 +501,29 @@ class Context:
         names = [self.lib.vib_kernel_name(i).decode() for i in range(nk)]
         return it_ms, dict(zip(names, k_ms))
 
-    def marginalize(self, slot, policy, samples=None, params=None):
+    def marginalize(self, slot, policy, samples=None, t_start=0.0, t_end=0.0, params=None):
+        """vib_marginalize; SECOND_NEW takes the raw IMU samples over [t_start, t_end]."""
         rep = VibMarginalReport()
-        sp = None if samples is None else f64p(np.ascontiguousarray(samples, np.float64))
-        cnt = 0 if samples is None else int(samples.shape[0])
-        st = self.lib.vib_marginalize(self.h, slot, policy, sp, cnt,
+        smp = None if samples is None else np.ascontiguousarray(samples, np.float64)
+        cnt = 0 if smp is None else int(smp.shape[0])
+        st = self.lib.vib_marginalize(self.h, slot, policy, None if smp is None else f64p(smp), cnt, t_start, t_end,
                                       None if params is None else C.byref(params), C.byref(rep))
         return st, rep
+
+    def read_window(self, slot):
+        """The whole resident window (vib_read_window) as a Window; observations in landmark order."""
+        K, L, F, E, rows, cols = self.shape(slot)
+        ncam = int(self._cams.get(slot, 0))
+        blocks = max(cols, 1)
+        buf = window_buffers(K, ncam, L, F, E, rows, cols, blocks, 2 * cols + 2)
+        cs = buf.cstruct()
+        cs.num_frames, cs.num_cameras, cs.num_landmarks, cs.num_observations, cs.num_imu = K, ncam, L, F, E
+        cs.prior_rows = rows
+        self.check(self.lib.vib_read_window(self.h, slot, C.byref(cs)), "vib_read_window")
+        nb = int(cs.prior_blocks)
+        reflen = sum(7 if k in (VIB_KEY_POSE, VIB_KEY_RIG) else 9 if k == VIB_KEY_SPEED_BIAS else 1
+                     for k in buf.prior_key_kind[:nb])
+        return trimmed_window(buf, cs, K, L, F, E, rows, cols, nb, reflen)
 
     def read_prior(self, slot, max_rows=2048, max_cols=2048, max_blocks=256):
         rows, cols, blocks = C.c_int32(), C.c_int32(), C.c_int32()

@@ -193,9 +193,10 @@ def gyro_at(samples, t):
 def generate(K=10, L=2000, n_cams=4, obs_range=(2, 8), seed=20261204, rig_free=False, td_free=False,
              pixel_sigma=1.0, pixel_noise=None, outlier_fraction=0.0, outlier_px=20.0, speed=1.5, ill_conditioned=False,
              sigma_g_c=1e-3, sigma_a_c=1e-2, init_pos=0.05, init_rot_deg=1.0, init_rho=0.05,
-             td_init=0.002, imu_noise=True, lib=None):
+             td_init=0.002, imu_noise=True, static_bearings=False, lib=None):
     """Generate one window. Returns a vibpy.abi.Window with the initial (perturbed) estimate; the
-    ground truth is in window.meta."""
+    ground truth is in window.meta. With static_bearings the measured bearing velocities are zero
+    (a window without time-offset information, whose tracks a SECOND_NEW slide may re-anchor)."""
     lib = lib or abi.load_library()
     rng = np.random.default_rng(seed)
     cam_R, cam_t = make_rig(rng, n_cams)
@@ -298,6 +299,8 @@ def generate(K=10, L=2000, n_cams=4, obs_range=(2, 8), seed=20261204, rig_free=F
     bvel = (pdot - bt * (bt * pdot).sum(1, keepdims=True)) / rn[:, None]
     # a feature-track bearing derivative is tangent to the measured bearing itself
     bvel = bvel - bearing * (bearing * bvel).sum(1, keepdims=True)
+    if static_bearings:
+        bvel = np.zeros_like(bvel)
     # split anchors / factors
     first_idx = np.concatenate([[0], np.cumsum([len(o) for o in obs_list])[:-1]])
     is_anchor = np.zeros(len(all_pairs), bool)
@@ -362,6 +365,68 @@ def generate(K=10, L=2000, n_cams=4, obs_range=(2, 8), seed=20261204, rig_free=F
                     true_rho=true_rho, true_rig=rig_true, samples=samples, frame_t=frame_t, imu_params=params,
                     trajectory=traj)
     return win
+
+
+def camera_world(win, k, c):
+    """World pose (R, t) of camera c of slot k: T_WB T_BR T_RC."""
+    Rb = matrix_from_quat(win.pose[k, 3:])
+    Re = matrix_from_quat(win.rig[3:])
+    return Rb @ Re @ win.cam_rotation[c], Rb @ (Re @ win.cam_translation[c] + win.rig[:3]) + win.pose[k, :3]
+
+
+def reanchor_at(win, frame, min_views=2):
+    """Re-anchor the tracks that are observed in `frame` and have at least `min_views` other views
+    on their view in `frame`: the anchor takes that view's measured bearing and the inverse range
+    of the current point along it, and the former anchor ray becomes an observation carrying the
+    re-anchored view's square-root information. Returns the new window and the re-anchored
+    landmark indices (the tracks of a window whose frames were anchored in a later slot)."""
+    w = win.copy()
+    obs_of = [[] for _ in range(w.L)]
+    for f in range(w.F):
+        obs_of[w.obs_landmark[f]].append(f)
+    moved = []
+    for l in range(w.L):
+        if w.lm_anchor_frame[l] == frame:
+            continue
+        here = [f for f in obs_of[l] if w.obs_frame[f] == frame]
+        if not here or len(obs_of[l]) - len(here) + 1 < min_views + 1:
+            continue
+        f = here[0]
+        Ra, ta = camera_world(w, w.lm_anchor_frame[l], w.lm_anchor_cam[l])
+        b = w.lm_bearing[l] / np.linalg.norm(w.lm_bearing[l])
+        world = Ra @ (b / w.lm_rho[l]) + ta
+        Rc, tc = camera_world(w, frame, w.obs_cam[f])
+        ray = w.obs_bearing[f] / np.linalg.norm(w.obs_bearing[f])
+        rng_ = float(ray @ (Rc.T @ (world - tc)))
+        if not rng_ > 0:
+            continue
+        old = (w.lm_anchor_frame[l], w.lm_anchor_cam[l], w.lm_bearing[l].copy(), w.lm_velocity[l].copy(), w.lm_tref[l])
+        w.lm_anchor_frame[l], w.lm_anchor_cam[l] = frame, w.obs_cam[f]
+        w.lm_bearing[l], w.lm_velocity[l], w.lm_tref[l] = w.obs_bearing[f], w.obs_velocity[f], w.obs_tref[f]
+        w.lm_rho[l] = 1.0 / rng_
+        w.obs_frame[f], w.obs_cam[f] = old[0], old[1]
+        w.obs_bearing[f], w.obs_velocity[f], w.obs_tref[f] = old[2], old[3], old[4]
+        moved.append(l)
+    return w, np.array(moved, int)
+
+
+def drop_landmarks(win, drop):
+    """The window without the landmarks in `drop` and their observations."""
+    w = win.copy()
+    keep = np.ones(w.L, bool)
+    keep[list(drop)] = False
+    new = np.cumsum(keep) - 1
+    fk = keep[w.obs_landmark]
+    lm = dict(lm_anchor_frame=w.lm_anchor_frame[keep], lm_anchor_cam=w.lm_anchor_cam[keep],
+              lm_bearing=w.lm_bearing[keep], lm_velocity=w.lm_velocity[keep], lm_tref=w.lm_tref[keep],
+              lm_rho=w.lm_rho[keep], lm_free=None if w.lm_free is None else w.lm_free[keep])
+    ob = dict(obs_landmark=new[w.obs_landmark[fk]], obs_frame=w.obs_frame[fk], obs_cam=w.obs_cam[fk],
+              obs_bearing=w.obs_bearing[fk], obs_velocity=w.obs_velocity[fk], obs_tref=w.obs_tref[fk],
+              obs_sqrt_info=w.obs_sqrt_info[fk])
+    kw = {k: v for k, v in w.__dict__.items()}
+    kw.update(lm)
+    kw.update(ob)
+    return abi.Window(**kw)
 
 
 def quantize_fp32(win):

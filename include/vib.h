@@ -192,6 +192,13 @@ typedef struct vib_solve_report {
     double seconds;               /* wall time of the solve on the host */
 } vib_solve_report;
 
+/* IMU handling of a SECOND_NEW slide ([Window] imu_mode, SlidingWindowManager.cpp:275-311) */
+#define VIB_IMU_MERGE_NONE 0          /* no IMU edge touched the frame                         */
+#define VIB_IMU_MERGE_REINTEGRATED 1  /* merged edge re-integrated at the current bias          */
+#define VIB_IMU_MERGE_INCREMENTAL 2   /* merged edge kept the left edge's reference bias        */
+#define VIB_IMU_MERGE_SUPPRESSED 3    /* merged interval longer than 10 s: no IMU factor left   */
+#define VIB_IMU_MERGE_BRIDGE 4        /* no replay (or its SB in the prior): edges go to Schur  */
+
 typedef struct vib_marginal_report {
     int32_t status;
     int32_t prior_reused;        /* SECOND_NEW with an untouched prior                   */
@@ -201,6 +208,10 @@ typedef struct vib_marginal_report {
     int32_t eliminated_landmarks;
     int32_t consumed_factors;
     double eigen_cutoff;
+    int32_t imu_mode;            /* VIB_IMU_MERGE_* (SECOND_NEW)                          */
+    int32_t reset_landmarks;     /* re-anchored on a surviving view (reset_tracks)        */
+    int32_t demoted_landmarks;   /* one surviving ray: left the window (demoted_tracks)   */
+    int32_t dropped_factors;     /* the frame's image measurements discarded              */
 } vib_marginal_report;
 
 typedef struct vib_latest_two_report {  /* SolveReport lightweight fields (:541-562) */
@@ -287,11 +298,26 @@ int vib_copy_probe(vib_context* ctx, int64_t bytes);
 /* ---------------- marginalization ---------------- */
 /* OLD: Schur-eliminate slot 0 (pose, speed/bias) plus every landmark anchored in or observed
  * by it into a new square-root prior (Marginalizer::compute, Marginalizer.cpp:284-474).
- * SECOND_NEW: drop slot K-2's visual factors, merge its two IMU edges by re-integrating the
- * raw samples given here, reuse the prior (SlidingWindowManager.cpp:275-311). */
+ * SECOND_NEW, the redundant-frame path (SlidingWindowManager.cpp:263-482, 587-680): slot K-2's
+ * two IMU edges become one edge K-3 -> K-1 re-integrated from the raw samples given here
+ * (sample_count rows of VIB_IMU_SAMPLE_STRIDE covering [t_start, t_end], the times of slots K-3
+ * and K-1; mergeBodyImuRecords, BodyImuFactor.cpp:306-388); without samples, or when K-2's
+ * speed/bias is in the prior, the edges go into the Schur prior instead. Landmarks anchored in
+ * K-2 whose bearings do not move are re-anchored on their newest surviving view (inverse range
+ * from the projection of the current point, <= 0.03 rad off the ray), or leave the window with
+ * one surviving ray; the others are Schur-eliminated with the slot. K-2's other image
+ * measurements are discarded. When no factor is left on K-2 and the prior does not hold it, the
+ * prior is reused unchanged. Either way the resident window slides by one frame. */
 int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy,
-                    const double* imu_samples, int32_t sample_count,
+                    const double* imu_samples, int32_t sample_count, double t_start, double t_end,
                     const vib_imu_params* imu_params, vib_marginal_report* report);
+/* Copy the whole resident window (topology, measurements, state and prior) into `window`, whose
+ * sizes (num_* and prior_rows) must match vib_window_shape and whose arrays the caller
+ * allocates at those sizes (prior keys: up to prior_cols entries, prior_ref: up to
+ * 7 * prior_cols / 6 + 1 doubles); prior_blocks is written. Observations come in landmark
+ * order. Lets the host mirror the window after a slide (no counterpart in the reference, whose
+ * window manager keeps the graph on the host). */
+int vib_read_window(vib_context* ctx, int32_t slot, vib_window* window);
 /* Read the resident prior: keys, references, A (rows x cols row-major) and r0. */
 int vib_read_prior(vib_context* ctx, int32_t slot, int32_t* rows, int32_t* cols, int32_t* blocks,
                    int32_t* key_kind, int64_t* key_id, double* ref, double* A, double* r0,

@@ -1028,21 +1028,12 @@ struct MargResult {
     int cols = 0;
 };
 
-MargResult marginalizeOld(const Window& w) {
+// Schur elimination of slot `victim` (pose, speed/bias) and the landmarks of elimLm into a new
+// prior, from the consumed visual factors `visual`, IMU edges `imus` and the previous prior
+// (always included once, Marginalizer.cpp:311-316) (Marginalizer::compute, :284-474).
+MargResult marginalizeSet(const Window& w, int victim, const std::vector<char>& elimLm,
+                          const std::vector<int>& visual, const std::vector<int>& imus) {
     MargResult out;
-    const int victim = 0;
-    // eliminated landmarks: anchored in or observed by the victim
-    std::vector<char> elimLm(w.L, 0);
-    for (int l = 0; l < w.L; ++l) if (w.lmFrame[l] == victim) elimLm[l] = 1;
-    for (int f = 0; f < w.F; ++f) if (w.obsFrame[f] == victim) elimLm[w.obsLm[f]] = 1;
-    // consumed factors
-    std::vector<int> visual;
-    for (int f = 0; f < w.F; ++f) {
-        const int l = w.obsLm[f];
-        if (elimLm[l] || w.obsFrame[f] == victim || w.lmFrame[l] == victim) visual.push_back(f);
-    }
-    std::vector<int> imus;
-    for (int e = 0; e < w.E; ++e) if (w.imuFrom[e] == victim || w.imuTo[e] == victim) imus.push_back(e);
     out.consumed = (int)visual.size() + (int)imus.size() + (w.priorRows ? 1 : 0);
     // keys touched, in BlockKey order: poses(id), sbs(id), rig, td, landmarks(id)
     std::set<int> poseSlots, sbSlots;
@@ -1280,6 +1271,23 @@ MargResult marginalizeOld(const Window& w) {
     return out;
 }
 
+// OLD: the victim is slot 0 with every landmark anchored in or observed by it
+// (SlidingWindowManager.cpp:624-650 with preserveOldestVisualInformation).
+MargResult marginalizeOld(const Window& w) {
+    const int victim = 0;
+    std::vector<char> elimLm(w.L, 0);
+    for (int l = 0; l < w.L; ++l) if (w.lmFrame[l] == victim) elimLm[l] = 1;
+    for (int f = 0; f < w.F; ++f) if (w.obsFrame[f] == victim) elimLm[w.obsLm[f]] = 1;
+    std::vector<int> visual;
+    for (int f = 0; f < w.F; ++f) {
+        const int l = w.obsLm[f];
+        if (elimLm[l] || w.obsFrame[f] == victim || w.lmFrame[l] == victim) visual.push_back(f);
+    }
+    std::vector<int> imus;
+    for (int e = 0; e < w.E; ++e) if (w.imuFrom[e] == victim || w.imuTo[e] == victim) imus.push_back(e);
+    return marginalizeSet(w, victim, elimLm, visual, imus);
+}
+
 // ------------------------------------------------------------------------------------
 // IMU preintegration (ImuError::redoPreintegration, ImuError.cpp:254-462) and
 // PseudoInverse::symmSqrtU (PseudoInverse.hpp:116-145).
@@ -1508,6 +1516,253 @@ int latestTwo(Window& w, const vib_solve_options& opt, int gate, double* nis, in
 
 using namespace oracle;
 
+// ------------------------------------------------------------------------------------
+// SECOND_NEW: the redundant-frame path (SlidingWindowManager.cpp:263-482, 587-680;
+// BodyImuFactor.cpp:306-388).
+// ------------------------------------------------------------------------------------
+// Window without slot `drop` (-1: none), restricted to the kept landmarks, factors and IMU
+// edges; slots above `drop` move down by one. The prior is copied unchanged.
+Window filterWindow(const Window& w, int drop, const std::vector<char>& keepLm, const std::vector<char>& keepF,
+                    const std::vector<char>& keepE) {
+    Window o = w;
+    auto slot = [&](int k) { return drop >= 0 && k > drop ? k - 1 : k; };
+    if (drop >= 0) {
+        o.K = w.K - 1;
+        o.frameId.erase(o.frameId.begin() + drop);
+        o.pose.erase(o.pose.begin() + 7 * drop, o.pose.begin() + 7 * drop + 7);
+        o.sb.erase(o.sb.begin() + 9 * drop, o.sb.begin() + 9 * drop + 9);
+        o.poseFree.erase(o.poseFree.begin() + drop);
+        o.sbWidth.erase(o.sbWidth.begin() + drop);
+    }
+    std::vector<int> newLm(w.L, -1);
+    o.lmFrame.clear(); o.lmCam.clear(); o.lmBearing.clear(); o.lmVel.clear(); o.lmTref.clear(); o.rho.clear(); o.lmFree.clear();
+    for (int l = 0; l < w.L; ++l) {
+        if (!keepLm[l]) continue;
+        newLm[l] = (int)o.lmFrame.size();
+        o.lmFrame.push_back(slot(w.lmFrame[l])); o.lmCam.push_back(w.lmCam[l]);
+        for (int i = 0; i < 3; ++i) { o.lmBearing.push_back(w.lmBearing[3 * l + i]); o.lmVel.push_back(w.lmVel[3 * l + i]); }
+        o.lmTref.push_back(w.lmTref[l]); o.rho.push_back(w.rho[l]); o.lmFree.push_back(w.lmFree[l]);
+    }
+    o.L = (int)o.lmFrame.size();
+    o.obsLm.clear(); o.obsFrame.clear(); o.obsCam.clear(); o.obsBearing.clear(); o.obsVel.clear(); o.obsTref.clear(); o.obsSqrt.clear();
+    for (int f = 0; f < w.F; ++f) {
+        if (!keepF[f] || newLm[w.obsLm[f]] < 0 || w.obsFrame[f] == drop) continue;
+        o.obsLm.push_back(newLm[w.obsLm[f]]); o.obsFrame.push_back(slot(w.obsFrame[f])); o.obsCam.push_back(w.obsCam[f]);
+        for (int i = 0; i < 3; ++i) {
+            o.obsBearing.push_back(w.obsBearing[3 * f + i]); o.obsVel.push_back(w.obsVel[3 * f + i]);
+            o.obsSqrt.push_back(w.obsSqrt[3 * f + i]);
+        }
+        o.obsTref.push_back(w.obsTref[f]);
+    }
+    o.F = (int)o.obsLm.size();
+    o.imuFrom.clear(); o.imuTo.clear(); o.imu.clear();
+    for (int e = 0; e < w.E; ++e) {
+        if (!keepE[e] || w.imuFrom[e] == drop || w.imuTo[e] == drop) continue;
+        o.imuFrom.push_back(slot(w.imuFrom[e])); o.imuTo.push_back(slot(w.imuTo[e]));
+        o.imu.insert(o.imu.end(), w.imu.begin() + (size_t)VIB_IMU_STRIDE * e, w.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+    }
+    o.E = (int)o.imuFrom.size();
+    return o;
+}
+
+// Linear interpolation of the gyroscope at t (gyroAt, BodyImuFactor.cpp:29-38).
+V3 gyroAt(const double* samples, int count, double t) {
+    int next = 0;
+    while (next < count && samples[7 * next] < t) ++next;
+    if (next == 0) return V3(&samples[1]);
+    if (next == count) return V3(&samples[7 * (count - 1) + 1]);
+    const double t0 = samples[7 * (next - 1)], alpha = (t - t0) / (samples[7 * next] - t0);
+    return (1 - alpha) * V3(&samples[7 * (next - 1) + 1]) + alpha * V3(&samples[7 * next + 1]);
+}
+
+// World pose of camera c of slot k: T_WB T_BR T_RC.
+void cameraWorld(const Window& w, int k, int c, M3& R, V3& t) {
+    const M3 Rb = Q::fromXyzw(&w.pose[7 * k + 3]).normalized().matrix();
+    const M3 Re = Q::fromXyzw(w.rig + 3).normalized().matrix();
+    const M3 Rc = fromPtr(&w.camR[9 * c]);
+    R = Rb * Re * Rc;
+    t = Rb * (Re * V3(&w.camT[3 * c]) + V3(w.rig)) + V3(&w.pose[7 * k]);
+}
+
+struct SecondNewResult {
+    int status = VIB_OK;
+    int imuMode = VIB_IMU_MERGE_NONE;
+    int reset = 0, demoted = 0, retired = 0, dropped = 0;
+    bool priorReused = false;
+    MargResult marg;
+    Window next;
+};
+
+SecondNewResult secondNew(const Window& w0, const double* samples, int count, double t0, double t1,
+                          const vib_imu_params* params) {
+    SecondNewResult out;
+    Window w = w0;
+    const int K = w.K;
+    if (K < 3) { out.status = VIB_ERR_INVALID_ARGUMENT; return out; }
+    const int v = K - 2, left = K - 3, right = K - 1;
+    // IMU (prepareEviction, SlidingWindowManager.cpp:275-311)
+    std::vector<int> incident;
+    for (int e = 0; e < w.E; ++e) if (w.imuFrom[e] == v || w.imuTo[e] == v) incident.push_back(e);
+    bool sbInPrior = false, priorTouches = false;
+    for (int b = 0; b < w.priorBlocks; ++b)
+        if ((w.priorKind[b] == VIB_KEY_SPEED_BIAS || w.priorKind[b] == VIB_KEY_POSE) && w.priorId[b] == w.frameId[v]) {
+            priorTouches = true;
+            if (w.priorKind[b] == VIB_KEY_SPEED_BIAS) sbInPrior = true;
+        }
+    out.imuMode = incident.empty() ? VIB_IMU_MERGE_NONE : VIB_IMU_MERGE_BRIDGE;
+    std::vector<char> keepE(w.E, 1);
+    std::vector<double> merged;
+    if (incident.size() == 2 && !sbInPrior && samples && count > 0 && params) {
+        int a = incident[0], b = incident[1];
+        if (w.imuFrom[a] != left) std::swap(a, b);
+        if (w.imuFrom[a] != left || w.imuTo[a] != v || w.imuFrom[b] != v || w.imuTo[b] != right) {
+            out.status = VIB_ERR_INVALID_ARGUMENT; return out;  // "IMU merge endpoint/layout mismatch"
+        }
+        const double* da = &w.imu[(size_t)VIB_IMU_STRIDE * a];
+        const double* db = &w.imu[(size_t)VIB_IMU_STRIDE * b];
+        if (!(t1 > t0) || std::fabs((t1 - t0) - (da[0] + db[0])) > 1e-9) { out.status = VIB_ERR_INVALID_ARGUMENT; return out; }
+        if (count > 10000) { out.status = VIB_ERR_INVALID_ARGUMENT; return out; }  // maxImuSamplesPerEpoch
+        for (int i = 0; i < count; ++i) {
+            for (int c = 0; c < 7; ++c) if (!std::isfinite(samples[7 * i + c])) { out.status = VIB_ERR_INVALID_ARGUMENT; return out; }
+            if (i && !(samples[7 * i] > samples[7 * (i - 1)])) { out.status = VIB_ERR_INVALID_ARGUMENT; return out; }
+        }
+        keepE[a] = keepE[b] = 0;
+        if (t1 - t0 > 10.0) {
+            out.imuMode = VIB_IMU_MERGE_SUPPRESSED;  // maxImuIntervalSeconds: kept out of the solve
+        } else {
+            // incremental reuse keeps the left edge's reference bias while the estimate is within
+            // 0.0003 rad/s (gyro) and 0.02 m/s^2 (accel) of it (BodyImuFactor.cpp:146-170)
+            double sbLin[9];
+            std::copy_n(&w.sb[9 * left], 9, sbLin);
+            const V3 dbg = V3(&sbLin[3]) - V3(da + 25), dba = V3(&sbLin[6]) - V3(da + 28);
+            const bool incremental = norm(dbg) <= 3e-4 && norm(dba) <= 0.02;
+            if (incremental) for (int i = 0; i < 6; ++i) sbLin[3 + i] = da[25 + i];
+            const V3 g0 = gyroAt(samples, count, t0), g1 = gyroAt(samples, count, t1);
+            double gs[3], ge[3];
+            g0.store(gs); g1.store(ge);
+            merged.assign(VIB_IMU_STRIDE, 0);
+            const int st = preintegrate(samples, count, t0, t1, sbLin, *params, gs, ge, merged.data());
+            if (st != VIB_OK) { out.status = st; return out; }
+            out.imuMode = incremental ? VIB_IMU_MERGE_INCREMENTAL : VIB_IMU_MERGE_REINTEGRATED;
+        }
+    }
+    if (!merged.empty()) {
+        w.imuFrom.push_back(left); w.imuTo.push_back(right);
+        w.imu.insert(w.imu.end(), merged.begin(), merged.end());
+        w.E += 1;
+        keepE.push_back(1);
+    }
+    // landmarks anchored in the victim: reset to a surviving view, demote or retire (:312-457)
+    std::vector<std::vector<int>> ofLm(w.L);
+    for (int f = 0; f < w.F; ++f) ofLm[w.obsLm[f]].push_back(f);
+    std::vector<char> keepLm(w.L, 1), keepF(w.F, 1), retired(w.L, 0);
+    const State x = stateOf(w);
+    for (int l = 0; l < w.L; ++l) {
+        if (w.lmFrame[l] != v) continue;
+        std::vector<int> surviving;
+        bool eligible = sqnorm(V3(&w.lmVel[3 * l])) == 0;
+        for (int f : ofLm[l]) {
+            eligible = eligible && sqnorm(V3(&w.obsVel[3 * f])) == 0;
+            if (w.obsFrame[f] != v) surviving.push_back(f);
+        }
+        if (surviving.empty() || !eligible || !w.lmFree[l]) { retired[l] = 1; continue; }
+        if (surviving.size() == 1) {  // one surviving ray: demoted out of the window
+            keepLm[l] = 0;
+            for (int f : ofLm[l]) keepF[f] = 0;
+            ++out.demoted;
+            continue;
+        }
+        // preferred anchors: newer frames first, then lower camera index (all frames are keyframes here)
+        std::stable_sort(surviving.begin(), surviving.end(), [&](int a, int b) {
+            if (w.obsFrame[a] != w.obsFrame[b]) return w.obsFrame[a] > w.obsFrame[b];
+            return w.obsCam[a] < w.obsCam[b];
+        });
+        M3 Ra; V3 ta;
+        cameraWorld(w, v, w.lmCam[l], Ra, ta);
+        const V3 world = Ra * (normalized(V3(&w.lmBearing[3 * l])) / w.rho[l]) + ta;
+        int chosen = -1;
+        double newRho = 0;
+        for (int cand : surviving) {
+            M3 Rc; V3 tc;
+            cameraWorld(w, w.obsFrame[cand], w.obsCam[cand], Rc, tc);
+            const V3 delta = transpose(Rc) * (world - tc);
+            const V3 ray(&w.obsBearing[3 * cand]);
+            const double range = dot(delta, ray);
+            if (!(range > 0) || !std::isfinite(range) || 1 / range < w.minRho ||
+                std::atan2(norm(cross(delta, ray)), range) > 0.03) continue;
+            // the rebuilt factors must evaluate (the NIS threshold of the factors is disabled)
+            Window t = w;
+            t.lmFrame[l] = w.obsFrame[cand]; t.lmCam[l] = w.obsCam[cand];
+            for (int i = 0; i < 3; ++i) { t.lmBearing[3 * l + i] = w.obsBearing[3 * cand + i]; t.lmVel[3 * l + i] = w.obsVel[3 * cand + i]; }
+            t.lmTref[l] = w.obsTref[cand];
+            bool valid = true;
+            for (int f : surviving) {
+                if (f == cand) continue;
+                double r[2];
+                if (!visualEvaluate(t, f, &x.pose[7 * t.lmFrame[l]], &x.pose[7 * w.obsFrame[f]], x.rig, x.td, 1 / range, r,
+                                    nullptr, false, false, false, false, false)) { valid = false; break; }
+            }
+            if (valid) { chosen = cand; newRho = 1 / range; break; }
+        }
+        if (chosen < 0) { retired[l] = 1; continue; }
+        w.lmFrame[l] = w.obsFrame[chosen]; w.lmCam[l] = w.obsCam[chosen];
+        for (int i = 0; i < 3; ++i) { w.lmBearing[3 * l + i] = w.obsBearing[3 * chosen + i]; w.lmVel[3 * l + i] = w.obsVel[3 * chosen + i]; }
+        w.lmTref[l] = w.obsTref[chosen];
+        w.rho[l] = newRho;
+        keepF[chosen] = 0;
+        for (int f : ofLm[l]) if (w.obsFrame[f] == v) keepF[f] = 0;
+        ++out.reset;
+    }
+    // the victim's image measurements of landmarks anchored elsewhere are discarded (:458-470)
+    for (int f = 0; f < w.F; ++f)
+        if (w.obsFrame[f] == v && w.lmFrame[w.obsLm[f]] != v && keepF[f]) { keepF[f] = 0; ++out.dropped; }
+    // Schur elimination of the victim with the retired landmarks (:623-660), or prior reuse (:299-310)
+    Window w1 = filterWindow(w, -1, keepLm, keepF, keepE);
+    std::vector<char> ret1;
+    for (int l = 0; l < w.L; ++l) if (keepLm[l]) ret1.push_back(retired[l]);
+    std::vector<int> visual, imus;
+    for (int f = 0; f < w1.F; ++f) if (ret1[w1.obsLm[f]]) visual.push_back(f);
+    for (int e = 0; e < w1.E; ++e) if (w1.imuFrom[e] == v || w1.imuTo[e] == v) imus.push_back(e);
+    for (char r : ret1) out.retired += r;
+    std::vector<char> keepLm1(w1.L), keepF1(w1.F, 1), keepE1(w1.E, 1);
+    for (int l = 0; l < w1.L; ++l) keepLm1[l] = !ret1[l];
+    for (int f : visual) keepF1[f] = 0;
+    for (int e : imus) keepE1[e] = 0;
+    Window next = filterWindow(w1, v, keepLm1, keepF1, keepE1);
+    if (visual.empty() && imus.empty() && !priorTouches) {
+        out.priorReused = true;
+        out.next = next;
+        return out;
+    }
+    out.marg = marginalizeSet(w1, v, ret1, visual, imus);
+    if (out.marg.status != VIB_OK) { out.status = out.marg.status; return out; }
+    next.priorRows = out.marg.rank; next.priorBlocks = (int)out.marg.keyKind.size();
+    next.priorKind = out.marg.keyKind; next.priorId = out.marg.keyId; next.priorRef = out.marg.ref;
+    next.priorA = out.marg.A; next.priorR0 = out.marg.r0;
+    out.next = next;
+    return out;
+}
+
+// Copies a window into caller arrays (capacities checked by the caller through `sizes`).
+void storeWindow(const Window& w, vib_window* o) {
+    auto put = [](const auto& v, auto* p) { if (p && !v.empty()) std::copy(v.begin(), v.end(), const_cast<std::remove_const_t<std::remove_pointer_t<decltype(p)>>*>(p)); };
+    o->num_frames = w.K; o->num_cameras = w.C; o->num_landmarks = w.L; o->num_observations = w.F; o->num_imu = w.E;
+    o->flags = w.flags; o->huber_delta = w.huber; o->min_inverse_depth = w.minRho;
+    o->td_lower = w.tdLower; o->td_upper = w.tdUpper;
+    put(w.frameId, o->frame_id); put(w.pose, o->pose); put(w.sb, o->speed_bias);
+    put(w.poseFree, o->pose_free); put(w.sbWidth, o->sb_width);
+    std::copy_n(w.rig, 7, o->rig); *o->td = w.td;
+    put(w.camR, o->cam_rotation); put(w.camT, o->cam_translation);
+    put(w.lmFrame, o->lm_anchor_frame); put(w.lmCam, o->lm_anchor_cam); put(w.lmBearing, o->lm_bearing);
+    put(w.lmVel, o->lm_velocity); put(w.lmTref, o->lm_tref); put(w.rho, o->lm_rho); put(w.lmFree, o->lm_free);
+    put(w.obsLm, o->obs_landmark); put(w.obsFrame, o->obs_frame); put(w.obsCam, o->obs_cam);
+    put(w.obsBearing, o->obs_bearing); put(w.obsVel, o->obs_velocity); put(w.obsTref, o->obs_tref); put(w.obsSqrt, o->obs_sqrt_info);
+    put(w.imuFrom, o->imu_from); put(w.imuTo, o->imu_to); put(w.imu, o->imu_data);
+    o->prior_rows = w.priorRows; o->prior_blocks = w.priorBlocks;
+    put(w.priorKind, o->prior_key_kind); put(w.priorId, o->prior_key_id); put(w.priorRef, o->prior_ref);
+    put(w.priorA, o->prior_A); put(w.priorR0, o->prior_r0);
+}
+
 extern "C" {
 
 // Evaluate one visual factor: residual (2) and tangent Jacobian blocks at the window state:
@@ -1703,6 +1958,44 @@ int oracle_marginal_system(const vib_window* win, int* n, double* H, double* b, 
     if (H) std::copy(m.Hfull.begin(), m.Hfull.end(), H);
     if (b) std::copy(m.bfull.begin(), m.bfull.end(), b);
     if (mask) std::copy(m.elimMask.begin(), m.elimMask.end(), mask);
+    return VIB_OK;
+}
+
+// SECOND_NEW marginalization. sizes (in: capacities {L, F, E, prior cells, prior rows, prior
+// blocks}; out: {K, L, F, E, prior rows, prior cols, prior blocks, ref length}), report fields
+// {status, prior_reused, imu_mode, reset, demoted, retired, dropped, rank}; `out` receives the
+// window after the slide when every capacity suffices.
+int oracle_marginalize_second_new(const vib_window* win, const double* samples, int count, double t0, double t1,
+                                  const vib_imu_params* params, int* sizes, int* report, vib_window* out) {
+    const Window w = load(*win);
+    SecondNewResult r = secondNew(w, samples, count, t0, t1, params);
+    report[0] = r.status; report[1] = r.priorReused; report[2] = r.imuMode; report[3] = r.reset;
+    report[4] = r.demoted; report[5] = r.retired; report[6] = r.dropped; report[7] = r.marg.rank;
+    if (r.status != VIB_OK) return r.status;
+    const Window& n = r.next;
+    int cols = 0;
+    for (int k : n.priorKind) cols += k == VIB_KEY_POSE || k == VIB_KEY_RIG ? 6 : (k == VIB_KEY_SPEED_BIAS ? 9 : 1);
+    const int cap[6] = {sizes[0], sizes[1], sizes[2], sizes[3], sizes[4], sizes[5]};
+    sizes[0] = n.K; sizes[1] = n.L; sizes[2] = n.F; sizes[3] = n.E; sizes[4] = n.priorRows; sizes[5] = cols;
+    sizes[6] = n.priorBlocks; sizes[7] = (int)n.priorRef.size();
+    if (out && n.L <= cap[0] && n.F <= cap[1] && n.E <= cap[2] && (size_t)n.priorRows * cols <= (size_t)cap[3] &&
+        n.priorRows <= cap[4] && n.priorBlocks <= cap[5])
+        storeWindow(n, out);
+    return VIB_OK;
+}
+
+// The whole linear system of the factors a SECOND_NEW slide consumes (as oracle_marginal_system;
+// *n = 0 when the prior is reused).
+int oracle_second_new_system(const vib_window* win, const double* samples, int count, double t0, double t1,
+                             const vib_imu_params* params, int* n, double* H, double* b, int* mask) {
+    const Window w = load(*win);
+    SecondNewResult r = secondNew(w, samples, count, t0, t1, params);
+    if (r.status != VIB_OK) return r.status;
+    *n = r.priorReused ? 0 : (int)r.marg.bfull.size();
+    if (r.priorReused) return VIB_OK;
+    if (H) std::copy(r.marg.Hfull.begin(), r.marg.Hfull.end(), H);
+    if (b) std::copy(r.marg.bfull.begin(), r.marg.bfull.end(), b);
+    if (mask) std::copy(r.marg.elimMask.begin(), r.marg.elimMask.end(), mask);
     return VIB_OK;
 }
 

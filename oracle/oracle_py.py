@@ -42,6 +42,10 @@ def load(abi):
                                             C.POINTER(abi.VibSolveReport)]
     lib.oracle_marginalize_old.argtypes = [W, ip, dp, dp, dp, dp, ip, lp, dp, dp]
     lib.oracle_marginal_system.argtypes = [W, ip, dp, dp, ip]
+    lib.oracle_marginalize_second_new.argtypes = [W, dp, C.c_int, C.c_double, C.c_double,
+                                                  C.POINTER(abi.VibImuParams), ip, ip, W]
+    lib.oracle_second_new_system.argtypes = [W, dp, C.c_int, C.c_double, C.c_double, C.POINTER(abi.VibImuParams),
+                                             ip, dp, dp, ip]
     lib.oracle_preintegrate.argtypes = [dp, C.c_int, C.c_double, C.c_double, dp, C.POINTER(abi.VibImuParams), dp, dp, dp]
     lib.oracle_pinhole_observation.argtypes = [C.c_double] * 7 + [dp, dp]
     lib.oracle_eigen_sym.argtypes = [dp, C.c_int, dp, dp]
@@ -168,6 +172,49 @@ class Oracle:
         b = np.zeros(n.value)
         mask = np.zeros(n.value, np.int32)
         st = self.lib.oracle_marginal_system(C.byref(s), C.byref(n), _p(H), _p(b), mask.ctypes.data_as(C.POINTER(C.c_int32)))
+        return st, dict(H=H, b=b, mask=mask.astype(bool))
+
+    def marginalize_second_new(self, win, samples, t0, t1, params):
+        """SECOND_NEW slide of the second-newest slot. Returns status, report dict and the window
+        after the slide (None on failure)."""
+        abi = self.abi
+        nr = win.layout()["nr"]
+        cap = nr + 16
+        blocks = 2 * win.K + 2
+        buf = abi.window_buffers(win.K - 1, int(win.cam_rotation.shape[0]), win.L, win.F, win.E, cap, cap, blocks,
+                                 7 * blocks)
+        out = buf.cstruct()
+        s = win.cstruct()
+        sizes = np.array([win.L, win.F, win.E, cap * cap, cap, blocks, 0, 0], np.int32)
+        rep = np.zeros(8, np.int32)
+        smp = None if samples is None else np.ascontiguousarray(samples, np.float64)
+        ipp = C.POINTER(C.c_int32)
+        st = self.lib.oracle_marginalize_second_new(C.byref(s), None if smp is None else _p(smp),
+                                                    0 if smp is None else smp.shape[0], t0, t1,
+                                                    None if params is None else C.byref(params),
+                                                    sizes.ctypes.data_as(ipp), rep.ctypes.data_as(ipp), C.byref(out))
+        report = dict(status=int(rep[0]), prior_reused=bool(rep[1]), imu_mode=int(rep[2]), reset=int(rep[3]),
+                      demoted=int(rep[4]), retired=int(rep[5]), dropped=int(rep[6]), rank=int(rep[7]))
+        if st != 0:
+            return st, report, None
+        K, L, F, E, rows, cols, nb, reflen = (int(v) for v in sizes)
+        return st, report, abi.trimmed_window(buf, out, K, L, F, E, rows, cols, nb, reflen)
+
+    def second_new_system(self, win, samples, t0, t1, params):
+        """Whole system (H, b) of the factors a SECOND_NEW slide consumes, eliminated mask (None
+        when the prior is reused)."""
+        s = win.cstruct()
+        smp = None if samples is None else np.ascontiguousarray(samples, np.float64)
+        n = C.c_int32()
+        args = (C.byref(s), None if smp is None else _p(smp), 0 if smp is None else smp.shape[0], t0, t1,
+                None if params is None else C.byref(params))
+        st = self.lib.oracle_second_new_system(*args, C.byref(n), None, None, None)
+        if st != 0 or n.value == 0:
+            return st, None
+        H = np.zeros((n.value, n.value))
+        b = np.zeros(n.value)
+        mask = np.zeros(n.value, np.int32)
+        st = self.lib.oracle_second_new_system(*args, C.byref(n), _p(H), _p(b), mask.ctypes.data_as(C.POINTER(C.c_int32)))
         return st, dict(H=H, b=b, mask=mask.astype(bool))
 
     def preintegrate(self, samples, t0, t1, sb, params, g0, g1):

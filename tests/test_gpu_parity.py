@@ -254,3 +254,91 @@ def test_marginalize_old_matches_oracle(ctx, oracle, abi):
     say("[gpu] solve after marginalization")
     st, srep = ctx.solve(0, abi.options(max_iterations=3))
     assert st == 0, ctx.error()
+
+
+def _second_new_case(name):
+    from vibpy import synth
+    if name == "retire":
+        w = synth.generate(K=6, L=300, n_cams=4, obs_range=(2, 7), seed=54, rig_free=True, td_free=True)
+        w, _ = synth.reanchor_at(w, w.K - 2)
+        return w, True
+    if name == "reset":
+        w = synth.generate(K=7, L=400, n_cams=4, obs_range=(2, 8), seed=53, static_bearings=True)
+        w, _ = synth.reanchor_at(w, w.K - 2)
+        v = w.K - 2
+        seen = {}
+        for f in range(w.F):
+            if w.obs_frame[f] != v:
+                seen[w.obs_landmark[f]] = seen.get(w.obs_landmark[f], 0) + 1
+        # tracks anchored in the frame with no surviving view would be retired into the prior
+        drop = [l for l in range(w.L) if w.lm_anchor_frame[l] == v and seen.get(l, 0) == 0]
+        return synth.drop_landmarks(w, drop), True
+    if name == "reuse":
+        return synth.generate(K=8, L=600, n_cams=4, obs_range=(2, 8), seed=41, static_bearings=True), True
+    w = synth.generate(K=6, L=250, n_cams=3, obs_range=(2, 6), seed=55)
+    return w, False
+
+
+def _canonical_obs(w):
+    order = np.lexsort((w.obs_cam, w.obs_frame, w.obs_landmark))
+    return {k: getattr(w, k)[order] for k in ("obs_landmark", "obs_frame", "obs_cam", "obs_bearing", "obs_velocity",
+                                               "obs_tref", "obs_sqrt_info")}
+
+
+@pytest.mark.parametrize("name", ["retire", "reset", "reuse", "bridge"])
+def test_second_new_matches_oracle(ctx, oracle, abi, name):
+    """SECOND_NEW on the device vs the oracle: the same IMU merge, track re-anchoring, demotion
+    and retirement; the window after the slide agrees exactly in topology, within 1e-12 in the
+    re-anchored inverse ranges and 1e-9 in the re-integrated IMU record; the prior is the same
+    (reused bit for bit, or A^T A and A^T r0 within 1e-6 on the oracle's kept subspace); both
+    windows then solve to the same state."""
+    w, with_samples = _second_new_case(name)
+    m = w.meta
+    ft = m["frame_t"]
+    args = (m["samples"], float(ft[w.K - 3]), float(ft[w.K - 1]), m["imu_params"]) if with_samples else \
+        (None, 0.0, 0.0, None)
+    st, orep, nxt = oracle.marginalize_second_new(w, *args)
+    assert st == 0
+    say(f"[gpu] SECOND_NEW {name}: K={w.K} L={w.L} F={w.F}")
+    ctx.upload(0, w)
+    st, rep = ctx.marginalize(0, abi.VIB_MARGIN_SECOND_NEW, *args[:3], params=args[3]) if with_samples else \
+        ctx.marginalize(0, abi.VIB_MARGIN_SECOND_NEW)
+    assert st == 0, ctx.error()
+    assert rep.imu_mode == orep["imu_mode"] and bool(rep.prior_reused) == orep["prior_reused"]
+    assert rep.reset_landmarks == orep["reset"] and rep.demoted_landmarks == orep["demoted"]
+    assert rep.eliminated_landmarks == orep["retired"] and rep.dropped_factors == orep["dropped"]
+    dev = ctx.read_window(0)
+    assert (dev.K, dev.L, dev.F, dev.E) == (nxt.K, nxt.L, nxt.F, nxt.E)
+    for k in ("lm_anchor_frame", "lm_anchor_cam", "lm_bearing", "lm_velocity", "imu_from", "imu_to", "frame_id"):
+        np.testing.assert_array_equal(getattr(dev, k), getattr(nxt, k), err_msg=k)
+    np.testing.assert_allclose(dev.lm_rho, nxt.lm_rho, rtol=1e-12)
+    np.testing.assert_allclose(dev.imu_data, nxt.imu_data, rtol=1e-9, atol=1e-12)
+    cd, co = _canonical_obs(dev), _canonical_obs(nxt)
+    for k in cd:
+        np.testing.assert_array_equal(cd[k], co[k], err_msg=k)
+    if orep["prior_reused"]:
+        np.testing.assert_array_equal(dev.prior_A, nxt.prior_A)
+        np.testing.assert_array_equal(dev.prior_r0, nxt.prior_r0)
+    else:
+        co_, no = _prior_columns(nxt.prior_key_kind, nxt.prior_key_id)
+        cd_, nd = _prior_columns(dev.prior_key_kind, dev.prior_key_id)
+        assert set(co_) == set(cd_) and no == nd
+        perm = np.concatenate([np.arange(cd_[k][0], cd_[k][0] + cd_[k][1]) for k in co_])
+        Ad = dev.prior_A[:, perm]
+        Gd, hd = Ad.T @ Ad, Ad.T @ dev.prior_r0
+        Ao = nxt.prior_A
+        Go, ho = Ao.T @ Ao, Ao.T @ nxt.prior_r0
+        P = np.linalg.pinv(Ao) @ Ao
+        assert np.linalg.norm(P @ Gd @ P - Go) <= 1e-6 * np.linalg.norm(Go)
+        assert np.linalg.norm(P @ hd - ho) <= 1e-6 * np.linalg.norm(ho)
+    say(f"[gpu] solve after SECOND_NEW {name}")
+    opts = abi.options(max_iterations=3)
+    st, srep = ctx.solve(0, opts)
+    assert st == 0, ctx.error()
+    got = ctx.read_window(0)
+    exp = nxt.copy()
+    ost, orep2 = oracle.solve(exp, opts)
+    assert ost == 0
+    assert abs(srep.final_cost - orep2.final_cost) <= 1e-8 * orep2.final_cost
+    assert np.abs(got.pose - exp.pose).max() < 1e-6
+    assert np.abs(got.lm_rho - exp.lm_rho).max() < 1e-6 * max(1.0, np.abs(exp.lm_rho).max())

@@ -446,3 +446,164 @@ def test_marginalize_old_is_the_schur_complement(oracle, abi):
         base = -0.5 * b[e] @ np.linalg.solve(Hmm, b[e])
         prior = 0.5 * np.sum((r0 + A @ dr) ** 2) - 0.5 * np.sum(r0 ** 2) + base
         assert abs(full - prior) <= 1e-7 * max(1.0, abs(full))
+
+
+# ---------------------------------------------------------------------------------------------
+# SECOND_NEW (SlidingWindowManager.cpp:263-482, 587-680; BodyImuFactor.cpp:306-388)
+# ---------------------------------------------------------------------------------------------
+def _slide_args(w):
+    m = w.meta
+    ft = m["frame_t"]
+    return m["samples"], float(ft[w.K - 3]), float(ft[w.K - 1]), m["imu_params"]
+
+
+def _qmul(a, b):  # Hamilton product, xyzw storage
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz])
+
+
+def _edge(w, a, b):
+    idx = [e for e in range(w.E) if w.imu_from[e] == a and w.imu_to[e] == b]
+    assert len(idx) == 1
+    return w.imu_data[idx[0]]
+
+
+def test_second_new_merged_imu_edge_composes_the_two_edges(oracle, abi):
+    """With the bias estimate at the edges' reference the merged edge keeps it (incremental
+    mode) and, the middle frame time being a sample time, midpoint integration over the joined
+    samples equals the composition of the two edges: dq = dq_a dq_b, dv = dv_a + R_a dv_b,
+    dp = dp_a + dv_a dt_b + R_a dp_b."""
+    from vibpy import synth
+    w = synth.generate(K=6, L=150, n_cams=2, obs_range=(2, 5), seed=51)
+    samples, t0, t1, params = _slide_args(w)
+    tv = float(w.meta["frame_t"][w.K - 2])
+    assert np.min(np.abs(samples[:, 0] - tv)) < 1e-12
+    st, rep, nxt = oracle.marginalize_second_new(w, samples, t0, t1, params)
+    assert st == 0 and rep["imu_mode"] == abi.VIB_IMU_MERGE_INCREMENTAL
+    K = w.K
+    da, db = _edge(w, K - 3, K - 2), _edge(w, K - 2, K - 1)
+    dm = _edge(nxt, K - 3, K - 2)
+    assert nxt.E == w.E - 1
+    assert abs(dm[0] - (da[0] + db[0])) < 1e-12
+    np.testing.assert_allclose(dm[25:31], da[25:31], atol=0)
+    Ra = synth.matrix_from_quat(da[15:19])
+    q = _qmul(da[15:19], db[15:19])
+    q = q if np.dot(q, dm[15:19]) > 0 else -q
+    np.testing.assert_allclose(dm[15:19], q, atol=1e-12)
+    np.testing.assert_allclose(dm[19:22], da[19:22] + Ra @ db[19:22], atol=1e-11)
+    np.testing.assert_allclose(dm[22:25], da[22:25] + da[19:22] * db[0] + Ra @ db[22:25], atol=1e-11)
+    # the other edges move down with their frames, unchanged
+    for k in range(K - 3):
+        np.testing.assert_array_equal(_edge(nxt, k, k + 1), _edge(w, k, k + 1))
+
+
+def test_second_new_reintegrates_at_a_moved_bias(oracle, abi):
+    """A gyro-bias estimate 1e-3 rad/s away from the reference (> 3e-4) re-integrates the joined
+    interval at the current estimate, with the gyroscope at both frame times interpolated from
+    the samples (gyroAt)."""
+    from vibpy import synth
+    w = synth.generate(K=6, L=150, n_cams=2, obs_range=(2, 5), seed=52)
+    w.speed_bias[w.K - 3, 3:6] += 1e-3
+    samples, t0, t1, params = _slide_args(w)
+    st, rep, nxt = oracle.marginalize_second_new(w, samples, t0, t1, params)
+    assert st == 0 and rep["imu_mode"] == abi.VIB_IMU_MERGE_REINTEGRATED
+    dm = _edge(nxt, w.K - 3, w.K - 2)
+    np.testing.assert_allclose(dm[25:31], w.speed_bias[w.K - 3, 3:9], atol=0)
+    st, direct = oracle.preintegrate(samples, t0, t1, w.speed_bias[w.K - 3], params,
+                                     synth.gyro_at(samples, t0), synth.gyro_at(samples, t1))
+    assert st == 0
+    np.testing.assert_allclose(dm, direct, rtol=1e-13, atol=1e-15)
+    # no samples: the two edges go into the Schur prior with the frame
+    st, rep, nxt = oracle.marginalize_second_new(w, None, 0.0, 0.0, None)
+    assert st == 0 and rep["imu_mode"] == abi.VIB_IMU_MERGE_BRIDGE and not rep["prior_reused"]
+    assert nxt.E == w.E - 2
+
+
+def test_second_new_reanchors_demotes_and_reuses_the_prior(oracle, abi):
+    """Static tracks anchored in the second-newest frame move to their newest surviving view
+    (lowest camera first) with rho = 1 / (ray . p_C) of the current point; a track with one
+    surviving ray leaves the window; nothing is left on the frame, so the prior is reused as it
+    is; the frame's other measurements are discarded."""
+    from vibpy import synth
+    w0 = synth.generate(K=7, L=400, n_cams=4, obs_range=(2, 8), seed=53, static_bearings=True)
+    v = w0.K - 2
+    w, moved = synth.reanchor_at(w0, v)
+    assert len(moved) > 20
+    obs_of = [[] for _ in range(w.L)]
+    for f in range(w.F):
+        obs_of[w.obs_landmark[f]].append(f)
+    anchored = [l for l in range(w.L) if w.lm_anchor_frame[l] == v]
+    surv = {l: [f for f in obs_of[l] if w.obs_frame[f] != v] for l in anchored}
+    choice = {}
+    for l in anchored:
+        if len(surv[l]) < 2:
+            continue
+        Ra, ta = synth.camera_world(w, v, w.lm_anchor_cam[l])
+        b = w.lm_bearing[l] / np.linalg.norm(w.lm_bearing[l])
+        world = Ra @ (b / w.lm_rho[l]) + ta
+        for f in sorted(surv[l], key=lambda f: (-w.obs_frame[f], w.obs_cam[f])):
+            Rc, tc = synth.camera_world(w, w.obs_frame[f], w.obs_cam[f])
+            d = Rc.T @ (world - tc)
+            ray = w.obs_bearing[f]
+            r = float(d @ ray)
+            if r > 0 and np.arctan2(np.linalg.norm(np.cross(d, ray)), r) <= 0.03:
+                choice[l] = (f, r)
+                break
+    # tracks that would be Schur-retired (no surviving view, or no acceptable one) are removed
+    # first, so that nothing is left on the frame
+    retired = [l for l in anchored if len(surv[l]) != 1 and l not in choice]
+    assert len(choice) > 20
+    w2 = synth.drop_landmarks(w, retired)
+    st, rep, nxt = oracle.marginalize_second_new(w2, *_slide_args(w2))
+    assert st == 0 and rep["prior_reused"]
+    np.testing.assert_array_equal(nxt.prior_A, w2.prior_A)
+    np.testing.assert_array_equal(nxt.prior_r0, w2.prior_r0)
+    np.testing.assert_array_equal(nxt.prior_key_id, w2.prior_key_id)
+    demoted = {l for l in anchored if len(surv[l]) == 1}
+    keep = [l for l in range(w.L) if l not in demoted and l not in retired]
+    removed_f = sum(1 for f in range(w2.F) if w2.obs_frame[f] == v) + sum(len(surv[l]) for l in demoted)
+    for l, (f, r) in choice.items():
+        i = keep.index(l)
+        fr = w.obs_frame[f] - (1 if w.obs_frame[f] > v else 0)
+        assert nxt.lm_anchor_frame[i] == fr and nxt.lm_anchor_cam[i] == w.obs_cam[f]
+        np.testing.assert_array_equal(nxt.lm_bearing[i], w.obs_bearing[f])
+        assert abs(nxt.lm_rho[i] - 1.0 / r) <= 1e-12 / r
+        removed_f += 1
+    assert rep["reset"] == len(choice) and rep["demoted"] == len(demoted) and rep["retired"] == 0
+    assert nxt.K == w.K - 1 and nxt.L == len(keep) and nxt.F == w2.F - removed_f
+    assert not np.any(nxt.obs_frame >= nxt.K)
+
+
+def _check_prior_is_schur(H, b, e, A, r0):
+    r = ~e
+    Hmm, Hmr, Hrr = H[np.ix_(e, e)], H[np.ix_(e, r)], H[np.ix_(r, r)]
+    X = np.linalg.solve(Hmm, np.column_stack([Hmr, b[e]]))
+    S = Hrr - Hmr.T @ X[:, :-1]
+    g = b[r] - Hmr.T @ X[:, -1]
+    lam, U = np.linalg.eigh(0.5 * (S + S.T))
+    keep = lam > max(1e-12, 1e-9 * lam[-1])
+    assert A.shape == (keep.sum(), S.shape[0])
+    P = U[:, keep] @ U[:, keep].T
+    assert np.linalg.norm(A.T @ A - P @ S @ P) <= 1e-8 * np.linalg.norm(S)
+    assert np.linalg.norm(A.T @ r0 - P @ g) <= 1e-8 * np.linalg.norm(g)
+
+
+def test_second_new_retires_moving_tracks_into_the_schur_prior(oracle, abi):
+    """Tracks anchored in the frame whose bearings move cannot be re-anchored: they are
+    eliminated with the frame, and the new prior's quadratic is the Schur complement of that
+    block in the system of the consumed factors and the previous prior."""
+    from vibpy import synth
+    w = synth.generate(K=6, L=300, n_cams=4, obs_range=(2, 7), seed=54, rig_free=True, td_free=True)
+    v = w.K - 2
+    w, _ = synth.reanchor_at(w, v)
+    args = _slide_args(w)
+    st, rep, nxt = oracle.marginalize_second_new(w, *args)
+    anchored = [l for l in range(w.L) if w.lm_anchor_frame[l] == v]
+    assert st == 0 and not rep["prior_reused"] and rep["reset"] == 0
+    assert rep["retired"] == len(anchored) > 0
+    st, sysm = oracle.second_new_system(w, *args)
+    assert st == 0
+    _check_prior_is_schur(sysm["H"], sysm["b"], sysm["mask"], nxt.prior_A, nxt.prior_r0)
+    assert nxt.L == w.L - len(anchored) and nxt.E == w.E - 1
