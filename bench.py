@@ -83,28 +83,58 @@ def batch_windows(n, procs):
 
 
 def batched_line(ctx, abi, wins, steps, warmup):
-    """Batched throughput: all windows in one graph-captured launch sequence per LM iteration,
-    against the same windows solved one after another (one window per launch sequence)."""
+    """Batched throughput: all windows in one graph-captured launch sequence per LM iteration
+    (FP64, then mixed precision), against the same windows solved one after another in FP64
+    (one window per launch sequence)."""
     slots = list(range(1000, 1000 + len(wins)))
     for s, w in zip(slots, wins):
         ctx.upload(s, w)
+    res = {}
+    for name, prec in (("fp64", abi.VIB_FP64), ("mixed", abi.VIB_MIXED)):
+        opts = abi.options(max_iterations=1, precision=prec, flags=abi.VIB_SOLVE_CONTINUOUS)
+        ctx.bench(slots, opts, warmup, kernels=False)
+        ctx.synchronize()
+        it_ms, _ = ctx.bench(slots, opts, steps, kernels=False)
+        ctx.synchronize()
+        res[name] = it_ms
     opts = abi.options(max_iterations=1, flags=abi.VIB_SOLVE_CONTINUOUS)
-    ctx.bench(slots, opts, warmup, kernels=False)
-    ctx.synchronize()
-    it_ms, _ = ctx.bench(slots, opts, steps, kernels=False)
-    ctx.synchronize()
     seq = []
     for s in slots:
         ms, _ = ctx.bench([s], opts, 3, kernels=False)
         seq.append(float(np.median(ms)))
     for s in slots:
         ctx.release(s)
-    batch_ms = float(np.median(it_ms))
+    batch_ms = float(np.median(res["fp64"]))
+    mixed_ms = float(np.median(res["mixed"]))
     seq_ms = float(np.sum(seq))
     return dict(windows=len(wins), factors=int(sum(w.F for w in wins)), landmarks=int(sum(w.L for w in wins)),
                 value=len(wins) / (batch_ms * 1e-3), unit="window-steps/s", ms_per_batched_step=batch_ms,
-                p99_ms=float(np.percentile(it_ms, 99)), sequential_value=len(wins) / (seq_ms * 1e-3),
-                sequential_ms=seq_ms, speedup_vs_sequential=seq_ms / batch_ms)
+                p99_ms=float(np.percentile(res["fp64"], 99)), mixed_value=len(wins) / (mixed_ms * 1e-3),
+                mixed_ms_per_batched_step=mixed_ms, mixed_p99_ms=float(np.percentile(res["mixed"], 99)),
+                sequential_value=len(wins) / (seq_ms * 1e-3), sequential_ms=seq_ms,
+                speedup_vs_sequential=seq_ms / batch_ms, mixed_speedup_vs_sequential_fp64=seq_ms / mixed_ms)
+
+
+def step_error(ctx, abi, win, slot, lam=1e-4, refinement=2):
+    """One LM step of `win` from its initial state at identical lambda in FP64 and in mixed
+    precision: ||dx_mixed - dx_fp64|| / ||dx_fp64|| over the full tangent step (states and
+    inverse ranges), and the relative differences of the linearization-point and trial costs."""
+    lay = win.layout()
+    n = lay["nr"] + lay["ne"]
+    out = {}
+    for name, prec in (("fp64", abi.VIB_FP64), ("mixed", abi.VIB_MIXED)):
+        ctx.upload(slot, win)
+        ctx.check(ctx.linearize(slot, prec), "vib_linearize")
+        st, rep = ctx.solve_step(slot, lam, prec, refinement)
+        ctx.check(st, "vib_solve_step")
+        out[name] = (ctx.read_step(slot, n), rep)
+    ctx.release(slot)
+    d64, r64 = out["fp64"]
+    d32, r32 = out["mixed"]
+    return dict(dx_rel_err=float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
+                cost_rel_err=abs(r32.cost - r64.cost) / r64.cost,
+                trial_cost_rel_err=abs(r32.trial_cost - r64.trial_cost) / r64.trial_cost,
+                lam=lam, refinement_steps=refinement)
 
 
 def cpu_baseline(win, seconds=12.0):
@@ -135,7 +165,6 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--batch-windows", type=int, default=256,
@@ -167,6 +196,20 @@ def main():
     print(f"[bench] timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
     _, k_ms = ctx.bench([0], opts, max(5, min(args.steps, 20)), kernels=True)
     ctx.synchronize()
+    # the same window in mixed precision, then the mixed-vs-FP64 step error on it and on the
+    # ill-conditioned window (depths 0.5-200 m, 30% low-parallax tracks, a near-static stretch)
+    mopts = abi.options(max_iterations=1, precision=abi.VIB_MIXED, flags=abi.VIB_SOLVE_CONTINUOUS)
+    ctx.bench([0], mopts, args.warmup, kernels=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    mit_ms, _ = ctx.bench([0], mopts, args.steps, kernels=False)
+    ctx.synchronize()
+    mwall = time.perf_counter() - t0
+    _, mk_ms = ctx.bench([0], mopts, max(5, min(args.steps, 20)), kernels=True)
+    ctx.synchronize()
+    mixed = dict(value=args.steps / mwall, unit="window-steps/s", ms_per_step=1e3 * mwall / args.steps,
+                 p50_ms=float(np.percentile(mit_ms, 50)), p99_ms=float(np.percentile(mit_ms, 99)), kernel_ms=mk_ms)
+    mixed["accuracy"] = {args.workload: step_error(ctx, abi, win, 1), "c3": step_error(ctx, abi, synth.generate(**synth.CONFIGS["c3"]), 1)}
     batched = None
     if bwins:
         print(f"[bench] batched workload: {len(bwins)} windows", file=sys.stderr, flush=True)
@@ -211,10 +254,11 @@ def main():
                    "algorithmic_bytes_per_step": algorithmic_bytes(win)},
         "kernel_ms": k_ms,
         "roofline": roof,
+        "mixed": mixed,
     }
     if batched is not None:
         out["batched"] = dict(batched, workload=f"{batched['windows']} independent windows, K 5-20, L 500-30000, "
-                                               "2-4 cameras, 2-10 observations per landmark, 5% outliers at 20 px, FP64")
+                                               "2-4 cameras, 2-10 observations per landmark, 5% outliers at 20 px")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(win, args.cpu_seconds)
     print(json.dumps(out))

@@ -30,7 +30,7 @@ extern "C" {
 const char* vibk_name(int k);
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, const vib::Chunk* lin,
                        int nlin, double** red, int max_len, int max_edges, int max_landmarks, int refinement, int mode,
-                       hipStream_t s);
+                       int mixed, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, hipStream_t s);
@@ -452,7 +452,7 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
 
 bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
     return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_lin, nlin, d_red, max_len, max_edges, max_landmarks, refinement,
-                       mode, s) == hipSuccess;
+                       mode, mixed, s) == hipSuccess;
 }
 
 bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(1, s, 0, 0) && launch(2, s, 0, 0); }
@@ -529,7 +529,7 @@ bool write_lm(vib_context* ctx, Slot& s, const LmState& lm) {
 // skipped on the device. Returns the per-window reports.
 int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options& o, std::vector<vib_solve_report>& reps) {
     const auto t0 = std::chrono::steady_clock::now();
-    if (o.precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    if (o.precision != VIB_FP64 && o.precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     const int maxit = std::max(1, o.max_iterations);
     for (Slot* s : slots) {
         LmState lm{};
@@ -539,6 +539,7 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
         if (!write_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
     }
     Batch batch;
+    batch.mixed = o.precision == VIB_MIXED;
     std::string err;
     if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     const bool graph = ctx->use_graph && batch.capture(ctx->stream, o.refinement_steps);
@@ -1485,7 +1486,7 @@ int vib_retained_dimension(vib_context* ctx, int32_t slot, int32_t* nr, int32_t*
 
 int vib_linearize(vib_context* ctx, int32_t slot, int32_t precision) {
     if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
-    if (precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    if (precision != VIB_FP64 && precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     std::lock_guard<std::mutex> lock(ctx->mutex);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
@@ -1494,26 +1495,30 @@ int vib_linearize(vib_context* ctx, int32_t slot, int32_t precision) {
     lm.new_point = 1; lm.status = 0; lm.iteration = 0;
     std::vector<Slot*> one{s};
     Batch b;
+    b.mixed = precision == VIB_MIXED;
     std::string err;
     if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, err) || !b.linearize(ctx->stream) ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(ctx, VIB_ERR_DEVICE, "linearization launch failed");
+    s->lin_precision = precision;
     return VIB_OK;
 }
 
 int vib_solve_step(vib_context* ctx, int32_t slot, double lambda, int32_t precision, int32_t refinement,
                    vib_step_report* rep) {
     if (!ctx || !rep || !(lambda > 0) || !std::isfinite(lambda)) return VIB_ERR_INVALID_ARGUMENT;
-    if (precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
     std::lock_guard<std::mutex> lock(ctx->mutex);
     *rep = vib_step_report{};
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    if (precision != s->lin_precision)
+        return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "step precision differs from the linearization's (vib_linearize)");
     LmState lm;
     if (!read_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state read failed");
     lm.lambda = lambda; lm.status = 0; lm.iteration = 0;
     std::vector<Slot*> one{s};
     Batch b;
+    b.mixed = precision == VIB_MIXED;
     std::string err;
     if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, err) || !b.step(ctx->stream, refinement, 1) ||
         !read_lm(ctx, *s, lm))
@@ -1636,7 +1641,7 @@ const char* vib_kernel_name(int32_t k) { return vibk_name(k); }
 int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_solve_options* o, int32_t iterations,
               double* iteration_ms, double* kernel_ms) {
     if (!ctx || !o || !ids || count <= 0 || iterations <= 0) return VIB_ERR_INVALID_ARGUMENT;
-    if (o->precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    if (o->precision != VIB_FP64 && o->precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     std::lock_guard<std::mutex> lock(ctx->mutex);
     hipSetDevice(ctx->device);
     std::vector<Slot*> slots;
@@ -1652,6 +1657,7 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
         if (!write_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
     }
     Batch batch;
+    batch.mixed = o->precision == VIB_MIXED;
     std::string err;
     if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     if (!batch.capture(ctx->stream, o->refinement_steps)) return fail(ctx, VIB_ERR_DEVICE, "graph capture failed");
@@ -1697,7 +1703,7 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
 int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* o, int32_t gate,
                          vib_latest_two_report* rep) {
     if (!ctx || !o || !rep) return VIB_ERR_INVALID_ARGUMENT;
-    if (o->precision != VIB_FP64) return fail(ctx, VIB_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    if (o->precision != VIB_FP64 && o->precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     std::lock_guard<std::mutex> lock(ctx->mutex);
     hipSetDevice(ctx->device);
     *rep = vib_latest_two_report{};

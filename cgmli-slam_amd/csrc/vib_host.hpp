@@ -72,6 +72,7 @@ struct Slot {
     size_t bytes = 0;
     double* reduced_lin = nullptr;
     double* reduced_schur = nullptr;
+    int lin_precision = VIB_FP64;  // coupling-stream format of the last vib_linearize
 
     ~Slot();
     void release();
@@ -84,6 +85,7 @@ struct Batch {
     int nwin = 0, nchunks = 0, nlin = 0, max_len = 1;
     int max_edges = 0;
     int max_landmarks = 0;
+    int mixed = 0;  // 1: the mixed-precision forms of the coupling and Schur kernels
     WinDev* d_wins = nullptr;
     Chunk* d_chunks = nullptr;
     Chunk* d_lin = nullptr;

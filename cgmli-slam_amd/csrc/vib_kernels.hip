@@ -18,6 +18,8 @@
 
 #include <cstdlib>
 
+#include <type_traits>
+
 #include "vib_device.hpp"
 #include "vib_math.hpp"
 
@@ -31,6 +33,7 @@ __device__ __forceinline__ int tri_index(int i, int j, int nv) {  // i <= j, pac
 }
 
 typedef double d4 __attribute__((ext_vector_type(4)));  // v_mfma_f64_16x16x4f64 accumulator
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 // Deterministic block reductions (fixed tree order).
 __device__ double block_sum(double v, double* red) {
@@ -362,6 +365,18 @@ __device__ __forceinline__ void scatter_gram(double* acc, double* gacc, const Wi
     }
 }
 
+// Per-factor coupling terms. FP64: fac_u[k F + f] (6 planes), fac_lm[q F + f] (FacLm planes).
+// Mixed precision: the same arrays hold FP32 planes (u: 6; the Hessian-side landmark terms d,
+// c_rig (6) and c_td as planes 0..7) with the landmark gradient g_rho kept in FP64 behind them
+// (fac_lm + 4 F doubles), so both forms fit the FP64 allocation.
+__device__ __forceinline__ int fl32_plane(int q) { return q == FL_D ? 0 : q - 1; }
+template <bool MIXED>
+__device__ __forceinline__ double load_u(const WinDev& w, size_t F, int k, int f) {
+    if constexpr (MIXED) return (double)((const float*)w.fac_u)[k * F + f];
+    return w.fac_u[k * F + f];
+}
+
+template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* lin) {
     __shared__ double acc[kAccSize];
     __shared__ FrameTable ft;
@@ -428,8 +443,17 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
             } else {
                 invalid = 1;
             }
-            for (int k = 0; k < 6; ++k) w.fac_u[k * F + f] = u[k];
-            for (int q = 0; q < FL_PLANES; ++q) w.fac_lm[q * F + f] = fl[q];
+            if constexpr (MIXED) {
+                float* u32 = (float*)w.fac_u;
+                float* l32 = (float*)w.fac_lm;
+                for (int k = 0; k < 6; ++k) u32[k * F + f] = (float)u[k];
+                for (int q = 0; q < FL_PLANES; ++q)
+                    if (q != FL_G) l32[fl32_plane(q) * F + f] = (float)fl[q];
+                w.fac_lm[4 * F + f] = fl[FL_G];
+            } else {
+                for (int k = 0; k < 6; ++k) w.fac_u[k * F + f] = u[k];
+                for (int q = 0; q < FL_PLANES; ++q) w.fac_lm[q * F + f] = fl[q];
+            }
         }
         keys[wave][lane] = key;
         wave_sync();
@@ -469,6 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
 // ---------------------------------------------------------------------------------------------
 // k_lm_sums: per-landmark d, g_rho, c_rig, c_td from the per-factor terms (fixed order).
 // ---------------------------------------------------------------------------------------------
+template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_lm_sums(const WinDev* wins, const Chunk* chunks) {
     const Chunk ch = chunks[blockIdx.x];
     const WinDev& w = wins[ch.win];
@@ -477,7 +502,13 @@ __global__ __launch_bounds__(kBlock) void k_lm_sums(const WinDev* wins, const Ch
     for (int l = ch.lm_begin + threadIdx.x; l < ch.lm_end; l += blockDim.x) {
         double sum[FL_PLANES] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int f = w.lm_off[l]; f < w.lm_off[l + 1]; ++f)
-            for (int q = 0; q < FL_PLANES; ++q) sum[q] += w.fac_lm[q * F + f];
+            if constexpr (MIXED) {
+                const float* l32 = (const float*)w.fac_lm;
+                for (int q = 0; q < FL_PLANES; ++q)
+                    sum[q] += q == FL_G ? w.fac_lm[4 * F + f] : (double)l32[fl32_plane(q) * F + f];
+            } else {
+                for (int q = 0; q < FL_PLANES; ++q) sum[q] += w.fac_lm[q * F + f];
+            }
         for (int q = 0; q < FL_PLANES; ++q) w.lm_out[q * L + l] = sum[q];
     }
 }
@@ -915,10 +946,15 @@ constexpr int kVisTiles = 8;                   // 128 / 16
 constexpr int kSchurTiles = kVisTiles * (kVisTiles + 1) / 2;
 constexpr int kSchurTilesPerWave = (kSchurTiles + kBlock / 64 - 1) / (kBlock / 64);
 
-// two workgroups per CU: 256 VGPRs (no AGPRs) and 74 KB of LDS each
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_schur(const WinDev* wins, const Chunk* chunks) {
-    __shared__ double crow[kSchurRows * kSchurLd];
-    __shared__ double sc[kSchurRows], qg[kSchurRows];
+// Mixed precision keeps the coupling rows in FP32 (37 KB of LDS) and multiplies them on
+// v_mfma_f32_16x16x4f32; the FP32 products of each 16-landmark group are carried into FP64
+// accumulators (the chunk partials, and their sum over chunks, stay FP64).
+template <bool MIXED>
+__device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chunks) {
+    using CT = typename std::conditional<MIXED, float, double>::type;
+    __shared__ CT crow[kSchurRows * kSchurLd];
+    __shared__ CT sc[kSchurRows];
+    __shared__ double qg[kSchurRows];
     const Chunk ch = chunks[blockIdx.x];
     const WinDev& w = wins[ch.win];
     if (!window_active(w)) return;
@@ -947,7 +983,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     }
     double gsum = 0;  // gradient correction of column t (t < nv)
     for (int base = ch.lm_begin; base < ch.lm_end; base += kSchurRows) {
-        for (int i = t; i < kSchurRows * kSchurLd; i += kBlock) crow[i] = 0.0;
+        for (int i = t; i < kSchurRows * kSchurLd; i += kBlock) crow[i] = CT(0);
         __syncthreads();
         // dense coupling rows, scale 1 / damped(d) and gradient coefficient g / damped(d)
         for (int r0 = 0; r0 < kSchurRows; r0 += groups) {
@@ -961,11 +997,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
                     inv = w.lm->marginal ? (dl > w.lm->marg_cut ? 1.0 / dl : 0.0) : 1.0 / damped_point(dl, lambda);
                     gk = w.lm_out[LO_G * L + l];
                 }
-                sc[r] = inv;
+                sc[r] = CT(inv);
                 qg[r] = inv * gk;
             }
             if (!live) continue;
-            double* row = crow + r * kSchurLd;
+            CT* row = crow + r * kSchurLd;
             const int a = w.lm_fc[l] >> 8;
             const int ac = w.pose_col[a];
             const int f0 = w.lm_off[l], f1 = w.lm_off[l + 1];
@@ -973,15 +1009,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
                 const int b = w.ob_fc[f] >> 8;
                 const int bc = b == a ? -1 : w.pose_col[b];
                 for (int k = 0; k < 6; ++k) {
-                    const double u = w.fac_u[k * F + f];
+                    const CT u = CT(load_u<MIXED>(w, F, k, f));
                     if (ac >= 0) atomicAdd(&row[ac + k], u);
                     if (bc >= 0) atomicAdd(&row[bc + k], -u);
                 }
             }
             if (lgroup == 0) {
                 if (w.rig_col >= 0)
-                    for (int k = 0; k < 6; ++k) row[w.rig_col + k] = w.lm_out[(LO_CR0 + k) * L + l];
-                if (w.td_col >= 0) row[w.td_col] = w.lm_out[LO_CT * L + l];
+                    for (int k = 0; k < 6; ++k) row[w.rig_col + k] = CT(w.lm_out[(LO_CR0 + k) * L + l]);
+                if (w.td_col >= 0) row[w.td_col] = CT(w.lm_out[LO_CT * L + l]);
             }
         }
         __syncthreads();
@@ -990,29 +1026,56 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         for (int q = 0; q < kSchurTilesPerWave; ++q) {
             if (tti[q] < 0) continue;
             const int ci = 16 * tti[q] + lane % 16, cj = 16 * ttj[q] + lane % 16;
-            for (int k0 = 0; k0 < kSchurRows; k0 += 4) {
-                const int k = k0 + lane / 16;
-                const double av = sc[k] * crow[k * kSchurLd + ci];
-                const double bv = crow[k * kSchurLd + cj];
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
+            if constexpr (MIXED) {
+                for (int g0 = 0; g0 < kSchurRows; g0 += 16) {
+                    f4 part = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int k0 = g0; k0 < g0 + 16; k0 += 4) {
+                        const int k = k0 + lane / 16;
+                        part = __builtin_amdgcn_mfma_f32_16x16x4f32(sc[k] * crow[k * kSchurLd + ci], crow[k * kSchurLd + cj],
+                                                                    part, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) acc[q][rr] += (double)part[rr];
+                }
+            } else {
+                for (int k0 = 0; k0 < kSchurRows; k0 += 4) {
+                    const int k = k0 + lane / 16;
+                    const double av = sc[k] * crow[k * kSchurLd + ci];
+                    const double bv = crow[k * kSchurLd + cj];
+                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
+                }
             }
         }
         if (t < nv)
-            for (int k = 0; k < kSchurRows; ++k) gsum += qg[k] * crow[k * kSchurLd + t];
+            for (int k = 0; k < kSchurRows; ++k) gsum += qg[k] * (double)crow[k * kSchurLd + t];
         __syncthreads();
     }
-    // chunk partial: packed upper triangle of -C^T diag(s) C, then -C^T (s g)
+    // chunk partial: packed upper triangle of -C^T diag(s) C, then -C^T (s g); output element
+    // (row, col) of the tile in register rr: f64 MFMA row 4 rr + lane / 16, f32 MFMA row
+    // 4 (lane / 16) + rr, column lane % 16 in both
     double* part = w.schur_part + (size_t)ch.local * (tri + nv);
 #pragma unroll
     for (int q = 0; q < kSchurTilesPerWave; ++q) {
         if (tti[q] < 0) continue;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-            const int i = 16 * tti[q] + 4 * rr + lane / 16, j = 16 * ttj[q] + lane % 16;
+            const int ri = MIXED ? 4 * (lane / 16) + rr : 4 * rr + lane / 16;
+            const int i = 16 * tti[q] + ri, j = 16 * ttj[q] + lane % 16;
             if (i <= j && j < nv) part[tri_index(i, j, nv)] = -acc[q][rr];
         }
     }
     if (t < nv) part[tri + t] = -gsum;
+}
+
+// two workgroups per CU: 256 VGPRs (no AGPRs) and 74 KB of LDS each
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_schur(const WinDev* wins,
+                                                                                              const Chunk* chunks) {
+    schur_body<false>(wins, chunks);
+}
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_schur_mixed(const WinDev* wins,
+                                                                                                    const Chunk* chunks) {
+    schur_body<true>(wins, chunks);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1723,6 +1786,7 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V,
 // k_backsub: delta_k = -(g_k + c_k^T dx) / damped(d_k), clamped; model decrease; rho trial
 // (VisualCudaBatch.cu:191-205).
 // ---------------------------------------------------------------------------------------------
+template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Chunk* chunks) {
     __shared__ double red[kBlock];
     const Chunk ch = chunks[blockIdx.x];
@@ -1748,7 +1812,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
                 if (b == a) continue;
                 const int bc = w.pose_col[b];
                 for (int k = 0; k < 6; ++k) {
-                    const double u = w.fac_u[k * F + f];
+                    const double u = load_u<MIXED>(w, F, k, f);
                     cd += u * ((ac >= 0 ? dx[ac + k] : 0.0) - (bc >= 0 ? dx[bc + k] : 0.0));
                 }
             }
@@ -1988,18 +2052,27 @@ const char* vibk_name(int k) {
 
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, const vib::Chunk* lin,
                        int nlin, double** red, int max_len, int max_edges, int max_landmarks, int refinement, int mode,
-                       hipStream_t s) {
+                       int mixed, hipStream_t s) {
     using namespace vib;
     const int slices = (max_len + kRedEntries - 1) / kRedEntries;
     switch (k) {
         case 0:
-            if (nlin) hipLaunchKernelGGL(k_lin_visual, dim3(nlin), dim3(kBlock), 0, s, wins, lin);
-            if (nchunks) hipLaunchKernelGGL(k_lm_sums, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            if (nlin) {
+                if (mixed) hipLaunchKernelGGL(k_lin_visual<true>, dim3(nlin), dim3(kBlock), 0, s, wins, lin);
+                else hipLaunchKernelGGL(k_lin_visual<false>, dim3(nlin), dim3(kBlock), 0, s, wins, lin);
+            }
+            if (nchunks) {
+                if (mixed) hipLaunchKernelGGL(k_lm_sums<true>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+                else hipLaunchKernelGGL(k_lm_sums<false>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            }
             break;
         case 1: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 0); break;
         case 2: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 0, red); break;
         case 3:
-            if (nchunks) hipLaunchKernelGGL(k_schur, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            if (nchunks) {
+                if (mixed) hipLaunchKernelGGL(k_schur_mixed, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+                else hipLaunchKernelGGL(k_schur, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            }
             break;
         case 4: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 1, red); break;
         case 5:
@@ -2008,7 +2081,10 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             break;
         case 6: hipLaunchKernelGGL(k_cholesky, dim3(nwin), dim3(kCholThreads), 0, s, wins, refinement); break;
         case 7:
-            if (nchunks) hipLaunchKernelGGL(k_backsub, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            if (nchunks) {
+                if (mixed) hipLaunchKernelGGL(k_backsub<true>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+                else hipLaunchKernelGGL(k_backsub<false>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            }
             break;
         case 8:
             if (nchunks) hipLaunchKernelGGL(k_trial_cost, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);

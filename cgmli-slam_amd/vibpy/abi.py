@@ -21,6 +21,7 @@ STATUS_NAMES = {
     13: "NO_DEVICE", 14: "NOT_FOUND", 15: "UNSUPPORTED", 16: "LIGHTWEIGHT_SUPPORT", 17: "IMU_BRIDGE",
     18: "IMU_GATE",
 }
+VIB_ERR_INVALID_ARGUMENT = 1
 VIB_ERR_UNSUPPORTED = 15
 VIB_ERR_LIGHTWEIGHT_SUPPORT = 16
 VIB_ERR_IMU_BRIDGE = 17

@@ -75,7 +75,9 @@ enum {
 
 /* precision of a solve */
 #define VIB_FP64 0   /* plain FP64 everywhere */
-#define VIB_MIXED 1  /* FP32 streams/Jacobian products, FP64 accumulation, FP64 LDLT + refinement */
+#define VIB_MIXED 1  /* FP32 Hessian-side coupling streams and landmark Schur products (FP32
+                        MFMA, FP64 sums over 16-landmark groups); factor evaluation, gradients,
+                        costs and the reduced-system Cholesky stay FP64 */
 
 /* marginalization policies (src/backend/SlidingWindowManager.cpp:503) */
 #define VIB_MARGIN_OLD 0
