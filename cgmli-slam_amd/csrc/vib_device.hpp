@@ -118,7 +118,8 @@ struct WinDev {
     const double* pf_an;       // LM_PLANES x F : the factor's anchor bearing, velocity, t_ref
     const int32_t* pf_ids;     // F : anchor frame << 24 | anchor cam << 16 | observer frame << 8 | cam
     const int32_t* pf_lm;      // F : landmark
-    const int32_t* pf_f;       // F : position of the factor in landmark order
+    const int32_t* pf_pos;     // F : pair-order position of each landmark-order factor (the per-factor
+                               //     coupling planes fac_u / fac_lm are stored in pair order)
     // linearization outputs
     double* fac_u;             // 6 x F : J_anchor^T J_rho per factor (weighted, landmark order)
     double* fac_lm;            // FL_PLANES x F : per-factor landmark terms (landmark order)

@@ -232,14 +232,14 @@ bool HostWindow::layout(std::string& err) {
     for (int f = 0; f < F; ++f) order[start[pair_key(f)]++] = f;
     pf_ob.assign((size_t)OB_PLANES * F, 0);
     pf_an.assign((size_t)LM_PLANES * F, 0);
-    pf_ids.assign(F, 0); pf_lm.assign(F, 0); pf_f.assign(F, 0);
+    pf_ids.assign(F, 0); pf_lm.assign(F, 0); pf_pos.assign(F, 0);
     for (int p = 0; p < F; ++p) {
         const int f = order[p], l = ob_lm[f];
         for (int q = 0; q < OB_PLANES; ++q) pf_ob[(size_t)q * F + p] = ob[(size_t)q * F + f];
         for (int q = 0; q < LM_PLANES; ++q) pf_an[(size_t)q * F + p] = lmd[(size_t)q * L + l];
         pf_ids[p] = (lm_frame[l] << 24) | (lm_cam[l] << 16) | ob_fc[f];
         pf_lm[p] = l;
-        pf_f[p] = f;
+        pf_pos[f] = p;
     }
     lin_chunks.clear();
     for (int p0 = 0, i = 0; p0 < F; p0 += kLinFactors, ++i) lin_chunks.push_back(Chunk{0, i, p0, std::min(F, p0 + kLinFactors)});
@@ -312,8 +312,8 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     double* pf_an = a.take<double>((size_t)LM_PLANES * h.F);
     int32_t* pf_ids = a.take<int32_t>(h.F);
     int32_t* pf_lm = a.take<int32_t>(h.F);
-    int32_t* pf_f = a.take<int32_t>(h.F);
-    d.pf_ob = pf_ob; d.pf_an = pf_an; d.pf_ids = pf_ids; d.pf_lm = pf_lm; d.pf_f = pf_f;
+    int32_t* pf_pos = a.take<int32_t>(h.F);
+    d.pf_ob = pf_ob; d.pf_an = pf_an; d.pf_ids = pf_ids; d.pf_lm = pf_lm; d.pf_pos = pf_pos;
     d.lm_out = a.take<double>((size_t)LO_PLANES * h.L);
     d.lm_delta = a.take<double>(h.L);
     int32_t* imu_from = a.take<int32_t>(h.E);
@@ -366,7 +366,7 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
               up(pfree, h.pfree.data(), 4 * h.pfree.size()) && up(colmap, h.colmap.data(), 4 * h.colmap.size()) &&
               up(pf_ob, h.pf_ob.data(), 8 * h.pf_ob.size()) && up(pf_an, h.pf_an.data(), 8 * h.pf_an.size()) &&
               up(pf_ids, h.pf_ids.data(), 4 * h.pf_ids.size()) && up(pf_lm, h.pf_lm.data(), 4 * h.pf_lm.size()) &&
-              up(pf_f, h.pf_f.data(), 4 * h.pf_f.size());
+              up(pf_pos, h.pf_pos.data(), 4 * h.pf_pos.size());
     // candidate starts equal to the state
     ok = ok && up(d.cpose, h.pose.data(), 8 * h.pose.size()) && up(d.csb, h.sb.data(), 8 * h.sb.size()) &&
          up(d.crig, h.rig, 56) && up(d.ctd, &h.td, 8) && up(d.crho, h.rho.data(), 8 * h.rho.size());
@@ -435,6 +435,16 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
         max_len = std::max(max_len, wins[i].tri + wins[i].nv);
         max_edges = std::max(max_edges, s.host.E);
         max_landmarks = std::max(max_landmarks, s.host.L);
+    }
+    // XCD-aware order of the visual work list: workgroup i runs on XCD i % 8, so list position
+    // 8 s + x holds chunk x * n8 + s, and each XCD takes one contiguous range of the pair-ordered
+    // chunks. The per-factor coupling stores of one anchor frame then meet in one L2 instead of
+    // leaving eight partial lines. Positions without a chunk are empty (local = -1).
+    {
+        const int n = (int)lin.size(), n8 = (n + kXcds - 1) / kXcds;
+        std::vector<Chunk> order((size_t)n8 * kXcds, Chunk{0, -1, 0, 0});
+        for (int c = 0; c < n; ++c) order[(size_t)(c % n8) * kXcds + c / n8] = lin[c];
+        lin.swap(order);
     }
     nchunks = (int)all.size();
     nlin = (int)lin.size();

@@ -21,7 +21,8 @@ constexpr int kKernelCount = 12;
 // Batches with at least this many landmark chunks (about 16 per CU) merge kChunkMerge of them
 // per workgroup (measured on the 256-window batched workload).
 constexpr size_t kMergeChunksAbove = 4096;
-constexpr int kChunkMerge = 4;  // kernels of one LM iteration (vibk_launch ids 0..11)
+constexpr int kChunkMerge = 4;
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X (workgroup i runs on XCD i % 8)  // kernels of one LM iteration (vibk_launch ids 0..11)
 
 // Host copy of one window: observation streams sorted by landmark, tangent column layout
 // [pose (6, free) ..., rig (6), td (1), speed/bias (width) ...] then free inverse ranges,
@@ -56,7 +57,7 @@ struct HostWindow {
     std::vector<Chunk> chunks;
     // factors in (anchor frame, observer frame) pair order
     std::vector<double> pf_ob, pf_an;
-    std::vector<int32_t> pf_ids, pf_lm, pf_f;
+    std::vector<int32_t> pf_ids, pf_lm, pf_pos;
     std::vector<Chunk> lin_chunks;  // factor ranges of the visual-normal kernel
 
     bool build(const vib_window& w, std::string& err);

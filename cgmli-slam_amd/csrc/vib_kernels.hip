@@ -365,15 +365,17 @@ __device__ __forceinline__ void scatter_gram(double* acc, double* gacc, const Wi
     }
 }
 
-// Per-factor coupling terms. FP64: fac_u[k F + f] (6 planes), fac_lm[q F + f] (FacLm planes).
+// Per-factor coupling terms, stored at the factor's pair-order position p (k_lin_visual writes
+// them contiguously; the landmark-ordered kernels read them through pf_pos). FP64: fac_u[k F + p]
+// (6 planes), fac_lm[q F + p] (FacLm planes).
 // Mixed precision: the same arrays hold FP32 planes (u: 6; the Hessian-side landmark terms d,
 // c_rig (6) and c_td as planes 0..7) with the landmark gradient g_rho kept in FP64 behind them
 // (fac_lm + 4 F doubles), so both forms fit the FP64 allocation.
 __device__ __forceinline__ int fl32_plane(int q) { return q == FL_D ? 0 : q - 1; }
 template <bool MIXED>
-__device__ __forceinline__ double load_u(const WinDev& w, size_t F, int k, int f) {
-    if constexpr (MIXED) return (double)((const float*)w.fac_u)[k * F + f];
-    return w.fac_u[k * F + f];
+__device__ __forceinline__ double load_u(const WinDev& w, size_t F, int k, int p) {
+    if constexpr (MIXED) return (double)((const float*)w.fac_u)[k * F + p];
+    return w.fac_u[k * F + p];
 }
 
 template <bool MIXED>
@@ -384,6 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     __shared__ int keys[kWaves][64];
     __shared__ double red[kBlock];
     const Chunk ch = lin[blockIdx.x];
+    if (ch.local < 0) return;  // an empty position of the XCD-ordered work list
     const WinDev& w = wins[ch.win];
     if (!window_active(w) || !w.lm->new_point) return;
     const int nv = w.nv, tri = w.tri;
@@ -405,7 +408,6 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
         for (int c = 0; c < 2 * kGramCols; ++c) V[2 * kGramCols * lane + c] = 0.0;
         if (p < ch.lm_end) {
             const int ids = w.pf_ids[p];
-            const int f = w.pf_f[p];
             LmData lm;
             lm.raw = {w.pf_an[LM_BX * F + p], w.pf_an[LM_BY * F + p], w.pf_an[LM_BZ * F + p]};
             lm.vel = {w.pf_an[LM_VX * F + p], w.pf_an[LM_VY * F + p], w.pf_an[LM_VZ * F + p]};
@@ -446,13 +448,13 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
             if constexpr (MIXED) {
                 float* u32 = (float*)w.fac_u;
                 float* l32 = (float*)w.fac_lm;
-                for (int k = 0; k < 6; ++k) u32[k * F + f] = (float)u[k];
+                for (int k = 0; k < 6; ++k) u32[k * F + p] = (float)u[k];
                 for (int q = 0; q < FL_PLANES; ++q)
-                    if (q != FL_G) l32[fl32_plane(q) * F + f] = (float)fl[q];
-                w.fac_lm[4 * F + f] = fl[FL_G];
+                    if (q != FL_G) l32[fl32_plane(q) * F + p] = (float)fl[q];
+                w.fac_lm[4 * F + p] = fl[FL_G];
             } else {
-                for (int k = 0; k < 6; ++k) w.fac_u[k * F + f] = u[k];
-                for (int q = 0; q < FL_PLANES; ++q) w.fac_lm[q * F + f] = fl[q];
+                for (int k = 0; k < 6; ++k) w.fac_u[k * F + p] = u[k];
+                for (int q = 0; q < FL_PLANES; ++q) w.fac_lm[q * F + p] = fl[q];
             }
         }
         keys[wave][lane] = key;
@@ -501,14 +503,16 @@ __global__ __launch_bounds__(kBlock) void k_lm_sums(const WinDev* wins, const Ch
     const size_t L = (size_t)w.L, F = (size_t)w.F;
     for (int l = ch.lm_begin + threadIdx.x; l < ch.lm_end; l += blockDim.x) {
         double sum[FL_PLANES] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int f = w.lm_off[l]; f < w.lm_off[l + 1]; ++f)
+        for (int f = w.lm_off[l]; f < w.lm_off[l + 1]; ++f) {
+            const int p = w.pf_pos[f];
             if constexpr (MIXED) {
                 const float* l32 = (const float*)w.fac_lm;
                 for (int q = 0; q < FL_PLANES; ++q)
-                    sum[q] += q == FL_G ? w.fac_lm[4 * F + f] : (double)l32[fl32_plane(q) * F + f];
+                    sum[q] += q == FL_G ? w.fac_lm[4 * F + p] : (double)l32[fl32_plane(q) * F + p];
             } else {
-                for (int q = 0; q < FL_PLANES; ++q) sum[q] += w.fac_lm[q * F + f];
+                for (int q = 0; q < FL_PLANES; ++q) sum[q] += w.fac_lm[q * F + p];
             }
+        }
         for (int q = 0; q < FL_PLANES; ++q) w.lm_out[q * L + l] = sum[q];
     }
 }
@@ -1009,7 +1013,7 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
                 const int b = w.ob_fc[f] >> 8;
                 const int bc = b == a ? -1 : w.pose_col[b];
                 for (int k = 0; k < 6; ++k) {
-                    const CT u = CT(load_u<MIXED>(w, F, k, f));
+                    const CT u = CT(load_u<MIXED>(w, F, k, w.pf_pos[f]));
                     if (ac >= 0) atomicAdd(&row[ac + k], u);
                     if (bc >= 0) atomicAdd(&row[bc + k], -u);
                 }
@@ -1812,7 +1816,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
                 if (b == a) continue;
                 const int bc = w.pose_col[b];
                 for (int k = 0; k < 6; ++k) {
-                    const double u = load_u<MIXED>(w, F, k, f);
+                    const double u = load_u<MIXED>(w, F, k, w.pf_pos[f]);
                     cd += u * ((ac >= 0 ? dx[ac + k] : 0.0) - (bc >= 0 ? dx[bc + k] : 0.0));
                 }
             }

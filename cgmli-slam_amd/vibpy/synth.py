@@ -364,7 +364,28 @@ def generate(K=10, L=2000, n_cams=4, obs_range=(2, 8), seed=20261204, rig_free=F
                     true_sb=np.concatenate([V, np.tile(bg_true, (K, 1)), np.tile(ba_true, (K, 1))], 1),
                     true_rho=true_rho, true_rig=rig_true, samples=samples, frame_t=frame_t, imu_params=params,
                     trajectory=traj)
-    return win
+    return by_anchor_frame(win)
+
+
+def by_anchor_frame(win):
+    """The window with its landmarks in anchor-frame order (stable) and the observations grouped
+    by landmark: the order in which a backend creates landmarks as frames arrive."""
+    order = np.argsort(win.lm_anchor_frame, kind="stable")
+    new = np.empty_like(order)
+    new[order] = np.arange(order.size)
+    w = win.copy()
+    for k in ("lm_anchor_frame", "lm_anchor_cam", "lm_bearing", "lm_velocity", "lm_tref", "lm_rho"):
+        setattr(w, k, getattr(win, k)[order].copy())
+    if win.lm_free is not None:
+        w.lm_free = win.lm_free[order].copy()
+    lm = new[win.obs_landmark]
+    fo = np.argsort(lm, kind="stable")
+    w.obs_landmark = lm[fo].astype(np.int32)
+    for k in ("obs_frame", "obs_cam", "obs_bearing", "obs_velocity", "obs_tref", "obs_sqrt_info"):
+        setattr(w, k, getattr(win, k)[fo].copy())
+    if "true_rho" in w.meta:
+        w.meta["true_rho"] = np.asarray(win.meta["true_rho"])[order]
+    return w
 
 
 def camera_world(win, k, c):
