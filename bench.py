@@ -137,6 +137,39 @@ def step_error(ctx, abi, win, slot, lam=1e-4, refinement=2):
                 lam=lam, refinement_steps=refinement)
 
 
+def _session_window(i):
+    from vibpy import synth
+    return synth.generate(**synth.session_config(i))
+
+
+def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb):
+    """Serving workload: n_sessions resident windows, Poisson requests at `rate` per second for
+    `seconds`, 1-3 LM iterations each, continuous batching by size bucket (vibpy.serve)."""
+    import multiprocessing as mp
+    from vibpy import serve
+    with mp.get_context("spawn").Pool(procs) as pool:
+        wins = pool.map(_session_window, range(n_sessions))
+    sessions = [serve.Session(i, w) for i, w in enumerate(wins)]
+    srv = serve.Server(ctx, sessions, int(budget_gb * (1 << 30)), full_iterations=True)
+    srv.preload()
+    ctx.synchronize()
+    reqs = serve.poisson_requests(n_sessions, rate, seconds, np.random.default_rng(20261208))
+    t0 = time.perf_counter()
+    done = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    for s in sessions:
+        if s.slot is not None:
+            ctx.release(s.slot)
+    out = serve.summary(done, wall)
+    out.update(sessions=n_sessions, arrival_rate=rate, offered_seconds=seconds, hbm_budget_gb=budget_gb,
+               uploads=srv.uploads, evictions=srv.evictions,
+               landmarks=int(sum(w.L for w in wins)), factors=int(sum(w.F for w in wins)),
+               workload=f"{n_sessions} sessions (4 cameras, K 8-20, L 500-30000, 2-10 observations per landmark), "
+                        f"Poisson {rate:g} requests/s, 1-3 full LM iterations each (relinearized, no early "
+                        "stop), FP64, windows resident (no slide inside a request)")
+    return out
+
+
 def cpu_baseline(win, seconds=12.0):
     """Oracle FP64 restatement of the reference CPU path, one thread: steps/s on a bounded sample
     (repeated LM steps on the same window until ~`seconds` of CPU work)."""
@@ -170,6 +203,10 @@ def main():
     ap.add_argument("--batch-windows", type=int, default=256,
                     help="windows of the batched workload line (0: skip)")
     ap.add_argument("--batch-procs", type=int, default=16)
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="seconds of Poisson arrivals of the serving line (0: skip)")
+    ap.add_argument("--serve-sessions", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the measurement, launch one plain device copy of this many MiB "
                          "(reference dispatch for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -214,6 +251,12 @@ def main():
     if bwins:
         print(f"[bench] batched workload: {len(bwins)} windows", file=sys.stderr, flush=True)
         batched = batched_line(ctx, abi, bwins, max(3, min(args.steps, 10)), max(1, args.warmup))
+    serving = None
+    if args.serve_seconds > 0:
+        print(f"[bench] serving: {args.serve_sessions} sessions, {args.serve_rate:g} requests/s", file=sys.stderr,
+              flush=True)
+        serving = serving_line(ctx, abi, args.serve_sessions, args.serve_rate, args.serve_seconds, args.batch_procs,
+                               64.0)
     if args.copy_probe_mb > 0:
         ctx.copy_probe(args.copy_probe_mb << 20)
     ctx.close()
@@ -259,6 +302,8 @@ def main():
     if batched is not None:
         out["batched"] = dict(batched, workload=f"{batched['windows']} independent windows, K 5-20, L 500-30000, "
                                                "2-4 cameras, 2-10 observations per landmark, 5% outliers at 20 px")
+    if serving is not None:
+        out["serving"] = serving
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(win, args.cpu_seconds)
     print(json.dumps(out))

@@ -448,10 +448,21 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     }
     nchunks = (int)all.size();
     nlin = (int)lin.size();
-    if (hipMalloc(&d_wins, sizeof(WinDev) * nwin) != hipSuccess ||
-        hipMalloc(&d_chunks, sizeof(Chunk) * std::max(1, nchunks)) != hipSuccess ||
-        hipMalloc(&d_lin, sizeof(Chunk) * std::max(1, nlin)) != hipSuccess ||
-        hipMalloc(&d_red, sizeof(double*) * 2 * nwin) != hipSuccess) { err = "batch allocation failed"; return false; }
+    // a rebuilt batch keeps its device buffers when they are large enough; a captured graph of
+    // the previous contents is dropped
+    if (graph_exec) { hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+    if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+    auto grow = [](auto*& ptr, size_t& cap, size_t need, size_t elem) {
+        if (ptr && cap >= need) return true;
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        cap = std::max(need, cap + cap / 2);
+        return hipMalloc((void**)&ptr, elem * cap) == hipSuccess;
+    };
+    if (!grow(d_wins, cap_wins, (size_t)nwin, sizeof(WinDev)) ||
+        !grow(d_red, cap_red, (size_t)nwin, 2 * sizeof(double*)) ||
+        !grow(d_chunks, cap_chunks, (size_t)std::max(1, nchunks), sizeof(Chunk)) ||
+        !grow(d_lin, cap_lin, (size_t)std::max(1, nlin), sizeof(Chunk))) { err = "batch allocation failed"; return false; }
     if (hipMemcpyAsync(d_wins, wins.data(), sizeof(WinDev) * nwin, hipMemcpyHostToDevice, stream) != hipSuccess ||
         (nchunks && hipMemcpyAsync(d_chunks, all.data(), sizeof(Chunk) * nchunks, hipMemcpyHostToDevice, stream) != hipSuccess) ||
         (nlin && hipMemcpyAsync(d_lin, lin.data(), sizeof(Chunk) * nlin, hipMemcpyHostToDevice, stream) != hipSuccess) ||
@@ -497,6 +508,7 @@ struct vib_context {
     std::map<int, std::unique_ptr<Slot>> slots;
     std::mutex mutex;
     bool use_graph = true;
+    vib::Batch requests;  // descriptors of the last vib_request_iterate set (buffers reused)
 };
 
 namespace {
@@ -535,19 +547,39 @@ bool write_lm(vib_context* ctx, Slot& s, const LmState& lm) {
     return hipMemcpyAsync(s.dev.lm, &lm, sizeof(LmState), hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
 }
 
+LmState fresh_lm(const vib_solve_options& o) {
+    LmState lm{};
+    lm.lambda = o.initial_lambda > 0 ? o.initial_lambda : 1e-4;
+    lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = std::max(1, o.max_iterations);
+    lm.continuous = (o.flags & VIB_SOLVE_CONTINUOUS) ? 1 : 0;
+    return lm;
+}
+
+// Report of a finished (or stopped) LM solve from its device state.
+int fill_report(const LmState& s, const Slot& slot, double secs, vib_solve_report& r) {
+    int st = status_from_lm(s.status);
+    if (s.status == 0 && !s.accepted && !s.converged) st = VIB_ERR_NO_ACCEPTABLE_STEP;
+    r.status = st;
+    r.iterations = s.iteration;
+    r.accepted_steps = s.accepted;
+    r.converged = s.converged;
+    r.effective_parameters = slot.host.nr + slot.host.ne;
+    r.retained_dimension = slot.host.nr;
+    r.initial_cost = s.initial_cost;
+    r.final_cost = s.cost;
+    r.final_lambda = s.lambda;
+    r.seconds = secs;
+    return st;
+}
+
 // Runs the LM loop for a set of slots. All windows iterate in lock-step; finished windows are
 // skipped on the device. Returns the per-window reports.
 int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options& o, std::vector<vib_solve_report>& reps) {
     const auto t0 = std::chrono::steady_clock::now();
     if (o.precision != VIB_FP64 && o.precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     const int maxit = std::max(1, o.max_iterations);
-    for (Slot* s : slots) {
-        LmState lm{};
-        lm.lambda = o.initial_lambda > 0 ? o.initial_lambda : 1e-4;
-        lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = maxit;
-        lm.continuous = (o.flags & VIB_SOLVE_CONTINUOUS) ? 1 : 0;
-        if (!write_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
-    }
+    for (Slot* s : slots)
+        if (!write_lm(ctx, *s, fresh_lm(o))) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
     Batch batch;
     batch.mixed = o.precision == VIB_MIXED;
     std::string err;
@@ -578,20 +610,7 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
     reps.assign(slots.size(), vib_solve_report{});
     int first_error = VIB_OK;
     for (size_t i = 0; i < slots.size(); ++i) {
-        const LmState& s = states[i];
-        vib_solve_report& r = reps[i];
-        int st = status_from_lm(s.status);
-        if (s.status == 0 && !s.accepted && !s.converged) st = VIB_ERR_NO_ACCEPTABLE_STEP;
-        r.status = st;
-        r.iterations = s.iteration;
-        r.accepted_steps = s.accepted;
-        r.converged = s.converged;
-        r.effective_parameters = slots[i]->host.nr + slots[i]->host.ne;
-        r.retained_dimension = slots[i]->host.nr;
-        r.initial_cost = s.initial_cost;
-        r.final_cost = s.cost;
-        r.final_lambda = s.lambda;
-        r.seconds = secs;
+        const int st = fill_report(states[i], *slots[i], secs, reps[i]);
         if (st != VIB_OK && first_error == VIB_OK) first_error = st;
     }
     if (first_error != VIB_OK) ctx->error = status_message(first_error);
@@ -1625,6 +1644,52 @@ int vib_solve_batch(vib_context* ctx, int32_t count, const int32_t* ids, const v
     const int st = run_lm(ctx, slots, *o, reps);
     for (size_t i = 0; i < reps.size(); ++i) reports[i] = reps[i];
     return st;
+}
+
+int vib_request_begin(vib_context* ctx, int32_t slot, const vib_solve_options* o) {
+    if (!ctx || !o) return VIB_ERR_INVALID_ARGUMENT;
+    if (o->precision != VIB_FP64 && o->precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    if (!write_lm(ctx, *s, fresh_lm(*o))) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
+    return VIB_OK;
+}
+
+int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* ids, int32_t precision, int32_t refinement,
+                        int32_t* done, vib_solve_report* reps) {
+    if (!ctx || !ids || !done || count <= 0) return VIB_ERR_INVALID_ARGUMENT;
+    if (precision != VIB_FP64 && precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<Slot*> slots;
+    for (int i = 0; i < count; ++i) {
+        Slot* s = find_slot(ctx, ids[i]);
+        if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+        for (Slot* o : slots)
+            if (o == s) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "a slot is listed twice");
+        slots.push_back(s);
+    }
+    Batch& b = ctx->requests;
+    b.mixed = precision == VIB_MIXED;
+    std::string err;
+    if (!b.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    if (!b.iteration(ctx->stream, refinement))
+        return fail(ctx, VIB_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+    std::vector<LmState> st(slots.size());
+    for (size_t i = 0; i < slots.size(); ++i)
+        if (hipMemcpyAsync(&st[i], slots[i]->dev.lm, sizeof(LmState), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+            return fail(ctx, VIB_ERR_DEVICE, "LM state read failed");
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(ctx, VIB_ERR_DEVICE, std::string("device failure: ") + hipGetErrorString(hipGetLastError()));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t i = 0; i < slots.size(); ++i) {
+        done[i] = st[i].status != 0;
+        if (reps) fill_report(st[i], *slots[i], secs, reps[i]);
+    }
+    return VIB_OK;
 }
 
 int vib_copy_probe(vib_context* ctx, int64_t bytes) {

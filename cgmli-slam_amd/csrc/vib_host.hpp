@@ -93,6 +93,7 @@ struct Batch {
     double** d_red = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    size_t cap_wins = 0, cap_red = 0, cap_chunks = 0, cap_lin = 0;  // grow-only device buffers (rebuilt batches)
 
     ~Batch();
     bool build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err);

@@ -317,6 +317,8 @@ def load_library(path=LIB_PATH):
         "vib_marginalize": ([vp, C.c_int32, C.c_int32, dp, C.c_int32, C.c_double, C.c_double,
                              C.POINTER(VibImuParams), C.POINTER(VibMarginalReport)], C.c_int),
         "vib_read_window": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
+        "vib_request_begin": ([vp, C.c_int32, C.POINTER(VibSolveOptions)], C.c_int),
+        "vib_request_iterate": ([vp, C.c_int32, ip, C.c_int32, C.c_int32, ip, C.POINTER(VibSolveReport)], C.c_int),
         "vib_read_prior": ([vp, C.c_int32, ip, ip, ip, ip, lp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "vib_kernel_count": ([], C.c_int),
         "vib_kernel_name": ([C.c_int32], C.c_char_p),
@@ -374,6 +376,16 @@ def trimmed_window(buf, cs, K, L, F, E, rows, cols, blocks, reflen):
                   huber_delta=cs.huber_delta, min_inverse_depth=cs.min_inverse_depth, td_lower=cs.td_lower,
                   td_upper=cs.td_upper, pose_free=buf.pose_free[:K].copy(), sb_width=buf.sb_width[:K].copy(),
                   lm_free=buf.lm_free[:L].copy(), **prior)
+
+
+def window_footprint(win, precision=VIB_FP64, lib=None):
+    """Device bytes the library allocates for `win` in one slot (vib_window_footprint)."""
+    lib = lib or load_library()
+    s = win.cstruct()
+    n = lib.vib_window_footprint(C.byref(s), precision)
+    if n < 0:
+        raise RuntimeError("vib_window_footprint rejected the window")
+    return int(n)
 
 
 def imu_params(sigma_g_c=1e-3, sigma_a_c=1e-2, sigma_gw_c=2.6867875817855036e-06,
@@ -510,6 +522,19 @@ class Context:
         st = self.lib.vib_marginalize(self.h, slot, policy, None if smp is None else f64p(smp), cnt, t_start, t_end,
                                       None if params is None else C.byref(params), C.byref(rep))
         return st, rep
+
+    def request_begin(self, slot, opts):
+        return self.check(self.lib.vib_request_begin(self.h, slot, C.byref(opts)), "vib_request_begin")
+
+    def request_iterate(self, slots, precision=VIB_FP64, refinement=0):
+        """One batched LM iteration of the running requests on `slots`: (done flags, reports)."""
+        n = len(slots)
+        ids = np.ascontiguousarray(slots, np.int32)
+        done = np.zeros(n, np.int32)
+        reps = (VibSolveReport * n)()
+        self.check(self.lib.vib_request_iterate(self.h, n, ids.ctypes.data_as(ip), precision, refinement,
+                                                done.ctypes.data_as(ip), reps), "vib_request_iterate")
+        return done.astype(bool), list(reps)
 
     def read_window(self, slot):
         """The whole resident window (vib_read_window) as a Window; observations in landmark order."""

@@ -476,3 +476,11 @@ def batch_config(i, seed_base=20261207):
     rng = np.random.default_rng(seed_base * 1000 + i)
     return dict(K=int(rng.integers(5, 21)), L=int(rng.integers(500, 30001)), n_cams=int(rng.integers(2, 5)),
                 obs_range=(2, 10), seed=seed_base * 1000 + i, outlier_fraction=0.05)
+
+
+def session_config(i, seed_base=20261208):
+    """C5: session i of the serving workload (its own 4-camera rig, 8-20 keyframes, 500-30,000
+    landmarks, 2-10 observations per landmark)."""
+    rng = np.random.default_rng(seed_base * 1000 + i)
+    return dict(K=int(rng.integers(8, 21)), L=int(rng.integers(500, 30001)), n_cams=4, obs_range=(2, 10),
+                seed=seed_base * 1000 + i)

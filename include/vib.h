@@ -282,6 +282,18 @@ int vib_solve_batch(vib_context* ctx, int32_t count, const int32_t* slots,
 int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options* options,
                          int32_t gate_new_imu_nis, vib_latest_two_report* report);
 
+/* ---------------- serving: continuous batching (no counterpart in the reference) ----------
+ * A request is a short LM solve on a resident window. vib_request_begin starts it (fresh LM
+ * state with the options' iteration budget and initial lambda); vib_request_iterate then runs
+ * one LM iteration for every listed slot in one batched launch sequence. done[i] = 1 once slot
+ * i's request has finished (converged, budget spent or failed; reports[i] then holds its
+ * result, status included). The set of slots may change between calls, so requests that arrive
+ * while others are running join at the next iteration: each window's result does not depend on
+ * which other windows share its launches. All slots of one call use one precision. */
+int vib_request_begin(vib_context* ctx, int32_t slot, const vib_solve_options* options);
+int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* slots, int32_t precision,
+                        int32_t refinement_steps, int32_t* done, vib_solve_report* reports);
+
 /* ---------------- measurement harness (no counterpart in the reference) ---------------- */
 /* Number of distinct kernels of one LM iteration and their names (for per-kernel timing). */
 int vib_kernel_count(void);
