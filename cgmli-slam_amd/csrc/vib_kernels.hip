@@ -1986,8 +1986,8 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
                     }
                 }
                 lm->iteration = i + 1;
-                if (lm->continuous) {
-                    if (lm->status > 0) lm->status = 0;
+                if (lm->continuous) {  // relinearize every iteration, stop only at the budget
+                    if (lm->status >= 0) lm->status = lm->iteration >= lm->max_iterations ? 1 : 0;
                     lm->new_point = 1;
                 } else {
                     if (lm->status == 0 && lm->iteration >= lm->max_iterations) lm->status = 1;

@@ -159,8 +159,9 @@ typedef struct vib_window {
 } vib_window;
 
 /* solve option flags */
-#define VIB_SOLVE_CONTINUOUS 1u  /* benchmark mode: relinearize every iteration, run the full
-                                    iteration budget without terminating on convergence  */
+#define VIB_SOLVE_CONTINUOUS 1u  /* relinearize every iteration and run the full iteration
+                                    budget without terminating on convergence (benchmarks,
+                                    serving load)                                          */
 
 typedef struct vib_solve_options {
     int32_t max_iterations;     /* LM iteration budget (reference: maxIterations)       */
