@@ -33,7 +33,7 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        int mixed, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
-hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, hipStream_t s);
+hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, hipStream_t s);
 }
 
 namespace vib {
@@ -872,27 +872,32 @@ WindowCopy filter_window(const WindowCopy& w, int drop, const std::vector<char>&
 // ascending, eigenvectors as the columns of U (row-major).
 int device_eigen(vib_context* ctx, const std::vector<double>& M, int n, std::vector<double>& ev, std::vector<double>& U) {
     double *dA = nullptr, *dV = nullptr, *de = nullptr;
+    int* dok = nullptr;
+    int conv = 0;
     const size_t bytes = 8 * (size_t)n * n;
     std::vector<double> V((size_t)n * n), e(n);
     const bool ok = hipMalloc(&dA, bytes) == hipSuccess && hipMalloc(&dV, bytes) == hipSuccess &&
-                    hipMalloc(&de, 8 * (size_t)n) == hipSuccess &&
+                    hipMalloc(&de, 8 * (size_t)n) == hipSuccess && hipMalloc(&dok, sizeof(int)) == hipSuccess &&
                     hipMemcpyAsync(dA, M.data(), bytes, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
-                    vibk_sym_eigen(dA, dV, de, n, n, ctx->stream) == hipSuccess &&
+                    vibk_sym_eigen(dA, dV, de, n, n, dok, ctx->stream) == hipSuccess &&
                     hipMemcpyAsync(V.data(), dV, bytes, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
                     hipMemcpyAsync(e.data(), de, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(&conv, dok, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
                     hipStreamSynchronize(ctx->stream) == hipSuccess;
     if (dA) hipFree(dA);
     if (dV) hipFree(dV);
     if (de) hipFree(de);
+    if (dok) hipFree(dok);
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, "device eigendecomposition failed");
+    if (!conv) return fail(ctx, VIB_ERR_EIGEN_FAILED, "eigensolver did not converge");
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e[a] < e[b]; });
     ev.resize(n);
     U.resize((size_t)n * n);
-    for (int c = 0; c < n; ++c) {
+    for (int c = 0; c < n; ++c) {  // k_sym_eigen returns eigenvector c as row c of V
         ev[c] = e[order[c]];
-        for (int r = 0; r < n; ++r) U[(size_t)r * n + c] = V[(size_t)r * n + order[c]];
+        for (int r = 0; r < n; ++r) U[(size_t)r * n + c] = V[(size_t)order[c] * n + r];
     }
     for (double v : ev) if (!std::isfinite(v)) return fail(ctx, VIB_ERR_EIGEN_FAILED, "eigensolver did not converge");
     return VIB_OK;

@@ -1708,82 +1708,150 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_sym_eigen: A = V diag(ev) V^T for one symmetric n x n matrix per workgroup (row-major, leading
-// dimension ld; A is overwritten, ev is unsorted, eigenvectors are the columns of V). Parallel
-// cyclic Jacobi: every step rotates the n/2 disjoint index pairs of one round of a round-robin
-// schedule at once, so each 2x2 block of A and each row pair of V is updated by one thread
-// without conflicts (A <- G^T A G, V <- V G); n-1 steps form a sweep; sweeps repeat until the
-// off-diagonal mass falls below 1e-32 of the total, at most 60. The marginalization's
-// pseudo-inverse of the eliminated state block and the prior's eigen cut and square root use it
-// (Marginalizer::psdSolve / compute, Marginalizer.cpp:240-283,430-466, whose GPU form is
-// MarginalCudaWorkspace::eigen, VisualCudaBatch.cu:887-957).
+// k_sym_eigen: A = Q diag(ev) Q^T for one symmetric n x n matrix (row-major, leading dimension
+// ld; A is overwritten), one workgroup. Q is returned transposed in V (row c = eigenvector c),
+// ev unsorted. Three phases (Golub & Van Loan, Householder tridiagonalization and the symmetric
+// QR step with implicit Wilkinson shift):
+//   1. n-2 Householder reflections H_k = I - 2 v v^T reduce A to tridiagonal (d, e); each step
+//      is a column norm, a matrix-vector product (one wave per row) and a rank-2 update;
+//      v is kept below the subdiagonal of A;
+//   2. Q^T = H_{n-3} ... H_0 accumulated into V;
+//   3. implicit QR sweeps on the unreduced trailing block: one lane chases the bulge and
+//      records the Givens rotations in LDS, then every thread applies them to its column of
+//      V^T (rows k, k+1 of V, coalesced); converged off-diagonals (|e| <= eps (|d_k|+|d_k+1|))
+//      deflate; at most 30 n sweeps.
+// Serves the marginalization's pseudo-inverse of the eliminated state block and the prior's
+// eigen cut and square root (Marginalizer::psdSolve / compute, Marginalizer.cpp:240-283,
+// 430-466, whose GPU form is MarginalCudaWorkspace::eigen, VisualCudaBatch.cu:887-957).
 // ---------------------------------------------------------------------------------------------
 constexpr int kEigMax = 512;
 constexpr int kEigThreads = 1024;
-__global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V, double* ev, int n, int ld) {
-    __shared__ double cs[kEigMax / 2], sn[kEigMax / 2];
-    __shared__ int pp[kEigMax / 2], qq[kEigMax / 2];
+__global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok) {
+    __shared__ double d[kEigMax], e[kEigMax], vec[kEigMax], q[kEigMax], gc[kEigMax], gs[kEigMax];
     __shared__ double red[kEigThreads / 64];
-    const int t = threadIdx.x, nt = blockDim.x;
-    const int m = n + (n & 1), half = m / 2;  // odd n: index n is a zero dummy row/column
+    __shared__ int lo, hi, state;
+    const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
+    auto a = [&](int r, int c) -> double& { return A[(size_t)r * ld + c]; };
+    // 1. tridiagonalization
+    for (int k = 0; k + 2 < n; ++k) {
+        const int m = n - k - 1;  // length of the reflected part (rows k+1 .. n-1)
+        double ss = 0;
+        for (int i = t; i < m; i += nt) { const double x = a(k + 1 + i, k); ss += x * x; }
+        ss = block_sum_waves(ss, red);
+        const double x0 = a(k + 1, k), nx = sqrt(ss);
+        const double alpha = x0 > 0 ? -nx : nx;
+        // v = (x - alpha e1) / |x - alpha e1|;  |x - alpha e1|^2 = 2 (ss - alpha x0)
+        const double vn2 = 2 * (ss - alpha * x0);
+        const double iv = vn2 > 0 ? 1 / sqrt(vn2) : 0.0;
+        for (int i = t; i < m; i += nt) vec[i] = (a(k + 1 + i, k) - (i == 0 ? alpha : 0.0)) * iv;
+        __syncthreads();
+        // p = A22 v (one wave per row)
+        for (int i = wave; i < m; i += nw) {
+            double s = 0;
+            for (int j = lane; j < m; j += 64) s += a(k + 1 + i, k + 1 + j) * vec[j];
+            s = wave_reduce_sum(s);
+            if (lane == 0) q[i] = s;
+        }
+        __syncthreads();
+        double kk = 0;
+        for (int i = t; i < m; i += nt) kk += vec[i] * q[i];
+        kk = block_sum_waves(kk, red);
+        for (int i = t; i < m; i += nt) q[i] -= kk * vec[i];
+        __syncthreads();
+        // A22 -= 2 (v q^T + q v^T); column k keeps v below the subdiagonal
+        for (int p = t; p < m * m; p += nt) {
+            const int i = p / m, j = p % m;
+            a(k + 1 + i, k + 1 + j) -= 2 * (vec[i] * q[j] + q[i] * vec[j]);
+        }
+        if (t == 0) { d[k] = a(k, k); e[k] = vn2 > 0 ? alpha : x0; }
+        __syncthreads();
+        for (int i = t; i < m; i += nt) a(k + 1 + i, k) = vn2 > 0 ? vec[i] : 0.0;
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (n >= 2) { d[n - 2] = a(n - 2, n - 2); e[n - 2] = a(n - 1, n - 2); }
+        d[n - 1] = a(n - 1, n - 1);
+        e[n - 1] = 0;
+    }
+    // 2. Q^T = H_{n-3} ... H_0 in V: start from I, apply H_k on the right for k = n-3 .. 0
+    //    (Q^T <- Q^T H_k acts on columns k+1 ..; H_0 last)
     for (int i = t; i < n * n; i += nt) V[(size_t)(i / n) * ld + i % n] = (i / n == i % n) ? 1.0 : 0.0;
     __syncthreads();
-    for (int sweep = 0; sweep < 60; ++sweep) {
-        double off = 0, tot = 0;
-        for (int i = t; i < n * n; i += nt) {
-            const double v = A[(size_t)(i / n) * ld + i % n];
-            tot += v * v;
-            if (i / n != i % n) off += v * v;
+    for (int k = n - 3; k >= 0; --k) {
+        const int m = n - k - 1;
+        for (int i = t; i < m; i += nt) vec[i] = a(k + 1 + i, k);
+        __syncthreads();
+        // w_r = sum_i V[r][k+1+i] v_i for the rows r that H_k reaches (rows >= k+1 of Q^T stay
+        // identity-like until their H is applied, but the full product is formed for clarity)
+        for (int r = wave; r < n; r += nw) {
+            double s = 0;
+            for (int i = lane; i < m; i += 64) s += V[(size_t)r * ld + k + 1 + i] * vec[i];
+            s = wave_reduce_sum(s);
+            if (lane == 0) q[r] = s;
         }
-        off = block_sum_waves(off, red);
-        tot = block_sum_waves(tot, red);
-        if (off == 0 || off <= 1e-32 * tot) break;
-        for (int r = 0; r < m - 1; ++r) {
-            if (t < half) {  // round r of the circle schedule (index m-1 fixed)
-                const int a = t == 0 ? r : (r + t) % (m - 1), b = t == 0 ? m - 1 : (r - t + (m - 1)) % (m - 1);
-                const int p = min(a, b), q = max(a, b);
-                double c = 1, s = 0;
-                if (q < n) {
-                    const double apq = A[(size_t)p * ld + q];
-                    if (apq != 0) {
-                        const double theta = (A[(size_t)q * ld + q] - A[(size_t)p * ld + p]) / (2 * apq);
-                        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
-                        c = 1 / sqrt(tt * tt + 1);
-                        s = tt * c;
-                    }
-                }
-                pp[t] = p; qq[t] = q; cs[t] = c; sn[t] = s;
-            }
-            __syncthreads();
-            for (int blk = t; blk < half * half; blk += nt) {
-                const int P = blk / half, R = blk % half;
-                const int p = pp[P], q = qq[P], u = pp[R], v = qq[R];
-                const double c1 = cs[P], s1 = sn[P], c2 = cs[R], s2 = sn[R];
-                const bool qn = q < n, vn = v < n;
-                const double apu = A[(size_t)p * ld + u];
-                const double apv = vn ? A[(size_t)p * ld + v] : 0.0;
-                const double aqu = qn ? A[(size_t)q * ld + u] : 0.0;
-                const double aqv = qn && vn ? A[(size_t)q * ld + v] : 0.0;
-                // rows: G^T (p: c1 p - s1 q, q: s1 p + c1 q), then columns by G of pair R
-                const double bpu = c1 * apu - s1 * aqu, bpv = c1 * apv - s1 * aqv;
-                const double bqu = s1 * apu + c1 * aqu, bqv = s1 * apv + c1 * aqv;
-                A[(size_t)p * ld + u] = c2 * bpu - s2 * bpv;
-                if (vn) A[(size_t)p * ld + v] = s2 * bpu + c2 * bpv;
-                if (qn) A[(size_t)q * ld + u] = c2 * bqu - s2 * bqv;
-                if (qn && vn) A[(size_t)q * ld + v] = s2 * bqu + c2 * bqv;
-            }
-            for (int e = t; e < n * half; e += nt) {
-                const int k = e / half, P = e % half;
-                const int p = pp[P], q = qq[P];
-                if (q >= n) continue;
-                const double x = V[(size_t)k * ld + p], y = V[(size_t)k * ld + q];
-                V[(size_t)k * ld + p] = cs[P] * x - sn[P] * y;
-                V[(size_t)k * ld + q] = sn[P] * x + cs[P] * y;
-            }
-            __syncthreads();
+        __syncthreads();
+        for (int p = t; p < n * m; p += nt) {
+            const int r = p / m, i = p % m;
+            V[(size_t)r * ld + k + 1 + i] -= 2 * q[r] * vec[i];
         }
+        __syncthreads();
     }
-    for (int i = t; i < n; i += nt) ev[i] = A[(size_t)i * ld + i];
+    // 3. implicit symmetric QR on (d, e); V holds Q^T: a rotation G on (k, k+1) replaces rows
+    //    k, k+1 of Q^T by (c row_k - s row_k+1, s row_k + c row_k+1)
+    if (t == 0) { state = 0; hi = n - 1; }
+    __syncthreads();
+    for (int sweep = 0; sweep < 30 * n; ++sweep) {
+        if (t == 0) {
+            const double eps = 2.220446049250313e-16;
+            int m = hi;
+            while (m > 0 && fabs(e[m - 1]) <= eps * (fabs(d[m - 1]) + fabs(d[m]))) { e[m - 1] = 0; --m; }
+            hi = m;
+            if (m == 0) {
+                state = 1;
+            } else {
+                int l = m - 1;
+                while (l > 0 && fabs(e[l - 1]) > eps * (fabs(d[l - 1]) + fabs(d[l]))) --l;
+                lo = l;
+                // Wilkinson shift from the trailing 2x2 block
+                const double dl = 0.5 * (d[m - 1] - d[m]), em = e[m - 1];
+                const double mu = d[m] - em * em / (dl + (dl >= 0 ? 1.0 : -1.0) * sqrt(dl * dl + em * em));
+                double x = d[l] - mu, z = e[l], bulge = 0;
+                for (int k = l; k < m; ++k) {
+                    const double h = hypot(x, z);
+                    const double c = h > 0 ? x / h : 1.0, s = h > 0 ? -z / h : 0.0;
+                    if (k > l) e[k - 1] = c * x - s * z;  // (x, z) = (e[k-1], bulge)
+                    const double ak = d[k], bk = e[k], ck = d[k + 1];
+                    d[k] = c * c * ak - 2 * c * s * bk + s * s * ck;
+                    d[k + 1] = s * s * ak + 2 * c * s * bk + c * c * ck;
+                    e[k] = c * s * (ak - ck) + (c * c - s * s) * bk;
+                    if (k + 1 < m) {
+                        const double r = e[k + 1];
+                        bulge = -s * r;
+                        e[k + 1] = c * r;
+                        x = e[k];
+                        z = bulge;
+                    }
+                    gc[k] = c;
+                    gs[k] = s;
+                }
+            }
+        }
+        __syncthreads();
+        if (state) break;
+        const int l = lo, m = hi;
+        for (int r = t; r < n; r += nt) {
+            double x = V[(size_t)l * ld + r];
+            for (int k = l; k < m; ++k) {
+                const double y = V[(size_t)(k + 1) * ld + r];
+                V[(size_t)k * ld + r] = gc[k] * x - gs[k] * y;
+                x = gs[k] * x + gc[k] * y;
+            }
+            V[(size_t)m * ld + r] = x;
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < n; i += nt) ev[i] = d[i];
+    if (t == 0) *ok = state;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2116,9 +2184,9 @@ hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, hipStream_t s) {
+hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, hipStream_t s) {
     if (n < 1 || n > vib::kEigMax || ld < n) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(vib::k_sym_eigen, dim3(1), dim3(vib::kEigThreads), 0, s, A, V, ev, n, ld);
+    hipLaunchKernelGGL(vib::k_sym_eigen, dim3(1), dim3(vib::kEigThreads), 0, s, A, V, ev, n, ld, ok);
     return hipGetLastError();
 }
 
