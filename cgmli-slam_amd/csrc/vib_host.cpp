@@ -402,6 +402,9 @@ Batch::~Batch() {
     if (d_red) hipFree(d_red);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
+    for (hipEvent_t e : fork_join)
+        if (e) hipEventDestroy(e);
+    if (side) hipStreamDestroy(side);
 }
 
 bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err) {
@@ -484,10 +487,36 @@ bool Batch::step(hipStream_t s, int refinement, int mode) {
     return true;
 }
 
-bool Batch::iteration(hipStream_t s, int refinement) { return linearize(s) && step(s, refinement, 0); }
+// One LM iteration. The retained factors (IMU edges and prior: a few workgroups per window)
+// do not depend on the visual kernels, so their linearization runs on a side stream beside the
+// visual linearization and Schur elimination, and their trial cost beside the back-substitution
+// and the visual trial cost; the main stream joins each before the kernel that reads it
+// (k_assemble, k_lm_update). Both streams have the default priority.
+bool Batch::make_side() {
+    if (side) return true;
+    if (hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess) {
+        side = nullptr;
+        return false;
+    }
+    for (hipEvent_t& e : fork_join)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+    return true;
+}
+
+bool Batch::iteration(hipStream_t s, int refinement) {
+    if (!make_side()) return false;
+    auto fork = [&](hipEvent_t e) { return hipEventRecord(e, s) == hipSuccess && hipStreamWaitEvent(side, e, 0) == hipSuccess; };
+    auto join = [&](hipEvent_t e) { return hipEventRecord(e, side) == hipSuccess && hipStreamWaitEvent(s, e, 0) == hipSuccess; };
+    return fork(fork_join[0]) && launch(1, side, 0, 0) && launch(0, s, 0, 0) && launch(2, s, 0, 0) &&
+           launch(3, s, refinement, 0) && launch(4, s, refinement, 0) && join(fork_join[1]) &&
+           launch(5, s, refinement, 0) && launch(6, s, refinement, 0) && fork(fork_join[2]) &&
+           launch(9, side, refinement, 0) && launch(7, s, refinement, 0) && launch(8, s, refinement, 0) &&
+           join(fork_join[3]) && launch(10, s, refinement, 0) && launch(11, s, refinement, 0);
+}
 
 bool Batch::capture(hipStream_t s, int refinement) {
     if (graph_exec) return true;
+    if (!make_side()) return false;  // before capture: stream and event creation stay out of it
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
     const bool ok = iteration(s, refinement);
     if (hipStreamEndCapture(s, &graph) != hipSuccess || !ok) return false;

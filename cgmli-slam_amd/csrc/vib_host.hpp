@@ -17,12 +17,12 @@ constexpr int kMaxFactorsPerLandmark = 13;
 // Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major, small n); eigenvalues
 // ascending, eigenvectors as columns of V (vib_preint.cpp).
 void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V);
-constexpr int kKernelCount = 12;
+constexpr int kKernelCount = 12;  // kernels of one LM iteration (vibk_launch ids 0..11)
 // Batches with at least this many landmark chunks (about 16 per CU) merge kChunkMerge of them
 // per workgroup (measured on the 256-window batched workload).
 constexpr size_t kMergeChunksAbove = 4096;
 constexpr int kChunkMerge = 4;
-constexpr int kXcds = 8;  // accelerator complex dies of an MI355X (workgroup i runs on XCD i % 8)  // kernels of one LM iteration (vibk_launch ids 0..11)
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X (workgroup i runs on XCD i % 8)
 
 // Host copy of one window: observation streams sorted by landmark, tangent column layout
 // [pose (6, free) ..., rig (6), td (1), speed/bias (width) ...] then free inverse ranges,
@@ -94,12 +94,15 @@ struct Batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     size_t cap_wins = 0, cap_red = 0, cap_chunks = 0, cap_lin = 0;  // grow-only device buffers (rebuilt batches)
+    hipStream_t side = nullptr;        // the retained-factor kernels of an iteration run here
+    hipEvent_t fork_join[4] = {};      // overlapped with the visual kernels on the main stream
 
     ~Batch();
     bool build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err);
     bool launch(int kernel, hipStream_t s, int refinement, int mode);
     bool linearize(hipStream_t s);
     bool step(hipStream_t s, int refinement, int mode);
+    bool make_side();
     bool iteration(hipStream_t s, int refinement);
     bool capture(hipStream_t s, int refinement);
 };

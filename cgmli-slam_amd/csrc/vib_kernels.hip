@@ -1127,6 +1127,11 @@ __device__ __forceinline__ double retained_entry(const WinDev& w, const int* ci,
     return h;
 }
 
+// Operand layout of a 16x16 tile: element (i, c) at (c / 4) * 64 + i + 16 (c % 4), so the 64
+// doubles of column group c / 4 are the A or B operand of one MFMA (lane i + 16 k). It is the
+// accumulator layout of the transposed tile; the factor tile store and the LDS panels use it.
+__device__ __forceinline__ int opnd_pos(int i, int c) { return (c >> 2) * 64 + i + 16 * (c & 3); }
+
 __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const* reduced) {
     __shared__ double red[kBlock];
     WinDev& w = wins[blockIdx.x];
@@ -1202,12 +1207,13 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
         w.Scopy[(size_t)i * nr + j] = damped(i, j);
     }
     for (int i = row0 + t; i < row1; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
-    // this workgroup's four rows of tile row ti of the factor tile store (register q of the
-    // accumulator layout; padding identity)
+    // this workgroup's four rows of tile row ti of the factor tile store, in operand layout
+    // (padding identity)
     for (int p = t; p < (ti + 1) * 64; p += nt) {
-        const int tj = p / 64, e = p % 64;
-        const int r = row0 + e / 16, c = kTile * tj + e % 16;
-        w.fac[tile_base(ti, tj) + 64 * q + e] = (r < nr && c < nr) ? damped(r, c) : (r == c ? 1.0 : 0.0);
+        const int tj = p / 64, e = p % 64, rho = e / 16, c = e % 16;
+        const int r = row0 + rho, cg = kTile * tj + c;
+        w.fac[tile_base(ti, tj) + opnd_pos(kAssembleRows * q + rho, c)] =
+            (r < nr && cg < nr) ? damped(r, cg) : (r == cg ? 1.0 : 0.0);
     }
 }
 
@@ -1232,10 +1238,8 @@ typedef __attribute__((address_space(1))) double gdouble;
 constexpr int kMaxTiles = (kMaxRetained + kTile - 1) / kTile;
 constexpr int kTileDoubles = kTile * kTile;
 constexpr int kCholThreads = 1024;
+static_assert(kCholThreads / 64 >= 3, "the factorization needs a diagonal wave and two trailing waves");
 
-// LDS operand layout of a 16x16 tile: element (i, c) at (c / 4) * 64 + i + 16 (c % 4), so the
-// 64 doubles of column group c / 4 are the A or B operand of one MFMA (lane i + 16 k).
-__device__ __forceinline__ int opnd_pos(int i, int c) { return (c >> 2) * 64 + i + 16 * (c & 3); }
 
 // Value of v in lane src (src wave-uniform).
 __device__ __forceinline__ double lane_bcast(double v, int src) {
@@ -1243,6 +1247,16 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 16 lanes of the lane's row (quad xor 1, xor 2, half-row and row mirrors); every
+// lane of the row gets the same bits.
+__device__ __forceinline__ double row16_sum(double v) {
+    v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm [1, 0, 3, 2]
+    v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm [2, 3, 0, 1]
+    v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    v += __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false);  // row_mirror
+    return v;
 }
 
 // Value of v in lane Q of the lane's own 16-lane row (one v_mov_b64_dpp row_newbcast).
@@ -1265,93 +1279,119 @@ template <int J, int C>
 __device__ __forceinline__ void factor_update(double (&v)[kTile]) {
     if constexpr (C < kTile) {
         v[C] = fma(-v[J], row_bcast<C>(v[J]), v[C]);
+        asm volatile("" : "+v"(v[C]));  // update in place now: the broadcast dies here
         factor_update<J, C + 1>(v);
     }
 }
 
-// Columns J.. of the factor (right-looking): lane r of a 16-lane row holds row r of the tile;
-// every pivot must be positive and finite.
-template <int J>
-__device__ __forceinline__ void factor_columns(double (&v)[kTile], int r, bool& ok) {
-    if constexpr (J < kTile) {
-        const double d = row_bcast<J>(v[J]);
-        if (!(d > 0) || !isfinite(d)) ok = false;
-        const double is = rsqrt_nr(d);
-        v[J] = r == J ? is : (r > J ? v[J] * is : 0.0);
-        // rows above J hold zeros in column J; entries right of the diagonal are never read
-        factor_update<J, J + 1>(v);
-        factor_columns<J + 1>(v, r, ok);
-    }
+// Row Q of M = L^-1 (forward elimination): lane (r, g) of the wave holds columns 4 cc + g of
+// row r of M in m[cc]; v[Q] is column Q of L (final) with 1 / L[Q][Q] on the diagonal lane.
+template <int Q>
+__device__ __forceinline__ void inverse_row(double (&m)[4], const double (&v)[kTile], int r) {
+    const double sc = r == Q ? row_bcast<Q>(v[Q]) : 1.0;
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+        if (4 * cc <= Q) m[cc] *= sc;
+    const double lq = r > Q ? v[Q] : 0.0;
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+        if (4 * cc <= Q) {
+            m[cc] = fma(-lq, row_bcast<Q>(m[cc]), m[cc]);
+            asm volatile("" : "+v"(m[cc]));
+        }
 }
 
-// Rows Q.. of M = L^-1 (forward elimination, one row per step): lane (r, g) of the wave holds
-// columns 4 cc + g of row r of M in m[cc]; v is row r of L with 1 / L[r][r] on the diagonal.
-template <int Q>
-__device__ __forceinline__ void inverse_rows(double (&m)[4], const double (&v)[kTile], int r) {
-    if constexpr (Q < kTile) {
-        const double sc = r == Q ? row_bcast<Q>(v[Q]) : 1.0;
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc)
-            if (4 * cc <= Q) m[cc] *= sc;
-        const double lq = r > Q ? v[Q] : 0.0;
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc)
-            if (4 * cc <= Q) m[cc] = fma(-lq, row_bcast<Q>(m[cc]), m[cc]);
-        inverse_rows<Q + 1>(m, v, r);
+// Step J of the right-looking factor (column J of L: pivot, scale, rank-1 update of the columns
+// right of J) fused with row J - 1 of the inverse, which needs only column J - 1 of L: the two
+// dependency chains are independent within a step and interleave. Lane r of a 16-lane row holds
+// row r of the tile; every pivot must be positive and finite.
+template <int J>
+__device__ __forceinline__ void factor_inverse(double (&v)[kTile], double (&m)[4], int r, bool& ok) {
+    if constexpr (J <= kTile) {
+        if constexpr (J < kTile) {
+            const double d = row_bcast<J>(v[J]);
+            ok = ok && d > 0 && isfinite(d);
+            const double is = rsqrt_nr(d);
+            const double sc = v[J] * is;
+            v[J] = r > J ? sc : (r == J ? is : 0.0);
+            // rows above J hold zeros in column J; entries right of the diagonal are never read
+            factor_update<J, J + 1>(v);
+        }
+        if constexpr (J > 0) inverse_row<J - 1>(m, v, r);
+        factor_inverse<J + 1>(v, m, r, ok);
     }
 }
 
 // Cholesky factor and inverse of one SPD 16x16 tile; every 16-lane row of the wave holds the
 // tile row-wise (v[c] = A[lane % 16][c], c <= lane % 16 used). The factor is formed redundantly
 // in the four rows (the diagonal keeps 1 / L[j][j]), the inverse split over them by column
-// (c % 4 = row index of the lane group); L^-1 goes to inv in operand layout. Returns false when
-// a pivot is not positive and finite.
-__device__ bool factor_diag_tile(double (&v)[kTile], double* inv) {
+// (c % 4 = row index of the lane group); L^-1 goes to inv (LDS) and ginv (the tile's slot in the
+// tile store, read by the substitutions) in operand layout. Returns false when a pivot is not
+// positive and finite.
+__device__ bool factor_diag_tile(double (&v)[kTile], double* inv, gdouble* ginv) {
     const int lane = threadIdx.x % 64, r = lane % kTile, g = lane / kTile;
     bool ok = true;
-    factor_columns<0>(v, r, ok);
     double m[4];
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) m[cc] = r == 4 * cc + g ? 1.0 : 0.0;
-    inverse_rows<0>(m, v, r);
+    factor_inverse<0>(v, m, r, ok);
 #pragma unroll
-    for (int cc = 0; cc < 4; ++cc) inv[cc * 64 + lane] = m[cc];
+    for (int cc = 0; cc < 4; ++cc) {
+        inv[cc * 64 + lane] = m[cc];
+        ginv[cc * 64 + lane] = m[cc];
+    }
     return ok;
 }
 
-// Updates the tile (ti, tj) of the trailing matrix with the solved panel of column k:
-// acc -= X_ti X_tj^T (panel tiles indexed relative to k).
+// Updates the tile (ti, tj) of the trailing matrix with the solved panel of column k,
+// T -= X_ti X_tj^T (panel tiles indexed relative to k). The accumulator holds T in operand
+// layout, which is the accumulator layout of T^T: the product is formed as T^T -= X_tj X_ti^T.
 __device__ __forceinline__ d4 trailing_mfma(d4 acc, const double* panel, int ai, int aj, int lane) {
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
-        const double a = -panel[(ai * 4 + kk) * 64 + lane];
-        const double b = panel[(aj * 4 + kk) * 64 + lane];
+        const double a = -panel[(aj * 4 + kk) * 64 + lane];
+        const double b = panel[(ai * 4 + kk) * 64 + lane];
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     return acc;
 }
 
-// Accumulator register rr of a lane -> tile element (i, j).
-__device__ __forceinline__ void acc_elem(int rr, int lane, int& i, int& j) { i = 4 * rr + lane / 16; j = lane % 16; }
-
+// Accumulator registers of a tile in operand layout -> LDS tile (lane-linear, conflict free).
 __device__ __forceinline__ void acc_to_panel(const d4& acc, double* tile, int lane) {
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-        int i, j;
-        acc_elem(rr, lane, i, j);
-        tile[opnd_pos(i, j)] = acc[rr];
-    }
+    for (int rr = 0; rr < 4; ++rr) tile[rr * 64 + lane] = acc[rr];
 }
 
 constexpr int kRegSlots = 7;   // register-resident trailing tiles per trailing wave
 constexpr int kGlobBatch = 3;  // tile-store trailing tiles in flight per wave
 
 // Panel solve of tile column k: X_a = A_a L_kk^-T for the panel tiles a = 1.. (relative to k),
-// written to the LDS panel (operand layout) and to the tile store; wave 0 first applies
-// L_kk^-1 to the right-hand side block y_k.
+// written to the LDS panel (operand layout) and to the tile store. Wave 0 solves only X_1, the
+// operand of the next diagonal tile, into LDS (its tile-store copy is written after the barrier
+// by wave 1, so wave 0 has no stores in flight when it needs its prefetched diagonal tile); the
+// other waves solve X_2.. round-robin, and the last wave first applies L_kk^-1 to the
+// right-hand side block y_k.
+__device__ __forceinline__ void solve_panel_tile(gdouble* Lt, double* panel, const double* inv, int k, int a, int lane,
+                                                 bool to_store) {
+    d4 acc = {0.0, 0.0, 0.0, 0.0};  // X^T = L_kk^-1 A_a^T: X in operand layout
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(inv[kk * 64 + lane], panel[(a * 4 + kk) * 64 + lane], acc, 0, 0, 0);
+    acc_to_panel(acc, panel + a * kTileDoubles, lane);
+    if (to_store) {
+        const int base = tile_base(k + a, k);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) Lt[base + rr * 64 + lane] = acc[rr];
+    }
+}
+
 __device__ __forceinline__ void panel_solve(gdouble* Lt, double* panel, const double* inv, double* y, int k, int m,
                                             int wave, int lane, int nw) {
     if (wave == 0) {
+        if (m >= 1) solve_panel_tile(Lt, panel, inv, k, 1, lane, false);
+        return;
+    }
+    if (wave == nw - 1) {
         double yr = 0;
         if (lane < kTile) {
 #pragma unroll
@@ -1360,41 +1400,42 @@ __device__ __forceinline__ void panel_solve(gdouble* Lt, double* panel, const do
         wave_sync();
         if (lane < kTile) y[kTile * k + lane] = yr;
     }
-    for (int a = 1 + wave; a <= m; a += nw) {
-        d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int a = 1 + wave; a <= m; a += nw - 1) solve_panel_tile(Lt, panel, inv, k, a, lane, true);
+}
+
+// Tile-store copy of the LDS panel tile X_1 of column k.
+__device__ __forceinline__ void store_panel_tile1(gdouble* Lt, const double* panel, int k, int lane) {
+    const int base = tile_base(k + 1, k);
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(panel[(a * 4 + kk) * 64 + lane], inv[kk * 64 + lane], acc, 0, 0, 0);
-        const int base = tile_base(k + a, k);
-        acc_to_panel(acc, panel + a * kTileDoubles, lane);
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) Lt[base + rr * 64 + lane] = acc[rr];
-    }
+    for (int rr = 0; rr < 4; ++rr) Lt[base + rr * 64 + lane] = panel[kTileDoubles + rr * 64 + lane];
 }
 
 // Lower Cholesky of the padded tile store Lt (T x T tiles) with the forward substitution of y
-// (LDS, 16 T entries, padding zero) fused. pnl: two LDS panels; dinv: inverse diagonal tiles.
-// Wave 0 runs the diagonal tiles (update, factor, inverse: one step ahead of the trailing
+// (LDS, 16 T entries, padding zero) fused. pnl: two LDS panels; dtile: the diagonal tiles 1..
+// (LDS, accumulator layout); dcur: the inverse diagonal tiles of the current and the next step
+// (LDS). Wave 0 runs the diagonal tiles (update, factor, inverse: one step ahead of the trailing
 // update); the other waves run the trailing update. The off-diagonal trailing tiles with the
 // largest column index (register_tile order) stay in accumulator registers of the trailing
-// waves for the whole factorization; the others and the diagonal tiles are read from and
-// written back to the tile store at every update. The two roles run separate step loops with
-// the same barrier sequence, so the resident tiles do not constrain the factor wave's registers.
-__device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dinv, double* y, int* flag) {
+// waves for the whole factorization, the diagonal tiles in LDS; the other off-diagonal tiles are
+// read from and written back to the tile store at every update. The inverse diagonal tiles
+// replace the diagonal tiles of the tile store. The two roles run separate step loops with the
+// same barrier sequence, so the resident tiles do not constrain the factor wave's registers.
+__device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dtile, double* dcur, double* y,
+                                                 int* flag) {
     gdouble* Lt = (gdouble*)Lt_;
     const int t = threadIdx.x, nt = blockDim.x, lane = t % 64, nw = nt / 64, nwt = nw - 1;
     const int wave = __builtin_amdgcn_readfirstlane(t / 64);
     constexpr int PANEL = kMaxTiles * kTileDoubles;
     for (int p = t; p < T * kTileDoubles; p += nt) {
         const int a = p / kTileDoubles, e = p % kTileDoubles;
-        int i, j;
-        acc_elem(e / 64, e % 64, i, j);
-        pnl[a * kTileDoubles + opnd_pos(i, j)] = Lt[tile_base(a, 0) + e];
+        pnl[p] = Lt[tile_base(a, 0) + e];
+        if (a > 0) dtile[p] = Lt[tile_base(a, a) + e];
     }
     __syncthreads();
+    auto inv_of = [&](int k) { return dcur + (k & 1) * kTileDoubles; };
     if (wave == 0) {
         // diagonal tile k: factor and inverse
-        auto factor = [&](double (&v)[kTile], int k) { return factor_diag_tile(v, dinv + k * kTileDoubles); };
+        auto factor = [&](double (&v)[kTile], int k) { return factor_diag_tile(v, inv_of(k), Lt + tile_base(k, k)); };
         {
             double v[kTile];
 #pragma unroll
@@ -1409,11 +1450,10 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
             const int m = T - k - 1;
             d4 acc = {0.0, 0.0, 0.0, 0.0};
             if (m > 0) {  // the next diagonal tile is final since the last step: fetch it early
-                const int base = tile_base(k + 1, k + 1);
 #pragma unroll
-                for (int rr = 0; rr < 4; ++rr) acc[rr] = Lt[base + rr * 64 + lane];
+                for (int rr = 0; rr < 4; ++rr) acc[rr] = dtile[(k + 1) * kTileDoubles + rr * 64 + lane];
             }
-            panel_solve(Lt, panel, dinv + k * kTileDoubles, y, k, m, wave, lane, nw);
+            panel_solve(Lt, panel, inv_of(k), y, k, m, wave, lane, nw);
             __syncthreads();
             if (m > 0) {
                 acc = trailing_mfma(acc, panel, 1, 1, lane);
@@ -1443,16 +1483,25 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
             const int base = tile_base(rti[s], rtj[s]);
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) racc[s][rr] = Lt[base + rr * 64 + lane];
+        } else {
+            racc[s] = d4{0.0, 0.0, 0.0, 0.0};
         }
     }
+    // the loads complete here, once: a load still pending at the step loop would make the first
+    // accumulator use of every step wait for all memory traffic in flight
+#pragma unroll
+    for (int s = 0; s < kRegSlots; ++s)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) asm volatile("" ::"v"(racc[s][rr]));
     __syncthreads();
     for (int k = 0; k < T; ++k) {
         if (*flag) return false;
         double* panel = pnl + (k & 1) * PANEL;
         double* next = pnl + ((k + 1) & 1) * PANEL;
         const int m = T - k - 1;
-        panel_solve(Lt, panel, dinv + k * kTileDoubles, y, k, m, wave, lane, nw);
+        panel_solve(Lt, panel, inv_of(k), y, k, m, wave, lane, nw);
         __syncthreads();
+        if (wave == 1 && m >= 1) store_panel_tile1(Lt, panel, k, lane);
         // right-hand side rows below the block
         for (int rr = (wave - 1) * 64 + lane; rr < kTile * m; rr += nwt * 64) {
             const int a = 1 + rr / kTile, i = rr % kTile;
@@ -1461,11 +1510,10 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
             for (int c = 0; c < kTile; ++c) s = fma(panel[a * kTileDoubles + opnd_pos(i, c)], y[kTile * k + c], s);
             y[kTile * (k + a) + i] -= s;
         }
-        // tile-store items of this step: diagonal tiles k + 2.., then the off-diagonal tiles
-        // outside the register set with column > k
-        const int nd = max(0, T - k - 2);
+        // tile-store items of this step: the off-diagonal tiles outside the register set with
+        // column > k
         const int Fk = k * (T - 1) - k * (k + 1) / 2;
-        const int nitems = nd + max(0, G - Fk);
+        const int nitems = max(0, G - Fk);
         d4 gacc[kGlobBatch];
         int gi[kGlobBatch], gj[kGlobBatch];
         int w0 = wave - 1;
@@ -1476,8 +1524,7 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
                 gi[b] = -1;
                 gj[b] = 0;
                 if (w < nitems) {
-                    if (w < nd) gi[b] = gj[b] = k + 2 + w;
-                    else store_tile(T, Fk + w - nd, gi[b], gj[b]);
+                    store_tile(T, Fk + w, gi[b], gj[b]);
                     const int base = tile_base(gi[b], gj[b]);
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr) gacc[b][rr] = Lt[base + rr * 64 + lane];
@@ -1505,6 +1552,16 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
             racc[s] = trailing_mfma(racc[s], panel, rti[s] - k, rtj[s] - k, lane);
             if (rtj[s] == k + 1) acc_to_panel(racc[s], next + (rti[s] - k - 1) * kTileDoubles, lane);
         }
+        // diagonal tiles k + 2.. (LDS)
+        for (int w = wave - 1; w < T - k - 2; w += nwt) {
+            double* dt = dtile + (k + 2 + w) * kTileDoubles;
+            d4 acc;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) acc[rr] = dt[rr * 64 + lane];
+            acc = trailing_mfma(acc, panel, w + 2, w + 2, lane);
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) dt[rr * 64 + lane] = acc[rr];
+        }
         finish_batch();
         for (w0 += kGlobBatch * nwt; w0 < nitems; w0 += kGlobBatch * nwt) {
             load_batch();
@@ -1517,14 +1574,16 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
 
 // L^T x = y in place (x in LDS, 16 T entries) with the panel tiles from the tile store and the
 // inverse diagonal tiles from dinv. part: 16 doubles per wave of LDS.
+// Every wave holds kBackSlots panel tiles of a tile column (T - 1 <= kBackSlots x waves).
+constexpr int kBackSlots = (kMaxTiles - 1 + kCholThreads / 64 - 1) / (kCholThreads / 64);
 __device__ void cholesky_backward(const double* Lt_, int T, const double* dinv, double* x, double* part) {
     const gdouble* Lt = (const gdouble*)Lt_;
     const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
-    d4 cur[2], nxt[2];
+    d4 cur[kBackSlots], nxt[kBackSlots];
     for (int k = T - 1; k >= 0; --k) {
         if (k == T - 1) {
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < kBackSlots; ++s) {
                 const int a = k + 1 + wave + s * nw;
                 if (a < T)
 #pragma unroll
@@ -1533,24 +1592,29 @@ __device__ void cholesky_backward(const double* Lt_, int T, const double* dinv, 
         }
         if (k > 0) {
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < kBackSlots; ++s) {
                 const int a = k + wave + s * nw;
                 if (a < T)
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr) nxt[s][rr] = Lt[tile_base(a, k - 1) + rr * 64 + lane];
             }
         }
-        double p = 0;
+        // (L_ak^T x_a)[c]: the lane holds L_ak[r][4 rr + q] (r = lane % 16, q = lane / 16)
+        double p[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < kBackSlots; ++s) {
             const int a = k + 1 + wave + s * nw;
-            if (a < T)
+            if (a < T) {
+                const double xr = x[kTile * a + lane % 16];
 #pragma unroll
-                for (int rr = 0; rr < 4; ++rr) p = fma(cur[s][rr], x[kTile * a + 4 * rr + lane / 16], p);
+                for (int rr = 0; rr < 4; ++rr) p[rr] = fma(cur[s][rr], xr, p[rr]);
+            }
         }
-        p += __shfl_xor(p, 16, 64);
-        p += __shfl_xor(p, 32, 64);
-        if (lane < kTile) part[wave * kTile + lane] = p;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            p[rr] = row16_sum(p[rr]);
+            if (lane % 16 == 0) part[wave * kTile + 4 * rr + lane / 16] = p[rr];
+        }
         __syncthreads();
         if (wave == 0) {
             const double* inv = dinv + k * kTileDoubles;
@@ -1566,7 +1630,7 @@ __device__ void cholesky_backward(const double* Lt_, int T, const double* dinv, 
         }
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 2; ++s) cur[s] = nxt[s];
+        for (int s = 0; s < kBackSlots; ++s) cur[s] = nxt[s];
     }
 }
 
@@ -1585,16 +1649,14 @@ __device__ void cholesky_solve_again(const double* Lt_, int T, const double* din
         }
         __syncthreads();
         for (int a = k + 1 + wave; a < T; a += nw) {
+            // (L_ak x_k)[r]: the lane holds L_ak[r][4 rr + q] (r = lane % 16, q = lane / 16)
             const int base = tile_base(a, k);
+            double v = 0;
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                double v = Lt[base + rr * 64 + lane] * x[kTile * k + lane % 16];
-                v += __shfl_xor(v, 1, 64);
-                v += __shfl_xor(v, 2, 64);
-                v += __shfl_xor(v, 4, 64);
-                v += __shfl_xor(v, 8, 64);
-                if (lane % 16 == 0) x[kTile * a + 4 * rr + lane / 16] -= v;
-            }
+            for (int rr = 0; rr < 4; ++rr) v = fma(Lt[base + rr * 64 + lane], x[kTile * k + 4 * rr + lane / 16], v);
+            v += __shfl_xor(v, 16, 64);
+            v += __shfl_xor(v, 32, 64);
+            if (lane < kTile) x[kTile * a + lane] -= v;
         }
         __syncthreads();
     }
@@ -1626,7 +1688,8 @@ __device__ double block_max_waves(double v, double* red) {
 
 __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int refinement) {
     __shared__ double pnl[2 * kMaxTiles * kTileDoubles];
-    __shared__ double dinv[kMaxTiles * kTileDoubles];
+    __shared__ double dtile[kMaxTiles * kTileDoubles];
+    __shared__ double dcur[2 * kTileDoubles];
     __shared__ double x[kMaxTiles * kTile];
     __shared__ double rres[kMaxTiles * kTile];
     __shared__ double part[kCholThreads / 64 * kTile];
@@ -1639,11 +1702,18 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     if (t == 0) flag = 0;
     for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[i] : 0.0;
     __syncthreads();
-    const bool ok = cholesky_forward(w.fac, T, pnl, dinv, x, &flag);
+    const bool ok = cholesky_forward(w.fac, T, pnl, dtile, dcur, x, &flag);
     if (!ok) {
         if (t == 0) w.lm->chol_ok = 0;
         return;
     }
+    // inverse diagonal tiles (diagonal slots of the tile store) -> LDS for the substitutions
+    double* dinv = pnl;
+    for (int p = t; p < T * kTileDoubles; p += nt) {
+        const int a = p / kTileDoubles;
+        dinv[p] = w.fac[tile_base(a, a) + p % kTileDoubles];
+    }
+    __syncthreads();
     cholesky_backward(w.fac, T, dinv, x, part);
     for (int it = 0; it < refinement; ++it) {
         for (int i = wave; i < N; i += nw) {
