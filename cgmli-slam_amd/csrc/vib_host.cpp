@@ -58,7 +58,9 @@ struct Arena {
 
 }  // namespace
 
-int HostWindow::chunk_size = 128;
+// 64 landmarks per chunk workgroup: a 20000-landmark window spreads over 313 workgroups (more
+// than one per CU), against twice the chunk partials to reduce (measured against 32..192).
+int HostWindow::chunk_size = 64;
 
 // ------------------------------------------------------------------------------------------
 // Host-side window copy with derived layout.

@@ -313,14 +313,24 @@ constexpr int kAccSize = kMaxVisual * (kMaxVisual + 1) / 2 + kMaxVisual;
 // Per-factor landmark terms go to fac_u / fac_lm in landmark order (k_lm_sums adds them up).
 // ---------------------------------------------------------------------------------------------
 constexpr int kGramCols = 16;
+// Per-wave row staging, column-major: element (row, col) at col * kRowPitch + row. A lane writes
+// its two rows of a column as one 16-byte store (consecutive lanes contiguous); the MFMA reads
+// (16 columns x 4 rows per step) fall on distinct bank pairs since kRowPitch = 4 mod 32.
+constexpr int kRowPitch = 132;
 constexpr int kWaves = kBlock / 64;
+constexpr int kGramDim = 14;  // [J_anchor (6) | J_rig (6) | J_td | r]
+constexpr int kGramEntries = kGramDim * (kGramDim + 1) / 2;
+constexpr int kGramsPerWave = 4;  // listed Gram matrices per wave and scatter pass
+__device__ __forceinline__ int gram_entry(int i, int j) { return i * kGramDim - i * (i - 1) / 2 + (j - i); }
 
 __device__ __forceinline__ void acc_add(double* acc, int nv, int ci, int cj, double v) {
     if (ci < 0 || cj < 0) return;
-    atomicAdd(&acc[ci <= cj ? tri_index(ci, cj, nv) : tri_index(cj, ci, nv)], v);
+    acc[ci <= cj ? tri_index(ci, cj, nv) : tri_index(cj, ci, nv)] += v;
 }
 
-// Gram entry G[i][j] (i <= j) of pair (a, b) into the visual block and gradient.
+// Gram entry G[i][j] (i <= j) of pair (a, b) into the visual block and gradient. The elements
+// one entry adds to are determined by the entry and the pair's blocks, so different entries of
+// one Gram matrix never meet.
 __device__ __forceinline__ void scatter_gram(double* acc, double* gacc, const WinDev& w, int a, int b, int i, int j,
                                              double g) {
     const int nv = w.nv;
@@ -355,12 +365,12 @@ __device__ __forceinline__ void scatter_gram(double* acc, double* gacc, const Wi
         }
     } else if (j == 13) {  // gradient J^T r
         if (i < 6) {
-            if (ac >= 0) atomicAdd(&gacc[ac + i], g);
-            if (bc >= 0) atomicAdd(&gacc[bc + i], -g);
+            if (ac >= 0) gacc[ac + i] += g;
+            if (bc >= 0) gacc[bc + i] -= g;
         } else if (i < 12) {
-            if (rc >= 0 && !same) atomicAdd(&gacc[rc + i - 6], g);
+            if (rc >= 0 && !same) gacc[rc + i - 6] += g;
         } else if (i == 12) {
-            if (tc >= 0) atomicAdd(&gacc[tc], g);
+            if (tc >= 0) gacc[tc] += g;
         }
     }
 }
@@ -382,8 +392,12 @@ template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* lin) {
     __shared__ double acc[kAccSize];
     __shared__ FrameTable ft;
-    __shared__ double rows[kWaves][128 * kGramCols];  // per wave: 2 rows per lane x 16 columns
+    __shared__ double rows[kWaves][kGramCols * kRowPitch];  // per wave: 2 rows per lane x 16 columns
     __shared__ int keys[kWaves][64];
+    __shared__ int nruns[2][kWaves];
+    __shared__ double glist[kWaves][kGramsPerWave][kGramEntries];
+    __shared__ int gkey[kWaves][kGramsPerWave];
+    __shared__ int ngram[kWaves];
     __shared__ double red[kBlock];
     const Chunk ch = lin[blockIdx.x];
     if (ch.local < 0) return;  // an empty position of the XCD-ordered work list
@@ -401,11 +415,24 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     const size_t F = (size_t)w.F;
     double cost = 0;
     int invalid = 0;
-    for (int p0 = ch.lm_begin + 64 * wave; p0 < ch.lm_end; p0 += kBlock) {
-        const int p = p0 + lane;
+    const int nrounds = (ch.lm_end - ch.lm_begin + kBlock - 1) / kBlock;  // the same for every wave
+    // scatter threads: Gram entry t = (gi, gj), summed over consecutive listed matrices of one
+    // pair (ckey) before it is added to the accumulator
+    int gi = 0, gj = t < kGramEntries ? t : 0, ckey = -1;
+    double cval = 0;
+    while (gj >= kGramDim - gi) {  // packed upper index -> (gi, gj); ends for every t < 105
+        gj -= kGramDim - gi;
+        ++gi;
+    }
+    gj += gi;
+    for (int round = 0; round < nrounds; ++round) {
+        const int p = ch.lm_begin + round * kBlock + 64 * wave + lane;
         int key = -1;
 #pragma unroll
-        for (int c = 0; c < 2 * kGramCols; ++c) V[2 * kGramCols * lane + c] = 0.0;
+        for (int c = 0; c < kGramCols; ++c) {
+            V[c * kRowPitch + 2 * lane] = 0.0;
+            V[c * kRowPitch + 2 * lane + 1] = 0.0;
+        }
         if (p < ch.lm_end) {
             const int ids = w.pf_ids[p];
             LmData lm;
@@ -425,15 +452,14 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
                 cost += huber_cost(w.huber, sq);
                 const double r0 = J.w * J.r[0], r1 = J.w * J.r[1];
                 const bool same = lm.frame == (fc >> 8);
-                double* v0 = V + 2 * kGramCols * lane;
-                double* v1 = v0 + kGramCols;
+                double* v0 = V + 2 * lane;  // rows 2 lane (v0[c kRowPitch]) and 2 lane + 1 (v0[... + 1])
 #pragma unroll
                 for (int k = 0; k < 6; ++k) {
-                    v0[k] = J.Ja[k]; v1[k] = J.Ja[6 + k];
-                    v0[6 + k] = J.Jr[k]; v1[6 + k] = J.Jr[6 + k];
+                    v0[k * kRowPitch] = J.Ja[k]; v0[k * kRowPitch + 1] = J.Ja[6 + k];
+                    v0[(6 + k) * kRowPitch] = J.Jr[k]; v0[(6 + k) * kRowPitch + 1] = J.Jr[6 + k];
                 }
-                v0[12] = J.Jt[0]; v1[12] = J.Jt[1];
-                v0[13] = r0; v1[13] = r1;
+                v0[12 * kRowPitch] = J.Jt[0]; v0[12 * kRowPitch + 1] = J.Jt[1];
+                v0[13 * kRowPitch] = r0; v0[13 * kRowPitch + 1] = r1;
                 key = lm.frame * 32 + (fc >> 8);
                 if (!same)
                     for (int k = 0; k < 6; ++k) u[k] = J.Ja[k] * J.Jrho[0] + J.Ja[6 + k] * J.Jrho[1];
@@ -459,28 +485,63 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
         }
         keys[wave][lane] = key;
         wave_sync();
-        // runs of one pair: Gram matrix on MFMA, then one scatter per run
+        // runs of one pair: Gram matrix on MFMA. A run occupies the lanes between its first and
+        // last factor (pair order), so only the MFMA steps over its rows run; the rows of other
+        // keys inside a boundary step are masked to zero. Every wave lists the packed upper
+        // entries of up to kGramsPerWave Gram matrices in LDS, then the workgroup scatters the
+        // lists, one thread per Gram entry and the lists in wave order: the entries of one
+        // Gram matrix never share an accumulator element, so the scatter needs no atomics and
+        // its sums do not depend on scheduling.
         unsigned long long pending = __ballot(key >= 0);
-        while (pending) {
-            const int leader = __ffsll(pending) - 1;
-            const int pk = __shfl(key, leader, 64);
-            d4 G = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-            for (int st = 0; st < 32; ++st) {
-                const int row = 4 * st + lane / 16;
-                const double x = keys[wave][row >> 1] == pk ? V[row * kGramCols + lane % 16] : 0.0;
-                G = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, G, 0, 0, 0);
-            }
-            const int a = pk / 32, b = pk % 32;
+        const int prev = __shfl_up(key, 1, 64);
+        const unsigned long long starts = __ballot(key >= 0 && (lane == 0 || prev != key));
+        if (lane == 0) nruns[round & 1][wave] = __popcll(starts);
+        __syncthreads();
+        int maxruns = 0;
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                const int i = 4 * rr + lane / 16, j = lane % 16;
-                if (i <= j && j < 14) scatter_gram(acc, gacc, w, a, b, i, j, G[rr]);
+        for (int q = 0; q < kWaves; ++q) maxruns = max(maxruns, nruns[round & 1][q]);
+        for (int b0 = 0; b0 < maxruns; b0 += kGramsPerWave) {
+            int cnt = 0;
+            while (pending && cnt < kGramsPerWave) {
+                const int leader = __ffsll(pending) - 1;
+                const int pk = __shfl(key, leader, 64);
+                const unsigned long long run = __ballot(key == pk);
+                const int last = 63 - __clzll(run);
+                d4 G = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+                for (int st = leader / 2; st <= last / 2; ++st) {
+                    const int row = 4 * st + lane / 16;
+                    const double x = keys[wave][row >> 1] == pk ? V[(lane % 16) * kRowPitch + row] : 0.0;
+                    G = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, G, 0, 0, 0);
+                }
+                double* gl = glist[wave][cnt];
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int i = 4 * rr + lane / 16, j = lane % 16;
+                    if (i <= j && j < kGramDim) gl[gram_entry(i, j)] = G[rr];
+                }
+                if (lane == 0) gkey[wave][cnt] = pk;
+                ++cnt;
+                pending &= ~run;
             }
-            pending &= ~__ballot(key == pk);
+            if (lane == 0) ngram[wave] = cnt;
+            __syncthreads();
+            if (t < kGramEntries)
+                for (int q = 0; q < kWaves; ++q)
+                    for (int g = 0; g < ngram[q]; ++g) {
+                        const int k2 = gkey[q][g];
+                        if (k2 != ckey) {
+                            if (ckey >= 0) scatter_gram(acc, gacc, w, ckey / 32, ckey % 32, gi, gj, cval);
+                            ckey = k2;
+                            cval = 0;
+                        }
+                        cval += glist[q][g][t];
+                    }
+            __syncthreads();
         }
         wave_sync();
     }
+    if (t < kGramEntries && ckey >= 0) scatter_gram(acc, gacc, w, ckey / 32, ckey % 32, gi, gj, cval);
     __syncthreads();
     double* part = w.lin_part + (size_t)ch.local * (tri + nv);
     for (int i = threadIdx.x; i < tri + nv; i += kBlock) part[i] = acc[i];

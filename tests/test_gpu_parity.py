@@ -344,6 +344,28 @@ def test_second_new_matches_oracle(ctx, oracle, abi, name):
     assert np.abs(got.lm_rho - exp.lm_rho).max() < 1e-6 * max(1.0, np.abs(exp.lm_rho).max())
 
 
+def test_step_is_bitwise_deterministic(ctx, abi):
+    """The same step from the same state, solved five times: costs and the full tangent step are
+    bit-identical (every sum on the device, including the visual normal block accumulated in LDS
+    by the workgroups' waves, runs in a fixed order)."""
+    from vibpy import synth
+    w = synth.generate(K=12, L=4000, n_cams=4, obs_range=(2, 8), seed=66, rig_free=True, td_free=True)
+    lay = w.layout()
+    n = lay["nr"] + lay["ne"]
+    say(f"[gpu] determinism K={w.K} L={w.L} F={w.F}")
+    runs = []
+    for _ in range(5):
+        ctx.upload(0, w)
+        ctx.check(ctx.linearize(0, abi.VIB_FP64), "linearize")
+        st, rep = ctx.solve_step(0, 1e-4, abi.VIB_FP64, 1)
+        assert st == 0, ctx.error()
+        runs.append((rep.cost, rep.trial_cost, ctx.read_step(0, n)))
+        ctx.release(0)
+    for cost, trial, dx in runs[1:]:
+        assert cost == runs[0][0] and trial == runs[0][1]
+        assert np.array_equal(dx, runs[0][2])
+
+
 @pytest.mark.parametrize("name,cfg,solve_tol", [
     ("k10_l1500", dict(K=10, L=1500, n_cams=4, obs_range=(2, 8), seed=61, rig_free=True, td_free=True), (1e-8, 1e-6)),
     # depths 0.5-200 m, 30% low-parallax tracks, a near-static stretch: four LM steps that each
