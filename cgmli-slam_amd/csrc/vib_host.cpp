@@ -404,9 +404,6 @@ Batch::~Batch() {
     if (d_red) hipFree(d_red);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
-    for (hipEvent_t e : fork_join)
-        if (e) hipEventDestroy(e);
-    if (side) hipStreamDestroy(side);
 }
 
 bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err) {
@@ -481,7 +478,9 @@ bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
                        mode, mixed, s) == hipSuccess;
 }
 
-bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(1, s, 0, 0) && launch(2, s, 0, 0); }
+// Kernel 1 (the retained items linearized alone) is not part of an iteration: kernel 0 runs the
+// same items as extra workgroups, beside the visual factors of the same launch.
+bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(2, s, 0, 0); }
 
 bool Batch::step(hipStream_t s, int refinement, int mode) {
     for (int k = 3; k < kKernelCount; ++k)
@@ -489,36 +488,10 @@ bool Batch::step(hipStream_t s, int refinement, int mode) {
     return true;
 }
 
-// One LM iteration. The retained factors (IMU edges and prior: a few workgroups per window)
-// do not depend on the visual kernels, so their linearization runs on a side stream beside the
-// visual linearization and Schur elimination, and their trial cost beside the back-substitution
-// and the visual trial cost; the main stream joins each before the kernel that reads it
-// (k_assemble, k_lm_update). Both streams have the default priority.
-bool Batch::make_side() {
-    if (side) return true;
-    if (hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess) {
-        side = nullptr;
-        return false;
-    }
-    for (hipEvent_t& e : fork_join)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
-    return true;
-}
-
-bool Batch::iteration(hipStream_t s, int refinement) {
-    if (!make_side()) return false;
-    auto fork = [&](hipEvent_t e) { return hipEventRecord(e, s) == hipSuccess && hipStreamWaitEvent(side, e, 0) == hipSuccess; };
-    auto join = [&](hipEvent_t e) { return hipEventRecord(e, side) == hipSuccess && hipStreamWaitEvent(s, e, 0) == hipSuccess; };
-    return fork(fork_join[0]) && launch(1, side, 0, 0) && launch(0, s, 0, 0) && launch(2, s, 0, 0) &&
-           launch(3, s, refinement, 0) && launch(4, s, refinement, 0) && join(fork_join[1]) &&
-           launch(5, s, refinement, 0) && launch(6, s, refinement, 0) && fork(fork_join[2]) &&
-           launch(9, side, refinement, 0) && launch(7, s, refinement, 0) && launch(8, s, refinement, 0) &&
-           join(fork_join[3]) && launch(10, s, refinement, 0) && launch(11, s, refinement, 0);
-}
+bool Batch::iteration(hipStream_t s, int refinement) { return linearize(s) && step(s, refinement, 0); }
 
 bool Batch::capture(hipStream_t s, int refinement) {
     if (graph_exec) return true;
-    if (!make_side()) return false;  // before capture: stream and event creation stay out of it
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
     const bool ok = iteration(s, refinement);
     if (hipStreamEndCapture(s, &graph) != hipSuccess || !ok) return false;
@@ -1793,7 +1766,9 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
         for (int it = 0; ok && it < iterations; ++it)
             for (int k = 0; ok && k < kKernelCount; ++k) {
                 hipEvent_t* e = &kev[2 * (it * kKernelCount + k)];
-                ok = hipEventRecord(e[0], ctx->stream) == hipSuccess && batch.launch(k, ctx->stream, o->refinement_steps, 0) &&
+                const bool in_iteration = k != 1;  // the retained linearization runs inside kernel 0
+                ok = hipEventRecord(e[0], ctx->stream) == hipSuccess &&
+                     (!in_iteration || batch.launch(k, ctx->stream, o->refinement_steps, 0)) &&
                      hipEventRecord(e[1], ctx->stream) == hipSuccess;
             }
         ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;

@@ -94,15 +94,12 @@ struct Batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     size_t cap_wins = 0, cap_red = 0, cap_chunks = 0, cap_lin = 0;  // grow-only device buffers (rebuilt batches)
-    hipStream_t side = nullptr;        // the retained-factor kernels of an iteration run here
-    hipEvent_t fork_join[4] = {};      // overlapped with the visual kernels on the main stream
 
     ~Batch();
     bool build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err);
     bool launch(int kernel, hipStream_t s, int refinement, int mode);
     bool linearize(hipStream_t s);
     bool step(hipStream_t s, int refinement, int mode);
-    bool make_side();
     bool iteration(hipStream_t s, int refinement);
     bool capture(hipStream_t s, int refinement);
 };

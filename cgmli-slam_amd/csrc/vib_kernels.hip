@@ -388,8 +388,24 @@ __device__ __forceinline__ double load_u(const WinDev& w, size_t F, int k, int p
     return w.fac_u[k * F + p];
 }
 
+// IMU edges and prior of a window (k_retained below). Their linearization runs as extra
+// workgroups of the visual linearization, so it overlaps the visual work in one launch. (The
+// trial cost stays a launch of its own: inside k_trial_cost its 280 registers would halve that
+// kernel's occupancy.)
+template <int MODE>
+__device__ void retained_body(const WinDev& w, int y, int lane, double* scratch);
+// The retained items as extra workgroups of a chunk kernel: item i = (window i / per,
+// edge or prior i % per), one per wave.
+template <int MODE>
+__device__ __forceinline__ void retained_items(const WinDev* wins, int first_block, int items, int per,
+                                               double* scratch_of_wave) {
+    const int item = ((int)blockIdx.x - first_block) * (kBlock / 64) + (int)threadIdx.x / 64;
+    if (item < items) retained_body<MODE>(wins[item / per], item % per, threadIdx.x % 64, scratch_of_wave);
+}
+
+// Workgroups nlin.. run the retained items (linearization, mode 0), items / per as above.
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* lin) {
+__global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* lin, int nlin, int items, int per) {
     __shared__ double acc[kAccSize];
     __shared__ FrameTable ft;
     __shared__ double rows[kWaves][kGramCols * kRowPitch];  // per wave: 2 rows per lane x 16 columns
@@ -399,6 +415,10 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     __shared__ int gkey[kWaves][kGramsPerWave];
     __shared__ int ngram[kWaves];
     __shared__ double red[kBlock];
+    if ((int)blockIdx.x >= nlin) {  // the row staging of the wave holds the prior's scratch
+        retained_items<0>(wins, nlin, items, per, rows[threadIdx.x / 64]);
+        return;
+    }
     const Chunk ch = lin[blockIdx.x];
     if (ch.local < 0) return;  // an empty position of the XCD-ordered work list
     const WinDev& w = wins[ch.win];
@@ -869,11 +889,14 @@ __device__ const double* prior_state(const WinDev& w, const PriorBlockDev& b, co
 // mode 1 evaluates the cost at the trial state. Results go to per-edge slots that k_assemble /
 // k_lm_update combine in a fixed order. grid = (windows, max_edges + 1), block = one wave.
 
-__global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
-    const WinDev& w = wins[blockIdx.x];
+// One retained item of a window (IMU edge y < E, or the prior at y == E) on one wave; scratch:
+// 2 kMaxRetained doubles of the wave's own LDS (the prior's chart and residual). MODE 0
+// linearizes (residual, Jacobians, prior chart derivative), MODE 1 evaluates the trial cost.
+template <int MODE>
+__device__ void retained_body(const WinDev& w, int y, int lane, double* scratch) {
+    constexpr int mode = MODE;
     if (!window_active(w)) return;
     if (mode == 0 && !w.lm->new_point) return;
-    const int y = blockIdx.y, lane = threadIdx.x;
     const int E = w.E;
     const double* pose = mode == 0 ? w.pose : w.cpose;
     const double* sb = mode == 0 ? w.sb : w.csb;
@@ -915,8 +938,8 @@ __global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
         if (lane == 0) { psc[mode] = 0; psc[2 + mode] = 0; }
         return;
     }
-    __shared__ double chart[kMaxRetained];
-    __shared__ double res[kMaxRetained];
+    double* chart = scratch;
+    double* res = scratch + kMaxRetained;
     const int m = w.prior_rows, n = w.prior_cols, B = w.prior_blocks;
     for (int bi = lane; bi < B; bi += 64) {
         const PriorBlockDev b = w.pblk[bi];
@@ -963,6 +986,13 @@ __global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
         psc[mode] = cost;
         psc[2 + mode] = bad ? 1.0 : 0.0;
     }
+}
+
+// The retained items alone: grid (window, item), one wave each.
+__global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
+    __shared__ double scratch[2 * kMaxRetained];
+    if (mode == 0) retained_body<0>(wins[blockIdx.x], blockIdx.y, threadIdx.x, scratch);
+    else retained_body<1>(wins[blockIdx.x], blockIdx.y, threadIdx.x, scratch);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2258,12 +2288,13 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        int mixed, hipStream_t s) {
     using namespace vib;
     const int slices = (max_len + kRedEntries - 1) / kRedEntries;
+    const int rper = max_edges + 1, ritems = nwin * rper;  // retained items: IMU edges + prior
     switch (k) {
-        case 0:
-            if (nlin) {
-                if (mixed) hipLaunchKernelGGL(k_lin_visual<true>, dim3(nlin), dim3(kBlock), 0, s, wins, lin);
-                else hipLaunchKernelGGL(k_lin_visual<false>, dim3(nlin), dim3(kBlock), 0, s, wins, lin);
-            }
+        case 0: {  // visual factors, then the retained items in the same launch
+            const int grid = nlin + (ritems + kWaves - 1) / kWaves;
+            if (mixed) hipLaunchKernelGGL(k_lin_visual<true>, dim3(grid), dim3(kBlock), 0, s, wins, lin, nlin, ritems, rper);
+            else hipLaunchKernelGGL(k_lin_visual<false>, dim3(grid), dim3(kBlock), 0, s, wins, lin, nlin, ritems, rper);
+        }
             if (nchunks) {
                 if (mixed) hipLaunchKernelGGL(k_lm_sums<true>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
                 else hipLaunchKernelGGL(k_lm_sums<false>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
