@@ -1663,65 +1663,88 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
     return true;
 }
 
+// acc += sum_c inv[(c, r)] u[c] with u[c] taken from lane c of the lane's 16-lane row.
+template <int C>
+__device__ __forceinline__ void bcast_dot(double& acc, const double* inv, double u, int r) {
+    if constexpr (C < kTile) {
+        acc = fma(inv[opnd_pos(C, r)], row_bcast<C>(u), acc);
+        bcast_dot<C + 1>(acc, inv, u, r);
+    }
+}
+
 // L^T x = y in place (x in LDS, 16 T entries) with the panel tiles from the tile store and the
-// inverse diagonal tiles from dinv. part: 16 doubles per wave of LDS.
-// Every wave holds kBackSlots panel tiles of a tile column (T - 1 <= kBackSlots x waves).
-constexpr int kBackSlots = (kMaxTiles - 1 + kCholThreads / 64 - 1) / (kCholThreads / 64);
+// inverse diagonal tiles from dinv; one barrier per tile row. In round j wave 0 forms
+//   x_j = L_jj^-T (y_j - L_{j+1,j}^T x_{j+1} - sum of the other waves' partials of round j + 1)
+// and then L_{j,j-1}^T x_j for round j - 1 (its tile prefetched), while the other waves form
+// the partials sum_{a >= j+1} L_{a,j-1}^T x_a of round j - 1, which need x only up to x_{j+1}.
+// The next round's tiles are fetched after this round's work. part: two buffers (round parity)
+// of 16 doubles per wave, then 16 doubles for wave 0's term.
+constexpr int kBackWaves = kCholThreads / 64 - 1;                          // partial waves
+constexpr int kBackSlots = (kMaxTiles - 2 + kBackWaves - 1) / kBackWaves;  // tiles per partial wave
+constexpr int kBackPart = 2 * (kCholThreads / 64) * kTile + kTile;         // doubles of part
 __device__ void cholesky_backward(const double* Lt_, int T, const double* dinv, double* x, double* part) {
     const gdouble* Lt = (const gdouble*)Lt_;
-    const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
-    d4 cur[kBackSlots], nxt[kBackSlots];
-    for (int k = T - 1; k >= 0; --k) {
-        if (k == T - 1) {
-#pragma unroll
-            for (int s = 0; s < kBackSlots; ++s) {
-                const int a = k + 1 + wave + s * nw;
-                if (a < T)
-#pragma unroll
-                    for (int rr = 0; rr < 4; ++rr) cur[s][rr] = Lt[tile_base(a, k) + rr * 64 + lane];
-            }
-        }
-        if (k > 0) {
-#pragma unroll
-            for (int s = 0; s < kBackSlots; ++s) {
-                const int a = k + wave + s * nw;
-                if (a < T)
-#pragma unroll
-                    for (int rr = 0; rr < 4; ++rr) nxt[s][rr] = Lt[tile_base(a, k - 1) + rr * 64 + lane];
-            }
-        }
-        // (L_ak^T x_a)[c]: the lane holds L_ak[r][4 rr + q] (r = lane % 16, q = lane / 16)
-        double p[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int s = 0; s < kBackSlots; ++s) {
-            const int a = k + 1 + wave + s * nw;
-            if (a < T) {
-                const double xr = x[kTile * a + lane % 16];
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) p[rr] = fma(cur[s][rr], xr, p[rr]);
-            }
-        }
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            p[rr] = row16_sum(p[rr]);
-            if (lane % 16 == 0) part[wave * kTile + 4 * rr + lane / 16] = p[rr];
-        }
-        __syncthreads();
+    const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64, r = lane % 16;
+    double* tterm = part + 2 * nw * kTile;
+    d4 cur[kBackSlots], nxt[kBackSlots];  // partial waves: tiles (a, j - 1), a >= j + 1
+    d4 tcur = {0.0, 0.0, 0.0, 0.0}, tnxt = {0.0, 0.0, 0.0, 0.0};  // wave 0: tile (j, j - 1)
+    double tj = 0;                                                 // wave 0: (L_{j+1,j}^T x_{j+1})[r]
+    auto load_round = [&](int j, d4 (&pw)[kBackSlots], d4& w0) {  // the tiles of round j
+        if (j < 1) return;
         if (wave == 0) {
-            const double* inv = dinv + k * kTileDoubles;
-            double u = 0;
-            if (lane < kTile) {
-                u = x[kTile * k + lane];
-                for (int w = 0; w < nw; ++w) u -= part[w * kTile + lane];
-            }
-            double xr = 0;
 #pragma unroll
-            for (int c = 0; c < kTile; ++c) xr = fma(inv[opnd_pos(c, lane % kTile)], lane_bcast(u, c), xr);
-            if (lane < kTile) x[kTile * k + lane] = xr;
+            for (int rr = 0; rr < 4; ++rr) w0[rr] = Lt[tile_base(j, j - 1) + rr * 64 + lane];
+        } else {
+#pragma unroll
+            for (int s = 0; s < kBackSlots; ++s) {
+                const int a = j + wave + s * kBackWaves;
+                if (a < T)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) pw[s][rr] = Lt[tile_base(a, j - 1) + rr * 64 + lane];
+            }
         }
+    };
+    load_round(T - 1, cur, tcur);
+    for (int j = T - 1; j >= 0; --j) {
+        if (wave == 0) {
+            double u = x[kTile * j + r] - tj;
+            if (j + 1 < T)
+                for (int w = 1; w < nw; ++w) u -= part[(j & 1) * nw * kTile + w * kTile + r];
+            const double* inv = dinv + j * kTileDoubles;
+            double xr = 0;
+            bcast_dot<0>(xr, inv, u, r);  // every 16-lane row holds u, so row broadcasts serve
+            if (lane < kTile) x[kTile * j + lane] = xr;
+            if (j >= 1) {  // (L_{j,j-1}^T x_j)[c]: the lane holds L[r][4 rr + q] (q = lane / 16)
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const double v = row16_sum(tcur[rr] * xr);
+                    if (r == 0) tterm[4 * rr + lane / 16] = v;
+                }
+                wave_sync();
+                tj = tterm[r];
+            }
+        } else if (j >= 1) {
+            double p[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < kBackSlots; ++s) {
+                const int a = j + wave + s * kBackWaves;
+                if (a < T) {
+                    const double xa = x[kTile * a + r];
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) p[rr] = fma(cur[s][rr], xa, p[rr]);
+                }
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                p[rr] = row16_sum(p[rr]);
+                if (r == 0) part[((j - 1) & 1) * nw * kTile + wave * kTile + 4 * rr + lane / 16] = p[rr];
+            }
+        }
+        load_round(j - 1, nxt, tnxt);  // after this round's work: its loads overlap the barrier
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < kBackSlots; ++s) cur[s] = nxt[s];
+        tcur = tnxt;
     }
 }
 
@@ -1783,7 +1806,7 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     __shared__ double dcur[2 * kTileDoubles];
     __shared__ double x[kMaxTiles * kTile];
     __shared__ double rres[kMaxTiles * kTile];
-    __shared__ double part[kCholThreads / 64 * kTile];
+    __shared__ double part[kBackPart];
     __shared__ double red[kCholThreads / 64];
     __shared__ int flag;
     WinDev& w = wins[blockIdx.x];
