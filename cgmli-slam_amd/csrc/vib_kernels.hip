@@ -1178,7 +1178,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 4))) 
 // reduced camera system S and rhs (prepareStateSolve, VisualCudaBatch.cu:163-172).
 // red_lin / red_schur: reduced partials (tri + nv) per window.
 // ---------------------------------------------------------------------------------------------
-constexpr int kAssembleRows = 4;                       // rows per workgroup (one register row of a tile row)
+constexpr int kAssembleRows = 1;                       // rows per workgroup (320 workgroups at nr = 307)
 constexpr int kAssembleSplit = kTile / kAssembleRows;  // workgroups per tile row
 
 // Retained (IMU + prior) normal-matrix entry (i, j) from the per-edge Jacobians and the prior
@@ -1298,10 +1298,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
         w.Scopy[(size_t)i * nr + j] = damped(i, j);
     }
     for (int i = row0 + t; i < row1; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
-    // this workgroup's four rows of tile row ti of the factor tile store, in operand layout
-    // (padding identity)
-    for (int p = t; p < (ti + 1) * 64; p += nt) {
-        const int tj = p / 64, e = p % 64, rho = e / 16, c = e % 16;
+    // this workgroup's rows of tile row ti of the factor tile store, in operand layout (padding
+    // identity)
+    constexpr int per_tile = kAssembleRows * kTile;
+    for (int p = t; p < (ti + 1) * per_tile; p += nt) {
+        const int tj = p / per_tile, e = p % per_tile, rho = e / kTile, c = e % kTile;
         const int r = row0 + rho, cg = kTile * tj + c;
         w.fac[tile_base(ti, tj) + opnd_pos(kAssembleRows * q + rho, c)] =
             (r < nr && cg < nr) ? damped(r, cg) : (r == cg ? 1.0 : 0.0);
