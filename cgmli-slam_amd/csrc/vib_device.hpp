@@ -21,6 +21,7 @@ constexpr int kMaxRetained = 15 * kMaxFrames + 7;
 constexpr int kChunkLanes = 8;      // lanes per landmark in the per-landmark kernels
 constexpr int kBlock = 256;         // threads per chunk workgroup
 constexpr int kTile = 16;           // tile edge of the reduced-system factorization
+constexpr int kMaxTiles = (kMaxRetained + kTile - 1) / kTile;
 
 // Tile store of the reduced system / Cholesky factor: the lower-triangle 16x16 tiles of the
 // padded system (16 T x 16 T, T = ceil(nr / 16), padding rows and columns = identity), tile
@@ -100,6 +101,10 @@ struct WinDev {
     int32_t tri;               // nv*(nv+1)/2
     double huber, min_rho, td_lo, td_hi;
     int32_t pose_col[kMaxFrames], sb_col[kMaxFrames], sb_w[kMaxFrames];
+    // Reduced-system factorization order: the speed/bias columns first, then the visual columns
+    // (a rotation of the tangent layout: column c < nv goes to c + nr - nv, c >= nv to c - nv).
+    // chol_mask[k] bit i: tile (i, k) of the factor in that order may be nonzero (i >= k).
+    uint32_t chol_mask[kMaxTiles];
     // state and trial candidate
     double* pose; double* sb; double* rig; double* td; double* rho;
     double* cpose; double* csb; double* crig; double* ctd; double* crho;

@@ -222,6 +222,45 @@ bool HostWindow::layout(std::string& err) {
         const PriorBlockDev& d = pblk[f >> 8];
         colmap[4 * (size_t)(d.col + (f & 255)) + 2] = f;
     }
+    // structural tiles of the factor in the speed/bias-first order: the visual block is dense
+    // (landmark Schur complement), an IMU edge couples its four blocks, the prior all of its
+    // columns; then the fill of a tile Cholesky (tile (i, j) once (i, k) and (j, k) are).
+    {
+        const int nsb = nr - nv, T = (nr + kTile - 1) / kTile;
+        auto tile_of = [&](int col) { return (col < nv ? col + nsb : col - nv) / kTile; };
+        std::vector<uint8_t> S((size_t)T * T, 0);
+        auto mark = [&](const std::vector<int>& cols) {
+            for (int a : cols)
+                for (int b : cols) {
+                    const int ta = tile_of(a), tb = tile_of(b);
+                    S[(size_t)std::max(ta, tb) * T + std::min(ta, tb)] = 1;
+                }
+        };
+        std::vector<int> cols;
+        for (int c = 0; c < nv; ++c) cols.push_back(c);
+        mark(cols);
+        for (int e = 0; e < E; ++e) {
+            cols.clear();
+            for (int s : {imu_from[e], imu_to[e]}) {
+                if (pose_col[s] >= 0) for (int i = 0; i < 6; ++i) cols.push_back(pose_col[s] + i);
+                if (sb_col[s] >= 0) for (int i = 0; i < sb_width[s]; ++i) cols.push_back(sb_col[s] + i);
+            }
+            mark(cols);
+        }
+        cols.clear();
+        for (const int32_t f : pfree) cols.push_back(pblk[f >> 8].col + (f & 255));
+        mark(cols);
+        for (int i = 0; i < T; ++i) S[(size_t)i * T + i] = 1;
+        for (int k = 0; k < T; ++k)
+            for (int i = k + 1; i < T; ++i)
+                if (S[(size_t)i * T + k])
+                    for (int j = k + 1; j <= i; ++j)
+                        if (S[(size_t)j * T + k]) S[(size_t)i * T + j] = 1;
+        chol_mask.assign(kMaxTiles, 0);
+        for (int k = 0; k < T; ++k)
+            for (int i = k; i < T; ++i)
+                if (S[(size_t)i * T + k]) chol_mask[k] |= 1u << i;
+    }
     // chunks
     chunks.clear();
     const int cs = std::max(1, chunk_size);
@@ -291,6 +330,7 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     d.nv = h.nv; d.nr = h.nr; d.ne = h.ne; d.rig_col = h.rig_col; d.td_col = h.td_col; d.flags = h.flags;
     d.tri = h.nv * (h.nv + 1) / 2;
     d.huber = h.huber; d.min_rho = h.min_rho; d.td_lo = h.td_lo; d.td_hi = h.td_hi;
+    for (int k = 0; k < kMaxTiles; ++k) d.chol_mask[k] = k < (int)h.chol_mask.size() ? h.chol_mask[k] : 0;
     for (int k = 0; k < kMaxFrames; ++k) {
         d.pose_col[k] = k < h.K ? h.pose_col[k] : -1;
         d.sb_col[k] = k < h.K ? h.sb_col[k] : -1;

@@ -54,6 +54,7 @@ struct HostWindow {
     std::vector<PriorBlockDev> pblk;
     std::vector<int32_t> pfree;
     std::vector<int32_t> colmap;  // nr x 4 retained-factor column map
+    std::vector<uint32_t> chol_mask;  // structural tiles of the reduced-system factor (WinDev)
     std::vector<Chunk> chunks;
     // factors in (anchor frame, observer frame) pair order
     std::vector<double> pf_ob, pf_an;

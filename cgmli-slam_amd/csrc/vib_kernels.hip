@@ -1298,14 +1298,17 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
         w.Scopy[(size_t)i * nr + j] = damped(i, j);
     }
     for (int i = row0 + t; i < row1; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
-    // this workgroup's rows of tile row ti of the factor tile store, in operand layout (padding
-    // identity)
-    constexpr int per_tile = kAssembleRows * kTile;
-    for (int p = t; p < (ti + 1) * per_tile; p += nt) {
-        const int tj = p / per_tile, e = p % per_tile, rho = e / kTile, c = e % kTile;
-        const int r = row0 + rho, cg = kTile * tj + c;
-        w.fac[tile_base(ti, tj) + opnd_pos(kAssembleRows * q + rho, c)] =
-            (r < nr && cg < nr) ? damped(r, cg) : (r == cg ? 1.0 : 0.0);
+    // this workgroup's rows of the factor tile store, in the factorization order (speed/bias
+    // columns first; padding rows and columns stay in place as identity) and operand layout
+    const int nsb = nr - nv;
+    for (int rho = 0; rho < kAssembleRows; ++rho) {
+        const int r = row0 + rho;
+        const int P = r >= nr ? r : (r < nv ? r + nsb : r - nv);
+        for (int pc = t; pc < (P / kTile + 1) * kTile; pc += nt) {
+            const int c = pc >= nr ? pc : (pc < nsb ? pc + nv : pc - nsb);
+            w.fac[tile_base(P / kTile, pc / kTile) + opnd_pos(P % kTile, pc % kTile)] =
+                (r < nr && c < nr) ? damped(r, c) : (r == c ? 1.0 : 0.0);
+        }
     }
 }
 
@@ -1327,7 +1330,6 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
 // Two workgroup barriers per tile column.
 // ---------------------------------------------------------------------------------------------
 typedef __attribute__((address_space(1))) double gdouble;
-constexpr int kMaxTiles = (kMaxRetained + kTile - 1) / kTile;
 constexpr int kTileDoubles = kTile * kTile;
 constexpr int kCholThreads = 1024;
 static_assert(kCholThreads / 64 >= 3, "the factorization needs a diagonal wave and two trailing waves");
@@ -1478,9 +1480,9 @@ __device__ __forceinline__ void solve_panel_tile(gdouble* Lt, double* panel, con
 }
 
 __device__ __forceinline__ void panel_solve(gdouble* Lt, double* panel, const double* inv, double* y, int k, int m,
-                                            int wave, int lane, int nw) {
+                                            int wave, int lane, int nw, unsigned mk) {
     if (wave == 0) {
-        if (m >= 1) solve_panel_tile(Lt, panel, inv, k, 1, lane, false);
+        if (m >= 1 && ((mk >> (k + 1)) & 1u)) solve_panel_tile(Lt, panel, inv, k, 1, lane, false);
         return;
     }
     if (wave == nw - 1) {
@@ -1492,7 +1494,8 @@ __device__ __forceinline__ void panel_solve(gdouble* Lt, double* panel, const do
         wave_sync();
         if (lane < kTile) y[kTile * k + lane] = yr;
     }
-    for (int a = 1 + wave; a <= m; a += nw - 1) solve_panel_tile(Lt, panel, inv, k, a, lane, true);
+    for (int a = 1 + wave; a <= m; a += nw - 1)
+        if ((mk >> (k + a)) & 1u) solve_panel_tile(Lt, panel, inv, k, a, lane, true);
 }
 
 // Tile-store copy of the LDS panel tile X_1 of column k.
@@ -1513,7 +1516,7 @@ __device__ __forceinline__ void store_panel_tile1(gdouble* Lt, const double* pan
 // replace the diagonal tiles of the tile store. The two roles run separate step loops with the
 // same barrier sequence, so the resident tiles do not constrain the factor wave's registers.
 __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl, double* dtile, double* dcur, double* y,
-                                                 int* flag) {
+                                                 int* flag, const uint32_t* mask) {
     gdouble* Lt = (gdouble*)Lt_;
     const int t = threadIdx.x, nt = blockDim.x, lane = t % 64, nw = nt / 64, nwt = nw - 1;
     const int wave = __builtin_amdgcn_readfirstlane(t / 64);
@@ -1545,10 +1548,11 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) acc[rr] = dtile[(k + 1) * kTileDoubles + rr * 64 + lane];
             }
-            panel_solve(Lt, panel, inv_of(k), y, k, m, wave, lane, nw);
+            const unsigned mk = mask[k];
+            panel_solve(Lt, panel, inv_of(k), y, k, m, wave, lane, nw, mk);
             __syncthreads();
             if (m > 0) {
-                acc = trailing_mfma(acc, panel, 1, 1, lane);
+                if ((mk >> (k + 1)) & 1u) acc = trailing_mfma(acc, panel, 1, 1, lane);
                 acc_to_panel(acc, next, lane);
                 wave_sync();
                 double v[kTile];
@@ -1591,12 +1595,15 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
         double* panel = pnl + (k & 1) * PANEL;
         double* next = pnl + ((k + 1) & 1) * PANEL;
         const int m = T - k - 1;
-        panel_solve(Lt, panel, inv_of(k), y, k, m, wave, lane, nw);
+        const unsigned mk = mask[k];  // tiles of panel column k that may be nonzero
+        auto live = [&](int ti) { return ((mk >> ti) & 1u) != 0; };
+        panel_solve(Lt, panel, inv_of(k), y, k, m, wave, lane, nw, mk);
         __syncthreads();
-        if (wave == 1 && m >= 1) store_panel_tile1(Lt, panel, k, lane);
+        if (wave == 1 && m >= 1 && live(k + 1)) store_panel_tile1(Lt, panel, k, lane);
         // right-hand side rows below the block
         for (int rr = (wave - 1) * 64 + lane; rr < kTile * m; rr += nwt * 64) {
             const int a = 1 + rr / kTile, i = rr % kTile;
+            if (!live(k + a)) continue;
             double s = 0;
 #pragma unroll
             for (int c = 0; c < kTile; ++c) s = fma(panel[a * kTileDoubles + opnd_pos(i, c)], y[kTile * k + c], s);
@@ -1617,6 +1624,11 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
                 gj[b] = 0;
                 if (w < nitems) {
                     store_tile(T, Fk + w, gi[b], gj[b]);
+                    // untouched and not moving to the next panel: nothing to do
+                    if (!(live(gi[b]) && live(gj[b])) && gj[b] != k + 1) {
+                        gi[b] = -1;
+                        continue;
+                    }
                     const int base = tile_base(gi[b], gj[b]);
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr) gacc[b][rr] = Lt[base + rr * 64 + lane];
@@ -1627,7 +1639,7 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
 #pragma unroll
             for (int b = 0; b < kGlobBatch; ++b) {
                 if (gi[b] < 0) continue;
-                gacc[b] = trailing_mfma(gacc[b], panel, gi[b] - k, gj[b] - k, lane);
+                if (live(gi[b]) && live(gj[b])) gacc[b] = trailing_mfma(gacc[b], panel, gi[b] - k, gj[b] - k, lane);
                 if (gj[b] == k + 1) {
                     acc_to_panel(gacc[b], next + (gi[b] - k - 1) * kTileDoubles, lane);
                 } else {
@@ -1641,11 +1653,12 @@ __device__ __forceinline__ bool cholesky_forward(double* Lt_, int T, double* pnl
 #pragma unroll
         for (int s = 0; s < kRegSlots; ++s) {
             if (rtj[s] <= k) continue;
-            racc[s] = trailing_mfma(racc[s], panel, rti[s] - k, rtj[s] - k, lane);
+            if (live(rti[s]) && live(rtj[s])) racc[s] = trailing_mfma(racc[s], panel, rti[s] - k, rtj[s] - k, lane);
             if (rtj[s] == k + 1) acc_to_panel(racc[s], next + (rti[s] - k - 1) * kTileDoubles, lane);
         }
         // diagonal tiles k + 2.. (LDS)
         for (int w = wave - 1; w < T - k - 2; w += nwt) {
+            if (!live(k + 2 + w)) continue;
             double* dt = dtile + (k + 2 + w) * kTileDoubles;
             d4 acc;
 #pragma unroll
@@ -1814,10 +1827,15 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     if (!window_active(w)) return;
     const int n = w.nr, t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
     const int T = (n + kTile - 1) / kTile, N = kTile * T;
+    // factorization order: speed/bias columns first (their IMU chain leaves most of their tiles
+    // zero, which the factorization skips), then the visual columns
+    const int nv = w.nv, nsb = n - nv;
+    auto to_solve = [&](int c) { return c < nv ? c + nsb : c - nv; };
+    auto to_tangent = [&](int p) { return p < nsb ? p + nv : p - nsb; };
     if (t == 0) flag = 0;
-    for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[i] : 0.0;
+    for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[to_tangent(i)] : 0.0;
     __syncthreads();
-    const bool ok = cholesky_forward(w.fac, T, pnl, dtile, dcur, x, &flag);
+    const bool ok = cholesky_forward(w.fac, T, pnl, dtile, dcur, x, &flag, w.chol_mask);
     if (!ok) {
         if (t == 0) w.lm->chol_ok = 0;
         return;
@@ -1831,20 +1849,25 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
     __syncthreads();
     cholesky_backward(w.fac, T, dinv, x, part);
     for (int it = 0; it < refinement; ++it) {
-        for (int i = wave; i < N; i += nw) {
+        for (int i = wave; i < N; i += nw) {  // residual row i of the factorization order
             double v = 0;
             if (i < n) {
-                const double* row = w.Scopy + (size_t)i * n;
-                for (int j = lane; j < n; j += 64) v += row[j] * x[j];
+                const double* row = w.Scopy + (size_t)to_tangent(i) * n;
+                for (int j = lane; j < n; j += 64) v += row[j] * x[to_solve(j)];
             }
             v = wave_reduce_sum(v);
-            if (lane == 0) rres[i] = i < n ? w.rhs[i] - v : 0.0;
+            if (lane == 0) rres[i] = i < n ? w.rhs[to_tangent(i)] - v : 0.0;
         }
         __syncthreads();
         cholesky_solve_again(w.fac, T, dinv, rres, part);
         for (int i = t; i < n; i += nt) x[i] += rres[i];
         __syncthreads();
     }
+    // the step back in the tangent layout
+    for (int c = t; c < N; c += nt) rres[c] = c < n ? x[to_solve(c)] : 0.0;
+    __syncthreads();
+    {
+    double* x = rres;
     // clamp to relative bounds: only td is bounded among the states
     if (t == 0 && w.td_col >= 0 && isfinite(x[w.td_col])) {
         const double tdv = *w.td;
@@ -1889,6 +1912,7 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
         w.lm->predicted = ps;
         w.lm->max_increment = maxinc;
         w.lm->trial_valid = nonfinite > 0 ? 0 : 1;
+    }
     }
 }
 
