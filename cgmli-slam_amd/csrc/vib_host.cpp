@@ -519,7 +519,9 @@ bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
 }
 
 // Kernel 1 (the retained items linearized alone) is not part of an iteration: kernel 0 runs the
-// same items as extra workgroups, beside the visual factors of the same launch.
+// same items as extra workgroups, beside the visual factors of the same launch. Kernel 3 sums the
+// visual-block partials beside the Schur products, so an iteration skips kernel 2; a
+// linearization alone (read back without a step) still runs it.
 bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(2, s, 0, 0); }
 
 bool Batch::step(hipStream_t s, int refinement, int mode) {
@@ -528,7 +530,7 @@ bool Batch::step(hipStream_t s, int refinement, int mode) {
     return true;
 }
 
-bool Batch::iteration(hipStream_t s, int refinement) { return linearize(s) && step(s, refinement, 0); }
+bool Batch::iteration(hipStream_t s, int refinement) { return launch(0, s, 0, 0) && step(s, refinement, 0); }
 
 bool Batch::capture(hipStream_t s, int refinement) {
     if (graph_exec) return true;
@@ -1806,7 +1808,8 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
         for (int it = 0; ok && it < iterations; ++it)
             for (int k = 0; ok && k < kKernelCount; ++k) {
                 hipEvent_t* e = &kev[2 * (it * kKernelCount + k)];
-                const bool in_iteration = k != 1;  // the retained linearization runs inside kernel 0
+                // the retained linearization runs inside kernel 0, the visual-block sums inside 3
+                const bool in_iteration = k != 1 && k != 2;
                 ok = hipEventRecord(e[0], ctx->stream) == hipSuccess &&
                      (!in_iteration || batch.launch(k, ctx->stream, o->refinement_steps, 0)) &&
                      hipEventRecord(e[1], ctx->stream) == hipSuccess;

@@ -1003,27 +1003,34 @@ __global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
 constexpr int kRedEntries = 16;               // entries per workgroup (one 128-byte line per chunk)
 constexpr int kRedLanes = kBlock / kRedEntries;  // chunk lanes per entry
 
-__global__ __launch_bounds__(kBlock) void k_reduce(const WinDev* wins, int which, double** outs) {
-    __shared__ double red[kRedLanes][kRedEntries];
-    const WinDev& w = wins[blockIdx.x];
+// Entries [slice E, slice E + E) of window w's partial sum, which 0: visual block, 1: Schur
+// correction; kBlock / E chunk lanes per entry, combined in lane order.
+template <int E>
+__device__ __forceinline__ void reduce_slice(const WinDev& w, int which, double* out, int slice, double* red) {
     if (!window_active(w)) return;
     if (which == 0 && !w.lm->new_point) return;
     const int len = w.tri + w.nv;
-    if ((int)blockIdx.y * kRedEntries >= len) return;
-    const int e = threadIdx.x % kRedEntries, cl = threadIdx.x / kRedEntries;
-    const int i = blockIdx.y * kRedEntries + e;
+    if (slice * E >= len) return;
+    constexpr int lanes = kBlock / E;
+    const int e = threadIdx.x % E, cl = threadIdx.x / E;
+    const int i = slice * E + e;
     const double* part = which == 0 ? w.lin_part : w.schur_part;
     const int nparts = which == 0 ? w.nlin : w.nchunks;
     double s = 0;
     if (i < len)
-        for (int c = cl; c < nparts; c += kRedLanes) s += part[(size_t)c * len + i];
-    red[cl][e] = s;
+        for (int c = cl; c < nparts; c += lanes) s += part[(size_t)c * len + i];
+    red[cl * E + e] = s;
     __syncthreads();
     if (cl == 0 && i < len) {
         double t = 0;
-        for (int q = 0; q < kRedLanes; ++q) t += red[q][e];
-        outs[2 * blockIdx.x + which][i] = t;
+        for (int q = 0; q < lanes; ++q) t += red[q * E + e];
+        out[i] = t;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce(const WinDev* wins, int which, double** outs) {
+    __shared__ double red[kBlock];
+    reduce_slice<kRedEntries>(wins[blockIdx.x], which, outs[2 * blockIdx.x + which], blockIdx.y, red);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1163,13 +1170,33 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
     if (t < nv) part[tri + t] = -gsum;
 }
 
-// two workgroups per CU: 256 VGPRs (no AGPRs) and 74 KB of LDS each
+// two workgroups per CU: 256 VGPRs (no AGPRs) and 74 KB of LDS each. Workgroups nchunks.. sum
+// the visual-block partials of the linearization (64 entries each, rslices per window), which
+// the Schur products do not need: k_assemble reads both sums.
+constexpr int kRedEntriesFused = 64;
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_schur(const WinDev* wins,
-                                                                                              const Chunk* chunks) {
+                                                                                              const Chunk* chunks,
+                                                                                              int nchunks, int rslices,
+                                                                                              double** outs) {
+    if ((int)blockIdx.x >= nchunks) {
+        __shared__ double red[kBlock];
+        const int r = blockIdx.x - nchunks, win = r / rslices;
+        reduce_slice<kRedEntriesFused>(wins[win], 0, outs[2 * win], r % rslices, red);
+        return;
+    }
     schur_body<false>(wins, chunks);
 }
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_schur_mixed(const WinDev* wins,
-                                                                                                    const Chunk* chunks) {
+                                                                                                    const Chunk* chunks,
+                                                                                                    int nchunks,
+                                                                                                    int rslices,
+                                                                                                    double** outs) {
+    if ((int)blockIdx.x >= nchunks) {
+        __shared__ double red[kBlock];
+        const int r = blockIdx.x - nchunks, win = r / rslices;
+        reduce_slice<kRedEntriesFused>(wins[win], 0, outs[2 * win], r % rslices, red);
+        return;
+    }
     schur_body<true>(wins, chunks);
 }
 
@@ -2350,12 +2377,13 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             break;
         case 1: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 0); break;
         case 2: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 0, red); break;
-        case 3:
-            if (nchunks) {
-                if (mixed) hipLaunchKernelGGL(k_schur_mixed, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
-                else hipLaunchKernelGGL(k_schur, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
-            }
+        case 3: {  // Schur products, then the visual-block partial sums in the same launch
+            const int rslices = (max_len + kRedEntriesFused - 1) / kRedEntriesFused;
+            const int grid = nchunks + nwin * rslices;
+            if (mixed) hipLaunchKernelGGL(k_schur_mixed, dim3(grid), dim3(kBlock), 0, s, wins, chunks, nchunks, rslices, red);
+            else hipLaunchKernelGGL(k_schur, dim3(grid), dim3(kBlock), 0, s, wins, chunks, nchunks, rslices, red);
             break;
+        }
         case 4: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 1, red); break;
         case 5:
             hipLaunchKernelGGL(k_assemble, dim3(nwin, kAssembleSplit * ((kMaxRetained + kTile - 1) / kTile)), dim3(kBlock), 0,
