@@ -77,6 +77,8 @@ def _batch_window(i):
 def batch_windows(n, procs):
     """The n windows of the batched workload (sizes drawn per window from the seeded stream),
     generated in `procs` spawned worker processes (no GPU state is inherited)."""
+    if procs <= 1:  # in this process (profilers that follow child processes)
+        return [_batch_window(i) for i in range(n)]
     import multiprocessing as mp
     with mp.get_context("spawn").Pool(procs) as pool:
         return pool.map(_batch_window, range(n))

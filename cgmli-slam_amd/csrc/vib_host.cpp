@@ -30,7 +30,7 @@ extern "C" {
 const char* vibk_name(int k);
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, const vib::Chunk* lin,
                        int nlin, double** red, int max_len, int max_edges, int max_landmarks, int refinement, int mode,
-                       int mixed, hipStream_t s);
+                       int mixed, int fuse_reduce, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, hipStream_t s);
@@ -515,14 +515,19 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
 
 bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
     return vibk_launch(k, d_wins, nwin, d_chunks, nchunks, d_lin, nlin, d_red, max_len, max_edges, max_landmarks, refinement,
-                       mode, mixed, s) == hipSuccess;
+                       mode, mixed, fuse_reduce() ? 1 : 0, s) == hipSuccess;
 }
 
-// Kernel 1 (the retained items linearized alone) is not part of an iteration: kernel 0 runs the
-// same items as extra workgroups, beside the visual factors of the same launch. Kernel 3 sums the
-// visual-block partials beside the Schur products, so an iteration skips kernel 2; a
-// linearization alone (read back without a step) still runs it.
-bool Batch::linearize(hipStream_t s) { return launch(0, s, 0, 0) && launch(2, s, 0, 0); }
+// Batches of up to kFuseReduceWindows windows run the retained linearization (kernel 1) as extra
+// workgroups of kernel 0 and the visual-block partial sums (kernel 2) as extra workgroups of
+// kernel 3, so an iteration skips kernels 1 and 2. Larger batches launch them separately: their
+// many small items would otherwise run at the occupancy of the large visual and Schur kernels.
+// A linearization alone always sums the partials.
+bool Batch::fuse_reduce() const { return nwin <= kFuseReduceWindows; }
+bool Batch::in_iteration(int k) const { return !((k == 1 || k == 2) && fuse_reduce()); }
+bool Batch::linearize(hipStream_t s) {
+    return launch(0, s, 0, 0) && (fuse_reduce() || launch(1, s, 0, 0)) && launch(2, s, 0, 0);
+}
 
 bool Batch::step(hipStream_t s, int refinement, int mode) {
     for (int k = 3; k < kKernelCount; ++k)
@@ -530,7 +535,9 @@ bool Batch::step(hipStream_t s, int refinement, int mode) {
     return true;
 }
 
-bool Batch::iteration(hipStream_t s, int refinement) { return launch(0, s, 0, 0) && step(s, refinement, 0); }
+bool Batch::iteration(hipStream_t s, int refinement) {
+    return launch(0, s, 0, 0) && (fuse_reduce() || (launch(1, s, 0, 0) && launch(2, s, 0, 0))) && step(s, refinement, 0);
+}
 
 bool Batch::capture(hipStream_t s, int refinement) {
     if (graph_exec) return true;
@@ -1808,20 +1815,20 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
         for (int it = 0; ok && it < iterations; ++it)
             for (int k = 0; ok && k < kKernelCount; ++k) {
                 hipEvent_t* e = &kev[2 * (it * kKernelCount + k)];
-                // the retained linearization runs inside kernel 0, the visual-block sums inside 3
-                const bool in_iteration = k != 1 && k != 2;
                 ok = hipEventRecord(e[0], ctx->stream) == hipSuccess &&
-                     (!in_iteration || batch.launch(k, ctx->stream, o->refinement_steps, 0)) &&
+                     (!batch.in_iteration(k) || batch.launch(k, ctx->stream, o->refinement_steps, 0)) &&
                      hipEventRecord(e[1], ctx->stream) == hipSuccess;
             }
         ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
         for (int k = 0; k < kKernelCount; ++k) kernel_ms[k] = 0;
+        std::vector<char> skipped(kKernelCount, 0);
+        for (int k = 0; k < kKernelCount; ++k) skipped[k] = !batch.in_iteration(k);
         for (int it = 0; ok && it < iterations; ++it)
             for (int k = 0; ok && k < kKernelCount; ++k) {
                 float ms = 0;
                 const hipEvent_t* e = &kev[2 * (it * kKernelCount + k)];
                 ok = hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess;
-                kernel_ms[k] += ms / iterations;
+                if (!skipped[k]) kernel_ms[k] += ms / iterations;
             }
         for (auto& e : kev) (void)hipEventDestroy(e);
         if (!ok) return fail(ctx, VIB_ERR_DEVICE, "kernel profiling failed");

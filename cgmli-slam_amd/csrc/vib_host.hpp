@@ -22,6 +22,7 @@ constexpr int kKernelCount = 12;  // kernels of one LM iteration (vibk_launch id
 // per workgroup (measured on the 256-window batched workload).
 constexpr size_t kMergeChunksAbove = 4096;
 constexpr int kChunkMerge = 4;
+constexpr int kFuseReduceWindows = 16;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X (workgroup i runs on XCD i % 8)
 
 // Host copy of one window: observation streams sorted by landmark, tangent column layout
@@ -102,6 +103,8 @@ struct Batch {
     bool linearize(hipStream_t s);
     bool step(hipStream_t s, int refinement, int mode);
     bool iteration(hipStream_t s, int refinement);
+    bool fuse_reduce() const;        // kernel 3 also sums the visual-block partials
+    bool in_iteration(int k) const;  // kernel k is launched by an iteration
     bool capture(hipStream_t s, int refinement);
 };
 

@@ -2360,13 +2360,14 @@ const char* vibk_name(int k) {
 
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, const vib::Chunk* lin,
                        int nlin, double** red, int max_len, int max_edges, int max_landmarks, int refinement, int mode,
-                       int mixed, hipStream_t s) {
+                       int mixed, int fuse_reduce, hipStream_t s) {
     using namespace vib;
     const int slices = (max_len + kRedEntries - 1) / kRedEntries;
-    const int rper = max_edges + 1, ritems = nwin * rper;  // retained items: IMU edges + prior
+    const int rper = max_edges + 1, ritems = fuse_reduce ? nwin * rper : 0;  // retained items inside kernel 0
     switch (k) {
-        case 0: {  // visual factors, then the retained items in the same launch
+        case 0: {  // visual factors, then (small batches) the retained items in the same launch
             const int grid = nlin + (ritems + kWaves - 1) / kWaves;
+            if (grid == 0) break;
             if (mixed) hipLaunchKernelGGL(k_lin_visual<true>, dim3(grid), dim3(kBlock), 0, s, wins, lin, nlin, ritems, rper);
             else hipLaunchKernelGGL(k_lin_visual<false>, dim3(grid), dim3(kBlock), 0, s, wins, lin, nlin, ritems, rper);
         }
@@ -2378,8 +2379,9 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
         case 1: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 0); break;
         case 2: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 0, red); break;
         case 3: {  // Schur products, then the visual-block partial sums in the same launch
-            const int rslices = (max_len + kRedEntriesFused - 1) / kRedEntriesFused;
+            const int rslices = fuse_reduce ? (max_len + kRedEntriesFused - 1) / kRedEntriesFused : 0;
             const int grid = nchunks + nwin * rslices;
+            if (grid == 0) break;
             if (mixed) hipLaunchKernelGGL(k_schur_mixed, dim3(grid), dim3(kBlock), 0, s, wins, chunks, nchunks, rslices, red);
             else hipLaunchKernelGGL(k_schur, dim3(grid), dim3(kBlock), 0, s, wins, chunks, nchunks, rslices, red);
             break;

@@ -303,9 +303,9 @@ const char* vib_kernel_name(int32_t kernel);
  * iteration, no host synchronization in between) and writes each iteration's device time,
  * measured with events on the library's stream, to iteration_ms. Then runs `iterations` more
  * iterations kernel by kernel with events around every launch and writes the average duration
- * of each kernel (vib_kernel_count entries) to kernel_ms. Either output may be NULL. Kernels 1
- * (the IMU edges and prior linearized alone) and 2 (the visual-block sums alone) report 0:
- * inside an iteration kernels 0 and 3 run them as extra workgroups. */
+ * of each kernel (vib_kernel_count entries) to kernel_ms. Either output may be NULL. For up to
+ * 16 slots kernels 1 (the IMU edges and prior linearized) and 2 (the visual-block sums) report
+ * 0: kernels 0 and 3 then run them as extra workgroups. */
 int vib_bench(vib_context* ctx, int32_t count, const int32_t* slots, const vib_solve_options* options,
               int32_t iterations, double* iteration_ms, double* kernel_ms);
 /* One launch of a plain streaming copy of `bytes` (multiple of 16) between two fresh device
