@@ -172,6 +172,53 @@ def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb):
     return out
 
 
+def preintegration_line(ctx, abi, count, reps=5, host_seconds=2.0):
+    """Device preintegration of `count` keyframe edges of 0.1 s (20 samples at 200 Hz each, every
+    edge at its own bias linearization point) in one launch, against the native host routine
+    (vib_preintegrate, one thread) on a bounded sample of the same edges."""
+    from vibpy import synth
+    rng = np.random.default_rng(20261210)
+    rate, span = 200.0, 0.1
+    n = int(round(count * span * rate)) + 2
+    t = np.arange(n) / rate
+    s = np.empty((n, 7))
+    s[:, 0] = t
+    for k in range(3):
+        s[:, 1 + k] = 0.3 * np.sin(1.7 * t + k) + 0.05 * rng.standard_normal(n)
+        s[:, 4 + k] = (9.81 if k == 2 else 0.0) + 0.8 * np.cos(2.3 * t + k) + 0.02 * rng.standard_normal(n)
+    t0 = np.arange(count) * span + 0.5 / rate
+    t1 = t0 + span
+    b = np.floor(t0 * rate).astype(np.int32)
+    e = (np.ceil(t1 * rate) + 1).astype(np.int32)
+    sb = np.zeros((count, 9))
+    sb[:, 3:6] = rng.normal(0, 1e-3, (count, 3))
+    sb[:, 6:9] = rng.normal(0, 1e-2, (count, 3))
+    g0 = np.array([synth.gyro_at(s, a) for a in t0])
+    g1 = np.array([synth.gyro_at(s, a) for a in t1])
+    p = abi.imu_params()
+    ms = []
+    for _ in range(reps):
+        out, st, k = ctx.preintegrate_batch(s, b, e, t0, t1, sb, p, g0, g1, timed=True)
+        if (st != abi.VIB_OK).any():
+            raise RuntimeError("device preintegration rejected an interval")
+        ms.append(k)
+    kms = float(np.median(ms))
+    m = 0
+    h0 = time.perf_counter()
+    while m < count:
+        ref = abi.preintegrate(s[b[m]:e[m]], t0[m], t1[m], sb[m], p, g0[m], g1[m])
+        m += 1
+        if time.perf_counter() - h0 >= host_seconds:
+            break
+    hs = time.perf_counter() - h0
+    dev = float(np.max(np.abs(out[m - 1, :76] - ref[:76]) / (np.abs(ref[:76]) + 1e-12)))
+    return dict(value=count / (kms * 1e-3), unit="intervals/s", kernel_ms=kms, intervals=count,
+                samples_per_interval=int(np.median(e - b)), host_value=m / hs, host_sample=f"{m} intervals, 1 thread",
+                max_rel_dev_vs_host=dev,
+                workload=f"{count} keyframe edges of 0.1 s at 200 Hz, one device launch (kernel time, inputs "
+                         "resident), host vib_preintegrate one thread")
+
+
 def cpu_baseline(win, seconds=12.0):
     """Oracle FP64 restatement of the reference CPU path, one thread: steps/s on a bounded sample
     (repeated LM steps on the same window until ~`seconds` of CPU work)."""
@@ -209,6 +256,8 @@ def main():
                     help="seconds of Poisson arrivals of the serving line (0: skip)")
     ap.add_argument("--serve-sessions", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--preint-intervals", type=int, default=4096,
+                    help="keyframe edges of the device preintegration line (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the measurement, launch one plain device copy of this many MiB "
                          "(reference dispatch for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -259,6 +308,10 @@ def main():
               flush=True)
         serving = serving_line(ctx, abi, args.serve_sessions, args.serve_rate, args.serve_seconds, args.batch_procs,
                                64.0)
+    preint = None
+    if args.preint_intervals > 0:
+        print(f"[bench] preintegration: {args.preint_intervals} intervals", file=sys.stderr, flush=True)
+        preint = preintegration_line(ctx, abi, args.preint_intervals)
     if args.copy_probe_mb > 0:
         ctx.copy_probe(args.copy_probe_mb << 20)
     ctx.close()
@@ -306,6 +359,8 @@ def main():
                                                "2-4 cameras, 2-10 observations per landmark, 5% outliers at 20 px")
     if serving is not None:
         out["serving"] = serving
+    if preint is not None:
+        out["preintegration"] = preint
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(win, args.cpu_seconds)
     print(json.dumps(out))

@@ -33,6 +33,9 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        int mixed, int fuse_reduce, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t vibk_preintegrate(int count, const double* smp, const int* begin, const int* end, const double* t0,
+                             const double* t1, const double* sbl, const vib_imu_params* prm, const double* g0,
+                             const double* g1, double* out, int* status, hipStream_t s);
 hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, hipStream_t s);
 }
 
@@ -1765,6 +1768,94 @@ int vib_copy_probe(vib_context* ctx, int64_t bytes) {
     (void)hipFree(src);
     (void)hipFree(dst);
     return ok ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "copy probe failed");
+}
+
+int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* samples, int32_t num_samples,
+                           const int32_t* sample_begin, const int32_t* sample_end, const double* t0, const double* t1,
+                           const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
+                           const double* gyro_end, double* out, int32_t* status, double* kernel_ms) {
+    if (!ctx || count <= 0 || !samples || num_samples < 2 || !sample_begin || !sample_end || !t0 || !t1 || !speed_bias ||
+        !params || !gyro_start || !gyro_end || !out || !status)
+        return VIB_ERR_INVALID_ARGUMENT;
+    const int S = VIB_IMU_SAMPLE_STRIDE;
+    // the checks of the single-interval routine, per interval; invalid intervals are not launched
+    std::vector<int> live;
+    live.reserve(count);
+    for (int e = 0; e < count; ++e) {
+        const int b = sample_begin[e], n = sample_end[e];
+        const bool ok = b >= 0 && n <= num_samples && n - b >= 2 && t1[e] > t0[e] && samples[(size_t)b * S] <= t0[e] &&
+                        samples[(size_t)(n - 1) * S] >= t1[e];
+        status[e] = ok ? VIB_OK : VIB_ERR_INVALID_ARGUMENT;
+        std::memset(out + (size_t)e * VIB_IMU_STRIDE, 0, sizeof(double) * VIB_IMU_STRIDE);
+        if (ok) live.push_back(e);
+    }
+    const int m = (int)live.size();
+    if (m == 0) return VIB_ERR_INVALID_ARGUMENT;
+    std::vector<int> hb(m), he(m);
+    std::vector<double> ht0(m), ht1(m), hsb((size_t)m * 9), hg0((size_t)m * 3), hg1((size_t)m * 3);
+    for (int i = 0; i < m; ++i) {
+        const int e = live[i];
+        hb[i] = sample_begin[e]; he[i] = sample_end[e]; ht0[i] = t0[e]; ht1[i] = t1[e];
+        std::memcpy(&hsb[(size_t)i * 9], speed_bias + (size_t)e * 9, 9 * sizeof(double));
+        std::memcpy(&hg0[(size_t)i * 3], gyro_start + (size_t)e * 3, 3 * sizeof(double));
+        std::memcpy(&hg1[(size_t)i * 3], gyro_end + (size_t)e * 3, 3 * sizeof(double));
+    }
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    // one allocation: samples | t0 | t1 | speed/bias | gyro | records | begin | end | status
+    const size_t nd = (size_t)num_samples * S + 2 * (size_t)m + 15 * (size_t)m + (size_t)m * VIB_IMU_STRIDE;
+    const size_t bytes = nd * sizeof(double) + 3 * (size_t)m * sizeof(int);
+    char* buf = nullptr;
+    if (hipMalloc(&buf, bytes) != hipSuccess) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "preintegration allocation failed");
+    double* d_smp = reinterpret_cast<double*>(buf);
+    double* d_t0 = d_smp + (size_t)num_samples * S;
+    double* d_t1 = d_t0 + m;
+    double* d_sb = d_t1 + m;
+    double* d_g0 = d_sb + (size_t)m * 9;
+    double* d_g1 = d_g0 + (size_t)m * 3;
+    double* d_out = d_g1 + (size_t)m * 3;
+    int* d_b = reinterpret_cast<int*>(d_out + (size_t)m * VIB_IMU_STRIDE);
+    int* d_e = d_b + m;
+    int* d_st = d_e + m;
+    hipStream_t st = ctx->stream;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ok = hipMemcpyAsync(d_smp, samples, (size_t)num_samples * S * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_t0, ht0.data(), m * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_t1, ht1.data(), m * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_sb, hsb.data(), hsb.size() * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_g0, hg0.data(), hg0.size() * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_g1, hg1.data(), hg1.size() * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_b, hb.data(), m * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_e, he.data(), m * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess;
+    if (ok && kernel_ms) ok = hipEventCreate(&ev0) == hipSuccess && hipEventCreate(&ev1) == hipSuccess &&
+                             hipEventRecord(ev0, st) == hipSuccess;
+    ok = ok && vibk_preintegrate(m, d_smp, d_b, d_e, d_t0, d_t1, d_sb, params, d_g0, d_g1, d_out, d_st, st) == hipSuccess;
+    if (ok && kernel_ms) ok = hipEventRecord(ev1, st) == hipSuccess;
+    std::vector<double> hout((size_t)m * VIB_IMU_STRIDE);
+    std::vector<int> hst(m);
+    ok = ok && hipMemcpyAsync(hout.data(), d_out, hout.size() * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(hst.data(), d_st, m * sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipStreamSynchronize(st) == hipSuccess;
+    if (ok && kernel_ms) {
+        float ms = 0;
+        ok = hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess;
+        *kernel_ms = ms;
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    (void)hipFree(buf);
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "preintegration launch failed");
+    int rc = m == count ? VIB_OK : VIB_ERR_INVALID_ARGUMENT;
+    for (int i = 0; i < m; ++i) {
+        const int e = live[i];
+        if (hst[i] != 0) {  // no sample step inside [t0, t1]
+            status[e] = VIB_ERR_INVALID_ARGUMENT;
+            rc = VIB_ERR_INVALID_ARGUMENT;
+            continue;
+        }
+        std::memcpy(out + (size_t)e * VIB_IMU_STRIDE, &hout[(size_t)i * VIB_IMU_STRIDE], sizeof(double) * VIB_IMU_STRIDE);
+    }
+    return rc;
 }
 
 int vib_kernel_count(void) { return kKernelCount; }

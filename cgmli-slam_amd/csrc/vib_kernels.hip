@@ -20,6 +20,7 @@
 
 #include <type_traits>
 
+#include "../../include/vib.h"
 #include "vib_device.hpp"
 #include "vib_math.hpp"
 
@@ -2341,6 +2342,306 @@ __global__ __launch_bounds__(kBlock) void k_prior_gram(const WinDev* wins, doubl
     }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// k_preintegrate: IMU preintegration of many intervals at once, one wavefront per interval, the
+// arithmetic of ImuError::redoPreintegration (src/factor/ImuError.cpp:254-462) as restated by
+// the host vib_preintegrate (midpoint rule, 15x15 covariance propagated per noise source through
+// the transition F, whitening W = Lambda^-1/2 U^T of PseudoInverse::symmSqrtU,
+// src/factor/PseudoInverse.hpp:116-145, by the same cyclic Jacobi). Every lane carries the
+// interval's delta rotation, velocity, position and bias Jacobians (identical arithmetic in all
+// lanes); the four 15x15 sensitivity matrices live in LDS, each F P F^T product spread over the
+// lanes (14 entries each).
+// ---------------------------------------------------------------------------------------------
+namespace preint {
+struct Mat3 {
+    double a[9];
+};
+__device__ __forceinline__ Mat3 zero3() { Mat3 m; for (int i = 0; i < 9; ++i) m.a[i] = 0; return m; }
+__device__ __forceinline__ Mat3 eye3() { Mat3 m = zero3(); m.a[0] = m.a[4] = m.a[8] = 1; return m; }
+__device__ __forceinline__ Mat3 mul(const Mat3& x, const Mat3& y) {
+    Mat3 o;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) o.a[3 * r + c] = x.a[3 * r] * y.a[c] + x.a[3 * r + 1] * y.a[3 + c] + x.a[3 * r + 2] * y.a[6 + c];
+    return o;
+}
+__device__ __forceinline__ Mat3 add(const Mat3& x, const Mat3& y) { Mat3 o; for (int i = 0; i < 9; ++i) o.a[i] = x.a[i] + y.a[i]; return o; }
+__device__ __forceinline__ Mat3 scl(double s, const Mat3& x) { Mat3 o; for (int i = 0; i < 9; ++i) o.a[i] = s * x.a[i]; return o; }
+__device__ __forceinline__ void mv(const Mat3& m, const double* v, double* o) {
+    for (int r = 0; r < 3; ++r) o[r] = m.a[3 * r] * v[0] + m.a[3 * r + 1] * v[1] + m.a[3 * r + 2] * v[2];
+}
+__device__ __forceinline__ Mat3 skew(const double* v) {
+    Mat3 m = zero3();
+    m.a[1] = -v[2]; m.a[2] = v[1]; m.a[3] = v[2]; m.a[5] = -v[0]; m.a[6] = -v[1]; m.a[7] = v[0];
+    return m;
+}
+struct Q4 {
+    double w, x, y, z;
+};
+__device__ __forceinline__ Q4 qmul(const Q4& a, const Q4& b) {
+    return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + b.w * a.x + a.y * b.z - a.z * b.y,
+            a.w * b.y + b.w * a.y + a.z * b.x - a.x * b.z, a.w * b.z + b.w * a.z + a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ Mat3 qmat(const Q4& q) {
+    const double tx = 2 * q.x, ty = 2 * q.y, tz = 2 * q.z;
+    const double twx = tx * q.w, twy = ty * q.w, twz = tz * q.w, txx = tx * q.x, txy = ty * q.x, txz = tz * q.x;
+    const double tyy = ty * q.y, tyz = tz * q.y, tzz = tz * q.z;
+    Mat3 r;
+    r.a[0] = 1 - (tyy + tzz); r.a[1] = txy - twz; r.a[2] = txz + twy;
+    r.a[3] = txy + twz; r.a[4] = 1 - (txx + tzz); r.a[5] = tyz - twx;
+    r.a[6] = txz - twy; r.a[7] = tyz + twx; r.a[8] = 1 - (txx + tyy);
+    return r;
+}
+__device__ __forceinline__ double pick(const Mat3& m, int q) {  // m.a[q] without a stack copy
+    double v = m.a[0];
+#pragma unroll
+    for (int i = 1; i < 9; ++i) v = q == i ? m.a[i] : v;
+    return v;
+}
+__device__ __forceinline__ double sinc(double x) {
+    if (fabs(x) > 1e-6) return sin(x) / x;
+    const double x2 = x * x;
+    return 1.0 - x2 / 6.0 + x2 * x2 / 120.0 - x2 * x2 * x2 / 5040.0;
+}
+__device__ __forceinline__ Mat3 right_jacobian(const double* phi) {  // Forster et al. RSS 2015 eq. (8)
+    const double n = sqrt(phi[0] * phi[0] + phi[1] * phi[1] + phi[2] * phi[2]);
+    const Mat3 px = skew(phi), px2 = mul(px, px);
+    if (n < 1.0e-4) return add(add(eye3(), scl(-0.5, px)), scl(1.0 / 6.0, px2));
+    return add(add(eye3(), scl(-(1.0 - cos(n)) / (n * n), px)), scl((n - sin(n)) / (n * n * n), px2));
+}
+}  // namespace preint
+
+__global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, const int* begin, const int* end,
+                                                     const double* t0s, const double* t1s, const double* sbl_all,
+                                                     vib_imu_params prm, const double* g0_all, const double* g1_all,
+                                                     double* out_all, int* status) {
+    using namespace preint;
+    __shared__ double dP[4][225];
+    __shared__ double Tm[4][225];
+    __shared__ double Fm[225];
+    __shared__ double Qd[4][15];
+    const int e = blockIdx.x, lane = threadIdx.x;
+    const int S = VIB_IMU_SAMPLE_STRIDE;
+    const double* smp = smp_all + (size_t)begin[e] * S;
+    const int count = end[e] - begin[e];
+    const double t0 = t0s[e], t1 = t1s[e];
+    const double* sbl = sbl_all + 9 * (size_t)e;
+    for (int i = lane; i < 4 * 225; i += 64) (&dP[0][0])[i] = 0;
+    Q4 Dq{1, 0, 0, 0};
+    Mat3 Cint = zero3(), Cdint = zero3(), cr = zero3(), dadbg = zero3(), dvdbg = zero3(), dpdbg = zero3();
+    double vint[3] = {0, 0, 0}, pint[3] = {0, 0, 0};
+    double time = t0;
+    bool started = false;
+    int steps = 0;
+    wave_sync();
+    for (int i = 0; i + 1 < count; ++i) {
+        const double* s0 = smp + i * S;
+        const double* s1 = smp + (i + 1) * S;
+        double w0[3] = {s0[1], s0[2], s0[3]}, a0[3] = {s0[4], s0[5], s0[6]};
+        double w1[3] = {s1[1], s1[2], s1[3]}, a1[3] = {s1[4], s1[5], s1[6]};
+        double next = s1[0];
+        double dt = next - time;
+        if (t1 < next) {
+            const double span = next - s0[0];
+            next = t1;
+            dt = next - time;
+            const double r = dt / span;
+            for (int k = 0; k < 3; ++k) { w1[k] = (1 - r) * w0[k] + r * w1[k]; a1[k] = (1 - r) * a0[k] + r * a1[k]; }
+        }
+        if (dt <= 0.0) continue;
+        if (!started) {
+            started = true;
+            const double r = dt / (next - s0[0]);
+            for (int k = 0; k < 3; ++k) { w0[k] = r * w0[k] + (1 - r) * w1[k]; a0[k] = r * a0[k] + (1 - r) * a1[k]; }
+        }
+        double gsat = 1, asat = 1;
+        for (int k = 0; k < 3; ++k) {
+            if (fabs(w0[k]) > prm.g_max || fabs(w1[k]) > prm.g_max) gsat = 100;
+            if (fabs(a0[k]) > prm.a_max || fabs(a1[k]) > prm.a_max) asat = 100;
+        }
+        double om[3], ac[3];
+        for (int k = 0; k < 3; ++k) {
+            om[k] = 0.5 * (w0[k] + w1[k]) - sbl[3 + k];
+            ac[k] = 0.5 * (a0[k] + a1[k]) - sbl[6 + k];
+        }
+        const double half = 0.5 * dt * sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+        const double sc = sinc(half) * 0.5 * dt;
+        const Q4 dq{cos(half), sc * om[0], sc * om[1], sc * om[2]};
+        const Q4 Dq1 = qmul(Dq, dq);
+        const Mat3 C0 = qmat(Dq), C1 = qmat(Dq1), CC = add(C0, C1);
+        double CCa[3];
+        mv(CC, ac, CCa);
+        const Mat3 Cint1 = add(Cint, scl(0.5 * dt, CC));
+        double vint1[3];
+        for (int k = 0; k < 3; ++k) vint1[k] = vint[k] + 0.5 * dt * CCa[k];
+        const Mat3 Cdint1 = add(add(Cdint, scl(dt, Cint)), scl(0.25 * dt * dt, CC));
+        double pint1[3];  // the position update uses the velocity before this step
+        for (int k = 0; k < 3; ++k) pint1[k] = pint[k] + dt * vint[k] + 0.25 * dt * dt * CCa[k];
+        double odt[3] = {om[0] * dt, om[1] * dt, om[2] * dt};
+        const Mat3 Jr = right_jacobian(odt);
+        const Mat3 dadbg1 = add(dadbg, scl(dt, mul(C1, Jr)));
+        const Q4 dqi{dq.w, -dq.x, -dq.y, -dq.z};
+        const Mat3 cr1 = add(mul(qmat(dqi), cr), scl(dt, Jr));
+        const Mat3 ax = skew(ac);
+        const Mat3 mix = add(mul(mul(C0, ax), cr), mul(mul(C1, ax), cr1));
+        const Mat3 dvdbg1 = add(dvdbg, scl(0.5 * dt, mix));
+        const Mat3 dpdbg1 = add(add(dpdbg, scl(dt, dvdbg)), scl(0.25 * dt * dt, mix));
+        // transition F (identity plus nine 3x3 blocks), built from the state before this step
+        double tmpA[3], tmpB[3];
+        for (int k = 0; k < 3; ++k) {
+            tmpA[k] = vint[k] * dt + 0.25 * dt * dt * CCa[k];
+            tmpB[k] = 0.5 * dt * CCa[k];
+        }
+        const Mat3 b03 = scl(-1, skew(tmpA)), b09 = add(scl(dt, dvdbg), scl(0.25 * dt * dt, mix));
+        const Mat3 b012 = add(scl(-dt, Cint), scl(0.25 * dt * dt, CC)), b39 = scl(-dt, C1);
+        const Mat3 b63 = scl(-1, skew(tmpB)), b69 = scl(0.5 * dt, mix), b612 = scl(-0.5 * dt, CC);
+        for (int idx = lane; idx < 225; idx += 64) {
+            const int r = idx / 15, c = idx % 15, br = r / 3, bc = c / 3, q = 3 * (r % 3) + c % 3;
+            double v = r == c ? 1.0 : 0.0;
+            if (br == 0) {
+                if (bc == 1) v = pick(b03, q);
+                else if (bc == 2) v = (r % 3 == c % 3) ? dt : 0.0;
+                else if (bc == 3) v = pick(b09, q);
+                else if (bc == 4) v = pick(b012, q);
+            } else if (br == 1) {
+                if (bc == 3) v = pick(b39, q);
+            } else if (br == 2) {
+                if (bc == 1) v = pick(b63, q);
+                else if (bc == 3) v = pick(b69, q);
+                else if (bc == 4) v = pick(b612, q);
+            }
+            Fm[idx] = v;
+        }
+        if (lane < 15) {
+            const int r = lane;
+            Qd[0][r] = (r >= 3 && r < 6) ? gsat * dt : 0.0;
+            Qd[1][r] = r < 3 ? 0.5 * dt * dt * dt * asat * asat * asat : ((r >= 6 && r < 9) ? asat * dt : 0.0);
+            Qd[2][r] = (r >= 9 && r < 12) ? dt : 0.0;
+            Qd[3][r] = r >= 12 ? dt : 0.0;
+        }
+        wave_sync();
+        for (int idx = lane; idx < 900; idx += 64) {  // T = F dP
+            const int j = idx / 225, r = (idx % 225) / 15, c = idx % 15;
+            double acc = 0;
+            for (int k = 0; k < 15; ++k) acc += Fm[r * 15 + k] * dP[j][k * 15 + c];
+            Tm[j][r * 15 + c] = acc;
+        }
+        wave_sync();
+        for (int idx = lane; idx < 900; idx += 64) {  // dP = T F^T + Q
+            const int j = idx / 225, r = (idx % 225) / 15, c = idx % 15;
+            double acc = 0;
+            for (int k = 0; k < 15; ++k) acc += Tm[j][r * 15 + k] * Fm[c * 15 + k];
+            dP[j][r * 15 + c] = acc + (r == c ? Qd[j][r] : 0.0);
+        }
+        wave_sync();
+        Dq = Dq1;
+        Cint = Cint1;
+        Cdint = Cdint1;
+        for (int k = 0; k < 3; ++k) { vint[k] = vint1[k]; pint[k] = pint1[k]; }
+        cr = cr1;
+        dadbg = dadbg1;
+        dvdbg = dvdbg1;
+        dpdbg = dpdbg1;
+        time = next;
+        ++steps;
+        if (next == t1) break;
+    }
+    if (steps <= 0) {
+        if (lane == 0) status[e] = 1;
+        return;
+    }
+    // P = sum_j sigma_j^2 sym(dP_j), then its eigendecomposition (cyclic Jacobi, the host's
+    // sweep order and stopping rule; one lane per row or column of the rotation)
+    const double s0 = prm.sigma_g_c * prm.sigma_g_c, s1 = prm.sigma_a_c * prm.sigma_a_c;
+    const double s2 = prm.sigma_gw_c * prm.sigma_gw_c, s3 = prm.sigma_aw_c * prm.sigma_aw_c;
+    double* A = Tm[0];
+    double* V = Tm[1];
+    for (int idx = lane; idx < 225; idx += 64) {
+        const int r = idx / 15, c = idx % 15;
+        const double v = (0.5 * dP[0][r * 15 + c] + 0.5 * dP[0][c * 15 + r]) * s0 +
+                         (0.5 * dP[1][r * 15 + c] + 0.5 * dP[1][c * 15 + r]) * s1 +
+                         (0.5 * dP[2][r * 15 + c] + 0.5 * dP[2][c * 15 + r]) * s2 +
+                         (0.5 * dP[3][r * 15 + c] + 0.5 * dP[3][c * 15 + r]) * s3;
+        A[idx] = v;
+        V[idx] = r == c ? 1.0 : 0.0;
+    }
+    wave_sync();
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < 225; ++i) {
+            const double v = A[i] * A[i];
+            tot += v;
+            if (i / 15 != i % 15) off += v;
+        }
+        if (off == 0 || off <= 1e-32 * tot) break;
+        for (int p = 0; p < 14; ++p)
+            for (int q = p + 1; q < 15; ++q) {
+                const double apq = A[p * 15 + q];
+                if (apq == 0) continue;
+                const double theta = (A[q * 15 + q] - A[p * 15 + p]) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                const double c = 1 / sqrt(t * t + 1), s = t * c;
+                wave_sync();
+                if (lane < 15) {
+                    const double x = A[lane * 15 + p], y = A[lane * 15 + q];
+                    A[lane * 15 + p] = c * x - s * y;
+                    A[lane * 15 + q] = s * x + c * y;
+                }
+                wave_sync();
+                if (lane < 15) {
+                    const double x = A[p * 15 + lane], y = A[q * 15 + lane];
+                    A[p * 15 + lane] = c * x - s * y;
+                    A[q * 15 + lane] = s * x + c * y;
+                    const double vx = V[lane * 15 + p], vy = V[lane * 15 + q];
+                    V[lane * 15 + p] = c * vx - s * vy;
+                    V[lane * 15 + q] = s * vx + c * vy;
+                }
+                wave_sync();
+            }
+    }
+    // eigenvalues ascending (selection sort as on the host, by lane 0 in LDS), then W rows
+    // s_i u_i^T
+    __shared__ double ev[15];
+    __shared__ int order[15];
+    if (lane == 0) {
+        for (int i = 0; i < 15; ++i) { ev[i] = A[i * 15 + i]; order[i] = i; }
+        for (int i = 0; i < 15; ++i) {
+            int m = i;
+            for (int j = i + 1; j < 15; ++j) if (ev[j] < ev[m]) m = j;
+            if (m != i) {
+                const double tv = ev[i]; ev[i] = ev[m]; ev[m] = tv;
+                const int to = order[i]; order[i] = order[m]; order[m] = to;
+            }
+        }
+    }
+    wave_sync();
+    const double eps = 2.220446049250313e-16;
+    const double tol = fmax(eps, eps * 15 * ev[14]);
+    double* out = out_all + (size_t)e * VIB_IMU_STRIDE;
+    for (int idx = lane; idx < 225; idx += 64) {
+        const int i = idx / 15, k = idx % 15;
+        const double sv = sqrt(ev[i] > tol ? 1.0 / ev[i] : 1.0 / tol);
+        out[76 + idx] = sv * V[k * 15 + order[i]];
+    }
+    if (lane == 0) {
+        const double* g0 = g0_all + 3 * (size_t)e;
+        const double* g1 = g1_all + 3 * (size_t)e;
+        out[0] = t1 - t0;
+        out[1] = prm.g;
+        for (int k = 0; k < 3; ++k) out[2 + k] = prm.p_BS[k];
+        for (int k = 0; k < 4; ++k) out[5 + k] = prm.q_BS[k];
+        for (int k = 0; k < 3; ++k) { out[9 + k] = g0[k]; out[12 + k] = g1[k]; }
+        out[15] = Dq.x; out[16] = Dq.y; out[17] = Dq.z; out[18] = Dq.w;
+        for (int k = 0; k < 3; ++k) { out[19 + k] = vint[k]; out[22 + k] = pint[k]; }
+        for (int k = 0; k < 6; ++k) out[25 + k] = sbl[3 + k];
+        for (int k = 0; k < 9; ++k) {
+            out[31 + k] = dadbg.a[k]; out[40 + k] = dvdbg.a[k]; out[49 + k] = dpdbg.a[k];
+            out[58 + k] = Cint.a[k]; out[67 + k] = Cdint.a[k];
+        }
+        status[e] = 0;
+    }
+}
+
 }  // namespace vib
 
 // ---------------------------------------------------------------------------------------------
@@ -2416,6 +2717,14 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
 __global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+hipError_t vibk_preintegrate(int count, const double* smp, const int* begin, const int* end, const double* t0,
+                             const double* t1, const double* sbl, const vib_imu_params* prm, const double* g0,
+                             const double* g1, double* out, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(vib::k_preintegrate, dim3(count), dim3(64), 0, s, smp, begin, end, t0, t1, sbl, *prm, g0, g1, out,
+                       status);
+    return hipGetLastError();
 }
 
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s) {

@@ -325,6 +325,8 @@ def load_library(path=LIB_PATH):
         "vib_bench": ([vp, C.c_int32, ip, C.POINTER(VibSolveOptions), C.c_int32, dp, dp], C.c_int),
         "vib_copy_probe": ([vp, C.c_int64], C.c_int),
         "vib_preintegrate": ([dp, C.c_int32, C.c_double, C.c_double, dp, C.POINTER(VibImuParams), dp, dp, dp], C.c_int),
+        "vib_preintegrate_batch": ([vp, C.c_int32, dp, C.c_int32, ip, ip, dp, dp, dp, C.POINTER(VibImuParams), dp, dp,
+                                    dp, ip, dp], C.c_int),
         "vib_pinhole_observations": ([C.c_double] * 5 + [C.c_int32, dp, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -337,6 +339,10 @@ def load_library(path=LIB_PATH):
 
 def f64p(a):
     return a.ctypes.data_as(dp)
+
+
+def i32p(a):
+    return a.ctypes.data_as(ip)
 
 
 def window_buffers(K, C_, L, F, E, prior_rows, prior_cols, prior_blocks, prior_ref):
@@ -428,6 +434,31 @@ class Context:
     def copy_probe(self, nbytes):
         """One plain device copy of nbytes (counter calibration)."""
         return self.check(self.lib.vib_copy_probe(self.h, int(nbytes)), "vib_copy_probe")
+
+    def preintegrate_batch(self, samples, begin, end, t0, t1, speed_bias, params, gyro_start, gyro_end, timed=False):
+        """Device preintegration of many intervals of one sample array (one launch).
+
+        Returns (records (n, IMU_STRIDE), status (n,)) and, with timed=True, the kernel's
+        device milliseconds as a third value. Raises only on a device failure; an interval that
+        fails the argument checks has status VIB_ERR_INVALID_ARGUMENT and a zero record."""
+        s = np.ascontiguousarray(samples, np.float64)
+        b = np.ascontiguousarray(begin, np.int32)
+        e = np.ascontiguousarray(end, np.int32)
+        n = int(b.shape[0])
+        a0 = np.ascontiguousarray(t0, np.float64)
+        a1 = np.ascontiguousarray(t1, np.float64)
+        sb = np.ascontiguousarray(speed_bias, np.float64).reshape(n, 9)
+        g0 = np.ascontiguousarray(gyro_start, np.float64).reshape(n, 3)
+        g1 = np.ascontiguousarray(gyro_end, np.float64).reshape(n, 3)
+        out = np.zeros((n, IMU_STRIDE))
+        status = np.zeros(n, np.int32)
+        ms = C.c_double(0)
+        st = self.lib.vib_preintegrate_batch(self.h, n, f64p(s), int(s.shape[0]), i32p(b), i32p(e), f64p(a0), f64p(a1),
+                                             f64p(sb), C.byref(params), f64p(g0), f64p(g1), f64p(out), i32p(status),
+                                             C.byref(ms) if timed else None)
+        if st not in (VIB_OK, VIB_ERR_INVALID_ARGUMENT):
+            self.check(st, "vib_preintegrate_batch")
+        return (out, status, ms.value) if timed else (out, status)
 
     def error(self):
         m = self.lib.vib_last_error(self.h)

@@ -340,6 +340,20 @@ int vib_read_prior(vib_context* ctx, int32_t slot, int32_t* rows, int32_t* cols,
                    int32_t* key_kind, int64_t* key_id, double* ref, double* A, double* r0,
                    int32_t capacity_rows, int32_t capacity_cols, int32_t capacity_blocks);
 
+/* Device preintegration of `count` intervals in one launch (one wavefront per interval), each
+ * the computation of vib_preintegrate (ImuError::redoPreintegration, ImuError.cpp:254-462,
+ * which the reference calls once per edge on the host: ImuError.cpp:864, BodyImuFactor.cpp:103,
+ * and the re-integration of mergeBodyImuRecords, BodyImuFactor.cpp:306-388). Interval e
+ * integrates samples[sample_begin[e] .. sample_end[e]) (rows of VIB_IMU_SAMPLE_STRIDE inside
+ * one shared array of num_samples rows) over [t0[e], t1[e]] at speed_bias[9e..9e+8] with gyro
+ * readings gyro_start/gyro_end[3e..3e+2], writing out[e * VIB_IMU_STRIDE ...]. status[e] is
+ * VIB_OK or VIB_ERR_INVALID_ARGUMENT (the checks of vib_preintegrate; that record is zeroed).
+ * kernel_ms (nullable) receives the kernel's device time. Host buffers in and out. */
+int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* samples, int32_t num_samples,
+                           const int32_t* sample_begin, const int32_t* sample_end, const double* t0, const double* t1,
+                           const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
+                           const double* gyro_end, double* out, int32_t* status, double* kernel_ms);
+
 /* ---------------- host-side helpers (CPU only, no device) ---------------- */
 /* IMU preintegration between t0 and t1 at the given speed/bias linearization point,
  * written as one VIB_IMU_STRIDE record (ImuError::redoPreintegration, ImuError.cpp:254-462). */
