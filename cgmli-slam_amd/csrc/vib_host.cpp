@@ -33,6 +33,9 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        int mixed, int fuse_reduce, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t vibk_nis_gate(const vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, double threshold,
+                         double* energy, uint8_t* decision, int F, double huber, double* stats, hipStream_t s);
+hipError_t vibk_nis_apply(const vib::WinDev* wins, const uint8_t* decision, int F, hipStream_t s);
 hipError_t vibk_preintegrate(int count, const double* smp, const int* begin, const int* end, const double* t0,
                              const double* t1, const double* sbl, const vib_imu_params* prm, const double* g0,
                              const double* g1, double* out, int* status, hipStream_t s);
@@ -1858,6 +1861,147 @@ int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* sample
     return rc;
 }
 
+namespace {
+
+// The statistical gate of one resident window (the lock is held by the caller).
+static int nis_gate_locked(vib_context* ctx, Slot* s, double threshold, int apply, vib_gate_report* rep, uint8_t* dec_out) {
+    *rep = vib_gate_report{};
+    rep->evaluated.worst_factor = rep->admitted.worst_factor = -1;
+    HostWindow& h = s->host;
+    const int F = h.F;
+    if (F == 0) return VIB_OK;
+    hipSetDevice(ctx->device);
+    std::vector<Slot*> one{s};
+    Batch b;
+    std::string err;
+    if (!b.build(one, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    constexpr int kStats = 18;
+    char* buf = nullptr;
+    if (hipMalloc(&buf, (size_t)F * 9 + kStats * sizeof(double)) != hipSuccess)
+        return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "gate allocation failed");
+    double* d_energy = reinterpret_cast<double*>(buf);
+    double* d_stats = d_energy + F;
+    uint8_t* d_dec = reinterpret_cast<uint8_t*>(d_stats + kStats);
+    std::vector<uint8_t> dec(F);
+    double st[kStats];
+    bool ok = vibk_nis_gate(b.d_wins, b.d_chunks, b.nchunks, threshold, d_energy, d_dec, F, h.huber, d_stats,
+                            ctx->stream) == hipSuccess &&
+              hipMemcpyAsync(dec.data(), d_dec, F, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              hipMemcpyAsync(st, d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              hipStreamSynchronize(ctx->stream) == hipSuccess;
+    int over = 0, invalid = 0, skipped = 0;
+    for (uint8_t d : dec) {
+        over += d == 1;
+        invalid += d == 2;
+        skipped += d == 3;
+    }
+    const bool reject = ok && apply && invalid == 0 && over > 0;
+    if (reject)
+        ok = vibk_nis_apply(b.d_wins, d_dec, F, ctx->stream) == hipSuccess && hipStreamSynchronize(ctx->stream) == hipSuccess;
+    (void)hipFree(buf);
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "gate launch failed");
+    auto fill = [](vib_residual_stats& o, const double* v) {
+        o.factors = v[0]; o.components = v[1]; o.squared_norm = v[2]; o.over_gate = v[3]; o.max_nis = v[4];
+        o.worst_factor = (int32_t)v[5]; o.robust_cost = v[6]; o.median_nis = v[7]; o.p95_nis = v[8];
+    };
+    fill(rep->evaluated, st);
+    fill(rep->admitted, st + 9);
+    rep->skipped = skipped;
+    rep->invalid = invalid;
+    if (reject) {  // the host mirror follows the device
+        const size_t Fs = F;
+        for (int f = 0; f < F; ++f) {
+            if (dec[f] != 1) continue;
+            const size_t p = h.pf_pos[f];
+            for (int q = OB_S00; q <= OB_S11; ++q) {
+                h.ob[q * Fs + f] = 0.0;
+                h.pf_ob[q * Fs + p] = 0.0;
+            }
+        }
+        rep->rejected = over;
+    }
+    if (dec_out) std::memcpy(dec_out, dec.data(), F);
+    rep->status = invalid ? VIB_ERR_INVALID_RESIDUAL : VIB_OK;
+    return invalid ? fail(ctx, VIB_ERR_INVALID_RESIDUAL, "measurement evaluation failed/nonfinite before robust loss")
+                   : VIB_OK;
+}
+
+}  // namespace
+
+int vib_nis_gate(vib_context* ctx, int32_t slot, double threshold, int32_t apply, vib_gate_report* report,
+                 uint8_t* decisions) {
+    if (!ctx || !report || !(threshold > 0)) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    return nis_gate_locked(ctx, s, threshold, apply, report, decisions);
+}
+
+int vib_solve_refit(vib_context* ctx, int32_t slot, const vib_solve_options* o, const vib_refit_options* r,
+                    vib_refit_report* rep) {
+    if (!ctx || !o || !r || !rep || !(r->nis_threshold > 0) || r->max_outlier_refits < 0) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    *rep = vib_refit_report{};
+    rep->pre.evaluated.worst_factor = rep->pre.admitted.worst_factor = -1;
+    rep->post.evaluated.worst_factor = rep->post.admitted.worst_factor = -1;
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    auto failed = [&](int st) {
+        rep->solve.status = st;
+        rep->termination = VIB_TERM_FAILURE;
+        return st;
+    };
+    if (!r->skip_pre_nis) {  // BackendOptimizer.cpp:681-690
+        const int st = nis_gate_locked(ctx, s, r->nis_threshold, 1, &rep->pre, nullptr);
+        if (st != VIB_OK) return failed(st);
+        rep->pre_rejected = rep->pre.rejected;
+    }
+    // passes share the iteration allowance (BackendOptimizer.cpp:720-737, 1036-1079)
+    int remaining = std::max(1, o->max_iterations), used_total = 0;
+    double seconds = 0, initial = 0;
+    for (int pass = 0; pass <= r->max_outlier_refits; ++pass) {
+        const int minimum_remaining = std::max(0, o->min_iterations - used_total);
+        if (remaining <= 0) {
+            rep->termination = VIB_TERM_ITERATION_BUDGET;
+            break;
+        }
+        const int reserve = std::min(r->max_outlier_refits - pass, std::max(0, remaining - std::max(1, minimum_remaining)));
+        vib_solve_options po = *o;
+        po.max_iterations = remaining - std::max(0, reserve);
+        std::vector<Slot*> one{s};
+        std::vector<vib_solve_report> reps;
+        const int st = run_lm(ctx, one, po, reps);
+        if (st != VIB_OK || reps.empty()) return failed(st != VIB_OK ? st : VIB_ERR_DEVICE);
+        const vib_solve_report& sr = reps[0];
+        if (sr.status != VIB_OK) {
+            rep->solve = sr;
+            return failed(sr.status);
+        }
+        ++rep->passes;
+        if (pass == 0) initial = sr.initial_cost;
+        seconds += sr.seconds;
+        used_total += sr.iterations;
+        remaining -= std::max(1, sr.iterations);  // a pass is charged at least one iteration
+        rep->solve = sr;
+        const int gst = nis_gate_locked(ctx, s, r->nis_threshold, 1, &rep->post, nullptr);
+        if (gst != VIB_OK) return failed(gst);
+        const int rejected = rep->post.rejected;
+        rep->post_rejected += rejected;
+        rep->unrefitted_outliers = rejected;
+        const bool iterations_ended = remaining <= 0, refits_ended = pass == r->max_outlier_refits;
+        if (!rejected || iterations_ended || refits_ended) {
+            rep->termination = iterations_ended ? VIB_TERM_ITERATION_BUDGET
+                               : rejected && refits_ended ? VIB_TERM_REFIT_BUDGET
+                                                          : VIB_TERM_SOLVER_FINISHED;
+            break;
+        }
+    }
+    rep->solve.iterations = used_total;
+    rep->solve.initial_cost = initial;
+    rep->solve.seconds = seconds;
+    return VIB_OK;
+}
+
 int vib_kernel_count(void) { return kKernelCount; }
 
 const char* vib_kernel_name(int32_t k) { return vibk_name(k); }
@@ -1979,6 +2123,35 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
     return ok ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "state write-back failed");
 }
 
+namespace {
+
+// A factor with zero square-root information was rejected by the statistical gate. Window
+// maintenance archives rejected factors before any marginalization linearization
+// (BackendOptimizer.cpp:1053-1054): the window is rebuilt without them, and without the landmarks
+// whose every factor was rejected, before the slide.
+static int strip_rejected(vib_context* ctx, Slot& s) {
+    HostWindow h;  // the mirror with the current device state
+    if (!pull_state(ctx, s, h)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    const size_t F = h.F;
+    std::vector<char> keep_f(h.F, 1), keep_lm(h.L, 1), keep_e(h.E, 1);
+    int rejected = 0;
+    for (size_t f = 0; f < F; ++f)
+        if (h.ob[OB_S00 * F + f] == 0.0 && h.ob[OB_S10 * F + f] == 0.0 && h.ob[OB_S11 * F + f] == 0.0) {
+            keep_f[f] = 0;
+            ++rejected;
+        }
+    if (!rejected) return VIB_OK;
+    for (int l = 0; l < h.L; ++l) {
+        bool any = h.lm_off[l + 1] == h.lm_off[l];
+        for (int f = h.lm_off[l]; f < h.lm_off[l + 1]; ++f) any = any || keep_f[f];
+        keep_lm[l] = any;
+    }
+    WindowCopy next = filter_window(copy_of(h), -1, keep_lm, keep_f, keep_e);
+    return install(ctx, s, next);
+}
+
+}  // namespace
+
 int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double* samples, int32_t count,
                     double t_start, double t_end, const vib_imu_params* params, vib_marginal_report* rep) {
     if (!ctx || !rep) return VIB_ERR_INVALID_ARGUMENT;
@@ -1987,7 +2160,11 @@ int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double
     *rep = vib_marginal_report{};
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
-    int st;
+    int st = policy == VIB_MARGIN_OLD || policy == VIB_MARGIN_SECOND_NEW ? strip_rejected(ctx, *s) : VIB_OK;
+    if (st != VIB_OK) {
+        rep->status = st;
+        return st;
+    }
     if (policy == VIB_MARGIN_OLD) st = marginalize_old(ctx, *s, rep);
     else if (policy == VIB_MARGIN_SECOND_NEW) st = marginalize_second_new(ctx, *s, samples, count, t_start, t_end, params, rep);
     else return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown marginalization policy");

@@ -2344,6 +2344,168 @@ __global__ __launch_bounds__(kBlock) void k_prior_gram(const WinDev* wins, doubl
 
 
 // ---------------------------------------------------------------------------------------------
+// Statistical gate (BackendOptimizer::statisticalGate, BackendOptimizer.cpp:394-472): the
+// whitened visual residual before the robust loss, energy = |r|^2, at the current state.
+// decision: 0 admitted, 1 over the gate, 2 evaluation failed / nonfinite (the reference aborts
+// the solve), 3 not participating (a factor rejected earlier: zero square-root information).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool factor_rejected(const double* ob, size_t F, int f) {
+    return ob[OB_S00 * F + f] == 0.0 && ob[OB_S10 * F + f] == 0.0 && ob[OB_S11 * F + f] == 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nis_gate(const WinDev* wins, const Chunk* chunks, double threshold,
+                                                     double* energy, uint8_t* decision) {
+    __shared__ FrameTable ft;
+    const Chunk ch = chunks[blockIdx.x];
+    const WinDev& w = wins[ch.win];
+    build_frame_table(w, w.pose, w.rig, ft);
+    __syncthreads();
+    const double td = *w.td;
+    const size_t F = (size_t)w.F;
+    const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
+    constexpr int groups = kBlock / kChunkLanes;
+    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
+        const int l = base + group;
+        if (l >= ch.lm_end) continue;
+        const LmData lm = load_landmark(w, w.rho, l);
+        for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
+            uint8_t d = 3;
+            double sq = 0;
+            if (!factor_rejected(w.ob, F, f)) {
+                FactorJ J;
+                if (!eval_factor<false>(w, ft, lm, w.ob, F, f, w.ob_fc[f], td, false, J)) {
+                    d = 2;
+                } else {
+                    sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
+                    d = !isfinite(sq) ? 2 : (sq > threshold ? 1 : 0);
+                    if (d == 2) sq = 0;
+                }
+            }
+            energy[f] = sq;
+            decision[f] = d;
+        }
+    }
+}
+
+// Rejection: the factors over the gate lose their square-root information in both observation
+// orders, so no later kernel counts them (the reference's FactorLifecycle::Rejected).
+__global__ __launch_bounds__(kBlock) void k_nis_apply(const WinDev* wins, const uint8_t* decision) {
+    const WinDev& w = wins[0];
+    const size_t F = (size_t)w.F;
+    double* ob = const_cast<double*>(w.ob);
+    double* pf = const_cast<double*>(w.pf_ob);
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < w.F; f += gridDim.x * blockDim.x) {
+        if (decision[f] != 1) continue;
+        const int p = w.pf_pos[f];
+        ob[OB_S00 * F + f] = 0.0; ob[OB_S10 * F + f] = 0.0; ob[OB_S11 * F + f] = 0.0;
+        pf[OB_S00 * F + p] = 0.0; pf[OB_S10 * F + p] = 0.0; pf[OB_S11 * F + p] = 0.0;
+    }
+}
+
+// Residual statistics of the evaluated (decision 0 or 1) and admitted (decision 0) factors
+// (ResidualStatistics, BackendTypes.hpp:209-214; accumulation and quantiles,
+// BackendOptimizer.cpp:436-465), 9 doubles per set: factors, components, squared norm, over
+// gate, max NIS, worst factor (first index of the maximum; -1 when empty), robust cost (Huber),
+// median NIS (sorted[n / 2]), p95 NIS (sorted[ceil(0.95 n) - 1]). Workgroup 0 sums; workgroups
+// 1..4 select one quantile each, exactly, by a most-significant-digit radix select over the bit
+// patterns of the energies (non-negative doubles order like their bits).
+constexpr int kStatSet = 9;
+__device__ __forceinline__ bool in_set(uint8_t d, int set) { return set == 0 ? d <= 1 : d == 0; }
+
+__global__ __launch_bounds__(kBlock) void k_nis_stats(const double* energy, const uint8_t* decision, int F,
+                                                      double huber, double* out) {
+    __shared__ double red[kBlock];
+    __shared__ int ridx[kBlock];
+    __shared__ unsigned hist[2048];
+    __shared__ unsigned long long s_prefix;
+    __shared__ long long s_rank;
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0) {
+        for (int set = 0; set < 2; ++set) {
+            double n = 0, sum = 0, over = 0, robust = 0, mx = -1;
+            int arg = -1;
+            for (int f = t; f < F; f += kBlock) {
+                const uint8_t d = decision[f];
+                if (!in_set(d, set)) continue;
+                const double e = energy[f];
+                n += 1; sum += e; robust += huber_cost(huber, e);
+                if (d == 1) over += 1;
+                if (e > mx) { mx = e; arg = f; }  // strided indices rise: the first maximum is kept
+            }
+            n = block_sum(n, red);
+            sum = block_sum(sum, red);
+            over = block_sum(over, red);
+            robust = block_sum(robust, red);
+            red[t] = mx;
+            ridx[t] = arg;
+            __syncthreads();
+            for (int s = kBlock / 2; s > 0; s >>= 1) {
+                if (t < s) {
+                    const double a = red[t], b = red[t + s];
+                    const int ia = ridx[t], ib = ridx[t + s];
+                    if (b > a || (b == a && ib >= 0 && (ia < 0 || ib < ia))) { red[t] = b; ridx[t] = ib; }
+                }
+                __syncthreads();
+            }
+            if (t == 0) {
+                double* o = out + set * kStatSet;
+                o[0] = n; o[1] = 2 * n; o[2] = sum; o[3] = over;
+                o[4] = n > 0 ? red[0] : 0.0;
+                o[5] = n > 0 ? (double)ridx[0] : -1.0;
+                o[6] = robust;
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    const int q = blockIdx.x - 1, set = q / 2, which = q % 2;
+    double nd = 0;
+    for (int f = t; f < F; f += kBlock) nd += in_set(decision[f], set) ? 1.0 : 0.0;
+    nd = block_sum(nd, red);
+    const long long n = (long long)nd;
+    double* o = out + set * kStatSet + 7 + which;
+    if (n == 0) {
+        if (t == 0) *o = 0.0;
+        return;
+    }
+    if (t == 0) {
+        s_prefix = 0;
+        s_rank = which == 0 ? n / 2 : (long long)ceil(0.95 * (double)n) - 1;
+    }
+    unsigned long long mask = 0;
+    for (int pass = 0; pass < 6; ++pass) {  // digits: bits 53-63, 42-52, 31-41, 20-30, 9-19, 0-8
+        const int sh = pass < 5 ? 53 - 11 * pass : 0;
+        const int width = pass < 5 ? 11 : 9;
+        const unsigned long long dmask = (1ull << width) - 1;
+        for (int b = t; b < 2048; b += kBlock) hist[b] = 0;
+        __syncthreads();
+        const unsigned long long prefix = s_prefix;
+        for (int f = t; f < F; f += kBlock) {
+            if (!in_set(decision[f], set)) continue;
+            const unsigned long long key = (unsigned long long)__double_as_longlong(energy[f]);
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> sh) & dmask], 1u);
+        }
+        __syncthreads();
+        if (t == 0) {
+            long long k = s_rank, cum = 0;
+            for (int b = 0; b <= (int)dmask; ++b) {
+                const long long c = hist[b];
+                if (cum + c > k) {
+                    s_prefix = prefix | ((unsigned long long)b << sh);
+                    s_rank = k - cum;
+                    break;
+                }
+                cum += c;
+            }
+        }
+        mask |= dmask << sh;
+        __syncthreads();
+    }
+    if (t == 0) *o = __longlong_as_double((long long)s_prefix);
+}
+
+
+// ---------------------------------------------------------------------------------------------
 // k_preintegrate: IMU preintegration of many intervals at once, one wavefront per interval, the
 // arithmetic of ImuError::redoPreintegration (src/factor/ImuError.cpp:254-462) as restated by
 // the host vib_preintegrate (midpoint rule, 15x15 covariance propagated per noise source through
@@ -2717,6 +2879,21 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
 __global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+hipError_t vibk_nis_gate(const vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, double threshold,
+                         double* energy, uint8_t* decision, int F, double huber, double* stats, hipStream_t s) {
+    if (nchunks > 0)
+        hipLaunchKernelGGL(vib::k_nis_gate, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks, threshold, energy,
+                           decision);
+    hipLaunchKernelGGL(vib::k_nis_stats, dim3(5), dim3(vib::kBlock), 0, s, energy, decision, F, huber, stats);
+    return hipGetLastError();
+}
+
+hipError_t vibk_nis_apply(const vib::WinDev* wins, const uint8_t* decision, int F, hipStream_t s) {
+    const int blocks = std::max(1, std::min(1024, (F + vib::kBlock - 1) / vib::kBlock));
+    hipLaunchKernelGGL(vib::k_nis_apply, dim3(blocks), dim3(vib::kBlock), 0, s, wins, decision);
+    return hipGetLastError();
 }
 
 hipError_t vibk_preintegrate(int count, const double* smp, const int* begin, const int* end, const double* t0,

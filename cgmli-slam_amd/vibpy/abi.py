@@ -81,6 +81,31 @@ class VibSolveReport(C.Structure):
                 ("final_cost", C.c_double), ("final_lambda", C.c_double), ("seconds", C.c_double)]
 
 
+class VibResidualStats(C.Structure):
+    _fields_ = [("factors", C.c_double), ("components", C.c_double), ("over_gate", C.c_double),
+                ("squared_norm", C.c_double), ("robust_cost", C.c_double), ("median_nis", C.c_double),
+                ("p95_nis", C.c_double), ("max_nis", C.c_double), ("worst_factor", C.c_int32), ("pad", C.c_int32)]
+
+
+class VibGateReport(C.Structure):
+    _fields_ = [("status", C.c_int32), ("rejected", C.c_int32), ("skipped", C.c_int32), ("invalid", C.c_int32),
+                ("evaluated", VibResidualStats), ("admitted", VibResidualStats)]
+
+
+class VibRefitOptions(C.Structure):
+    _fields_ = [("nis_threshold", C.c_double), ("max_outlier_refits", C.c_int32), ("skip_pre_nis", C.c_int32)]
+
+
+class VibRefitReport(C.Structure):
+    _fields_ = [("solve", VibSolveReport), ("passes", C.c_int32), ("pre_rejected", C.c_int32),
+                ("post_rejected", C.c_int32), ("unrefitted_outliers", C.c_int32), ("termination", C.c_int32),
+                ("pad", C.c_int32), ("pre", VibGateReport), ("post", VibGateReport)]
+
+
+VISUAL_NIS_THRESHOLD = 9.210340371976184  # chi-square, 2 dof, 0.99 (BackendTypes.hpp:186)
+TERM_FAILURE, TERM_SOLVER_FINISHED, TERM_TIME_BUDGET, TERM_ITERATION_BUDGET, TERM_REFIT_BUDGET = range(5)
+
+
 class VibLatestTwoReport(C.Structure):
     _fields_ = [("solve", VibSolveReport), ("old_imu_nis_before", C.c_double), ("old_imu_nis_after", C.c_double),
                 ("new_imu_nis_before", C.c_double), ("new_imu_nis_after", C.c_double),
@@ -325,6 +350,9 @@ def load_library(path=LIB_PATH):
         "vib_bench": ([vp, C.c_int32, ip, C.POINTER(VibSolveOptions), C.c_int32, dp, dp], C.c_int),
         "vib_copy_probe": ([vp, C.c_int64], C.c_int),
         "vib_preintegrate": ([dp, C.c_int32, C.c_double, C.c_double, dp, C.POINTER(VibImuParams), dp, dp, dp], C.c_int),
+        "vib_nis_gate": ([vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(VibGateReport), C.POINTER(C.c_uint8)], C.c_int),
+        "vib_solve_refit": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.POINTER(VibRefitOptions),
+                             C.POINTER(VibRefitReport)], C.c_int),
         "vib_preintegrate_batch": ([vp, C.c_int32, dp, C.c_int32, ip, ip, dp, dp, dp, C.POINTER(VibImuParams), dp, dp,
                                     dp, ip, dp], C.c_int),
         "vib_pinhole_observations": ([C.c_double] * 5 + [C.c_int32, dp, dp, dp], C.c_int),
@@ -521,6 +549,23 @@ class Context:
     def solve(self, slot, opts):
         rep = VibSolveReport()
         st = self.lib.vib_solve(self.h, slot, C.byref(opts), C.byref(rep))
+        return st, rep
+
+    def nis_gate(self, slot, threshold=VISUAL_NIS_THRESHOLD, apply=False):
+        """Statistical gate of a resident window: (status, VibGateReport, decisions (F,) uint8 in
+        landmark order: 0 admitted, 1 over the gate, 2 invalid, 3 not participating)."""
+        n = self.shape(slot)[2]
+        dec = np.zeros(max(n, 1), np.uint8)
+        rep = VibGateReport()
+        st = self.lib.vib_nis_gate(self.h, slot, float(threshold), 1 if apply else 0, C.byref(rep),
+                                   dec.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return st, rep, dec[:n]
+
+    def solve_refit(self, slot, opts, threshold=VISUAL_NIS_THRESHOLD, max_refits=3, skip_pre=False):
+        """Solve with the pre-gate and the bounded outlier-refit loop: (status, VibRefitReport)."""
+        ro = VibRefitOptions(float(threshold), int(max_refits), 1 if skip_pre else 0)
+        rep = VibRefitReport()
+        st = self.lib.vib_solve_refit(self.h, slot, C.byref(opts), C.byref(ro), C.byref(rep))
         return st, rep
 
     def solve_latest_two(self, slot, opts, gate=False):

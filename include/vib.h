@@ -340,6 +340,67 @@ int vib_read_prior(vib_context* ctx, int32_t slot, int32_t* rows, int32_t* cols,
                    int32_t* key_kind, int64_t* key_id, double* ref, double* A, double* r0,
                    int32_t capacity_rows, int32_t capacity_cols, int32_t capacity_blocks);
 
+/* ---------------- statistical gate and outlier refits ---------------- */
+/* Residual statistics of one factor set (ResidualStatistics, BackendTypes.hpp:209-214), visual
+ * category. NIS = squared whitened residual before the robust loss. worst_factor is a factor
+ * index in landmark order (the order vib_read_window returns), -1 for an empty set. */
+typedef struct vib_residual_stats {
+    double factors, components, over_gate;
+    double squared_norm, robust_cost;   /* robust cost: sum of the Huber 0.5 rho(NIS)       */
+    double median_nis, p95_nis, max_nis; /* sorted[n/2], sorted[ceil(0.95 n) - 1], maximum    */
+    int32_t worst_factor, pad;
+} vib_residual_stats;
+
+typedef struct vib_gate_report {
+    int32_t status;        /* VIB_OK, or VIB_ERR_INVALID_RESIDUAL when a factor failed to
+                              evaluate or was nonfinite (the reference rolls the solve back)   */
+    int32_t rejected;      /* factors rejected by this call (0 unless applied)                */
+    int32_t skipped;       /* factors rejected earlier (not participating)                    */
+    int32_t invalid;       /* factors that failed to evaluate                                 */
+    vib_residual_stats evaluated, admitted;
+} vib_gate_report;
+
+/* Statistical gate of the resident window at its current state
+ * (BackendOptimizer::statisticalGate, BackendOptimizer.cpp:394-472): every participating visual
+ * factor's NIS against `threshold` (the reference's visualNisThreshold 9.210340371976184,
+ * BackendTypes.hpp:186). With apply != 0 and no invalid factor, the factors over the gate are
+ * rejected for the rest of the window's life: their square-root information becomes zero on the
+ * device and in the host mirror (FactorLifecycle::Rejected), and the next slide removes them
+ * (window maintenance archives rejected factors before marginalization). decisions (nullable, F
+ * bytes, landmark order): 0 admitted, 1 over the gate, 2 invalid, 3 not participating. */
+int vib_nis_gate(vib_context* ctx, int32_t slot, double threshold, int32_t apply, vib_gate_report* report,
+                 uint8_t* decisions);
+
+/* SolveTermination (BackendTypes.hpp:216) */
+#define VIB_TERM_FAILURE 0
+#define VIB_TERM_SOLVER_FINISHED 1
+#define VIB_TERM_TIME_BUDGET 2
+#define VIB_TERM_ITERATION_BUDGET 3
+#define VIB_TERM_REFIT_BUDGET 4
+
+typedef struct vib_refit_options {  /* BackendOptions (BackendTypes.hpp:186-190)             */
+    double nis_threshold;           /* visualNisThreshold                                     */
+    int32_t max_outlier_refits;     /* maxOutlierRefits (3)                                   */
+    int32_t skip_pre_nis;           /* SolveBudget::skipPreNis                                */
+} vib_refit_options;
+
+typedef struct vib_refit_report {
+    vib_solve_report solve;         /* the last pass; iterations and seconds summed over passes,
+                                       initial_cost of the first pass                          */
+    int32_t passes, pre_rejected, post_rejected, unrefitted_outliers;
+    int32_t termination;            /* VIB_TERM_*                                             */
+    int32_t pad;
+    vib_gate_report pre, post;      /* pre-gate (evaluated/admitted), last post-gate          */
+} vib_refit_report;
+
+/* The solve with the pre-gate and the bounded outlier-refit loop of BackendOptimizer::solve
+ * (BackendOptimizer.cpp:680-690, 720-737, 1036-1079): gate, then up to max_outlier_refits + 1
+ * LM passes sharing max_iterations (each pass reserves one iteration per possible later refit
+ * and is charged at least one), each followed by a post-gate that rejects live outliers; the
+ * loop ends when a post-gate rejects nothing or a budget is spent. */
+int vib_solve_refit(vib_context* ctx, int32_t slot, const vib_solve_options* options,
+                    const vib_refit_options* refit, vib_refit_report* report);
+
 /* Device preintegration of `count` intervals in one launch (one wavefront per interval), each
  * the computation of vib_preintegrate (ImuError::redoPreintegration, ImuError.cpp:254-462,
  * which the reference calls once per edge on the host: ImuError.cpp:864, BodyImuFactor.cpp:103,
