@@ -219,6 +219,25 @@ def preintegration_line(ctx, abi, count, reps=5, host_seconds=2.0):
                          "resident), host vib_preintegrate one thread")
 
 
+def gate_line(ctx, abi, win, slot=0, reps=10):
+    """Statistical gate of the resident window (every visual factor's NIS, decisions, residual
+    statistics with exact quantiles), wall time of the host call."""
+    ms = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        st, rep, _ = ctx.nis_gate(slot, abi.VISUAL_NIS_THRESHOLD, apply=False)
+        ms.append(1e3 * (time.perf_counter() - t0))
+        if st != abi.VIB_OK:
+            raise RuntimeError(f"gate failed: {st}")
+    m = float(np.median(ms))
+    ev = rep.evaluated
+    return dict(value=win.F / (m * 1e-3), unit="factors/s", wall_ms=m, factors=win.F, over_gate=int(ev.over_gate),
+                median_nis=ev.median_nis, p95_nis=ev.p95_nis,
+                workload="the timed window at its state after the bench steps: NIS of every visual factor against "
+                         "9.21, decisions and the evaluated/admitted statistics; host call wall time (descriptor "
+                         "upload, allocation, three kernels, download)")
+
+
 def cpu_baseline(win, seconds=12.0):
     """Oracle FP64 restatement of the reference CPU path, one thread: steps/s on a bounded sample
     (repeated LM steps on the same window until ~`seconds` of CPU work)."""
@@ -284,6 +303,7 @@ def main():
     print(f"[bench] timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
     _, k_ms = ctx.bench([0], opts, max(5, min(args.steps, 20)), kernels=True)
     ctx.synchronize()
+    gate = gate_line(ctx, abi, win)
     # the same window in mixed precision, then the mixed-vs-FP64 step error on it and on the
     # ill-conditioned window (depths 0.5-200 m, 30% low-parallax tracks, a near-static stretch)
     mopts = abi.options(max_iterations=1, precision=abi.VIB_MIXED, flags=abi.VIB_SOLVE_CONTINUOUS)
@@ -359,6 +379,7 @@ def main():
                                                "2-4 cameras, 2-10 observations per landmark, 5% outliers at 20 px")
     if serving is not None:
         out["serving"] = serving
+    out["gate"] = gate
     if preint is not None:
         out["preintegration"] = preint
     if not args.no_cpu_baseline:
