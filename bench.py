@@ -238,6 +238,34 @@ def gate_line(ctx, abi, win, slot=0, reps=10):
                          "upload, allocation, three kernels, download)")
 
 
+def whitening_line(ctx, abi, count, reps=5):
+    """Device whitening of `count` pixel observations over 4 pinhole cameras (one sigma), kernel
+    time with inputs resident, against the native host routine (vib_pinhole_observations, one
+    thread) on the same pixels."""
+    rng = np.random.default_rng(20261211)
+    intr = np.array([[458.6, 457.3, 367.2, 248.4, 752, 480], [700.0, 700.0, 640.0, 360.0, 1280, 720],
+                     [320.5, 318.0, 315.0, 242.0, 640, 480], [1000.0, 990.0, 800.0, 600.0, 1600, 1200]])
+    cam = rng.integers(0, 4, count).astype(np.int32)
+    px = np.stack([rng.uniform(0, intr[cam, 4]), rng.uniform(0, intr[cam, 5])], 1)
+    ms = []
+    for _ in range(reps):
+        b, s, st, k = ctx.whiten(px, intr, cam, 1.0, timed=True)
+        if (st != 0).any():
+            raise RuntimeError("device whitening rejected an observation")
+        ms.append(k)
+    kms = float(np.median(ms))
+    h0 = time.perf_counter()
+    for c in range(4):
+        abi.pinhole_observations(*intr[c, :4], 1.0, px[cam == c])
+    hs = time.perf_counter() - h0
+    per = 16 + 4 + 48 + 4  # pixel, camera index, bearing + square-root information, status
+    return dict(value=count / (kms * 1e-3), unit="observations/s", kernel_ms=kms, observations=count,
+                bytes_per_observation=per, achieved_gbps=count * per / (kms * 1e-3) / 1e9, peak_gbps=8000.0,
+                host_value=count / hs, host_sample=f"{count} observations, 1 thread",
+                workload=f"{count} pixel observations over 4 undistorted pinhole cameras, one launch (kernel time, "
+                         "inputs resident); host vib_pinhole_observations one thread")
+
+
 def cpu_baseline(win, seconds=12.0):
     """Oracle FP64 restatement of the reference CPU path, one thread: steps/s on a bounded sample
     (repeated LM steps on the same window until ~`seconds` of CPU work)."""
@@ -277,6 +305,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--preint-intervals", type=int, default=4096,
                     help="keyframe edges of the device preintegration line (0: skip)")
+    ap.add_argument("--whiten-observations", type=int, default=1 << 22,
+                    help="pixel observations of the device whitening line (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the measurement, launch one plain device copy of this many MiB "
                          "(reference dispatch for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -332,6 +362,10 @@ def main():
     if args.preint_intervals > 0:
         print(f"[bench] preintegration: {args.preint_intervals} intervals", file=sys.stderr, flush=True)
         preint = preintegration_line(ctx, abi, args.preint_intervals)
+    whiten = None
+    if args.whiten_observations > 0:
+        print(f"[bench] whitening: {args.whiten_observations} observations", file=sys.stderr, flush=True)
+        whiten = whitening_line(ctx, abi, args.whiten_observations)
     if args.copy_probe_mb > 0:
         ctx.copy_probe(args.copy_probe_mb << 20)
     ctx.close()
@@ -380,6 +414,8 @@ def main():
     if serving is not None:
         out["serving"] = serving
     out["gate"] = gate
+    if whiten is not None:
+        out["whitening"] = whiten
     if preint is not None:
         out["preintegration"] = preint
     if not args.no_cpu_baseline:

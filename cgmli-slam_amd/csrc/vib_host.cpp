@@ -33,6 +33,9 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        int mixed, int fuse_reduce, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t vibk_whiten(int count, const double* px, const int32_t* cam, const double* intr, int ncam,
+                       const double* sigma, double sigma_all, double* bearing, double* sqrt_info, int32_t* status,
+                       hipStream_t s);
 hipError_t vibk_nis_gate(const vib::WinDev* wins, const vib::Chunk* chunks, int nchunks, double threshold,
                          double* energy, uint8_t* decision, int F, double huber, double* stats, hipStream_t s);
 hipError_t vibk_nis_apply(const vib::WinDev* wins, const uint8_t* decision, int F, hipStream_t s);
@@ -1999,6 +2002,57 @@ int vib_solve_refit(vib_context* ctx, int32_t slot, const vib_solve_options* o, 
     rep->solve.iterations = used_total;
     rep->solve.initial_cost = initial;
     rep->solve.seconds = seconds;
+    return VIB_OK;
+}
+
+int vib_whiten_observations(vib_context* ctx, int32_t count, const double* pixels, const int32_t* camera,
+                            const double* intrinsics, int32_t num_cameras, const double* sigma, double sigma_all,
+                            double* bearings, double* sqrt_info, int32_t* status, double* kernel_ms) {
+    if (!ctx || count < 0 || !intrinsics || num_cameras <= 0 || (count > 0 && (!pixels || !bearings || !sqrt_info || !status)))
+        return VIB_ERR_INVALID_ARGUMENT;
+    if (!camera && num_cameras != 1) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "camera indices required for several cameras");
+    if (count == 0) return VIB_OK;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    const size_t n = count;
+    // one allocation: pixels | intrinsics | sigma | bearings | sqrt_info | camera | status
+    const size_t nd = 2 * n + 6 * (size_t)num_cameras + (sigma ? n : 0) + 6 * n;
+    const size_t bytes = nd * sizeof(double) + 2 * n * sizeof(int32_t);
+    char* buf = nullptr;
+    if (hipMalloc(&buf, bytes) != hipSuccess) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "whitening allocation failed");
+    double* d_px = reinterpret_cast<double*>(buf);
+    double* d_in = d_px + 2 * n;
+    double* d_sg = d_in + 6 * (size_t)num_cameras;
+    double* d_b = d_sg + (sigma ? n : 0);
+    double* d_s = d_b + 3 * n;
+    int32_t* d_c = reinterpret_cast<int32_t*>(d_s + 3 * n);
+    int32_t* d_st = d_c + n;
+    hipStream_t st = ctx->stream;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ok = hipMemcpyAsync(d_px, pixels, 2 * n * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_in, intrinsics, 6 * (size_t)num_cameras * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              (!sigma || hipMemcpyAsync(d_sg, sigma, n * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess) &&
+              (!camera || hipMemcpyAsync(d_c, camera, n * sizeof(int32_t), hipMemcpyHostToDevice, st) == hipSuccess);
+    if (ok && kernel_ms) ok = hipEventCreate(&ev0) == hipSuccess && hipEventCreate(&ev1) == hipSuccess &&
+                             hipEventRecord(ev0, st) == hipSuccess;
+    ok = ok && vibk_whiten(count, d_px, camera ? d_c : nullptr, d_in, num_cameras, sigma ? d_sg : nullptr, sigma_all, d_b,
+                           d_s, d_st, st) == hipSuccess;
+    if (ok && kernel_ms) ok = hipEventRecord(ev1, st) == hipSuccess;
+    ok = ok && hipMemcpyAsync(bearings, d_b, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(sqrt_info, d_s, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(status, d_st, n * sizeof(int32_t), hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipStreamSynchronize(st) == hipSuccess;
+    if (ok && kernel_ms) {
+        float ms = 0;
+        ok = hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess;
+        *kernel_ms = ms;
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    (void)hipFree(buf);
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "whitening launch failed");
+    for (size_t i = 0; i < n; ++i)
+        if (status[i] != 0) return VIB_ERR_INVALID_ARGUMENT;
     return VIB_OK;
 }
 

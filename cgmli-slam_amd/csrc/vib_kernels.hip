@@ -2506,6 +2506,73 @@ __global__ __launch_bounds__(kBlock) void k_nis_stats(const double* energy, cons
 
 
 // ---------------------------------------------------------------------------------------------
+// k_whiten: pixel observations of undistorted pinhole cameras -> unit bearing and the
+// lower-triangular square-root information (s00, s10, s11) of the bearing in the sphere basis
+// (FrameStore::observation, FrameStore.cpp:102-124; PinholeCamera::backProject, the
+// undistorted case). One thread per observation; the record written is the 48-byte
+// (bearing, square-root information) pair of the window's observation planes.
+// status: 0 ok, 1 outside the image, 2 back-projection failed, 3 singular covariance or
+// invalid whitening, 4 invalid sigma or camera index.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_whiten(int count, const double* px, const int32_t* cam,
+                                                   const double* intr, int ncam, const double* sigma,
+                                                   double sigma_all, double* bearing, double* sqrt_info,
+                                                   int32_t* status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int c = cam ? cam[i] : 0;
+    const double sg = sigma ? sigma[i] : sigma_all;
+    int st = 0;
+    double b0 = 0, b1 = 0, b2 = 0, s00 = 0, s10 = 0, s11 = 0;
+    if (c < 0 || c >= ncam || !isfinite(sg) || !(sg > 0)) {
+        st = 4;
+    } else {
+        const double* k = intr + 6 * c;  // fx fy cx cy width height
+        const double u = px[2 * i], v = px[2 * i + 1];
+        if (!(u >= 0) || !(v >= 0) || !(u < k[4]) || !(v < k[5])) {
+            st = 1;
+        } else {
+            const V3<double> ray = {(u - k[2]) / k[0], (v - k[3]) / k[1], 1.0};
+            const double rn = sqrt(sqnorm(ray));
+            if (!isfinite(rn) || rn < 1e-12 || !isfinite(1.0 / k[0]) || !isfinite(1.0 / k[1])) {
+                st = 2;
+            } else {
+                const V3<double> b = {ray.x / rn, ray.y / rn, ray.z / rn};
+                V3<double> c0, c1;
+                sphere_basis(b, c0, c1);
+                // D = (I - b b^T) J / |ray|, J = d ray / d pixel = [1/fx 0; 0 1/fy; 0 0]; M = B^T D
+                const double j0 = 1.0 / k[0], j1 = 1.0 / k[1];
+                const V3<double> d0 = {(1 - b.x * b.x) * j0 / rn, (-b.y * b.x) * j0 / rn, (-b.z * b.x) * j0 / rn};
+                const V3<double> d1 = {(-b.x * b.y) * j1 / rn, (1 - b.y * b.y) * j1 / rn, (-b.z * b.y) * j1 / rn};
+                const double m00 = dot(c0, d0), m01 = dot(c0, d1), m10 = dot(c1, d0), m11 = dot(c1, d1);
+                const double s2 = sg * sg;
+                const double cov00 = s2 * (m00 * m00 + m01 * m01);
+                const double cov10 = s2 * (m10 * m00 + m11 * m01);
+                const double cov11 = s2 * (m10 * m10 + m11 * m11);
+                const double l00 = sqrt(cov00), l10 = cov10 / l00, l11 = sqrt(cov11 - l10 * l10);
+                if (!(l00 > 0) || !(l11 > 0)) {
+                    st = 3;
+                } else {
+                    s00 = 1 / l00;
+                    s10 = -l10 / (l00 * l11);
+                    s11 = 1 / l11;
+                    if (!isfinite(s00) || !isfinite(s10) || !isfinite(s11)) {
+                        st = 3;
+                        s00 = s10 = s11 = 0;
+                    } else {
+                        b0 = b.x; b1 = b.y; b2 = b.z;
+                    }
+                }
+            }
+        }
+    }
+    bearing[3 * (size_t)i] = b0; bearing[3 * (size_t)i + 1] = b1; bearing[3 * (size_t)i + 2] = b2;
+    sqrt_info[3 * (size_t)i] = s00; sqrt_info[3 * (size_t)i + 1] = s10; sqrt_info[3 * (size_t)i + 2] = s11;
+    status[i] = st;
+}
+
+
+// ---------------------------------------------------------------------------------------------
 // k_preintegrate: IMU preintegration of many intervals at once, one wavefront per interval, the
 // arithmetic of ImuError::redoPreintegration (src/factor/ImuError.cpp:254-462) as restated by
 // the host vib_preintegrate (midpoint rule, 15x15 covariance propagated per noise source through
@@ -2893,6 +2960,14 @@ hipError_t vibk_nis_gate(const vib::WinDev* wins, const vib::Chunk* chunks, int 
 hipError_t vibk_nis_apply(const vib::WinDev* wins, const uint8_t* decision, int F, hipStream_t s) {
     const int blocks = std::max(1, std::min(1024, (F + vib::kBlock - 1) / vib::kBlock));
     hipLaunchKernelGGL(vib::k_nis_apply, dim3(blocks), dim3(vib::kBlock), 0, s, wins, decision);
+    return hipGetLastError();
+}
+
+hipError_t vibk_whiten(int count, const double* px, const int32_t* cam, const double* intr, int ncam,
+                       const double* sigma, double sigma_all, double* bearing, double* sqrt_info, int32_t* status,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(vib::k_whiten, dim3((count + vib::kBlock - 1) / vib::kBlock), dim3(vib::kBlock), 0, s, count, px,
+                       cam, intr, ncam, sigma, sigma_all, bearing, sqrt_info, status);
     return hipGetLastError();
 }
 

@@ -353,6 +353,7 @@ def load_library(path=LIB_PATH):
         "vib_nis_gate": ([vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(VibGateReport), C.POINTER(C.c_uint8)], C.c_int),
         "vib_solve_refit": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.POINTER(VibRefitOptions),
                              C.POINTER(VibRefitReport)], C.c_int),
+        "vib_whiten_observations": ([vp, C.c_int32, dp, ip, dp, C.c_int32, dp, C.c_double, dp, dp, ip, dp], C.c_int),
         "vib_preintegrate_batch": ([vp, C.c_int32, dp, C.c_int32, ip, ip, dp, dp, dp, C.POINTER(VibImuParams), dp, dp,
                                     dp, ip, dp], C.c_int),
         "vib_pinhole_observations": ([C.c_double] * 5 + [C.c_int32, dp, dp, dp], C.c_int),
@@ -462,6 +463,27 @@ class Context:
     def copy_probe(self, nbytes):
         """One plain device copy of nbytes (counter calibration)."""
         return self.check(self.lib.vib_copy_probe(self.h, int(nbytes)), "vib_copy_probe")
+
+    def whiten(self, pixels, intrinsics, camera=None, sigma=1.0, timed=False):
+        """Device whitening of pixel observations: (bearings (n,3), sqrt_info (n,3), status (n,)) and,
+        with timed=True, the kernel's device milliseconds. intrinsics: (C, 6) fx fy cx cy width height;
+        sigma: a scalar or one value per observation."""
+        px = np.ascontiguousarray(pixels, np.float64).reshape(-1, 2)
+        n = px.shape[0]
+        intr = np.ascontiguousarray(intrinsics, np.float64).reshape(-1, 6)
+        cam = None if camera is None else np.ascontiguousarray(camera, np.int32)
+        sg = None if np.isscalar(sigma) else np.ascontiguousarray(sigma, np.float64)
+        b = np.zeros((n, 3))
+        s = np.zeros((n, 3))
+        status = np.zeros(n, np.int32)
+        ms = C.c_double(0)
+        st = self.lib.vib_whiten_observations(self.h, n, f64p(px), None if cam is None else i32p(cam), f64p(intr),
+                                              intr.shape[0], None if sg is None else f64p(sg),
+                                              float(sigma) if sg is None else 0.0, f64p(b), f64p(s), i32p(status),
+                                              C.byref(ms) if timed else None)
+        if st not in (VIB_OK, VIB_ERR_INVALID_ARGUMENT):
+            self.check(st, "vib_whiten_observations")
+        return (b, s, status, ms.value) if timed else (b, s, status)
 
     def preintegrate_batch(self, samples, begin, end, t0, t1, speed_bias, params, gyro_start, gyro_end, timed=False):
         """Device preintegration of many intervals of one sample array (one launch).

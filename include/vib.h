@@ -415,6 +415,20 @@ int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* sample
                            const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
                            const double* gyro_end, double* out, int32_t* status, double* kernel_ms);
 
+/* Device observation whitening (FrameStore::observation, FrameStore.cpp:102-124; undistorted
+ * PinholeCamera::backProject, PinholeCamera.hpp:778-807) of `count` pixels (u, v interleaved)
+ * from the cameras camera[i] (nullable when num_cameras == 1) described by intrinsics
+ * (num_cameras x 6: fx, fy, cx, cy, width, height), pixel sigma[i] (nullable: sigma_all for
+ * all). Writes unit bearings (count x 3) and (s00, s10, s11) of the lower-triangular
+ * square-root information in the sphere basis (count x 3): the observation record of
+ * vib_window. status[i]: 0 ok, 1 outside the image ("observation is outside valid camera
+ * image"), 2 back-projection failed, 3 singular covariance / invalid whitening, 4 invalid sigma
+ * or camera index; failed records are zero. Returns VIB_ERR_INVALID_ARGUMENT when any status is
+ * nonzero. kernel_ms (nullable) receives the kernel's device time. Host buffers in and out. */
+int vib_whiten_observations(vib_context* ctx, int32_t count, const double* pixels, const int32_t* camera,
+                            const double* intrinsics, int32_t num_cameras, const double* sigma, double sigma_all,
+                            double* bearings, double* sqrt_info, int32_t* status, double* kernel_ms);
+
 /* ---------------- host-side helpers (CPU only, no device) ---------------- */
 /* IMU preintegration between t0 and t1 at the given speed/bias linearization point,
  * written as one VIB_IMU_STRIDE record (ImuError::redoPreintegration, ImuError.cpp:254-462). */
