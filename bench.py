@@ -203,6 +203,10 @@ def preintegration_line(ctx, abi, count, reps=5, host_seconds=2.0):
             raise RuntimeError("device preintegration rejected an interval")
         ms.append(k)
     kms = float(np.median(ms))
+    one = []  # latency of a launch with a single edge
+    for _ in range(reps):
+        _, st, k = ctx.preintegrate_batch(s, b[:1], e[:1], t0[:1], t1[:1], sb[:1], p, g0[:1], g1[:1], timed=True)
+        one.append(k)
     m = 0
     h0 = time.perf_counter()
     while m < count:
@@ -213,6 +217,7 @@ def preintegration_line(ctx, abi, count, reps=5, host_seconds=2.0):
     hs = time.perf_counter() - h0
     dev = float(np.max(np.abs(out[m - 1, :76] - ref[:76]) / (np.abs(ref[:76]) + 1e-12)))
     return dict(value=count / (kms * 1e-3), unit="intervals/s", kernel_ms=kms, intervals=count,
+                single_interval_kernel_ms=float(np.median(one)),
                 samples_per_interval=int(np.median(e - b)), host_value=m / hs, host_sample=f"{m} intervals, 1 thread",
                 max_rel_dev_vs_host=dev,
                 workload=f"{count} keyframe edges of 0.1 s at 200 Hz, one device launch (kernel time, inputs "

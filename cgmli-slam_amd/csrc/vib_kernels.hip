@@ -2410,27 +2410,40 @@ __global__ __launch_bounds__(kBlock) void k_nis_apply(const WinDev* wins, const 
 // 1..4 select one quantile each, exactly, by a most-significant-digit radix select over the bit
 // patterns of the energies (non-negative doubles order like their bits).
 constexpr int kStatSet = 9;
+constexpr int kStatThreads = 1024;
 __device__ __forceinline__ bool in_set(uint8_t d, int set) { return set == 0 ? d <= 1 : d == 0; }
 
-__global__ __launch_bounds__(kBlock) void k_nis_stats(const double* energy, const uint8_t* decision, int F,
-                                                      double huber, double* out) {
-    __shared__ double red[kBlock];
-    __shared__ int ridx[kBlock];
+__global__ __launch_bounds__(kStatThreads) void k_nis_stats(const double* energy, const uint8_t* decision, int F,
+                                                            double huber, double* out) {
+    __shared__ double red[kStatThreads];
+    __shared__ int ridx[kStatThreads];
     __shared__ unsigned hist[2048];
+    __shared__ unsigned scan[kStatThreads];
     __shared__ unsigned long long s_prefix;
     __shared__ long long s_rank;
     const int t = threadIdx.x;
+    constexpr int U = 4;  // elements per thread and round, loaded before they are used
+    constexpr int stride = kStatThreads * U;
     if (blockIdx.x == 0) {
         for (int set = 0; set < 2; ++set) {
             double n = 0, sum = 0, over = 0, robust = 0, mx = -1;
             int arg = -1;
-            for (int f = t; f < F; f += kBlock) {
-                const uint8_t d = decision[f];
-                if (!in_set(d, set)) continue;
-                const double e = energy[f];
-                n += 1; sum += e; robust += huber_cost(huber, e);
-                if (d == 1) over += 1;
-                if (e > mx) { mx = e; arg = f; }  // strided indices rise: the first maximum is kept
+            for (int base = 0; base < F; base += stride) {
+                double e[U];
+                uint8_t d[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int f = base + u * kStatThreads + t;
+                    d[u] = f < F ? decision[f] : 3;
+                    e[u] = f < F ? energy[f] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!in_set(d[u], set)) continue;
+                    n += 1; sum += e[u]; robust += huber_cost(huber, e[u]);
+                    if (d[u] == 1) over += 1;
+                    if (e[u] > mx) { mx = e[u]; arg = base + u * kStatThreads + t; }  // indices rise: first maximum kept
+                }
             }
             n = block_sum(n, red);
             sum = block_sum(sum, red);
@@ -2439,7 +2452,7 @@ __global__ __launch_bounds__(kBlock) void k_nis_stats(const double* energy, cons
             red[t] = mx;
             ridx[t] = arg;
             __syncthreads();
-            for (int s = kBlock / 2; s > 0; s >>= 1) {
+            for (int s = kStatThreads / 2; s > 0; s >>= 1) {
                 if (t < s) {
                     const double a = red[t], b = red[t + s];
                     const int ia = ridx[t], ib = ridx[t + s];
@@ -2460,7 +2473,7 @@ __global__ __launch_bounds__(kBlock) void k_nis_stats(const double* energy, cons
     }
     const int q = blockIdx.x - 1, set = q / 2, which = q % 2;
     double nd = 0;
-    for (int f = t; f < F; f += kBlock) nd += in_set(decision[f], set) ? 1.0 : 0.0;
+    for (int f = t; f < F; f += kStatThreads) nd += in_set(decision[f], set) ? 1.0 : 0.0;
     nd = block_sum(nd, red);
     const long long n = (long long)nd;
     double* o = out + set * kStatSet + 7 + which;
@@ -2477,26 +2490,43 @@ __global__ __launch_bounds__(kBlock) void k_nis_stats(const double* energy, cons
         const int sh = pass < 5 ? 53 - 11 * pass : 0;
         const int width = pass < 5 ? 11 : 9;
         const unsigned long long dmask = (1ull << width) - 1;
-        for (int b = t; b < 2048; b += kBlock) hist[b] = 0;
+        for (int b = t; b < 2048; b += kStatThreads) hist[b] = 0;
         __syncthreads();
         const unsigned long long prefix = s_prefix;
-        for (int f = t; f < F; f += kBlock) {
-            if (!in_set(decision[f], set)) continue;
-            const unsigned long long key = (unsigned long long)__double_as_longlong(energy[f]);
-            if ((key & mask) == prefix) atomicAdd(&hist[(key >> sh) & dmask], 1u);
+        for (int base = 0; base < F; base += stride) {
+            unsigned long long key[U];
+            uint8_t d[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int f = base + u * kStatThreads + t;
+                d[u] = f < F ? decision[f] : 3;
+                key[u] = f < F ? (unsigned long long)__double_as_longlong(energy[f]) : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (in_set(d[u], set) && (key[u] & mask) == prefix) atomicAdd(&hist[(key[u] >> sh) & dmask], 1u);
         }
         __syncthreads();
-        if (t == 0) {
-            long long k = s_rank, cum = 0;
-            for (int b = 0; b <= (int)dmask; ++b) {
-                const long long c = hist[b];
-                if (cum + c > k) {
-                    s_prefix = prefix | ((unsigned long long)b << sh);
-                    s_rank = k - cum;
-                    break;
-                }
-                cum += c;
-            }
+        // the bin holding rank k: exclusive prefix of the counts (two bins per thread), the one
+        // thread whose bins span k records the digit
+        const unsigned c0 = hist[2 * t], c1 = hist[2 * t + 1];
+        scan[t] = c0 + c1;
+        __syncthreads();
+        for (int off = 1; off < kStatThreads; off <<= 1) {
+            const unsigned v = t >= off ? scan[t - off] : 0u;
+            __syncthreads();
+            scan[t] += v;
+            __syncthreads();
+        }
+        const long long k = s_rank;
+        const long long excl = (long long)scan[t] - (long long)(c0 + c1);
+        __syncthreads();
+        if (k >= excl && k < excl + (long long)c0) {
+            s_prefix = prefix | ((unsigned long long)(2 * t) << sh);
+            s_rank = k - excl;
+        } else if (k >= excl + (long long)c0 && k < excl + (long long)(c0 + c1)) {
+            s_prefix = prefix | ((unsigned long long)(2 * t + 1) << sh);
+            s_rank = k - excl - c0;
         }
         mask |= dmask << sh;
         __syncthreads();
@@ -2518,57 +2548,66 @@ __global__ __launch_bounds__(kBlock) void k_whiten(int count, const double* px, 
                                                    const double* intr, int ncam, const double* sigma,
                                                    double sigma_all, double* bearing, double* sqrt_info,
                                                    int32_t* status) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const int c = cam ? cam[i] : 0;
-    const double sg = sigma ? sigma[i] : sigma_all;
+    __shared__ double stage[6 * kBlock];  // the block's records, stored back as contiguous runs
+    const int t = threadIdx.x, i0 = blockIdx.x * kBlock, i = i0 + t;
     int st = 0;
     double b0 = 0, b1 = 0, b2 = 0, s00 = 0, s10 = 0, s11 = 0;
-    if (c < 0 || c >= ncam || !isfinite(sg) || !(sg > 0)) {
-        st = 4;
-    } else {
-        const double* k = intr + 6 * c;  // fx fy cx cy width height
-        const double u = px[2 * i], v = px[2 * i + 1];
-        if (!(u >= 0) || !(v >= 0) || !(u < k[4]) || !(v < k[5])) {
-            st = 1;
+    if (i < count) {
+        const int c = cam ? cam[i] : 0;
+        const double sg = sigma ? sigma[i] : sigma_all;
+        if (c < 0 || c >= ncam || !isfinite(sg) || !(sg > 0)) {
+            st = 4;
         } else {
-            const V3<double> ray = {(u - k[2]) / k[0], (v - k[3]) / k[1], 1.0};
-            const double rn = sqrt(sqnorm(ray));
-            if (!isfinite(rn) || rn < 1e-12 || !isfinite(1.0 / k[0]) || !isfinite(1.0 / k[1])) {
-                st = 2;
+            const double* k = intr + 6 * c;  // fx fy cx cy width height
+            const double2 uv = reinterpret_cast<const double2*>(px)[i];
+            const double u = uv.x, v = uv.y;
+            if (!(u >= 0) || !(v >= 0) || !(u < k[4]) || !(v < k[5])) {
+                st = 1;
             } else {
-                const V3<double> b = {ray.x / rn, ray.y / rn, ray.z / rn};
-                V3<double> c0, c1;
-                sphere_basis(b, c0, c1);
-                // D = (I - b b^T) J / |ray|, J = d ray / d pixel = [1/fx 0; 0 1/fy; 0 0]; M = B^T D
-                const double j0 = 1.0 / k[0], j1 = 1.0 / k[1];
-                const V3<double> d0 = {(1 - b.x * b.x) * j0 / rn, (-b.y * b.x) * j0 / rn, (-b.z * b.x) * j0 / rn};
-                const V3<double> d1 = {(-b.x * b.y) * j1 / rn, (1 - b.y * b.y) * j1 / rn, (-b.z * b.y) * j1 / rn};
-                const double m00 = dot(c0, d0), m01 = dot(c0, d1), m10 = dot(c1, d0), m11 = dot(c1, d1);
-                const double s2 = sg * sg;
-                const double cov00 = s2 * (m00 * m00 + m01 * m01);
-                const double cov10 = s2 * (m10 * m00 + m11 * m01);
-                const double cov11 = s2 * (m10 * m10 + m11 * m11);
-                const double l00 = sqrt(cov00), l10 = cov10 / l00, l11 = sqrt(cov11 - l10 * l10);
-                if (!(l00 > 0) || !(l11 > 0)) {
-                    st = 3;
+                const V3<double> ray = {(u - k[2]) / k[0], (v - k[3]) / k[1], 1.0};
+                const double rn = sqrt(sqnorm(ray));
+                if (!isfinite(rn) || rn < 1e-12 || !isfinite(1.0 / k[0]) || !isfinite(1.0 / k[1])) {
+                    st = 2;
                 } else {
-                    s00 = 1 / l00;
-                    s10 = -l10 / (l00 * l11);
-                    s11 = 1 / l11;
-                    if (!isfinite(s00) || !isfinite(s10) || !isfinite(s11)) {
+                    const V3<double> b = {ray.x / rn, ray.y / rn, ray.z / rn};
+                    V3<double> c0, c1;
+                    sphere_basis(b, c0, c1);
+                    // D = (I - b b^T) J / |ray|, J = d ray / d pixel = [1/fx 0; 0 1/fy; 0 0]; M = B^T D
+                    const double j0 = 1.0 / k[0], j1 = 1.0 / k[1];
+                    const V3<double> d0 = {(1 - b.x * b.x) * j0 / rn, (-b.y * b.x) * j0 / rn, (-b.z * b.x) * j0 / rn};
+                    const V3<double> d1 = {(-b.x * b.y) * j1 / rn, (1 - b.y * b.y) * j1 / rn, (-b.z * b.y) * j1 / rn};
+                    const double m00 = dot(c0, d0), m01 = dot(c0, d1), m10 = dot(c1, d0), m11 = dot(c1, d1);
+                    const double s2 = sg * sg;
+                    const double cov00 = s2 * (m00 * m00 + m01 * m01);
+                    const double cov10 = s2 * (m10 * m00 + m11 * m01);
+                    const double cov11 = s2 * (m10 * m10 + m11 * m11);
+                    const double l00 = sqrt(cov00), l10 = cov10 / l00, l11 = sqrt(cov11 - l10 * l10);
+                    if (!(l00 > 0) || !(l11 > 0)) {
                         st = 3;
-                        s00 = s10 = s11 = 0;
                     } else {
-                        b0 = b.x; b1 = b.y; b2 = b.z;
+                        s00 = 1 / l00;
+                        s10 = -l10 / (l00 * l11);
+                        s11 = 1 / l11;
+                        if (!isfinite(s00) || !isfinite(s10) || !isfinite(s11)) {
+                            st = 3;
+                            s00 = s10 = s11 = 0;
+                        } else {
+                            b0 = b.x; b1 = b.y; b2 = b.z;
+                        }
                     }
                 }
             }
         }
+        status[i] = st;
     }
-    bearing[3 * (size_t)i] = b0; bearing[3 * (size_t)i + 1] = b1; bearing[3 * (size_t)i + 2] = b2;
-    sqrt_info[3 * (size_t)i] = s00; sqrt_info[3 * (size_t)i + 1] = s10; sqrt_info[3 * (size_t)i + 2] = s11;
-    status[i] = st;
+    stage[3 * t] = b0; stage[3 * t + 1] = b1; stage[3 * t + 2] = b2;
+    stage[3 * kBlock + 3 * t] = s00; stage[3 * kBlock + 3 * t + 1] = s10; stage[3 * kBlock + 3 * t + 2] = s11;
+    __syncthreads();
+    const int n3 = 3 * min(kBlock, count - i0);
+    for (int j = t; j < n3; j += kBlock) {
+        bearing[3 * (size_t)i0 + j] = stage[j];
+        sqrt_info[3 * (size_t)i0 + j] = stage[3 * kBlock + j];
+    }
 }
 
 
@@ -2586,17 +2625,29 @@ namespace preint {
 struct Mat3 {
     double a[9];
 };
-__device__ __forceinline__ Mat3 zero3() { Mat3 m; for (int i = 0; i < 9; ++i) m.a[i] = 0; return m; }
+__device__ __forceinline__ Mat3 zero3() {
+    Mat3 m;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m.a[i] = 0;
+    return m;
+}
 __device__ __forceinline__ Mat3 eye3() { Mat3 m = zero3(); m.a[0] = m.a[4] = m.a[8] = 1; return m; }
 __device__ __forceinline__ Mat3 mul(const Mat3& x, const Mat3& y) {
     Mat3 o;
+#pragma unroll
     for (int r = 0; r < 3; ++r)
+#pragma unroll
         for (int c = 0; c < 3; ++c) o.a[3 * r + c] = x.a[3 * r] * y.a[c] + x.a[3 * r + 1] * y.a[3 + c] + x.a[3 * r + 2] * y.a[6 + c];
     return o;
 }
-__device__ __forceinline__ Mat3 add(const Mat3& x, const Mat3& y) { Mat3 o; for (int i = 0; i < 9; ++i) o.a[i] = x.a[i] + y.a[i]; return o; }
-__device__ __forceinline__ Mat3 scl(double s, const Mat3& x) { Mat3 o; for (int i = 0; i < 9; ++i) o.a[i] = s * x.a[i]; return o; }
+__device__ __forceinline__ Mat3 add(const Mat3& x, const Mat3& y) { Mat3 o;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) o.a[i] = x.a[i] + y.a[i]; return o; }
+__device__ __forceinline__ Mat3 scl(double s, const Mat3& x) { Mat3 o;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) o.a[i] = s * x.a[i]; return o; }
 __device__ __forceinline__ void mv(const Mat3& m, const double* v, double* o) {
+#pragma unroll
     for (int r = 0; r < 3; ++r) o[r] = m.a[3 * r] * v[0] + m.a[3 * r + 1] * v[1] + m.a[3 * r + 2] * v[2];
 }
 __device__ __forceinline__ Mat3 skew(const double* v) {
@@ -2675,20 +2726,24 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
             next = t1;
             dt = next - time;
             const double r = dt / span;
+#pragma unroll
             for (int k = 0; k < 3; ++k) { w1[k] = (1 - r) * w0[k] + r * w1[k]; a1[k] = (1 - r) * a0[k] + r * a1[k]; }
         }
         if (dt <= 0.0) continue;
         if (!started) {
             started = true;
             const double r = dt / (next - s0[0]);
+#pragma unroll
             for (int k = 0; k < 3; ++k) { w0[k] = r * w0[k] + (1 - r) * w1[k]; a0[k] = r * a0[k] + (1 - r) * a1[k]; }
         }
         double gsat = 1, asat = 1;
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (fabs(w0[k]) > prm.g_max || fabs(w1[k]) > prm.g_max) gsat = 100;
             if (fabs(a0[k]) > prm.a_max || fabs(a1[k]) > prm.a_max) asat = 100;
         }
         double om[3], ac[3];
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             om[k] = 0.5 * (w0[k] + w1[k]) - sbl[3 + k];
             ac[k] = 0.5 * (a0[k] + a1[k]) - sbl[6 + k];
@@ -2702,9 +2757,11 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         mv(CC, ac, CCa);
         const Mat3 Cint1 = add(Cint, scl(0.5 * dt, CC));
         double vint1[3];
+#pragma unroll
         for (int k = 0; k < 3; ++k) vint1[k] = vint[k] + 0.5 * dt * CCa[k];
         const Mat3 Cdint1 = add(add(Cdint, scl(dt, Cint)), scl(0.25 * dt * dt, CC));
         double pint1[3];  // the position update uses the velocity before this step
+#pragma unroll
         for (int k = 0; k < 3; ++k) pint1[k] = pint[k] + dt * vint[k] + 0.25 * dt * dt * CCa[k];
         double odt[3] = {om[0] * dt, om[1] * dt, om[2] * dt};
         const Mat3 Jr = right_jacobian(odt);
@@ -2717,6 +2774,7 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         const Mat3 dpdbg1 = add(add(dpdbg, scl(dt, dvdbg)), scl(0.25 * dt * dt, mix));
         // transition F (identity plus nine 3x3 blocks), built from the state before this step
         double tmpA[3], tmpB[3];
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             tmpA[k] = vint[k] * dt + 0.25 * dt * dt * CCa[k];
             tmpB[k] = 0.5 * dt * CCa[k];
@@ -2752,6 +2810,7 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         for (int idx = lane; idx < 900; idx += 64) {  // T = F dP
             const int j = idx / 225, r = (idx % 225) / 15, c = idx % 15;
             double acc = 0;
+#pragma unroll
             for (int k = 0; k < 15; ++k) acc += Fm[r * 15 + k] * dP[j][k * 15 + c];
             Tm[j][r * 15 + c] = acc;
         }
@@ -2759,6 +2818,7 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         for (int idx = lane; idx < 900; idx += 64) {  // dP = T F^T + Q
             const int j = idx / 225, r = (idx % 225) / 15, c = idx % 15;
             double acc = 0;
+#pragma unroll
             for (int k = 0; k < 15; ++k) acc += Tm[j][r * 15 + k] * Fm[c * 15 + k];
             dP[j][r * 15 + c] = acc + (r == c ? Qd[j][r] : 0.0);
         }
@@ -2766,6 +2826,7 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         Dq = Dq1;
         Cint = Cint1;
         Cdint = Cdint1;
+#pragma unroll
         for (int k = 0; k < 3; ++k) { vint[k] = vint1[k]; pint[k] = pint1[k]; }
         cr = cr1;
         dadbg = dadbg1;
@@ -2779,8 +2840,11 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         if (lane == 0) status[e] = 1;
         return;
     }
-    // P = sum_j sigma_j^2 sym(dP_j), then its eigendecomposition (cyclic Jacobi, the host's
-    // sweep order and stopping rule; one lane per row or column of the rotation)
+    // P = sum_j sigma_j^2 sym(dP_j), then its eigendecomposition: Jacobi with the host's
+    // rotation and stopping rule, in round-robin order (Brent and Luk 1985): each of the 15
+    // rounds of a sweep applies 7 rotations of disjoint index pairs at once, so a sweep has 15
+    // dependent phases instead of 105. The eigenvectors agree with the host's cyclic order up to
+    // sign and rounding; the whitening they give is the same.
     const double s0 = prm.sigma_g_c * prm.sigma_g_c, s1 = prm.sigma_a_c * prm.sigma_a_c;
     const double s2 = prm.sigma_gw_c * prm.sigma_gw_c, s3 = prm.sigma_aw_c * prm.sigma_aw_c;
     double* A = Tm[0];
@@ -2794,6 +2858,8 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         A[idx] = v;
         V[idx] = r == c ? 1.0 : 0.0;
     }
+    __shared__ double rot_c[8], rot_s[8];
+    __shared__ int rot_p[8], rot_q[8];
     wave_sync();
     for (int sweep = 0; sweep < 60; ++sweep) {
         double off = 0, tot = 0;
@@ -2803,30 +2869,41 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
             if (i / 15 != i % 15) off += v;
         }
         if (off == 0 || off <= 1e-32 * tot) break;
-        for (int p = 0; p < 14; ++p)
-            for (int q = p + 1; q < 15; ++q) {
+        for (int round = 0; round < 15; ++round) {
+            if (lane < 7) {  // pair k = lane + 1 of the tournament on 15 indices
+                const int x = (round + lane + 1) % 15, y = (round + 14 - lane) % 15;
+                const int p = min(x, y), q = max(x, y);
                 const double apq = A[p * 15 + q];
-                if (apq == 0) continue;
-                const double theta = (A[q * 15 + q] - A[p * 15 + p]) / (2 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
-                const double c = 1 / sqrt(t * t + 1), s = t * c;
-                wave_sync();
-                if (lane < 15) {
-                    const double x = A[lane * 15 + p], y = A[lane * 15 + q];
-                    A[lane * 15 + p] = c * x - s * y;
-                    A[lane * 15 + q] = s * x + c * y;
+                double c = 1, sn = 0;
+                if (apq != 0) {
+                    const double theta = (A[q * 15 + q] - A[p * 15 + p]) / (2 * apq);
+                    const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                    c = 1 / sqrt(t * t + 1);
+                    sn = t * c;
                 }
-                wave_sync();
-                if (lane < 15) {
-                    const double x = A[p * 15 + lane], y = A[q * 15 + lane];
-                    A[p * 15 + lane] = c * x - s * y;
-                    A[q * 15 + lane] = s * x + c * y;
-                    const double vx = V[lane * 15 + p], vy = V[lane * 15 + q];
-                    V[lane * 15 + p] = c * vx - s * vy;
-                    V[lane * 15 + q] = s * vx + c * vy;
-                }
-                wave_sync();
+                rot_c[lane] = c; rot_s[lane] = sn; rot_p[lane] = p; rot_q[lane] = q;
             }
+            wave_sync();
+            for (int idx = lane; idx < 105; idx += 64) {  // columns p, q of every row
+                const int k = idx / 15, r = idx % 15, p = rot_p[k], q = rot_q[k];
+                const double c = rot_c[k], sn = rot_s[k];
+                const double x = A[r * 15 + p], y = A[r * 15 + q];
+                A[r * 15 + p] = c * x - sn * y;
+                A[r * 15 + q] = sn * x + c * y;
+            }
+            wave_sync();
+            for (int idx = lane; idx < 105; idx += 64) {  // rows p, q of every column, and V
+                const int k = idx / 15, r = idx % 15, p = rot_p[k], q = rot_q[k];
+                const double c = rot_c[k], sn = rot_s[k];
+                const double x = A[p * 15 + r], y = A[q * 15 + r];
+                A[p * 15 + r] = c * x - sn * y;
+                A[q * 15 + r] = sn * x + c * y;
+                const double vx = V[r * 15 + p], vy = V[r * 15 + q];
+                V[r * 15 + p] = c * vx - sn * vy;
+                V[r * 15 + q] = sn * vx + c * vy;
+            }
+            wave_sync();
+        }
     }
     // eigenvalues ascending (selection sort as on the host, by lane 0 in LDS), then W rows
     // s_i u_i^T
@@ -2857,10 +2934,13 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         const double* g1 = g1_all + 3 * (size_t)e;
         out[0] = t1 - t0;
         out[1] = prm.g;
+#pragma unroll
         for (int k = 0; k < 3; ++k) out[2 + k] = prm.p_BS[k];
         for (int k = 0; k < 4; ++k) out[5 + k] = prm.q_BS[k];
+#pragma unroll
         for (int k = 0; k < 3; ++k) { out[9 + k] = g0[k]; out[12 + k] = g1[k]; }
         out[15] = Dq.x; out[16] = Dq.y; out[17] = Dq.z; out[18] = Dq.w;
+#pragma unroll
         for (int k = 0; k < 3; ++k) { out[19 + k] = vint[k]; out[22 + k] = pint[k]; }
         for (int k = 0; k < 6; ++k) out[25 + k] = sbl[3 + k];
         for (int k = 0; k < 9; ++k) {
@@ -2953,7 +3033,7 @@ hipError_t vibk_nis_gate(const vib::WinDev* wins, const vib::Chunk* chunks, int 
     if (nchunks > 0)
         hipLaunchKernelGGL(vib::k_nis_gate, dim3(nchunks), dim3(vib::kBlock), 0, s, wins, chunks, threshold, energy,
                            decision);
-    hipLaunchKernelGGL(vib::k_nis_stats, dim3(5), dim3(vib::kBlock), 0, s, energy, decision, F, huber, stats);
+    hipLaunchKernelGGL(vib::k_nis_stats, dim3(5), dim3(vib::kStatThreads), 0, s, energy, decision, F, huber, stats);
     return hipGetLastError();
 }
 
