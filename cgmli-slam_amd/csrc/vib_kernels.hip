@@ -2696,9 +2696,12 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
                                                      vib_imu_params prm, const double* g0_all, const double* g1_all,
                                                      double* out_all, int* status) {
     using namespace preint;
-    __shared__ double dP[4][225];
-    __shared__ double Tm[4][225];
-    __shared__ double Fm[225];
+    // 15x15 matrices padded to 16x16 tiles (zero row and column 15) at a pitch of 17 doubles, so
+    // the strided operand reads of the MFMA products fall in distinct LDS banks
+    constexpr int P = 17, PT = 16 * P;
+    __shared__ double dP[4][PT];
+    __shared__ double Tm[4][PT];
+    __shared__ double Fm[PT];
     __shared__ double Qd[4][15];
     const int e = blockIdx.x, lane = threadIdx.x;
     const int S = VIB_IMU_SAMPLE_STRIDE;
@@ -2706,7 +2709,7 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
     const int count = end[e] - begin[e];
     const double t0 = t0s[e], t1 = t1s[e];
     const double* sbl = sbl_all + 9 * (size_t)e;
-    for (int i = lane; i < 4 * 225; i += 64) (&dP[0][0])[i] = 0;
+    for (int i = lane; i < 4 * PT; i += 64) (&dP[0][0])[i] = 0;
     Q4 Dq{1, 0, 0, 0};
     Mat3 Cint = zero3(), Cdint = zero3(), cr = zero3(), dadbg = zero3(), dvdbg = zero3(), dpdbg = zero3();
     double vint[3] = {0, 0, 0}, pint[3] = {0, 0, 0};
@@ -2782,9 +2785,13 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
         const Mat3 b03 = scl(-1, skew(tmpA)), b09 = add(scl(dt, dvdbg), scl(0.25 * dt * dt, mix));
         const Mat3 b012 = add(scl(-dt, Cint), scl(0.25 * dt * dt, CC)), b39 = scl(-dt, C1);
         const Mat3 b63 = scl(-1, skew(tmpB)), b69 = scl(0.5 * dt, mix), b612 = scl(-0.5 * dt, CC);
-        for (int idx = lane; idx < 225; idx += 64) {
-            const int r = idx / 15, c = idx % 15, br = r / 3, bc = c / 3, q = 3 * (r % 3) + c % 3;
-            double v = r == c ? 1.0 : 0.0;
+        for (int idx = lane; idx < PT; idx += 64) {
+            const int r = idx / P, c = idx % P, br = r / 3, bc = c / 3, q = 3 * (r % 3) + c % 3;
+            double v = r == c && r < 15 ? 1.0 : 0.0;
+            if (r >= 15 || c >= 15) {
+                Fm[idx] = 0.0;
+                continue;
+            }
             if (br == 0) {
                 if (bc == 1) v = pick(b03, q);
                 else if (bc == 2) v = (r % 3 == c % 3) ? dt : 0.0;
@@ -2807,22 +2814,35 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
             Qd[3][r] = r >= 12 ? dt : 0.0;
         }
         wave_sync();
-        for (int idx = lane; idx < 900; idx += 64) {  // T = F dP
-            const int j = idx / 225, r = (idx % 225) / 15, c = idx % 15;
-            double acc = 0;
+        {
+            // T_j = F dP_j, then dP_j = T_j F^T + Q_j, on v_mfma_f64_16x16x4f64: lane l gives
+            // A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16]; accumulator register rr holds
+            // D[4 rr + l / 16][l % 16]
+            const int li = lane % 16, lk = lane / 16;
 #pragma unroll
-            for (int k = 0; k < 15; ++k) acc += Fm[r * 15 + k] * dP[j][k * 15 + c];
-            Tm[j][r * 15 + c] = acc;
-        }
-        wave_sync();
-        for (int idx = lane; idx < 900; idx += 64) {  // dP = T F^T + Q
-            const int j = idx / 225, r = (idx % 225) / 15, c = idx % 15;
-            double acc = 0;
+            for (int j = 0; j < 4; ++j) {
+                d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < 15; ++k) acc += Tm[j][r * 15 + k] * Fm[c * 15 + k];
-            dP[j][r * 15 + c] = acc + (r == c ? Qd[j][r] : 0.0);
+                for (int k0 = 0; k0 < 16; k0 += 4)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Fm[li * P + k0 + lk], dP[j][(k0 + lk) * P + li], acc, 0, 0, 0);
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) Tm[j][(4 * rr + lk) * P + li] = acc[rr];
+            }
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < 16; k0 += 4)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Tm[j][li * P + k0 + lk], Fm[li * P + k0 + lk], acc, 0, 0, 0);
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int r = 4 * rr + lk;
+                    dP[j][r * P + li] = acc[rr] + (r == li && r < 15 ? Qd[j][r] : 0.0);
+                }
+            }
+            wave_sync();
         }
-        wave_sync();
         Dq = Dq1;
         Cint = Cint1;
         Cdint = Cdint1;
@@ -2851,10 +2871,10 @@ __global__ __launch_bounds__(64) void k_preintegrate(const double* smp_all, cons
     double* V = Tm[1];
     for (int idx = lane; idx < 225; idx += 64) {
         const int r = idx / 15, c = idx % 15;
-        const double v = (0.5 * dP[0][r * 15 + c] + 0.5 * dP[0][c * 15 + r]) * s0 +
-                         (0.5 * dP[1][r * 15 + c] + 0.5 * dP[1][c * 15 + r]) * s1 +
-                         (0.5 * dP[2][r * 15 + c] + 0.5 * dP[2][c * 15 + r]) * s2 +
-                         (0.5 * dP[3][r * 15 + c] + 0.5 * dP[3][c * 15 + r]) * s3;
+        const double v = (0.5 * dP[0][r * P + c] + 0.5 * dP[0][c * P + r]) * s0 +
+                         (0.5 * dP[1][r * P + c] + 0.5 * dP[1][c * P + r]) * s1 +
+                         (0.5 * dP[2][r * P + c] + 0.5 * dP[2][c * P + r]) * s2 +
+                         (0.5 * dP[3][r * P + c] + 0.5 * dP[3][c * P + r]) * s3;
         A[idx] = v;
         V[idx] = r == c ? 1.0 : 0.0;
     }
