@@ -2553,13 +2553,13 @@ __global__ __launch_bounds__(kBlock) void k_whiten(int count, const double* px, 
     int st = 0;
     double b0 = 0, b1 = 0, b2 = 0, s00 = 0, s10 = 0, s11 = 0;
     if (i < count) {
+        const double2 uv = reinterpret_cast<const double2*>(px)[i];  // issued with the index loads
         const int c = cam ? cam[i] : 0;
         const double sg = sigma ? sigma[i] : sigma_all;
         if (c < 0 || c >= ncam || !isfinite(sg) || !(sg > 0)) {
             st = 4;
         } else {
             const double* k = intr + 6 * c;  // fx fy cx cy width height
-            const double2 uv = reinterpret_cast<const double2*>(px)[i];
             const double u = uv.x, v = uv.y;
             if (!(u >= 0) || !(v >= 0) || !(u < k[4]) || !(v < k[5])) {
                 st = 1;
