@@ -33,6 +33,8 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        int mixed, int fuse_reduce, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, int count, int32_t* chosen,
+                         double* new_rho, hipStream_t s);
 hipError_t vibk_whiten(int count, const double* px, const int32_t* cam, const double* intr, int ncam,
                        const double* sigma, double sigma_all, double* bearing, double* sqrt_info, int32_t* status,
                        hipStream_t s);
@@ -1239,36 +1241,6 @@ void camera_world(const WindowCopy& w, int k, int c, M3<double>& R, V3<double>& 
     t = mul(Rb, mul(Re, load3(&w.cam_t[3 * c])) + load3(w.rig)) + load3(&w.pose[7 * k]);
 }
 
-// Whether factor f of landmark l evaluates (validity rules of VisualSphereBatchMath.hpp:67-174)
-// with the landmark anchored at (frame, cam) on `bearing` with inverse range rho; the bearings
-// of the tracks this is used for do not move.
-bool static_factor_valid(const WindowCopy& w, int f, int frame, int cam, V3<double> bearing, double rho) {
-    if (!(rho > 0) || !std::isfinite(rho)) return false;
-    M3<double> Ra, Rb;
-    V3<double> ta, tb;
-    camera_world(w, frame, cam, Ra, ta);
-    camera_world(w, w.ob_frame[f], w.ob_cam[f], Rb, tb);
-    const V3<double> ba = (1.0 / std::sqrt(sqnorm(bearing))) * bearing;
-    const V3<double> raw = load3(&w.ob_bearing[3 * f]);
-    const V3<double> bb = (1.0 / std::sqrt(sqnorm(raw))) * raw;
-    const V3<double> pc = mulT(Rb, mul(Ra, (1.0 / rho) * ba) + ta - tb);
-    // sphere_residual's rules: a finite p_C of length >= 1e-10, not within 1e-8 of antipodal,
-    // and a finite residual P f(theta) (n - c b)
-    const double range = std::sqrt(sqnorm(pc));
-    if (!std::isfinite(pc.x) || !std::isfinite(pc.y) || !std::isfinite(pc.z) || !std::isfinite(range) || range < 1e-10)
-        return false;
-    const V3<double> n = (1.0 / range) * pc;
-    const double c = std::fmin(1.0, std::fmax(-1.0, dot(bb, n))), sn = std::sqrt(sqnorm(cross(bb, n)));
-    if (c < -1 + 1e-8) return false;
-    const double f_theta = sn < 1e-8 ? 1.0 : std::atan2(sn, c) / sn;
-    const V3<double> lg = f_theta * (n - c * bb);
-    V3<double> c0, c1;
-    sphere_basis(raw, c0, c1);
-    const double s00 = w.ob_sqrt[3 * f], s10 = w.ob_sqrt[3 * f + 1], s11 = w.ob_sqrt[3 * f + 2];
-    const double r0 = s00 * dot(c0, lg), r1 = s10 * dot(c0, lg) + s11 * dot(c1, lg);
-    return std::isfinite(r0) && std::isfinite(r1);
-}
-
 // Linear interpolation of the gyroscope at t (gyroAt, BodyImuFactor.cpp:29-38).
 void gyro_at(const double* samples, int count, double t, double* out) {
     int next = 0;
@@ -1287,6 +1259,32 @@ void gyro_at(const double* samples, int count, double t, double* out) {
 // landmarks anchored in K-2 are re-anchored on a surviving view, demoted (one surviving ray) or
 // retired into the prior, the slot's other image measurements are discarded, and the prior is
 // reused when nothing else touches the slot.
+// k_reanchor over the listed landmarks of the resident window (anchored in slot v): the chosen
+// view (landmark-order observation, -1: none) and its inverse range per landmark.
+bool reanchor_on_device(vib_context* ctx, Slot& s, int v, const std::vector<int32_t>& lms, std::vector<int32_t>& chosen,
+                        std::vector<double>& rho) {
+    const size_t n = lms.size();
+    chosen.assign(n, -1);
+    rho.assign(n, 0.0);
+    if (n == 0) return true;
+    std::vector<Slot*> one{&s};
+    Batch b;
+    std::string err;
+    if (!b.build(one, ctx->stream, err)) return false;
+    char* buf = nullptr;
+    if (hipMalloc(&buf, n * (sizeof(double) + 2 * sizeof(int32_t))) != hipSuccess) return false;
+    double* d_rho = reinterpret_cast<double*>(buf);
+    int32_t* d_lms = reinterpret_cast<int32_t*>(d_rho + n);
+    int32_t* d_chosen = d_lms + n;
+    const bool ok = hipMemcpyAsync(d_lms, lms.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+                    vibk_reanchor(b.d_wins, v, d_lms, (int)n, d_chosen, d_rho, ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(chosen.data(), d_chosen, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(rho.data(), d_rho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                    hipStreamSynchronize(ctx->stream) == hipSuccess;
+    (void)hipFree(buf);
+    return ok;
+}
+
 int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int count, double t0, double t1,
                            const vib_imu_params* params, vib_marginal_report* rep) {
     HostWindow cur;
@@ -1357,6 +1355,7 @@ int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int
     for (int f = 0; f < w.F; ++f) of_lm[w.ob_lm[f]].push_back(f);
     std::vector<char> keep_lm(w.L, 1), keep_f(w.F, 1), retired(w.L, 0);
     int reset = 0, demoted = 0, dropped = 0;
+    std::vector<int32_t> pending;  // landmarks with two or more surviving views
     for (int l = 0; l < w.L; ++l) {
         if (w.lm_frame[l] != v) continue;
         std::vector<int> surviving;
@@ -1372,44 +1371,24 @@ int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int
             ++demoted;
             continue;
         }
-        // preferred anchors: newer frames first, then lower camera index (every retained frame
-        // counts as a keyframe)
-        std::stable_sort(surviving.begin(), surviving.end(), [&](int a, int b) {
-            if (w.ob_frame[a] != w.ob_frame[b]) return w.ob_frame[a] > w.ob_frame[b];
-            return w.ob_cam[a] < w.ob_cam[b];
-        });
-        M3<double> Ra;
-        V3<double> ta;
-        camera_world(w, v, w.lm_cam[l], Ra, ta);
-        const V3<double> raw = load3(&w.lm_bearing[3 * l]);
-        const V3<double> world = mul(Ra, (1.0 / (w.rho[l] * std::sqrt(sqnorm(raw)))) * raw) + ta;
-        int chosen = -1;
-        double new_rho = 0;
-        for (int cand : surviving) {
-            M3<double> Rc;
-            V3<double> tc;
-            camera_world(w, w.ob_frame[cand], w.ob_cam[cand], Rc, tc);
-            const V3<double> delta = mulT(Rc, world - tc);
-            const V3<double> ray = load3(&w.ob_bearing[3 * cand]);
-            const double range = dot(delta, ray);
-            if (!(range > 0) || !std::isfinite(range) || 1 / range < w.min_rho ||
-                std::atan2(std::sqrt(sqnorm(cross(delta, ray))), range) > 0.03) continue;
-            bool valid = true;
-            for (int f : surviving)
-                if (f != cand && !static_factor_valid(w, f, w.ob_frame[cand], w.ob_cam[cand], ray, 1 / range)) {
-                    valid = false;
-                    break;
-                }
-            if (valid) { chosen = cand; new_rho = 1 / range; break; }
-        }
+        pending.push_back(l);  // anchor search on the device, below
+    }
+    // anchor search (preferred anchors: newer frames first, then lower camera index; every
+    // retained frame counts as a keyframe), one device thread per landmark
+    std::vector<int32_t> chosen_view;
+    std::vector<double> chosen_rho;
+    if (!reanchor_on_device(ctx, s, v, pending, chosen_view, chosen_rho))
+        return fail(ctx, VIB_ERR_DEVICE, "anchor search launch failed");
+    for (size_t i = 0; i < pending.size(); ++i) {
+        const int l = pending[i], chosen = chosen_view[i];
         if (chosen < 0) { retired[l] = 1; continue; }
         w.lm_frame[l] = w.ob_frame[chosen]; w.lm_cam[l] = w.ob_cam[chosen];
-        for (int i = 0; i < 3; ++i) {
-            w.lm_bearing[3 * l + i] = w.ob_bearing[3 * chosen + i];
-            w.lm_vel[3 * l + i] = w.ob_vel[3 * chosen + i];
+        for (int k = 0; k < 3; ++k) {
+            w.lm_bearing[3 * l + k] = w.ob_bearing[3 * chosen + k];
+            w.lm_vel[3 * l + k] = w.ob_vel[3 * chosen + k];
         }
         w.lm_tref[l] = w.ob_tref[chosen];
-        w.rho[l] = new_rho;
+        w.rho[l] = chosen_rho[i];
         keep_f[chosen] = 0;
         for (int f : of_lm[l]) if (w.ob_frame[f] == v) keep_f[f] = 0;
         ++reset;

@@ -2612,6 +2612,94 @@ __global__ __launch_bounds__(kBlock) void k_whiten(int count, const double* px, 
 
 
 // ---------------------------------------------------------------------------------------------
+// k_reanchor: the anchor search of a SECOND_NEW slide (SlidingWindowManager.cpp:312-457) for
+// landmarks anchored in the redundant slot v, one thread per landmark. The surviving views
+// (observations outside v) are tried in the reference's preference order (newer frame first,
+// then lower camera index, then observation order); the first view whose ray meets the
+// landmark's world point at positive range, inverse range >= min_rho and within 0.03 rad, and
+// under which every other surviving observation still evaluates, becomes the new anchor.
+// chosen[i]: that observation (landmark order) or -1 (the landmark is retired); rho[i]: 1/range.
+// ---------------------------------------------------------------------------------------------
+__device__ bool reanchor_view_valid(const WinDev& w, const FrameTable& ft, int f, int frame, int cam, V3<double> bearing,
+                                    double rho) {
+    if (!(rho > 0) || !isfinite(rho)) return false;
+    const size_t F = (size_t)w.F;
+    const int fb = w.ob_fc[f] >> 8, cb = w.ob_fc[f] & 255;
+    const M3<double> Ra = *reinterpret_cast<const M3<double>*>(ft.Rc[frame][cam]);
+    const M3<double> Rb = *reinterpret_cast<const M3<double>*>(ft.Rc[fb][cb]);
+    const V3<double> ta = load3(ft.tc[frame][cam]), tb = load3(ft.tc[fb][cb]);
+    const V3<double> ba = (1.0 / sqrt(sqnorm(bearing))) * bearing;
+    const V3<double> raw = {w.ob[OB_BX * F + f], w.ob[OB_BY * F + f], w.ob[OB_BZ * F + f]};
+    const V3<double> bb = (1.0 / sqrt(sqnorm(raw))) * raw;
+    const V3<double> pc = mulT(Rb, mul(Ra, (1.0 / rho) * ba) + ta - tb);
+    const double range = sqrt(sqnorm(pc));
+    if (!isfinite(pc.x) || !isfinite(pc.y) || !isfinite(pc.z) || !isfinite(range) || range < 1e-10) return false;
+    const V3<double> n = (1.0 / range) * pc;
+    const double c = fmin(1.0, fmax(-1.0, dot(bb, n))), sn = sqrt(sqnorm(cross(bb, n)));
+    if (c < -1 + 1e-8) return false;
+    const double f_theta = sn < 1e-8 ? 1.0 : atan2(sn, c) / sn;
+    const V3<double> lg = f_theta * (n - c * bb);
+    V3<double> c0, c1;
+    sphere_basis(raw, c0, c1);
+    const double s00 = w.ob[OB_S00 * F + f], s10 = w.ob[OB_S10 * F + f], s11 = w.ob[OB_S11 * F + f];
+    const double r0 = s00 * dot(c0, lg), r1 = s10 * dot(c0, lg) + s11 * dot(c1, lg);
+    return isfinite(r0) && isfinite(r1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_reanchor(const WinDev* wins, int v, const int32_t* lms, int count,
+                                                     int32_t* chosen, double* new_rho) {
+    __shared__ FrameTable ft;
+    const WinDev& w = wins[0];
+    build_frame_table(w, w.pose, w.rig, ft);
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int l = lms[i];
+    const size_t L = (size_t)w.L, F = (size_t)w.F;
+    const int ca = w.lm_fc[l] & 255;
+    const M3<double> Ra = *reinterpret_cast<const M3<double>*>(ft.Rc[v][ca]);
+    const V3<double> ta = load3(ft.tc[v][ca]);
+    const V3<double> raw = {w.lmd[LM_BX * L + l], w.lmd[LM_BY * L + l], w.lmd[LM_BZ * L + l]};
+    const V3<double> world = mul(Ra, (1.0 / (w.rho[l] * sqrt(sqnorm(raw)))) * raw) + ta;
+    const int f0 = w.lm_off[l], f1 = w.lm_off[l + 1];
+    auto key = [&](int f) {  // preference order: newer frame, lower camera, earlier observation
+        return ((kMaxFrames - 1 - (w.ob_fc[f] >> 8)) << 24) | ((w.ob_fc[f] & 255) << 16) | (f - f0);
+    };
+    int result = -1;
+    double rho_out = 0;
+    int prev = -1;
+    for (;;) {
+        int cand = -1, best = 0x7fffffff;
+        for (int f = f0; f < f1; ++f) {
+            if ((w.ob_fc[f] >> 8) == v) continue;
+            const int k = key(f);
+            if (k > prev && k < best) { best = k; cand = f; }
+        }
+        if (cand < 0) break;
+        prev = best;
+        const int fc = w.ob_fc[cand] >> 8, cc = w.ob_fc[cand] & 255;
+        const M3<double> Rc = *reinterpret_cast<const M3<double>*>(ft.Rc[fc][cc]);
+        const V3<double> delta = mulT(Rc, world - load3(ft.tc[fc][cc]));
+        const V3<double> ray = {w.ob[OB_BX * F + cand], w.ob[OB_BY * F + cand], w.ob[OB_BZ * F + cand]};
+        const double range = dot(delta, ray);
+        if (!(range > 0) || !isfinite(range) || 1 / range < w.min_rho ||
+            atan2(sqrt(sqnorm(cross(delta, ray))), range) > 0.03)
+            continue;
+        bool valid = true;
+        for (int f = f0; f < f1 && valid; ++f)
+            if ((w.ob_fc[f] >> 8) != v && f != cand) valid = reanchor_view_valid(w, ft, f, fc, cc, ray, 1 / range);
+        if (valid) {
+            result = cand;
+            rho_out = 1 / range;
+            break;
+        }
+    }
+    chosen[i] = result;
+    new_rho[i] = rho_out;
+}
+
+
+// ---------------------------------------------------------------------------------------------
 // k_preintegrate: IMU preintegration of many intervals at once, one wavefront per interval, the
 // arithmetic of ImuError::redoPreintegration (src/factor/ImuError.cpp:254-462) as restated by
 // the host vib_preintegrate (midpoint rule, 15x15 covariance propagated per noise source through
@@ -3068,6 +3156,13 @@ hipError_t vibk_whiten(int count, const double* px, const int32_t* cam, const do
                        hipStream_t s) {
     hipLaunchKernelGGL(vib::k_whiten, dim3((count + vib::kBlock - 1) / vib::kBlock), dim3(vib::kBlock), 0, s, count, px,
                        cam, intr, ncam, sigma, sigma_all, bearing, sqrt_info, status);
+    return hipGetLastError();
+}
+
+hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, int count, int32_t* chosen,
+                         double* new_rho, hipStream_t s) {
+    hipLaunchKernelGGL(vib::k_reanchor, dim3((count + vib::kBlock - 1) / vib::kBlock), dim3(vib::kBlock), 0, s, wins, v,
+                       lms, count, chosen, new_rho);
     return hipGetLastError();
 }
 
