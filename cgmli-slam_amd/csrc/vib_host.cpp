@@ -33,6 +33,7 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                        int mixed, int fuse_reduce, hipStream_t s);
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t vibk_elim_set(const vib::WinDev* wins, int L, int victim, uint8_t* elim, hipStream_t s);
 hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, int count, int32_t* chosen,
                          double* new_rho, hipStream_t s);
 hipError_t vibk_whiten(int count, const double* px, const int32_t* cam, const double* intr, int ncam,
@@ -1201,11 +1202,19 @@ int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
     if (!pull_state(ctx, s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
     const int victim = 0;
     if (cur.K < 2) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "marginalization needs at least two frames");
-    std::vector<char> elim(cur.L, 0);
-    for (int l = 0; l < cur.L; ++l) {
-        bool e = cur.lm_frame[l] == victim;
-        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) e = e || (cur.ob_fc[f] >> 8) == victim;
-        elim[l] = e;
+    std::vector<char> elim(cur.L, 0);  // the eliminated landmarks, selected on the device
+    if (cur.L > 0) {
+        std::vector<Slot*> one{&s};
+        Batch b;
+        std::string err;
+        uint8_t* d_elim = nullptr;
+        if (!b.build(one, ctx->stream, err) || hipMalloc(&d_elim, cur.L) != hipSuccess)
+            return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
+        const bool ok = vibk_elim_set(b.d_wins, cur.L, victim, d_elim, ctx->stream) == hipSuccess &&
+                        hipMemcpyAsync(elim.data(), d_elim, cur.L, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                        hipStreamSynchronize(ctx->stream) == hipSuccess;
+        (void)hipFree(d_elim);
+        if (!ok) return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
     }
     std::vector<int> imu_used;
     for (int e = 0; e < cur.E; ++e) if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) imu_used.push_back(e);

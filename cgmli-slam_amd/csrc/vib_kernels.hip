@@ -2700,6 +2700,20 @@ __global__ __launch_bounds__(kBlock) void k_reanchor(const WinDev* wins, int v, 
 
 
 // ---------------------------------------------------------------------------------------------
+// k_elim_set: the landmarks a MARGIN_OLD slide of slot `victim` eliminates
+// (Marginalizer::compute's landmark set: anchored in the slot or observed from it), one thread
+// per landmark; elim[l] = 1 for those.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_elim_set(const WinDev* wins, int victim, uint8_t* elim) {
+    const WinDev& w = wins[0];
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= w.L) return;
+    bool e = (w.lm_fc[l] >> 8) == victim;
+    for (int f = w.lm_off[l]; f < w.lm_off[l + 1] && !e; ++f) e = (w.ob_fc[f] >> 8) == victim;
+    elim[l] = e ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------------------
 // k_preintegrate: IMU preintegration of many intervals at once, one wavefront per interval, the
 // arithmetic of ImuError::redoPreintegration (src/factor/ImuError.cpp:254-462) as restated by
 // the host vib_preintegrate (midpoint rule, 15x15 covariance propagated per noise source through
@@ -3163,6 +3177,12 @@ hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, int
                          double* new_rho, hipStream_t s) {
     hipLaunchKernelGGL(vib::k_reanchor, dim3((count + vib::kBlock - 1) / vib::kBlock), dim3(vib::kBlock), 0, s, wins, v,
                        lms, count, chosen, new_rho);
+    return hipGetLastError();
+}
+
+hipError_t vibk_elim_set(const vib::WinDev* wins, int L, int victim, uint8_t* elim, hipStream_t s) {
+    hipLaunchKernelGGL(vib::k_elim_set, dim3((L + vib::kBlock - 1) / vib::kBlock), dim3(vib::kBlock), 0, s, wins, victim,
+                       elim);
     return hipGetLastError();
 }
 
