@@ -271,6 +271,34 @@ def whitening_line(ctx, abi, count, reps=5):
                          "inputs resident); host vib_pinhole_observations one thread")
 
 
+def slide_line(ctx, abi, win, reps=3):
+    """Window slides of the timed window: MARGIN_OLD (slot 0 and its landmarks eliminated into the
+    prior) and SECOND_NEW (the redundant second-newest slot removed, its IMU edges merged and
+    re-integrated, tracks re-anchored); wall time of the host call, upload excluded."""
+    out = {}
+    m = win.meta
+    t = m["frame_t"]
+    for name, policy in (("old", abi.VIB_MARGIN_OLD), ("second_new", abi.VIB_MARGIN_SECOND_NEW)):
+        ms = []
+        for _ in range(reps):
+            ctx.upload(0, win)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            if policy == abi.VIB_MARGIN_OLD:
+                st, rep = ctx.marginalize(0, policy)
+            else:
+                st, rep = ctx.marginalize(0, policy, m["samples"], float(t[win.K - 3]), float(t[win.K - 1]),
+                                          m["imu_params"])
+            ms.append(1e3 * (time.perf_counter() - t0))
+            if st != abi.VIB_OK:
+                raise RuntimeError(f"slide {name} failed: {st}")
+        out[name + "_ms"] = float(np.median(ms))
+        out[name + "_eliminated_landmarks"] = int(rep.eliminated_landmarks)
+    out["workload"] = ("the timed window uploaded afresh, then one slide; wall time of vib_marginalize (state "
+                       "download, host topology edits, marginal system on the device, re-upload)")
+    return out
+
+
 def cpu_baseline(win, seconds=12.0):
     """Oracle FP64 restatement of the reference CPU path, one thread: steps/s on a bounded sample
     (repeated LM steps on the same window until ~`seconds` of CPU work)."""
@@ -312,6 +340,7 @@ def main():
                     help="keyframe edges of the device preintegration line (0: skip)")
     ap.add_argument("--whiten-observations", type=int, default=1 << 22,
                     help="pixel observations of the device whitening line (0: skip)")
+    ap.add_argument("--no-slides", dest="slides", action="store_false", help="skip the window-slide line")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the measurement, launch one plain device copy of this many MiB "
                          "(reference dispatch for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -371,6 +400,7 @@ def main():
     if args.whiten_observations > 0:
         print(f"[bench] whitening: {args.whiten_observations} observations", file=sys.stderr, flush=True)
         whiten = whitening_line(ctx, abi, args.whiten_observations)
+    slide = slide_line(ctx, abi, win) if args.slides else None
     if args.copy_probe_mb > 0:
         ctx.copy_probe(args.copy_probe_mb << 20)
     ctx.close()
@@ -419,6 +449,8 @@ def main():
     if serving is not None:
         out["serving"] = serving
     out["gate"] = gate
+    if slide is not None:
+        out["slide"] = slide
     if whiten is not None:
         out["whitening"] = whiten
     if preint is not None:

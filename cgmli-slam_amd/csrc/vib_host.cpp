@@ -308,6 +308,7 @@ void Slot::release() {
     if (arena) hipFree(arena);
     arena = nullptr;
     bytes = 0;
+    capacity = 0;
     used = false;
 }
 
@@ -331,10 +332,14 @@ size_t Slot::footprint(const HostWindow& h) {
 }
 
 bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
-    release();
+    const size_t need = footprint(h);
+    if (!arena || need > capacity) {  // a re-upload that fits (a slide) keeps the arena
+        release();
+        if (hipMalloc(&arena, need) != hipSuccess) { arena = nullptr; err = "device allocation failed"; return false; }
+        capacity = need;
+    }
     host = h;
-    bytes = footprint(h);
-    if (hipMalloc(&arena, bytes) != hipSuccess) { arena = nullptr; err = "device allocation failed"; return false; }
+    bytes = need;
     Arena a{reinterpret_cast<char*>(arena), bytes, 0};
     WinDev& d = dev;
     d = WinDev{};
@@ -864,6 +869,9 @@ WindowCopy copy_of(const HostWindow& h) {
     }
     w.lm_frame = h.lm_frame; w.lm_cam = h.lm_cam; w.rho = h.rho; w.lm_free = h.lm_free;
     const size_t L = h.L, F = h.F;
+    w.lm_bearing.reserve(3 * L); w.lm_vel.reserve(3 * L); w.lm_tref.reserve(L);
+    w.ob_lm.reserve(F); w.ob_frame.reserve(F); w.ob_cam.reserve(F); w.ob_tref.reserve(F);
+    w.ob_bearing.reserve(3 * F); w.ob_vel.reserve(3 * F); w.ob_sqrt.reserve(3 * F);
     for (size_t l = 0; l < L; ++l) {
         for (int i = 0; i < 3; ++i) {
             w.lm_bearing.push_back(h.lmd[(LM_BX + i) * L + l]);

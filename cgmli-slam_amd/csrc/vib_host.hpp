@@ -72,7 +72,8 @@ struct Slot {
     HostWindow host;
     WinDev dev{};
     void* arena = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;     // footprint of the resident window
+    size_t capacity = 0;  // size of the arena (at least bytes)
     double* reduced_lin = nullptr;
     double* reduced_schur = nullptr;
     int lin_precision = VIB_FP64;  // coupling-stream format of the last vib_linearize
