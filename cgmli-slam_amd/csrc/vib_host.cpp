@@ -2180,6 +2180,14 @@ namespace {
 // (BackendOptimizer.cpp:1053-1054): the window is rebuilt without them, and without the landmarks
 // whose every factor was rejected, before the slide.
 static int strip_rejected(vib_context* ctx, Slot& s) {
+    {  // the mirror's square-root information follows every rejection: scan it before any download
+        const size_t F = s.host.F;
+        const double* ob = s.host.ob.data();
+        bool any = false;
+        for (size_t f = 0; f < F && !any; ++f)
+            any = ob[OB_S00 * F + f] == 0.0 && ob[OB_S10 * F + f] == 0.0 && ob[OB_S11 * F + f] == 0.0;
+        if (!any) return VIB_OK;
+    }
     HostWindow h;  // the mirror with the current device state
     if (!pull_state(ctx, s, h)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
     const size_t F = h.F;
