@@ -2544,69 +2544,73 @@ __global__ __launch_bounds__(kStatThreads) void k_nis_stats(const double* energy
 // status: 0 ok, 1 outside the image, 2 back-projection failed, 3 singular covariance or
 // invalid whitening, 4 invalid sigma or camera index.
 // ---------------------------------------------------------------------------------------------
+// One observation of k_whiten: writes the record and returns the status.
+__device__ __forceinline__ int whiten_one(double2 uv, int c, double sg, const double* intr, int ncam, double& b0,
+                                          double& b1, double& b2, double& s00, double& s10, double& s11) {
+    int st = 0;
+    b0 = b1 = b2 = s00 = s10 = s11 = 0;
+    if (c < 0 || c >= ncam || !isfinite(sg) || !(sg > 0)) return 4;
+    const double* k = intr + 6 * c;  // fx fy cx cy width height
+    const double u = uv.x, v = uv.y;
+    if (!(u >= 0) || !(v >= 0) || !(u < k[4]) || !(v < k[5])) return 1;
+    const V3<double> ray = {(u - k[2]) / k[0], (v - k[3]) / k[1], 1.0};
+    const double rn = sqrt(sqnorm(ray));
+    if (!isfinite(rn) || rn < 1e-12 || !isfinite(1.0 / k[0]) || !isfinite(1.0 / k[1])) return 2;
+    const V3<double> b = {ray.x / rn, ray.y / rn, ray.z / rn};
+    V3<double> c0, c1;
+    sphere_basis(b, c0, c1);
+    // D = (I - b b^T) J / |ray|, J = d ray / d pixel = [1/fx 0; 0 1/fy; 0 0]; M = B^T D
+    const double j0 = 1.0 / k[0], j1 = 1.0 / k[1];
+    const V3<double> d0 = {(1 - b.x * b.x) * j0 / rn, (-b.y * b.x) * j0 / rn, (-b.z * b.x) * j0 / rn};
+    const V3<double> d1 = {(-b.x * b.y) * j1 / rn, (1 - b.y * b.y) * j1 / rn, (-b.z * b.y) * j1 / rn};
+    const double m00 = dot(c0, d0), m01 = dot(c0, d1), m10 = dot(c1, d0), m11 = dot(c1, d1);
+    const double s2 = sg * sg;
+    const double cov00 = s2 * (m00 * m00 + m01 * m01);
+    const double cov10 = s2 * (m10 * m00 + m11 * m01);
+    const double cov11 = s2 * (m10 * m10 + m11 * m11);
+    const double l00 = sqrt(cov00), l10 = cov10 / l00, l11 = sqrt(cov11 - l10 * l10);
+    if (!(l00 > 0) || !(l11 > 0)) return 3;
+    const double a00 = 1 / l00, a10 = -l10 / (l00 * l11), a11 = 1 / l11;
+    if (!isfinite(a00) || !isfinite(a10) || !isfinite(a11)) return 3;
+    s00 = a00; s10 = a10; s11 = a11;
+    b0 = b.x; b1 = b.y; b2 = b.z;
+    return st;
+}
+
+constexpr int kWhitenPer = 2;  // observations per thread (their loads issued together)
+
 __global__ __launch_bounds__(kBlock) void k_whiten(int count, const double* px, const int32_t* cam,
                                                    const double* intr, int ncam, const double* sigma,
                                                    double sigma_all, double* bearing, double* sqrt_info,
                                                    int32_t* status) {
-    __shared__ double stage[6 * kBlock];  // the block's records, stored back as contiguous runs
-    const int t = threadIdx.x, i0 = blockIdx.x * kBlock, i = i0 + t;
-    int st = 0;
-    double b0 = 0, b1 = 0, b2 = 0, s00 = 0, s10 = 0, s11 = 0;
-    if (i < count) {
-        const double2 uv = reinterpret_cast<const double2*>(px)[i];  // issued with the index loads
-        const int c = cam ? cam[i] : 0;
-        const double sg = sigma ? sigma[i] : sigma_all;
-        if (c < 0 || c >= ncam || !isfinite(sg) || !(sg > 0)) {
-            st = 4;
-        } else {
-            const double* k = intr + 6 * c;  // fx fy cx cy width height
-            const double u = uv.x, v = uv.y;
-            if (!(u >= 0) || !(v >= 0) || !(u < k[4]) || !(v < k[5])) {
-                st = 1;
-            } else {
-                const V3<double> ray = {(u - k[2]) / k[0], (v - k[3]) / k[1], 1.0};
-                const double rn = sqrt(sqnorm(ray));
-                if (!isfinite(rn) || rn < 1e-12 || !isfinite(1.0 / k[0]) || !isfinite(1.0 / k[1])) {
-                    st = 2;
-                } else {
-                    const V3<double> b = {ray.x / rn, ray.y / rn, ray.z / rn};
-                    V3<double> c0, c1;
-                    sphere_basis(b, c0, c1);
-                    // D = (I - b b^T) J / |ray|, J = d ray / d pixel = [1/fx 0; 0 1/fy; 0 0]; M = B^T D
-                    const double j0 = 1.0 / k[0], j1 = 1.0 / k[1];
-                    const V3<double> d0 = {(1 - b.x * b.x) * j0 / rn, (-b.y * b.x) * j0 / rn, (-b.z * b.x) * j0 / rn};
-                    const V3<double> d1 = {(-b.x * b.y) * j1 / rn, (1 - b.y * b.y) * j1 / rn, (-b.z * b.y) * j1 / rn};
-                    const double m00 = dot(c0, d0), m01 = dot(c0, d1), m10 = dot(c1, d0), m11 = dot(c1, d1);
-                    const double s2 = sg * sg;
-                    const double cov00 = s2 * (m00 * m00 + m01 * m01);
-                    const double cov10 = s2 * (m10 * m00 + m11 * m01);
-                    const double cov11 = s2 * (m10 * m10 + m11 * m11);
-                    const double l00 = sqrt(cov00), l10 = cov10 / l00, l11 = sqrt(cov11 - l10 * l10);
-                    if (!(l00 > 0) || !(l11 > 0)) {
-                        st = 3;
-                    } else {
-                        s00 = 1 / l00;
-                        s10 = -l10 / (l00 * l11);
-                        s11 = 1 / l11;
-                        if (!isfinite(s00) || !isfinite(s10) || !isfinite(s11)) {
-                            st = 3;
-                            s00 = s10 = s11 = 0;
-                        } else {
-                            b0 = b.x; b1 = b.y; b2 = b.z;
-                        }
-                    }
-                }
-            }
-        }
-        status[i] = st;
+    constexpr int nb = kWhitenPer * kBlock;  // observations of the block
+    __shared__ double stage[6 * nb];         // the block's records, stored back as contiguous runs
+    const int t = threadIdx.x, i0 = blockIdx.x * nb;
+    double2 uv[kWhitenPer];
+    int c[kWhitenPer];
+    double sg[kWhitenPer];
+#pragma unroll
+    for (int r = 0; r < kWhitenPer; ++r) {
+        const int i = i0 + r * kBlock + t;
+        const bool in = i < count;
+        uv[r] = in ? reinterpret_cast<const double2*>(px)[i] : make_double2(0.0, 0.0);
+        c[r] = in && cam ? cam[i] : 0;
+        sg[r] = in && sigma ? sigma[i] : sigma_all;
     }
-    stage[3 * t] = b0; stage[3 * t + 1] = b1; stage[3 * t + 2] = b2;
-    stage[3 * kBlock + 3 * t] = s00; stage[3 * kBlock + 3 * t + 1] = s10; stage[3 * kBlock + 3 * t + 2] = s11;
+#pragma unroll
+    for (int r = 0; r < kWhitenPer; ++r) {
+        const int i = i0 + r * kBlock + t, o = r * kBlock + t;
+        double b0, b1, b2, s00, s10, s11;
+        const int st = whiten_one(uv[r], c[r], sg[r], intr, ncam, b0, b1, b2, s00, s10, s11);
+        if (i < count) status[i] = st;
+        stage[3 * o] = b0; stage[3 * o + 1] = b1; stage[3 * o + 2] = b2;
+        stage[3 * nb + 3 * o] = s00; stage[3 * nb + 3 * o + 1] = s10; stage[3 * nb + 3 * o + 2] = s11;
+    }
     __syncthreads();
-    const int n3 = 3 * min(kBlock, count - i0);
+    const int n3 = 3 * min(nb, count - i0);
     for (int j = t; j < n3; j += kBlock) {
         bearing[3 * (size_t)i0 + j] = stage[j];
-        sqrt_info[3 * (size_t)i0 + j] = stage[3 * kBlock + j];
+        sqrt_info[3 * (size_t)i0 + j] = stage[3 * nb + j];
     }
 }
 
@@ -3168,7 +3172,8 @@ hipError_t vibk_nis_apply(const vib::WinDev* wins, const uint8_t* decision, int 
 hipError_t vibk_whiten(int count, const double* px, const int32_t* cam, const double* intr, int ncam,
                        const double* sigma, double sigma_all, double* bearing, double* sqrt_info, int32_t* status,
                        hipStream_t s) {
-    hipLaunchKernelGGL(vib::k_whiten, dim3((count + vib::kBlock - 1) / vib::kBlock), dim3(vib::kBlock), 0, s, count, px,
+    hipLaunchKernelGGL(vib::k_whiten, dim3((count + vib::kWhitenPer * vib::kBlock - 1) / (vib::kWhitenPer * vib::kBlock)),
+                       dim3(vib::kBlock), 0, s, count, px,
                        cam, intr, ncam, sigma, sigma_all, bearing, sqrt_info, status);
     return hipGetLastError();
 }
