@@ -333,6 +333,9 @@ size_t Slot::footprint(const HostWindow& h) {
 
 bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     const size_t need = footprint(h);
+    // The slot is live only once every part of the window is on the device: a failure on any
+    // path below leaves it unused (find_slot no longer returns it) with its arena kept for reuse.
+    used = false;
     if (!arena || need > capacity) {  // a re-upload that fits (a slide) keeps the arena
         release();
         if (hipMalloc(&arena, need) != hipSuccess) { arena = nullptr; err = "device allocation failed"; return false; }
@@ -407,8 +410,13 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     reduced_lin = a.take<double>(len); reduced_schur = a.take<double>(len);
     d.lm = a.take<LmState>(1);
     if (a.used > bytes) { err = "arena overflow"; return false; }
+    // every copy into the arena stays inside the carved footprint (host-side bounds check)
+    const char* lo = reinterpret_cast<const char*>(arena);
     auto up = [&](void* dst, const void* src, size_t n) {
-        return n == 0 || hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
+        if (n == 0) return true;
+        const char* p = reinterpret_cast<const char*>(dst);
+        if (!src || p < lo || p + n > lo + bytes) return false;
+        return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
     };
     std::vector<int32_t> fc(h.L);
     for (int l = 0; l < h.L; ++l) fc[l] = (h.lm_frame[l] << 8) | h.lm_cam[l];
@@ -437,13 +445,14 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
         // G = A^T A once per prior (one-window launch)
         WinDev* dw = nullptr;
         double** gp = nullptr;
-        if (hipMalloc(&dw, sizeof(WinDev)) != hipSuccess || hipMalloc(&gp, sizeof(double*)) != hipSuccess) {
-            err = "device allocation failed"; return false;
-        }
-        ok = up(dw, &d, sizeof(WinDev)) && up(gp, &pG, sizeof(double*)) &&
+        auto put = [&](void* dst, const void* src, size_t n) {
+            return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
+        };
+        ok = hipMalloc(&dw, sizeof(WinDev)) == hipSuccess && hipMalloc(&gp, sizeof(double*)) == hipSuccess &&
+             put(dw, &d, sizeof(WinDev)) && put(gp, &pG, sizeof(double*)) &&
              vibk_prior_gram(dw, 1, gp, stream) == hipSuccess && hipStreamSynchronize(stream) == hipSuccess;
-        hipFree(dw);
-        hipFree(gp);
+        if (dw) (void)hipFree(dw);
+        if (gp) (void)hipFree(gp);
         if (!ok) { err = "prior Gram matrix failed"; return false; }
     }
     if (hipStreamSynchronize(stream) != hipSuccess) { err = "upload synchronization failed"; return false; }
@@ -507,6 +516,15 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     }
     nchunks = (int)all.size();
     nlin = (int)lin.size();
+    // host-side bounds check of the work lists against the windows they index
+    for (const Chunk& c : all)
+        if (c.win < 0 || c.win >= nwin || c.lm_begin < 0 || c.lm_begin > c.lm_end || c.lm_end > wins[c.win].L ||
+            c.local >= wins[c.win].nchunks) { err = "landmark chunk outside its window"; return false; }
+    for (const Chunk& c : lin)
+        if (c.local >= 0 && (c.win < 0 || c.win >= nwin || c.lm_begin < 0 || c.lm_begin > c.lm_end ||
+                             c.lm_end > wins[c.win].F || c.local >= wins[c.win].nlin)) {
+            err = "visual factor chunk outside its window"; return false;
+        }
     // a rebuilt batch keeps its device buffers when they are large enough; a captured graph of
     // the previous contents is dropped
     if (graph_exec) { hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
@@ -2268,6 +2286,7 @@ int vib_read_window(vib_context* ctx, int32_t slot, vib_window* out) {
 int vib_read_prior(vib_context* ctx, int32_t slot, int32_t* rows, int32_t* cols, int32_t* blocks, int32_t* kind,
                    int64_t* id, double* ref, double* A, double* r0, int32_t cap_rows, int32_t cap_cols, int32_t cap_blocks) {
     if (!ctx || !rows || !cols || !blocks) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
     const HostWindow& h = s->host;

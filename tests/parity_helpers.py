@@ -1,0 +1,49 @@
+"""Comparison helpers shared by the GPU parity tests (not collected by pytest)."""
+import sys
+
+import numpy as np
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    d = np.linalg.norm(a - b)
+    n = max(np.linalg.norm(b), 1e-300)
+    return d / n
+
+
+def prior_columns(kind, ids):
+    """Tangent column range of every prior block, keyed by (kind, frame id)."""
+    widths = {0: 6, 1: 9, 2: 6, 3: 1}
+    out, c = {}, 0
+    for k, i in zip(kind, ids):
+        key = (int(k), int(i) if k in (0, 1) else 0)
+        out[key] = (c, widths[int(k)])
+        c += widths[int(k)]
+    return out, c
+
+
+def prior_errors(A_dev, r0_dev, kind_dev, id_dev, A_ref, r0_ref, kind_ref, id_ref):
+    """Relative errors of A^T A and A^T r0 of the device prior against the reference prior, on the
+    reference's kept subspace (eigenvector signs and rotations inside the cut are not comparable).
+    Also checks that both priors carry the same blocks."""
+    co, no = prior_columns(kind_ref, id_ref)
+    cd, nd = prior_columns(kind_dev, id_dev)
+    assert set(co) == set(cd) and no == nd, (sorted(co), sorted(cd))
+    perm = np.concatenate([np.arange(cd[k][0], cd[k][0] + cd[k][1]) for k in co])
+    Ad = A_dev[:, perm]
+    Gd, hd = Ad.T @ Ad, Ad.T @ r0_dev
+    Go, ho = A_ref.T @ A_ref, A_ref.T @ r0_ref
+    P = np.linalg.pinv(A_ref) @ A_ref
+    eg = np.linalg.norm(P @ Gd @ P - Go) / np.linalg.norm(Go)
+    eh = np.linalg.norm(P @ hd - ho) / max(np.linalg.norm(ho), 1e-300)
+    return eg, eh
+
+
+def canonical_obs(w):
+    order = np.lexsort((w.obs_cam, w.obs_frame, w.obs_landmark))
+    return {k: getattr(w, k)[order] for k in ("obs_landmark", "obs_frame", "obs_cam", "obs_bearing", "obs_velocity",
+                                               "obs_tref", "obs_sqrt_info")}
