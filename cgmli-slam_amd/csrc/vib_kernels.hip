@@ -1034,6 +1034,15 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const WinDev* wins, int which
     reduce_slice<kRedEntries>(wins[blockIdx.x], which, outs[2 * blockIdx.x + which], blockIdx.y, red);
 }
 
+// The visual-block partials are summed with 64 entries per workgroup (4 chunk lanes per entry)
+// whether the sum runs inside the Schur launch or as a launch of its own: one summation order,
+// so a window's result does not depend on the size of the batch it is solved in.
+constexpr int kRedEntriesFused = 64;
+__global__ __launch_bounds__(kBlock) void k_reduce_lin(const WinDev* wins, double** outs) {
+    __shared__ double red[kBlock];
+    reduce_slice<kRedEntriesFused>(wins[blockIdx.x], 0, outs[2 * blockIdx.x], blockIdx.y, red);
+}
+
 // ---------------------------------------------------------------------------------------------
 // k_schur: -c c^T / damped(d) over each landmark's support (VisualCudaBatch.cu:145-161).
 // ---------------------------------------------------------------------------------------------
@@ -1174,7 +1183,6 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
 // two workgroups per CU: 256 VGPRs (no AGPRs) and 74 KB of LDS each. Workgroups nchunks.. sum
 // the visual-block partials of the linearization (64 entries each, rslices per window), which
 // the Schur products do not need: k_assemble reads both sums.
-constexpr int kRedEntriesFused = 64;
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_schur(const WinDev* wins,
                                                                                               const Chunk* chunks,
                                                                                               int nchunks, int rslices,
@@ -3113,7 +3121,10 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             }
             break;
         case 1: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 0); break;
-        case 2: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 0, red); break;
+        case 2:
+            hipLaunchKernelGGL(k_reduce_lin, dim3(nwin, (max_len + kRedEntriesFused - 1) / kRedEntriesFused), dim3(kBlock), 0,
+                               s, wins, red);
+            break;
         case 3: {  // Schur products, then the visual-block partial sums in the same launch
             const int rslices = fuse_reduce ? (max_len + kRedEntriesFused - 1) / kRedEntriesFused : 0;
             const int grid = nchunks + nwin * rslices;

@@ -1312,7 +1312,8 @@ template <int N>
 M<N, N> mulABt(const M<N, N>& A, const M<N, N>& B) { return A * transpose(B); }
 
 int preintegrate(const double* samples, int count, double t0, double t1, const double* sbLin,
-                 const vib_imu_params& p, const double* gyroStart, const double* gyroEnd, double* out) {
+                 const vib_imu_params& p, const double* gyroStart, const double* gyroEnd, double* out,
+                 double* covOut = nullptr) {
     if (count < 2 || !(t1 > t0)) return VIB_ERR_INVALID_ARGUMENT;
     auto T = [&](int i) { return samples[i * 7]; };
     if (!(T(0) <= t0) || !(T(count - 1) >= t1)) return VIB_ERR_INVALID_ARGUMENT;
@@ -1399,6 +1400,7 @@ int preintegrate(const double* samples, int count, double t0, double t1, const d
         const M<15, 15> sym = 0.5 * dP[j] + 0.5 * transpose(dP[j]);
         for (int i = 0; i < 225; ++i) P[i] += sym.a[i] * sig[j] * sig[j];
     }
+    if (covOut) std::copy(P.begin(), P.end(), covOut);
     std::vector<double> ev, U;
     if (!jacobiEigen(P, 15, ev, U)) return VIB_ERR_EIGEN_FAILED;
     const double eps = std::numeric_limits<double>::epsilon();
@@ -2002,6 +2004,13 @@ int oracle_second_new_system(const vib_window* win, const double* samples, int c
 int oracle_preintegrate(const double* samples, int count, double t0, double t1, const double* sb,
                         const vib_imu_params* p, const double* g0, const double* g1, double* out) {
     return preintegrate(samples, count, t0, t1, sb, *p, g0, g1, out);
+}
+
+// The same preintegration, also returning the 15x15 covariance P (row-major, state order
+// [p, alpha, v, b_g, b_a]) that the whitening W is computed from.
+int oracle_preintegrate_cov(const double* samples, int count, double t0, double t1, const double* sb,
+                            const vib_imu_params* p, const double* g0, const double* g1, double* out, double* cov) {
+    return preintegrate(samples, count, t0, t1, sb, *p, g0, g1, out, cov);
 }
 
 // FrameStore::observation for an undistorted pinhole (FrameStore.cpp:102-124,

@@ -47,6 +47,8 @@ def load(abi):
     lib.oracle_second_new_system.argtypes = [W, dp, C.c_int, C.c_double, C.c_double, C.POINTER(abi.VibImuParams),
                                              ip, dp, dp, ip]
     lib.oracle_preintegrate.argtypes = [dp, C.c_int, C.c_double, C.c_double, dp, C.POINTER(abi.VibImuParams), dp, dp, dp]
+    lib.oracle_preintegrate_cov.argtypes = [dp, C.c_int, C.c_double, C.c_double, dp, C.POINTER(abi.VibImuParams), dp,
+                                            dp, dp, dp]
     lib.oracle_pinhole_observation.argtypes = [C.c_double] * 7 + [dp, dp]
     lib.oracle_eigen_sym.argtypes = [dp, C.c_int, dp, dp]
     lib.oracle_pose_plus.argtypes = [dp, dp, dp]
@@ -225,6 +227,18 @@ class Oracle:
         g1 = np.ascontiguousarray(g1, np.float64)
         st = self.lib.oracle_preintegrate(_p(s), s.shape[0], t0, t1, _p(sb), C.byref(params), _p(g0), _p(g1), _p(out))
         return st, out
+
+    def preintegrate_cov(self, samples, t0, t1, sb, params, g0, g1):
+        """Preintegration record and the 15x15 covariance it whitens."""
+        out = np.zeros(self.abi.IMU_STRIDE)
+        P = np.zeros((15, 15))
+        s = np.ascontiguousarray(samples, np.float64)
+        sb = np.ascontiguousarray(sb, np.float64)
+        g0 = np.ascontiguousarray(g0, np.float64)
+        g1 = np.ascontiguousarray(g1, np.float64)
+        st = self.lib.oracle_preintegrate_cov(_p(s), s.shape[0], t0, t1, _p(sb), C.byref(params), _p(g0), _p(g1),
+                                              _p(out), _p(P))
+        return st, out, P
 
     def pinhole_observation(self, fx, fy, cx, cy, u, v, sigma):
         b = np.zeros(3)

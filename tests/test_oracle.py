@@ -245,6 +245,223 @@ def test_preintegration_covariance_scaling(oracle, abi):
     assert np.allclose(W2.T @ W2 * 4, W1.T @ W1, rtol=1e-8)
 
 
+
+# ---------------------------------------------------------------------------------------------
+# Preintegration pins (ImuError::redoPreintegration, ImuError.cpp:254-462): the bias Jacobians
+# against central differences of the deltas re-integrated at b +- h, and the 15x15 covariance
+# against closed-form sums of the error-state recursion for motions where it solves by hand.
+# ---------------------------------------------------------------------------------------------
+def _so3_log(R):
+    c = np.clip((np.trace(R) - 1) / 2, -1, 1)
+    a = np.arccos(c)
+    v = np.array([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]]) / 2
+    return v if a < 1e-12 else v * a / np.sin(a)
+
+
+def _skew(a):
+    return np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+
+
+def _preint_pair(oracle, abi, samples, t0, t1, sb, params, g0, g1):
+    """(oracle record, native vib_preintegrate record) of one interval."""
+    st, ref = oracle.preintegrate(samples, t0, t1, sb, params, g0, g1)
+    assert st == 0
+    return ref, abi.preintegrate(samples, t0, t1, sb, params, g0, g1)
+
+
+@pytest.mark.parametrize("impl", ["oracle", "native"])
+def test_preintegration_bias_jacobians_central_differences(oracle, abi, impl):
+    """Record entries 31-75 (d alpha/d b_g, dv/d b_g, dp/d b_g, the C integral = -dv/d b_a and the
+    C double integral = -dp/d b_a) equal central differences of the preintegrated deltas with the
+    interval re-integrated at the bias +- h. Rotation: Dq(b_g + d) = Exp(-A d) Dq(b_g) (left
+    perturbation in the interval's start frame)."""
+    from vibpy import synth
+    w = synth.generate(K=4, L=20, seed=109)
+    t = w.meta["frame_t"]
+    smp, prm = w.meta["samples"], w.meta["imu_params"]
+    sb0 = w.speed_bias[1].copy()
+    sb0[3:6] += [0.02, -0.015, 0.01]   # bias far from zero so the Jacobians are exercised
+    sb0[6:9] += [0.05, -0.03, 0.04]
+    g0, g1 = w.imu_data[1, 9:12], w.imu_data[1, 12:15]
+
+    def rec(sb):
+        pair = _preint_pair(oracle, abi, smp, t[1], t[2], sb, prm, g0, g1)
+        return pair[0] if impl == "oracle" else pair[1]
+
+    r0 = rec(sb0)
+    R0 = synth.matrix_from_quat(r0[15:19])
+    A, Vg, Pg = r0[31:40].reshape(3, 3), r0[40:49].reshape(3, 3), r0[49:58].reshape(3, 3)
+    Cint, Cdint = r0[58:67].reshape(3, 3), r0[67:76].reshape(3, 3)
+    for i in range(3):
+        h = 1e-5
+        sp, sm = sb0.copy(), sb0.copy()
+        sp[3 + i] += h
+        sm[3 + i] -= h
+        rp, rm = rec(sp), rec(sm)
+        dth = (_so3_log(synth.matrix_from_quat(rp[15:19]) @ R0.T) - _so3_log(synth.matrix_from_quat(rm[15:19]) @ R0.T)) / (2 * h)
+        np.testing.assert_allclose(dth, -A[:, i], rtol=1e-6, atol=1e-9 * np.abs(A).max())
+        np.testing.assert_allclose((rp[19:22] - rm[19:22]) / (2 * h), Vg[:, i], rtol=1e-6, atol=1e-8 * np.abs(Vg).max())
+        np.testing.assert_allclose((rp[22:25] - rm[22:25]) / (2 * h), Pg[:, i], rtol=1e-6, atol=1e-8 * np.abs(Pg).max())
+        # accelerometer bias: the deltas are linear in b_a (exact differences)
+        h = 1e-3
+        sp, sm = sb0.copy(), sb0.copy()
+        sp[6 + i] += h
+        sm[6 + i] -= h
+        rp, rm = rec(sp), rec(sm)
+        np.testing.assert_allclose((rp[19:22] - rm[19:22]) / (2 * h), -Cint[:, i], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose((rp[22:25] - rm[22:25]) / (2 * h), -Cdint[:, i], rtol=1e-9, atol=1e-12)
+        np.testing.assert_array_equal(rp[15:19], r0[15:19])  # the rotation does not depend on b_a
+
+
+def _still_samples(N, T, accel):
+    """N + 1 samples over [0, T]: zero angular rate and a constant specific force."""
+    smp = np.zeros((N + 1, 7))
+    smp[:, 0] = np.arange(N + 1) * (T / N)
+    smp[-1, 0] = T
+    smp[:, 4:7] = accel
+    return smp
+
+
+def _cov(oracle, abi, smp, T, sig, accel=(0.0, 0.0, 0.0)):
+    p = abi.imu_params(sigma_g_c=sig[0], sigma_a_c=sig[1], sigma_gw_c=sig[2], sigma_aw_c=sig[3])
+    st, rec, P = oracle.preintegrate_cov(smp, 0.0, T, np.zeros(9), p, np.zeros(3), np.zeros(3))
+    assert st == 0
+    return rec, P
+
+
+def _blocks(P):
+    return {(a, b): P[3 * a:3 * a + 3, 3 * b:3 * b + 3] for a in range(5) for b in range(5)}
+
+
+def test_preintegration_covariance_closed_forms_zero_motion(oracle, abi):
+    """Zero rate, zero specific force, zero bias, N steps of dt (T = N dt), one noise density at a
+    time. The error-state recursion (state [p, alpha, v, b_g, b_a], ImuError.cpp:370-420) reduces
+    per axis to scalar sums with these closed forms (S2 = sum_{k<N} k^2 = (N-1)N(2N-1)/6):
+      gyro noise:        P_aa = s^2 T
+      accel noise:       P_vv = s^2 T, P_pv = s^2 dt^2 N(N-1)/2, P_pp = s^2 dt^3 (S2 + N/2)
+                         (the last term is the reference's 0.5 dt^3 position noise)
+      gyro bias walk:    P_gg = s^2 T, P_ag = -s^2 dt^2 N(N-1)/2, P_aa = s^2 dt^3 S2
+      accel bias walk:   P_bb = s^2 T, P_vb = -s^2 dt^2 N(N-1)/2, P_vv = s^2 dt^3 S2, and the
+                         position rows from the explicit solution p_N = sum_j a_j w_j with
+                         a_j = -dt^2 [m(m-1)/2 + N(N-1)/2 - j(j+1)/2 - m/2], m = N-1-j
+                         (F_pb = -dt C_int + dt^2/2 I double-counts the earlier steps' bias)
+    every other block is zero. Also the deltas: Dq = 1, dv = dp = 0, A = T I, C_int = T I,
+    C_dint = T^2/2 I."""
+    N, T = 20, 0.1
+    dt = T / N
+    I3 = np.eye(3)
+    smp = _still_samples(N, T, (0.0, 0.0, 0.0))
+    S2 = (N - 1) * N * (2 * N - 1) / 6
+    tri = N * (N - 1) / 2
+    s = 0.3
+    rec, _ = _cov(oracle, abi, smp, T, (s, 0, 0, 0))
+    np.testing.assert_allclose(rec[15:19], [0, 0, 0, 1], atol=1e-15)
+    np.testing.assert_allclose(rec[19:25], 0, atol=1e-15)
+    np.testing.assert_allclose(rec[31:40].reshape(3, 3), T * I3, rtol=1e-13)
+    np.testing.assert_allclose(rec[58:67].reshape(3, 3), T * I3, rtol=1e-13)
+    np.testing.assert_allclose(rec[67:76].reshape(3, 3), T * T / 2 * I3, rtol=1e-13)
+    np.testing.assert_allclose(rec[40:58], 0, atol=1e-15)
+    cases = []
+    exp = np.zeros((15, 15))
+    exp[3:6, 3:6] = s * s * T * I3
+    cases.append(((s, 0, 0, 0), exp))
+    exp = np.zeros((15, 15))
+    exp[6:9, 6:9] = s * s * T * I3
+    exp[0:3, 6:9] = exp[6:9, 0:3] = s * s * dt * dt * tri * I3
+    exp[0:3, 0:3] = s * s * dt ** 3 * (S2 + N / 2) * I3
+    cases.append(((0, s, 0, 0), exp))
+    exp = np.zeros((15, 15))
+    exp[9:12, 9:12] = s * s * T * I3
+    exp[3:6, 9:12] = exp[9:12, 3:6] = -s * s * dt * dt * tri * I3
+    exp[3:6, 3:6] = s * s * dt ** 3 * S2 * I3
+    cases.append(((0, 0, s, 0), exp))
+    exp = np.zeros((15, 15))
+    j = np.arange(N)
+    m = N - 1 - j
+    a = -dt * dt * (m * (m - 1) / 2 + N * (N - 1) / 2 - j * (j + 1) / 2 - m / 2)
+    b = -dt * m
+    coef = np.stack([a, b, np.ones(N)])       # rows p, v, b_a
+    G = s * s * dt * coef @ coef.T
+    idx = [slice(0, 3), slice(6, 9), slice(12, 15)]
+    for r in range(3):
+        for c in range(3):
+            exp[idx[r], idx[c]] = G[r, c] * I3
+    np.testing.assert_allclose(G[2, 2], s * s * T, rtol=1e-13)
+    np.testing.assert_allclose(G[1, 2], -s * s * dt * dt * tri, rtol=1e-13)
+    np.testing.assert_allclose(G[1, 1], s * s * dt ** 3 * S2, rtol=1e-13)
+    cases.append(((0, 0, 0, s), exp))
+    for sig, exp in cases:
+        _, P = _cov(oracle, abi, smp, T, sig)
+        np.testing.assert_allclose(P, exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max(), err_msg=str(sig))
+
+
+def test_preintegration_covariance_closed_forms_constant_force(oracle, abi):
+    """Zero rate and a constant specific force a: the rotation error moves the velocity and
+    position through -[a]x (F_va = -dt [a]x, F_pa = -dt^2 (k + 1/2) [a]x at step k), and the gyro
+    bias through dv/db_g = dt^2 k^2 / 2 [a]x (F_vg = dt^2 (2k+1)/2 [a]x, F_pg = dt^3 (k^2/2 + k/2 +
+    1/4) [a]x). Gyro noise and gyro bias walk, one at a time, against the explicit solutions
+    x_N = sum_j G_j w_j of the recursion (coefficients summed per step below); the deltas are
+    dv = T a, dp = T^2/2 a, dv/db_g = T^2/2 [a]x."""
+    N, T = 20, 0.1
+    dt = T / N
+    acc = np.array([0.7, -1.3, 9.81])
+    ax = _skew(acc)
+    I3 = np.eye(3)
+    smp = _still_samples(N, T, acc)
+    s = 0.3
+    rec, P = _cov(oracle, abi, smp, T, (s, 0, 0, 0))
+    np.testing.assert_allclose(rec[19:22], T * acc, rtol=1e-13)
+    np.testing.assert_allclose(rec[22:25], T * T / 2 * acc, rtol=1e-13)
+    np.testing.assert_allclose(rec[40:49].reshape(3, 3), T * T / 2 * ax, rtol=1e-12, atol=1e-15)
+    # gyro white noise w_j enters alpha after step j
+    G = []
+    for jj in range(N):
+        ca = 1.0
+        cv = -dt * sum(1.0 for k in range(jj + 1, N))                      # v_N coefficient of [a]x
+        cp = sum(-dt * dt * (k - 1 - jj) for k in range(jj + 2, N)) - sum(dt * dt * (k + 0.5) for k in range(jj + 1, N))
+        G.append(np.vstack([cp * ax, ca * I3, cv * ax]))                   # rows p, alpha, v
+    G = np.array(G)
+    exp = s * s * dt * np.einsum("jab,jcb->ac", G, G)
+    np.testing.assert_allclose(P[0:9, 0:9], exp, rtol=1e-11, atol=1e-12 * np.abs(exp).max())
+    np.testing.assert_allclose(P[9:, :], 0, atol=1e-18)
+    # gyro bias random walk w_j enters b_g after step j
+    rec, P = _cov(oracle, abi, smp, T, (0, 0, s, 0))
+    G = []
+    for jj in range(N):
+        def bg(k):
+            return 1.0 if k > jj else 0.0
+
+        def al(k):  # alpha_k coefficient
+            return -dt * max(k - 1 - jj, 0)
+
+        def vk(k):  # v_k coefficient (of [a]x)
+            return sum(dt * dt * max(i - 1 - jj, 0) + 0.5 * dt * dt * (2 * i + 1) * bg(i) for i in range(k))
+
+        cv = vk(N)
+        cp = sum(dt * vk(k) + dt * dt * (k + 0.5) * dt * max(k - 1 - jj, 0) + dt ** 3 * (0.5 * k * k + 0.5 * k + 0.25) * bg(k)
+                 for k in range(N))
+        G.append(np.vstack([cp * ax, al(N) * I3, cv * ax, bg(N) * I3]))      # rows p, alpha, v, b_g
+    G = np.array(G)
+    exp = s * s * dt * np.einsum("jab,jcb->ac", G, G)
+    np.testing.assert_allclose(P[0:12, 0:12], exp, rtol=1e-11, atol=1e-12 * np.abs(exp).max())
+
+
+def test_preintegration_whitening_inverts_the_covariance(oracle, abi):
+    """The record's whitening W (entries 76-300) satisfies W^T W = P^-1 for a full-rank P, in
+    both the oracle and the native vib_preintegrate (IMU residual r = W e, ImuError.cpp:436-462)."""
+    from vibpy import synth
+    w = synth.generate(K=3, L=10, seed=110)
+    t = w.meta["frame_t"]
+    smp, prm = w.meta["samples"], w.meta["imu_params"]
+    st, rec, P = oracle.preintegrate_cov(smp, t[0], t[1], w.speed_bias[0], prm, w.imu_data[0, 9:12], w.imu_data[0, 12:15])
+    assert st == 0
+    native = abi.preintegrate(smp, t[0], t[1], w.speed_bias[0], prm, w.imu_data[0, 9:12], w.imu_data[0, 12:15])
+    Pinv = np.linalg.inv(P)
+    for r in (rec, native):
+        W = r[76:].reshape(15, 15)
+        np.testing.assert_allclose(W.T @ W, Pinv, rtol=1e-7, atol=1e-9 * np.abs(Pinv).max())
+
+
 def dense_system(win, oracle, lam):
     """Brute force: full dense normal equations over [states, rho] and the LM-damped solve."""
     lin = oracle.linearize(win)
