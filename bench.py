@@ -299,26 +299,77 @@ def slide_line(ctx, abi, win, reps=3):
     return out
 
 
-def cpu_baseline(win, seconds=12.0):
-    """Oracle FP64 restatement of the reference CPU path, one thread: steps/s on a bounded sample
-    (repeated LM steps on the same window until ~`seconds` of CPU work)."""
+def host_info():
+    """Host cores usable by this process, CPU model and NUMA layout (printed beside the CPU line)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nodes = []
+    base = "/sys/devices/system/node"
+    if os.path.isdir(base):
+        for d in sorted(os.listdir(base)):
+            if d.startswith("node") and d[4:].isdigit():
+                try:
+                    with open(os.path.join(base, d, "cpulist")) as f:
+                        nodes.append(f"{d}: cpus {f.read().strip()}")
+                except OSError:
+                    pass
+    return dict(nproc=os.cpu_count(), usable_cpus=usable, cpu_model=model, numa=nodes)
+
+
+def cpu_baseline(win, seconds=12.0, threads=None, bwins=None):
+    """The oracle's block-sparse FP64 LM step (per-landmark Schur on the landmark's support
+    columns, std::thread over landmark ranges) timed on this host: all usable cores and one
+    thread, p50/p99 per step, on a bounded sample (about `seconds` in total). With `bwins`, also
+    the batched workload: every window stepped once per round, whole windows spread over the
+    threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     from vibpy import abi
     o = oracle_py.Oracle(abi)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        st, _ = o.solve_step(win, 1e-4)
+    info = host_info()
+    # one GPU's share of the host is 16 cores, whatever the affinity mask shows
+    threads = threads or max(1, min(info["usable_cpus"], 16))
+    print(f"[bench] cpu baseline: {info['cpu_model']}, {info['usable_cpus']} usable of {info['nproc']} cpus, "
+          f"numa {info['numa']}", file=sys.stderr, flush=True)
+    st, ms1 = o.time_steps(win, 1e-4, 1, 2)       # probe one step on one thread to size the samples
+    if st != 0:
+        raise RuntimeError(f"oracle step failed: {st}")
+    per = float(np.median(ms1)) * 1e-3
+    n1 = int(max(5, min(200, 0.35 * seconds / per)))
+    n_all = int(max(10, min(2000, 0.35 * seconds / per * threads / 2)))
+    st, ms_all = o.time_steps(win, 1e-4, threads, n_all)
+    if st != 0:
+        raise RuntimeError(f"oracle step failed: {st}")
+    st, ms_one = o.time_steps(win, 1e-4, 1, n1)
+    if st != 0:
+        raise RuntimeError(f"oracle step failed: {st}")
+    out = dict(value=1e3 / float(np.mean(ms_all)), unit="window-steps/s", cores=threads, kind="port",
+               p50_ms=float(np.percentile(ms_all, 50)), p99_ms=float(np.percentile(ms_all, 99)),
+               one_thread=dict(value=1e3 / float(np.mean(ms_one)), p50_ms=float(np.percentile(ms_one, 50)),
+                               p99_ms=float(np.percentile(ms_one, 99)), steps=n1),
+               host=info,
+               sample=f"{n_all} LM steps (linearize + landmark Schur + Cholesky + back-substitution + trial cost) of "
+                      f"the benchmark window on {threads} host threads, and {n1} on 1 thread; oracle block-sparse "
+                      f"FP64 step (restatement of the reference CPU path's math; Ceres is not available)")
+    if bwins:
+        st, sec = o.time_windows(bwins, 1e-4, threads, 1)
         if st != 0:
-            raise RuntimeError(f"oracle step failed: {st}")
-        n += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    dt = time.perf_counter() - t0
-    return dict(value=n / dt, unit="window-steps/s", cores=1, kind="port",
-                sample=f"{n} LM steps (linearize + Schur + Cholesky + back-substitution + trial cost) "
-                       f"of the same window on 1 host thread, {dt:.1f} s")
+            raise RuntimeError(f"oracle batched step failed: {st}")
+        out["batched"] = dict(value=len(bwins) / sec, unit="window-steps/s", cores=threads, seconds=sec,
+                              sample=f"one LM step of each of the {len(bwins)} batched windows, whole windows "
+                                     f"spread over {threads} host threads")
+    return out
 
 
 def main():
@@ -456,7 +507,7 @@ def main():
     if preint is not None:
         out["preintegration"] = preint
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(win, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(win, args.cpu_seconds, bwins=bwins)
     print(json.dumps(out))
 
 

@@ -562,6 +562,56 @@ bool priorEvaluate(const Window& w, const Layout& lay, const State& x, std::vect
     return true;
 }
 
+// IMU and prior rows added into the dense retained system H (nr x nr, full), g (first nr
+// entries) and cost (GpuRetainedFactorMath.hpp:53-140, VisualCudaBatch.cu:216-289).
+bool retainedInto(const Window& w, const Layout& lay, const State& x, std::vector<double>& H,
+                  std::vector<double>& g, double& cost) {
+    bool valid = true;
+    // IMU
+    for (int e = 0; e < w.E; ++e) {
+        const Frozen imu = unpackImu(&w.imu[(size_t)VIB_IMU_STRIDE * e]);
+        const int s0 = w.imuFrom[e], s1 = w.imuTo[e];
+        double r[15], J[4][15 * 9];
+        if (!imuEvaluate(imu, &x.pose[7 * s0], &x.sb[9 * s0], &x.pose[7 * s1], &x.sb[9 * s1], r, J)) {
+            valid = false; continue;
+        }
+        const int col[4] = {lay.poseCol[s0], lay.sbCol[s0], lay.poseCol[s1], lay.sbCol[s1]};
+        const int width[4] = {col[0] >= 0 ? 6 : 0, col[1] >= 0 ? w.sbWidth[s0] : 0,
+                              col[2] >= 0 ? 6 : 0, col[3] >= 0 ? w.sbWidth[s1] : 0};
+        for (int i = 0; i < 15; ++i) cost += 0.5 * r[i] * r[i];
+        for (int bi = 0; bi < 4; ++bi)
+            for (int ci = 0; ci < width[bi]; ++ci) {
+                const int gi = col[bi] + ci;
+                double gsum = 0;
+                for (int i = 0; i < 15; ++i) gsum += J[bi][i * 9 + ci] * r[i];
+                g[gi] += gsum;
+                for (int bj = 0; bj < 4; ++bj)
+                    for (int cj = 0; cj < width[bj]; ++cj) {
+                        double s = 0;
+                        for (int i = 0; i < 15; ++i) s += J[bi][i * 9 + ci] * J[bj][i * 9 + cj];
+                        H[(size_t)gi * lay.nr + col[bj] + cj] += s;
+                    }
+            }
+    }
+    // prior
+    if (w.priorRows) {
+        std::vector<double> r, J;
+        if (!priorEvaluate(w, lay, x, r, &J)) valid = false;
+        else {
+            for (int i = 0; i < w.priorRows; ++i) cost += 0.5 * r[i] * r[i];
+            for (int i = 0; i < w.priorRows; ++i) {
+                const double* row = &J[(size_t)i * lay.nr];
+                for (int a = 0; a < lay.nr; ++a) {
+                    if (row[a] == 0) continue;
+                    g[a] += row[a] * r[i];
+                    for (int b = 0; b < lay.nr; ++b) H[(size_t)a * lay.nr + b] += row[a] * row[b];
+                }
+            }
+        }
+    }
+    return valid;
+}
+
 // Full linearization at x (visual + IMU + prior). Mirrors launchVisualLinearization +
 // evaluateRetained: robustified tangent rows accumulated into pose block, cross, diag, g.
 Linear linearize(const Window& w, const Layout& lay, const State& x) {
@@ -617,48 +667,7 @@ Linear linearize(const Window& w, const Layout& lay, const State& x) {
             }
         }
     }
-    // IMU
-    for (int e = 0; e < w.E; ++e) {
-        const Frozen imu = unpackImu(&w.imu[(size_t)VIB_IMU_STRIDE * e]);
-        const int s0 = w.imuFrom[e], s1 = w.imuTo[e];
-        double r[15], J[4][15 * 9];
-        if (!imuEvaluate(imu, &x.pose[7 * s0], &x.sb[9 * s0], &x.pose[7 * s1], &x.sb[9 * s1], r, J)) {
-            L.valid = false; continue;
-        }
-        const int col[4] = {lay.poseCol[s0], lay.sbCol[s0], lay.poseCol[s1], lay.sbCol[s1]};
-        const int width[4] = {col[0] >= 0 ? 6 : 0, col[1] >= 0 ? w.sbWidth[s0] : 0,
-                              col[2] >= 0 ? 6 : 0, col[3] >= 0 ? w.sbWidth[s1] : 0};
-        for (int i = 0; i < 15; ++i) L.cost += 0.5 * r[i] * r[i];
-        for (int bi = 0; bi < 4; ++bi)
-            for (int ci = 0; ci < width[bi]; ++ci) {
-                const int gi = col[bi] + ci;
-                double gsum = 0;
-                for (int i = 0; i < 15; ++i) gsum += J[bi][i * 9 + ci] * r[i];
-                L.g[gi] += gsum;
-                for (int bj = 0; bj < 4; ++bj)
-                    for (int cj = 0; cj < width[bj]; ++cj) {
-                        double s = 0;
-                        for (int i = 0; i < 15; ++i) s += J[bi][i * 9 + ci] * J[bj][i * 9 + cj];
-                        L.H[(size_t)gi * lay.nr + col[bj] + cj] += s;
-                    }
-            }
-    }
-    // prior
-    if (w.priorRows) {
-        std::vector<double> r, J;
-        if (!priorEvaluate(w, lay, x, r, &J)) L.valid = false;
-        else {
-            for (int i = 0; i < w.priorRows; ++i) L.cost += 0.5 * r[i] * r[i];
-            for (int i = 0; i < w.priorRows; ++i) {
-                const double* row = &J[(size_t)i * lay.nr];
-                for (int a = 0; a < lay.nr; ++a) {
-                    if (row[a] == 0) continue;
-                    L.g[a] += row[a] * r[i];
-                    for (int b = 0; b < lay.nr; ++b) L.H[(size_t)a * lay.nr + b] += row[a] * row[b];
-                }
-            }
-        }
-    }
+    if (!retainedInto(w, lay, x, L.H, L.g, L.cost)) L.valid = false;
     if (!std::isfinite(L.cost)) L.valid = false;
     return L;
 }
@@ -843,6 +852,283 @@ Step solveStep(const Window& w, const Layout& lay, const Linear& L, const State&
     st.trialCost = tc;
     st.trialValid = costOk && std::isfinite(tc);
     return st;
+}
+
+// ------------------------------------------------------------------------------------
+// Block-sparse, multi-threaded LM step (the timed CPU baseline). The same arithmetic as
+// linearize + solveStep, with the landmark Schur complement formed landmark by landmark on the
+// landmark's support columns only (anchor and observer poses, rig, td), as Ceres'
+// SchurEliminator does for the reference CPU path (backend_visual_batch: cpu,
+// BackendOptimizer.cpp:900-989, DENSE_SCHUR), and the landmark work split over std::thread
+// ranges. Sums are combined in thread order, so the result depends only on the thread count.
+// Checked against the dense solveStep in tests/test_oracle.py.
+// ------------------------------------------------------------------------------------
+struct Prepared {  // topology shared by every step of one window
+    Window w;
+    Layout lay;
+    Bounds abs;
+    std::vector<int> lmOff, lmFac;  // factors of each landmark (CSR)
+};
+
+Prepared prepare(const vib_window& win) {
+    Prepared p;
+    p.w = load(win);
+    p.lay = layoutOf(p.w);
+    p.abs = boundsOf(p.w, p.lay);
+    p.lmOff.assign(p.w.L + 1, 0);
+    for (int f = 0; f < p.w.F; ++f) p.lmOff[p.w.obsLm[f] + 1]++;
+    for (int l = 0; l < p.w.L; ++l) p.lmOff[l + 1] += p.lmOff[l];
+    std::vector<int> cur(p.lmOff.begin(), p.lmOff.end() - 1);
+    p.lmFac.assign(p.w.F, 0);
+    for (int f = 0; f < p.w.F; ++f) p.lmFac[cur[p.w.obsLm[f]]++] = f;
+    return p;
+}
+
+struct LmTerms {  // one landmark's coupling on its support columns
+    int begin = 0, count = 0;  // range in the thread's column / value lists
+    double d = 0, g = 0;
+};
+
+struct ThreadPart {
+    std::vector<double> H, Sc, g, gr;  // visual block, Schur correction, gradients (nr)
+    std::vector<int> col;
+    std::vector<double> val;
+    std::vector<LmTerms> lm;  // landmarks of this thread's range, in order
+    double cost = 0, pred = 0, maxInc = 0;
+    bool valid = true, invalid = false;
+};
+
+template <class Fn>
+void parallelRanges(int n, int threads, Fn fn) {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) {
+        const int a = (int)((int64_t)n * t / threads), b = (int)((int64_t)n * (t + 1) / threads);
+        if (threads == 1) fn(t, a, b);
+        else pool.emplace_back(fn, t, a, b);
+    }
+    for (auto& th : pool) th.join();
+}
+
+// Upper-triangle row-major Cholesky S = U^T U in place, right-looking (row updates vectorize).
+bool choleskyUpper(std::vector<double>& A, int n) {
+    for (int k = 0; k < n; ++k) {
+        double* rk = &A[(size_t)k * n];
+        const double d = rk[k];
+        if (!(d > 0) || !std::isfinite(d)) return false;
+        const double piv = std::sqrt(d);
+        rk[k] = piv;
+        for (int j = k + 1; j < n; ++j) rk[j] /= piv;
+        for (int i = k + 1; i < n; ++i) {
+            double* ri = &A[(size_t)i * n];
+            const double u = rk[i];
+            if (u == 0) continue;
+            for (int j = i; j < n; ++j) ri[j] -= u * rk[j];
+        }
+    }
+    return true;
+}
+void choleskyUpperSolve(const std::vector<double>& U, int n, std::vector<double>& b) {
+    for (int i = 0; i < n; ++i) {  // U^T y = b
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= U[(size_t)k * n + i] * b[k];
+        b[i] = s / U[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {  // U x = y
+        double s = b[i];
+        for (int k = i + 1; k < n; ++k) s -= U[(size_t)i * n + k] * b[k];
+        b[i] = s / U[(size_t)i * n + i];
+    }
+}
+
+bool retainedCost(const Window& w, const Layout& lay, const State& x, double& cost) {
+    for (int e = 0; e < w.E; ++e) {
+        const Frozen imu = unpackImu(&w.imu[(size_t)VIB_IMU_STRIDE * e]);
+        const int s0 = w.imuFrom[e], s1 = w.imuTo[e];
+        double r[15];
+        if (!imuEvaluate(imu, &x.pose[7 * s0], &x.sb[9 * s0], &x.pose[7 * s1], &x.sb[9 * s1], r, nullptr)) return false;
+        for (int i = 0; i < 15; ++i) cost += 0.5 * r[i] * r[i];
+    }
+    if (w.priorRows) {
+        std::vector<double> r;
+        if (!priorEvaluate(w, lay, x, r, nullptr)) return false;
+        for (double v : r) cost += 0.5 * v * v;
+    }
+    return true;
+}
+
+int blockStep(const Prepared& P, const State& x, double lambda, int threads, std::vector<double>& delta,
+              double* metrics) {
+    const Window& w = P.w;
+    const Layout& lay = P.lay;
+    const int nr = lay.nr;
+    threads = std::max(1, std::min(threads, std::max(1, w.L)));
+    std::vector<ThreadPart> parts(threads);
+    // linearize the visual factors and eliminate each landmark on its support (one pass)
+    parallelRanges(w.L, threads, [&](int t, int l0, int l1) {
+        ThreadPart& tp = parts[t];
+        tp.H.assign((size_t)nr * nr, 0); tp.Sc.assign((size_t)nr * nr, 0);
+        tp.g.assign(nr, 0); tp.gr.assign(nr, 0);
+        std::vector<int> local(nr, -1);
+        double plusA[42], plusB[42], plusR[42];
+        int cols[20];
+        double v0[20], v1[20];
+        for (int l = l0; l < l1; ++l) {
+            LmTerms lt;
+            lt.begin = (int)tp.col.size();
+            const int rhoCol = lay.lmCol[l];
+            for (int q = P.lmOff[l]; q < P.lmOff[l + 1]; ++q) {
+                const int f = P.lmFac[q], a = w.lmFrame[l], b = w.obsFrame[f];
+                const int ca = lay.poseCol[a], cb = a == b ? -1 : lay.poseCol[b];
+                double r[2];
+                VisualJ J;
+                if (!visualEvaluate(w, f, &x.pose[7 * a], &x.pose[7 * b], x.rig, x.td, x.rho[l], r, &J, ca >= 0,
+                                    cb >= 0, lay.rigCol >= 0, lay.tdCol >= 0, rhoCol >= 0)) { tp.valid = false; continue; }
+                const double sq = r[0] * r[0] + r[1] * r[1];
+                if (!std::isfinite(sq)) { tp.valid = false; continue; }
+                tp.cost += robustCost(w.huber, sq);
+                const double wgt = huberWeight(w.huber, sq);
+                int n = 0;
+                auto add = [&](int col, const double* amb, int size, const double* lift, int width) {
+                    for (int c = 0; c < width; ++c) {
+                        double s0 = 0, s1 = 0;
+                        for (int k = 0; k < size; ++k) {
+                            s0 += amb[k] * lift[k * width + c];
+                            s1 += amb[size + k] * lift[k * width + c];
+                        }
+                        cols[n] = col + c; v0[n] = wgt * s0; v1[n] = wgt * s1; ++n;
+                    }
+                };
+                if (ca >= 0) { plusJacobian(&x.pose[7 * a], plusA); add(ca, J.anchor, 7, plusA, 6); }
+                if (cb >= 0) { plusJacobian(&x.pose[7 * b], plusB); add(cb, J.observer, 7, plusB, 6); }
+                if (lay.rigCol >= 0) { plusJacobian(x.rig, plusR); add(lay.rigCol, J.rig, 7, plusR, 6); }
+                const double one = 1;
+                if (lay.tdCol >= 0) add(lay.tdCol, J.td, 1, &one, 1);
+                const double rr0 = wgt * r[0], rr1 = wgt * r[1];
+                for (int i = 0; i < n; ++i) {
+                    tp.g[cols[i]] += v0[i] * rr0 + v1[i] * rr1;
+                    double* Hi = &tp.H[(size_t)cols[i] * nr];
+                    for (int j = 0; j < n; ++j) Hi[cols[j]] += v0[i] * v0[j] + v1[i] * v1[j];
+                }
+                if (rhoCol < 0) continue;
+                const double jr0 = wgt * J.rho[0], jr1 = wgt * J.rho[1];
+                lt.d += jr0 * jr0 + jr1 * jr1;
+                lt.g += jr0 * rr0 + jr1 * rr1;
+                for (int i = 0; i < n; ++i) {
+                    int& li = local[cols[i]];
+                    if (li < 0) { li = (int)tp.col.size(); tp.col.push_back(cols[i]); tp.val.push_back(0); }
+                    tp.val[li] += jr0 * v0[i] + jr1 * v1[i];
+                }
+            }
+            lt.count = (int)tp.col.size() - lt.begin;
+            for (int i = lt.begin; i < lt.begin + lt.count; ++i) local[tp.col[i]] = -1;
+            if (rhoCol >= 0) {  // S -= c c^T / damped(d), g_r -= c g / damped(d) on the support
+                const double inv = 1 / damped(lt.d, lambda);
+                const int* c = &tp.col[lt.begin];
+                const double* v = &tp.val[lt.begin];
+                for (int i = 0; i < lt.count; ++i) {
+                    double* Si = &tp.Sc[(size_t)c[i] * nr];
+                    const double vi = inv * v[i];
+                    for (int j = 0; j < lt.count; ++j) Si[c[j]] -= vi * v[j];
+                    tp.gr[c[i]] -= vi * lt.g;
+                }
+            }
+            tp.lm.push_back(lt);
+        }
+    });
+    std::vector<double> H((size_t)nr * nr, 0), g(nr, 0), Sc((size_t)nr * nr, 0), gr(nr, 0);
+    double cost = 0;
+    bool valid = true;
+    for (const ThreadPart& tp : parts) {
+        for (size_t i = 0; i < H.size(); ++i) { H[i] += tp.H[i]; Sc[i] += tp.Sc[i]; }
+        for (int i = 0; i < nr; ++i) { g[i] += tp.g[i]; gr[i] += tp.gr[i]; }
+        cost += tp.cost;
+        valid = valid && tp.valid;
+    }
+    if (!retainedInto(w, lay, x, H, g, cost)) valid = false;
+    if (!valid || !std::isfinite(cost)) return VIB_ERR_INVALID_RESIDUAL;
+    std::vector<double> S(H), rhs(nr);
+    for (size_t i = 0; i < S.size(); ++i) S[i] += Sc[i];
+    for (int i = 0; i < nr; ++i) {
+        S[(size_t)i * nr + i] += lambda * std::fmax(H[(size_t)i * nr + i], 1e-6);
+        rhs[i] = -(g[i] + gr[i]);
+    }
+    if (!choleskyUpper(S, nr)) return VIB_ERR_NOT_POSITIVE_DEFINITE;
+    std::vector<double> ds(rhs);
+    choleskyUpperSolve(S, nr, ds);
+    for (int i = 0; i < nr; ++i) {
+        const double v = i == lay.tdCol ? x.td : 0.0;
+        if (std::isfinite(ds[i])) ds[i] = std::fmax(P.abs.lower[i] - v, std::fmin(P.abs.upper[i] - v, ds[i]));
+    }
+    double pred = 0;
+    for (int i = 0; i < nr; ++i) {
+        double hd = 0;
+        for (int j = 0; j < nr; ++j) hd += H[(size_t)i * nr + j] * ds[j];
+        pred += -ds[i] * (g[i] + 0.5 * hd);
+    }
+    delta.assign(nr + lay.ne, 0);
+    std::copy(ds.begin(), ds.end(), delta.begin());
+    // back-substitution of the inverse depths, then the candidate
+    parallelRanges(w.L, threads, [&](int t, int l0, int l1) {
+        ThreadPart& tp = parts[t];
+        tp.pred = 0;
+        for (int l = l0, q = 0; l < l1; ++l, ++q) {
+            const int i = lay.lmCol[l];
+            if (i < 0) continue;
+            const LmTerms& lt = tp.lm[q];
+            double cd = 0;
+            for (int k = lt.begin; k < lt.begin + lt.count; ++k) cd += tp.val[k] * ds[tp.col[k]];
+            const double dl = -(lt.g + cd) / damped(lt.d, lambda);
+            const double lo = P.abs.lower[i] - x.rho[l], hi = P.abs.upper[i] - x.rho[l];
+            delta[i] = std::isfinite(dl) ? std::fmax(lo, std::fmin(hi, dl)) : dl;
+            tp.pred += -delta[i] * (lt.g + cd + 0.5 * lt.d * delta[i]);
+        }
+    });
+    for (const ThreadPart& tp : parts) pred += tp.pred;
+    State y = x;
+    bool invalid = false;
+    double maxInc = 0;
+    auto track = [&](const double* d, int width) {
+        for (int k = 0; k < width; ++k) {
+            if (!std::isfinite(d[k])) invalid = true;
+            maxInc = std::fmax(maxInc, std::fabs(d[k]));
+        }
+    };
+    for (int k = 0; k < w.K; ++k) {
+        if (lay.poseCol[k] >= 0) { track(&delta[lay.poseCol[k]], 6); posePlus(&x.pose[7 * k], &delta[lay.poseCol[k]], &y.pose[7 * k], false); }
+        if (lay.sbCol[k] >= 0) {
+            track(&delta[lay.sbCol[k]], w.sbWidth[k]);
+            for (int j = 0; j < w.sbWidth[k]; ++j) y.sb[9 * k + j] = x.sb[9 * k + j] + delta[lay.sbCol[k] + j];
+        }
+    }
+    if (lay.rigCol >= 0) { track(&delta[lay.rigCol], 6); posePlus(x.rig, &delta[lay.rigCol], y.rig, false); }
+    if (lay.tdCol >= 0) { track(&delta[lay.tdCol], 1); y.td = std::fmax(w.tdLower, std::fmin(w.tdUpper, x.td + delta[lay.tdCol])); }
+    for (int l = 0; l < w.L; ++l)
+        if (lay.lmCol[l] >= 0) { track(&delta[lay.lmCol[l]], 1); y.rho[l] = std::fmax(w.minRho, x.rho[l] + delta[lay.lmCol[l]]); }
+    // trial cost (residual-only pass)
+    double tc = 0;
+    bool costOk = !invalid;
+    if (costOk) {
+        parallelRanges(w.L, threads, [&](int t, int l0, int l1) {
+            ThreadPart& tp = parts[t];
+            tp.cost = 0;
+            tp.invalid = false;
+            for (int l = l0; l < l1 && !tp.invalid; ++l)
+                for (int q = P.lmOff[l]; q < P.lmOff[l + 1]; ++q) {
+                    const int f = P.lmFac[q], a = w.lmFrame[l], b = w.obsFrame[f];
+                    double r[2];
+                    if (!visualEvaluate(w, f, &y.pose[7 * a], &y.pose[7 * b], y.rig, y.td, y.rho[l], r, nullptr, false,
+                                        false, false, false, false)) { tp.invalid = true; break; }
+                    const double sq = r[0] * r[0] + r[1] * r[1];
+                    if (!std::isfinite(sq)) { tp.invalid = true; break; }
+                    tp.cost += robustCost(w.huber, sq);
+                }
+        });
+        for (const ThreadPart& tp : parts) { tc += tp.cost; costOk = costOk && !tp.invalid; }
+        costOk = costOk && retainedCost(w, lay, y, tc);
+    }
+    metrics[0] = cost; metrics[1] = pred; metrics[2] = tc; metrics[3] = maxInc;
+    metrics[4] = costOk && std::isfinite(tc) ? 1 : 0;
+    return VIB_OK;
 }
 
 void writeState(const State& s, const Window& w, vib_window* out) {
@@ -1896,6 +2182,58 @@ int oracle_solve_step(const vib_window* win, double lambda, double* delta, doubl
     if (rhs) std::copy(st.rhs.begin(), st.rhs.end(), rhs);
     metrics[0] = L.cost; metrics[1] = st.predicted; metrics[2] = st.trialCost;
     metrics[3] = st.maxIncrement; metrics[4] = st.trialValid ? 1 : 0;
+    return VIB_OK;
+}
+
+// The same step as oracle_solve_step, block-sparse on `threads` host threads; delta (nr + ne),
+// metrics [cost, predicted, trial cost, max increment, trial valid].
+int oracle_solve_step_threaded(const vib_window* win, double lambda, int threads, double* delta, double* metrics) {
+    const Prepared P = prepare(*win);
+    std::vector<double> d;
+    const int st = blockStep(P, stateOf(P.w), lambda, threads, d, metrics);
+    if (st == VIB_OK) std::copy(d.begin(), d.end(), delta);
+    return st;
+}
+
+// Timed CPU baseline: `steps` block-sparse LM steps of one window on `threads` host threads
+// (topology prepared once, as a resident window), wall milliseconds of each step in ms[].
+int oracle_time_steps(const vib_window* win, double lambda, int threads, int steps, double* ms) {
+    const Prepared P = prepare(*win);
+    const State x = stateOf(P.w);
+    std::vector<double> d;
+    double m[5];
+    for (int s = 0; s < steps; ++s) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const int st = blockStep(P, x, lambda, threads, d, m);
+        ms[s] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (st != VIB_OK) return st;
+    }
+    return VIB_OK;
+}
+
+// Timed CPU baseline over many windows: each of `threads` host threads takes whole windows
+// (window i goes to thread i % threads) and runs one single-threaded block-sparse LM step of
+// each, `rounds` times; returns the wall seconds of all rounds.
+int oracle_time_windows(const vib_window* wins, int count, double lambda, int threads, int rounds, double* seconds) {
+    std::vector<Prepared> P;
+    P.reserve(count);
+    for (int i = 0; i < count; ++i) P.push_back(prepare(wins[i]));
+    std::vector<int> status(threads, VIB_OK);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] {
+            std::vector<double> d;
+            double m[5];
+            for (int r = 0; r < rounds; ++r)
+                for (int i = t; i < count; i += threads) {
+                    const int st = blockStep(P[i], stateOf(P[i].w), lambda, 1, d, m);
+                    if (st != VIB_OK) status[t] = st;
+                }
+        });
+    for (auto& th : pool) th.join();
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int st : status) if (st != VIB_OK) return st;
     return VIB_OK;
 }
 

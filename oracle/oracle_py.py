@@ -37,6 +37,9 @@ def load(abi):
     lib.oracle_linearize.argtypes = [W, dp, dp, dp, dp, dp, ip]
     lib.oracle_solve_step.argtypes = [W, C.c_double, dp, dp, dp, dp]
     lib.oracle_cost.argtypes = [W, dp]
+    lib.oracle_solve_step_threaded.argtypes = [W, C.c_double, C.c_int, dp, dp]
+    lib.oracle_time_steps.argtypes = [W, C.c_double, C.c_int, C.c_int, dp]
+    lib.oracle_time_windows.argtypes = [W, C.c_int, C.c_double, C.c_int, C.c_int, dp]
     lib.oracle_solve.argtypes = [W, C.POINTER(abi.VibSolveOptions), C.POINTER(abi.VibSolveReport)]
     lib.oracle_solve_latest_two.argtypes = [W, C.POINTER(abi.VibSolveOptions), C.c_int, dp, ip,
                                             C.POINTER(abi.VibSolveReport)]
@@ -115,6 +118,31 @@ class Oracle:
         st = self.lib.oracle_solve_step(C.byref(s), lam, _p(delta), _p(S), _p(rhs), _p(m))
         return st, dict(delta=delta, S=S, rhs=rhs, cost=m[0], predicted=m[1], trial_cost=m[2], max_increment=m[3],
                         trial_valid=bool(m[4]))
+
+    def solve_step_threaded(self, win, lam, threads):
+        """Block-sparse step on `threads` host threads: (status, delta, metrics dict)."""
+        lay = win.layout()
+        delta = np.zeros(lay["nr"] + lay["ne"])
+        m = np.zeros(5)
+        s = win.cstruct()
+        st = self.lib.oracle_solve_step_threaded(C.byref(s), lam, threads, _p(delta), _p(m))
+        return st, delta, dict(cost=m[0], predicted=m[1], trial_cost=m[2], max_increment=m[3], trial_valid=bool(m[4]))
+
+    def time_steps(self, win, lam, threads, steps):
+        """Wall milliseconds of `steps` block-sparse steps of one window on `threads` threads."""
+        ms = np.zeros(steps)
+        s = win.cstruct()
+        st = self.lib.oracle_time_steps(C.byref(s), lam, threads, steps, _p(ms))
+        return st, ms
+
+    def time_windows(self, wins, lam, threads, rounds=1):
+        """Wall seconds of `rounds` single-threaded steps of every window, windows spread over
+        `threads` host threads."""
+        structs = [w.cstruct() for w in wins]
+        arr = (self.abi.VibWindow * len(structs))(*structs)
+        sec = C.c_double()
+        st = self.lib.oracle_time_windows(arr, len(structs), lam, threads, rounds, C.byref(sec))
+        return st, sec.value
 
     def cost(self, win):
         s = win.cstruct()

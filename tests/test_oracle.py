@@ -824,3 +824,24 @@ def test_second_new_retires_moving_tracks_into_the_schur_prior(oracle, abi):
     assert st == 0
     _check_prior_is_schur(sysm["H"], sysm["b"], sysm["mask"], nxt.prior_A, nxt.prior_r0)
     assert nxt.L == w.L - len(anchored) and nxt.E == w.E - 1
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_block_step_equals_dense_step(oracle, threads):
+    """The timed CPU baseline (block-sparse per-landmark Schur on host threads) takes the dense
+    oracle step: same state step within 1e-10 relative, same costs and model decrease within
+    1e-11 (sums in another order)."""
+    from vibpy import synth
+    w = synth.generate(K=8, L=600, n_cams=4, obs_range=(2, 8), seed=111, rig_free=True, td_free=True,
+                       outlier_fraction=0.05)
+    for lam in (1e-4, 1e-1):
+        st, ref = oracle.solve_step(w, lam)
+        assert st == 0
+        st, delta, m = oracle.solve_step_threaded(w, lam, threads)
+        assert st == 0
+        assert np.linalg.norm(delta - ref["delta"]) <= 1e-10 * np.linalg.norm(ref["delta"])
+        assert abs(m["cost"] - ref["cost"]) <= 1e-12 * ref["cost"]
+        assert abs(m["trial_cost"] - ref["trial_cost"]) <= 1e-11 * ref["trial_cost"]
+        assert abs(m["predicted"] - ref["predicted"]) <= 1e-9 * abs(ref["predicted"])
+        assert abs(m["max_increment"] - ref["max_increment"]) <= 1e-10 * ref["max_increment"]
+        assert m["trial_valid"] == ref["trial_valid"]
