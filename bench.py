@@ -25,6 +25,9 @@ sys.path.insert(0, os.path.join(ROOT, "cgmli-slam_amd"))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# refinement passes of every mixed-precision line: the count at which the accuracy line meets
+# the metric's bound (dx relative error <= 1e-6 against plain FP64)
+MIXED_REFINEMENT = 2
 
 
 def algorithmic_bytes(win, precision="fp64"):
@@ -92,8 +95,8 @@ def batched_line(ctx, abi, wins, steps, warmup):
     for s, w in zip(slots, wins):
         ctx.upload(s, w)
     res = {}
-    for name, prec in (("fp64", abi.VIB_FP64), ("mixed", abi.VIB_MIXED)):
-        opts = abi.options(max_iterations=1, precision=prec, flags=abi.VIB_SOLVE_CONTINUOUS)
+    for name, prec, refine in (("fp64", abi.VIB_FP64, 0), ("mixed", abi.VIB_MIXED, MIXED_REFINEMENT)):
+        opts = abi.options(max_iterations=1, precision=prec, refinement_steps=refine, flags=abi.VIB_SOLVE_CONTINUOUS)
         ctx.bench(slots, opts, warmup, kernels=False)
         ctx.synchronize()
         it_ms, _ = ctx.bench(slots, opts, steps, kernels=False)
@@ -118,16 +121,17 @@ def batched_line(ctx, abi, wins, steps, warmup):
 
 
 def step_error(ctx, abi, win, slot, lam=1e-4, refinement=2):
-    """One LM step of `win` from its initial state at identical lambda in FP64 and in mixed
-    precision: ||dx_mixed - dx_fp64|| / ||dx_fp64|| over the full tangent step (states and
-    inverse ranges), and the relative differences of the linearization-point and trial costs."""
+    """One LM step of `win` from its initial state at identical lambda in plain FP64 and in mixed
+    precision with `refinement` passes: ||dx_mixed - dx_fp64|| / ||dx_fp64|| over the full
+    tangent step (states and inverse ranges), and the relative differences of the
+    linearization-point and trial costs."""
     lay = win.layout()
     n = lay["nr"] + lay["ne"]
     out = {}
-    for name, prec in (("fp64", abi.VIB_FP64), ("mixed", abi.VIB_MIXED)):
+    for name, prec, refine in (("fp64", abi.VIB_FP64, 0), ("mixed", abi.VIB_MIXED, refinement)):
         ctx.upload(slot, win)
         ctx.check(ctx.linearize(slot, prec), "vib_linearize")
-        st, rep = ctx.solve_step(slot, lam, prec, refinement)
+        st, rep = ctx.solve_step(slot, lam, prec, refine)
         ctx.check(st, "vib_solve_step")
         out[name] = (ctx.read_step(slot, n), rep)
     ctx.release(slot)
@@ -421,7 +425,8 @@ def main():
     gate = gate_line(ctx, abi, win)
     # the same window in mixed precision, then the mixed-vs-FP64 step error on it and on the
     # ill-conditioned window (depths 0.5-200 m, 30% low-parallax tracks, a near-static stretch)
-    mopts = abi.options(max_iterations=1, precision=abi.VIB_MIXED, flags=abi.VIB_SOLVE_CONTINUOUS)
+    mopts = abi.options(max_iterations=1, precision=abi.VIB_MIXED, refinement_steps=MIXED_REFINEMENT,
+                        flags=abi.VIB_SOLVE_CONTINUOUS)
     ctx.bench([0], mopts, args.warmup, kernels=False)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -432,7 +437,9 @@ def main():
     ctx.synchronize()
     mixed = dict(value=args.steps / mwall, unit="window-steps/s", ms_per_step=1e3 * mwall / args.steps,
                  p50_ms=float(np.percentile(mit_ms, 50)), p99_ms=float(np.percentile(mit_ms, 99)), kernel_ms=mk_ms)
-    mixed["accuracy"] = {args.workload: step_error(ctx, abi, win, 1), "c3": step_error(ctx, abi, synth.generate(**synth.CONFIGS["c3"]), 1)}
+    mixed["accuracy"] = {args.workload: step_error(ctx, abi, win, 1, refinement=MIXED_REFINEMENT),
+                         "c3": step_error(ctx, abi, synth.generate(**synth.CONFIGS["c3"]), 1, refinement=MIXED_REFINEMENT)}
+    mixed["refinement_steps"] = MIXED_REFINEMENT
     batched = None
     if bwins:
         print(f"[bench] batched workload: {len(bwins)} windows", file=sys.stderr, flush=True)

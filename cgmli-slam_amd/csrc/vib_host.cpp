@@ -559,19 +559,39 @@ bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
 // many small items would otherwise run at the occupancy of the large visual and Schur kernels.
 // A linearization alone always sums the partials.
 bool Batch::fuse_reduce() const { return nwin <= kFuseReduceWindows; }
-bool Batch::in_iteration(int k) const { return !((k == 1 || k == 2) && fuse_reduce()); }
 bool Batch::linearize(hipStream_t s) {
     return launch(0, s, 0, 0) && (fuse_reduce() || launch(1, s, 0, 0)) && launch(2, s, 0, 0);
 }
 
+// A mixed-precision step with refinement runs `refinement` passes of the exact-operator sum
+// (kernel 12) and the correction solve (kernel 13, mode 1 on the last pass, which also finishes
+// the step) between the factorization and the back-substitution.
+std::vector<std::pair<int, int>> Batch::sequence(bool iteration, int refinement, int mode) const {
+    std::vector<std::pair<int, int>> q;
+    if (iteration) {
+        q.push_back({0, 0});
+        if (!fuse_reduce()) { q.push_back({1, 0}); q.push_back({2, 0}); }
+    }
+    for (int k = 3; k <= 6; ++k) q.push_back({k, mode});
+    if (mixed && refinement > 0)
+        for (int i = 0; i < refinement; ++i) {
+            q.push_back({12, 0});
+            q.push_back({13, i + 1 == refinement ? 1 : 0});
+        }
+    for (int k = 7; k < 12; ++k) q.push_back({k, mode});
+    return q;
+}
+
 bool Batch::step(hipStream_t s, int refinement, int mode) {
-    for (int k = 3; k < kKernelCount; ++k)
-        if (!launch(k, s, refinement, mode)) return false;
+    for (const auto& km : sequence(false, refinement, mode))
+        if (!launch(km.first, s, refinement, km.second)) return false;
     return true;
 }
 
 bool Batch::iteration(hipStream_t s, int refinement) {
-    return launch(0, s, 0, 0) && (fuse_reduce() || (launch(1, s, 0, 0) && launch(2, s, 0, 0))) && step(s, refinement, 0);
+    for (const auto& km : sequence(true, refinement, 0))
+        if (!launch(km.first, s, refinement, km.second)) return false;
+    return true;
 }
 
 bool Batch::capture(hipStream_t s, int refinement) {
@@ -2112,26 +2132,28 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
     for (auto& e : ev) (void)hipEventDestroy(e);
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, std::string("timed iterations failed: ") + hipGetErrorString(hipGetLastError()));
     if (kernel_ms) {
-        std::vector<hipEvent_t> kev(2 * kKernelCount * iterations);
+        // the iteration's launches one by one, each between two events; a kernel launched more
+        // than once per iteration (the refinement passes) reports the sum of its launches
+        const auto seq = batch.sequence(true, o->refinement_steps, 0);
+        const size_t per = seq.size();
+        std::vector<hipEvent_t> kev(2 * per * iterations);
         for (auto& e : kev)
             if (hipEventCreate(&e) != hipSuccess) return fail(ctx, VIB_ERR_DEVICE, "event creation failed");
         for (int it = 0; ok && it < iterations; ++it)
-            for (int k = 0; ok && k < kKernelCount; ++k) {
-                hipEvent_t* e = &kev[2 * (it * kKernelCount + k)];
+            for (size_t q = 0; ok && q < per; ++q) {
+                hipEvent_t* e = &kev[2 * (it * per + q)];
                 ok = hipEventRecord(e[0], ctx->stream) == hipSuccess &&
-                     (!batch.in_iteration(k) || batch.launch(k, ctx->stream, o->refinement_steps, 0)) &&
+                     batch.launch(seq[q].first, ctx->stream, o->refinement_steps, seq[q].second) &&
                      hipEventRecord(e[1], ctx->stream) == hipSuccess;
             }
         ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
         for (int k = 0; k < kKernelCount; ++k) kernel_ms[k] = 0;
-        std::vector<char> skipped(kKernelCount, 0);
-        for (int k = 0; k < kKernelCount; ++k) skipped[k] = !batch.in_iteration(k);
         for (int it = 0; ok && it < iterations; ++it)
-            for (int k = 0; ok && k < kKernelCount; ++k) {
+            for (size_t q = 0; ok && q < per; ++q) {
                 float ms = 0;
-                const hipEvent_t* e = &kev[2 * (it * kKernelCount + k)];
+                const hipEvent_t* e = &kev[2 * (it * per + q)];
                 ok = hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess;
-                if (!skipped[k]) kernel_ms[k] += ms / iterations;
+                kernel_ms[seq[q].first] += ms / iterations;
             }
         for (auto& e : kev) (void)hipEventDestroy(e);
         if (!ok) return fail(ctx, VIB_ERR_DEVICE, "kernel profiling failed");

@@ -17,7 +17,8 @@ constexpr int kMaxFactorsPerLandmark = 13;
 // Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major, small n); eigenvalues
 // ascending, eigenvectors as columns of V (vib_preint.cpp).
 void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V);
-constexpr int kKernelCount = 12;  // kernels of one LM iteration (vibk_launch ids 0..11)
+constexpr int kKernelCount = 14;  // kernels of one LM iteration (vibk_launch ids 0..13; 12 and 13 are the
+                                  // refinement passes of a mixed-precision step)
 // Batches with at least this many landmark chunks (about 16 per CU) merge kChunkMerge of them
 // per workgroup (measured on the 256-window batched workload).
 constexpr size_t kMergeChunksAbove = 4096;
@@ -105,7 +106,8 @@ struct Batch {
     bool step(hipStream_t s, int refinement, int mode);
     bool iteration(hipStream_t s, int refinement);
     bool fuse_reduce() const;        // kernel 3 also sums the visual-block partials
-    bool in_iteration(int k) const;  // kernel k is launched by an iteration
+    // launches of one LM iteration (iteration = true) or of a step alone, as (kernel, mode)
+    std::vector<std::pair<int, int>> sequence(bool iteration, int refinement, int mode) const;
     bool capture(hipStream_t s, int refinement);
 };
 

@@ -379,14 +379,32 @@ __device__ __forceinline__ void scatter_gram(double* acc, double* gacc, const Wi
 // Per-factor coupling terms, stored at the factor's pair-order position p (k_lin_visual writes
 // them contiguously; the landmark-ordered kernels read them through pf_pos). FP64: fac_u[k F + p]
 // (6 planes), fac_lm[q F + p] (FacLm planes).
-// Mixed precision: the same arrays hold FP32 planes (u: 6; the Hessian-side landmark terms d,
-// c_rig (6) and c_td as planes 0..7) with the landmark gradient g_rho kept in FP64 behind them
-// (fac_lm + 4 F doubles), so both forms fit the FP64 allocation.
+// Mixed precision: the same arrays hold each term as an FP32 pair (hi, lo) with
+// hi = fl32(v), lo = fl32(v - hi), so hi + lo carries v to about 2^-48: u in float planes
+// k (hi) and 6 + k (lo); the Hessian-side landmark terms d, c_rig (6) and c_td in float planes
+// j (hi) and 8 + j (lo), j = fl32_plane(q); the landmark gradient g_rho in FP64 behind them
+// (fac_lm + 8 F doubles). Both forms fit the FP64 allocation. The Schur product reads the hi
+// planes only (FP32 rows, FP32 MFMA); the refinement operator and the back-substitution read
+// hi + lo.
 __device__ __forceinline__ int fl32_plane(int q) { return q == FL_D ? 0 : q - 1; }
+constexpr int kFl32Planes = FL_PLANES - 1;  // FP32 Hessian-side planes of fac_lm
 template <bool MIXED>
 __device__ __forceinline__ double load_u(const WinDev& w, size_t F, int k, int p) {
     if constexpr (MIXED) return (double)((const float*)w.fac_u)[k * F + p];
     return w.fac_u[k * F + p];
+}
+template <bool MIXED>
+__device__ __forceinline__ double load_u_exact(const WinDev& w, size_t F, int k, int p) {
+    if constexpr (MIXED) {
+        const float* u32 = (const float*)w.fac_u;
+        return (double)u32[k * F + p] + (double)u32[(6 + k) * F + p];
+    }
+    return w.fac_u[k * F + p];
+}
+__device__ __forceinline__ void store_pair(float* planes, size_t F, int hi_plane, int lo_plane, size_t p, double v) {
+    const float hi = (float)v;
+    planes[hi_plane * F + p] = hi;
+    planes[lo_plane * F + p] = (float)(v - (double)hi);
 }
 
 // IMU edges and prior of a window (k_retained below). Their linearization runs as extra
@@ -495,10 +513,10 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
             if constexpr (MIXED) {
                 float* u32 = (float*)w.fac_u;
                 float* l32 = (float*)w.fac_lm;
-                for (int k = 0; k < 6; ++k) u32[k * F + p] = (float)u[k];
+                for (int k = 0; k < 6; ++k) store_pair(u32, F, k, 6 + k, p, u[k]);
                 for (int q = 0; q < FL_PLANES; ++q)
-                    if (q != FL_G) l32[fl32_plane(q) * F + p] = (float)fl[q];
-                w.fac_lm[4 * F + p] = fl[FL_G];
+                    if (q != FL_G) store_pair(l32, F, fl32_plane(q), kFl32Planes + fl32_plane(q), p, fl[q]);
+                w.fac_lm[kFl32Planes * F + p] = fl[FL_G];
             } else {
                 for (int k = 0; k < 6; ++k) w.fac_u[k * F + p] = u[k];
                 for (int q = 0; q < FL_PLANES; ++q) w.fac_lm[q * F + p] = fl[q];
@@ -590,7 +608,9 @@ __global__ __launch_bounds__(kBlock) void k_lm_sums(const WinDev* wins, const Ch
             if constexpr (MIXED) {
                 const float* l32 = (const float*)w.fac_lm;
                 for (int q = 0; q < FL_PLANES; ++q)
-                    sum[q] += q == FL_G ? w.fac_lm[4 * F + p] : (double)l32[fl32_plane(q) * F + p];
+                    sum[q] += q == FL_G ? w.fac_lm[kFl32Planes * F + p]
+                                        : (double)l32[fl32_plane(q) * F + p] +
+                                              (double)l32[(kFl32Planes + fl32_plane(q)) * F + p];
             } else {
                 for (int q = 0; q < FL_PLANES; ++q) sum[q] += w.fac_lm[q * F + p];
             }
@@ -1850,68 +1870,17 @@ __device__ double block_max_waves(double v, double* red) {
     return s;
 }
 
-__global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int refinement) {
-    __shared__ double pnl[2 * kMaxTiles * kTileDoubles];
-    __shared__ double dtile[kMaxTiles * kTileDoubles];
-    __shared__ double dcur[2 * kTileDoubles];
-    __shared__ double x[kMaxTiles * kTile];
-    __shared__ double rres[kMaxTiles * kTile];
-    __shared__ double part[kBackPart];
-    __shared__ double red[kCholThreads / 64];
-    __shared__ int flag;
-    WinDev& w = wins[blockIdx.x];
-    if (!window_active(w)) return;
+// The end of a step once the state step x (tangent order, LDS, N entries) is final: clamp to
+// the relative bounds (only td is bounded among the states), the state model decrease (here only
+// for a window without landmarks; k_backsub spreads it over its chunk blocks otherwise), the
+// step norm and the state trial.
+__device__ void step_tail(WinDev& w, double* x, double* red) {
     const int n = w.nr, t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
-    const int T = (n + kTile - 1) / kTile, N = kTile * T;
-    // factorization order: speed/bias columns first (their IMU chain leaves most of their tiles
-    // zero, which the factorization skips), then the visual columns
-    const int nv = w.nv, nsb = n - nv;
-    auto to_solve = [&](int c) { return c < nv ? c + nsb : c - nv; };
-    auto to_tangent = [&](int p) { return p < nsb ? p + nv : p - nsb; };
-    if (t == 0) flag = 0;
-    for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[to_tangent(i)] : 0.0;
-    __syncthreads();
-    const bool ok = cholesky_forward(w.fac, T, pnl, dtile, dcur, x, &flag, w.chol_mask);
-    if (!ok) {
-        if (t == 0) w.lm->chol_ok = 0;
-        return;
-    }
-    // inverse diagonal tiles (diagonal slots of the tile store) -> LDS for the substitutions
-    double* dinv = pnl;
-    for (int p = t; p < T * kTileDoubles; p += nt) {
-        const int a = p / kTileDoubles;
-        dinv[p] = w.fac[tile_base(a, a) + p % kTileDoubles];
-    }
-    __syncthreads();
-    cholesky_backward(w.fac, T, dinv, x, part);
-    for (int it = 0; it < refinement; ++it) {
-        for (int i = wave; i < N; i += nw) {  // residual row i of the factorization order
-            double v = 0;
-            if (i < n) {
-                const double* row = w.Scopy + (size_t)to_tangent(i) * n;
-                for (int j = lane; j < n; j += 64) v += row[j] * x[to_solve(j)];
-            }
-            v = wave_reduce_sum(v);
-            if (lane == 0) rres[i] = i < n ? w.rhs[to_tangent(i)] - v : 0.0;
-        }
-        __syncthreads();
-        cholesky_solve_again(w.fac, T, dinv, rres, part);
-        for (int i = t; i < n; i += nt) x[i] += rres[i];
-        __syncthreads();
-    }
-    // the step back in the tangent layout
-    for (int c = t; c < N; c += nt) rres[c] = c < n ? x[to_solve(c)] : 0.0;
-    __syncthreads();
-    {
-    double* x = rres;
-    // clamp to relative bounds: only td is bounded among the states
     if (t == 0 && w.td_col >= 0 && isfinite(x[w.td_col])) {
         const double tdv = *w.td;
         x[w.td_col] = fmax(w.td_lo - tdv, fmin(w.td_hi - tdv, x[w.td_col]));
     }
     __syncthreads();
-    // state model decrease -dx^T (g + 0.5 H dx) with the undamped H: k_backsub spreads its rows
-    // over the chunk blocks; only a window without landmarks (no chunks) sums it here
     double pred = 0;
     if (w.nchunks == 0)
         for (int i = wave; i < n; i += nw) {
@@ -1949,7 +1918,200 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
         w.lm->max_increment = maxinc;
         w.lm->trial_valid = nonfinite > 0 ? 0 : 1;
     }
+}
+
+// refinement > 0 and defer == 0: iterative refinement against the saved damped system (Scopy),
+// which is the exact operator in FP64. defer != 0 (mixed precision, whose Scopy holds FP32
+// Schur products): the unrefined step goes to dstate and k_schur_apply / k_refine refine it
+// against the exact operator and finish the step.
+__global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int refinement, int defer) {
+    __shared__ double pnl[2 * kMaxTiles * kTileDoubles];
+    __shared__ double dtile[kMaxTiles * kTileDoubles];
+    __shared__ double dcur[2 * kTileDoubles];
+    __shared__ double x[kMaxTiles * kTile];
+    __shared__ double rres[kMaxTiles * kTile];
+    __shared__ double part[kBackPart];
+    __shared__ double red[kCholThreads / 64];
+    __shared__ int flag;
+    WinDev& w = wins[blockIdx.x];
+    if (!window_active(w)) return;
+    const int n = w.nr, t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
+    const int T = (n + kTile - 1) / kTile, N = kTile * T;
+    // factorization order: speed/bias columns first (their IMU chain leaves most of their tiles
+    // zero, which the factorization skips), then the visual columns
+    const int nv = w.nv, nsb = n - nv;
+    auto to_solve = [&](int c) { return c < nv ? c + nsb : c - nv; };
+    auto to_tangent = [&](int p) { return p < nsb ? p + nv : p - nsb; };
+    if (t == 0) flag = 0;
+    for (int i = t; i < N; i += nt) x[i] = i < n ? w.rhs[to_tangent(i)] : 0.0;
+    __syncthreads();
+    const bool ok = cholesky_forward(w.fac, T, pnl, dtile, dcur, x, &flag, w.chol_mask);
+    if (!ok) {
+        if (t == 0) w.lm->chol_ok = 0;
+        return;
     }
+    // inverse diagonal tiles (diagonal slots of the tile store) -> LDS for the substitutions
+    double* dinv = pnl;
+    for (int p = t; p < T * kTileDoubles; p += nt) {
+        const int a = p / kTileDoubles;
+        dinv[p] = w.fac[tile_base(a, a) + p % kTileDoubles];
+    }
+    __syncthreads();
+    cholesky_backward(w.fac, T, dinv, x, part);
+    if (defer) {
+        for (int c = t; c < n; c += nt) w.dstate[c] = x[to_solve(c)];
+        if (t == 0) w.lm->chol_ok = 1;
+        return;
+    }
+    for (int it = 0; it < refinement; ++it) {
+        for (int i = wave; i < N; i += nw) {  // residual row i of the factorization order
+            double v = 0;
+            if (i < n) {
+                const double* row = w.Scopy + (size_t)to_tangent(i) * n;
+                for (int j = lane; j < n; j += 64) v += row[j] * x[to_solve(j)];
+            }
+            v = wave_reduce_sum(v);
+            if (lane == 0) rres[i] = i < n ? w.rhs[to_tangent(i)] - v : 0.0;
+        }
+        __syncthreads();
+        cholesky_solve_again(w.fac, T, dinv, rres, part);
+        for (int i = t; i < n; i += nt) x[i] += rres[i];
+        __syncthreads();
+    }
+    // the step back in the tangent layout
+    for (int c = t; c < N; c += nt) rres[c] = c < n ? x[to_solve(c)] : 0.0;
+    __syncthreads();
+    step_tail(w, rres, red);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Refinement of the state step against the exact reduced operator (mixed precision).
+// The reduced residual at x is
+//   r = -g - (H + lambda D) x + sum_k c_k (g_k + c_k^T x) / damped(d_k)
+// with H, g the undamped FP64 state block and gradient, D = diag(max(H_ii, 1e-6)), and c_k,
+// d_k, g_k landmark k's coupling column, Hessian and gradient (VisualCudaBatch.cu:138-172).
+// k_schur_apply forms the landmark sum per chunk from the couplings' exact (hi + lo) values,
+// in FP64, as a chunk partial over the nv visual columns; k_refine sums the partials in fixed
+// order, forms r, solves with the FP64 factor of the mixed system and adds the correction. Each
+// pass contracts the step error by the factor's relative error (about 1e-4 on ill-conditioned
+// windows), so two passes reach the FP64 step.
+// ---------------------------------------------------------------------------------------------
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, const Chunk* chunks) {
+    __shared__ double crow[kSchurRows * kSchurLd];
+    __shared__ double xs[kMaxVisual + 1];
+    __shared__ double wk[kSchurRows];
+    const Chunk ch = chunks[blockIdx.x];
+    const WinDev& w = wins[ch.win];
+    if (!window_active(w) || !w.lm->chol_ok) return;
+    const int nv = w.nv, t = threadIdx.x;
+    const double lambda = w.lm->lambda;
+    const int lgroup = t % kChunkLanes, group = t / kChunkLanes;
+    constexpr int groups = kBlock / kChunkLanes;
+    const size_t L = (size_t)w.L, F = (size_t)w.F;
+    for (int j = t; j < nv; j += kBlock) xs[j] = w.dstate[j];
+    double zsum = 0;  // column t of the chunk's landmark sum
+    for (int base = ch.lm_begin; base < ch.lm_end; base += kSchurRows) {
+        for (int i = t; i < kSchurRows * kSchurLd; i += kBlock) crow[i] = 0.0;
+        __syncthreads();
+        for (int r0 = 0; r0 < kSchurRows; r0 += groups) {
+            const int r = r0 + group, l = base + r;
+            if (r >= kSchurRows || l >= ch.lm_end || w.lm_col[l] < 0) continue;
+            double* row = crow + r * kSchurLd;
+            const int a = w.lm_fc[l] >> 8;
+            const int ac = w.pose_col[a];
+            const int f0 = w.lm_off[l], f1 = w.lm_off[l + 1];
+            for (int f = f0 + lgroup; f < f1; f += kChunkLanes) {
+                const int b = w.ob_fc[f] >> 8;
+                const int bc = b == a ? -1 : w.pose_col[b];
+                for (int k = 0; k < 6; ++k) {
+                    const double u = load_u_exact<MIXED>(w, F, k, w.pf_pos[f]);
+                    if (ac >= 0) atomicAdd(&row[ac + k], u);
+                    if (bc >= 0) atomicAdd(&row[bc + k], -u);
+                }
+            }
+            if (lgroup == 0) {
+                if (w.rig_col >= 0)
+                    for (int k = 0; k < 6; ++k) row[w.rig_col + k] = w.lm_out[(LO_CR0 + k) * L + l];
+                if (w.td_col >= 0) row[w.td_col] = w.lm_out[LO_CT * L + l];
+            }
+        }
+        __syncthreads();
+        // w_k = (g_k + c_k^T x) / damped(d_k): four lanes per row, combined in lane order
+        {
+            const int r = t / 4, q = t % 4, l = base + r;
+            double v = 0;
+            for (int j = q; j < nv; j += 4) v += crow[r * kSchurLd + j] * xs[j];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            if (q == 0) {
+                double wv = 0;
+                if (l < ch.lm_end && w.lm_col[l] >= 0)
+                    wv = (w.lm_out[LO_G * L + l] + v) / damped_point(w.lm_out[LO_D * L + l], lambda);
+                wk[r] = wv;
+            }
+        }
+        __syncthreads();
+        if (t < nv)
+            for (int k = 0; k < kSchurRows; ++k) zsum += wk[k] * crow[k * kSchurLd + t];
+        __syncthreads();
+    }
+    if (t < nv) w.schur_part[(size_t)ch.local * (w.tri + nv) + t] = zsum;
+}
+
+// last != 0: the final pass, which also finishes the step (clamp, model decrease, trial).
+__global__ __launch_bounds__(kCholThreads) void k_refine(WinDev* wins, int last) {
+    __shared__ double dinv[kMaxTiles * kTileDoubles];
+    __shared__ double x[kMaxTiles * kTile];
+    __shared__ double rres[kMaxTiles * kTile];
+    __shared__ double zp[kCholThreads];
+    __shared__ double part[kBackPart];
+    __shared__ double red[kCholThreads / 64];
+    WinDev& w = wins[blockIdx.x];
+    if (!window_active(w) || !w.lm->chol_ok) return;
+    const int n = w.nr, t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
+    const int T = (n + kTile - 1) / kTile, N = kTile * T;
+    const int nv = w.nv, nsb = n - nv, len = w.tri + nv;
+    auto to_solve = [&](int c) { return c < nv ? c + nsb : c - nv; };
+    const double lambda = w.lm->lambda;
+    for (int p = t; p < T * kTileDoubles; p += nt) {
+        const int a = p / kTileDoubles;
+        dinv[p] = w.fac[tile_base(a, a) + p % kTileDoubles];
+    }
+    for (int c = t; c < N; c += nt) x[c] = c < n ? w.dstate[c] : 0.0;
+    // landmark sum z over the chunk partials: eight chunk lanes per column, in lane order
+    {
+        const int j = t / 8, q = t % 8;
+        double v = 0;
+        if (j < nv)
+            for (int c = q; c < w.nchunks; c += 8) v += w.schur_part[(size_t)c * len + j];
+        zp[t] = v;
+    }
+    __syncthreads();
+    // reduced residual, tangent order, one row per wave
+    for (int i = wave; i < n; i += nw) {
+        const double* row = w.Hfull + (size_t)i * n;
+        double v = 0;
+        for (int j = lane; j < n; j += 64) v += row[j] * x[j];
+        v = wave_reduce_sum(v);
+        if (lane == 0) {
+            double z = 0;
+            if (i < nv)
+                for (int q = 0; q < 8; ++q) z += zp[8 * i + q];
+            const double damp = lambda * fmax(row[i], 1e-6);
+            rres[to_solve(i)] = -w.gfull[i] - v - damp * x[i] + z;
+        }
+    }
+    for (int i = n + t; i < N; i += nt) rres[i] = 0.0;
+    __syncthreads();
+    cholesky_solve_again(w.fac, T, dinv, rres, part);
+    for (int c = t; c < n; c += nt) x[c] += rres[to_solve(c)];
+    __syncthreads();
+    if (!last) {
+        for (int c = t; c < n; c += nt) w.dstate[c] = x[c];
+        return;
+    }
+    step_tail(w, x, red);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2129,7 +2291,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
                 if (b == a) continue;
                 const int bc = w.pose_col[b];
                 for (int k = 0; k < 6; ++k) {
-                    const double u = load_u<MIXED>(w, F, k, w.pf_pos[f]);
+                    const double u = load_u_exact<MIXED>(w, F, k, w.pf_pos[f]);
                     cd += u * ((ac >= 0 ? dx[ac + k] : 0.0) - (bc >= 0 ? dx[bc + k] : 0.0));
                 }
             }
@@ -3098,8 +3260,8 @@ extern "C" {
 const char* vibk_name(int k) {
     static const char* names[] = {"k_lin_visual", "k_retained_lin", "k_reduce_lin", "k_schur", "k_reduce_schur",
                                   "k_assemble", "k_cholesky", "k_backsub", "k_trial_cost", "k_retained_trial",
-                                  "k_lm_update", "k_accept"};
-    return k >= 0 && k < 12 ? names[k] : "";
+                                  "k_lm_update", "k_accept", "k_schur_apply", "k_refine"};
+    return k >= 0 && k < 14 ? names[k] : "";
 }
 
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, const vib::Chunk* lin,
@@ -3138,7 +3300,10 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             hipLaunchKernelGGL(k_assemble, dim3(nwin, kAssembleSplit * ((kMaxRetained + kTile - 1) / kTile)), dim3(kBlock), 0,
                                s, wins, red);
             break;
-        case 6: hipLaunchKernelGGL(k_cholesky, dim3(nwin), dim3(kCholThreads), 0, s, wins, refinement); break;
+        case 6:
+            hipLaunchKernelGGL(k_cholesky, dim3(nwin), dim3(kCholThreads), 0, s, wins, refinement,
+                               mixed && refinement > 0 ? 1 : 0);
+            break;
         case 7:
             if (nchunks) {
                 if (mixed) hipLaunchKernelGGL(k_backsub<true>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
@@ -3154,6 +3319,13 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             hipLaunchKernelGGL(k_accept, dim3(nwin, (max_landmarks + kAcceptSlice - 1) / kAcceptSlice + 1), dim3(kBlock), 0, s,
                                wins);
             break;
+        case 12:  // exact-operator landmark sum of one refinement pass (mixed precision)
+            if (nchunks) {
+                if (mixed) hipLaunchKernelGGL(k_schur_apply<true>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+                else hipLaunchKernelGGL(k_schur_apply<false>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            }
+            break;
+        case 13: hipLaunchKernelGGL(k_refine, dim3(nwin), dim3(kCholThreads), 0, s, wins, mode); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

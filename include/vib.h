@@ -75,9 +75,13 @@ enum {
 
 /* precision of a solve */
 #define VIB_FP64 0   /* plain FP64 everywhere */
-#define VIB_MIXED 1  /* FP32 Hessian-side coupling streams and landmark Schur products (FP32
-                        MFMA, FP64 sums over 16-landmark groups); factor evaluation, gradients,
-                        costs and the reduced-system Cholesky stay FP64 */
+#define VIB_MIXED 1  /* Hessian-side coupling streams as FP32 (hi, lo) pairs; the landmark Schur
+                        products of the reduced system from the hi parts (FP32 MFMA, FP64 sums
+                        over 16-landmark groups); factor evaluation, gradients, costs, the
+                        reduced-system Cholesky and the back-substitution stay FP64. With
+                        refinement_steps > 0 the state step is refined against the exact FP64
+                        reduced operator (matrix-free landmark pass over hi + lo), which brings
+                        it to the FP64 step; with 0 it keeps the FP32 products' error. */
 
 /* marginalization policies (src/backend/SlidingWindowManager.cpp:503) */
 #define VIB_MARGIN_OLD 0
@@ -169,7 +173,8 @@ typedef struct vib_solve_options {
     double initial_lambda;      /* 1e-4 in the reference (:167)                          */
     double time_limit_seconds;  /* <= 0: unlimited                                       */
     int32_t precision;          /* VIB_FP64 / VIB_MIXED                                  */
-    int32_t refinement_steps;   /* iterative-refinement steps of the RCS solve           */
+    int32_t refinement_steps;   /* iterative-refinement steps of the RCS solve (VIB_MIXED:
+                                   against the exact operator; 2 reach the FP64 step)    */
     uint32_t flags;             /* VIB_SOLVE_*                                           */
 } vib_solve_options;
 
@@ -305,7 +310,9 @@ const char* vib_kernel_name(int32_t kernel);
  * iterations kernel by kernel with events around every launch and writes the average duration
  * of each kernel (vib_kernel_count entries) to kernel_ms. Either output may be NULL. For up to
  * 16 slots kernels 1 (the IMU edges and prior linearized) and 2 (the visual-block sums) report
- * 0: kernels 0 and 3 then run them as extra workgroups. */
+ * 0: kernels 0 and 3 then run them as extra workgroups. Kernels 12 and 13 (the refinement
+ * passes of a VIB_MIXED step) run refinement_steps times per iteration and report the sum of
+ * their launches; 0 otherwise. */
 int vib_bench(vib_context* ctx, int32_t count, const int32_t* slots, const vib_solve_options* options,
               int32_t iterations, double* iteration_ms, double* kernel_ms);
 /* One launch of a plain streaming copy of `bytes` (multiple of 16) between two fresh device

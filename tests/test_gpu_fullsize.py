@@ -219,3 +219,32 @@ def test_batch_of_twenty_windows(ctx, oracle, abi):
     assert bitwise == 20
     for s in slots:
         ctx.release(s)
+
+
+@pytest.mark.parametrize("name", ["c3", "c2"])
+def test_fullsize_mixed_step_meets_the_bound(ctx, abi, name):
+    """The metric's accuracy bound at full size: a mixed-precision step with two refinement passes
+    against the exact operator vs the plain FP64 step at identical lambda, from the same
+    linearization point: ||dx_mixed - dx_fp64|| / ||dx_fp64|| <= 1e-6 over states and inverse
+    ranges, trial cost within 1e-9 relative, at lambda 1e-4 and 1e-1. C3 is the ill-conditioned
+    window on which the unrefined FP32 Schur products miss the bound."""
+    w = window(name)
+    lay = w.layout()
+    n = lay["nr"] + lay["ne"]
+    for lam in (1e-4, 1e-1):
+        out = {}
+        for prec, refine in ((abi.VIB_FP64, 0), (abi.VIB_MIXED, 2), (abi.VIB_MIXED, 0)):
+            ctx.upload(0, w)
+            ctx.check(ctx.linearize(0, prec), "linearize")
+            st, rep = ctx.solve_step(0, lam, prec, refine)
+            assert st == 0, ctx.error()
+            out[(prec, refine)] = (ctx.read_step(0, n), rep)
+        d64, r64 = out[(abi.VIB_FP64, 0)]
+        d32, r32 = out[(abi.VIB_MIXED, 2)]
+        e = rel(d32, d64)
+        tc = abs(r32.trial_cost - r64.trial_cost) / r64.trial_cost
+        say(f"[gpu] {name} mixed lambda={lam}: dx rel {e:.2e} (unrefined {rel(out[(abi.VIB_MIXED, 0)][0], d64):.2e}), "
+            f"trial cost rel {tc:.2e}")
+        assert e <= 1e-6
+        assert tc <= 1e-9
+        assert r32.cost == r64.cost
