@@ -1818,7 +1818,10 @@ __device__ void cholesky_backward(const double* Lt_, int T, const double* dinv, 
     }
 }
 
-// Forward + backward solve with an existing factor (iterative refinement).
+// Forward + backward solve with an existing factor (iterative refinement). The forward
+// substitution runs tile column by tile column: wave 0 solves y_k with the inverse diagonal
+// tile, then the waves update the rows below it (L_ak y_k; in the operand layout the lane holds
+// L_ak[r][4 rr + q], r = lane % 16, q = lane / 16).
 __device__ void cholesky_solve_again(const double* Lt_, int T, const double* dinv, double* x, double* part) {
     const gdouble* Lt = (const gdouble*)Lt_;
     const int t = threadIdx.x, wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
@@ -1833,7 +1836,6 @@ __device__ void cholesky_solve_again(const double* Lt_, int T, const double* din
         }
         __syncthreads();
         for (int a = k + 1 + wave; a < T; a += nw) {
-            // (L_ak x_k)[r]: the lane holds L_ak[r][4 rr + q] (r = lane % 16, q = lane / 16)
             const int base = tile_base(a, k);
             double v = 0;
 #pragma unroll
@@ -1874,7 +1876,7 @@ __device__ double block_max_waves(double v, double* red) {
 // the relative bounds (only td is bounded among the states), the state model decrease (here only
 // for a window without landmarks; k_backsub spreads it over its chunk blocks otherwise), the
 // step norm and the state trial.
-__device__ void step_tail(WinDev& w, double* x, double* red) {
+__device__ __forceinline__ void step_tail(WinDev& w, double* x, double* red) {
     const int n = w.nr, t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
     if (t == 0 && w.td_col >= 0 && isfinite(x[w.td_col])) {
         const double tdv = *w.td;
@@ -1996,67 +1998,92 @@ __global__ __launch_bounds__(kCholThreads) void k_cholesky(WinDev* wins, int ref
 // pass contracts the step error by the factor's relative error (about 1e-4 on ill-conditioned
 // windows), so two passes reach the FP64 step.
 // ---------------------------------------------------------------------------------------------
+// Workgroups nchunks.. form the state part of the residual, -g - (H + lambda D) x, sixteen rows
+// each (hslices per window), into the window's scratch vector.
+constexpr int kResidRows = kBlock / 16;
+__device__ __forceinline__ void residual_rows(const WinDev& w, int slice) {
+    const int n = w.nr, t = threadIdx.x, i = slice * kResidRows + t / 16, l16 = t % 16;
+    if (i >= n) return;
+    const double* row = w.Hfull + (size_t)i * n;
+    const double* x = w.dstate;
+    double v = 0;
+#pragma unroll 4
+    for (int j = l16; j < n; j += 16) v += row[j] * x[j];
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1) v += __shfl_xor(v, m, 16);
+    if (l16 == 0) w.scratch[i] = -w.gfull[i] - v - w.lm->lambda * fmax(row[i], 1e-6) * x[i];
+}
+
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, const Chunk* chunks) {
-    __shared__ double crow[kSchurRows * kSchurLd];
-    __shared__ double xs[kMaxVisual + 1];
-    __shared__ double wk[kSchurRows];
+__global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, const Chunk* chunks, int nchunks,
+                                                        int hslices) {
+    constexpr int kZPitch = kMaxVisual + 1;
+    __shared__ double zw[kBlock / 64][kZPitch];  // one accumulator per wave
+    __shared__ double xs[kZPitch];
+    if ((int)blockIdx.x >= nchunks) {
+        const int r = blockIdx.x - nchunks;
+        const WinDev& w = wins[r / hslices];
+        if (window_active(w) && w.lm->chol_ok) residual_rows(w, r % hslices);
+        return;
+    }
     const Chunk ch = chunks[blockIdx.x];
     const WinDev& w = wins[ch.win];
     if (!window_active(w) || !w.lm->chol_ok) return;
-    const int nv = w.nv, t = threadIdx.x;
+    const int nv = w.nv, t = threadIdx.x, wave = t / 64;
     const double lambda = w.lm->lambda;
-    const int lgroup = t % kChunkLanes, group = t / kChunkLanes;
+    const int lane = t % kChunkLanes, group = t / kChunkLanes;
     constexpr int groups = kBlock / kChunkLanes;
     const size_t L = (size_t)w.L, F = (size_t)w.F;
-    for (int j = t; j < nv; j += kBlock) xs[j] = w.dstate[j];
-    double zsum = 0;  // column t of the chunk's landmark sum
-    for (int base = ch.lm_begin; base < ch.lm_end; base += kSchurRows) {
-        for (int i = t; i < kSchurRows * kSchurLd; i += kBlock) crow[i] = 0.0;
-        __syncthreads();
-        for (int r0 = 0; r0 < kSchurRows; r0 += groups) {
-            const int r = r0 + group, l = base + r;
-            if (r >= kSchurRows || l >= ch.lm_end || w.lm_col[l] < 0) continue;
-            double* row = crow + r * kSchurLd;
-            const int a = w.lm_fc[l] >> 8;
-            const int ac = w.pose_col[a];
-            const int f0 = w.lm_off[l], f1 = w.lm_off[l + 1];
-            for (int f = f0 + lgroup; f < f1; f += kChunkLanes) {
+    for (int j = t; j < kZPitch; j += kBlock) xs[j] = j < nv ? w.dstate[j] : 0.0;
+    for (int j = t; j < (kBlock / 64) * kZPitch; j += kBlock) (&zw[0][0])[j] = 0.0;
+    __syncthreads();
+    double* z = zw[wave];
+    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
+        const int l = base + group;
+        const bool free = l < ch.lm_end && w.lm_col[l] >= 0;
+        int a = 0, ac = -1;
+        double cd = 0;
+        if (free) {
+            a = w.lm_fc[l] >> 8;
+            ac = w.pose_col[a];
+            for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
                 const int b = w.ob_fc[f] >> 8;
-                const int bc = b == a ? -1 : w.pose_col[b];
-                for (int k = 0; k < 6; ++k) {
-                    const double u = load_u_exact<MIXED>(w, F, k, w.pf_pos[f]);
-                    if (ac >= 0) atomicAdd(&row[ac + k], u);
-                    if (bc >= 0) atomicAdd(&row[bc + k], -u);
-                }
-            }
-            if (lgroup == 0) {
-                if (w.rig_col >= 0)
-                    for (int k = 0; k < 6; ++k) row[w.rig_col + k] = w.lm_out[(LO_CR0 + k) * L + l];
-                if (w.td_col >= 0) row[w.td_col] = w.lm_out[LO_CT * L + l];
+                if (b == a) continue;  // same-epoch factors carry no pose coupling
+                const int bc = w.pose_col[b], p = w.pf_pos[f];
+                for (int k = 0; k < 6; ++k)
+                    cd += load_u_exact<MIXED>(w, F, k, p) * ((ac >= 0 ? xs[ac + k] : 0.0) - (bc >= 0 ? xs[bc + k] : 0.0));
             }
         }
-        __syncthreads();
-        // w_k = (g_k + c_k^T x) / damped(d_k): four lanes per row, combined in lane order
-        {
-            const int r = t / 4, q = t % 4, l = base + r;
-            double v = 0;
-            for (int j = q; j < nv; j += 4) v += crow[r * kSchurLd + j] * xs[j];
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            if (q == 0) {
-                double wv = 0;
-                if (l < ch.lm_end && w.lm_col[l] >= 0)
-                    wv = (w.lm_out[LO_G * L + l] + v) / damped_point(w.lm_out[LO_D * L + l], lambda);
-                wk[r] = wv;
+        cd = group_sum(cd);
+        if (!free) continue;
+        if (w.rig_col >= 0)
+            for (int k = 0; k < 6; ++k) cd += w.lm_out[(LO_CR0 + k) * L + l] * xs[w.rig_col + k];
+        if (w.td_col >= 0) cd += w.lm_out[LO_CT * L + l] * xs[w.td_col];
+        const double wk = (w.lm_out[LO_G * L + l] + cd) / damped_point(w.lm_out[LO_D * L + l], lambda);
+        // z += c_k w_k on the landmark's support; the lanes of one wave add in a fixed order
+        for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
+            const int b = w.ob_fc[f] >> 8;
+            if (b == a) continue;
+            const int bc = w.pose_col[b], p = w.pf_pos[f];
+            for (int k = 0; k < 6; ++k) {
+                const double v = wk * load_u_exact<MIXED>(w, F, k, p);
+                if (ac >= 0) atomicAdd(&z[ac + k], v);
+                if (bc >= 0) atomicAdd(&z[bc + k], -v);
             }
         }
-        __syncthreads();
-        if (t < nv)
-            for (int k = 0; k < kSchurRows; ++k) zsum += wk[k] * crow[k * kSchurLd + t];
-        __syncthreads();
+        if (lane == 0) {
+            if (w.rig_col >= 0)
+                for (int k = 0; k < 6; ++k) atomicAdd(&z[w.rig_col + k], wk * w.lm_out[(LO_CR0 + k) * L + l]);
+            if (w.td_col >= 0) atomicAdd(&z[w.td_col], wk * w.lm_out[LO_CT * L + l]);
+        }
     }
-    if (t < nv) w.schur_part[(size_t)ch.local * (w.tri + nv) + t] = zsum;
+    __syncthreads();
+    if (t < nv) {
+        double v = 0;
+#pragma unroll
+        for (int q = 0; q < kBlock / 64; ++q) v += zw[q][t];
+        w.schur_part[(size_t)ch.local * (w.tri + nv) + t] = v;
+    }
 }
 
 // last != 0: the final pass, which also finishes the step (clamp, model decrease, trial).
@@ -2073,34 +2100,42 @@ __global__ __launch_bounds__(kCholThreads) void k_refine(WinDev* wins, int last)
     const int T = (n + kTile - 1) / kTile, N = kTile * T;
     const int nv = w.nv, nsb = n - nv, len = w.tri + nv;
     auto to_solve = [&](int c) { return c < nv ? c + nsb : c - nv; };
-    const double lambda = w.lm->lambda;
     for (int p = t; p < T * kTileDoubles; p += nt) {
         const int a = p / kTileDoubles;
         dinv[p] = w.fac[tile_base(a, a) + p % kTileDoubles];
     }
     for (int c = t; c < N; c += nt) x[c] = c < n ? w.dstate[c] : 0.0;
-    // landmark sum z over the chunk partials: eight chunk lanes per column, in lane order
+    // landmark sum z over the chunk partials: eight chunk lanes per column, each adding its
+    // partials in chunk order; a lane issues the loads of up to kZBatch partials at once
+    // (absent ones read as zero, which leaves the sum unchanged)
     {
+        constexpr int kZBatch = 24;
         const int j = t / 8, q = t % 8;
         double v = 0;
-        if (j < nv)
-            for (int c = q; c < w.nchunks; c += 8) v += w.schur_part[(size_t)c * len + j];
+        if (j < nv) {
+            const double* src = w.schur_part + j;
+            const int nch = w.nchunks;
+            for (int c0 = q; c0 < nch; c0 += 8 * kZBatch) {
+                double b[kZBatch];
+#pragma unroll
+                for (int e = 0; e < kZBatch; ++e) {
+                    const int c = c0 + 8 * e;
+                    b[e] = c < nch ? src[(size_t)c * len] : 0.0;
+                }
+#pragma unroll
+                for (int e = 0; e < kZBatch; ++e) v += b[e];
+            }
+        }
         zp[t] = v;
     }
     __syncthreads();
-    // reduced residual, tangent order, one row per wave
-    for (int i = wave; i < n; i += nw) {
-        const double* row = w.Hfull + (size_t)i * n;
-        double v = 0;
-        for (int j = lane; j < n; j += 64) v += row[j] * x[j];
-        v = wave_reduce_sum(v);
-        if (lane == 0) {
-            double z = 0;
-            if (i < nv)
-                for (int q = 0; q < 8; ++q) z += zp[8 * i + q];
-            const double damp = lambda * fmax(row[i], 1e-6);
-            rres[to_solve(i)] = -w.gfull[i] - v - damp * x[i] + z;
-        }
+    // reduced residual: the state part from k_schur_apply's row workgroups, plus z
+    for (int i = t; i < n; i += nt) {
+        double z = 0;
+        if (i < nv)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) z += zp[8 * i + q];
+        rres[to_solve(i)] = w.scratch[i] + z;
     }
     for (int i = n + t; i < N; i += nt) rres[i] = 0.0;
     __syncthreads();
@@ -3319,12 +3354,13 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             hipLaunchKernelGGL(k_accept, dim3(nwin, (max_landmarks + kAcceptSlice - 1) / kAcceptSlice + 1), dim3(kBlock), 0, s,
                                wins);
             break;
-        case 12:  // exact-operator landmark sum of one refinement pass (mixed precision)
-            if (nchunks) {
-                if (mixed) hipLaunchKernelGGL(k_schur_apply<true>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
-                else hipLaunchKernelGGL(k_schur_apply<false>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
-            }
+        case 12: {  // exact-operator landmark sum and state residual of one refinement pass
+            const int hslices = (kMaxRetained + kResidRows - 1) / kResidRows;
+            const int grid = nchunks + nwin * hslices;
+            if (mixed) hipLaunchKernelGGL(k_schur_apply<true>, dim3(grid), dim3(kBlock), 0, s, wins, chunks, nchunks, hslices);
+            else hipLaunchKernelGGL(k_schur_apply<false>, dim3(grid), dim3(kBlock), 0, s, wins, chunks, nchunks, hslices);
             break;
+        }
         case 13: hipLaunchKernelGGL(k_refine, dim3(nwin), dim3(kCholThreads), 0, s, wins, mode); break;
         default: return hipErrorInvalidValue;
     }
