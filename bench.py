@@ -94,7 +94,7 @@ def batched_line(ctx, abi, wins, steps, warmup):
     slots = list(range(1000, 1000 + len(wins)))
     for s, w in zip(slots, wins):
         ctx.upload(s, w)
-    res = {}
+    res, kern = {}, {}
     for name, prec, refine in (("fp64", abi.VIB_FP64, 0), ("mixed", abi.VIB_MIXED, MIXED_REFINEMENT)):
         opts = abi.options(max_iterations=1, precision=prec, refinement_steps=refine, flags=abi.VIB_SOLVE_CONTINUOUS)
         ctx.bench(slots, opts, warmup, kernels=False)
@@ -102,6 +102,9 @@ def batched_line(ctx, abi, wins, steps, warmup):
         it_ms, _ = ctx.bench(slots, opts, steps, kernels=False)
         ctx.synchronize()
         res[name] = it_ms
+        # per-kernel device time of the batched launches alone (events around every launch)
+        _, kern[name] = ctx.bench(slots, opts, 2, kernels=True)
+        ctx.synchronize()
     opts = abi.options(max_iterations=1, flags=abi.VIB_SOLVE_CONTINUOUS)
     seq = []
     for s in slots:
@@ -117,7 +120,8 @@ def batched_line(ctx, abi, wins, steps, warmup):
                 p99_ms=float(np.percentile(res["fp64"], 99)), mixed_value=len(wins) / (mixed_ms * 1e-3),
                 mixed_ms_per_batched_step=mixed_ms, mixed_p99_ms=float(np.percentile(res["mixed"], 99)),
                 sequential_value=len(wins) / (seq_ms * 1e-3), sequential_ms=seq_ms,
-                speedup_vs_sequential=seq_ms / batch_ms, mixed_speedup_vs_sequential_fp64=seq_ms / mixed_ms)
+                speedup_vs_sequential=seq_ms / batch_ms, mixed_speedup_vs_sequential_fp64=seq_ms / mixed_ms,
+                mixed_refinement_steps=MIXED_REFINEMENT, kernel_ms=kern["fp64"], mixed_kernel_ms=kern["mixed"])
 
 
 def step_error(ctx, abi, win, slot, lam=1e-4, refinement=2):
@@ -396,6 +400,8 @@ def main():
     ap.add_argument("--whiten-observations", type=int, default=1 << 22,
                     help="pixel observations of the device whitening line (0: skip)")
     ap.add_argument("--no-slides", dest="slides", action="store_false", help="skip the window-slide line")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched workload line (prints its JSON object; for per-kernel profiles)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the measurement, launch one plain device copy of this many MiB "
                          "(reference dispatch for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -406,6 +412,13 @@ def main():
         sys.exit(2)
 
     from vibpy import abi, synth
+    if args.batched_only:
+        bwins = batch_windows(args.batch_windows, args.batch_procs)
+        ctx = abi.Context(0)
+        batched = batched_line(ctx, abi, bwins, max(3, min(args.steps, 10)), max(1, args.warmup))
+        ctx.close()
+        print(json.dumps(dict(batched=batched)))
+        return
     cfg = synth.CONFIGS[args.workload]
     win = synth.generate(**cfg)
     bwins = batch_windows(args.batch_windows, args.batch_procs) if args.batch_windows > 0 else []

@@ -484,10 +484,17 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     // Landmark chunks are merged `merge` at a time when the batch alone fills the device many
     // times over: fewer per-chunk Schur partials to write and reduce. The per-slot arrays are
     // sized for the unmerged count, so any merge fits.
-    size_t fine = 0;
-    for (Slot* s : slots) fine += s->host.chunks.size();
+    size_t fine = 0, fine_lin = 0;
+    for (Slot* s : slots) {
+        fine += s->host.chunks.size();
+        fine_lin += s->host.lin_chunks.size();
+    }
     int merge = fine >= kMergeChunksAbove ? kChunkMerge : 1;
     if (const char* m = std::getenv("VIB_CHUNK_MERGE")) merge = std::max(1, std::atoi(m));
+    // the same for the visual factor ranges: a merged range is one longer pair-ordered range,
+    // whose workgroup zeroes, and writes back, its visual-block accumulator once
+    int lin_merge = fine_lin >= kMergeLinAbove ? kLinMerge : 1;
+    if (const char* m = std::getenv("VIB_LIN_MERGE")) lin_merge = std::max(1, std::atoi(m));
     for (int i = 0; i < nwin; ++i) {
         Slot& s = *slots[i];
         wins[i] = s.dev;
@@ -497,7 +504,12 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
             all.push_back(Chunk{i, (int)(c0 / merge), hc[c0].lm_begin, hc[c1 - 1].lm_end});
         }
         wins[i].nchunks = (int)((hc.size() + merge - 1) / merge);
-        for (Chunk c : s.host.lin_chunks) { c.win = i; lin.push_back(c); }
+        const std::vector<Chunk>& hl = s.host.lin_chunks;
+        for (size_t c0 = 0; c0 < hl.size(); c0 += lin_merge) {
+            const size_t c1 = std::min(hl.size(), c0 + lin_merge);
+            lin.push_back(Chunk{i, (int)(c0 / lin_merge), hl[c0].lm_begin, hl[c1 - 1].lm_end});
+        }
+        wins[i].nlin = (int)((hl.size() + lin_merge - 1) / lin_merge);
         red[2 * i] = s.reduced_lin;
         red[2 * i + 1] = s.reduced_schur;
         max_len = std::max(max_len, wins[i].tri + wins[i].nv);

@@ -23,6 +23,11 @@ constexpr int kKernelCount = 14;  // kernels of one LM iteration (vibk_launch id
 // per workgroup (measured on the 256-window batched workload).
 constexpr size_t kMergeChunksAbove = 4096;
 constexpr int kChunkMerge = 4;
+// Visual factor ranges (kLinFactors each) are merged kLinMerge at a time per workgroup above
+// this many ranges in a batch: the per-workgroup fixed costs (zeroing and writing back the
+// dense visual-block accumulator, the frame table) are about 7 us per workgroup.
+constexpr size_t kMergeLinAbove = 2048;
+constexpr int kLinMerge = 4;
 constexpr int kFuseReduceWindows = 16;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X (workgroup i runs on XCD i % 8)
 

@@ -112,10 +112,12 @@ def test_full_solve_matches_oracle(ctx, oracle, abi, name, cfg):
 
 
 def test_merged_chunks_match_oracle(ctx, oracle, abi, monkeypatch):
-    """Landmark chunks merged three at a time per workgroup (the large-batch launch shape, here
-    forced on a small window with a ragged last group) give the same LM step as the oracle."""
+    """Landmark chunks and visual factor ranges merged three at a time per workgroup (the
+    large-batch launch shape, here forced on a small window with ragged last groups) give the
+    same LM step as the oracle."""
     from vibpy import synth
     monkeypatch.setenv("VIB_CHUNK_MERGE", "3")
+    monkeypatch.setenv("VIB_LIN_MERGE", "3")
     w = synth.generate(K=8, L=900, n_cams=4, obs_range=(2, 8), seed=31, rig_free=True, td_free=True)
     lay = w.layout()
     say("[gpu] step with merged chunks")
