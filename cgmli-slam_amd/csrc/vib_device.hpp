@@ -56,7 +56,7 @@ enum LmOut { LO_D, LO_G, LO_CR0, LO_CR1, LO_CR2, LO_CR3, LO_CR4, LO_CR5, LO_CT, 
 // per-factor landmark terms, summed per landmark into LmOut (doubles, each F long)
 enum FacLm { FL_D, FL_G, FL_CR0, FL_CR1, FL_CR2, FL_CR3, FL_CR4, FL_CR5, FL_CT, FL_PLANES };
 // per-chunk scalars
-enum ChunkScalar { CS_COST, CS_INVALID, CS_PRED, CS_MAXINC, CS_TRIAL, CS_TINVALID, CS_PLANES };
+enum ChunkScalar { CS_COST, CS_INVALID, CS_PRED, CS_MAXINC, CS_TRIAL, CS_TINVALID, CS_NONFINITE, CS_PLANES };
 
 // LM bookkeeping per window (BackendOptimizer.cpp:167-307), updated on the device.
 struct LmState {
@@ -72,6 +72,7 @@ struct LmState {
     int32_t accepted;
     int32_t converged;
     int32_t lin_valid, chol_ok, trial_valid;
+    int32_t step_nonfinite; // some entry of the last step (states or inverse ranges) is not finite
     int32_t max_iterations;
     int32_t evaluate_only;  // piecewise API: compute metrics, never accept
     int32_t continuous;     // benchmark mode: relinearize every iteration, never terminate
@@ -149,6 +150,8 @@ struct WinDev {
     double* ret_sc;            // 4 E + 4 : IMU costs (lin, trial), invalid (lin, trial); prior same
     const int32_t* colmap;     // nr x 4 : IMU (edge << 8 | local) x2, prior (block << 8 | index), -1
     // dense per-window buffers
+    const double* hostH;       // nr x nr normal matrix of host-evaluated factors, or null
+    const double* hostG;       // nr gradient of host-evaluated factors, or null
     double* Hfull;             // nr x nr undamped full normal matrix of the states
     double* gfull;             // nr
     double* fac;               // factor_doubles(nr): tile store of the damped reduced system,

@@ -77,19 +77,23 @@ int HostWindow::chunk_size = 64;
 // ------------------------------------------------------------------------------------------
 // Host-side window copy with derived layout.
 // ------------------------------------------------------------------------------------------
-bool HostWindow::build(const vib_window& w, std::string& err) {
+int HostWindow::build(const vib_window& w, std::string& err) {
+    const int BAD = VIB_ERR_INVALID_ARGUMENT;
     *this = HostWindow();
     K = w.num_frames; C = w.num_cameras; L = w.num_landmarks; F = w.num_observations; E = w.num_imu;
     flags = w.flags; huber = w.huber_delta; min_rho = w.min_inverse_depth;
     td_lo = w.td_lower; td_hi = w.td_upper;
-    if (K < 1 || K > kMaxFrames) { err = "window must hold 1..20 frames"; return false; }
-    if (C < 1 || C > kMaxCameras) { err = "window must hold 1..4 cameras"; return false; }
-    if (L < 0 || F < 0 || E < 0) { err = "negative sizes"; return false; }
-    if (!w.pose || !w.speed_bias || !w.rig || !w.td || !w.cam_rotation || !w.cam_translation) { err = "missing state"; return false; }
-    if (L && (!w.lm_anchor_frame || !w.lm_anchor_cam || !w.lm_bearing || !w.lm_velocity || !w.lm_tref || !w.lm_rho)) { err = "missing landmarks"; return false; }
-    if (F && (!w.obs_landmark || !w.obs_frame || !w.obs_cam || !w.obs_bearing || !w.obs_velocity || !w.obs_tref || !w.obs_sqrt_info)) { err = "missing observations"; return false; }
-    if (E && (!w.imu_from || !w.imu_to || !w.imu_data)) { err = "missing IMU"; return false; }
-    if (!(huber >= 0) || !(min_rho > 0)) { err = "invalid huber / inverse depth bound"; return false; }
+    if (K < 1 || C < 1) { err = "window must hold at least one frame and one camera"; return BAD; }
+    // limits of the device layout (the visual block and the reduced system are sized for them)
+    if (K > kMaxFrames) { err = "unsupported Schur dimensions: more than 20 frames"; return VIB_ERR_UNSUPPORTED_DIMENSIONS; }
+    if (C > kMaxCameras) { err = "unsupported Schur dimensions: more than 4 cameras"; return VIB_ERR_UNSUPPORTED_DIMENSIONS; }
+    if (L < 0 || F < 0 || E < 0) { err = "negative sizes"; return BAD; }
+    if (!w.pose || !w.speed_bias || !w.rig || !w.td || !w.cam_rotation || !w.cam_translation) { err = "missing state"; return BAD; }
+    if (L && (!w.lm_anchor_frame || !w.lm_anchor_cam || !w.lm_bearing || !w.lm_velocity || !w.lm_tref || !w.lm_rho)) { err = "missing landmarks"; return BAD; }
+    if (F && (!w.obs_landmark || !w.obs_frame || !w.obs_cam || !w.obs_bearing || !w.obs_velocity || !w.obs_tref || !w.obs_sqrt_info)) { err = "missing observations"; return BAD; }
+    if (E && (!w.imu_from || !w.imu_to || !w.imu_data)) { err = "missing IMU"; return BAD; }
+    if (!(huber >= 0) || !std::isfinite(huber)) { err = "unsupported visual loss (Huber scale must be finite and >= 0)"; return VIB_ERR_UNSUPPORTED_LOSS; }
+    if (!(min_rho > 0)) { err = "invalid inverse depth bound"; return BAD; }
     frame_id.assign(K, 0);
     for (int k = 0; k < K; ++k) frame_id[k] = w.frame_id ? w.frame_id[k] : k;
     pose.assign(w.pose, w.pose + 7 * K);
@@ -99,7 +103,7 @@ bool HostWindow::build(const vib_window& w, std::string& err) {
     for (int k = 0; k < K; ++k) {
         if (w.pose_free) pose_free[k] = w.pose_free[k] ? 1 : 0;
         if (w.sb_width) sb_width[k] = w.sb_width[k];
-        if (sb_width[k] != 0 && sb_width[k] != 3 && sb_width[k] != 9) { err = "sb_width must be 0, 3 or 9"; return false; }
+        if (sb_width[k] != 0 && sb_width[k] != 3 && sb_width[k] != 9) { err = "sb_width must be 0, 3 or 9"; return BAD; }
     }
     std::copy_n(w.rig, 7, rig);
     td = *w.td;
@@ -113,7 +117,7 @@ bool HostWindow::build(const vib_window& w, std::string& err) {
     rho.assign(L, 0);
     for (int l = 0; l < L; ++l) {
         lm_frame[l] = w.lm_anchor_frame[l]; lm_cam[l] = w.lm_anchor_cam[l];
-        if (lm_frame[l] < 0 || lm_frame[l] >= K || lm_cam[l] < 0 || lm_cam[l] >= C) { err = "landmark anchor out of range"; return false; }
+        if (lm_frame[l] < 0 || lm_frame[l] >= K || lm_cam[l] < 0 || lm_cam[l] >= C) { err = "landmark anchor out of range"; return BAD; }
         for (int i = 0; i < 3; ++i) {
             lmd[(size_t)(LM_BX + i) * L + l] = w.lm_bearing[3 * l + i];
             lmd[(size_t)(LM_VX + i) * L + l] = w.lm_velocity[3 * l + i];
@@ -126,14 +130,11 @@ bool HostWindow::build(const vib_window& w, std::string& err) {
     lm_off.assign(L + 1, 0);
     for (int f = 0; f < F; ++f) {
         const int l = w.obs_landmark[f];
-        if (l < 0 || l >= L) { err = "observation landmark out of range"; return false; }
-        if (w.obs_frame[f] < 0 || w.obs_frame[f] >= K || w.obs_cam[f] < 0 || w.obs_cam[f] >= C) { err = "observation frame/camera out of range"; return false; }
+        if (l < 0 || l >= L) { err = "observation landmark out of range"; return BAD; }
+        if (w.obs_frame[f] < 0 || w.obs_frame[f] >= K || w.obs_cam[f] < 0 || w.obs_cam[f] >= C) { err = "observation frame/camera out of range"; return BAD; }
         lm_off[l + 1]++;
     }
-    for (int l = 0; l < L; ++l) {
-        if (lm_off[l + 1] > kMaxFactorsPerLandmark) { err = "a landmark has more than 13 factors"; return false; }
-        lm_off[l + 1] += lm_off[l];
-    }
+    for (int l = 0; l < L; ++l) lm_off[l + 1] += lm_off[l];
     std::vector<int> cursor(lm_off.begin(), lm_off.end() - 1);
     ob_fc.assign(F, 0); ob_lm.assign(F, 0);
     ob.assign((size_t)OB_PLANES * F, 0);
@@ -154,18 +155,18 @@ bool HostWindow::build(const vib_window& w, std::string& err) {
     imu.assign(w.imu_data, w.imu_data + (size_t)VIB_IMU_STRIDE * E);
     for (int e = 0; e < E; ++e)
         if (imu_from[e] < 0 || imu_from[e] >= K || imu_to[e] < 0 || imu_to[e] >= K || imu_from[e] == imu_to[e]) {
-            err = "IMU edge out of range"; return false;
+            err = "IMU edge out of range"; return BAD;
         }
     // prior
     prior_rows = w.prior_rows > 0 ? w.prior_rows : 0;
     prior_cols = 0;
     if (prior_rows) {
         const int B = w.prior_blocks;
-        if (B <= 0 || !w.prior_key_kind || !w.prior_ref || !w.prior_A || !w.prior_r0) { err = "incomplete prior"; return false; }
+        if (B <= 0 || !w.prior_key_kind || !w.prior_ref || !w.prior_A || !w.prior_r0) { err = "incomplete prior"; return BAD; }
         int refs = 0;
         for (int b = 0; b < B; ++b) {
             const int k = w.prior_key_kind[b];
-            if (k < 0 || k > 3) { err = "invalid prior key kind"; return false; }
+            if (k < 0 || k > 3) { err = "invalid prior key kind"; return BAD; }
             prior_kind.push_back(k);
             prior_id.push_back(w.prior_key_id ? w.prior_key_id[b] : 0);
             refs += ambient_size(k);
@@ -175,7 +176,7 @@ bool HostWindow::build(const vib_window& w, std::string& err) {
         prior_A.assign(w.prior_A, w.prior_A + (size_t)prior_rows * prior_cols);
         prior_r0.assign(w.prior_r0, w.prior_r0 + prior_rows);
     }
-    return layout(err);
+    return layout(err) ? VIB_OK : BAD;
 }
 
 bool HostWindow::layout(std::string& err) {
@@ -305,7 +306,11 @@ bool HostWindow::layout(std::string& err) {
 Slot::~Slot() { release(); }
 
 void Slot::release() {
-    if (arena) hipFree(arena);
+    if (arena) (void)hipFree(arena);
+    if (host_terms) (void)hipFree(host_terms);
+    host_terms = nullptr;
+    host_cap = 0;
+    host_set = false;
     arena = nullptr;
     bytes = 0;
     capacity = 0;
@@ -331,18 +336,24 @@ size_t Slot::footprint(const HostWindow& h) {
     return 8 * d + 4 * i + sizeof(PriorBlockDev) * B + sizeof(LmState) + 64 * 256;
 }
 
-bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
+int Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     const size_t need = footprint(h);
     // The slot is live only once every part of the window is on the device: a failure on any
     // path below leaves it unused (find_slot no longer returns it) with its arena kept for reuse.
     used = false;
     if (!arena || need > capacity) {  // a re-upload that fits (a slide) keeps the arena
         release();
-        if (hipMalloc(&arena, need) != hipSuccess) { arena = nullptr; err = "device allocation failed"; return false; }
+        if (hipMalloc(&arena, need) != hipSuccess) {
+            arena = nullptr;
+            (void)hipGetLastError();
+            err = "device allocation failed";
+            return VIB_ERR_OUT_OF_MEMORY;
+        }
         capacity = need;
     }
     host = h;
     bytes = need;
+    host_set = false;  // host-evaluated terms belong to the previous window
     Arena a{reinterpret_cast<char*>(arena), bytes, 0};
     WinDev& d = dev;
     d = WinDev{};
@@ -409,7 +420,7 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     d.nlin = (int)nlin;
     reduced_lin = a.take<double>(len); reduced_schur = a.take<double>(len);
     d.lm = a.take<LmState>(1);
-    if (a.used > bytes) { err = "arena overflow"; return false; }
+    if (a.used > bytes) { err = "arena overflow"; return VIB_ERR_DEVICE; }
     // every copy into the arena stays inside the carved footprint (host-side bounds check)
     const char* lo = reinterpret_cast<const char*>(arena);
     auto up = [&](void* dst, const void* src, size_t n) {
@@ -440,7 +451,7 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     LmState lm{};
     lm.lambda = 1e-4; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1;
     ok = ok && up(d.lm, &lm, sizeof(lm));
-    if (!ok) { err = "upload failed"; return false; }
+    if (!ok) { err = "upload failed"; return VIB_ERR_DEVICE; }
     if (h.prior_rows) {
         // G = A^T A once per prior (one-window launch)
         WinDev* dw = nullptr;
@@ -453,11 +464,11 @@ bool Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
              vibk_prior_gram(dw, 1, gp, stream) == hipSuccess && hipStreamSynchronize(stream) == hipSuccess;
         if (dw) (void)hipFree(dw);
         if (gp) (void)hipFree(gp);
-        if (!ok) { err = "prior Gram matrix failed"; return false; }
+        if (!ok) { err = "prior Gram matrix failed"; return VIB_ERR_DEVICE; }
     }
-    if (hipStreamSynchronize(stream) != hipSuccess) { err = "upload synchronization failed"; return false; }
+    if (hipStreamSynchronize(stream) != hipSuccess) { err = "upload synchronization failed"; return VIB_ERR_DEVICE; }
     used = true;
-    return true;
+    return VIB_OK;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -498,6 +509,8 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
     for (int i = 0; i < nwin; ++i) {
         Slot& s = *slots[i];
         wins[i] = s.dev;
+        wins[i].hostH = s.host_set ? s.host_terms : nullptr;
+        wins[i].hostG = s.host_set ? s.host_terms + (size_t)s.host.nr * s.host.nr : nullptr;
         const std::vector<Chunk>& hc = s.host.chunks;
         for (size_t c0 = 0; c0 < hc.size(); c0 += merge) {
             const size_t c1 = std::min(hc.size(), c0 + merge);
@@ -629,6 +642,12 @@ struct vib_context {
     std::mutex mutex;
     bool use_graph = true;
     vib::Batch requests;  // descriptors of the last vib_request_iterate set (buffers reused)
+    size_t hbm_budget = 0;       // bytes of resident windows allowed (0: unlimited)
+    void* preint_buf = nullptr;  // grow-only device buffer of the preintegration launches
+    size_t preint_cap = 0;
+    ~vib_context() {
+        if (preint_buf) (void)hipFree(preint_buf);
+    }
 };
 
 namespace {
@@ -698,6 +717,10 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
     const auto t0 = std::chrono::steady_clock::now();
     if (o.precision != VIB_FP64 && o.precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     const int maxit = std::max(1, o.max_iterations);
+    for (Slot* s : slots)
+        if (s->host_set)
+            return fail(ctx, VIB_ERR_UNSUPPORTED,
+                        "a window with host-evaluated factors is solved through vib_linearize / vib_solve_step");
     for (Slot* s : slots)
         if (!write_lm(ctx, *s, fresh_lm(o))) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
     Batch batch;
@@ -1125,7 +1148,7 @@ int schur_prior(vib_context* ctx, const HostWindow& cur, int victim, const std::
     if (nvict == 0) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "no eliminated state block");
     // device: linearize the consumed factors, landmark pseudo-inverse Schur, reduced system
     Slot tmp;
-    if (!tmp.upload(m, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    if (const int st = tmp.upload(m, ctx->stream, err)) return fail(ctx, st, err);
     LmState lm{};
     lm.lambda = 0; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1; lm.evaluate_only = 1;
     lm.marginal = 1;
@@ -1249,8 +1272,8 @@ int install(vib_context* ctx, Slot& s, WindowCopy& w) {
     HostWindow h;
     std::string err;
     const vib_window v = w.view();
-    if (!h.build(v, err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
-    if (!s.upload(h, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    if (const int st = h.build(v, err)) return fail(ctx, st, err);
+    if (const int st = s.upload(h, ctx->stream, err)) return fail(ctx, st, err);
     return VIB_OK;
 }
 
@@ -1352,6 +1375,11 @@ bool reanchor_on_device(vib_context* ctx, Slot& s, int v, const std::vector<int3
     return ok;
 }
 
+int preintegrate_locked(vib_context* ctx, int32_t count, const double* samples, int32_t num_samples,
+                        const int32_t* sample_begin, const int32_t* sample_end, const double* t0, const double* t1,
+                        const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
+                        const double* gyro_end, double* out, int32_t* status, double* kernel_ms);
+
 int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int count, double t0, double t1,
                            const vib_imu_params* params, vib_marginal_report* rep) {
     HostWindow cur;
@@ -1405,9 +1433,14 @@ int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int
             double g0[3], g1[3];
             gyro_at(samples, count, t0, g0);
             gyro_at(samples, count, t1, g1);
+            // the re-integration runs on the device (k_preintegrate), as for any batch of edges
             merged.assign(VIB_IMU_STRIDE, 0);
-            const int st = vib_preintegrate(samples, count, t0, t1, lin, params, g0, g1, merged.data());
-            if (st != VIB_OK) return fail(ctx, st, "IMU re-integration failed");
+            const int32_t b0 = 0, e0 = count;
+            int32_t est = VIB_OK;
+            const int st = preintegrate_locked(ctx, 1, samples, count, &b0, &e0, &t0, &t1, lin, params, g0, g1,
+                                               merged.data(), &est, nullptr);
+            if (st == VIB_ERR_DEVICE || st == VIB_ERR_OUT_OF_MEMORY) return st;
+            if (st != VIB_OK || est != VIB_OK) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "IMU re-integration failed");
             imu_mode = incremental ? VIB_IMU_MERGE_INCREMENTAL : VIB_IMU_MERGE_REINTEGRATED;
         }
     }
@@ -1495,7 +1528,7 @@ int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int
     std::string err;
     {
         const vib_window view = w1.view();
-        if (!h1.build(view, err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
+        if (const int st = h1.build(view, err)) return fail(ctx, st, err);
     }
     SchurPrior p;
     const int st = schur_prior(ctx, h1, v, ret1, imus, p);
@@ -1560,7 +1593,7 @@ int64_t vib_window_footprint(const vib_window* w, int32_t precision) {
     if (!w) return -1;
     HostWindow h;
     std::string err;
-    if (!h.build(*w, err)) return -1;
+    if (h.build(*w, err) != VIB_OK) return -1;
     (void)precision;
     return (int64_t)Slot::footprint(h);
 }
@@ -1571,10 +1604,77 @@ int vib_window_upload(vib_context* ctx, int32_t slot, const vib_window* w) {
     hipSetDevice(ctx->device);
     auto h = std::make_unique<HostWindow>();
     std::string err;
-    if (!h->build(*w, err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
+    // a failed upload leaves the slot empty (its previous window is released)
+    auto drop = [&](int code) {
+        ctx->slots.erase(slot);
+        return fail(ctx, code, err);
+    };
+    if (const int st = h->build(*w, err)) return drop(st);
+    const size_t need = Slot::footprint(*h);
+    if (ctx->hbm_budget > 0) {
+        size_t resident = 0;
+        for (const auto& kv : ctx->slots)
+            if (kv.first != slot && kv.second && kv.second->used) resident += kv.second->capacity;
+        if (resident + need > ctx->hbm_budget) {
+            err = "window does not fit the context's HBM budget";
+            return drop(VIB_ERR_OUT_OF_MEMORY);
+        }
+    }
     auto& s = ctx->slots[slot];
     if (!s) s = std::make_unique<Slot>();
-    if (!s->upload(*h, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    if (const int st = s->upload(*h, ctx->stream, err)) return drop(st);
+    return VIB_OK;
+}
+
+int vib_set_hbm_budget(vib_context* ctx, int64_t bytes) {
+    if (!ctx || bytes < 0) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    ctx->hbm_budget = (size_t)bytes;
+    return VIB_OK;
+}
+
+int vib_resident_bytes(vib_context* ctx, int64_t* bytes) {
+    if (!ctx || !bytes) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    size_t total = 0;
+    for (const auto& kv : ctx->slots)
+        if (kv.second && kv.second->used) total += kv.second->capacity;
+    *bytes = (int64_t)total;
+    return VIB_OK;
+}
+
+int vib_set_host_retained(vib_context* ctx, int32_t slot, const double* H, const double* g) {
+    if (!ctx || (!H) != (!g)) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    if (!H) {
+        s->host_set = false;
+        return VIB_OK;
+    }
+    const size_t nr = s->host.nr, need = nr * nr + nr;
+    for (size_t i = 0; i < nr * nr; ++i)
+        if (!std::isfinite(H[i])) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "non-finite host normal matrix");
+    for (size_t i = 0; i < nr; ++i)
+        if (!std::isfinite(g[i])) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "non-finite host gradient");
+    if (need > s->host_cap) {
+        if (s->host_terms) (void)hipFree(s->host_terms);
+        s->host_terms = nullptr;
+        s->host_cap = 0;
+        s->host_set = false;
+        if (hipMalloc(&s->host_terms, need * sizeof(double)) != hipSuccess) {
+            s->host_terms = nullptr;
+            (void)hipGetLastError();
+            return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "host retained terms allocation failed");
+        }
+        s->host_cap = need;
+    }
+    const bool ok = hipMemcpyAsync(s->host_terms, H, nr * nr * sizeof(double), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(s->host_terms + nr * nr, g, nr * sizeof(double), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+                    hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "host retained terms upload failed");
+    s->host_set = true;
     return VIB_OK;
 }
 
@@ -1672,6 +1772,12 @@ int vib_solve_step(vib_context* ctx, int32_t slot, double lambda, int32_t precis
     rep->trial_valid = lm.trial_valid;
     rep->status = status_from_lm(lm.status);
     if (rep->status != VIB_OK) return fail(ctx, rep->status, status_message(rep->status));
+    // the increment goes back to the caller: a non-finite one is the reference's
+    // "nonfinite CUDA Schur step" (BackendOptimizer.cpp:253-254)
+    if (lm.step_nonfinite) {
+        rep->status = VIB_ERR_NONFINITE_STEP;
+        return fail(ctx, VIB_ERR_NONFINITE_STEP, "nonfinite Schur step");
+    }
     return VIB_OK;
 }
 
@@ -1783,6 +1889,9 @@ int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* ids, int
         if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
         for (Slot* o : slots)
             if (o == s) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "a slot is listed twice");
+        if (s->host_set)
+            return fail(ctx, VIB_ERR_UNSUPPORTED,
+                        "a window with host-evaluated factors is solved through vib_linearize / vib_solve_step");
         slots.push_back(s);
     }
     Batch& b = ctx->requests;
@@ -1822,13 +1931,16 @@ int vib_copy_probe(vib_context* ctx, int64_t bytes) {
     return ok ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "copy probe failed");
 }
 
-int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* samples, int32_t num_samples,
-                           const int32_t* sample_begin, const int32_t* sample_end, const double* t0, const double* t1,
-                           const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
-                           const double* gyro_end, double* out, int32_t* status, double* kernel_ms) {
-    if (!ctx || count <= 0 || !samples || num_samples < 2 || !sample_begin || !sample_end || !t0 || !t1 || !speed_bias ||
-        !params || !gyro_start || !gyro_end || !out || !status)
-        return VIB_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+namespace {
+
+// Device preintegration of `count` intervals of one sample array (one k_preintegrate launch),
+// with the context's lock held. Inputs are staged in the context's grow-only device buffer.
+int preintegrate_locked(vib_context* ctx, int32_t count, const double* samples, int32_t num_samples,
+                        const int32_t* sample_begin, const int32_t* sample_end, const double* t0, const double* t1,
+                        const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
+                        const double* gyro_end, double* out, int32_t* status, double* kernel_ms) {
     const int S = VIB_IMU_SAMPLE_STRIDE;
     // the checks of the single-interval routine, per interval; invalid intervals are not launched
     std::vector<int> live;
@@ -1852,14 +1964,21 @@ int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* sample
         std::memcpy(&hg0[(size_t)i * 3], gyro_start + (size_t)e * 3, 3 * sizeof(double));
         std::memcpy(&hg1[(size_t)i * 3], gyro_end + (size_t)e * 3, 3 * sizeof(double));
     }
-    std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
-    // one allocation: samples | t0 | t1 | speed/bias | gyro | records | begin | end | status
+    // one buffer: samples | t0 | t1 | speed/bias | gyro | records | begin | end | status
     const size_t nd = (size_t)num_samples * S + 2 * (size_t)m + 15 * (size_t)m + (size_t)m * VIB_IMU_STRIDE;
     const size_t bytes = nd * sizeof(double) + 3 * (size_t)m * sizeof(int);
-    char* buf = nullptr;
-    if (hipMalloc(&buf, bytes) != hipSuccess) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "preintegration allocation failed");
-    double* d_smp = reinterpret_cast<double*>(buf);
+    if (bytes > ctx->preint_cap) {
+        if (ctx->preint_buf) (void)hipFree(ctx->preint_buf);
+        ctx->preint_buf = nullptr;
+        ctx->preint_cap = 0;
+        const size_t cap = std::max(bytes, (size_t)1 << 20);
+        if (hipMalloc(&ctx->preint_buf, cap) != hipSuccess) {
+            ctx->preint_buf = nullptr;
+            return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "preintegration allocation failed");
+        }
+        ctx->preint_cap = cap;
+    }
+    double* d_smp = reinterpret_cast<double*>(ctx->preint_buf);
     double* d_t0 = d_smp + (size_t)num_samples * S;
     double* d_t1 = d_t0 + m;
     double* d_sb = d_t1 + m;
@@ -1895,7 +2014,6 @@ int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* sample
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    (void)hipFree(buf);
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, "preintegration launch failed");
     int rc = m == count ? VIB_OK : VIB_ERR_INVALID_ARGUMENT;
     for (int i = 0; i < m; ++i) {
@@ -1908,6 +2026,23 @@ int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* sample
         std::memcpy(out + (size_t)e * VIB_IMU_STRIDE, &hout[(size_t)i * VIB_IMU_STRIDE], sizeof(double) * VIB_IMU_STRIDE);
     }
     return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* samples, int32_t num_samples,
+                           const int32_t* sample_begin, const int32_t* sample_end, const double* t0, const double* t1,
+                           const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
+                           const double* gyro_end, double* out, int32_t* status, double* kernel_ms) {
+    if (!ctx || count <= 0 || !samples || num_samples < 2 || !sample_begin || !sample_end || !t0 || !t1 || !speed_bias ||
+        !params || !gyro_start || !gyro_end || !out || !status)
+        return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    return preintegrate_locked(ctx, count, samples, num_samples, sample_begin, sample_end, t0, t1, speed_bias, params,
+                               gyro_start, gyro_end, out, status, kernel_ms);
 }
 
 namespace {
@@ -2190,7 +2325,7 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
     rep->support_landmarks = lt.support;
     if (st != VIB_OK) return fail(ctx, st, err);
     Slot local;
-    if (!local.upload(lt.win, ctx->stream, err)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, err);
+    if (const int st = local.upload(lt.win, ctx->stream, err)) return fail(ctx, st, err);
     double nis[2];
     if (!imu_nis(ctx, local, lt.edges, nis)) return fail(ctx, VIB_ERR_DEVICE, "IMU residual evaluation failed");
     rep->old_imu_nis_before = nis[0];

@@ -12,8 +12,6 @@
 
 namespace vib {
 
-constexpr int kMaxFactorsPerLandmark = 13;
-
 // Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major, small n); eigenvalues
 // ascending, eigenvectors as columns of V (vib_preint.cpp).
 void symmetric_eigen(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V);
@@ -68,7 +66,9 @@ struct HostWindow {
     std::vector<int32_t> pf_ids, pf_lm, pf_pos;
     std::vector<Chunk> lin_chunks;  // factor ranges of the visual-normal kernel
 
-    bool build(const vib_window& w, std::string& err);
+    // VIB_OK, or VIB_ERR_INVALID_ARGUMENT (malformed), VIB_ERR_UNSUPPORTED_DIMENSIONS (beyond the
+    // window limits) or VIB_ERR_UNSUPPORTED_LOSS, with the message in err
+    int build(const vib_window& w, std::string& err);
     bool layout(std::string& err);
 };
 
@@ -83,11 +83,15 @@ struct Slot {
     double* reduced_lin = nullptr;
     double* reduced_schur = nullptr;
     int lin_precision = VIB_FP64;  // coupling-stream format of the last vib_linearize
+    double* host_terms = nullptr;  // nr x nr + nr: normal equations of host-evaluated factors
+    size_t host_cap = 0;           // doubles of host_terms
+    bool host_set = false;         // host terms are added at the next linearizations
 
     ~Slot();
     void release();
     static size_t footprint(const HostWindow& h);
-    bool upload(const HostWindow& h, hipStream_t stream, std::string& err);
+    // VIB_OK, VIB_ERR_OUT_OF_MEMORY (allocation) or VIB_ERR_DEVICE (copies, launches)
+    int upload(const HostWindow& h, hipStream_t stream, std::string& err);
 };
 
 // Descriptors of a set of windows solved together.

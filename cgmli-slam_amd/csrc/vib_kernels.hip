@@ -1297,6 +1297,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
             const int i = row0 + p / nr, j = p % nr;
             double h = retained_entry(w, w.colmap + 4 * i, w.colmap + 4 * j);
             if (i < nv && j < nv) h += rl[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
+            if (w.hostH) h += w.hostH[(size_t)i * nr + j];
             w.Hfull[(size_t)i * nr + j] = h;
         }
         for (int i = row0 + t; i < row1; i += nt) {
@@ -1316,6 +1317,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
                 for (int x = 0; x < b.tsize; ++x)
                     g += (se3 ? w.pD[bi * 36 + x * 6 + ii] : (x == ii ? 1.0 : 0.0)) * w.patr[b.chart + x];
             }
+            if (w.hostG) g += w.hostG[i];
             w.gfull[i] = g;
         }
         if (blockIdx.y == 0) {  // cost at the linearization point (fixed-order sums)
@@ -1919,6 +1921,7 @@ __device__ __forceinline__ void step_tail(WinDev& w, double* x, double* red) {
         w.lm->predicted = ps;
         w.lm->max_increment = maxinc;
         w.lm->trial_valid = nonfinite > 0 ? 0 : 1;
+        w.lm->step_nonfinite = nonfinite > 0 ? 1 : 0;
     }
 }
 
@@ -2038,20 +2041,37 @@ __global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, cons
     for (int j = t; j < (kBlock / 64) * kZPitch; j += kBlock) (&zw[0][0])[j] = 0.0;
     __syncthreads();
     double* z = zw[wave];
+    constexpr int kCached = 2;  // factors per lane kept in registers between the dot and the scatter
     for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
         const int l = base + group;
         const bool free = l < ch.lm_end && w.lm_col[l] >= 0;
         int a = 0, ac = -1;
         double cd = 0;
+        double uc[kCached][6];
+        int bcc[kCached];
+#pragma unroll
+        for (int e = 0; e < kCached; ++e) bcc[e] = -2;  // -2: no factor (same-epoch or absent)
         if (free) {
             a = w.lm_fc[l] >> 8;
             ac = w.pose_col[a];
-            for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
+            int e = 0;
+            for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes, ++e) {
                 const int b = w.ob_fc[f] >> 8;
                 if (b == a) continue;  // same-epoch factors carry no pose coupling
                 const int bc = w.pose_col[b], p = w.pf_pos[f];
-                for (int k = 0; k < 6; ++k)
-                    cd += load_u_exact<MIXED>(w, F, k, p) * ((ac >= 0 ? xs[ac + k] : 0.0) - (bc >= 0 ? xs[bc + k] : 0.0));
+                double u[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    u[k] = load_u_exact<MIXED>(w, F, k, p);
+                    cd += u[k] * ((ac >= 0 ? xs[ac + k] : 0.0) - (bc >= 0 ? xs[bc + k] : 0.0));
+                }
+#pragma unroll
+                for (int q = 0; q < kCached; ++q)
+                    if (q == e) {
+                        bcc[q] = bc;
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) uc[q][k] = u[k];
+                    }
             }
         }
         cd = group_sum(cd);
@@ -2061,7 +2081,16 @@ __global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, cons
         if (w.td_col >= 0) cd += w.lm_out[LO_CT * L + l] * xs[w.td_col];
         const double wk = (w.lm_out[LO_G * L + l] + cd) / damped_point(w.lm_out[LO_D * L + l], lambda);
         // z += c_k w_k on the landmark's support; the lanes of one wave add in a fixed order
-        for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
+#pragma unroll
+        for (int q = 0; q < kCached; ++q) {
+            if (bcc[q] == -2) continue;
+            for (int k = 0; k < 6; ++k) {
+                const double v = wk * uc[q][k];
+                if (ac >= 0) atomicAdd(&z[ac + k], v);
+                if (bcc[q] >= 0) atomicAdd(&z[bcc[q] + k], -v);
+            }
+        }
+        for (int f = w.lm_off[l] + lane + kCached * kChunkLanes; f < w.lm_off[l + 1]; f += kChunkLanes) {
             const int b = w.ob_fc[f] >> 8;
             if (b == a) continue;
             const int bc = w.pose_col[b], p = w.pf_pos[f];
@@ -2372,6 +2401,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
         sc[CS_PRED] = ps;
         sc[CS_MAXINC] = m;
         sc[CS_TINVALID] = b;
+        sc[CS_NONFINITE] = b;
     }
 }
 
@@ -2430,7 +2460,7 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
     } else {
         // per-chunk and retained scalars of the trial (fixed-order block sums)
         const int E = w.E;
-        double pred = 0, trial = 0, mx = 0, tinv = 0;
+        double pred = 0, trial = 0, mx = 0, tinv = 0, nonfinite = 0;
         for (int e = t; e < E; e += blockDim.x) {
             trial += w.ret_sc[E + e];
             tinv = fmax(tinv, w.ret_sc[3 * E + e]);
@@ -2441,8 +2471,10 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
             trial += sc[CS_TRIAL];
             mx = fmax(mx, sc[CS_MAXINC]);
             tinv = fmax(tinv, sc[CS_TINVALID]);
+            nonfinite = fmax(nonfinite, sc[CS_NONFINITE]);
         }
         pred = block_sum(pred, red);
+        nonfinite = block_max(nonfinite, red);
         trial = block_sum(trial, red);
         mx = block_max(mx, red);
         tinv = block_max(tinv, red);
@@ -2463,6 +2495,7 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
                 }
             }
             if (status == 0 && !lm->chol_ok) status = -4;
+            if (nonfinite > 0) lm->step_nonfinite = 1;
             lm->predicted = pred;
             lm->trial_cost = trial;
             lm->max_increment = mx;

@@ -22,6 +22,16 @@ STATUS_NAMES = {
     18: "IMU_GATE",
 }
 VIB_ERR_INVALID_ARGUMENT = 1
+VIB_ERR_UNSUPPORTED_DIMENSIONS = 2
+VIB_ERR_INVALID_RESIDUAL = 3
+VIB_ERR_NOT_POSITIVE_DEFINITE = 4
+VIB_ERR_COST_MISMATCH = 5
+VIB_ERR_NO_ACCEPTABLE_STEP = 6
+VIB_ERR_UNSUPPORTED_LOSS = 7
+VIB_ERR_DEVICE = 8
+VIB_ERR_OUT_OF_MEMORY = 9
+VIB_ERR_NONFINITE_STEP = 10
+VIB_ERR_NOT_FOUND = 14
 VIB_ERR_UNSUPPORTED = 15
 VIB_ERR_LIGHTWEIGHT_SUPPORT = 16
 VIB_ERR_IMU_BRIDGE = 17
@@ -326,6 +336,9 @@ def load_library(path=LIB_PATH):
         "vib_window_upload": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
         "vib_window_footprint": ([C.POINTER(VibWindow), C.c_int32], C.c_int64),
         "vib_window_release": ([vp, C.c_int32], C.c_int),
+        "vib_set_hbm_budget": ([vp, C.c_int64], C.c_int),
+        "vib_set_host_retained": ([vp, C.c_int32, dp, dp], C.c_int),
+        "vib_resident_bytes": ([vp, lp], C.c_int),
         "vib_read_state": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
         "vib_window_shape": ([vp, C.c_int32] + [ip] * 6, C.c_int),
         "vib_linearize": ([vp, C.c_int32, C.c_int32], C.c_int),
@@ -523,6 +536,33 @@ class Context:
         s = win.cstruct()
         self._cams[slot] = int(win.cam_rotation.shape[0])
         return self.check(self.lib.vib_window_upload(self.h, slot, C.byref(s)), "vib_window_upload")
+
+    def set_host_retained(self, slot, H=None, g=None):
+        """Normal equations of host-evaluated factors (None clears them)."""
+        if H is None:
+            return self.check(self.lib.vib_set_host_retained(self.h, slot, None, None), "vib_set_host_retained")
+        H = np.ascontiguousarray(H, np.float64)
+        g = np.ascontiguousarray(g, np.float64)
+        return self.check(self.lib.vib_set_host_retained(self.h, slot, f64p(H), f64p(g)), "vib_set_host_retained")
+
+    def set_hbm_budget(self, nbytes):
+        return self.check(self.lib.vib_set_hbm_budget(self.h, int(nbytes)), "vib_set_hbm_budget")
+
+    def resident_bytes(self):
+        v = C.c_int64()
+        self.check(self.lib.vib_resident_bytes(self.h, C.byref(v)), "vib_resident_bytes")
+        return v.value
+
+    def try_upload(self, slot, win):
+        """vib_window_upload returning its status instead of raising."""
+        s = win.cstruct()
+        return self.lib.vib_window_upload(self.h, slot, C.byref(s))
+
+    def release_if_present(self, slot):
+        """vib_window_release that ignores an empty slot."""
+        st = self.lib.vib_window_release(self.h, slot)
+        if st not in (VIB_OK, VIB_ERR_NOT_FOUND):
+            self.check(st, "vib_window_release")
 
     def release(self, slot):
         return self.check(self.lib.vib_window_release(self.h, slot), "vib_window_release")

@@ -247,6 +247,17 @@ const char* vib_last_error(const vib_context* ctx);
  * streams, IMU and prior are copied to HBM (reference linearizeSchur(newTopology) +
  * configureRetained + configureResident, VisualCudaBatch.cu:433-511,606-624,673-691). */
 int vib_window_upload(vib_context* ctx, int32_t slot, const vib_window* window);
+/* Errors of an upload: VIB_ERR_INVALID_ARGUMENT (malformed window), VIB_ERR_UNSUPPORTED_DIMENSIONS
+ * (more than 20 frames or 4 cameras: the reference's "unsupported Schur dimensions",
+ * BackendOptimizer.cpp:67-68, at this library's limits), VIB_ERR_UNSUPPORTED_LOSS (Huber scale
+ * negative or not finite; BackendOptimizer.cpp:112), VIB_ERR_OUT_OF_MEMORY (allocation, or the
+ * HBM budget of vib_set_hbm_budget), VIB_ERR_DEVICE (copies). A failed upload leaves the slot
+ * empty (VIB_ERR_NOT_FOUND afterwards). A landmark may have any number of factors. */
+/* HBM admission (no counterpart in the reference; for callers that serve many sessions): an
+ * upload fails with VIB_ERR_OUT_OF_MEMORY when the resident windows' footprint plus the new one
+ * would exceed `bytes` (0: no budget). vib_resident_bytes reports the current footprint. */
+int vib_set_hbm_budget(vib_context* ctx, int64_t bytes);
+int vib_resident_bytes(vib_context* ctx, int64_t* bytes);
 /* Bytes of HBM a window of this shape occupies (admission accounting). */
 int64_t vib_window_footprint(const vib_window* window, int32_t precision);
 int vib_window_release(vib_context* ctx, int32_t slot);
@@ -259,9 +270,20 @@ int vib_window_shape(vib_context* ctx, int32_t slot, int32_t* frames, int32_t* l
 /* ---------------- one LM iteration, exposed piecewise (VisualCudaBatch API) ---------------- */
 /* Linearize visual + IMU + prior factors at the resident state (linearizeResident, :693-700). */
 int vib_linearize(vib_context* ctx, int32_t slot, int32_t precision);
-/* Damped Schur step, back-substitution, resident trial and trial cost (solveOnline, :530-604). */
+/* Damped Schur step, back-substitution, resident trial and trial cost (solveOnline, :530-604).
+ * Returns VIB_ERR_NONFINITE_STEP when an entry of the step (vib_read_step) is not finite. */
 int vib_solve_step(vib_context* ctx, int32_t slot, double lambda, int32_t precision,
                    int32_t refinement_steps, vib_step_report* report);
+/* Normal equations of factors the caller evaluates on the host (the retainedH / retainedG
+ * arguments of solveOnline, VisualCudaBatch.hpp:61-62; BackendOptimizer.cpp:210-228): H is the
+ * full nr x nr matrix (row-major, the slot's tangent column layout, vib_retained_dimension) and g
+ * the nr gradient of those factors at the current state. They are added to the window's own
+ * terms at every following vib_linearize (damping then uses diag(H + H_host), as the reference)
+ * until replaced, cleared (H = g = NULL) or the slot is re-uploaded. As in the reference, whose
+ * solve leaves the device-resident mode when host factors exist, such a window takes the
+ * piecewise API: the caller adds its factors' cost to the reported costs, and vib_solve,
+ * vib_solve_batch and the request API return VIB_ERR_UNSUPPORTED for it. */
+int vib_set_host_retained(vib_context* ctx, int32_t slot, const double* H, const double* g);
 /* Adopt the last trial state (acceptResident, :702). */
 int vib_accept(vib_context* ctx, int32_t slot);
 /* Debug/parity read-outs of the last step: full tangent step [states, rho] (length nr + L)

@@ -47,3 +47,18 @@ def canonical_obs(w):
     order = np.lexsort((w.obs_cam, w.obs_frame, w.obs_landmark))
     return {k: getattr(w, k)[order] for k in ("obs_landmark", "obs_frame", "obs_cam", "obs_bearing", "obs_velocity",
                                                "obs_tref", "obs_sqrt_info")}
+
+
+def assert_imu_records_close(got, exp, rtol=1e-9):
+    """IMU records (VIB_IMU_STRIDE rows) agree: the preintegrated deltas, reference bias and bias
+    Jacobians (entries 0..75) elementwise, the whitening W (entries 76..300) through W^T W. The
+    eigenvectors behind W carry free signs (the reference's eigensolver fixes none), so only
+    W^T W, the inverse covariance the factor uses, is comparable."""
+    import numpy as np
+    got = np.atleast_2d(got)
+    exp = np.atleast_2d(exp)
+    np.testing.assert_allclose(got[:, :76], exp[:, :76], rtol=rtol, atol=1e-12)
+    for a, b in zip(got, exp):
+        Wa, Wb = a[76:].reshape(15, 15), b[76:].reshape(15, 15)
+        Ga, Gb = Wa.T @ Wa, Wb.T @ Wb
+        assert np.linalg.norm(Ga - Gb) <= 1e-8 * np.linalg.norm(Gb)

@@ -20,7 +20,7 @@ slide; C2 is five LM iterations then a SECOND_NEW slide.
 import numpy as np
 import pytest
 
-from parity_helpers import canonical_obs, prior_errors, rel, say
+from parity_helpers import assert_imu_records_close, canonical_obs, prior_errors, rel, say
 
 pytestmark = pytest.mark.gpu
 
@@ -125,8 +125,8 @@ def test_c1_recipe_step_then_old(ctx, oracle, abi):
 def test_c2_recipe_five_iterations_then_second_new(ctx, oracle, abi):
     """C2: five LM iterations, then SECOND_NEW with the raw IMU samples (the two IMU edges of the
     redundant frame merge into one re-integrated edge). The slid window matches the oracle's in
-    topology exactly, the merged IMU record within 1e-9 and the prior within 1e-6; both windows
-    then take the same LM step."""
+    topology exactly, the merged IMU record within 1e-9 (whitening as W^T W) and the prior within
+    1e-6; both windows then take the same LM step."""
     w = window("c2")
     say(f"[gpu] C2 recipe: K={w.K} L={w.L} F={w.F}")
     _solve_both(ctx, oracle, abi, w, 5)
@@ -148,7 +148,7 @@ def test_c2_recipe_five_iterations_then_second_new(ctx, oracle, abi):
     for k in ("lm_anchor_frame", "lm_anchor_cam", "lm_bearing", "lm_velocity", "imu_from", "imu_to", "frame_id"):
         np.testing.assert_array_equal(getattr(dev, k), getattr(nxt, k), err_msg=k)
     np.testing.assert_allclose(dev.lm_rho, nxt.lm_rho, rtol=1e-12)
-    np.testing.assert_allclose(dev.imu_data, nxt.imu_data, rtol=1e-9, atol=1e-12)
+    assert_imu_records_close(dev.imu_data, nxt.imu_data)
     cd, co = canonical_obs(dev), canonical_obs(nxt)
     for k in cd:
         np.testing.assert_array_equal(cd[k], co[k], err_msg=k)
