@@ -70,6 +70,42 @@ struct Arena {
 
 }  // namespace
 
+// Scratch of the no-allocation gate (vib_nis_gate_static): the window descriptor, its landmark
+// chunks, the 18 statistics, the per-factor energies and decisions of one call, in one
+// file-scope device array. Every GPU the library runs on holds its own copy until the process
+// ends; host code reaches it only through the per-device address that vib_create looks up
+// (the array's name is not a device address in host code).
+constexpr size_t kStaticGateFactors = size_t(1) << 20;  // factors of a window
+constexpr size_t kStaticGateChunks = size_t(1) << 17;   // landmark chunks of a window
+constexpr size_t kScratchChunkOffset = 4096;            // after the descriptor (sizeof(WinDev) < 4 KB)
+constexpr size_t kScratchStatsOffset = kScratchChunkOffset + kStaticGateChunks * sizeof(Chunk);
+constexpr size_t kScratchEnergyOffset = kScratchStatsOffset + 256;
+constexpr size_t kScratchDecisionOffset = kScratchEnergyOffset + kStaticGateFactors * sizeof(double);
+constexpr size_t kScratchBytes = kScratchDecisionOffset + kStaticGateFactors;
+static_assert(sizeof(WinDev) <= kScratchChunkOffset, "descriptor region");
+static_assert(18 * sizeof(double) <= 256, "statistics region");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (`device`), looked up once per device.
+int scratch_address(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return VIB_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return VIB_ERR_DEVICE;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return VIB_OK;
+}
+
 // 64 landmarks per chunk workgroup: a 20000-landmark window spreads over 313 workgroups (more
 // than one per CU), against twice the chunk partials to reduce (measured against 32..192).
 int HostWindow::chunk_size = 64;
@@ -643,6 +679,7 @@ struct vib_context {
     bool use_graph = true;
     vib::Batch requests;  // descriptors of the last vib_request_iterate set (buffers reused)
     size_t hbm_budget = 0;       // bytes of resident windows allowed (0: unlimited)
+    void* scratch = nullptr;     // device address of this device's `scratch` array
     void* preint_buf = nullptr;  // grow-only device buffer of the preintegration launches
     size_t preint_cap = 0;
     ~vib_context() {
@@ -1564,6 +1601,13 @@ int vib_create(vib_context** out, int device) {
         delete ctx;
         return VIB_ERR_DEVICE;
     }
+    void* sp = nullptr;  // the no-allocation gate's scratch on this device
+    if (const int st = scratch_address(device, &sp)) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return st;
+    }
+    ctx->scratch = sp;
     const char* g = std::getenv("VIB_NO_GRAPH");
     ctx->use_graph = !(g && g[0] == '1');
     if (const char* c = std::getenv("VIB_CHUNK")) HostWindow::chunk_size = std::max(8, std::atoi(c));
@@ -2048,30 +2092,28 @@ int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* sample
 namespace {
 
 // The statistical gate of one resident window (the lock is held by the caller).
-static int nis_gate_locked(vib_context* ctx, Slot* s, double threshold, int apply, vib_gate_report* rep, uint8_t* dec_out) {
-    *rep = vib_gate_report{};
-    rep->evaluated.worst_factor = rep->admitted.worst_factor = -1;
+// Device buffers of one gate call: the window descriptor and its landmark chunks, the per-factor
+// energies and decisions, and the 18 statistics.
+struct GateBuffers {
+    const WinDev* wins;
+    const Chunk* chunks;
+    int nchunks;
+    double* energy;
+    uint8_t* decision;
+    double* stats;
+};
+constexpr int kGateStats = 18;
+
+static int nis_gate_core(vib_context* ctx, Slot* s, double threshold, int apply, vib_gate_report* rep, uint8_t* dec_out,
+                         const GateBuffers& g) {
     HostWindow& h = s->host;
     const int F = h.F;
-    if (F == 0) return VIB_OK;
-    hipSetDevice(ctx->device);
-    std::vector<Slot*> one{s};
-    Batch b;
-    std::string err;
-    if (!b.build(one, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
-    constexpr int kStats = 18;
-    char* buf = nullptr;
-    if (hipMalloc(&buf, (size_t)F * 9 + kStats * sizeof(double)) != hipSuccess)
-        return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "gate allocation failed");
-    double* d_energy = reinterpret_cast<double*>(buf);
-    double* d_stats = d_energy + F;
-    uint8_t* d_dec = reinterpret_cast<uint8_t*>(d_stats + kStats);
     std::vector<uint8_t> dec(F);
-    double st[kStats];
-    bool ok = vibk_nis_gate(b.d_wins, b.d_chunks, b.nchunks, threshold, d_energy, d_dec, F, h.huber, d_stats,
+    double st[kGateStats];
+    bool ok = vibk_nis_gate(g.wins, g.chunks, g.nchunks, threshold, g.energy, g.decision, F, h.huber, g.stats,
                             ctx->stream) == hipSuccess &&
-              hipMemcpyAsync(dec.data(), d_dec, F, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-              hipMemcpyAsync(st, d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              hipMemcpyAsync(dec.data(), g.decision, F, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              hipMemcpyAsync(st, g.stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
               hipStreamSynchronize(ctx->stream) == hipSuccess;
     int over = 0, invalid = 0, skipped = 0;
     for (uint8_t d : dec) {
@@ -2081,8 +2123,7 @@ static int nis_gate_locked(vib_context* ctx, Slot* s, double threshold, int appl
     }
     const bool reject = ok && apply && invalid == 0 && over > 0;
     if (reject)
-        ok = vibk_nis_apply(b.d_wins, d_dec, F, ctx->stream) == hipSuccess && hipStreamSynchronize(ctx->stream) == hipSuccess;
-    (void)hipFree(buf);
+        ok = vibk_nis_apply(g.wins, g.decision, F, ctx->stream) == hipSuccess && hipStreamSynchronize(ctx->stream) == hipSuccess;
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, "gate launch failed");
     auto fill = [](vib_residual_stats& o, const double* v) {
         o.factors = v[0]; o.components = v[1]; o.squared_norm = v[2]; o.over_gate = v[3]; o.max_nis = v[4];
@@ -2110,6 +2151,28 @@ static int nis_gate_locked(vib_context* ctx, Slot* s, double threshold, int appl
                    : VIB_OK;
 }
 
+static int nis_gate_locked(vib_context* ctx, Slot* s, double threshold, int apply, vib_gate_report* rep, uint8_t* dec_out) {
+    *rep = vib_gate_report{};
+    rep->evaluated.worst_factor = rep->admitted.worst_factor = -1;
+    const int F = s->host.F;
+    if (F == 0) return VIB_OK;
+    hipSetDevice(ctx->device);
+    std::vector<Slot*> one{s};
+    Batch b;
+    std::string err;
+    if (!b.build(one, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    char* buf = nullptr;
+    if (hipMalloc(&buf, (size_t)F * 9 + kGateStats * sizeof(double)) != hipSuccess)
+        return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "gate allocation failed");
+    double* d_energy = reinterpret_cast<double*>(buf);
+    double* d_stats = d_energy + F;
+    uint8_t* d_dec = reinterpret_cast<uint8_t*>(d_stats + kGateStats);
+    const int st = nis_gate_core(ctx, s, threshold, apply, rep, dec_out,
+                                 GateBuffers{b.d_wins, b.d_chunks, b.nchunks, d_energy, d_dec, d_stats});
+    (void)hipFree(buf);
+    return st;
+}
+
 }  // namespace
 
 int vib_nis_gate(vib_context* ctx, int32_t slot, double threshold, int32_t apply, vib_gate_report* report,
@@ -2119,6 +2182,36 @@ int vib_nis_gate(vib_context* ctx, int32_t slot, double threshold, int32_t apply
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
     return nis_gate_locked(ctx, s, threshold, apply, report, decisions);
+}
+
+int vib_nis_gate_static(vib_context* ctx, int32_t slot, double threshold, int32_t apply, vib_gate_report* report,
+                        uint8_t* decisions) {
+    if (!ctx || !report || !(threshold > 0)) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    *report = vib_gate_report{};
+    report->evaluated.worst_factor = report->admitted.worst_factor = -1;
+    const HostWindow& h = s->host;
+    if (h.F == 0) return VIB_OK;
+    if ((size_t)h.F > kStaticGateFactors || h.chunks.size() > kStaticGateChunks)
+        return fail(ctx, VIB_ERR_UNSUPPORTED_DIMENSIONS,
+                    "window beyond the no-allocation gate's scratch (2^20 factors, 2^17 landmark chunks)");
+    hipSetDevice(ctx->device);
+    char* base = reinterpret_cast<char*>(ctx->scratch);  // device address of `scratch`, never the name
+    WinDev wd = s->dev;
+    wd.nchunks = (int)h.chunks.size();
+    wd.hostH = wd.hostG = nullptr;
+    const int nch = (int)h.chunks.size();
+    if (hipMemcpyAsync(base, &wd, sizeof(WinDev), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        (nch && hipMemcpyAsync(base + kScratchChunkOffset, h.chunks.data(), nch * sizeof(Chunk), hipMemcpyHostToDevice,
+                               ctx->stream) != hipSuccess))
+        return fail(ctx, VIB_ERR_DEVICE, "gate descriptor copy failed");
+    const GateBuffers g{reinterpret_cast<const WinDev*>(base), reinterpret_cast<const Chunk*>(base + kScratchChunkOffset),
+                        nch, reinterpret_cast<double*>(base + kScratchEnergyOffset),
+                        reinterpret_cast<uint8_t*>(base + kScratchDecisionOffset),
+                        reinterpret_cast<double*>(base + kScratchStatsOffset)};
+    return nis_gate_core(ctx, s, threshold, apply, report, decisions, g);
 }
 
 int vib_solve_refit(vib_context* ctx, int32_t slot, const vib_solve_options* o, const vib_refit_options* r,

@@ -364,6 +364,8 @@ def load_library(path=LIB_PATH):
         "vib_copy_probe": ([vp, C.c_int64], C.c_int),
         "vib_preintegrate": ([dp, C.c_int32, C.c_double, C.c_double, dp, C.POINTER(VibImuParams), dp, dp, dp], C.c_int),
         "vib_nis_gate": ([vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(VibGateReport), C.POINTER(C.c_uint8)], C.c_int),
+        "vib_nis_gate_static": ([vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(VibGateReport), C.POINTER(C.c_uint8)],
+                                C.c_int),
         "vib_solve_refit": ([vp, C.c_int32, C.POINTER(VibSolveOptions), C.POINTER(VibRefitOptions),
                              C.POINTER(VibRefitReport)], C.c_int),
         "vib_whiten_observations": ([vp, C.c_int32, dp, ip, dp, C.c_int32, dp, C.c_double, dp, dp, ip, dp], C.c_int),
@@ -621,6 +623,15 @@ class Context:
         rep = VibGateReport()
         st = self.lib.vib_nis_gate(self.h, slot, float(threshold), 1 if apply else 0, C.byref(rep),
                                    dec.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return st, rep, dec[:n]
+
+    def nis_gate_static(self, slot, threshold=VISUAL_NIS_THRESHOLD, apply=False):
+        """vib_nis_gate_static (the gate without device allocation), returned like nis_gate."""
+        n = self.shape(slot)[2]
+        dec = np.zeros(max(n, 1), np.uint8)
+        rep = VibGateReport()
+        st = self.lib.vib_nis_gate_static(self.h, slot, float(threshold), 1 if apply else 0, C.byref(rep),
+                                          dec.ctypes.data_as(C.POINTER(C.c_uint8)))
         return st, rep, dec[:n]
 
     def solve_refit(self, slot, opts, threshold=VISUAL_NIS_THRESHOLD, max_refits=3, skip_pre=False):

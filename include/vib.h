@@ -399,6 +399,16 @@ typedef struct vib_gate_report {
  * bytes, landmark order): 0 admitted, 1 over the gate, 2 invalid, 3 not participating. */
 int vib_nis_gate(vib_context* ctx, int32_t slot, double threshold, int32_t apply, vib_gate_report* report,
                  uint8_t* decisions);
+/* vib_nis_gate without device allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same kernels and results as vib_nis_gate; the
+ * per-call descriptors, energies, decisions and statistics live in one file-scope device array
+ * of the library (about 11.5 MB per GPU, held until the process ends) instead of fresh
+ * allocations. Windows beyond 2^20 factors or 2^17 landmark chunks return
+ * VIB_ERR_UNSUPPORTED_DIMENSIONS. Every call on one GPU shares that array: calls that overlap on
+ * one GPU (from several contexts or threads) race. */
+int vib_nis_gate_static(vib_context* ctx, int32_t slot, double threshold, int32_t apply, vib_gate_report* report,
+                        uint8_t* decisions);
 
 /* SolveTermination (BackendTypes.hpp:216) */
 #define VIB_TERM_FAILURE 0
