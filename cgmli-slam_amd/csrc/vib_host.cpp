@@ -45,7 +45,8 @@ hipError_t vibk_nis_apply(const vib::WinDev* wins, const uint8_t* decision, int 
 hipError_t vibk_preintegrate(int count, const double* smp, const int* begin, const int* end, const double* t0,
                              const double* t1, const double* sbl, const vib_imu_params* prm, const double* g0,
                              const double* g1, double* out, int* status, hipStream_t s);
-hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, hipStream_t s);
+hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, double* rot, hipStream_t s);
+size_t vibk_sym_eigen_log(int n);
 }
 
 namespace vib {
@@ -364,6 +365,7 @@ size_t Slot::footprint(const HostWindow& h) {
     d += m * n + m + n * n + h.prior_ref.size();
     d += 2 * nr * nr + factor_doubles((int)nr + kTile) + 8 * nr + 64;
     d += 465 * (size_t)h.E + 36 * B + n + 4 * (size_t)h.E + 4;
+    d += (sizeof(WinDev) + 256) / 8 + 1;  // the descriptor and pointer of the prior Gram launch
     const size_t nlin = h.lin_chunks.size();
     d += 2 * nch * (tri + nv) + nch * CS_PLANES + 2 * (tri + nv);
     d += nlin * (tri + nv + CS_PLANES);
@@ -456,6 +458,8 @@ int Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     d.nlin = (int)nlin;
     reduced_lin = a.take<double>(len); reduced_schur = a.take<double>(len);
     d.lm = a.take<LmState>(1);
+    WinDev* gram_win = a.take<WinDev>(1);
+    double** gram_out = a.take<double*>(1);
     if (a.used > bytes) { err = "arena overflow"; return VIB_ERR_DEVICE; }
     // every copy into the arena stays inside the carved footprint (host-side bounds check)
     const char* lo = reinterpret_cast<const char*>(arena);
@@ -490,16 +494,8 @@ int Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     if (!ok) { err = "upload failed"; return VIB_ERR_DEVICE; }
     if (h.prior_rows) {
         // G = A^T A once per prior (one-window launch)
-        WinDev* dw = nullptr;
-        double** gp = nullptr;
-        auto put = [&](void* dst, const void* src, size_t n) {
-            return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
-        };
-        ok = hipMalloc(&dw, sizeof(WinDev)) == hipSuccess && hipMalloc(&gp, sizeof(double*)) == hipSuccess &&
-             put(dw, &d, sizeof(WinDev)) && put(gp, &pG, sizeof(double*)) &&
-             vibk_prior_gram(dw, 1, gp, stream) == hipSuccess && hipStreamSynchronize(stream) == hipSuccess;
-        if (dw) (void)hipFree(dw);
-        if (gp) (void)hipFree(gp);
+        ok = up(gram_win, &d, sizeof(WinDev)) && up(gram_out, &pG, sizeof(double*)) &&
+             vibk_prior_gram(gram_win, 1, gram_out, stream) == hipSuccess && hipStreamSynchronize(stream) == hipSuccess;
         if (!ok) { err = "prior Gram matrix failed"; return VIB_ERR_DEVICE; }
     }
     if (hipStreamSynchronize(stream) != hipSuccess) { err = "upload synchronization failed"; return VIB_ERR_DEVICE; }
@@ -682,8 +678,20 @@ struct vib_context {
     void* scratch = nullptr;     // device address of this device's `scratch` array
     void* preint_buf = nullptr;  // grow-only device buffer of the preintegration launches
     size_t preint_cap = 0;
+    // Grow-only device resources of the piecewise API and the slides, so that steady-state
+    // calls allocate nothing: descriptors of one window and of a solve's window set, the
+    // eigensolver's matrices, small per-landmark buffers, and the slots of the marginal system
+    // and of the two-frame solve.
+    vib::Batch one, solve;
+    void* eig_buf = nullptr;
+    size_t eig_cap = 0;
+    void* aux_buf = nullptr;
+    size_t aux_cap = 0;
+    std::unique_ptr<vib::Slot> marginal_slot, two_frame_slot;
     ~vib_context() {
         if (preint_buf) (void)hipFree(preint_buf);
+        if (eig_buf) (void)hipFree(eig_buf);
+        if (aux_buf) (void)hipFree(aux_buf);
     }
 };
 
@@ -692,6 +700,22 @@ namespace {
 int fail(vib_context* ctx, int code, const std::string& msg) {
     if (ctx) ctx->error = msg;
     return code;
+}
+
+// A grow-only device buffer of at least `bytes` (null when the allocation fails).
+void* grow_device(void*& buf, size_t& cap, size_t bytes) {
+    if (bytes <= cap && buf) return buf;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    const size_t want = std::max(bytes, (size_t)1 << 16);
+    if (hipMalloc(&buf, want) != hipSuccess) {
+        buf = nullptr;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    cap = want;
+    return buf;
 }
 
 Slot* find_slot(vib_context* ctx, int slot) {
@@ -760,7 +784,7 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
                         "a window with host-evaluated factors is solved through vib_linearize / vib_solve_step");
     for (Slot* s : slots)
         if (!write_lm(ctx, *s, fresh_lm(o))) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
-    Batch batch;
+    Batch& batch = ctx->solve;
     batch.mixed = o.precision == VIB_MIXED;
     std::string err;
     if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
@@ -900,7 +924,8 @@ bool imu_nis(vib_context* ctx, Slot& s, const int edges[2], double nis[2]) {
     LmState lm{};
     lm.lambda = 1e-4; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1;
     std::vector<Slot*> one{&s};
-    Batch b;
+    Batch& b = ctx->one;
+    b.mixed = 0;
     std::string err;
     std::vector<double> r((size_t)15 * s.host.E);
     if (!write_lm(ctx, s, lm) || !b.build(one, ctx->stream, err) || !b.launch(1, ctx->stream, 0, 0) ||
@@ -1054,23 +1079,24 @@ WindowCopy filter_window(const WindowCopy& w, int drop, const std::vector<char>&
 // Eigendecomposition of a symmetric n x n matrix on the device (k_sym_eigen): eigenvalues
 // ascending, eigenvectors as the columns of U (row-major).
 int device_eigen(vib_context* ctx, const std::vector<double>& M, int n, std::vector<double>& ev, std::vector<double>& U) {
-    double *dA = nullptr, *dV = nullptr, *de = nullptr;
-    int* dok = nullptr;
     int conv = 0;
     const size_t bytes = 8 * (size_t)n * n;
     std::vector<double> V((size_t)n * n), e(n);
-    const bool ok = hipMalloc(&dA, bytes) == hipSuccess && hipMalloc(&dV, bytes) == hipSuccess &&
-                    hipMalloc(&de, 8 * (size_t)n) == hipSuccess && hipMalloc(&dok, sizeof(int)) == hipSuccess &&
-                    hipMemcpyAsync(dA, M.data(), bytes, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
-                    vibk_sym_eigen(dA, dV, de, n, n, dok, ctx->stream) == hipSuccess &&
+    // A | V | eigenvalues | rotation log | convergence flag in the context's grow-only eigen buffer
+    const size_t logd = vibk_sym_eigen_log(n);
+    char* buf = static_cast<char*>(grow_device(ctx->eig_buf, ctx->eig_cap, 2 * bytes + 8 * (size_t)n + 8 * logd + 256));
+    if (!buf) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "eigensolver buffer allocation failed");
+    double* dA = reinterpret_cast<double*>(buf);
+    double* dV = dA + (size_t)n * n;
+    double* de = dV + (size_t)n * n;
+    double* drot = de + n;
+    int* dok = reinterpret_cast<int*>(drot + logd);
+    const bool ok = hipMemcpyAsync(dA, M.data(), bytes, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+                    vibk_sym_eigen(dA, dV, de, n, n, dok, drot, ctx->stream) == hipSuccess &&
                     hipMemcpyAsync(V.data(), dV, bytes, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
                     hipMemcpyAsync(e.data(), de, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
                     hipMemcpyAsync(&conv, dok, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
                     hipStreamSynchronize(ctx->stream) == hipSuccess;
-    if (dA) hipFree(dA);
-    if (dV) hipFree(dV);
-    if (de) hipFree(de);
-    if (dok) hipFree(dok);
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, "device eigendecomposition failed");
     if (!conv) return fail(ctx, VIB_ERR_EIGEN_FAILED, "eigensolver did not converge");
     std::vector<int> order(n);
@@ -1184,13 +1210,15 @@ int schur_prior(vib_context* ctx, const HostWindow& cur, int victim, const std::
     const int nvict = (int)victim_cols.size(), np = (int)kept_cols.size();
     if (nvict == 0) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "no eliminated state block");
     // device: linearize the consumed factors, landmark pseudo-inverse Schur, reduced system
-    Slot tmp;
+    if (!ctx->marginal_slot) ctx->marginal_slot = std::make_unique<Slot>();
+    Slot& tmp = *ctx->marginal_slot;  // its arena is kept from slide to slide
     if (const int st = tmp.upload(m, ctx->stream, err)) return fail(ctx, st, err);
     LmState lm{};
     lm.lambda = 0; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1; lm.evaluate_only = 1;
     lm.marginal = 1;
     std::vector<Slot*> one{&tmp};
-    Batch b;
+    Batch& b = ctx->one;
+    b.mixed = 0;
     if (!write_lm(ctx, tmp, lm) || !b.build(one, ctx->stream, err) || !b.linearize(ctx->stream))
         return fail(ctx, VIB_ERR_DEVICE, "marginal linearization failed");
     std::vector<double> dvals(m.L);
@@ -1323,15 +1351,15 @@ int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
     std::vector<char> elim(cur.L, 0);  // the eliminated landmarks, selected on the device
     if (cur.L > 0) {
         std::vector<Slot*> one{&s};
-        Batch b;
+        Batch& b = ctx->one;
+        b.mixed = 0;
         std::string err;
-        uint8_t* d_elim = nullptr;
-        if (!b.build(one, ctx->stream, err) || hipMalloc(&d_elim, cur.L) != hipSuccess)
-            return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
+        uint8_t* d_elim = static_cast<uint8_t*>(grow_device(ctx->aux_buf, ctx->aux_cap, cur.L));
+        if (!d_elim) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "eliminated-set buffer allocation failed");
+        if (!b.build(one, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
         const bool ok = vibk_elim_set(b.d_wins, cur.L, victim, d_elim, ctx->stream) == hipSuccess &&
                         hipMemcpyAsync(elim.data(), d_elim, cur.L, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
                         hipStreamSynchronize(ctx->stream) == hipSuccess;
-        (void)hipFree(d_elim);
         if (!ok) return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
     }
     std::vector<int> imu_used;
@@ -1395,11 +1423,12 @@ bool reanchor_on_device(vib_context* ctx, Slot& s, int v, const std::vector<int3
     rho.assign(n, 0.0);
     if (n == 0) return true;
     std::vector<Slot*> one{&s};
-    Batch b;
+    Batch& b = ctx->one;
+    b.mixed = 0;
     std::string err;
     if (!b.build(one, ctx->stream, err)) return false;
-    char* buf = nullptr;
-    if (hipMalloc(&buf, n * (sizeof(double) + 2 * sizeof(int32_t))) != hipSuccess) return false;
+    char* buf = static_cast<char*>(grow_device(ctx->aux_buf, ctx->aux_cap, n * (sizeof(double) + 2 * sizeof(int32_t))));
+    if (!buf) return false;
     double* d_rho = reinterpret_cast<double*>(buf);
     int32_t* d_lms = reinterpret_cast<int32_t*>(d_rho + n);
     int32_t* d_chosen = d_lms + n;
@@ -1408,7 +1437,6 @@ bool reanchor_on_device(vib_context* ctx, Slot& s, int v, const std::vector<int3
                     hipMemcpyAsync(chosen.data(), d_chosen, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
                     hipMemcpyAsync(rho.data(), d_rho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
                     hipStreamSynchronize(ctx->stream) == hipSuccess;
-    (void)hipFree(buf);
     return ok;
 }
 
@@ -1780,7 +1808,7 @@ int vib_linearize(vib_context* ctx, int32_t slot, int32_t precision) {
     if (!read_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state read failed");
     lm.new_point = 1; lm.status = 0; lm.iteration = 0;
     std::vector<Slot*> one{s};
-    Batch b;
+    Batch& b = ctx->one;
     b.mixed = precision == VIB_MIXED;
     std::string err;
     if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, err) || !b.linearize(ctx->stream) ||
@@ -1803,7 +1831,7 @@ int vib_solve_step(vib_context* ctx, int32_t slot, double lambda, int32_t precis
     if (!read_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state read failed");
     lm.lambda = lambda; lm.status = 0; lm.iteration = 0;
     std::vector<Slot*> one{s};
-    Batch b;
+    Batch& b = ctx->one;
     b.mixed = precision == VIB_MIXED;
     std::string err;
     if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, err) || !b.step(ctx->stream, refinement, 1) ||
@@ -1831,7 +1859,8 @@ int vib_accept(vib_context* ctx, int32_t slot) {
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
     std::vector<Slot*> one{s};
-    Batch b;
+    Batch& b = ctx->one;
+    b.mixed = 0;
     std::string err;
     if (!b.build(one, ctx->stream, err) || !b.launch(10, ctx->stream, 0, 2) || !b.launch(11, ctx->stream, 0, 0) ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -2158,18 +2187,17 @@ static int nis_gate_locked(vib_context* ctx, Slot* s, double threshold, int appl
     if (F == 0) return VIB_OK;
     hipSetDevice(ctx->device);
     std::vector<Slot*> one{s};
-    Batch b;
+    Batch& b = ctx->one;
+    b.mixed = 0;
     std::string err;
     if (!b.build(one, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
-    char* buf = nullptr;
-    if (hipMalloc(&buf, (size_t)F * 9 + kGateStats * sizeof(double)) != hipSuccess)
-        return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "gate allocation failed");
+    char* buf = static_cast<char*>(grow_device(ctx->aux_buf, ctx->aux_cap, (size_t)F * 9 + kGateStats * sizeof(double)));
+    if (!buf) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "gate allocation failed");
     double* d_energy = reinterpret_cast<double*>(buf);
     double* d_stats = d_energy + F;
     uint8_t* d_dec = reinterpret_cast<uint8_t*>(d_stats + kGateStats);
     const int st = nis_gate_core(ctx, s, threshold, apply, rep, dec_out,
                                  GateBuffers{b.d_wins, b.d_chunks, b.nchunks, d_energy, d_dec, d_stats});
-    (void)hipFree(buf);
     return st;
 }
 
@@ -2292,8 +2320,8 @@ int vib_whiten_observations(vib_context* ctx, int32_t count, const double* pixel
     // one allocation: pixels | intrinsics | sigma | bearings | sqrt_info | camera | status
     const size_t nd = 2 * n + 6 * (size_t)num_cameras + (sigma ? n : 0) + 6 * n;
     const size_t bytes = nd * sizeof(double) + 2 * n * sizeof(int32_t);
-    char* buf = nullptr;
-    if (hipMalloc(&buf, bytes) != hipSuccess) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "whitening allocation failed");
+    char* buf = static_cast<char*>(grow_device(ctx->aux_buf, ctx->aux_cap, bytes));
+    if (!buf) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "whitening allocation failed");
     double* d_px = reinterpret_cast<double*>(buf);
     double* d_in = d_px + 2 * n;
     double* d_sg = d_in + 6 * (size_t)num_cameras;
@@ -2323,7 +2351,6 @@ int vib_whiten_observations(vib_context* ctx, int32_t count, const double* pixel
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    (void)hipFree(buf);
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, "whitening launch failed");
     for (size_t i = 0; i < n; ++i)
         if (status[i] != 0) return VIB_ERR_INVALID_ARGUMENT;
@@ -2352,7 +2379,7 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
         lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1 << 30; lm.continuous = 1;
         if (!write_lm(ctx, *s, lm)) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
     }
-    Batch batch;
+    Batch& batch = ctx->solve;
     batch.mixed = o->precision == VIB_MIXED;
     std::string err;
     if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
@@ -2417,7 +2444,8 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
     int st = latest_two_window(cur, lt, err);
     rep->support_landmarks = lt.support;
     if (st != VIB_OK) return fail(ctx, st, err);
-    Slot local;
+    if (!ctx->two_frame_slot) ctx->two_frame_slot = std::make_unique<Slot>();
+    Slot& local = *ctx->two_frame_slot;  // its arena is kept from call to call
     if (const int st = local.upload(lt.win, ctx->stream, err)) return fail(ctx, st, err);
     double nis[2];
     if (!imu_nis(ctx, local, lt.edges, nis)) return fail(ctx, VIB_ERR_DEVICE, "IMU residual evaluation failed");

@@ -1022,7 +1022,6 @@ __global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
 // grid.x = window, grid.y = slice
 // ---------------------------------------------------------------------------------------------
 constexpr int kRedEntries = 16;               // entries per workgroup (one 128-byte line per chunk)
-constexpr int kRedLanes = kBlock / kRedEntries;  // chunk lanes per entry
 
 // Entries [slice E, slice E + E) of window w's partial sum, which 0: visual block, 1: Schur
 // correction; kBlock / E chunk lanes per entry, combined in lane order.
@@ -2125,7 +2124,7 @@ __global__ __launch_bounds__(kCholThreads) void k_refine(WinDev* wins, int last)
     __shared__ double red[kCholThreads / 64];
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w) || !w.lm->chol_ok) return;
-    const int n = w.nr, t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
+    const int n = w.nr, t = threadIdx.x, nt = blockDim.x;
     const int T = (n + kTile - 1) / kTile, N = kTile * T;
     const int nv = w.nv, nsb = n - nv, len = w.tri + nv;
     auto to_solve = [&](int c) { return c < nv ? c + nsb : c - nv; };
@@ -2187,22 +2186,34 @@ __global__ __launch_bounds__(kCholThreads) void k_refine(WinDev* wins, int last)
 //      is a column norm, a matrix-vector product (one wave per row) and a rank-2 update;
 //      v is kept below the subdiagonal of A;
 //   2. Q^T = H_{n-3} ... H_0 accumulated into V;
-//   3. implicit QR sweeps on the unreduced trailing block: one lane chases the bulge and
-//      records the Givens rotations in LDS, then every thread applies them to its column of
-//      V^T (rows k, k+1 of V, coalesced); converged off-diagonals (|e| <= eps (|d_k|+|d_k+1|))
-//      deflate; at most 30 n sweeps.
+//   3. implicit QR sweeps on the unreduced trailing block: one wave runs up to 64 sweeps on the
+//      tridiagonal alone (deflation scans 64 entries at a time, one lane chases the bulge) and
+//      logs their Givens rotations (global memory), then every thread applies the logged
+//      rotations to its column of V^T (rows k, k+1 of V, coalesced); converged off-diagonals
+//      (|e| <= eps (|d_k|+|d_k+1|)) deflate; at most 30 n sweeps.
 // Serves the marginalization's pseudo-inverse of the eliminated state block and the prior's
 // eigen cut and square root (Marginalizer::psdSolve / compute, Marginalizer.cpp:240-283,
 // 430-466, whose GPU form is MarginalCudaWorkspace::eigen, VisualCudaBatch.cu:887-957).
 // ---------------------------------------------------------------------------------------------
 constexpr int kEigMax = 512;
 constexpr int kEigThreads = 1024;
-__global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok) {
-    __shared__ double d[kEigMax], e[kEigMax], vec[kEigMax], q[kEigMax], gc[kEigMax], gs[kEigMax];
-    __shared__ double red[kEigThreads / 64];
-    __shared__ int lo, hi, state;
+// NL > 0: the matrix, then the eigenvectors, are staged in the workgroup's LDS (n <= NL); every
+// step of the three phases then reads and writes LDS instead of global memory. NL == 0: both
+// stay in global memory (any n <= kEigMax).
+constexpr int kEigLogSweeps = 64;  // QR sweeps logged per round of the rotation pass
+// Doubles of the rotation log of an n x n problem (k_sym_eigen's `rot`).
+__host__ __device__ inline size_t eigen_log_doubles(int n) { return 2 * (size_t)kEigLogSweeps * n + kEigLogSweeps; }
+template <int NL>
+__device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, int* ok, double* rot, double* lds,
+                               double* d, double* e, double* vec, double* q, double* red, int& lo, int& hi, int& state) {
     const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
-    auto a = [&](int r, int c) -> double& { return A[(size_t)r * ld + c]; };
+    double* Ap = NL ? lds : A;
+    const int lda = NL ? n : ld;
+    if (NL) {
+        for (int p = t; p < n * n; p += nt) Ap[p] = A[(size_t)(p / n) * ld + p % n];
+        __syncthreads();
+    }
+    auto a = [&](int r, int c) -> double& { return Ap[(size_t)r * lda + c]; };
     // 1. tridiagonalization
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;  // length of the reflected part (rows k+1 .. n-1)
@@ -2244,85 +2255,176 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V,
         d[n - 1] = a(n - 1, n - 1);
         e[n - 1] = 0;
     }
+    if (NL) {  // the reflectors (column k, rows k+1 ..) go back to global memory: LDS takes V next
+        __syncthreads();
+        for (int p = t; p < n * n; p += nt) {
+            const int r = p / n, c = p % n;
+            if (r > c) A[(size_t)r * ld + c] = Ap[p];
+        }
+        __syncthreads();
+    }
+    double* Vp = NL ? lds : V;
+    const int ldv = NL ? n : ld;
+    auto av = [&](int r, int c) -> double { return A[(size_t)r * ld + c]; };
     // 2. Q^T = H_{n-3} ... H_0 in V: start from I, apply H_k on the right for k = n-3 .. 0
     //    (Q^T <- Q^T H_k acts on columns k+1 ..; H_0 last)
-    for (int i = t; i < n * n; i += nt) V[(size_t)(i / n) * ld + i % n] = (i / n == i % n) ? 1.0 : 0.0;
+    for (int i = t; i < n * n; i += nt) Vp[(size_t)(i / n) * ldv + i % n] = (i / n == i % n) ? 1.0 : 0.0;
     __syncthreads();
     for (int k = n - 3; k >= 0; --k) {
         const int m = n - k - 1;
-        for (int i = t; i < m; i += nt) vec[i] = a(k + 1 + i, k);
+        for (int i = t; i < m; i += nt) vec[i] = av(k + 1 + i, k);
         __syncthreads();
         // w_r = sum_i V[r][k+1+i] v_i for the rows r that H_k reaches (rows >= k+1 of Q^T stay
         // identity-like until their H is applied, but the full product is formed for clarity)
         for (int r = wave; r < n; r += nw) {
             double s = 0;
-            for (int i = lane; i < m; i += 64) s += V[(size_t)r * ld + k + 1 + i] * vec[i];
+            for (int i = lane; i < m; i += 64) s += Vp[(size_t)r * ldv + k + 1 + i] * vec[i];
             s = wave_reduce_sum(s);
             if (lane == 0) q[r] = s;
         }
         __syncthreads();
         for (int p = t; p < n * m; p += nt) {
             const int r = p / m, i = p % m;
-            V[(size_t)r * ld + k + 1 + i] -= 2 * q[r] * vec[i];
+            Vp[(size_t)r * ldv + k + 1 + i] -= 2 * q[r] * vec[i];
         }
         __syncthreads();
     }
     // 3. implicit symmetric QR on (d, e); V holds Q^T: a rotation G on (k, k+1) replaces rows
-    //    k, k+1 of Q^T by (c row_k - s row_k+1, s row_k + c row_k+1)
-    if (t == 0) { state = 0; hi = n - 1; }
+    //    k, k+1 of Q^T by (c row_k - s row_k+1, s row_k + c row_k+1). Wave 0 runs up to
+    //    kEigLogSweeps sweeps on (d, e) alone, logging each sweep's block [l, m] and rotations in
+    //    `rot`; then every thread applies the logged rotations, in order, to its column of V^T.
+    //    Repeated until every off-diagonal has deflated (at most 30 n sweeps).
+    int2* span = reinterpret_cast<int2*>(rot + 2 * (size_t)kEigLogSweeps * n);
+    if (t == 0) { state = 0; hi = n - 1; lo = 0; }
     __syncthreads();
-    for (int sweep = 0; sweep < 30 * n; ++sweep) {
-        if (t == 0) {
+    for (int done = 0; done < 30 * n;) {
+        if (wave == 0) {
             const double eps = 2.220446049250313e-16;
-            int m = hi;
-            while (m > 0 && fabs(e[m - 1]) <= eps * (fabs(d[m - 1]) + fabs(d[m]))) { e[m - 1] = 0; --m; }
-            hi = m;
-            if (m == 0) {
-                state = 1;
-            } else {
-                int l = m - 1;
-                while (l > 0 && fabs(e[l - 1]) > eps * (fabs(d[l - 1]) + fabs(d[l]))) --l;
-                lo = l;
-                // Wilkinson shift from the trailing 2x2 block
-                const double dl = 0.5 * (d[m - 1] - d[m]), em = e[m - 1];
-                const double mu = d[m] - em * em / (dl + (dl >= 0 ? 1.0 : -1.0) * sqrt(dl * dl + em * em));
-                double x = d[l] - mu, z = e[l], bulge = 0;
-                for (int k = l; k < m; ++k) {
-                    const double h = hypot(x, z);
-                    const double c = h > 0 ? x / h : 1.0, s = h > 0 ? -z / h : 0.0;
-                    if (k > l) e[k - 1] = c * x - s * z;  // (x, z) = (e[k-1], bulge)
-                    const double ak = d[k], bk = e[k], ck = d[k + 1];
-                    d[k] = c * c * ak - 2 * c * s * bk + s * s * ck;
-                    d[k + 1] = s * s * ak + 2 * c * s * bk + c * c * ck;
-                    e[k] = c * s * (ak - ck) + (c * c - s * s) * bk;
-                    if (k + 1 < m) {
-                        const double r = e[k + 1];
-                        bulge = -s * r;
-                        e[k + 1] = c * r;
-                        x = e[k];
-                        z = bulge;
-                    }
-                    gc[k] = c;
-                    gs[k] = s;
+            auto small = [&](int i) { return fabs(e[i]) <= eps * (fabs(d[i]) + fabs(d[i + 1])); };
+            int count = 0;
+            while (count < kEigLogSweeps && done + count < 30 * n) {
+                // deflation at the bottom: the trailing off-diagonals that are small are set to
+                // zero; m = the last row of the unreduced block (64 entries per wave step)
+                int m = hi;
+                while (m > 0) {
+                    const int i = m - 1 - lane;
+                    const bool keep = i >= 0 && !small(i);
+                    const unsigned long long b = __ballot(keep);
+                    const int top = b ? m - (__ffsll(b) - 1) : m - 64;  // rows > top - 1 deflated
+                    if (i >= 0 && i >= top) e[i] = 0;
+                    m = b ? top : (m > 64 ? m - 64 : 0);
+                    if (b) break;
                 }
+                if (lane == 0) hi = m;
+                if (m == 0) {
+                    if (lane == 0) state = 1;
+                    break;
+                }
+                // l = first row of the block: the nearest small off-diagonal above m ends it
+                int l = 0;
+                for (int base = m - 2; base >= 0; base -= 64) {
+                    const int i = base - lane;
+                    const unsigned long long b = __ballot(i >= 0 && small(i));
+                    if (b) {
+                        l = base - (__ffsll(b) - 1) + 1;
+                        break;
+                    }
+                }
+                wave_sync();
+                if (lane == 0) {
+                    // Wilkinson shift from the trailing 2x2 block
+                    const double dl = 0.5 * (d[m - 1] - d[m]), em = e[m - 1];
+                    const double mu = d[m] - em * em / (dl + (dl >= 0 ? 1.0 : -1.0) * sqrt(dl * dl + em * em));
+                    // the bulge chase; the running diagonal and off-diagonal entries stay in
+                    // registers (ak = d[k], bk = e[k] as updated by the previous rotation), so a
+                    // step loads only the untouched d[k+1], e[k+1] (e[m] is read and ignored)
+                    double* cs = rot + 2 * (size_t)count * n;
+                    double ak = d[l], bk = e[l];
+                    double x = ak - mu, z = bk;
+                    for (int k = l; k < m; ++k) {
+                        const double ck = d[k + 1];
+                        const double en = e[k + 1];
+                        const double rn = k + 1 < m ? en : 0.0;
+                        const double h2 = x * x + z * z;
+                        // 1 / sqrt(h2): the hardware estimate and two Newton steps, relative error
+                        // a few ulp (h2 in the normal range), else the correctly rounded form
+                        double ih;
+                        if (h2 > 1e-280 && h2 < 1e280) {
+                            double y = __builtin_amdgcn_rsq(h2);
+                            double r = fma(-h2 * y, y, 1.0);
+                            y = fma(0.5 * y, r, y);
+                            r = fma(-h2 * y, y, 1.0);
+                            ih = fma(0.5 * y, r, y);
+                        } else {
+                            ih = h2 > 0 ? 1.0 / sqrt(h2) : 0.0;
+                        }
+                        const double c = h2 > 0 ? x * ih : 1.0, sn = h2 > 0 ? -z * ih : 0.0;
+                        if (k > l) e[k - 1] = c * x - sn * z;  // (x, z) = (e[k-1], bulge)
+                        const double dk = c * c * ak - 2 * c * sn * bk + sn * sn * ck;
+                        const double dk1 = sn * sn * ak + 2 * c * sn * bk + c * c * ck;
+                        const double ek = c * sn * (ak - ck) + (c * c - sn * sn) * bk;
+                        d[k] = dk;
+                        cs[2 * k] = c;
+                        cs[2 * k + 1] = sn;
+                        x = ek;
+                        z = -sn * rn;  // the bulge
+                        ak = dk1;
+                        bk = c * rn;
+                    }
+                    d[m] = ak;
+                    e[m - 1] = x;
+                    span[count] = make_int2(l, m);
+                }
+                wave_sync();
+                ++count;
             }
+            if (lane == 0) lo = count;  // sweeps logged in this round
         }
         __syncthreads();
-        if (state) break;
-        const int l = lo, m = hi;
-        for (int r = t; r < n; r += nt) {
-            double x = V[(size_t)l * ld + r];
-            for (int k = l; k < m; ++k) {
-                const double y = V[(size_t)(k + 1) * ld + r];
-                V[(size_t)k * ld + r] = gc[k] * x - gs[k] * y;
-                x = gs[k] * x + gc[k] * y;
+        const int count = lo;
+        for (int r = t; r < n; r += nt)
+            for (int sw = 0; sw < count; ++sw) {
+                const int2 lm = span[sw];
+                const double* cs = rot + 2 * (size_t)sw * n;
+                double x = Vp[(size_t)lm.x * ldv + r];
+                for (int k = lm.x; k < lm.y; ++k) {
+                    const double y = Vp[(size_t)(k + 1) * ldv + r];
+                    const double c = cs[2 * k], sn = cs[2 * k + 1];
+                    Vp[(size_t)k * ldv + r] = c * x - sn * y;
+                    x = sn * x + c * y;
+                }
+                Vp[(size_t)lm.y * ldv + r] = x;
             }
-            V[(size_t)m * ld + r] = x;
-        }
+        __syncthreads();
+        done += count;
+        if (state) break;
         __syncthreads();
     }
+    if (NL)
+        for (int p = t; p < n * n; p += nt) V[(size_t)(p / n) * ld + p % n] = Vp[p];
     for (int i = t; i < n; i += nt) ev[i] = d[i];
     if (t == 0) *ok = state;
+}
+
+__global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok,
+                                                           double* rot) {
+    __shared__ double d[kEigMax], e[kEigMax], vec[kEigMax], q[kEigMax];
+    __shared__ double red[kEigThreads / 64];
+    __shared__ int lo, hi, state;
+    sym_eigen_body<0>(A, V, ev, n, ld, ok, rot, nullptr, d, e, vec, q, red, lo, hi, state);
+}
+
+// Matrices of up to kEigLds rows (every prior of a window of at most 20 frames: 19 poses, one
+// speed/bias block, rig and time offset give 130) in LDS: 148 KB for the matrix, 6.5 KB of
+// vectors.
+constexpr int kEigLds = 136;
+__global__ __launch_bounds__(kEigThreads) void k_sym_eigen_lds(double* A, double* V, double* ev, int n, int ld, int* ok,
+                                                               double* rot) {
+    __shared__ double lds[kEigLds * kEigLds];
+    __shared__ double d[kEigLds], e[kEigLds], vec[kEigLds], q[kEigLds];
+    __shared__ double red[kEigThreads / 64];
+    __shared__ int lo, hi, state;
+    sym_eigen_body<kEigLds>(A, V, ev, n, ld, ok, rot, lds, d, e, vec, q, red, lo, hi, state);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3457,9 +3559,13 @@ hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, hipStream_t s) {
-    if (n < 1 || n > vib::kEigMax || ld < n) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(vib::k_sym_eigen, dim3(1), dim3(vib::kEigThreads), 0, s, A, V, ev, n, ld, ok);
+size_t vibk_sym_eigen_log(int n) { return vib::eigen_log_doubles(n); }
+
+hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, double* rot, hipStream_t s) {
+    if (n < 1 || n > vib::kEigMax || ld < n || !rot) return hipErrorInvalidValue;
+    if (n <= vib::kEigLds)
+        hipLaunchKernelGGL(vib::k_sym_eigen_lds, dim3(1), dim3(vib::kEigThreads), 0, s, A, V, ev, n, ld, ok, rot);
+    else hipLaunchKernelGGL(vib::k_sym_eigen, dim3(1), dim3(vib::kEigThreads), 0, s, A, V, ev, n, ld, ok, rot);
     return hipGetLastError();
 }
 
