@@ -1416,6 +1416,13 @@ __device__ __forceinline__ double row_bcast(double v) {
     return __builtin_amdgcn_mov_dpp(v, 0x150 + Q, 0xF, 0xF, false);
 }
 
+// Sum over the 64 lanes, the same bits in every lane: row sums by DPP, then the four rows'
+// sums read from lanes 0, 16, 32, 48 and added in that order.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v = row16_sum(v);
+    return (lane_bcast(v, 0) + lane_bcast(v, 16)) + (lane_bcast(v, 32) + lane_bcast(v, 48));
+}
+
 // 1 / sqrt(d) from the hardware estimate and two Newton steps.
 __device__ __forceinline__ double rsqrt_nr(double d) {
     double q = __builtin_amdgcn_rsq(d);
@@ -2183,8 +2190,8 @@ __global__ __launch_bounds__(kCholThreads) void k_refine(WinDev* wins, int last)
 // ev unsorted. Three phases (Golub & Van Loan, Householder tridiagonalization and the symmetric
 // QR step with implicit Wilkinson shift):
 //   1. n-2 Householder reflections H_k = I - 2 v v^T reduce A to tridiagonal (d, e); each step
-//      is a column norm, a matrix-vector product (one wave per row) and a rank-2 update;
-//      v is kept below the subdiagonal of A;
+//      is a column norm, a matrix-vector product (one wave per row) and a rank-2 update,
+//      two barriers; v is kept in row k of A's upper triangle;
 //   2. Q^T = H_{n-3} ... H_0 accumulated into V;
 //   3. implicit QR sweeps on the unreduced trailing block: one wave runs up to 64 sweeps on the
 //      tridiagonal alone (deflation scans 64 entries at a time, one lane chases the bulge) and
@@ -2205,7 +2212,7 @@ constexpr int kEigLogSweeps = 64;  // QR sweeps logged per round of the rotation
 __host__ __device__ inline size_t eigen_log_doubles(int n) { return 2 * (size_t)kEigLogSweeps * n + kEigLogSweeps; }
 template <int NL>
 __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, int* ok, double* rot, double* lds,
-                               double* d, double* e, double* vec, double* q, double* red, int& lo, int& hi, int& state) {
+                               double* d, double* e, double* q, int& lo, int& hi, int& state) {
     const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
     double* Ap = NL ? lds : A;
     const int lda = NL ? n : ld;
@@ -2214,78 +2221,78 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
         __syncthreads();
     }
     auto a = [&](int r, int c) -> double& { return Ap[(size_t)r * lda + c]; };
-    // 1. tridiagonalization
+    // 1. tridiagonalization. Every wave forms the reflector's scalars itself from column k (no
+    //    barrier); v_i = (a(k+1+i, k) - alpha [i == 0]) / |x - alpha e1| is read from column k on
+    //    the fly. Row k's upper part (dead after step k) receives v for phase 2.
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;  // length of the reflected part (rows k+1 .. n-1)
         double ss = 0;
-        for (int i = t; i < m; i += nt) { const double x = a(k + 1 + i, k); ss += x * x; }
-        ss = block_sum_waves(ss, red);
+        for (int i = lane; i < m; i += 64) { const double x = a(k + 1 + i, k); ss += x * x; }
+        ss = wave_sum_dpp(ss);
         const double x0 = a(k + 1, k), nx = sqrt(ss);
         const double alpha = x0 > 0 ? -nx : nx;
-        // v = (x - alpha e1) / |x - alpha e1|;  |x - alpha e1|^2 = 2 (ss - alpha x0)
+        // |x - alpha e1|^2 = 2 (ss - alpha x0)
         const double vn2 = 2 * (ss - alpha * x0);
         const double iv = vn2 > 0 ? 1 / sqrt(vn2) : 0.0;
-        for (int i = t; i < m; i += nt) vec[i] = (a(k + 1 + i, k) - (i == 0 ? alpha : 0.0)) * iv;
-        __syncthreads();
-        // p = A22 v (one wave per row)
-        for (int i = wave; i < m; i += nw) {
+        auto v = [&](int i) { return (a(k + 1 + i, k) - (i == 0 ? alpha : 0.0)) * iv; };
+        // p = A22 v: thread i runs down column i of the symmetric A22 (the lanes of a wave read
+        // consecutive entries of one row)
+        for (int i = t; i < m; i += nt) {
             double s = 0;
-            for (int j = lane; j < m; j += 64) s += a(k + 1 + i, k + 1 + j) * vec[j];
-            s = wave_reduce_sum(s);
-            if (lane == 0) q[i] = s;
+#pragma unroll 8
+            for (int j = 0; j < m; ++j) s += a(k + 1 + j, k + 1 + i) * v(j);
+            q[i] = s;
         }
         __syncthreads();
         double kk = 0;
-        for (int i = t; i < m; i += nt) kk += vec[i] * q[i];
-        kk = block_sum_waves(kk, red);
-        for (int i = t; i < m; i += nt) q[i] -= kk * vec[i];
-        __syncthreads();
-        // A22 -= 2 (v q^T + q v^T); column k keeps v below the subdiagonal
+        for (int i = lane; i < m; i += 64) kk += v(i) * q[i];
+        kk = wave_sum_dpp(kk);
+        // A22 -= 2 (v w^T + w v^T), w = p - (v^T p) v
         for (int p = t; p < m * m; p += nt) {
             const int i = p / m, j = p % m;
-            a(k + 1 + i, k + 1 + j) -= 2 * (vec[i] * q[j] + q[i] * vec[j]);
+            const double vi = v(i), vj = v(j);
+            a(k + 1 + i, k + 1 + j) -= 2 * (vi * (q[j] - kk * vj) + (q[i] - kk * vi) * vj);
         }
+        __syncthreads();
+        // no reader of column k or row k is left in this step or the next ones
         if (t == 0) { d[k] = a(k, k); e[k] = vn2 > 0 ? alpha : x0; }
-        __syncthreads();
-        for (int i = t; i < m; i += nt) a(k + 1 + i, k) = vn2 > 0 ? vec[i] : 0.0;
-        __syncthreads();
+        for (int i = t; i < m; i += nt) a(k, k + 1 + i) = v(i);
     }
     if (t == 0) {
         if (n >= 2) { d[n - 2] = a(n - 2, n - 2); e[n - 2] = a(n - 1, n - 2); }
         d[n - 1] = a(n - 1, n - 1);
         e[n - 1] = 0;
     }
-    if (NL) {  // the reflectors (column k, rows k+1 ..) go back to global memory: LDS takes V next
-        __syncthreads();
+    __syncthreads();
+    if (NL) {  // the reflectors (row k, columns k+1 ..) go back to global memory: LDS takes V next
         for (int p = t; p < n * n; p += nt) {
             const int r = p / n, c = p % n;
-            if (r > c) A[(size_t)r * ld + c] = Ap[p];
+            if (c > r) A[(size_t)r * ld + c] = Ap[p];
         }
         __syncthreads();
     }
     double* Vp = NL ? lds : V;
     const int ldv = NL ? n : ld;
-    auto av = [&](int r, int c) -> double { return A[(size_t)r * ld + c]; };
     // 2. Q^T = H_{n-3} ... H_0 in V: start from I, apply H_k on the right for k = n-3 .. 0
-    //    (Q^T <- Q^T H_k acts on columns k+1 ..; H_0 last)
+    //    (Q^T <- Q^T H_k acts on columns k+1 ..; H_0 last). Before H_k the product acts on rows
+    //    and columns >= k+2 only, so rows <= k have zeros in columns > k and H_k changes rows
+    //    k+1 .. n-1 alone.
     for (int i = t; i < n * n; i += nt) Vp[(size_t)(i / n) * ldv + i % n] = (i / n == i % n) ? 1.0 : 0.0;
     __syncthreads();
     for (int k = n - 3; k >= 0; --k) {
         const int m = n - k - 1;
-        for (int i = t; i < m; i += nt) vec[i] = av(k + 1 + i, k);
-        __syncthreads();
-        // w_r = sum_i V[r][k+1+i] v_i for the rows r that H_k reaches (rows >= k+1 of Q^T stay
-        // identity-like until their H is applied, but the full product is formed for clarity)
-        for (int r = wave; r < n; r += nw) {
+        const double* vk = A + (size_t)k * ld + k + 1;  // the reflector of step k
+        for (int rr = wave; rr < m; rr += nw) {
+            const double* row = Vp + (size_t)(k + 1 + rr) * ldv + k + 1;
             double s = 0;
-            for (int i = lane; i < m; i += 64) s += Vp[(size_t)r * ldv + k + 1 + i] * vec[i];
-            s = wave_reduce_sum(s);
-            if (lane == 0) q[r] = s;
+            for (int i = lane; i < m; i += 64) s += row[i] * vk[i];
+            s = wave_sum_dpp(s);
+            if (lane == 0) q[rr] = s;
         }
         __syncthreads();
-        for (int p = t; p < n * m; p += nt) {
-            const int r = p / m, i = p % m;
-            Vp[(size_t)r * ldv + k + 1 + i] -= 2 * q[r] * vec[i];
+        for (int p = t; p < m * m; p += nt) {
+            const int rr = p / m, i = p % m;
+            Vp[(size_t)(k + 1 + rr) * ldv + k + 1 + i] -= 2 * q[rr] * vk[i];
         }
         __syncthreads();
     }
@@ -2348,16 +2355,8 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
                         const double h2 = x * x + z * z;
                         // 1 / sqrt(h2): the hardware estimate and two Newton steps, relative error
                         // a few ulp (h2 in the normal range), else the correctly rounded form
-                        double ih;
-                        if (h2 > 1e-280 && h2 < 1e280) {
-                            double y = __builtin_amdgcn_rsq(h2);
-                            double r = fma(-h2 * y, y, 1.0);
-                            y = fma(0.5 * y, r, y);
-                            r = fma(-h2 * y, y, 1.0);
-                            ih = fma(0.5 * y, r, y);
-                        } else {
-                            ih = h2 > 0 ? 1.0 / sqrt(h2) : 0.0;
-                        }
+                        const double ih = h2 > 1e-280 && h2 < 1e280 ? rsqrt_nr(h2)
+                                                                     : (h2 > 0 ? 1.0 / sqrt(h2) : 0.0);
                         const double c = h2 > 0 ? x * ih : 1.0, sn = h2 > 0 ? -z * ih : 0.0;
                         if (k > l) e[k - 1] = c * x - sn * z;  // (x, z) = (e[k-1], bulge)
                         const double dk = c * c * ak - 2 * c * sn * bk + sn * sn * ck;
@@ -2387,7 +2386,24 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
                 const int2 lm = span[sw];
                 const double* cs = rot + 2 * (size_t)sw * n;
                 double x = Vp[(size_t)lm.x * ldv + r];
-                for (int k = lm.x; k < lm.y; ++k) {
+                int k = lm.x;
+                // groups of 8 rotations: rows k+1 .. k+8 still hold their values from before
+                // this sweep, so their loads (and the 8 rotations') are issued together
+                for (; k + 8 <= lm.y; k += 8) {
+                    double y[8], c[8], sn[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        y[j] = Vp[(size_t)(k + 1 + j) * ldv + r];
+                        c[j] = cs[2 * (k + j)];
+                        sn[j] = cs[2 * (k + j) + 1];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        Vp[(size_t)(k + j) * ldv + r] = c[j] * x - sn[j] * y[j];
+                        x = sn[j] * x + c[j] * y[j];
+                    }
+                }
+                for (; k < lm.y; ++k) {
                     const double y = Vp[(size_t)(k + 1) * ldv + r];
                     const double c = cs[2 * k], sn = cs[2 * k + 1];
                     Vp[(size_t)k * ldv + r] = c * x - sn * y;
@@ -2408,23 +2424,21 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
 
 __global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok,
                                                            double* rot) {
-    __shared__ double d[kEigMax], e[kEigMax], vec[kEigMax], q[kEigMax];
-    __shared__ double red[kEigThreads / 64];
+    __shared__ double d[kEigMax], e[kEigMax], q[kEigMax];
     __shared__ int lo, hi, state;
-    sym_eigen_body<0>(A, V, ev, n, ld, ok, rot, nullptr, d, e, vec, q, red, lo, hi, state);
+    sym_eigen_body<0>(A, V, ev, n, ld, ok, rot, nullptr, d, e, q, lo, hi, state);
 }
 
 // Matrices of up to kEigLds rows (every prior of a window of at most 20 frames: 19 poses, one
-// speed/bias block, rig and time offset give 130) in LDS: 148 KB for the matrix, 6.5 KB of
+// speed/bias block, rig and time offset give 130) in LDS: 148 KB for the matrix, 3.3 KB of
 // vectors.
 constexpr int kEigLds = 136;
 __global__ __launch_bounds__(kEigThreads) void k_sym_eigen_lds(double* A, double* V, double* ev, int n, int ld, int* ok,
                                                                double* rot) {
     __shared__ double lds[kEigLds * kEigLds];
-    __shared__ double d[kEigLds], e[kEigLds], vec[kEigLds], q[kEigLds];
-    __shared__ double red[kEigThreads / 64];
+    __shared__ double d[kEigLds], e[kEigLds], q[kEigLds];
     __shared__ int lo, hi, state;
-    sym_eigen_body<kEigLds>(A, V, ev, n, ld, ok, rot, lds, d, e, vec, q, red, lo, hi, state);
+    sym_eigen_body<kEigLds>(A, V, ev, n, ld, ok, rot, lds, d, e, q, lo, hi, state);
 }
 
 // ---------------------------------------------------------------------------------------------
