@@ -302,8 +302,9 @@ def slide_line(ctx, abi, win, reps=3):
                 raise RuntimeError(f"slide {name} failed: {st}")
         out[name + "_ms"] = float(np.median(ms))
         out[name + "_eliminated_landmarks"] = int(rep.eliminated_landmarks)
-    out["workload"] = ("the timed window uploaded afresh, then one slide; wall time of vib_marginalize (state "
-                       "download, host topology edits, marginal system on the device, re-upload)")
+    out["workload"] = ("the timed window uploaded afresh, then one slide; wall time of vib_marginalize (topology "
+                       "selection, marginal system and the slid window as device edits of the resident window, "
+                       "the prior's eigen square root on the device; frame-level state and prior through the host)")
     return out
 
 

@@ -177,4 +177,18 @@ struct Chunk {
 
 constexpr int kLinFactors = 512;  // factors per visual-normal workgroup (pair order)
 
+// One window edit (vib_edit.hip): which landmarks and observations of a resident window the
+// edited window keeps, and how its frames are renumbered.
+struct EditSpec {
+    int32_t renum[kMaxFrames];  // new slot of each source frame; -1: the frame is removed
+    int32_t lm_mode;            // inverse ranges of kept landmarks: 0 as before, 1 all free, 2 all constant
+    int32_t nr;                 // retained dimension of the edited window (free rho columns follow it)
+    int32_t keep_lm_invert;     // 1: keep_lm marks the landmarks removed instead
+    int32_t pad;
+    const uint8_t* keep_lm;     // L keep flags (null: every landmark)
+    const uint8_t* keep_f;      // F keep flags in landmark order (null: every observation of a kept landmark)
+    const int32_t* over_f;      // L (null: none): >= 0 re-anchors the landmark in that observation
+    const double* over_rho;     // L: the inverse range in the new anchor
+};
+
 }  // namespace vib

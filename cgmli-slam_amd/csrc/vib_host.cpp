@@ -34,8 +34,27 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
 hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t vibk_elim_set(const vib::WinDev* wins, int L, int victim, uint8_t* elim, hipStream_t s);
-hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, int count, int32_t* chosen,
-                         double* new_rho, hipStream_t s);
+hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, const uint8_t* cls, int count,
+                         int32_t* chosen, double* new_rho, hipStream_t s);
+int vibk_edit_scan_tiles(int n);
+int vibk_edit_sort_blocks(int n);
+int vibk_edit_pair_keys(void);
+hipError_t vibk_edit_scan(const void* in, int n, void* tile_sum, void* out, int wide, hipStream_t s);
+hipError_t vibk_edit_mark(const vib::WinDev* src, int L, const void* spec, unsigned long long* lmv, uint32_t* fv,
+                          int32_t* lm_of, int* bad, hipStream_t s);
+hipError_t vibk_edit_gather(const vib::WinDev* src, const vib::WinDev* dst, int Ls, int Fs, int Fd, const void* spec,
+                            const unsigned long long* lscan, const uint32_t* fscan, const int32_t* lm_of, int32_t* lmn,
+                            hipStream_t s);
+hipError_t vibk_edit_pair_hist(const vib::WinDev* dst, int F, const int32_t* lmn, uint32_t* counts, hipStream_t s);
+hipError_t vibk_edit_pair_scatter(const vib::WinDev* dst, int F, const int32_t* lmn, const uint32_t* offs, hipStream_t s);
+hipError_t vibk_strip_select(const vib::WinDev* src, int L, uint8_t* keep_lm, uint8_t* keep_f, int* out, hipStream_t s);
+hipError_t vibk_second_new_classify(const vib::WinDev* src, int L, int v, uint8_t* cls, hipStream_t s);
+hipError_t vibk_second_new_flags(const vib::WinDev* src, int L, int v, const uint8_t* cls, const int32_t* chosen,
+                                 const double* chosen_rho, uint8_t* keep_lm, uint8_t* keep_f, int32_t* over_f,
+                                 double* over_rho, uint8_t* marg_lm, int* counts, hipStream_t s);
+hipError_t vibk_touch_frames(const vib::WinDev* src, int L, const uint8_t* sel, int* out, hipStream_t s);
+hipError_t vibk_two_frame_select(const vib::WinDev* src, int L, int prev, int last, uint8_t* keep_lm, uint8_t* keep_f,
+                                 int* out, hipStream_t s);
 hipError_t vibk_whiten(int count, const double* px, const int32_t* cam, const double* intr, int ncam,
                        const double* sigma, double sigma_all, double* bearing, double* sqrt_info, int32_t* status,
                        hipStream_t s);
@@ -217,6 +236,41 @@ int HostWindow::build(const vib_window& w, std::string& err) {
 }
 
 bool HostWindow::layout(std::string& err) {
+    if (!layout_states(err)) return false;
+    lm_col.assign(L, -1);
+    ne = 0;
+    for (int l = 0; l < L; ++l) if (lm_free[l]) lm_col[l] = nr + ne++;
+    make_chunks();
+    // factors in (anchor frame, observer frame) pair order (stable counting sort)
+    std::vector<int> start(kMaxFrames * kMaxFrames + 1, 0), order(F);
+    auto pair_key = [&](int f) { return lm_frame[ob_lm[f]] * kMaxFrames + (ob_fc[f] >> 8); };
+    for (int f = 0; f < F; ++f) start[pair_key(f) + 1]++;
+    for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
+    for (int f = 0; f < F; ++f) order[start[pair_key(f)]++] = f;
+    pf_ob.assign((size_t)OB_PLANES * F, 0);
+    pf_an.assign((size_t)LM_PLANES * F, 0);
+    pf_ids.assign(F, 0); pf_lm.assign(F, 0); pf_pos.assign(F, 0);
+    for (int p = 0; p < F; ++p) {
+        const int f = order[p], l = ob_lm[f];
+        for (int q = 0; q < OB_PLANES; ++q) pf_ob[(size_t)q * F + p] = ob[(size_t)q * F + f];
+        for (int q = 0; q < LM_PLANES; ++q) pf_an[(size_t)q * F + p] = lmd[(size_t)q * L + l];
+        pf_ids[p] = (lm_frame[l] << 24) | (lm_cam[l] << 16) | ob_fc[f];
+        pf_lm[p] = l;
+        pf_pos[f] = p;
+    }
+    bulk = true;
+    return true;
+}
+
+void HostWindow::make_chunks() {
+    chunks.clear();
+    const int cs = std::max(1, chunk_size);
+    for (int l0 = 0, i = 0; l0 < L; l0 += cs, ++i) chunks.push_back(Chunk{0, i, l0, std::min(L, l0 + cs)});
+    lin_chunks.clear();
+    for (int p0 = 0, i = 0; p0 < F; p0 += kLinFactors, ++i) lin_chunks.push_back(Chunk{0, i, p0, std::min(F, p0 + kLinFactors)});
+}
+
+bool HostWindow::layout_states(std::string& err) {
     int c = 0;
     pose_col.assign(K, -1); sb_col.assign(K, -1);
     for (int k = 0; k < K; ++k) if (pose_free[k]) { pose_col[k] = c; c += 6; }
@@ -227,9 +281,6 @@ bool HostWindow::layout(std::string& err) {
     nv = c;
     for (int k = 0; k < K; ++k) if (sb_width[k]) { sb_col[k] = c; c += sb_width[k]; }
     nr = c;
-    lm_col.assign(L, -1);
-    ne = 0;
-    for (int l = 0; l < L; ++l) if (lm_free[l]) lm_col[l] = nr + ne++;
     if (nr <= 0) { err = "no retained state to solve"; return false; }
     // prior blocks
     pblk.clear(); pfree.clear();
@@ -311,29 +362,6 @@ bool HostWindow::layout(std::string& err) {
             for (int i = k; i < T; ++i)
                 if (S[(size_t)i * T + k]) chol_mask[k] |= 1u << i;
     }
-    // chunks
-    chunks.clear();
-    const int cs = std::max(1, chunk_size);
-    for (int l0 = 0, i = 0; l0 < L; l0 += cs, ++i) chunks.push_back(Chunk{0, i, l0, std::min(L, l0 + cs)});
-    // factors in (anchor frame, observer frame) pair order (stable counting sort)
-    std::vector<int> start(kMaxFrames * kMaxFrames + 1, 0), order(F);
-    auto pair_key = [&](int f) { return lm_frame[ob_lm[f]] * kMaxFrames + (ob_fc[f] >> 8); };
-    for (int f = 0; f < F; ++f) start[pair_key(f) + 1]++;
-    for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
-    for (int f = 0; f < F; ++f) order[start[pair_key(f)]++] = f;
-    pf_ob.assign((size_t)OB_PLANES * F, 0);
-    pf_an.assign((size_t)LM_PLANES * F, 0);
-    pf_ids.assign(F, 0); pf_lm.assign(F, 0); pf_pos.assign(F, 0);
-    for (int p = 0; p < F; ++p) {
-        const int f = order[p], l = ob_lm[f];
-        for (int q = 0; q < OB_PLANES; ++q) pf_ob[(size_t)q * F + p] = ob[(size_t)q * F + f];
-        for (int q = 0; q < LM_PLANES; ++q) pf_an[(size_t)q * F + p] = lmd[(size_t)q * L + l];
-        pf_ids[p] = (lm_frame[l] << 24) | (lm_cam[l] << 16) | ob_fc[f];
-        pf_lm[p] = l;
-        pf_pos[f] = p;
-    }
-    lin_chunks.clear();
-    for (int p0 = 0, i = 0; p0 < F; p0 += kLinFactors, ++i) lin_chunks.push_back(Chunk{0, i, p0, std::min(F, p0 + kLinFactors)});
     return true;
 }
 
@@ -374,6 +402,119 @@ size_t Slot::footprint(const HostWindow& h) {
     return 8 * d + 4 * i + sizeof(PriorBlockDev) * B + sizeof(LmState) + 64 * 256;
 }
 
+bool Slot::carve(const HostWindow& h, char* base, size_t bytes, WinDev& d, SlotCarve& x) {
+    Arena a{base, bytes, 0};
+    d = WinDev{};
+    d.K = h.K; d.C = h.C; d.L = h.L; d.F = h.F; d.E = h.E;
+    d.nv = h.nv; d.nr = h.nr; d.ne = h.ne; d.rig_col = h.rig_col; d.td_col = h.td_col; d.flags = h.flags;
+    d.tri = h.nv * (h.nv + 1) / 2;
+    d.huber = h.huber; d.min_rho = h.min_rho; d.td_lo = h.td_lo; d.td_hi = h.td_hi;
+    for (int k = 0; k < kMaxTiles; ++k) d.chol_mask[k] = k < (int)h.chol_mask.size() ? h.chol_mask[k] : 0;
+    for (int k = 0; k < kMaxFrames; ++k) {
+        d.pose_col[k] = k < h.K ? h.pose_col[k] : -1;
+        d.sb_col[k] = k < h.K ? h.sb_col[k] : -1;
+        d.sb_w[k] = k < h.K ? h.sb_width[k] : 0;
+    }
+    d.pose = a.take<double>(7 * h.K); d.sb = a.take<double>(9 * h.K); d.rig = a.take<double>(7);
+    d.td = a.take<double>(1); d.rho = a.take<double>(h.L);
+    d.cpose = a.take<double>(7 * h.K); d.csb = a.take<double>(9 * h.K); d.crig = a.take<double>(7);
+    d.ctd = a.take<double>(1); d.crho = a.take<double>(h.L);
+    d.cam = a.take<double>(12 * h.C);
+    d.lm_fc = a.take<int32_t>(h.L);
+    d.lm_off = a.take<int32_t>(h.L + 1);
+    d.lm_col = a.take<int32_t>(h.L);
+    d.lmd = a.take<double>((size_t)LM_PLANES * h.L);
+    d.ob_fc = a.take<int32_t>(h.F);
+    d.ob = a.take<double>((size_t)OB_PLANES * h.F);
+    d.fac_u = a.take<double>((size_t)6 * h.F);
+    d.fac_lm = a.take<double>((size_t)FL_PLANES * h.F);
+    d.pf_ob = a.take<double>((size_t)OB_PLANES * h.F);
+    d.pf_an = a.take<double>((size_t)LM_PLANES * h.F);
+    d.pf_ids = a.take<int32_t>(h.F);
+    d.pf_lm = a.take<int32_t>(h.F);
+    d.pf_pos = a.take<int32_t>(h.F);
+    d.lm_out = a.take<double>((size_t)LO_PLANES * h.L);
+    d.lm_delta = a.take<double>(h.L);
+    d.imu_from = a.take<int32_t>(h.E);
+    d.imu_to = a.take<int32_t>(h.E);
+    d.imu = a.take<double>((size_t)VIB_IMU_STRIDE * h.E);
+    d.prior_rows = h.prior_rows; d.prior_cols = h.prior_cols; d.prior_blocks = (int)h.pblk.size();
+    d.pblk = a.take<PriorBlockDev>(h.pblk.size());
+    d.pA = a.take<double>((size_t)h.prior_rows * h.prior_cols);
+    d.pr0 = a.take<double>(h.prior_rows);
+    d.pG = a.take<double>((size_t)h.prior_cols * h.prior_cols);
+    d.pref = a.take<double>(h.prior_ref.size());
+    d.pfree = a.take<int32_t>(h.pfree.size());
+    d.prior_free = (int)h.pfree.size();
+    const size_t nr = h.nr;
+    d.imu_J = a.take<double>((size_t)450 * h.E); d.imu_r = a.take<double>((size_t)15 * h.E);
+    d.pD = a.take<double>(36 * h.pblk.size()); d.patr = a.take<double>(h.prior_cols);
+    d.ret_sc = a.take<double>(4 * (size_t)h.E + 4);
+    d.colmap = a.take<int32_t>(4 * nr);
+    d.Hfull = a.take<double>(nr * nr); d.gfull = a.take<double>(nr);
+    d.fac = a.take<double>(factor_doubles((int)nr + kTile)); d.Scopy = a.take<double>(nr * nr);
+    d.rhs = a.take<double>(nr); d.dstate = a.take<double>(nr); d.scratch = a.take<double>(4 * nr + 64);
+    const size_t len = d.tri + h.nv, nch = h.chunks.size();
+    const size_t nlin = h.lin_chunks.size();
+    d.lin_part = a.take<double>(nlin * len); d.schur_part = a.take<double>(nch * len);
+    d.lin_sc = a.take<double>(nlin * CS_PLANES);
+    d.chunk_sc = a.take<double>(nch * CS_PLANES);
+    d.nchunks = (int)nch;
+    d.nlin = (int)nlin;
+    x.reduced_lin = a.take<double>(len); x.reduced_schur = a.take<double>(len);
+    d.lm = a.take<LmState>(1);
+    x.gram_win = a.take<WinDev>(1);
+    x.gram_out = a.take<double*>(1);
+    return a.used <= bytes;
+}
+
+bool Slot::fill(const HostWindow& h, const WinDev& d, const SlotCarve& x, char* base, size_t bytes, bool streams,
+                hipStream_t stream) {
+    // every copy into the arena stays inside the carved footprint (host-side bounds check)
+    const char* lo = base;
+    auto up = [&](const void* dst, const void* src, size_t n) {
+        if (n == 0) return true;
+        const char* p = reinterpret_cast<const char*>(dst);
+        if (!src || p < lo || p + n > lo + bytes) return false;
+        return hipMemcpyAsync(const_cast<void*>(dst), src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
+    };
+    bool ok = up(d.pose, h.pose.data(), 8 * h.pose.size()) && up(d.sb, h.sb.data(), 8 * h.sb.size()) &&
+              up(d.rig, h.rig, 56) && up(d.td, &h.td, 8) && up(d.cam, h.cam.data(), 8 * h.cam.size()) &&
+              up(d.imu_from, h.imu_from.data(), 4 * h.E) && up(d.imu_to, h.imu_to.data(), 4 * h.E) &&
+              up(d.imu, h.imu.data(), 8 * h.imu.size()) &&
+              up(d.pblk, h.pblk.data(), sizeof(PriorBlockDev) * h.pblk.size()) &&
+              up(d.pA, h.prior_A.data(), 8 * h.prior_A.size()) && up(d.pr0, h.prior_r0.data(), 8 * h.prior_r0.size()) &&
+              up(d.pref, h.prior_ref.data(), 8 * h.prior_ref.size()) &&
+              up(d.pfree, h.pfree.data(), 4 * h.pfree.size()) && up(d.colmap, h.colmap.data(), 4 * h.colmap.size());
+    if (streams) {
+        std::vector<int32_t> fc(h.L);
+        for (int l = 0; l < h.L; ++l) fc[l] = (h.lm_frame[l] << 8) | h.lm_cam[l];
+        ok = ok && up(d.rho, h.rho.data(), 8 * h.rho.size()) && up(d.lm_fc, fc.data(), 4 * fc.size()) &&
+             up(d.lm_off, h.lm_off.data(), 4 * h.lm_off.size()) && up(d.lm_col, h.lm_col.data(), 4 * h.lm_col.size()) &&
+             up(d.lmd, h.lmd.data(), 8 * h.lmd.size()) && up(d.ob_fc, h.ob_fc.data(), 4 * h.ob_fc.size()) &&
+             up(d.ob, h.ob.data(), 8 * h.ob.size()) && up(d.pf_ob, h.pf_ob.data(), 8 * h.pf_ob.size()) &&
+             up(d.pf_an, h.pf_an.data(), 8 * h.pf_an.size()) && up(d.pf_ids, h.pf_ids.data(), 4 * h.pf_ids.size()) &&
+             up(d.pf_lm, h.pf_lm.data(), 4 * h.pf_lm.size()) && up(d.pf_pos, h.pf_pos.data(), 4 * h.pf_pos.size()) &&
+             up(d.crho, h.rho.data(), 8 * h.rho.size());
+        // the layouts of the copies match the carved footprint
+        ok = ok && (int)h.rho.size() == h.L && (int)h.lm_off.size() == h.L + 1 && h.lmd.size() == (size_t)LM_PLANES * h.L &&
+             (int)h.ob_fc.size() == h.F && h.ob.size() == (size_t)OB_PLANES * h.F && h.pf_ob.size() == (size_t)OB_PLANES * h.F;
+    }
+    // candidate starts equal to the state
+    ok = ok && up(d.cpose, h.pose.data(), 8 * h.pose.size()) && up(d.csb, h.sb.data(), 8 * h.sb.size()) &&
+         up(d.crig, h.rig, 56) && up(d.ctd, &h.td, 8);
+    LmState lm{};
+    lm.lambda = 1e-4; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1;
+    ok = ok && up(d.lm, &lm, sizeof(lm));
+    if (ok && h.prior_rows) {
+        // G = A^T A once per prior (one-window launch)
+        ok = up(x.gram_win, &d, sizeof(WinDev)) && up(x.gram_out, &d.pG, sizeof(double*)) &&
+             vibk_prior_gram(x.gram_win, 1, x.gram_out, stream) == hipSuccess;
+    }
+    // the staging above (landmark keys, LM state) is local: the copies complete before return
+    return ok && hipStreamSynchronize(stream) == hipSuccess;
+}
+
 int Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     const size_t need = footprint(h);
     // The slot is live only once every part of the window is on the device: a failure on any
@@ -392,113 +533,15 @@ int Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     host = h;
     bytes = need;
     host_set = false;  // host-evaluated terms belong to the previous window
-    Arena a{reinterpret_cast<char*>(arena), bytes, 0};
-    WinDev& d = dev;
-    d = WinDev{};
-    d.K = h.K; d.C = h.C; d.L = h.L; d.F = h.F; d.E = h.E;
-    d.nv = h.nv; d.nr = h.nr; d.ne = h.ne; d.rig_col = h.rig_col; d.td_col = h.td_col; d.flags = h.flags;
-    d.tri = h.nv * (h.nv + 1) / 2;
-    d.huber = h.huber; d.min_rho = h.min_rho; d.td_lo = h.td_lo; d.td_hi = h.td_hi;
-    for (int k = 0; k < kMaxTiles; ++k) d.chol_mask[k] = k < (int)h.chol_mask.size() ? h.chol_mask[k] : 0;
-    for (int k = 0; k < kMaxFrames; ++k) {
-        d.pose_col[k] = k < h.K ? h.pose_col[k] : -1;
-        d.sb_col[k] = k < h.K ? h.sb_col[k] : -1;
-        d.sb_w[k] = k < h.K ? h.sb_width[k] : 0;
+    SlotCarve x;
+    if (!carve(h, static_cast<char*>(arena), bytes, dev, x)) { err = "arena overflow"; return VIB_ERR_DEVICE; }
+    reduced_lin = x.reduced_lin;
+    reduced_schur = x.reduced_schur;
+    if (!fill(h, dev, x, static_cast<char*>(arena), bytes, true, stream) || hipStreamSynchronize(stream) != hipSuccess) {
+        (void)hipGetLastError();
+        err = "upload failed";
+        return VIB_ERR_DEVICE;
     }
-    d.pose = a.take<double>(7 * h.K); d.sb = a.take<double>(9 * h.K); d.rig = a.take<double>(7);
-    d.td = a.take<double>(1); d.rho = a.take<double>(h.L);
-    d.cpose = a.take<double>(7 * h.K); d.csb = a.take<double>(9 * h.K); d.crig = a.take<double>(7);
-    d.ctd = a.take<double>(1); d.crho = a.take<double>(h.L);
-    double* cam = a.take<double>(12 * h.C);
-    int32_t* lm_fc = a.take<int32_t>(h.L);
-    int32_t* lm_off = a.take<int32_t>(h.L + 1);
-    int32_t* lm_col = a.take<int32_t>(h.L);
-    double* lmd = a.take<double>((size_t)LM_PLANES * h.L);
-    int32_t* ob_fc = a.take<int32_t>(h.F);
-    double* ob = a.take<double>((size_t)OB_PLANES * h.F);
-    d.cam = cam; d.lm_fc = lm_fc; d.lm_off = lm_off; d.lm_col = lm_col; d.lmd = lmd; d.ob_fc = ob_fc; d.ob = ob;
-    d.fac_u = a.take<double>((size_t)6 * h.F);
-    d.fac_lm = a.take<double>((size_t)FL_PLANES * h.F);
-    double* pf_ob = a.take<double>((size_t)OB_PLANES * h.F);
-    double* pf_an = a.take<double>((size_t)LM_PLANES * h.F);
-    int32_t* pf_ids = a.take<int32_t>(h.F);
-    int32_t* pf_lm = a.take<int32_t>(h.F);
-    int32_t* pf_pos = a.take<int32_t>(h.F);
-    d.pf_ob = pf_ob; d.pf_an = pf_an; d.pf_ids = pf_ids; d.pf_lm = pf_lm; d.pf_pos = pf_pos;
-    d.lm_out = a.take<double>((size_t)LO_PLANES * h.L);
-    d.lm_delta = a.take<double>(h.L);
-    int32_t* imu_from = a.take<int32_t>(h.E);
-    int32_t* imu_to = a.take<int32_t>(h.E);
-    double* imu = a.take<double>((size_t)VIB_IMU_STRIDE * h.E);
-    d.imu_from = imu_from; d.imu_to = imu_to; d.imu = imu;
-    d.prior_rows = h.prior_rows; d.prior_cols = h.prior_cols; d.prior_blocks = (int)h.pblk.size();
-    PriorBlockDev* pblk = a.take<PriorBlockDev>(h.pblk.size());
-    double* pA = a.take<double>((size_t)h.prior_rows * h.prior_cols);
-    double* pr0 = a.take<double>(h.prior_rows);
-    double* pG = a.take<double>((size_t)h.prior_cols * h.prior_cols);
-    double* pref = a.take<double>(h.prior_ref.size());
-    int32_t* pfree = a.take<int32_t>(h.pfree.size());
-    d.pblk = pblk; d.pA = pA; d.pr0 = pr0; d.pG = pG; d.pref = pref; d.pfree = pfree;
-    d.prior_free = (int)h.pfree.size();
-    const size_t nr = h.nr;
-    d.imu_J = a.take<double>((size_t)450 * h.E); d.imu_r = a.take<double>((size_t)15 * h.E);
-    d.pD = a.take<double>(36 * h.pblk.size()); d.patr = a.take<double>(h.prior_cols);
-    d.ret_sc = a.take<double>(4 * (size_t)h.E + 4);
-    int32_t* colmap = a.take<int32_t>(4 * nr);
-    d.colmap = colmap;
-    d.Hfull = a.take<double>(nr * nr); d.gfull = a.take<double>(nr);
-    d.fac = a.take<double>(factor_doubles((int)nr + kTile)); d.Scopy = a.take<double>(nr * nr);
-    d.rhs = a.take<double>(nr); d.dstate = a.take<double>(nr); d.scratch = a.take<double>(4 * nr + 64);
-    const size_t len = d.tri + h.nv, nch = h.chunks.size();
-    const size_t nlin = h.lin_chunks.size();
-    d.lin_part = a.take<double>(nlin * len); d.schur_part = a.take<double>(nch * len);
-    d.lin_sc = a.take<double>(nlin * CS_PLANES);
-    d.chunk_sc = a.take<double>(nch * CS_PLANES);
-    d.nchunks = (int)nch;
-    d.nlin = (int)nlin;
-    reduced_lin = a.take<double>(len); reduced_schur = a.take<double>(len);
-    d.lm = a.take<LmState>(1);
-    WinDev* gram_win = a.take<WinDev>(1);
-    double** gram_out = a.take<double*>(1);
-    if (a.used > bytes) { err = "arena overflow"; return VIB_ERR_DEVICE; }
-    // every copy into the arena stays inside the carved footprint (host-side bounds check)
-    const char* lo = reinterpret_cast<const char*>(arena);
-    auto up = [&](void* dst, const void* src, size_t n) {
-        if (n == 0) return true;
-        const char* p = reinterpret_cast<const char*>(dst);
-        if (!src || p < lo || p + n > lo + bytes) return false;
-        return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
-    };
-    std::vector<int32_t> fc(h.L);
-    for (int l = 0; l < h.L; ++l) fc[l] = (h.lm_frame[l] << 8) | h.lm_cam[l];
-    bool ok = up(d.pose, h.pose.data(), 8 * h.pose.size()) && up(d.sb, h.sb.data(), 8 * h.sb.size()) &&
-              up(d.rig, h.rig, 56) && up(d.td, &h.td, 8) && up(d.rho, h.rho.data(), 8 * h.rho.size()) &&
-              up(cam, h.cam.data(), 8 * h.cam.size()) && up(lm_fc, fc.data(), 4 * fc.size()) &&
-              up(lm_off, h.lm_off.data(), 4 * h.lm_off.size()) && up(lm_col, h.lm_col.data(), 4 * h.lm_col.size()) &&
-              up(lmd, h.lmd.data(), 8 * h.lmd.size()) && up(ob_fc, h.ob_fc.data(), 4 * h.ob_fc.size()) &&
-              up(ob, h.ob.data(), 8 * h.ob.size()) && up(imu_from, h.imu_from.data(), 4 * h.E) &&
-              up(imu_to, h.imu_to.data(), 4 * h.E) && up(imu, h.imu.data(), 8 * h.imu.size()) &&
-              up(pblk, h.pblk.data(), sizeof(PriorBlockDev) * h.pblk.size()) &&
-              up(pA, h.prior_A.data(), 8 * h.prior_A.size()) && up(pr0, h.prior_r0.data(), 8 * h.prior_r0.size()) &&
-              up(pref, h.prior_ref.data(), 8 * h.prior_ref.size()) &&
-              up(pfree, h.pfree.data(), 4 * h.pfree.size()) && up(colmap, h.colmap.data(), 4 * h.colmap.size()) &&
-              up(pf_ob, h.pf_ob.data(), 8 * h.pf_ob.size()) && up(pf_an, h.pf_an.data(), 8 * h.pf_an.size()) &&
-              up(pf_ids, h.pf_ids.data(), 4 * h.pf_ids.size()) && up(pf_lm, h.pf_lm.data(), 4 * h.pf_lm.size()) &&
-              up(pf_pos, h.pf_pos.data(), 4 * h.pf_pos.size());
-    // candidate starts equal to the state
-    ok = ok && up(d.cpose, h.pose.data(), 8 * h.pose.size()) && up(d.csb, h.sb.data(), 8 * h.sb.size()) &&
-         up(d.crig, h.rig, 56) && up(d.ctd, &h.td, 8) && up(d.crho, h.rho.data(), 8 * h.rho.size());
-    LmState lm{};
-    lm.lambda = 1e-4; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1;
-    ok = ok && up(d.lm, &lm, sizeof(lm));
-    if (!ok) { err = "upload failed"; return VIB_ERR_DEVICE; }
-    if (h.prior_rows) {
-        // G = A^T A once per prior (one-window launch)
-        ok = up(gram_win, &d, sizeof(WinDev)) && up(gram_out, &pG, sizeof(double*)) &&
-             vibk_prior_gram(gram_win, 1, gram_out, stream) == hipSuccess && hipStreamSynchronize(stream) == hipSuccess;
-        if (!ok) { err = "prior Gram matrix failed"; return VIB_ERR_DEVICE; }
-    }
-    if (hipStreamSynchronize(stream) != hipSuccess) { err = "upload synchronization failed"; return VIB_ERR_DEVICE; }
     used = true;
     return VIB_OK;
 }
@@ -688,10 +731,19 @@ struct vib_context {
     void* aux_buf = nullptr;
     size_t aux_cap = 0;
     std::unique_ptr<vib::Slot> marginal_slot, two_frame_slot;
+    // Window edits: the buffers of the scans and selections, and a spare arena that an edit in
+    // place (a slide) builds the new window in before it takes the slot's place (the slot's old
+    // arena becomes the spare).
+    void* edit_buf = nullptr;
+    size_t edit_cap = 0;
+    void* spare = nullptr;
+    size_t spare_cap = 0;
     ~vib_context() {
         if (preint_buf) (void)hipFree(preint_buf);
         if (eig_buf) (void)hipFree(eig_buf);
         if (aux_buf) (void)hipFree(aux_buf);
+        if (edit_buf) (void)hipFree(edit_buf);
+        if (spare) (void)hipFree(spare);
     }
 };
 
@@ -822,9 +874,14 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
 }
 
 // Host copy of a slot with the state it holds on the device now.
-bool pull_state(vib_context* ctx, Slot& s, HostWindow& h) {
-    h = s.host;
+// Downloads the resident state (poses, speed/bias, rig, time offset, inverse ranges) into the
+// slot's host mirror.
+bool refresh_state(vib_context* ctx, Slot& s) {
+    HostWindow& h = s.host;
     const WinDev& d = s.dev;
+    h.pose.resize(7 * (size_t)h.K);
+    h.sb.resize(9 * (size_t)h.K);
+    h.rho.resize(h.L);
     return hipMemcpyAsync(h.pose.data(), d.pose, 56 * (size_t)d.K, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
            hipMemcpyAsync(h.sb.data(), d.sb, 72 * (size_t)d.K, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
            hipMemcpyAsync(h.rig, d.rig, 56, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
@@ -833,90 +890,245 @@ bool pull_state(vib_context* ctx, Slot& s, HostWindow& h) {
            hipStreamSynchronize(ctx->stream) == hipSuccess;
 }
 
-struct LatestTwo {
-    HostWindow win;
-    int edges[2] = {-1, -1};  // local IMU edge ending at K-2, edge K-2 -> K-1
-    int support = 0;
+// Makes the mirror's landmark and observation vectors current after a device edit (they are
+// downloaded from the resident window; the host never rebuilds them).
+bool ensure_bulk(vib_context* ctx, Slot& s) {
+    HostWindow& h = s.host;
+    if (h.bulk) return true;
+    const size_t L = h.L, F = h.F;
+    const WinDev& d = s.dev;
+    std::vector<int32_t> fc(L);
+    h.lm_off.assign(L + 1, 0); h.lm_col.assign(L, -1); h.lmd.assign((size_t)LM_PLANES * L, 0); h.rho.assign(L, 0);
+    h.ob_fc.assign(F, 0); h.ob.assign((size_t)OB_PLANES * F, 0);
+    h.pf_ob.assign((size_t)OB_PLANES * F, 0); h.pf_an.assign((size_t)LM_PLANES * F, 0);
+    h.pf_ids.assign(F, 0); h.pf_lm.assign(F, 0); h.pf_pos.assign(F, 0);
+    auto dn = [&](void* dst, const void* src, size_t n) {
+        return n == 0 || hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess;
+    };
+    const bool ok = dn(fc.data(), d.lm_fc, 4 * L) && dn(h.lm_off.data(), d.lm_off, 4 * (L + 1)) &&
+                    dn(h.lm_col.data(), d.lm_col, 4 * L) && dn(h.lmd.data(), d.lmd, 8 * h.lmd.size()) &&
+                    dn(h.rho.data(), d.rho, 8 * L) && dn(h.ob_fc.data(), d.ob_fc, 4 * F) &&
+                    dn(h.ob.data(), d.ob, 8 * h.ob.size()) && dn(h.pf_ob.data(), d.pf_ob, 8 * h.pf_ob.size()) &&
+                    dn(h.pf_an.data(), d.pf_an, 8 * h.pf_an.size()) && dn(h.pf_ids.data(), d.pf_ids, 4 * F) &&
+                    dn(h.pf_lm.data(), d.pf_lm, 4 * F) && dn(h.pf_pos.data(), d.pf_pos, 4 * F) &&
+                    hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) return false;
+    h.lm_frame.assign(L, 0); h.lm_cam.assign(L, 0); h.lm_free.assign(L, 0); h.ob_lm.assign(F, 0);
+    for (size_t l = 0; l < L; ++l) {
+        h.lm_frame[l] = fc[l] >> 8;
+        h.lm_cam[l] = fc[l] & 255;
+        h.lm_free[l] = h.lm_col[l] >= 0;
+        for (int f = h.lm_off[l]; f < h.lm_off[l + 1]; ++f) h.ob_lm[f] = (int)l;
+    }
+    h.bulk = true;
+    return true;
+}
+
+// The frame-level part of a window (frames, state, cameras, IMU, prior) without its landmark
+// and observation vectors: the host part of an edited window.
+HostWindow frame_meta(const HostWindow& cur) {
+    HostWindow m;
+    m.K = cur.K; m.C = cur.C; m.E = cur.E;
+    m.flags = cur.flags; m.huber = cur.huber; m.min_rho = cur.min_rho; m.td_lo = cur.td_lo; m.td_hi = cur.td_hi;
+    m.frame_id = cur.frame_id; m.pose = cur.pose; m.sb = cur.sb; m.pose_free = cur.pose_free; m.sb_width = cur.sb_width;
+    std::copy_n(cur.rig, 7, m.rig);
+    m.td = cur.td; m.cam = cur.cam;
+    m.imu_from = cur.imu_from; m.imu_to = cur.imu_to; m.imu = cur.imu;
+    m.prior_rows = cur.prior_rows; m.prior_cols = cur.prior_cols; m.prior_kind = cur.prior_kind;
+    m.prior_id = cur.prior_id; m.prior_ref = cur.prior_ref; m.prior_A = cur.prior_A; m.prior_r0 = cur.prior_r0;
+    m.bulk = false;
+    return m;
+}
+
+// Removes frame `drop` from the frame-level vectors (later frames move down by one).
+void drop_frame(HostWindow& m, int drop) {
+    m.K -= 1;
+    m.frame_id.erase(m.frame_id.begin() + drop);
+    m.pose.erase(m.pose.begin() + 7 * drop, m.pose.begin() + 7 * drop + 7);
+    m.sb.erase(m.sb.begin() + 9 * drop, m.sb.begin() + 9 * drop + 9);
+    m.pose_free.erase(m.pose_free.begin() + drop);
+    m.sb_width.erase(m.sb_width.begin() + drop);
+}
+
+// The IMU edges e with keep[e] that do not touch frame `drop` (-1: none), renumbered as
+// drop_frame renumbers the frames.
+void keep_edges(HostWindow& m, const std::vector<int>& from, const std::vector<int>& to, const std::vector<double>& rec,
+                const std::vector<char>& keep, int drop) {
+    auto slot = [&](int k) { return drop >= 0 && k > drop ? k - 1 : k; };
+    m.imu_from.clear(); m.imu_to.clear(); m.imu.clear();
+    for (size_t e = 0; e < from.size(); ++e) {
+        if (!keep[e] || from[e] == drop || to[e] == drop) continue;
+        m.imu_from.push_back(slot(from[e]));
+        m.imu_to.push_back(slot(to[e]));
+        m.imu.insert(m.imu.end(), rec.begin() + (size_t)VIB_IMU_STRIDE * e, rec.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+    }
+    m.E = (int)m.imu_from.size();
+}
+
+// Device buffers of the window edits of one slide (sized for the source window), carved from
+// the context's grow-only edit buffer.
+struct EditBufs {
+    WinDev* src = nullptr;
+    WinDev* dst = nullptr;
+    int* bad = nullptr;  // kept anchors or observations of removed frames (must stay 0)
+    int* sel = nullptr;  // 16 counters of the selection kernels
+    unsigned long long *lmv = nullptr, *lscan = nullptr, *ltile = nullptr;
+    uint32_t *fv = nullptr, *fscan = nullptr, *ftile = nullptr;
+    int32_t *lm_of = nullptr, *lmn = nullptr;
+    uint32_t *counts = nullptr, *offs = nullptr, *ctile = nullptr;
+    uint8_t *keep_lm = nullptr, *keep_f = nullptr, *marg_lm = nullptr, *cls = nullptr;
+    int32_t *over_f = nullptr, *chosen = nullptr;
+    double *over_rho = nullptr, *chosen_rho = nullptr;
 };
 
-// Local graph of the two-frame solve (BackendOptimizer.cpp:476-538): factors touching the pose
-// or speed/bias of slots K-2, K-1; those blocks free with the velocity-only speed/bias tangent,
-// everything else constant.
-int latest_two_window(const HostWindow& h, LatestTwo& out, std::string& err) {
-    if (h.K < 2) { err = "missing pose or velocity block in two-frame solve"; return VIB_ERR_LIGHTWEIGHT_SUPPORT; }
-    const int prev = h.K - 2, last = h.K - 1;
-    auto is_free = [&](int k) { return k == prev || k == last; };
-    HostWindow& w = out.win;
-    w = HostWindow();
-    w.K = h.K; w.C = h.C; w.E = 0;
-    w.flags = h.flags & ~(VIB_RIG_FREE | VIB_TD_FREE);
-    w.huber = h.huber; w.min_rho = h.min_rho; w.td_lo = h.td_lo; w.td_hi = h.td_hi;
-    w.frame_id = h.frame_id; w.pose = h.pose; w.sb = h.sb;
-    std::copy_n(h.rig, 7, w.rig);
-    w.td = h.td; w.cam = h.cam;
-    w.pose_free.assign(h.K, 0);
-    w.sb_width.assign(h.K, 0);
-    w.pose_free[prev] = w.pose_free[last] = 1;
-    w.sb_width[prev] = w.sb_width[last] = 3;
-    // visual factors touching a free pose, and their landmarks (held constant)
-    std::vector<int> keep_lm, keep_f;
-    std::vector<int> off{0};
-    std::vector<char> seen(h.L, 0);
-    for (int l = 0; l < h.L; ++l) {
-        const size_t before = keep_f.size();
-        for (int f = h.lm_off[l]; f < h.lm_off[l + 1]; ++f)
-            if (is_free(h.lm_frame[l]) || is_free(h.ob_fc[f] >> 8)) {
-                keep_f.push_back(f);
-                if ((h.ob_fc[f] >> 8) == last && !seen[l]) { seen[l] = 1; ++out.support; }
+bool edit_buffers(vib_context* ctx, int L, int F, EditBufs& b) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) & ~size_t(255);
+        const size_t o = off;
+        off += std::max<size_t>(bytes, 1);
+        return o;
+    };
+    const size_t Ls = L, Fs = F;
+    const size_t lt = (size_t)vibk_edit_scan_tiles(L) + 1, ft = (size_t)vibk_edit_scan_tiles(F) + 1;
+    const size_t nkeys = (size_t)vibk_edit_pair_keys() * vibk_edit_sort_blocks(F);
+    const size_t ct = (size_t)vibk_edit_scan_tiles((int)nkeys) + 1;
+    const size_t o_src = take(sizeof(WinDev)), o_dst = take(sizeof(WinDev)), o_bad = take(4), o_sel = take(64);
+    const size_t o_lmv = take(8 * Ls), o_lscan = take(8 * (Ls + 1)), o_ltile = take(8 * lt);
+    const size_t o_fv = take(4 * Fs), o_fscan = take(4 * (Fs + 1)), o_ftile = take(4 * ft);
+    const size_t o_lmof = take(4 * Fs), o_lmn = take(4 * Fs);
+    const size_t o_counts = take(4 * (nkeys + 1)), o_offs = take(4 * (nkeys + 1)), o_ctile = take(4 * ct);
+    const size_t o_klm = take(Ls), o_kf = take(Fs), o_mlm = take(Ls), o_cls = take(Ls);
+    const size_t o_overf = take(4 * Ls), o_chosen = take(4 * Ls), o_overr = take(8 * Ls), o_chr = take(8 * Ls);
+    char* base = static_cast<char*>(grow_device(ctx->edit_buf, ctx->edit_cap, off));
+    if (!base) return false;
+    b.src = reinterpret_cast<WinDev*>(base + o_src);
+    b.dst = reinterpret_cast<WinDev*>(base + o_dst);
+    b.bad = reinterpret_cast<int*>(base + o_bad);
+    b.sel = reinterpret_cast<int*>(base + o_sel);
+    b.lmv = reinterpret_cast<unsigned long long*>(base + o_lmv);
+    b.lscan = reinterpret_cast<unsigned long long*>(base + o_lscan);
+    b.ltile = reinterpret_cast<unsigned long long*>(base + o_ltile);
+    b.fv = reinterpret_cast<uint32_t*>(base + o_fv);
+    b.fscan = reinterpret_cast<uint32_t*>(base + o_fscan);
+    b.ftile = reinterpret_cast<uint32_t*>(base + o_ftile);
+    b.lm_of = reinterpret_cast<int32_t*>(base + o_lmof);
+    b.lmn = reinterpret_cast<int32_t*>(base + o_lmn);
+    b.counts = reinterpret_cast<uint32_t*>(base + o_counts);
+    b.offs = reinterpret_cast<uint32_t*>(base + o_offs);
+    b.ctile = reinterpret_cast<uint32_t*>(base + o_ctile);
+    b.keep_lm = reinterpret_cast<uint8_t*>(base + o_klm);
+    b.keep_f = reinterpret_cast<uint8_t*>(base + o_kf);
+    b.marg_lm = reinterpret_cast<uint8_t*>(base + o_mlm);
+    b.cls = reinterpret_cast<uint8_t*>(base + o_cls);
+    b.over_f = reinterpret_cast<int32_t*>(base + o_overf);
+    b.chosen = reinterpret_cast<int32_t*>(base + o_chosen);
+    b.over_rho = reinterpret_cast<double*>(base + o_overr);
+    b.chosen_rho = reinterpret_cast<double*>(base + o_chr);
+    return true;
+}
+
+// Copies the slot's descriptor to b.src and clears the selection counters.
+bool edit_begin(vib_context* ctx, const Slot& s, const EditBufs& b) {
+    const int zero[16] = {};
+    return hipMemcpyAsync(b.src, &s.dev, sizeof(WinDev), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+           hipMemcpyAsync(b.sel, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+           hipStreamSynchronize(ctx->stream) == hipSuccess;
+}
+
+// The window that `sp` selects from the resident window of `src`, made the resident window of
+// `dst`. meta holds its frame-level part (frame_meta, then the caller's changes); its landmark
+// and observation counts, free inverse ranges and work lists follow from the device scans. With
+// dst == src (a slide in place) the edited window is built in the context's spare arena and
+// then takes the slot's place; a failure leaves src as it was. `budgeted`: the result counts
+// against the context's HBM budget like an upload to dst.
+int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slot& dst, const EditBufs& b, bool budgeted) {
+    std::string err;
+    if (!meta.layout_states(err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
+    sp.nr = meta.nr;
+    const int Ls = src.host.L, Fs = src.host.F;
+    const hipStream_t st = ctx->stream;
+    auto h2d = [&](void* dst_p, const void* src_p, size_t n) {
+        return hipMemcpyAsync(dst_p, src_p, n, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    auto d2h = [&](void* dst_p, const void* src_p, size_t n) {
+        return hipMemcpyAsync(dst_p, src_p, n, hipMemcpyDeviceToHost, st) == hipSuccess;
+    };
+    const int zero = 0;
+    unsigned long long ltot = 0;
+    uint32_t ftot = 0;
+    int bad = 0;
+    bool ok = h2d(b.src, &src.dev, sizeof(WinDev)) && h2d(b.bad, &zero, sizeof(int)) &&
+              vibk_edit_mark(b.src, Ls, &sp, b.lmv, b.fv, b.lm_of, b.bad, st) == hipSuccess &&
+              vibk_edit_scan(b.lmv, Ls, b.ltile, b.lscan, 1, st) == hipSuccess &&
+              vibk_edit_scan(b.fv, Fs, b.ftile, b.fscan, 0, st) == hipSuccess &&
+              d2h(&ltot, b.lscan + Ls, sizeof(ltot)) && d2h(&ftot, b.fscan + Fs, sizeof(ftot)) &&
+              d2h(&bad, b.bad, sizeof(bad)) && hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        return fail(ctx, VIB_ERR_DEVICE, "window edit failed");
+    }
+    if (bad) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "window edit keeps an anchor or observation of a removed frame");
+    meta.L = (int)(uint32_t)ltot;
+    meta.ne = (int)(ltot >> 32);
+    meta.F = (int)ftot;
+    meta.bulk = false;
+    meta.lm_frame.clear(); meta.lm_cam.clear(); meta.lm_off.clear(); meta.lm_free.clear(); meta.lmd.clear();
+    meta.rho.clear(); meta.ob_fc.clear(); meta.ob_lm.clear(); meta.ob.clear(); meta.lm_col.clear();
+    meta.pf_ob.clear(); meta.pf_an.clear(); meta.pf_ids.clear(); meta.pf_lm.clear(); meta.pf_pos.clear();
+    meta.make_chunks();
+    const size_t need = Slot::footprint(meta);
+    if (budgeted && ctx->hbm_budget > 0) {
+        size_t resident = 0;
+        for (const auto& kv : ctx->slots)
+            if (kv.second && kv.second.get() != &dst && kv.second->used) resident += kv.second->capacity;
+        if (resident + need > ctx->hbm_budget)
+            return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "edited window does not fit the context's HBM budget");
+    }
+    const bool in_place = &dst == &src;
+    char* base = nullptr;
+    if (in_place) {
+        base = static_cast<char*>(grow_device(ctx->spare, ctx->spare_cap, need));
+        if (!base) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "window edit allocation failed");
+    } else {
+        dst.used = false;
+        if (!dst.arena || need > dst.capacity) {
+            dst.release();
+            if (hipMalloc(&dst.arena, need) != hipSuccess) {
+                dst.arena = nullptr;
+                (void)hipGetLastError();
+                return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "window edit allocation failed");
             }
-        if (keep_f.size() > before) {
-            keep_lm.push_back(l);
-            off.push_back((int)keep_f.size());
+            dst.capacity = need;
         }
+        base = static_cast<char*>(dst.arena);
     }
-    if (out.support < 6) { err = "fewer than six current landmarks in two-frame solve"; return VIB_ERR_LIGHTWEIGHT_SUPPORT; }
-    w.L = (int)keep_lm.size();
-    w.F = (int)keep_f.size();
-    w.lm_off = off;
-    w.lm_frame.resize(w.L); w.lm_cam.resize(w.L); w.lm_free.assign(w.L, 0); w.rho.resize(w.L);
-    w.lmd.assign((size_t)LM_PLANES * w.L, 0);
-    for (int i = 0; i < w.L; ++i) {
-        const int l = keep_lm[i];
-        w.lm_frame[i] = h.lm_frame[l]; w.lm_cam[i] = h.lm_cam[l]; w.rho[i] = h.rho[l];
-        for (int p = 0; p < LM_PLANES; ++p) w.lmd[(size_t)p * w.L + i] = h.lmd[(size_t)p * h.L + l];
+    WinDev nd;
+    SlotCarve x;
+    if (!Slot::carve(meta, base, need, nd, x)) return fail(ctx, VIB_ERR_DEVICE, "arena overflow");
+    const int Fd = meta.F, nkeys = vibk_edit_pair_keys() * vibk_edit_sort_blocks(Fd);
+    ok = Slot::fill(meta, nd, x, base, need, false, st) && h2d(b.dst, &nd, sizeof(WinDev)) &&
+         vibk_edit_gather(b.src, b.dst, Ls, Fs, Fd, &sp, b.lscan, b.fscan, b.lm_of, b.lmn, st) == hipSuccess &&
+         (Fd == 0 || (vibk_edit_pair_hist(b.dst, Fd, b.lmn, b.counts, st) == hipSuccess &&
+                      vibk_edit_scan(b.counts, nkeys, b.ctile, b.offs, 0, st) == hipSuccess &&
+                      vibk_edit_pair_scatter(b.dst, Fd, b.lmn, b.offs, st) == hipSuccess)) &&
+         hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        return fail(ctx, VIB_ERR_DEVICE, "window edit failed");
     }
-    w.ob_fc.resize(w.F); w.ob_lm.resize(w.F);
-    w.ob.assign((size_t)OB_PLANES * w.F, 0);
-    for (int i = 0, li = 0; i < w.F; ++i) {
-        while (i >= w.lm_off[li + 1]) ++li;
-        const int f = keep_f[i];
-        w.ob_fc[i] = h.ob_fc[f];
-        w.ob_lm[i] = li;
-        for (int p = 0; p < OB_PLANES; ++p) w.ob[(size_t)p * w.F + i] = h.ob[(size_t)p * h.F + f];
+    if (in_place) {
+        std::swap(dst.arena, ctx->spare);
+        std::swap(dst.capacity, ctx->spare_cap);
     }
-    // the IMU bridge: exactly the edge ending at K-2 and the edge K-2 -> K-1
-    for (int e = 0; e < h.E; ++e) {
-        if (!is_free(h.imu_from[e]) && !is_free(h.imu_to[e])) continue;
-        const int i = w.E++;
-        w.imu_from.push_back(h.imu_from[e]);
-        w.imu_to.push_back(h.imu_to[e]);
-        w.imu.insert(w.imu.end(), h.imu.begin() + (size_t)VIB_IMU_STRIDE * e, h.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
-        if (h.imu_to[e] == prev) out.edges[0] = i;
-        if (h.imu_from[e] == prev && h.imu_to[e] == last) out.edges[1] = i;
-    }
-    if (w.E != 2 || out.edges[0] < 0 || out.edges[1] < 0) { err = "two-frame IMU bridge missing"; return VIB_ERR_IMU_BRIDGE; }
-    // the prior takes part when one of its blocks is a free block
-    bool touches = false;
-    for (size_t b = 0; b < h.prior_kind.size(); ++b)
-        if ((h.prior_kind[b] == VIB_KEY_POSE || h.prior_kind[b] == VIB_KEY_SPEED_BIAS) &&
-            (h.prior_id[b] == h.frame_id[prev] || h.prior_id[b] == h.frame_id[last]))
-            touches = true;
-    if (touches) {
-        w.prior_rows = h.prior_rows; w.prior_cols = h.prior_cols;
-        w.prior_kind = h.prior_kind; w.prior_id = h.prior_id;
-        w.prior_ref = h.prior_ref; w.prior_A = h.prior_A; w.prior_r0 = h.prior_r0;
-    }
-    return w.layout(err) ? VIB_OK : VIB_ERR_INVALID_ARGUMENT;
+    dst.dev = nd;
+    dst.reduced_lin = x.reduced_lin;
+    dst.reduced_schur = x.reduced_schur;
+    dst.host = std::move(meta);
+    dst.bytes = need;
+    dst.host_set = false;
+    dst.used = true;
+    return VIB_OK;
 }
 
 // Squared whitened residuals (NIS) of two IMU edges of a slot at its resident state.
@@ -1029,53 +1241,6 @@ WindowCopy copy_of(const HostWindow& h) {
     return w;
 }
 
-// The window without slot `drop` (-1: none), restricted to the kept landmarks, factors and IMU
-// edges (factors and edges of the dropped slot go with it); later slots move down by one.
-WindowCopy filter_window(const WindowCopy& w, int drop, const std::vector<char>& keep_lm,
-                         const std::vector<char>& keep_f, const std::vector<char>& keep_e) {
-    WindowCopy o = w;
-    auto slot = [&](int k) { return drop >= 0 && k > drop ? k - 1 : k; };
-    if (drop >= 0) {
-        o.K = w.K - 1;
-        o.frame_id.erase(o.frame_id.begin() + drop);
-        o.pose.erase(o.pose.begin() + 7 * drop, o.pose.begin() + 7 * drop + 7);
-        o.sb.erase(o.sb.begin() + 9 * drop, o.sb.begin() + 9 * drop + 9);
-        o.pose_free.erase(o.pose_free.begin() + drop);
-        o.sb_width.erase(o.sb_width.begin() + drop);
-    }
-    std::vector<int> nl(w.L, -1);
-    o.lm_frame.clear(); o.lm_cam.clear(); o.lm_bearing.clear(); o.lm_vel.clear(); o.lm_tref.clear();
-    o.rho.clear(); o.lm_free.clear();
-    for (int l = 0; l < w.L; ++l) {
-        if (!keep_lm[l]) continue;
-        nl[l] = (int)o.lm_frame.size();
-        o.lm_frame.push_back(slot(w.lm_frame[l])); o.lm_cam.push_back(w.lm_cam[l]);
-        o.lm_bearing.insert(o.lm_bearing.end(), w.lm_bearing.begin() + 3 * l, w.lm_bearing.begin() + 3 * l + 3);
-        o.lm_vel.insert(o.lm_vel.end(), w.lm_vel.begin() + 3 * l, w.lm_vel.begin() + 3 * l + 3);
-        o.lm_tref.push_back(w.lm_tref[l]); o.rho.push_back(w.rho[l]); o.lm_free.push_back(w.lm_free[l]);
-    }
-    o.L = (int)o.lm_frame.size();
-    o.ob_lm.clear(); o.ob_frame.clear(); o.ob_cam.clear(); o.ob_bearing.clear(); o.ob_vel.clear();
-    o.ob_tref.clear(); o.ob_sqrt.clear();
-    for (int f = 0; f < w.F; ++f) {
-        if (!keep_f[f] || nl[w.ob_lm[f]] < 0 || w.ob_frame[f] == drop) continue;
-        o.ob_lm.push_back(nl[w.ob_lm[f]]); o.ob_frame.push_back(slot(w.ob_frame[f])); o.ob_cam.push_back(w.ob_cam[f]);
-        o.ob_bearing.insert(o.ob_bearing.end(), w.ob_bearing.begin() + 3 * f, w.ob_bearing.begin() + 3 * f + 3);
-        o.ob_vel.insert(o.ob_vel.end(), w.ob_vel.begin() + 3 * f, w.ob_vel.begin() + 3 * f + 3);
-        o.ob_sqrt.insert(o.ob_sqrt.end(), w.ob_sqrt.begin() + 3 * f, w.ob_sqrt.begin() + 3 * f + 3);
-        o.ob_tref.push_back(w.ob_tref[f]);
-    }
-    o.F = (int)o.ob_lm.size();
-    o.imu_from.clear(); o.imu_to.clear(); o.imu.clear();
-    for (int e = 0; e < w.E; ++e) {
-        if (!keep_e[e] || w.imu_from[e] == drop || w.imu_to[e] == drop) continue;
-        o.imu_from.push_back(slot(w.imu_from[e])); o.imu_to.push_back(slot(w.imu_to[e]));
-        o.imu.insert(o.imu.end(), w.imu.begin() + (size_t)VIB_IMU_STRIDE * e, w.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
-    }
-    o.E = (int)o.imu_from.size();
-    return o;
-}
-
 // Eigendecomposition of a symmetric n x n matrix on the device (k_sym_eigen): eigenvalues
 // ascending, eigenvectors as the columns of U (row-major).
 int device_eigen(vib_context* ctx, const std::vector<double>& M, int n, std::vector<double>& ev, std::vector<double>& U) {
@@ -1120,27 +1285,26 @@ struct SchurPrior {
     std::vector<double> ref, A, r0;
 };
 
-// Schur elimination of slot `victim` (pose, speed/bias) together with the landmarks of `elim`
-// (all their factors) and the IMU edges `imu_used` into a square-root prior on the separator;
-// the previous prior always takes part (Marginalizer::compute, Marginalizer.cpp:284-474).
-// Consumed visual factors linearize on the device, the landmarks are eliminated there with the
-// diagonal pseudo-inverse and the reduced system is assembled on the device; the victim block
-// is eliminated on the host (eigen pseudo-inverse) and the prior's square root is the
-// semidefinite tile Cholesky on the device.
-int schur_prior(vib_context* ctx, const HostWindow& cur, int victim, const std::vector<char>& elim,
-                const std::vector<int>& imu_used, SchurPrior& out) {
+// Schur elimination of slot `victim` (pose, speed/bias) together with the landmarks selected by
+// d_sel (device flags; all their factors) and the IMU edges (from, to, record) into a
+// square-root prior on the separator; the previous prior always takes part
+// (Marginalizer::compute, Marginalizer.cpp:284-474). frames / observed: the frames the selected
+// landmarks touch (anchor and observers) and whether any of them has an observation
+// (k_touch_frames). The marginal system is a device edit of the resident window into the
+// context's marginal slot; its factors linearize on the device, the landmarks are eliminated
+// there with the diagonal pseudo-inverse and the reduced system is assembled on the device; the
+// victim block's eigen pseudo-inverse and the prior's eigen square root use k_sym_eigen.
+int schur_prior(vib_context* ctx, Slot& src, const EditBufs& eb, const uint8_t* d_sel, int frames, bool observed,
+                int victim, const std::vector<int>& e_from, const std::vector<int>& e_to, const std::vector<double>& e_rec,
+                SchurPrior& out) {
+    const HostWindow& cur = src.host;
     const int K = cur.K;
     std::vector<char> tpose(K, 0), tsb(K, 0);
-    bool rig_used = false, td_used = false;
-    for (int l = 0; l < cur.L; ++l) {
-        if (!elim[l]) continue;
-        tpose[cur.lm_frame[l]] = 1;
-        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) tpose[cur.ob_fc[f] >> 8] = 1;
-        if (cur.lm_off[l + 1] > cur.lm_off[l]) rig_used = td_used = true;
-    }
-    for (int e : imu_used) {
-        tpose[cur.imu_from[e]] = tpose[cur.imu_to[e]] = 1;
-        tsb[cur.imu_from[e]] = tsb[cur.imu_to[e]] = 1;
+    bool rig_used = observed, td_used = observed;
+    for (int k = 0; k < K; ++k) if ((frames >> k) & 1) tpose[k] = 1;
+    for (size_t e = 0; e < e_from.size(); ++e) {
+        tpose[e_from[e]] = tpose[e_to[e]] = 1;
+        tsb[e_from[e]] = tsb[e_to[e]] = 1;
     }
     for (size_t b = 0; b < cur.prior_kind.size(); ++b) {
         int slot = -1;
@@ -1153,52 +1317,23 @@ int schur_prior(vib_context* ctx, const HostWindow& cur, int victim, const std::
         }
     }
     // the marginal system: consumed factors only, every touched block a variable
-    HostWindow m;
-    m.K = K; m.C = cur.C;
-    m.flags = (rig_used ? VIB_RIG_FREE : 0u) | (td_used ? VIB_TD_FREE : 0u);
-    m.huber = cur.huber; m.min_rho = cur.min_rho; m.td_lo = -1e300; m.td_hi = 1e300;
-    m.frame_id = cur.frame_id; m.pose = cur.pose; m.sb = cur.sb;
-    std::copy_n(cur.rig, 7, m.rig);
-    m.td = cur.td; m.cam = cur.cam;
-    m.pose_free.assign(tpose.begin(), tpose.end());
-    m.sb_width.assign(K, 0);
-    for (int k = 0; k < K; ++k) m.sb_width[k] = tsb[k] ? 9 : 0;
-    std::vector<int> keep_lm;
-    for (int l = 0; l < cur.L; ++l) if (elim[l]) keep_lm.push_back(l);
-    m.L = (int)keep_lm.size();
-    m.lm_off.assign(1, 0);
-    m.lm_frame.resize(m.L); m.lm_cam.resize(m.L); m.lm_free.assign(m.L, 1); m.rho.resize(m.L);
-    m.lmd.assign((size_t)LM_PLANES * m.L, 0);
-    std::vector<int> keep_f;
-    for (int i = 0; i < m.L; ++i) {
-        const int l = keep_lm[i];
-        m.lm_frame[i] = cur.lm_frame[l]; m.lm_cam[i] = cur.lm_cam[l]; m.rho[i] = cur.rho[l];
-        for (int p = 0; p < LM_PLANES; ++p) m.lmd[(size_t)p * m.L + i] = cur.lmd[(size_t)p * cur.L + l];
-        for (int f = cur.lm_off[l]; f < cur.lm_off[l + 1]; ++f) keep_f.push_back(f);
-        m.lm_off.push_back((int)keep_f.size());
-    }
-    m.F = (int)keep_f.size();
-    m.ob_fc.resize(m.F); m.ob_lm.resize(m.F);
-    m.ob.assign((size_t)OB_PLANES * m.F, 0);
-    for (int i = 0, li = 0; i < m.F; ++i) {
-        while (i >= m.lm_off[li + 1]) ++li;
-        const int f = keep_f[i];
-        m.ob_fc[i] = cur.ob_fc[f];
-        m.ob_lm[i] = li;
-        for (int p = 0; p < OB_PLANES; ++p) m.ob[(size_t)p * m.F + i] = cur.ob[(size_t)p * cur.F + f];
-    }
-    for (int e : imu_used) {
-        m.imu_from.push_back(cur.imu_from[e]);
-        m.imu_to.push_back(cur.imu_to[e]);
-        m.imu.insert(m.imu.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e,
-                     cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
-    }
-    m.E = (int)imu_used.size();
-    m.prior_rows = cur.prior_rows; m.prior_cols = cur.prior_cols;
-    m.prior_kind = cur.prior_kind; m.prior_id = cur.prior_id;
-    m.prior_ref = cur.prior_ref; m.prior_A = cur.prior_A; m.prior_r0 = cur.prior_r0;
+    HostWindow mm = frame_meta(cur);
+    mm.flags = (rig_used ? VIB_RIG_FREE : 0u) | (td_used ? VIB_TD_FREE : 0u);
+    mm.td_lo = -1e300; mm.td_hi = 1e300;
+    mm.pose_free.assign(tpose.begin(), tpose.end());
+    mm.sb_width.assign(K, 0);
+    for (int k = 0; k < K; ++k) mm.sb_width[k] = tsb[k] ? 9 : 0;
+    mm.imu_from = e_from; mm.imu_to = e_to; mm.imu = e_rec;
+    mm.E = (int)e_from.size();
+    EditSpec sp{};
+    for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < K ? k : -1;
+    sp.lm_mode = 1;  // every eliminated inverse range is a variable
+    sp.keep_lm = d_sel;
+    if (!ctx->marginal_slot) ctx->marginal_slot = std::make_unique<Slot>();
+    Slot& tmp = *ctx->marginal_slot;  // its arena is kept from slide to slide
+    if (const int st = edit_window(ctx, src, sp, std::move(mm), tmp, eb, false)) return st;
+    const HostWindow& m = tmp.host;
     std::string err;
-    if (!m.layout(err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
     const int nr = m.nr;
     // eliminated state columns (victim pose and speed/bias) and retained columns in layout order
     std::vector<int> victim_cols, kept_cols;
@@ -1210,9 +1345,6 @@ int schur_prior(vib_context* ctx, const HostWindow& cur, int victim, const std::
     const int nvict = (int)victim_cols.size(), np = (int)kept_cols.size();
     if (nvict == 0) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "no eliminated state block");
     // device: linearize the consumed factors, landmark pseudo-inverse Schur, reduced system
-    if (!ctx->marginal_slot) ctx->marginal_slot = std::make_unique<Slot>();
-    Slot& tmp = *ctx->marginal_slot;  // its arena is kept from slide to slide
-    if (const int st = tmp.upload(m, ctx->stream, err)) return fail(ctx, st, err);
     LmState lm{};
     lm.lambda = 0; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1; lm.evaluate_only = 1;
     lm.marginal = 1;
@@ -1327,54 +1459,69 @@ int schur_prior(vib_context* ctx, const HostWindow& cur, int victim, const std::
     return VIB_OK;
 }
 
-void set_prior(WindowCopy& w, const SchurPrior& p) {
-    w.prior_rows = p.rows; w.prior_kind = p.kind; w.prior_id = p.id; w.prior_ref = p.ref;
-    w.prior_A = p.A; w.prior_r0 = p.r0;
+// The prior of an edited window: the square root of a marginalization (no rows: no prior).
+void set_prior(HostWindow& m, const SchurPrior& p) {
+    m.prior_rows = p.rows;
+    if (p.rows == 0) {
+        m.prior_cols = 0;
+        m.prior_kind.clear(); m.prior_id.clear(); m.prior_ref.clear(); m.prior_A.clear(); m.prior_r0.clear();
+        return;
+    }
+    m.prior_cols = p.cols;
+    m.prior_kind = p.kind; m.prior_id = p.id; m.prior_ref = p.ref; m.prior_A = p.A; m.prior_r0 = p.r0;
 }
 
-// Makes `w` the resident window of slot s.
-int install(vib_context* ctx, Slot& s, WindowCopy& w) {
-    HostWindow h;
-    std::string err;
-    const vib_window v = w.view();
-    if (const int st = h.build(v, err)) return fail(ctx, st, err);
-    if (const int st = s.upload(h, ctx->stream, err)) return fail(ctx, st, err);
-    return VIB_OK;
+// The frames of the landmarks selected by d_sel (k_touch_frames): bit mask and whether any of
+// them has an observation.
+bool touched_frames(vib_context* ctx, const EditBufs& b, int L, const uint8_t* d_sel, int& frames, bool& observed) {
+    int out[2] = {0, 0};
+    const bool ok = vibk_touch_frames(b.src, L, d_sel, b.sel + 8, ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(out, b.sel + 8, sizeof(out), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                    hipStreamSynchronize(ctx->stream) == hipSuccess;
+    frames = out[0];
+    observed = out[1] != 0;
+    return ok;
 }
 
-// OLD: slot 0 with every landmark anchored in or observed by it and its IMU edges.
+// OLD: slot 0 with every landmark anchored in or observed by it and its IMU edges. The
+// eliminated set and the frames it touches are found on the device; the marginal system and
+// the slid window are device edits of the resident window.
 int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
-    HostWindow cur;
-    if (!pull_state(ctx, s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    if (!refresh_state(ctx, s)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    const HostWindow& cur = s.host;
     const int victim = 0;
     if (cur.K < 2) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "marginalization needs at least two frames");
-    std::vector<char> elim(cur.L, 0);  // the eliminated landmarks, selected on the device
-    if (cur.L > 0) {
-        std::vector<Slot*> one{&s};
-        Batch& b = ctx->one;
-        b.mixed = 0;
-        std::string err;
-        uint8_t* d_elim = static_cast<uint8_t*>(grow_device(ctx->aux_buf, ctx->aux_cap, cur.L));
-        if (!d_elim) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "eliminated-set buffer allocation failed");
-        if (!b.build(one, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
-        const bool ok = vibk_elim_set(b.d_wins, cur.L, victim, d_elim, ctx->stream) == hipSuccess &&
-                        hipMemcpyAsync(elim.data(), d_elim, cur.L, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-                        hipStreamSynchronize(ctx->stream) == hipSuccess;
-        if (!ok) return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
-    }
-    std::vector<int> imu_used;
-    for (int e = 0; e < cur.E; ++e) if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) imu_used.push_back(e);
+    EditBufs b;
+    if (!edit_buffers(ctx, cur.L, cur.F, b)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "slide buffer allocation failed");
+    int frames = 0;
+    bool observed = false;
+    if (!edit_begin(ctx, s, b) ||
+        (cur.L && vibk_elim_set(b.src, cur.L, victim, b.marg_lm, ctx->stream) != hipSuccess) ||
+        !touched_frames(ctx, b, cur.L, b.marg_lm, frames, observed))
+        return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
+    std::vector<int> e_from, e_to;
+    std::vector<double> e_rec;
+    for (int e = 0; e < cur.E; ++e)
+        if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) {
+            e_from.push_back(cur.imu_from[e]);
+            e_to.push_back(cur.imu_to[e]);
+            e_rec.insert(e_rec.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e,
+                         cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+        }
     SchurPrior p;
-    const int st = schur_prior(ctx, cur, victim, elim, imu_used, p);
+    int st = schur_prior(ctx, s, b, b.marg_lm, frames, observed, victim, e_from, e_to, e_rec, p);
     if (st != VIB_OK) return st;
     // slide: drop slot 0, its landmarks, factors and IMU edges
-    WindowCopy w = copy_of(cur);
-    std::vector<char> keep_lm(cur.L), keep_f(cur.F, 1), keep_e(cur.E, 1);
-    for (int l = 0; l < cur.L; ++l) keep_lm[l] = !elim[l];
-    WindowCopy next = filter_window(w, victim, keep_lm, keep_f, keep_e);
+    HostWindow next = frame_meta(cur);
+    drop_frame(next, victim);
+    keep_edges(next, cur.imu_from, cur.imu_to, cur.imu, std::vector<char>(cur.E, 1), victim);
     set_prior(next, p);
-    const int up = install(ctx, s, next);
-    if (up != VIB_OK) return up;
+    EditSpec sp{};
+    for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < cur.K ? (k == victim ? -1 : (k > victim ? k - 1 : k)) : -1;
+    sp.keep_lm = b.marg_lm;
+    sp.keep_lm_invert = 1;
+    st = edit_window(ctx, s, sp, std::move(next), s, b, true);
+    if (st != VIB_OK) return st;
     rep->status = VIB_OK;
     rep->prior_reused = 0;
     rep->eliminated_dimension = p.eliminated;
@@ -1383,17 +1530,8 @@ int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
     rep->eliminated_landmarks = p.landmarks;
     rep->consumed_factors = p.consumed;
     rep->eigen_cutoff = p.cutoff;
-    rep->imu_mode = imu_used.empty() ? VIB_IMU_MERGE_NONE : VIB_IMU_MERGE_BRIDGE;
+    rep->imu_mode = e_from.empty() ? VIB_IMU_MERGE_NONE : VIB_IMU_MERGE_BRIDGE;
     return VIB_OK;
-}
-
-// World pose of camera c of slot k: T_WB T_BR T_RC.
-void camera_world(const WindowCopy& w, int k, int c, M3<double>& R, V3<double>& t) {
-    const M3<double> Rb = quat_to_matrix(&w.pose[7 * k + 3]), Re = quat_to_matrix(w.rig + 3);
-    M3<double> Rc;
-    for (int i = 0; i < 9; ++i) Rc.m[i] = w.cam_R[9 * c + i];
-    R = mul(mul(Rb, Re), Rc);
-    t = mul(Rb, mul(Re, load3(&w.cam_t[3 * c])) + load3(w.rig)) + load3(&w.pose[7 * k]);
 }
 
 // Linear interpolation of the gyroscope at t (gyroAt, BodyImuFactor.cpp:29-38).
@@ -1414,32 +1552,6 @@ void gyro_at(const double* samples, int count, double t, double* out) {
 // landmarks anchored in K-2 are re-anchored on a surviving view, demoted (one surviving ray) or
 // retired into the prior, the slot's other image measurements are discarded, and the prior is
 // reused when nothing else touches the slot.
-// k_reanchor over the listed landmarks of the resident window (anchored in slot v): the chosen
-// view (landmark-order observation, -1: none) and its inverse range per landmark.
-bool reanchor_on_device(vib_context* ctx, Slot& s, int v, const std::vector<int32_t>& lms, std::vector<int32_t>& chosen,
-                        std::vector<double>& rho) {
-    const size_t n = lms.size();
-    chosen.assign(n, -1);
-    rho.assign(n, 0.0);
-    if (n == 0) return true;
-    std::vector<Slot*> one{&s};
-    Batch& b = ctx->one;
-    b.mixed = 0;
-    std::string err;
-    if (!b.build(one, ctx->stream, err)) return false;
-    char* buf = static_cast<char*>(grow_device(ctx->aux_buf, ctx->aux_cap, n * (sizeof(double) + 2 * sizeof(int32_t))));
-    if (!buf) return false;
-    double* d_rho = reinterpret_cast<double*>(buf);
-    int32_t* d_lms = reinterpret_cast<int32_t*>(d_rho + n);
-    int32_t* d_chosen = d_lms + n;
-    const bool ok = hipMemcpyAsync(d_lms, lms.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
-                    vibk_reanchor(b.d_wins, v, d_lms, (int)n, d_chosen, d_rho, ctx->stream) == hipSuccess &&
-                    hipMemcpyAsync(chosen.data(), d_chosen, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-                    hipMemcpyAsync(rho.data(), d_rho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-                    hipStreamSynchronize(ctx->stream) == hipSuccess;
-    return ok;
-}
-
 int preintegrate_locked(vib_context* ctx, int32_t count, const double* samples, int32_t num_samples,
                         const int32_t* sample_begin, const int32_t* sample_end, const double* t0, const double* t1,
                         const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
@@ -1447,30 +1559,30 @@ int preintegrate_locked(vib_context* ctx, int32_t count, const double* samples, 
 
 int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int count, double t0, double t1,
                            const vib_imu_params* params, vib_marginal_report* rep) {
-    HostWindow cur;
-    if (!pull_state(ctx, s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    if (!refresh_state(ctx, s)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    const HostWindow& cur = s.host;
     if (cur.K < 3) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "SECOND_NEW needs at least three frames");
-    WindowCopy w = copy_of(cur);
-    const int K = w.K, v = K - 2, left = K - 3, right = K - 1;
+    const int K = cur.K, v = K - 2, left = K - 3, right = K - 1;
     // IMU edges of the slot (SlidingWindowManager.cpp:275-311)
     std::vector<int> incident;
-    for (int e = 0; e < w.E; ++e) if (w.imu_from[e] == v || w.imu_to[e] == v) incident.push_back(e);
+    for (int e = 0; e < cur.E; ++e) if (cur.imu_from[e] == v || cur.imu_to[e] == v) incident.push_back(e);
     bool sb_in_prior = false, prior_touches = false;
-    for (size_t b = 0; b < w.prior_kind.size(); ++b)
-        if ((w.prior_kind[b] == VIB_KEY_POSE || w.prior_kind[b] == VIB_KEY_SPEED_BIAS) && w.prior_id[b] == w.frame_id[v]) {
+    for (size_t b = 0; b < cur.prior_kind.size(); ++b)
+        if ((cur.prior_kind[b] == VIB_KEY_POSE || cur.prior_kind[b] == VIB_KEY_SPEED_BIAS) &&
+            cur.prior_id[b] == cur.frame_id[v]) {
             prior_touches = true;
-            if (w.prior_kind[b] == VIB_KEY_SPEED_BIAS) sb_in_prior = true;
+            if (cur.prior_kind[b] == VIB_KEY_SPEED_BIAS) sb_in_prior = true;
         }
     int imu_mode = incident.empty() ? VIB_IMU_MERGE_NONE : VIB_IMU_MERGE_BRIDGE;
-    std::vector<char> keep_e(w.E, 1);
+    std::vector<char> keep_e(cur.E, 1);
     std::vector<double> merged;
     if (incident.size() == 2 && !sb_in_prior && samples && count > 0 && params) {
         int a = incident[0], b = incident[1];
-        if (w.imu_from[a] != left) std::swap(a, b);
-        if (w.imu_from[a] != left || w.imu_to[a] != v || w.imu_from[b] != v || w.imu_to[b] != right)
+        if (cur.imu_from[a] != left) std::swap(a, b);
+        if (cur.imu_from[a] != left || cur.imu_to[a] != v || cur.imu_from[b] != v || cur.imu_to[b] != right)
             return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "IMU merge endpoint/layout mismatch");
-        const double* da = &w.imu[(size_t)VIB_IMU_STRIDE * a];
-        const double* db = &w.imu[(size_t)VIB_IMU_STRIDE * b];
+        const double* da = &cur.imu[(size_t)VIB_IMU_STRIDE * a];
+        const double* db = &cur.imu[(size_t)VIB_IMU_STRIDE * b];
         if (!(t1 > t0) || std::fabs((t1 - t0) - (da[0] + db[0])) > 1e-9)
             return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "IMU merge interval does not span the two edges");
         if (count > 10000) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "merged IMU interval exceeds sample budget");
@@ -1487,7 +1599,7 @@ int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int
             // the left edge's reference bias is kept while the estimate is within 0.0003 rad/s
             // (gyro) and 0.02 m/s^2 (accel) of it (BodyImuFactor.cpp:146-170)
             double lin[9];
-            std::copy_n(&w.sb[9 * left], 9, lin);
+            std::copy_n(&cur.sb[9 * left], 9, lin);
             double dg = 0, dacc = 0;
             for (int i = 0; i < 3; ++i) {
                 dg += (lin[3 + i] - da[25 + i]) * (lin[3 + i] - da[25 + i]);
@@ -1509,98 +1621,82 @@ int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int
             imu_mode = incremental ? VIB_IMU_MERGE_INCREMENTAL : VIB_IMU_MERGE_REINTEGRATED;
         }
     }
+    // the IMU edges after the merge: the kept ones, then the merged edge
+    std::vector<int> w_from, w_to;
+    std::vector<double> w_rec;
+    for (int e = 0; e < cur.E; ++e) {
+        if (!keep_e[e]) continue;
+        w_from.push_back(cur.imu_from[e]);
+        w_to.push_back(cur.imu_to[e]);
+        w_rec.insert(w_rec.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e, cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+    }
     if (!merged.empty()) {
-        w.imu_from.push_back(left); w.imu_to.push_back(right);
-        w.imu.insert(w.imu.end(), merged.begin(), merged.end());
-        w.E += 1;
-        keep_e.push_back(1);
+        w_from.push_back(left);
+        w_to.push_back(right);
+        w_rec.insert(w_rec.end(), merged.begin(), merged.end());
     }
-    // landmarks anchored in the slot (SlidingWindowManager.cpp:312-457)
-    std::vector<std::vector<int>> of_lm(w.L);
-    for (int f = 0; f < w.F; ++f) of_lm[w.ob_lm[f]].push_back(f);
-    std::vector<char> keep_lm(w.L, 1), keep_f(w.F, 1), retired(w.L, 0);
-    int reset = 0, demoted = 0, dropped = 0;
-    std::vector<int32_t> pending;  // landmarks with two or more surviving views
-    for (int l = 0; l < w.L; ++l) {
-        if (w.lm_frame[l] != v) continue;
-        std::vector<int> surviving;
-        bool eligible = sqnorm(load3(&w.lm_vel[3 * l])) == 0;
-        for (int f : of_lm[l]) {
-            eligible = eligible && sqnorm(load3(&w.ob_vel[3 * f])) == 0;
-            if (w.ob_frame[f] != v) surviving.push_back(f);
-        }
-        if (surviving.empty() || !eligible || !w.lm_free[l]) { retired[l] = 1; continue; }
-        if (surviving.size() == 1) {
-            keep_lm[l] = 0;
-            for (int f : of_lm[l]) keep_f[f] = 0;
-            ++demoted;
-            continue;
-        }
-        pending.push_back(l);  // anchor search on the device, below
+    // landmarks anchored in the slot (SlidingWindowManager.cpp:312-457) on the device: classes,
+    // the anchor search (preferred anchors: newer frames first, then lower camera index; every
+    // retained frame counts as a keyframe), keep flags of the next window and of the marginal
+    // system; the slot's image measurements of landmarks anchored elsewhere are discarded
+    // (:458-470)
+    EditBufs b;
+    if (!edit_buffers(ctx, cur.L, cur.F, b)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "slide buffer allocation failed");
+    int cnt[4] = {0, 0, 0, 0};
+    int frames = 0;
+    bool observed = false;
+    bool ok = edit_begin(ctx, s, b);
+    if (ok && cur.L) {
+        ok = vibk_second_new_classify(b.src, cur.L, v, b.cls, ctx->stream) == hipSuccess &&
+             vibk_reanchor(b.src, v, nullptr, b.cls, cur.L, b.chosen, b.chosen_rho, ctx->stream) == hipSuccess &&
+             vibk_second_new_flags(b.src, cur.L, v, b.cls, b.chosen, b.chosen_rho, b.keep_lm, b.keep_f, b.over_f,
+                                   b.over_rho, b.marg_lm, b.sel, ctx->stream) == hipSuccess &&
+             hipMemcpyAsync(cnt, b.sel, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+             touched_frames(ctx, b, cur.L, b.marg_lm, frames, observed);
     }
-    // anchor search (preferred anchors: newer frames first, then lower camera index; every
-    // retained frame counts as a keyframe), one device thread per landmark
-    std::vector<int32_t> chosen_view;
-    std::vector<double> chosen_rho;
-    if (!reanchor_on_device(ctx, s, v, pending, chosen_view, chosen_rho))
-        return fail(ctx, VIB_ERR_DEVICE, "anchor search launch failed");
-    for (size_t i = 0; i < pending.size(); ++i) {
-        const int l = pending[i], chosen = chosen_view[i];
-        if (chosen < 0) { retired[l] = 1; continue; }
-        w.lm_frame[l] = w.ob_frame[chosen]; w.lm_cam[l] = w.ob_cam[chosen];
-        for (int k = 0; k < 3; ++k) {
-            w.lm_bearing[3 * l + k] = w.ob_bearing[3 * chosen + k];
-            w.lm_vel[3 * l + k] = w.ob_vel[3 * chosen + k];
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "slide topology launch failed");
+    // IMU edges incident to the slot after the merge are eliminated with it
+    std::vector<int> m_from, m_to;
+    std::vector<double> m_rec;
+    for (size_t e = 0; e < w_from.size(); ++e)
+        if (w_from[e] == v || w_to[e] == v) {
+            m_from.push_back(w_from[e]);
+            m_to.push_back(w_to[e]);
+            m_rec.insert(m_rec.end(), w_rec.begin() + (size_t)VIB_IMU_STRIDE * e, w_rec.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
         }
-        w.lm_tref[l] = w.ob_tref[chosen];
-        w.rho[l] = chosen_rho[i];
-        keep_f[chosen] = 0;
-        for (int f : of_lm[l]) if (w.ob_frame[f] == v) keep_f[f] = 0;
-        ++reset;
-    }
-    // the slot's image measurements of landmarks anchored elsewhere are discarded (:458-470)
-    for (int f = 0; f < w.F; ++f)
-        if (w.ob_frame[f] == v && w.lm_frame[w.ob_lm[f]] != v && keep_f[f]) { keep_f[f] = 0; ++dropped; }
-    WindowCopy w1 = filter_window(w, -1, keep_lm, keep_f, keep_e);
-    std::vector<char> ret1;
-    for (int l = 0; l < w.L; ++l) if (keep_lm[l]) ret1.push_back(retired[l]);
-    std::vector<int> imus;
-    for (int e = 0; e < w1.E; ++e) if (w1.imu_from[e] == v || w1.imu_to[e] == v) imus.push_back(e);
-    bool consumed = !imus.empty();
-    int nret = 0;
-    for (int f = 0; f < w1.F; ++f) consumed = consumed || ret1[w1.ob_lm[f]];
-    for (char r : ret1) nret += r;
-    std::vector<char> keep_lm1(w1.L), keep_f1(w1.F, 1), keep_e1(w1.E, 1);
-    for (int l = 0; l < w1.L; ++l) keep_lm1[l] = !ret1[l];
-    WindowCopy next = filter_window(w1, v, keep_lm1, keep_f1, keep_e1);
+    const bool consumed = !m_from.empty() || observed;
     *rep = vib_marginal_report{};
     rep->imu_mode = imu_mode;
-    rep->reset_landmarks = reset;
-    rep->demoted_landmarks = demoted;
-    rep->dropped_factors = dropped;
-    rep->eliminated_landmarks = nret;
+    rep->reset_landmarks = cnt[0];
+    rep->demoted_landmarks = cnt[1];
+    rep->dropped_factors = cnt[2];
+    rep->eliminated_landmarks = cnt[3];
+    HostWindow next = frame_meta(cur);
+    drop_frame(next, v);
+    keep_edges(next, w_from, w_to, w_rec, std::vector<char>(w_from.size(), 1), v);
+    EditSpec sp{};
+    for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < K ? (k == v ? -1 : (k > v ? k - 1 : k)) : -1;
+    sp.keep_lm = b.keep_lm;
+    sp.keep_f = b.keep_f;
+    sp.over_f = b.over_f;
+    sp.over_rho = b.over_rho;
     if (!consumed && !prior_touches) {
         // Marginalizer.cpp:299-310: nothing incident is left, the frozen prior is kept as it is
-        const int up = install(ctx, s, next);
+        const int up = edit_window(ctx, s, sp, std::move(next), s, b, true);
         if (up != VIB_OK) return up;
+        const HostWindow& h = s.host;
         rep->status = VIB_OK;
         rep->prior_reused = 1;
-        rep->retained_dimension = next.prior_rows ? (int)(next.prior_A.size() / next.prior_rows) : 0;
-        rep->prior_rank = next.prior_rows;
+        rep->retained_dimension = h.prior_rows ? (int)(h.prior_A.size() / h.prior_rows) : 0;
+        rep->prior_rank = h.prior_rows;
         return VIB_OK;
     }
-    HostWindow h1;
-    std::string err;
-    {
-        const vib_window view = w1.view();
-        if (const int st = h1.build(view, err)) return fail(ctx, st, err);
-    }
     SchurPrior p;
-    const int st = schur_prior(ctx, h1, v, ret1, imus, p);
+    int st = schur_prior(ctx, s, b, b.marg_lm, frames, observed, v, m_from, m_to, m_rec, p);
     if (st != VIB_OK) return st;
     set_prior(next, p);
-    const int up = install(ctx, s, next);
-    if (up != VIB_OK) return up;
+    st = edit_window(ctx, s, sp, std::move(next), s, b, true);
+    if (st != VIB_OK) return st;
     rep->status = VIB_OK;
     rep->eliminated_dimension = p.eliminated;
     rep->retained_dimension = p.cols;
@@ -1870,8 +1966,10 @@ int vib_accept(vib_context* ctx, int32_t slot) {
 
 int vib_read_step(vib_context* ctx, int32_t slot, double* delta, int32_t capacity) {
     if (!ctx || !delta) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    if (!ensure_bulk(ctx, *s)) return fail(ctx, VIB_ERR_DEVICE, "window download failed");
     const HostWindow& h = s->host;
     if (capacity < h.nr + h.ne) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "capacity too small");
     std::vector<double> dl(h.L);
@@ -2162,7 +2260,7 @@ static int nis_gate_core(vib_context* ctx, Slot* s, double threshold, int apply,
     fill(rep->admitted, st + 9);
     rep->skipped = skipped;
     rep->invalid = invalid;
-    if (reject) {  // the host mirror follows the device
+    if (reject && h.bulk) {  // a downloaded host mirror follows the device
         const size_t Fs = F;
         for (int f = 0; f < F; ++f) {
             if (dec[f] != 1) continue;
@@ -2172,8 +2270,8 @@ static int nis_gate_core(vib_context* ctx, Slot* s, double threshold, int apply,
                 h.pf_ob[q * Fs + p] = 0.0;
             }
         }
-        rep->rejected = over;
     }
+    if (reject) rep->rejected = over;
     if (dec_out) std::memcpy(dec_out, dec.data(), F);
     rep->status = invalid ? VIB_ERR_INVALID_RESIDUAL : VIB_OK;
     return invalid ? fail(ctx, VIB_ERR_INVALID_RESIDUAL, "measurement evaluation failed/nonfinite before robust loss")
@@ -2437,18 +2535,66 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
     *rep = vib_latest_two_report{};
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
-    HostWindow cur;
-    if (!pull_state(ctx, *s, cur)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
-    LatestTwo lt;
-    std::string err;
-    int st = latest_two_window(cur, lt, err);
-    rep->support_landmarks = lt.support;
-    if (st != VIB_OK) return fail(ctx, st, err);
+    // Local graph of the two-frame solve (BackendOptimizer.cpp:476-538): factors touching the pose
+    // or speed/bias of slots K-2, K-1; those blocks free with the velocity-only speed/bias
+    // tangent, everything else constant. A device edit of the resident window into the
+    // context's two-frame slot.
+    if (!refresh_state(ctx, *s)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    const HostWindow& cur = s->host;
+    if (cur.K < 2) return fail(ctx, VIB_ERR_LIGHTWEIGHT_SUPPORT, "missing pose or velocity block in two-frame solve");
+    const int prev = cur.K - 2, last = cur.K - 1;
+    auto is_free = [&](int k) { return k == prev || k == last; };
+    EditBufs eb;
+    if (!edit_buffers(ctx, cur.L, cur.F, eb)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "two-frame buffer allocation failed");
+    int support = 0;
+    if (!edit_begin(ctx, *s, eb) ||
+        vibk_two_frame_select(eb.src, cur.L, prev, last, eb.keep_lm, eb.keep_f, eb.sel, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(&support, eb.sel, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(ctx, VIB_ERR_DEVICE, "two-frame selection failed");
+    rep->support_landmarks = support;
+    if (support < 6) return fail(ctx, VIB_ERR_LIGHTWEIGHT_SUPPORT, "fewer than six current landmarks in two-frame solve");
+    HostWindow w = frame_meta(cur);
+    w.flags = cur.flags & ~(VIB_RIG_FREE | VIB_TD_FREE);
+    w.pose_free.assign(cur.K, 0);
+    w.sb_width.assign(cur.K, 0);
+    w.pose_free[prev] = w.pose_free[last] = 1;
+    w.sb_width[prev] = w.sb_width[last] = 3;
+    // the IMU bridge: exactly the edge ending at K-2 and the edge K-2 -> K-1
+    int edges[2] = {-1, -1};
+    w.imu_from.clear(); w.imu_to.clear(); w.imu.clear();
+    for (int e = 0; e < cur.E; ++e) {
+        if (!is_free(cur.imu_from[e]) && !is_free(cur.imu_to[e])) continue;
+        const int i = (int)w.imu_from.size();
+        w.imu_from.push_back(cur.imu_from[e]);
+        w.imu_to.push_back(cur.imu_to[e]);
+        w.imu.insert(w.imu.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e, cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+        if (cur.imu_to[e] == prev) edges[0] = i;
+        if (cur.imu_from[e] == prev && cur.imu_to[e] == last) edges[1] = i;
+    }
+    w.E = (int)w.imu_from.size();
+    if (w.E != 2 || edges[0] < 0 || edges[1] < 0) return fail(ctx, VIB_ERR_IMU_BRIDGE, "two-frame IMU bridge missing");
+    // the prior takes part when one of its blocks is a free block
+    bool touches = false;
+    for (size_t b = 0; b < cur.prior_kind.size(); ++b)
+        if ((cur.prior_kind[b] == VIB_KEY_POSE || cur.prior_kind[b] == VIB_KEY_SPEED_BIAS) &&
+            (cur.prior_id[b] == cur.frame_id[prev] || cur.prior_id[b] == cur.frame_id[last]))
+            touches = true;
+    if (!touches) {
+        w.prior_rows = w.prior_cols = 0;
+        w.prior_kind.clear(); w.prior_id.clear(); w.prior_ref.clear(); w.prior_A.clear(); w.prior_r0.clear();
+    }
+    EditSpec sp{};
+    for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < cur.K ? k : -1;
+    sp.lm_mode = 2;  // landmarks held constant
+    sp.keep_lm = eb.keep_lm;
+    sp.keep_f = eb.keep_f;
     if (!ctx->two_frame_slot) ctx->two_frame_slot = std::make_unique<Slot>();
     Slot& local = *ctx->two_frame_slot;  // its arena is kept from call to call
-    if (const int st = local.upload(lt.win, ctx->stream, err)) return fail(ctx, st, err);
+    int st = edit_window(ctx, *s, sp, std::move(w), local, eb, false);
+    if (st != VIB_OK) return st;
     double nis[2];
-    if (!imu_nis(ctx, local, lt.edges, nis)) return fail(ctx, VIB_ERR_DEVICE, "IMU residual evaluation failed");
+    if (!imu_nis(ctx, local, edges, nis)) return fail(ctx, VIB_ERR_DEVICE, "IMU residual evaluation failed");
     rep->old_imu_nis_before = nis[0];
     rep->new_imu_nis_before = nis[1];
     if (!std::isfinite(nis[0]) || !std::isfinite(nis[1]))
@@ -2463,7 +2609,7 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
     st = run_lm(ctx, one, lo, reps);
     if (!reps.empty()) rep->solve = reps[0];
     if (st != VIB_OK) return st;
-    if (!imu_nis(ctx, local, lt.edges, nis)) return fail(ctx, VIB_ERR_DEVICE, "IMU residual evaluation failed");
+    if (!imu_nis(ctx, local, edges, nis)) return fail(ctx, VIB_ERR_DEVICE, "IMU residual evaluation failed");
     rep->old_imu_nis_after = nis[0];
     rep->new_imu_nis_after = nis[1];
     if (!std::isfinite(nis[0]) || !std::isfinite(nis[1]))
@@ -2471,7 +2617,6 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
     if (gate && nis[1] > 60.0)
         return fail(ctx, VIB_ERR_IMU_GATE, "new IMU edge residual exceeded lightweight health gate");
     // write back poses and velocities of K-2 and K-1 (biases are constant in this solve)
-    const int prev = cur.K - 2;
     bool ok = hipMemcpyAsync(s->dev.pose + 7 * prev, local.dev.pose + 7 * prev, 14 * sizeof(double),
                              hipMemcpyDeviceToDevice, ctx->stream) == hipSuccess;
     for (int k = prev; k < prev + 2; ++k)
@@ -2488,32 +2633,22 @@ namespace {
 // (BackendOptimizer.cpp:1053-1054): the window is rebuilt without them, and without the landmarks
 // whose every factor was rejected, before the slide.
 static int strip_rejected(vib_context* ctx, Slot& s) {
-    {  // the mirror's square-root information follows every rejection: scan it before any download
-        const size_t F = s.host.F;
-        const double* ob = s.host.ob.data();
-        bool any = false;
-        for (size_t f = 0; f < F && !any; ++f)
-            any = ob[OB_S00 * F + f] == 0.0 && ob[OB_S10 * F + f] == 0.0 && ob[OB_S11 * F + f] == 0.0;
-        if (!any) return VIB_OK;
-    }
-    HostWindow h;  // the mirror with the current device state
-    if (!pull_state(ctx, s, h)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
-    const size_t F = h.F;
-    std::vector<char> keep_f(h.F, 1), keep_lm(h.L, 1), keep_e(h.E, 1);
+    const HostWindow& h = s.host;
+    if (h.L == 0 || h.F == 0) return VIB_OK;
+    EditBufs b;
+    if (!edit_buffers(ctx, h.L, h.F, b)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "slide buffer allocation failed");
     int rejected = 0;
-    for (size_t f = 0; f < F; ++f)
-        if (h.ob[OB_S00 * F + f] == 0.0 && h.ob[OB_S10 * F + f] == 0.0 && h.ob[OB_S11 * F + f] == 0.0) {
-            keep_f[f] = 0;
-            ++rejected;
-        }
+    if (!edit_begin(ctx, s, b) || vibk_strip_select(b.src, h.L, b.keep_lm, b.keep_f, b.sel, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(&rejected, b.sel, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(ctx, VIB_ERR_DEVICE, "rejected-factor scan failed");
     if (!rejected) return VIB_OK;
-    for (int l = 0; l < h.L; ++l) {
-        bool any = h.lm_off[l + 1] == h.lm_off[l];
-        for (int f = h.lm_off[l]; f < h.lm_off[l + 1]; ++f) any = any || keep_f[f];
-        keep_lm[l] = any;
-    }
-    WindowCopy next = filter_window(copy_of(h), -1, keep_lm, keep_f, keep_e);
-    return install(ctx, s, next);
+    if (!refresh_state(ctx, s)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    EditSpec sp{};
+    for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < h.K ? k : -1;
+    sp.keep_lm = b.keep_lm;
+    sp.keep_f = b.keep_f;
+    return edit_window(ctx, s, sp, frame_meta(h), s, b, true);
 }
 
 }  // namespace
@@ -2544,8 +2679,8 @@ int vib_read_window(vib_context* ctx, int32_t slot, vib_window* out) {
     hipSetDevice(ctx->device);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
-    HostWindow h;
-    if (!pull_state(ctx, *s, h)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    if (!ensure_bulk(ctx, *s) || !refresh_state(ctx, *s)) return fail(ctx, VIB_ERR_DEVICE, "window download failed");
+    const HostWindow& h = s->host;
     if (out->num_frames != h.K || out->num_cameras != h.C || out->num_landmarks != h.L ||
         out->num_observations != h.F || out->num_imu != h.E || out->prior_rows != h.prior_rows)
         return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "window shape mismatch");

@@ -65,11 +65,27 @@ struct HostWindow {
     std::vector<double> pf_ob, pf_an;
     std::vector<int32_t> pf_ids, pf_lm, pf_pos;
     std::vector<Chunk> lin_chunks;  // factor ranges of the visual-normal kernel
+    // The per-landmark and per-observation vectors (lm_frame .. rho, ob_fc, ob_lm, ob, lm_col and
+    // the pair-order planes) hold the resident window. False after a device edit: the device
+    // holds them, and ensure_bulk downloads them when a host path needs them.
+    bool bulk = true;
 
     // VIB_OK, or VIB_ERR_INVALID_ARGUMENT (malformed), VIB_ERR_UNSUPPORTED_DIMENSIONS (beyond the
     // window limits) or VIB_ERR_UNSUPPORTED_LOSS, with the message in err
     int build(const vib_window& w, std::string& err);
     bool layout(std::string& err);
+    // the state-column part of layout: columns, prior blocks, retained column map, factor tiles
+    bool layout_states(std::string& err);
+    // the landmark chunks and visual factor ranges from L and F
+    void make_chunks();
+};
+
+// Device pointers of a slot's arena that the window descriptor does not carry.
+struct SlotCarve {
+    double* reduced_lin = nullptr;
+    double* reduced_schur = nullptr;
+    WinDev* gram_win = nullptr;  // descriptor and output pointer of the prior Gram launch
+    double** gram_out = nullptr;
 };
 
 // One device-resident window (a session's window and prior live here across solves).
@@ -90,6 +106,15 @@ struct Slot {
     ~Slot();
     void release();
     static size_t footprint(const HostWindow& h);
+    // Lays window h out over [base, base + bytes): the descriptor d and the other pointers x;
+    // false if the footprint does not fit.
+    static bool carve(const HostWindow& h, char* base, size_t bytes, WinDev& d, SlotCarve& x);
+    // Copies the host-held parts of window h into a carved arena (state and its trial copy,
+    // cameras, IMU, prior, retained column map, LM state; with streams also the landmark and
+    // observation streams in both orders), then forms the prior's Gram matrix; returns after
+    // `stream` has completed them, false if a copy or launch fails.
+    static bool fill(const HostWindow& h, const WinDev& d, const SlotCarve& x, char* base, size_t bytes, bool streams,
+                     hipStream_t stream);
     // VIB_OK, VIB_ERR_OUT_OF_MEMORY (allocation) or VIB_ERR_DEVICE (copies, launches)
     int upload(const HostWindow& h, hipStream_t stream, std::string& err);
 };

@@ -2977,6 +2977,7 @@ __global__ __launch_bounds__(kBlock) void k_whiten(int count, const double* px, 
 // landmark's world point at positive range, inverse range >= min_rho and within 0.03 rad, and
 // under which every other surviving observation still evaluates, becomes the new anchor.
 // chosen[i]: that observation (landmark order) or -1 (the landmark is retired); rho[i]: 1/range.
+// With lms null, i runs over every landmark and only those of class 3 in cls are searched.
 // ---------------------------------------------------------------------------------------------
 __device__ bool reanchor_view_valid(const WinDev& w, const FrameTable& ft, int f, int frame, int cam, V3<double> bearing,
                                     double rho) {
@@ -3004,15 +3005,20 @@ __device__ bool reanchor_view_valid(const WinDev& w, const FrameTable& ft, int f
     return isfinite(r0) && isfinite(r1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_reanchor(const WinDev* wins, int v, const int32_t* lms, int count,
-                                                     int32_t* chosen, double* new_rho) {
+__global__ __launch_bounds__(kBlock) void k_reanchor(const WinDev* wins, int v, const int32_t* lms, const uint8_t* cls,
+                                                     int count, int32_t* chosen, double* new_rho) {
     __shared__ FrameTable ft;
     const WinDev& w = wins[0];
     build_frame_table(w, w.pose, w.rig, ft);
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const int l = lms[i];
+    const int l = lms ? lms[i] : i;
+    if (!lms && cls[l] != 3) {
+        chosen[i] = -1;
+        new_rho[i] = 0;
+        return;
+    }
     const size_t L = (size_t)w.L, F = (size_t)w.F;
     const int ca = w.lm_fc[l] & 255;
     const M3<double> Ra = *reinterpret_cast<const M3<double>*>(ft.Rc[v][ca]);
@@ -3546,10 +3552,11 @@ hipError_t vibk_whiten(int count, const double* px, const int32_t* cam, const do
     return hipGetLastError();
 }
 
-hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, int count, int32_t* chosen,
-                         double* new_rho, hipStream_t s) {
+hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, const uint8_t* cls, int count,
+                         int32_t* chosen, double* new_rho, hipStream_t s) {
+    if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(vib::k_reanchor, dim3((count + vib::kBlock - 1) / vib::kBlock), dim3(vib::kBlock), 0, s, wins, v,
-                       lms, count, chosen, new_rho);
+                       lms, cls, count, chosen, new_rho);
     return hipGetLastError();
 }
 

@@ -1,0 +1,133 @@
+"""Window edits on the device (slides, stripping of gate rejections, the two-frame window)
+against the host layout.
+
+A slide builds the next window on the device: stream compaction of the landmark and observation
+streams and a stable (anchor frame, observer frame) sort of the pair-order planes. The same
+window read back (vib_read_window) and uploaded through the host path (vib_window_upload, which
+sorts on the host) must be the identical device layout, so the two slots take one LM step bit for
+bit alike (normal equations, reduced system, step, costs). Cases: OLD after a step on the 2000-landmark window,
+SECOND_NEW with the IMU merge on the 20000-landmark rig+td window, re-anchored and retired
+tracks, gate rejections stripped before a slide, and three slides in a row on one slot (the
+spare arena changes places each time). The topology and prior themselves are compared with the
+oracle in test_gpu_parity and test_gpu_fullsize.
+"""
+import numpy as np
+import pytest
+
+from parity_helpers import say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(abi):
+    c = abi.Context(0)
+    yield c
+    c.close()
+
+
+def _same_layout(ctx, abi, a, b):
+    """Slot a (device-edited) and its window uploaded to slot b step bit for bit alike."""
+    w = ctx.read_window(a)
+    ctx.upload(b, w)
+    lay = w.layout()
+    nr, ne = lay["nr"], lay["ne"]
+    for s in (a, b):
+        ctx.check(ctx.linearize(s), "linearize")
+    sa, ra = ctx.solve_step(a, 1e-4)
+    sb, rb = ctx.solve_step(b, 1e-4)
+    assert sa == 0 and sb == 0, ctx.error()
+    # the normal equations and reduced system of the step, the step, its trial cost
+    Ha, ga, ca = ctx.read_normal(a, nr)
+    Hb, gb, cb = ctx.read_normal(b, nr)
+    assert np.all(np.isfinite(Ha)) and np.abs(Ha).max() > 0
+    assert np.array_equal(Ha, Hb) and np.array_equal(ga, gb) and ca == cb
+    Sa, rha = ctx.read_reduced(a, nr)
+    Sb, rhb = ctx.read_reduced(b, nr)
+    assert np.array_equal(Sa, Sb) and np.array_equal(rha, rhb)
+    assert np.array_equal(ctx.read_step(a, nr + ne), ctx.read_step(b, nr + ne))
+    assert ra.cost == rb.cost and ra.trial_cost == rb.trial_cost and ra.predicted_reduction == rb.predicted_reduction
+    ctx.release(b)
+    return w
+
+
+def _second_new_args(w):
+    m = w.meta
+    ft = m["frame_t"]
+    return m["samples"], float(ft[w.K - 3]), float(ft[w.K - 1]), m["imu_params"]
+
+
+def test_old_after_a_step_keeps_the_host_layout(ctx, abi):
+    from vibpy import synth
+    w = synth.generate(**synth.CONFIGS["c1"])
+    say(f"[gpu] edit: OLD K={w.K} L={w.L} F={w.F}")
+    ctx.upload(0, w)
+    st, _ = ctx.solve(0, abi.options(max_iterations=1))
+    assert st == 0, ctx.error()
+    st, rep = ctx.marginalize(0, abi.VIB_MARGIN_OLD)
+    assert st == 0, ctx.error()
+    nxt = _same_layout(ctx, abi, 0, 1)
+    assert nxt.K == w.K - 1 and rep.eliminated_landmarks > 0 and nxt.L == w.L - rep.eliminated_landmarks
+
+
+def test_second_new_with_merge_keeps_the_host_layout(ctx, abi):
+    from vibpy import synth
+    w = synth.generate(**synth.CONFIGS["c2"])
+    say(f"[gpu] edit: SECOND_NEW K={w.K} L={w.L} F={w.F}")
+    ctx.upload(0, w)
+    st, _ = ctx.solve(0, abi.options(max_iterations=2))
+    assert st == 0, ctx.error()
+    smp, t0, t1, prm = _second_new_args(w)
+    st, rep = ctx.marginalize(0, abi.VIB_MARGIN_SECOND_NEW, smp, t0, t1, params=prm)
+    assert st == 0, ctx.error()
+    assert rep.imu_mode in (abi.VIB_IMU_MERGE_INCREMENTAL, abi.VIB_IMU_MERGE_REINTEGRATED)
+    nxt = _same_layout(ctx, abi, 0, 1)
+    assert nxt.K == w.K - 1 and nxt.E == w.E - 1
+
+
+@pytest.mark.parametrize("seed", [53, 54])
+def test_reanchored_tracks_keep_the_host_layout(ctx, abi, seed):
+    from vibpy import synth
+    w = synth.generate(K=7, L=400, n_cams=4, obs_range=(2, 8), seed=seed, static_bearings=True, rig_free=seed == 54,
+                       td_free=seed == 54)
+    w, _ = synth.reanchor_at(w, w.K - 2)
+    say(f"[gpu] edit: re-anchor seed {seed}")
+    ctx.upload(0, w)
+    smp, t0, t1, prm = _second_new_args(w)
+    st, rep = ctx.marginalize(0, abi.VIB_MARGIN_SECOND_NEW, smp, t0, t1, params=prm)
+    assert st == 0, ctx.error()
+    assert rep.reset_landmarks + rep.eliminated_landmarks + rep.demoted_landmarks > 0
+    _same_layout(ctx, abi, 0, 1)
+
+
+def test_gate_rejections_are_stripped_on_the_device(ctx, abi):
+    from vibpy import synth
+    w = synth.generate(K=10, L=600, n_cams=2, obs_range=(2, 8), seed=71, outlier_fraction=0.05)
+    say("[gpu] edit: strip rejected")
+    ctx.upload(0, w)
+    st, _ = ctx.solve(0, abi.options(max_iterations=3))
+    assert st == 0, ctx.error()
+    st, grep, dec = ctx.nis_gate(0, apply=True)
+    assert st == 0 and grep.rejected > 0
+    K, L, F = ctx.shape(0)[:3]
+    st, rep = ctx.marginalize(0, abi.VIB_MARGIN_OLD)
+    assert st == 0, ctx.error()
+    nxt = _same_layout(ctx, abi, 0, 1)
+    # no rejected observation survives the slide
+    s = nxt.obs_sqrt_info.reshape(-1, 3)
+    assert not np.any(np.all(s == 0, axis=1))
+    assert nxt.F < F - grep.rejected + 1
+
+
+def test_three_slides_in_a_row(ctx, abi):
+    from vibpy import synth
+    w = synth.generate(K=12, L=1500, n_cams=4, obs_range=(2, 8), seed=77, rig_free=True, td_free=True)
+    say("[gpu] edit: three slides")
+    ctx.upload(0, w)
+    for i, pol in enumerate((abi.VIB_MARGIN_OLD, abi.VIB_MARGIN_SECOND_NEW, abi.VIB_MARGIN_OLD)):
+        st, _ = ctx.solve(0, abi.options(max_iterations=1))
+        assert st == 0, ctx.error()
+        st, rep = ctx.marginalize(0, pol)
+        assert st == 0, ctx.error()
+        nxt = _same_layout(ctx, abi, 0, 1)
+        assert nxt.K == w.K - 1 - i
