@@ -131,6 +131,7 @@ struct WinDev {
     double* fac_lm;            // FL_PLANES x F : per-factor landmark terms (landmark order)
     double* lm_out;            // LO_PLANES x L
     double* lm_delta;          // L (tangent step of rho; 0 for constant)
+    int64_t* lm_uid;           // L stable landmark ids (strictly increasing; kept through edits)
     // IMU
     const int32_t* imu_from; const int32_t* imu_to; const double* imu;
     // prior
@@ -189,6 +190,19 @@ struct EditSpec {
     const uint8_t* keep_f;      // F keep flags in landmark order (null: every observation of a kept landmark)
     const int32_t* over_f;      // L (null: none): >= 0 re-anchors the landmark in that observation
     const double* over_rho;     // L: the inverse range in the new anchor
+    // inserted landmarks (after the kept ones) and observations (an appended keyframe's), all in
+    // the edited window's frame numbering
+    int32_t ins_lm, ins_obs;
+    const int32_t* add_cnt;      // L + ins_lm: inserted observations per landmark (null: none)
+    const int32_t* ins_lm_fc;    // ins_lm: anchor frame << 8 | camera
+    const double* ins_lmd;       // LM_PLANES x ins_lm
+    const double* ins_rho;       // ins_lm
+    const uint8_t* ins_free;     // ins_lm
+    const int64_t* ins_uid;      // ins_lm
+    const int32_t* ins_obs_lm;   // ins_obs: source landmark, or L + j for inserted landmark j; -1 dropped
+    const int32_t* ins_obs_rank; // ins_obs: position among the inserted observations of its landmark
+    const int32_t* ins_obs_fc;   // ins_obs: observer frame << 8 | camera
+    const double* ins_ob;        // OB_PLANES x ins_obs
 };
 
 }  // namespace vib

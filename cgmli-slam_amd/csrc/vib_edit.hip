@@ -109,11 +109,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const T* in, int n,
 
 // ---------------------------------------------------------------------------------------------
 // k_edit_mark: one thread per source landmark. Landmark flag: 1 (kept) | (kept and free) << 32;
-// observation flag: kept landmark and kept observation; lm_of[f] = the observation's landmark.
+// observation flag: kept landmark and kept observation; cnt[l]: the landmark's observations in
+// the edited window (kept ones and inserted ones); lm_of[f] = the observation's landmark.
 // bad[0] counts kept landmarks anchored in a removed frame and kept observations from one.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kEditBlock) void k_edit_mark(const WinDev* srcp, EditSpec sp, unsigned long long* lmv,
-                                                          uint32_t* fv, int32_t* lm_of, int* bad) {
+                                                          uint32_t* fv, uint32_t* cnt, int32_t* lm_of, int* bad) {
     const WinDev& s = *srcp;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= s.L) return;
@@ -123,23 +124,35 @@ __global__ __launch_bounds__(kEditBlock) void k_edit_mark(const WinDev* srcp, Ed
     int wrong = kl && sp.renum[anchor] < 0;
     const bool fr = sp.lm_mode == 1 ? true : (sp.lm_mode == 2 ? false : s.lm_col[l] >= 0);
     lmv[l] = kl ? (1ull | ((unsigned long long)(fr ? 1 : 0) << 32)) : 0ull;
+    uint32_t kept = 0;
     for (int f = s.lm_off[l]; f < s.lm_off[l + 1]; ++f) {
         const bool kf = kl && (sp.keep_f ? sp.keep_f[f] != 0 : true);
         wrong += kf && sp.renum[s.ob_fc[f] >> 8] < 0;
         fv[f] = kf ? 1u : 0u;
+        kept += kf;
         lm_of[f] = l;
     }
+    cnt[l] = kl ? kept + (sp.add_cnt ? (uint32_t)sp.add_cnt[l] : 0u) : 0u;
     if (wrong) atomicAdd(bad, wrong);
 }
 
+// k_insert_mark: the inserted landmarks' flags and observation counts (after the source's).
+__global__ __launch_bounds__(kEditBlock) void k_insert_mark(int L, EditSpec sp, unsigned long long* lmv, uint32_t* cnt) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= sp.ins_lm) return;
+    const bool fr = sp.lm_mode == 1 ? true : (sp.lm_mode == 2 ? false : sp.ins_free[j] != 0);
+    lmv[L + j] = 1ull | ((unsigned long long)(fr ? 1 : 0) << 32);
+    cnt[L + j] = sp.add_cnt ? (uint32_t)sp.add_cnt[L + j] : 0u;
+}
+
 // k_edit_landmarks: the kept landmarks at their new index (anchor, planes, inverse range and its
-// trial copy, rho column, observation offsets).
+// trial copy, rho column, observation offsets, id).
 __global__ __launch_bounds__(kEditBlock) void k_edit_landmarks(const WinDev* srcp, const WinDev* dstp, EditSpec sp,
-                                                               const unsigned long long* lscan, const uint32_t* fscan) {
+                                                               const unsigned long long* lscan, const uint32_t* cscan) {
     const WinDev& s = *srcp;
     const WinDev& d = *dstp;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l == 0) const_cast<int32_t*>(d.lm_off)[d.L] = (int32_t)fscan[s.F];
+    if (l == 0) const_cast<int32_t*>(d.lm_off)[d.L] = (int32_t)cscan[s.L + sp.ins_lm];
     if (l >= s.L) return;
     const unsigned long long a = lscan[l], b = lscan[l + 1];
     if ((uint32_t)b == (uint32_t)a) return;  // removed
@@ -163,30 +176,102 @@ __global__ __launch_bounds__(kEditBlock) void k_edit_landmarks(const WinDev* src
     }
     const_cast<int32_t*>(d.lm_fc)[ln] = (sp.renum[frame] << 8) | cam;
     const_cast<int32_t*>(d.lm_col)[ln] = fr ? sp.nr + rank : -1;
-    const_cast<int32_t*>(d.lm_off)[ln] = (int32_t)fscan[s.lm_off[l]];
+    const_cast<int32_t*>(d.lm_off)[ln] = (int32_t)cscan[l];
 #pragma unroll
     for (int q = 0; q < LM_PLANES; ++q) const_cast<double*>(d.lmd)[q * Ld + ln] = pl[q];
     d.rho[ln] = rho;
     d.crho[ln] = rho;
+    d.lm_uid[ln] = s.lm_uid[l];
 }
 
-// k_edit_factors: the kept observations at their new index (landmark order), and the new
-// landmark of each (for the pair-order sort).
+// k_insert_landmarks: the inserted landmarks after the kept ones.
+__global__ __launch_bounds__(kEditBlock) void k_insert_landmarks(int L, const WinDev* dstp, EditSpec sp,
+                                                                 const unsigned long long* lscan, const uint32_t* cscan) {
+    const WinDev& d = *dstp;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= sp.ins_lm) return;
+    const unsigned long long a = lscan[L + j], b = lscan[L + j + 1];
+    const int ln = (int)(uint32_t)a, rank = (int)(a >> 32);
+    const bool fr = (b >> 32) != (a >> 32);
+    const size_t Ld = (size_t)d.L, Ni = (size_t)sp.ins_lm;
+    const_cast<int32_t*>(d.lm_fc)[ln] = sp.ins_lm_fc[j];
+    const_cast<int32_t*>(d.lm_col)[ln] = fr ? sp.nr + rank : -1;
+    const_cast<int32_t*>(d.lm_off)[ln] = (int32_t)cscan[L + j];
+#pragma unroll
+    for (int q = 0; q < LM_PLANES; ++q) const_cast<double*>(d.lmd)[q * Ld + ln] = sp.ins_lmd[q * Ni + j];
+    d.rho[ln] = sp.ins_rho[j];
+    d.crho[ln] = sp.ins_rho[j];
+    d.lm_uid[ln] = sp.ins_uid[j];
+}
+
+// k_edit_factors: the kept observations at their new index (landmark order: the landmark's
+// offset plus the observation's rank among its kept ones), and the new landmark of each (for the
+// pair-order sort).
 __global__ __launch_bounds__(kEditBlock) void k_edit_factors(const WinDev* srcp, const WinDev* dstp, EditSpec sp,
                                                              const unsigned long long* lscan, const uint32_t* fscan,
-                                                             const int32_t* lm_of, int32_t* lmn) {
+                                                             const uint32_t* cscan, const int32_t* lm_of, int32_t* lmn) {
     const WinDev& s = *srcp;
     const WinDev& d = *dstp;
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= s.F) return;
-    const uint32_t fn = fscan[f];
-    if (fscan[f + 1] == fn) return;  // removed
+    if (fscan[f + 1] == fscan[f]) return;  // removed
+    const int l = lm_of[f];
+    const uint32_t fn = cscan[l] + (fscan[f] - fscan[s.lm_off[l]]);
     const size_t Fs = (size_t)s.F, Fd = (size_t)d.F;
     const int fc = s.ob_fc[f];
     const_cast<int32_t*>(d.ob_fc)[fn] = (sp.renum[fc >> 8] << 8) | (fc & 255);
 #pragma unroll
     for (int q = 0; q < OB_PLANES; ++q) const_cast<double*>(d.ob)[q * Fd + fn] = s.ob[q * Fs + f];
-    lmn[fn] = (int32_t)(uint32_t)lscan[lm_of[f]];
+    lmn[fn] = (int32_t)(uint32_t)lscan[l];
+}
+
+// k_insert_factors: the inserted observations after their landmark's kept ones.
+__global__ __launch_bounds__(kEditBlock) void k_insert_factors(const WinDev* srcp, const WinDev* dstp, EditSpec sp,
+                                                               const unsigned long long* lscan, const uint32_t* fscan,
+                                                               const uint32_t* cscan, int32_t* lmn) {
+    const WinDev& s = *srcp;
+    const WinDev& d = *dstp;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sp.ins_obs) return;
+    const int l = sp.ins_obs_lm[i];
+    if (l < 0 || (uint32_t)lscan[l + 1] == (uint32_t)lscan[l]) return;  // unresolved or removed landmark
+    const uint32_t kept = l < s.L ? fscan[s.lm_off[l + 1]] - fscan[s.lm_off[l]] : 0u;
+    const uint32_t fn = cscan[l] + kept + (uint32_t)sp.ins_obs_rank[i];
+    const size_t Fd = (size_t)d.F, Ni = (size_t)sp.ins_obs;
+    const_cast<int32_t*>(d.ob_fc)[fn] = sp.ins_obs_fc[i];
+#pragma unroll
+    for (int q = 0; q < OB_PLANES; ++q) const_cast<double*>(d.ob)[q * Fd + fn] = sp.ins_ob[q * Ni + i];
+    lmn[fn] = (int32_t)(uint32_t)lscan[l];
+}
+
+// k_insert_resolve: the landmark of each inserted observation from its landmark id (binary search
+// over the source window's ids, then the inserted landmarks' ids; both strictly increasing) and
+// the inserted observations per landmark. Unresolved ids (landmarks no longer in the window) are
+// counted in unmatched[0] and their observations dropped.
+__global__ __launch_bounds__(kEditBlock) void k_insert_resolve(const WinDev* srcp, const int64_t* obs_uid, int nobs,
+                                                               const int64_t* new_uid, int nnew, int32_t* obs_lm,
+                                                               int32_t* add_cnt, int* unmatched) {
+    const WinDev& s = *srcp;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nobs) return;
+    const int64_t u = obs_uid[i];
+    auto find = [&](const int64_t* ids, int n) {
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (ids[mid] < u) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo < n && ids[lo] == u ? lo : -1;
+    };
+    int l = find(s.lm_uid, s.L);
+    if (l < 0) {
+        const int j = find(new_uid, nnew);
+        l = j < 0 ? -1 : s.L + j;
+    }
+    obs_lm[i] = l;
+    if (l < 0) atomicAdd(unmatched, 1);
+    else atomicAdd(&add_cnt[l], 1);
 }
 
 // Stable rank of each lane's key among the lanes of its wave with the same key (lanes in order),
@@ -409,25 +494,43 @@ hipError_t vibk_edit_scan(const void* in, int n, void* tile_sum, void* out, int 
 
 // spec points to a host vib::EditSpec (passed by value to the kernels).
 hipError_t vibk_edit_mark(const vib::WinDev* src, int L, const void* spec, unsigned long long* lmv, uint32_t* fv,
-                          int32_t* lm_of, int* bad, hipStream_t s) {
+                          uint32_t* cnt, int32_t* lm_of, int* bad, hipStream_t s) {
     using namespace vib;
-    if (L == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_edit_mark, dim3((L + kEditBlock - 1) / kEditBlock), dim3(kEditBlock), 0, s, src,
-                       *static_cast<const EditSpec*>(spec), lmv, fv, lm_of, bad);
+    const EditSpec& sp = *static_cast<const EditSpec*>(spec);
+    if (L > 0)
+        hipLaunchKernelGGL(k_edit_mark, dim3((L + kEditBlock - 1) / kEditBlock), dim3(kEditBlock), 0, s, src, sp, lmv, fv,
+                           cnt, lm_of, bad);
+    if (sp.ins_lm > 0)
+        hipLaunchKernelGGL(k_insert_mark, dim3((sp.ins_lm + kEditBlock - 1) / kEditBlock), dim3(kEditBlock), 0, s, L, sp,
+                           lmv, cnt);
     return hipGetLastError();
 }
 
-hipError_t vibk_edit_gather(const vib::WinDev* src, const vib::WinDev* dst, int Ls, int Fs, int Fd, const void* spec,
-                            const unsigned long long* lscan, const uint32_t* fscan, const int32_t* lm_of, int32_t* lmn,
-                            hipStream_t s) {
+hipError_t vibk_edit_gather(const vib::WinDev* src, const vib::WinDev* dst, int Ls, int Fs, const void* spec,
+                            const unsigned long long* lscan, const uint32_t* fscan, const uint32_t* cscan,
+                            const int32_t* lm_of, int32_t* lmn, hipStream_t s) {
     using namespace vib;
     const EditSpec& sp = *static_cast<const EditSpec*>(spec);
     hipLaunchKernelGGL(k_edit_landmarks, dim3((Ls + kEditBlock) / kEditBlock), dim3(kEditBlock), 0, s, src, dst, sp, lscan,
-                       fscan);
+                       cscan);
+    if (sp.ins_lm > 0)
+        hipLaunchKernelGGL(k_insert_landmarks, dim3((sp.ins_lm + kEditBlock - 1) / kEditBlock), dim3(kEditBlock), 0, s, Ls,
+                           dst, sp, lscan, cscan);
     if (Fs > 0)
         hipLaunchKernelGGL(k_edit_factors, dim3((Fs + kEditBlock - 1) / kEditBlock), dim3(kEditBlock), 0, s, src, dst, sp,
-                           lscan, fscan, lm_of, lmn);
-    (void)Fd;
+                           lscan, fscan, cscan, lm_of, lmn);
+    if (sp.ins_obs > 0)
+        hipLaunchKernelGGL(k_insert_factors, dim3((sp.ins_obs + kEditBlock - 1) / kEditBlock), dim3(kEditBlock), 0, s, src,
+                           dst, sp, lscan, fscan, cscan, lmn);
+    return hipGetLastError();
+}
+
+hipError_t vibk_insert_resolve(const vib::WinDev* src, const int64_t* obs_uid, int nobs, const int64_t* new_uid, int nnew,
+                               int32_t* obs_lm, int32_t* add_cnt, int* unmatched, hipStream_t s) {
+    using namespace vib;
+    if (nobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_insert_resolve, dim3((nobs + kEditBlock - 1) / kEditBlock), dim3(kEditBlock), 0, s, src, obs_uid,
+                       nobs, new_uid, nnew, obs_lm, add_cnt, unmatched);
     return hipGetLastError();
 }
 

@@ -41,10 +41,12 @@ int vibk_edit_sort_blocks(int n);
 int vibk_edit_pair_keys(void);
 hipError_t vibk_edit_scan(const void* in, int n, void* tile_sum, void* out, int wide, hipStream_t s);
 hipError_t vibk_edit_mark(const vib::WinDev* src, int L, const void* spec, unsigned long long* lmv, uint32_t* fv,
-                          int32_t* lm_of, int* bad, hipStream_t s);
-hipError_t vibk_edit_gather(const vib::WinDev* src, const vib::WinDev* dst, int Ls, int Fs, int Fd, const void* spec,
-                            const unsigned long long* lscan, const uint32_t* fscan, const int32_t* lm_of, int32_t* lmn,
-                            hipStream_t s);
+                          uint32_t* cnt, int32_t* lm_of, int* bad, hipStream_t s);
+hipError_t vibk_edit_gather(const vib::WinDev* src, const vib::WinDev* dst, int Ls, int Fs, const void* spec,
+                            const unsigned long long* lscan, const uint32_t* fscan, const uint32_t* cscan,
+                            const int32_t* lm_of, int32_t* lmn, hipStream_t s);
+hipError_t vibk_insert_resolve(const vib::WinDev* src, const int64_t* obs_uid, int nobs, const int64_t* new_uid, int nnew,
+                               int32_t* obs_lm, int32_t* add_cnt, int* unmatched, hipStream_t s);
 hipError_t vibk_edit_pair_hist(const vib::WinDev* dst, int F, const int32_t* lmn, uint32_t* counts, hipStream_t s);
 hipError_t vibk_edit_pair_scatter(const vib::WinDev* dst, int F, const int32_t* lmn, const uint32_t* offs, hipStream_t s);
 hipError_t vibk_strip_select(const vib::WinDev* src, int L, uint8_t* keep_lm, uint8_t* keep_f, int* out, hipStream_t s);
@@ -171,6 +173,12 @@ int HostWindow::build(const vib_window& w, std::string& err) {
     lm_frame.assign(L, 0); lm_cam.assign(L, 0); lm_free.assign(L, 1);
     lmd.assign((size_t)LM_PLANES * L, 0);
     rho.assign(L, 0);
+    lm_uid.assign(L, 0);
+    for (int l = 0; l < L; ++l) {
+        lm_uid[l] = w.lm_id ? w.lm_id[l] : l;
+        if (l && lm_uid[l] <= lm_uid[l - 1]) { err = "landmark ids must be strictly increasing"; return BAD; }
+    }
+    uid_next = L ? lm_uid[L - 1] + 1 : 0;
     for (int l = 0; l < L; ++l) {
         lm_frame[l] = w.lm_anchor_frame[l]; lm_cam[l] = w.lm_anchor_cam[l];
         if (lm_frame[l] < 0 || lm_frame[l] >= K || lm_cam[l] < 0 || lm_cam[l] >= C) { err = "landmark anchor out of range"; return BAD; }
@@ -389,6 +397,7 @@ size_t Slot::footprint(const HostWindow& h) {
     d += 2 * (16 * h.K + 8 + h.L);                     // state + candidate
     d += 12 * h.C + LM_PLANES * h.L + OB_PLANES * h.F;  // streams
     d += 6 * h.F + LO_PLANES * h.L + h.L;                // linearization outputs
+    d += h.L;                                            // landmark ids
     d += (size_t)VIB_IMU_STRIDE * h.E;
     d += m * n + m + n * n + h.prior_ref.size();
     d += 2 * nr * nr + factor_doubles((int)nr + kTile) + 8 * nr + 64;
@@ -435,6 +444,7 @@ bool Slot::carve(const HostWindow& h, char* base, size_t bytes, WinDev& d, SlotC
     d.pf_pos = a.take<int32_t>(h.F);
     d.lm_out = a.take<double>((size_t)LO_PLANES * h.L);
     d.lm_delta = a.take<double>(h.L);
+    d.lm_uid = a.take<int64_t>(h.L);
     d.imu_from = a.take<int32_t>(h.E);
     d.imu_to = a.take<int32_t>(h.E);
     d.imu = a.take<double>((size_t)VIB_IMU_STRIDE * h.E);
@@ -495,9 +505,9 @@ bool Slot::fill(const HostWindow& h, const WinDev& d, const SlotCarve& x, char* 
              up(d.ob, h.ob.data(), 8 * h.ob.size()) && up(d.pf_ob, h.pf_ob.data(), 8 * h.pf_ob.size()) &&
              up(d.pf_an, h.pf_an.data(), 8 * h.pf_an.size()) && up(d.pf_ids, h.pf_ids.data(), 4 * h.pf_ids.size()) &&
              up(d.pf_lm, h.pf_lm.data(), 4 * h.pf_lm.size()) && up(d.pf_pos, h.pf_pos.data(), 4 * h.pf_pos.size()) &&
-             up(d.crho, h.rho.data(), 8 * h.rho.size());
+             up(d.crho, h.rho.data(), 8 * h.rho.size()) && up(d.lm_uid, h.lm_uid.data(), 8 * h.lm_uid.size());
         // the layouts of the copies match the carved footprint
-        ok = ok && (int)h.rho.size() == h.L && (int)h.lm_off.size() == h.L + 1 && h.lmd.size() == (size_t)LM_PLANES * h.L &&
+        ok = ok && (int)h.rho.size() == h.L && (int)h.lm_uid.size() == h.L && (int)h.lm_off.size() == h.L + 1 && h.lmd.size() == (size_t)LM_PLANES * h.L &&
              (int)h.ob_fc.size() == h.F && h.ob.size() == (size_t)OB_PLANES * h.F && h.pf_ob.size() == (size_t)OB_PLANES * h.F;
     }
     // candidate starts equal to the state
@@ -899,6 +909,7 @@ bool ensure_bulk(vib_context* ctx, Slot& s) {
     const WinDev& d = s.dev;
     std::vector<int32_t> fc(L);
     h.lm_off.assign(L + 1, 0); h.lm_col.assign(L, -1); h.lmd.assign((size_t)LM_PLANES * L, 0); h.rho.assign(L, 0);
+    h.lm_uid.assign(L, 0);
     h.ob_fc.assign(F, 0); h.ob.assign((size_t)OB_PLANES * F, 0);
     h.pf_ob.assign((size_t)OB_PLANES * F, 0); h.pf_an.assign((size_t)LM_PLANES * F, 0);
     h.pf_ids.assign(F, 0); h.pf_lm.assign(F, 0); h.pf_pos.assign(F, 0);
@@ -907,7 +918,8 @@ bool ensure_bulk(vib_context* ctx, Slot& s) {
     };
     const bool ok = dn(fc.data(), d.lm_fc, 4 * L) && dn(h.lm_off.data(), d.lm_off, 4 * (L + 1)) &&
                     dn(h.lm_col.data(), d.lm_col, 4 * L) && dn(h.lmd.data(), d.lmd, 8 * h.lmd.size()) &&
-                    dn(h.rho.data(), d.rho, 8 * L) && dn(h.ob_fc.data(), d.ob_fc, 4 * F) &&
+                    dn(h.rho.data(), d.rho, 8 * L) && dn(h.lm_uid.data(), d.lm_uid, 8 * L) &&
+                    dn(h.ob_fc.data(), d.ob_fc, 4 * F) &&
                     dn(h.ob.data(), d.ob, 8 * h.ob.size()) && dn(h.pf_ob.data(), d.pf_ob, 8 * h.pf_ob.size()) &&
                     dn(h.pf_an.data(), d.pf_an, 8 * h.pf_an.size()) && dn(h.pf_ids.data(), d.pf_ids, 4 * F) &&
                     dn(h.pf_lm.data(), d.pf_lm, 4 * F) && dn(h.pf_pos.data(), d.pf_pos, 4 * F) &&
@@ -936,6 +948,7 @@ HostWindow frame_meta(const HostWindow& cur) {
     m.imu_from = cur.imu_from; m.imu_to = cur.imu_to; m.imu = cur.imu;
     m.prior_rows = cur.prior_rows; m.prior_cols = cur.prior_cols; m.prior_kind = cur.prior_kind;
     m.prior_id = cur.prior_id; m.prior_ref = cur.prior_ref; m.prior_A = cur.prior_A; m.prior_r0 = cur.prior_r0;
+    m.uid_next = cur.uid_next;
     m.bulk = false;
     return m;
 }
@@ -965,23 +978,34 @@ void keep_edges(HostWindow& m, const std::vector<int>& from, const std::vector<i
     m.E = (int)m.imu_from.size();
 }
 
-// Device buffers of the window edits of one slide (sized for the source window), carved from
-// the context's grow-only edit buffer.
+// Device buffers of the window edits of one slide or append (sized for the source window and
+// nl inserted landmarks, nf inserted observations), carved from the context's grow-only edit
+// buffer.
 struct EditBufs {
     WinDev* src = nullptr;
     WinDev* dst = nullptr;
     int* bad = nullptr;  // kept anchors or observations of removed frames (must stay 0)
     int* sel = nullptr;  // 16 counters of the selection kernels
     unsigned long long *lmv = nullptr, *lscan = nullptr, *ltile = nullptr;
+    uint32_t *cnt = nullptr, *cscan = nullptr, *cctile = nullptr;
     uint32_t *fv = nullptr, *fscan = nullptr, *ftile = nullptr;
     int32_t *lm_of = nullptr, *lmn = nullptr;
     uint32_t *counts = nullptr, *offs = nullptr, *ctile = nullptr;
     uint8_t *keep_lm = nullptr, *keep_f = nullptr, *marg_lm = nullptr, *cls = nullptr;
     int32_t *over_f = nullptr, *chosen = nullptr;
     double *over_rho = nullptr, *chosen_rho = nullptr;
+    // inserted data
+    int32_t* add_cnt = nullptr;
+    int64_t* ins_obs_uid = nullptr;
+    int32_t *ins_obs_lm = nullptr, *ins_obs_rank = nullptr, *ins_obs_fc = nullptr;
+    double* ins_ob = nullptr;
+    int32_t* ins_lm_fc = nullptr;
+    double *ins_lmd = nullptr, *ins_rho = nullptr;
+    uint8_t* ins_free = nullptr;
+    int64_t* ins_uid = nullptr;
 };
 
-bool edit_buffers(vib_context* ctx, int L, int F, EditBufs& b) {
+bool edit_buffers(vib_context* ctx, int L, int F, EditBufs& b, int nl = 0, int nf = 0) {
     size_t off = 0;
     auto take = [&](size_t bytes) {
         off = (off + 255) & ~size_t(255);
@@ -989,42 +1013,61 @@ bool edit_buffers(vib_context* ctx, int L, int F, EditBufs& b) {
         off += std::max<size_t>(bytes, 1);
         return o;
     };
-    const size_t Ls = L, Fs = F;
-    const size_t lt = (size_t)vibk_edit_scan_tiles(L) + 1, ft = (size_t)vibk_edit_scan_tiles(F) + 1;
-    const size_t nkeys = (size_t)vibk_edit_pair_keys() * vibk_edit_sort_blocks(F);
+    const size_t Ls = L, Fs = F, Lx = (size_t)L + nl, Fx = (size_t)F + nf, NL = nl, NF = nf;
+    const size_t lt = (size_t)vibk_edit_scan_tiles((int)Lx) + 1, ft = (size_t)vibk_edit_scan_tiles(F) + 1;
+    const size_t nkeys = (size_t)vibk_edit_pair_keys() * vibk_edit_sort_blocks((int)Fx);
     const size_t ct = (size_t)vibk_edit_scan_tiles((int)nkeys) + 1;
     const size_t o_src = take(sizeof(WinDev)), o_dst = take(sizeof(WinDev)), o_bad = take(4), o_sel = take(64);
-    const size_t o_lmv = take(8 * Ls), o_lscan = take(8 * (Ls + 1)), o_ltile = take(8 * lt);
+    const size_t o_lmv = take(8 * Lx), o_lscan = take(8 * (Lx + 1)), o_ltile = take(8 * lt);
+    const size_t o_cnt = take(4 * Lx), o_cscan = take(4 * (Lx + 1)), o_cctile = take(4 * lt);
     const size_t o_fv = take(4 * Fs), o_fscan = take(4 * (Fs + 1)), o_ftile = take(4 * ft);
-    const size_t o_lmof = take(4 * Fs), o_lmn = take(4 * Fs);
+    const size_t o_lmof = take(4 * Fs), o_lmn = take(4 * Fx);
     const size_t o_counts = take(4 * (nkeys + 1)), o_offs = take(4 * (nkeys + 1)), o_ctile = take(4 * ct);
     const size_t o_klm = take(Ls), o_kf = take(Fs), o_mlm = take(Ls), o_cls = take(Ls);
     const size_t o_overf = take(4 * Ls), o_chosen = take(4 * Ls), o_overr = take(8 * Ls), o_chr = take(8 * Ls);
+    const size_t o_add = take(4 * Lx), o_iou = take(8 * NF), o_iol = take(4 * NF), o_ior = take(4 * NF),
+                 o_iof = take(4 * NF), o_iob = take(8 * OB_PLANES * NF), o_ilf = take(4 * NL),
+                 o_ild = take(8 * LM_PLANES * NL), o_ilr = take(8 * NL), o_ilfr = take(NL), o_ilu = take(8 * NL);
     char* base = static_cast<char*>(grow_device(ctx->edit_buf, ctx->edit_cap, off));
     if (!base) return false;
-    b.src = reinterpret_cast<WinDev*>(base + o_src);
-    b.dst = reinterpret_cast<WinDev*>(base + o_dst);
-    b.bad = reinterpret_cast<int*>(base + o_bad);
-    b.sel = reinterpret_cast<int*>(base + o_sel);
-    b.lmv = reinterpret_cast<unsigned long long*>(base + o_lmv);
-    b.lscan = reinterpret_cast<unsigned long long*>(base + o_lscan);
-    b.ltile = reinterpret_cast<unsigned long long*>(base + o_ltile);
-    b.fv = reinterpret_cast<uint32_t*>(base + o_fv);
-    b.fscan = reinterpret_cast<uint32_t*>(base + o_fscan);
-    b.ftile = reinterpret_cast<uint32_t*>(base + o_ftile);
-    b.lm_of = reinterpret_cast<int32_t*>(base + o_lmof);
-    b.lmn = reinterpret_cast<int32_t*>(base + o_lmn);
-    b.counts = reinterpret_cast<uint32_t*>(base + o_counts);
-    b.offs = reinterpret_cast<uint32_t*>(base + o_offs);
-    b.ctile = reinterpret_cast<uint32_t*>(base + o_ctile);
-    b.keep_lm = reinterpret_cast<uint8_t*>(base + o_klm);
-    b.keep_f = reinterpret_cast<uint8_t*>(base + o_kf);
-    b.marg_lm = reinterpret_cast<uint8_t*>(base + o_mlm);
-    b.cls = reinterpret_cast<uint8_t*>(base + o_cls);
-    b.over_f = reinterpret_cast<int32_t*>(base + o_overf);
-    b.chosen = reinterpret_cast<int32_t*>(base + o_chosen);
-    b.over_rho = reinterpret_cast<double*>(base + o_overr);
-    b.chosen_rho = reinterpret_cast<double*>(base + o_chr);
+    auto at = [&](size_t o) { return static_cast<void*>(base + o); };
+    b.src = static_cast<WinDev*>(at(o_src));
+    b.dst = static_cast<WinDev*>(at(o_dst));
+    b.bad = static_cast<int*>(at(o_bad));
+    b.sel = static_cast<int*>(at(o_sel));
+    b.lmv = static_cast<unsigned long long*>(at(o_lmv));
+    b.lscan = static_cast<unsigned long long*>(at(o_lscan));
+    b.ltile = static_cast<unsigned long long*>(at(o_ltile));
+    b.cnt = static_cast<uint32_t*>(at(o_cnt));
+    b.cscan = static_cast<uint32_t*>(at(o_cscan));
+    b.cctile = static_cast<uint32_t*>(at(o_cctile));
+    b.fv = static_cast<uint32_t*>(at(o_fv));
+    b.fscan = static_cast<uint32_t*>(at(o_fscan));
+    b.ftile = static_cast<uint32_t*>(at(o_ftile));
+    b.lm_of = static_cast<int32_t*>(at(o_lmof));
+    b.lmn = static_cast<int32_t*>(at(o_lmn));
+    b.counts = static_cast<uint32_t*>(at(o_counts));
+    b.offs = static_cast<uint32_t*>(at(o_offs));
+    b.ctile = static_cast<uint32_t*>(at(o_ctile));
+    b.keep_lm = static_cast<uint8_t*>(at(o_klm));
+    b.keep_f = static_cast<uint8_t*>(at(o_kf));
+    b.marg_lm = static_cast<uint8_t*>(at(o_mlm));
+    b.cls = static_cast<uint8_t*>(at(o_cls));
+    b.over_f = static_cast<int32_t*>(at(o_overf));
+    b.chosen = static_cast<int32_t*>(at(o_chosen));
+    b.over_rho = static_cast<double*>(at(o_overr));
+    b.chosen_rho = static_cast<double*>(at(o_chr));
+    b.add_cnt = static_cast<int32_t*>(at(o_add));
+    b.ins_obs_uid = static_cast<int64_t*>(at(o_iou));
+    b.ins_obs_lm = static_cast<int32_t*>(at(o_iol));
+    b.ins_obs_rank = static_cast<int32_t*>(at(o_ior));
+    b.ins_obs_fc = static_cast<int32_t*>(at(o_iof));
+    b.ins_ob = static_cast<double*>(at(o_iob));
+    b.ins_lm_fc = static_cast<int32_t*>(at(o_ilf));
+    b.ins_lmd = static_cast<double*>(at(o_ild));
+    b.ins_rho = static_cast<double*>(at(o_ilr));
+    b.ins_free = static_cast<uint8_t*>(at(o_ilfr));
+    b.ins_uid = static_cast<int64_t*>(at(o_ilu));
     return true;
 }
 
@@ -1046,7 +1089,7 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
     std::string err;
     if (!meta.layout_states(err)) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, err);
     sp.nr = meta.nr;
-    const int Ls = src.host.L, Fs = src.host.F;
+    const int Ls = src.host.L, Fs = src.host.F, Lx = Ls + sp.ins_lm;
     const hipStream_t st = ctx->stream;
     auto h2d = [&](void* dst_p, const void* src_p, size_t n) {
         return hipMemcpyAsync(dst_p, src_p, n, hipMemcpyHostToDevice, st) == hipSuccess;
@@ -1059,10 +1102,11 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
     uint32_t ftot = 0;
     int bad = 0;
     bool ok = h2d(b.src, &src.dev, sizeof(WinDev)) && h2d(b.bad, &zero, sizeof(int)) &&
-              vibk_edit_mark(b.src, Ls, &sp, b.lmv, b.fv, b.lm_of, b.bad, st) == hipSuccess &&
-              vibk_edit_scan(b.lmv, Ls, b.ltile, b.lscan, 1, st) == hipSuccess &&
+              vibk_edit_mark(b.src, Ls, &sp, b.lmv, b.fv, b.cnt, b.lm_of, b.bad, st) == hipSuccess &&
+              vibk_edit_scan(b.lmv, Lx, b.ltile, b.lscan, 1, st) == hipSuccess &&
+              vibk_edit_scan(b.cnt, Lx, b.cctile, b.cscan, 0, st) == hipSuccess &&
               vibk_edit_scan(b.fv, Fs, b.ftile, b.fscan, 0, st) == hipSuccess &&
-              d2h(&ltot, b.lscan + Ls, sizeof(ltot)) && d2h(&ftot, b.fscan + Fs, sizeof(ftot)) &&
+              d2h(&ltot, b.lscan + Lx, sizeof(ltot)) && d2h(&ftot, b.cscan + Lx, sizeof(ftot)) &&
               d2h(&bad, b.bad, sizeof(bad)) && hipStreamSynchronize(st) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
@@ -1074,7 +1118,7 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
     meta.F = (int)ftot;
     meta.bulk = false;
     meta.lm_frame.clear(); meta.lm_cam.clear(); meta.lm_off.clear(); meta.lm_free.clear(); meta.lmd.clear();
-    meta.rho.clear(); meta.ob_fc.clear(); meta.ob_lm.clear(); meta.ob.clear(); meta.lm_col.clear();
+    meta.rho.clear(); meta.lm_uid.clear(); meta.ob_fc.clear(); meta.ob_lm.clear(); meta.ob.clear(); meta.lm_col.clear();
     meta.pf_ob.clear(); meta.pf_an.clear(); meta.pf_ids.clear(); meta.pf_lm.clear(); meta.pf_pos.clear();
     meta.make_chunks();
     const size_t need = Slot::footprint(meta);
@@ -1108,7 +1152,7 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
     if (!Slot::carve(meta, base, need, nd, x)) return fail(ctx, VIB_ERR_DEVICE, "arena overflow");
     const int Fd = meta.F, nkeys = vibk_edit_pair_keys() * vibk_edit_sort_blocks(Fd);
     ok = Slot::fill(meta, nd, x, base, need, false, st) && h2d(b.dst, &nd, sizeof(WinDev)) &&
-         vibk_edit_gather(b.src, b.dst, Ls, Fs, Fd, &sp, b.lscan, b.fscan, b.lm_of, b.lmn, st) == hipSuccess &&
+         vibk_edit_gather(b.src, b.dst, Ls, Fs, &sp, b.lscan, b.fscan, b.cscan, b.lm_of, b.lmn, st) == hipSuccess &&
          (Fd == 0 || (vibk_edit_pair_hist(b.dst, Fd, b.lmn, b.counts, st) == hipSuccess &&
                       vibk_edit_scan(b.counts, nkeys, b.ctile, b.offs, 0, st) == hipSuccess &&
                       vibk_edit_pair_scatter(b.dst, Fd, b.lmn, b.offs, st) == hipSuccess)) &&
@@ -1557,6 +1601,112 @@ int preintegrate_locked(vib_context* ctx, int32_t count, const double* samples, 
                         const double* speed_bias, const vib_imu_params* params, const double* gyro_start,
                         const double* gyro_end, double* out, int32_t* status, double* kernel_ms);
 
+// The keyframe as the window's newest frame: its state and IMU edge join the frame-level part
+// on the host; its landmarks and observations are inserted into the resident streams by a device
+// edit (observations sorted by landmark id here, resolved to resident landmarks on the device).
+int append_keyframe(vib_context* ctx, Slot& s, const vib_keyframe& kf, int32_t* unmatched) {
+    const int BAD = VIB_ERR_INVALID_ARGUMENT;
+    const HostWindow& cur = s.host;
+    if (cur.K >= kMaxFrames) return fail(ctx, VIB_ERR_UNSUPPORTED_DIMENSIONS, "unsupported Schur dimensions: the window holds 20 frames");
+    const int nl = kf.num_landmarks, nf = kf.num_observations;
+    if (nl < 0 || nf < 0 || !kf.pose || !kf.speed_bias) return fail(ctx, BAD, "incomplete keyframe");
+    if (nl && (!kf.lm_id || !kf.lm_anchor_cam || !kf.lm_bearing || !kf.lm_velocity || !kf.lm_tref || !kf.lm_rho))
+        return fail(ctx, BAD, "missing keyframe landmarks");
+    if (nf && (!kf.obs_landmark_id || !kf.obs_cam || !kf.obs_bearing || !kf.obs_velocity || !kf.obs_tref ||
+               !kf.obs_sqrt_info))
+        return fail(ctx, BAD, "missing keyframe observations");
+    if (kf.sb_width != 0 && kf.sb_width != 3 && kf.sb_width != 9) return fail(ctx, BAD, "sb_width must be 0, 3 or 9");
+    for (int j = 0; j < nl; ++j) {
+        if (kf.lm_anchor_cam[j] < 0 || kf.lm_anchor_cam[j] >= cur.C) return fail(ctx, BAD, "landmark anchor out of range");
+        if (kf.lm_id[j] < (j ? kf.lm_id[j - 1] + 1 : cur.uid_next))
+            return fail(ctx, BAD, "new landmark ids must increase above every id the window held");
+    }
+    for (int i = 0; i < nf; ++i)
+        if (kf.obs_cam[i] < 0 || kf.obs_cam[i] >= cur.C) return fail(ctx, BAD, "observation camera out of range");
+    if (!refresh_state(ctx, s)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
+    const int k = cur.K;
+    HostWindow meta = frame_meta(cur);
+    meta.K = k + 1;
+    meta.frame_id.push_back(kf.frame_id);
+    meta.pose.insert(meta.pose.end(), kf.pose, kf.pose + 7);
+    meta.sb.insert(meta.sb.end(), kf.speed_bias, kf.speed_bias + 9);
+    meta.pose_free.push_back(kf.pose_free ? 1 : 0);
+    meta.sb_width.push_back((uint8_t)kf.sb_width);
+    if (kf.imu_data) {
+        if (k < 1) return fail(ctx, BAD, "an IMU edge needs a previous frame");
+        meta.imu_from.push_back(k - 1);
+        meta.imu_to.push_back(k);
+        meta.imu.insert(meta.imu.end(), kf.imu_data, kf.imu_data + VIB_IMU_STRIDE);
+        meta.E += 1;
+    }
+    if (nl) meta.uid_next = kf.lm_id[nl - 1] + 1;
+    // observations in landmark-id order (stable), each with its rank among those of its landmark
+    std::vector<int> order(nf);
+    for (int i = 0; i < nf; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return kf.obs_landmark_id[a] < kf.obs_landmark_id[b]; });
+    const size_t NF = nf, NL = nl;
+    std::vector<int64_t> ouid(NF);
+    std::vector<int32_t> orank(NF), ofc(NF), lfc(NL);
+    std::vector<double> ob((size_t)OB_PLANES * NF), lmd((size_t)LM_PLANES * NL);
+    std::vector<uint8_t> lfree(NL);
+    for (size_t p = 0; p < NF; ++p) {
+        const int i = order[p];
+        ouid[p] = kf.obs_landmark_id[i];
+        orank[p] = p && ouid[p] == ouid[p - 1] ? orank[p - 1] + 1 : 0;
+        ofc[p] = (k << 8) | kf.obs_cam[i];
+        for (int q = 0; q < 3; ++q) {
+            ob[(OB_BX + q) * NF + p] = kf.obs_bearing[3 * i + q];
+            ob[(OB_VX + q) * NF + p] = kf.obs_velocity[3 * i + q];
+            ob[(OB_S00 + q) * NF + p] = kf.obs_sqrt_info[3 * i + q];
+        }
+        ob[OB_T * NF + p] = kf.obs_tref[i];
+    }
+    for (size_t j = 0; j < NL; ++j) {
+        lfc[j] = (k << 8) | kf.lm_anchor_cam[j];
+        for (int q = 0; q < 3; ++q) {
+            lmd[(LM_BX + q) * NL + j] = kf.lm_bearing[3 * j + q];
+            lmd[(LM_VX + q) * NL + j] = kf.lm_velocity[3 * j + q];
+        }
+        lmd[LM_T * NL + j] = kf.lm_tref[j];
+        lfree[j] = kf.lm_free ? (kf.lm_free[j] ? 1 : 0) : 1;
+    }
+    EditBufs b;
+    if (!edit_buffers(ctx, cur.L, cur.F, b, nl, nf)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "append buffer allocation failed");
+    auto h2d = [&](void* dst, const void* src, size_t n) {
+        return n == 0 || hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
+    };
+    int miss = 0;
+    const bool ok = edit_begin(ctx, s, b) && h2d(b.ins_obs_uid, ouid.data(), 8 * NF) &&
+                    h2d(b.ins_obs_rank, orank.data(), 4 * NF) && h2d(b.ins_obs_fc, ofc.data(), 4 * NF) &&
+                    h2d(b.ins_ob, ob.data(), 8 * ob.size()) && h2d(b.ins_lm_fc, lfc.data(), 4 * NL) &&
+                    h2d(b.ins_lmd, lmd.data(), 8 * lmd.size()) && h2d(b.ins_rho, kf.lm_rho, 8 * NL) &&
+                    h2d(b.ins_free, lfree.data(), NL) && h2d(b.ins_uid, kf.lm_id, 8 * NL) &&
+                    hipMemsetAsync(b.add_cnt, 0, 4 * ((size_t)cur.L + NL), ctx->stream) == hipSuccess &&
+                    vibk_insert_resolve(b.src, b.ins_obs_uid, nf, b.ins_uid, nl, b.ins_obs_lm, b.add_cnt, b.sel,
+                                        ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(&miss, b.sel, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                    hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "keyframe upload failed");
+    EditSpec sp{};
+    for (int f = 0; f < kMaxFrames; ++f) sp.renum[f] = f < k ? f : -1;
+    sp.ins_lm = nl;
+    sp.ins_obs = nf;
+    sp.add_cnt = b.add_cnt;
+    sp.ins_lm_fc = b.ins_lm_fc;
+    sp.ins_lmd = b.ins_lmd;
+    sp.ins_rho = b.ins_rho;
+    sp.ins_free = b.ins_free;
+    sp.ins_uid = b.ins_uid;
+    sp.ins_obs_lm = b.ins_obs_lm;
+    sp.ins_obs_rank = b.ins_obs_rank;
+    sp.ins_obs_fc = b.ins_obs_fc;
+    sp.ins_ob = b.ins_ob;
+    const int st = edit_window(ctx, s, sp, std::move(meta), s, b, true);
+    if (st == VIB_OK && unmatched) *unmatched = miss;
+    return st;
+}
+
 int marginalize_second_new(vib_context* ctx, Slot& s, const double* samples, int count, double t0, double t1,
                            const vib_imu_params* params, vib_marginal_report* rep) {
     if (!refresh_state(ctx, s)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
@@ -1844,6 +1994,15 @@ int vib_set_host_retained(vib_context* ctx, int32_t slot, const double* H, const
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, "host retained terms upload failed");
     s->host_set = true;
     return VIB_OK;
+}
+
+int vib_window_append(vib_context* ctx, int32_t slot, const vib_keyframe* kf, int32_t* unmatched) {
+    if (!ctx || !kf) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    return append_keyframe(ctx, *s, *kf, unmatched);
 }
 
 int vib_window_release(vib_context* ctx, int32_t slot) {
@@ -2705,6 +2864,7 @@ int vib_read_window(vib_context* ctx, int32_t slot, vib_window* out) {
     out->prior_blocks = (int32_t)w.prior_kind.size();
     put(w.prior_kind, out->prior_key_kind); put(w.prior_id, out->prior_key_id); put(w.prior_ref, out->prior_ref);
     put(w.prior_A, out->prior_A); put(w.prior_r0, out->prior_r0);
+    put(h.lm_uid, out->lm_id);
     return VIB_OK;
 }
 

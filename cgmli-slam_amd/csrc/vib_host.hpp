@@ -45,6 +45,8 @@ struct HostWindow {
     std::vector<int> lm_frame, lm_cam, lm_off;
     std::vector<uint8_t> lm_free;
     std::vector<double> lmd, rho;
+    std::vector<int64_t> lm_uid;  // stable landmark ids, strictly increasing
+    int64_t uid_next = 0;         // ids at or above this one were never in the window
     std::vector<int> ob_fc, ob_lm;
     std::vector<double> ob;
     std::vector<int> imu_from, imu_to;
@@ -65,8 +67,8 @@ struct HostWindow {
     std::vector<double> pf_ob, pf_an;
     std::vector<int32_t> pf_ids, pf_lm, pf_pos;
     std::vector<Chunk> lin_chunks;  // factor ranges of the visual-normal kernel
-    // The per-landmark and per-observation vectors (lm_frame .. rho, ob_fc, ob_lm, ob, lm_col and
-    // the pair-order planes) hold the resident window. False after a device edit: the device
+    // The per-landmark and per-observation vectors (lm_frame .. rho, lm_uid, ob_fc, ob_lm, ob,
+    // lm_col and the pair-order planes) hold the resident window. False after a device edit: the device
     // holds them, and ensure_bulk downloads them when a host path needs them.
     bool bulk = true;
 

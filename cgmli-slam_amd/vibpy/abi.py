@@ -69,7 +69,17 @@ class VibWindow(C.Structure):
         ("obs_velocity", dp), ("obs_tref", dp), ("obs_sqrt_info", dp),
         ("imu_from", ip), ("imu_to", ip), ("imu_data", dp),
         ("prior_rows", C.c_int32), ("prior_blocks", C.c_int32), ("prior_key_kind", ip),
-        ("prior_key_id", lp), ("prior_ref", dp), ("prior_A", dp), ("prior_r0", dp),
+        ("prior_key_id", lp), ("prior_ref", dp), ("prior_A", dp), ("prior_r0", dp), ("lm_id", lp),
+    ]
+
+
+class VibKeyframe(C.Structure):
+    _fields_ = [
+        ("frame_id", C.c_int64), ("pose", dp), ("speed_bias", dp), ("pose_free", C.c_int32), ("sb_width", C.c_int32),
+        ("imu_data", dp), ("num_landmarks", C.c_int32), ("num_observations", C.c_int32),
+        ("lm_id", lp), ("lm_anchor_cam", ip), ("lm_bearing", dp), ("lm_velocity", dp), ("lm_tref", dp),
+        ("lm_rho", dp), ("lm_free", up), ("obs_landmark_id", lp), ("obs_cam", ip), ("obs_bearing", dp),
+        ("obs_velocity", dp), ("obs_tref", dp), ("obs_sqrt_info", dp),
     ]
 
 
@@ -188,6 +198,7 @@ class Window:
     prior_ref: np.ndarray = None
     prior_A: np.ndarray = None   # (m, n)
     prior_r0: np.ndarray = None
+    lm_id: np.ndarray = None     # (L,) strictly increasing landmark ids (None: 0 .. L-1)
     meta: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -202,7 +213,7 @@ class Window:
         for name in ("lm_anchor_frame", "lm_anchor_cam", "obs_landmark", "obs_frame", "obs_cam", "imu_from",
                      "imu_to", "prior_key_kind"):
             setattr(self, name, i32(getattr(self, name)))
-        for name in ("frame_id", "prior_key_id"):
+        for name in ("frame_id", "prior_key_id", "lm_id"):
             setattr(self, name, i64(getattr(self, name)))
         for name in ("pose_free", "sb_width", "lm_free"):
             setattr(self, name, u8(getattr(self, name)))
@@ -279,6 +290,7 @@ class Window:
         w.min_inverse_depth = self.min_inverse_depth
         w.td_lower, w.td_upper = self.td_lower, self.td_upper
         w.frame_id = _ptr(self.frame_id, C.c_int64)
+        w.lm_id = _ptr(self.lm_id, C.c_int64)
         w.pose = _ptr(self.pose, C.c_double)
         w.speed_bias = _ptr(self.speed_bias, C.c_double)
         w.pose_free = _ptr(self.pose_free, C.c_uint8)
@@ -329,6 +341,7 @@ def load_library(path=LIB_PATH):
     vp = C.c_void_p
     sig = {
         "vib_abi_version": ([], C.c_int),
+        "vib_window_append": ([vp, C.c_int32, C.POINTER(VibKeyframe), ip], C.c_int),
         "vib_create": ([C.POINTER(vp), C.c_int], C.c_int),
         "vib_destroy": ([vp], C.c_int),
         "vib_last_error": ([vp], C.c_char_p),
@@ -389,6 +402,68 @@ def i32p(a):
     return a.ctypes.data_as(ip)
 
 
+@dataclass
+class Keyframe:
+    """A new keyframe for vib_window_append: state, IMU edge from the newest frame, the landmarks
+    it anchors (ids above every id of the window) and its observations (by landmark id)."""
+    frame_id: int
+    pose: np.ndarray                 # (7,)
+    speed_bias: np.ndarray           # (9,)
+    lm_id: np.ndarray
+    lm_anchor_cam: np.ndarray
+    lm_bearing: np.ndarray           # (N, 3)
+    lm_velocity: np.ndarray          # (N, 3)
+    lm_tref: np.ndarray
+    lm_rho: np.ndarray
+    obs_landmark_id: np.ndarray
+    obs_cam: np.ndarray
+    obs_bearing: np.ndarray          # (M, 3)
+    obs_velocity: np.ndarray         # (M, 3)
+    obs_tref: np.ndarray
+    obs_sqrt_info: np.ndarray        # (M, 3)
+    imu_data: np.ndarray = None      # (IMU_STRIDE,)
+    lm_free: np.ndarray = None
+    pose_free: int = 1
+    sb_width: int = 9
+
+    def __post_init__(self):
+        for name in ("pose", "speed_bias", "lm_bearing", "lm_velocity", "lm_tref", "lm_rho", "obs_bearing",
+                     "obs_velocity", "obs_tref", "obs_sqrt_info", "imu_data"):
+            a = getattr(self, name)
+            setattr(self, name, None if a is None else np.ascontiguousarray(a, dtype=np.float64))
+        for name in ("lm_anchor_cam", "obs_cam"):
+            setattr(self, name, np.ascontiguousarray(getattr(self, name), dtype=np.int32))
+        for name in ("lm_id", "obs_landmark_id"):
+            setattr(self, name, np.ascontiguousarray(getattr(self, name), dtype=np.int64))
+        if self.lm_free is not None:
+            self.lm_free = np.ascontiguousarray(self.lm_free, dtype=np.uint8)
+
+    def cstruct(self):
+        k = VibKeyframe()
+        k.frame_id = int(self.frame_id)
+        k.pose = _ptr(self.pose, C.c_double)
+        k.speed_bias = _ptr(self.speed_bias, C.c_double)
+        k.pose_free = int(self.pose_free)
+        k.sb_width = int(self.sb_width)
+        k.imu_data = _ptr(self.imu_data, C.c_double)
+        k.num_landmarks = int(self.lm_id.shape[0])
+        k.num_observations = int(self.obs_landmark_id.shape[0])
+        k.lm_id = _ptr(self.lm_id, C.c_int64)
+        k.lm_anchor_cam = _ptr(self.lm_anchor_cam, C.c_int32)
+        k.lm_bearing = _ptr(self.lm_bearing, C.c_double)
+        k.lm_velocity = _ptr(self.lm_velocity, C.c_double)
+        k.lm_tref = _ptr(self.lm_tref, C.c_double)
+        k.lm_rho = _ptr(self.lm_rho, C.c_double)
+        k.lm_free = _ptr(self.lm_free, C.c_uint8)
+        k.obs_landmark_id = _ptr(self.obs_landmark_id, C.c_int64)
+        k.obs_cam = _ptr(self.obs_cam, C.c_int32)
+        k.obs_bearing = _ptr(self.obs_bearing, C.c_double)
+        k.obs_velocity = _ptr(self.obs_velocity, C.c_double)
+        k.obs_tref = _ptr(self.obs_tref, C.c_double)
+        k.obs_sqrt_info = _ptr(self.obs_sqrt_info, C.c_double)
+        return k
+
+
 def window_buffers(K, C_, L, F, E, prior_rows, prior_cols, prior_blocks, prior_ref):
     """Zeroed arrays of a window at the given capacities, for the calls that write a window."""
     z = np.zeros
@@ -402,7 +477,7 @@ def window_buffers(K, C_, L, F, E, prior_rows, prior_cols, prior_blocks, prior_r
                   pose_free=z(K, np.uint8), sb_width=z(K, np.uint8), lm_free=z(L, np.uint8),
                   prior_key_kind=z(max(prior_blocks, 1), np.int32), prior_key_id=z(max(prior_blocks, 1), np.int64),
                   prior_ref=z(max(prior_ref, 1)), prior_A=z((max(prior_rows, 1), max(prior_cols, 1))),
-                  prior_r0=z(max(prior_rows, 1)))
+                  prior_r0=z(max(prior_rows, 1)), lm_id=z(L, np.int64))
 
 
 def trimmed_window(buf, cs, K, L, F, E, rows, cols, blocks, reflen):
@@ -425,7 +500,7 @@ def trimmed_window(buf, cs, K, L, F, E, rows, cols, blocks, reflen):
                   imu_to=buf.imu_to[:E].copy(), imu_data=buf.imu_data[:E].copy(), flags=int(cs.flags),
                   huber_delta=cs.huber_delta, min_inverse_depth=cs.min_inverse_depth, td_lower=cs.td_lower,
                   td_upper=cs.td_upper, pose_free=buf.pose_free[:K].copy(), sb_width=buf.sb_width[:K].copy(),
-                  lm_free=buf.lm_free[:L].copy(), **prior)
+                  lm_free=buf.lm_free[:L].copy(), lm_id=buf.lm_id[:L].copy(), **prior)
 
 
 def window_footprint(win, precision=VIB_FP64, lib=None):
@@ -538,6 +613,14 @@ class Context:
         s = win.cstruct()
         self._cams[slot] = int(win.cam_rotation.shape[0])
         return self.check(self.lib.vib_window_upload(self.h, slot, C.byref(s)), "vib_window_upload")
+
+    def append(self, slot, kf):
+        """vib_window_append: the keyframe becomes the window's newest frame; returns the number of
+        its observations whose landmark is no longer in the window (skipped)."""
+        k = kf.cstruct()
+        miss = C.c_int32()
+        self.check(self.lib.vib_window_append(self.h, slot, C.byref(k), C.byref(miss)), "vib_window_append")
+        return int(miss.value)
 
     def set_host_retained(self, slot, H=None, g=None):
         """Normal equations of host-evaluated factors (None clears them)."""

@@ -484,3 +484,140 @@ def session_config(i, seed_base=20261208):
     rng = np.random.default_rng(seed_base * 1000 + i)
     return dict(K=int(rng.integers(8, 21)), L=int(rng.integers(500, 30001)), n_cams=4, obs_range=(2, 10),
                 seed=seed_base * 1000 + i)
+
+
+def stream(K, n_frames, lm_per_frame, n_cams=4, obs_range=(2, 8), span=8, seed=20261211, rig_free=False,
+           td_free=False, pixel_sigma=1.0, outlier_fraction=0.0, outlier_px=20.0, speed=1.5, static_bearings=False,
+           sigma_g_c=1e-3, sigma_a_c=1e-2, init_pos=0.05, init_rot_deg=1.0, init_rho=0.05, lib=None):
+    """A session's stream of keyframes for the sliding-window request loop: the first window
+    (frames 0 .. K-1) and the keyframes K .. n_frames-1 that the window appends one per request.
+
+    Every frame gives birth to `lm_per_frame` landmarks: a random pixel of a random camera at a
+    depth of 2-20 m, anchored in that (frame, camera) and observed in up to obs_range[1] - 1 more
+    (frame, camera) pairs among the following `span` frames that see it. Landmark ids are global
+    and increase with the anchor frame (the order in which a backend creates them). Measurements,
+    noise, IMU samples and the perturbed initial estimates follow `generate`. Returns (window,
+    keyframes) with window.meta holding the truth, the IMU samples, frame times and parameters."""
+    lib = lib or abi.load_library()
+    rng = np.random.default_rng(seed)
+    cam_R, cam_t = make_rig(rng, n_cams)
+    R_BR_true = random_small_rotation(rng, np.deg2rad(1.0))
+    t_BR_true = rng.uniform(-0.02, 0.02, 3)
+    N = n_frames
+    frame_t = np.arange(N) * round(KEYFRAME_DT * IMU_RATE) / IMU_RATE
+    traj = Trajectory(rng, speed=speed, duration=max(KEYFRAME_DT * (N - 1), 0.5))
+    P, R, V, Wb = traj.position(frame_t), traj.rotation(frame_t), traj.velocity(frame_t), traj.body_rate(frame_t)
+    R_WC = np.einsum('kij,jl,clm->kcim', R, R_BR_true, cam_R)
+    t_WC = P[:, None, :] + np.einsum('kij,cj->kci', R, (R_BR_true @ cam_t.T).T + t_BR_true)
+    pts, pairs = [], []  # per landmark: world point, (frame, cam) pairs with the anchor first
+    for j in range(N):
+        m = lm_per_frame
+        c = rng.integers(0, n_cams, m)
+        u = rng.uniform(20, WIDTH - 20, m)
+        v = rng.uniform(20, HEIGHT - 20, m)
+        depth = rng.uniform(2.0, 20.0, m)
+        ray = np.stack([(u - CX) / FX, (v - CY) / FY, np.ones(m)], 1)
+        ray /= np.linalg.norm(ray, axis=1, keepdims=True)
+        world = t_WC[j, c] + np.einsum('nij,nj->ni', R_WC[j, c], ray * depth[:, None])
+        later = np.arange(j, min(N, j + span + 1))
+        d = world[:, None, None, :] - t_WC[later][None]
+        pc = np.einsum('kcji,mkcj->mkci', R_WC[later], d)
+        z = pc[..., 2]
+        with np.errstate(divide='ignore', invalid='ignore'):
+            uu = FX * pc[..., 0] / z + CX
+            vv = FY * pc[..., 1] / z + CY
+        vis = (z > 0.3) & (uu >= 1) & (uu < WIDTH - 1) & (vv >= 1) & (vv < HEIGHT - 1)
+        for i in range(m):
+            cand = [(int(later[a]), int(b)) for a, b in zip(*np.nonzero(vis[i])) if not (later[a] == j and b == c[i])]
+            if not cand:
+                continue
+            n = int(rng.integers(obs_range[0], obs_range[1] + 1)) - 1
+            pick = sorted(cand[x] for x in rng.choice(len(cand), min(n, len(cand)), replace=False))
+            pts.append(world[i])
+            pairs.append([(j, int(c[i]))] + pick)
+    L = len(pts)
+    pts = np.array(pts)
+    lm_of = np.concatenate([np.full(len(p), i) for i, p in enumerate(pairs)])
+    fk = np.concatenate([[f for f, _ in p] for p in pairs])
+    fc = np.concatenate([[cc for _, cc in p] for p in pairs])
+    d = pts[lm_of] - t_WC[fk, fc]
+    pc = np.einsum('nji,nj->ni', R_WC[fk, fc], d)
+    px = np.stack([FX * pc[:, 0] / pc[:, 2] + CX, FY * pc[:, 1] / pc[:, 2] + CY], 1)
+    noisy = px + rng.normal(0, pixel_sigma, px.shape)
+    if outlier_fraction > 0:
+        out = rng.random(len(noisy)) < outlier_fraction
+        ang = rng.uniform(0, 2 * np.pi, out.sum())
+        noisy[out] += outlier_px * np.stack([np.cos(ang), np.sin(ang)], 1)
+    bearing, sqrt_info = abi.pinhole_observations(FX, FY, CX, CY, pixel_sigma, noisy, lib=lib)
+    w_W = np.einsum('kij,kj->ki', R, Wb)
+    t_BC = (R_BR_true @ cam_t.T).T + t_BR_true
+    w_C = np.einsum('kcji,kj->kci', R_WC, w_W)
+    v_WC = V[:, None, :] + np.cross(w_W[:, None, :], np.einsum('kij,cj->kci', R, t_BC))
+    v_C = np.einsum('kcji,kcj->kci', R_WC, v_WC)
+    pdot = -np.cross(w_C[fk, fc], pc) - v_C[fk, fc]
+    rn = np.linalg.norm(pc, axis=1)
+    bt = pc / rn[:, None]
+    bvel = (pdot - bt * (bt * pdot).sum(1, keepdims=True)) / rn[:, None]
+    bvel = bvel - bearing * (bearing * bvel).sum(1, keepdims=True)
+    if static_bearings:
+        bvel = np.zeros_like(bvel)
+    first = np.concatenate([[0], np.cumsum([len(p) for p in pairs])[:-1]])
+    is_anchor = np.zeros(len(fk), bool)
+    is_anchor[first] = True
+    a_frame, a_cam = fk[first], fc[first]
+    true_rho = 1.0 / rn[first]
+    rho0 = true_rho * (1 + rng.uniform(-init_rho, init_rho, L))
+    # IMU and the perturbed initial estimate of every frame (frame 0 at truth: the gauge)
+    bg_true = rng.normal(0, 1e-3, 3)
+    ba_true = rng.normal(0, 1e-2, 3)
+    samples, _ = imu_samples(traj, frame_t[0], frame_t[-1], rng, sigma_g_c, sigma_a_c, bg_true, ba_true)
+    params = abi.imu_params(sigma_g_c=sigma_g_c, sigma_a_c=sigma_a_c)
+    pose = np.zeros((N, 7))
+    sb = np.zeros((N, 9))
+    for k in range(N):
+        Rk = R[k] if k == 0 else random_small_rotation(rng, np.deg2rad(init_rot_deg)) @ R[k]
+        pose[k, :3] = P[k] + (0 if k == 0 else rng.uniform(-1, 1, 3) * init_pos)
+        pose[k, 3:] = quat_from_matrix(Rk)
+        sb[k, :3] = V[k] + (0 if k == 0 else rng.normal(0, 0.02, 3))
+    imu = np.zeros((max(N - 1, 0), abi.IMU_STRIDE))
+    for e in range(N - 1):
+        imu[e] = abi.preintegrate(samples, frame_t[e], frame_t[e + 1], sb[e], params,
+                                  gyro_at(samples, frame_t[e]), gyro_at(samples, frame_t[e + 1]), lib=lib)
+    rig_true = np.concatenate([t_BR_true, quat_from_matrix(R_BR_true)])
+    rig = rig_true.copy()
+    if rig_free:
+        Rr = random_small_rotation(rng, np.deg2rad(0.3)) @ R_BR_true
+        rig = np.concatenate([t_BR_true + rng.uniform(-0.005, 0.005, 3), quat_from_matrix(Rr)])
+    flags = (abi.VIB_RIG_FREE if rig_free else 0) | (abi.VIB_TD_FREE if td_free else 0)
+    # the first window
+    lm_in = np.nonzero(a_frame < K)[0]
+    ob_in = np.nonzero(~is_anchor & (fk < K))[0]
+    ob_in = ob_in[np.argsort(lm_of[ob_in], kind="stable")]
+    sig = np.array([1e-3] * 3 + [1e-3] * 3 + [0.1] * 3 + [0.01] * 3 + [0.1] * 3)
+    win = abi.Window(
+        frame_id=np.arange(K) + 1000, pose=pose[:K], speed_bias=sb[:K], rig=rig, td=np.array([0.002 if td_free else 0.0]),
+        cam_rotation=cam_R, cam_translation=cam_t,
+        lm_anchor_frame=a_frame[lm_in], lm_anchor_cam=a_cam[lm_in], lm_bearing=bearing[first[lm_in]],
+        lm_velocity=bvel[first[lm_in]], lm_tref=np.zeros(lm_in.size), lm_rho=rho0[lm_in],
+        obs_landmark=np.searchsorted(lm_in, lm_of[ob_in]), obs_frame=fk[ob_in], obs_cam=fc[ob_in],
+        obs_bearing=bearing[ob_in], obs_velocity=bvel[ob_in], obs_tref=np.zeros(ob_in.size),
+        obs_sqrt_info=sqrt_info[ob_in], imu_from=np.arange(K - 1), imu_to=np.arange(1, K), imu_data=imu[:K - 1],
+        flags=flags, huber_delta=1.0, min_inverse_depth=1e-8, td_lower=-0.05, td_upper=0.05,
+        prior_key_kind=np.array([abi.VIB_KEY_POSE, abi.VIB_KEY_SPEED_BIAS]), prior_key_id=np.array([1000, 1000]),
+        prior_ref=np.concatenate([pose[0], sb[0]]), prior_A=np.diag(1.0 / sig), prior_r0=np.zeros(15),
+        lm_id=lm_in)
+    # the keyframes
+    kfs = []
+    for j in range(K, N):
+        new = np.nonzero(a_frame == j)[0]
+        ob = np.nonzero(~is_anchor & (fk == j))[0]
+        kfs.append(abi.Keyframe(
+            frame_id=1000 + j, pose=pose[j], speed_bias=sb[j], imu_data=imu[j - 1],
+            lm_id=new, lm_anchor_cam=a_cam[new], lm_bearing=bearing[first[new]], lm_velocity=bvel[first[new]],
+            lm_tref=np.zeros(new.size), lm_rho=rho0[new],
+            obs_landmark_id=lm_of[ob], obs_cam=fc[ob], obs_bearing=bearing[ob], obs_velocity=bvel[ob],
+            obs_tref=np.zeros(ob.size), obs_sqrt_info=sqrt_info[ob]))
+    win.meta = dict(seed=seed, true_pose=np.concatenate([P, np.array([quat_from_matrix(r) for r in R])], 1),
+                    true_rho=true_rho, true_rig=rig_true, samples=samples, frame_t=frame_t, imu_params=params,
+                    lm_anchor_frame=a_frame, obs=(lm_of, fk, fc, is_anchor))
+    return win, kfs

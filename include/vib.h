@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define VIB_ABI_VERSION 1
+#define VIB_ABI_VERSION 2
 
 /* ---- status codes (reference fallback classes, src/backend/BackendOptimizer.cpp) ---- */
 enum {
@@ -160,7 +160,36 @@ typedef struct vib_window {
     const double* prior_ref;        /* concatenated reference values (7/9/7/1 per key) */
     const double* prior_A;          /* [m * n] row-major, n = sum of tangent sizes     */
     const double* prior_r0;         /* [m]                                            */
+    /* landmark ids (the reference's LandmarkId): vib_window_append's observations name landmarks
+       by id, and the ids follow the landmarks through slides */
+    const int64_t* lm_id;           /* [L] strictly increasing; NULL = 0 .. L-1         */
 } vib_window;
+
+/* A new keyframe for vib_window_append: its state, the IMU edge from the window's newest frame,
+ * the landmarks it anchors and the observations it makes of resident or new landmarks. */
+typedef struct vib_keyframe {
+    int64_t frame_id;
+    const double* pose;             /* [7] initial estimate                           */
+    const double* speed_bias;       /* [9]                                            */
+    int32_t pose_free;              /* 1 free / 0 constant                            */
+    int32_t sb_width;               /* 9, 3 or 0                                      */
+    const double* imu_data;         /* [VIB_IMU_STRIDE] edge newest -> this frame; NULL: none */
+    int32_t num_landmarks;          /* N landmarks anchored in this frame              */
+    int32_t num_observations;       /* M observations made from this frame             */
+    const int64_t* lm_id;           /* [N] strictly increasing, above every id the window held */
+    const int32_t* lm_anchor_cam;   /* [N]                                            */
+    const double* lm_bearing;       /* [N*3]                                          */
+    const double* lm_velocity;      /* [N*3]                                          */
+    const double* lm_tref;          /* [N]                                            */
+    const double* lm_rho;           /* [N]                                            */
+    const uint8_t* lm_free;         /* [N] NULL = all free                            */
+    const int64_t* obs_landmark_id; /* [M] id of the observed landmark                 */
+    const int32_t* obs_cam;         /* [M]                                            */
+    const double* obs_bearing;      /* [M*3]                                          */
+    const double* obs_velocity;     /* [M*3]                                          */
+    const double* obs_tref;         /* [M]                                            */
+    const double* obs_sqrt_info;    /* [M*3]                                          */
+} vib_keyframe;
 
 /* solve option flags */
 #define VIB_SOLVE_CONTINUOUS 1u  /* relinearize every iteration and run the full iteration
@@ -261,6 +290,15 @@ int vib_resident_bytes(vib_context* ctx, int64_t* bytes);
 /* Bytes of HBM a window of this shape occupies (admission accounting). */
 int64_t vib_window_footprint(const vib_window* window, int32_t precision);
 int vib_window_release(vib_context* ctx, int32_t slot);
+/* Append a keyframe as the window's newest frame (FixedLagBackend::append,
+ * src/backend/FixedLagBackend.cpp:17-22, with the topology change that the next solve uploads,
+ * BackendOptimizer.cpp:169-209). A device edit inserts its landmarks and observations into the
+ * resident streams in the layout of an upload; nothing of the window is re-uploaded. Observations
+ * of ids no longer in the window (landmarks marginalized meanwhile) are skipped and counted in
+ * *unmatched (may be NULL). Errors: VIB_ERR_UNSUPPORTED_DIMENSIONS (the window already holds 20
+ * frames), VIB_ERR_INVALID_ARGUMENT (missing arrays, ids not increasing, camera out of range),
+ * VIB_ERR_OUT_OF_MEMORY (HBM budget); a failed append leaves the window as it was. */
+int vib_window_append(vib_context* ctx, int32_t slot, const vib_keyframe* keyframe, int32_t* unmatched);
 /* Copy the resident state back into the (in/out) arrays of `window` (readResident, :704-713). */
 int vib_read_state(vib_context* ctx, int32_t slot, vib_window* window);
 /* Shape of the resident window after marginalizations. */

@@ -62,3 +62,29 @@ def assert_imu_records_close(got, exp, rtol=1e-9):
         Wa, Wb = a[76:].reshape(15, 15), b[76:].reshape(15, 15)
         Ga, Gb = Wa.T @ Wa, Wb.T @ Wb
         assert np.linalg.norm(Ga - Gb) <= 1e-8 * np.linalg.norm(Gb)
+
+
+def same_layout(ctx, a, b):
+    """The window of slot a (edited on the device) uploaded to slot b through the host path steps
+    bit for bit alike: normal equations, reduced system, step and costs. Slot b is released."""
+    import numpy as np
+    w = ctx.read_window(a)
+    ctx.upload(b, w)
+    lay = w.layout()
+    nr, ne = lay["nr"], lay["ne"]
+    for s in (a, b):
+        ctx.check(ctx.linearize(s), "linearize")
+    sa, ra = ctx.solve_step(a, 1e-4)
+    sb, rb = ctx.solve_step(b, 1e-4)
+    assert sa == 0 and sb == 0, ctx.error()
+    Ha, ga, ca = ctx.read_normal(a, nr)
+    Hb, gb, cb = ctx.read_normal(b, nr)
+    assert np.all(np.isfinite(Ha)) and np.abs(Ha).max() > 0
+    assert np.array_equal(Ha, Hb) and np.array_equal(ga, gb) and ca == cb
+    Sa, rha = ctx.read_reduced(a, nr)
+    Sb, rhb = ctx.read_reduced(b, nr)
+    assert np.array_equal(Sa, Sb) and np.array_equal(rha, rhb)
+    assert np.array_equal(ctx.read_step(a, nr + ne), ctx.read_step(b, nr + ne))
+    assert ra.cost == rb.cost and ra.trial_cost == rb.trial_cost and ra.predicted_reduction == rb.predicted_reduction
+    ctx.release(b)
+    return w

@@ -22,7 +22,7 @@ def test_library_exports_every_declared_symbol(abi):
     exported = set(re.findall(r"\bT (vib_[a-z_0-9]+)", out))
     missing = [f for f in declared_functions() if f not in exported]
     assert not missing, missing
-    assert lib.vib_abi_version() == 1
+    assert lib.vib_abi_version() == 2
 
 
 def test_create_without_device_fails_cleanly(abi):
@@ -93,4 +93,20 @@ def test_result_structs_match_the_header(abi, tmp_path):
     want = [C.sizeof(abi.VibResidualStats), C.sizeof(abi.VibGateReport), C.sizeof(abi.VibRefitOptions),
             C.sizeof(abi.VibRefitReport), C.sizeof(abi.VibSolveReport), abi.VibRefitReport.pre.offset,
             abi.VibGateReport.admitted.offset]
+    assert got == want
+
+
+def test_window_and_keyframe_structs_match_the_header(abi, tmp_path):
+    """The ctypes mirrors of vib_window and vib_keyframe have the C compiler's sizes and offsets."""
+    import ctypes as C
+    src = tmp_path / "win.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "vib.h"\nint main(void) {\n'
+                   'printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(vib_window), offsetof(vib_window, prior_rows),\n'
+                   'offsetof(vib_window, lm_id), sizeof(vib_keyframe), offsetof(vib_keyframe, num_landmarks),\n'
+                   'offsetof(vib_keyframe, obs_sqrt_info));\nreturn 0;\n}\n')
+    exe = tmp_path / "win"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(abi.VibWindow), abi.VibWindow.prior_rows.offset, abi.VibWindow.lm_id.offset,
+            C.sizeof(abi.VibKeyframe), abi.VibKeyframe.num_landmarks.offset, abi.VibKeyframe.obs_sqrt_info.offset]
     assert got == want

@@ -14,7 +14,7 @@ oracle in test_gpu_parity and test_gpu_fullsize.
 import numpy as np
 import pytest
 
-from parity_helpers import say
+from parity_helpers import same_layout, say
 
 pytestmark = pytest.mark.gpu
 
@@ -24,31 +24,6 @@ def ctx(abi):
     c = abi.Context(0)
     yield c
     c.close()
-
-
-def _same_layout(ctx, abi, a, b):
-    """Slot a (device-edited) and its window uploaded to slot b step bit for bit alike."""
-    w = ctx.read_window(a)
-    ctx.upload(b, w)
-    lay = w.layout()
-    nr, ne = lay["nr"], lay["ne"]
-    for s in (a, b):
-        ctx.check(ctx.linearize(s), "linearize")
-    sa, ra = ctx.solve_step(a, 1e-4)
-    sb, rb = ctx.solve_step(b, 1e-4)
-    assert sa == 0 and sb == 0, ctx.error()
-    # the normal equations and reduced system of the step, the step, its trial cost
-    Ha, ga, ca = ctx.read_normal(a, nr)
-    Hb, gb, cb = ctx.read_normal(b, nr)
-    assert np.all(np.isfinite(Ha)) and np.abs(Ha).max() > 0
-    assert np.array_equal(Ha, Hb) and np.array_equal(ga, gb) and ca == cb
-    Sa, rha = ctx.read_reduced(a, nr)
-    Sb, rhb = ctx.read_reduced(b, nr)
-    assert np.array_equal(Sa, Sb) and np.array_equal(rha, rhb)
-    assert np.array_equal(ctx.read_step(a, nr + ne), ctx.read_step(b, nr + ne))
-    assert ra.cost == rb.cost and ra.trial_cost == rb.trial_cost and ra.predicted_reduction == rb.predicted_reduction
-    ctx.release(b)
-    return w
 
 
 def _second_new_args(w):
@@ -66,7 +41,7 @@ def test_old_after_a_step_keeps_the_host_layout(ctx, abi):
     assert st == 0, ctx.error()
     st, rep = ctx.marginalize(0, abi.VIB_MARGIN_OLD)
     assert st == 0, ctx.error()
-    nxt = _same_layout(ctx, abi, 0, 1)
+    nxt = same_layout(ctx, 0, 1)
     assert nxt.K == w.K - 1 and rep.eliminated_landmarks > 0 and nxt.L == w.L - rep.eliminated_landmarks
 
 
@@ -81,7 +56,7 @@ def test_second_new_with_merge_keeps_the_host_layout(ctx, abi):
     st, rep = ctx.marginalize(0, abi.VIB_MARGIN_SECOND_NEW, smp, t0, t1, params=prm)
     assert st == 0, ctx.error()
     assert rep.imu_mode in (abi.VIB_IMU_MERGE_INCREMENTAL, abi.VIB_IMU_MERGE_REINTEGRATED)
-    nxt = _same_layout(ctx, abi, 0, 1)
+    nxt = same_layout(ctx, 0, 1)
     assert nxt.K == w.K - 1 and nxt.E == w.E - 1
 
 
@@ -97,7 +72,7 @@ def test_reanchored_tracks_keep_the_host_layout(ctx, abi, seed):
     st, rep = ctx.marginalize(0, abi.VIB_MARGIN_SECOND_NEW, smp, t0, t1, params=prm)
     assert st == 0, ctx.error()
     assert rep.reset_landmarks + rep.eliminated_landmarks + rep.demoted_landmarks > 0
-    _same_layout(ctx, abi, 0, 1)
+    same_layout(ctx, 0, 1)
 
 
 def test_gate_rejections_are_stripped_on_the_device(ctx, abi):
@@ -112,7 +87,7 @@ def test_gate_rejections_are_stripped_on_the_device(ctx, abi):
     K, L, F = ctx.shape(0)[:3]
     st, rep = ctx.marginalize(0, abi.VIB_MARGIN_OLD)
     assert st == 0, ctx.error()
-    nxt = _same_layout(ctx, abi, 0, 1)
+    nxt = same_layout(ctx, 0, 1)
     # no rejected observation survives the slide
     s = nxt.obs_sqrt_info.reshape(-1, 3)
     assert not np.any(np.all(s == 0, axis=1))
@@ -129,5 +104,5 @@ def test_three_slides_in_a_row(ctx, abi):
         assert st == 0, ctx.error()
         st, rep = ctx.marginalize(0, pol)
         assert st == 0, ctx.error()
-        nxt = _same_layout(ctx, abi, 0, 1)
+        nxt = same_layout(ctx, 0, 1)
         assert nxt.K == w.K - 1 - i
