@@ -147,36 +147,54 @@ def step_error(ctx, abi, win, slot, lam=1e-4, refinement=2):
                 lam=lam, refinement_steps=refinement)
 
 
-def _session_window(i):
+def _session_stream(args):
+    """Session i of the serving workload: its first window and `n` keyframes to append. A request
+    solves a window of K frames (K 8-20): K - 1 resident frames and the appended keyframe; the
+    slide brings it back to K - 1."""
+    i, n = args
     from vibpy import synth
-    return synth.generate(**synth.session_config(i))
+    cfg = synth.session_config(i)
+    K = cfg["K"]
+    return synth.stream(K - 1, K + n, max(1, cfg["L"] // K), n_cams=4, obs_range=(2, 10), seed=cfg["seed"])
 
 
-def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb):
-    """Serving workload: n_sessions resident windows, Poisson requests at `rate` per second for
-    `seconds`, 1-3 LM iterations each, continuous batching by size bucket (vibpy.serve)."""
+def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb, precision, refinement):
+    """Serving workload: n_sessions sliding-window sessions; Poisson requests at `rate` per second
+    for `seconds`; each request appends its session's next keyframe, runs 1-3 LM iterations
+    batched with the other running requests of its size bucket and slides OLD back to the window
+    size, the prior staying resident (the library's request server, vib_server_*)."""
     import multiprocessing as mp
     from vibpy import serve
-    with mp.get_context("spawn").Pool(procs) as pool:
-        wins = pool.map(_session_window, range(n_sessions))
-    sessions = [serve.Session(i, w) for i, w in enumerate(wins)]
-    srv = serve.Server(ctx, sessions, int(budget_gb * (1 << 30)), full_iterations=True)
-    srv.preload()
-    ctx.synchronize()
     reqs = serve.poisson_requests(n_sessions, rate, seconds, np.random.default_rng(20261208))
-    t0 = time.perf_counter()
-    done = srv.run(reqs)
-    wall = time.perf_counter() - t0
-    for s in sessions:
-        if s.slot is not None:
-            ctx.release(s.slot)
+    need = np.bincount([r["session"] for r in reqs], minlength=n_sessions)
+    with mp.get_context("spawn").Pool(procs) as pool:
+        streams = pool.map(_session_stream, [(i, int(need[i])) for i in range(n_sessions)])
+    wins = [w for w, _ in streams]
+    kfs = [k for _, k in streams]
+    srv = serve.Server(ctx, wins, int(budget_gb * (1 << 30)), precision=precision, refinement=refinement,
+                       full_iterations=True)
+    try:
+        srv.preload()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        done = srv.run(reqs, keyframes=kfs)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
     out = serve.summary(done, wall)
     out.update(sessions=n_sessions, arrival_rate=rate, offered_seconds=seconds, hbm_budget_gb=budget_gb,
-               uploads=srv.uploads, evictions=srv.evictions,
-               landmarks=int(sum(w.L for w in wins)), factors=int(sum(w.F for w in wins)),
-               workload=f"{n_sessions} sessions (4 cameras, K 8-20, L 500-30000, 2-10 observations per landmark), "
-                        f"Poisson {rate:g} requests/s, 1-3 full LM iterations each (relinearized, no early "
-                        "stop), FP64, windows resident (no slide inside a request)")
+               uploads=st["uploads"], evictions=st["evictions"], launches=st["launches"], slides=st["slides"],
+               slides_per_s=st["slides"] / wall, appends=st["appends"],
+               phase_seconds=dict(admit=st["admit_seconds"], iterate=st["iterate_seconds"], slide=st["slide_seconds"]),
+               unmatched_observations=int(sum(r["unmatched"] for r in done)),
+               window_landmarks=int(sum(w.L for w in wins)), window_factors=int(sum(w.F for w in wins)),
+               precision="mixed" if precision == abi.VIB_MIXED else "fp64", refinement_steps=refinement,
+               workload=f"{n_sessions} sessions (own 4-camera rig, K 8-20 keyframes solved per request, 500-30000 landmarks per window, "
+                        f"2-10 observations per landmark); Poisson {rate:g} requests/s for {seconds:g} s; each request "
+                        "appends the session's next keyframe, runs 1-3 full LM iterations (relinearized, no early "
+                        "stop) batched by size bucket, then slides OLD back to the window size (prior resident); "
+                        "latency from arrival to the end of the slide")
     return out
 
 
@@ -392,7 +410,7 @@ def main():
     ap.add_argument("--batch-windows", type=int, default=256,
                     help="windows of the batched workload line (0: skip)")
     ap.add_argument("--batch-procs", type=int, default=16)
-    ap.add_argument("--serve-seconds", type=float, default=3.0,
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="seconds of Poisson arrivals of the serving line (0: skip)")
     ap.add_argument("--serve-sessions", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
@@ -401,6 +419,8 @@ def main():
     ap.add_argument("--whiten-observations", type=int, default=1 << 22,
                     help="pixel observations of the device whitening line (0: skip)")
     ap.add_argument("--no-slides", dest="slides", action="store_false", help="skip the window-slide line")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving line (prints its JSON object)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched workload line (prints its JSON object; for per-kernel profiles)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -413,6 +433,13 @@ def main():
         sys.exit(2)
 
     from vibpy import abi, synth
+    if args.serving_only:
+        ctx = abi.Context(0)
+        serving = serving_line(ctx, abi, args.serve_sessions, args.serve_rate, args.serve_seconds, args.batch_procs,
+                               64.0, abi.VIB_MIXED, MIXED_REFINEMENT)
+        ctx.close()
+        print(json.dumps(dict(serving=serving)))
+        return
     if args.batched_only:
         bwins = batch_windows(args.batch_windows, args.batch_procs)
         ctx = abi.Context(0)
@@ -463,7 +490,7 @@ def main():
         print(f"[bench] serving: {args.serve_sessions} sessions, {args.serve_rate:g} requests/s", file=sys.stderr,
               flush=True)
         serving = serving_line(ctx, abi, args.serve_sessions, args.serve_rate, args.serve_seconds, args.batch_procs,
-                               64.0)
+                               64.0, abi.VIB_MIXED, MIXED_REFINEMENT)
     preint = None
     if args.preint_intervals > 0:
         print(f"[bench] preintegration: {args.preint_intervals} intervals", file=sys.stderr, flush=True)

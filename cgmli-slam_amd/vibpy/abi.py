@@ -73,6 +73,25 @@ class VibWindow(C.Structure):
     ]
 
 
+class VibServerOptions(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("precision", C.c_int32), ("refinement_steps", C.c_int32),
+                ("slide_policy", C.c_int32), ("full_iterations", C.c_int32), ("first_slot", C.c_int32),
+                ("num_buckets", C.c_int32), ("bucket_limit", C.c_int32 * 8)]
+
+
+class VibServed(C.Structure):
+    _fields_ = [("request", C.c_int64), ("session", C.c_int32), ("status", C.c_int32), ("iterations", C.c_int32),
+                ("unmatched", C.c_int32), ("final_cost", C.c_double), ("queued_seconds", C.c_double),
+                ("latency_seconds", C.c_double)]
+
+
+class VibServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("submitted", "finished", "failed", "turns", "launches", "window_iterations",
+                                         "uploads", "evictions", "appends", "slides", "resident_bytes")] + \
+               [(n, C.c_int32) for n in ("queued", "running", "resident", "pending_results")] + \
+               [(n, C.c_double) for n in ("admit_seconds", "iterate_seconds", "slide_seconds")]
+
+
 class VibKeyframe(C.Structure):
     _fields_ = [
         ("frame_id", C.c_int64), ("pose", dp), ("speed_bias", dp), ("pose_free", C.c_int32), ("sb_width", C.c_int32),
@@ -342,6 +361,14 @@ def load_library(path=LIB_PATH):
     sig = {
         "vib_abi_version": ([], C.c_int),
         "vib_window_append": ([vp, C.c_int32, C.POINTER(VibKeyframe), ip], C.c_int),
+        "vib_server_create": ([vp, C.POINTER(VibServerOptions), C.POINTER(vp)], C.c_int),
+        "vib_server_destroy": ([vp], C.c_int),
+        "vib_server_add_session": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
+        "vib_server_submit": ([vp, C.c_int64, C.c_int32, C.c_int32, C.POINTER(VibKeyframe)], C.c_int),
+        "vib_server_turn": ([vp, C.POINTER(VibServed), C.c_int32, ip], C.c_int),
+        "vib_server_get_stats": ([vp, C.POINTER(VibServerStats)], C.c_int),
+        "vib_server_preload": ([vp], C.c_int),
+        "vib_server_session_slot": ([vp, C.c_int32, ip], C.c_int),
         "vib_create": ([C.POINTER(vp), C.c_int], C.c_int),
         "vib_destroy": ([vp], C.c_int),
         "vib_last_error": ([vp], C.c_char_p),
@@ -769,10 +796,11 @@ class Context:
                                                 done.ctypes.data_as(ip), reps), "vib_request_iterate")
         return done.astype(bool), list(reps)
 
-    def read_window(self, slot):
-        """The whole resident window (vib_read_window) as a Window; observations in landmark order."""
+    def read_window(self, slot, cameras=None):
+        """The whole resident window (vib_read_window) as a Window; observations in landmark order.
+        cameras: the window's camera count (known from the upload through this object otherwise)."""
         K, L, F, E, rows, cols = self.shape(slot)
-        ncam = int(self._cams.get(slot, 0))
+        ncam = int(self._cams.get(slot, 0) if cameras is None else cameras)
         blocks = max(cols, 1)
         buf = window_buffers(K, ncam, L, F, E, rows, cols, blocks, 2 * cols + 2)
         cs = buf.cstruct()

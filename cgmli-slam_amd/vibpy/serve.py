@@ -1,19 +1,11 @@
-"""Serving loop: continuous batching of window requests from many sessions on one device.
+"""Driver of the library's request server (vib_server_*, the serving loop in C++).
 
-Each session keeps its window (and its marginalization prior) resident in a slot of the
-library. A request asks for 1-3 LM iterations on its session's window. Requests arrive over
-time; a session serves its requests in order. Every loop turn:
-  1. moves the requests that have arrived into their sessions' queues;
-  2. admits the first queued request of every idle session: if the session's window is not
-     resident, it is uploaded when its HBM footprint fits the budget, after evicting resident
-     windows of idle sessions (least recently used first); otherwise the request waits;
-  3. runs one LM iteration for the running requests of one size bucket (retained dimension)
-     in one batched launch sequence (vib_request_iterate), the buckets taking turns;
-  4. retires the requests that finished, with their latency (arrival to completion).
-Requests that arrive while others are on the device therefore join at the next iteration of
-their bucket. The LM work and every per-window decision run in the library.
+The scheduling (admission by HBM footprint, size buckets, continuous batching, the per-request
+append / LM iterations / slide) runs in the library. This module plays the clients: requests of
+many sessions arrive over time (Poisson), each is submitted once its arrival time has passed,
+with its session's next keyframe, and the loop turns until every request has finished.
 """
-import collections
+import ctypes as C
 import time
 
 import numpy as np
@@ -21,128 +13,95 @@ import numpy as np
 from . import abi
 
 
-class Session:
-    def __init__(self, sid, window):
-        self.sid = sid
-        self.window = window
-        self.slot = None
-        self.footprint = abi.window_footprint(window)
-        self.nr = window.layout()["nr"]
-        self.queue = collections.deque()
-        self.active = None
-        self.last_used = 0.0
-        self.dirty = False  # resident state newer than self.window
-
-
 class Server:
-    def __init__(self, ctx, sessions, hbm_budget_bytes, buckets=(128, 224, 320), precision=abi.VIB_FP64,
-                 first_slot=5000, full_iterations=False):
-        """full_iterations: every request runs its whole iteration budget, relinearizing each
-        time, instead of stopping at convergence (the load of fresh frames on windows that do
-        not slide)."""
+    def __init__(self, ctx, windows, hbm_budget_bytes=0, buckets=(128, 224, 320), precision=abi.VIB_FP64,
+                 refinement=0, slide_policy=abi.VIB_MARGIN_OLD, full_iterations=False, first_slot=5000):
+        """windows: the sessions' first windows (session i = windows[i]). full_iterations: every
+        request runs its whole iteration budget, relinearizing each time."""
         self.ctx = ctx
-        self.flags = abi.VIB_SOLVE_CONTINUOUS if full_iterations else 0
-        self.sessions = sessions
-        self.budget = hbm_budget_bytes
-        self.buckets = buckets
-        self.precision = precision
-        self.resident = 0
-        self.free_slots = collections.deque(range(first_slot, first_slot + len(sessions)))
-        self.turn = 0
-        self.uploads = 0
-        self.evictions = 0
-        self.on_done = None  # callback(request, session) after a request finishes
+        self.lib = ctx.lib
+        o = abi.VibServerOptions()
+        o.hbm_budget = int(hbm_budget_bytes)
+        o.precision = precision
+        o.refinement_steps = refinement
+        o.slide_policy = slide_policy
+        o.full_iterations = 1 if full_iterations else 0
+        o.first_slot = first_slot
+        o.num_buckets = len(buckets)
+        for i, b in enumerate(buckets):
+            o.bucket_limit[i] = int(b)
+        self.h = C.c_void_p()
+        ctx.check(self.lib.vib_server_create(ctx.h, C.byref(o), C.byref(self.h)), "vib_server_create")
+        for i, w in enumerate(windows):
+            cs = w.cstruct()
+            ctx.check(self.lib.vib_server_add_session(self.h, i, C.byref(cs)), "vib_server_add_session")
+        self._out = (abi.VibServed * 512)()
 
-    def bucket_of(self, s):
-        for i, b in enumerate(self.buckets):
-            if s.nr <= b:
-                return i
-        return len(self.buckets) - 1
-
-    def _evict(self, s):
-        if s.dirty:
-            self.ctx.read_state(s.slot, s.window)
-            s.dirty = False
-        self.ctx.release(s.slot)
-        self.free_slots.append(s.slot)
-        s.slot = None
-        self.resident -= s.footprint
-        self.evictions += 1
-
-    def _make_resident(self, s, now):
-        if s.slot is not None:
-            return True
-        if s.footprint > self.budget:
-            raise RuntimeError(f"session {s.sid} needs {s.footprint} bytes, more than the HBM budget")
-        idle = sorted((o for o in self.sessions if o.slot is not None and o.active is None and not o.queue),
-                      key=lambda o: o.last_used)
-        while self.resident + s.footprint > self.budget and idle:
-            self._evict(idle.pop(0))
-        if self.resident + s.footprint > self.budget:
-            return False
-        s.slot = self.free_slots.popleft()
-        self.ctx.upload(s.slot, s.window)
-        self.resident += s.footprint
-        self.uploads += 1
-        s.last_used = now
-        return True
+    def close(self):
+        if self.h:
+            self.lib.vib_server_destroy(self.h)
+            self.h = C.c_void_p()
 
     def preload(self):
-        """Makes as many sessions resident as the budget allows (before a timed run)."""
-        for s in self.sessions:
-            if not self._make_resident(s, 0.0):
-                break
+        """Makes the sessions' windows resident (session order) until the HBM budget is full."""
+        self.ctx.check(self.lib.vib_server_preload(self.h), "vib_server_preload")
 
-    def run(self, requests, clock=time.perf_counter):
-        """Serves `requests` (dicts with session, iterations, arrival in seconds from the start;
-        sorted by arrival) in real time. Returns the finished requests with `done` (seconds from
-        the start), `latency`, `status`, `final_cost`, `iterations_run`."""
-        pending = collections.deque(requests)
+    def submit(self, request, session, iterations, keyframe=None):
+        k = None if keyframe is None else C.byref(keyframe.cstruct())
+        self.ctx.check(self.lib.vib_server_submit(self.h, int(request), int(session), int(iterations), k),
+                       "vib_server_submit")
+
+    def turn(self):
+        n = C.c_int32()
+        self.ctx.check(self.lib.vib_server_turn(self.h, self._out, len(self._out), C.byref(n)), "vib_server_turn")
+        return [dict(id=int(r.request), session=int(r.session), status=int(r.status), iterations_run=int(r.iterations),
+                     unmatched=int(r.unmatched), final_cost=float(r.final_cost), queued=float(r.queued_seconds),
+                     latency=float(r.latency_seconds)) for r in self._out[:n.value]]
+
+    def stats(self):
+        st = abi.VibServerStats()
+        self.ctx.check(self.lib.vib_server_get_stats(self.h, C.byref(st)), "vib_server_get_stats")
+        return {name: getattr(st, name) for name, _ in abi.VibServerStats._fields_}
+
+    def slot(self, session):
+        v = C.c_int32()
+        self.ctx.check(self.lib.vib_server_session_slot(self.h, int(session), C.byref(v)), "vib_server_session_slot")
+        return int(v.value)
+
+    def run(self, requests, keyframes=None, on_done=None, clock=time.perf_counter):
+        """Serves `requests` (dicts with id, session, iterations, arrival in seconds from the start;
+        sorted by arrival) in real time: each is submitted once its arrival has passed, with the
+        next keyframe of keyframes[session] when keyframes is given. on_done(result) is called
+        right after the turn that finished the request (its session's window is still resident
+        and unchanged then). Returns the finished requests."""
+        pending = list(requests)
+        arrival = {r["id"]: r["arrival"] for r in pending}
+        nxt = {}
         t0 = clock()
-        finished = []
-        active = {}  # slot -> session
-        while pending or active or any(s.queue for s in self.sessions):
+        finished, i = [], 0
+        while len(finished) < len(pending):
             now = clock() - t0
-            while pending and pending[0]["arrival"] <= now:
-                r = pending.popleft()
-                self.sessions[r["session"]].queue.append(r)
-            for s in self.sessions:
-                if s.active is None and s.queue and self._make_resident(s, now):
-                    r = s.queue.popleft()
-                    opts = abi.options(max_iterations=r["iterations"], precision=self.precision, flags=self.flags)
-                    self.ctx.request_begin(s.slot, opts)
-                    r["start"] = now
-                    s.active = r
-                    active[s.slot] = s
-            if not active:
-                if pending:
-                    wait = pending[0]["arrival"] - (clock() - t0)
+            while i < len(pending) and pending[i]["arrival"] <= now:
+                r = pending[i]
+                kf = None
+                if keyframes is not None:
+                    j = nxt.get(r["session"], 0)
+                    kf = keyframes[r["session"]][j]
+                    nxt[r["session"]] = j + 1
+                self.submit(r["id"], r["session"], r["iterations"], kf)
+                i += 1
+            out = self.turn()
+            for res in out:
+                res["arrival"] = arrival[res["id"]]
+                if on_done:
+                    on_done(res)
+            finished.extend(out)
+            if not out and i < len(pending):
+                st = self.stats()
+                if st["running"] == 0 and st["queued"] == 0:
+                    wait = pending[i]["arrival"] - (clock() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 1e-3))
-                continue
-            # the next bucket (in turn) that has running requests
-            by_bucket = collections.defaultdict(list)
-            for slot, s in active.items():
-                by_bucket[self.bucket_of(s)].append(slot)
-            keys = sorted(by_bucket)
-            b = keys[self.turn % len(keys)]
-            self.turn += 1
-            slots = by_bucket[b]
-            done, reps = self.ctx.request_iterate(slots, self.precision)
-            t = clock() - t0
-            for slot, d, rep in zip(slots, done, reps):
-                if not d:
-                    continue
-                s = active.pop(slot)
-                r = s.active
-                s.active = None
-                s.dirty = True
-                s.last_used = t
-                r.update(done=t, latency=t - r["arrival"], status=rep.status, final_cost=rep.final_cost,
-                         iterations_run=rep.iterations)
-                finished.append(r)
-                if self.on_done:
-                    self.on_done(r, s)
         return finished
 
 
@@ -164,4 +123,6 @@ def summary(finished, wall):
     return dict(requests=len(finished), window_steps=steps, wall_s=wall, steps_per_s=steps / wall,
                 requests_per_s=len(finished) / wall, p50_ms=float(np.percentile(lat, 50)),
                 p99_ms=float(np.percentile(lat, 99)), max_ms=float(lat.max()),
-                failed=int(sum(1 for r in finished if r["status"] != 0)))
+                failed=int(sum(1 for r in finished if r["status"] != 0)),
+                statuses={abi.STATUS_NAMES.get(k, str(k)): int(v) for k, v in
+                          zip(*np.unique([r["status"] for r in finished], return_counts=True))})

@@ -527,14 +527,18 @@ def stream(K, n_frames, lm_per_frame, n_cams=4, obs_range=(2, 8), span=8, seed=2
             uu = FX * pc[..., 0] / z + CX
             vv = FY * pc[..., 1] / z + CY
         vis = (z > 0.3) & (uu >= 1) & (uu < WIDTH - 1) & (vv >= 1) & (vv < HEIGHT - 1)
-        for i in range(m):
-            cand = [(int(later[a]), int(b)) for a, b in zip(*np.nonzero(vis[i])) if not (later[a] == j and b == c[i])]
-            if not cand:
-                continue
-            n = int(rng.integers(obs_range[0], obs_range[1] + 1)) - 1
-            pick = sorted(cand[x] for x in rng.choice(len(cand), min(n, len(cand)), replace=False))
+        vis[np.arange(m), 0, c] = False  # the anchor view itself
+        vis = vis.reshape(m, -1)         # (frame, camera) pairs in increasing order
+        # n - 1 further views per landmark, uniformly among the visible ones: the smallest
+        # random keys of the visible pairs
+        n = rng.integers(obs_range[0], obs_range[1] + 1, m) - 1
+        keys = np.where(vis, rng.random(vis.shape), np.inf)
+        rank = np.argsort(np.argsort(keys, axis=1), axis=1)
+        pick = vis & (rank < n[:, None])
+        for i in np.nonzero(pick.any(1))[0]:
+            sel = np.nonzero(pick[i])[0]
             pts.append(world[i])
-            pairs.append([(j, int(c[i]))] + pick)
+            pairs.append([(j, int(c[i]))] + [(int(later[q // n_cams]), int(q % n_cams)) for q in sel])
     L = len(pts)
     pts = np.array(pts)
     lm_of = np.concatenate([np.full(len(p), i) for i, p in enumerate(pairs)])

@@ -23,8 +23,8 @@
  * one of the reference's fallback reasons (src/backend/BackendOptimizer.cpp:32-308);
  * vib_last_error() gives the message. No function throws.
  *
- * Threading: one context may be driven by one host thread at a time; the serving loop
- * (vib_serve_*) is the only multi-request client and serializes internally.
+ * Threading: every entry point takes the context's lock, so threads may share a context; the
+ * request server (vib_server_*) is a client of these entry points driven by one thread.
  */
 #ifndef VIB_H
 #define VIB_H
@@ -359,6 +359,67 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
 int vib_request_begin(vib_context* ctx, int32_t slot, const vib_solve_options* options);
 int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* slots, int32_t precision,
                         int32_t refinement_steps, int32_t* done, vib_solve_report* reports);
+
+/* The request server: many sessions, each a sliding window whose requests bring the next
+ * keyframe. Serving a request is the reference's per-frame cycle (FixedLagBackend::append, then
+ * step: the solve, then the slide once the window is over its size;
+ * src/backend/FixedLagBackend.cpp:17-22, 405-451): vib_window_append, LM iterations batched
+ * with the other running requests of its size bucket, and vib_marginalize back to the
+ * session's window size, the prior staying resident. Each vib_server_turn (1) admits the first
+ * queued request of every idle session whose window is resident or fits the HBM budget (idle
+ * sessions with nothing queued are evicted to host memory, least recently used first), (2) runs
+ * one LM iteration for the running requests of one size bucket (the buckets take turns), and (3)
+ * slides and reports the requests that finished. Requests submitted while others run join at
+ * the next turn. The server uses context slots from first_slot on; one thread drives it. */
+typedef struct vib_server vib_server;
+typedef struct vib_server_options {
+    int64_t hbm_budget;          /* bytes of resident windows the server admits (0: no limit)  */
+    int32_t precision;           /* VIB_FP64 or VIB_MIXED                                       */
+    int32_t refinement_steps;    /* refinement passes of a mixed-precision step                */
+    int32_t slide_policy;        /* VIB_MARGIN_OLD or VIB_MARGIN_SECOND_NEW after a request that
+                                    grew the window; -1: no slide                             */
+    int32_t full_iterations;     /* 1: every request runs its whole budget (VIB_SOLVE_CONTINUOUS) */
+    int32_t first_slot;          /* context slots first_slot, first_slot + 1, ... are the server's */
+    int32_t num_buckets;         /* size buckets by retained dimension (<= 8; 0: one bucket)   */
+    int32_t bucket_limit[8];     /* largest retained dimension of each bucket (ascending)      */
+} vib_server_options;
+typedef struct vib_served {      /* one finished request */
+    int64_t request;
+    int32_t session;
+    int32_t status;              /* VIB_OK or the first error of its append, solve or slide   */
+    int32_t iterations;          /* LM iterations run                                          */
+    int32_t unmatched;           /* keyframe observations of landmarks no longer resident      */
+    double final_cost;
+    double queued_seconds;       /* submission to admission                                   */
+    double latency_seconds;      /* submission to completion, slide included                  */
+} vib_served;
+typedef struct vib_server_stats {
+    int64_t submitted, finished, failed, turns, launches, window_iterations;
+    int64_t uploads, evictions, appends, slides, resident_bytes;
+    int32_t queued, running, resident, pending_results;
+    /* host wall time spent in each phase */
+    double admit_seconds;        /* uploads, evictions, appends, request starts               */
+    double iterate_seconds;      /* batched LM iterations                                    */
+    double slide_seconds;        /* slides                                                   */
+} vib_server_stats;
+int vib_server_create(vib_context* ctx, const vib_server_options* options, vib_server** server);
+/* Releases the server's resident windows. */
+int vib_server_destroy(vib_server* server);
+/* A session and its first window (copied; uploaded when the session's first request is admitted).
+ * Its frame count is the size the session's window slides back to. */
+int vib_server_add_session(vib_server* server, int32_t session, const vib_window* window);
+/* Queues a request of `session`: append `keyframe` (copied; NULL: none), then at most
+ * `iterations` LM iterations, then the slide. Requests of one session are served in order. */
+int vib_server_submit(vib_server* server, int64_t request, int32_t session, int32_t iterations,
+                      const vib_keyframe* keyframe);
+/* One turn of the loop; up to `capacity` finished requests are copied to `finished` (the rest
+ * are kept for the next call), their number in *count. */
+int vib_server_turn(vib_server* server, vib_served* finished, int32_t capacity, int32_t* count);
+/* Makes the sessions' windows resident in session order until the HBM budget is full. */
+int vib_server_preload(vib_server* server);
+int vib_server_get_stats(const vib_server* server, vib_server_stats* stats);
+/* The context slot holding the session's window, -1 while it is evicted. */
+int vib_server_session_slot(const vib_server* server, int32_t session, int32_t* slot);
 
 /* ---------------- measurement harness (no counterpart in the reference) ---------------- */
 /* Number of distinct kernels of one LM iteration and their names (for per-kernel timing). */

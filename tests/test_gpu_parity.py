@@ -432,47 +432,3 @@ def test_mixed_step_within_1e6_of_fp64(ctx, abi, name, cfg, solve_tol):
     assert abs(s32.final_cost - s64.final_cost) <= solve_tol[0] * s64.final_cost
     assert np.abs(g32.pose - g64.pose).max() < 1e-8
     assert np.abs(g32.lm_rho - g64.lm_rho).max() < solve_tol[1] * max(1.0, np.abs(g64.lm_rho).max())
-
-
-def test_served_requests_equal_alone(ctx, abi):
-    """Continuous batching: requests of six sessions arrive (Poisson) while others are on the
-    device and join the running launches of their size bucket; an HBM budget below the six
-    windows' footprint forces evictions and re-uploads. Every request's result (final cost,
-    poses and inverse ranges right after it) equals the same request sequence of its session
-    served alone, one window per launch: cost within 1e-10 relative, states within 1e-7."""
-    from vibpy import serve, synth
-    cfgs = [dict(K=6 + i, L=200 + 150 * i, n_cams=2 + i % 3, seed=70 + i, obs_range=(2, 7)) for i in range(6)]
-    wins = [synth.generate(**c) for c in cfgs]
-    reqs = serve.poisson_requests(len(wins), 400.0, 0.15, np.random.default_rng(5))
-    assert len(reqs) > 20
-    sessions = [serve.Session(i, w.copy()) for i, w in enumerate(wins)]
-    budget = sum(sorted(s.footprint for s in sessions)[-4:])
-    srv = serve.Server(ctx, sessions, budget, buckets=(64, 128, 320))
-    snaps = {}
-
-    def on_done(r, s):
-        snap = s.window.copy()
-        ctx.read_state(s.slot, snap)
-        snaps[r["id"]] = (r["final_cost"], r["status"], snap)
-
-    srv.on_done = on_done
-    say(f"[gpu] serving {len(reqs)} requests of {len(wins)} sessions")
-    finished = srv.run([dict(r) for r in reqs])
-    assert len(finished) == len(reqs)
-    assert srv.evictions > 0
-    say(f"[gpu] served: {srv.uploads} uploads, {srv.evictions} evictions; replaying alone")
-    for i, w in enumerate(wins):
-        ctx.upload(0, w)
-        cur = w.copy()
-        for r in (r for r in reqs if r["session"] == i):
-            ctx.request_begin(0, abi.options(max_iterations=r["iterations"]))
-            while True:
-                done, reps = ctx.request_iterate([0])
-                if done[0]:
-                    break
-            ctx.read_state(0, cur)
-            cost, status, snap = snaps[r["id"]]
-            assert reps[0].status == status
-            assert abs(reps[0].final_cost - cost) <= 1e-10 * abs(cost)
-            assert np.abs(cur.pose - snap.pose).max() < 1e-7
-            assert np.abs(cur.lm_rho - snap.lm_rho).max() < 1e-7 * max(1.0, np.abs(snap.lm_rho).max())
