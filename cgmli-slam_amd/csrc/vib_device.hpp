@@ -178,6 +178,16 @@ struct Chunk {
 
 constexpr int kLinFactors = 512;  // factors per visual-normal workgroup (pair order)
 
+// One symmetric eigendecomposition of a batched k_sym_eigen launch (one workgroup per job).
+struct EigenJob {
+    double* A;     // n x n (leading dimension ld), overwritten
+    double* V;     // eigenvectors as rows
+    double* ev;    // eigenvalues (unsorted)
+    double* rot;   // rotation log (eigen_log_doubles(n))
+    int32_t* ok;   // 1: converged
+    int32_t n, ld;
+};
+
 // One window edit (vib_edit.hip): which landmarks and observations of a resident window the
 // edited window keeps, and how its frames are renumbered.
 struct EditSpec {

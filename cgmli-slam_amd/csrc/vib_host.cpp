@@ -66,7 +66,8 @@ hipError_t vibk_nis_apply(const vib::WinDev* wins, const uint8_t* decision, int 
 hipError_t vibk_preintegrate(int count, const double* smp, const int* begin, const int* end, const double* t0,
                              const double* t1, const double* sbl, const vib_imu_params* prm, const double* g0,
                              const double* g1, double* out, int* status, hipStream_t s);
-hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, double* rot, hipStream_t s);
+hipError_t vibk_sym_eigen_batch(const vib::EigenJob* jobs, int count, int max_n, hipStream_t s);
+int vibk_sym_eigen_max(void);
 size_t vibk_sym_eigen_log(int n);
 }
 
@@ -740,7 +741,10 @@ struct vib_context {
     size_t eig_cap = 0;
     void* aux_buf = nullptr;
     size_t aux_cap = 0;
-    std::unique_ptr<vib::Slot> marginal_slot, two_frame_slot;
+    std::unique_ptr<vib::Slot> two_frame_slot;
+    // the marginal systems and the edit buffers of the slides of one batch (one each per slot)
+    std::vector<std::unique_ptr<vib::Slot>> marginal_pool;
+    std::vector<std::pair<void*, size_t>> edit_pool;
     // Window edits: the buffers of the scans and selections, and a spare arena that an edit in
     // place (a slide) builds the new window in before it takes the slot's place (the slot's old
     // arena becomes the spare).
@@ -754,6 +758,8 @@ struct vib_context {
         if (aux_buf) (void)hipFree(aux_buf);
         if (edit_buf) (void)hipFree(edit_buf);
         if (spare) (void)hipFree(spare);
+        for (auto& e : edit_pool)
+            if (e.first) (void)hipFree(e.first);
     }
 };
 
@@ -1005,7 +1011,7 @@ struct EditBufs {
     int64_t* ins_uid = nullptr;
 };
 
-bool edit_buffers(vib_context* ctx, int L, int F, EditBufs& b, int nl = 0, int nf = 0) {
+bool edit_buffers(void*& buf, size_t& cap, int L, int F, EditBufs& b, int nl = 0, int nf = 0) {
     size_t off = 0;
     auto take = [&](size_t bytes) {
         off = (off + 255) & ~size_t(255);
@@ -1028,7 +1034,7 @@ bool edit_buffers(vib_context* ctx, int L, int F, EditBufs& b, int nl = 0, int n
     const size_t o_add = take(4 * Lx), o_iou = take(8 * NF), o_iol = take(4 * NF), o_ior = take(4 * NF),
                  o_iof = take(4 * NF), o_iob = take(8 * OB_PLANES * NF), o_ilf = take(4 * NL),
                  o_ild = take(8 * LM_PLANES * NL), o_ilr = take(8 * NL), o_ilfr = take(NL), o_ilu = take(8 * NL);
-    char* base = static_cast<char*>(grow_device(ctx->edit_buf, ctx->edit_cap, off));
+    char* base = static_cast<char*>(grow_device(buf, cap, off));
     if (!base) return false;
     auto at = [&](size_t o) { return static_cast<void*>(base + o); };
     b.src = static_cast<WinDev*>(at(o_src));
@@ -1069,6 +1075,15 @@ bool edit_buffers(vib_context* ctx, int L, int F, EditBufs& b, int nl = 0, int n
     b.ins_free = static_cast<uint8_t*>(at(o_ilfr));
     b.ins_uid = static_cast<int64_t*>(at(o_ilu));
     return true;
+}
+
+// The edit buffers of the context (one edit at a time) and of job i of a batch of slides.
+bool edit_buffers(vib_context* ctx, int L, int F, EditBufs& b, int nl = 0, int nf = 0) {
+    return edit_buffers(ctx->edit_buf, ctx->edit_cap, L, F, b, nl, nf);
+}
+bool job_edit_buffers(vib_context* ctx, size_t i, int L, int F, EditBufs& b) {
+    while (ctx->edit_pool.size() <= i) ctx->edit_pool.emplace_back(nullptr, 0);
+    return edit_buffers(ctx->edit_pool[i].first, ctx->edit_pool[i].second, L, F, b);
 }
 
 // Copies the slot's descriptor to b.src and clears the selection counters.
@@ -1285,39 +1300,74 @@ WindowCopy copy_of(const HostWindow& h) {
     return w;
 }
 
-// Eigendecomposition of a symmetric n x n matrix on the device (k_sym_eigen): eigenvalues
-// ascending, eigenvectors as the columns of U (row-major).
-int device_eigen(vib_context* ctx, const std::vector<double>& M, int n, std::vector<double>& ev, std::vector<double>& U) {
-    int conv = 0;
-    const size_t bytes = 8 * (size_t)n * n;
-    std::vector<double> V((size_t)n * n), e(n);
-    // A | V | eigenvalues | rotation log | convergence flag in the context's grow-only eigen buffer
-    const size_t logd = vibk_sym_eigen_log(n);
-    char* buf = static_cast<char*>(grow_device(ctx->eig_buf, ctx->eig_cap, 2 * bytes + 8 * (size_t)n + 8 * logd + 256));
-    if (!buf) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "eigensolver buffer allocation failed");
-    double* dA = reinterpret_cast<double*>(buf);
-    double* dV = dA + (size_t)n * n;
-    double* de = dV + (size_t)n * n;
-    double* drot = de + n;
-    int* dok = reinterpret_cast<int*>(drot + logd);
-    const bool ok = hipMemcpyAsync(dA, M.data(), bytes, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
-                    vibk_sym_eigen(dA, dV, de, n, n, dok, drot, ctx->stream) == hipSuccess &&
-                    hipMemcpyAsync(V.data(), dV, bytes, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-                    hipMemcpyAsync(e.data(), de, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-                    hipMemcpyAsync(&conv, dok, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-                    hipStreamSynchronize(ctx->stream) == hipSuccess;
-    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "device eigendecomposition failed");
-    if (!conv) return fail(ctx, VIB_ERR_EIGEN_FAILED, "eigensolver did not converge");
-    std::vector<int> order(n);
-    for (int i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e[a] < e[b]; });
-    ev.resize(n);
-    U.resize((size_t)n * n);
-    for (int c = 0; c < n; ++c) {  // k_sym_eigen returns eigenvector c as row c of V
-        ev[c] = e[order[c]];
-        for (int r = 0; r < n; ++r) U[(size_t)r * n + c] = V[(size_t)order[c] * n + r];
+// Eigendecompositions of symmetric matrices on the device in one launch (k_sym_eigen, one
+// workgroup each): eigenvalues ascending, eigenvectors as the columns of U[j] (row-major).
+// status[j]: VIB_OK or VIB_ERR_EIGEN_FAILED (no convergence, or an eigenvalue not finite). The
+// return value is VIB_OK or the failure of the launch as a whole (allocation, device).
+int device_eigen_batch(vib_context* ctx, const std::vector<const std::vector<double>*>& M, const std::vector<int>& n,
+                       std::vector<std::vector<double>>& ev, std::vector<std::vector<double>>& U, std::vector<int>& status) {
+    const size_t J = M.size();
+    ev.assign(J, {});
+    U.assign(J, {});
+    status.assign(J, VIB_OK);
+    if (J == 0) return VIB_OK;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    std::vector<size_t> off(J);
+    size_t bytes = al(sizeof(EigenJob) * J);
+    int max_n = 0;
+    for (size_t j = 0; j < J; ++j) {
+        if (n[j] < 1 || n[j] > vibk_sym_eigen_max())
+            return fail(ctx, VIB_ERR_UNSUPPORTED_DIMENSIONS, "eigenproblem beyond the device eigensolver");
+        max_n = std::max(max_n, n[j]);
+        off[j] = bytes;
+        const size_t nn = (size_t)n[j] * n[j];
+        bytes += al(8 * (2 * nn + n[j] + vibk_sym_eigen_log(n[j])) + 8);
     }
-    for (double v : ev) if (!std::isfinite(v)) return fail(ctx, VIB_ERR_EIGEN_FAILED, "eigensolver did not converge");
+    // jobs | per job: A | V | eigenvalues | rotation log | convergence flag, in the context's
+    // grow-only eigen buffer
+    char* buf = static_cast<char*>(grow_device(ctx->eig_buf, ctx->eig_cap, bytes));
+    if (!buf) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "eigensolver buffer allocation failed");
+    std::vector<EigenJob> jobs(J);
+    std::vector<std::vector<double>> V(J), e(J);
+    std::vector<int32_t> conv(J, 0);
+    bool ok = true;
+    for (size_t j = 0; j < J; ++j) {
+        const size_t nn = (size_t)n[j] * n[j];
+        EigenJob& q = jobs[j];
+        q.A = reinterpret_cast<double*>(buf + off[j]);
+        q.V = q.A + nn;
+        q.ev = q.V + nn;
+        q.rot = q.ev + n[j];
+        q.ok = reinterpret_cast<int32_t*>(q.rot + vibk_sym_eigen_log(n[j]));
+        q.n = q.ld = n[j];
+        V[j].resize(nn);
+        e[j].resize(n[j]);
+        ok = ok && hipMemcpyAsync(q.A, M[j]->data(), 8 * nn, hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
+    }
+    const EigenJob* djobs = reinterpret_cast<const EigenJob*>(buf);
+    ok = ok && hipMemcpyAsync(buf, jobs.data(), sizeof(EigenJob) * J, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+         vibk_sym_eigen_batch(djobs, (int)J, max_n, ctx->stream) == hipSuccess;
+    for (size_t j = 0; j < J && ok; ++j)
+        ok = hipMemcpyAsync(V[j].data(), jobs[j].V, 8 * V[j].size(), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+             hipMemcpyAsync(e[j].data(), jobs[j].ev, 8 * (size_t)n[j], hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+             hipMemcpyAsync(&conv[j], jobs[j].ok, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, "device eigendecomposition failed");
+    for (size_t j = 0; j < J; ++j) {
+        const int m = n[j];
+        std::vector<int> order(m);
+        for (int i = 0; i < m; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e[j][a] < e[j][b]; });
+        ev[j].resize(m);
+        U[j].resize((size_t)m * m);
+        for (int c = 0; c < m; ++c) {  // k_sym_eigen returns eigenvector c as row c of V
+            ev[j][c] = e[j][order[c]];
+            for (int r = 0; r < m; ++r) U[j][(size_t)r * m + c] = V[j][(size_t)order[c] * m + r];
+        }
+        bool finite = conv[j] != 0;
+        for (double v : ev[j]) finite = finite && std::isfinite(v);
+        if (!finite) status[j] = VIB_ERR_EIGEN_FAILED;
+    }
     return VIB_OK;
 }
 
@@ -1329,177 +1379,288 @@ struct SchurPrior {
     std::vector<double> ref, A, r0;
 };
 
-// Schur elimination of slot `victim` (pose, speed/bias) together with the landmarks selected by
-// d_sel (device flags; all their factors) and the IMU edges (from, to, record) into a
-// square-root prior on the separator; the previous prior always takes part
-// (Marginalizer::compute, Marginalizer.cpp:284-474). frames / observed: the frames the selected
-// landmarks touch (anchor and observers) and whether any of them has an observation
-// (k_touch_frames). The marginal system is a device edit of the resident window into the
-// context's marginal slot; its factors linearize on the device, the landmarks are eliminated
-// there with the diagonal pseudo-inverse and the reduced system is assembled on the device; the
-// victim block's eigen pseudo-inverse and the prior's eigen square root use k_sym_eigen.
-int schur_prior(vib_context* ctx, Slot& src, const EditBufs& eb, const uint8_t* d_sel, int frames, bool observed,
-                int victim, const std::vector<int>& e_from, const std::vector<int>& e_to, const std::vector<double>& e_rec,
-                SchurPrior& out) {
-    const HostWindow& cur = src.host;
-    const int K = cur.K;
-    std::vector<char> tpose(K, 0), tsb(K, 0);
-    bool rig_used = observed, td_used = observed;
-    for (int k = 0; k < K; ++k) if ((frames >> k) & 1) tpose[k] = 1;
-    for (size_t e = 0; e < e_from.size(); ++e) {
-        tpose[e_from[e]] = tpose[e_to[e]] = 1;
-        tsb[e_from[e]] = tsb[e_to[e]] = 1;
-    }
-    for (size_t b = 0; b < cur.prior_kind.size(); ++b) {
-        int slot = -1;
-        for (int k = 0; k < K; ++k) if (cur.frame_id[k] == cur.prior_id[b]) slot = k;
-        switch (cur.prior_kind[b]) {
-            case VIB_KEY_POSE: if (slot >= 0) tpose[slot] = 1; break;
-            case VIB_KEY_SPEED_BIAS: if (slot >= 0) tsb[slot] = 1; break;
-            case VIB_KEY_RIG: rig_used = true; break;
-            default: td_used = true;
-        }
-    }
-    // the marginal system: consumed factors only, every touched block a variable
-    HostWindow mm = frame_meta(cur);
-    mm.flags = (rig_used ? VIB_RIG_FREE : 0u) | (td_used ? VIB_TD_FREE : 0u);
-    mm.td_lo = -1e300; mm.td_hi = 1e300;
-    mm.pose_free.assign(tpose.begin(), tpose.end());
-    mm.sb_width.assign(K, 0);
-    for (int k = 0; k < K; ++k) mm.sb_width[k] = tsb[k] ? 9 : 0;
-    mm.imu_from = e_from; mm.imu_to = e_to; mm.imu = e_rec;
-    mm.E = (int)e_from.size();
-    EditSpec sp{};
-    for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < K ? k : -1;
-    sp.lm_mode = 1;  // every eliminated inverse range is a variable
-    sp.keep_lm = d_sel;
-    if (!ctx->marginal_slot) ctx->marginal_slot = std::make_unique<Slot>();
-    Slot& tmp = *ctx->marginal_slot;  // its arena is kept from slide to slide
-    if (const int st = edit_window(ctx, src, sp, std::move(mm), tmp, eb, false)) return st;
-    const HostWindow& m = tmp.host;
-    std::string err;
-    const int nr = m.nr;
-    // eliminated state columns (victim pose and speed/bias) and retained columns in layout order
+// One marginalization of a batch (Marginalizer::compute, Marginalizer.cpp:284-474): slot
+// `victim` (pose, speed/bias) of the window of `src` with the landmarks of d_sel (device flags;
+// all their factors) and the IMU edges (from, to, record) eliminated into a square-root prior
+// on the separator; the previous prior always takes part. frames / observed: the frames the
+// selected landmarks touch and whether any of them has an observation (k_touch_frames).
+struct PriorJob {
+    Slot* src = nullptr;
+    const EditBufs* eb = nullptr;
+    const uint8_t* d_sel = nullptr;
+    int frames = 0;
+    bool observed = false;
+    int victim = 0;
+    std::vector<int> e_from, e_to;
+    std::vector<double> e_rec;
+    Slot* marg = nullptr;  // the marginal system's slot
+    int status = VIB_OK;
+    std::string error;
     std::vector<int> victim_cols, kept_cols;
-    std::vector<char> is_victim(nr, 0);
-    if (m.pose_col[victim] >= 0) for (int i = 0; i < 6; ++i) victim_cols.push_back(m.pose_col[victim] + i);
-    if (m.sb_col[victim] >= 0) for (int i = 0; i < 9; ++i) victim_cols.push_back(m.sb_col[victim] + i);
-    for (int c : victim_cols) is_victim[c] = 1;
-    for (int c = 0; c < nr; ++c) if (!is_victim[c]) kept_cols.push_back(c);
-    const int nvict = (int)victim_cols.size(), np = (int)kept_cols.size();
-    if (nvict == 0) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "no eliminated state block");
-    // device: linearize the consumed factors, landmark pseudo-inverse Schur, reduced system
+    std::vector<double> S, rhs, Hp, bp;
+    SchurPrior out;
+};
+
+void job_fail(vib_context* ctx, PriorJob& j, int code, const std::string& msg) {
+    j.status = code;
+    j.error = msg;
+    ctx->error = msg;
+}
+
+// The marginalizations of a batch of windows: each marginal system is a device edit of its
+// window (consumed factors only, every touched block a variable); they linearize, eliminate
+// their landmarks through the diagonal pseudo-inverse and assemble their reduced systems in
+// one batch of launches; the eliminated state blocks' eigen pseudo-inverses (cutoff
+// max(1e-12, 1e-9 lambda_max), Marginalizer::psdSolve, Marginalizer.cpp:240-283) run as one
+// eigen launch, and so do the priors' eigen square roots (A = sqrt(lambda) u^T, r0 =
+// u^T b / sqrt(lambda) above the same cutoff; indefinite below -10 cutoff,
+// Marginalizer.cpp:430-466). A job's failure is its status; the return value is a failure of
+// the batch as a whole.
+int schur_priors(vib_context* ctx, std::vector<PriorJob>& jobs) {
+    // 1. the marginal systems
+    for (PriorJob& j : jobs) {
+        const HostWindow& cur = j.src->host;
+        const int K = cur.K;
+        std::vector<char> tpose(K, 0), tsb(K, 0);
+        bool rig_used = j.observed, td_used = j.observed;
+        for (int k = 0; k < K; ++k) if ((j.frames >> k) & 1) tpose[k] = 1;
+        for (size_t e = 0; e < j.e_from.size(); ++e) {
+            tpose[j.e_from[e]] = tpose[j.e_to[e]] = 1;
+            tsb[j.e_from[e]] = tsb[j.e_to[e]] = 1;
+        }
+        for (size_t b = 0; b < cur.prior_kind.size(); ++b) {
+            int slot = -1;
+            for (int k = 0; k < K; ++k) if (cur.frame_id[k] == cur.prior_id[b]) slot = k;
+            switch (cur.prior_kind[b]) {
+                case VIB_KEY_POSE: if (slot >= 0) tpose[slot] = 1; break;
+                case VIB_KEY_SPEED_BIAS: if (slot >= 0) tsb[slot] = 1; break;
+                case VIB_KEY_RIG: rig_used = true; break;
+                default: td_used = true;
+            }
+        }
+        HostWindow mm = frame_meta(cur);
+        mm.flags = (rig_used ? VIB_RIG_FREE : 0u) | (td_used ? VIB_TD_FREE : 0u);
+        mm.td_lo = -1e300; mm.td_hi = 1e300;
+        mm.pose_free.assign(tpose.begin(), tpose.end());
+        mm.sb_width.assign(K, 0);
+        for (int k = 0; k < K; ++k) mm.sb_width[k] = tsb[k] ? 9 : 0;
+        mm.imu_from = j.e_from; mm.imu_to = j.e_to; mm.imu = j.e_rec;
+        mm.E = (int)j.e_from.size();
+        EditSpec sp{};
+        for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < K ? k : -1;
+        sp.lm_mode = 1;  // every eliminated inverse range is a variable
+        sp.keep_lm = j.d_sel;
+        if (const int st = edit_window(ctx, *j.src, sp, std::move(mm), *j.marg, *j.eb, false)) {
+            job_fail(ctx, j, st, ctx->error);
+            continue;
+        }
+        const HostWindow& m = j.marg->host;
+        std::vector<char> is_victim(m.nr, 0);
+        if (m.pose_col[j.victim] >= 0) for (int i = 0; i < 6; ++i) j.victim_cols.push_back(m.pose_col[j.victim] + i);
+        if (m.sb_col[j.victim] >= 0) for (int i = 0; i < 9; ++i) j.victim_cols.push_back(m.sb_col[j.victim] + i);
+        for (int c : j.victim_cols) is_victim[c] = 1;
+        for (int c = 0; c < m.nr; ++c) if (!is_victim[c]) j.kept_cols.push_back(c);
+        if (j.victim_cols.empty()) job_fail(ctx, j, VIB_ERR_INVALID_ARGUMENT, "no eliminated state block");
+    }
+    // 2. linearization, landmark elimination and reduced systems, batched
+    std::vector<PriorJob*> live;
+    for (PriorJob& j : jobs) if (j.status == VIB_OK) live.push_back(&j);
+    if (live.empty()) return VIB_OK;
     LmState lm{};
     lm.lambda = 0; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1; lm.evaluate_only = 1;
     lm.marginal = 1;
-    std::vector<Slot*> one{&tmp};
+    std::vector<Slot*> slots;
+    for (PriorJob* j : live) {
+        if (!write_lm(ctx, *j->marg, lm)) return fail(ctx, VIB_ERR_DEVICE, "marginal LM state upload failed");
+        slots.push_back(j->marg);
+    }
     Batch& b = ctx->one;
     b.mixed = 0;
-    if (!write_lm(ctx, tmp, lm) || !b.build(one, ctx->stream, err) || !b.linearize(ctx->stream))
+    std::string err;
+    if (!b.build(slots, ctx->stream, err) || !b.linearize(ctx->stream))
         return fail(ctx, VIB_ERR_DEVICE, "marginal linearization failed");
-    std::vector<double> dvals(m.L);
-    if ((m.L && hipMemcpyAsync(dvals.data(), tmp.dev.lm_out + (size_t)LO_D * m.L, 8 * (size_t)m.L, hipMemcpyDeviceToHost,
-                               ctx->stream) != hipSuccess) ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
-        return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
-    double dmax = 0;
-    for (double d : dvals) dmax = std::max(dmax, d);
-    lm.marg_cut = std::max(1e-12, 1e-9 * dmax);
-    std::vector<double> S((size_t)nr * nr), rhs(nr);
-    if (!write_lm(ctx, tmp, lm) || !b.launch(3, ctx->stream, 0, 0) || !b.launch(4, ctx->stream, 0, 0) ||
-        !b.launch(5, ctx->stream, 0, 0) ||
-        hipMemcpyAsync(S.data(), tmp.dev.Scopy, 8 * S.size(), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(rhs.data(), tmp.dev.rhs, 8 * (size_t)nr, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
-        return fail(ctx, VIB_ERR_DEVICE, "marginal Schur failed");
-    {
+    std::vector<std::vector<double>> dvals(live.size());
+    bool ok = true;
+    for (size_t i = 0; i < live.size() && ok; ++i) {
+        const Slot& t = *live[i]->marg;
+        dvals[i].resize(t.host.L);
+        ok = t.host.L == 0 || hipMemcpyAsync(dvals[i].data(), t.dev.lm_out + (size_t)LO_D * t.host.L, 8 * (size_t)t.host.L,
+                                             hipMemcpyDeviceToHost, ctx->stream) == hipSuccess;
+    }
+    if (!ok || hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
+    for (size_t i = 0; i < live.size(); ++i) {
+        double dmax = 0;
+        for (double d : dvals[i]) dmax = std::max(dmax, d);
+        LmState l2 = lm;
+        l2.marg_cut = std::max(1e-12, 1e-9 * dmax);
+        if (!write_lm(ctx, *live[i]->marg, l2)) return fail(ctx, VIB_ERR_DEVICE, "marginal LM state upload failed");
+    }
+    ok = b.launch(3, ctx->stream, 0, 0) && b.launch(4, ctx->stream, 0, 0) && b.launch(5, ctx->stream, 0, 0);
+    for (PriorJob* j : live) {
+        const Slot& t = *j->marg;
+        const size_t nr = t.host.nr;
+        j->S.resize(nr * nr);
+        j->rhs.resize(nr);
+        ok = ok && hipMemcpyAsync(j->S.data(), t.dev.Scopy, 8 * nr * nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+             hipMemcpyAsync(j->rhs.data(), t.dev.rhs, 8 * nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess;
+    }
+    if (!ok || hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, VIB_ERR_DEVICE, "marginal Schur failed");
+    for (PriorJob* j : live) {
         LmState chk;
-        if (!read_lm(ctx, tmp, chk)) return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
-        if (!chk.lin_valid) return fail(ctx, VIB_ERR_INVALID_RESIDUAL, "invalid residual in a consumed factor");
+        if (!read_lm(ctx, *j->marg, chk)) return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
+        if (!chk.lin_valid) job_fail(ctx, *j, VIB_ERR_INVALID_RESIDUAL, "invalid residual in a consumed factor");
     }
-    // eliminated state block: eigen pseudo-inverse with cutoff max(1e-12, 1e-9 lambda_max)
-    // (Marginalizer::psdSolve, Marginalizer.cpp:240-283), H_prior = S_rr - S_rm S_mm^+ S_mr
-    std::vector<double> Smm((size_t)nvict * nvict), ev, U;
-    for (int i = 0; i < nvict; ++i)
-        for (int j = 0; j < nvict; ++j)
-            Smm[(size_t)i * nvict + j] = 0.5 * (S[(size_t)victim_cols[i] * nr + victim_cols[j]] +
-                                                 S[(size_t)victim_cols[j] * nr + victim_cols[i]]);
-    {
-        const int st = device_eigen(ctx, Smm, nvict, ev, U);
-        if (st != VIB_OK) return st;
+    // 3. the eliminated state blocks' eigen pseudo-inverses; H_prior = S_rr - S_rm S_mm^+ S_mr
+    std::vector<PriorJob*> stage;
+    std::vector<std::vector<double>> mats;
+    for (PriorJob* j : live) {
+        if (j->status != VIB_OK) continue;
+        const int nv = (int)j->victim_cols.size(), nr = j->marg->host.nr;
+        std::vector<double> Smm((size_t)nv * nv);
+        for (int a = 0; a < nv; ++a)
+            for (int c = 0; c < nv; ++c)
+                Smm[(size_t)a * nv + c] = 0.5 * (j->S[(size_t)j->victim_cols[a] * nr + j->victim_cols[c]] +
+                                                  j->S[(size_t)j->victim_cols[c] * nr + j->victim_cols[a]]);
+        stage.push_back(j);
+        mats.push_back(std::move(Smm));
     }
-    const double cut_v = std::max(1e-12, 1e-9 * std::max(0.0, ev[nvict - 1]));
-    if (ev[0] < -10.0 * cut_v) return fail(ctx, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
-    std::vector<double> Z, zb;  // U_k^T S_mr / sqrt(lambda_k), U_k^T b_m / sqrt(lambda_k)
-    for (int q = 0; q < nvict; ++q) {
-        if (!(ev[q] > cut_v)) continue;
-        const double is = 1.0 / std::sqrt(ev[q]);
-        double bz = 0;
-        for (int i = 0; i < nvict; ++i) bz += U[(size_t)i * nvict + q] * -rhs[victim_cols[i]];
-        zb.push_back(bz * is);
-        for (int c = 0; c < np; ++c) {
-            double v = 0;
-            for (int i = 0; i < nvict; ++i) v += U[(size_t)i * nvict + q] * S[(size_t)victim_cols[i] * nr + kept_cols[c]];
-            Z.push_back(v * is);
+    std::vector<const std::vector<double>*> mp;
+    std::vector<int> ns;
+    for (size_t i = 0; i < stage.size(); ++i) {
+        mp.push_back(&mats[i]);
+        ns.push_back((int)stage[i]->victim_cols.size());
+    }
+    std::vector<std::vector<double>> evs, Us;
+    std::vector<int> est;
+    if (const int st = device_eigen_batch(ctx, mp, ns, evs, Us, est)) return st;
+    for (size_t i = 0; i < stage.size(); ++i) {
+        PriorJob& j = *stage[i];
+        if (est[i] != VIB_OK) {
+            job_fail(ctx, j, est[i], "eigensolver did not converge");
+            continue;
+        }
+        const int nv = ns[i], np = (int)j.kept_cols.size(), nr = j.marg->host.nr;
+        const std::vector<double>& ev = evs[i];
+        const std::vector<double>& U = Us[i];
+        const double cut_v = std::max(1e-12, 1e-9 * std::max(0.0, ev[nv - 1]));
+        if (ev[0] < -10.0 * cut_v) {
+            job_fail(ctx, j, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
+            continue;
+        }
+        std::vector<double> Z, zb;  // U_k^T S_mr / sqrt(lambda_k), U_k^T b_m / sqrt(lambda_k)
+        for (int q = 0; q < nv; ++q) {
+            if (!(ev[q] > cut_v)) continue;
+            const double is = 1.0 / std::sqrt(ev[q]);
+            double bz = 0;
+            for (int a = 0; a < nv; ++a) bz += U[(size_t)a * nv + q] * -j.rhs[j.victim_cols[a]];
+            zb.push_back(bz * is);
+            for (int c = 0; c < np; ++c) {
+                double v = 0;
+                for (int a = 0; a < nv; ++a) v += U[(size_t)a * nv + q] * j.S[(size_t)j.victim_cols[a] * nr + j.kept_cols[c]];
+                Z.push_back(v * is);
+            }
+        }
+        const int nk = (int)zb.size();
+        j.Hp.assign((size_t)np * np, 0);
+        j.bp.assign(np, 0);
+        for (int a = 0; a < np; ++a) {
+            double g = -j.rhs[j.kept_cols[a]];
+            for (int q = 0; q < nk; ++q) g -= Z[(size_t)q * np + a] * zb[q];
+            j.bp[a] = g;
+            for (int c = 0; c <= a; ++c) {
+                double h = 0.5 * (j.S[(size_t)j.kept_cols[a] * nr + j.kept_cols[c]] +
+                                  j.S[(size_t)j.kept_cols[c] * nr + j.kept_cols[a]]);
+                for (int q = 0; q < nk; ++q) h -= Z[(size_t)q * np + a] * Z[(size_t)q * np + c];
+                j.Hp[(size_t)a * np + c] = j.Hp[(size_t)c * np + a] = h;
+            }
         }
     }
-    const int nk = (int)zb.size();
-    std::vector<double> Hp((size_t)np * np), bp(np);
-    for (int i = 0; i < np; ++i) {
-        double g = -rhs[kept_cols[i]];
-        for (int q = 0; q < nk; ++q) g -= Z[(size_t)q * np + i] * zb[q];
-        bp[i] = g;
-        for (int j = 0; j <= i; ++j) {
-            double h = 0.5 * (S[(size_t)kept_cols[i] * nr + kept_cols[j]] + S[(size_t)kept_cols[j] * nr + kept_cols[i]]);
-            for (int q = 0; q < nk; ++q) h -= Z[(size_t)q * np + i] * Z[(size_t)q * np + j];
-            Hp[(size_t)i * np + j] = Hp[(size_t)j * np + i] = h;
+    // 4. the priors' eigen square roots
+    stage.clear();
+    mp.clear();
+    ns.clear();
+    for (PriorJob* j : live)
+        if (j->status == VIB_OK) {
+            stage.push_back(j);
+            mp.push_back(&j->Hp);
+            ns.push_back((int)j->kept_cols.size());
         }
-    }
-    // square root of H_prior: A = sqrt(lambda) u^T, r0 = u^T b / sqrt(lambda) for the eigenvalues
-    // above max(1e-12, 1e-9 lambda_max); indefinite below -10 cutoff (Marginalizer.cpp:430-466)
-    std::vector<double> pev, PU;
-    {
-        const int st = device_eigen(ctx, Hp, np, pev, PU);
-        if (st != VIB_OK) return st;
-    }
-    const double cut = std::max(1e-12, 1e-9 * std::max(0.0, pev[np - 1]));
-    if (pev[0] < -10.0 * cut) return fail(ctx, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
-    out.cols = np;
-    out.rows = 0;
-    for (int q = 0; q < np; ++q) {
-        if (!(pev[q] > cut)) continue;
-        const double root = std::sqrt(pev[q]);
-        double ub = 0;
-        for (int k = 0; k < np; ++k) {
-            out.A.push_back(root * PU[(size_t)k * np + q]);
-            ub += PU[(size_t)k * np + q] * bp[k];
+    if (const int st = device_eigen_batch(ctx, mp, ns, evs, Us, est)) return st;
+    for (size_t i = 0; i < stage.size(); ++i) {
+        PriorJob& j = *stage[i];
+        if (est[i] != VIB_OK) {
+            job_fail(ctx, j, est[i], "eigensolver did not converge");
+            continue;
         }
-        out.r0.push_back(ub / root);
-        ++out.rows;
-    }
-    // prior keys in column order: poses, rig, td, speed/bias (the victim excluded)
-    for (int k = 0; k < K; ++k)
-        if (k != victim && m.pose_col[k] >= 0) {
-            out.kind.push_back(VIB_KEY_POSE); out.id.push_back(cur.frame_id[k]);
-            out.ref.insert(out.ref.end(), cur.pose.begin() + 7 * k, cur.pose.begin() + 7 * k + 7);
+        const int np = ns[i];
+        const std::vector<double>& pev = evs[i];
+        const std::vector<double>& PU = Us[i];
+        const double cut = std::max(1e-12, 1e-9 * std::max(0.0, pev[np - 1]));
+        if (pev[0] < -10.0 * cut) {
+            job_fail(ctx, j, VIB_ERR_INDEFINITE, "Schur prior is indefinite");
+            continue;
         }
-    if (m.rig_col >= 0) {
-        out.kind.push_back(VIB_KEY_RIG); out.id.push_back(0);
-        out.ref.insert(out.ref.end(), cur.rig, cur.rig + 7);
-    }
-    if (m.td_col >= 0) {
-        out.kind.push_back(VIB_KEY_TD); out.id.push_back(0);
-        out.ref.push_back(cur.td);
-    }
-    for (int k = 0; k < K; ++k)
-        if (k != victim && m.sb_col[k] >= 0) {
-            out.kind.push_back(VIB_KEY_SPEED_BIAS); out.id.push_back(cur.frame_id[k]);
-            out.ref.insert(out.ref.end(), cur.sb.begin() + 9 * k, cur.sb.begin() + 9 * k + 9);
+        SchurPrior& out = j.out;
+        out.cols = np;
+        out.rows = 0;
+        for (int q = 0; q < np; ++q) {
+            if (!(pev[q] > cut)) continue;
+            const double root = std::sqrt(pev[q]);
+            double ub = 0;
+            for (int k = 0; k < np; ++k) {
+                out.A.push_back(root * PU[(size_t)k * np + q]);
+                ub += PU[(size_t)k * np + q] * j.bp[k];
+            }
+            out.r0.push_back(ub / root);
+            ++out.rows;
         }
-    out.eliminated = nvict + m.L;
-    out.landmarks = m.L;
-    out.consumed = m.F + m.E + (cur.prior_rows ? 1 : 0);
-    out.cutoff = cut;
+        // prior keys in column order: poses, rig, td, speed/bias (the victim excluded)
+        const HostWindow& cur = j.src->host;
+        const HostWindow& m = j.marg->host;
+        for (int k = 0; k < cur.K; ++k)
+            if (k != j.victim && m.pose_col[k] >= 0) {
+                out.kind.push_back(VIB_KEY_POSE); out.id.push_back(cur.frame_id[k]);
+                out.ref.insert(out.ref.end(), cur.pose.begin() + 7 * k, cur.pose.begin() + 7 * k + 7);
+            }
+        if (m.rig_col >= 0) {
+            out.kind.push_back(VIB_KEY_RIG); out.id.push_back(0);
+            out.ref.insert(out.ref.end(), cur.rig, cur.rig + 7);
+        }
+        if (m.td_col >= 0) {
+            out.kind.push_back(VIB_KEY_TD); out.id.push_back(0);
+            out.ref.push_back(cur.td);
+        }
+        for (int k = 0; k < cur.K; ++k)
+            if (k != j.victim && m.sb_col[k] >= 0) {
+                out.kind.push_back(VIB_KEY_SPEED_BIAS); out.id.push_back(cur.frame_id[k]);
+                out.ref.insert(out.ref.end(), cur.sb.begin() + 9 * k, cur.sb.begin() + 9 * k + 9);
+            }
+        out.eliminated = (int)j.victim_cols.size() + m.L;
+        out.landmarks = m.L;
+        out.consumed = m.F + m.E + (cur.prior_rows ? 1 : 0);
+        out.cutoff = cut;
+    }
+    return VIB_OK;
+}
+
+// The marginal slot of job i (grow-only pool of the context).
+Slot& marginal_slot(vib_context* ctx, size_t i) {
+    while (ctx->marginal_pool.size() <= i) ctx->marginal_pool.push_back(std::make_unique<Slot>());
+    return *ctx->marginal_pool[i];
+}
+
+// One marginalization (a batch of one).
+int schur_prior(vib_context* ctx, Slot& src, const EditBufs& eb, const uint8_t* d_sel, int frames, bool observed,
+                int victim, const std::vector<int>& e_from, const std::vector<int>& e_to, const std::vector<double>& e_rec,
+                SchurPrior& out) {
+    std::vector<PriorJob> jobs(1);
+    PriorJob& j = jobs[0];
+    j.src = &src; j.eb = &eb; j.d_sel = d_sel; j.frames = frames; j.observed = observed; j.victim = victim;
+    j.e_from = e_from; j.e_to = e_to; j.e_rec = e_rec;
+    j.marg = &marginal_slot(ctx, 0);
+    if (const int st = schur_priors(ctx, jobs)) return st;
+    if (j.status != VIB_OK) return fail(ctx, j.status, j.error);
+    out = std::move(j.out);
     return VIB_OK;
 }
 
@@ -1525,57 +1686,6 @@ bool touched_frames(vib_context* ctx, const EditBufs& b, int L, const uint8_t* d
     frames = out[0];
     observed = out[1] != 0;
     return ok;
-}
-
-// OLD: slot 0 with every landmark anchored in or observed by it and its IMU edges. The
-// eliminated set and the frames it touches are found on the device; the marginal system and
-// the slid window are device edits of the resident window.
-int marginalize_old(vib_context* ctx, Slot& s, vib_marginal_report* rep) {
-    if (!refresh_state(ctx, s)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");
-    const HostWindow& cur = s.host;
-    const int victim = 0;
-    if (cur.K < 2) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "marginalization needs at least two frames");
-    EditBufs b;
-    if (!edit_buffers(ctx, cur.L, cur.F, b)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "slide buffer allocation failed");
-    int frames = 0;
-    bool observed = false;
-    if (!edit_begin(ctx, s, b) ||
-        (cur.L && vibk_elim_set(b.src, cur.L, victim, b.marg_lm, ctx->stream) != hipSuccess) ||
-        !touched_frames(ctx, b, cur.L, b.marg_lm, frames, observed))
-        return fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
-    std::vector<int> e_from, e_to;
-    std::vector<double> e_rec;
-    for (int e = 0; e < cur.E; ++e)
-        if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) {
-            e_from.push_back(cur.imu_from[e]);
-            e_to.push_back(cur.imu_to[e]);
-            e_rec.insert(e_rec.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e,
-                         cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
-        }
-    SchurPrior p;
-    int st = schur_prior(ctx, s, b, b.marg_lm, frames, observed, victim, e_from, e_to, e_rec, p);
-    if (st != VIB_OK) return st;
-    // slide: drop slot 0, its landmarks, factors and IMU edges
-    HostWindow next = frame_meta(cur);
-    drop_frame(next, victim);
-    keep_edges(next, cur.imu_from, cur.imu_to, cur.imu, std::vector<char>(cur.E, 1), victim);
-    set_prior(next, p);
-    EditSpec sp{};
-    for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < cur.K ? (k == victim ? -1 : (k > victim ? k - 1 : k)) : -1;
-    sp.keep_lm = b.marg_lm;
-    sp.keep_lm_invert = 1;
-    st = edit_window(ctx, s, sp, std::move(next), s, b, true);
-    if (st != VIB_OK) return st;
-    rep->status = VIB_OK;
-    rep->prior_reused = 0;
-    rep->eliminated_dimension = p.eliminated;
-    rep->retained_dimension = p.cols;
-    rep->prior_rank = p.rows;
-    rep->eliminated_landmarks = p.landmarks;
-    rep->consumed_factors = p.consumed;
-    rep->eigen_cutoff = p.cutoff;
-    rep->imu_mode = e_from.empty() ? VIB_IMU_MERGE_NONE : VIB_IMU_MERGE_BRIDGE;
-    return VIB_OK;
 }
 
 // Linear interpolation of the gyroscope at t (gyroAt, BodyImuFactor.cpp:29-38).
@@ -2810,6 +2920,99 @@ static int strip_rejected(vib_context* ctx, Slot& s) {
     return edit_window(ctx, s, sp, frame_meta(h), s, b, true);
 }
 
+// OLD slides of a batch of windows: slot 0 of each with every landmark anchored in or observed
+// by it and its IMU edges (Marginalizer::compute with the elimination set of
+// SlidingWindowManager.cpp:627-652). Per window: rejected observations stripped, the eliminated
+// set and the frames it touches found on the device; then the marginalizations of all windows
+// as one batch (schur_priors); then each window slides (a device edit in place). reps[i]
+// receives window i's result; the return value is the first failure, if any.
+int marginalize_old_batch(vib_context* ctx, const std::vector<Slot*>& slots, vib_marginal_report* reps) {
+    const int victim = 0;
+    const size_t n = slots.size();
+    std::vector<EditBufs> bufs(n);
+    std::vector<PriorJob> jobs;
+    std::vector<size_t> job_of;  // window of each job
+    int first = VIB_OK;
+    auto note = [&](size_t i, int st) {
+        reps[i].status = st;
+        if (first == VIB_OK) first = st;
+    };
+    for (size_t i = 0; i < n; ++i) {
+        reps[i] = vib_marginal_report{};
+        Slot& s = *slots[i];
+        int st = strip_rejected(ctx, s);
+        if (st == VIB_OK && !refresh_state(ctx, s)) st = fail(ctx, VIB_ERR_DEVICE, "state download failed");
+        if (st == VIB_OK && s.host.K < 2) st = fail(ctx, VIB_ERR_INVALID_ARGUMENT, "marginalization needs at least two frames");
+        if (st == VIB_OK && !job_edit_buffers(ctx, i, s.host.L, s.host.F, bufs[i]))
+            st = fail(ctx, VIB_ERR_OUT_OF_MEMORY, "slide buffer allocation failed");
+        const HostWindow& cur = s.host;
+        PriorJob j;
+        if (st == VIB_OK &&
+            (!edit_begin(ctx, s, bufs[i]) ||
+             (cur.L && vibk_elim_set(bufs[i].src, cur.L, victim, bufs[i].marg_lm, ctx->stream) != hipSuccess) ||
+             !touched_frames(ctx, bufs[i], cur.L, bufs[i].marg_lm, j.frames, j.observed)))
+            st = fail(ctx, VIB_ERR_DEVICE, "eliminated-set launch failed");
+        if (st != VIB_OK) {
+            note(i, st);
+            continue;
+        }
+        j.src = &s;
+        j.eb = &bufs[i];
+        j.d_sel = bufs[i].marg_lm;
+        j.victim = victim;
+        for (int e = 0; e < cur.E; ++e)
+            if (cur.imu_from[e] == victim || cur.imu_to[e] == victim) {
+                j.e_from.push_back(cur.imu_from[e]);
+                j.e_to.push_back(cur.imu_to[e]);
+                j.e_rec.insert(j.e_rec.end(), cur.imu.begin() + (size_t)VIB_IMU_STRIDE * e,
+                               cur.imu.begin() + (size_t)VIB_IMU_STRIDE * (e + 1));
+            }
+        j.marg = &marginal_slot(ctx, jobs.size());
+        jobs.push_back(std::move(j));
+        job_of.push_back(i);
+    }
+    if (const int st = schur_priors(ctx, jobs)) {
+        for (size_t q = 0; q < jobs.size(); ++q) note(job_of[q], st);
+        return first;
+    }
+    for (size_t q = 0; q < jobs.size(); ++q) {
+        PriorJob& j = jobs[q];
+        const size_t i = job_of[q];
+        if (j.status != VIB_OK) {
+            ctx->error = j.error;
+            note(i, j.status);
+            continue;
+        }
+        // slide: drop slot 0, its landmarks, factors and IMU edges
+        Slot& s = *slots[i];
+        const HostWindow& cur = s.host;
+        HostWindow next = frame_meta(cur);
+        drop_frame(next, victim);
+        keep_edges(next, cur.imu_from, cur.imu_to, cur.imu, std::vector<char>(cur.E, 1), victim);
+        set_prior(next, j.out);
+        EditSpec sp{};
+        for (int k = 0; k < kMaxFrames; ++k) sp.renum[k] = k < cur.K ? (k == victim ? -1 : (k > victim ? k - 1 : k)) : -1;
+        sp.keep_lm = bufs[i].marg_lm;
+        sp.keep_lm_invert = 1;
+        const bool bridged = !j.e_from.empty();
+        if (const int st = edit_window(ctx, s, sp, std::move(next), s, bufs[i], true)) {
+            note(i, st);
+            continue;
+        }
+        vib_marginal_report& rep = reps[i];
+        rep.status = VIB_OK;
+        rep.prior_reused = 0;
+        rep.eliminated_dimension = j.out.eliminated;
+        rep.retained_dimension = j.out.cols;
+        rep.prior_rank = j.out.rows;
+        rep.eliminated_landmarks = j.out.landmarks;
+        rep.consumed_factors = j.out.consumed;
+        rep.eigen_cutoff = j.out.cutoff;
+        rep.imu_mode = bridged ? VIB_IMU_MERGE_BRIDGE : VIB_IMU_MERGE_NONE;
+    }
+    return first;
+}
+
 }  // namespace
 
 int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double* samples, int32_t count,
@@ -2820,16 +3023,38 @@ int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double
     *rep = vib_marginal_report{};
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
-    int st = policy == VIB_MARGIN_OLD || policy == VIB_MARGIN_SECOND_NEW ? strip_rejected(ctx, *s) : VIB_OK;
-    if (st != VIB_OK) {
-        rep->status = st;
-        return st;
-    }
-    if (policy == VIB_MARGIN_OLD) st = marginalize_old(ctx, *s, rep);
-    else if (policy == VIB_MARGIN_SECOND_NEW) st = marginalize_second_new(ctx, *s, samples, count, t_start, t_end, params, rep);
-    else return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown marginalization policy");
+    if (policy == VIB_MARGIN_OLD) return marginalize_old_batch(ctx, {s}, rep);
+    if (policy != VIB_MARGIN_SECOND_NEW) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown marginalization policy");
+    int st = strip_rejected(ctx, *s);
+    if (st == VIB_OK) st = marginalize_second_new(ctx, *s, samples, count, t_start, t_end, params, rep);
     rep->status = st;
     return st;
+}
+
+int vib_marginalize_batch(vib_context* ctx, int32_t count, const int32_t* ids, int32_t policy,
+                          vib_marginal_report* reps) {
+    if (!ctx || !ids || !reps || count <= 0) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    std::vector<Slot*> slots;
+    for (int i = 0; i < count; ++i) {
+        reps[i] = vib_marginal_report{};
+        Slot* s = find_slot(ctx, ids[i]);
+        if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+        for (Slot* o : slots)
+            if (o == s) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "a slot is listed twice");
+        slots.push_back(s);
+    }
+    if (policy == VIB_MARGIN_OLD) return marginalize_old_batch(ctx, slots, reps);
+    if (policy != VIB_MARGIN_SECOND_NEW) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown marginalization policy");
+    int first = VIB_OK;
+    for (int i = 0; i < count; ++i) {  // without IMU samples: the slot's edges are bridged
+        int st = strip_rejected(ctx, *slots[i]);
+        if (st == VIB_OK) st = marginalize_second_new(ctx, *slots[i], nullptr, 0, 0.0, 0.0, nullptr, &reps[i]);
+        reps[i].status = st;
+        if (first == VIB_OK) first = st;
+    }
+    return first;
 }
 
 int vib_read_window(vib_context* ctx, int32_t slot, vib_window* out) {

@@ -2185,8 +2185,9 @@ __global__ __launch_bounds__(kCholThreads) void k_refine(WinDev* wins, int last)
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_sym_eigen: A = Q diag(ev) Q^T for one symmetric n x n matrix (row-major, leading dimension
-// ld; A is overwritten), one workgroup. Q is returned transposed in V (row c = eigenvector c),
+// k_sym_eigen: A = Q diag(ev) Q^T for a batch of symmetric matrices, one workgroup per job
+// (EigenJob: n x n, row-major, leading dimension ld; A is overwritten; the slides of many sessions
+// share one launch). Q is returned transposed in V (row c = eigenvector c),
 // ev unsorted. Three phases (Golub & Van Loan, Householder tridiagonalization and the symmetric
 // QR step with implicit Wilkinson shift):
 //   1. n-2 Householder reflections H_k = I - 2 v v^T reduce A to tridiagonal (d, e); each step
@@ -2422,23 +2423,23 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
     if (t == 0) *ok = state;
 }
 
-__global__ __launch_bounds__(kEigThreads) void k_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok,
-                                                           double* rot) {
+__global__ __launch_bounds__(kEigThreads) void k_sym_eigen(const EigenJob* jobs) {
     __shared__ double d[kEigMax], e[kEigMax], q[kEigMax];
     __shared__ int lo, hi, state;
-    sym_eigen_body<0>(A, V, ev, n, ld, ok, rot, nullptr, d, e, q, lo, hi, state);
+    const EigenJob j = jobs[blockIdx.x];
+    sym_eigen_body<0>(j.A, j.V, j.ev, j.n, j.ld, j.ok, j.rot, nullptr, d, e, q, lo, hi, state);
 }
 
 // Matrices of up to kEigLds rows (every prior of a window of at most 20 frames: 19 poses, one
 // speed/bias block, rig and time offset give 130) in LDS: 148 KB for the matrix, 3.3 KB of
 // vectors.
 constexpr int kEigLds = 136;
-__global__ __launch_bounds__(kEigThreads) void k_sym_eigen_lds(double* A, double* V, double* ev, int n, int ld, int* ok,
-                                                               double* rot) {
+__global__ __launch_bounds__(kEigThreads) void k_sym_eigen_lds(const EigenJob* jobs) {
     __shared__ double lds[kEigLds * kEigLds];
     __shared__ double d[kEigLds], e[kEigLds], q[kEigLds];
     __shared__ int lo, hi, state;
-    sym_eigen_body<kEigLds>(A, V, ev, n, ld, ok, rot, lds, d, e, q, lo, hi, state);
+    const EigenJob j = jobs[blockIdx.x];
+    sym_eigen_body<kEigLds>(j.A, j.V, j.ev, j.n, j.ld, j.ok, j.rot, lds, d, e, q, lo, hi, state);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3582,11 +3583,14 @@ hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t
 
 size_t vibk_sym_eigen_log(int n) { return vib::eigen_log_doubles(n); }
 
-hipError_t vibk_sym_eigen(double* A, double* V, double* ev, int n, int ld, int* ok, double* rot, hipStream_t s) {
-    if (n < 1 || n > vib::kEigMax || ld < n || !rot) return hipErrorInvalidValue;
-    if (n <= vib::kEigLds)
-        hipLaunchKernelGGL(vib::k_sym_eigen_lds, dim3(1), dim3(vib::kEigThreads), 0, s, A, V, ev, n, ld, ok, rot);
-    else hipLaunchKernelGGL(vib::k_sym_eigen, dim3(1), dim3(vib::kEigThreads), 0, s, A, V, ev, n, ld, ok, rot);
+int vibk_sym_eigen_max(void) { return vib::kEigMax; }
+
+// count jobs (device array), every n <= max_n <= kEigMax (checked by the caller on the host).
+hipError_t vibk_sym_eigen_batch(const vib::EigenJob* jobs, int count, int max_n, hipStream_t s) {
+    if (count < 1 || max_n < 1 || max_n > vib::kEigMax) return hipErrorInvalidValue;
+    if (max_n <= vib::kEigLds)
+        hipLaunchKernelGGL(vib::k_sym_eigen_lds, dim3(count), dim3(vib::kEigThreads), 0, s, jobs);
+    else hipLaunchKernelGGL(vib::k_sym_eigen, dim3(count), dim3(vib::kEigThreads), 0, s, jobs);
     return hipGetLastError();
 }
 

@@ -403,23 +403,33 @@ int vib_server_turn(vib_server* s, vib_served* finished, int32_t capacity, int32
         s->stats.iterate_seconds += since(t_iter);
         s->stats.launches++;
         s->stats.window_iterations += (int64_t)ids.size();
+        // 3. the finished requests' slides back to their windows' sizes (FixedLagBackend.cpp:434-451),
+        //    all in one batch
+        std::vector<size_t> fin, slide;
+        std::vector<int> status(run.size(), VIB_OK);
         for (size_t i = 0; i < run.size(); ++i) {
-            Session& se = *run[i];
             if (st == VIB_OK && !dn[i]) continue;
-            int status = st != VIB_OK ? st : reps[i].status;
-            // 3. the slide back to the session's window size (FixedLagBackend.cpp:434-451)
+            fin.push_back(i);
+            status[i] = st != VIB_OK ? st : reps[i].status;
             int32_t K = 0, L, F, E, rows, cols;
-            if (status == VIB_OK && s->opt.slide_policy >= 0 &&
-                vib_window_shape(s->ctx, se.slot, &K, &L, &F, &E, &rows, &cols) == VIB_OK && K > se.frames) {
-                vib_marginal_report mr{};
-                const auto t_slide = Clock::now();
-                status = vib_marginalize(s->ctx, se.slot, s->opt.slide_policy, nullptr, 0, 0.0, 0.0, nullptr, &mr);
-                s->stats.slide_seconds += since(t_slide);
-                s->stats.slides++;
-            }
-            finish(s, se, std::move(se.running), status, st == VIB_OK ? reps[i].iterations : 0,
-                   st == VIB_OK ? reps[i].final_cost : 0.0);
+            if (status[i] == VIB_OK && s->opt.slide_policy >= 0 &&
+                vib_window_shape(s->ctx, run[i]->slot, &K, &L, &F, &E, &rows, &cols) == VIB_OK && K > run[i]->frames)
+                slide.push_back(i);
         }
+        if (!slide.empty()) {
+            std::vector<int32_t> sl;
+            for (size_t i : slide) sl.push_back(run[i]->slot);
+            std::vector<vib_marginal_report> mr(slide.size());
+            const auto t_slide = Clock::now();
+            (void)vib_marginalize_batch(s->ctx, (int32_t)sl.size(), sl.data(), s->opt.slide_policy, mr.data());
+            s->stats.slide_seconds += since(t_slide);
+            s->stats.slides += (int64_t)slide.size();
+            s->stats.slide_launches++;
+            for (size_t q = 0; q < slide.size(); ++q) status[slide[q]] = mr[q].status;
+        }
+        for (size_t i : fin)
+            finish(s, *run[i], std::move(run[i]->running), status[i], st == VIB_OK ? reps[i].iterations : 0,
+                   st == VIB_OK ? reps[i].final_cost : 0.0);
     }
     while (*count < capacity && !s->done.empty()) {
         finished[(*count)++] = s->done.front();

@@ -87,7 +87,8 @@ class VibServed(C.Structure):
 
 class VibServerStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("submitted", "finished", "failed", "turns", "launches", "window_iterations",
-                                         "uploads", "evictions", "appends", "slides", "resident_bytes")] + \
+                                         "uploads", "evictions", "appends", "slides", "slide_launches",
+                                         "resident_bytes")] + \
                [(n, C.c_int32) for n in ("queued", "running", "resident", "pending_results")] + \
                [(n, C.c_double) for n in ("admit_seconds", "iterate_seconds", "slide_seconds")]
 
@@ -361,6 +362,7 @@ def load_library(path=LIB_PATH):
     sig = {
         "vib_abi_version": ([], C.c_int),
         "vib_window_append": ([vp, C.c_int32, C.POINTER(VibKeyframe), ip], C.c_int),
+        "vib_marginalize_batch": ([vp, C.c_int32, ip, C.c_int32, C.POINTER(VibMarginalReport)], C.c_int),
         "vib_server_create": ([vp, C.POINTER(VibServerOptions), C.POINTER(vp)], C.c_int),
         "vib_server_destroy": ([vp], C.c_int),
         "vib_server_add_session": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
@@ -782,6 +784,14 @@ class Context:
         st = self.lib.vib_marginalize(self.h, slot, policy, None if smp is None else f64p(smp), cnt, t_start, t_end,
                                       None if params is None else C.byref(params), C.byref(rep))
         return st, rep
+
+    def marginalize_batch(self, slots, policy):
+        """vib_marginalize_batch: (first failure, one report per slot)."""
+        n = len(slots)
+        ids = np.ascontiguousarray(slots, np.int32)
+        reps = (VibMarginalReport * n)()
+        st = self.lib.vib_marginalize_batch(self.h, n, ids.ctypes.data_as(ip), policy, reps)
+        return st, list(reps)
 
     def request_begin(self, slot, opts):
         return self.check(self.lib.vib_request_begin(self.h, slot, C.byref(opts)), "vib_request_begin")

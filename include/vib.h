@@ -395,7 +395,7 @@ typedef struct vib_served {      /* one finished request */
 } vib_served;
 typedef struct vib_server_stats {
     int64_t submitted, finished, failed, turns, launches, window_iterations;
-    int64_t uploads, evictions, appends, slides, resident_bytes;
+    int64_t uploads, evictions, appends, slides, slide_launches, resident_bytes;
     int32_t queued, running, resident, pending_results;
     /* host wall time spent in each phase */
     double admit_seconds;        /* uploads, evictions, appends, request starts               */
@@ -456,6 +456,14 @@ int vib_copy_probe(vib_context* ctx, int64_t bytes);
 int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy,
                     const double* imu_samples, int32_t sample_count, double t_start, double t_end,
                     const vib_imu_params* imu_params, vib_marginal_report* report);
+/* The slides of many windows at once (no counterpart in the reference; for the request server,
+ * whose sessions slide together): with VIB_MARGIN_OLD the marginal systems linearize and reduce
+ * in one batch of launches and their eigendecompositions run as one launch per stage (one
+ * workgroup per window); each window's result equals vib_marginalize's. SECOND_NEW slides the
+ * windows one by one without IMU samples. reports[i] holds window i's result; the return value
+ * is the first failure. */
+int vib_marginalize_batch(vib_context* ctx, int32_t count, const int32_t* slots, int32_t policy,
+                          vib_marginal_report* reports);
 /* Copy the whole resident window (topology, measurements, state and prior) into `window`, whose
  * sizes (num_* and prior_rows) must match vib_window_shape and whose arrays the caller
  * allocates at those sizes (prior keys: up to prior_cols entries, prior_ref: up to

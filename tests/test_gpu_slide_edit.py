@@ -106,3 +106,31 @@ def test_three_slides_in_a_row(ctx, abi):
         assert st == 0, ctx.error()
         nxt = same_layout(ctx, 0, 1)
         assert nxt.K == w.K - 1 - i
+
+
+def test_batched_slides_equal_single_slides(ctx, abi):
+    """vib_marginalize_batch of five uneven windows (one marginal-system batch, one eigen launch
+    per stage) leaves each window exactly as vib_marginalize alone does: report, topology, prior
+    and state bit for bit."""
+    from vibpy import synth
+    wins = [synth.generate(K=6 + 2 * i, L=300 + 200 * i, n_cams=2 + i % 3, seed=60 + i, rig_free=i % 2 == 1,
+                           td_free=i % 2 == 1) for i in range(5)]
+    say("[gpu] edit: batched slides")
+    for i, w in enumerate(wins):
+        ctx.upload(10 + i, w)
+        ctx.upload(20 + i, w)
+        ctx.check(ctx.solve(10 + i, abi.options(max_iterations=1))[0], "solve")
+        ctx.check(ctx.solve(20 + i, abi.options(max_iterations=1))[0], "solve")
+    st, reps = ctx.marginalize_batch([10 + i for i in range(5)], abi.VIB_MARGIN_OLD)
+    assert st == 0, ctx.error()
+    for i, w in enumerate(wins):
+        st1, r1 = ctx.marginalize(20 + i, abi.VIB_MARGIN_OLD)
+        assert st1 == 0 and reps[i].status == 0
+        for f in ("eliminated_landmarks", "prior_rank", "retained_dimension", "consumed_factors", "eigen_cutoff"):
+            assert getattr(reps[i], f) == getattr(r1, f), f
+        a, b = ctx.read_window(10 + i), ctx.read_window(20 + i)
+        for f in ("lm_id", "lm_anchor_frame", "obs_landmark", "obs_frame", "prior_A", "prior_r0", "prior_ref",
+                  "pose", "speed_bias", "lm_rho", "imu_data"):
+            np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+        ctx.release(10 + i)
+        ctx.release(20 + i)
