@@ -28,6 +28,7 @@ FP64_PEAK_TFLOPS = 78.6
 # refinement passes of every mixed-precision line: the count at which the accuracy line meets
 # the metric's bound (dx relative error <= 1e-6 against plain FP64)
 MIXED_REFINEMENT = 2
+SERVE_SUSTAINED_RATE = 300.0  # requests/s of the serving line's second schedule (below the loop's capacity)
 
 
 def algorithmic_bytes(win, precision="fp64"):
@@ -94,9 +95,11 @@ def batched_line(ctx, abi, wins, steps, warmup):
     slots = list(range(1000, 1000 + len(wins)))
     for s, w in zip(slots, wins):
         ctx.upload(s, w)
-    res, kern = {}, {}
+    res, kern, tune = {}, {}, {}
     for name, prec, refine in (("fp64", abi.VIB_FP64, 0), ("mixed", abi.VIB_MIXED, MIXED_REFINEMENT)):
         opts = abi.options(max_iterations=1, precision=prec, refinement_steps=refine, flags=abi.VIB_SOLVE_CONTINUOUS)
+        # work-list shapes of this batch from the autotuner (kept by the context for the timed runs)
+        tune[name] = ctx.autotune(slots, opts, 2)
         ctx.bench(slots, opts, warmup, kernels=False)
         ctx.synchronize()
         it_ms, _ = ctx.bench(slots, opts, steps, kernels=False)
@@ -121,7 +124,8 @@ def batched_line(ctx, abi, wins, steps, warmup):
                 mixed_ms_per_batched_step=mixed_ms, mixed_p99_ms=float(np.percentile(res["mixed"], 99)),
                 sequential_value=len(wins) / (seq_ms * 1e-3), sequential_ms=seq_ms,
                 speedup_vs_sequential=seq_ms / batch_ms, mixed_speedup_vs_sequential_fp64=seq_ms / mixed_ms,
-                mixed_refinement_steps=MIXED_REFINEMENT, kernel_ms=kern["fp64"], mixed_kernel_ms=kern["mixed"])
+                mixed_refinement_steps=MIXED_REFINEMENT, kernel_ms=kern["fp64"], mixed_kernel_ms=kern["mixed"],
+                autotune=tune)
 
 
 def step_error(ctx, abi, win, slot, lam=1e-4, refinement=2):
@@ -166,23 +170,34 @@ def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb, precisio
     import multiprocessing as mp
     from vibpy import serve
     reqs = serve.poisson_requests(n_sessions, rate, seconds, np.random.default_rng(20261208))
-    need = np.bincount([r["session"] for r in reqs], minlength=n_sessions)
+    # a second schedule below the loop's capacity: latency without the backlog of the offered load
+    low = serve.poisson_requests(n_sessions, SERVE_SUSTAINED_RATE, 2.0, np.random.default_rng(20261209))
+    need = np.maximum(np.bincount([r["session"] for r in reqs], minlength=n_sessions),
+                      np.bincount([r["session"] for r in low], minlength=n_sessions))
     with mp.get_context("spawn").Pool(procs) as pool:
         streams = pool.map(_session_stream, [(i, int(need[i])) for i in range(n_sessions)])
     wins = [w for w, _ in streams]
     kfs = [k for _, k in streams]
-    srv = serve.Server(ctx, wins, int(budget_gb * (1 << 30)), precision=precision, refinement=refinement,
-                       full_iterations=True)
-    try:
-        srv.preload()
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        done = srv.run(reqs, keyframes=kfs)
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-    finally:
-        srv.close()
+    def serve_once(schedule):
+        srv = serve.Server(ctx, wins, int(budget_gb * (1 << 30)), precision=precision, refinement=refinement,
+                           full_iterations=True)
+        try:
+            srv.preload()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            done = srv.run(schedule, keyframes=kfs)
+            wall = time.perf_counter() - t0
+            return done, wall, srv.stats()
+        finally:
+            srv.close()
+
+    done, wall, st = serve_once(reqs)
+    ldone, lwall, _ = serve_once(low)
     out = serve.summary(done, wall)
+    sustained = serve.summary(ldone, lwall)
+    out["sustained"] = dict(arrival_rate=SERVE_SUSTAINED_RATE, offered_seconds=2.0, requests=sustained["requests"],
+                            requests_per_s=sustained["requests_per_s"], steps_per_s=sustained["steps_per_s"],
+                            p50_ms=sustained["p50_ms"], p99_ms=sustained["p99_ms"], failed=sustained["failed"])
     out.update(sessions=n_sessions, arrival_rate=rate, offered_seconds=seconds, hbm_budget_gb=budget_gb,
                uploads=st["uploads"], evictions=st["evictions"], launches=st["launches"], slides=st["slides"],
                slides_per_s=st["slides"] / wall, appends=st["appends"],
@@ -324,6 +339,31 @@ def slide_line(ctx, abi, win, reps=3):
                        "selection, marginal system and the slid window as device edits of the resident window, "
                        "the prior's eigen square root on the device; frame-level state and prior through the host)")
     return out
+
+
+def two_frame_line(ctx, abi, win, reps=20):
+    """Two-frame solve latency (optimizeLatestTwo, BackendOptimizer.cpp:476-590; budget <= 3
+    iterations, <= 8 ms, :549-553): the timed window resident, the two newest frames free with
+    their velocities, everything else constant; wall time of vib_solve_latest_two (the local
+    window built on the device from the resident one, its IMU bridge checked, the solve, the
+    write-back)."""
+    ctx.upload(0, win)
+    opts = abi.options(max_iterations=3)
+    ms, its = [], []
+    for i in range(reps + 2):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        st, rep = ctx.solve_latest_two(0, opts)
+        dt = 1e3 * (time.perf_counter() - t0)
+        if st != abi.VIB_OK:
+            raise RuntimeError(f"two-frame solve failed: {abi.STATUS_NAMES.get(st, st)}")
+        if i >= 2:
+            ms.append(dt)
+            its.append(rep.solve.iterations)
+    ctx.release(0)
+    return dict(p50_ms=float(np.percentile(ms, 50)), p99_ms=float(np.percentile(ms, 99)), budget_ms=8.0,
+                iterations=int(np.median(its)), support_landmarks=int(rep.support_landmarks), calls=reps,
+                workload="the timed window; vib_solve_latest_two with at most 3 iterations (FP64), wall time per call")
 
 
 def host_info():
@@ -500,6 +540,7 @@ def main():
         print(f"[bench] whitening: {args.whiten_observations} observations", file=sys.stderr, flush=True)
         whiten = whitening_line(ctx, abi, args.whiten_observations)
     slide = slide_line(ctx, abi, win) if args.slides else None
+    two_frame = two_frame_line(ctx, abi, win)
     if args.copy_probe_mb > 0:
         ctx.copy_probe(args.copy_probe_mb << 20)
     ctx.close()
@@ -550,6 +591,7 @@ def main():
     out["gate"] = gate
     if slide is not None:
         out["slide"] = slide
+    out["two_frame"] = two_frame
     if whiten is not None:
         out["whitening"] = whiten
     if preint is not None:

@@ -586,11 +586,11 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
         fine += s->host.chunks.size();
         fine_lin += s->host.lin_chunks.size();
     }
-    int merge = fine >= kMergeChunksAbove ? kChunkMerge : 1;
+    int merge = chunk_merge > 0 ? chunk_merge : (fine >= kMergeChunksAbove ? kChunkMerge : 1);
     if (const char* m = std::getenv("VIB_CHUNK_MERGE")) merge = std::max(1, std::atoi(m));
     // the same for the visual factor ranges: a merged range is one longer pair-ordered range,
     // whose workgroup zeroes, and writes back, its visual-block accumulator once
-    int lin_merge = fine_lin >= kMergeLinAbove ? kLinMerge : 1;
+    int lin_merge = this->lin_merge > 0 ? this->lin_merge : (fine_lin >= kMergeLinAbove ? kLinMerge : 1);
     if (const char* m = std::getenv("VIB_LIN_MERGE")) lin_merge = std::max(1, std::atoi(m));
     for (int i = 0; i < nwin; ++i) {
         Slot& s = *slots[i];
@@ -752,12 +752,18 @@ struct vib_context {
     size_t edit_cap = 0;
     void* spare = nullptr;
     size_t spare_cap = 0;
+    // work-list shapes chosen by vib_autotune: (precision, log2 of a batch's visual factor
+    // ranges) -> (chunk merge, range merge)
+    std::map<std::pair<int, int>, std::pair<int, int>> tuned;
+    void* tune_buf = nullptr;  // state snapshot of the windows being tuned
+    size_t tune_cap = 0;
     ~vib_context() {
         if (preint_buf) (void)hipFree(preint_buf);
         if (eig_buf) (void)hipFree(eig_buf);
         if (aux_buf) (void)hipFree(aux_buf);
         if (edit_buf) (void)hipFree(edit_buf);
         if (spare) (void)hipFree(spare);
+        if (tune_buf) (void)hipFree(tune_buf);
         for (auto& e : edit_pool)
             if (e.first) (void)hipFree(e.first);
     }
@@ -784,6 +790,23 @@ void* grow_device(void*& buf, size_t& cap, size_t bytes) {
     }
     cap = want;
     return buf;
+}
+
+// Size class of a batch for the autotuned work-list shapes: log2 of its visual factor ranges.
+int tune_class(const std::vector<Slot*>& slots) {
+    size_t n = 0;
+    for (const Slot* s : slots) n += s->host.lin_chunks.size();
+    int c = 0;
+    while ((size_t(1) << (c + 1)) <= n) ++c;
+    return c;
+}
+
+// The work-list shapes vib_autotune chose for batches like this one (the built-in rule when it
+// did not tune this precision and size class).
+void apply_tuning(vib_context* ctx, Batch& b, const std::vector<Slot*>& slots) {
+    auto it = ctx->tuned.find({b.mixed, tune_class(slots)});
+    b.chunk_merge = it == ctx->tuned.end() ? 0 : it->second.first;
+    b.lin_merge = it == ctx->tuned.end() ? 0 : it->second.second;
 }
 
 Slot* find_slot(vib_context* ctx, int slot) {
@@ -854,6 +877,7 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
         if (!write_lm(ctx, *s, fresh_lm(o))) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
     Batch& batch = ctx->solve;
     batch.mixed = o.precision == VIB_MIXED;
+    apply_tuning(ctx, batch, slots);
     std::string err;
     if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     const bool graph = ctx->use_graph && batch.capture(ctx->stream, o.refinement_steps);
@@ -2336,6 +2360,7 @@ int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* ids, int
     }
     Batch& b = ctx->requests;
     b.mixed = precision == VIB_MIXED;
+    apply_tuning(ctx, b, slots);
     std::string err;
     if (!b.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     if (!b.iteration(ctx->stream, refinement))
@@ -2748,6 +2773,7 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
     }
     Batch& batch = ctx->solve;
     batch.mixed = o->precision == VIB_MIXED;
+    apply_tuning(ctx, batch, slots);
     std::string err;
     if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     if (!batch.capture(ctx->stream, o->refinement_steps)) return fail(ctx, VIB_ERR_DEVICE, "graph capture failed");
@@ -2792,6 +2818,88 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
         for (auto& e : kev) (void)hipEventDestroy(e);
         if (!ok) return fail(ctx, VIB_ERR_DEVICE, "kernel profiling failed");
     }
+    return VIB_OK;
+}
+
+int vib_autotune(vib_context* ctx, int32_t count, const int32_t* ids, const vib_solve_options* o, int32_t iterations,
+                 vib_tune_report* rep) {
+    if (!ctx || !o || !ids || !rep || count <= 0 || iterations <= 0) return VIB_ERR_INVALID_ARGUMENT;
+    if (o->precision != VIB_FP64 && o->precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    hipSetDevice(ctx->device);
+    *rep = vib_tune_report{};
+    std::vector<Slot*> slots;
+    for (int i = 0; i < count; ++i) {
+        Slot* s = find_slot(ctx, ids[i]);
+        if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+        if (s->host_set) return fail(ctx, VIB_ERR_UNSUPPORTED, "a window with host-evaluated factors is not tuned");
+        slots.push_back(s);
+    }
+    // the windows' states before the timed iterations, restored afterwards
+    size_t snap = 0;
+    for (const Slot* s : slots) snap += 16 * (size_t)s->host.K + 8 + s->host.L;
+    double* keep = static_cast<double*>(grow_device(ctx->tune_buf, ctx->tune_cap, 8 * snap));
+    if (!keep) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "tuning buffer allocation failed");
+    auto copy_state = [&](bool save) {
+        double* p = keep;
+        bool ok = true;
+        for (const Slot* s : slots) {
+            const WinDev& d = s->dev;
+            const size_t part[5] = {7 * (size_t)d.K, 9 * (size_t)d.K, 7, 1, (size_t)d.L};
+            double* const arr[5] = {d.pose, d.sb, d.rig, d.td, d.rho};
+            for (int q = 0; q < 5 && ok; ++q) {
+                if (part[q])
+                    ok = hipMemcpyAsync(save ? p : arr[q], save ? arr[q] : p, 8 * part[q], hipMemcpyDeviceToDevice,
+                                        ctx->stream) == hipSuccess;
+                p += part[q];
+            }
+        }
+        return ok;
+    };
+    if (!copy_state(true)) return fail(ctx, VIB_ERR_DEVICE, "state snapshot failed");
+    const int merges[4] = {1, 2, 4, 8};
+    Batch& batch = ctx->solve;
+    batch.mixed = o->precision == VIB_MIXED;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(ctx, VIB_ERR_DEVICE, "event creation failed");
+    std::string err;
+    bool ok = true;
+    double best = 1e300;
+    for (int a = 0; a < 4 && ok; ++a)
+        for (int b = 0; b < 4 && ok; ++b) {
+            for (Slot* s : slots) {
+                LmState lm{};
+                lm.lambda = o->initial_lambda > 0 ? o->initial_lambda : 1e-4;
+                lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1 << 30; lm.continuous = 1;
+                ok = ok && write_lm(ctx, *s, lm);
+            }
+            batch.chunk_merge = merges[a];
+            batch.lin_merge = merges[b];
+            ok = ok && batch.build(slots, ctx->stream, err) && batch.capture(ctx->stream, o->refinement_steps) &&
+                 hipGraphLaunch(batch.graph_exec, ctx->stream) == hipSuccess &&  // warm-up
+                 hipEventRecord(e0, ctx->stream) == hipSuccess;
+            for (int it = 0; ok && it < iterations; ++it) ok = hipGraphLaunch(batch.graph_exec, ctx->stream) == hipSuccess;
+            float ms = 0;
+            ok = ok && hipEventRecord(e1, ctx->stream) == hipSuccess && hipStreamSynchronize(ctx->stream) == hipSuccess &&
+                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            const int k = rep->candidates++;
+            rep->chunk_merge[k] = merges[a];
+            rep->range_merge[k] = merges[b];
+            rep->ms_per_iteration[k] = ms / iterations;
+            if (ok && ms / iterations < best) {
+                best = ms / iterations;
+                rep->best = k;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ok = ok && copy_state(false) && hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) return fail(ctx, VIB_ERR_DEVICE, std::string("tuning launches failed: ") + err);
+    rep->landmarks_per_chunk = HostWindow::chunk_size;
+    rep->factors_per_range = kLinFactors;
+    rep->size_class = tune_class(slots);
+    ctx->tuned[{batch.mixed, rep->size_class}] = {rep->chunk_merge[rep->best], rep->range_merge[rep->best]};
     return VIB_OK;
 }
 

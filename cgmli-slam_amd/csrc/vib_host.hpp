@@ -127,6 +127,9 @@ struct Batch {
     int max_edges = 0;
     int max_landmarks = 0;
     int mixed = 0;  // 1: the mixed-precision forms of the coupling and Schur kernels
+    // landmark chunks / visual factor ranges merged per workgroup (0: the built-in rule, merged
+    // by kChunkMerge / kLinMerge above kMergeChunksAbove / kMergeLinAbove; vib_autotune sets them)
+    int chunk_merge = 0, lin_merge = 0;
     WinDev* d_wins = nullptr;
     Chunk* d_chunks = nullptr;
     Chunk* d_lin = nullptr;

@@ -73,6 +73,12 @@ class VibWindow(C.Structure):
     ]
 
 
+class VibTuneReport(C.Structure):
+    _fields_ = [("candidates", C.c_int32), ("best", C.c_int32), ("size_class", C.c_int32),
+                ("landmarks_per_chunk", C.c_int32), ("factors_per_range", C.c_int32),
+                ("chunk_merge", C.c_int32 * 16), ("range_merge", C.c_int32 * 16), ("ms_per_iteration", C.c_double * 16)]
+
+
 class VibServerOptions(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("precision", C.c_int32), ("refinement_steps", C.c_int32),
                 ("slide_policy", C.c_int32), ("full_iterations", C.c_int32), ("first_slot", C.c_int32),
@@ -363,6 +369,7 @@ def load_library(path=LIB_PATH):
         "vib_abi_version": ([], C.c_int),
         "vib_window_append": ([vp, C.c_int32, C.POINTER(VibKeyframe), ip], C.c_int),
         "vib_marginalize_batch": ([vp, C.c_int32, ip, C.c_int32, C.POINTER(VibMarginalReport)], C.c_int),
+        "vib_autotune": ([vp, C.c_int32, ip, C.POINTER(VibSolveOptions), C.c_int32, C.POINTER(VibTuneReport)], C.c_int),
         "vib_server_create": ([vp, C.POINTER(VibServerOptions), C.POINTER(vp)], C.c_int),
         "vib_server_destroy": ([vp], C.c_int),
         "vib_server_add_session": ([vp, C.c_int32, C.POINTER(VibWindow)], C.c_int),
@@ -784,6 +791,18 @@ class Context:
         st = self.lib.vib_marginalize(self.h, slot, policy, None if smp is None else f64p(smp), cnt, t_start, t_end,
                                       None if params is None else C.byref(params), C.byref(rep))
         return st, rep
+
+    def autotune(self, slots, opts, iterations=3):
+        """vib_autotune over `slots`: the measured work-list shapes and the chosen one (dict)."""
+        ids = np.ascontiguousarray(slots, np.int32)
+        r = VibTuneReport()
+        self.check(self.lib.vib_autotune(self.h, len(slots), ids.ctypes.data_as(ip), C.byref(opts), iterations,
+                                         C.byref(r)), "vib_autotune")
+        n = r.candidates
+        table = [dict(chunk_merge=r.chunk_merge[i], range_merge=r.range_merge[i], ms=r.ms_per_iteration[i])
+                 for i in range(n)]
+        return dict(best=table[r.best], size_class=r.size_class, landmarks_per_chunk=r.landmarks_per_chunk,
+                    factors_per_range=r.factors_per_range, candidates=table)
 
     def marginalize_batch(self, slots, policy):
         """vib_marginalize_batch: (first failure, one report per slot)."""

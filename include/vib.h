@@ -436,6 +436,27 @@ const char* vib_kernel_name(int32_t kernel);
  * their launches; 0 otherwise. */
 int vib_bench(vib_context* ctx, int32_t count, const int32_t* slots, const vib_solve_options* options,
               int32_t iterations, double* iteration_ms, double* kernel_ms);
+/* The work-list shapes of batched solves, chosen by measurement (no counterpart in the
+ * reference): every combination of landmark chunks merged per workgroup (1, 2, 4, 8 chunks of
+ * landmarks_per_chunk) and visual factor ranges merged per workgroup (1, 2, 4, 8 ranges of
+ * factors_per_range) runs `iterations` LM iterations of the listed windows (captured graph,
+ * continuous mode, timed with events); the fastest is kept by the context for batches of the
+ * same precision and size class (log2 of their visual factor ranges) and used by vib_solve,
+ * vib_solve_batch, the request API and vib_bench. The windows' states are restored afterwards.
+ * Merged work lists change the order of some sums: a window's result in a tuned batch can
+ * differ in the last bits from its result alone. */
+typedef struct vib_tune_report {
+    int32_t candidates;          /* combinations measured (16)                               */
+    int32_t best;                /* index of the chosen one                                  */
+    int32_t size_class;          /* the batch's size class                                   */
+    int32_t landmarks_per_chunk; /* layout constants the merges multiply                     */
+    int32_t factors_per_range;
+    int32_t chunk_merge[16];
+    int32_t range_merge[16];
+    double ms_per_iteration[16];
+} vib_tune_report;
+int vib_autotune(vib_context* ctx, int32_t count, const int32_t* slots, const vib_solve_options* options,
+                 int32_t iterations, vib_tune_report* report);
 /* One launch of a plain streaming copy of `bytes` (multiple of 16) between two fresh device
  * buffers: the known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters. */
 int vib_copy_probe(vib_context* ctx, int64_t bytes);
