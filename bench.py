@@ -341,29 +341,40 @@ def slide_line(ctx, abi, win, reps=3):
     return out
 
 
-def two_frame_line(ctx, abi, win, reps=20):
+def two_frame_line(ctx, abi, reps=12):
     """Two-frame solve latency (optimizeLatestTwo, BackendOptimizer.cpp:476-590; budget <= 3
-    iterations, <= 8 ms, :549-553): the timed window resident, the two newest frames free with
-    their velocities, everything else constant; wall time of vib_solve_latest_two (the local
-    window built on the device from the resident one, its IMU bridge checked, the solve, the
-    write-back)."""
+    iterations, <= 8 ms, :549-553) where the reference runs it: right after a new keyframe joins
+    a solved window. A C2-sized stream (4 cameras, rig and time offset online, about 1000
+    landmarks born per frame, 20 frames with the new one): per sample the next keyframe is
+    appended, vib_solve_latest_two runs (timed: the local window built on the device, its IMU
+    bridge checked, at most 3 iterations, the write-back), then the window solves 2 iterations and
+    slides OLD back to 19 frames (untimed)."""
+    from vibpy import synth
+    win, kfs = synth.stream(19, 19 + reps + 2, 1000, n_cams=4, obs_range=(4, 8), seed=20261212, rig_free=True,
+                            td_free=True)
     ctx.upload(0, win)
-    opts = abi.options(max_iterations=3)
+    ctx.check(ctx.solve(0, abi.options(max_iterations=5))[0], "vib_solve")
     ms, its = [], []
-    for i in range(reps + 2):
+    for i, kf in enumerate(kfs[:reps + 2]):
+        ctx.append(0, kf)
         ctx.synchronize()
         t0 = time.perf_counter()
-        st, rep = ctx.solve_latest_two(0, opts)
+        st, rep = ctx.solve_latest_two(0, abi.options(max_iterations=3))
         dt = 1e3 * (time.perf_counter() - t0)
         if st != abi.VIB_OK:
             raise RuntimeError(f"two-frame solve failed: {abi.STATUS_NAMES.get(st, st)}")
         if i >= 2:
             ms.append(dt)
             its.append(rep.solve.iterations)
+        ctx.check(ctx.solve(0, abi.options(max_iterations=2))[0], "vib_solve")
+        ctx.check(ctx.marginalize(0, abi.VIB_MARGIN_OLD)[0], "vib_marginalize")
+    shape = ctx.shape(0)
     ctx.release(0)
     return dict(p50_ms=float(np.percentile(ms, 50)), p99_ms=float(np.percentile(ms, 99)), budget_ms=8.0,
                 iterations=int(np.median(its)), support_landmarks=int(rep.support_landmarks), calls=reps,
-                workload="the timed window; vib_solve_latest_two with at most 3 iterations (FP64), wall time per call")
+                window=dict(frames=20, landmarks=int(shape[1]), observations=int(shape[2])),
+                workload="C2-sized keyframe stream (20 frames with the new one, rig+td online); each call right "
+                         "after the next keyframe's append; vib_solve_latest_two, at most 3 iterations, FP64; wall time")
 
 
 def host_info():
@@ -540,7 +551,7 @@ def main():
         print(f"[bench] whitening: {args.whiten_observations} observations", file=sys.stderr, flush=True)
         whiten = whitening_line(ctx, abi, args.whiten_observations)
     slide = slide_line(ctx, abi, win) if args.slides else None
-    two_frame = two_frame_line(ctx, abi, win)
+    two_frame = two_frame_line(ctx, abi)
     if args.copy_probe_mb > 0:
         ctx.copy_probe(args.copy_probe_mb << 20)
     ctx.close()
