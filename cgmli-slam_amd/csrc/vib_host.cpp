@@ -31,7 +31,7 @@ const char* vibk_name(int k);
 hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chunks, int nchunks, const vib::Chunk* lin,
                        int nlin, double** red, int max_len, int max_edges, int max_landmarks, int refinement, int mode,
                        int mixed, int fuse_reduce, hipStream_t s);
-hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s);
+hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, int max_cols, hipStream_t s);
 hipError_t vibk_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t vibk_elim_set(const vib::WinDev* wins, int L, int victim, uint8_t* elim, hipStream_t s);
 hipError_t vibk_reanchor(const vib::WinDev* wins, int v, const int32_t* lms, const uint8_t* cls, int count,
@@ -520,7 +520,7 @@ bool Slot::fill(const HostWindow& h, const WinDev& d, const SlotCarve& x, char* 
     if (ok && h.prior_rows) {
         // G = A^T A once per prior (one-window launch)
         ok = up(x.gram_win, &d, sizeof(WinDev)) && up(x.gram_out, &d.pG, sizeof(double*)) &&
-             vibk_prior_gram(x.gram_win, 1, x.gram_out, stream) == hipSuccess;
+             vibk_prior_gram(x.gram_win, 1, x.gram_out, h.prior_cols, stream) == hipSuccess;
     }
     // the staging above (landmark keys, LM state) is local: the copies complete before return
     return ok && hipStreamSynchronize(stream) == hipSuccess;
@@ -1235,15 +1235,8 @@ bool imu_nis(vib_context* ctx, Slot& s, const int edges[2], double nis[2]) {
     return true;
 }
 
-// OLD marginalization (Marginalizer::compute, Marginalizer.cpp:284-474, with the elimination set
-// of SlidingWindowManager.cpp:627-652): slot 0's pose and speed/bias and every landmark anchored
-// in or observed by slot 0 are eliminated from the factors they touch (all blocks of those
-// factors as variables). On the device: linearization of the consumed factors and the landmark
-// Schur complement through the diagonal pseudo-inverse. On the host: the 15 x 15 state block's
-// eigen pseudo-inverse and the rank-15 update to H_prior, b_prior. On the device again: a
-// semidefinite MFMA tile Cholesky H_prior = L L^T with r0 = L^-1 b_prior, so A = L^T gives
-// A^T A = H_prior and A^T r0 = b_prior (dropped pivots drop rows). The window then slides.
-// Editable copy of a window in the C-ABI layout; view() points a vib_window into it.
+// Copy of a resident window in the C-ABI layout (vib_read_window); view() points a vib_window
+// into it.
 struct WindowCopy {
     int K = 0, C = 0, L = 0, F = 0, E = 0;
     uint32_t flags = 0;

@@ -164,9 +164,9 @@ __device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData&
     const bool tdd = WANT_J && want_td;
     V3<double> da = {0, 0, 0}, db = {0, 0, 0}, ba, bb;
     if (movingA) ba = time_bearing(lm.raw, lm.vel, lm.tref - td, tdd, da);
-    else { const double n = sqrt(sqnorm(lm.raw)); ba = {lm.raw.x / n, lm.raw.y / n, lm.raw.z / n}; }
+    else { const double in = inv_sqrt(sqnorm(lm.raw)); ba = {lm.raw.x * in, lm.raw.y * in, lm.raw.z * in}; }
     if (movingB) bb = time_bearing(braw, bvel, btref - td, tdd, db);
-    else { const double n = sqrt(sqnorm(braw)); bb = {braw.x / n, braw.y / n, braw.z / n}; }
+    else { const double in = inv_sqrt(sqnorm(braw)); bb = {braw.x * in, braw.y * in, braw.z * in}; }
     if (!isfinite(ba.x) || !isfinite(ba.y) || !isfinite(ba.z) || !isfinite(bb.x) || !isfinite(bb.y) || !isfinite(bb.z))
         return false;
     // camera chain
@@ -174,7 +174,8 @@ __device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData&
     for (int i = 0; i < 9; ++i) { Rwa.m[i] = ft.Rc[a][ca][i]; Rwb.m[i] = ft.Rc[b][cb][i]; }
     const V3<double> twa = {ft.tc[a][ca][0], ft.tc[a][ca][1], ft.tc[a][ca][2]};
     const V3<double> twb = {ft.tc[b][cb][0], ft.tc[b][cb][1], ft.tc[b][cb][2]};
-    const V3<double> pw = mul(Rwa, (1.0 / rho) * ba) + twa;
+    const double irho = 1.0 / rho;
+    const V3<double> pw = mul(Rwa, irho * ba) + twa;
     const V3<double> pc = mulT(Rwb, pw - twb);
     // measurement projection: P = S B^T (transported when the bearing moves)
     V3<double> c0, c1;
@@ -182,17 +183,18 @@ __device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData&
     double M[6] = {s00 * c0.x, s00 * c0.y, s00 * c0.z,
                    s10 * c0.x + s11 * c1.x, s10 * c0.y + s11 * c1.y, s10 * c0.z + s11 * c1.z};
     double P[6];
-    double den = 2;
+    double den = 2, iden = 0.5;
     double K[9];
     if (movingB) {
         den = 1 + dot(braw, bb);
+        iden = 1.0 / den;
         const V3<double> k = cross(braw, bb);
         K[0] = 0; K[1] = -k.z; K[2] = k.y; K[3] = k.z; K[4] = 0; K[5] = -k.x; K[6] = -k.y; K[7] = k.x; K[8] = 0;
         double Tr[9];  // transport = I + K + K K / den
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) {
                 double kk = K[r * 3] * K[c] + K[r * 3 + 1] * K[3 + c] + K[r * 3 + 2] * K[6 + c];
-                Tr[r * 3 + c] = (r == c ? 1.0 : 0.0) + K[r * 3 + c] + kk / den;
+                Tr[r * 3 + c] = (r == c ? 1.0 : 0.0) + K[r * 3 + c] + kk * iden;
             }
         for (int r = 0; r < 2; ++r)
             for (int c = 0; c < 3; ++c)  // P = M * Tr^T
@@ -244,7 +246,7 @@ __device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData&
     }
     // rho: D Rwa (-ba / rho^2)
     {
-        const V3<double> v = mul(Rwa, (-1.0 / (rho * rho)) * ba);
+        const V3<double> v = mul(Rwa, (-irho * irho) * ba);
         o.Jrho[0] = wt * (D[0] * v.x + D[1] * v.y + D[2] * v.z);
         o.Jrho[1] = wt * (D[3] * v.x + D[4] * v.y + D[5] * v.z);
         if (!isfinite(o.Jrho[0]) || !isfinite(o.Jrho[1])) return false;
@@ -252,7 +254,7 @@ __device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData&
     o.Jt[0] = o.Jt[1] = 0;
     if (tdd && (movingA || movingB)) {
         // J = D Rwa (da/rho) + P Jb db + M dR^T log  (VisualSphereBatchMath.hpp:162-171)
-        const V3<double> v = mul(Rwa, (1.0 / rho) * da);
+        const V3<double> v = mul(Rwa, irho * da);
         double t1[2] = {D[0] * v.x + D[1] * v.y + D[2] * v.z, D[3] * v.x + D[4] * v.y + D[5] * v.z};
         // Jb db = f(-bb (n.db) - c db) + g u (n.db)
         const double ndb = dot(e.n, db);
@@ -262,7 +264,7 @@ __device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData&
         if (movingB) {
             const V3<double> kd = cross(braw, db);
             double dK[9] = {0, -kd.z, kd.y, kd.z, 0, -kd.x, -kd.y, kd.x, 0};
-            const double mdb = dot(braw, db);
+            const double mdb = dot(braw, db), mdd = mdb * (iden * iden);
             double dR[9];
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) {
@@ -272,7 +274,7 @@ __device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData&
                         a2 += K[r * 3 + m] * dK[m * 3 + c];
                         kk += K[r * 3 + m] * K[m * 3 + c];
                     }
-                    dR[r * 3 + c] = dK[r * 3 + c] + (a1 + a2) / den - kk * (mdb / (den * den));
+                    dR[r * 3 + c] = dK[r * 3 + c] + (a1 + a2) * iden - kk * mdd;
                 }
             // dR^T log
             const V3<double> lg = e.logv;
@@ -304,6 +306,7 @@ __device__ __forceinline__ LmData load_landmark(const WinDev& w, const double* r
 __device__ __forceinline__ bool window_active(const WinDev& w) { return w.lm->status == 0; }
 
 constexpr int kAccSize = kMaxVisual * (kMaxVisual + 1) / 2 + kMaxVisual;
+constexpr int kLinXcds = 8;  // XCDs of an MI355X: workgroup i of a launch runs on XCD i % 8
 
 // ---------------------------------------------------------------------------------------------
 // k_lin_visual: visual factors in (anchor frame, observer frame) pair order, one factor per lane
@@ -422,9 +425,12 @@ __device__ __forceinline__ void retained_items(const WinDev* wins, int first_blo
     if (item < items) retained_body<MODE>(wins[item / per], item % per, threadIdx.x % 64, scratch_of_wave);
 }
 
-// Workgroups nlin.. run the retained items (linearization, mode 0), items / per as above.
+// Workgroups 0 .. rb-1 run the retained items (linearization, mode 0), items / per as above;
+// they come first so that their longer evaluation overlaps the visual chunks instead of
+// trailing them. rb is a multiple of the XCD count, so visual chunk j (workgroup rb + j) keeps
+// the XCD j % 8 of the work-list order.
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* lin, int nlin, int items, int per) {
+__global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const Chunk* lin, int rb, int items, int per) {
     __shared__ double acc[kAccSize];
     __shared__ FrameTable ft;
     __shared__ double rows[kWaves][kGramCols * kRowPitch];  // per wave: 2 rows per lane x 16 columns
@@ -434,11 +440,11 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     __shared__ int gkey[kWaves][kGramsPerWave];
     __shared__ int ngram[kWaves];
     __shared__ double red[kBlock];
-    if ((int)blockIdx.x >= nlin) {  // the row staging of the wave holds the prior's scratch
-        retained_items<0>(wins, nlin, items, per, rows[threadIdx.x / 64]);
+    if ((int)blockIdx.x < rb) {  // the row staging of the wave holds the prior's scratch
+        retained_items<0>(wins, 0, items, per, rows[threadIdx.x / 64]);
         return;
     }
-    const Chunk ch = lin[blockIdx.x];
+    const Chunk ch = lin[blockIdx.x - rb];
     if (ch.local < 0) return;  // an empty position of the XCD-ordered work list
     const WinDev& w = wins[ch.win];
     if (!window_active(w) || !w.lm->new_point) return;
@@ -2683,18 +2689,33 @@ __global__ __launch_bounds__(kBlock) void k_accept(WinDev* wins) {
 
 
 // ---------------------------------------------------------------------------------------------
-// Prior Gram matrix G = A^T A (cols x cols), one workgroup per window, after a prior upload.
+// Prior Gram matrix G = A^T A (cols x cols) after a prior upload: one 256-thread workgroup per
+// 16 x 16 tile of G (blockIdx.x = tile row * tiles + tile column, blockIdx.y = window), one
+// output per thread. The two 16-column strips of A pass through LDS 16 rows at a time; each
+// output sums its rows in row order.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_prior_gram(const WinDev* wins, double* const* G) {
-    const WinDev& w = wins[blockIdx.x];
+constexpr int kGramTile = 16;
+
+__global__ __launch_bounds__(kBlock) void k_prior_gram(const WinDev* wins, double* const* G, int tiles) {
+    __shared__ double a[kGramTile][kGramTile + 1], b[kGramTile][kGramTile + 1];
+    const WinDev& w = wins[blockIdx.y];
     const int m = w.prior_rows, n = w.prior_cols;
-    double* out = G[blockIdx.x];
-    for (size_t p = threadIdx.x; p < (size_t)n * n; p += blockDim.x) {
-        const int i = (int)(p / n), j = (int)(p % n);
-        double s = 0;
-        for (int r = 0; r < m; ++r) s += w.pA[(size_t)r * n + i] * w.pA[(size_t)r * n + j];
-        out[p] = s;
+    const int ti = blockIdx.x / tiles, tj = blockIdx.x % tiles;
+    const int i0 = ti * kGramTile, j0 = tj * kGramTile;
+    if (i0 >= n || j0 >= n) return;  // uniform over the workgroup
+    const int t = threadIdx.x, r16 = t / kGramTile, c16 = t % kGramTile;
+    const int i = i0 + r16, j = j0 + c16;
+    double s = 0;
+    for (int r0 = 0; r0 < m; r0 += kGramTile) {
+        const int r = r0 + r16;
+        a[r16][c16] = (r < m && i0 + c16 < n) ? w.pA[(size_t)r * n + i0 + c16] : 0.0;
+        b[r16][c16] = (r < m && j0 + c16 < n) ? w.pA[(size_t)r * n + j0 + c16] : 0.0;
+        __syncthreads();
+        const int rn = min(kGramTile, m - r0);
+        for (int q = 0; q < rn; ++q) s += a[q][r16] * b[q][c16];
+        __syncthreads();
     }
+    if (i < n && j < n) G[blockIdx.y][(size_t)i * n + j] = s;
 }
 
 
@@ -3462,11 +3483,12 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
     const int slices = (max_len + kRedEntries - 1) / kRedEntries;
     const int rper = max_edges + 1, ritems = fuse_reduce ? nwin * rper : 0;  // retained items inside kernel 0
     switch (k) {
-        case 0: {  // visual factors, then (small batches) the retained items in the same launch
-            const int grid = nlin + (ritems + kWaves - 1) / kWaves;
+        case 0: {  // (small batches) the retained items, then the visual factors, in one launch
+            const int rb = ritems ? ((ritems + kWaves - 1) / kWaves + kLinXcds - 1) / kLinXcds * kLinXcds : 0;
+            const int grid = rb + nlin;
             if (grid == 0) break;
-            if (mixed) hipLaunchKernelGGL(k_lin_visual<true>, dim3(grid), dim3(kBlock), 0, s, wins, lin, nlin, ritems, rper);
-            else hipLaunchKernelGGL(k_lin_visual<false>, dim3(grid), dim3(kBlock), 0, s, wins, lin, nlin, ritems, rper);
+            if (mixed) hipLaunchKernelGGL(k_lin_visual<true>, dim3(grid), dim3(kBlock), 0, s, wins, lin, rb, ritems, rper);
+            else hipLaunchKernelGGL(k_lin_visual<false>, dim3(grid), dim3(kBlock), 0, s, wins, lin, rb, ritems, rper);
         }
             if (nchunks) {
                 if (mixed) hipLaunchKernelGGL(k_lm_sums<true>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
@@ -3594,8 +3616,10 @@ hipError_t vibk_sym_eigen_batch(const vib::EigenJob* jobs, int count, int max_n,
     return hipGetLastError();
 }
 
-hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, hipStream_t s) {
-    hipLaunchKernelGGL(vib::k_prior_gram, dim3(nwin), dim3(vib::kBlock), 0, s, wins, G);
+hipError_t vibk_prior_gram(vib::WinDev* wins, int nwin, double* const* G, int max_cols, hipStream_t s) {
+    if (nwin < 1 || max_cols < 1) return hipErrorInvalidValue;
+    const int tiles = (max_cols + vib::kGramTile - 1) / vib::kGramTile;
+    hipLaunchKernelGGL(vib::k_prior_gram, dim3(tiles * tiles, nwin), dim3(vib::kBlock), 0, s, wins, G, tiles);
     return hipGetLastError();
 }
 

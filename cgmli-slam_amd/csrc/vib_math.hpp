@@ -37,6 +37,25 @@ struct M3 {  // row-major
     T m[9];
 };
 
+// 1 / sqrt(x) for x > 0. In device code, inside the normal range, the hardware estimate refined
+// by two Newton steps (a few ulp, no division); elsewhere the quotient of the rounded root.
+VIB_HD double inv_sqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (x > 1e-280 && x < 1e280) {
+        double q = __builtin_amdgcn_rsq(x);
+        const double h = 0.5 * x;
+        q = q * fma(-h * q, q, 1.5);
+        q = q * fma(-h * q, q, 1.5);
+        return q;
+    }
+#endif
+    return 1.0 / sqrt(x);
+}
+VIB_HD float inv_sqrt(float x) { return 1.0f / sqrtf(x); }
+// sin and cos of one argument (one range reduction).
+VIB_HD void sin_cos(double a, double& s, double& c) { sincos(a, &s, &c); }
+VIB_HD void sin_cos(float a, float& s, float& c) { sincosf(a, &s, &c); }
+
 template <class T> VIB_HD V3<T> v3(T x, T y, T z) { return V3<T>{x, y, z}; }
 template <class T> VIB_HD V3<T> load3(const T* p) { return V3<T>{p[0], p[1], p[2]}; }
 template <class T> VIB_HD V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -155,8 +174,8 @@ template <class T>
 VIB_HD void sphere_basis(V3<T> b, V3<T>& c0, V3<T>& c1) {
     const V3<T> axis = fabs(b.x) < T(0.8) ? v3<T>(1, 0, 0) : v3<T>(0, 1, 0);
     V3<T> t = axis - dot(b, axis) * b;
-    const T n = sqrt(sqnorm(t));
-    c0 = {t.x / n, t.y / n, t.z / n};
+    const T in = inv_sqrt(sqnorm(t));
+    c0 = {t.x * in, t.y * in, t.z * in};
     c1 = cross(b, c0);
 }
 
@@ -168,21 +187,24 @@ VIB_HD V3<T> time_bearing(V3<T> raw, V3<T> v, T dt, bool differentiate, V3<T>& d
         const T angle = sqrt(sqnorm(u));
         if (angle < T(1e-8)) {
             const V3<T> s = raw + u;
-            const T n = sqrt(sqnorm(s));
-            return {s.x / n, s.y / n, s.z / n};
+            const T in = inv_sqrt(sqnorm(s));
+            return {s.x * in, s.y * in, s.z * in};
         }
-        return cos(angle) * raw + (sin(angle) / angle) * u;
+        T sn, cs;
+        sin_cos(angle, sn, cs);
+        return cs * raw + (sn / angle) * u;
     }
     const T speed = sqrt(sqnorm(v)), angle = speed * dt;
     if (fabs(angle) < T(1e-8)) {
         const V3<T> dir = raw + dt * v;
-        const T len = sqrt(sqnorm(dir));
-        const V3<T> unit = {dir.x / len, dir.y / len, dir.z / len};
+        const T il = inv_sqrt(sqnorm(dir));
+        const V3<T> unit = {dir.x * il, dir.y * il, dir.z * il};
         const V3<T> pv = v - dot(unit, v) * unit;
-        derivative = {-pv.x / len, -pv.y / len, -pv.z / len};
+        derivative = {-pv.x * il, -pv.y * il, -pv.z * il};
         return unit;
     }
-    const T sn = sin(angle), cs = cos(angle);
+    T sn, cs;
+    sin_cos(angle, sn, cs);
     derivative = (speed * sn) * raw - cs * v;
     return cs * raw + (sn / speed) * v;
 }
@@ -203,7 +225,8 @@ VIB_HD bool sphere_residual(V3<T> pc, V3<T> bb, const T* P, T denominator, bool 
                             SphereEval<T>& e) {
     const T range = sqrt(sqnorm(pc));
     if (!isfinite(pc.x) || !isfinite(pc.y) || !isfinite(pc.z) || !isfinite(range) || range < T(1e-10)) return false;
-    const V3<T> n = {pc.x / range, pc.y / range, pc.z / range};
+    const T ir = T(1) / range;
+    const V3<T> n = {pc.x * ir, pc.y * ir, pc.z * ir};
     const T c = fmin(T(1), fmax(T(-1), dot(bb, n)));
     const T s = sqrt(sqnorm(cross(bb, n)));
     if (c < T(-1) + T(1e-8) || denominator < T(1e-8)) return false;
@@ -211,7 +234,8 @@ VIB_HD bool sphere_residual(V3<T> pc, V3<T> bb, const T* P, T denominator, bool 
     const T d = 1 - c;
     const bool series = c > 0 && s < T(1e-3);
     const T theta = series ? T(0) : atan2(s, c);
-    const T f = s < T(1e-8) ? T(1) : (series ? 1 + d / 3 + 2 * d * d / 15 + 2 * d * d * d / 35 : theta / s);
+    const T is = series ? T(0) : T(1) / s;  // only read where s >= 1e-3
+    const T f = s < T(1e-8) ? T(1) : (series ? 1 + d / 3 + 2 * d * d / 15 + 2 * d * d * d / 35 : theta * is);
     const V3<T> lg = f * u;
     e.r[0] = P[0] * lg.x + P[1] * lg.y + P[2] * lg.z;
     e.r[1] = P[3] * lg.x + P[4] * lg.y + P[5] * lg.z;
@@ -219,7 +243,7 @@ VIB_HD bool sphere_residual(V3<T> pc, V3<T> bb, const T* P, T denominator, bool 
     e.n = n; e.u = u; e.logv = lg; e.c = c; e.f = f; e.range = range;
     if (!want_derivative) return true;
     const T g = s < T(1e-8) ? T(0) : (series ? (T(-1) / T(3) - 4 * d / T(15) - 6 * d * d / T(35))
-                                            : (theta * c - s) / (s * s * s));
+                                            : (theta * c - s) * (is * is * is));
     e.g = g;
     for (int r = 0; r < 2; ++r) {
         const T* Pr = P + 3 * r;
@@ -228,9 +252,9 @@ VIB_HD bool sphere_residual(V3<T> pc, V3<T> bb, const T* P, T denominator, bool 
         const T k = g * Pu - f * Pb;
         const T pr0 = f * Pr[0] + k * bb.x, pr1 = f * Pr[1] + k * bb.y, pr2 = f * Pr[2] + k * bb.z;
         const T pn = pr0 * n.x + pr1 * n.y + pr2 * n.z;
-        e.reduce[3 * r + 0] = (pr0 - pn * n.x) / range;
-        e.reduce[3 * r + 1] = (pr1 - pn * n.y) / range;
-        e.reduce[3 * r + 2] = (pr2 - pn * n.z) / range;
+        e.reduce[3 * r + 0] = (pr0 - pn * n.x) * ir;
+        e.reduce[3 * r + 1] = (pr1 - pn * n.y) * ir;
+        e.reduce[3 * r + 2] = (pr2 - pn * n.z) * ir;
     }
     return true;
 }
