@@ -434,7 +434,6 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     __shared__ double acc[kAccSize];
     __shared__ FrameTable ft;
     __shared__ double rows[kWaves][kGramCols * kRowPitch];  // per wave: 2 rows per lane x 16 columns
-    __shared__ int keys[kWaves][64];
     __shared__ int nruns[2][kWaves];
     __shared__ double glist[kWaves][kGramsPerWave][kGramEntries];
     __shared__ int gkey[kWaves][kGramsPerWave];
@@ -528,7 +527,6 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
                 for (int q = 0; q < FL_PLANES; ++q) w.fac_lm[q * F + p] = fl[q];
             }
         }
-        keys[wave][lane] = key;
         wave_sync();
         // runs of one pair: Gram matrix on MFMA. A run occupies the lanes between its first and
         // last factor (pair order), so only the MFMA steps over its rows run; the rows of other
@@ -537,6 +535,11 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
         // lists, one thread per Gram entry and the lists in wave order: the entries of one
         // Gram matrix never share an accumulator element, so the scatter needs no atomics and
         // its sums do not depend on scheduling.
+        // the wave's staged rows as MFMA operands: step st covers rows 4 st .. 4 st + 3 (the
+        // factors of lanes 2 st and 2 st + 1); lane l supplies row 4 st + l / 16 (factor
+        // 2 st + l / 32), column l % 16
+        const int half = lane / 32;
+        const double* vcol = V + (lane % 16) * kRowPitch + lane / 16;
         unsigned long long pending = __ballot(key >= 0);
         const int prev = __shfl_up(key, 1, 64);
         const unsigned long long starts = __ballot(key >= 0 && (lane == 0 || prev != key));
@@ -552,12 +555,25 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
                 const int pk = __shfl(key, leader, 64);
                 const unsigned long long run = __ballot(key == pk);
                 const int last = 63 - __clzll(run);
+                // groups of four steps; run membership of a step's rows comes from the
+                // wave-uniform `run` mask, the exit is wave-uniform and the loads unconditional
+                // (index clamped to the wave's 32 steps), so the chain has no divergent branch
                 d4 G = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-                for (int st = leader / 2; st <= last / 2; ++st) {
-                    const int row = 4 * st + lane / 16;
-                    const double x = keys[wave][row >> 1] == pk ? V[(lane % 16) * kRowPitch + row] : 0.0;
-                    G = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, G, 0, 0, 0);
+                const int s0 = leader / 2, s1 = last / 2;
+#pragma unroll
+                for (int g = 0; g < 8; ++g) {
+                    const int st = s0 + 4 * g;
+                    if (st > s1) break;
+                    double x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int sc = min(st + u, 31);
+                        const double v = vcol[4 * sc];
+                        const bool in = (st + u <= s1) & (((run >> (2 * sc + half)) & 1ull) != 0ull);
+                        x[u] = in ? v : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) G = __builtin_amdgcn_mfma_f64_16x16x4f64(x[u], x[u], G, 0, 0, 0);
                 }
                 double* gl = glist[wave][cnt];
 #pragma unroll
