@@ -94,6 +94,20 @@ struct PriorBlockDev {
     int32_t pad;
 };
 
+// Device code sees the buffers of a window descriptor in the global address space: loads and
+// stores through them compile to global memory instructions, which the compiler can schedule
+// around LDS accesses (a generic pointer might address LDS). The host sees plain pointers; the
+// layout is the same. Host-only translation units define VIB_HOST_VIEW (their device pass then
+// sees the host's types).
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(VIB_HOST_VIEW)
+#define VIB_G __attribute__((address_space(1)))
+#else
+#define VIB_G
+#endif
+// The edit kernels fill buffers that the solve kernels only read.
+template <class T>
+__host__ __device__ inline VIB_G T* writable(VIB_G const T* p) { return const_cast<VIB_G T*>(p); }
+
 struct WinDev {
     int32_t K, C, L, F, E;
     int32_t nv, nr, ne;
@@ -107,67 +121,67 @@ struct WinDev {
     // chol_mask[k] bit i: tile (i, k) of the factor in that order may be nonzero (i >= k).
     uint32_t chol_mask[kMaxTiles];
     // state and trial candidate
-    double* pose; double* sb; double* rig; double* td; double* rho;
-    double* cpose; double* csb; double* crig; double* ctd; double* crho;
-    const double* cam;         // C x 12: R (9, row-major R_RC) then t (3)
+    VIB_G double* pose; VIB_G double* sb; VIB_G double* rig; VIB_G double* td; VIB_G double* rho;
+    VIB_G double* cpose; VIB_G double* csb; VIB_G double* crig; VIB_G double* ctd; VIB_G double* crho;
+    VIB_G const double* cam;         // C x 12: R (9, row-major R_RC) then t (3)
     // landmarks
-    const int32_t* lm_fc;      // anchor frame << 8 | camera
-    const int32_t* lm_off;     // L + 1 observation offsets
-    const int32_t* lm_col;     // rho column or -1
-    const double* lmd;         // LM_PLANES x L
+    VIB_G const int32_t* lm_fc;      // anchor frame << 8 | camera
+    VIB_G const int32_t* lm_off;     // L + 1 observation offsets
+    VIB_G const int32_t* lm_col;     // rho column or -1
+    VIB_G const double* lmd;         // LM_PLANES x L
     // observations (sorted by landmark)
-    const int32_t* ob_fc;      // observer frame << 8 | camera
-    const double* ob;          // OB_PLANES x F
+    VIB_G const int32_t* ob_fc;      // observer frame << 8 | camera
+    VIB_G const double* ob;          // OB_PLANES x F
     // observations again in (anchor frame, observer frame) pair order, for the visual normal
     // equations (runs of one pair reduce on MFMA)
-    const double* pf_ob;       // OB_PLANES x F
-    const double* pf_an;       // LM_PLANES x F : the factor's anchor bearing, velocity, t_ref
-    const int32_t* pf_ids;     // F : anchor frame << 24 | anchor cam << 16 | observer frame << 8 | cam
-    const int32_t* pf_lm;      // F : landmark
-    const int32_t* pf_pos;     // F : pair-order position of each landmark-order factor (the per-factor
+    VIB_G const double* pf_ob;       // OB_PLANES x F
+    VIB_G const double* pf_an;       // LM_PLANES x F : the factor's anchor bearing, velocity, t_ref
+    VIB_G const int32_t* pf_ids;     // F : anchor frame << 24 | anchor cam << 16 | observer frame << 8 | cam
+    VIB_G const int32_t* pf_lm;      // F : landmark
+    VIB_G const int32_t* pf_pos;     // F : pair-order position of each landmark-order factor (the per-factor
                                //     coupling planes fac_u / fac_lm are stored in pair order)
     // linearization outputs
-    double* fac_u;             // 6 x F : J_anchor^T J_rho per factor (weighted, landmark order)
-    double* fac_lm;            // FL_PLANES x F : per-factor landmark terms (landmark order)
-    double* lm_out;            // LO_PLANES x L
-    double* lm_delta;          // L (tangent step of rho; 0 for constant)
-    int64_t* lm_uid;           // L stable landmark ids (strictly increasing; kept through edits)
+    VIB_G double* fac_u;             // 6 x F : J_anchor^T J_rho per factor (weighted, landmark order)
+    VIB_G double* fac_lm;            // FL_PLANES x F : per-factor landmark terms (landmark order)
+    VIB_G double* lm_out;            // LO_PLANES x L
+    VIB_G double* lm_delta;          // L (tangent step of rho; 0 for constant)
+    VIB_G int64_t* lm_uid;           // L stable landmark ids (strictly increasing; kept through edits)
     // IMU
-    const int32_t* imu_from; const int32_t* imu_to; const double* imu;
+    VIB_G const int32_t* imu_from; VIB_G const int32_t* imu_to; VIB_G const double* imu;
     // prior
     int32_t prior_rows, prior_cols, prior_blocks, pad0;
-    const PriorBlockDev* pblk;
-    const double* pA;          // rows x cols
-    const double* pr0;
-    const double* pG;          // cols x cols = A^T A
-    const double* pref;
-    const int32_t* pfree;      // free prior columns: block << 8 | index within the block
+    VIB_G const PriorBlockDev* pblk;
+    VIB_G const double* pA;          // rows x cols
+    VIB_G const double* pr0;
+    VIB_G const double* pG;          // cols x cols = A^T A
+    VIB_G const double* pref;
+    VIB_G const int32_t* pfree;      // free prior columns: block << 8 | index within the block
     int32_t prior_free, pad2;
     // retained-factor linearization (written by k_retained, assembled row-parallel)
-    double* imu_J;             // E x 15 x 30 whitened Jacobians [pose0 | sb0 | pose1 | sb1]
-    double* imu_r;             // E x 15 whitened residuals
-    double* pD;                // prior_blocks x 36 chart derivatives (6 x 6, row-major)
-    double* patr;              // prior_cols : A^T r
-    double* ret_sc;            // 4 E + 4 : IMU costs (lin, trial), invalid (lin, trial); prior same
-    const int32_t* colmap;     // nr x 4 : IMU (edge << 8 | local) x2, prior (block << 8 | index), -1
+    VIB_G double* imu_J;             // E x 15 x 30 whitened Jacobians [pose0 | sb0 | pose1 | sb1]
+    VIB_G double* imu_r;             // E x 15 whitened residuals
+    VIB_G double* pD;                // prior_blocks x 36 chart derivatives (6 x 6, row-major)
+    VIB_G double* patr;              // prior_cols : A^T r
+    VIB_G double* ret_sc;            // 4 E + 4 : IMU costs (lin, trial), invalid (lin, trial); prior same
+    VIB_G const int32_t* colmap;     // nr x 4 : IMU (edge << 8 | local) x2, prior (block << 8 | index), -1
     // dense per-window buffers
-    const double* hostH;       // nr x nr normal matrix of host-evaluated factors, or null
-    const double* hostG;       // nr gradient of host-evaluated factors, or null
-    double* Hfull;             // nr x nr undamped full normal matrix of the states
-    double* gfull;             // nr
-    double* fac;               // factor_doubles(nr): tile store of the damped reduced system,
+    VIB_G const double* hostH;       // nr x nr normal matrix of host-evaluated factors, or null
+    VIB_G const double* hostG;       // nr gradient of host-evaluated factors, or null
+    VIB_G double* Hfull;             // nr x nr undamped full normal matrix of the states
+    VIB_G double* gfull;             // nr
+    VIB_G double* fac;               // factor_doubles(nr): tile store of the damped reduced system,
                                 // overwritten by its lower Cholesky factor
-    double* Scopy;             // nr x nr damped reduced system (row-major), kept for refinement
-    double* rhs;               // nr
-    double* dstate;            // nr state step
-    double* scratch;           // 4 * nr
+    VIB_G double* Scopy;             // nr x nr damped reduced system (row-major), kept for refinement
+    VIB_G double* rhs;               // nr
+    VIB_G double* dstate;            // nr state step
+    VIB_G double* scratch;           // 4 * nr
     // partials
-    double* lin_part;          // nlin x (tri + nv) (one per factor range of the visual kernel)
-    double* lin_sc;            // nlin x CS_PLANES
-    double* schur_part;        // nchunks x (tri + nv)
-    double* chunk_sc;          // nchunks x CS_PLANES
+    VIB_G double* lin_part;          // nlin x (tri + nv) (one per factor range of the visual kernel)
+    VIB_G double* lin_sc;            // nlin x CS_PLANES
+    VIB_G double* schur_part;        // nchunks x (tri + nv)
+    VIB_G double* chunk_sc;          // nchunks x CS_PLANES
     int32_t nlin, nchunks;
-    LmState* lm;               // persistent per-window LM state (device)
+    VIB_G LmState* lm;               // persistent per-window LM state (device)
 };
 
 struct Chunk {
@@ -180,11 +194,11 @@ constexpr int kLinFactors = 512;  // factors per visual-normal workgroup (pair o
 
 // One symmetric eigendecomposition of a batched k_sym_eigen launch (one workgroup per job).
 struct EigenJob {
-    double* A;     // n x n (leading dimension ld), overwritten
-    double* V;     // eigenvectors as rows
-    double* ev;    // eigenvalues (unsorted)
-    double* rot;   // rotation log (eigen_log_doubles(n))
-    int32_t* ok;   // 1: converged
+    VIB_G double* A;     // n x n (leading dimension ld), overwritten
+    VIB_G double* V;     // eigenvectors as rows
+    VIB_G double* ev;    // eigenvalues (unsorted)
+    VIB_G double* rot;   // rotation log (eigen_log_doubles(n))
+    VIB_G int32_t* ok;   // 1: converged
     int32_t n, ld;
 };
 

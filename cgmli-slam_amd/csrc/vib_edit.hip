@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kEditBlock) void k_edit_landmarks(const WinDev* src
     const WinDev& s = *srcp;
     const WinDev& d = *dstp;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l == 0) const_cast<int32_t*>(d.lm_off)[d.L] = (int32_t)cscan[s.L + sp.ins_lm];
+    if (l == 0) writable(d.lm_off)[d.L] = (int32_t)cscan[s.L + sp.ins_lm];
     if (l >= s.L) return;
     const unsigned long long a = lscan[l], b = lscan[l + 1];
     if ((uint32_t)b == (uint32_t)a) return;  // removed
@@ -174,11 +174,11 @@ __global__ __launch_bounds__(kEditBlock) void k_edit_landmarks(const WinDev* src
         for (int q = 0; q < LM_PLANES; ++q) pl[q] = s.lmd[q * Ls + l];
         rho = s.rho[l];
     }
-    const_cast<int32_t*>(d.lm_fc)[ln] = (sp.renum[frame] << 8) | cam;
-    const_cast<int32_t*>(d.lm_col)[ln] = fr ? sp.nr + rank : -1;
-    const_cast<int32_t*>(d.lm_off)[ln] = (int32_t)cscan[l];
+    writable(d.lm_fc)[ln] = (sp.renum[frame] << 8) | cam;
+    writable(d.lm_col)[ln] = fr ? sp.nr + rank : -1;
+    writable(d.lm_off)[ln] = (int32_t)cscan[l];
 #pragma unroll
-    for (int q = 0; q < LM_PLANES; ++q) const_cast<double*>(d.lmd)[q * Ld + ln] = pl[q];
+    for (int q = 0; q < LM_PLANES; ++q) writable(d.lmd)[q * Ld + ln] = pl[q];
     d.rho[ln] = rho;
     d.crho[ln] = rho;
     d.lm_uid[ln] = s.lm_uid[l];
@@ -194,11 +194,11 @@ __global__ __launch_bounds__(kEditBlock) void k_insert_landmarks(int L, const Wi
     const int ln = (int)(uint32_t)a, rank = (int)(a >> 32);
     const bool fr = (b >> 32) != (a >> 32);
     const size_t Ld = (size_t)d.L, Ni = (size_t)sp.ins_lm;
-    const_cast<int32_t*>(d.lm_fc)[ln] = sp.ins_lm_fc[j];
-    const_cast<int32_t*>(d.lm_col)[ln] = fr ? sp.nr + rank : -1;
-    const_cast<int32_t*>(d.lm_off)[ln] = (int32_t)cscan[L + j];
+    writable(d.lm_fc)[ln] = sp.ins_lm_fc[j];
+    writable(d.lm_col)[ln] = fr ? sp.nr + rank : -1;
+    writable(d.lm_off)[ln] = (int32_t)cscan[L + j];
 #pragma unroll
-    for (int q = 0; q < LM_PLANES; ++q) const_cast<double*>(d.lmd)[q * Ld + ln] = sp.ins_lmd[q * Ni + j];
+    for (int q = 0; q < LM_PLANES; ++q) writable(d.lmd)[q * Ld + ln] = sp.ins_lmd[q * Ni + j];
     d.rho[ln] = sp.ins_rho[j];
     d.crho[ln] = sp.ins_rho[j];
     d.lm_uid[ln] = sp.ins_uid[j];
@@ -219,9 +219,9 @@ __global__ __launch_bounds__(kEditBlock) void k_edit_factors(const WinDev* srcp,
     const uint32_t fn = cscan[l] + (fscan[f] - fscan[s.lm_off[l]]);
     const size_t Fs = (size_t)s.F, Fd = (size_t)d.F;
     const int fc = s.ob_fc[f];
-    const_cast<int32_t*>(d.ob_fc)[fn] = (sp.renum[fc >> 8] << 8) | (fc & 255);
+    writable(d.ob_fc)[fn] = (sp.renum[fc >> 8] << 8) | (fc & 255);
 #pragma unroll
-    for (int q = 0; q < OB_PLANES; ++q) const_cast<double*>(d.ob)[q * Fd + fn] = s.ob[q * Fs + f];
+    for (int q = 0; q < OB_PLANES; ++q) writable(d.ob)[q * Fd + fn] = s.ob[q * Fs + f];
     lmn[fn] = (int32_t)(uint32_t)lscan[l];
 }
 
@@ -238,9 +238,9 @@ __global__ __launch_bounds__(kEditBlock) void k_insert_factors(const WinDev* src
     const uint32_t kept = l < s.L ? fscan[s.lm_off[l + 1]] - fscan[s.lm_off[l]] : 0u;
     const uint32_t fn = cscan[l] + kept + (uint32_t)sp.ins_obs_rank[i];
     const size_t Fd = (size_t)d.F, Ni = (size_t)sp.ins_obs;
-    const_cast<int32_t*>(d.ob_fc)[fn] = sp.ins_obs_fc[i];
+    writable(d.ob_fc)[fn] = sp.ins_obs_fc[i];
 #pragma unroll
-    for (int q = 0; q < OB_PLANES; ++q) const_cast<double*>(d.ob)[q * Fd + fn] = sp.ins_ob[q * Ni + i];
+    for (int q = 0; q < OB_PLANES; ++q) writable(d.ob)[q * Fd + fn] = sp.ins_ob[q * Ni + i];
     lmn[fn] = (int32_t)(uint32_t)lscan[l];
 }
 
@@ -335,13 +335,13 @@ __global__ __launch_bounds__(kSortBlock) void k_pair_scatter(const WinDev* dstp,
     const size_t F = (size_t)d.F, L = (size_t)d.L;
     const int l = lmn[f];
 #pragma unroll
-    for (int q = 0; q < OB_PLANES; ++q) const_cast<double*>(d.pf_ob)[q * F + p] = d.ob[q * F + f];
+    for (int q = 0; q < OB_PLANES; ++q) writable(d.pf_ob)[q * F + p] = d.ob[q * F + f];
 #pragma unroll
-    for (int q = 0; q < LM_PLANES; ++q) const_cast<double*>(d.pf_an)[q * F + p] = d.lmd[q * L + l];
+    for (int q = 0; q < LM_PLANES; ++q) writable(d.pf_an)[q * F + p] = d.lmd[q * L + l];
     const int a = d.lm_fc[l];
-    const_cast<int32_t*>(d.pf_ids)[p] = ((a >> 8) << 24) | ((a & 255) << 16) | d.ob_fc[f];
-    const_cast<int32_t*>(d.pf_lm)[p] = l;
-    const_cast<int32_t*>(d.pf_pos)[f] = (int32_t)p;
+    writable(d.pf_ids)[p] = ((a >> 8) << 24) | ((a & 255) << 16) | d.ob_fc[f];
+    writable(d.pf_lm)[p] = l;
+    writable(d.pf_pos)[f] = (int32_t)p;
 }
 
 // ---------------------------------------------------------------------------------------------

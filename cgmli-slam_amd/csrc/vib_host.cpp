@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/vib.h"
+#define VIB_HOST_VIEW 1  // host code only: the descriptor's pointers stay plain in both passes
 #include "vib_device.hpp"
 #include "vib_host.hpp"
 #include "vib_math.hpp"

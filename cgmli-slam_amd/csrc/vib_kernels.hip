@@ -1309,8 +1309,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
     if (kTile * ti >= nr) return;
     const int row0 = kTile * ti + kAssembleRows * q;
     const int row1 = min(nr, row0 + kAssembleRows);
-    const double* rl = reduced[2 * blockIdx.x + 0];
-    const double* rs = reduced[2 * blockIdx.x + 1];
+    const VIB_G double* rl = (const VIB_G double*)reduced[2 * blockIdx.x + 0];
+    const VIB_G double* rs = (const VIB_G double*)reduced[2 * blockIdx.x + 1];
     const int t = threadIdx.x, nt = blockDim.x;
     const bool fresh = w.lm->new_point;
     if (fresh) {
@@ -2233,17 +2233,26 @@ constexpr int kEigThreads = 1024;
 constexpr int kEigLogSweeps = 64;  // QR sweeps logged per round of the rotation pass
 // Doubles of the rotation log of an n x n problem (k_sym_eigen's `rot`).
 __host__ __device__ inline size_t eigen_log_doubles(int n) { return 2 * (size_t)kEigLogSweeps * n + kEigLogSweeps; }
+typedef __attribute__((address_space(3))) double ldouble;  // LDS
+typedef __attribute__((address_space(3))) int lint;
+// the matrix, then the eigenvectors, in LDS (NL > 0) or in global memory
+template <int NL> struct EigenStore { typedef ldouble type; };
+template <> struct EigenStore<0> { typedef VIB_G double type; };
 template <int NL>
-__device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, int* ok, double* rot, double* lds,
-                               double* d, double* e, double* q, int& lo, int& hi, int& state) {
+__device__ void sym_eigen_body(VIB_G double* A, VIB_G double* V, VIB_G double* ev, int n, int ld, VIB_G int* ok,
+                               VIB_G double* rot, ldouble* lds, ldouble* d, ldouble* e, ldouble* q, lint& lo, lint& hi,
+                               lint& state) {
     const int t = threadIdx.x, nt = blockDim.x, wave = t / 64, lane = t % 64, nw = nt / 64;
-    double* Ap = NL ? lds : A;
+    typedef typename EigenStore<NL>::type ET;
+    ET* Ap;
+    if constexpr (NL > 0) Ap = lds;
+    else Ap = A;
     const int lda = NL ? n : ld;
     if (NL) {
         for (int p = t; p < n * n; p += nt) Ap[p] = A[(size_t)(p / n) * ld + p % n];
         __syncthreads();
     }
-    auto a = [&](int r, int c) -> double& { return Ap[(size_t)r * lda + c]; };
+    auto a = [&](int r, int c) -> ET& { return Ap[(size_t)r * lda + c]; };
     // 1. tridiagonalization. Every wave forms the reflector's scalars itself from column k (no
     //    barrier); v_i = (a(k+1+i, k) - alpha [i == 0]) / |x - alpha e1| is read from column k on
     //    the fly. Row k's upper part (dead after step k) receives v for phase 2.
@@ -2294,7 +2303,9 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
         }
         __syncthreads();
     }
-    double* Vp = NL ? lds : V;
+    ET* Vp;
+    if constexpr (NL > 0) Vp = lds;
+    else Vp = V;
     const int ldv = NL ? n : ld;
     // 2. Q^T = H_{n-3} ... H_0 in V: start from I, apply H_k on the right for k = n-3 .. 0
     //    (Q^T <- Q^T H_k acts on columns k+1 ..; H_0 last). Before H_k the product acts on rows
@@ -2304,9 +2315,9 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
     __syncthreads();
     for (int k = n - 3; k >= 0; --k) {
         const int m = n - k - 1;
-        const double* vk = A + (size_t)k * ld + k + 1;  // the reflector of step k
+        const VIB_G double* vk = A + (size_t)k * ld + k + 1;  // the reflector of step k
         for (int rr = wave; rr < m; rr += nw) {
-            const double* row = Vp + (size_t)(k + 1 + rr) * ldv + k + 1;
+            const ET* row = Vp + (size_t)(k + 1 + rr) * ldv + k + 1;
             double s = 0;
             for (int i = lane; i < m; i += 64) s += row[i] * vk[i];
             s = wave_sum_dpp(s);
@@ -2324,7 +2335,7 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
     //    kEigLogSweeps sweeps on (d, e) alone, logging each sweep's block [l, m] and rotations in
     //    `rot`; then every thread applies the logged rotations, in order, to its column of V^T.
     //    Repeated until every off-diagonal has deflated (at most 30 n sweeps).
-    int2* span = reinterpret_cast<int2*>(rot + 2 * (size_t)kEigLogSweeps * n);
+    VIB_G int2* span = reinterpret_cast<VIB_G int2*>(rot + 2 * (size_t)kEigLogSweeps * n);
     if (t == 0) { state = 0; hi = n - 1; lo = 0; }
     __syncthreads();
     for (int done = 0; done < 30 * n;) {
@@ -2368,7 +2379,7 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
                     // the bulge chase; the running diagonal and off-diagonal entries stay in
                     // registers (ak = d[k], bk = e[k] as updated by the previous rotation), so a
                     // step loads only the untouched d[k+1], e[k+1] (e[m] is read and ignored)
-                    double* cs = rot + 2 * (size_t)count * n;
+                    VIB_G double* cs = rot + 2 * (size_t)count * n;
                     double ak = d[l], bk = e[l];
                     double x = ak - mu, z = bk;
                     for (int k = l; k < m; ++k) {
@@ -2407,7 +2418,7 @@ __device__ void sym_eigen_body(double* A, double* V, double* ev, int n, int ld, 
         for (int r = t; r < n; r += nt)
             for (int sw = 0; sw < count; ++sw) {
                 const int2 lm = span[sw];
-                const double* cs = rot + 2 * (size_t)sw * n;
+                const VIB_G double* cs = rot + 2 * (size_t)sw * n;
                 double x = Vp[(size_t)lm.x * ldv + r];
                 int k = lm.x;
                 // groups of 8 rotations: rows k+1 .. k+8 still hold their values from before
@@ -2449,7 +2460,8 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eigen(const EigenJob* jobs)
     __shared__ double d[kEigMax], e[kEigMax], q[kEigMax];
     __shared__ int lo, hi, state;
     const EigenJob j = jobs[blockIdx.x];
-    sym_eigen_body<0>(j.A, j.V, j.ev, j.n, j.ld, j.ok, j.rot, nullptr, d, e, q, lo, hi, state);
+    sym_eigen_body<0>(j.A, j.V, j.ev, j.n, j.ld, j.ok, j.rot, nullptr, (ldouble*)d, (ldouble*)e, (ldouble*)q,
+                      *(lint*)&lo, *(lint*)&hi, *(lint*)&state);
 }
 
 // Matrices of up to kEigLds rows (every prior of a window of at most 20 frames: 19 poses, one
@@ -2461,7 +2473,8 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eigen_lds(const EigenJob* j
     __shared__ double d[kEigLds], e[kEigLds], q[kEigLds];
     __shared__ int lo, hi, state;
     const EigenJob j = jobs[blockIdx.x];
-    sym_eigen_body<kEigLds>(j.A, j.V, j.ev, j.n, j.ld, j.ok, j.rot, lds, d, e, q, lo, hi, state);
+    sym_eigen_body<kEigLds>(j.A, j.V, j.ev, j.n, j.ld, j.ok, j.rot, (ldouble*)lds, (ldouble*)d, (ldouble*)e,
+                            (ldouble*)q, *(lint*)&lo, *(lint*)&hi, *(lint*)&state);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2784,8 +2797,8 @@ __global__ __launch_bounds__(kBlock) void k_nis_gate(const WinDev* wins, const C
 __global__ __launch_bounds__(kBlock) void k_nis_apply(const WinDev* wins, const uint8_t* decision) {
     const WinDev& w = wins[0];
     const size_t F = (size_t)w.F;
-    double* ob = const_cast<double*>(w.ob);
-    double* pf = const_cast<double*>(w.pf_ob);
+    double* ob = writable(w.ob);
+    double* pf = writable(w.pf_ob);
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < w.F; f += gridDim.x * blockDim.x) {
         if (decision[f] != 1) continue;
         const int p = w.pf_pos[f];

@@ -57,7 +57,8 @@ VIB_HD void sin_cos(double a, double& s, double& c) { sincos(a, &s, &c); }
 VIB_HD void sin_cos(float a, float& s, float& c) { sincosf(a, &s, &c); }
 
 template <class T> VIB_HD V3<T> v3(T x, T y, T z) { return V3<T>{x, y, z}; }
-template <class T> VIB_HD V3<T> load3(const T* p) { return V3<T>{p[0], p[1], p[2]}; }
+// (any pointer: decltype(+p[0]) is the element type without qualifiers or address space)
+template <class P> VIB_HD auto load3(P p) -> V3<decltype(+p[0])> { return {p[0], p[1], p[2]}; }
 template <class T> VIB_HD V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 template <class T> VIB_HD V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 template <class T> VIB_HD V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
@@ -85,8 +86,8 @@ template <class T> VIB_HD M3<T> mul(const M3<T>& A, const M3<T>& B) {
 
 // Eigen convention xyzw quaternion -> rotation matrix of the normalized quaternion
 // (QuaternionBase::toRotationMatrix on q.normalized()).
-template <class T>
-VIB_HD M3<T> quat_to_matrix(const T* q) {
+template <class P, class T = decltype(+(*(P)nullptr))>
+VIB_HD M3<T> quat_to_matrix(P q) {
     const T n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     const T x = q[0] / n, y = q[1] / n, z = q[2] / n, w = q[3] / n;
     const T tx = 2 * x, ty = 2 * y, tz = 2 * z;
@@ -101,7 +102,7 @@ VIB_HD M3<T> quat_to_matrix(const T* q) {
 
 // ---- quaternion algebra (Hamilton, storage x y z w) -------------------------------------
 template <class T> struct Qt { T x, y, z, w; };
-template <class T> VIB_HD Qt<T> qload(const T* p) { return {p[0], p[1], p[2], p[3]}; }
+template <class P> VIB_HD auto qload(P p) -> Qt<decltype(+p[0])> { return {p[0], p[1], p[2], p[3]}; }
 template <class T> VIB_HD Qt<T> qmul(Qt<T> a, Qt<T> b) {
     return {a.w * b.x + b.w * a.x + a.y * b.z - a.z * b.y, a.w * b.y + b.w * a.y + a.z * b.x - a.x * b.z,
             a.w * b.z + b.w * a.z + a.x * b.y - a.y * b.x, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
@@ -144,8 +145,8 @@ VIB_HD V3<T> rotation_jacobian_mul(V3<T> a, V3<T> v) {
     const V3<T> wv = cross(a, v);
     return v + c1 * wv + c2 * cross(a, wv);
 }
-template <class T>
-VIB_HD void pose_plus(const T* x, const T* d, T* y) {
+template <class PX, class PD, class PY, class T = decltype(+(*(PX)nullptr))>
+VIB_HD void pose_plus(PX x, PD d, PY y) {
     const V3<T> angle = load3(d + 3);
     const Qt<T> q = rotation_exp(angle);
     const V3<T> p = qrotate(q, load3(x)) + rotation_jacobian_mul(angle, load3(d));
@@ -153,8 +154,8 @@ VIB_HD void pose_plus(const T* x, const T* d, T* y) {
     y[0] = p.x; y[1] = p.y; y[2] = p.z;
     y[3] = yq.x; y[4] = yq.y; y[5] = yq.z; y[6] = yq.w;
 }
-template <class T>
-VIB_HD void pose_minus(const T* x, const T* ref, T* d) {
+template <class PX, class PR, class PD, class T = decltype(+(*(PX)nullptr))>
+VIB_HD void pose_minus(PX x, PR ref, PD d) {
     Qt<T> q = qnormalized(qmul(qload(x + 3), qconj(qload(ref + 3))));
     if (q.w < 0) q = {-q.x, -q.y, -q.z, -q.w};
     const T s = sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
