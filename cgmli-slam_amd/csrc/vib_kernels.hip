@@ -2558,39 +2558,45 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_trial_cost: robust visual cost at the trial state.
+// k_trial_cost: robust visual cost at the trial state (VisualCudaBatch.cu:715-741), one factor
+// per lane over the pair-ordered visual work list: every lane evaluates a factor, whatever the
+// landmarks' observation counts (in landmark order, 8 lanes per landmark idle where a landmark
+// has fewer observations). Per-range partials go to lin_sc; k_lm_update sums them in list order.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const Chunk* chunks) {
+__global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const Chunk* lin) {
     __shared__ FrameTable ft;
     __shared__ double red[kBlock];
-    const Chunk ch = chunks[blockIdx.x];
+    const Chunk ch = lin[blockIdx.x];
+    if (ch.local < 0) return;  // an empty position of the XCD-ordered work list
     const WinDev& w = wins[ch.win];
     if (!window_active(w)) return;
     build_frame_table(w, w.cpose, w.crig, ft);
     __syncthreads();
     const double td = *w.ctd;
+    const size_t F = (size_t)w.F;
     double cost = 0;
     int invalid = 0;
-    const int lane = threadIdx.x % kChunkLanes, group = threadIdx.x / kChunkLanes;
-    constexpr int groups = kBlock / kChunkLanes;
-    for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
-        const int l = base + group;
-        if (l >= ch.lm_end) continue;
-        const LmData lm = load_landmark(w, w.crho, l);
-        for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
-            FactorJ J;
-            if (!eval_factor<false>(w, ft, lm, w.ob, (size_t)w.F, f, w.ob_fc[f], td, false, J)) { invalid = 1; continue; }
-            const double sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
-            if (!isfinite(sq)) { invalid = 1; continue; }
-            cost += huber_cost(w.huber, sq);
-        }
+    for (int p = ch.lm_begin + (int)threadIdx.x; p < ch.lm_end; p += kBlock) {
+        const int ids = w.pf_ids[p];
+        LmData lm;
+        lm.raw = {w.pf_an[LM_BX * F + p], w.pf_an[LM_BY * F + p], w.pf_an[LM_BZ * F + p]};
+        lm.vel = {w.pf_an[LM_VX * F + p], w.pf_an[LM_VY * F + p], w.pf_an[LM_VZ * F + p]};
+        lm.tref = w.pf_an[LM_T * F + p];
+        lm.rho = w.crho[w.pf_lm[p]];
+        lm.frame = ids >> 24;
+        lm.cam = (ids >> 16) & 255;
+        FactorJ J;
+        if (!eval_factor<false>(w, ft, lm, w.pf_ob, F, p, ids & 0xffff, td, false, J)) { invalid = 1; continue; }
+        const double sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
+        if (!isfinite(sq)) { invalid = 1; continue; }
+        cost += huber_cost(w.huber, sq);
     }
     const double c = block_sum(cost, red);
     const double b = block_max(invalid ? 1.0 : 0.0, red);
     if (threadIdx.x == 0) {
-        double* sc = w.chunk_sc + (size_t)ch.local * CS_PLANES;
+        double* sc = w.lin_sc + (size_t)ch.local * CS_PLANES;
         sc[CS_TRIAL] = c;
-        sc[CS_TINVALID] = fmax(sc[CS_TINVALID], b);
+        sc[CS_TINVALID] = b;
     }
 }
 
@@ -2617,13 +2623,17 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
             trial += w.ret_sc[E + e];
             tinv = fmax(tinv, w.ret_sc[3 * E + e]);
         }
-        for (int k = t; k < w.nchunks; k += blockDim.x) {
+        for (int k = t; k < w.nchunks; k += blockDim.x) {  // back-substitution (landmark chunks)
             const double* sc = w.chunk_sc + (size_t)k * CS_PLANES;
             pred += sc[CS_PRED];
-            trial += sc[CS_TRIAL];
             mx = fmax(mx, sc[CS_MAXINC]);
             tinv = fmax(tinv, sc[CS_TINVALID]);
             nonfinite = fmax(nonfinite, sc[CS_NONFINITE]);
+        }
+        for (int k = t; k < w.nlin; k += blockDim.x) {  // trial cost (visual factor ranges)
+            const double* sc = w.lin_sc + (size_t)k * CS_PLANES;
+            trial += sc[CS_TRIAL];
+            tinv = fmax(tinv, sc[CS_TINVALID]);
         }
         pred = block_sum(pred, red);
         nonfinite = block_max(nonfinite, red);
@@ -3553,7 +3563,7 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             }
             break;
         case 8:
-            if (nchunks) hipLaunchKernelGGL(k_trial_cost, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
+            if (nlin) hipLaunchKernelGGL(k_trial_cost, dim3(nlin), dim3(kBlock), 0, s, wins, lin);
             break;
         case 9: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 1); break;
         case 10: hipLaunchKernelGGL(k_lm_update, dim3(nwin), dim3(kBlock), 0, s, wins, mode); break;
