@@ -162,7 +162,7 @@ def _session_stream(args):
     return synth.stream(K - 1, K + n, max(1, cfg["L"] // K), n_cams=4, obs_range=(2, 10), seed=cfg["seed"])
 
 
-def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb, precision, refinement):
+def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb, precision, refinement, workers=1):
     """Serving workload: n_sessions sliding-window sessions; Poisson requests at `rate` per second
     for `seconds`; each request appends its session's next keyframe, runs 1-3 LM iterations
     batched with the other running requests of its size bucket and slides OLD back to the window
@@ -180,7 +180,7 @@ def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb, precisio
     kfs = [k for _, k in streams]
     def serve_once(schedule):
         srv = serve.Server(ctx, wins, int(budget_gb * (1 << 30)), precision=precision, refinement=refinement,
-                           full_iterations=True)
+                           full_iterations=True, workers=workers)
         try:
             srv.preload()
             ctx.synchronize()
@@ -198,7 +198,7 @@ def serving_line(ctx, abi, n_sessions, rate, seconds, procs, budget_gb, precisio
     out["sustained"] = dict(arrival_rate=SERVE_SUSTAINED_RATE, offered_seconds=2.0, requests=sustained["requests"],
                             requests_per_s=sustained["requests_per_s"], steps_per_s=sustained["steps_per_s"],
                             p50_ms=sustained["p50_ms"], p99_ms=sustained["p99_ms"], failed=sustained["failed"])
-    out.update(sessions=n_sessions, arrival_rate=rate, offered_seconds=seconds, hbm_budget_gb=budget_gb,
+    out.update(sessions=n_sessions, workers=workers, arrival_rate=rate, offered_seconds=seconds, hbm_budget_gb=budget_gb,
                uploads=st["uploads"], evictions=st["evictions"], launches=st["launches"], slides=st["slides"],
                slides_per_s=st["slides"] / wall, appends=st["appends"],
                phase_seconds=dict(admit=st["admit_seconds"], iterate=st["iterate_seconds"], slide=st["slide_seconds"]),
@@ -465,6 +465,8 @@ def main():
                     help="seconds of Poisson arrivals of the serving line (0: skip)")
     ap.add_argument("--serve-sessions", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-workers", type=int, default=4,
+                    help="server workers (contexts serving disjoint sessions on threads of their own)")
     ap.add_argument("--preint-intervals", type=int, default=4096,
                     help="keyframe edges of the device preintegration line (0: skip)")
     ap.add_argument("--whiten-observations", type=int, default=1 << 22,
@@ -487,7 +489,7 @@ def main():
     if args.serving_only:
         ctx = abi.Context(0)
         serving = serving_line(ctx, abi, args.serve_sessions, args.serve_rate, args.serve_seconds, args.batch_procs,
-                               64.0, abi.VIB_MIXED, MIXED_REFINEMENT)
+                               64.0, abi.VIB_MIXED, MIXED_REFINEMENT, args.serve_workers)
         ctx.close()
         print(json.dumps(dict(serving=serving)))
         return
@@ -541,7 +543,7 @@ def main():
         print(f"[bench] serving: {args.serve_sessions} sessions, {args.serve_rate:g} requests/s", file=sys.stderr,
               flush=True)
         serving = serving_line(ctx, abi, args.serve_sessions, args.serve_rate, args.serve_seconds, args.batch_procs,
-                               64.0, abi.VIB_MIXED, MIXED_REFINEMENT)
+                               64.0, abi.VIB_MIXED, MIXED_REFINEMENT, args.serve_workers)
     preint = None
     if args.preint_intervals > 0:
         print(f"[bench] preintegration: {args.preint_intervals} intervals", file=sys.stderr, flush=True)

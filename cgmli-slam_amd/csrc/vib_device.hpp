@@ -202,6 +202,12 @@ struct EigenJob {
     int32_t n, ld;
 };
 
+// One copy of a batched transfer (k_copy_segments): `bytes` from src to dst, both device
+// addresses; src 0 fills dst with zeros.
+struct CopySeg {
+    uint64_t dst, src, bytes;
+};
+
 // One window edit (vib_edit.hip): which landmarks and observations of a resident window the
 // edited window keeps, and how its frames are renumbered.
 struct EditSpec {

@@ -12,6 +12,7 @@
 // oldest frame are eliminated).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "vib_device.hpp"
@@ -460,7 +461,44 @@ __global__ __launch_bounds__(kEditBlock) void k_two_frame_select(const WinDev* s
 // ---------------------------------------------------------------------------------------------
 // Launch wrappers used by the host driver.
 // ---------------------------------------------------------------------------------------------
+namespace vib {
+
+// ---------------------------------------------------------------------------------------------
+// Batched copies: workgroup (x, y) copies slice y of segment x, in 4-byte words when both ends
+// are 4-byte aligned (the tail bytes by the first threads of slice 0), byte by byte otherwise.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kEditBlock) void k_copy_segments(const CopySeg* segs, int n) {
+    if ((int)blockIdx.x >= n) return;
+    const CopySeg g = segs[blockIdx.x];
+    unsigned char* d = reinterpret_cast<unsigned char*>(g.dst);
+    const unsigned char* s = reinterpret_cast<const unsigned char*>(g.src);
+    const uint64_t first = (uint64_t)blockIdx.y * kEditBlock + threadIdx.x, step = (uint64_t)gridDim.y * kEditBlock;
+    if (((g.dst | g.src) & 3) == 0) {
+        const uint64_t words = g.bytes >> 2;
+        uint32_t* dw = reinterpret_cast<uint32_t*>(d);
+        const uint32_t* sw = reinterpret_cast<const uint32_t*>(s);
+        for (uint64_t i = first; i < words; i += step) dw[i] = s ? sw[i] : 0u;
+        if (blockIdx.y == 0 && threadIdx.x < (g.bytes & 3)) {
+            const uint64_t i = 4 * words + threadIdx.x;
+            d[i] = s ? s[i] : 0;
+        }
+    } else {
+        for (uint64_t i = first; i < g.bytes; i += step) d[i] = s ? s[i] : 0;
+    }
+}
+
+}  // namespace vib
+
 extern "C" {
+
+hipError_t vibk_copy_segments(const vib::CopySeg* segs, int n, size_t max_bytes, hipStream_t s) {
+    using namespace vib;
+    if (n == 0) return hipSuccess;
+    const size_t per = (size_t)4 * kEditBlock;  // bytes of one slice pass
+    const int slices = (int)std::min<size_t>(32, std::max<size_t>(1, (max_bytes + per - 1) / per));
+    hipLaunchKernelGGL(k_copy_segments, dim3(n, slices), dim3(kEditBlock), 0, s, segs, n);
+    return hipGetLastError();
+}
 
 int vibk_edit_scan_tiles(int n) { return (n + vib::kScanTile - 1) / vib::kScanTile; }
 int vibk_edit_sort_blocks(int n) { return (n + vib::kSortBlock - 1) / vib::kSortBlock; }

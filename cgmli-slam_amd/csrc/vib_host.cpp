@@ -67,6 +67,7 @@ hipError_t vibk_nis_apply(const vib::WinDev* wins, const uint8_t* decision, int 
 hipError_t vibk_preintegrate(int count, const double* smp, const int* begin, const int* end, const double* t0,
                              const double* t1, const double* sbl, const vib_imu_params* prm, const double* g0,
                              const double* g1, double* out, int* status, hipStream_t s);
+hipError_t vibk_copy_segments(const vib::CopySeg* segs, int n, size_t max_bytes, hipStream_t s);
 hipError_t vibk_sym_eigen_batch(const vib::EigenJob* jobs, int count, int max_n, hipStream_t s);
 int vibk_sym_eigen_max(void);
 size_t vibk_sym_eigen_log(int n);
@@ -378,6 +379,96 @@ bool HostWindow::layout_states(std::string& err) {
 // ------------------------------------------------------------------------------------------
 // Slot: device arena holding one window.
 // ------------------------------------------------------------------------------------------
+Staging::~Staging() {
+    if (buf) (void)hipFree(buf);
+    for (hipEvent_t e : done)
+        if (e) (void)hipEventDestroy(e);
+}
+
+void CopyBatch::up(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    const size_t off = (pay_.size() + 7) & ~size_t(7);
+    pay_.resize(off + bytes);
+    std::memcpy(pay_.data() + off, src, bytes);
+    ups_.push_back(Up{dst, off, bytes});
+}
+
+void CopyBatch::zero(void* dst, size_t bytes) {
+    if (bytes) ups_.push_back(Up{dst, SIZE_MAX, bytes});
+}
+
+void CopyBatch::down(void* host, const void* dev, size_t bytes) {
+    if (bytes) downs_.push_back(Down{host, dev, bytes});
+}
+
+bool CopyBatch::flush(hipStream_t s, Staging& st) {
+    if (ups_.empty() && downs_.empty()) return true;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t nu = ups_.size(), nd = downs_.size(), n = nu + nd;
+    std::vector<size_t> doff(nd);
+    size_t dbytes = 0, max_up = 0, max_down = 0;
+    for (size_t j = 0; j < nd; ++j) {
+        doff[j] = dbytes;
+        dbytes += (downs_[j].bytes + 7) & ~size_t(7);
+        max_down = std::max(max_down, downs_[j].bytes);
+    }
+    for (const Up& u : ups_) max_up = std::max(max_up, u.bytes);
+    // device staging: upload payload | segment table | download area
+    const size_t seg_off = al(pay_.size()), down_off = al(seg_off + n * sizeof(CopySeg)), total = down_off + dbytes;
+    auto drop = [&]() {
+        ups_.clear();
+        downs_.clear();
+        pay_.clear();
+        (void)hipGetLastError();
+        return false;
+    };
+    if (total > st.cap) {  // grow: the old buffer may still be read by issued work
+        if (st.buf && (hipStreamSynchronize(s) != hipSuccess || hipFree(st.buf) != hipSuccess)) return drop();
+        st.buf = nullptr;
+        st.cap = 0;
+        const size_t want = std::max(total, size_t(1) << 20);
+        if (hipMalloc(&st.buf, want) != hipSuccess) {
+            st.buf = nullptr;
+            return drop();
+        }
+        st.cap = want;
+    }
+    const int k = st.next;
+    st.next ^= 1;
+    if (!st.done[k] && hipEventCreateWithFlags(&st.done[k], hipEventDisableTiming) != hipSuccess) {
+        st.done[k] = nullptr;
+        return drop();
+    }
+    if (st.recorded[k] && hipEventSynchronize(st.done[k]) != hipSuccess) return drop();  // the transfer that last read host[k]
+    char* dev = static_cast<char*>(st.buf);
+    std::vector<char>& h = st.host[k];
+    h.assign(seg_off + n * sizeof(CopySeg), 0);
+    if (!pay_.empty()) std::memcpy(h.data(), pay_.data(), pay_.size());
+    CopySeg* seg = reinterpret_cast<CopySeg*>(h.data() + seg_off);
+    for (size_t i = 0; i < nu; ++i)
+        seg[i] = CopySeg{(uint64_t)(uintptr_t)ups_[i].dst,
+                         ups_[i].off == SIZE_MAX ? 0 : (uint64_t)(uintptr_t)(dev + ups_[i].off), ups_[i].bytes};
+    for (size_t j = 0; j < nd; ++j)
+        seg[nu + j] = CopySeg{(uint64_t)(uintptr_t)(dev + down_off + doff[j]), (uint64_t)(uintptr_t)downs_[j].dev,
+                              downs_[j].bytes};
+    const CopySeg* dseg = reinterpret_cast<const CopySeg*>(dev + seg_off);
+    bool ok = hipMemcpyAsync(dev, h.data(), h.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipEventRecord(st.done[k], s) == hipSuccess && (st.recorded[k] = true) &&
+              vibk_copy_segments(dseg, (int)nu, max_up, s) == hipSuccess &&
+              vibk_copy_segments(dseg + nu, (int)nd, max_down, s) == hipSuccess;
+    if (ok && nd) {
+        st.back.resize(dbytes);
+        ok = hipMemcpyAsync(st.back.data(), dev + down_off, dbytes, hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+        for (size_t j = 0; ok && j < nd; ++j) std::memcpy(downs_[j].host, st.back.data() + doff[j], downs_[j].bytes);
+    }
+    if (!ok) return drop();
+    ups_.clear();
+    downs_.clear();
+    pay_.clear();
+    return true;
+}
+
 Slot::~Slot() { release(); }
 
 void Slot::release() {
@@ -481,53 +572,72 @@ bool Slot::carve(const HostWindow& h, char* base, size_t bytes, WinDev& d, SlotC
 }
 
 bool Slot::fill(const HostWindow& h, const WinDev& d, const SlotCarve& x, char* base, size_t bytes, bool streams,
-                hipStream_t stream) {
+                hipStream_t stream, Staging& st, CopyBatch& cb) {
     // every copy into the arena stays inside the carved footprint (host-side bounds check)
     const char* lo = base;
-    auto up = [&](const void* dst, const void* src, size_t n) {
-        if (n == 0) return true;
+    bool ok = true;
+    auto inside = [&](const void* dst, const void* src, size_t n) {
         const char* p = reinterpret_cast<const char*>(dst);
-        if (!src || p < lo || p + n > lo + bytes) return false;
-        return hipMemcpyAsync(const_cast<void*>(dst), src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
+        return src && p >= lo && p + n <= lo + bytes;
     };
-    bool ok = up(d.pose, h.pose.data(), 8 * h.pose.size()) && up(d.sb, h.sb.data(), 8 * h.sb.size()) &&
-              up(d.rig, h.rig, 56) && up(d.td, &h.td, 8) && up(d.cam, h.cam.data(), 8 * h.cam.size()) &&
-              up(d.imu_from, h.imu_from.data(), 4 * h.E) && up(d.imu_to, h.imu_to.data(), 4 * h.E) &&
-              up(d.imu, h.imu.data(), 8 * h.imu.size()) &&
-              up(d.pblk, h.pblk.data(), sizeof(PriorBlockDev) * h.pblk.size()) &&
-              up(d.pA, h.prior_A.data(), 8 * h.prior_A.size()) && up(d.pr0, h.prior_r0.data(), 8 * h.prior_r0.size()) &&
-              up(d.pref, h.prior_ref.data(), 8 * h.prior_ref.size()) &&
-              up(d.pfree, h.pfree.data(), 4 * h.pfree.size()) && up(d.colmap, h.colmap.data(), 4 * h.colmap.size());
+    auto up = [&](const void* dst, const void* src, size_t n) {  // small: one batched transfer
+        if (n == 0) return;
+        if (!inside(dst, src, n)) ok = false;
+        else cb.up(const_cast<void*>(dst), src, n);
+    };
+    auto big = [&](const void* dst, const void* src, size_t n) {  // streams: transfers of their own
+        if (n == 0) return true;
+        return inside(dst, src, n) &&
+               hipMemcpyAsync(const_cast<void*>(dst), src, n, hipMemcpyHostToDevice, stream) == hipSuccess;
+    };
+    up(d.pose, h.pose.data(), 8 * h.pose.size());
+    up(d.sb, h.sb.data(), 8 * h.sb.size());
+    up(d.rig, h.rig, 56);
+    up(d.td, &h.td, 8);
+    up(d.cam, h.cam.data(), 8 * h.cam.size());
+    up(d.imu_from, h.imu_from.data(), 4 * h.E);
+    up(d.imu_to, h.imu_to.data(), 4 * h.E);
+    up(d.imu, h.imu.data(), 8 * h.imu.size());
+    up(d.pblk, h.pblk.data(), sizeof(PriorBlockDev) * h.pblk.size());
+    up(d.pA, h.prior_A.data(), 8 * h.prior_A.size());
+    up(d.pr0, h.prior_r0.data(), 8 * h.prior_r0.size());
+    up(d.pref, h.prior_ref.data(), 8 * h.prior_ref.size());
+    up(d.pfree, h.pfree.data(), 4 * h.pfree.size());
+    up(d.colmap, h.colmap.data(), 4 * h.colmap.size());
+    // candidate starts equal to the state
+    up(d.cpose, h.pose.data(), 8 * h.pose.size());
+    up(d.csb, h.sb.data(), 8 * h.sb.size());
+    up(d.crig, h.rig, 56);
+    up(d.ctd, &h.td, 8);
+    LmState lm{};
+    lm.lambda = 1e-4; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1;
+    up(d.lm, &lm, sizeof(lm));
+    if (h.prior_rows) {  // descriptor and output of the prior Gram launch
+        up(x.gram_win, &d, sizeof(WinDev));
+        up(x.gram_out, &d.pG, sizeof(double*));
+    }
+    std::vector<int32_t> fc;  // staged landmark keys: kept until the stream has completed
     if (streams) {
-        std::vector<int32_t> fc(h.L);
+        fc.resize(h.L);
         for (int l = 0; l < h.L; ++l) fc[l] = (h.lm_frame[l] << 8) | h.lm_cam[l];
-        ok = ok && up(d.rho, h.rho.data(), 8 * h.rho.size()) && up(d.lm_fc, fc.data(), 4 * fc.size()) &&
-             up(d.lm_off, h.lm_off.data(), 4 * h.lm_off.size()) && up(d.lm_col, h.lm_col.data(), 4 * h.lm_col.size()) &&
-             up(d.lmd, h.lmd.data(), 8 * h.lmd.size()) && up(d.ob_fc, h.ob_fc.data(), 4 * h.ob_fc.size()) &&
-             up(d.ob, h.ob.data(), 8 * h.ob.size()) && up(d.pf_ob, h.pf_ob.data(), 8 * h.pf_ob.size()) &&
-             up(d.pf_an, h.pf_an.data(), 8 * h.pf_an.size()) && up(d.pf_ids, h.pf_ids.data(), 4 * h.pf_ids.size()) &&
-             up(d.pf_lm, h.pf_lm.data(), 4 * h.pf_lm.size()) && up(d.pf_pos, h.pf_pos.data(), 4 * h.pf_pos.size()) &&
-             up(d.crho, h.rho.data(), 8 * h.rho.size()) && up(d.lm_uid, h.lm_uid.data(), 8 * h.lm_uid.size());
+        ok = ok && big(d.rho, h.rho.data(), 8 * h.rho.size()) && big(d.lm_fc, fc.data(), 4 * fc.size()) &&
+             big(d.lm_off, h.lm_off.data(), 4 * h.lm_off.size()) && big(d.lm_col, h.lm_col.data(), 4 * h.lm_col.size()) &&
+             big(d.lmd, h.lmd.data(), 8 * h.lmd.size()) && big(d.ob_fc, h.ob_fc.data(), 4 * h.ob_fc.size()) &&
+             big(d.ob, h.ob.data(), 8 * h.ob.size()) && big(d.pf_ob, h.pf_ob.data(), 8 * h.pf_ob.size()) &&
+             big(d.pf_an, h.pf_an.data(), 8 * h.pf_an.size()) && big(d.pf_ids, h.pf_ids.data(), 4 * h.pf_ids.size()) &&
+             big(d.pf_lm, h.pf_lm.data(), 4 * h.pf_lm.size()) && big(d.pf_pos, h.pf_pos.data(), 4 * h.pf_pos.size()) &&
+             big(d.crho, h.rho.data(), 8 * h.rho.size()) && big(d.lm_uid, h.lm_uid.data(), 8 * h.lm_uid.size());
         // the layouts of the copies match the carved footprint
         ok = ok && (int)h.rho.size() == h.L && (int)h.lm_uid.size() == h.L && (int)h.lm_off.size() == h.L + 1 && h.lmd.size() == (size_t)LM_PLANES * h.L &&
              (int)h.ob_fc.size() == h.F && h.ob.size() == (size_t)OB_PLANES * h.F && h.pf_ob.size() == (size_t)OB_PLANES * h.F;
     }
-    // candidate starts equal to the state
-    ok = ok && up(d.cpose, h.pose.data(), 8 * h.pose.size()) && up(d.csb, h.sb.data(), 8 * h.sb.size()) &&
-         up(d.crig, h.rig, 56) && up(d.ctd, &h.td, 8);
-    LmState lm{};
-    lm.lambda = 1e-4; lm.increase = 2; lm.new_point = 1; lm.status = 0; lm.max_iterations = 1;
-    ok = ok && up(d.lm, &lm, sizeof(lm));
-    if (ok && h.prior_rows) {
-        // G = A^T A once per prior (one-window launch)
-        ok = up(x.gram_win, &d, sizeof(WinDev)) && up(x.gram_out, &d.pG, sizeof(double*)) &&
-             vibk_prior_gram(x.gram_win, 1, x.gram_out, h.prior_cols, stream) == hipSuccess;
-    }
-    // the staging above (landmark keys, LM state) is local: the copies complete before return
+    ok = ok && cb.flush(stream, st);
+    // G = A^T A once per prior (one-window launch)
+    if (ok && h.prior_rows) ok = vibk_prior_gram(x.gram_win, 1, x.gram_out, h.prior_cols, stream) == hipSuccess;
     return ok && hipStreamSynchronize(stream) == hipSuccess;
 }
 
-int Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
+int Slot::upload(const HostWindow& h, hipStream_t stream, Staging& st, std::string& err) {
     const size_t need = footprint(h);
     // The slot is live only once every part of the window is on the device: a failure on any
     // path below leaves it unused (find_slot no longer returns it) with its arena kept for reuse.
@@ -549,7 +659,8 @@ int Slot::upload(const HostWindow& h, hipStream_t stream, std::string& err) {
     if (!carve(h, static_cast<char*>(arena), bytes, dev, x)) { err = "arena overflow"; return VIB_ERR_DEVICE; }
     reduced_lin = x.reduced_lin;
     reduced_schur = x.reduced_schur;
-    if (!fill(h, dev, x, static_cast<char*>(arena), bytes, true, stream) || hipStreamSynchronize(stream) != hipSuccess) {
+    CopyBatch cb;
+    if (!fill(h, dev, x, static_cast<char*>(arena), bytes, true, stream, st, cb)) {
         (void)hipGetLastError();
         err = "upload failed";
         return VIB_ERR_DEVICE;
@@ -758,6 +869,7 @@ struct vib_context {
     std::map<std::pair<int, int>, std::pair<int, int>> tuned;
     void* tune_buf = nullptr;  // state snapshot of the windows being tuned
     size_t tune_cap = 0;
+    vib::Staging xfer;         // batched small transfers (CopyBatch)
     ~vib_context() {
         if (preint_buf) (void)hipFree(preint_buf);
         if (eig_buf) (void)hipFree(eig_buf);
@@ -923,12 +1035,13 @@ bool refresh_state(vib_context* ctx, Slot& s) {
     h.pose.resize(7 * (size_t)h.K);
     h.sb.resize(9 * (size_t)h.K);
     h.rho.resize(h.L);
-    return hipMemcpyAsync(h.pose.data(), d.pose, 56 * (size_t)d.K, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-           hipMemcpyAsync(h.sb.data(), d.sb, 72 * (size_t)d.K, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-           hipMemcpyAsync(h.rig, d.rig, 56, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-           hipMemcpyAsync(&h.td, d.td, 8, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-           (d.L == 0 || hipMemcpyAsync(h.rho.data(), d.rho, 8 * (size_t)d.L, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
-           hipStreamSynchronize(ctx->stream) == hipSuccess;
+    CopyBatch cb;
+    cb.down(h.pose.data(), d.pose, 56 * (size_t)d.K);
+    cb.down(h.sb.data(), d.sb, 72 * (size_t)d.K);
+    cb.down(h.rig, d.rig, 56);
+    cb.down(&h.td, d.td, 8);
+    cb.down(h.rho.data(), d.rho, 8 * (size_t)d.L);
+    return cb.flush(ctx->stream, ctx->xfer);
 }
 
 // Makes the mirror's landmark and observation vectors current after a device edit (they are
@@ -1113,10 +1226,10 @@ bool job_edit_buffers(vib_context* ctx, size_t i, int L, int F, EditBufs& b) {
 
 // Copies the slot's descriptor to b.src and clears the selection counters.
 bool edit_begin(vib_context* ctx, const Slot& s, const EditBufs& b) {
-    const int zero[16] = {};
-    return hipMemcpyAsync(b.src, &s.dev, sizeof(WinDev), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
-           hipMemcpyAsync(b.sel, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
-           hipStreamSynchronize(ctx->stream) == hipSuccess;
+    CopyBatch cb;
+    cb.up(b.src, &s.dev, sizeof(WinDev));
+    cb.zero(b.sel, 16 * sizeof(int));
+    return cb.flush(ctx->stream, ctx->xfer);
 }
 
 // The window that `sp` selects from the resident window of `src`, made the resident window of
@@ -1131,23 +1244,21 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
     sp.nr = meta.nr;
     const int Ls = src.host.L, Fs = src.host.F, Lx = Ls + sp.ins_lm;
     const hipStream_t st = ctx->stream;
-    auto h2d = [&](void* dst_p, const void* src_p, size_t n) {
-        return hipMemcpyAsync(dst_p, src_p, n, hipMemcpyHostToDevice, st) == hipSuccess;
-    };
-    auto d2h = [&](void* dst_p, const void* src_p, size_t n) {
-        return hipMemcpyAsync(dst_p, src_p, n, hipMemcpyDeviceToHost, st) == hipSuccess;
-    };
-    const int zero = 0;
     unsigned long long ltot = 0;
     uint32_t ftot = 0;
     int bad = 0;
-    bool ok = h2d(b.src, &src.dev, sizeof(WinDev)) && h2d(b.bad, &zero, sizeof(int)) &&
+    CopyBatch cb;
+    cb.up(b.src, &src.dev, sizeof(WinDev));
+    cb.zero(b.bad, sizeof(int));
+    bool ok = cb.flush(st, ctx->xfer) &&
               vibk_edit_mark(b.src, Ls, &sp, b.lmv, b.fv, b.cnt, b.lm_of, b.bad, st) == hipSuccess &&
               vibk_edit_scan(b.lmv, Lx, b.ltile, b.lscan, 1, st) == hipSuccess &&
               vibk_edit_scan(b.cnt, Lx, b.cctile, b.cscan, 0, st) == hipSuccess &&
-              vibk_edit_scan(b.fv, Fs, b.ftile, b.fscan, 0, st) == hipSuccess &&
-              d2h(&ltot, b.lscan + Lx, sizeof(ltot)) && d2h(&ftot, b.cscan + Lx, sizeof(ftot)) &&
-              d2h(&bad, b.bad, sizeof(bad)) && hipStreamSynchronize(st) == hipSuccess;
+              vibk_edit_scan(b.fv, Fs, b.ftile, b.fscan, 0, st) == hipSuccess;
+    cb.down(&ltot, b.lscan + Lx, sizeof(ltot));
+    cb.down(&ftot, b.cscan + Lx, sizeof(ftot));
+    cb.down(&bad, b.bad, sizeof(bad));
+    ok = ok && cb.flush(st, ctx->xfer);
     if (!ok) {
         (void)hipGetLastError();
         return fail(ctx, VIB_ERR_DEVICE, "window edit failed");
@@ -1191,7 +1302,8 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
     SlotCarve x;
     if (!Slot::carve(meta, base, need, nd, x)) return fail(ctx, VIB_ERR_DEVICE, "arena overflow");
     const int Fd = meta.F, nkeys = vibk_edit_pair_keys() * vibk_edit_sort_blocks(Fd);
-    ok = Slot::fill(meta, nd, x, base, need, false, st) && h2d(b.dst, &nd, sizeof(WinDev)) &&
+    cb.up(b.dst, &nd, sizeof(WinDev));
+    ok = Slot::fill(meta, nd, x, base, need, false, st, ctx->xfer, cb) &&
          vibk_edit_gather(b.src, b.dst, Ls, Fs, &sp, b.lscan, b.fscan, b.cscan, b.lm_of, b.lmn, st) == hipSuccess &&
          (Fd == 0 || (vibk_edit_pair_hist(b.dst, Fd, b.lmn, b.counts, st) == hipSuccess &&
                       vibk_edit_scan(b.counts, nkeys, b.ctile, b.offs, 0, st) == hipSuccess &&
@@ -1801,20 +1913,25 @@ int append_keyframe(vib_context* ctx, Slot& s, const vib_keyframe& kf, int32_t* 
     }
     EditBufs b;
     if (!edit_buffers(ctx, cur.L, cur.F, b, nl, nf)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "append buffer allocation failed");
-    auto h2d = [&](void* dst, const void* src, size_t n) {
-        return n == 0 || hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
-    };
     int miss = 0;
-    const bool ok = edit_begin(ctx, s, b) && h2d(b.ins_obs_uid, ouid.data(), 8 * NF) &&
-                    h2d(b.ins_obs_rank, orank.data(), 4 * NF) && h2d(b.ins_obs_fc, ofc.data(), 4 * NF) &&
-                    h2d(b.ins_ob, ob.data(), 8 * ob.size()) && h2d(b.ins_lm_fc, lfc.data(), 4 * NL) &&
-                    h2d(b.ins_lmd, lmd.data(), 8 * lmd.size()) && h2d(b.ins_rho, kf.lm_rho, 8 * NL) &&
-                    h2d(b.ins_free, lfree.data(), NL) && h2d(b.ins_uid, kf.lm_id, 8 * NL) &&
-                    hipMemsetAsync(b.add_cnt, 0, 4 * ((size_t)cur.L + NL), ctx->stream) == hipSuccess &&
-                    vibk_insert_resolve(b.src, b.ins_obs_uid, nf, b.ins_uid, nl, b.ins_obs_lm, b.add_cnt, b.sel,
-                                        ctx->stream) == hipSuccess &&
-                    hipMemcpyAsync(&miss, b.sel, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-                    hipStreamSynchronize(ctx->stream) == hipSuccess;
+    CopyBatch cb;  // the source descriptor and the keyframe's records in one transfer
+    cb.up(b.src, &s.dev, sizeof(WinDev));
+    cb.zero(b.sel, 16 * sizeof(int));
+    cb.up(b.ins_obs_uid, ouid.data(), 8 * NF);
+    cb.up(b.ins_obs_rank, orank.data(), 4 * NF);
+    cb.up(b.ins_obs_fc, ofc.data(), 4 * NF);
+    cb.up(b.ins_ob, ob.data(), 8 * ob.size());
+    cb.up(b.ins_lm_fc, lfc.data(), 4 * NL);
+    cb.up(b.ins_lmd, lmd.data(), 8 * lmd.size());
+    cb.up(b.ins_rho, kf.lm_rho, 8 * NL);
+    cb.up(b.ins_free, lfree.data(), NL);
+    cb.up(b.ins_uid, kf.lm_id, 8 * NL);
+    cb.zero(b.add_cnt, 4 * ((size_t)cur.L + NL));
+    bool ok = cb.flush(ctx->stream, ctx->xfer) &&
+              vibk_insert_resolve(b.src, b.ins_obs_uid, nf, b.ins_uid, nl, b.ins_obs_lm, b.add_cnt, b.sel,
+                                  ctx->stream) == hipSuccess;
+    cb.down(&miss, b.sel, sizeof(int));
+    ok = ok && cb.flush(ctx->stream, ctx->xfer);
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, "keyframe upload failed");
     EditSpec sp{};
     for (int f = 0; f < kMaxFrames; ++f) sp.renum[f] = f < k ? f : -1;
@@ -2017,6 +2134,12 @@ int vib_create(vib_context** out, int device) {
     return VIB_OK;
 }
 
+int vib_context_device(const vib_context* ctx, int32_t* device) {
+    if (!ctx || !device) return VIB_ERR_INVALID_ARGUMENT;
+    *device = ctx->device;
+    return VIB_OK;
+}
+
 int vib_destroy(vib_context* ctx) {
     if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
     hipSetDevice(ctx->device);
@@ -2068,7 +2191,7 @@ int vib_window_upload(vib_context* ctx, int32_t slot, const vib_window* w) {
     }
     auto& s = ctx->slots[slot];
     if (!s) s = std::make_unique<Slot>();
-    if (const int st = s->upload(*h, ctx->stream, err)) return drop(st);
+    if (const int st = s->upload(*h, ctx->stream, ctx->xfer, err)) return drop(st);
     return VIB_OK;
 }
 

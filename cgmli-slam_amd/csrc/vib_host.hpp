@@ -82,6 +82,42 @@ struct HostWindow {
     void make_chunks();
 };
 
+// Staging of a context's batched transfers: a grow-only device buffer, and two host buffers
+// used in turn, each reused only after the transfer that last read it has completed (`done`).
+struct Staging {
+    void* buf = nullptr;
+    size_t cap = 0;
+    std::vector<char> host[2], back;
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool recorded[2] = {false, false};
+    int next = 0;
+    ~Staging();
+};
+
+// The small copies of one host call as one transfer and one kernel launch. Each hipMemcpyAsync
+// of pageable host memory costs several microseconds of host time (a staging copy and a copy
+// launch), and a window edit used to make dozens of them. Uploads are packed into one host
+// buffer, moved in one host-to-device transfer and scattered to their destinations by one
+// launch (k_copy_segments); downloads are gathered by one launch into the staging buffer and
+// moved in one device-to-host transfer. The batch owns its host buffers, so its sources may be
+// temporaries, and the batch may be dropped once flushed.
+class CopyBatch {
+  public:
+    void up(void* dst, const void* src, size_t bytes);  // host src, copied into the batch now
+    void zero(void* dst, size_t bytes);
+    void down(void* host, const void* dev, size_t bytes);  // valid after flush returns true
+    // Issues the uploads on `s`; with downloads also gathers them, synchronizes `s` and unpacks
+    // them into their host destinations. Every pending copy is consumed.
+    bool flush(hipStream_t s, Staging& st);
+
+  private:
+    struct Up { void* dst; size_t off; size_t bytes; };  // off: payload offset (SIZE_MAX: zeros)
+    struct Down { void* host; const void* dev; size_t bytes; };
+    std::vector<Up> ups_;
+    std::vector<Down> downs_;
+    std::vector<char> pay_;
+};
+
 // Device pointers of a slot's arena that the window descriptor does not carry.
 struct SlotCarve {
     double* reduced_lin = nullptr;
@@ -115,10 +151,12 @@ struct Slot {
     // cameras, IMU, prior, retained column map, LM state; with streams also the landmark and
     // observation streams in both orders), then forms the prior's Gram matrix; returns after
     // `stream` has completed them, false if a copy or launch fails.
+    // The small copies go through one batched transfer (`cb`, which may already hold the
+    // caller's copies).
     static bool fill(const HostWindow& h, const WinDev& d, const SlotCarve& x, char* base, size_t bytes, bool streams,
-                     hipStream_t stream);
+                     hipStream_t stream, Staging& st, CopyBatch& cb);
     // VIB_OK, VIB_ERR_OUT_OF_MEMORY (allocation) or VIB_ERR_DEVICE (copies, launches)
-    int upload(const HostWindow& h, hipStream_t stream, std::string& err);
+    int upload(const HostWindow& h, hipStream_t stream, Staging& st, std::string& err);
 };
 
 // Descriptors of a set of windows solved together.

@@ -1093,31 +1093,45 @@ __device__ __forceinline__ double damped_point(double d, double lambda) { return
 // coupling rows (c_k over the nv visual columns: anchor pose sum_f u_f, observer poses -u_f,
 // rig, td), formed 64 landmarks at a time in LDS and multiplied on MFMA (v_mfma_f64_16x16x4f64,
 // upper 16x16 tiles of the padded 128 x 128 block, 9 tiles per wave).
+// FP64: row k holds sqrt(s_k) c_k and, in column nv, sqrt(s_k) g_k (s_k = 1 / damped(d_k) > 0,
+// or the pseudo-inverse weight of a marginal system), so the tiles of C~^T C~ carry the
+// gradient correction -sum_k s_k g_k c_k in column nv and the products need no scaling.
 constexpr int kSchurRows = 64;                 // landmarks per LDS pass
 constexpr int kSchurLd = 144;                  // LDS row stride (doubles; offsets rows by 128 B)
 constexpr int kVisTiles = 8;                   // 128 / 16
 constexpr int kSchurTiles = kVisTiles * (kVisTiles + 1) / 2;
 constexpr int kSchurTilesPerWave = (kSchurTiles + kBlock / 64 - 1) / (kBlock / 64);
+constexpr int kSchurGroups = kBlock / kChunkLanes;            // landmark groups per workgroup
+constexpr int kSchurRowsPerGroup = kSchurRows / kSchurGroups;  // rows one group builds per pass
+static_assert(kSchurRowsPerGroup * kSchurGroups == kSchurRows, "rows per pass");
+static_assert(kMaxVisual < kVisTiles * 16, "the gradient column fits the padded block");
 
 // Mixed precision keeps the coupling rows in FP32 (37 KB of LDS) and multiplies them on
 // v_mfma_f32_16x16x4f32; the FP32 products of each 16-landmark group are carried into FP64
-// accumulators (the chunk partials, and their sum over chunks, stay FP64).
+// accumulators (the chunk partials, and their sum over chunks, stay FP64). Its gradient
+// correction is summed in FP64 from the unscaled rows.
 template <bool MIXED>
 __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chunks) {
     using CT = typename std::conditional<MIXED, float, double>::type;
     __shared__ CT crow[kSchurRows * kSchurLd];
     __shared__ CT sc[kSchurRows];
     __shared__ double qg[kSchurRows];
+    __shared__ int pcol[kMaxFrames];
     const Chunk ch = chunks[blockIdx.x];
     const WinDev& w = wins[ch.win];
     if (!window_active(w)) return;
     const int nv = w.nv, tri = w.tri;
     const double lambda = w.lm->lambda;
+    const int marginal = w.lm->marginal;
+    const double marg_cut = w.lm->marg_cut;
+    const int rig_col = w.rig_col, td_col = w.td_col;
     const int t = threadIdx.x, lane = t % 64, wave = t / 64;
     const int lgroup = t % kChunkLanes, group = t / kChunkLanes;
-    constexpr int groups = kBlock / kChunkLanes;
     const size_t L = (size_t)w.L, F = (size_t)w.F;
-    // this wave's output tiles (ti <= tj) of the padded block
+    if (t < kMaxFrames) pcol[t] = t < w.K ? w.pose_col[t] : -1;
+    // this wave's output tiles (ti <= tj) of the padded block; FP64 also needs the tiles of
+    // column nv (the gradient)
+    const int last_col = MIXED ? nv - 1 : nv;
     int tti[kSchurTilesPerWave], ttj[kSchurTilesPerWave];
     d4 acc[kSchurTilesPerWave];
 #pragma unroll
@@ -1131,50 +1145,90 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
             while (r >= kVisTiles - ti) { r -= kVisTiles - ti; ++ti; }
             tti[q] = ti;
             ttj[q] = ti + r;
-            if (16 * ttj[q] >= nv) tti[q] = -1;
+            if (16 * ttj[q] > last_col) tti[q] = -1;
         }
     }
-    double gsum = 0;  // gradient correction of column t (t < nv)
+    double gsum = 0;  // MIXED: gradient correction of column t (t < nv)
     for (int base = ch.lm_begin; base < ch.lm_end; base += kSchurRows) {
         for (int i = t; i < kSchurRows * kSchurLd; i += kBlock) crow[i] = CT(0);
-        __syncthreads();
-        // dense coupling rows, scale 1 / damped(d) and gradient coefficient g / damped(d)
-        for (int r0 = 0; r0 < kSchurRows; r0 += groups) {
-            const int r = r0 + group, l = base + r;
-            const bool live = r < kSchurRows && l < ch.lm_end && w.lm_col[l] >= 0;
-            if (r < kSchurRows && lgroup == 0) {
-                double inv = 0, gk = 0;
-                if (live) {
-                    const double dl = w.lm_out[LO_D * L + l];
-                    // marginalization: pseudo-inverse of the diagonal landmark block (Marginalizer.cpp:373-409)
-                    inv = w.lm->marginal ? (dl > w.lm->marg_cut ? 1.0 / dl : 0.0) : 1.0 / damped_point(dl, lambda);
-                    gk = w.lm_out[LO_G * L + l];
-                }
-                sc[r] = CT(inv);
-                qg[r] = inv * gk;
+        // Dense coupling rows. Group g builds rows g + j groups (j < kSchurRowsPerGroup): every
+        // load of its rows (landmark record, first factor of each lane, its coupling terms) is
+        // issued before the first LDS update, so the dependent global round trips of the rows
+        // overlap instead of following each other.
+        int lmk[kSchurRowsPerGroup], f1[kSchurRowsPerGroup], fa[kSchurRowsPerGroup], pa[kSchurRowsPerGroup];
+        bool live[kSchurRowsPerGroup];
+        double dl[kSchurRowsPerGroup], gl[kSchurRowsPerGroup];
+#pragma unroll
+        for (int j = 0; j < kSchurRowsPerGroup; ++j) {
+            const int l = base + group + j * kSchurGroups;
+            const int lc = l < ch.lm_end ? l : ch.lm_begin;  // the loads stay inside the chunk
+            lmk[j] = l;
+            live[j] = l < ch.lm_end && w.lm_col[lc] >= 0;
+            fa[j] = w.lm_off[lc] + lgroup;
+            f1[j] = live[j] ? w.lm_off[lc + 1] : 0;  // a row that is not live has no factor to load
+            pa[j] = w.lm_fc[lc] >> 8;
+            dl[j] = w.lm_out[LO_D * L + lc];
+            gl[j] = w.lm_out[LO_G * L + lc];
+        }
+        int fb[kSchurRowsPerGroup], fp[kSchurRowsPerGroup];
+#pragma unroll
+        for (int j = 0; j < kSchurRowsPerGroup; ++j) {
+            fb[j] = -1;
+            fp[j] = 0;
+            if (fa[j] < f1[j]) {
+                fb[j] = w.ob_fc[fa[j]] >> 8;
+                fp[j] = w.pf_pos[fa[j]];
             }
-            if (!live) continue;
+        }
+        double u[kSchurRowsPerGroup][6];
+#pragma unroll
+        for (int j = 0; j < kSchurRowsPerGroup; ++j)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) u[j][k] = fa[j] < f1[j] ? load_u<MIXED>(w, F, k, fp[j]) : 0.0;
+        __syncthreads();  // rows cleared, pose columns staged
+#pragma unroll
+        for (int j = 0; j < kSchurRowsPerGroup; ++j) {
+            const int r = group + j * kSchurGroups, l = lmk[j];
+            // marginalization: pseudo-inverse of the diagonal landmark block (Marginalizer.cpp:373-409)
+            double inv = 0;
+            if (live[j]) inv = marginal ? (dl[j] > marg_cut ? 1.0 / dl[j] : 0.0) : 1.0 / damped_point(dl[j], lambda);
+            const double rs = MIXED ? 1.0 : sqrt(inv);  // FP64 rows carry sqrt(s_k)
+            if (lgroup == 0) {
+                sc[r] = CT(inv);
+                qg[r] = inv * gl[j];
+            }
+            if (!live[j]) continue;
             CT* row = crow + r * kSchurLd;
-            const int a = w.lm_fc[l] >> 8;
-            const int ac = w.pose_col[a];
-            const int f0 = w.lm_off[l], f1 = w.lm_off[l + 1];
-            for (int f = f0 + lgroup; f < f1; f += kChunkLanes) {
-                const int b = w.ob_fc[f] >> 8;
-                const int bc = b == a ? -1 : w.pose_col[b];
+            const int a = pa[j], ac = pcol[a];
+            if (fa[j] < f1[j]) {
+                const int bc = fb[j] == a ? -1 : pcol[fb[j]];
+#pragma unroll
                 for (int k = 0; k < 6; ++k) {
-                    const CT u = CT(load_u<MIXED>(w, F, k, w.pf_pos[f]));
-                    if (ac >= 0) atomicAdd(&row[ac + k], u);
-                    if (bc >= 0) atomicAdd(&row[bc + k], -u);
+                    const CT v = CT(rs * u[j][k]);
+                    if (ac >= 0) atomicAdd(&row[ac + k], v);
+                    if (bc >= 0) atomicAdd(&row[bc + k], -v);
+                }
+            }
+            for (int f = fa[j] + kChunkLanes; f < f1[j]; f += kChunkLanes) {  // landmarks with more factors than lanes
+                const int b = w.ob_fc[f] >> 8;
+                const int bc = b == a ? -1 : pcol[b];
+                const int p = w.pf_pos[f];
+                for (int k = 0; k < 6; ++k) {
+                    const CT v = CT(rs * load_u<MIXED>(w, F, k, p));
+                    if (ac >= 0) atomicAdd(&row[ac + k], v);
+                    if (bc >= 0) atomicAdd(&row[bc + k], -v);
                 }
             }
             if (lgroup == 0) {
-                if (w.rig_col >= 0)
-                    for (int k = 0; k < 6; ++k) row[w.rig_col + k] = CT(w.lm_out[(LO_CR0 + k) * L + l]);
-                if (w.td_col >= 0) row[w.td_col] = CT(w.lm_out[LO_CT * L + l]);
+                if (rig_col >= 0)
+                    for (int k = 0; k < 6; ++k) row[rig_col + k] = CT(rs * w.lm_out[(LO_CR0 + k) * L + l]);
+                if (td_col >= 0) row[td_col] = CT(rs * w.lm_out[LO_CT * L + l]);
+                if constexpr (!MIXED) row[nv] = CT(rs * gl[j]);
             }
         }
         __syncthreads();
-        // C^T diag(s) C on MFMA: A[i][k] = s_k C[k][16 ti + i] (lane i + 16 k), B[k][j] = C[k][16 tj + j]
+        // FP64: C~^T C~ on MFMA, A[i][k] = C~[k][16 ti + i] (lane i + 16 k), B[k][j] = C~[k][16 tj + j].
+        // MIXED: C^T diag(s) C with the scale on the A operand.
 #pragma unroll
         for (int q = 0; q < kSchurTilesPerWave; ++q) {
             if (tti[q] < 0) continue;
@@ -1192,16 +1246,18 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
                     for (int rr = 0; rr < 4; ++rr) acc[q][rr] += (double)part[rr];
                 }
             } else {
+#pragma unroll 4
                 for (int k0 = 0; k0 < kSchurRows; k0 += 4) {
                     const int k = k0 + lane / 16;
-                    const double av = sc[k] * crow[k * kSchurLd + ci];
-                    const double bv = crow[k * kSchurLd + cj];
-                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
+                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(crow[k * kSchurLd + ci], crow[k * kSchurLd + cj], acc[q],
+                                                                 0, 0, 0);
                 }
             }
         }
-        if (t < nv)
-            for (int k = 0; k < kSchurRows; ++k) gsum += qg[k] * (double)crow[k * kSchurLd + t];
+        if constexpr (MIXED) {
+            if (t < nv)
+                for (int k = 0; k < kSchurRows; ++k) gsum += qg[k] * (double)crow[k * kSchurLd + t];
+        }
         __syncthreads();
     }
     // chunk partial: packed upper triangle of -C^T diag(s) C, then -C^T (s g); output element
@@ -1216,9 +1272,10 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
             const int ri = MIXED ? 4 * (lane / 16) + rr : 4 * rr + lane / 16;
             const int i = 16 * tti[q] + ri, j = 16 * ttj[q] + lane % 16;
             if (i <= j && j < nv) part[tri_index(i, j, nv)] = -acc[q][rr];
+            else if (!MIXED && j == nv && i < nv) part[tri + i] = -acc[q][rr];
         }
     }
-    if (t < nv) part[tri + t] = -gsum;
+    if (MIXED && t < nv) part[tri + t] = -gsum;
 }
 
 // two workgroups per CU: 256 VGPRs (no AGPRs) and 74 KB of LDS each. Workgroups nchunks.. sum

@@ -15,10 +15,14 @@
 // footprint, size buckets taken in turn, per-request accounting.
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/vib.h"
@@ -171,13 +175,15 @@ struct Request {
     int64_t id = 0;
     int32_t session = 0, iterations = 1, unmatched = 0;
     bool has_keyframe = false;
+    bool starting = false;  // admitted, keyframe not yet appended and LM state not yet reset
     OwnedKeyframe keyframe;
     Clock::time_point submitted, admitted;
 };
 
 struct Session {
     int32_t id = 0;
-    int32_t slot = -1;     // context slot while resident
+    int32_t worker = 0;    // the worker (context) that holds the session
+    int32_t slot = -1;     // slot of the worker's context while resident
     int32_t frames = 0;    // the window size a slide restores
     int32_t cameras = 0;
     OwnedWindow stored;    // the window while not resident
@@ -186,16 +192,82 @@ struct Session {
     Clock::time_point last_used;
 };
 
+// One worker: a context of its own (worker 0: the caller's), its sessions and slots. Workers
+// serve disjoint sessions, so their parts of a turn run on separate threads and streams: a
+// turn's host work (appends, slides, their synchronizations) is mostly latency, and the device
+// is idle for most of it.
+struct Worker {
+    vib_context* ctx = nullptr;
+    bool owned = false;
+    std::vector<int32_t> sessions;  // ids, ascending
+    std::deque<int32_t> free_slots;
+    int32_t next_slot = 0;
+    int64_t turn = 0;               // bucket rotation
+    int64_t footprint = 0;          // footprints of its sessions' first windows (assignment)
+    vib_server_stats stats{};
+    std::deque<vib_served> done;    // finished in this worker's last turn
+};
+
+// Threads 1 .. n - 1 run worker i's part of a turn; the calling thread runs worker 0.
+struct Pool {
+    std::vector<std::thread> threads;
+    std::mutex m;
+    std::condition_variable start, finished;
+    int64_t generation = 0;
+    int remaining = 0;
+    bool quit = false;
+    std::function<void(int)> job;
+
+    void launch(int n) {
+        for (int i = 1; i < n; ++i)
+            threads.emplace_back([this, i] {
+                int64_t seen = 0;
+                for (;;) {
+                    std::unique_lock<std::mutex> lk(m);
+                    start.wait(lk, [&] { return quit || generation != seen; });
+                    if (quit) return;
+                    seen = generation;
+                    std::function<void(int)> f = job;
+                    lk.unlock();
+                    f(i);
+                    lk.lock();
+                    if (--remaining == 0) finished.notify_one();
+                }
+            });
+    }
+    void run(const std::function<void(int)>& f) {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            job = f;
+            remaining = (int)threads.size();
+            ++generation;
+        }
+        start.notify_all();
+        f(0);
+        std::unique_lock<std::mutex> lk(m);
+        finished.wait(lk, [&] { return remaining == 0; });
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            quit = true;
+        }
+        start.notify_all();
+        for (std::thread& t : threads) t.join();
+        threads.clear();
+    }
+};
+
 }  // namespace
 
 struct vib_server {
     vib_context* ctx = nullptr;
     vib_server_options opt{};
     std::map<int32_t, Session> sessions;
-    std::deque<int32_t> free_slots;
-    int32_t next_slot = 0;
-    int64_t turn = 0;
-    vib_server_stats stats{};
+    std::vector<Worker> workers;
+    Pool pool;
+    int64_t turns = 0;
+    double residency_seconds = 0;  // admission by HBM footprint (calling thread)
     std::deque<vib_served> done;  // finished requests not yet returned
 };
 
@@ -207,71 +279,85 @@ int bucket_of(const vib_server* s, int nr) {
     return std::max(0, s->opt.num_buckets - 1);
 }
 
-int32_t take_slot(vib_server* s) {
-    if (!s->free_slots.empty()) {
-        const int32_t v = s->free_slots.front();
-        s->free_slots.pop_front();
+int32_t take_slot(const vib_server* s, Worker& w) {
+    if (!w.free_slots.empty()) {
+        const int32_t v = w.free_slots.front();
+        w.free_slots.pop_front();
         return v;
     }
-    return s->opt.first_slot + s->next_slot++;
+    return s->opt.first_slot + w.next_slot++;
 }
 
 // The session's window back into host memory, its slot freed.
-int evict(vib_server* s, Session& se) {
+int evict(Worker& w, Session& se) {
     int32_t K, L, F, E, rows, cols;
-    int st = vib_window_shape(s->ctx, se.slot, &K, &L, &F, &E, &rows, &cols);
+    int st = vib_window_shape(w.ctx, se.slot, &K, &L, &F, &E, &rows, &cols);
     if (st != VIB_OK) return st;
     se.stored.allocate(K, se.cameras, L, F, E, rows, cols);
-    vib_window w = se.stored.view();
-    st = vib_read_window(s->ctx, se.slot, &w);
+    vib_window win = se.stored.view();
+    st = vib_read_window(w.ctx, se.slot, &win);
     if (st != VIB_OK) return st;
-    se.stored.flags = w.flags; se.stored.huber = w.huber_delta; se.stored.min_rho = w.min_inverse_depth;
-    se.stored.td_lo = w.td_lower; se.stored.td_hi = w.td_upper; se.stored.blocks = w.prior_blocks;
-    st = vib_window_release(s->ctx, se.slot);
+    se.stored.flags = win.flags; se.stored.huber = win.huber_delta; se.stored.min_rho = win.min_inverse_depth;
+    se.stored.td_lo = win.td_lower; se.stored.td_hi = win.td_upper; se.stored.blocks = win.prior_blocks;
+    st = vib_window_release(w.ctx, se.slot);
     if (st != VIB_OK) return st;
-    s->free_slots.push_back(se.slot);
+    w.free_slots.push_back(se.slot);
     se.slot = -1;
-    s->stats.evictions++;
+    w.stats.evictions++;
     return VIB_OK;
 }
 
-// Makes the session's window resident within the HBM budget (idle sessions with nothing queued
-// are evicted, least recently used first). False: it does not fit now.
+// Makes the session's window resident within the server's HBM budget (idle sessions with nothing
+// queued are evicted, least recently used first, whichever worker holds them). False: it does
+// not fit now; with *status set: it never will, or the upload failed. Runs on the calling thread
+// while no worker runs.
 bool make_resident(vib_server* s, Session& se, int* status) {
     *status = VIB_OK;
     if (se.slot >= 0) return true;
-    vib_window w = se.stored.view();
-    const int64_t need = vib_window_footprint(&w, s->opt.precision);
+    Worker& w = s->workers[se.worker];
+    vib_window win = se.stored.view();
+    const int64_t need = vib_window_footprint(&win, s->opt.precision);
     if (need < 0) {
         *status = VIB_ERR_INVALID_ARGUMENT;
         return false;
     }
-    if (s->opt.hbm_budget > 0) {
-        int64_t resident = 0;
-        vib_resident_bytes(s->ctx, &resident);
-        while (resident + need > s->opt.hbm_budget) {
+    const int64_t budget = s->opt.hbm_budget;
+    if (budget > 0) {
+        if (need > budget) {  // does not fit even with every other window evicted
+            *status = VIB_ERR_OUT_OF_MEMORY;
+            return false;
+        }
+        auto resident = [&]() {
+            int64_t total = 0;
+            for (const Worker& x : s->workers) {
+                int64_t b = 0;
+                vib_resident_bytes(x.ctx, &b);
+                total += b;
+            }
+            return total;
+        };
+        for (int64_t used = resident(); used + need > budget; used = resident()) {
             Session* victim = nullptr;
             for (auto& kv : s->sessions) {
                 Session& o = kv.second;
                 if (o.slot >= 0 && !o.running && o.queue.empty() && (!victim || o.last_used < victim->last_used)) victim = &o;
             }
             if (!victim) return false;
-            if ((*status = evict(s, *victim)) != VIB_OK) return false;
-            vib_resident_bytes(s->ctx, &resident);
+            if ((*status = evict(s->workers[victim->worker], *victim)) != VIB_OK) return false;
         }
     }
-    const int32_t slot = take_slot(s);
-    if ((*status = vib_window_upload(s->ctx, slot, &w)) != VIB_OK) {
-        s->free_slots.push_back(slot);
+    const int32_t slot = take_slot(s, w);
+    if ((*status = vib_window_upload(w.ctx, slot, &win)) != VIB_OK) {
+        w.free_slots.push_back(slot);
         return false;
     }
     se.slot = slot;
     se.stored.clear();
-    s->stats.uploads++;
+    w.stats.uploads++;
     return true;
 }
 
-void finish(vib_server* s, Session& se, std::unique_ptr<Request> r, int status, int iterations, double cost) {
+void finish(Worker& w, Session& se, std::unique_ptr<Request> r, int status, int iterations, double cost) {
     const auto now = Clock::now();
     vib_served out{};
     out.request = r->id;
@@ -283,9 +369,98 @@ void finish(vib_server* s, Session& se, std::unique_ptr<Request> r, int status, 
     out.queued_seconds = std::chrono::duration<double>(r->admitted - r->submitted).count();
     out.latency_seconds = std::chrono::duration<double>(now - r->submitted).count();
     se.last_used = now;
-    s->done.push_back(out);
-    s->stats.finished++;
-    if (status != VIB_OK) s->stats.failed++;
+    w.done.push_back(out);
+    w.stats.finished++;
+    if (status != VIB_OK) w.stats.failed++;
+}
+
+// One worker's part of a turn: the admitted requests started (keyframe appended, LM state reset),
+// one LM iteration of the running requests of one size bucket, slides.
+void turn_worker(vib_server* s, Worker& w) {
+    auto since = [](Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); };
+    const auto t_admit = Clock::now();
+    // 1. the requests admitted by this turn's residency pass
+    for (int32_t id : w.sessions) {
+        Session& se = s->sessions[id];
+        if (!se.running || !se.running->starting) continue;
+        Request& r = *se.running;
+        r.starting = false;
+        int st = VIB_OK;
+        if (r.has_keyframe) {
+            vib_keyframe k = r.keyframe.view();
+            st = vib_window_append(w.ctx, se.slot, &k, &r.unmatched);
+            w.stats.appends++;
+            r.keyframe = OwnedKeyframe();  // the window holds it now
+        }
+        vib_solve_options o{};
+        o.max_iterations = r.iterations;
+        o.min_iterations = 0;
+        o.initial_lambda = 1e-4;
+        o.precision = s->opt.precision;
+        o.refinement_steps = s->opt.refinement_steps;
+        o.flags = s->opt.full_iterations ? VIB_SOLVE_CONTINUOUS : 0u;
+        if (st == VIB_OK) st = vib_request_begin(w.ctx, se.slot, &o);
+        if (st != VIB_OK) finish(w, se, std::move(se.running), st, 0, 0.0);
+    }
+    w.stats.admit_seconds += since(t_admit);
+    // 2. one LM iteration of the running requests of one size bucket (buckets in turn)
+    std::map<int, std::vector<Session*>> by_bucket;
+    for (int32_t id : w.sessions) {
+        Session& se = s->sessions[id];
+        if (!se.running) continue;
+        int32_t nr = 0, nv = 0;
+        vib_retained_dimension(w.ctx, se.slot, &nr, &nv);
+        by_bucket[bucket_of(s, nr)].push_back(&se);
+    }
+    if (by_bucket.empty()) return;
+    std::vector<int> keys;
+    for (auto& kv : by_bucket) keys.push_back(kv.first);
+    const int b = keys[w.turn++ % keys.size()];
+    std::vector<Session*>& run = by_bucket[b];
+    std::vector<int32_t> ids, dn(run.size());
+    std::vector<vib_solve_report> reps(run.size());
+    for (Session* se : run) ids.push_back(se->slot);
+    const auto t_iter = Clock::now();
+    const int st = vib_request_iterate(w.ctx, (int32_t)ids.size(), ids.data(), s->opt.precision, s->opt.refinement_steps,
+                                       dn.data(), reps.data());
+    w.stats.iterate_seconds += since(t_iter);
+    w.stats.launches++;
+    w.stats.window_iterations += (int64_t)ids.size();
+    // 3. the finished requests' slides back to their windows' sizes (FixedLagBackend.cpp:434-451),
+    //    all in one batch
+    std::vector<size_t> fin, slide;
+    std::vector<int> status(run.size(), VIB_OK);
+    for (size_t i = 0; i < run.size(); ++i) {
+        if (st == VIB_OK && !dn[i]) continue;
+        fin.push_back(i);
+        status[i] = st != VIB_OK ? st : reps[i].status;
+        int32_t K = 0, L, F, E, rows, cols;
+        if (status[i] == VIB_OK && s->opt.slide_policy >= 0 &&
+            vib_window_shape(w.ctx, run[i]->slot, &K, &L, &F, &E, &rows, &cols) == VIB_OK && K > run[i]->frames)
+            slide.push_back(i);
+    }
+    if (!slide.empty()) {
+        std::vector<int32_t> sl;
+        for (size_t i : slide) sl.push_back(run[i]->slot);
+        std::vector<vib_marginal_report> mr(slide.size());
+        const auto t_slide = Clock::now();
+        (void)vib_marginalize_batch(w.ctx, (int32_t)sl.size(), sl.data(), s->opt.slide_policy, mr.data());
+        w.stats.slide_seconds += since(t_slide);
+        w.stats.slides += (int64_t)slide.size();
+        w.stats.slide_launches++;
+        for (size_t q = 0; q < slide.size(); ++q) status[slide[q]] = mr[q].status;
+    }
+    for (size_t i : fin)
+        finish(w, *run[i], std::move(run[i]->running), status[i], st == VIB_OK ? reps[i].iterations : 0,
+               st == VIB_OK ? reps[i].final_cost : 0.0);
+}
+
+void release_all(vib_server* s) {
+    s->pool.stop();
+    for (auto& kv : s->sessions)
+        if (kv.second.slot >= 0) (void)vib_window_release(s->workers[kv.second.worker].ctx, kv.second.slot);
+    for (Worker& w : s->workers)
+        if (w.owned && w.ctx) (void)vib_destroy(w.ctx);
 }
 
 }  // namespace
@@ -298,17 +473,35 @@ int vib_server_create(vib_context* ctx, const vib_server_options* o, vib_server*
     if (o->num_buckets < 0 || o->num_buckets > 8 || o->first_slot < 0) return VIB_ERR_INVALID_ARGUMENT;
     if (o->slide_policy != -1 && o->slide_policy != VIB_MARGIN_OLD && o->slide_policy != VIB_MARGIN_SECOND_NEW)
         return VIB_ERR_INVALID_ARGUMENT;
+    if (o->workers < 0 || o->workers > VIB_SERVER_MAX_WORKERS || o->hbm_budget < 0) return VIB_ERR_INVALID_ARGUMENT;
+    int32_t device = 0;
+    if (vib_context_device(ctx, &device) != VIB_OK) return VIB_ERR_INVALID_ARGUMENT;
     auto* s = new vib_server();
     s->ctx = ctx;
     s->opt = *o;
+    const int n = std::max(1, o->workers);
+    s->workers.resize(n);
+    for (int i = 0; i < n; ++i) {
+        Worker& w = s->workers[i];
+        if (i == 0) {
+            w.ctx = ctx;
+        } else if (vib_create(&w.ctx, device) == VIB_OK) {
+            w.owned = true;
+        } else {
+            w.ctx = nullptr;
+            release_all(s);
+            delete s;
+            return VIB_ERR_DEVICE;
+        }
+    }
+    s->pool.launch(n);
     *out = s;
     return VIB_OK;
 }
 
 int vib_server_destroy(vib_server* s) {
     if (!s) return VIB_ERR_INVALID_ARGUMENT;
-    for (auto& kv : s->sessions)
-        if (kv.second.slot >= 0) (void)vib_window_release(s->ctx, kv.second.slot);
+    release_all(s);
     delete s;
     return VIB_OK;
 }
@@ -321,6 +514,15 @@ int vib_server_add_session(vib_server* s, int32_t session, const vib_window* w) 
     if (!se.stored.copy_from(*w)) return VIB_ERR_INVALID_ARGUMENT;
     se.frames = w->num_frames;
     se.cameras = w->num_cameras;
+    // the worker with the smallest share of session footprints
+    const int64_t fp = std::max<int64_t>(0, vib_window_footprint(w, s->opt.precision));
+    int best = 0;
+    for (int i = 1; i < (int)s->workers.size(); ++i)
+        if (s->workers[i].footprint < s->workers[best].footprint) best = i;
+    se.worker = best;
+    Worker& wk = s->workers[best];
+    wk.footprint += fp;
+    wk.sessions.insert(std::upper_bound(wk.sessions.begin(), wk.sessions.end(), session), session);
     s->sessions.emplace(session, std::move(se));
     return VIB_OK;
 }
@@ -337,99 +539,38 @@ int vib_server_submit(vib_server* s, int64_t request, int32_t session, int32_t i
     if (kf && !r->keyframe.copy_from(*kf)) return VIB_ERR_INVALID_ARGUMENT;
     r->submitted = Clock::now();
     it->second.queue.push_back(std::move(r));
-    s->stats.submitted++;
+    s->workers[it->second.worker].stats.submitted++;
     return VIB_OK;
 }
 
 int vib_server_turn(vib_server* s, vib_served* finished, int32_t capacity, int32_t* count) {
     if (!s || !count || capacity < 0 || (capacity > 0 && !finished)) return VIB_ERR_INVALID_ARGUMENT;
     *count = 0;
-    s->stats.turns++;
-    auto since = [](Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); };
-    const auto t_admit = Clock::now();
-    // 1. admission: the first queued request of every idle session whose window is (made) resident
+    s->turns++;
+    // admission: the first queued request of every idle session whose window is (made) resident
+    const auto t0 = Clock::now();
     for (auto& kv : s->sessions) {
         Session& se = kv.second;
         if (se.running || se.queue.empty()) continue;
         int st = VIB_OK;
         if (!make_resident(s, se, &st)) {
-            if (st != VIB_OK) finish(s, se, std::move(se.queue.front()), st, 0, 0.0), se.queue.pop_front();
+            if (st != VIB_OK) {
+                finish(s->workers[se.worker], se, std::move(se.queue.front()), st, 0, 0.0);
+                se.queue.pop_front();
+            }
             continue;
         }
-        std::unique_ptr<Request> r = std::move(se.queue.front());
+        se.running = std::move(se.queue.front());
         se.queue.pop_front();
-        r->admitted = Clock::now();
-        if (r->has_keyframe) {
-            vib_keyframe k = r->keyframe.view();
-            st = vib_window_append(s->ctx, se.slot, &k, &r->unmatched);
-            s->stats.appends++;
-            r->keyframe = OwnedKeyframe();  // the window holds it now
-        }
-        vib_solve_options o{};
-        o.max_iterations = r->iterations;
-        o.min_iterations = 0;
-        o.initial_lambda = 1e-4;
-        o.precision = s->opt.precision;
-        o.refinement_steps = s->opt.refinement_steps;
-        o.flags = s->opt.full_iterations ? VIB_SOLVE_CONTINUOUS : 0u;
-        if (st == VIB_OK) st = vib_request_begin(s->ctx, se.slot, &o);
-        if (st != VIB_OK) {
-            finish(s, se, std::move(r), st, 0, 0.0);
-            continue;
-        }
-        se.running = std::move(r);
+        se.running->admitted = Clock::now();
+        se.running->starting = true;
     }
-    s->stats.admit_seconds += since(t_admit);
-    // 2. one LM iteration of the running requests of one size bucket (buckets in turn)
-    std::map<int, std::vector<Session*>> by_bucket;
-    for (auto& kv : s->sessions) {
-        Session& se = kv.second;
-        if (!se.running) continue;
-        int32_t nr = 0, nv = 0;
-        vib_retained_dimension(s->ctx, se.slot, &nr, &nv);
-        by_bucket[bucket_of(s, nr)].push_back(&se);
-    }
-    if (!by_bucket.empty()) {
-        std::vector<int> keys;
-        for (auto& kv : by_bucket) keys.push_back(kv.first);
-        const int b = keys[s->turn++ % keys.size()];
-        std::vector<Session*>& run = by_bucket[b];
-        std::vector<int32_t> ids, dn(run.size());
-        std::vector<vib_solve_report> reps(run.size());
-        for (Session* se : run) ids.push_back(se->slot);
-        const auto t_iter = Clock::now();
-        const int st = vib_request_iterate(s->ctx, (int32_t)ids.size(), ids.data(), s->opt.precision,
-                                           s->opt.refinement_steps, dn.data(), reps.data());
-        s->stats.iterate_seconds += since(t_iter);
-        s->stats.launches++;
-        s->stats.window_iterations += (int64_t)ids.size();
-        // 3. the finished requests' slides back to their windows' sizes (FixedLagBackend.cpp:434-451),
-        //    all in one batch
-        std::vector<size_t> fin, slide;
-        std::vector<int> status(run.size(), VIB_OK);
-        for (size_t i = 0; i < run.size(); ++i) {
-            if (st == VIB_OK && !dn[i]) continue;
-            fin.push_back(i);
-            status[i] = st != VIB_OK ? st : reps[i].status;
-            int32_t K = 0, L, F, E, rows, cols;
-            if (status[i] == VIB_OK && s->opt.slide_policy >= 0 &&
-                vib_window_shape(s->ctx, run[i]->slot, &K, &L, &F, &E, &rows, &cols) == VIB_OK && K > run[i]->frames)
-                slide.push_back(i);
-        }
-        if (!slide.empty()) {
-            std::vector<int32_t> sl;
-            for (size_t i : slide) sl.push_back(run[i]->slot);
-            std::vector<vib_marginal_report> mr(slide.size());
-            const auto t_slide = Clock::now();
-            (void)vib_marginalize_batch(s->ctx, (int32_t)sl.size(), sl.data(), s->opt.slide_policy, mr.data());
-            s->stats.slide_seconds += since(t_slide);
-            s->stats.slides += (int64_t)slide.size();
-            s->stats.slide_launches++;
-            for (size_t q = 0; q < slide.size(); ++q) status[slide[q]] = mr[q].status;
-        }
-        for (size_t i : fin)
-            finish(s, *run[i], std::move(run[i]->running), status[i], st == VIB_OK ? reps[i].iterations : 0,
-                   st == VIB_OK ? reps[i].final_cost : 0.0);
+    s->residency_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
+    if (s->workers.size() == 1) turn_worker(s, s->workers[0]);
+    else s->pool.run([s](int i) { turn_worker(s, s->workers[i]); });
+    for (Worker& w : s->workers) {
+        for (const vib_served& r : w.done) s->done.push_back(r);
+        w.done.clear();
     }
     while (*count < capacity && !s->done.empty()) {
         finished[(*count)++] = s->done.front();
@@ -450,27 +591,39 @@ int vib_server_preload(vib_server* s) {
 
 int vib_server_get_stats(const vib_server* s, vib_server_stats* out) {
     if (!s || !out) return VIB_ERR_INVALID_ARGUMENT;
-    *out = s->stats;
-    out->queued = 0;
-    out->running = 0;
-    out->resident = 0;
+    *out = vib_server_stats{};
+    for (const Worker& w : s->workers) {  // counters and phase times summed over the workers
+        const vib_server_stats& x = w.stats;
+        out->submitted += x.submitted; out->finished += x.finished; out->failed += x.failed;
+        out->launches += x.launches; out->window_iterations += x.window_iterations; out->uploads += x.uploads;
+        out->evictions += x.evictions; out->appends += x.appends; out->slides += x.slides;
+        out->slide_launches += x.slide_launches; out->admit_seconds += x.admit_seconds;
+        out->iterate_seconds += x.iterate_seconds; out->slide_seconds += x.slide_seconds;
+        int64_t bytes = 0;
+        vib_resident_bytes(w.ctx, &bytes);
+        out->resident_bytes += bytes;
+    }
+    out->turns = s->turns;
+    out->admit_seconds += s->residency_seconds;
     for (const auto& kv : s->sessions) {
         out->queued += (int32_t)kv.second.queue.size();
         out->running += kv.second.running ? 1 : 0;
         out->resident += kv.second.slot >= 0 ? 1 : 0;
     }
     out->pending_results = (int32_t)s->done.size();
-    int64_t bytes = 0;
-    vib_resident_bytes(s->ctx, &bytes);
-    out->resident_bytes = bytes;
     return VIB_OK;
 }
 
 int vib_server_session_slot(const vib_server* s, int32_t session, int32_t* slot) {
+    return vib_server_session_context(s, session, nullptr, slot);
+}
+
+int vib_server_session_context(const vib_server* s, int32_t session, vib_context** ctx, int32_t* slot) {
     if (!s || !slot) return VIB_ERR_INVALID_ARGUMENT;
     auto it = s->sessions.find(session);
     if (it == s->sessions.end()) return VIB_ERR_NOT_FOUND;
     *slot = it->second.slot;
+    if (ctx) *ctx = s->workers[it->second.worker].ctx;
     return VIB_OK;
 }
 

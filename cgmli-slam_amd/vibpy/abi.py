@@ -82,7 +82,7 @@ class VibTuneReport(C.Structure):
 class VibServerOptions(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("precision", C.c_int32), ("refinement_steps", C.c_int32),
                 ("slide_policy", C.c_int32), ("full_iterations", C.c_int32), ("first_slot", C.c_int32),
-                ("num_buckets", C.c_int32), ("bucket_limit", C.c_int32 * 8)]
+                ("num_buckets", C.c_int32), ("bucket_limit", C.c_int32 * 8), ("workers", C.c_int32)]
 
 
 class VibServed(C.Structure):
@@ -378,6 +378,8 @@ def load_library(path=LIB_PATH):
         "vib_server_get_stats": ([vp, C.POINTER(VibServerStats)], C.c_int),
         "vib_server_preload": ([vp], C.c_int),
         "vib_server_session_slot": ([vp, C.c_int32, ip], C.c_int),
+        "vib_server_session_context": ([vp, C.c_int32, C.POINTER(vp), ip], C.c_int),
+        "vib_context_device": ([vp, ip], C.c_int),
         "vib_create": ([C.POINTER(vp), C.c_int], C.c_int),
         "vib_destroy": ([vp], C.c_int),
         "vib_last_error": ([vp], C.c_char_p),
@@ -570,11 +572,23 @@ class Context:
         if st != VIB_OK:
             raise RuntimeError(f"vib_create failed: {STATUS_NAMES.get(st, st)}")
         self.h = h
+        self._owned = True
         self._cams = {}  # cameras per uploaded slot (the shape query does not report them)
+
+    @classmethod
+    def borrowed(cls, handle, lib=None):
+        """A non-owning view of a context that something else created (a server's worker)."""
+        self = cls.__new__(cls)
+        self.lib = lib or load_library()
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self._owned = False
+        self._cams = {}
+        return self
 
     def close(self):
         if self.h:
-            self.lib.vib_destroy(self.h)
+            if self._owned:
+                self.lib.vib_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -15,9 +15,10 @@ from . import abi
 
 class Server:
     def __init__(self, ctx, windows, hbm_budget_bytes=0, buckets=(128, 224, 320), precision=abi.VIB_FP64,
-                 refinement=0, slide_policy=abi.VIB_MARGIN_OLD, full_iterations=False, first_slot=5000):
+                 refinement=0, slide_policy=abi.VIB_MARGIN_OLD, full_iterations=False, first_slot=5000, workers=1):
         """windows: the sessions' first windows (session i = windows[i]). full_iterations: every
-        request runs its whole iteration budget, relinearizing each time."""
+        request runs its whole iteration budget, relinearizing each time. workers: contexts that
+        serve disjoint sessions concurrently (worker 0 is ctx)."""
         self.ctx = ctx
         self.lib = ctx.lib
         o = abi.VibServerOptions()
@@ -27,6 +28,7 @@ class Server:
         o.slide_policy = slide_policy
         o.full_iterations = 1 if full_iterations else 0
         o.first_slot = first_slot
+        o.workers = int(workers)
         o.num_buckets = len(buckets)
         for i, b in enumerate(buckets):
             o.bucket_limit[i] = int(b)
@@ -67,6 +69,13 @@ class Server:
         v = C.c_int32()
         self.ctx.check(self.lib.vib_server_session_slot(self.h, int(session), C.byref(v)), "vib_server_session_slot")
         return int(v.value)
+
+    def session_view(self, session):
+        """(context, slot) holding the session's window: a non-owning Context of its worker."""
+        h, v = C.c_void_p(), C.c_int32()
+        self.ctx.check(self.lib.vib_server_session_context(self.h, int(session), C.byref(h), C.byref(v)),
+                       "vib_server_session_context")
+        return abi.Context.borrowed(h, self.lib), int(v.value)
 
     def run(self, requests, keyframes=None, on_done=None, clock=time.perf_counter):
         """Serves `requests` (dicts with id, session, iterations, arrival in seconds from the start;

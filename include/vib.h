@@ -24,7 +24,8 @@
  * vib_last_error() gives the message. No function throws.
  *
  * Threading: every entry point takes the context's lock, so threads may share a context; the
- * request server (vib_server_*) is a client of these entry points driven by one thread.
+ * request server (vib_server_*) is a client of these entry points driven by one thread, and runs
+ * its workers' contexts on threads of its own.
  */
 #ifndef VIB_H
 #define VIB_H
@@ -35,7 +36,7 @@
 extern "C" {
 #endif
 
-#define VIB_ABI_VERSION 2
+#define VIB_ABI_VERSION 3
 
 /* ---- status codes (reference fallback classes, src/backend/BackendOptimizer.cpp) ---- */
 enum {
@@ -265,6 +266,8 @@ typedef struct vib_context vib_context;
 /* ---------------- context ---------------- */
 int vib_abi_version(void);
 int vib_create(vib_context** ctx, int device);
+/* The device a context was created on. */
+int vib_context_device(const vib_context* ctx, int32_t* device);
 int vib_destroy(vib_context* ctx);
 /* Wait for all work of the context's device (hipDeviceSynchronize). */
 int vib_device_synchronize(vib_context* ctx);
@@ -370,10 +373,14 @@ int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* slots, i
  * sessions with nothing queued are evicted to host memory, least recently used first), (2) runs
  * one LM iteration for the running requests of one size bucket (the buckets take turns), and (3)
  * slides and reports the requests that finished. Requests submitted while others run join at
- * the next turn. The server uses context slots from first_slot on; one thread drives it. */
+ * the next turn. The server uses context slots from first_slot on; one thread drives it.
+ * With several workers every session belongs to one worker (the least loaded by window
+ * footprint when it is added) and a turn runs the workers' parts concurrently. */
 typedef struct vib_server vib_server;
 typedef struct vib_server_options {
-    int64_t hbm_budget;          /* bytes of resident windows the server admits (0: no limit)  */
+    int64_t hbm_budget;          /* bytes of resident windows the server admits (0: no limit); a
+                                    request whose window alone exceeds it fails with
+                                    VIB_ERR_OUT_OF_MEMORY                                      */
     int32_t precision;           /* VIB_FP64 or VIB_MIXED                                       */
     int32_t refinement_steps;    /* refinement passes of a mixed-precision step                */
     int32_t slide_policy;        /* VIB_MARGIN_OLD or VIB_MARGIN_SECOND_NEW after a request that
@@ -382,7 +389,13 @@ typedef struct vib_server_options {
     int32_t first_slot;          /* context slots first_slot, first_slot + 1, ... are the server's */
     int32_t num_buckets;         /* size buckets by retained dimension (<= 8; 0: one bucket)   */
     int32_t bucket_limit[8];     /* largest retained dimension of each bucket (ascending)      */
+    int32_t workers;             /* contexts serving disjoint sessions concurrently, each on a
+                                    thread and stream of its own (<= VIB_SERVER_MAX_WORKERS;
+                                    0 or 1: the caller's context alone). Worker 0 is the
+                                    caller's context, the others are created on its device;
+                                    hbm_budget bounds the windows resident in all of them.   */
 } vib_server_options;
+#define VIB_SERVER_MAX_WORKERS 16
 typedef struct vib_served {      /* one finished request */
     int64_t request;
     int32_t session;
@@ -420,6 +433,8 @@ int vib_server_preload(vib_server* server);
 int vib_server_get_stats(const vib_server* server, vib_server_stats* stats);
 /* The context slot holding the session's window, -1 while it is evicted. */
 int vib_server_session_slot(const vib_server* server, int32_t session, int32_t* slot);
+/* The same, with the worker context the slot belongs to (the caller's context with one worker). */
+int vib_server_session_context(const vib_server* server, int32_t session, vib_context** ctx, int32_t* slot);
 
 /* ---------------- measurement harness (no counterpart in the reference) ---------------- */
 /* Number of distinct kernels of one LM iteration and their names (for per-kernel timing). */

@@ -22,7 +22,7 @@ def test_library_exports_every_declared_symbol(abi):
     exported = set(re.findall(r"\bT (vib_[a-z_0-9]+)", out))
     missing = [f for f in declared_functions() if f not in exported]
     assert not missing, missing
-    assert lib.vib_abi_version() == 2
+    assert lib.vib_abi_version() == 3
 
 
 def test_create_without_device_fails_cleanly(abi):

@@ -2,7 +2,8 @@
 
 Six sessions of uneven size stream keyframes; requests (append the session's next keyframe, 1-3
 LM iterations, slide OLD back to the window size) arrive as Poisson while others run and join
-the running launches of their size bucket. An HBM budget below the sessions' footprint forces
+the running launches of their size bucket. With several workers the sessions are spread over
+contexts whose turns run concurrently on threads and streams of their own. An HBM budget below the sessions' footprint forces
 evictions (window read back to host memory) and re-uploads. Every request's result (status,
 final cost, iterations) and the session's window right after it (poses, inverse ranges, shape)
 equal the same request sequence of that session served alone through the C-ABI (append,
@@ -33,8 +34,8 @@ def _streams(synth, reqs, n):
     return out
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_served_requests_equal_alone(ctx, abi, precision):
+@pytest.mark.parametrize("precision,workers", [("fp64", 1), ("mixed", 1), ("fp64", 3)])
+def test_served_requests_equal_alone(ctx, abi, precision, workers):
     from vibpy import serve, synth
     prec = abi.VIB_FP64 if precision == "fp64" else abi.VIB_MIXED
     refine = 2 if prec == abi.VIB_MIXED else 0
@@ -45,22 +46,23 @@ def test_served_requests_equal_alone(ctx, abi, precision):
     wins = [w for w, _ in streams]
     kfs = [k for _, k in streams]
     budget = sum(sorted(abi.window_footprint(w, prec) for w in wins)[-4:])
-    srv = serve.Server(ctx, wins, budget, buckets=(64, 128, 320), precision=prec, refinement=refine)
+    srv = serve.Server(ctx, wins, budget, buckets=(64, 128, 320), precision=prec, refinement=refine, workers=workers)
     snaps = {}
 
     def on_done(res):
-        slot = srv.slot(res["session"])
+        view, slot = srv.session_view(res["session"])
         cams = wins[res["session"]].cam_rotation.shape[0]
-        snaps[res["id"]] = (res, ctx.read_window(slot, cams) if slot >= 0 else None)
+        snaps[res["id"]] = (res, view.read_window(slot, cams) if slot >= 0 else None)
 
-    say(f"[gpu] serving {len(reqs)} requests of {n} sessions ({precision})")
+    say(f"[gpu] serving {len(reqs)} requests of {n} sessions ({precision}, {workers} worker(s))")
     try:
         done = srv.run([dict(r) for r in reqs], keyframes=kfs, on_done=on_done)
         st = srv.stats()
     finally:
         srv.close()
     assert len(done) == len(reqs) and st["failed"] == 0, [r["status"] for r in done if r["status"]]
-    assert st["evictions"] > 0 and st["slides"] == len(reqs) and st["appends"] == len(reqs)
+    assert st["slides"] == len(reqs) and st["appends"] == len(reqs)
+    assert st["evictions"] > 0 or workers > 1  # one worker: the budget holds four of the six windows
     say(f"[gpu] served: {st['uploads']} uploads, {st['evictions']} evictions, {st['launches']} launches; replaying")
     for i in range(n):
         ctx.upload(0, wins[i])
