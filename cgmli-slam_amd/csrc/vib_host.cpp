@@ -889,13 +889,15 @@ int fail(vib_context* ctx, int code, const std::string& msg) {
     return code;
 }
 
-// A grow-only device buffer of at least `bytes` (null when the allocation fails).
+// A grow-only device buffer of at least `bytes` (null when the allocation fails). It grows by at
+// least half its size: the windows of many sessions pass through one buffer, and a reallocation
+// (hipFree waits for the whole device) each time a larger one comes would stall every stream.
 void* grow_device(void*& buf, size_t& cap, size_t bytes) {
     if (bytes <= cap && buf) return buf;
+    const size_t want = std::max({bytes, cap + cap / 2, (size_t)1 << 16});
     if (buf) (void)hipFree(buf);
     buf = nullptr;
     cap = 0;
-    const size_t want = std::max(bytes, (size_t)1 << 16);
     if (hipMalloc(&buf, want) != hipSuccess) {
         buf = nullptr;
         (void)hipGetLastError();
@@ -1288,13 +1290,15 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
     } else {
         dst.used = false;
         if (!dst.arena || need > dst.capacity) {
+            // the marginal systems of many sessions pass through one slot: room to grow
+            const size_t want = std::max(need, dst.capacity + dst.capacity / 2);
             dst.release();
-            if (hipMalloc(&dst.arena, need) != hipSuccess) {
+            if (hipMalloc(&dst.arena, want) != hipSuccess) {
                 dst.arena = nullptr;
                 (void)hipGetLastError();
                 return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "window edit allocation failed");
             }
-            dst.capacity = need;
+            dst.capacity = want;
         }
         base = static_cast<char*>(dst.arena);
     }
@@ -1314,8 +1318,25 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
         return fail(ctx, VIB_ERR_DEVICE, "window edit failed");
     }
     if (in_place) {
-        std::swap(dst.arena, ctx->spare);
-        std::swap(dst.capacity, ctx->spare_cap);
+        if (need <= dst.capacity) {
+            // the slot keeps its arena (and its footprint): the edited window is copied into it
+            // in stream order, at the offsets the spare arena had (the layout depends only on
+            // the window), and the spare stays the context's largest
+            WinDev nd2;
+            SlotCarve x2;
+            if (!Slot::carve(meta, static_cast<char*>(dst.arena), dst.capacity, nd2, x2))
+                return fail(ctx, VIB_ERR_DEVICE, "arena overflow");
+            if (hipMemcpyAsync(dst.arena, base, need, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+                (void)hipGetLastError();
+                dst.used = false;  // the arena may hold part of the edited window
+                return fail(ctx, VIB_ERR_DEVICE, "window edit failed");
+            }
+            nd = nd2;
+            x = x2;
+        } else {
+            std::swap(dst.arena, ctx->spare);
+            std::swap(dst.capacity, ctx->spare_cap);
+        }
     }
     dst.dev = nd;
     dst.reduced_lin = x.reduced_lin;
@@ -2209,6 +2230,15 @@ int vib_resident_bytes(vib_context* ctx, int64_t* bytes) {
     for (const auto& kv : ctx->slots)
         if (kv.second && kv.second->used) total += kv.second->capacity;
     *bytes = (int64_t)total;
+    return VIB_OK;
+}
+
+int vib_slot_bytes(vib_context* ctx, int32_t slot, int64_t* bytes) {
+    if (!ctx || !bytes) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    Slot* s = find_slot(ctx, slot);
+    if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
+    *bytes = (int64_t)s->capacity;
     return VIB_OK;
 }
 

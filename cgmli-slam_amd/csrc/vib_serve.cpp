@@ -327,12 +327,12 @@ bool make_resident(vib_server* s, Session& se, int* status) {
             *status = VIB_ERR_OUT_OF_MEMORY;
             return false;
         }
-        auto resident = [&]() {
+        auto resident = [&]() {  // the server's own windows (other slots of the contexts do not count)
             int64_t total = 0;
-            for (const Worker& x : s->workers) {
+            for (const auto& kv : s->sessions) {
                 int64_t b = 0;
-                vib_resident_bytes(x.ctx, &b);
-                total += b;
+                if (kv.second.slot >= 0 && vib_slot_bytes(s->workers[kv.second.worker].ctx, kv.second.slot, &b) == VIB_OK)
+                    total += b;
             }
             return total;
         };
@@ -599,9 +599,11 @@ int vib_server_get_stats(const vib_server* s, vib_server_stats* out) {
         out->evictions += x.evictions; out->appends += x.appends; out->slides += x.slides;
         out->slide_launches += x.slide_launches; out->admit_seconds += x.admit_seconds;
         out->iterate_seconds += x.iterate_seconds; out->slide_seconds += x.slide_seconds;
-        int64_t bytes = 0;
-        vib_resident_bytes(w.ctx, &bytes);
-        out->resident_bytes += bytes;
+    }
+    for (const auto& kv : s->sessions) {
+        int64_t b = 0;
+        if (kv.second.slot >= 0 && vib_slot_bytes(s->workers[kv.second.worker].ctx, kv.second.slot, &b) == VIB_OK)
+            out->resident_bytes += b;
     }
     out->turns = s->turns;
     out->admit_seconds += s->residency_seconds;

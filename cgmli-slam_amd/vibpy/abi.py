@@ -380,6 +380,7 @@ def load_library(path=LIB_PATH):
         "vib_server_session_slot": ([vp, C.c_int32, ip], C.c_int),
         "vib_server_session_context": ([vp, C.c_int32, C.POINTER(vp), ip], C.c_int),
         "vib_context_device": ([vp, ip], C.c_int),
+        "vib_slot_bytes": ([vp, C.c_int32, C.POINTER(C.c_int64)], C.c_int),
         "vib_create": ([C.POINTER(vp), C.c_int], C.c_int),
         "vib_destroy": ([vp], C.c_int),
         "vib_last_error": ([vp], C.c_char_p),

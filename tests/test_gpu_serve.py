@@ -85,3 +85,24 @@ def test_served_requests_equal_alone(ctx, abi, precision, workers):
             np.testing.assert_array_equal(alone.lm_id, win.lm_id)
             assert np.abs(alone.pose - win.pose).max() < 1e-7
             assert np.abs(alone.lm_rho - win.lm_rho).max() < 1e-7 * max(1.0, np.abs(win.lm_rho).max())
+
+
+def test_window_over_the_budget_fails_instead_of_waiting(ctx, abi):
+    """A request whose window alone exceeds the server's HBM budget finishes with
+    VIB_ERR_OUT_OF_MEMORY at its first turn; windows that other slots of the context hold do not
+    count against the server's budget."""
+    from vibpy import serve, synth
+    small, kf_small = synth.stream(6, 8, 80, n_cams=2, seed=301, obs_range=(2, 6))
+    large, kf_large = synth.stream(9, 11, 400, n_cams=3, seed=302, obs_range=(2, 6))
+    fs = abi.window_footprint(small, abi.VIB_FP64)
+    fl = abi.window_footprint(large, abi.VIB_FP64)
+    assert fl > 2 * fs
+    ctx.upload(0, large)  # a slot outside the server, larger than the budget
+    srv = serve.Server(ctx, [small, large], hbm_budget_bytes=fl - 1, workers=2)
+    try:
+        reqs = [dict(id=1, session=0, iterations=1, arrival=0.0), dict(id=2, session=1, iterations=1, arrival=0.0)]
+        done = {r["id"]: r for r in srv.run(reqs, keyframes=[kf_small, kf_large])}
+    finally:
+        srv.close()
+    assert done[1]["status"] == abi.VIB_OK
+    assert done[2]["status"] == abi.VIB_ERR_OUT_OF_MEMORY
