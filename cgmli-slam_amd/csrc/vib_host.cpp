@@ -379,6 +379,25 @@ bool HostWindow::layout_states(std::string& err) {
 // ------------------------------------------------------------------------------------------
 // Slot: device arena holding one window.
 // ------------------------------------------------------------------------------------------
+bool PinnedBuffer::reserve(size_t bytes) {
+    if (bytes <= cap && p) return true;
+    const size_t want = std::max({bytes, cap + cap / 2, (size_t)1 << 16});
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), want, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return false;
+    }
+    cap = want;
+    return true;
+}
+
+PinnedBuffer::~PinnedBuffer() {
+    if (p) (void)hipHostFree(p);
+}
+
 Staging::~Staging() {
     if (buf) (void)hipFree(buf);
     for (hipEvent_t e : done)
@@ -441,10 +460,11 @@ bool CopyBatch::flush(hipStream_t s, Staging& st) {
     }
     if (st.recorded[k] && hipEventSynchronize(st.done[k]) != hipSuccess) return drop();  // the transfer that last read host[k]
     char* dev = static_cast<char*>(st.buf);
-    std::vector<char>& h = st.host[k];
-    h.assign(seg_off + n * sizeof(CopySeg), 0);
-    if (!pay_.empty()) std::memcpy(h.data(), pay_.data(), pay_.size());
-    CopySeg* seg = reinterpret_cast<CopySeg*>(h.data() + seg_off);
+    const size_t hbytes = seg_off + n * sizeof(CopySeg);
+    if (!st.host[k].reserve(hbytes)) return drop();
+    char* h = st.host[k].p;
+    if (!pay_.empty()) std::memcpy(h, pay_.data(), pay_.size());
+    CopySeg* seg = reinterpret_cast<CopySeg*>(h + seg_off);
     for (size_t i = 0; i < nu; ++i)
         seg[i] = CopySeg{(uint64_t)(uintptr_t)ups_[i].dst,
                          ups_[i].off == SIZE_MAX ? 0 : (uint64_t)(uintptr_t)(dev + ups_[i].off), ups_[i].bytes};
@@ -452,15 +472,15 @@ bool CopyBatch::flush(hipStream_t s, Staging& st) {
         seg[nu + j] = CopySeg{(uint64_t)(uintptr_t)(dev + down_off + doff[j]), (uint64_t)(uintptr_t)downs_[j].dev,
                               downs_[j].bytes};
     const CopySeg* dseg = reinterpret_cast<const CopySeg*>(dev + seg_off);
-    bool ok = hipMemcpyAsync(dev, h.data(), h.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+    bool ok = hipMemcpyAsync(dev, h, hbytes, hipMemcpyHostToDevice, s) == hipSuccess &&
               hipEventRecord(st.done[k], s) == hipSuccess && (st.recorded[k] = true) &&
               vibk_copy_segments(dseg, (int)nu, max_up, s) == hipSuccess &&
               vibk_copy_segments(dseg + nu, (int)nd, max_down, s) == hipSuccess;
     if (ok && nd) {
-        st.back.resize(dbytes);
-        ok = hipMemcpyAsync(st.back.data(), dev + down_off, dbytes, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        ok = st.back.reserve(dbytes) &&
+             hipMemcpyAsync(st.back.p, dev + down_off, dbytes, hipMemcpyDeviceToHost, s) == hipSuccess &&
              hipStreamSynchronize(s) == hipSuccess;
-        for (size_t j = 0; ok && j < nd; ++j) std::memcpy(downs_[j].host, st.back.data() + doff[j], downs_[j].bytes);
+        for (size_t j = 0; ok && j < nd; ++j) std::memcpy(downs_[j].host, st.back.p + doff[j], downs_[j].bytes);
     }
     if (!ok) return drop();
     ups_.clear();
@@ -681,7 +701,7 @@ Batch::~Batch() {
     if (graph) hipGraphDestroy(graph);
 }
 
-bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err) {
+bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, Staging& xs, std::string& err) {
     nwin = (int)slots.size();
     std::vector<WinDev> wins(nwin);
     std::vector<Chunk> all;
@@ -763,11 +783,12 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, std::string& er
         !grow(d_red, cap_red, (size_t)nwin, 2 * sizeof(double*)) ||
         !grow(d_chunks, cap_chunks, (size_t)std::max(1, nchunks), sizeof(Chunk)) ||
         !grow(d_lin, cap_lin, (size_t)std::max(1, nlin), sizeof(Chunk))) { err = "batch allocation failed"; return false; }
-    if (hipMemcpyAsync(d_wins, wins.data(), sizeof(WinDev) * nwin, hipMemcpyHostToDevice, stream) != hipSuccess ||
-        (nchunks && hipMemcpyAsync(d_chunks, all.data(), sizeof(Chunk) * nchunks, hipMemcpyHostToDevice, stream) != hipSuccess) ||
-        (nlin && hipMemcpyAsync(d_lin, lin.data(), sizeof(Chunk) * nlin, hipMemcpyHostToDevice, stream) != hipSuccess) ||
-        hipMemcpyAsync(d_red, red.data(), sizeof(double*) * 2 * nwin, hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess) { err = "batch upload failed"; return false; }
+    CopyBatch cb;  // descriptors and work lists in one transfer, in stream order before the launches
+    cb.up(d_wins, wins.data(), sizeof(WinDev) * nwin);
+    cb.up(d_chunks, all.data(), sizeof(Chunk) * nchunks);
+    cb.up(d_lin, lin.data(), sizeof(Chunk) * nlin);
+    cb.up(d_red, red.data(), sizeof(double*) * 2 * nwin);
+    if (!cb.flush(stream, xs)) { err = "batch upload failed"; return false; }
     return true;
 }
 
@@ -945,12 +966,15 @@ const char* status_message(int code) {
 }
 
 bool read_lm(vib_context* ctx, Slot& s, LmState& lm) {
-    return hipMemcpyAsync(&lm, s.dev.lm, sizeof(LmState), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-           hipStreamSynchronize(ctx->stream) == hipSuccess;
+    CopyBatch cb;
+    cb.down(&lm, s.dev.lm, sizeof(LmState));
+    return cb.flush(ctx->stream, ctx->xfer);
 }
 
 bool write_lm(vib_context* ctx, Slot& s, const LmState& lm) {
-    return hipMemcpyAsync(s.dev.lm, &lm, sizeof(LmState), hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
+    CopyBatch cb;
+    cb.up(s.dev.lm, &lm, sizeof(LmState));
+    return cb.flush(ctx->stream, ctx->xfer);
 }
 
 LmState fresh_lm(const vib_solve_options& o) {
@@ -994,7 +1018,7 @@ int run_lm(vib_context* ctx, std::vector<Slot*>& slots, const vib_solve_options&
     batch.mixed = o.precision == VIB_MIXED;
     apply_tuning(ctx, batch, slots);
     std::string err;
-    if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    if (!batch.build(slots, ctx->stream, ctx->xfer, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     const bool graph = ctx->use_graph && batch.capture(ctx->stream, o.refinement_steps);
     std::vector<LmState> states(slots.size());
     const auto t_iter = std::chrono::steady_clock::now();  // the time budget covers the iterations
@@ -1357,7 +1381,7 @@ bool imu_nis(vib_context* ctx, Slot& s, const int edges[2], double nis[2]) {
     b.mixed = 0;
     std::string err;
     std::vector<double> r((size_t)15 * s.host.E);
-    if (!write_lm(ctx, s, lm) || !b.build(one, ctx->stream, err) || !b.launch(1, ctx->stream, 0, 0) ||
+    if (!write_lm(ctx, s, lm) || !b.build(one, ctx->stream, ctx->xfer, err) || !b.launch(1, ctx->stream, 0, 0) ||
         hipMemcpyAsync(r.data(), s.dev.imu_r, r.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return false;
@@ -1493,16 +1517,18 @@ int device_eigen_batch(vib_context* ctx, const std::vector<const std::vector<dou
         q.n = q.ld = n[j];
         V[j].resize(nn);
         e[j].resize(n[j]);
-        ok = ok && hipMemcpyAsync(q.A, M[j]->data(), 8 * nn, hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
     }
+    CopyBatch cb;  // matrices and job list in, eigenvectors, eigenvalues and flags out: one transfer each way
+    for (size_t j = 0; j < J; ++j) cb.up(jobs[j].A, M[j]->data(), 8 * (size_t)n[j] * n[j]);
+    cb.up(buf, jobs.data(), sizeof(EigenJob) * J);
     const EigenJob* djobs = reinterpret_cast<const EigenJob*>(buf);
-    ok = ok && hipMemcpyAsync(buf, jobs.data(), sizeof(EigenJob) * J, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
-         vibk_sym_eigen_batch(djobs, (int)J, max_n, ctx->stream) == hipSuccess;
-    for (size_t j = 0; j < J && ok; ++j)
-        ok = hipMemcpyAsync(V[j].data(), jobs[j].V, 8 * V[j].size(), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-             hipMemcpyAsync(e[j].data(), jobs[j].ev, 8 * (size_t)n[j], hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-             hipMemcpyAsync(&conv[j], jobs[j].ok, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess;
-    ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
+    ok = cb.flush(ctx->stream, ctx->xfer) && vibk_sym_eigen_batch(djobs, (int)J, max_n, ctx->stream) == hipSuccess;
+    for (size_t j = 0; j < J; ++j) {
+        cb.down(V[j].data(), jobs[j].V, 8 * V[j].size());
+        cb.down(e[j].data(), jobs[j].ev, 8 * (size_t)n[j]);
+        cb.down(&conv[j], jobs[j].ok, sizeof(int32_t));
+    }
+    ok = ok && cb.flush(ctx->stream, ctx->xfer);
     if (!ok) return fail(ctx, VIB_ERR_DEVICE, "device eigendecomposition failed");
     for (size_t j = 0; j < J; ++j) {
         const int m = n[j];
@@ -1628,39 +1654,39 @@ int schur_priors(vib_context* ctx, std::vector<PriorJob>& jobs) {
     Batch& b = ctx->one;
     b.mixed = 0;
     std::string err;
-    if (!b.build(slots, ctx->stream, err) || !b.linearize(ctx->stream))
+    if (!b.build(slots, ctx->stream, ctx->xfer, err) || !b.linearize(ctx->stream))
         return fail(ctx, VIB_ERR_DEVICE, "marginal linearization failed");
     std::vector<std::vector<double>> dvals(live.size());
-    bool ok = true;
-    for (size_t i = 0; i < live.size() && ok; ++i) {
+    CopyBatch cb;
+    for (size_t i = 0; i < live.size(); ++i) {
         const Slot& t = *live[i]->marg;
         dvals[i].resize(t.host.L);
-        ok = t.host.L == 0 || hipMemcpyAsync(dvals[i].data(), t.dev.lm_out + (size_t)LO_D * t.host.L, 8 * (size_t)t.host.L,
-                                             hipMemcpyDeviceToHost, ctx->stream) == hipSuccess;
+        cb.down(dvals[i].data(), t.dev.lm_out + (size_t)LO_D * t.host.L, 8 * (size_t)t.host.L);
     }
-    if (!ok || hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
+    if (!cb.flush(ctx->stream, ctx->xfer)) return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
+    std::vector<LmState> l2(live.size(), lm);
     for (size_t i = 0; i < live.size(); ++i) {
         double dmax = 0;
         for (double d : dvals[i]) dmax = std::max(dmax, d);
-        LmState l2 = lm;
-        l2.marg_cut = std::max(1e-12, 1e-9 * dmax);
-        if (!write_lm(ctx, *live[i]->marg, l2)) return fail(ctx, VIB_ERR_DEVICE, "marginal LM state upload failed");
+        l2[i].marg_cut = std::max(1e-12, 1e-9 * dmax);
+        cb.up(live[i]->marg->dev.lm, &l2[i], sizeof(LmState));
     }
-    ok = b.launch(3, ctx->stream, 0, 0) && b.launch(4, ctx->stream, 0, 0) && b.launch(5, ctx->stream, 0, 0);
-    for (PriorJob* j : live) {
+    bool ok = cb.flush(ctx->stream, ctx->xfer) && b.launch(3, ctx->stream, 0, 0) && b.launch(4, ctx->stream, 0, 0) &&
+              b.launch(5, ctx->stream, 0, 0);
+    std::vector<LmState> chk(live.size());
+    for (size_t i = 0; i < live.size(); ++i) {
+        PriorJob* j = live[i];
         const Slot& t = *j->marg;
         const size_t nr = t.host.nr;
         j->S.resize(nr * nr);
         j->rhs.resize(nr);
-        ok = ok && hipMemcpyAsync(j->S.data(), t.dev.Scopy, 8 * nr * nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-             hipMemcpyAsync(j->rhs.data(), t.dev.rhs, 8 * nr, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess;
+        cb.down(j->S.data(), t.dev.Scopy, 8 * nr * nr);
+        cb.down(j->rhs.data(), t.dev.rhs, 8 * nr);
+        cb.down(&chk[i], t.dev.lm, sizeof(LmState));
     }
-    if (!ok || hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, VIB_ERR_DEVICE, "marginal Schur failed");
-    for (PriorJob* j : live) {
-        LmState chk;
-        if (!read_lm(ctx, *j->marg, chk)) return fail(ctx, VIB_ERR_DEVICE, "marginal read-back failed");
-        if (!chk.lin_valid) job_fail(ctx, *j, VIB_ERR_INVALID_RESIDUAL, "invalid residual in a consumed factor");
-    }
+    if (!ok || !cb.flush(ctx->stream, ctx->xfer)) return fail(ctx, VIB_ERR_DEVICE, "marginal Schur failed");
+    for (size_t i = 0; i < live.size(); ++i)
+        if (!chk[i].lin_valid) job_fail(ctx, *live[i], VIB_ERR_INVALID_RESIDUAL, "invalid residual in a consumed factor");
     // 3. the eliminated state blocks' eigen pseudo-inverses; H_prior = S_rr - S_rm S_mm^+ S_mr
     std::vector<PriorJob*> stage;
     std::vector<std::vector<double>> mats;
@@ -1831,9 +1857,9 @@ void set_prior(HostWindow& m, const SchurPrior& p) {
 // them has an observation.
 bool touched_frames(vib_context* ctx, const EditBufs& b, int L, const uint8_t* d_sel, int& frames, bool& observed) {
     int out[2] = {0, 0};
+    CopyBatch cb;
     const bool ok = vibk_touch_frames(b.src, L, d_sel, b.sel + 8, ctx->stream) == hipSuccess &&
-                    hipMemcpyAsync(out, b.sel + 8, sizeof(out), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-                    hipStreamSynchronize(ctx->stream) == hipSuccess;
+                    (cb.down(out, b.sel + 8, sizeof(out)), cb.flush(ctx->stream, ctx->xfer));
     frames = out[0];
     observed = out[1] != 0;
     return ok;
@@ -2347,7 +2373,7 @@ int vib_linearize(vib_context* ctx, int32_t slot, int32_t precision) {
     Batch& b = ctx->one;
     b.mixed = precision == VIB_MIXED;
     std::string err;
-    if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, err) || !b.linearize(ctx->stream) ||
+    if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, ctx->xfer, err) || !b.linearize(ctx->stream) ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(ctx, VIB_ERR_DEVICE, "linearization launch failed");
     s->lin_precision = precision;
@@ -2370,7 +2396,7 @@ int vib_solve_step(vib_context* ctx, int32_t slot, double lambda, int32_t precis
     Batch& b = ctx->one;
     b.mixed = precision == VIB_MIXED;
     std::string err;
-    if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, err) || !b.step(ctx->stream, refinement, 1) ||
+    if (!write_lm(ctx, *s, lm) || !b.build(one, ctx->stream, ctx->xfer, err) || !b.step(ctx->stream, refinement, 1) ||
         !read_lm(ctx, *s, lm))
         return fail(ctx, VIB_ERR_DEVICE, "step launch failed");
     rep->cost = lm.cost;
@@ -2398,7 +2424,7 @@ int vib_accept(vib_context* ctx, int32_t slot) {
     Batch& b = ctx->one;
     b.mixed = 0;
     std::string err;
-    if (!b.build(one, ctx->stream, err) || !b.launch(10, ctx->stream, 0, 2) || !b.launch(11, ctx->stream, 0, 0) ||
+    if (!b.build(one, ctx->stream, ctx->xfer, err) || !b.launch(10, ctx->stream, 0, 2) || !b.launch(11, ctx->stream, 0, 0) ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(ctx, VIB_ERR_DEVICE, "accept failed");
     return VIB_OK;
@@ -2509,14 +2535,13 @@ int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* ids, int
     b.mixed = precision == VIB_MIXED;
     apply_tuning(ctx, b, slots);
     std::string err;
-    if (!b.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    if (!b.build(slots, ctx->stream, ctx->xfer, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     if (!b.iteration(ctx->stream, refinement))
         return fail(ctx, VIB_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
     std::vector<LmState> st(slots.size());
-    for (size_t i = 0; i < slots.size(); ++i)
-        if (hipMemcpyAsync(&st[i], slots[i]->dev.lm, sizeof(LmState), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
-            return fail(ctx, VIB_ERR_DEVICE, "LM state read failed");
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    CopyBatch cb;
+    for (size_t i = 0; i < slots.size(); ++i) cb.down(&st[i], slots[i]->dev.lm, sizeof(LmState));
+    if (!cb.flush(ctx->stream, ctx->xfer))
         return fail(ctx, VIB_ERR_DEVICE, std::string("device failure: ") + hipGetErrorString(hipGetLastError()));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (size_t i = 0; i < slots.size(); ++i) {
@@ -2729,7 +2754,7 @@ static int nis_gate_locked(vib_context* ctx, Slot* s, double threshold, int appl
     Batch& b = ctx->one;
     b.mixed = 0;
     std::string err;
-    if (!b.build(one, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    if (!b.build(one, ctx->stream, ctx->xfer, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     char* buf = static_cast<char*>(grow_device(ctx->aux_buf, ctx->aux_cap, (size_t)F * 9 + kGateStats * sizeof(double)));
     if (!buf) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "gate allocation failed");
     double* d_energy = reinterpret_cast<double*>(buf);
@@ -2922,7 +2947,7 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
     batch.mixed = o->precision == VIB_MIXED;
     apply_tuning(ctx, batch, slots);
     std::string err;
-    if (!batch.build(slots, ctx->stream, err)) return fail(ctx, VIB_ERR_DEVICE, err);
+    if (!batch.build(slots, ctx->stream, ctx->xfer, err)) return fail(ctx, VIB_ERR_DEVICE, err);
     if (!batch.capture(ctx->stream, o->refinement_steps)) return fail(ctx, VIB_ERR_DEVICE, "graph capture failed");
     std::vector<hipEvent_t> ev(iterations + 1);
     for (auto& e : ev)
@@ -3023,7 +3048,7 @@ int vib_autotune(vib_context* ctx, int32_t count, const int32_t* ids, const vib_
             }
             batch.chunk_merge = merges[a];
             batch.lin_merge = merges[b];
-            ok = ok && batch.build(slots, ctx->stream, err) && batch.capture(ctx->stream, o->refinement_steps) &&
+            ok = ok && batch.build(slots, ctx->stream, ctx->xfer, err) && batch.capture(ctx->stream, o->refinement_steps) &&
                  hipGraphLaunch(batch.graph_exec, ctx->stream) == hipSuccess &&  // warm-up
                  hipEventRecord(e0, ctx->stream) == hipSuccess;
             for (int it = 0; ok && it < iterations; ++it) ok = hipGraphLaunch(batch.graph_exec, ctx->stream) == hipSuccess;
@@ -3162,9 +3187,10 @@ static int strip_rejected(vib_context* ctx, Slot& s) {
     EditBufs b;
     if (!edit_buffers(ctx, h.L, h.F, b)) return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "slide buffer allocation failed");
     int rejected = 0;
+    CopyBatch cb;
+    cb.down(&rejected, b.sel, sizeof(int));
     if (!edit_begin(ctx, s, b) || vibk_strip_select(b.src, h.L, b.keep_lm, b.keep_f, b.sel, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(&rejected, b.sel, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        !cb.flush(ctx->stream, ctx->xfer))
         return fail(ctx, VIB_ERR_DEVICE, "rejected-factor scan failed");
     if (!rejected) return VIB_OK;
     if (!refresh_state(ctx, s)) return fail(ctx, VIB_ERR_DEVICE, "state download failed");

@@ -82,12 +82,23 @@ struct HostWindow {
     void make_chunks();
 };
 
-// Staging of a context's batched transfers: a grow-only device buffer, and two host buffers
-// used in turn, each reused only after the transfer that last read it has completed (`done`).
+// Grow-only pinned host buffer (hipHostMalloc): transfers from and to it are plain DMA, with no
+// staging copy inside the runtime (whose pageable-memory path serializes the transfers of all
+// threads of the process).
+struct PinnedBuffer {
+    char* p = nullptr;
+    size_t cap = 0;
+    bool reserve(size_t bytes);
+    ~PinnedBuffer();
+};
+
+// Staging of a context's batched transfers: a grow-only device buffer, and two pinned host
+// buffers used in turn, each reused only after the transfer that last read it has completed
+// (`done`), and one for downloads.
 struct Staging {
     void* buf = nullptr;
     size_t cap = 0;
-    std::vector<char> host[2], back;
+    PinnedBuffer host[2], back;
     hipEvent_t done[2] = {nullptr, nullptr};
     bool recorded[2] = {false, false};
     int next = 0;
@@ -177,7 +188,7 @@ struct Batch {
     size_t cap_wins = 0, cap_red = 0, cap_chunks = 0, cap_lin = 0;  // grow-only device buffers (rebuilt batches)
 
     ~Batch();
-    bool build(std::vector<Slot*>& slots, hipStream_t stream, std::string& err);
+    bool build(std::vector<Slot*>& slots, hipStream_t stream, Staging& xs, std::string& err);
     bool launch(int kernel, hipStream_t s, int refinement, int mode);
     bool linearize(hipStream_t s);
     bool step(hipStream_t s, int refinement, int mode);

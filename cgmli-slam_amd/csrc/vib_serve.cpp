@@ -6,9 +6,15 @@
 // the window is over its size, src/backend/FixedLagBackend.cpp:17-22, 405-451): the keyframe is
 // appended on the device, the LM iterations run batched with the other running requests of the
 // same size bucket, and the slide brings the window back to its size. Continuous batching:
-// a request that arrives while others are on the device is admitted at the next turn (its
-// session idle, its window resident or uploaded within the HBM budget, idle sessions evicted
-// least recently used first) and joins the next launch of its bucket.
+// a request that arrives while others are on the device is admitted at its worker's next pass
+// (its session idle, its window resident or uploaded within the HBM budget, idle sessions
+// evicted least recently used first) and joins the next launch of its bucket.
+//
+// Workers: each is a context (worker 0: the caller's) with a thread and stream of its own,
+// serving a fixed share of the sessions in a loop of passes (admit, start, one LM iteration of
+// one bucket, slides). A pass is mostly host latency (edits, synchronizations) with the device
+// idle, so workers that do not wait for each other overlap it. The driving thread only queues
+// requests and collects results (vib_server_turn).
 //
 // The server is a client of the library's own C-ABI (upload, append, request begin/iterate,
 // marginalize, read/release for eviction); it adds the scheduling: admission by HBM
@@ -18,7 +24,6 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -182,20 +187,21 @@ struct Request {
 
 struct Session {
     int32_t id = 0;
-    int32_t worker = 0;    // the worker (context) that holds the session
+    int32_t worker = 0;    // the worker (context) that serves the session
     int32_t slot = -1;     // slot of the worker's context while resident
     int32_t frames = 0;    // the window size a slide restores
     int32_t cameras = 0;
+    bool held = false;     // its last result has not been handed out by a turn call yet
     OwnedWindow stored;    // the window while not resident
     std::deque<std::unique_ptr<Request>> queue;
     std::unique_ptr<Request> running;
     Clock::time_point last_used;
 };
 
-// One worker: a context of its own (worker 0: the caller's), its sessions and slots. Workers
-// serve disjoint sessions, so their parts of a turn run on separate threads and streams: a
-// turn's host work (appends, slides, their synchronizations) is mostly latency, and the device
-// is idle for most of it.
+// One worker: a context of its own (worker 0: the caller's) on a thread of its own, serving its
+// sessions in a loop: admit, start (append, LM state), one LM iteration of one size bucket,
+// slides. The workers do not wait for each other: a worker whose slides take long (a large
+// prior's eigendecomposition) does not hold back the others.
 struct Worker {
     vib_context* ctx = nullptr;
     bool owned = false;
@@ -204,58 +210,8 @@ struct Worker {
     int32_t next_slot = 0;
     int64_t turn = 0;               // bucket rotation
     int64_t footprint = 0;          // footprints of its sessions' first windows (assignment)
-    vib_server_stats stats{};
-    std::deque<vib_served> done;    // finished in this worker's last turn
-};
-
-// Threads 1 .. n - 1 run worker i's part of a turn; the calling thread runs worker 0.
-struct Pool {
-    std::vector<std::thread> threads;
-    std::mutex m;
-    std::condition_variable start, finished;
-    int64_t generation = 0;
-    int remaining = 0;
-    bool quit = false;
-    std::function<void(int)> job;
-
-    void launch(int n) {
-        for (int i = 1; i < n; ++i)
-            threads.emplace_back([this, i] {
-                int64_t seen = 0;
-                for (;;) {
-                    std::unique_lock<std::mutex> lk(m);
-                    start.wait(lk, [&] { return quit || generation != seen; });
-                    if (quit) return;
-                    seen = generation;
-                    std::function<void(int)> f = job;
-                    lk.unlock();
-                    f(i);
-                    lk.lock();
-                    if (--remaining == 0) finished.notify_one();
-                }
-            });
-    }
-    void run(const std::function<void(int)>& f) {
-        {
-            std::lock_guard<std::mutex> lk(m);
-            job = f;
-            remaining = (int)threads.size();
-            ++generation;
-        }
-        start.notify_all();
-        f(0);
-        std::unique_lock<std::mutex> lk(m);
-        finished.wait(lk, [&] { return remaining == 0; });
-    }
-    void stop() {
-        {
-            std::lock_guard<std::mutex> lk(m);
-            quit = true;
-        }
-        start.notify_all();
-        for (std::thread& t : threads) t.join();
-        threads.clear();
-    }
+    vib_server_stats stats{};       // guarded by the server's mutex
+    std::thread thread;
 };
 
 }  // namespace
@@ -263,12 +219,18 @@ struct Pool {
 struct vib_server {
     vib_context* ctx = nullptr;
     vib_server_options opt{};
-    std::map<int32_t, Session> sessions;
+    std::map<int32_t, Session> sessions;  // added before the first request; the map is not modified afterwards
     std::vector<Worker> workers;
-    Pool pool;
+    // The mutex guards the sessions' queues and states (running, held, resident), the results,
+    // the HBM accounting and the statistics; the workers' device work runs outside it.
+    std::mutex mu;
+    std::condition_variable work;      // a request was queued or a session was released
+    std::condition_variable progress;  // a request finished
+    std::deque<vib_served> done;       // finished requests not yet handed out
+    std::vector<int32_t> handed;       // sessions whose results the last turn call handed out
+    bool quit = false;
     int64_t turns = 0;
-    double residency_seconds = 0;  // admission by HBM footprint (calling thread)
-    std::deque<vib_served> done;  // finished requests not yet returned
+    double residency_seconds = 0;      // admission (uploads, evictions), summed over the workers
 };
 
 namespace {
@@ -288,7 +250,8 @@ int32_t take_slot(const vib_server* s, Worker& w) {
     return s->opt.first_slot + w.next_slot++;
 }
 
-// The session's window back into host memory, its slot freed.
+// The session's window back into host memory, its slot freed (under the server's mutex; the
+// session is idle, so no worker is using it).
 int evict(Worker& w, Session& se) {
     int32_t K, L, F, E, rows, cols;
     int st = vib_window_shape(w.ctx, se.slot, &K, &L, &F, &E, &rows, &cols);
@@ -308,9 +271,9 @@ int evict(Worker& w, Session& se) {
 }
 
 // Makes the session's window resident within the server's HBM budget (idle sessions with nothing
-// queued are evicted, least recently used first, whichever worker holds them). False: it does
-// not fit now; with *status set: it never will, or the upload failed. Runs on the calling thread
-// while no worker runs.
+// queued are evicted, least recently used first, whichever worker serves them). False: it does
+// not fit now; with *status set: it never will, or the upload failed. Runs under the server's
+// mutex.
 bool make_resident(vib_server* s, Session& se, int* status) {
     *status = VIB_OK;
     if (se.slot >= 0) return true;
@@ -340,7 +303,8 @@ bool make_resident(vib_server* s, Session& se, int* status) {
             Session* victim = nullptr;
             for (auto& kv : s->sessions) {
                 Session& o = kv.second;
-                if (o.slot >= 0 && !o.running && o.queue.empty() && (!victim || o.last_used < victim->last_used)) victim = &o;
+                if (o.slot >= 0 && !o.running && !o.held && o.queue.empty() && (!victim || o.last_used < victim->last_used))
+                    victim = &o;
             }
             if (!victim) return false;
             if ((*status = evict(s->workers[victim->worker], *victim)) != VIB_OK) return false;
@@ -357,7 +321,9 @@ bool make_resident(vib_server* s, Session& se, int* status) {
     return true;
 }
 
-void finish(Worker& w, Session& se, std::unique_ptr<Request> r, int status, int iterations, double cost) {
+// A finished request: its result queued for the next turn call, the session held until then
+// (under the server's mutex).
+void finish(vib_server* s, Worker& w, Session& se, std::unique_ptr<Request> r, int status, int iterations, double cost) {
     const auto now = Clock::now();
     vib_served out{};
     out.request = r->id;
@@ -369,50 +335,89 @@ void finish(Worker& w, Session& se, std::unique_ptr<Request> r, int status, int 
     out.queued_seconds = std::chrono::duration<double>(r->admitted - r->submitted).count();
     out.latency_seconds = std::chrono::duration<double>(now - r->submitted).count();
     se.last_used = now;
-    w.done.push_back(out);
+    se.held = true;
+    s->done.push_back(out);
     w.stats.finished++;
     if (status != VIB_OK) w.stats.failed++;
 }
 
-// One worker's part of a turn: the admitted requests started (keyframe appended, LM state reset),
-// one LM iteration of the running requests of one size bucket, slides.
-void turn_worker(vib_server* s, Worker& w) {
-    auto since = [](Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); };
-    const auto t_admit = Clock::now();
-    // 1. the requests admitted by this turn's residency pass
+// Admission for one worker (under the server's mutex): the first queued request of every idle
+// session of the worker whose window is (made) resident. Returns the worker's running sessions.
+std::vector<Session*> admit(vib_server* s, Worker& w) {
+    const auto t0 = Clock::now();
+    std::vector<Session*> run;
     for (int32_t id : w.sessions) {
         Session& se = s->sessions[id];
-        if (!se.running || !se.running->starting) continue;
-        Request& r = *se.running;
-        r.starting = false;
-        int st = VIB_OK;
-        if (r.has_keyframe) {
-            vib_keyframe k = r.keyframe.view();
-            st = vib_window_append(w.ctx, se.slot, &k, &r.unmatched);
-            w.stats.appends++;
-            r.keyframe = OwnedKeyframe();  // the window holds it now
+        if (!se.running && !se.held && !se.queue.empty()) {
+            int st = VIB_OK;
+            if (!make_resident(s, se, &st)) {
+                if (st != VIB_OK) {
+                    finish(s, w, se, std::move(se.queue.front()), st, 0, 0.0);
+                    se.queue.pop_front();
+                }
+                continue;
+            }
+            se.running = std::move(se.queue.front());
+            se.queue.pop_front();
+            se.running->admitted = Clock::now();
+            se.running->starting = true;
         }
-        vib_solve_options o{};
-        o.max_iterations = r.iterations;
-        o.min_iterations = 0;
-        o.initial_lambda = 1e-4;
-        o.precision = s->opt.precision;
-        o.refinement_steps = s->opt.refinement_steps;
-        o.flags = s->opt.full_iterations ? VIB_SOLVE_CONTINUOUS : 0u;
-        if (st == VIB_OK) st = vib_request_begin(w.ctx, se.slot, &o);
-        if (st != VIB_OK) finish(w, se, std::move(se.running), st, 0, 0.0);
+        if (se.running) run.push_back(&se);
     }
-    w.stats.admit_seconds += since(t_admit);
-    // 2. one LM iteration of the running requests of one size bucket (buckets in turn)
+    s->residency_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
+    return run;
+}
+
+struct Outcome {
+    Session* se;
+    int status, iterations;
+    double cost;
+};
+
+// One pass of a worker over its running sessions (outside the server's mutex: only the worker
+// touches running sessions and their windows): the admitted requests started (keyframe appended,
+// LM state reset), one LM iteration of the running requests of one size bucket, and the slides
+// of the requests that finished.
+std::vector<Outcome> serve_pass(vib_server* s, Worker& w, const std::vector<Session*>& run_all, vib_server_stats& st) {
+    auto since = [](Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); };
+    std::vector<Outcome> out;
+    const auto t_admit = Clock::now();
+    std::vector<Session*> live;
+    for (Session* se : run_all) {
+        Request& r = *se->running;
+        if (r.starting) {
+            r.starting = false;
+            int code = VIB_OK;
+            if (r.has_keyframe) {
+                vib_keyframe k = r.keyframe.view();
+                code = vib_window_append(w.ctx, se->slot, &k, &r.unmatched);
+                st.appends++;
+                r.keyframe = OwnedKeyframe();  // the window holds it now
+            }
+            vib_solve_options o{};
+            o.max_iterations = r.iterations;
+            o.min_iterations = 0;
+            o.initial_lambda = 1e-4;
+            o.precision = s->opt.precision;
+            o.refinement_steps = s->opt.refinement_steps;
+            o.flags = s->opt.full_iterations ? VIB_SOLVE_CONTINUOUS : 0u;
+            if (code == VIB_OK) code = vib_request_begin(w.ctx, se->slot, &o);
+            if (code != VIB_OK) {
+                out.push_back(Outcome{se, code, 0, 0.0});
+                continue;
+            }
+        }
+        live.push_back(se);
+    }
+    st.admit_seconds += since(t_admit);
+    // one LM iteration of the running requests of one size bucket (buckets in turn)
     std::map<int, std::vector<Session*>> by_bucket;
-    for (int32_t id : w.sessions) {
-        Session& se = s->sessions[id];
-        if (!se.running) continue;
+    for (Session* se : live) {
         int32_t nr = 0, nv = 0;
-        vib_retained_dimension(w.ctx, se.slot, &nr, &nv);
-        by_bucket[bucket_of(s, nr)].push_back(&se);
+        vib_retained_dimension(w.ctx, se->slot, &nr, &nv);
+        by_bucket[bucket_of(s, nr)].push_back(se);
     }
-    if (by_bucket.empty()) return;
+    if (by_bucket.empty()) return out;
     std::vector<int> keys;
     for (auto& kv : by_bucket) keys.push_back(kv.first);
     const int b = keys[w.turn++ % keys.size()];
@@ -421,19 +426,19 @@ void turn_worker(vib_server* s, Worker& w) {
     std::vector<vib_solve_report> reps(run.size());
     for (Session* se : run) ids.push_back(se->slot);
     const auto t_iter = Clock::now();
-    const int st = vib_request_iterate(w.ctx, (int32_t)ids.size(), ids.data(), s->opt.precision, s->opt.refinement_steps,
-                                       dn.data(), reps.data());
-    w.stats.iterate_seconds += since(t_iter);
-    w.stats.launches++;
-    w.stats.window_iterations += (int64_t)ids.size();
-    // 3. the finished requests' slides back to their windows' sizes (FixedLagBackend.cpp:434-451),
-    //    all in one batch
+    const int code = vib_request_iterate(w.ctx, (int32_t)ids.size(), ids.data(), s->opt.precision,
+                                         s->opt.refinement_steps, dn.data(), reps.data());
+    st.iterate_seconds += since(t_iter);
+    st.launches++;
+    st.window_iterations += (int64_t)ids.size();
+    // the finished requests' slides back to their windows' sizes (FixedLagBackend.cpp:434-451),
+    // all in one batch
     std::vector<size_t> fin, slide;
     std::vector<int> status(run.size(), VIB_OK);
     for (size_t i = 0; i < run.size(); ++i) {
-        if (st == VIB_OK && !dn[i]) continue;
+        if (code == VIB_OK && !dn[i]) continue;
         fin.push_back(i);
-        status[i] = st != VIB_OK ? st : reps[i].status;
+        status[i] = code != VIB_OK ? code : reps[i].status;
         int32_t K = 0, L, F, E, rows, cols;
         if (status[i] == VIB_OK && s->opt.slide_policy >= 0 &&
             vib_window_shape(w.ctx, run[i]->slot, &K, &L, &F, &E, &rows, &cols) == VIB_OK && K > run[i]->frames)
@@ -445,18 +450,63 @@ void turn_worker(vib_server* s, Worker& w) {
         std::vector<vib_marginal_report> mr(slide.size());
         const auto t_slide = Clock::now();
         (void)vib_marginalize_batch(w.ctx, (int32_t)sl.size(), sl.data(), s->opt.slide_policy, mr.data());
-        w.stats.slide_seconds += since(t_slide);
-        w.stats.slides += (int64_t)slide.size();
-        w.stats.slide_launches++;
+        st.slide_seconds += since(t_slide);
+        st.slides += (int64_t)slide.size();
+        st.slide_launches++;
         for (size_t q = 0; q < slide.size(); ++q) status[slide[q]] = mr[q].status;
     }
     for (size_t i : fin)
-        finish(w, *run[i], std::move(run[i]->running), status[i], st == VIB_OK ? reps[i].iterations : 0,
-               st == VIB_OK ? reps[i].final_cost : 0.0);
+        out.push_back(Outcome{run[i], status[i], code == VIB_OK ? reps[i].iterations : 0,
+                              code == VIB_OK ? reps[i].final_cost : 0.0});
+    return out;
+}
+
+bool has_work(const vib_server* s, const Worker& w) {
+    for (int32_t id : w.sessions) {
+        const Session& se = s->sessions.at(id);
+        if (se.running || (!se.held && !se.queue.empty())) return true;
+    }
+    return false;
+}
+
+void worker_loop(vib_server* s, int index) {
+    Worker& w = s->workers[index];
+    for (;;) {
+        std::vector<Session*> run;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->work.wait(lk, [&] { return s->quit || has_work(s, w); });
+            if (s->quit) return;
+            run = admit(s, w);
+        }
+        if (run.empty()) {  // nothing admissible now (budget): wait for a release or a new request
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->work.wait_for(lk, std::chrono::milliseconds(1));
+            continue;
+        }
+        vib_server_stats st{};
+        std::vector<Outcome> out = serve_pass(s, w, run, st);
+        std::lock_guard<std::mutex> lk(s->mu);
+        w.stats.appends += st.appends; w.stats.launches += st.launches; w.stats.window_iterations += st.window_iterations;
+        w.stats.slides += st.slides; w.stats.slide_launches += st.slide_launches; w.stats.admit_seconds += st.admit_seconds;
+        w.stats.iterate_seconds += st.iterate_seconds; w.stats.slide_seconds += st.slide_seconds;
+        for (Outcome& o : out) finish(s, w, *o.se, std::move(o.se->running), o.status, o.iterations, o.cost);
+        if (!out.empty()) s->progress.notify_all();
+    }
+}
+
+void stop_workers(vib_server* s) {
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->quit = true;
+    }
+    s->work.notify_all();
+    for (Worker& w : s->workers)
+        if (w.thread.joinable()) w.thread.join();
 }
 
 void release_all(vib_server* s) {
-    s->pool.stop();
+    stop_workers(s);
     for (auto& kv : s->sessions)
         if (kv.second.slot >= 0) (void)vib_window_release(s->workers[kv.second.worker].ctx, kv.second.slot);
     for (Worker& w : s->workers)
@@ -494,7 +544,7 @@ int vib_server_create(vib_context* ctx, const vib_server_options* o, vib_server*
             return VIB_ERR_DEVICE;
         }
     }
-    s->pool.launch(n);
+    for (int i = 0; i < n; ++i) s->workers[i].thread = std::thread(worker_loop, s, i);
     *out = s;
     return VIB_OK;
 }
@@ -508,6 +558,7 @@ int vib_server_destroy(vib_server* s) {
 
 int vib_server_add_session(vib_server* s, int32_t session, const vib_window* w) {
     if (!s || !w) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(s->mu);
     if (s->sessions.count(session)) return VIB_ERR_INVALID_ARGUMENT;
     Session se;
     se.id = session;
@@ -529,58 +580,47 @@ int vib_server_add_session(vib_server* s, int32_t session, const vib_window* w) 
 
 int vib_server_submit(vib_server* s, int64_t request, int32_t session, int32_t iterations, const vib_keyframe* kf) {
     if (!s || iterations < 1) return VIB_ERR_INVALID_ARGUMENT;
-    auto it = s->sessions.find(session);
-    if (it == s->sessions.end()) return VIB_ERR_NOT_FOUND;
     auto r = std::make_unique<Request>();
     r->id = request;
     r->session = session;
     r->iterations = iterations;
     r->has_keyframe = kf != nullptr;
     if (kf && !r->keyframe.copy_from(*kf)) return VIB_ERR_INVALID_ARGUMENT;
-    r->submitted = Clock::now();
-    it->second.queue.push_back(std::move(r));
-    s->workers[it->second.worker].stats.submitted++;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->sessions.find(session);
+        if (it == s->sessions.end()) return VIB_ERR_NOT_FOUND;
+        r->submitted = Clock::now();
+        it->second.queue.push_back(std::move(r));
+        s->workers[it->second.worker].stats.submitted++;
+    }
+    s->work.notify_all();
     return VIB_OK;
 }
 
 int vib_server_turn(vib_server* s, vib_served* finished, int32_t capacity, int32_t* count) {
     if (!s || !count || capacity < 0 || (capacity > 0 && !finished)) return VIB_ERR_INVALID_ARGUMENT;
     *count = 0;
-    s->turns++;
-    // admission: the first queued request of every idle session whose window is (made) resident
-    const auto t0 = Clock::now();
-    for (auto& kv : s->sessions) {
-        Session& se = kv.second;
-        if (se.running || se.queue.empty()) continue;
-        int st = VIB_OK;
-        if (!make_resident(s, se, &st)) {
-            if (st != VIB_OK) {
-                finish(s->workers[se.worker], se, std::move(se.queue.front()), st, 0, 0.0);
-                se.queue.pop_front();
-            }
-            continue;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->turns++;
+        // the sessions handed out by the previous call may take their next request now
+        for (int32_t id : s->handed) s->sessions[id].held = false;
+        s->handed.clear();
+        if (s->done.empty()) s->progress.wait_for(lk, std::chrono::milliseconds(1));
+        while (*count < capacity && !s->done.empty()) {
+            finished[(*count)++] = s->done.front();
+            s->handed.push_back(s->done.front().session);
+            s->done.pop_front();
         }
-        se.running = std::move(se.queue.front());
-        se.queue.pop_front();
-        se.running->admitted = Clock::now();
-        se.running->starting = true;
     }
-    s->residency_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
-    if (s->workers.size() == 1) turn_worker(s, s->workers[0]);
-    else s->pool.run([s](int i) { turn_worker(s, s->workers[i]); });
-    for (Worker& w : s->workers) {
-        for (const vib_served& r : w.done) s->done.push_back(r);
-        w.done.clear();
-    }
-    while (*count < capacity && !s->done.empty()) {
-        finished[(*count)++] = s->done.front();
-        s->done.pop_front();
-    }
+    s->work.notify_all();
     return VIB_OK;
 }
 
 int vib_server_preload(vib_server* s) {
     if (!s) return VIB_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(s->mu);
     for (auto& kv : s->sessions) {
         int st = VIB_OK;
         if (!make_resident(s, kv.second, &st)) return st;
@@ -589,8 +629,10 @@ int vib_server_preload(vib_server* s) {
     return VIB_OK;
 }
 
-int vib_server_get_stats(const vib_server* s, vib_server_stats* out) {
-    if (!s || !out) return VIB_ERR_INVALID_ARGUMENT;
+int vib_server_get_stats(const vib_server* cs, vib_server_stats* out) {
+    if (!cs || !out) return VIB_ERR_INVALID_ARGUMENT;
+    vib_server* s = const_cast<vib_server*>(cs);  // the mutex
+    std::lock_guard<std::mutex> lk(s->mu);
     *out = vib_server_stats{};
     for (const Worker& w : s->workers) {  // counters and phase times summed over the workers
         const vib_server_stats& x = w.stats;
@@ -604,14 +646,12 @@ int vib_server_get_stats(const vib_server* s, vib_server_stats* out) {
         int64_t b = 0;
         if (kv.second.slot >= 0 && vib_slot_bytes(s->workers[kv.second.worker].ctx, kv.second.slot, &b) == VIB_OK)
             out->resident_bytes += b;
-    }
-    out->turns = s->turns;
-    out->admit_seconds += s->residency_seconds;
-    for (const auto& kv : s->sessions) {
         out->queued += (int32_t)kv.second.queue.size();
         out->running += kv.second.running ? 1 : 0;
         out->resident += kv.second.slot >= 0 ? 1 : 0;
     }
+    out->turns = s->turns;
+    out->admit_seconds += s->residency_seconds;
     out->pending_results = (int32_t)s->done.size();
     return VIB_OK;
 }
@@ -620,8 +660,10 @@ int vib_server_session_slot(const vib_server* s, int32_t session, int32_t* slot)
     return vib_server_session_context(s, session, nullptr, slot);
 }
 
-int vib_server_session_context(const vib_server* s, int32_t session, vib_context** ctx, int32_t* slot) {
-    if (!s || !slot) return VIB_ERR_INVALID_ARGUMENT;
+int vib_server_session_context(const vib_server* cs, int32_t session, vib_context** ctx, int32_t* slot) {
+    if (!cs || !slot) return VIB_ERR_INVALID_ARGUMENT;
+    vib_server* s = const_cast<vib_server*>(cs);
+    std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->sessions.find(session);
     if (it == s->sessions.end()) return VIB_ERR_NOT_FOUND;
     *slot = it->second.slot;

@@ -370,14 +370,16 @@ int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* slots, i
  * step: the solve, then the slide once the window is over its size;
  * src/backend/FixedLagBackend.cpp:17-22, 405-451): vib_window_append, LM iterations batched
  * with the other running requests of its size bucket, and vib_marginalize back to the
- * session's window size, the prior staying resident. Each vib_server_turn (1) admits the first
- * queued request of every idle session whose window is resident or fits the HBM budget (idle
- * sessions with nothing queued are evicted to host memory, least recently used first), (2) runs
- * one LM iteration for the running requests of one size bucket (the buckets take turns), and (3)
- * slides and reports the requests that finished. Requests submitted while others run join at
- * the next turn. The server uses context slots from first_slot on; one thread drives it.
- * With several workers every session belongs to one worker (the least loaded by window
- * footprint when it is added) and a turn runs the workers' parts concurrently. */
+ * session's window size, the prior staying resident. Every session belongs to one worker (the
+ * least loaded by window footprint when it is added); each worker runs passes on a thread of its
+ * own: (1) admit the first queued request of every idle session of the worker whose window is
+ * resident or fits the HBM budget (idle sessions with nothing queued are evicted to host memory,
+ * least recently used first), (2) start them (append, LM state), (3) run one LM iteration for its
+ * running requests of one size bucket (the buckets take turns), and (4) slide the requests that
+ * finished. Requests submitted while others run join at the next pass. A session whose request
+ * finished waits until vib_server_turn has handed the result out and has been called again, so
+ * between two turn calls the caller may read the windows of the sessions just reported. The
+ * server uses context slots from first_slot on; one thread drives it (submit, turn). */
 typedef struct vib_server vib_server;
 typedef struct vib_server_options {
     int64_t hbm_budget;          /* bytes of resident windows the server admits (0: no limit); a
@@ -427,8 +429,8 @@ int vib_server_add_session(vib_server* server, int32_t session, const vib_window
  * `iterations` LM iterations, then the slide. Requests of one session are served in order. */
 int vib_server_submit(vib_server* server, int64_t request, int32_t session, int32_t iterations,
                       const vib_keyframe* keyframe);
-/* One turn of the loop; up to `capacity` finished requests are copied to `finished` (the rest
- * are kept for the next call), their number in *count. */
+/* Collects finished requests: up to `capacity` are copied to `finished` (the rest are kept for the
+ * next call), their number in *count; waits up to a millisecond when none has finished. */
 int vib_server_turn(vib_server* server, vib_served* finished, int32_t capacity, int32_t* count);
 /* Makes the sessions' windows resident in session order until the HBM budget is full. */
 int vib_server_preload(vib_server* server);
