@@ -1047,6 +1047,8 @@ constexpr int kRedEntries = 16;               // entries per workgroup (one 128-
 
 // Entries [slice E, slice E + E) of window w's partial sum, which 0: visual block, 1: Schur
 // correction; kBlock / E chunk lanes per entry, combined in lane order.
+// Threads kBlock.. of a larger workgroup only take part in the barrier, so the summation order is
+// the same in every launch that runs it.
 template <int E>
 __device__ __forceinline__ void reduce_slice(const WinDev& w, int which, double* out, int slice, double* red) {
     if (!window_active(w)) return;
@@ -1054,16 +1056,17 @@ __device__ __forceinline__ void reduce_slice(const WinDev& w, int which, double*
     const int len = w.tri + w.nv;
     if (slice * E >= len) return;
     constexpr int lanes = kBlock / E;
+    const bool act = threadIdx.x < kBlock;
     const int e = threadIdx.x % E, cl = threadIdx.x / E;
     const int i = slice * E + e;
     const double* part = which == 0 ? w.lin_part : w.schur_part;
     const int nparts = which == 0 ? w.nlin : w.nchunks;
     double s = 0;
-    if (i < len)
+    if (act && i < len)
         for (int c = cl; c < nparts; c += lanes) s += part[(size_t)c * len + i];
-    red[cl * E + e] = s;
+    if (act) red[cl * E + e] = s;
     __syncthreads();
-    if (cl == 0 && i < len) {
+    if (act && cl == 0 && i < len) {
         double t = 0;
         for (int q = 0; q < lanes; ++q) t += red[q * E + e];
         out[i] = t;
@@ -1100,10 +1103,11 @@ constexpr int kSchurRows = 64;                 // landmarks per LDS pass
 constexpr int kSchurLd = 144;                  // LDS row stride (doubles; offsets rows by 128 B)
 constexpr int kVisTiles = 8;                   // 128 / 16
 constexpr int kSchurTiles = kVisTiles * (kVisTiles + 1) / 2;
-constexpr int kSchurTilesPerWave = (kSchurTiles + kBlock / 64 - 1) / (kBlock / 64);
-constexpr int kSchurGroups = kBlock / kChunkLanes;            // landmark groups per workgroup
-constexpr int kSchurRowsPerGroup = kSchurRows / kSchurGroups;  // rows one group builds per pass
-static_assert(kSchurRowsPerGroup * kSchurGroups == kSchurRows, "rows per pass");
+// FP64: 512 threads (8 waves) per workgroup at 128 VGPRs: two workgroups fit a CU by LDS, so 16
+// waves share its SIMDs and hide each other's load latency while the rows are built (256
+// threads at 210 VGPRs ran 8 waves per CU). Mixed precision: 256 threads (its FP32 products
+// need more registers per tile).
+template <bool MIXED> constexpr int schur_block() { return MIXED ? 256 : 512; }
 static_assert(kMaxVisual < kVisTiles * 16, "the gradient column fits the padded block");
 
 // Mixed precision keeps the coupling rows in FP32 (37 KB of LDS) and multiplies them on
@@ -1113,6 +1117,12 @@ static_assert(kMaxVisual < kVisTiles * 16, "the gradient column fits the padded 
 template <bool MIXED>
 __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chunks) {
     using CT = typename std::conditional<MIXED, float, double>::type;
+    constexpr int kSchurBlock = schur_block<MIXED>();
+    constexpr int kSchurWaves = kSchurBlock / 64;
+    constexpr int kSchurTilesPerWave = (kSchurTiles + kSchurWaves - 1) / kSchurWaves;
+    constexpr int kSchurGroups = kSchurBlock / kChunkLanes;       // landmark groups per workgroup
+    constexpr int kSchurRowsPerGroup = kSchurRows / kSchurGroups;  // rows one group builds per pass
+    static_assert(kSchurRowsPerGroup * kSchurGroups == kSchurRows, "rows per pass");
     __shared__ CT crow[kSchurRows * kSchurLd];
     __shared__ CT sc[kSchurRows];
     __shared__ double qg[kSchurRows];
@@ -1136,7 +1146,7 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
     d4 acc[kSchurTilesPerWave];
 #pragma unroll
     for (int q = 0; q < kSchurTilesPerWave; ++q) {
-        const int idx = wave + q * (kBlock / 64);
+        const int idx = wave + q * kSchurWaves;
         tti[q] = -1;
         ttj[q] = 0;
         acc[q] = d4{0.0, 0.0, 0.0, 0.0};
@@ -1150,7 +1160,7 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
     }
     double gsum = 0;  // MIXED: gradient correction of column t (t < nv)
     for (int base = ch.lm_begin; base < ch.lm_end; base += kSchurRows) {
-        for (int i = t; i < kSchurRows * kSchurLd; i += kBlock) crow[i] = CT(0);
+        for (int i = t; i < kSchurRows * kSchurLd; i += kSchurBlock) crow[i] = CT(0);
         // Dense coupling rows. Group g builds rows g + j groups (j < kSchurRowsPerGroup): every
         // load of its rows (landmark record, first factor of each lane, its coupling terms) is
         // issued before the first LDS update, so the dependent global round trips of the rows
@@ -1180,11 +1190,11 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
                 fp[j] = w.pf_pos[fa[j]];
             }
         }
-        double u[kSchurRowsPerGroup][6];
+        CT u[kSchurRowsPerGroup][6];  // FP32 hi parts in mixed precision (the products use them alone)
 #pragma unroll
         for (int j = 0; j < kSchurRowsPerGroup; ++j)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) u[j][k] = fa[j] < f1[j] ? load_u<MIXED>(w, F, k, fp[j]) : 0.0;
+            for (int k = 0; k < 6; ++k) u[j][k] = fa[j] < f1[j] ? CT(load_u<MIXED>(w, F, k, fp[j])) : CT(0);
         __syncthreads();  // rows cleared, pose columns staged
 #pragma unroll
         for (int j = 0; j < kSchurRowsPerGroup; ++j) {
@@ -1278,13 +1288,10 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
     if (MIXED && t < nv) part[tri + t] = -gsum;
 }
 
-// two workgroups per CU: 256 VGPRs (no AGPRs) and 74 KB of LDS each. Workgroups nchunks.. sum
-// the visual-block partials of the linearization (64 entries each, rslices per window), which
-// the Schur products do not need: k_assemble reads both sums.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_schur(const WinDev* wins,
-                                                                                              const Chunk* chunks,
-                                                                                              int nchunks, int rslices,
-                                                                                              double** outs) {
+// Workgroups nchunks.. sum the visual-block partials of the linearization (64 entries each,
+// rslices per window), which the Schur products do not need: k_assemble reads both sums.
+__global__ __launch_bounds__(schur_block<false>()) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_schur(const WinDev* wins, const Chunk* chunks, int nchunks, int rslices,
+                                                      double** outs) {
     if ((int)blockIdx.x >= nchunks) {
         __shared__ double red[kBlock];
         const int r = blockIdx.x - nchunks, win = r / rslices;
@@ -1293,11 +1300,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     }
     schur_body<false>(wins, chunks);
 }
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_schur_mixed(const WinDev* wins,
-                                                                                                    const Chunk* chunks,
-                                                                                                    int nchunks,
-                                                                                                    int rslices,
-                                                                                                    double** outs) {
+__global__ __launch_bounds__(schur_block<true>()) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_schur_mixed(const WinDev* wins, const Chunk* chunks, int nchunks,
+                                                            int rslices, double** outs) {
     if ((int)blockIdx.x >= nchunks) {
         __shared__ double red[kBlock];
         const int r = blockIdx.x - nchunks, win = r / rslices;
@@ -3600,8 +3604,9 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
             const int rslices = fuse_reduce ? (max_len + kRedEntriesFused - 1) / kRedEntriesFused : 0;
             const int grid = nchunks + nwin * rslices;
             if (grid == 0) break;
-            if (mixed) hipLaunchKernelGGL(k_schur_mixed, dim3(grid), dim3(kBlock), 0, s, wins, chunks, nchunks, rslices, red);
-            else hipLaunchKernelGGL(k_schur, dim3(grid), dim3(kBlock), 0, s, wins, chunks, nchunks, rslices, red);
+            if (mixed)
+                hipLaunchKernelGGL(k_schur_mixed, dim3(grid), dim3(schur_block<true>()), 0, s, wins, chunks, nchunks, rslices, red);
+            else hipLaunchKernelGGL(k_schur, dim3(grid), dim3(schur_block<false>()), 0, s, wins, chunks, nchunks, rslices, red);
             break;
         }
         case 4: hipLaunchKernelGGL(k_reduce, dim3(nwin, slices), dim3(kBlock), 0, s, wins, 1, red); break;
