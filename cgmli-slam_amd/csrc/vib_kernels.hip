@@ -2112,6 +2112,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, cons
     constexpr int kZPitch = kMaxVisual + 1;
     __shared__ double zw[kBlock / 64][kZPitch];  // one accumulator per wave
     __shared__ double xs[kZPitch];
+    __shared__ int pcol[kMaxFrames];
     if ((int)blockIdx.x >= nchunks) {
         const int r = blockIdx.x - nchunks;
         const WinDev& w = wins[r / hslices];
@@ -2127,6 +2128,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, cons
     constexpr int groups = kBlock / kChunkLanes;
     const size_t L = (size_t)w.L, F = (size_t)w.F;
     for (int j = t; j < kZPitch; j += kBlock) xs[j] = j < nv ? w.dstate[j] : 0.0;
+    if (t < kMaxFrames) pcol[t] = t < w.K ? w.pose_col[t] : -1;
     for (int j = t; j < (kBlock / 64) * kZPitch; j += kBlock) (&zw[0][0])[j] = 0.0;
     __syncthreads();
     double* z = zw[wave];
@@ -2142,12 +2144,12 @@ __global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, cons
         for (int e = 0; e < kCached; ++e) bcc[e] = -2;  // -2: no factor (same-epoch or absent)
         if (free) {
             a = w.lm_fc[l] >> 8;
-            ac = w.pose_col[a];
+            ac = pcol[a];
             int e = 0;
             for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes, ++e) {
                 const int b = w.ob_fc[f] >> 8;
                 if (b == a) continue;  // same-epoch factors carry no pose coupling
-                const int bc = w.pose_col[b], p = w.pf_pos[f];
+                const int bc = pcol[b], p = w.pf_pos[f];
                 double u[6];
 #pragma unroll
                 for (int k = 0; k < 6; ++k) {
@@ -2182,7 +2184,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_apply(const WinDev* wins, cons
         for (int f = w.lm_off[l] + lane + kCached * kChunkLanes; f < w.lm_off[l + 1]; f += kChunkLanes) {
             const int b = w.ob_fc[f] >> 8;
             if (b == a) continue;
-            const int bc = w.pose_col[b], p = w.pf_pos[f];
+            const int bc = pcol[b], p = w.pf_pos[f];
             for (int k = 0; k < 6; ++k) {
                 const double v = wk * load_u_exact<MIXED>(w, F, k, p);
                 if (ac >= 0) atomicAdd(&z[ac + k], v);
@@ -2545,6 +2547,8 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eigen_lds(const EigenJob* j
 template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Chunk* chunks) {
     __shared__ double red[kBlock];
+    __shared__ double dxv[kMaxVisual];  // the visual columns of the state step
+    __shared__ int pcol[kMaxFrames];
     const Chunk ch = chunks[blockIdx.x];
     const WinDev& w = wins[ch.win];
     if (!window_active(w)) return;
@@ -2553,6 +2557,11 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
     constexpr int groups = kBlock / kChunkLanes;
     const size_t L = (size_t)w.L, F = (size_t)w.F;
     const double* dx = w.dstate;
+    // the step's visual columns and the pose columns in LDS: the coupling sums index them per
+    // factor, one dependent global load fewer each
+    for (int j = threadIdx.x; j < w.nv; j += kBlock) dxv[j] = dx[j];
+    if (threadIdx.x < kMaxFrames) pcol[threadIdx.x] = (int)threadIdx.x < w.K ? w.pose_col[threadIdx.x] : -1;
+    __syncthreads();
     double pred = 0, mx = 0;
     int bad = 0;
     for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
@@ -2562,14 +2571,15 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
         double cd = 0;
         if (free) {
             const int a = w.lm_fc[l] >> 8;
-            const int ac = w.pose_col[a];
+            const int ac = pcol[a];
             for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
                 const int b = w.ob_fc[f] >> 8;
                 if (b == a) continue;
-                const int bc = w.pose_col[b];
+                const int bc = pcol[b];
+                const int p = w.pf_pos[f];
                 for (int k = 0; k < 6; ++k) {
-                    const double u = load_u_exact<MIXED>(w, F, k, w.pf_pos[f]);
-                    cd += u * ((ac >= 0 ? dx[ac + k] : 0.0) - (bc >= 0 ? dx[bc + k] : 0.0));
+                    const double u = load_u_exact<MIXED>(w, F, k, p);
+                    cd += u * ((ac >= 0 ? dxv[ac + k] : 0.0) - (bc >= 0 ? dxv[bc + k] : 0.0));
                 }
             }
         }
@@ -2578,8 +2588,8 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
             const double rho = w.rho[l];
             if (free) {
                 if (w.rig_col >= 0)
-                    for (int k = 0; k < 6; ++k) cd += w.lm_out[(LO_CR0 + k) * L + l] * dx[w.rig_col + k];
-                if (w.td_col >= 0) cd += w.lm_out[LO_CT * L + l] * dx[w.td_col];
+                    for (int k = 0; k < 6; ++k) cd += w.lm_out[(LO_CR0 + k) * L + l] * dxv[w.rig_col + k];
+                if (w.td_col >= 0) cd += w.lm_out[LO_CT * L + l] * dxv[w.td_col];
                 const double dd = w.lm_out[LO_D * L + l], gk = w.lm_out[LO_G * L + l];
                 const double d = -(gk + cd) / damped_point(dd, lambda);
                 const double delta = isfinite(d) ? fmax(w.min_rho - rho, d) : d;
