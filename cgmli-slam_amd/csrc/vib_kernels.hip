@@ -90,6 +90,7 @@ struct FrameTable {
     double tc[kMaxFrames][kMaxCameras][3];
     int valid[kMaxFrames];  // finite and |q|^2 > 1e-16
     int unit[kMaxFrames];   // ||q|^2 - 1| <= 1e-6 (required for a pose Jacobian)
+    int pcol[kMaxFrames];   // pose column of every frame (-1: constant), read per factor
     int rig_valid, rig_unit;
 };
 
@@ -103,6 +104,7 @@ __device__ void build_frame_table(const WinDev& w, const double* pose, const dou
         ft.rig_valid = ok && n2 > 1e-16;
         ft.rig_unit = isfinite(n2) && fabs(n2 - 1) <= 1e-6;
     }
+    if (t < kMaxFrames) ft.pcol[t] = t < w.K ? w.pose_col[t] : -1;
     if (t < w.K) {
         const double* x = pose + 7 * t;
         bool ok = true;
@@ -216,7 +218,7 @@ __device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData&
             D[r * 3 + k] = e.reduce[r * 3] * Rwb.m[k * 3] + e.reduce[r * 3 + 1] * Rwb.m[k * 3 + 1] +
                            e.reduce[r * 3 + 2] * Rwb.m[k * 3 + 2];
     // anchor pose: [D, -D [p_w]x] ; rows: [D_r, p_w x D_r]
-    const int ac = w.pose_col[a], bc = w.pose_col[b];
+    const int ac = ft.pcol[a], bc = ft.pcol[b];
     if (same) {
         for (int i = 0; i < 12; ++i) { o.Ja[i] = 0; o.Jr[i] = 0; }
     } else {
@@ -335,11 +337,11 @@ __device__ __forceinline__ void acc_add(double* acc, int nv, int ci, int cj, dou
 // Gram entry G[i][j] (i <= j) of pair (a, b) into the visual block and gradient. The elements
 // one entry adds to are determined by the entry and the pair's blocks, so different entries of
 // one Gram matrix never meet.
-__device__ __forceinline__ void scatter_gram(double* acc, double* gacc, const WinDev& w, int a, int b, int i, int j,
-                                             double g) {
+__device__ __forceinline__ void scatter_gram(double* acc, double* gacc, const WinDev& w, const FrameTable& ft, int a,
+                                             int b, int i, int j, double g) {
     const int nv = w.nv;
     const bool same = a == b;
-    const int ac = same ? -1 : w.pose_col[a], bc = same ? -1 : w.pose_col[b];
+    const int ac = same ? -1 : ft.pcol[a], bc = same ? -1 : ft.pcol[b];
     const int rc = w.rig_col, tc = w.td_col;
     if (j < 6) {  // anchor/observer pose blocks
         acc_add(acc, nv, ac >= 0 ? ac + i : -1, ac >= 0 ? ac + j : -1, g);
@@ -592,7 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
                     for (int g = 0; g < ngram[q]; ++g) {
                         const int k2 = gkey[q][g];
                         if (k2 != ckey) {
-                            if (ckey >= 0) scatter_gram(acc, gacc, w, ckey / 32, ckey % 32, gi, gj, cval);
+                            if (ckey >= 0) scatter_gram(acc, gacc, w, ft, ckey / 32, ckey % 32, gi, gj, cval);
                             ckey = k2;
                             cval = 0;
                         }
@@ -602,7 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
         }
         wave_sync();
     }
-    if (t < kGramEntries && ckey >= 0) scatter_gram(acc, gacc, w, ckey / 32, ckey % 32, gi, gj, cval);
+    if (t < kGramEntries && ckey >= 0) scatter_gram(acc, gacc, w, ft, ckey / 32, ckey % 32, gi, gj, cval);
     __syncthreads();
     double* part = w.lin_part + (size_t)ch.local * (tri + nv);
     for (int i = threadIdx.x; i < tri + nv; i += kBlock) part[i] = acc[i];
