@@ -148,20 +148,34 @@ struct FactorJ {
     double w;        // Huber weight
 };
 
-// One visual factor: observation record f of the planes `ob` (stride F), observer frame/camera fc.
+// One observation record (planes of stride F): bearing, bearing velocity, reference time, the
+// square-root information (lower 2x2).
+struct ObsData {
+    V3<double> raw, vel;
+    double tref, s00, s10, s11;
+};
+__device__ __forceinline__ ObsData load_obs(const double* ob, size_t F, int f) {
+    ObsData o;
+    o.raw = {ob[OB_BX * F + f], ob[OB_BY * F + f], ob[OB_BZ * F + f]};
+    o.vel = {ob[OB_VX * F + f], ob[OB_VY * F + f], ob[OB_VZ * F + f]};
+    o.tref = ob[OB_T * F + f];
+    o.s00 = ob[OB_S00 * F + f]; o.s10 = ob[OB_S10 * F + f]; o.s11 = ob[OB_S11 * F + f];
+    return o;
+}
+
+// One visual factor: observation `obs`, observer frame/camera fc.
 template <bool WANT_J>
-__device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData& lm, const double* ob, size_t F,
-                            int f, int fc, double td, bool want_td, FactorJ& o) {
+__device__ bool eval_factor(const WinDev& w, const FrameTable& ft, const LmData& lm, const ObsData& obs, int fc,
+                            double td, bool want_td, FactorJ& o) {
     const int b = fc >> 8, cb = fc & 255;
     const int a = lm.frame, ca = lm.cam;
     const double rho = lm.rho;
     if (!(rho > 0) || !isfinite(rho) || !isfinite(td)) return false;
     if (!ft.valid[a] || !ft.valid[b] || !ft.rig_valid) return false;
     const bool same = a == b;
-    const V3<double> braw = {ob[OB_BX * F + f], ob[OB_BY * F + f], ob[OB_BZ * F + f]};
-    const V3<double> bvel = {ob[OB_VX * F + f], ob[OB_VY * F + f], ob[OB_VZ * F + f]};
-    const double btref = ob[OB_T * F + f];
-    const double s00 = ob[OB_S00 * F + f], s10 = ob[OB_S10 * F + f], s11 = ob[OB_S11 * F + f];
+    const V3<double> braw = obs.raw, bvel = obs.vel;
+    const double btref = obs.tref;
+    const double s00 = obs.s00, s10 = obs.s10, s11 = obs.s11;
     const bool movingA = sqnorm(lm.vel) > 0, movingB = sqnorm(bvel) > 0;
     const bool tdd = WANT_J && want_td;
     V3<double> da = {0, 0, 0}, db = {0, 0, 0}, ba, bb;
@@ -471,8 +485,30 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
         ++gi;
     }
     gj += gi;
+    // a factor's records (observation, anchor bearing, ids); each round's are loaded during the
+    // previous round's Gram phase, so their global latency overlaps it
+    struct LinIn {
+        ObsData ob;
+        V3<double> raw, vel;
+        double tref;
+        int ids, lmi;
+    };
+    auto load_in = [&](int q) {
+        LinIn in{};
+        if (q < ch.lm_end) {
+            in.ob = load_obs(w.pf_ob, F, q);
+            in.raw = {w.pf_an[LM_BX * F + q], w.pf_an[LM_BY * F + q], w.pf_an[LM_BZ * F + q]};
+            in.vel = {w.pf_an[LM_VX * F + q], w.pf_an[LM_VY * F + q], w.pf_an[LM_VZ * F + q]};
+            in.tref = w.pf_an[LM_T * F + q];
+            in.ids = w.pf_ids[q];
+            in.lmi = w.pf_lm[q];
+        }
+        return in;
+    };
+    LinIn next_in = load_in(ch.lm_begin + 64 * wave + lane);
     for (int round = 0; round < nrounds; ++round) {
         const int p = ch.lm_begin + round * kBlock + 64 * wave + lane;
+        const LinIn cur = next_in;
         int key = -1;
 #pragma unroll
         for (int c = 0; c < kGramCols; ++c) {
@@ -480,17 +516,17 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
             V[c * kRowPitch + 2 * lane + 1] = 0.0;
         }
         if (p < ch.lm_end) {
-            const int ids = w.pf_ids[p];
+            const int ids = cur.ids;
             LmData lm;
-            lm.raw = {w.pf_an[LM_BX * F + p], w.pf_an[LM_BY * F + p], w.pf_an[LM_BZ * F + p]};
-            lm.vel = {w.pf_an[LM_VX * F + p], w.pf_an[LM_VY * F + p], w.pf_an[LM_VZ * F + p]};
-            lm.tref = w.pf_an[LM_T * F + p];
-            lm.rho = w.rho[w.pf_lm[p]];
+            lm.raw = cur.raw;
+            lm.vel = cur.vel;
+            lm.tref = cur.tref;
+            lm.rho = w.rho[cur.lmi];
             lm.frame = ids >> 24;
             lm.cam = (ids >> 16) & 255;
             const int fc = ids & 0xffff;
             FactorJ J;
-            bool ok = eval_factor<true>(w, ft, lm, w.pf_ob, F, p, fc, td, want_td, J);
+            bool ok = eval_factor<true>(w, ft, lm, cur.ob, fc, td, want_td, J);
             const double sq = ok ? J.r[0] * J.r[0] + J.r[1] * J.r[1] : 0.0;
             ok = ok && isfinite(sq);
             double u[6] = {0, 0, 0, 0, 0, 0}, fl[FL_PLANES] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -530,6 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
             }
         }
         wave_sync();
+        if (round + 1 < nrounds) next_in = load_in(p + kBlock);
         // runs of one pair: Gram matrix on MFMA. A run occupies the lanes between its first and
         // last factor (pair order), so only the MFMA steps over its rows run; the rows of other
         // keys inside a boundary step are masked to zero. Every wave lists the packed upper
@@ -2659,7 +2696,7 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
         lm.frame = ids >> 24;
         lm.cam = (ids >> 16) & 255;
         FactorJ J;
-        if (!eval_factor<false>(w, ft, lm, w.pf_ob, F, p, ids & 0xffff, td, false, J)) { invalid = 1; continue; }
+        if (!eval_factor<false>(w, ft, lm, load_obs(w.pf_ob, F, p), ids & 0xffff, td, false, J)) { invalid = 1; continue; }
         const double sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
         if (!isfinite(sq)) { invalid = 1; continue; }
         cost += huber_cost(w.huber, sq);
@@ -2861,7 +2898,7 @@ __global__ __launch_bounds__(kBlock) void k_nis_gate(const WinDev* wins, const C
             double sq = 0;
             if (!factor_rejected(w.ob, F, f)) {
                 FactorJ J;
-                if (!eval_factor<false>(w, ft, lm, w.ob, F, f, w.ob_fc[f], td, false, J)) {
+                if (!eval_factor<false>(w, ft, lm, load_obs(w.ob, F, f), w.ob_fc[f], td, false, J)) {
                     d = 2;
                 } else {
                     sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
