@@ -2686,17 +2686,38 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
     const size_t F = (size_t)w.F;
     double cost = 0;
     int invalid = 0;
-    for (int p = ch.lm_begin + (int)threadIdx.x; p < ch.lm_end; p += kBlock) {
-        const int ids = w.pf_ids[p];
+    // the next factor's records are loaded before this one is evaluated
+    struct TrialIn {
+        ObsData ob;
+        V3<double> raw, vel;
+        double tref, rho;
+        int ids;
+    };
+    auto load_in = [&](int q) {
+        TrialIn in{};
+        in.ob = load_obs(w.pf_ob, F, q);
+        in.raw = {w.pf_an[LM_BX * F + q], w.pf_an[LM_BY * F + q], w.pf_an[LM_BZ * F + q]};
+        in.vel = {w.pf_an[LM_VX * F + q], w.pf_an[LM_VY * F + q], w.pf_an[LM_VZ * F + q]};
+        in.tref = w.pf_an[LM_T * F + q];
+        in.rho = w.crho[w.pf_lm[q]];
+        in.ids = w.pf_ids[q];
+        return in;
+    };
+    int p = ch.lm_begin + (int)threadIdx.x;
+    TrialIn next = p < ch.lm_end ? load_in(p) : TrialIn{};
+    for (; p < ch.lm_end; p += kBlock) {
+        const TrialIn cur = next;
+        if (p + kBlock < ch.lm_end) next = load_in(p + kBlock);
+        const int ids = cur.ids;
         LmData lm;
-        lm.raw = {w.pf_an[LM_BX * F + p], w.pf_an[LM_BY * F + p], w.pf_an[LM_BZ * F + p]};
-        lm.vel = {w.pf_an[LM_VX * F + p], w.pf_an[LM_VY * F + p], w.pf_an[LM_VZ * F + p]};
-        lm.tref = w.pf_an[LM_T * F + p];
-        lm.rho = w.crho[w.pf_lm[p]];
+        lm.raw = cur.raw;
+        lm.vel = cur.vel;
+        lm.tref = cur.tref;
+        lm.rho = cur.rho;
         lm.frame = ids >> 24;
         lm.cam = (ids >> 16) & 255;
         FactorJ J;
-        if (!eval_factor<false>(w, ft, lm, load_obs(w.pf_ob, F, p), ids & 0xffff, td, false, J)) { invalid = 1; continue; }
+        if (!eval_factor<false>(w, ft, lm, cur.ob, ids & 0xffff, td, false, J)) { invalid = 1; continue; }
         const double sq = J.r[0] * J.r[0] + J.r[1] * J.r[1];
         if (!isfinite(sq)) { invalid = 1; continue; }
         cost += huber_cost(w.huber, sq);
