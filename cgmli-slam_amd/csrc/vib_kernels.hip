@@ -1359,8 +1359,10 @@ constexpr int kAssembleRows = 1;                       // rows per workgroup (32
 constexpr int kAssembleSplit = kTile / kAssembleRows;  // workgroups per tile row
 
 // Retained (IMU + prior) normal-matrix entry (i, j) from the per-edge Jacobians and the prior
-// chart derivatives / Gram matrix, using the static column map.
-__device__ __forceinline__ double retained_entry(const WinDev& w, const int* ci, const int* cj) {
+// chart derivatives / Gram matrix, using the static column map. pv: row i of G D_i^T-side
+// product, pv[c] = sum_x D_i[x][ii] G[chart_i + x][c] over the prior's chart columns c (formed
+// once per row by prior_row), so the prior part of an entry is at most six products.
+__device__ __forceinline__ double retained_entry(const WinDev& w, const int* ci, const int* cj, const double* pv) {
     double h = 0;
     for (int a = 0; a < 2; ++a) {
         if (ci[a] < 0) continue;
@@ -1375,24 +1377,54 @@ __device__ __forceinline__ double retained_entry(const WinDev& w, const int* ci,
         }
     }
     if (ci[2] >= 0 && cj[2] >= 0) {
-        const int bi = ci[2] >> 8, ii = ci[2] & 255, bj = cj[2] >> 8, jj = cj[2] & 255;
-        const PriorBlockDev b1 = w.pblk[bi], b2 = w.pblk[bj];
-        const bool se1 = b1.kind == 0 || b1.kind == 2, se2 = b2.kind == 0 || b2.kind == 2;
-        const int n = w.prior_cols;
-        for (int x = 0; x < b1.tsize; ++x) {
-            const double d1 = se1 ? w.pD[bi * 36 + x * 6 + ii] : (x == ii ? 1.0 : 0.0);
-            if (d1 == 0) continue;
-            const double* G = w.pG + (size_t)(b1.chart + x) * n + b2.chart;
-            double inner = 0;
+        const int bj = cj[2] >> 8, jj = cj[2] & 255;
+        const PriorBlockDev b2 = w.pblk[bj];
+        const bool se2 = b2.kind == 0 || b2.kind == 2;
+        if (pv) {
             if (se2) {
-                for (int yy = 0; yy < 6; ++yy) inner += G[yy] * w.pD[bj * 36 + yy * 6 + jj];
+                double inner = 0;
+                for (int yy = 0; yy < 6; ++yy) inner += pv[b2.chart + yy] * w.pD[bj * 36 + yy * 6 + jj];
+                h += inner;
             } else {
-                inner = G[jj];
+                h += pv[b2.chart + jj];
             }
-            h += d1 * inner;
+        } else {  // a prior wider than the row buffer: the entry's own products
+            const int bi = ci[2] >> 8, ii = ci[2] & 255;
+            const PriorBlockDev b1 = w.pblk[bi];
+            const bool se1 = b1.kind == 0 || b1.kind == 2;
+            const int n = w.prior_cols;
+            for (int x = 0; x < b1.tsize; ++x) {
+                const double d1 = se1 ? w.pD[bi * 36 + x * 6 + ii] : (x == ii ? 1.0 : 0.0);
+                if (d1 == 0) continue;
+                const double* G = w.pG + (size_t)(b1.chart + x) * n + b2.chart;
+                double inner = 0;
+                if (se2) {
+                    for (int yy = 0; yy < 6; ++yy) inner += G[yy] * w.pD[bj * 36 + yy * 6 + jj];
+                } else {
+                    inner = G[jj];
+                }
+                h += d1 * inner;
+            }
         }
     }
     return h;
+}
+
+// pv[c] = sum_x D_i[x][ii] G[chart_i + x][c] for the prior's chart columns c of the row whose
+// column map is ci (a prior column); the workgroup's threads split the columns.
+__device__ __forceinline__ void prior_row(const WinDev& w, const int* ci, double* pv) {
+    const int bi = ci[2] >> 8, ii = ci[2] & 255;
+    const PriorBlockDev b1 = w.pblk[bi];
+    const bool se1 = b1.kind == 0 || b1.kind == 2;
+    const int n = w.prior_cols;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        double v = 0;
+        for (int x = 0; x < b1.tsize; ++x) {
+            const double d1 = se1 ? w.pD[bi * 36 + x * 6 + ii] : (x == ii ? 1.0 : 0.0);
+            if (d1 != 0) v += d1 * w.pG[(size_t)(b1.chart + x) * n + c];
+        }
+        pv[c] = v;
+    }
 }
 
 // Operand layout of a 16x16 tile: element (i, c) at (c / 4) * 64 + i + 16 (c % 4), so the 64
@@ -1414,9 +1446,16 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
     const int t = threadIdx.x, nt = blockDim.x;
     const bool fresh = w.lm->new_point;
     if (fresh) {
+        static_assert(kAssembleRows == 1, "one prior row product per workgroup");
+        // the prior products of this workgroup's row (a padding row of the last tile, row0 >= nr,
+        // has no column map entry and no entries)
+        __shared__ double prow[kMaxRetained];
+        const double* pv = w.prior_cols <= kMaxRetained ? prow : nullptr;
+        if (pv && row0 < nr && w.colmap[4 * row0 + 2] >= 0) prior_row(w, w.colmap + 4 * row0, prow);
+        __syncthreads();
         for (int p = t; p < (row1 - row0) * nr; p += nt) {
             const int i = row0 + p / nr, j = p % nr;
-            double h = retained_entry(w, w.colmap + 4 * i, w.colmap + 4 * j);
+            double h = retained_entry(w, w.colmap + 4 * i, w.colmap + 4 * j, pv);
             if (i < nv && j < nv) h += rl[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
             if (w.hostH) h += w.hostH[(size_t)i * nr + j];
             w.Hfull[(size_t)i * nr + j] = h;
