@@ -663,14 +663,16 @@ int Slot::upload(const HostWindow& h, hipStream_t stream, Staging& st, std::stri
     // path below leaves it unused (find_slot no longer returns it) with its arena kept for reuse.
     used = false;
     if (!arena || need > capacity) {  // a re-upload that fits (a slide) keeps the arena
+        // a quarter of headroom: the window's next appends are edited in place
+        const size_t want = need + need / 4;
         release();
-        if (hipMalloc(&arena, need) != hipSuccess) {
+        if (hipMalloc(&arena, want) != hipSuccess) {
             arena = nullptr;
             (void)hipGetLastError();
             err = "device allocation failed";
             return VIB_ERR_OUT_OF_MEMORY;
         }
-        capacity = need;
+        capacity = want;
     }
     host = h;
     bytes = need;
@@ -1302,7 +1304,7 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
     if (budgeted && ctx->hbm_budget > 0) {
         size_t resident = 0;
         for (const auto& kv : ctx->slots)
-            if (kv.second && kv.second.get() != &dst && kv.second->used) resident += kv.second->capacity;
+            if (kv.second && kv.second.get() != &dst && kv.second->used) resident += kv.second->bytes;
         if (resident + need > ctx->hbm_budget)
             return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "edited window does not fit the context's HBM budget");
     }
@@ -1342,25 +1344,32 @@ int edit_window(vib_context* ctx, Slot& src, EditSpec sp, HostWindow&& meta, Slo
         return fail(ctx, VIB_ERR_DEVICE, "window edit failed");
     }
     if (in_place) {
-        if (need <= dst.capacity) {
-            // the slot keeps its arena (and its footprint): the edited window is copied into it
-            // in stream order, at the offsets the spare arena had (the layout depends only on
-            // the window), and the spare stays the context's largest
-            WinDev nd2;
-            SlotCarve x2;
-            if (!Slot::carve(meta, static_cast<char*>(dst.arena), dst.capacity, nd2, x2))
-                return fail(ctx, VIB_ERR_DEVICE, "arena overflow");
-            if (hipMemcpyAsync(dst.arena, base, need, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+        // The edited window is copied into the slot's arena in stream order, at the offsets the
+        // spare arena had (the layout depends only on the window), so the spare stays the
+        // context's largest. A window that outgrew its arena (an append) first gets a new one
+        // with a quarter of headroom, which the next appends fit.
+        if (need > dst.capacity) {
+            void* grown = nullptr;
+            const size_t want = need + need / 4;
+            if (hipMalloc(&grown, want) != hipSuccess) {
                 (void)hipGetLastError();
-                dst.used = false;  // the arena may hold part of the edited window
-                return fail(ctx, VIB_ERR_DEVICE, "window edit failed");
+                return fail(ctx, VIB_ERR_OUT_OF_MEMORY, "window edit allocation failed");
             }
-            nd = nd2;
-            x = x2;
-        } else {
-            std::swap(dst.arena, ctx->spare);
-            std::swap(dst.capacity, ctx->spare_cap);
+            (void)hipFree(dst.arena);
+            dst.arena = grown;
+            dst.capacity = want;
         }
+        WinDev nd2;
+        SlotCarve x2;
+        if (!Slot::carve(meta, static_cast<char*>(dst.arena), dst.capacity, nd2, x2))
+            return fail(ctx, VIB_ERR_DEVICE, "arena overflow");
+        if (hipMemcpyAsync(dst.arena, base, need, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            (void)hipGetLastError();
+            dst.used = false;  // the arena may hold part of the edited window
+            return fail(ctx, VIB_ERR_DEVICE, "window edit failed");
+        }
+        nd = nd2;
+        x = x2;
     }
     dst.dev = nd;
     dst.reduced_lin = x.reduced_lin;
@@ -2230,7 +2239,7 @@ int vib_window_upload(vib_context* ctx, int32_t slot, const vib_window* w) {
     if (ctx->hbm_budget > 0) {
         size_t resident = 0;
         for (const auto& kv : ctx->slots)
-            if (kv.first != slot && kv.second && kv.second->used) resident += kv.second->capacity;
+            if (kv.first != slot && kv.second && kv.second->used) resident += kv.second->bytes;
         if (resident + need > ctx->hbm_budget) {
             err = "window does not fit the context's HBM budget";
             return drop(VIB_ERR_OUT_OF_MEMORY);
@@ -2254,7 +2263,7 @@ int vib_resident_bytes(vib_context* ctx, int64_t* bytes) {
     std::lock_guard<std::mutex> lock(ctx->mutex);
     size_t total = 0;
     for (const auto& kv : ctx->slots)
-        if (kv.second && kv.second->used) total += kv.second->capacity;
+        if (kv.second && kv.second->used) total += kv.second->bytes;
     *bytes = (int64_t)total;
     return VIB_OK;
 }
@@ -2264,7 +2273,7 @@ int vib_slot_bytes(vib_context* ctx, int32_t slot, int64_t* bytes) {
     std::lock_guard<std::mutex> lock(ctx->mutex);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
-    *bytes = (int64_t)s->capacity;
+    *bytes = (int64_t)s->bytes;
     return VIB_OK;
 }
 

@@ -290,7 +290,9 @@ int vib_window_upload(vib_context* ctx, int32_t slot, const vib_window* window);
  * would exceed `bytes` (0: no budget). vib_resident_bytes reports the current footprint. */
 int vib_set_hbm_budget(vib_context* ctx, int64_t bytes);
 int vib_resident_bytes(vib_context* ctx, int64_t* bytes);
-/* Device bytes held by one resident slot (its arena; part of vib_resident_bytes). */
+/* Footprint of one resident slot's window (part of vib_resident_bytes). Budgets count these
+ * footprints (vib_window_footprint of the resident windows); a slot's arena keeps up to a
+ * quarter more, so that the window's appends are edited in place. */
 int vib_slot_bytes(vib_context* ctx, int32_t slot, int64_t* bytes);
 /* Bytes of HBM a window of this shape occupies (admission accounting). */
 int64_t vib_window_footprint(const vib_window* window, int32_t precision);
