@@ -626,17 +626,35 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
             }
             if (lane == 0) ngram[wave] = cnt;
             __syncthreads();
-            if (t < kGramEntries)
-                for (int q = 0; q < kWaves; ++q)
-                    for (int g = 0; g < ngram[q]; ++g) {
-                        const int k2 = gkey[q][g];
-                        if (k2 != ckey) {
-                            if (ckey >= 0) scatter_gram(acc, gacc, w, ft, ckey / 32, ckey % 32, gi, gj, cval);
-                            ckey = k2;
-                            cval = 0;
-                        }
-                        cval += glist[q][g][t];
+            if (t < kGramEntries) {
+                // listed matrices in (wave, list position) order; the next one's key and entry
+                // are read from LDS while the current one is merged
+                static_assert(kWaves * kGramsPerWave <= 32, "one bit per listed matrix");
+                unsigned listed = 0;
+#pragma unroll
+                for (int q = 0; q < kWaves; ++q) listed |= ((1u << ngram[q]) - 1u) << (kGramsPerWave * q);
+                const int* keys = &gkey[0][0];
+                const double* vals = &glist[0][0][0] + t;
+                int s = listed ? __builtin_ctz(listed) : -1;
+                int knext = s >= 0 ? keys[s] : -1;
+                double vnext = s >= 0 ? vals[s * kGramEntries] : 0.0;
+                while (s >= 0) {
+                    const int k2 = knext;
+                    const double val = vnext;
+                    listed &= listed - 1u;
+                    s = listed ? __builtin_ctz(listed) : -1;
+                    if (s >= 0) {
+                        knext = keys[s];
+                        vnext = vals[s * kGramEntries];
                     }
+                    if (k2 != ckey) {
+                        if (ckey >= 0) scatter_gram(acc, gacc, w, ft, ckey / 32, ckey % 32, gi, gj, cval);
+                        ckey = k2;
+                        cval = 0;
+                    }
+                    cval += val;
+                }
+            }
             __syncthreads();
         }
         wave_sync();
