@@ -1452,6 +1452,7 @@ __device__ __forceinline__ int opnd_pos(int i, int c) { return (c >> 2) * 64 + i
 
 __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const* reduced) {
     __shared__ double red[kBlock];
+    __shared__ double srow[kAssembleRows][kMaxRetained];  // the rows: undamped, then damped
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
     const int nr = w.nr, nv = w.nv;
@@ -1477,6 +1478,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
             if (i < nv && j < nv) h += rl[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
             if (w.hostH) h += w.hostH[(size_t)i * nr + j];
             w.Hfull[(size_t)i * nr + j] = h;
+            srow[i - row0][j] = h;
         }
         for (int i = row0 + t; i < row1; i += nt) {
             const int* ci = w.colmap + 4 * i;
@@ -1521,19 +1523,20 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
         }
         __syncthreads();
     }
+    // the damped row, once: to the saved system and to LDS, from where the tile store below
+    // takes it in the factorization order
     const double lambda = w.lm->lambda;
-    auto damped = [&](int i, int j) {
-        const double h = w.Hfull[(size_t)i * nr + j];
+    for (int p = t; p < (row1 - row0) * nr; p += nt) {
+        const int i = row0 + p / nr, j = p % nr;
+        const double h = fresh ? srow[i - row0][j] : w.Hfull[(size_t)i * nr + j];
         double v = h;
         if (i < nv && j < nv) v += rs[i <= j ? tri_index(i, j, nv) : tri_index(j, i, nv)];
         if (i == j) v += lambda * fmax(h, 1e-6);
-        return v;
-    };
-    for (int p = t; p < (row1 - row0) * nr; p += nt) {
-        const int i = row0 + p / nr, j = p % nr;
-        w.Scopy[(size_t)i * nr + j] = damped(i, j);
+        w.Scopy[(size_t)i * nr + j] = v;
+        srow[i - row0][j] = v;
     }
     for (int i = row0 + t; i < row1; i += nt) w.rhs[i] = -(w.gfull[i] + (i < nv ? rs[w.tri + i] : 0.0));
+    __syncthreads();
     // this workgroup's rows of the factor tile store, in the factorization order (speed/bias
     // columns first; padding rows and columns stay in place as identity) and operand layout
     const int nsb = nr - nv;
@@ -1543,7 +1546,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
         for (int pc = t; pc < (P / kTile + 1) * kTile; pc += nt) {
             const int c = pc >= nr ? pc : (pc < nsb ? pc + nv : pc - nsb);
             w.fac[tile_base(P / kTile, pc / kTile) + opnd_pos(P % kTile, pc % kTile)] =
-                (r < nr && c < nr) ? damped(r, c) : (r == c ? 1.0 : 0.0);
+                (r < nr && c < nr) ? srow[rho][c] : (r == c ? 1.0 : 0.0);
         }
     }
 }
@@ -2750,21 +2753,27 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
         double tref, rho;
         int ids;
     };
-    auto load_in = [&](int q) {
+    // (the inverse range through the factor's landmark index, which is read one factor earlier
+    // still, so the gather does not wait on the index load)
+    auto load_in = [&](int q, int lmi) {
         TrialIn in{};
         in.ob = load_obs(w.pf_ob, F, q);
         in.raw = {w.pf_an[LM_BX * F + q], w.pf_an[LM_BY * F + q], w.pf_an[LM_BZ * F + q]};
         in.vel = {w.pf_an[LM_VX * F + q], w.pf_an[LM_VY * F + q], w.pf_an[LM_VZ * F + q]};
         in.tref = w.pf_an[LM_T * F + q];
-        in.rho = w.crho[w.pf_lm[q]];
+        in.rho = w.crho[lmi];
         in.ids = w.pf_ids[q];
         return in;
     };
     int p = ch.lm_begin + (int)threadIdx.x;
-    TrialIn next = p < ch.lm_end ? load_in(p) : TrialIn{};
+    int lmi_next = p + kBlock < ch.lm_end ? w.pf_lm[p + kBlock] : 0;
+    TrialIn next = p < ch.lm_end ? load_in(p, w.pf_lm[p]) : TrialIn{};
     for (; p < ch.lm_end; p += kBlock) {
         const TrialIn cur = next;
-        if (p + kBlock < ch.lm_end) next = load_in(p + kBlock);
+        if (p + kBlock < ch.lm_end) {
+            next = load_in(p + kBlock, lmi_next);
+            lmi_next = p + 2 * kBlock < ch.lm_end ? w.pf_lm[p + 2 * kBlock] : 0;
+        }
         const int ids = cur.ids;
         LmData lm;
         lm.raw = cur.raw;
