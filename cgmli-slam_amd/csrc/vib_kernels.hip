@@ -2663,15 +2663,36 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
     __syncthreads();
     double pred = 0, mx = 0;
     int bad = 0;
+    // a landmark's records; the next pass's are loaded before this pass chases its factors
+    struct LmIn {
+        int col, fc, f0, f1;
+        double rho, dd, gk;
+    };
+    auto load_lm = [&](int l) {
+        LmIn in{-1, 0, 0, 0, 0.0, 0.0, 0.0};
+        if (l < ch.lm_end) {
+            in.col = w.lm_col[l];
+            in.fc = w.lm_fc[l];
+            in.f0 = w.lm_off[l];
+            in.f1 = w.lm_off[l + 1];
+            in.rho = w.rho[l];
+            in.dd = w.lm_out[LO_D * L + l];
+            in.gk = w.lm_out[LO_G * L + l];
+        }
+        return in;
+    };
+    LmIn next = load_lm(ch.lm_begin + group);
     for (int base = ch.lm_begin; base < ch.lm_end; base += groups) {
         const int l = base + group;
+        const LmIn cur = next;
+        next = load_lm(l + groups);
         const bool live = l < ch.lm_end;
-        const bool free = live && w.lm_col[l] >= 0;
+        const bool free = live && cur.col >= 0;
         double cd = 0;
         if (free) {
-            const int a = w.lm_fc[l] >> 8;
+            const int a = cur.fc >> 8;
             const int ac = pcol[a];
-            for (int f = w.lm_off[l] + lane; f < w.lm_off[l + 1]; f += kChunkLanes) {
+            for (int f = cur.f0 + lane; f < cur.f1; f += kChunkLanes) {
                 const int b = w.ob_fc[f] >> 8;
                 if (b == a) continue;
                 const int bc = pcol[b];
@@ -2684,12 +2705,12 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
         }
         cd = group_sum(cd);
         if (live && lane == 0) {
-            const double rho = w.rho[l];
+            const double rho = cur.rho;
             if (free) {
                 if (w.rig_col >= 0)
                     for (int k = 0; k < 6; ++k) cd += w.lm_out[(LO_CR0 + k) * L + l] * dxv[w.rig_col + k];
                 if (w.td_col >= 0) cd += w.lm_out[LO_CT * L + l] * dxv[w.td_col];
-                const double dd = w.lm_out[LO_D * L + l], gk = w.lm_out[LO_G * L + l];
+                const double dd = cur.dd, gk = cur.gk;
                 const double d = -(gk + cd) / damped_point(dd, lambda);
                 const double delta = isfinite(d) ? fmax(w.min_rho - rho, d) : d;
                 pred += -delta * (gk + cd + 0.5 * dd * delta);
