@@ -1216,42 +1216,71 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
         }
     }
     double gsum = 0;  // MIXED: gradient correction of column t (t < nv)
-    for (int base = ch.lm_begin; base < ch.lm_end; base += kSchurRows) {
-        for (int i = t; i < kSchurRows * kSchurLd; i += kSchurBlock) crow[i] = CT(0);
-        // Dense coupling rows. Group g builds rows g + j groups (j < kSchurRowsPerGroup): every
-        // load of its rows (landmark record, first factor of each lane, its coupling terms) is
-        // issued before the first LDS update, so the dependent global round trips of the rows
-        // overlap instead of following each other.
-        int lmk[kSchurRowsPerGroup], f1[kSchurRowsPerGroup], fa[kSchurRowsPerGroup], pa[kSchurRowsPerGroup];
-        bool live[kSchurRowsPerGroup];
-        double dl[kSchurRowsPerGroup], gl[kSchurRowsPerGroup];
+    // Dense coupling rows. Group g builds rows g + j groups (j < kSchurRowsPerGroup) of a pass from
+    // three dependent global loads: the landmark record, the first factor of each lane (observer,
+    // pair position), its coupling terms. In FP64 the passes are pipelined: a pass's landmark
+    // records are loaded while the previous pass builds its rows, its factors during the previous
+    // pass's MFMAs, so only the coupling-term round trip is left at the head of a pass. (Mixed
+    // precision builds two rows per group; holding the next pass's loads would spill there.)
+    constexpr bool kPipe = !MIXED;
+    struct RowIn {
+        int lmk, f1, fa, pa, fb, fp;
+        bool live;
+        double dl, gl;
+    };
+    auto load_records = [&](int base, RowIn* r) {
 #pragma unroll
         for (int j = 0; j < kSchurRowsPerGroup; ++j) {
             const int l = base + group + j * kSchurGroups;
             const int lc = l < ch.lm_end ? l : ch.lm_begin;  // the loads stay inside the chunk
-            lmk[j] = l;
-            live[j] = l < ch.lm_end && w.lm_col[lc] >= 0;
-            fa[j] = w.lm_off[lc] + lgroup;
-            f1[j] = live[j] ? w.lm_off[lc + 1] : 0;  // a row that is not live has no factor to load
-            pa[j] = w.lm_fc[lc] >> 8;
-            dl[j] = w.lm_out[LO_D * L + lc];
-            gl[j] = w.lm_out[LO_G * L + lc];
+            r[j].lmk = l;
+            r[j].live = l < ch.lm_end && w.lm_col[lc] >= 0;
+            r[j].fa = w.lm_off[lc] + lgroup;
+            r[j].f1 = r[j].live ? w.lm_off[lc + 1] : 0;  // a row that is not live has no factor to load
+            r[j].pa = w.lm_fc[lc] >> 8;
+            r[j].dl = w.lm_out[LO_D * L + lc];
+            r[j].gl = w.lm_out[LO_G * L + lc];
         }
-        int fb[kSchurRowsPerGroup], fp[kSchurRowsPerGroup];
+    };
+    auto load_factors = [&](RowIn* r) {
 #pragma unroll
         for (int j = 0; j < kSchurRowsPerGroup; ++j) {
-            fb[j] = -1;
-            fp[j] = 0;
-            if (fa[j] < f1[j]) {
-                fb[j] = w.ob_fc[fa[j]] >> 8;
-                fp[j] = w.pf_pos[fa[j]];
+            r[j].fb = -1;
+            r[j].fp = 0;
+            if (r[j].fa < r[j].f1) {
+                r[j].fb = w.ob_fc[r[j].fa] >> 8;
+                r[j].fp = w.pf_pos[r[j].fa];
             }
         }
+    };
+    RowIn cur[kSchurRowsPerGroup], nxt[kSchurRowsPerGroup];
+    if constexpr (kPipe) {
+        load_records(ch.lm_begin, cur);
+        load_factors(cur);
+    }
+    for (int base = ch.lm_begin; base < ch.lm_end; base += kSchurRows) {
+        if constexpr (!kPipe) {
+            load_records(base, cur);
+            load_factors(cur);
+        }
+        for (int i = t; i < kSchurRows * kSchurLd; i += kSchurBlock) crow[i] = CT(0);
         CT u[kSchurRowsPerGroup][6];  // FP32 hi parts in mixed precision (the products use them alone)
 #pragma unroll
         for (int j = 0; j < kSchurRowsPerGroup; ++j)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) u[j][k] = fa[j] < f1[j] ? CT(load_u<MIXED>(w, F, k, fp[j])) : CT(0);
+            for (int k = 0; k < 6; ++k)
+                u[j][k] = cur[j].fa < cur[j].f1 ? CT(load_u<MIXED>(w, F, k, cur[j].fp)) : CT(0);
+        const bool more = kPipe && base + kSchurRows < ch.lm_end;
+        if (more) load_records(base + kSchurRows, nxt);
+        int lmk[kSchurRowsPerGroup], f1[kSchurRowsPerGroup], fa[kSchurRowsPerGroup], pa[kSchurRowsPerGroup];
+        int fb[kSchurRowsPerGroup];
+        bool live[kSchurRowsPerGroup];
+        double dl[kSchurRowsPerGroup], gl[kSchurRowsPerGroup];
+#pragma unroll
+        for (int j = 0; j < kSchurRowsPerGroup; ++j) {
+            lmk[j] = cur[j].lmk; f1[j] = cur[j].f1; fa[j] = cur[j].fa; pa[j] = cur[j].pa; fb[j] = cur[j].fb;
+            live[j] = cur[j].live; dl[j] = cur[j].dl; gl[j] = cur[j].gl;
+        }
         __syncthreads();  // rows cleared, pose columns staged
 #pragma unroll
         for (int j = 0; j < kSchurRowsPerGroup; ++j) {
@@ -1294,6 +1323,7 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
             }
         }
         __syncthreads();
+        if (more) load_factors(nxt);
         // FP64: C~^T C~ on MFMA, A[i][k] = C~[k][16 ti + i] (lane i + 16 k), B[k][j] = C~[k][16 tj + j].
         // MIXED: C^T diag(s) C with the scale on the A operand.
 #pragma unroll
@@ -1326,6 +1356,9 @@ __device__ __forceinline__ void schur_body(const WinDev* wins, const Chunk* chun
                 for (int k = 0; k < kSchurRows; ++k) gsum += qg[k] * (double)crow[k * kSchurLd + t];
         }
         __syncthreads();
+        if constexpr (kPipe)
+#pragma unroll
+            for (int j = 0; j < kSchurRowsPerGroup; ++j) cur[j] = nxt[j];
     }
     // chunk partial: packed upper triangle of -C^T diag(s) C, then -C^T (s g); output element
     // (row, col) of the tile in register rr: f64 MFMA row 4 rr + lane / 16, f32 MFMA row
