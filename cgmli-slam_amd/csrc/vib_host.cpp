@@ -695,12 +695,12 @@ int Slot::upload(const HostWindow& h, hipStream_t stream, Staging& st, std::stri
 // Batch: device copies of the WinDev descriptors and the concatenated chunk list.
 // ------------------------------------------------------------------------------------------
 Batch::~Batch() {
-    if (d_wins) hipFree(d_wins);
-    if (d_chunks) hipFree(d_chunks);
-    if (d_lin) hipFree(d_lin);
-    if (d_red) hipFree(d_red);
-    if (graph_exec) hipGraphExecDestroy(graph_exec);
-    if (graph) hipGraphDestroy(graph);
+    if (d_wins) (void)hipFree(d_wins);
+    if (d_chunks) (void)hipFree(d_chunks);
+    if (d_lin) (void)hipFree(d_lin);
+    if (d_red) (void)hipFree(d_red);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
 }
 
 bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, Staging& xs, std::string& err) {
@@ -772,11 +772,11 @@ bool Batch::build(std::vector<Slot*>& slots, hipStream_t stream, Staging& xs, st
         }
     // a rebuilt batch keeps its device buffers when they are large enough; a captured graph of
     // the previous contents is dropped
-    if (graph_exec) { hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-    if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+    if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     auto grow = [](auto*& ptr, size_t& cap, size_t need, size_t elem) {
         if (ptr && cap >= need) return true;
-        if (ptr) hipFree(ptr);
+        if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         cap = std::max(need, cap + cap / 2);
         return hipMalloc((void**)&ptr, elem * cap) == hipSuccess;
@@ -2198,17 +2198,17 @@ int vib_context_device(const vib_context* ctx, int32_t* device) {
 
 int vib_destroy(vib_context* ctx) {
     if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->slots.clear();
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VIB_OK;
 }
 
 int vib_device_synchronize(vib_context* ctx) {
     if (!ctx) return VIB_ERR_INVALID_ARGUMENT;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     return hipDeviceSynchronize() == hipSuccess ? VIB_OK : fail(ctx, VIB_ERR_DEVICE, "device synchronization failed");
 }
 
@@ -2226,7 +2226,7 @@ int64_t vib_window_footprint(const vib_window* w, int32_t precision) {
 int vib_window_upload(vib_context* ctx, int32_t slot, const vib_window* w) {
     if (!ctx || !w || slot < 0) return VIB_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     auto h = std::make_unique<HostWindow>();
     std::string err;
     // a failed upload leaves the slot empty (its previous window is released)
@@ -2280,7 +2280,7 @@ int vib_slot_bytes(vib_context* ctx, int32_t slot, int64_t* bytes) {
 int vib_set_host_retained(vib_context* ctx, int32_t slot, const double* H, const double* g) {
     if (!ctx || (!H) != (!g)) return VIB_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
     if (!H) {
@@ -2315,7 +2315,7 @@ int vib_set_host_retained(vib_context* ctx, int32_t slot, const double* H, const
 int vib_window_append(vib_context* ctx, int32_t slot, const vib_keyframe* kf, int32_t* unmatched) {
     if (!ctx || !kf) return VIB_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
     return append_keyframe(ctx, *s, *kf, unmatched);
@@ -2515,7 +2515,7 @@ int vib_request_begin(vib_context* ctx, int32_t slot, const vib_solve_options* o
     if (!ctx || !o) return VIB_ERR_INVALID_ARGUMENT;
     if (o->precision != VIB_FP64 && o->precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
     if (!write_lm(ctx, *s, fresh_lm(*o))) return fail(ctx, VIB_ERR_DEVICE, "LM state upload failed");
@@ -2527,7 +2527,7 @@ int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* ids, int
     if (!ctx || !ids || !done || count <= 0) return VIB_ERR_INVALID_ARGUMENT;
     if (precision != VIB_FP64 && precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<Slot*> slots;
     for (int i = 0; i < count; ++i) {
@@ -2563,7 +2563,7 @@ int vib_request_iterate(vib_context* ctx, int32_t count, const int32_t* ids, int
 int vib_copy_probe(vib_context* ctx, int64_t bytes) {
     if (!ctx || bytes <= 0 || bytes % 16) return VIB_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     void *src = nullptr, *dst = nullptr;
     if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) {
         if (src) (void)hipFree(src);
@@ -2686,7 +2686,7 @@ int vib_preintegrate_batch(vib_context* ctx, int32_t count, const double* sample
         !params || !gyro_start || !gyro_end || !out || !status)
         return VIB_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     return preintegrate_locked(ctx, count, samples, num_samples, sample_begin, sample_end, t0, t1, speed_bias, params,
                                gyro_start, gyro_end, out, status, kernel_ms);
 }
@@ -2758,7 +2758,7 @@ static int nis_gate_locked(vib_context* ctx, Slot* s, double threshold, int appl
     rep->evaluated.worst_factor = rep->admitted.worst_factor = -1;
     const int F = s->host.F;
     if (F == 0) return VIB_OK;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     std::vector<Slot*> one{s};
     Batch& b = ctx->one;
     b.mixed = 0;
@@ -2798,7 +2798,7 @@ int vib_nis_gate_static(vib_context* ctx, int32_t slot, double threshold, int32_
     if ((size_t)h.F > kStaticGateFactors || h.chunks.size() > kStaticGateChunks)
         return fail(ctx, VIB_ERR_UNSUPPORTED_DIMENSIONS,
                     "window beyond the no-allocation gate's scratch (2^20 factors, 2^17 landmark chunks)");
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     char* base = reinterpret_cast<char*>(ctx->scratch);  // device address of `scratch`, never the name
     WinDev wd = s->dev;
     wd.nchunks = (int)h.chunks.size();
@@ -2888,7 +2888,7 @@ int vib_whiten_observations(vib_context* ctx, int32_t count, const double* pixel
     if (!camera && num_cameras != 1) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "camera indices required for several cameras");
     if (count == 0) return VIB_OK;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     const size_t n = count;
     // one allocation: pixels | intrinsics | sigma | bearings | sqrt_info | camera | status
     const size_t nd = 2 * n + 6 * (size_t)num_cameras + (sigma ? n : 0) + 6 * n;
@@ -2939,7 +2939,7 @@ int vib_bench(vib_context* ctx, int32_t count, const int32_t* ids, const vib_sol
     if (!ctx || !o || !ids || count <= 0 || iterations <= 0) return VIB_ERR_INVALID_ARGUMENT;
     if (o->precision != VIB_FP64 && o->precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     std::vector<Slot*> slots;
     for (int i = 0; i < count; ++i) {
         Slot* s = find_slot(ctx, ids[i]);
@@ -3007,7 +3007,7 @@ int vib_autotune(vib_context* ctx, int32_t count, const int32_t* ids, const vib_
     if (!ctx || !o || !ids || !rep || count <= 0 || iterations <= 0) return VIB_ERR_INVALID_ARGUMENT;
     if (o->precision != VIB_FP64 && o->precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     *rep = vib_tune_report{};
     std::vector<Slot*> slots;
     for (int i = 0; i < count; ++i) {
@@ -3089,7 +3089,7 @@ int vib_solve_latest_two(vib_context* ctx, int32_t slot, const vib_solve_options
     if (!ctx || !o || !rep) return VIB_ERR_INVALID_ARGUMENT;
     if (o->precision != VIB_FP64 && o->precision != VIB_MIXED) return fail(ctx, VIB_ERR_INVALID_ARGUMENT, "unknown precision");
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     *rep = vib_latest_two_report{};
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
@@ -3309,7 +3309,7 @@ int vib_marginalize(vib_context* ctx, int32_t slot, int32_t policy, const double
                     double t_start, double t_end, const vib_imu_params* params, vib_marginal_report* rep) {
     if (!ctx || !rep) return VIB_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     *rep = vib_marginal_report{};
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
@@ -3325,7 +3325,7 @@ int vib_marginalize_batch(vib_context* ctx, int32_t count, const int32_t* ids, i
                           vib_marginal_report* reps) {
     if (!ctx || !ids || !reps || count <= 0) return VIB_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     std::vector<Slot*> slots;
     for (int i = 0; i < count; ++i) {
         reps[i] = vib_marginal_report{};
@@ -3350,7 +3350,7 @@ int vib_marginalize_batch(vib_context* ctx, int32_t count, const int32_t* ids, i
 int vib_read_window(vib_context* ctx, int32_t slot, vib_window* out) {
     if (!ctx || !out) return VIB_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     Slot* s = find_slot(ctx, slot);
     if (!s) return fail(ctx, VIB_ERR_NOT_FOUND, "unknown slot");
     if (!ensure_bulk(ctx, *s) || !refresh_state(ctx, *s)) return fail(ctx, VIB_ERR_DEVICE, "window download failed");
