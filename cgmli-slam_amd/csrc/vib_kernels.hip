@@ -430,8 +430,11 @@ __device__ __forceinline__ void store_pair(float* planes, size_t F, int hi_plane
 // workgroups of the visual linearization, so it overlaps the visual work in one launch. (The
 // trial cost stays a launch of its own: inside k_trial_cost its 280 registers would halve that
 // kernel's occupancy.)
+// LDS doubles of one retained item's wave: the prior's chart and residual, or an IMU edge's
+// whitening, whitened residual and raw Jacobian
+constexpr int kRetainedScratch = 2 * kMaxRetained > 240 + 450 ? 2 * kMaxRetained : 240 + 450;
 template <int MODE>
-__device__ void retained_body(const WinDev& w, int y, int lane, double* scratch);
+__device__ __forceinline__ void retained_body(const WinDev& w, int y, int lane, double* scratch);
 // The retained items as extra workgroups of a chunk kernel: item i = (window i / per,
 // edge or prior i % per), one per wave.
 template <int MODE>
@@ -456,6 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     __shared__ int ngram[kWaves];
     __shared__ double red[kBlock];
     if ((int)blockIdx.x < rb) {  // the row staging of the wave holds the prior's scratch
+        static_assert(kGramCols * kRowPitch >= kRetainedScratch, "a wave's row staging holds a retained item");
         retained_items<0>(wins, 0, items, per, rows[threadIdx.x / 64]);
         return;
     }
@@ -728,123 +732,6 @@ __device__ __forceinline__ void mm3(const double* A, const double* B, double* C)
         for (int c = 0; c < 3; ++c) C[r * 3 + c] = A[r * 3] * B[c] + A[r * 3 + 1] * B[3 + c] + A[r * 3 + 2] * B[6 + c];
 }
 
-// Evaluates one IMU edge: r (15) and, if J, the 15 x 30 tangent Jacobian [pose0 6 | sb0 9 | pose1 6 | sb1 9]
-// (row-major, whitened). Returns false on non-finite results.
-__device__ bool eval_imu(const double* d, const double* pose0, const double* sb0, const double* pose1,
-                         const double* sb1, double* r, double* J) {
-    const ImuSensor s0 = imu_sensor(d, pose0, sb0, 0), s1 = imu_sensor(d, pose1, sb1, 1);
-    const M3<double> R = qmatrix(qconj(s0.q));
-    const double dt = d[0];
-    const V3<double> grav = {0, 0, d[1]};
-    const V3<double> bg = load3(sb0 + 3) - load3(d + 25), ba = load3(sb0 + 6) - load3(d + 28);
-    const V3<double> dp = s0.p - s1.p + dt * s0.v - (0.5 * dt * dt) * grav;
-    const V3<double> dv = s0.v - s1.v - dt * grav;
-    const V3<double> phi = -1.0 * m3v(d + 31, bg);
-    const Qt<double> corr = rotation_exp(phi);
-    const Qt<double> Dq = qload(d + 15);
-    const Qt<double> dq = qmul(corr, Dq);
-    const Qt<double> rel = qmul(qconj(s1.q), s0.q);
-    const Qt<double> tail = qmul(qmul(Dq, qconj(s1.q)), s0.q);
-    double e[15];
-    const V3<double> ep = mul(R, dp) + load3(d + 22) + m3v(d + 49, bg) - m3v(d + 67, ba);
-    const V3<double> eq = 2.0 * qvec(qmul(dq, rel));
-    const V3<double> ev = mul(R, dv) + load3(d + 19) + m3v(d + 40, bg) - m3v(d + 58, ba);
-    e[0] = ep.x; e[1] = ep.y; e[2] = ep.z; e[3] = eq.x; e[4] = eq.y; e[5] = eq.z; e[6] = ev.x; e[7] = ev.y; e[8] = ev.z;
-    for (int i = 0; i < 6; ++i) e[9 + i] = sb0[3 + i] - sb1[3 + i];
-    const double* W = d + 76;
-    bool ok = true;
-    for (int i = 0; i < 15; ++i) {
-        double s = 0;
-        for (int k = 0; k < 15; ++k) s += W[i * 15 + k] * e[k];
-        r[i] = s;
-        ok = ok && isfinite(s);
-    }
-    if (!ok || !J) return ok;
-    // gyroAngle = -J_r(-phi)... rotationJacobian(-phi) * A, as columns
-    double GA[9];
-    {
-        // rotationJacobian(v) = I + c1 [v]x + c2 [v]x^2 at v = -phi
-        const V3<double> v = -1.0 * phi;
-        const double a2 = sqnorm(v), n = sqrt(a2);
-        const double c1 = a2 < 1e-10 ? 0.5 - a2 / 24 : (1 - cos(n)) / a2;
-        const double c2 = a2 < 1e-10 ? 1.0 / 6 - a2 / 120 : (n - sin(n)) / (a2 * n);
-        double Sk[9], S2[9], Jl[9];
-        skew_of(v, Sk);
-        mm3(Sk, Sk, S2);
-        for (int i = 0; i < 9; ++i) Jl[i] = (i % 4 == 0 ? 1.0 : 0.0) + c1 * Sk[i] + c2 * S2[i];
-        mm3(Jl, d + 31, GA);
-        for (int i = 0; i < 9; ++i) GA[i] = -GA[i];
-    }
-    double Jrot[9], Jbias[9];
-    const Qt<double> left = qmul(dq, qconj(s1.q));
-    for (int j = 0; j < 3; ++j) {
-        const Qt<double> ax = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0, 0.0};
-        const V3<double> c1 = qvec(qmul(qmul(left, ax), s0.q));
-        const Qt<double> dbq = {GA[j], GA[3 + j], GA[6 + j], 0.0};
-        const V3<double> c2 = qvec(qmul(qmul(corr, dbq), tail));
-        Jrot[j] = c1.x; Jrot[3 + j] = c1.y; Jrot[6 + j] = c1.z;
-        Jbias[j] = c2.x; Jbias[3 + j] = c2.y; Jbias[6 + j] = c2.z;
-    }
-    // raw (unwhitened) Jacobian blocks, 15 x 30
-    double Jr[15 * 30];
-    for (int i = 0; i < 15 * 30; ++i) Jr[i] = 0;
-    auto put = [&](int r0, int c0, const double* m, double s) {
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) Jr[(r0 + a) * 30 + c0 + b] = s * m[a * 3 + b];
-    };
-    auto add = [&](int r0, int c0, const double* m, double s) {
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) Jr[(r0 + a) * 30 + c0 + b] += s * m[a * 3 + b];
-    };
-    double tmp[9], tmp2[9], Sk[9];
-    for (int k = 0; k < 4; ++k) {
-        const bool first = k < 2;
-        const double sign = first ? 1.0 : -1.0;
-        const ImuSensor& s = first ? s0 : s1;
-        const int c0 = k == 0 ? 0 : (k == 1 ? 6 : (k == 2 ? 15 : 21));
-        if (k % 2 == 0) {
-            put(0, c0, R.m, sign);
-            skew_of(s.p, Sk); mm3(R.m, Sk, tmp); put(0, c0 + 3, tmp, -sign);
-            put(3, c0 + 3, Jrot, sign);
-            skew_of(s.lever, Sk); mm3(R.m, Sk, tmp); put(6, c0 + 3, tmp, -sign);
-            if (first) {
-                double Sdp[9], Sl[9];
-                skew_of(dp, Sdp); skew_of(s.lever, Sl);
-                for (int i = 0; i < 9; ++i) tmp2[i] = Sdp[i] - dt * Sl[i];
-                mm3(R.m, tmp2, tmp); add(0, c0 + 3, tmp, 1.0);
-                skew_of(dv, Sk); mm3(R.m, Sk, tmp); add(6, c0 + 3, tmp, 1.0);
-            }
-        } else {
-            double lb[9];
-            const M3<double> Rs = qmatrix(s.q);
-            skew_of(s.arm, Sk); mm3(Sk, Rs.m, lb);
-            put(6, c0, R.m, sign);
-            mm3(R.m, lb, tmp); put(6, c0 + 3, tmp, sign);
-            double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-            put(9, c0 + 3, I3, sign);
-            put(12, c0 + 6, I3, sign);
-            if (first) {
-                put(0, c0, R.m, dt);
-                mm3(R.m, lb, tmp);
-                for (int i = 0; i < 9; ++i) tmp[i] = dt * tmp[i] + d[49 + i];
-                put(0, c0 + 3, tmp, 1.0);
-                put(0, c0 + 6, d + 67, -1.0);
-                put(3, c0 + 3, Jbias, 1.0);
-                add(6, c0 + 3, d + 40, 1.0);
-                put(6, c0 + 6, d + 58, -1.0);
-            }
-        }
-    }
-    for (int i = 0; i < 15; ++i)
-        for (int c = 0; c < 30; ++c) {
-            double s = 0;
-            for (int m = 0; m < 15; ++m) s += W[i * 15 + m] * Jr[m * 30 + c];
-            J[i * 30 + c] = s;
-            ok = ok && isfinite(s);
-        }
-    return ok;
-}
-
 // Quantities of one IMU edge shared by all Jacobian columns (GpuRetainedFactorMath.hpp:71-101).
 struct ImuShared {
     ImuSensor s0, s1;
@@ -852,8 +739,7 @@ struct ImuShared {
     double dt;
     V3<double> dp, dv;
     double Jrot[9], Jbias[9];
-    double r[15];
-    bool ok;
+    double e[15];  // unwhitened residual
 };
 
 __device__ void imu_shared(const double* d, const double* pose0, const double* sb0, const double* pose1,
@@ -871,20 +757,12 @@ __device__ void imu_shared(const double* d, const double* pose0, const double* s
     const Qt<double> Dq = qload(d + 15);
     const Qt<double> dq = qmul(corr, Dq);
     const Qt<double> rel = qmul(qconj(o.s1.q), o.s0.q);
-    double e[15];
+    double* e = o.e;
     const V3<double> ep = mul(o.R, o.dp) + load3(d + 22) + m3v(d + 49, bg) - m3v(d + 67, ba);
     const V3<double> eq = 2.0 * qvec(qmul(dq, rel));
     const V3<double> ev = mul(o.R, o.dv) + load3(d + 19) + m3v(d + 40, bg) - m3v(d + 58, ba);
     e[0] = ep.x; e[1] = ep.y; e[2] = ep.z; e[3] = eq.x; e[4] = eq.y; e[5] = eq.z; e[6] = ev.x; e[7] = ev.y; e[8] = ev.z;
     for (int i = 0; i < 6; ++i) e[9 + i] = sb0[3 + i] - sb1[3 + i];
-    const double* W = d + 76;
-    o.ok = true;
-    for (int i = 0; i < 15; ++i) {
-        double v = 0;
-        for (int k = 0; k < 15; ++k) v += W[i * 15 + k] * e[k];
-        o.r[i] = v;
-        o.ok = o.ok && isfinite(v);
-    }
     if (!jac) return;
     double GA[9];
     {
@@ -919,9 +797,18 @@ __device__ void imu_raw_column(const double* d, const ImuShared& o, int c, doubl
     const int cc = c - (k == 0 ? 0 : (k == 1 ? 6 : (k == 2 ? 15 : 21)));
     const bool first = k < 2;
     const double sign = first ? 1.0 : -1.0;
-    const ImuSensor& s = first ? o.s0 : o.s1;
+    ImuSensor s;  // the edge's sensor at this column's end (selected by value: no indexed copy)
+    s.p = first ? o.s0.p : o.s1.p;
+    s.v = first ? o.s0.v : o.s1.v;
+    s.arm = first ? o.s0.arm : o.s1.arm;
+    s.lever = first ? o.s0.lever : o.s1.lever;
+    s.q = first ? o.s0.q : o.s1.q;
     const double* R = o.R.m;
-    auto mcol = [&](const double* M, int j, double* out3) { out3[0] = M[j]; out3[1] = M[3 + j]; out3[2] = M[6 + j]; };
+    // column j < 3 of a row-major 3 x 3 matrix, by selects so that the matrix stays in registers
+    auto mcol = [](const double* M, int j, double* out3) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out3[i] = j == 0 ? M[3 * i] : (j == 1 ? M[3 * i + 1] : M[3 * i + 2]);
+    };
     double v[3], tmp[9], Sk[9];
     if (k % 2 == 0) {
         if (cc < 3) {
@@ -960,7 +847,8 @@ __device__ void imu_raw_column(const double* d, const ImuShared& o, int c, doubl
             col[9 + j] = sign;
             if (first) {
                 for (int i = 0; i < 3; ++i) col[i] = o.dt * v[i] + d[49 + i * 3 + j];
-                for (int i = 0; i < 3; ++i) col[3 + i] = o.Jbias[i * 3 + j];
+                mcol(o.Jbias, j, v);
+                for (int i = 0; i < 3; ++i) col[3 + i] = v[i];
                 for (int i = 0; i < 3; ++i) col[6 + i] += d[40 + i * 3 + j];
             }
         } else {
@@ -993,7 +881,7 @@ __device__ const double* prior_state(const WinDev& w, const PriorBlockDev& b, co
 // 2 kMaxRetained doubles of the wave's own LDS (the prior's chart and residual). MODE 0
 // linearizes (residual, Jacobians, prior chart derivative), MODE 1 evaluates the trial cost.
 template <int MODE>
-__device__ void retained_body(const WinDev& w, int y, int lane, double* scratch) {
+__device__ __forceinline__ void retained_body(const WinDev& w, int y, int lane, double* scratch) {
     constexpr int mode = MODE;
     if (!window_active(w)) return;
     if (mode == 0 && !w.lm->new_point) return;
@@ -1004,31 +892,44 @@ __device__ void retained_body(const WinDev& w, int y, int lane, double* scratch)
     const double* td = mode == 0 ? w.td : w.ctd;
     double* sc = w.ret_sc;
     if (y < E) {
+        // the whitening W (15 x 15) and the raw Jacobian columns in the wave's LDS; the products
+        // are split over the lanes (one per residual row, then one per Jacobian entry)
         const int s0 = w.imu_from[y], s1 = w.imu_to[y];
         const double* d = w.imu + (size_t)y * 301;
+        double* Ws = scratch;
+        double* rs = scratch + 225;
+        double* cols = scratch + 240;  // column c of the raw Jacobian at cols[15 c ..]
+        for (int k = lane; k < 225; k += 64) Ws[k] = d[76 + k];
         ImuShared o;
         imu_shared(d, pose + 7 * s0, sb + 9 * s0, pose + 7 * s1, sb + 9 * s1, mode == 0, o);
-        int bad = !o.ok;
-        if (mode == 0 && lane < 30) {
-            double col[15];
-            imu_raw_column(d, o, lane, col);
-            const double* W = d + 76;
+        wave_sync();
+        int bad = 0;
+        if (lane < 15) {
+            double v = 0;
+            for (int k = 0; k < 15; ++k) v += Ws[lane * 15 + k] * o.e[k];
+            rs[lane] = v;
+            bad = !isfinite(v);
+            if (mode == 0) w.imu_r[(size_t)y * 15 + lane] = v;
+        }
+        if (mode == 0) {
+            if (lane < 30) imu_raw_column(d, o, lane, cols + 15 * lane);
+            wave_sync();
             double* J = w.imu_J + (size_t)y * 450;
-            for (int i = 0; i < 15; ++i) {
+            for (int q = lane; q < 450; q += 64) {
+                const int i = q / 30, c = q % 30;
                 double v = 0;
-                for (int m = 0; m < 15; ++m) v += W[i * 15 + m] * col[m];
-                J[i * 30 + lane] = v;
+                for (int m = 0; m < 15; ++m) v += Ws[i * 15 + m] * cols[15 * c + m];
+                J[q] = v;
                 if (!isfinite(v)) bad = 1;
             }
         }
+        wave_sync();
         for (int m = 32; m > 0; m >>= 1) bad |= __shfl_xor(bad, m, 64);
         if (lane == 0) {
             double c = 0;
-            for (int i = 0; i < 15; ++i) c += 0.5 * o.r[i] * o.r[i];
+            for (int i = 0; i < 15; ++i) c += 0.5 * rs[i] * rs[i];
             sc[mode * E + y] = c;
             sc[2 * E + mode * E + y] = bad ? 1.0 : 0.0;
-            if (mode == 0)
-                for (int i = 0; i < 15; ++i) w.imu_r[(size_t)y * 15 + i] = o.r[i];
         }
         return;
     }
@@ -1048,19 +949,29 @@ __device__ void retained_body(const WinDev& w, int y, int lane, double* scratch)
         const bool se3 = b.kind == 0 || b.kind == 2;
         if (se3) pose_minus(x, ref, chart + b.chart);
         else for (int c = 0; c < b.tsize; ++c) chart[b.chart + c] = x[c] - ref[c];
-        if (mode == 0 && se3 && b.width > 0) {
-            const double h = 1e-7;
-            double step[6] = {0, 0, 0, 0, 0, 0}, pp[7], pm[7], dp[6], dm[6];
-            double* D = w.pD + (size_t)bi * 36;
-            for (int c = 0; c < 6; ++c) {
-                step[c] = h; pose_plus(x, step, pp);
-                step[c] = -h; pose_plus(x, step, pm);
-                step[c] = 0;
-                pose_minus(pp, ref, dp); pose_minus(pm, ref, dm);
-                for (int r = 0; r < 6; ++r) D[r * 6 + c] = (dp[r] - dm[r]) / (2 * h);
-            }
-        }
     }
+    // chart derivatives of the SE(3) blocks, one lane per (block, tangent column)
+    if (mode == 0)
+        for (int it = lane; it < 6 * B; it += 64) {
+            const int bi = it / 6, c = it % 6;
+            const PriorBlockDev b = w.pblk[bi];
+            if (!((b.kind == 0 || b.kind == 2) && b.width > 0)) continue;
+            const double* x = prior_state(w, b, pose, sb, rig, td);
+            const double* ref = w.pref + b.ref;
+            const double h = 1e-7;
+            double step[6], pp[7], pm[7], dp[6], dm[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) step[k] = k == c ? h : 0.0;
+            pose_plus(x, step, pp);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) step[k] = k == c ? -h : 0.0;
+            pose_plus(x, step, pm);
+            pose_minus(pp, ref, dp);
+            pose_minus(pm, ref, dm);
+            double* D = w.pD + (size_t)bi * 36;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) D[r * 6 + c] = (dp[r] - dm[r]) / (2 * h);
+        }
     wave_sync();
     double cost = 0;
     int bad = 0;
@@ -1090,7 +1001,7 @@ __device__ void retained_body(const WinDev& w, int y, int lane, double* scratch)
 
 // The retained items alone: grid (window, item), one wave each.
 __global__ __launch_bounds__(64) void k_retained(const WinDev* wins, int mode) {
-    __shared__ double scratch[2 * kMaxRetained];
+    __shared__ double scratch[kRetainedScratch];
     if (mode == 0) retained_body<0>(wins[blockIdx.x], blockIdx.y, threadIdx.x, scratch);
     else retained_body<1>(wins[blockIdx.x], blockIdx.y, threadIdx.x, scratch);
 }
