@@ -800,9 +800,10 @@ bool Batch::launch(int k, hipStream_t s, int refinement, int mode) {
 }
 
 // Batches of up to kFuseReduceWindows windows run the retained linearization (kernel 1) as extra
-// workgroups of kernel 0 and the visual-block partial sums (kernel 2) as extra workgroups of
-// kernel 3, so an iteration skips kernels 1 and 2. Larger batches launch them separately: their
-// many small items would otherwise run at the occupancy of the large visual and Schur kernels.
+// workgroups of kernel 0, the visual-block partial sums (kernel 2) as extra workgroups of
+// kernel 3 and the retained trial cost (kernel 9) as extra workgroups of kernel 8, so an
+// iteration skips kernels 1, 2 and 9. Larger batches launch them separately: their many small
+// items would otherwise run at the occupancy of the large visual, Schur and trial kernels.
 // A linearization alone always sums the partials.
 bool Batch::fuse_reduce() const { return nwin <= kFuseReduceWindows; }
 bool Batch::linearize(hipStream_t s) {
@@ -824,7 +825,8 @@ std::vector<std::pair<int, int>> Batch::sequence(bool iteration, int refinement,
             q.push_back({12, 0});
             q.push_back({13, i + 1 == refinement ? 1 : 0});
         }
-    for (int k = 7; k < 12; ++k) q.push_back({k, mode});
+    for (int k = 7; k < 12; ++k)
+        if (k != 9 || !fuse_reduce()) q.push_back({k, mode});
     return q;
 }
 

@@ -2698,10 +2698,21 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
 // landmarks' observation counts (in landmark order, 8 lanes per landmark idle where a landmark
 // has fewer observations). Per-range partials go to lin_sc; k_lm_update sums them in list order.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const Chunk* lin) {
+// FUSED (small batches): workgroups 0 .. rb-1 evaluate the retained items at the trial state
+// first (items / per as in the linearization), so the retained trial cost needs no launch of its
+// own; rb is a multiple of the XCD count like there.
+template <bool FUSED>
+__global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const Chunk* lin, int rb, int items, int per) {
+    if constexpr (FUSED) {
+        if ((int)blockIdx.x < rb) {
+            __shared__ double rscratch[kWaves][kRetainedScratch];
+            retained_items<1>(wins, 0, items, per, rscratch[threadIdx.x / 64]);
+            return;
+        }
+    }
     __shared__ FrameTable ft;
     __shared__ double red[kBlock];
-    const Chunk ch = lin[blockIdx.x];
+    const Chunk ch = lin[blockIdx.x - rb];
     if (ch.local < 0) return;  // an empty position of the XCD-ordered work list
     const WinDev& w = wins[ch.win];
     if (!window_active(w)) return;
@@ -3725,9 +3736,12 @@ hipError_t vibk_launch(int k, vib::WinDev* wins, int nwin, const vib::Chunk* chu
                 else hipLaunchKernelGGL(k_backsub<false>, dim3(nchunks), dim3(kBlock), 0, s, wins, chunks);
             }
             break;
-        case 8:
-            if (nlin) hipLaunchKernelGGL(k_trial_cost, dim3(nlin), dim3(kBlock), 0, s, wins, lin);
+        case 8: {  // (small batches) the retained trial items, then the visual trial cost
+            const int rb = ritems ? ((ritems + kWaves - 1) / kWaves + kLinXcds - 1) / kLinXcds * kLinXcds : 0;
+            if (rb) hipLaunchKernelGGL(k_trial_cost<true>, dim3(rb + nlin), dim3(kBlock), 0, s, wins, lin, rb, ritems, rper);
+            else if (nlin) hipLaunchKernelGGL(k_trial_cost<false>, dim3(nlin), dim3(kBlock), 0, s, wins, lin, 0, 0, 1);
             break;
+        }
         case 9: hipLaunchKernelGGL(k_retained, dim3(nwin, max_edges + 1), dim3(64), 0, s, wins, 1); break;
         case 10: hipLaunchKernelGGL(k_lm_update, dim3(nwin), dim3(kBlock), 0, s, wins, mode); break;
         case 11:
