@@ -1030,8 +1030,19 @@ __device__ __forceinline__ void reduce_slice(const WinDev& w, int which, double*
     const double* part = which == 0 ? w.lin_part : w.schur_part;
     const int nparts = which == 0 ? w.nlin : w.nchunks;
     double s = 0;
-    if (act && i < len)
-        for (int c = cl; c < nparts; c += lanes) s += part[(size_t)c * len + i];
+    if (act && i < len) {
+        // four partials' loads in flight at a time, added in the same order
+        int c = cl;
+        for (; c + 3 * lanes < nparts; c += 4 * lanes) {
+            const double a0 = part[(size_t)c * len + i], a1 = part[(size_t)(c + lanes) * len + i];
+            const double a2 = part[(size_t)(c + 2 * lanes) * len + i], a3 = part[(size_t)(c + 3 * lanes) * len + i];
+            s += a0;
+            s += a1;
+            s += a2;
+            s += a3;
+        }
+        for (; c < nparts; c += lanes) s += part[(size_t)c * len + i];
+    }
     if (act) red[cl * E + e] = s;
     __syncthreads();
     if (act && cl == 0 && i < len) {
