@@ -453,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     __shared__ double acc[kAccSize];
     __shared__ FrameTable ft;
     __shared__ double rows[kWaves][kGramCols * kRowPitch];  // per wave: 2 rows per lane x 16 columns
-    __shared__ int nruns[2][kWaves];
+    __shared__ int more[kWaves];
     __shared__ double glist[kWaves][kGramsPerWave][kGramEntries];
     __shared__ int gkey[kWaves][kGramsPerWave];
     __shared__ int ngram[kWaves];
@@ -583,15 +583,10 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
         // 2 st + l / 32), column l % 16
         const int half = lane / 32;
         const double* vcol = V + (lane % 16) * kRowPitch + lane / 16;
+        // passes of up to kGramsPerWave listed matrices per wave, until no wave has runs left (the
+        // flags are read between the pass's two barriers, so every wave takes the same passes)
         unsigned long long pending = __ballot(key >= 0);
-        const int prev = __shfl_up(key, 1, 64);
-        const unsigned long long starts = __ballot(key >= 0 && (lane == 0 || prev != key));
-        if (lane == 0) nruns[round & 1][wave] = __popcll(starts);
-        __syncthreads();
-        int maxruns = 0;
-#pragma unroll
-        for (int q = 0; q < kWaves; ++q) maxruns = max(maxruns, nruns[round & 1][q]);
-        for (int b0 = 0; b0 < maxruns; b0 += kGramsPerWave) {
+        for (;;) {
             int cnt = 0;
             while (pending && cnt < kGramsPerWave) {
                 const int leader = __ffsll(pending) - 1;
@@ -628,8 +623,14 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
                 ++cnt;
                 pending &= ~run;
             }
-            if (lane == 0) ngram[wave] = cnt;
+            if (lane == 0) {
+                ngram[wave] = cnt;
+                more[wave] = pending != 0ull;
+            }
             __syncthreads();
+            bool again = false;
+#pragma unroll
+            for (int q = 0; q < kWaves; ++q) again = again || more[q] != 0;
             if (t < kGramEntries) {
                 // listed matrices in (wave, list position) order; the next one's key and entry
                 // are read from LDS while the current one is merged
@@ -660,6 +661,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
                 }
             }
             __syncthreads();
+            if (!again) break;
         }
         wave_sync();
     }
