@@ -37,30 +37,34 @@ typedef double d4 __attribute__((ext_vector_type(4)));  // v_mfma_f64_16x16x4f64
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // Deterministic block reductions (fixed tree order).
-__device__ double block_sum(double v, double* red) {
+// Pairwise tree over the workgroup (level s: element t < s takes t + s, s = n/2 .. 1). The
+// levels s >= 64 go through LDS with a barrier each; the last six run in wave 0, lane t taking
+// lane t + s's value by a shuffle, which is the same operation on the same operands.
+template <bool MAX>
+__device__ __forceinline__ double block_reduce(double v, double* red) {
     const int t = threadIdx.x, n = blockDim.x;
     red[t] = v;
     __syncthreads();
-    for (int s = n / 2; s > 0; s >>= 1) {
-        if (t < s) red[t] += red[t + s];
+    int s = n / 2;
+    for (; s >= 64; s >>= 1) {
+        if (t < s) red[t] = MAX ? fmax(red[t], red[t + s]) : red[t] + red[t + s];
         __syncthreads();
     }
+    if (t < 64) {
+        double x = red[t];
+        for (; s > 0; s >>= 1) {
+            const double y = __shfl_down(x, s, 64);
+            if (t < s) x = MAX ? fmax(x, y) : x + y;
+        }
+        if (t == 0) red[0] = x;
+    }
+    __syncthreads();
     const double r = red[0];
     __syncthreads();
     return r;
 }
-__device__ double block_max(double v, double* red) {
-    const int t = threadIdx.x, n = blockDim.x;
-    red[t] = v;
-    __syncthreads();
-    for (int s = n / 2; s > 0; s >>= 1) {
-        if (t < s) red[t] = fmax(red[t], red[t + s]);
-        __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
-}
+__device__ double block_sum(double v, double* red) { return block_reduce<false>(v, red); }
+__device__ double block_max(double v, double* red) { return block_reduce<true>(v, red); }
 
 // Orders LDS traffic between the lanes of one wavefront (single-wave phases of a workgroup).
 __device__ __forceinline__ void wave_sync() {
