@@ -65,6 +65,43 @@ __device__ __forceinline__ double block_reduce(double v, double* red) {
 }
 __device__ double block_sum(double v, double* red) { return block_reduce<false>(v, red); }
 __device__ double block_max(double v, double* red) { return block_reduce<true>(v, red); }
+// N values through the same tree, sharing its barriers: value q is a maximum if bit q of MAXMASK
+// is set, else a sum; red holds N * blockDim.x doubles.
+template <int N, unsigned MAXMASK>
+__device__ __forceinline__ void block_reduce_n(double (&v)[N], double* red) {
+    const int t = threadIdx.x, n = blockDim.x;
+#pragma unroll
+    for (int q = 0; q < N; ++q) red[q * n + t] = v[q];
+    __syncthreads();
+    int s = n / 2;
+    for (; s >= 64; s >>= 1) {
+        if (t < s)
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+                double* r = red + q * n;
+                r[t] = ((MAXMASK >> q) & 1u) ? fmax(r[t], r[t + s]) : r[t] + r[t + s];
+            }
+        __syncthreads();
+    }
+    if (t < 64) {
+        double x[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) x[q] = red[q * n + t];
+        for (; s > 0; s >>= 1)
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+                const double y = __shfl_down(x[q], s, 64);
+                if (t < s) x[q] = ((MAXMASK >> q) & 1u) ? fmax(x[q], y) : x[q] + y;
+            }
+        if (t == 0)
+#pragma unroll
+            for (int q = 0; q < N; ++q) red[q * n] = x[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = red[q * n];
+    __syncthreads();
+}
 
 // Orders LDS traffic between the lanes of one wavefront (single-wave phases of a workgroup).
 __device__ __forceinline__ void wave_sync() {
@@ -2606,7 +2643,7 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eigen_lds(const EigenJob* j
 // ---------------------------------------------------------------------------------------------
 template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Chunk* chunks) {
-    __shared__ double red[kBlock];
+    __shared__ double red[3 * kBlock];
     __shared__ double dxv[kMaxVisual];  // the visual columns of the state step
     __shared__ int pcol[kMaxFrames];
     const Chunk ch = chunks[blockIdx.x];
@@ -2697,9 +2734,9 @@ __global__ __launch_bounds__(kBlock) void k_backsub(const WinDev* wins, const Ch
             if (l64 == 0) pred += -dx[i] * (w.gfull[i] + 0.5 * h);
         }
     }
-    const double ps = block_sum(pred, red);
-    const double m = block_max(mx, red);
-    const double b = block_max(bad ? 1.0 : 0.0, red);
+    double sums[3] = {pred, mx, bad ? 1.0 : 0.0};
+    block_reduce_n<3, 0b110u>(sums, red);  // maxima: mx, bad
+    const double ps = sums[0], m = sums[1], b = sums[2];
     if (threadIdx.x == 0) {
         double* sc = w.chunk_sc + (size_t)ch.local * CS_PLANES;
         sc[CS_PRED] = ps;
@@ -2795,7 +2832,7 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
 // mode 0: LM step; 1: evaluate only (metrics, no state change); 2: accept the trial state.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
-    __shared__ double red[kBlock];
+    __shared__ double red[5 * kBlock];
     WinDev& w = wins[blockIdx.x];
     LmState* lm = w.lm;
     const int t = threadIdx.x;
@@ -2825,11 +2862,13 @@ __global__ __launch_bounds__(kBlock) void k_lm_update(WinDev* wins, int mode) {
             trial += sc[CS_TRIAL];
             tinv = fmax(tinv, sc[CS_TINVALID]);
         }
-        pred = block_sum(pred, red);
-        nonfinite = block_max(nonfinite, red);
-        trial = block_sum(trial, red);
-        mx = block_max(mx, red);
-        tinv = block_max(tinv, red);
+        double sums[5] = {pred, nonfinite, trial, mx, tinv};
+        block_reduce_n<5, 0b11010u>(sums, red);  // maxima: nonfinite, mx, tinv
+        pred = sums[0];
+        nonfinite = sums[1];
+        trial = sums[2];
+        mx = sums[3];
+        tinv = sums[4];
         if (t == 0) {
             int accept_flag = 0;
             pred += lm->predicted;
