@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     __shared__ double glist[kWaves][kGramsPerWave][kGramEntries];
     __shared__ int gkey[kWaves][kGramsPerWave];
     __shared__ int ngram[kWaves];
-    __shared__ double red[kBlock];
+    __shared__ double red[2 * kBlock];
     if ((int)blockIdx.x < rb) {  // the row staging of the wave holds the prior's scratch
         static_assert(kGramCols * kRowPitch >= kRetainedScratch, "a wave's row staging holds a retained item");
         retained_items<0>(wins, 0, items, per, rows[threadIdx.x / 64]);
@@ -710,8 +710,9 @@ __global__ __launch_bounds__(kBlock) void k_lin_visual(const WinDev* wins, const
     __syncthreads();
     double* part = w.lin_part + (size_t)ch.local * (tri + nv);
     for (int i = threadIdx.x; i < tri + nv; i += kBlock) part[i] = acc[i];
-    const double c = block_sum(cost, red);
-    const double inv = block_max(invalid ? 1.0 : 0.0, red);
+    double sums[2] = {cost, invalid ? 1.0 : 0.0};
+    block_reduce_n<2, 0b10u>(sums, red);  // the cost, then the invalid flag (maximum)
+    const double c = sums[0], inv = sums[1];
     if (threadIdx.x == 0) {
         w.lin_sc[(size_t)ch.local * CS_PLANES + CS_COST] = c;
         w.lin_sc[(size_t)ch.local * CS_PLANES + CS_INVALID] = inv;
@@ -1449,7 +1450,7 @@ __device__ __forceinline__ void prior_row(const WinDev& w, const int* ci, double
 __device__ __forceinline__ int opnd_pos(int i, int c) { return (c >> 2) * 64 + i + 16 * (c & 3); }
 
 __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const* reduced) {
-    __shared__ double red[kBlock];
+    __shared__ double red[2 * kBlock];
     __shared__ double srow[kAssembleRows][kMaxRetained];  // the rows: undamped, then damped
     WinDev& w = wins[blockIdx.x];
     if (!window_active(w)) return;
@@ -1512,8 +1513,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble(WinDev* wins, double* const
                 cost += w.ret_sc[4 * w.E];
                 invalid = fmax(invalid, w.ret_sc[4 * w.E + 2]);
             }
-            cost = block_sum(cost, red);
-            invalid = block_max(invalid, red);
+            double sums[2] = {cost, invalid};
+            block_reduce_n<2, 0b10u>(sums, red);
+            cost = sums[0];
+            invalid = sums[1];
             if (t == 0) {
                 w.lm->lin_cost = cost;
                 w.lm->lin_valid = !(invalid > 0) && isfinite(cost);
@@ -2765,7 +2768,7 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
         }
     }
     __shared__ FrameTable ft;
-    __shared__ double red[kBlock];
+    __shared__ double red[2 * kBlock];
     const Chunk ch = lin[blockIdx.x - rb];
     if (ch.local < 0) return;  // an empty position of the XCD-ordered work list
     const WinDev& w = wins[ch.win];
@@ -2818,8 +2821,9 @@ __global__ __launch_bounds__(kBlock) void k_trial_cost(const WinDev* wins, const
         if (!isfinite(sq)) { invalid = 1; continue; }
         cost += huber_cost(w.huber, sq);
     }
-    const double c = block_sum(cost, red);
-    const double b = block_max(invalid ? 1.0 : 0.0, red);
+    double sums[2] = {cost, invalid ? 1.0 : 0.0};
+    block_reduce_n<2, 0b10u>(sums, red);
+    const double c = sums[0], b = sums[1];
     if (threadIdx.x == 0) {
         double* sc = w.lin_sc + (size_t)ch.local * CS_PLANES;
         sc[CS_TRIAL] = c;
